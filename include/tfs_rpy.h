@@ -1,0 +1,150 @@
+/*
+ * tfs_rpy.h -- C-ABI of the MI355X (gfx950) RPY mobility library.
+ *
+ * Drop-in for the RPY mobility back-end of TimFilSim: the abstract class
+ * `mobility_solver` (src/cilia/mobility/mobility_solver.hpp:19-127) and its
+ * GPU implementation `rpy_mobility_solver`
+ * (src/cilia/mobility/rpy_mobility_solver.cu:1-337).  Plain pointers and
+ * sizes only; no C++ or torch types cross this boundary.
+ *
+ * Layouts are the reference's host layouts (rpy_mobility_solver.cu:65-71):
+ *   x_segs [3*Ns]  x y z       per segment, body -> filament -> segment
+ *   f_segs [6*Ns]  fx fy fz tx ty tz
+ *   v_segs [6*Ns]  vx vy vz wx wy wz   (w written as 0 when seg_dof == 3)
+ *   x_blobs, f_blobs, v_blobs [3*Nb]
+ * where Ns = NSWIM*NFIL*NSEG and Nb = NSWIM*NBLOB.
+ *
+ * Every entry point returns 0 (TFS_OK) or a TFS_ERR_* code; tfs_last_error()
+ * gives the message of the last failure on the calling thread.  Work is
+ * enqueued on the handle's stream; results are visible after tfs_rpy_sync()
+ * (the reference's wait_for_device, rpy_mobility_solver.cu:247-256).  A handle
+ * is not thread-safe: use one handle per host thread.
+ */
+#ifndef TFS_RPY_H
+#define TFS_RPY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TFS_OK 0
+#define TFS_ERR_INVALID_ARGUMENT 1
+#define TFS_ERR_HIP 2
+#define TFS_ERR_NON_FINITE 3
+#define TFS_ERR_NOT_CONVERGED 4
+#define TFS_ERR_NO_DEVICE 5
+#define TFS_ERR_OUT_OF_MEMORY 6
+
+/* Evaluation precision of the pair tensors. */
+#define TFS_PRECISION_FP64 0       /* FP64 tiles, FP64 sums (reference FP64 build) */
+#define TFS_PRECISION_FP32_ACC64 1 /* FP32 tiles, FP64 accumulation                */
+
+/* Write rule of the blob-segment block (cuda_functions.cu:739-751). */
+#define TFS_BS_OVERWRITE 0         /* GMRES-for-linear-system, elastic filaments   */
+#define TFS_BS_ACCUMULATE 1        /* PRESCRIBED_CILIA || USE_BROYDEN_FOR_EVERYTHING */
+
+/* Mobility blocks (bit mask). */
+#define TFS_BLOCK_SS 1u            /* V_s  = M_ss F_s   Mss_mult cuda_functions.cu:362-474 */
+#define TFS_BLOCK_SB 2u            /* V_s += M_sb F_b   Msb_mult cuda_functions.cu:551-636 */
+#define TFS_BLOCK_BB 4u            /* V_b  = M_bb F_b   Mbb_mult cuda_functions.cu:476-549 */
+#define TFS_BLOCK_BS 8u            /* V_b (+)= M_bs F_s Mbs_mult cuda_functions.cu:638-757 */
+#define TFS_BLOCK_ALL 15u          /* evaluate_full_mobility rpy_mobility_solver.cu:330-337 */
+
+/* Where caller pointers live. */
+#define TFS_MEMORY_HOST 0
+#define TFS_MEMORY_DEVICE 1
+
+/* Arrays moved by tfs_rpy_upload / tfs_rpy_download. */
+#define TFS_SEG_POSITIONS 0
+#define TFS_SEG_FORCES 1
+#define TFS_BLOB_POSITIONS 2
+#define TFS_BLOB_FORCES 3
+#define TFS_SEG_VELOCITIES 4
+#define TFS_BLOB_VELOCITIES 5
+
+typedef struct tfs_rpy_desc {
+    int32_t num_segs;      /* NSWIM*NFIL*NSEG (>= 0)                                   */
+    int32_t num_blobs;     /* NSWIM*NBLOB (>= 0)                                       */
+    int32_t seg_dof;       /* 6: <6,6>,<6,3>,<3,6> (!PRESCRIBED_CILIA); 3: <3,3>        */
+    int32_t precision;     /* TFS_PRECISION_*                                          */
+    int32_t bs_mode;       /* TFS_BS_*                                                  */
+    int32_t memory;        /* TFS_MEMORY_*: where the pointers passed later live       */
+    double seg_radius;     /* RSEG  (config.hpp:256)                                   */
+    double blob_radius;    /* RBLOB (config.hpp:257)                                   */
+    double mu;             /* MU    (config.hpp:254)                                   */
+    void *stream;          /* hipStream_t to run on, or NULL for a library-owned one   */
+} tfs_rpy_desc;
+
+typedef struct tfs_rpy tfs_rpy;
+
+/* Allocates the device buffers (allocate_device_memory, rpy_mobility_solver.cu:91-117). */
+int tfs_rpy_create(const tfs_rpy_desc *desc, tfs_rpy **out);
+/* free_device_memory (rpy_mobility_solver.cu:29-53). */
+int tfs_rpy_destroy(tfs_rpy *h);
+
+/* copy_{segment,blob}_{positions,forces}_to_device (rpy_mobility_solver.cu:119-161):
+ * `what` is TFS_SEG_POSITIONS / TFS_SEG_FORCES / TFS_BLOB_POSITIONS / TFS_BLOB_FORCES,
+ * `src` the full AoS array in the desc's memory space. Asynchronous.
+ * TFS_SEG_VELOCITIES / TFS_BLOB_VELOCITIES are accepted too: they set the
+ * velocities that an accumulating block (SB, or BS in TFS_BS_ACCUMULATE) adds to. */
+int tfs_rpy_upload(tfs_rpy *h, int32_t what, const double *src);
+
+/* copy_{segment,blob}_velocities_to_host (rpy_mobility_solver.cu:177-205):
+ * `what` is TFS_SEG_VELOCITIES / TFS_BLOB_VELOCITIES. Asynchronous. */
+int tfs_rpy_download(tfs_rpy *h, int32_t what, double *dst);
+
+/* evaluate_{segment_segment,segment_blob,blob_blob,blob_segment,full}_mobility
+ * (rpy_mobility_solver.cu:258-337) on the uploaded positions/forces, into the
+ * handle's velocity buffers, with the reference's write rules: SS '=', SB '+=',
+ * BB '=', BS '+=' or '=' per bs_mode; blocks given together are applied in the
+ * reference's order (SS then SB; BB then BS). */
+int tfs_rpy_evaluate(tfs_rpy *h, uint32_t blocks);
+
+/* One call: V = M F for the requested blocks with caller arrays (any of them may
+ * be NULL when its block is absent).  v_* are read first where a block
+ * accumulates.  Positions must have been uploaded. */
+int tfs_rpy_apply(tfs_rpy *h, uint32_t blocks, const double *f_segs, const double *f_blobs,
+                  double *v_segs, double *v_blobs);
+
+/* wait_for_device (rpy_mobility_solver.cu:247-256). */
+int tfs_rpy_sync(tfs_rpy *h);
+
+/* Switch the pair-tensor precision of later evaluations. */
+int tfs_rpy_set_precision(tfs_rpy *h, int32_t precision);
+
+/* Override the launch shape (threads per workgroup 64/128/256, targets per
+ * thread 1/2, sources per work item: a multiple of block) for a precision. */
+int tfs_rpy_set_tile_shape(tfs_rpy *h, int32_t precision, int32_t block, int32_t per_thread,
+                           int32_t chunk);
+
+/* Device address of an internal array (TFS_SEG_POSITIONS ... TFS_BLOB_VELOCITIES,
+ * AoS reference layout), for callers that keep data resident on the GPU. */
+int tfs_rpy_device_buffer(tfs_rpy *h, int32_t what, double **out);
+
+/* Live kernel timing (no counterpart in the reference, whose DISPLAYTIME
+ * buckets bracket whole phases with cudaDeviceSynchronize, mobility_solver.cpp:2576-2657):
+ * while enabled, HIP events are recorded on the handle's stream around every
+ * pair-kernel launch; profile_read synchronises, returns the summed kernel
+ * milliseconds and the launch count since the last read, and resets them. */
+int tfs_rpy_profile(tfs_rpy *h, int32_t enable);
+int tfs_rpy_profile_read(tfs_rpy *h, double *pair_ms_total, int32_t *launches);
+
+/* Pinned host memory, as the reference's cudaHostAlloc(..., cudaHostAllocPortable)
+ * host arrays (rpy_mobility_solver.cu:65-71; freed at :14-27). */
+int tfs_host_alloc(size_t bytes, void **out);
+int tfs_host_free(void *p);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *tfs_last_error(void);
+
+/* Library version string. */
+const char *tfs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TFS_RPY_H */

@@ -1,0 +1,137 @@
+/*
+ * tfs_solver.h -- C-ABI of the per-timestep mobility solve on the MI355X.
+ *
+ * Replaces the generic solver layer of the reference's `mobility_solver`
+ * (src/cilia/mobility/mobility_solver.cpp) for prescribed cilia on a body whose
+ * velocity is prescribed (config.hpp: CILIA_TYPE 3, PRESCRIBED_BODY_VELOCITIES):
+ *   system_matrix_mult_new   mobility_solver.cpp:1806-2090 -> tfs_solver_system_mult
+ *   apply_preconditioner_new mobility_solver.cpp:1040-1513 -> tfs_solver_precondition
+ *   solve_linear_system      mobility_solver.cpp:2092-2570 -> tfs_solver_solve
+ *   assemble_rhs + compute_velocities + filament::initial_guess
+ *                            mobility_solver.cpp:240-440, 2574-2659,
+ *                            filament.cpp:1083-1273       -> tfs_beat_step
+ *
+ * Unknown vector (mobility_solver.cpp:107-140), n = 3*Ns + 3*Nb + [Nfil] + [Nfil]:
+ *   [ segment forces (xyz per segment) | blob forces (xyz per blob) |
+ *     phase speeds (if dynamic_phase) | shape-rotation speeds (if dynamic_angle) ]
+ *
+ * The Krylov solver is GMRES(m) with right preconditioning and FP64 Arnoldi
+ * (classical Gram-Schmidt applied twice).  restart = min(n, 350) with
+ * max_cycles = 1 and tol = 1e-4 reproduces the reference's single-cycle solve
+ * (config.hpp:312-321); restarted cycles and mixed-precision iterative
+ * refinement are additions.  All statuses are TFS_* codes of tfs_rpy.h.
+ */
+#ifndef TFS_SOLVER_H
+#define TFS_SOLVER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "tfs_rpy.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TFS_SOLVE_FP64 0   /* every matvec in FP64 tiles                               */
+#define TFS_SOLVE_MIXED 1  /* inner GMRES on FP32-tile matvecs, FP64 residuals outside */
+
+typedef struct tfs_solver_desc {
+    int32_t num_fil;         /* NSWIM*NFIL                                          */
+    int32_t num_seg;         /* NSEG (segments per filament)                        */
+    int32_t num_blobs;       /* NSWIM*NBLOB                                         */
+    int32_t dynamic_phase;   /* DYNAMIC_PHASE_EVOLUTION (config.hpp:100)            */
+    int32_t dynamic_angle;   /* DYNAMIC_SHAPE_ROTATION (config.hpp:104)             */
+    int32_t mode;            /* TFS_SOLVE_*                                          */
+    double seg_radius, blob_radius, mu;
+    int32_t restart;         /* Krylov vectors per cycle (reference: min(n, 350))   */
+    int32_t max_cycles;      /* restart cycles per (inner) solve                     */
+    double tol;              /* target ||b - A x|| / ||b|| (reference: 1e-4)         */
+    double inner_tol;        /* mixed: reduction asked of each inner solve           */
+    int32_t max_refinements; /* mixed: iterative-refinement steps                    */
+    void *stream;            /* hipStream_t or NULL                                   */
+} tfs_solver_desc;
+
+typedef struct tfs_solve_stats {
+    int32_t iterations;       /* Krylov iterations (all cycles, all refinements)     */
+    int32_t cycles;           /* GMRES cycles                                        */
+    int32_t refinements;      /* mixed: outer refinement steps taken                 */
+    int32_t fp64_matvecs;     /* RPY matvecs in FP64 tiles                           */
+    int32_t fp32_matvecs;     /* RPY matvecs in FP32 tiles                           */
+    int32_t converged;        /* 1 if the tolerance was met                          */
+    double relres;            /* last relative residual (true residual in mixed mode,
+                                 GMRES estimate of the last cycle otherwise)         */
+    double initial_relres;    /* ||b - A x0|| / ||b|| of the warm start              */
+} tfs_solve_stats;
+
+typedef struct tfs_solver tfs_solver;
+
+int tfs_solver_create(const tfs_solver_desc *desc, tfs_solver **out);
+int tfs_solver_destroy(tfs_solver *s);
+/* System size n. */
+int tfs_solver_size(tfs_solver *s, int64_t *n);
+/* Geometry of the current step: segment and blob positions (3 per particle)
+ * and the phase / angle velocity directions vel_dir_phase, vel_dir_angle
+ * (3 per segment, filament.cpp:1083-1273); NULL keeps what was set before.
+ * `memory` is TFS_MEMORY_HOST or TFS_MEMORY_DEVICE. */
+int tfs_solver_set_geometry(tfs_solver *s, const double *x_segs, const double *x_blobs,
+                            const double *vel_dir_phase, const double *vel_dir_angle,
+                            int32_t memory);
+/* out = A in (system_matrix_mult_new); precision is TFS_PRECISION_*. */
+int tfs_solver_system_mult(tfs_solver *s, const double *in, double *out, int32_t precision,
+                           int32_t memory);
+/* out = P^{-1} in (apply_preconditioner_new). */
+int tfs_solver_precondition(tfs_solver *s, const double *in, double *out, int32_t memory);
+/* Solve A x = rhs; x holds the warm start on entry (solve_linear_system,
+ * including its zero-rhs and rubbish-guess rules, mobility_solver.cpp:2123-2322).
+ * Returns TFS_ERR_NOT_CONVERGED (with x updated) if the tolerance is not met,
+ * TFS_ERR_NON_FINITE if a residual is not finite. */
+int tfs_solver_solve(tfs_solver *s, const double *rhs, double *x, int32_t memory,
+                     tfs_solve_stats *stats);
+/* ||rhs - A x|| / ||rhs|| with the given matvec precision. */
+int tfs_solver_residual(tfs_solver *s, const double *rhs, const double *x, int32_t precision,
+                        int32_t memory, double *relres);
+
+/* ---- Prescribed-beat timestep driver (one body, body velocity prescribed) ---- */
+typedef struct tfs_beat_desc {
+    tfs_solver_desc solver;
+    double dt;                       /* DT                                             */
+    double fil_length;               /* FIL_LENGTH = DL*(NSEG-1) (config.hpp:512)       */
+    const double *frames;            /* 9*num_fil: e1 normal, e2 beat direction, e3     */
+    const double *bases;             /* 3*num_fil: segment-0 positions                  */
+    const double *x_blobs;           /* 3*num_blobs                                     */
+    const double *s_values;          /* num_seg shape parameters, NULL: n/(NSEG-1)      */
+    const double *omega0;            /* num_fil natural frequencies                     */
+    const double *phase0;            /* num_fil initial phases                          */
+    const double *angle0;            /* num_fil initial shape-rotation angles or NULL   */
+    const double *beat_A;            /* 24: Fourier cosine coefficients [x|y][4][3]     */
+    const double *beat_B;            /* 18: Fourier sine coefficients  [x|y][3][3]      */
+    const double *gen_phase_forces;  /* reference generalised phase forces, one period  */
+    int32_t n_gen_phase;
+    const double *gen_angle_forces;  /* reference generalised angle forces, one period  */
+    int32_t n_gen_angle;
+    double gen_force_factor;         /* GEN_FORCE_MAGNITUDE_FACTOR                      */
+    double torsion_factor;           /* TORSIONAL_SPRING_MAGNITUDE_FACTOR               */
+    double body_x[3];                /* body centre                                     */
+    double body_v[6];                /* prescribed body velocity and angular velocity  */
+} tfs_beat_desc;
+
+typedef struct tfs_beat tfs_beat;
+
+int tfs_beat_create(const tfs_beat_desc *desc, tfs_beat **out);
+int tfs_beat_destroy(tfs_beat *b);
+/* One timestep: phase update from the previous solution (step > 0), beat
+ * kinematics, right-hand side, warm-started solve. */
+int tfs_beat_step(tfs_beat *b, tfs_solve_stats *stats);
+/* Host copies of the current state (any pointer may be NULL). */
+int tfs_beat_state(tfs_beat *b, double *phase, double *angle, double *x_segs, double *solution);
+/* The solver handle inside the driver (to call tfs_solver_* on its state). */
+int tfs_beat_solver(tfs_beat *b, tfs_solver **out);
+/* Switch between TFS_SOLVE_FP64 and TFS_SOLVE_MIXED for later steps. */
+int tfs_beat_set_mode(tfs_beat *b, int32_t mode);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TFS_SOLVER_H */

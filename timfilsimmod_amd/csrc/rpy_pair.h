@@ -1,0 +1,178 @@
+// rpy_pair.h -- Rotne-Prager-Yamakawa pair couplings on the device (gfx950).
+//
+// Restates the unbounded branch of rpy_interaction<vel_dof, force_dof>
+// (src/cilia/cuda_functions.cu:207-354) for one target i and one source j:
+//
+//   far field  r > ai + aj            : closed RPY forms written with the
+//                                       unnormalised separation d = xi - xj
+//                                       and 1/r from a refined rsqrt;
+//   overlap    |ai - aj| < r <= ai+aj : Zuk et al. (2014) forms, evaluated
+//                                       exactly as the reference does (r from
+//                                       sqrt, 1/(r + 1e-20));
+//   inner      r <= |ai - aj|         : rigid-body limit (includes r = 0).
+//
+// The far branch is the hot one; the other two are taken by a handful of
+// neighbours per particle, so they sit behind a wave-uniform-skippable branch.
+// Template parameters: T is the evaluation type (double or float), VT the
+// number of velocity components written (3: v, 6: v and omega), FS the
+// number of source components read (3: force, 6: force and torque).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace tfs {
+
+// Pair-independent constants, computed once on the host for given radii and mu.
+template <typename T>
+struct PairConsts {
+    T sum_a2;      // (ai + aj)^2          far-field threshold on r^2
+    T s3;          // (ai^2 + aj^2)/3
+    T c8;          // 1/(8 pi mu)
+    // overlap / inner branch (evaluated in double for both evaluation types)
+    double ai, aj, mu;
+};
+
+__device__ __forceinline__ double rsqrt_refined(double x) {
+    // v_rsq_f64 seed, then two Newton-Raphson steps y <- y + y/2 (1 - x y^2).
+    double y = __builtin_amdgcn_rsq(x);
+    double e = __builtin_fma(-x * y, y, 1.0);
+    y = __builtin_fma(0.5 * y, e, y);
+    e = __builtin_fma(-x * y, y, 1.0);
+    y = __builtin_fma(0.5 * y, e, y);
+    return y;
+}
+
+__device__ __forceinline__ float rsqrt_refined(float x) {
+    return __builtin_amdgcn_rsqf(x);
+}
+
+// Far field, r^2 > (ai+aj)^2.  rm1 = 1/r.
+template <typename T, int VT, int FS>
+__device__ __forceinline__ void pair_far(T* __restrict__ v, T dx, T dy, T dz, T rm1,
+                                         const T* __restrict__ f, T s3, T c8) {
+    const T rm2 = rm1 * rm1;
+    const T c = c8 * rm1;                       // 1/(8 pi mu r)
+    const T A = c + (c * s3) * rm2;             // (1 + (ai^2+aj^2)/(3 r^2)) / (8 pi mu r)
+    const T B = (c - T(3) * (c * s3) * rm2) * rm2;
+    const T dF = dx * f[0] + dy * f[1] + dz * f[2];
+    const T Bd = B * dF;
+    v[0] += A * f[0] + Bd * dx;
+    v[1] += A * f[1] + Bd * dy;
+    v[2] += A * f[2] + Bd * dz;
+    if constexpr (FS == 6 || VT == 6) {
+        const T C = c * rm2;                    // 1/(8 pi mu r^3)
+        if constexpr (FS == 6) {
+            // translation-rotation: v += C (tau x d)
+            v[0] += C * (f[4] * dz - f[5] * dy);
+            v[1] += C * (f[5] * dx - f[3] * dz);
+            v[2] += C * (f[3] * dy - f[4] * dx);
+        }
+        if constexpr (VT == 6) {
+            // rotation-translation: w += C (F x d)
+            v[3] += C * (f[1] * dz - f[2] * dy);
+            v[4] += C * (f[2] * dx - f[0] * dz);
+            v[5] += C * (f[0] * dy - f[1] * dx);
+            if constexpr (FS == 6) {
+                // rotation-rotation: w += -C/2 [tau - 3 (tau.d) d / r^2]
+                const T Ar = T(-0.5) * C;
+                const T Br = T(-3) * Ar * rm2 * (dx * f[3] + dy * f[4] + dz * f[5]);
+                v[3] += Ar * f[3] + Br * dx;
+                v[4] += Ar * f[4] + Br * dy;
+                v[5] += Ar * f[5] + Br * dz;
+            }
+        }
+    }
+}
+
+// Overlap and inner branches: a term-by-term restatement of
+// cuda_functions.cu:207-354, in double.  Accumulates into v.
+template <int VT, int FS>
+__device__ __forceinline__ void pair_near(double* __restrict__ v, double dx, double dy, double dz,
+                                       const double* __restrict__ f, double ai, double aj,
+                                       double mu) {
+    const double PI = 3.14159265358979323846;
+    const double r = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+    const double inv_r = 1.0 / (r + 1e-20);
+    const double da = __builtin_fabs(ai - aj);
+    const double ex = dx * inv_r, ey = dy * inv_r, ez = dz * inv_r;
+    double A, B;
+    const double e_dot = ex * f[0] + ey * f[1] + ez * f[2];
+    if (r > ai + aj) {
+        const double s = inv_r * inv_r * (ai * ai + aj * aj) / 3.0;
+        const double c = inv_r / (8.0 * PI * mu);
+        A = (1.0 + s) * c;
+        B = (1.0 - 3.0 * s) * c * e_dot;
+    } else if (r > da) {
+        const double den = 32.0 * r * r * r;
+        const double p = da * da + 3.0 * r * r;
+        const double q = da * da - r * r;
+        const double c = 1.0 / (6.0 * PI * mu * ai * aj);
+        A = (0.5 * (ai + aj) - p * p / den) * c;
+        B = (3.0 * q * q / den) * c * e_dot;
+    } else {
+        A = 1.0 / (6.0 * PI * mu * (ai > aj ? ai : aj));
+        B = 0.0;
+    }
+    v[0] += A * f[0] + B * ex;
+    v[1] += A * f[1] + B * ey;
+    v[2] += A * f[2] + B * ez;
+    if constexpr (FS == 6) {
+        double C;
+        if (r > ai + aj) {
+            C = inv_r * inv_r / (8.0 * PI * mu);
+        } else if (r > da) {
+            const double g = aj - ai + r;
+            C = inv_r * inv_r * g * g * (ai * ai + 2.0 * ai * (r + aj) - 3.0 * (aj - r) * (aj - r))
+                / (128.0 * PI * mu * ai * aj * aj * aj);
+        } else {
+            C = (aj > ai ? 1.0 : 0.0) * r / (8.0 * PI * mu * aj * aj * aj);
+        }
+        v[0] += C * (f[4] * ez - f[5] * ey);
+        v[1] += C * (f[5] * ex - f[3] * ez);
+        v[2] += C * (f[3] * ey - f[4] * ex);
+    }
+    if constexpr (VT == 6) {
+        double C;
+        if (r > ai + aj) {
+            C = inv_r * inv_r / (8.0 * PI * mu);
+        } else if (r > da) {
+            const double g = ai - aj + r;
+            C = inv_r * inv_r * g * g * (aj * aj + 2.0 * aj * (r + ai) - 3.0 * (ai - r) * (ai - r))
+                / (128.0 * PI * mu * aj * ai * ai * ai);
+        } else {
+            C = (ai > aj ? 1.0 : 0.0) * r / (8.0 * PI * mu * ai * ai * ai);
+        }
+        v[3] += C * (f[1] * ez - f[2] * ey);
+        v[4] += C * (f[2] * ex - f[0] * ez);
+        v[5] += C * (f[0] * ey - f[1] * ex);
+        if constexpr (FS == 6) {
+            const double t_dot = ex * f[3] + ey * f[4] + ez * f[5];
+            double Ar, Br;
+            if (r > ai + aj) {
+                Ar = -inv_r * inv_r * inv_r / (16.0 * PI * mu);
+                Br = -3.0 * t_dot * Ar;
+            } else if (r > da) {
+                const double r2 = r * r, r3 = r2 * r;
+                const double ai2 = ai * ai, aj2 = aj * aj;
+                const double d4 = da * da * da * da;
+                const double na = 5.0 * r3 * r3 - 27.0 * r2 * r2 * (ai2 + aj2)
+                                + 32.0 * r3 * (ai2 * ai + aj2 * aj)
+                                - 9.0 * r2 * (ai2 - aj2) * (ai2 - aj2)
+                                - d4 * (ai2 + aj2 + 4.0 * ai * aj);
+                const double w = (ai - aj) * (ai - aj) - r2;
+                const double nb = 3.0 * w * w * (ai2 + aj2 + 4.0 * ai * aj - r2);
+                const double c = 1.0 / (512.0 * PI * mu * r3 * ai2 * ai * aj2 * aj);
+                Ar = na * c;
+                Br = nb * c * t_dot;
+            } else {
+                const double am = ai > aj ? ai : aj;
+                Ar = 1.0 / (8.0 * PI * mu * am * am * am);
+                Br = 0.0;
+            }
+            v[3] += Ar * f[3] + Br * ex;
+            v[4] += Ar * f[4] + Br * ey;
+            v[5] += Ar * f[5] + Br * ez;
+        }
+    }
+}
+
+}  // namespace tfs

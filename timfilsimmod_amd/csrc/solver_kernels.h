@@ -1,0 +1,67 @@
+// solver_kernels.h -- launchers of solver_kernels.hip (library-internal).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "tfs_internal.h"
+
+namespace tfs {
+
+// Offsets into the saddle-point vector
+//   [ lambda_s (3*nfil*nseg) | lambda_b (3*nblob) | phi_dot (nfil) | theta_dot (nfil) ].
+struct SaddleDims {
+    int nfil, nseg, nblob;
+    int dyn_phase, dyn_angle;
+    int64_t off_blob, off_phase, off_angle, n;
+};
+
+struct BeatArgs {
+    int nfil, nseg;
+    double length;            // filament length L = DL*(NSEG-1)
+    const double* frames;     // 9 per filament: e1 (normal), e2 (beat direction), e3
+    const double* bases;      // 3 per filament: position of segment 0
+    const double* s_values;   // nseg arclength parameters in [0, 1]
+    const double* phase;      // nfil
+    const double* angle;      // nfil or null
+};
+
+struct RhsArgs {
+    SaddleDims d;
+    const double* phase;
+    const double* angle;
+    const double* omega0;     // nfil natural frequencies
+    const double* gen_phase;  // reference generalised phase forces over one period
+    int n_gen_phase;
+    const double* gen_angle;  // reference generalised angle forces over one period
+    int n_gen_angle;
+    double gen_force_factor;  // GEN_FORCE_MAGNITUDE_FACTOR
+    double torsion_factor;    // TORSIONAL_SPRING_MAGNITUDE_FACTOR
+    double length;
+    const double* body_x;     // 3
+    const double* body_v;     // 6: V, Omega
+};
+
+hipError_t k_dot_cols(const double* V, int64_t ld, int ncols, const double* w, int n, double* partial, int G,
+                      hipStream_t s);
+hipError_t k_reduce_cols(const double* partial, int G, int ncols, double* h, double* Hcol, hipStream_t s);
+hipError_t k_gemv(const double* V, int64_t ld, int ncols, const double* h, double* w, int n, bool subtract,
+                  hipStream_t s);
+hipError_t k_norm(const double* w, int n, double* partial, int G, double* out, hipStream_t s);
+hipError_t k_givens(double* H, int ldh, int j, double* cs, double* sn, double* g, const double* wnorm,
+                    double* scal, hipStream_t s);
+hipError_t k_scale(double* v, const double* w, const double* scal, int idx, int n, hipStream_t s);
+hipError_t k_trisolve(const double* H, int ldh, const double* g, int k, double* y, hipStream_t s);
+hipError_t k_axpby(double a, const double* x, double b, double* y, int n, hipStream_t s);
+hipError_t k_sub(const double* x, const double* y, double* out, int n, hipStream_t s);
+hipError_t k_saddle(const double* in, double* out, const double* kphi, const double* kth, const SaddleDims& d,
+                    hipStream_t s);
+hipError_t k_precond(const double* in, double* out, const double* kphi, const double* kth, const SaddleDims& d,
+                     double gs, double gb, hipStream_t s);
+hipError_t k_beat_coeffs(const double* A, const double* B);
+hipError_t k_beat(const BeatArgs& a, double* x_segs, double* kphi, double* kth, hipStream_t s);
+hipError_t k_rhs(const RhsArgs& a, const double* x_segs, const double* x_blobs, double* rhs, hipStream_t s);
+hipError_t k_phase_update(const double* x, const SaddleDims& d, double dt, double* phase, double* angle,
+                          hipStream_t s);
+
+}  // namespace tfs

@@ -1,0 +1,479 @@
+// solver_kernels.hip -- device kernels of the per-timestep solve (gfx950).
+//
+// * Krylov vector kernels for GMRES with FP64 Arnoldi: classical Gram-Schmidt
+//   done twice (CGS2) as transposed/untransposed GEMVs over the basis V,
+//   block reductions with wavefront shuffles, Givens rotations and the
+//   triangular solve on one wave.
+// * The saddle-point couplings of the prescribed-cilia system
+//   (mobility_solver.cpp:1806-2090 system_matrix_mult_new, the K_phi / K_theta
+//   terms) and its right preconditioner (mobility_solver.cpp:1040-1513
+//   apply_preconditioner_new), one wave per filament.
+// * Prescribed-beat kinematics (filament.cpp:680-983 Fourier fit of the
+//   Fulford & Blake beat, filament.cpp:1083-1273 positions and phase/angle
+//   velocity directions), the generalised-force right-hand side
+//   (mobility_solver.cpp:240-280 cyclic cubic Hermite interpolation,
+//   :282-440 assemble_rhs) and the explicit phase update (filament.cpp:1087-1097).
+#include <hip/hip_runtime.h>
+
+#include "solver_kernels.h"
+
+namespace tfs {
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+    return v;  // valid in lane 0
+}
+
+// Sum over a 256-thread block; result valid in thread 0.
+__device__ __forceinline__ double block_sum256(double v, double* sh) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) s = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return s;
+}
+
+// partial[c*G + b] = sum over the rows of block b of V[r, c] * w[r], c < ncols.
+__global__ __launch_bounds__(256) void dot_cols_kernel(const double* __restrict__ V, int64_t ld, int ncols,
+                                                      const double* __restrict__ w, int n,
+                                                      double* __restrict__ partial) {
+    __shared__ double sh[4];
+    const int G = gridDim.x;
+    const int r0 = blockIdx.x * 256 + threadIdx.x;
+    // Each thread owns the rows r0, r0 + 256*G, ...
+    double wv[4];
+    int nr = 0;
+    for (int r = r0; r < n && nr < 4; r += 256 * G) wv[nr++] = w[r];
+    for (int c = 0; c < ncols; ++c) {
+        double s = 0.0;
+        int k = 0;
+        for (int r = r0; r < n && k < 4; r += 256 * G, ++k) s += V[(int64_t)c * ld + r] * wv[k];
+        s = block_sum256(s, sh);
+        if (threadIdx.x == 0) partial[(int64_t)c * G + blockIdx.x] = s;
+    }
+}
+
+// h[c] = sum_b partial[c*G + b]; H[c] += h[c] if H is given.  One block per column.
+__global__ __launch_bounds__(256) void reduce_cols_kernel(const double* __restrict__ partial, int G,
+                                                         double* __restrict__ h, double* __restrict__ Hcol) {
+    __shared__ double sh[4];
+    const int c = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < G; b += 256) s += partial[(int64_t)c * G + b];
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) {
+        h[c] = s;
+        if (Hcol) Hcol[c] += s;
+    }
+}
+
+// w[r] -= sum_c V[r, c] h[c]  (alpha = -1) or out[r] = sum_c V[r,c] h[c] (alpha = 0).
+__global__ __launch_bounds__(256) void gemv_kernel(const double* __restrict__ V, int64_t ld, int ncols,
+                                                  const double* __restrict__ h, double* __restrict__ w, int n,
+                                                  int subtract) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= n) return;
+    double s = 0.0;
+    for (int c = 0; c < ncols; ++c) s += V[(int64_t)c * ld + r] * h[c];
+    if (subtract) w[r] -= s;
+    else w[r] = s;
+}
+
+// partial[b] = sum over block b of w[r]^2 (grid-stride).
+__global__ __launch_bounds__(256) void sumsq_kernel(const double* __restrict__ w, int n,
+                                                   double* __restrict__ partial) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int r = blockIdx.x * 256 + threadIdx.x; r < n; r += 256 * gridDim.x) s += w[r] * w[r];
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// out[0] = sqrt(sum_b partial[b]); one block.
+__global__ __launch_bounds__(256) void norm_finish_kernel(const double* __restrict__ partial, int G,
+                                                         double* __restrict__ out) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < G; b += 256) s += partial[b];
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) out[0] = __builtin_sqrt(s);
+}
+
+// One Arnoldi step's bookkeeping (restates the rotation logic of
+// mobility_solver.cpp:2370-2419): H(j+1,j) = ||w||; apply the previous
+// rotations to column j; form the new rotation; update g; publish |g(j+1)|
+// and 1/||w|| for the basis normalisation.
+__global__ void givens_kernel(double* __restrict__ H, int ldh, int j, double* __restrict__ cs,
+                              double* __restrict__ sn, double* __restrict__ g,
+                              const double* __restrict__ wnorm, double* __restrict__ scal) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double* col = H + (int64_t)j * ldh;
+    const double hn = wnorm[0];
+    col[j + 1] = hn;
+    scal[1] = hn > 0.0 ? 1.0 / hn : 0.0;   // scale of the next basis vector
+    for (int i = 1; i <= j; ++i) {
+        const double t = cs[i - 1] * col[i - 1] + sn[i - 1] * col[i];
+        col[i] = cs[i - 1] * col[i] - sn[i - 1] * col[i - 1];
+        col[i - 1] = t;
+    }
+    double c, s;
+    const double a = col[j], b = col[j + 1];
+    if (b == 0.0) {
+        c = 1.0;
+        s = 0.0;
+    } else if (__builtin_fabs(b) > __builtin_fabs(a)) {
+        const double t = a / b;
+        s = (b > 0 ? 1.0 : -1.0) / __builtin_sqrt(1.0 + t * t);
+        c = t * s;
+    } else {
+        const double t = b / a;
+        c = (a > 0 ? 1.0 : -1.0) / __builtin_sqrt(1.0 + t * t);
+        s = t * c;
+    }
+    cs[j] = c;
+    sn[j] = s;
+    col[j] = c * a + s * b;
+    col[j + 1] = 0.0;
+    g[j + 1] = -s * g[j];
+    g[j] = c * g[j];
+    scal[0] = __builtin_fabs(g[j + 1]);   // residual norm estimate
+}
+
+// v = w * scal[idx]
+__global__ __launch_bounds__(256) void scale_kernel(double* __restrict__ v, const double* __restrict__ w,
+                                                   const double* __restrict__ scal, int idx, int n) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r < n) v[r] = w[r] * scal[idx];
+}
+
+// Back substitution H(0:k,0:k) y = g (mobility_solver.cpp:2421-2438).
+__global__ void trisolve_kernel(const double* __restrict__ H, int ldh, const double* __restrict__ g, int k,
+                                double* __restrict__ y) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int i = k - 1; i >= 0; --i) {
+        double s = g[i];
+        for (int j = i + 1; j < k; ++j) s -= H[(int64_t)j * ldh + i] * y[j];
+        y[i] = s / H[(int64_t)i * ldh + i];
+    }
+}
+
+// y = a*x + b*y
+__global__ __launch_bounds__(256) void axpby_kernel(double a, const double* __restrict__ x, double b,
+                                                   double* __restrict__ y, int n) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r < n) y[r] = a * x[r] + b * y[r];
+}
+
+// out = x - y
+__global__ __launch_bounds__(256) void sub_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                                                 double* __restrict__ out, int n) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r < n) out[r] = x[r] - y[r];
+}
+
+// ---------------------------------------------------------------------------
+// Saddle-point couplings, one wave per filament.
+// Vector layout (mobility_solver.cpp:107-140, prescribed body velocities):
+//   [ lambda_s (3*Ns, xyz per segment) | lambda_b (3*Nb) | phi_dot (Nfil) | theta_dot (Nfil) ]
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void saddle_k_kernel(const double* __restrict__ in, double* __restrict__ out,
+                                                      const double* __restrict__ kphi,
+                                                      const double* __restrict__ kth, SaddleDims d) {
+    const int m = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (m >= d.nfil) return;
+    const int len = 3 * d.nseg;
+    const int64_t seg0 = (int64_t)m * len;
+    const double pd = d.dyn_phase ? in[d.off_phase + m] : 0.0;
+    const double td = d.dyn_angle ? in[d.off_angle + m] : 0.0;
+    double sp = 0.0, st = 0.0;
+    for (int k = lane; k < len; k += kWave) {
+        const double kp = kphi[seg0 + k], kt = kth[seg0 + k];
+        const double lam = in[seg0 + k];
+        out[seg0 + k] -= pd * kp + td * kt;
+        sp += lam * kp;
+        st += lam * kt;
+    }
+    sp = wave_sum(sp);
+    st = wave_sum(st);
+    if (lane == 0) {
+        if (d.dyn_phase) out[d.off_phase + m] = -sp;
+        if (d.dyn_angle) out[d.off_angle + m] = -st;
+    }
+}
+
+// Right preconditioner P^{-1}: the exact inverse of [D -K; -K^T 0] with
+// D = I/(6 pi mu a) (mobility_solver.cpp:1040-1116).  Blob rows are scaled by
+// 6 pi mu a_b.  Filament waves first, then a grid-stride over blob entries.
+__global__ __launch_bounds__(256) void precond_kernel(const double* __restrict__ in, double* __restrict__ out,
+                                                     const double* __restrict__ kphi,
+                                                     const double* __restrict__ kth, SaddleDims d,
+                                                     double gs, double gb) {
+    const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int nwaves = gridDim.x * blockDim.x / kWave;
+    for (int m = gw; m < d.nfil; m += nwaves) {
+        const int len = 3 * d.nseg;
+        const int64_t seg0 = (int64_t)m * len;
+        double dp = 0.0, dt = 0.0, c11 = 0.0, c22 = 0.0, c12 = 0.0;
+        for (int k = lane; k < len; k += kWave) {
+            const double kp = kphi[seg0 + k], kt = kth[seg0 + k];
+            const double s = gs * in[seg0 + k];
+            dp += kp * s;
+            dt += kt * s;
+            c11 += kp * kp;
+            c22 += kt * kt;
+            c12 += kp * kt;
+        }
+        dp = wave_sum(dp);
+        dt = wave_sum(dt);
+        c11 = wave_sum(c11);
+        c22 = wave_sum(c22);
+        c12 = wave_sum(c12);
+        double p = 0.0, q = 0.0;
+        if (lane == 0) {
+            if (d.dyn_phase && d.dyn_angle) {
+                const double r1 = -in[d.off_phase + m] - dp;
+                const double r2 = -in[d.off_angle + m] - dt;
+                const double det = c11 * c22 - c12 * c12;
+                p = (c22 * r1 - c12 * r2) / det / gs;
+                q = (-c12 * r1 + c11 * r2) / det / gs;
+            } else if (d.dyn_phase) {
+                p = (-in[d.off_phase + m] - dp) / (c11 * gs);
+            } else if (d.dyn_angle) {
+                q = (-in[d.off_angle + m] - dt) / (c22 * gs);
+            }
+            if (d.dyn_phase) out[d.off_phase + m] = p;
+            if (d.dyn_angle) out[d.off_angle + m] = q;
+        }
+        p = __shfl(p, 0, kWave);
+        q = __shfl(q, 0, kWave);
+        for (int k = lane; k < len; k += kWave)
+            out[seg0 + k] = gs * (in[seg0 + k] + p * kphi[seg0 + k] + q * kth[seg0 + k]);
+    }
+    const int nb3 = 3 * d.nblob;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb3; i += gridDim.x * blockDim.x)
+        out[d.off_blob + i] = gb * in[d.off_blob + i];
+}
+
+// ---------------------------------------------------------------------------
+// Prescribed beat.
+// ---------------------------------------------------------------------------
+__constant__ double c_beat_A[2][4][3];  // [x|y][mode 0..3][power s, s^2, s^3]
+__constant__ double c_beat_B[2][3][3];  // [x|y][mode 1..3][power]
+
+// Local beat-plane shape (x, y) and its phase derivative at arclength
+// parameter s (filament.cpp:770-833).
+__device__ void beat_shape(double s, double phase, double* px, double* py, double* dx, double* dy) {
+    const double sv[3] = {s, s * s, s * s * s};
+    double cs[4], sn[4];
+    cs[0] = 1.0;
+    sn[0] = 0.0;
+    for (int n = 1; n < 4; ++n) {
+        cs[n] = __builtin_cos(n * phase);
+        sn[n] = __builtin_sin(n * phase);
+    }
+    double r[2] = {0, 0}, v[2] = {0, 0};
+    for (int ax = 0; ax < 2; ++ax) {
+        for (int p = 0; p < 3; ++p) {
+            double a = 0.0, da = 0.0;
+            for (int n = 0; n < 4; ++n) {
+                a += cs[n] * c_beat_A[ax][n][p];
+                da += -n * sn[n] * c_beat_A[ax][n][p];
+            }
+            for (int n = 1; n < 4; ++n) {
+                a += sn[n] * c_beat_B[ax][n - 1][p];
+                da += n * cs[n] * c_beat_B[ax][n - 1][p];
+            }
+            r[ax] += a * sv[p];
+            v[ax] += da * sv[p];
+        }
+    }
+    *px = r[0];
+    *py = r[1];
+    *dx = v[0];
+    *dy = v[1];
+}
+
+// One thread per segment: positions (into x_segs, 3 per segment), K_phi and
+// K_theta columns (3 per segment).
+__global__ __launch_bounds__(256) void beat_kernel(BeatArgs a, double* __restrict__ x_segs,
+                                                  double* __restrict__ kphi, double* __restrict__ kth) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nfil * a.nseg) return;
+    const int m = t / a.nseg, n = t - m * a.nseg;
+    const double* fr = a.frames + 9 * m;  // e1 (normal), e2 (beat direction), e3
+    const double* x0 = a.bases + 3 * m;
+    double pos[3] = {x0[0], x0[1], x0[2]}, kp[3] = {0, 0, 0}, kt[3] = {0, 0, 0};
+    if (n > 0) {
+        const double s = a.s_values[n];
+        double px, py, dx, dy;
+        beat_shape(s, a.phase[m], &px, &py, &dx, &dy);
+        const double th = a.angle ? a.angle[m] : 0.0;
+        const double ct = __builtin_cos(th), st = __builtin_sin(th);
+        // shape rotated about the local z axis by theta, scaled by the length
+        const double rx = a.length * (ct * px - st * py), ry = a.length * (st * px + ct * py);
+        const double vx = a.length * (ct * dx - st * dy), vy = a.length * (st * dx + ct * dy);
+        for (int c = 0; c < 3; ++c) {
+            pos[c] += rx * fr[c] + ry * fr[3 + c];
+            kp[c] = vx * fr[c] + vy * fr[3 + c];
+            // e_z x ref = (-ref_y, ref_x, 0) in the local frame
+            kt[c] = -ry * fr[c] + rx * fr[3 + c];
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        x_segs[3 * t + c] = pos[c];
+        kphi[3 * t + c] = kp[c];
+        kth[3 * t + c] = kt[c];
+    }
+}
+
+// Cyclic cubic Hermite interpolation of a table sampled uniformly over one
+// period (mobility_solver.cpp:240-280).
+__device__ double cyclic_hermite(double phase, const double* tab, int nt) {
+    const double PI = 3.14159265358979323846;
+    double u = 0.5 * phase / PI;
+    u -= __builtin_floor(u);
+    u *= nt;
+    int lo = (int)u;
+    if (lo >= nt) return tab[0];
+    const int hi = (lo == nt - 1) ? 0 : lo + 1;
+    const double ql = tab[lo];
+    const double dql = (lo == 0) ? (tab[1] - tab[0]) : 0.5 * (tab[hi] - tab[lo - 1]);
+    const double qh = tab[hi];
+    const double dqh = (hi == 0) ? (tab[0] - tab[nt - 1]) : 0.5 * (tab[(hi == nt - 1) ? 0 : hi + 1] - tab[lo]);
+    const double t = u - lo, t2 = t * t, t3 = t2 * t;
+    return ql * (2 * t3 - 3 * t2 + 1) + dql * (t3 - 2 * t2 + t) + qh * (3 * t2 - 2 * t3) + dqh * (t3 - t2);
+}
+
+// Right-hand side (mobility_solver.cpp:282-440 with PRESCRIBED_BODY_VELOCITIES):
+// minus the generalised driving forces in the phase/angle rows, and the rigid
+// body motion K V_body in the segment and blob rows.
+__global__ __launch_bounds__(256) void rhs_kernel(RhsArgs a, const double* __restrict__ x_segs,
+                                                 const double* __restrict__ x_blobs, double* __restrict__ rhs) {
+    const double PI = 3.14159265358979323846;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ns = a.d.nfil * a.d.nseg;
+    if (t < ns + a.d.nblob) {
+        const double* x = t < ns ? x_segs + 3 * t : x_blobs + 3 * (t - ns);
+        const double r0 = x[0] - a.body_x[0], r1 = x[1] - a.body_x[1], r2 = x[2] - a.body_x[2];
+        const double* V = a.body_v;  // (V, Omega)
+        double* o = t < ns ? rhs + 3 * t : rhs + a.d.off_blob + 3 * (t - ns);
+        o[0] = V[0] + V[4] * r2 - V[5] * r1;
+        o[1] = V[1] + V[5] * r0 - V[3] * r2;
+        o[2] = V[2] + V[3] * r1 - V[4] * r0;
+    }
+    if (t < a.d.nfil) {
+        const double w = a.omega0[t] * 0.5 / PI;
+        if (a.d.dyn_phase) {
+            const double q = cyclic_hermite(a.phase[t], a.gen_phase, a.n_gen_phase) * w * a.gen_force_factor;
+            rhs[a.d.off_phase + t] = -q;
+        }
+        if (a.d.dyn_angle) {
+            double q = cyclic_hermite(a.phase[t], a.gen_angle, a.n_gen_angle) * w;
+            const double L = a.length;
+            q -= a.torsion_factor * (a.omega0[t] * L * L * L) * a.angle[t];
+            rhs[a.d.off_angle + t] = -q;
+        }
+    }
+}
+
+// phase += phase_dot*dt, angle += angle_dot*dt (filament.cpp:1087-1097).
+__global__ void phase_update_kernel(const double* __restrict__ x, SaddleDims d, double dt, double* phase,
+                                    double* angle) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= d.nfil) return;
+    if (d.dyn_phase) phase[m] += x[d.off_phase + m] * dt;
+    if (d.dyn_angle) angle[m] += x[d.off_angle + m] * dt;
+}
+
+}  // namespace
+
+static inline int blocks_for(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
+
+hipError_t k_dot_cols(const double* V, int64_t ld, int ncols, const double* w, int n, double* partial, int G,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(dot_cols_kernel, dim3(G), dim3(256), 0, s, V, ld, ncols, w, n, partial);
+    return hipGetLastError();
+}
+hipError_t k_reduce_cols(const double* partial, int G, int ncols, double* h, double* Hcol, hipStream_t s) {
+    hipLaunchKernelGGL(reduce_cols_kernel, dim3(ncols), dim3(256), 0, s, partial, G, h, Hcol);
+    return hipGetLastError();
+}
+hipError_t k_gemv(const double* V, int64_t ld, int ncols, const double* h, double* w, int n, bool subtract,
+                  hipStream_t s) {
+    hipLaunchKernelGGL(gemv_kernel, dim3(blocks_for(n)), dim3(256), 0, s, V, ld, ncols, h, w, n, (int)subtract);
+    return hipGetLastError();
+}
+hipError_t k_norm(const double* w, int n, double* partial, int G, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(sumsq_kernel, dim3(G), dim3(256), 0, s, w, n, partial);
+    hipLaunchKernelGGL(norm_finish_kernel, dim3(1), dim3(256), 0, s, partial, G, out);
+    return hipGetLastError();
+}
+hipError_t k_givens(double* H, int ldh, int j, double* cs, double* sn, double* g, const double* wnorm,
+                    double* scal, hipStream_t s) {
+    hipLaunchKernelGGL(givens_kernel, dim3(1), dim3(64), 0, s, H, ldh, j, cs, sn, g, wnorm, scal);
+    return hipGetLastError();
+}
+hipError_t k_scale(double* v, const double* w, const double* scal, int idx, int n, hipStream_t s) {
+    hipLaunchKernelGGL(scale_kernel, dim3(blocks_for(n)), dim3(256), 0, s, v, w, scal, idx, n);
+    return hipGetLastError();
+}
+hipError_t k_trisolve(const double* H, int ldh, const double* g, int k, double* y, hipStream_t s) {
+    hipLaunchKernelGGL(trisolve_kernel, dim3(1), dim3(64), 0, s, H, ldh, g, k, y);
+    return hipGetLastError();
+}
+hipError_t k_axpby(double a, const double* x, double b, double* y, int n, hipStream_t s) {
+    hipLaunchKernelGGL(axpby_kernel, dim3(blocks_for(n)), dim3(256), 0, s, a, x, b, y, n);
+    return hipGetLastError();
+}
+hipError_t k_sub(const double* x, const double* y, double* out, int n, hipStream_t s) {
+    hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(n)), dim3(256), 0, s, x, y, out, n);
+    return hipGetLastError();
+}
+hipError_t k_saddle(const double* in, double* out, const double* kphi, const double* kth, const SaddleDims& d,
+                    hipStream_t s) {
+    if (d.nfil == 0 || !(d.dyn_phase || d.dyn_angle)) return hipSuccess;
+    hipLaunchKernelGGL(saddle_k_kernel, dim3(blocks_for((int64_t)d.nfil * kWave)), dim3(256), 0, s, in, out,
+                       kphi, kth, d);
+    return hipGetLastError();
+}
+hipError_t k_precond(const double* in, double* out, const double* kphi, const double* kth, const SaddleDims& d,
+                     double gs, double gb, hipStream_t s) {
+    const int64_t work = (int64_t)d.nfil * kWave > 3LL * d.nblob ? (int64_t)d.nfil * kWave : 3LL * d.nblob;
+    hipLaunchKernelGGL(precond_kernel, dim3(blocks_for(work)), dim3(256), 0, s, in, out, kphi, kth, d, gs, gb);
+    return hipGetLastError();
+}
+hipError_t k_beat_coeffs(const double* A, const double* B) {
+    TFS_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_beat_A), A, sizeof(double) * 24));
+    TFS_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_beat_B), B, sizeof(double) * 18));
+    return hipSuccess;
+}
+hipError_t k_beat(const BeatArgs& a, double* x_segs, double* kphi, double* kth, hipStream_t s) {
+    const int n = a.nfil * a.nseg;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(beat_kernel, dim3(blocks_for(n)), dim3(256), 0, s, a, x_segs, kphi, kth);
+    return hipGetLastError();
+}
+hipError_t k_rhs(const RhsArgs& a, const double* x_segs, const double* x_blobs, double* rhs, hipStream_t s) {
+    const int n = a.d.nfil * a.d.nseg + a.d.nblob;
+    const int m = n > a.d.nfil ? n : a.d.nfil;
+    if (m == 0) return hipSuccess;
+    hipLaunchKernelGGL(rhs_kernel, dim3(blocks_for(m)), dim3(256), 0, s, a, x_segs, x_blobs, rhs);
+    return hipGetLastError();
+}
+hipError_t k_phase_update(const double* x, const SaddleDims& d, double dt, double* phase, double* angle,
+                          hipStream_t s) {
+    if (d.nfil == 0) return hipSuccess;
+    hipLaunchKernelGGL(phase_update_kernel, dim3(blocks_for(d.nfil)), dim3(256), 0, s, x, d, dt, phase, angle);
+    return hipGetLastError();
+}
+
+}  // namespace tfs

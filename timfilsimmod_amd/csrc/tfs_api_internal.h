@@ -1,0 +1,61 @@
+// tfs_api_internal.h -- the handle behind the C-ABI (not part of the C-ABI).
+#pragma once
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/tfs_rpy.h"
+#include "tfs_internal.h"
+
+namespace tfs {
+
+// A cached work list for one block mask and launch shape.
+struct Plan {
+    std::vector<MatvecTile> tiles;
+    std::vector<MatvecItem> items;
+    MatvecTile* d_tiles = nullptr;
+    MatvecItem* d_items = nullptr;
+};
+
+}  // namespace tfs
+
+struct tfs_rpy {
+    tfs_rpy_desc desc{};
+    int device = 0;
+    int num_cus = 256;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    double* arena = nullptr;
+    // Reference-layout (AoS) device arrays, carved out of `arena`.
+    double* x_segs = nullptr;
+    double* f_segs = nullptr;
+    double* x_blobs = nullptr;
+    double* f_blobs = nullptr;
+    double* v_segs = nullptr;
+    double* v_blobs = nullptr;
+    // Partial sums of the work items (grown on demand).
+    double* partial = nullptr;
+    size_t partial_len = 0;
+    tfs::TileShape shape[2];
+    // Live kernel timing: event pairs recorded around each pair-kernel launch.
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used, ev_free;
+    std::map<std::tuple<uint32_t, int, int, int>, tfs::Plan> plans;
+};
+
+namespace tfs {
+
+extern thread_local std::string g_last_error;
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* where);
+TileShape default_shape(int precision, int ns, int nb, int num_cus);
+Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err);
+
+// Matvec with explicit views: forces fs (seg_dof comps) / fb (3 comps) and
+// outputs vs (seg_out_slots comps) / vb (3 comps), with the write rules of
+// `blocks` and the handle's positions.
+int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, View vs, View vb,
+               int seg_out_slots);
+
+}  // namespace tfs

@@ -1,0 +1,104 @@
+// tfs_internal.h -- shared declarations of the HIP library (not part of the C-ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace tfs {
+
+// Strided view of an n x c array: element (particle i, component k) lives at
+// p[i*ld_p + k*ld_c].  AoS with 6 slots: ld_p = 6, ld_c = 1.  SoA: ld_p = 1,
+// ld_c = n.
+struct CView {
+    const double* p;
+    int64_t ld_p;
+    int64_t ld_c;
+};
+struct View {
+    double* p;
+    int64_t ld_p;
+    int64_t ld_c;
+};
+
+// One unit of matvec work: the pairs between the targets of `tile` and the
+// sources [s_begin, s_end) of class src_cls (0 segments, 1 blobs).  Its
+// partial sums go to partial slot = the item's index.
+struct MatvecItem {
+    int32_t tile;
+    int32_t src_cls;
+    int32_t s_begin;
+    int32_t s_end;
+};
+
+// A tile of targets [t_begin, t_end) of class cls; its partial sums live in
+// slots [first_item, first_item + n_items).
+struct MatvecTile {
+    int32_t cls;
+    int32_t t_begin;
+    int32_t t_end;
+    int32_t first_item;
+    int32_t n_items;
+    int32_t write_mode;  // 0 overwrite, 1 accumulate into the output
+};
+
+// Launch shape of the pair kernel; chosen by the autotuner.
+struct TileShape {
+    int block;       // threads per workgroup (64, 128, 256)
+    int per_thread;  // targets per thread (1 or 2)
+    int chunk;       // sources per work item (a multiple of block)
+};
+
+// Everything the pair kernel needs besides the work list.
+struct MatvecArgs {
+    CView xs, xb;      // positions (3 components)
+    CView fs, fb;      // source strengths (seg_dof or 3 components)
+    int ns, nb;
+    double a_seg, a_blob, mu;
+};
+
+struct OutArgs {
+    View vs, vb;       // outputs; vs has seg_out_slots components
+    int seg_out_slots; // 6 in the reference layout, 3 for a packed 3-dof vector
+};
+
+// Builds the work list for one evaluation.  blocks: bit 0 SS, 1 SB, 2 BB, 3 BS.
+// seg_write / blob_write: -1 no write, 0 overwrite, 1 accumulate.
+void build_worklist(int ns, int nb, int tile_targets, int chunk, bool ss, bool sb, bool bb,
+                    bool bs, int seg_write, int blob_write, std::vector<MatvecTile>& tiles,
+                    std::vector<MatvecItem>& items);
+
+// Launches pair kernel + reduce on `stream`.  d_tiles/d_items/partial are device
+// buffers sized for the work list; precision 0 = fp64, 1 = fp32 tiles with fp64
+// accumulation.
+// ev_begin/ev_end (may be null) are recorded around the pair kernel only.
+hipError_t launch_matvec(int precision, int seg_dof, const TileShape& shape, const MatvecArgs& a,
+                         const OutArgs& o, const MatvecTile* d_tiles, int n_tiles,
+                         const MatvecItem* d_items, int n_items, double* partial,
+                         hipStream_t stream, hipEvent_t ev_begin = nullptr,
+                         hipEvent_t ev_end = nullptr);
+
+// Flow-field evaluation (Stokeslet + rotlet mode and RPY mode).
+struct FlowArgs {
+    CView xt;          // targets (3 components)
+    View vt;           // velocities out
+    int nt;
+    CView xs, fs;      // segment sources (6 components)
+    CView xb, fb;      // blob sources (3 components)
+    int ns_total, nb_total;
+    int ns_per_body, nb_per_body;
+    const double* x_bodies;  // 3 per body (AoS), for net-force removal
+    int remove_net_force;
+    int mode;          // 0 Stokeslet+rotlet (+Blakelet without blobs), 1 RPY
+    double a_seg, a_blob, mu;
+};
+hipError_t launch_flow(int precision, const FlowArgs& a, int block, hipStream_t stream);
+
+}  // namespace tfs
+
+#define TFS_HIP_CHECK(expr)                                 \
+    do {                                                    \
+        hipError_t _e = (expr);                             \
+        if (_e != hipSuccess) return _e;                    \
+    } while (0)

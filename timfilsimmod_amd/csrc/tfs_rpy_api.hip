@@ -1,0 +1,395 @@
+// tfs_rpy_api.hip -- the C-ABI of include/tfs_rpy.h over the gfx950 kernels.
+//
+// Mirrors the data ownership of rpy_mobility_solver
+// (src/cilia/mobility/rpy_mobility_solver.cu:54-117): the handle owns one set
+// of device arrays in the reference's AoS layouts (positions 3, forces 6/3,
+// velocities 6/3 per particle).  Unlike the reference there is no per-GPU
+// target split and no host staging between kernels: the four blocks run as one
+// fused launch (plus a reduce) on the handle's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "tfs_api_internal.h"
+
+namespace tfs {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+    return fail(TFS_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+TileShape default_shape(int precision, int ns, int nb, int num_cus) {
+    TileShape s;
+    s.block = 256;
+    s.per_thread = 1;
+    (void)precision;
+    const int tt = s.block * s.per_thread;
+    const long n_tiles = (ns + tt - 1) / tt + (nb + tt - 1) / tt;
+    const long n_src = (long)ns + nb;
+    // Aim for ~8 workgroups per CU over the whole work list.
+    const long want_items = 8L * (num_cus > 0 ? num_cus : 256);
+    long chunk = n_tiles > 0 ? (n_src * n_tiles + want_items - 1) / want_items : n_src;
+    chunk = ((chunk + s.block - 1) / s.block) * s.block;
+    if (chunk < s.block) chunk = s.block;
+    s.chunk = (int)chunk;
+    return s;
+}
+
+Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err) {
+    const bool ss = blocks & TFS_BLOCK_SS, sb = blocks & TFS_BLOCK_SB;
+    bool bb = blocks & TFS_BLOCK_BB;
+    const bool bs = blocks & TFS_BLOCK_BS;
+    // Sequential semantics: with an overwriting BS, an earlier BB is overwritten.
+    if (bs && bb && h->desc.bs_mode == TFS_BS_OVERWRITE) bb = false;
+    const int seg_write = ss ? 0 : (sb ? 1 : -1);
+    int blob_write = -1;
+    if (bb) blob_write = 0;
+    else if (bs) blob_write = h->desc.bs_mode == TFS_BS_OVERWRITE ? 0 : 1;
+    const auto key = std::make_tuple((uint32_t)((ss << 0) | (sb << 1) | (bb << 2) | (bs << 3)),
+                                     sh.block, sh.per_thread, sh.chunk);
+    auto it = h->plans.find(key);
+    if (it != h->plans.end()) return &it->second;
+    Plan p;
+    build_worklist(h->desc.num_segs, h->desc.num_blobs, sh.block * sh.per_thread, sh.chunk, ss, sb, bb,
+                   bs, seg_write, blob_write, p.tiles, p.items);
+    if (!p.tiles.empty()) {
+        hipError_t e = hipMalloc(&p.d_tiles, sizeof(MatvecTile) * p.tiles.size());
+        if (e == hipSuccess && !p.items.empty())
+            e = hipMalloc(&p.d_items, sizeof(MatvecItem) * p.items.size());
+        if (e == hipSuccess)
+            e = hipMemcpy(p.d_tiles, p.tiles.data(), sizeof(MatvecTile) * p.tiles.size(),
+                          hipMemcpyHostToDevice);
+        if (e == hipSuccess && !p.items.empty())
+            e = hipMemcpy(p.d_items, p.items.data(), sizeof(MatvecItem) * p.items.size(),
+                          hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            hipFree(p.d_tiles);
+            hipFree(p.d_items);
+            *err = hip_fail(e, "work list upload");
+            return nullptr;
+        }
+        const size_t need = p.items.size() * 6 * (size_t)(sh.block * sh.per_thread);
+        if (need > h->partial_len) {
+            if (h->partial) hipFree(h->partial);
+            h->partial = nullptr;
+            h->partial_len = 0;
+            e = hipMalloc(&h->partial, need * sizeof(double));
+            if (e != hipSuccess) {
+                *err = hip_fail(e, "partial-sum buffer");
+                return nullptr;
+            }
+            h->partial_len = need;
+        }
+    }
+    auto res = h->plans.emplace(key, std::move(p));
+    return &res.first->second;
+}
+
+int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, View vs, View vb,
+               int seg_out_slots) {
+    const TileShape& sh = h->shape[precision];
+    int err = TFS_OK;
+    Plan* p = get_plan(h, blocks, sh, &err);
+    if (!p) return err;
+    if (p->tiles.empty()) return TFS_OK;
+    MatvecArgs a;
+    a.xs = {h->x_segs, 3, 1};
+    a.xb = {h->x_blobs, 3, 1};
+    a.fs = fs;
+    a.fb = fb;
+    a.ns = h->desc.num_segs;
+    a.nb = h->desc.num_blobs;
+    a.a_seg = h->desc.seg_radius;
+    a.a_blob = h->desc.blob_radius;
+    a.mu = h->desc.mu;
+    OutArgs o;
+    o.vs = vs;
+    o.vb = vb;
+    o.seg_out_slots = seg_out_slots;
+    hipEvent_t eb = nullptr, ee = nullptr;
+    if (h->profiling && !p->items.empty()) {
+        if (h->ev_free.empty()) {
+            hipEvent_t a0, a1;
+            hipError_t ce = hipEventCreate(&a0);
+            if (ce == hipSuccess) ce = hipEventCreate(&a1);
+            if (ce != hipSuccess) return hip_fail(ce, "profiling events");
+            h->ev_free.push_back({a0, a1});
+        }
+        eb = h->ev_free.back().first;
+        ee = h->ev_free.back().second;
+        h->ev_used.push_back(h->ev_free.back());
+        h->ev_free.pop_back();
+    }
+    hipError_t e = launch_matvec(precision, h->desc.seg_dof, sh, a, o, p->d_tiles, (int)p->tiles.size(),
+                                 p->d_items, (int)p->items.size(), h->partial, h->stream, eb, ee);
+    if (e != hipSuccess) return hip_fail(e, "matvec launch");
+    return TFS_OK;
+}
+
+}  // namespace tfs
+
+using namespace tfs;
+
+extern "C" {
+
+const char* tfs_last_error(void) { return g_last_error.c_str(); }
+
+const char* tfs_version(void) { return "tfs-rpy-mi355x 0.1 (gfx950)"; }
+
+int tfs_rpy_create(const tfs_rpy_desc* desc, tfs_rpy** out) {
+    g_last_error.clear();
+    if (!desc || !out) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_create: null argument");
+    *out = nullptr;
+    if (desc->num_segs < 0 || desc->num_blobs < 0)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_create: negative particle count");
+    if (desc->seg_dof != 3 && desc->seg_dof != 6)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_create: seg_dof must be 3 or 6");
+    if (desc->precision != TFS_PRECISION_FP64 && desc->precision != TFS_PRECISION_FP32_ACC64)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_create: unknown precision");
+    if (desc->bs_mode != TFS_BS_OVERWRITE && desc->bs_mode != TFS_BS_ACCUMULATE)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_create: unknown bs_mode");
+    if (desc->memory != TFS_MEMORY_HOST && desc->memory != TFS_MEMORY_DEVICE)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_create: unknown memory space");
+    if (!(desc->seg_radius > 0) || !(desc->blob_radius > 0) || !(desc->mu > 0))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_create: radii and mu must be positive");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(TFS_ERR_NO_DEVICE, "tfs_rpy_create: no HIP device visible");
+    tfs_rpy* h = new tfs_rpy();
+    h->desc = *desc;
+    hipGetDevice(&h->device);
+    hipDeviceProp_t prop;
+    int cus = 256;
+    if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) cus = prop.multiProcessorCount;
+    h->num_cus = cus;
+    if (desc->stream) {
+        h->stream = (hipStream_t)desc->stream;
+        h->own_stream = false;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete h;
+            return hip_fail(e, "tfs_rpy_create: stream");
+        }
+        h->own_stream = true;
+    }
+    const size_t ns = desc->num_segs, nb = desc->num_blobs;
+    // One allocation for the six AoS arrays (at least one element each).
+    const size_t lens[6] = {3 * ns, 6 * ns, 3 * nb, 3 * nb, 6 * ns, 3 * nb};
+    size_t total = 0;
+    for (size_t l : lens) total += std::max<size_t>(l, 1) + 32;  // keep 256-B alignment
+    hipError_t e = hipMalloc(&h->arena, total * sizeof(double));
+    if (e != hipSuccess) {
+        tfs_rpy_destroy(h);
+        return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_rpy_create: device arrays");
+    }
+    double* p = h->arena;
+    double** dst[6] = {&h->x_segs, &h->f_segs, &h->x_blobs, &h->f_blobs, &h->v_segs, &h->v_blobs};
+    for (int i = 0; i < 6; ++i) {
+        *dst[i] = p;
+        p += ((std::max<size_t>(lens[i], 1) + 31) / 32) * 32;
+    }
+    e = hipMemsetAsync(h->arena, 0, total * sizeof(double), h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        tfs_rpy_destroy(h);
+        return hip_fail(e, "tfs_rpy_create: clear");
+    }
+    for (int pr = 0; pr < 2; ++pr) h->shape[pr] = default_shape(pr, desc->num_segs, desc->num_blobs, cus);
+    *out = h;
+    return TFS_OK;
+}
+
+int tfs_rpy_destroy(tfs_rpy* h) {
+    if (!h) return TFS_OK;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (auto& kv : h->plans) {
+        hipFree(kv.second.d_tiles);
+        hipFree(kv.second.d_items);
+    }
+    for (auto* v : {&h->ev_used, &h->ev_free})
+        for (auto& pr : *v) {
+            hipEventDestroy(pr.first);
+            hipEventDestroy(pr.second);
+        }
+    if (h->partial) hipFree(h->partial);
+    if (h->arena) hipFree(h->arena);
+    if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return TFS_OK;
+}
+
+static int array_info(tfs_rpy* h, int32_t what, double** dev, size_t* len) {
+    const size_t ns = h->desc.num_segs, nb = h->desc.num_blobs;
+    switch (what) {
+        case TFS_SEG_POSITIONS: *dev = h->x_segs; *len = 3 * ns; return TFS_OK;
+        case TFS_SEG_FORCES: *dev = h->f_segs; *len = 6 * ns; return TFS_OK;
+        case TFS_BLOB_POSITIONS: *dev = h->x_blobs; *len = 3 * nb; return TFS_OK;
+        case TFS_BLOB_FORCES: *dev = h->f_blobs; *len = 3 * nb; return TFS_OK;
+        case TFS_SEG_VELOCITIES: *dev = h->v_segs; *len = 6 * ns; return TFS_OK;
+        case TFS_BLOB_VELOCITIES: *dev = h->v_blobs; *len = 3 * nb; return TFS_OK;
+        default: return fail(TFS_ERR_INVALID_ARGUMENT, "unknown array id");
+    }
+}
+
+int tfs_rpy_upload(tfs_rpy* h, int32_t what, const double* src) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_upload: null handle");
+    if (what > TFS_BLOB_FORCES && what != TFS_SEG_VELOCITIES && what != TFS_BLOB_VELOCITIES)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_upload: not an input array");
+    double* dev;
+    size_t len;
+    int rc = array_info(h, what, &dev, &len);
+    if (rc) return rc;
+    if (len == 0) return TFS_OK;
+    if (!src) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_upload: null source");
+    hipError_t e = hipMemcpyAsync(dev, src, len * sizeof(double),
+                                  h->desc.memory == TFS_MEMORY_HOST ? hipMemcpyHostToDevice
+                                                                    : hipMemcpyDeviceToDevice,
+                                  h->stream);
+    return e == hipSuccess ? TFS_OK : hip_fail(e, "tfs_rpy_upload");
+}
+
+int tfs_rpy_download(tfs_rpy* h, int32_t what, double* dst) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_download: null handle");
+    if (what != TFS_SEG_VELOCITIES && what != TFS_BLOB_VELOCITIES)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_download: not an output array");
+    double* dev;
+    size_t len;
+    int rc = array_info(h, what, &dev, &len);
+    if (rc) return rc;
+    if (len == 0) return TFS_OK;
+    if (!dst) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_download: null destination");
+    hipError_t e = hipMemcpyAsync(dst, dev, len * sizeof(double),
+                                  h->desc.memory == TFS_MEMORY_HOST ? hipMemcpyDeviceToHost
+                                                                    : hipMemcpyDeviceToDevice,
+                                  h->stream);
+    return e == hipSuccess ? TFS_OK : hip_fail(e, "tfs_rpy_download");
+}
+
+int tfs_rpy_evaluate(tfs_rpy* h, uint32_t blocks) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_evaluate: null handle");
+    if (blocks & ~TFS_BLOCK_ALL) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_evaluate: bad block mask");
+    return run_matvec(h, blocks, h->desc.precision, CView{h->f_segs, 6, 1}, CView{h->f_blobs, 3, 1},
+                      View{h->v_segs, 6, 1}, View{h->v_blobs, 3, 1}, 6);
+}
+
+int tfs_rpy_apply(tfs_rpy* h, uint32_t blocks, const double* f_segs, const double* f_blobs,
+                  double* v_segs, double* v_blobs) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_apply: null handle");
+    if (blocks & ~TFS_BLOCK_ALL) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_apply: bad block mask");
+    const bool need_fs = blocks & (TFS_BLOCK_SS | TFS_BLOCK_BS);
+    const bool need_fb = blocks & (TFS_BLOCK_SB | TFS_BLOCK_BB);
+    const bool out_s = blocks & (TFS_BLOCK_SS | TFS_BLOCK_SB);
+    const bool out_b = blocks & (TFS_BLOCK_BB | TFS_BLOCK_BS);
+    const bool ns = h->desc.num_segs > 0, nb = h->desc.num_blobs > 0;
+    if ((need_fs && ns && !f_segs) || (need_fb && nb && !f_blobs) || (out_s && ns && !v_segs) ||
+        (out_b && nb && !v_blobs))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_apply: missing array for requested block");
+    if (h->desc.memory == TFS_MEMORY_DEVICE) {
+        return run_matvec(h, blocks, h->desc.precision, CView{f_segs ? f_segs : h->f_segs, 6, 1},
+                          CView{f_blobs ? f_blobs : h->f_blobs, 3, 1}, View{v_segs ? v_segs : h->v_segs, 6, 1},
+                          View{v_blobs ? v_blobs : h->v_blobs, 3, 1}, 6);
+    }
+    int rc = TFS_OK;
+    if (need_fs && ns) rc = tfs_rpy_upload(h, TFS_SEG_FORCES, f_segs);
+    if (!rc && need_fb && nb) rc = tfs_rpy_upload(h, TFS_BLOB_FORCES, f_blobs);
+    // Accumulating blocks read the caller's current velocities.
+    const bool acc_s = (blocks & TFS_BLOCK_SB) && !(blocks & TFS_BLOCK_SS);
+    const bool acc_b = (blocks & TFS_BLOCK_BS) && !(blocks & TFS_BLOCK_BB) &&
+                       h->desc.bs_mode == TFS_BS_ACCUMULATE;
+    if (!rc && acc_s && ns) rc = tfs_rpy_upload(h, TFS_SEG_VELOCITIES, v_segs);
+    if (!rc && acc_b && nb) rc = tfs_rpy_upload(h, TFS_BLOB_VELOCITIES, v_blobs);
+    if (!rc) rc = tfs_rpy_evaluate(h, blocks);
+    if (!rc && out_s && ns) rc = tfs_rpy_download(h, TFS_SEG_VELOCITIES, v_segs);
+    if (!rc && out_b && nb) rc = tfs_rpy_download(h, TFS_BLOB_VELOCITIES, v_blobs);
+    return rc;
+}
+
+int tfs_rpy_sync(tfs_rpy* h) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_sync: null handle");
+    hipError_t e = hipStreamSynchronize(h->stream);
+    return e == hipSuccess ? TFS_OK : hip_fail(e, "tfs_rpy_sync");
+}
+
+int tfs_rpy_set_precision(tfs_rpy* h, int32_t precision) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_precision: null handle");
+    if (precision != TFS_PRECISION_FP64 && precision != TFS_PRECISION_FP32_ACC64)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_precision: unknown precision");
+    h->desc.precision = precision;
+    return TFS_OK;
+}
+
+int tfs_rpy_set_tile_shape(tfs_rpy* h, int32_t precision, int32_t block, int32_t per_thread,
+                           int32_t chunk) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_tile_shape: null handle");
+    if (precision != 0 && precision != 1)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_tile_shape: unknown precision");
+    if (block != 64 && block != 128 && block != 256)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_tile_shape: block must be 64, 128 or 256");
+    if (per_thread != 1 && per_thread != 2)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_tile_shape: per_thread must be 1 or 2");
+    if (chunk < block || chunk % block)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_tile_shape: chunk must be a positive multiple of block");
+    h->shape[precision] = TileShape{block, per_thread, chunk};
+    return TFS_OK;
+}
+
+int tfs_rpy_profile(tfs_rpy* h, int32_t enable) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_profile: null handle");
+    h->profiling = enable != 0;
+    return TFS_OK;
+}
+
+int tfs_rpy_profile_read(tfs_rpy* h, double* pair_ms_total, int32_t* launches) {
+    if (!h || !pair_ms_total || !launches)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_profile_read: null argument");
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(e, "tfs_rpy_profile_read");
+    double total = 0.0;
+    for (auto& pr : h->ev_used) {
+        float ms = 0.0f;
+        e = hipEventElapsedTime(&ms, pr.first, pr.second);
+        if (e != hipSuccess) return hip_fail(e, "tfs_rpy_profile_read: elapsed");
+        total += ms;
+        h->ev_free.push_back(pr);
+    }
+    *pair_ms_total = total;
+    *launches = (int32_t)h->ev_used.size();
+    h->ev_used.clear();
+    return TFS_OK;
+}
+
+int tfs_host_alloc(size_t bytes, void** out) {
+    if (!out) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_host_alloc: null argument");
+    *out = nullptr;
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipHostMalloc(out, bytes, hipHostMallocPortable);
+    return e == hipSuccess ? TFS_OK : hip_fail(e, "tfs_host_alloc");
+}
+
+int tfs_host_free(void* p) {
+    if (!p) return TFS_OK;
+    hipError_t e = hipHostFree(p);
+    return e == hipSuccess ? TFS_OK : hip_fail(e, "tfs_host_free");
+}
+
+int tfs_rpy_device_buffer(tfs_rpy* h, int32_t what, double** out) {
+    if (!h || !out) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_device_buffer: null argument");
+    size_t len;
+    return array_info(h, what, out, &len);
+}
+
+}  // extern "C"
