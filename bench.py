@@ -102,7 +102,11 @@ def main():
     from timfilsimmod_amd.mobility import RpyHandle
 
     torch.cuda.set_device(0)
-    stream = torch.cuda.current_stream().cuda_stream
+    # A dedicated (default-priority) stream: the library's kernels and torch's
+    # timing events go on the same queue.
+    tstream = torch.cuda.Stream()
+    torch.cuda.set_stream(tstream)
+    stream = tstream.cuda_stream
     body = G.production_body()
     ns, nb = body.num_segs, body.num_blobs
 

@@ -47,14 +47,10 @@ __global__ __launch_bounds__(256) void dot_cols_kernel(const double* __restrict_
     __shared__ double sh[4];
     const int G = gridDim.x;
     const int r0 = blockIdx.x * 256 + threadIdx.x;
-    // Each thread owns the rows r0, r0 + 256*G, ...
-    double wv[4];
-    int nr = 0;
-    for (int r = r0; r < n && nr < 4; r += 256 * G) wv[nr++] = w[r];
     for (int c = 0; c < ncols; ++c) {
+        const double* v = V + (int64_t)c * ld;
         double s = 0.0;
-        int k = 0;
-        for (int r = r0; r < n && k < 4; r += 256 * G, ++k) s += V[(int64_t)c * ld + r] * wv[k];
+        for (int r = r0; r < n; r += 256 * G) s += v[r] * w[r];
         s = block_sum256(s, sh);
         if (threadIdx.x == 0) partial[(int64_t)c * G + blockIdx.x] = s;
     }
@@ -168,7 +164,7 @@ __global__ void trisolve_kernel(const double* __restrict__ H, int ldh, const dou
 __global__ __launch_bounds__(256) void axpby_kernel(double a, const double* __restrict__ x, double b,
                                                    double* __restrict__ y, int n) {
     const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r < n) y[r] = a * x[r] + b * y[r];
+    if (r < n) y[r] = b == 0.0 ? a * x[r] : a * x[r] + b * y[r];
 }
 
 // out = x - y
@@ -277,8 +273,8 @@ __device__ void beat_shape(double s, double phase, double* px, double* py, doubl
     cs[0] = 1.0;
     sn[0] = 0.0;
     for (int n = 1; n < 4; ++n) {
-        cs[n] = __builtin_cos(n * phase);
-        sn[n] = __builtin_sin(n * phase);
+        cs[n] = cos(n * phase);
+        sn[n] = sin(n * phase);
     }
     double r[2] = {0, 0}, v[2] = {0, 0};
     for (int ax = 0; ax < 2; ++ax) {
@@ -317,7 +313,7 @@ __global__ __launch_bounds__(256) void beat_kernel(BeatArgs a, double* __restric
         double px, py, dx, dy;
         beat_shape(s, a.phase[m], &px, &py, &dx, &dy);
         const double th = a.angle ? a.angle[m] : 0.0;
-        const double ct = __builtin_cos(th), st = __builtin_sin(th);
+        const double ct = cos(th), st = sin(th);
         // shape rotated about the local z axis by theta, scaled by the length
         const double rx = a.length * (ct * px - st * py), ry = a.length * (st * px + ct * py);
         const double vx = a.length * (ct * dx - st * dy), vy = a.length * (st * dx + ct * dy);
@@ -342,6 +338,7 @@ __device__ double cyclic_hermite(double phase, const double* tab, int nt) {
     double u = 0.5 * phase / PI;
     u -= __builtin_floor(u);
     u *= nt;
+    if (!(u >= 0.0)) u = 0.0;  // a non-finite phase must not index the table
     int lo = (int)u;
     if (lo >= nt) return tab[0];
     const int hi = (lo == nt - 1) ? 0 : lo + 1;

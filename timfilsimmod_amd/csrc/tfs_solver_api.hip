@@ -1,0 +1,627 @@
+// tfs_solver_api.hip -- device-resident per-timestep solve behind include/tfs_solver.h.
+//
+// The reference solves its saddle-point system on the host: every GMRES
+// iteration copies the Krylov vector to the GPU, runs the four RPY kernels,
+// copies velocities back and does modified Gram-Schmidt on host columns
+// (mobility_solver.cpp:1806-2090, 2335-2419).  Here every vector lives in HBM
+// and only one scalar -- the residual estimate -- crosses to the host per
+// iteration.  Arnoldi stays in FP64 (CGS2 as GEMVs over the basis); the RPY
+// matvec runs in FP64 tiles or, inside mixed-precision iterative refinement,
+// in FP32 tiles with FP64 accumulation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tfs_solver.h"
+#include "solver_kernels.h"
+#include "tfs_api_internal.h"
+
+using namespace tfs;
+
+namespace {
+
+constexpr int kMaxRestart = 350;
+
+__global__ void fill_kernel(double* p, double v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t k_fill(double* p, double v, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+struct tfs_solver {
+    tfs_solver_desc desc{};
+    SaddleDims dims{};
+    tfs_rpy* rpy = nullptr;       // seg_dof 3, Mbs accumulating (prescribed cilia)
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int m = 50;                   // restart length
+    int G = 256;                  // reduction blocks
+    // device memory
+    double* arena = nullptr;
+    double *kphi = nullptr, *kth = nullptr;
+    double *b = nullptr, *x = nullptr, *r = nullptr, *w = nullptr, *z = nullptr, *t = nullptr, *d = nullptr,
+           *rin = nullptr;
+    double* V = nullptr;          // n x (m+1), column-major, ld = ldv
+    int64_t ldv = 0;
+    double *H = nullptr, *cs = nullptr, *sn = nullptr, *g = nullptr, *y = nullptr, *h = nullptr;
+    double *partial = nullptr, *scal = nullptr, *nrm = nullptr;
+    double* host_scal = nullptr;  // pinned
+    double gs = 0, gb = 0;        // 6 pi mu a for segments / blobs
+};
+
+namespace {
+
+int check(hipError_t e, const char* where) { return e == hipSuccess ? TFS_OK : hip_fail(e, where); }
+
+// Read one device double (synchronises the stream).
+int read_scalar(tfs_solver* s, const double* dev, double* out) {
+    hipError_t e = hipMemcpyAsync(s->host_scal, dev, sizeof(double), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return hip_fail(e, "read_scalar");
+    *out = *s->host_scal;
+    return TFS_OK;
+}
+
+int norm(tfs_solver* s, const double* v, double* out) {
+    int rc = check(k_norm(v, (int)s->dims.n, s->partial, s->G, s->nrm, s->stream), "norm");
+    if (rc) return rc;
+    return read_scalar(s, s->nrm, out);
+}
+
+// out = A in.
+int apply_A(tfs_solver* s, const double* in, double* out, int precision, tfs_solve_stats* st) {
+    const SaddleDims& dm = s->dims;
+    int rc = run_matvec(s->rpy, TFS_BLOCK_ALL, precision, CView{in, 3, 1}, CView{in + dm.off_blob, 3, 1},
+                        View{out, 3, 1}, View{out + dm.off_blob, 3, 1}, 3);
+    if (rc) return rc;
+    if (st) {
+        if (precision == TFS_PRECISION_FP64) st->fp64_matvecs++;
+        else st->fp32_matvecs++;
+    }
+    return check(k_saddle(in, out, s->kphi, s->kth, dm, s->stream), "saddle couplings");
+}
+
+int apply_P(tfs_solver* s, const double* in, double* out) {
+    return check(k_precond(in, out, s->kphi, s->kth, s->dims, s->gs, s->gb, s->stream), "preconditioner");
+}
+
+// r = b - A x (x may be known to be zero: then r = b).
+int residual(tfs_solver* s, const double* b, const double* x, bool x_zero, double* r, int precision,
+             tfs_solve_stats* st) {
+    const int n = (int)s->dims.n;
+    if (x_zero) return check(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream), "copy");
+    int rc = apply_A(s, x, s->t, precision, st);
+    if (rc) return rc;
+    return check(k_sub(b, s->t, r, n, s->stream), "residual");
+}
+
+// One or more GMRES(m) cycles on A x = b with matvecs in `precision`, until
+// ||b - A x|| <= abs_tol or max_cycles.  x is updated in place.  On entry r
+// holds b - A x and rnorm its norm.  Returns TFS_OK (converged or not: see
+// *converged), or an error.
+int gmres(tfs_solver* s, int precision, const double* b, double* x, double* r, double rnorm, double abs_tol,
+          int max_cycles, tfs_solve_stats* st, int* converged, double* final_res) {
+    const int n = (int)s->dims.n;
+    const int m = s->m;
+    const int ldh = m + 1;
+    *converged = rnorm <= abs_tol;
+    *final_res = rnorm;
+    for (int cyc = 0; cyc < max_cycles && !*converged; ++cyc) {
+        if (cyc > 0) {
+            int rc = residual(s, b, x, false, r, precision, st);
+            if (!rc) rc = norm(s, r, &rnorm);
+            if (rc) return rc;
+            if (!std::isfinite(rnorm)) return fail(TFS_ERR_NON_FINITE, "GMRES: non-finite residual");
+            *final_res = rnorm;
+            if (rnorm <= abs_tol) {
+                *converged = 1;
+                break;
+            }
+        }
+        st->cycles++;
+        int rc = check(k_fill(s->H, 0.0, (int64_t)ldh * m, s->stream), "H");
+        if (!rc) rc = check(k_fill(s->g, 0.0, ldh, s->stream), "g");
+        if (!rc) rc = check(k_fill(s->g, rnorm, 1, s->stream), "g0");
+        // v0 = r / ||r||
+        if (!rc) rc = check(k_axpby(1.0 / rnorm, r, 0.0, s->V, n, s->stream), "v0");
+        if (rc) return rc;
+        int k = 0;
+        double est = rnorm;
+        for (int j = 0; j < m; ++j) {
+            double* vj = s->V + (int64_t)j * s->ldv;
+            double* vn = s->V + (int64_t)(j + 1) * s->ldv;
+            double* Hj = s->H + (int64_t)j * ldh;
+            rc = apply_P(s, vj, s->z);
+            if (!rc) rc = apply_A(s, s->z, s->w, precision, st);
+            // CGS2: two passes of classical Gram-Schmidt against v_0..v_j.
+            for (int pass = 0; pass < 2 && !rc; ++pass) {
+                rc = check(k_dot_cols(s->V, s->ldv, j + 1, s->w, n, s->partial, s->G, s->stream), "dots");
+                if (!rc) rc = check(k_reduce_cols(s->partial, s->G, j + 1, s->h, Hj, s->stream), "reduce");
+                if (!rc) rc = check(k_gemv(s->V, s->ldv, j + 1, s->h, s->w, n, true, s->stream), "orth");
+            }
+            if (!rc) rc = check(k_norm(s->w, n, s->partial, s->G, s->nrm, s->stream), "wnorm");
+            if (!rc) rc = check(k_givens(s->H, ldh, j, s->cs, s->sn, s->g, s->nrm, s->scal, s->stream), "givens");
+            if (!rc) rc = check(k_scale(vn, s->w, s->scal, 1, n, s->stream), "normalise");
+            if (!rc) rc = read_scalar(s, s->scal, &est);
+            if (rc) return rc;
+            st->iterations++;
+            k = j + 1;
+            if (!std::isfinite(est)) return fail(TFS_ERR_NON_FINITE, "GMRES: non-finite residual estimate");
+            if (est <= abs_tol) break;
+        }
+        // x += P^{-1} V(:, 0:k) y
+        rc = check(k_trisolve(s->H, ldh, s->g, k, s->y, s->stream), "trisolve");
+        if (!rc) rc = check(k_gemv(s->V, s->ldv, k, s->y, s->t, n, false, s->stream), "Vy");
+        if (!rc) rc = apply_P(s, s->t, s->z);
+        if (!rc) rc = check(k_axpby(1.0, s->z, 1.0, x, n, s->stream), "update");
+        if (rc) return rc;
+        *final_res = est;
+        if (est <= abs_tol) *converged = 1;
+    }
+    return TFS_OK;
+}
+
+int alloc_solver(tfs_solver* s) {
+    const int64_t n = s->dims.n;
+    s->ldv = ((n + 31) / 32) * 32;
+    const int m = s->m, ldh = m + 1;
+    s->G = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256));
+    const int64_t ns3 = 3LL * s->desc.num_fil * s->desc.num_seg;
+    std::vector<std::pair<double**, int64_t>> parts = {
+        {&s->kphi, std::max<int64_t>(ns3, 1)}, {&s->kth, std::max<int64_t>(ns3, 1)},
+        {&s->b, s->ldv}, {&s->x, s->ldv}, {&s->r, s->ldv}, {&s->w, s->ldv}, {&s->z, s->ldv},
+        {&s->t, s->ldv}, {&s->d, s->ldv}, {&s->rin, s->ldv}, {&s->V, s->ldv * (m + 1)}, {&s->H, (int64_t)ldh * m},
+        {&s->cs, m}, {&s->sn, m}, {&s->g, ldh}, {&s->y, ldh}, {&s->h, ldh},
+        {&s->partial, (int64_t)s->G * ldh}, {&s->scal, 8}, {&s->nrm, 8}};
+    int64_t total = 0;
+    for (auto& p : parts) total += ((p.second + 31) / 32) * 32;
+    hipError_t e = hipMalloc(&s->arena, total * sizeof(double));
+    if (e != hipSuccess) return fail(TFS_ERR_OUT_OF_MEMORY, "solver: device memory");
+    double* p = s->arena;
+    for (auto& q : parts) {
+        *q.first = p;
+        p += ((q.second + 31) / 32) * 32;
+    }
+    e = hipMemsetAsync(s->arena, 0, total * sizeof(double), s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return hip_fail(e, "solver: clear");
+    e = hipHostMalloc((void**)&s->host_scal, 64, hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(e, "solver: pinned scalar");
+    return TFS_OK;
+}
+
+hipMemcpyKind kind_in(int32_t memory) {
+    return memory == TFS_MEMORY_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+}
+hipMemcpyKind kind_out(int32_t memory) {
+    return memory == TFS_MEMORY_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+}
+
+// Core solve on device vectors b (rhs) and x (warm start in, solution out).
+int solve_device(tfs_solver* s, const double* b, double* x, tfs_solve_stats* st) {
+    const int n = (int)s->dims.n;
+    const tfs_solver_desc& D = s->desc;
+    *st = tfs_solve_stats{};
+    double bnorm = 0.0;
+    int rc = norm(s, b, &bnorm);
+    if (rc) return rc;
+    if (!std::isfinite(bnorm)) return fail(TFS_ERR_NON_FINITE, "solve: non-finite right-hand side");
+    if (bnorm == 0.0) {  // mobility_solver.cpp:2127-2194
+        st->converged = 1;
+        return check(k_fill(x, 0.0, n, s->stream), "zero solution");
+    }
+    const int outer_prec = TFS_PRECISION_FP64;
+    rc = residual(s, b, x, false, s->r, outer_prec, st);
+    double rnorm = 0.0;
+    if (!rc) rc = norm(s, s->r, &rnorm);
+    if (rc) return rc;
+    if (!std::isfinite(rnorm)) return fail(TFS_ERR_NON_FINITE, "solve: non-finite initial residual");
+    st->initial_relres = rnorm / bnorm;
+    const double abs_tol = D.tol * bnorm;
+    if (rnorm <= abs_tol) {  // mobility_solver.cpp:2304-2308: the warm start is good enough
+        st->converged = 1;
+        st->relres = rnorm / bnorm;
+        return TFS_OK;
+    }
+    if (rnorm > bnorm) {  // mobility_solver.cpp:2310-2317: drop a rubbish initial guess
+        rc = check(k_fill(x, 0.0, n, s->stream), "reset");
+        if (!rc) rc = check(hipMemcpyAsync(s->r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream), "r=b");
+        if (rc) return rc;
+        rnorm = bnorm;
+    }
+    int conv = 0;
+    double res = rnorm;
+    if (D.mode == TFS_SOLVE_FP64) {
+        rc = gmres(s, TFS_PRECISION_FP64, b, x, s->r, rnorm, abs_tol, D.max_cycles, st, &conv, &res);
+        if (rc) return rc;
+        st->converged = conv;
+        st->relres = res / bnorm;
+    } else {
+        // Mixed-precision iterative refinement: the residual of x is formed with
+        // FP64 matvecs; each correction d solves A d = r with FP32-tile matvecs
+        // (FP64 Arnoldi) to a relative reduction inner_tol.
+        for (int it = 0; it < D.max_refinements && rnorm > abs_tol; ++it) {
+            st->refinements++;
+            rc = check(hipMemcpyAsync(s->rin, s->r, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream),
+                       "inner rhs");
+            if (!rc) rc = check(k_fill(s->d, 0.0, n, s->stream), "d=0");
+            int inner_conv = 0;
+            double inner_res = 0.0;
+            if (!rc)
+                rc = gmres(s, TFS_PRECISION_FP32_ACC64, s->rin, s->d, s->r, rnorm,
+                           std::max(D.inner_tol * rnorm, 0.5 * abs_tol), D.max_cycles, st, &inner_conv,
+                           &inner_res);
+            if (!rc) rc = check(k_axpby(1.0, s->d, 1.0, x, n, s->stream), "x += d");
+            if (!rc) rc = residual(s, b, x, false, s->r, TFS_PRECISION_FP64, st);
+            if (!rc) rc = norm(s, s->r, &rnorm);
+            if (rc) return rc;
+            if (!std::isfinite(rnorm)) return fail(TFS_ERR_NON_FINITE, "solve: non-finite residual");
+        }
+        st->converged = rnorm <= abs_tol;
+        st->relres = rnorm / bnorm;
+    }
+    if (!std::isfinite(st->relres)) return fail(TFS_ERR_NON_FINITE, "solve: non-finite residual");
+    return st->converged ? TFS_OK : fail(TFS_ERR_NOT_CONVERGED, "solve: tolerance not met");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI: solver
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int tfs_solver_create(const tfs_solver_desc* desc, tfs_solver** out) {
+    g_last_error.clear();
+    if (!desc || !out) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_create: null argument");
+    *out = nullptr;
+    const tfs_solver_desc& D = *desc;
+    if (D.num_fil < 0 || D.num_seg < 0 || D.num_blobs < 0 || (D.num_fil > 0 && D.num_seg < 1))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_create: bad sizes");
+    if ((int64_t)D.num_fil * D.num_seg + D.num_blobs == 0)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_create: empty system");
+    if (D.mode != TFS_SOLVE_FP64 && D.mode != TFS_SOLVE_MIXED)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_create: unknown mode");
+    if (D.restart < 1 || D.restart > kMaxRestart)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_create: restart must be in [1, 350]");
+    if (D.max_cycles < 1 || !(D.tol > 0) || (D.mode == TFS_SOLVE_MIXED && (!(D.inner_tol > 0) || D.max_refinements < 1)))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_create: bad tolerances or iteration limits");
+    if (!(D.seg_radius > 0) || !(D.blob_radius > 0) || !(D.mu > 0))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_create: radii and mu must be positive");
+    tfs_solver* s = new tfs_solver();
+    s->desc = D;
+    SaddleDims& dm = s->dims;
+    dm.nfil = D.num_fil;
+    dm.nseg = D.num_seg;
+    dm.nblob = D.num_blobs;
+    dm.dyn_phase = D.dynamic_phase ? 1 : 0;
+    dm.dyn_angle = D.dynamic_angle ? 1 : 0;
+    const int64_t ns = (int64_t)D.num_fil * D.num_seg;
+    dm.off_blob = 3 * ns;
+    dm.off_phase = dm.off_blob + 3LL * D.num_blobs;
+    dm.off_angle = dm.off_phase + (dm.dyn_phase ? D.num_fil : 0);
+    dm.n = dm.off_angle + (dm.dyn_angle ? D.num_fil : 0);
+    if (dm.n > INT32_MAX / 2) {
+        delete s;
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_create: system too large");
+    }
+    s->m = (int)std::min<int64_t>(D.restart, dm.n);
+    const double PI = 3.14159265358979323846;
+    s->gs = 6.0 * PI * D.mu * D.seg_radius;
+    s->gb = 6.0 * PI * D.mu * D.blob_radius;
+    tfs_rpy_desc rd{};
+    rd.num_segs = (int32_t)ns;
+    rd.num_blobs = D.num_blobs;
+    rd.seg_dof = 3;
+    rd.precision = TFS_PRECISION_FP64;
+    rd.bs_mode = TFS_BS_ACCUMULATE;
+    rd.memory = TFS_MEMORY_DEVICE;
+    rd.seg_radius = D.seg_radius;
+    rd.blob_radius = D.blob_radius;
+    rd.mu = D.mu;
+    rd.stream = D.stream;
+    int rc = tfs_rpy_create(&rd, &s->rpy);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    s->stream = s->rpy->stream;
+    rc = alloc_solver(s);
+    if (rc) {
+        tfs_solver_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return TFS_OK;
+}
+
+int tfs_solver_destroy(tfs_solver* s) {
+    if (!s) return TFS_OK;
+    if (s->stream) hipStreamSynchronize(s->stream);
+    if (s->arena) hipFree(s->arena);
+    if (s->host_scal) hipHostFree(s->host_scal);
+    if (s->rpy) tfs_rpy_destroy(s->rpy);
+    delete s;
+    return TFS_OK;
+}
+
+int tfs_solver_size(tfs_solver* s, int64_t* n) {
+    if (!s || !n) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_size: null argument");
+    *n = s->dims.n;
+    return TFS_OK;
+}
+
+int tfs_solver_set_geometry(tfs_solver* s, const double* x_segs, const double* x_blobs,
+                            const double* vel_dir_phase, const double* vel_dir_angle, int32_t memory) {
+    if (!s) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_geometry: null handle");
+    if (memory != TFS_MEMORY_HOST && memory != TFS_MEMORY_DEVICE)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_geometry: unknown memory space");
+    const size_t ns3 = (size_t)3 * s->desc.num_fil * s->desc.num_seg, nb3 = (size_t)3 * s->desc.num_blobs;
+    const hipMemcpyKind k = kind_in(memory);
+    hipError_t e = hipSuccess;
+    if (x_segs && ns3) e = hipMemcpyAsync(s->rpy->x_segs, x_segs, ns3 * 8, k, s->stream);
+    if (e == hipSuccess && x_blobs && nb3) e = hipMemcpyAsync(s->rpy->x_blobs, x_blobs, nb3 * 8, k, s->stream);
+    if (e == hipSuccess && vel_dir_phase && ns3) e = hipMemcpyAsync(s->kphi, vel_dir_phase, ns3 * 8, k, s->stream);
+    if (e == hipSuccess && vel_dir_angle && ns3) e = hipMemcpyAsync(s->kth, vel_dir_angle, ns3 * 8, k, s->stream);
+    if (e == hipSuccess && memory == TFS_MEMORY_HOST) e = hipStreamSynchronize(s->stream);
+    return check(e, "tfs_solver_set_geometry");
+}
+
+int tfs_solver_system_mult(tfs_solver* s, const double* in, double* out, int32_t precision, int32_t memory) {
+    if (!s || !in || !out) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_system_mult: null argument");
+    if (precision != TFS_PRECISION_FP64 && precision != TFS_PRECISION_FP32_ACC64)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_system_mult: unknown precision");
+    const size_t n = s->dims.n;
+    const double* din = in;
+    double* dout = out;
+    int rc = TFS_OK;
+    if (memory == TFS_MEMORY_HOST) {
+        rc = check(hipMemcpyAsync(s->b, in, n * 8, hipMemcpyHostToDevice, s->stream), "upload");
+        din = s->b;
+        dout = s->x;
+    }
+    if (!rc) rc = apply_A(s, din, dout, precision, nullptr);
+    if (!rc && memory == TFS_MEMORY_HOST)
+        rc = check(hipMemcpyAsync(out, dout, n * 8, hipMemcpyDeviceToHost, s->stream), "download");
+    if (!rc) rc = check(hipStreamSynchronize(s->stream), "sync");
+    return rc;
+}
+
+int tfs_solver_precondition(tfs_solver* s, const double* in, double* out, int32_t memory) {
+    if (!s || !in || !out) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_precondition: null argument");
+    const size_t n = s->dims.n;
+    const double* din = in;
+    double* dout = out;
+    int rc = TFS_OK;
+    if (memory == TFS_MEMORY_HOST) {
+        rc = check(hipMemcpyAsync(s->b, in, n * 8, hipMemcpyHostToDevice, s->stream), "upload");
+        din = s->b;
+        dout = s->x;
+    }
+    if (!rc) rc = apply_P(s, din, dout);
+    if (!rc && memory == TFS_MEMORY_HOST)
+        rc = check(hipMemcpyAsync(out, dout, n * 8, hipMemcpyDeviceToHost, s->stream), "download");
+    if (!rc) rc = check(hipStreamSynchronize(s->stream), "sync");
+    return rc;
+}
+
+int tfs_solver_solve(tfs_solver* s, const double* rhs, double* x, int32_t memory, tfs_solve_stats* stats) {
+    if (!s || !rhs || !x) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_solve: null argument");
+    const size_t n = s->dims.n;
+    tfs_solve_stats local{};
+    tfs_solve_stats* st = stats ? stats : &local;
+    int rc = check(hipMemcpyAsync(s->b, rhs, n * 8, kind_in(memory), s->stream), "rhs");
+    if (!rc) rc = check(hipMemcpyAsync(s->x, x, n * 8, kind_in(memory), s->stream), "x0");
+    if (rc) return rc;
+    const int solve_rc = solve_device(s, s->b, s->x, st);
+    if (solve_rc != TFS_OK && solve_rc != TFS_ERR_NOT_CONVERGED) return solve_rc;
+    rc = check(hipMemcpyAsync(x, s->x, n * 8, kind_out(memory), s->stream), "x");
+    if (!rc) rc = check(hipStreamSynchronize(s->stream), "sync");
+    return rc ? rc : solve_rc;
+}
+
+int tfs_solver_residual(tfs_solver* s, const double* rhs, const double* x, int32_t precision, int32_t memory,
+                        double* relres) {
+    if (!s || !rhs || !x || !relres) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_residual: null argument");
+    const size_t n = s->dims.n;
+    int rc = check(hipMemcpyAsync(s->b, rhs, n * 8, kind_in(memory), s->stream), "rhs");
+    if (!rc) rc = check(hipMemcpyAsync(s->d, x, n * 8, kind_in(memory), s->stream), "x");
+    if (!rc) rc = residual(s, s->b, s->d, false, s->r, precision, nullptr);
+    double rn = 0, bn = 0;
+    if (!rc) rc = norm(s, s->r, &rn);
+    if (!rc) rc = norm(s, s->b, &bn);
+    if (rc) return rc;
+    *relres = bn > 0 ? rn / bn : rn;
+    return TFS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Prescribed-beat timestep driver
+// ---------------------------------------------------------------------------
+struct tfs_beat {
+    tfs_beat_desc desc{};
+    tfs_solver* solver = nullptr;
+    int64_t step = 0;
+    double* arena = nullptr;
+    double *frames = nullptr, *bases = nullptr, *s_values = nullptr, *phase = nullptr, *angle = nullptr,
+           *omega0 = nullptr, *gen_phase = nullptr, *gen_angle = nullptr, *body = nullptr, *rhs = nullptr,
+           *sol = nullptr;
+    std::vector<double> beat_A, beat_B;
+};
+
+extern "C" {
+
+int tfs_beat_create(const tfs_beat_desc* desc, tfs_beat** out) {
+    g_last_error.clear();
+    if (!desc || !out) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_create: null argument");
+    *out = nullptr;
+    const tfs_beat_desc& D = *desc;
+    const int nf = D.solver.num_fil, nseg = D.solver.num_seg;
+    if (nf < 1 || nseg < 2) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_create: need filaments of >= 2 segments");
+    if (!D.frames || !D.bases || !D.omega0 || !D.phase0 || !D.beat_A || !D.beat_B)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_create: missing geometry or beat tables");
+    if (D.solver.num_blobs > 0 && !D.x_blobs) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_create: missing blobs");
+    if (D.solver.dynamic_phase && (!D.gen_phase_forces || D.n_gen_phase < 2))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_create: missing generalised phase forces");
+    if (D.solver.dynamic_angle && (!D.gen_angle_forces || D.n_gen_angle < 2))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_create: missing generalised angle forces");
+    if (!(D.dt > 0) || !(D.fil_length > 0)) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_create: dt, length > 0");
+    tfs_beat* b = new tfs_beat();
+    b->desc = D;
+    int rc = tfs_solver_create(&D.solver, &b->solver);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    tfs_solver* s = b->solver;
+    const int ngp = D.solver.dynamic_phase ? D.n_gen_phase : 1, nga = D.solver.dynamic_angle ? D.n_gen_angle : 1;
+    const int64_t n = s->dims.n;
+    std::vector<std::pair<double**, int64_t>> parts = {
+        {&b->frames, 9LL * nf}, {&b->bases, 3LL * nf}, {&b->s_values, nseg}, {&b->phase, nf}, {&b->angle, nf},
+        {&b->omega0, nf}, {&b->gen_phase, ngp}, {&b->gen_angle, nga}, {&b->body, 9}, {&b->rhs, n}, {&b->sol, n}};
+    int64_t total = 0;
+    for (auto& p : parts) total += ((p.second + 31) / 32) * 32;
+    if (hipMalloc(&b->arena, total * 8) != hipSuccess) {
+        tfs_beat_destroy(b);
+        return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_beat_create: device memory");
+    }
+    double* p = b->arena;
+    for (auto& q : parts) {
+        *q.first = p;
+        p += ((q.second + 31) / 32) * 32;
+    }
+    std::vector<double> sv(nseg), zeros(std::max(nf, 1), 0.0), body(9, 0.0);
+    for (int i = 0; i < nseg; ++i) sv[i] = D.s_values ? D.s_values[i] : double(i) / double(nseg - 1);
+    for (int i = 0; i < 3; ++i) body[i] = D.body_x[i];
+    for (int i = 0; i < 6; ++i) body[3 + i] = D.body_v[i];
+    hipStream_t st = s->stream;
+    hipError_t e = hipMemsetAsync(b->arena, 0, total * 8, st);
+    auto up = [&](double* dst, const double* src, int64_t cnt) {
+        if (e == hipSuccess && cnt > 0) e = hipMemcpyAsync(dst, src, cnt * 8, hipMemcpyHostToDevice, st);
+    };
+    up(b->frames, D.frames, 9LL * nf);
+    up(b->bases, D.bases, 3LL * nf);
+    up(b->s_values, sv.data(), nseg);
+    up(b->phase, D.phase0, nf);
+    up(b->angle, D.angle0 ? D.angle0 : zeros.data(), nf);
+    up(b->omega0, D.omega0, nf);
+    if (D.solver.dynamic_phase) up(b->gen_phase, D.gen_phase_forces, D.n_gen_phase);
+    if (D.solver.dynamic_angle) up(b->gen_angle, D.gen_angle_forces, D.n_gen_angle);
+    up(b->body, body.data(), 9);
+    if (e == hipSuccess && D.solver.num_blobs > 0)
+        e = hipMemcpyAsync(s->rpy->x_blobs, D.x_blobs, 24LL * D.solver.num_blobs, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    b->beat_A.assign(D.beat_A, D.beat_A + 24);
+    b->beat_B.assign(D.beat_B, D.beat_B + 18);
+    if (e != hipSuccess) {
+        tfs_beat_destroy(b);
+        return hip_fail(e, "tfs_beat_create: upload");
+    }
+    *out = b;
+    return TFS_OK;
+}
+
+int tfs_beat_destroy(tfs_beat* b) {
+    if (!b) return TFS_OK;
+    if (b->solver && b->solver->stream) hipStreamSynchronize(b->solver->stream);
+    if (b->arena) hipFree(b->arena);
+    tfs_solver_destroy(b->solver);
+    delete b;
+    return TFS_OK;
+}
+
+int tfs_beat_step(tfs_beat* b, tfs_solve_stats* stats) {
+    if (!b) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_step: null handle");
+    tfs_solver* s = b->solver;
+    const tfs_beat_desc& D = b->desc;
+    hipStream_t st = s->stream;
+    // The beat coefficients live in constant memory shared by every handle.
+    int rc = check(k_beat_coeffs(b->beat_A.data(), b->beat_B.data()), "beat coefficients");
+    if (rc) return rc;
+    // filament::initial_guess: explicit phase/angle update from the last solve.
+    if (b->step > 0) {
+        rc = check(k_phase_update(b->sol, s->dims, D.dt, b->phase, b->angle, st), "phase update");
+        if (rc) return rc;
+    }
+    BeatArgs ba;
+    ba.nfil = D.solver.num_fil;
+    ba.nseg = D.solver.num_seg;
+    ba.length = D.fil_length;
+    ba.frames = b->frames;
+    ba.bases = b->bases;
+    ba.s_values = b->s_values;
+    ba.phase = b->phase;
+    ba.angle = D.solver.dynamic_angle ? b->angle : nullptr;
+    rc = check(k_beat(ba, s->rpy->x_segs, s->kphi, s->kth, st), "beat kinematics");
+    if (rc) return rc;
+    RhsArgs ra;
+    ra.d = s->dims;
+    ra.phase = b->phase;
+    ra.angle = b->angle;
+    ra.omega0 = b->omega0;
+    ra.gen_phase = b->gen_phase;
+    ra.n_gen_phase = D.n_gen_phase;
+    ra.gen_angle = b->gen_angle;
+    ra.n_gen_angle = D.n_gen_angle;
+    ra.gen_force_factor = D.gen_force_factor;
+    ra.torsion_factor = D.torsion_factor;
+    ra.length = D.fil_length;
+    ra.body_x = b->body;
+    ra.body_v = b->body + 3;
+    rc = check(k_rhs(ra, s->rpy->x_segs, s->rpy->x_blobs, b->rhs, st), "rhs");
+    if (rc) return rc;
+    tfs_solve_stats local{};
+    tfs_solve_stats* sst = stats ? stats : &local;
+    // Warm start: the previous step's solution (mobility_solver.cpp:2201-2293).
+    const int solve_rc = solve_device(s, b->rhs, b->sol, sst);
+    b->step++;
+    return solve_rc;
+}
+
+int tfs_beat_state(tfs_beat* b, double* phase, double* angle, double* x_segs, double* solution) {
+    if (!b) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_state: null handle");
+    tfs_solver* s = b->solver;
+    const int nf = b->desc.solver.num_fil;
+    hipError_t e = hipSuccess;
+    auto down = [&](double* dst, const double* src, int64_t cnt) {
+        if (e == hipSuccess && dst && cnt > 0) e = hipMemcpyAsync(dst, src, cnt * 8, hipMemcpyDeviceToHost, s->stream);
+    };
+    down(phase, b->phase, nf);
+    down(angle, b->angle, nf);
+    down(x_segs, s->rpy->x_segs, 3LL * nf * b->desc.solver.num_seg);
+    down(solution, b->sol, s->dims.n);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    return check(e, "tfs_beat_state");
+}
+
+int tfs_beat_solver(tfs_beat* b, tfs_solver** out) {
+    if (!b || !out) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_solver: null argument");
+    *out = b->solver;
+    return TFS_OK;
+}
+
+int tfs_beat_set_mode(tfs_beat* b, int32_t mode) {
+    if (!b) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_set_mode: null handle");
+    if (mode != TFS_SOLVE_FP64 && mode != TFS_SOLVE_MIXED)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_set_mode: unknown mode");
+    if (mode == TFS_SOLVE_MIXED && (!(b->solver->desc.inner_tol > 0) || b->solver->desc.max_refinements < 1))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_set_mode: mixed mode needs inner_tol and max_refinements");
+    b->solver->desc.mode = mode;
+    b->desc.solver.mode = mode;
+    return TFS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,242 @@
+"""The per-timestep solve over the C-ABI of include/tfs_solver.h.
+
+``Solver`` mirrors the generic layer of the reference's ``mobility_solver``
+for prescribed cilia (src/cilia/mobility/mobility_solver.cpp):
+``system_matrix_mult`` (system_matrix_mult_new, :1806-2090),
+``apply_preconditioner`` (apply_preconditioner_new, :1040-1513) and
+``solve_linear_system`` (:2092-2570), on device vectors.  ``BeatSimulation``
+is the time loop of a prescribed-beat body (cilia_sim_main.cu:355-390,
+filament::initial_guess, assemble_rhs, compute_velocities).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from . import beat as B
+
+SOLVE_FP64 = 0
+SOLVE_MIXED = 1
+
+
+class SolverDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_fil", ctypes.c_int32), ("num_seg", ctypes.c_int32), ("num_blobs", ctypes.c_int32),
+        ("dynamic_phase", ctypes.c_int32), ("dynamic_angle", ctypes.c_int32), ("mode", ctypes.c_int32),
+        ("seg_radius", ctypes.c_double), ("blob_radius", ctypes.c_double), ("mu", ctypes.c_double),
+        ("restart", ctypes.c_int32), ("max_cycles", ctypes.c_int32), ("tol", ctypes.c_double),
+        ("inner_tol", ctypes.c_double), ("max_refinements", ctypes.c_int32), ("stream", ctypes.c_void_p),
+    ]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int32), ("cycles", ctypes.c_int32), ("refinements", ctypes.c_int32),
+        ("fp64_matvecs", ctypes.c_int32), ("fp32_matvecs", ctypes.c_int32), ("converged", ctypes.c_int32),
+        ("relres", ctypes.c_double), ("initial_relres", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_dp = ctypes.c_void_p
+
+
+class BeatDesc(ctypes.Structure):
+    _fields_ = [
+        ("solver", SolverDesc), ("dt", ctypes.c_double), ("fil_length", ctypes.c_double),
+        ("frames", _dp), ("bases", _dp), ("x_blobs", _dp), ("s_values", _dp), ("omega0", _dp),
+        ("phase0", _dp), ("angle0", _dp), ("beat_A", _dp), ("beat_B", _dp),
+        ("gen_phase_forces", _dp), ("n_gen_phase", ctypes.c_int32),
+        ("gen_angle_forces", _dp), ("n_gen_angle", ctypes.c_int32),
+        ("gen_force_factor", ctypes.c_double), ("torsion_factor", ctypes.c_double),
+        ("body_x", ctypes.c_double * 3), ("body_v", ctypes.c_double * 6),
+    ]
+
+
+_i32 = ctypes.c_int32
+L.extra_signatures({
+    "tfs_solver_create": (ctypes.c_int, [ctypes.POINTER(SolverDesc), ctypes.POINTER(_dp)]),
+    "tfs_solver_destroy": (ctypes.c_int, [_dp]),
+    "tfs_solver_size": (ctypes.c_int, [_dp, ctypes.POINTER(ctypes.c_int64)]),
+    "tfs_solver_set_geometry": (ctypes.c_int, [_dp, _dp, _dp, _dp, _dp, _i32]),
+    "tfs_solver_system_mult": (ctypes.c_int, [_dp, _dp, _dp, _i32, _i32]),
+    "tfs_solver_precondition": (ctypes.c_int, [_dp, _dp, _dp, _i32]),
+    "tfs_solver_solve": (ctypes.c_int, [_dp, _dp, _dp, _i32, ctypes.POINTER(SolveStats)]),
+    "tfs_solver_residual": (ctypes.c_int, [_dp, _dp, _dp, _i32, _i32, ctypes.POINTER(ctypes.c_double)]),
+    "tfs_beat_create": (ctypes.c_int, [ctypes.POINTER(BeatDesc), ctypes.POINTER(_dp)]),
+    "tfs_beat_destroy": (ctypes.c_int, [_dp]),
+    "tfs_beat_step": (ctypes.c_int, [_dp, ctypes.POINTER(SolveStats)]),
+    "tfs_beat_state": (ctypes.c_int, [_dp, _dp, _dp, _dp, _dp]),
+    "tfs_beat_solver": (ctypes.c_int, [_dp, ctypes.POINTER(_dp)]),
+    "tfs_beat_set_mode": (ctypes.c_int, [_dp, _i32]),
+})
+
+
+def solver_desc(num_fil, num_seg, num_blobs, *, dynamic_phase=True, dynamic_angle=True, mode=SOLVE_FP64,
+                seg_radius=1.0, blob_radius=1.0, mu=1.0, restart=50, max_cycles=20, tol=1e-8,
+                inner_tol=1e-3, max_refinements=10, stream=None) -> SolverDesc:
+    d = SolverDesc()
+    d.num_fil, d.num_seg, d.num_blobs = num_fil, num_seg, num_blobs
+    d.dynamic_phase, d.dynamic_angle, d.mode = int(dynamic_phase), int(dynamic_angle), mode
+    d.seg_radius, d.blob_radius, d.mu = seg_radius, blob_radius, mu
+    d.restart, d.max_cycles, d.tol = restart, max_cycles, tol
+    d.inner_tol, d.max_refinements = inner_tol, max_refinements
+    d.stream = stream
+    return d
+
+
+class Solver:
+    """Saddle-point system of one prescribed-cilia body on the GPU."""
+
+    def __init__(self, desc: SolverDesc):
+        self.desc = desc
+        h = ctypes.c_void_p()
+        L.check(L.lib().tfs_solver_create(ctypes.byref(desc), ctypes.byref(h)))
+        self.h = h
+        n = ctypes.c_int64()
+        L.check(L.lib().tfs_solver_size(self.h, ctypes.byref(n)))
+        self.n = n.value
+
+    def close(self):
+        if self.h:
+            L.lib().tfs_solver_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_geometry(self, x_segs=None, x_blobs=None, vel_dir_phase=None, vel_dir_angle=None):
+        a = [None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+             for v in (x_segs, x_blobs, vel_dir_phase, vel_dir_angle)]
+        L.check(L.lib().tfs_solver_set_geometry(self.h, *[L.ptr(v) for v in a], L.MEMORY_HOST))
+
+    def system_mult(self, v, precision=L.PRECISION_FP64):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros(self.n)
+        L.check(L.lib().tfs_solver_system_mult(self.h, L.ptr(v), L.ptr(out), precision, L.MEMORY_HOST))
+        return out
+
+    def precondition(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros(self.n)
+        L.check(L.lib().tfs_solver_precondition(self.h, L.ptr(v), L.ptr(out), L.MEMORY_HOST))
+        return out
+
+    def solve(self, rhs, x0=None, *, allow_unconverged=False):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        st = SolveStats()
+        rc = L.lib().tfs_solver_solve(self.h, L.ptr(rhs), L.ptr(x), L.MEMORY_HOST, ctypes.byref(st))
+        if not (allow_unconverged and rc == L.TFS_ERR_NOT_CONVERGED):
+            L.check(rc)
+        return x, st.as_dict()
+
+    def residual(self, rhs, x, precision=L.PRECISION_FP64):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = ctypes.c_double()
+        L.check(L.lib().tfs_solver_residual(self.h, L.ptr(rhs), L.ptr(x), precision, L.MEMORY_HOST,
+                                            ctypes.byref(out)))
+        return out.value
+
+
+@dataclass
+class BeatSetup:
+    """Host-side inputs of a prescribed-beat body (kept alive for the C call)."""
+
+    frames: np.ndarray
+    bases: np.ndarray
+    x_blobs: np.ndarray
+    s_values: np.ndarray
+    omega0: np.ndarray
+    phase0: np.ndarray
+    angle0: np.ndarray
+    gen_phase: np.ndarray
+    gen_angle: np.ndarray
+    fil_length: float
+
+
+def beat_setup(body, *, phase_seed: int = 3, period: float = 1.0) -> BeatSetup:
+    """Inputs for a geometry.Body: F&B beat along each filament's polar
+    direction, natural frequency 2*pi/period, seeded uniform initial phases."""
+    from . import geometry as G
+    nf, nseg = body.num_fil, body.num_seg
+    fil_length = body.dl * (nseg - 1)
+    q_phase, q_angle = B.generalised_force_tables(nseg, fil_length, a=body.a)
+    return BeatSetup(
+        frames=B.frames(body.fil_normal),
+        bases=np.ascontiguousarray(body.fil_base.reshape(-1)),
+        x_blobs=np.ascontiguousarray(body.x_blobs),
+        s_values=B.uniform_s(nseg),
+        omega0=np.full(nf, 2.0 * np.pi / period),
+        phase0=2.0 * np.pi * G.uniform_open(phase_seed, nf),
+        angle0=np.zeros(nf),
+        gen_phase=q_phase, gen_angle=q_angle, fil_length=fil_length)
+
+
+class BeatSimulation:
+    """Prescribed-beat time loop of one body, fully on the GPU."""
+
+    def __init__(self, body, *, mode=SOLVE_FP64, dt=1.0 / 300.0, restart=50, tol=1e-8, inner_tol=1e-3,
+                 max_cycles=20, max_refinements=10, dynamic_phase=True, dynamic_angle=True,
+                 gen_force_factor=1.0, torsion_factor=0.046, stream=None, setup: BeatSetup | None = None):
+        self.setup = setup or beat_setup(body)
+        su = self.setup
+        d = BeatDesc()
+        d.solver = solver_desc(body.num_fil, body.num_seg, body.num_blobs, dynamic_phase=dynamic_phase,
+                               dynamic_angle=dynamic_angle, mode=mode, seg_radius=body.a,
+                               blob_radius=body.a, restart=restart, max_cycles=max_cycles, tol=tol,
+                               inner_tol=inner_tol, max_refinements=max_refinements, stream=stream)
+        d.dt, d.fil_length = dt, su.fil_length
+        self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                      (su.frames, su.bases, su.x_blobs, su.s_values, su.omega0, su.phase0, su.angle0,
+                       B.BEAT_A.reshape(-1), B.BEAT_B.reshape(-1), su.gen_phase, su.gen_angle)]
+        (d.frames, d.bases, d.x_blobs, d.s_values, d.omega0, d.phase0, d.angle0, d.beat_A, d.beat_B,
+         d.gen_phase_forces, d.gen_angle_forces) = [a.ctypes.data for a in self._keep]
+        d.n_gen_phase, d.n_gen_angle = len(su.gen_phase), len(su.gen_angle)
+        d.gen_force_factor, d.torsion_factor = gen_force_factor, torsion_factor
+        for i in range(3):
+            d.body_x[i] = float(body.centre[i])
+        self.desc = d
+        self.num_fil, self.num_seg = body.num_fil, body.num_seg
+        h = ctypes.c_void_p()
+        L.check(L.lib().tfs_beat_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        sp = ctypes.c_void_p()
+        L.check(L.lib().tfs_beat_solver(self.h, ctypes.byref(sp)))
+        n = ctypes.c_int64()
+        L.check(L.lib().tfs_solver_size(sp, ctypes.byref(n)))
+        self.n = n.value
+
+    def close(self):
+        if self.h:
+            L.lib().tfs_beat_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_mode(self, mode: int):
+        L.check(L.lib().tfs_beat_set_mode(self.h, mode))
+
+    def step(self):
+        st = SolveStats()
+        L.check(L.lib().tfs_beat_step(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def state(self):
+        nf, ns = self.num_fil, self.num_fil * self.num_seg
+        ph, an, xs, sol = np.zeros(nf), np.zeros(nf), np.zeros(3 * ns), np.zeros(self.n)
+        L.check(L.lib().tfs_beat_state(self.h, L.ptr(ph), L.ptr(an), L.ptr(xs), L.ptr(sol)))
+        return {"phase": ph, "angle": an, "x_segs": xs, "solution": sol}
