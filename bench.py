@@ -1,14 +1,20 @@
 """Benchmark of the RPY mobility hot path on one MI355X.
 
-Workload: the production body -- 640 filaments x 20 segments (12,800
-segments, 6 DOF each: force+torque in, velocity+rotation out) and 9,000 blobs
-on a sphere of radius 50a -- with seeded synthetic positions and N(0,1)
-forces/torques.  Inputs are resident in HBM before timing starts.
+Workload (production body): 640 filaments x 20 segments (12,800 segments) and
+9,000 surface blobs on a sphere of radius 50a (a = 1, mu = 1), seeded
+synthetic geometry, resident in HBM before timing starts.
 
-One step = one full grand-mobility matvec (all four blocks, dimension
-6*12,800 + 3*9,000) in the mixed precision (FP32 pair tensors, FP64
-accumulation); the same matvec in plain FP64 is timed beside it, and the
-relative L2 error of the mixed result against the FP64 result is reported.
+One step = one prescribed-beat timestep of that body: phase update, beat
+kinematics, right-hand side and the warm-started saddle-point solve
+(RPY <3,3> mobility + K_phi/K_theta couplings, n = 3*12,800 + 3*9,000 +
+2*640 = 66,680) to a relative residual of 1e-8 with GMRES(50) and FP64
+Arnoldi.  The headline is the mixed-precision form (FP32-tile matvecs inside
+iterative refinement, FP64 residuals outside); the all-FP64 form runs the
+same steps from the same state beside it.
+
+Also timed: the full 6-DOF grand-mobility matvec (dimension 6*12,800 +
+3*9,000) in FP64 and FP32-tile/FP64-accumulate forms -- the "RPY matvec GB/s"
+part of the metric -- with its relative L2 error against FP64.
 
 Prints one JSON line.  `--gpus` above 1 is not supported (one GPU).
 """
@@ -26,18 +32,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_VALU_TFLOPS = 78.6     # MI355X vector FP64, published
-PEAK_FP32_VALU_TFLOPS = 157.3    # MI355X vector FP32, published (packed)
+PEAK_FP32_VALU_TFLOPS = 157.3    # MI355X vector FP32, published (packed issue)
 PEAK_HBM_GBPS = 8000.0
 
 # Algorithmic flops per pair (canonical far-field count: +,-,* = 1, FMA = 2,
 # sqrt/rsqrt/div = 1): tt 36, tr 16, rt 15, rr 21.
+FLOPS_TT = 36
 FLOPS_SS6, FLOPS_SB6, FLOPS_BS6, FLOPS_BB = 88, 52, 52, 36
+DL = 2.2
 
 
 def matvec_flops(ns: int, nb: int, seg_dof: int) -> float:
     if seg_dof == 6:
         return FLOPS_SS6 * ns * ns + FLOPS_SB6 * ns * nb + FLOPS_BS6 * nb * ns + FLOPS_BB * nb * nb
-    return 36.0 * (ns + nb) ** 2
+    return float(FLOPS_TT) * (ns + nb) ** 2
 
 
 def matvec_bytes(ns: int, nb: int, seg_dof: int) -> float:
@@ -62,21 +70,53 @@ def matvec_leg(torch, handle, precision: int, steps: int, warmup: int):
     stop.record()
     torch.cuda.synchronize()
     total_ms = start.elapsed_time(stop)
-    pair_ms, launches = handle.profile_read()
+    ms, n = handle.profile_read()[precision]
     handle.profile(False)
-    return {"ms_per_step": total_ms / steps, "pair_kernel_ms": pair_ms / max(launches, 1),
-            "launches": launches}
+    return {"ms_per_step": total_ms / steps, "pair_kernel_ms": ms / max(n, 1), "launches": n}
 
 
-def cpu_baseline(body, budget_s: float = 12.0):
-    """FP64 oracle direct sum on the host cores (reported, not gating)."""
+def timestep_leg(torch, body, mode: int, steps: int, warmup: int, stream):
+    """Warm-up steps, then `steps` timed timesteps; returns timings and state."""
+    from timfilsimmod_amd import _lib as L
+    from timfilsimmod_amd.mobility import profile_read
+    from timfilsimmod_amd.solver import BeatSimulation
+    sim = BeatSimulation(body, mode=mode, tol=1e-8, restart=50, inner_tol=1e-3, max_cycles=40,
+                         max_refinements=20, stream=stream)
+    rpy = sim.rpy_handle()
+    for _ in range(warmup):
+        sim.step()
+    torch.cuda.synchronize()
+    L.check(L.lib().tfs_rpy_profile(rpy, 1))
+    profile_read(rpy)
+    stats = []
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(steps):
+        stats.append(sim.step())
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern = profile_read(rpy)
+    L.check(L.lib().tfs_rpy_profile(rpy, 0))
+    state = sim.state()
+    sim.close()
+    return {"ms_per_step": start.elapsed_time(stop) / steps, "wall_s": wall, "stats": stats,
+            "kernels": kern, "state": state}
+
+
+def cpu_baseline(body, budget_s: float = 15.0):
+    """FP64 oracle direct sum on the host cores (reported, not gating): the
+    prescribed-beat <3,3> matvec of the production body, timed."""
     from oracle import oracle as O
     O.full_mobility(body.x_segs[:300], body.x_blobs[:300], body.f_segs[:600], body.f_blobs[:300])
     times = []
     t_begin = time.perf_counter()
     while True:
         t0 = time.perf_counter()
-        O.full_mobility(body.x_segs, body.x_blobs, body.f_segs, body.f_blobs, seg_dof=6)
+        O.full_mobility(body.x_segs, body.x_blobs, body.f_segs, body.f_blobs, seg_dof=3,
+                        bs_accumulate=True)
         times.append(time.perf_counter() - t0)
         if time.perf_counter() - t_begin > budget_s or len(times) >= 5:
             break
@@ -86,8 +126,9 @@ def cpu_baseline(body, budget_s: float = 12.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20, help="timed timesteps per precision")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--matvec-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -100,6 +141,7 @@ def main():
     from timfilsimmod_amd import _lib as L
     from timfilsimmod_amd import geometry as G
     from timfilsimmod_amd.mobility import RpyHandle
+    from timfilsimmod_amd.solver import SOLVE_FP64, SOLVE_MIXED
 
     torch.cuda.set_device(0)
     # A dedicated (default-priority) stream: the library's kernels and torch's
@@ -110,49 +152,81 @@ def main():
     body = G.production_body()
     ns, nb = body.num_segs, body.num_blobs
 
-    # Resident inputs: upload once, evaluate on the device buffers.
-    # The full grand mobility: Mbs accumulates onto Mbb (the reference's
-    # accumulate build, cuda_functions.cu:739-743).
+    # ---- timestep legs (headline) ------------------------------------------
+    print("bench: timestep leg, mixed precision", file=sys.stderr, flush=True)
+    mixed = timestep_leg(torch, body, SOLVE_MIXED, args.steps, args.warmup, stream)
+    print("bench: timestep leg, FP64", file=sys.stderr, flush=True)
+    fp64 = timestep_leg(torch, body, SOLVE_FP64, args.steps, args.warmup, stream)
+    ts_mixed = 1000.0 / mixed["ms_per_step"]
+    ts_fp64 = 1000.0 / fp64["ms_per_step"]
+    dev = float(np.abs(mixed["state"]["x_segs"] - fp64["state"]["x_segs"]).max() / DL)
+    sol_err = float(np.linalg.norm(mixed["state"]["solution"] - fp64["state"]["solution"]) /
+                    np.linalg.norm(fp64["state"]["solution"]))
+    nsys = mixed["state"]["solution"].size
+
+    def summ(leg):
+        st = leg["stats"]
+        return {"ms_per_step": leg["ms_per_step"],
+                "timesteps_per_s": 1000.0 / leg["ms_per_step"],
+                "gmres_iterations_mean": float(np.mean([s["iterations"] for s in st])),
+                "fp32_matvecs_per_step": float(np.mean([s["fp32_matvecs"] for s in st])),
+                "fp64_matvecs_per_step": float(np.mean([s["fp64_matvecs"] for s in st])),
+                "refinements_mean": float(np.mean([s["refinements"] for s in st])),
+                "final_relres_max": float(max(s["relres"] for s in st)),
+                "converged_all": bool(all(s["converged"] for s in st))}
+
+    # Dominant kernel of the headline: the FP32-tile <3,3> pair kernel.
+    flops3 = matvec_flops(ns, nb, 3)
+    k32_ms, k32_n = mixed["kernels"][L.PRECISION_FP32_ACC64]
+    k64_ms, k64_n = fp64["kernels"][L.PRECISION_FP64]
+    tf32 = flops3 * k32_n / (k32_ms * 1e-3) / 1e12 if k32_n else 0.0
+    tf64 = flops3 * k64_n / (k64_ms * 1e-3) / 1e12 if k64_n else 0.0
+
+    # ---- 6-DOF matvec legs ---------------------------------------------------
+    print("bench: 6-DOF matvec legs", file=sys.stderr, flush=True)
     h = RpyHandle(ns, nb, seg_dof=6, precision=L.PRECISION_FP64, bs_accumulate=True, stream=stream)
     h.upload(L.SEG_POSITIONS, body.x_segs)
     h.upload(L.BLOB_POSITIONS, body.x_blobs)
     h.upload(L.SEG_FORCES, body.f_segs)
     h.upload(L.BLOB_FORCES, body.f_blobs)
     h.sync()
-
-    fp64 = matvec_leg(torch, h, L.PRECISION_FP64, args.steps, args.warmup)
+    mv64 = matvec_leg(torch, h, L.PRECISION_FP64, args.matvec_steps, 3)
     v64s, v64b = np.zeros(6 * ns), np.zeros(3 * nb)
     h.download(L.SEG_VELOCITIES, v64s)
     h.download(L.BLOB_VELOCITIES, v64b)
     h.sync()
-    mixed = matvec_leg(torch, h, L.PRECISION_FP32_ACC64, args.steps, args.warmup)
+    mv32 = matvec_leg(torch, h, L.PRECISION_FP32_ACC64, args.matvec_steps, 3)
     v32s, v32b = np.zeros(6 * ns), np.zeros(3 * nb)
     h.download(L.SEG_VELOCITIES, v32s)
     h.download(L.BLOB_VELOCITIES, v32b)
     h.sync()
+    h.close()
     v64 = np.concatenate([v64s, v64b])
-    err = float(np.linalg.norm(np.concatenate([v32s, v32b]) - v64) / np.linalg.norm(v64))
+    mv_err = float(np.linalg.norm(np.concatenate([v32s, v32b]) - v64) / np.linalg.norm(v64))
+    flops6 = matvec_flops(ns, nb, 6)
+    bytes6 = matvec_bytes(ns, nb, 6)
 
-    flops = matvec_flops(ns, nb, 6)
-    nbytes = matvec_bytes(ns, nb, 6)
-    gbps_mixed = nbytes / (mixed["ms_per_step"] * 1e-3) / 1e9
-    gbps_fp64 = nbytes / (fp64["ms_per_step"] * 1e-3) / 1e9
-    tf_mixed_kernel = flops / (mixed["pair_kernel_ms"] * 1e-3) / 1e12
-    tf_fp64_kernel = flops / (fp64["pair_kernel_ms"] * 1e-3) / 1e12
+    def mv(leg, peak):
+        t = leg["pair_kernel_ms"] * 1e-3
+        return {"ms": leg["ms_per_step"], "GB_s": bytes6 / (leg["ms_per_step"] * 1e-3) / 1e9,
+                "pair_kernel_ms": leg["pair_kernel_ms"], "TFLOP_s": flops6 / t / 1e12,
+                "valu_frac": flops6 / t / 1e12 / peak}
 
     cpu = None
     if not args.no_cpu_baseline:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        per_step = summ(fp64)["fp64_matvecs_per_step"]
         t_cpu, reps, cores = cpu_baseline(body)
-        cpu = {"value": nbytes / t_cpu / 1e9, "unit": "GB/s (matvec, compulsory FP64 bytes)",
-               "cores": cores, "kind": "port",
-               "sample": f"full production-body matvec (6*12800+3*9000), FP64 oracle, OpenMP, "
-                         f"median of {reps} after 1 warm-up; {t_cpu * 1e3:.1f} ms per matvec, "
-                         f"{flops / t_cpu / 1e9:.1f} GFLOP/s"}
+        cpu = {"value": 1.0 / (t_cpu * per_step), "unit": "timesteps/s", "cores": cores, "kind": "port",
+               "sample": (f"FP64 oracle direct sum (OpenMP) of the production body's <3,3> matvec "
+                          f"(21,800 particles), median of {reps} after 1 warm-up: {t_cpu * 1e3:.1f} ms "
+                          f"({flops3 / t_cpu / 1e9:.1f} GFLOP/s); a CPU timestep is counted as "
+                          f"{per_step:.1f} such matvecs, the FP64 GPU solve's mean per step")}
 
     out = {
         "metric": "timesteps/sec + RPY matvec GB/s at 12.8k segs/9k blobs; mixed vs FP64 @1e-6",
-        "value": gbps_mixed,
-        "unit": "GB/s (RPY matvec, compulsory FP64 bytes / time)",
+        "value": ts_mixed,
+        "unit": "timesteps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -160,29 +234,30 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair tensors, f64 accumulation",
-        "data": "synthetic (seeded Fibonacci sphere + straight filaments, N(0,1) forces/torques)",
-        "config": {"workload": "single-body RPY grand-mobility matvec, 12,800 segments (6 DOF) + "
-                               "9,000 blobs, R=50a, a=1, mu=1",
-                   "num_segs": ns, "num_blobs": nb, "matvec_dim": 6 * ns + 3 * nb},
-        "matvec": {
-            "mixed": {"ms": mixed["ms_per_step"], "GB_s": gbps_mixed,
-                      "pair_kernel_ms": mixed["pair_kernel_ms"], "TFLOP_s": tf_mixed_kernel,
-                      "rel_l2_err_vs_fp64": err},
-            "fp64": {"ms": fp64["ms_per_step"], "GB_s": gbps_fp64,
-                     "pair_kernel_ms": fp64["pair_kernel_ms"], "TFLOP_s": tf_fp64_kernel},
-            "speedup_mixed_over_fp64": fp64["ms_per_step"] / mixed["ms_per_step"],
-            "algorithmic_gflop": flops / 1e9,
-        },
-        "roofline": {"bound": "valu", "achieved": tf_mixed_kernel, "peak": PEAK_FP32_VALU_TFLOPS,
-                     "unit": "TFLOP/s", "frac": tf_mixed_kernel / PEAK_FP32_VALU_TFLOPS,
-                     "traffic": None,
-                     "kernel": "rpy_pairs_kernel<float,6>",
-                     "fp64_kernel": {"achieved": tf_fp64_kernel, "peak": PEAK_FP64_VALU_TFLOPS,
-                                     "frac": tf_fp64_kernel / PEAK_FP64_VALU_TFLOPS}},
+        "dtype": "f32 pair tensors with f64 accumulation inside f64 iterative refinement",
+        "data": "synthetic (seeded Fibonacci sphere + filaments, Fulford-Blake prescribed beat)",
+        "config": {"workload": "prescribed-beat timestep, one body: 640 filaments x 20 segments + "
+                               "9,000 blobs, R=50a, dt=T/300, GMRES(50) to 1e-8, dynamic phase "
+                               "and shape rotation",
+                   "num_segs": ns, "num_blobs": nb, "system_size": nsys},
+        "timestep": {"mixed": summ(mixed), "fp64": summ(fp64),
+                     "speedup_mixed_over_fp64": ts_mixed / ts_fp64,
+                     "trajectory_dev_over_DL_after_steps": dev,
+                     "solution_rel_err_mixed_vs_fp64": sol_err,
+                     "warmup_plus_timed_steps": args.warmup + args.steps},
+        "matvec": {"workload": "6-DOF grand mobility, dim 6*12800+3*9000",
+                   "fp64": mv(mv64, PEAK_FP64_VALU_TFLOPS), "mixed": mv(mv32, PEAK_FP32_VALU_TFLOPS),
+                   "mixed_rel_l2_err_vs_fp64": mv_err,
+                   "algorithmic_gflop": flops6 / 1e9, "compulsory_bytes": bytes6},
+        "roofline": {"bound": "valu", "achieved": tf32, "peak": PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tf32 / PEAK_FP32_VALU_TFLOPS, "traffic": None,
+                     "kernel": "rpy_pairs_kernel<float,3> (timestep matvec, 17.11 GFLOP per launch)",
+                     "launches": k32_n, "avg_ms": k32_ms / max(k32_n, 1),
+                     "fp64_kernel": {"achieved": tf64, "peak": PEAK_FP64_VALU_TFLOPS,
+                                     "frac": tf64 / PEAK_FP64_VALU_TFLOPS,
+                                     "avg_ms": k64_ms / max(k64_n, 1), "launches": k64_n}},
         "cpu_baseline": cpu,
     }
-    h.close()
     print(json.dumps(out), flush=True)
 
 

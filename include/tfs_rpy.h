@@ -127,10 +127,11 @@ int tfs_rpy_device_buffer(tfs_rpy *h, int32_t what, double **out);
 /* Live kernel timing (no counterpart in the reference, whose DISPLAYTIME
  * buckets bracket whole phases with cudaDeviceSynchronize, mobility_solver.cpp:2576-2657):
  * while enabled, HIP events are recorded on the handle's stream around every
- * pair-kernel launch; profile_read synchronises, returns the summed kernel
- * milliseconds and the launch count since the last read, and resets them. */
+ * pair-kernel launch.  profile_read synchronises and returns, indexed by
+ * precision (TFS_PRECISION_FP64, TFS_PRECISION_FP32_ACC64), the summed kernel
+ * milliseconds and launch counts since the last read, then resets them. */
 int tfs_rpy_profile(tfs_rpy *h, int32_t enable);
-int tfs_rpy_profile_read(tfs_rpy *h, double *pair_ms_total, int32_t *launches);
+int tfs_rpy_profile_read(tfs_rpy *h, double pair_ms[2], int32_t launches[2]);
 
 /* Pinned host memory, as the reference's cudaHostAlloc(..., cudaHostAllocPortable)
  * host arrays (rpy_mobility_solver.cu:65-71; freed at :14-27). */

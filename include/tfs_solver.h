@@ -92,6 +92,10 @@ int tfs_solver_solve(tfs_solver *s, const double *rhs, double *x, int32_t memory
 int tfs_solver_residual(tfs_solver *s, const double *rhs, const double *x, int32_t precision,
                         int32_t memory, double *relres);
 
+/* The RPY matvec handle the solver evaluates with (for tfs_rpy_profile,
+ * tfs_rpy_set_tile_shape, ...).  Owned by the solver. */
+int tfs_solver_rpy(tfs_solver *s, tfs_rpy **out);
+
 /* ---- Prescribed-beat timestep driver (one body, body velocity prescribed) ---- */
 typedef struct tfs_beat_desc {
     tfs_solver_desc solver;

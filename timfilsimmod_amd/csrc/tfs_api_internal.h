@@ -40,7 +40,11 @@ struct tfs_rpy {
     tfs::TileShape shape[2];
     // Live kernel timing: event pairs recorded around each pair-kernel launch.
     bool profiling = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used, ev_free;
+    struct EvPair {
+        hipEvent_t begin, end;
+        int precision;
+    };
+    std::vector<EvPair> ev_used, ev_free;
     std::map<std::tuple<uint32_t, int, int, int>, tfs::Plan> plans;
 };
 

@@ -126,12 +126,14 @@ int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, V
             hipError_t ce = hipEventCreate(&a0);
             if (ce == hipSuccess) ce = hipEventCreate(&a1);
             if (ce != hipSuccess) return hip_fail(ce, "profiling events");
-            h->ev_free.push_back({a0, a1});
+            h->ev_free.push_back({a0, a1, 0});
         }
-        eb = h->ev_free.back().first;
-        ee = h->ev_free.back().second;
-        h->ev_used.push_back(h->ev_free.back());
+        tfs_rpy::EvPair ep = h->ev_free.back();
         h->ev_free.pop_back();
+        ep.precision = precision;
+        eb = ep.begin;
+        ee = ep.end;
+        h->ev_used.push_back(ep);
     }
     hipError_t e = launch_matvec(precision, h->desc.seg_dof, sh, a, o, p->d_tiles, (int)p->tiles.size(),
                                  p->d_items, (int)p->items.size(), h->partial, h->stream, eb, ee);
@@ -222,8 +224,8 @@ int tfs_rpy_destroy(tfs_rpy* h) {
     }
     for (auto* v : {&h->ev_used, &h->ev_free})
         for (auto& pr : *v) {
-            hipEventDestroy(pr.first);
-            hipEventDestroy(pr.second);
+            hipEventDestroy(pr.begin);
+            hipEventDestroy(pr.end);
         }
     if (h->partial) hipFree(h->partial);
     if (h->arena) hipFree(h->arena);
@@ -353,21 +355,20 @@ int tfs_rpy_profile(tfs_rpy* h, int32_t enable) {
     return TFS_OK;
 }
 
-int tfs_rpy_profile_read(tfs_rpy* h, double* pair_ms_total, int32_t* launches) {
-    if (!h || !pair_ms_total || !launches)
-        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_profile_read: null argument");
+int tfs_rpy_profile_read(tfs_rpy* h, double* pair_ms, int32_t* launches) {
+    if (!h || !pair_ms || !launches) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_profile_read: null argument");
     hipError_t e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return hip_fail(e, "tfs_rpy_profile_read");
-    double total = 0.0;
+    pair_ms[0] = pair_ms[1] = 0.0;
+    launches[0] = launches[1] = 0;
     for (auto& pr : h->ev_used) {
         float ms = 0.0f;
-        e = hipEventElapsedTime(&ms, pr.first, pr.second);
+        e = hipEventElapsedTime(&ms, pr.begin, pr.end);
         if (e != hipSuccess) return hip_fail(e, "tfs_rpy_profile_read: elapsed");
-        total += ms;
+        pair_ms[pr.precision] += ms;
+        launches[pr.precision] += 1;
         h->ev_free.push_back(pr);
     }
-    *pair_ms_total = total;
-    *launches = (int32_t)h->ev_used.size();
     h->ev_used.clear();
     return TFS_OK;
 }

@@ -356,6 +356,12 @@ int tfs_solver_destroy(tfs_solver* s) {
     return TFS_OK;
 }
 
+int tfs_solver_rpy(tfs_solver* s, tfs_rpy** out) {
+    if (!s || !out) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_rpy: null argument");
+    *out = s->rpy;
+    return TFS_OK;
+}
+
 int tfs_solver_size(tfs_solver* s, int64_t* n) {
     if (!s || !n) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_size: null argument");
     *n = s->dims.n;

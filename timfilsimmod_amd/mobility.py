@@ -35,6 +35,13 @@ class PinnedArray:
             self._p = ctypes.c_void_p()
 
 
+def profile_read(h):
+    ms = (ctypes.c_double * 2)()
+    n = (ctypes.c_int32 * 2)()
+    L.check(L.lib().tfs_rpy_profile_read(h, ms, n))
+    return {L.PRECISION_FP64: (ms[0], n[0]), L.PRECISION_FP32_ACC64: (ms[1], n[1])}
+
+
 class RpyHandle:
     """Thin owner of one ``tfs_rpy`` handle."""
 
@@ -94,11 +101,8 @@ class RpyHandle:
         L.check(L.lib().tfs_rpy_profile(self.h, int(enable)))
 
     def profile_read(self):
-        """(summed pair-kernel milliseconds, launches) since the last read."""
-        ms = ctypes.c_double()
-        n = ctypes.c_int32()
-        L.check(L.lib().tfs_rpy_profile_read(self.h, ctypes.byref(ms), ctypes.byref(n)))
-        return ms.value, n.value
+        """{precision: (summed pair-kernel ms, launches)} since the last read."""
+        return profile_read(self.h)
 
     def device_buffer(self, what: int) -> int:
         p = ctypes.c_void_p()

@@ -230,6 +230,13 @@ class BeatSimulation:
     def set_mode(self, mode: int):
         L.check(L.lib().tfs_beat_set_mode(self.h, mode))
 
+    def rpy_handle(self) -> ctypes.c_void_p:
+        """The tfs_rpy handle the driver's solver evaluates with."""
+        sp, rp = ctypes.c_void_p(), ctypes.c_void_p()
+        L.check(L.lib().tfs_beat_solver(self.h, ctypes.byref(sp)))
+        L.check(L.lib().tfs_solver_rpy(sp, ctypes.byref(rp)))
+        return rp
+
     def step(self):
         st = SolveStats()
         L.check(L.lib().tfs_beat_step(self.h, ctypes.byref(st)))
