@@ -1,7 +1,7 @@
 """MI355X-native (gfx950) RPY mobility hot path of TimFilSim.
 
-Submodules: mobility (matvec back-end), solver (per-timestep saddle-point
+Submodules: mobility (matvec back-end), flow (flow-field evaluator), solver (per-timestep saddle-point
 solve and prescribed-beat driver), geometry and beat (synthetic inputs).
 All compute runs in libtfs_rpy.so (include/tfs_rpy.h, include/tfs_solver.h).
 """
-from . import _lib, geometry, beat, mobility, solver  # noqa: F401
+from . import _lib, geometry, beat, mobility, solver, flow  # noqa: F401

@@ -82,18 +82,20 @@ hipError_t launch_matvec(int precision, int seg_dof, const TileShape& shape, con
 // Flow-field evaluation (Stokeslet + rotlet mode and RPY mode).
 struct FlowArgs {
     CView xt;          // targets (3 components)
-    View vt;           // velocities out
-    int nt;
-    CView xs, fs;      // segment sources (6 components)
-    CView xb, fb;      // blob sources (3 components)
+    View vt;           // velocities out (3 components)
+    int64_t nt;
+    CView xs, fs;      // segment sources: positions, force+torque (6 components)
+    CView xb, fb;      // blob sources: positions, force (3 components)
     int ns_total, nb_total;
-    int ns_per_body, nb_per_body;
-    const double* x_bodies;  // 3 per body (AoS), for net-force removal
+    int ns_per_body, nb_per_body, num_bodies;
+    const double* x_bodies;  // 3 per body, for net-force removal
     int remove_net_force;
-    int mode;          // 0 Stokeslet+rotlet (+Blakelet without blobs), 1 RPY
+    int mode;          // 0 Stokeslet + rotlet (+ Blakelet without blobs), 1 RPY point tracer
     double a_seg, a_blob, mu;
 };
-hipError_t launch_flow(int precision, const FlowArgs& a, int block, hipStream_t stream);
+// net[3*b..] = total force on body b (segments + blobs).
+hipError_t launch_flow_net(const FlowArgs& a, double* net, int nbod, hipStream_t stream);
+hipError_t launch_flow_eval(int precision, const FlowArgs& a, const double* net, int block, hipStream_t stream);
 
 }  // namespace tfs
 
