@@ -106,6 +106,54 @@ def timestep_leg(torch, body, mode: int, steps: int, warmup: int, stream):
             "kernels": kern, "state": state}
 
 
+def flow_leg(torch, stream):
+    """Flow-field velocities of the flow-source body (1,280 x 20 segments +
+    9,000 blobs, N(0,1) forces/torques) at a 2048^2 plane through the centre
+    and a 256^3 volume grid, Stokeslet+rotlet mode, device-resident points."""
+    from timfilsimmod_amd import _lib as L
+    from timfilsimmod_amd import geometry as G
+    from timfilsimmod_amd.flow import STOKESLET_ROTLET, FlowFieldEvaluator
+    body = G.flow_source_body()
+    half = body.radius + 1.2 * body.dl * (body.num_seg - 1)
+    dev = torch.device("cuda", 0)
+    t = torch.linspace(-half, half, 2048, dtype=torch.float64, device=dev)
+    gy, gx = torch.meshgrid(t, t, indexing="ij")
+    plane = torch.stack([gx.reshape(-1), gy.reshape(-1), torch.zeros_like(gx).reshape(-1)], 1).contiguous()
+    t3 = torch.linspace(-half, half, 256, dtype=torch.float64, device=dev)
+    gz, gy3, gx3 = torch.meshgrid(t3, t3, t3, indexing="ij")
+    vol = torch.stack([gx3.reshape(-1), gy3.reshape(-1), gz.reshape(-1)], 1).contiguous()
+    del gz, gy3, gx3, gx, gy
+    srcs = [torch.tensor(a, dtype=torch.float64, device=dev) for a in
+            (body.x_segs, body.f_segs, body.x_blobs, body.f_blobs, body.centre)]
+    flops_per_target = 44.0 * body.num_segs + 28.0 * body.num_blobs
+    out = {"sources": {"segments": body.num_segs, "blobs": body.num_blobs},
+           "algorithmic_flop_per_target": flops_per_target}
+    ref_plane = None
+    for prec in ("fp64", "fp32"):
+        ev = FlowFieldEvaluator(1, body.num_segs, body.num_blobs, mode=STOKESLET_ROTLET, precision=prec,
+                                memory=L.MEMORY_DEVICE, stream=stream)
+        ev.set_sources(*srcs)
+        for name, pts in (("plane_2048x2048", plane), ("volume_256x256x256", vol)):
+            v = torch.empty_like(pts)
+            small = pts[:4096].contiguous()
+            ev.velocities(small, torch.empty_like(small))      # warm-up
+            ev.velocities(pts, v)
+            ms = ev.last_kernel_ms()
+            n = pts.shape[0]
+            entry = {"points": n, "kernel_ms": ms, "points_per_s": n / (ms * 1e-3),
+                     "TFLOP_s": n * flops_per_target / (ms * 1e-3) / 1e12}
+            if name.startswith("plane"):
+                if prec == "fp64":
+                    ref_plane = v.clone()
+                else:
+                    entry["rel_l2_err_vs_fp64"] = float(torch.linalg.norm(v - ref_plane) /
+                                                        torch.linalg.norm(ref_plane))
+            out[f"{name}_{prec}"] = entry
+            del v
+        ev.close()
+    return out
+
+
 def cpu_baseline(body, budget_s: float = 15.0):
     """FP64 oracle direct sum on the host cores (reported, not gating): the
     prescribed-beat <3,3> matvec of the production body, timed."""
@@ -212,6 +260,10 @@ def main():
                 "pair_kernel_ms": leg["pair_kernel_ms"], "TFLOP_s": flops6 / t / 1e12,
                 "valu_frac": flops6 / t / 1e12 / peak}
 
+    print("bench: flow-field leg", file=sys.stderr, flush=True)
+    flow = flow_leg(torch, stream)
+    torch.cuda.empty_cache()
+
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
@@ -249,6 +301,7 @@ def main():
                    "fp64": mv(mv64, PEAK_FP64_VALU_TFLOPS), "mixed": mv(mv32, PEAK_FP32_VALU_TFLOPS),
                    "mixed_rel_l2_err_vs_fp64": mv_err,
                    "algorithmic_gflop": flops6 / 1e9, "compulsory_bytes": bytes6},
+        "flow_field": flow,
         "roofline": {"bound": "valu", "achieved": tf32, "peak": PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
                      "frac": tf32 / PEAK_FP32_VALU_TFLOPS, "traffic": None,
                      "kernel": "rpy_pairs_kernel<float,3> (timestep matvec, 17.11 GFLOP per launch)",

@@ -124,6 +124,18 @@ int tfs_rpy_set_tile_shape(tfs_rpy *h, int32_t precision, int32_t block, int32_t
  * AoS reference layout), for callers that keep data resident on the GPU. */
 int tfs_rpy_device_buffer(tfs_rpy *h, int32_t what, double **out);
 
+/* Positions were written in place through tfs_rpy_device_buffer: the
+ * overlapping-pair lists must be rebuilt before the next evaluation. */
+int tfs_rpy_positions_changed(tfs_rpy *h);
+
+/* Independent bodies: the arrays hold num_bodies bodies one after another
+ * (segs_per_body[i] segments and blobs_per_body[i] blobs each, summing to the
+ * handle's sizes); evaluations couple particles of the same body only, i.e.
+ * a block-diagonal grand mobility.  The reference couples all NSWIM bodies
+ * (cuda_functions.cu:399, 504, 580, 675); one body is the default. */
+int tfs_rpy_set_bodies(tfs_rpy *h, int32_t num_bodies, const int32_t *segs_per_body,
+                       const int32_t *blobs_per_body);
+
 /* Live kernel timing (no counterpart in the reference, whose DISPLAYTIME
  * buckets bracket whole phases with cudaDeviceSynchronize, mobility_solver.cpp:2576-2657):
  * while enabled, HIP events are recorded on the handle's stream around every

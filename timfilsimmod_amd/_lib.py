@@ -69,6 +69,8 @@ SIGNATURES = {
     "tfs_rpy_profile": (ctypes.c_int, [_vp, _i32]),
     "tfs_rpy_profile_read": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i32)]),
     "tfs_solver_rpy": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "tfs_rpy_positions_changed": (ctypes.c_int, [_vp]),
+    "tfs_rpy_set_bodies": (ctypes.c_int, [_vp, _i32, _vp, _vp]),
     "tfs_host_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "tfs_host_free": (ctypes.c_int, [_vp]),
     "tfs_last_error": (ctypes.c_char_p, []),

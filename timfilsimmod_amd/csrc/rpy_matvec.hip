@@ -2,14 +2,22 @@
 //
 // Replaces the four reference kernels Mss_mult / Msb_mult / Mbb_mult /
 // Mbs_mult (src/cilia/cuda_functions.cu:362-757) with one launch over a work
-// list.  A work item pairs a tile of targets (segments or blobs) with a chunk
-// of sources (segments or blobs).  A workgroup stages its chunk through LDS in
-// batches of `BLOCK` sources (one coalesced load per lane, converted to the
-// evaluation type as it is stored), every lane then walks the batch with
-// broadcast LDS reads and accumulates PER_THREAD targets in registers.  Each
-// item writes its partial sums to its own slot; a second kernel adds the slots
-// of a tile in a fixed order and applies the reference's write rule (= or +=)
-// -- so results are deterministic run to run, unlike atomics.
+// list.  A work item pairs a tile of targets (segments or blobs of one body)
+// with a chunk of sources (segments or blobs of the same body).  A workgroup
+// stages its chunk through LDS in batches of BLOCK sources (one coalesced
+// load per lane, converted to the evaluation type as it is stored); every
+// lane then walks the batch with broadcast LDS reads and accumulates
+// PER_THREAD targets in registers.  Each item writes its partial sums to its
+// own slot; a second kernel adds the slots of a tile in a fixed order and
+// applies the reference's write rule (= or +=), so results are deterministic
+// run to run, unlike atomics.
+//
+// The pair loop is branch-free: it evaluates the far-field RPY forms and
+// drops (by a select) every pair with r^2 <= (ai+aj)^2, which includes the
+// self pair.  Those overlapping pairs -- a handful per particle -- are listed
+// once per geometry by build_near_list, using the pair loop's own arithmetic
+// for r^2, and the reduce kernel adds them with the reference's overlap and
+// inner-branch formulas in FP64.  Every pair is counted exactly once.
 //
 // Splitting the source range into chunks is what fills the chip: 21,800
 // targets are only 341 wave64s, far fewer than the 1,024 SIMDs of an MI355X;
@@ -21,6 +29,8 @@
 //            so near pairs -- the ones that dominate -- get an exact hi
 //            difference, by Sterbenz's lemma); per-batch FP32 partial sums are
 //            flushed into FP64 accumulators after every LDS batch.
+#include <algorithm>
+
 #include "rpy_pair.h"
 #include "tfs_internal.h"
 
@@ -35,7 +45,56 @@ constexpr int rec_len() {
     if constexpr (sizeof(T) == 8) {
         return FS == 6 ? 10 : 6;   // x y z f0..f(FS-1) [pad]
     } else {
-        return 12;                 // xh yh zh xl yl zl f0..f5 (3-dof: padded)
+        return 12;                 // xh yh zh xl yl zl f0..f(FS-1) [pad to 16 B]
+    }
+}
+
+// Separation and squared distance exactly as the pair loop evaluates them;
+// build_near_list calls the same function so the two agree bit for bit.
+template <typename T>
+__device__ __forceinline__ T separation(const double* xi, const T* xh, const T* xl, const T* rec, T& dx, T& dy,
+                                        T& dz) {
+    if constexpr (sizeof(T) == 4) {
+        dx = (xh[0] - rec[0]) + (xl[0] - rec[3]);
+        dy = (xh[1] - rec[1]) + (xl[1] - rec[4]);
+        dz = (xh[2] - rec[2]) + (xl[2] - rec[5]);
+    } else {
+        dx = xi[0] - rec[0];
+        dy = xi[1] - rec[1];
+        dz = xi[2] - rec[2];
+    }
+    return __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+}
+
+template <typename T>
+__device__ __forceinline__ void split_position(double x, T& h, T& l) {
+    if constexpr (sizeof(T) == 4) {
+        h = T(x);
+        l = T(x - double(h));
+    } else {
+        h = T(x);
+        l = T(0);
+    }
+}
+
+template <typename T, int FS>
+__device__ __forceinline__ void stage_source(T* rec, const CView& xsrc, const CView& fsrc, int s, bool with_force) {
+    const double p0 = xsrc.p[(int64_t)s * xsrc.ld_p];
+    const double p1 = xsrc.p[(int64_t)s * xsrc.ld_p + xsrc.ld_c];
+    const double p2 = xsrc.p[(int64_t)s * xsrc.ld_p + 2 * xsrc.ld_c];
+    constexpr int F0 = sizeof(T) == 4 ? 6 : 3;
+    if constexpr (sizeof(T) == 4) {
+        split_position(p0, rec[0], rec[3]);
+        split_position(p1, rec[1], rec[4]);
+        split_position(p2, rec[2], rec[5]);
+    } else {
+        rec[0] = p0;
+        rec[1] = p1;
+        rec[2] = p2;
+    }
+    if (with_force) {
+#pragma unroll
+        for (int k = 0; k < FS; ++k) rec[F0 + k] = T(fsrc.p[(int64_t)s * fsrc.ld_p + k * fsrc.ld_c]);
     }
 }
 
@@ -44,7 +103,7 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
                                          double* __restrict__ slot, T* __restrict__ lds) {
     constexpr int REC = rec_len<T, FS>();
     constexpr int TILE = BLOCK * R;
-    constexpr bool F32 = sizeof(T) == 4;
+    constexpr int F0 = sizeof(T) == 4 ? 6 : 3;
     const int tid = threadIdx.x;
 
     const CView xt = tl.cls == 0 ? a.xs : a.xb;
@@ -52,11 +111,10 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
     const CView fsrc = it.src_cls == 0 ? a.fs : a.fb;
     const double ai = tl.cls == 0 ? a.a_seg : a.a_blob;
     const double aj = it.src_cls == 0 ? a.a_seg : a.a_blob;
-    const double mu = a.mu;
     const double PI = 3.14159265358979323846;
     const T sum_a2 = T((ai + aj) * (ai + aj));
     const T s3 = T((ai * ai + aj * aj) / 3.0);
-    const T c8 = T(1.0 / (8.0 * PI * mu));
+    const T c8 = T(1.0 / (8.0 * PI * a.mu));
 
     // Targets of this lane (clamped into the tile; surplus lanes compute a
     // duplicate and their slot entries are never read).
@@ -68,104 +126,55 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
         t = t < tl.t_end ? t : tl.t_end - 1;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            xi[r][k] = xt.p[t * xt.ld_p + k * xt.ld_c];
-            if constexpr (F32) {
-                xh[r][k] = T(xi[r][k]);
-                xl[r][k] = T(xi[r][k] - double(xh[r][k]));
-            }
+            xi[r][k] = xt.p[(int64_t)t * xt.ld_p + k * xt.ld_c];
+            split_position(xi[r][k], xh[r][k], xl[r][k]);
         }
     }
 
+    // FP64 accumulates straight into acc; FP32 sums one LDS batch in `part`
+    // and flushes it into the FP64 accumulators.
+    constexpr bool F32 = sizeof(T) == 4;
     double acc[R][VT];
-    T part[R][VT];
+    T part[R][F32 ? VT : 1];
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
-        for (int k = 0; k < VT; ++k) {
-            acc[r][k] = 0.0;
-            part[r][k] = T(0);
-        }
+        for (int k = 0; k < VT; ++k) acc[r][k] = 0.0;
 
     for (int s0 = it.s_begin; s0 < it.s_end; s0 += BLOCK) {
         const int n = (it.s_end - s0) < BLOCK ? (it.s_end - s0) : BLOCK;
         __syncthreads();  // the previous batch has been consumed
-        if (tid < n) {
-            const int s = s0 + tid;
-            T* rec = lds + tid * REC;
-            double xs0 = xsrc.p[s * xsrc.ld_p];
-            double xs1 = xsrc.p[s * xsrc.ld_p + xsrc.ld_c];
-            double xs2 = xsrc.p[s * xsrc.ld_p + 2 * xsrc.ld_c];
-            if constexpr (F32) {
-                const float h0 = float(xs0), h1 = float(xs1), h2 = float(xs2);
-                rec[0] = h0; rec[1] = h1; rec[2] = h2;
-                rec[3] = float(xs0 - double(h0));
-                rec[4] = float(xs1 - double(h1));
-                rec[5] = float(xs2 - double(h2));
-#pragma unroll
-                for (int k = 0; k < FS; ++k) rec[6 + k] = float(fsrc.p[s * fsrc.ld_p + k * fsrc.ld_c]);
-            } else {
-                rec[0] = xs0; rec[1] = xs1; rec[2] = xs2;
-#pragma unroll
-                for (int k = 0; k < FS; ++k) rec[3 + k] = fsrc.p[s * fsrc.ld_p + k * fsrc.ld_c];
-            }
-        }
+        if (tid < n) stage_source<T, FS>(lds + tid * REC, xsrc, fsrc, s0 + tid, true);
         __syncthreads();
+        if constexpr (F32) {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int k = 0; k < VT; ++k) part[r][k] = T(0);
+        }
 
 #pragma unroll 2
         for (int j = 0; j < n; ++j) {
             const T* rec = lds + j * REC;
             T f[FS];
-            if constexpr (F32) {
 #pragma unroll
-                for (int k = 0; k < FS; ++k) f[k] = rec[6 + k];
-            } else {
-#pragma unroll
-                for (int k = 0; k < FS; ++k) f[k] = rec[3 + k];
-            }
+            for (int k = 0; k < FS; ++k) f[k] = rec[F0 + k];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 T dx, dy, dz;
-                if constexpr (F32) {
-                    dx = (xh[r][0] - rec[0]) + (xl[r][0] - rec[3]);
-                    dy = (xh[r][1] - rec[1]) + (xl[r][1] - rec[4]);
-                    dz = (xh[r][2] - rec[2]) + (xl[r][2] - rec[5]);
-                } else {
-                    dx = xi[r][0] - rec[0];
-                    dy = xi[r][1] - rec[1];
-                    dz = xi[r][2] - rec[2];
-                }
-                const T r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 > sum_a2) {
-                    const T rm1 = rsqrt_refined(r2);
-                    if constexpr (F32) {
-                        pair_far<T, VT, FS>(part[r], dx, dy, dz, rm1, f, s3, c8);
-                    } else {
-                        pair_far<T, VT, FS>(acc[r], dx, dy, dz, rm1, f, s3, c8);
-                    }
-                } else {
-                    double fd[FS];
-#pragma unroll
-                    for (int k = 0; k < FS; ++k) fd[k] = double(f[k]);
-                    double ddx, ddy, ddz;
-                    if constexpr (F32) {
-                        ddx = xi[r][0] - (double(rec[0]) + double(rec[3]));
-                        ddy = xi[r][1] - (double(rec[1]) + double(rec[4]));
-                        ddz = xi[r][2] - (double(rec[2]) + double(rec[5]));
-                    } else {
-                        ddx = dx; ddy = dy; ddz = dz;
-                    }
-                    pair_near<VT, FS>(acc[r], ddx, ddy, ddz, fd, ai, aj, mu);
-                }
+                const T r2 = separation<T>(xi[r], xh[r], xl[r], rec, dx, dy, dz);
+                // Overlapping pairs (and the self pair) are added by the reduce
+                // kernel; a zero 1/r removes them here without a branch.
+                const T rm1 = r2 > sum_a2 ? rsqrt_refined(r2) : T(0);
+                if constexpr (F32) pair_far<T, VT, FS>(part[r], dx, dy, dz, rm1, f, s3, c8);
+                else pair_far<T, VT, FS>(acc[r], dx, dy, dz, rm1, f, s3, c8);
             }
         }
         if constexpr (F32) {
 #pragma unroll
             for (int r = 0; r < R; ++r)
 #pragma unroll
-                for (int k = 0; k < VT; ++k) {
-                    acc[r][k] += double(part[r][k]);
-                    part[r][k] = T(0);
-                }
+                for (int k = 0; k < VT; ++k) acc[r][k] += double(part[r][k]);
         }
     }
 
@@ -193,28 +202,125 @@ __global__ __launch_bounds__(BLOCK) void rpy_pairs_kernel(const MatvecItem* __re
     }
 }
 
-// Adds the partial slots of each tile in item order and applies the write rule.
-template <int SEGDOF>
-__global__ void rpy_reduce_kernel(const MatvecTile* __restrict__ tiles, const double* __restrict__ partial,
-                                  int tile_targets, OutArgs o) {
-    const MatvecTile tl = tiles[blockIdx.x];
-    const int vt = tl.cls == 0 ? SEGDOF : 3;
-    const int slots = tl.cls == 0 ? o.seg_out_slots : 3;
-    const View out = tl.cls == 0 ? o.vs : o.vb;
-    const size_t slot_len = (size_t)kMaxDof * tile_targets;
-    for (int l = threadIdx.x; l < tile_targets; l += blockDim.x) {
-        const int t = tl.t_begin + l;
-        if (t >= tl.t_end) break;
-        for (int k = 0; k < slots; ++k) {
-            double s = 0.0;
-            if (k < vt) {
-                const double* p = partial + (size_t)tl.first_item * slot_len + (size_t)k * tile_targets + l;
-                for (int m = 0; m < tl.n_items; ++m) s += p[(size_t)m * slot_len];
+// Near-pair listing: the same work decomposition as the pair kernel (one
+// target per lane, BLOCK = 256), testing r^2 <= (ai+aj)^2 with the pair
+// kernel's arithmetic.  Entries are appended with atomics; the reduce kernel
+// sorts them before summing.
+template <typename T>
+__global__ __launch_bounds__(256) void near_build_kernel(const MatvecItem* __restrict__ items,
+                                                         const MatvecTile* __restrict__ tiles, MatvecArgs a,
+                                                         int32_t* __restrict__ cnt, int32_t* __restrict__ idx,
+                                                         int cap, int32_t* __restrict__ max_cnt) {
+    constexpr int REC = rec_len<T, 3>();
+    __shared__ __align__(16) T lds[256 * REC];
+    const MatvecItem it = items[blockIdx.x];
+    const MatvecTile tl = tiles[it.tile];
+    const int tid = threadIdx.x;
+    const CView xt = tl.cls == 0 ? a.xs : a.xb;
+    const CView xsrc = it.src_cls == 0 ? a.xs : a.xb;
+    const double ai = tl.cls == 0 ? a.a_seg : a.a_blob;
+    const double aj = it.src_cls == 0 ? a.a_seg : a.a_blob;
+    const T sum_a2 = T((ai + aj) * (ai + aj));
+    const int t_local = tl.t_begin + tid;
+    const bool active = t_local < tl.t_end;
+    const int t = active ? t_local : tl.t_end - 1;
+    double xi[3];
+    T xh[3], xl[3];
+    for (int k = 0; k < 3; ++k) {
+        xi[k] = xt.p[(int64_t)t * xt.ld_p + k * xt.ld_c];
+        split_position(xi[k], xh[k], xl[k]);
+    }
+    const int tg = (tl.cls == 0 ? 0 : a.ns) + t;  // global target index
+    const int src_base = it.src_cls == 0 ? 0 : a.ns;
+    for (int s0 = it.s_begin; s0 < it.s_end; s0 += 256) {
+        const int n = (it.s_end - s0) < 256 ? (it.s_end - s0) : 256;
+        __syncthreads();
+        if (tid < n) stage_source<T, 3>(lds + tid * REC, xsrc, a.fb, s0 + tid, false);
+        __syncthreads();
+        if (!active) continue;
+        for (int j = 0; j < n; ++j) {
+            T dx, dy, dz;
+            const T r2 = separation<T>(xi, xh, xl, lds + j * REC, dx, dy, dz);
+            if (!(r2 > sum_a2)) {
+                const int k = atomicAdd(&cnt[tg], 1);
+                if (k < cap) idx[(int64_t)tg * cap + k] = src_base + s0 + j;
+                else atomicMax(max_cnt, k + 1);
             }
-            double* dst = out.p + t * out.ld_p + k * out.ld_c;
-            if (tl.write_mode == 0) *dst = s;
-            else if (k < vt) *dst += s;
         }
+    }
+}
+
+__global__ void fill_int_kernel(int32_t* p, int32_t v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <int VT>
+__device__ void add_near(double* acc, int src, const double* xi, double ai, const MatvecArgs& a, int seg_dof,
+                         uint32_t cls_mask) {
+    const bool seg = src < a.ns;
+    if (!(cls_mask & (seg ? 1u : 2u))) return;  // that block is not active
+    const int s = seg ? src : src - a.ns;
+    const CView xv = seg ? a.xs : a.xb;
+    const CView fv = seg ? a.fs : a.fb;
+    const double d0 = xi[0] - xv.p[(int64_t)s * xv.ld_p];
+    const double d1 = xi[1] - xv.p[(int64_t)s * xv.ld_p + xv.ld_c];
+    const double d2 = xi[2] - xv.p[(int64_t)s * xv.ld_p + 2 * xv.ld_c];
+    double f[6];
+    const int fs = seg ? seg_dof : 3;
+    for (int k = 0; k < 6; ++k) f[k] = k < fs ? fv.p[(int64_t)s * fv.ld_p + k * fv.ld_c] : 0.0;
+    const double aj = seg ? a.a_seg : a.a_blob;
+    if (fs == 6) pair_near<VT, 6>(acc, d0, d1, d2, f, ai, aj, a.mu);
+    else pair_near<VT, 3>(acc, d0, d1, d2, f, ai, aj, a.mu);
+}
+
+// Adds the partial slots of each tile in item order, then the overlapping
+// pairs (sorted by source index), and applies the write rule.
+template <int SEGDOF>
+__global__ __launch_bounds__(64) void rpy_reduce_kernel(const MatvecTile* __restrict__ tiles,
+                                                        const double* __restrict__ partial, int tile_targets,
+                                                        OutArgs o, MatvecArgs a, NearList near) {
+    const MatvecTile tl = tiles[blockIdx.x / ((tile_targets + 63) / 64)];
+    const int l = (blockIdx.x % ((tile_targets + 63) / 64)) * 64 + threadIdx.x;
+    const int t = tl.t_begin + l;
+    if (l >= tile_targets || t >= tl.t_end) return;
+    const bool seg_t = tl.cls == 0;
+    const int slots = seg_t ? o.seg_out_slots : 3;
+    const View out = seg_t ? o.vs : o.vb;
+    const size_t slot_len = (size_t)kMaxDof * tile_targets;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const int vt = seg_t ? SEGDOF : 3;
+    const double* p = partial + (size_t)tl.first_item * slot_len + l;
+    for (int m = 0; m < tl.n_items; ++m)
+        for (int k = 0; k < vt; ++k) acc[k] += p[(size_t)m * slot_len + (size_t)k * tile_targets];
+    // Active source classes for this target class: bit 0 segments, bit 1 blobs.
+    const uint32_t cls_mask = seg_t ? ((o.blocks & 1u) ? 1u : 0u) | ((o.blocks & 2u) ? 2u : 0u)
+                                    : ((o.blocks & 8u) ? 1u : 0u) | ((o.blocks & 4u) ? 2u : 0u);
+    if (near.cnt && cls_mask) {
+        const int tg = (seg_t ? 0 : a.ns) + t;
+        const int c = near.cnt[tg] < near.cap ? near.cnt[tg] : near.cap;
+        const int32_t* e = near.idx + (int64_t)tg * near.cap;
+        const CView xv = seg_t ? a.xs : a.xb;
+        const double xi[3] = {xv.p[(int64_t)t * xv.ld_p], xv.p[(int64_t)t * xv.ld_p + xv.ld_c],
+                              xv.p[(int64_t)t * xv.ld_p + 2 * xv.ld_c]};
+        const double ai = seg_t ? a.a_seg : a.a_blob;
+        // Visit the entries in increasing source order (selection by successive minima).
+        int last = -1;
+        for (int q = 0; q < c; ++q) {
+            int best = 0x7fffffff;
+            for (int u = 0; u < c; ++u) {
+                const int v = e[u];
+                if (v > last && v < best) best = v;
+            }
+            last = best;
+            if (seg_t && SEGDOF == 6) add_near<6>(acc, best, xi, ai, a, SEGDOF, cls_mask);
+            else add_near<3>(acc, best, xi, ai, a, SEGDOF, cls_mask);
+        }
+    }
+    for (int k = 0; k < slots; ++k) {
+        double* dst = out.p + (int64_t)t * out.ld_p + k * out.ld_c;
+        if (tl.write_mode == 0) *dst = k < vt ? acc[k] : 0.0;
+        else if (k < vt) *dst += acc[k];
     }
 }
 
@@ -240,39 +346,61 @@ hipError_t dispatch_shape(const TileShape& sh, const MatvecArgs& a, const Matvec
 
 }  // namespace
 
-void build_worklist(int ns, int nb, int tile_targets, int chunk, bool ss, bool sb, bool bb, bool bs,
-                    int seg_write, int blob_write, std::vector<MatvecTile>& tiles,
+void build_worklist(const std::vector<BodyRange>& bodies, int tile_targets, int chunk, bool ss, bool sb,
+                    bool bb, bool bs, int seg_write, int blob_write, std::vector<MatvecTile>& tiles,
                     std::vector<MatvecItem>& items) {
     tiles.clear();
     items.clear();
-    auto add_tiles = [&](int cls, int n, bool from_segs, bool from_blobs, int write) {
-        if (write < 0) return;
-        for (int t0 = 0; t0 < n; t0 += tile_targets) {
-            MatvecTile tl;
-            tl.cls = cls;
-            tl.t_begin = t0;
-            tl.t_end = t0 + tile_targets < n ? t0 + tile_targets : n;
-            tl.first_item = (int)items.size();
-            tl.write_mode = write;
-            const int tile_id = (int)tiles.size();
-            auto add_src = [&](int src_cls, int m) {
-                for (int s0 = 0; s0 < m; s0 += chunk)
-                    items.push_back({tile_id, src_cls, s0, s0 + chunk < m ? s0 + chunk : m});
-            };
-            if (from_segs) add_src(0, ns);
-            if (from_blobs) add_src(1, nb);
-            tl.n_items = (int)items.size() - tl.first_item;
-            tiles.push_back(tl);
-        }
-    };
-    add_tiles(0, ns, ss, sb, seg_write);
-    add_tiles(1, nb, bs, bb, blob_write);
+    for (const BodyRange& body : bodies) {
+        auto add_tiles = [&](int cls, int t_lo, int t_hi, bool from_segs, bool from_blobs, int write) {
+            if (write < 0) return;
+            for (int t0 = t_lo; t0 < t_hi; t0 += tile_targets) {
+                MatvecTile tl;
+                tl.cls = cls;
+                tl.t_begin = t0;
+                tl.t_end = std::min(t0 + tile_targets, t_hi);
+                tl.first_item = (int)items.size();
+                tl.write_mode = write;
+                const int tile_id = (int)tiles.size();
+                auto add_src = [&](int src_cls, int lo, int hi) {
+                    for (int s0 = lo; s0 < hi; s0 += chunk) items.push_back({tile_id, src_cls, s0, std::min(s0 + chunk, hi)});
+                };
+                if (from_segs) add_src(0, body.seg_begin, body.seg_end);
+                if (from_blobs) add_src(1, body.blob_begin, body.blob_end);
+                tl.n_items = (int)items.size() - tl.first_item;
+                tiles.push_back(tl);
+            }
+        };
+        add_tiles(0, body.seg_begin, body.seg_end, ss, sb, seg_write);
+        add_tiles(1, body.blob_begin, body.blob_end, bs, bb, blob_write);
+    }
+}
+
+hipError_t build_near_list(int precision, const MatvecArgs& a, const MatvecItem* d_items, int n_items,
+                           const MatvecTile* d_tiles, NearList& near, int n_targets, int* max_count,
+                           int* host_scratch, hipStream_t stream) {
+    const int64_t n = (int64_t)n_targets;
+    hipLaunchKernelGGL(fill_int_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, near.cnt, 0, n);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, stream, near.max_cnt, 0, (int64_t)1);
+    if (n_items > 0) {
+        if (precision == 0)
+            hipLaunchKernelGGL(near_build_kernel<double>, dim3(n_items), dim3(256), 0, stream, d_items, d_tiles, a,
+                               near.cnt, near.idx, near.cap, near.max_cnt);
+        else
+            hipLaunchKernelGGL(near_build_kernel<float>, dim3(n_items), dim3(256), 0, stream, d_items, d_tiles, a,
+                               near.cnt, near.idx, near.cap, near.max_cnt);
+    }
+    TFS_HIP_CHECK(hipGetLastError());
+    TFS_HIP_CHECK(hipMemcpyAsync(host_scratch, near.max_cnt, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    TFS_HIP_CHECK(hipStreamSynchronize(stream));
+    *max_count = *host_scratch;
+    return hipSuccess;
 }
 
 hipError_t launch_matvec(int precision, int seg_dof, const TileShape& shape, const MatvecArgs& a,
                          const OutArgs& o, const MatvecTile* d_tiles, int n_tiles,
-                         const MatvecItem* d_items, int n_items, double* partial, hipStream_t stream,
-                         hipEvent_t ev_begin, hipEvent_t ev_end) {
+                         const MatvecItem* d_items, int n_items, double* partial, const NearList& near,
+                         hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end) {
     if (n_tiles == 0) return hipSuccess;
     if (n_items > 0) {
         hipError_t e;
@@ -288,11 +416,11 @@ hipError_t launch_matvec(int precision, int seg_dof, const TileShape& shape, con
         if (ev_end) TFS_HIP_CHECK(hipEventRecord(ev_end, stream));
     }
     const int tt = shape.block * shape.per_thread;
-    const int rb = tt < 256 ? tt : 256;
+    const unsigned grid = (unsigned)n_tiles * (unsigned)((tt + 63) / 64);
     if (seg_dof == 6)
-        hipLaunchKernelGGL(rpy_reduce_kernel<6>, dim3(n_tiles), dim3(rb), 0, stream, d_tiles, partial, tt, o);
+        hipLaunchKernelGGL(rpy_reduce_kernel<6>, dim3(grid), dim3(64), 0, stream, d_tiles, partial, tt, o, a, near);
     else
-        hipLaunchKernelGGL(rpy_reduce_kernel<3>, dim3(n_tiles), dim3(rb), 0, stream, d_tiles, partial, tt, o);
+        hipLaunchKernelGGL(rpy_reduce_kernel<3>, dim3(grid), dim3(64), 0, stream, d_tiles, partial, tt, o, a, near);
     return hipGetLastError();
 }
 

@@ -16,6 +16,7 @@ struct Plan {
     std::vector<MatvecItem> items;
     MatvecTile* d_tiles = nullptr;
     MatvecItem* d_items = nullptr;
+    uint32_t blocks = 0;  // blocks actually evaluated (after the reference's overwrite rule)
 };
 
 }  // namespace tfs
@@ -38,6 +39,12 @@ struct tfs_rpy {
     double* partial = nullptr;
     size_t partial_len = 0;
     tfs::TileShape shape[2];
+    // Independent bodies (one by default) and, per precision, the overlapping
+    // pairs of the current positions.
+    std::vector<tfs::BodyRange> bodies;
+    tfs::NearList near[2];
+    bool near_valid[2] = {false, false};
+    int32_t* host_int = nullptr;  // pinned scratch
     // Live kernel timing: event pairs recorded around each pair-kernel launch.
     bool profiling = false;
     struct EvPair {
@@ -54,7 +61,10 @@ extern thread_local std::string g_last_error;
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* where);
 TileShape default_shape(int precision, int ns, int nb, int num_cus);
-Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err);
+// near_plan: a work list over all four blocks in 256-target tiles, for build_near_list.
+Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err, bool near_plan = false);
+// Positions changed: the near-pair lists must be rebuilt before the next matvec.
+void invalidate_near(tfs_rpy* h);
 
 // Matvec with explicit views: forces fs (seg_dof comps) / fb (3 comps) and
 // outputs vs (seg_out_slots comps) / vb (3 comps), with the write rules of

@@ -22,6 +22,12 @@ struct View {
     int64_t ld_c;
 };
 
+// One independent body: its segments [seg_begin, seg_end) and blobs
+// [blob_begin, blob_end) in the concatenated arrays.  Bodies do not interact.
+struct BodyRange {
+    int32_t seg_begin, seg_end, blob_begin, blob_end;
+};
+
 // One unit of matvec work: the pairs between the targets of `tile` and the
 // sources [s_begin, s_end) of class src_cls (0 segments, 1 blobs).  Its
 // partial sums go to partial slot = the item's index.
@@ -43,6 +49,18 @@ struct MatvecTile {
     int32_t write_mode;  // 0 overwrite, 1 accumulate into the output
 };
 
+// Overlapping pairs (r^2 <= (ai+aj)^2 as evaluated by the pair kernel of one
+// precision): excluded from the branch-free pair kernel and added exactly, in
+// FP64 with the reference's overlap formulas, by the reduce kernel.  Target t
+// (segments first, then blobs) owns idx[t*cap .. t*cap + cnt[t]); an entry is
+// the source index, segments first then blobs.
+struct NearList {
+    int32_t* cnt = nullptr;
+    int32_t* idx = nullptr;
+    int32_t* max_cnt = nullptr;
+    int cap = 0;
+};
+
 // Launch shape of the pair kernel; chosen by the autotuner.
 struct TileShape {
     int block;       // threads per workgroup (64, 128, 256)
@@ -61,23 +79,29 @@ struct MatvecArgs {
 struct OutArgs {
     View vs, vb;       // outputs; vs has seg_out_slots components
     int seg_out_slots; // 6 in the reference layout, 3 for a packed 3-dof vector
+    uint32_t blocks;   // active blocks (bit 0 SS, 1 SB, 2 BB, 3 BS) for the near-pair terms
 };
 
-// Builds the work list for one evaluation.  blocks: bit 0 SS, 1 SB, 2 BB, 3 BS.
+// Builds the work list for one evaluation of independent bodies.
 // seg_write / blob_write: -1 no write, 0 overwrite, 1 accumulate.
-void build_worklist(int ns, int nb, int tile_targets, int chunk, bool ss, bool sb, bool bb,
-                    bool bs, int seg_write, int blob_write, std::vector<MatvecTile>& tiles,
+void build_worklist(const std::vector<BodyRange>& bodies, int tile_targets, int chunk, bool ss, bool sb,
+                    bool bb, bool bs, int seg_write, int blob_write, std::vector<MatvecTile>& tiles,
                     std::vector<MatvecItem>& items);
+
+// Rebuilds `near` for the given precision from a work list covering all four
+// blocks; returns the largest per-target count in *max_count (synchronises).
+hipError_t build_near_list(int precision, const MatvecArgs& a, const MatvecItem* d_items, int n_items,
+                           const MatvecTile* d_tiles, NearList& near, int n_targets, int* max_count,
+                           int* host_scratch, hipStream_t stream);
 
 // Launches pair kernel + reduce on `stream`.  d_tiles/d_items/partial are device
 // buffers sized for the work list; precision 0 = fp64, 1 = fp32 tiles with fp64
-// accumulation.
-// ev_begin/ev_end (may be null) are recorded around the pair kernel only.
+// accumulation.  ev_begin/ev_end (may be null) are recorded around the pair
+// kernel only.
 hipError_t launch_matvec(int precision, int seg_dof, const TileShape& shape, const MatvecArgs& a,
                          const OutArgs& o, const MatvecTile* d_tiles, int n_tiles,
-                         const MatvecItem* d_items, int n_items, double* partial,
-                         hipStream_t stream, hipEvent_t ev_begin = nullptr,
-                         hipEvent_t ev_end = nullptr);
+                         const MatvecItem* d_items, int n_items, double* partial, const NearList& near,
+                         hipStream_t stream, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 
 // Flow-field evaluation (Stokeslet + rotlet mode and RPY mode).
 struct FlowArgs {

@@ -48,23 +48,28 @@ TileShape default_shape(int precision, int ns, int nb, int num_cus) {
     return s;
 }
 
-Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err) {
-    const bool ss = blocks & TFS_BLOCK_SS, sb = blocks & TFS_BLOCK_SB;
-    bool bb = blocks & TFS_BLOCK_BB;
-    const bool bs = blocks & TFS_BLOCK_BS;
-    // Sequential semantics: with an overwriting BS, an earlier BB is overwritten.
-    if (bs && bb && h->desc.bs_mode == TFS_BS_OVERWRITE) bb = false;
-    const int seg_write = ss ? 0 : (sb ? 1 : -1);
-    int blob_write = -1;
-    if (bb) blob_write = 0;
-    else if (bs) blob_write = h->desc.bs_mode == TFS_BS_OVERWRITE ? 0 : 1;
-    const auto key = std::make_tuple((uint32_t)((ss << 0) | (sb << 1) | (bb << 2) | (bs << 3)),
-                                     sh.block, sh.per_thread, sh.chunk);
+Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err, bool near_plan) {
+    bool ss = blocks & TFS_BLOCK_SS, sb = blocks & TFS_BLOCK_SB;
+    bool bb = blocks & TFS_BLOCK_BB, bs = blocks & TFS_BLOCK_BS;
+    int seg_write = -1, blob_write = -1;
+    if (near_plan) {
+        ss = sb = bb = bs = true;
+        seg_write = blob_write = 0;
+    } else {
+        // Sequential semantics: with an overwriting BS, an earlier BB is overwritten.
+        if (bs && bb && h->desc.bs_mode == TFS_BS_OVERWRITE) bb = false;
+        seg_write = ss ? 0 : (sb ? 1 : -1);
+        if (bb) blob_write = 0;
+        else if (bs) blob_write = h->desc.bs_mode == TFS_BS_OVERWRITE ? 0 : 1;
+    }
+    const uint32_t eff = (ss ? 1u : 0u) | (sb ? 2u : 0u) | (bb ? 4u : 0u) | (bs ? 8u : 0u);
+    const auto key = std::make_tuple(eff | (near_plan ? 16u : 0u), sh.block, sh.per_thread, sh.chunk);
     auto it = h->plans.find(key);
     if (it != h->plans.end()) return &it->second;
     Plan p;
-    build_worklist(h->desc.num_segs, h->desc.num_blobs, sh.block * sh.per_thread, sh.chunk, ss, sb, bb,
-                   bs, seg_write, blob_write, p.tiles, p.items);
+    p.blocks = eff;
+    build_worklist(h->bodies, sh.block * sh.per_thread, sh.chunk, ss, sb, bb, bs, seg_write, blob_write, p.tiles,
+                   p.items);
     if (!p.tiles.empty()) {
         hipError_t e = hipMalloc(&p.d_tiles, sizeof(MatvecTile) * p.tiles.size());
         if (e == hipSuccess && !p.items.empty())
@@ -81,9 +86,12 @@ Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err) {
             *err = hip_fail(e, "work list upload");
             return nullptr;
         }
-        const size_t need = p.items.size() * 6 * (size_t)(sh.block * sh.per_thread);
+        const size_t need = near_plan ? 0 : p.items.size() * 6 * (size_t)(sh.block * sh.per_thread);
         if (need > h->partial_len) {
-            if (h->partial) hipFree(h->partial);
+            if (h->partial) {
+                hipStreamSynchronize(h->stream);
+                hipFree(h->partial);
+            }
             h->partial = nullptr;
             h->partial_len = 0;
             e = hipMalloc(&h->partial, need * sizeof(double));
@@ -98,13 +106,16 @@ Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err) {
     return &res.first->second;
 }
 
-int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, View vs, View vb,
-               int seg_out_slots) {
-    const TileShape& sh = h->shape[precision];
-    int err = TFS_OK;
-    Plan* p = get_plan(h, blocks, sh, &err);
-    if (!p) return err;
-    if (p->tiles.empty()) return TFS_OK;
+void invalidate_near(tfs_rpy* h) { h->near_valid[0] = h->near_valid[1] = false; }
+
+void free_near(NearList& nl) {
+    hipFree(nl.cnt);
+    hipFree(nl.idx);
+    hipFree(nl.max_cnt);
+    nl = NearList{};
+}
+
+MatvecArgs matvec_args(tfs_rpy* h, CView fs, CView fb) {
     MatvecArgs a;
     a.xs = {h->x_segs, 3, 1};
     a.xb = {h->x_blobs, 3, 1};
@@ -115,10 +126,67 @@ int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, V
     a.a_seg = h->desc.seg_radius;
     a.a_blob = h->desc.blob_radius;
     a.mu = h->desc.mu;
+    return a;
+}
+
+// Lists the overlapping pairs of the current positions for `precision`,
+// growing the per-target capacity until every list fits.
+int ensure_near(tfs_rpy* h, int precision) {
+    if (h->near_valid[precision]) return TFS_OK;
+    const int nt = h->desc.num_segs + h->desc.num_blobs;
+    if (nt == 0) {
+        h->near_valid[precision] = true;
+        return TFS_OK;
+    }
+    int err = TFS_OK;
+    const TileShape nsh{256, 1, 1024};
+    Plan* p = get_plan(h, TFS_BLOCK_ALL, nsh, &err, true);
+    if (!p) return err;
+    NearList& nl = h->near[precision];
+    MatvecArgs a = matvec_args(h, CView{h->f_segs, 6, 1}, CView{h->f_blobs, 3, 1});
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        if (!nl.cnt) {
+            if (nl.cap == 0) nl.cap = 32;
+            hipError_t e = hipMalloc(&nl.cnt, sizeof(int32_t) * nt);
+            if (e == hipSuccess) e = hipMalloc(&nl.idx, sizeof(int32_t) * (size_t)nt * nl.cap);
+            if (e == hipSuccess) e = hipMalloc(&nl.max_cnt, sizeof(int32_t));
+            if (e != hipSuccess) {
+                free_near(nl);
+                return fail(TFS_ERR_OUT_OF_MEMORY, "near-pair lists");
+            }
+        }
+        int max_count = 0;
+        hipError_t e = build_near_list(precision, a, p->d_items, (int)p->items.size(), p->d_tiles, nl, nt,
+                                       &max_count, h->host_int, h->stream);
+        if (e != hipSuccess) return hip_fail(e, "near-pair lists");
+        if (max_count <= nl.cap) {
+            h->near_valid[precision] = true;
+            return TFS_OK;
+        }
+        int cap = nl.cap;
+        while (cap < max_count) cap *= 2;
+        hipStreamSynchronize(h->stream);
+        free_near(nl);
+        nl.cap = cap;
+    }
+    return fail(TFS_ERR_INVALID_ARGUMENT, "near-pair lists: too many overlapping neighbours");
+}
+
+int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, View vs, View vb,
+               int seg_out_slots) {
+    const TileShape& sh = h->shape[precision];
+    int err = TFS_OK;
+    Plan* p = get_plan(h, blocks, sh, &err);
+    if (!p) return err;
+    if (p->tiles.empty()) return TFS_OK;
+    err = ensure_near(h, precision);
+    if (err) return err;
+    MatvecArgs a = matvec_args(h, fs, fb);
     OutArgs o;
     o.vs = vs;
     o.vb = vb;
     o.seg_out_slots = seg_out_slots;
+    o.blocks = p->blocks;
     hipEvent_t eb = nullptr, ee = nullptr;
     if (h->profiling && !p->items.empty()) {
         if (h->ev_free.empty()) {
@@ -136,7 +204,8 @@ int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, V
         h->ev_used.push_back(ep);
     }
     hipError_t e = launch_matvec(precision, h->desc.seg_dof, sh, a, o, p->d_tiles, (int)p->tiles.size(),
-                                 p->d_items, (int)p->items.size(), h->partial, h->stream, eb, ee);
+                                 p->d_items, (int)p->items.size(), h->partial, h->near[precision], h->stream,
+                                 eb, ee);
     if (e != hipSuccess) return hip_fail(e, "matvec launch");
     return TFS_OK;
 }
@@ -211,6 +280,12 @@ int tfs_rpy_create(const tfs_rpy_desc* desc, tfs_rpy** out) {
         return hip_fail(e, "tfs_rpy_create: clear");
     }
     for (int pr = 0; pr < 2; ++pr) h->shape[pr] = default_shape(pr, desc->num_segs, desc->num_blobs, cus);
+    h->bodies.push_back(BodyRange{0, desc->num_segs, 0, desc->num_blobs});
+    e = hipHostMalloc((void**)&h->host_int, 64, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        tfs_rpy_destroy(h);
+        return hip_fail(e, "tfs_rpy_create: pinned scratch");
+    }
     *out = h;
     return TFS_OK;
 }
@@ -229,6 +304,8 @@ int tfs_rpy_destroy(tfs_rpy* h) {
         }
     if (h->partial) hipFree(h->partial);
     if (h->arena) hipFree(h->arena);
+    for (auto& nl : h->near) free_near(nl);
+    if (h->host_int) hipHostFree(h->host_int);
     if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
     delete h;
     return TFS_OK;
@@ -257,6 +334,7 @@ int tfs_rpy_upload(tfs_rpy* h, int32_t what, const double* src) {
     if (rc) return rc;
     if (len == 0) return TFS_OK;
     if (!src) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_upload: null source");
+    if (what == TFS_SEG_POSITIONS || what == TFS_BLOB_POSITIONS) invalidate_near(h);
     hipError_t e = hipMemcpyAsync(dev, src, len * sizeof(double),
                                   h->desc.memory == TFS_MEMORY_HOST ? hipMemcpyHostToDevice
                                                                     : hipMemcpyDeviceToDevice,
@@ -346,6 +424,39 @@ int tfs_rpy_set_tile_shape(tfs_rpy* h, int32_t precision, int32_t block, int32_t
     if (chunk < block || chunk % block)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_tile_shape: chunk must be a positive multiple of block");
     h->shape[precision] = TileShape{block, per_thread, chunk};
+    return TFS_OK;
+}
+
+int tfs_rpy_positions_changed(tfs_rpy* h) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_positions_changed: null handle");
+    invalidate_near(h);
+    return TFS_OK;
+}
+
+int tfs_rpy_set_bodies(tfs_rpy* h, int32_t num_bodies, const int32_t* segs_per_body,
+                       const int32_t* blobs_per_body) {
+    if (!h || num_bodies < 1 || !segs_per_body || !blobs_per_body)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_bodies: bad arguments");
+    std::vector<BodyRange> bodies;
+    int64_t s = 0, b = 0;
+    for (int i = 0; i < num_bodies; ++i) {
+        if (segs_per_body[i] < 0 || blobs_per_body[i] < 0)
+            return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_bodies: negative count");
+        bodies.push_back(BodyRange{(int32_t)s, (int32_t)(s + segs_per_body[i]), (int32_t)b,
+                                   (int32_t)(b + blobs_per_body[i])});
+        s += segs_per_body[i];
+        b += blobs_per_body[i];
+    }
+    if (s != h->desc.num_segs || b != h->desc.num_blobs)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_bodies: counts do not add up to the handle's sizes");
+    hipStreamSynchronize(h->stream);
+    for (auto& kv : h->plans) {
+        hipFree(kv.second.d_tiles);
+        hipFree(kv.second.d_items);
+    }
+    h->plans.clear();
+    h->bodies = bodies;
+    invalidate_near(h);
     return TFS_OK;
 }
 

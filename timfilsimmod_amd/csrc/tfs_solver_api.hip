@@ -376,6 +376,7 @@ int tfs_solver_set_geometry(tfs_solver* s, const double* x_segs, const double* x
     const size_t ns3 = (size_t)3 * s->desc.num_fil * s->desc.num_seg, nb3 = (size_t)3 * s->desc.num_blobs;
     const hipMemcpyKind k = kind_in(memory);
     hipError_t e = hipSuccess;
+    if (x_segs || x_blobs) invalidate_near(s->rpy);
     if (x_segs && ns3) e = hipMemcpyAsync(s->rpy->x_segs, x_segs, ns3 * 8, k, s->stream);
     if (e == hipSuccess && x_blobs && nb3) e = hipMemcpyAsync(s->rpy->x_blobs, x_blobs, nb3 * 8, k, s->stream);
     if (e == hipSuccess && vel_dir_phase && ns3) e = hipMemcpyAsync(s->kphi, vel_dir_phase, ns3 * 8, k, s->stream);
@@ -529,6 +530,7 @@ int tfs_beat_create(const tfs_beat_desc* desc, tfs_beat** out) {
     up(b->body, body.data(), 9);
     if (e == hipSuccess && D.solver.num_blobs > 0)
         e = hipMemcpyAsync(s->rpy->x_blobs, D.x_blobs, 24LL * D.solver.num_blobs, hipMemcpyHostToDevice, st);
+    invalidate_near(s->rpy);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     b->beat_A.assign(D.beat_A, D.beat_A + 24);
     b->beat_B.assign(D.beat_B, D.beat_B + 18);
@@ -573,6 +575,7 @@ int tfs_beat_step(tfs_beat* b, tfs_solve_stats* stats) {
     ba.angle = D.solver.dynamic_angle ? b->angle : nullptr;
     rc = check(k_beat(ba, s->rpy->x_segs, s->kphi, s->kth, st), "beat kinematics");
     if (rc) return rc;
+    invalidate_near(s->rpy);
     RhsArgs ra;
     ra.d = s->dims;
     ra.phase = b->phase;
