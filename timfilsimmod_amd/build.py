@@ -62,9 +62,25 @@ def build(verbose: bool = False) -> str:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")
     os.replace(tmp, LIB)
+    build_abi_consumer()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+def build_abi_consumer() -> str:
+    """The C++ caller used by tests/test_capi_cpp_gpu.py (plain g++ against the C-ABI)."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "abi_consumer.cpp")
+    out = os.path.join(root, "tests", "cpp", "abi_consumer")
+    if not os.path.exists(src):
+        return ""
+    cmd = ["g++", "-O2", "-std=c++17", "-I", INCLUDE, src, "-o", out, "-L", HERE, "-ltfs_rpy",
+           f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../timfilsimmod_amd"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g++ failed on abi_consumer.cpp:\n{res.stderr}")
+    return out
 
 
 if __name__ == "__main__":
