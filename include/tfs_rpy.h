@@ -136,6 +136,33 @@ int tfs_rpy_positions_changed(tfs_rpy *h);
 int tfs_rpy_set_bodies(tfs_rpy *h, int32_t num_bodies, const int32_t *segs_per_body,
                        const int32_t *blobs_per_body);
 
+/* Persistent launches (0 = off): a fixed grid of workgroups_per_cu workgroups
+ * per CU walks the work list instead of one workgroup per item. */
+int tfs_rpy_set_persistent(tfs_rpy *h, int32_t precision, int32_t workgroups_per_cu);
+/* Current launch shape of a precision: block, per_thread, chunk, persistent. */
+int tfs_rpy_get_tile_shape(tfs_rpy *h, int32_t precision, int32_t shape[4]);
+/* Per-body completion times of the last persistent launch, in ms from its
+ * first work item's start (enable before the launch; reading synchronises). */
+int tfs_rpy_body_timing(tfs_rpy *h, int32_t enable);
+int tfs_rpy_body_times(tfs_rpy *h, double *ms_per_body);
+
+/* hipGraph of `evaluations` consecutive tfs_rpy_evaluate(h, blocks) calls,
+ * replayed by tfs_graph_launch on the handle's stream (no counterpart in the
+ * reference, which launches kernels one by one, rpy_mobility_solver.cu:258-337).
+ * The work list, partial buffer and near-pair lists are fixed at capture:
+ * re-create the graph after positions change. */
+typedef struct tfs_graph tfs_graph;
+int tfs_rpy_graph_create(tfs_rpy *h, uint32_t blocks, int32_t evaluations, tfs_graph **out);
+int tfs_graph_launch(tfs_graph *g);
+int tfs_graph_destroy(tfs_graph *g);
+
+/* Autotuner (no counterpart in the reference, whose launch shape is fixed at
+ * THREADS_PER_BLOCK 64, config.hpp:294): times every launch shape (block 64/128/256,
+ * 1 or 2 targets per thread, three chunk sizes, queued or one workgroup per
+ * item) on the handle's current data and keeps the fastest for `precision`;
+ * *best_ms is its mean time per evaluation. */
+int tfs_rpy_autotune(tfs_rpy *h, int32_t precision, uint32_t blocks, int32_t reps, double *best_ms);
+
 /* Live kernel timing (no counterpart in the reference, whose DISPLAYTIME
  * buckets bracket whole phases with cudaDeviceSynchronize, mobility_solver.cpp:2576-2657):
  * while enabled, HIP events are recorded on the handle's stream around every

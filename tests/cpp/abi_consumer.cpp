@@ -54,7 +54,7 @@ int main(int argc, char** argv) {
     ok += std::fread(f_segs, sizeof(double), 6 * ns, f);
     ok += std::fread(f_blobs, sizeof(double), 3 * nb, f);
     std::fclose(f);
-    if (ok != size_t(12 * ns + 6 * nb)) return 1;
+    if (ok != size_t(9 * ns + 6 * nb)) return 1;
 
     tfs_rpy* h = nullptr;
     CHECK(tfs_rpy_create(&d, &h));                       // allocate_device_memory

@@ -4,4 +4,4 @@ Submodules: mobility (matvec back-end), flow (flow-field evaluator), solver (per
 solve and prescribed-beat driver), geometry and beat (synthetic inputs).
 All compute runs in libtfs_rpy.so (include/tfs_rpy.h, include/tfs_solver.h).
 """
-from . import _lib, geometry, beat, mobility, solver, flow  # noqa: F401
+from . import _lib, geometry, beat, mobility, solver, flow, ensemble  # noqa: F401

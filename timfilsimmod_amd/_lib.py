@@ -70,6 +70,14 @@ SIGNATURES = {
     "tfs_rpy_profile_read": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i32)]),
     "tfs_solver_rpy": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "tfs_rpy_positions_changed": (ctypes.c_int, [_vp]),
+    "tfs_rpy_set_persistent": (ctypes.c_int, [_vp, _i32, _i32]),
+    "tfs_rpy_get_tile_shape": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_i32)]),
+    "tfs_rpy_body_timing": (ctypes.c_int, [_vp, _i32]),
+    "tfs_rpy_body_times": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
+    "tfs_rpy_graph_create": (ctypes.c_int, [_vp, _u32, _i32, ctypes.POINTER(_vp)]),
+    "tfs_graph_launch": (ctypes.c_int, [_vp]),
+    "tfs_graph_destroy": (ctypes.c_int, [_vp]),
+    "tfs_rpy_autotune": (ctypes.c_int, [_vp, _i32, _u32, _i32, ctypes.POINTER(ctypes.c_double)]),
     "tfs_rpy_set_bodies": (ctypes.c_int, [_vp, _i32, _vp, _vp]),
     "tfs_host_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "tfs_host_free": (ctypes.c_int, [_vp]),

@@ -202,6 +202,34 @@ __global__ __launch_bounds__(BLOCK) void rpy_pairs_kernel(const MatvecItem* __re
     }
 }
 
+// Persistent form: a fixed grid of `gridDim.x` workgroups walks the work list
+// with stride gridDim.x (the list is ordered by body, items of similar size),
+// so every workgroup's loop count and exit are uniform -- no atomics and no
+// lane-divergent code in the loop.  With timing on, every lane of a workgroup
+// stores the same per-item start/finish stamps (wall_clock64 ticks).
+template <typename T, int SEGDOF, int BLOCK, int R>
+__global__ __launch_bounds__(BLOCK) void rpy_pairs_persistent_kernel(const MatvecItem* __restrict__ items,
+                                                                     const MatvecTile* __restrict__ tiles,
+                                                                     MatvecArgs a, double* __restrict__ partial,
+                                                                     int n_items, PersistState ps) {
+    constexpr int REC = rec_len<T, 6>() > rec_len<T, 3>() ? rec_len<T, 6>() : rec_len<T, 3>();
+    __shared__ __align__(16) T lds[BLOCK * REC];
+    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+        if (ps.item_stamps) ps.item_stamps[2 * item] = (int64_t)wall_clock64();
+        const MatvecItem it = items[item];
+        const MatvecTile tl = tiles[it.tile];
+        double* slot = partial + (size_t)item * (kMaxDof * BLOCK * R);
+        if (tl.cls == 0) {
+            if (it.src_cls == 0) run_item<SEGDOF, SEGDOF, T, BLOCK, R>(it, tl, a, slot, lds);
+            else run_item<SEGDOF, 3, T, BLOCK, R>(it, tl, a, slot, lds);
+        } else {
+            if (it.src_cls == 0) run_item<3, SEGDOF, T, BLOCK, R>(it, tl, a, slot, lds);
+            else run_item<3, 3, T, BLOCK, R>(it, tl, a, slot, lds);
+        }
+        if (ps.item_stamps) ps.item_stamps[2 * item + 1] = (int64_t)wall_clock64();
+    }
+}
+
 // Near-pair listing: the same work decomposition as the pair kernel (one
 // target per lane, BLOCK = 256), testing r^2 <= (ai+aj)^2 with the pair
 // kernel's arithmetic.  Entries are appended with atomics; the reduce kernel
@@ -325,19 +353,27 @@ __global__ __launch_bounds__(64) void rpy_reduce_kernel(const MatvecTile* __rest
 }
 
 template <typename T, int SEGDOF, int BLOCK, int R>
-hipError_t launch_pairs(const MatvecArgs& a, const MatvecItem* d_items, int n_items,
-                        const MatvecTile* d_tiles, double* partial, hipStream_t stream) {
-    hipLaunchKernelGGL((rpy_pairs_kernel<T, SEGDOF, BLOCK, R>), dim3(n_items), dim3(BLOCK), 0, stream,
-                       d_items, d_tiles, a, partial);
+hipError_t launch_pairs(const TileShape& sh, const MatvecArgs& a, const MatvecItem* d_items, int n_items,
+                        const MatvecTile* d_tiles, double* partial, const PersistState& ps, hipStream_t stream) {
+    if (sh.persistent > 0) {
+        int grid = sh.persistent * ps.num_cus;
+        if (grid > n_items) grid = n_items;
+        hipLaunchKernelGGL((rpy_pairs_persistent_kernel<T, SEGDOF, BLOCK, R>), dim3(grid), dim3(BLOCK), 0, stream,
+                           d_items, d_tiles, a, partial, n_items, ps);
+    } else {
+        hipLaunchKernelGGL((rpy_pairs_kernel<T, SEGDOF, BLOCK, R>), dim3(n_items), dim3(BLOCK), 0, stream,
+                           d_items, d_tiles, a, partial);
+    }
     return hipGetLastError();
 }
 
 template <typename T, int SEGDOF>
 hipError_t dispatch_shape(const TileShape& sh, const MatvecArgs& a, const MatvecItem* d_items, int n_items,
-                          const MatvecTile* d_tiles, double* partial, hipStream_t stream) {
+                          const MatvecTile* d_tiles, double* partial, const PersistState& ps,
+                          hipStream_t stream) {
 #define TFS_SHAPE(B, R)                                                                       \
     if (sh.block == B && sh.per_thread == R)                                                 \
-        return launch_pairs<T, SEGDOF, B, R>(a, d_items, n_items, d_tiles, partial, stream);
+        return launch_pairs<T, SEGDOF, B, R>(sh, a, d_items, n_items, d_tiles, partial, ps, stream);
     TFS_SHAPE(64, 1) TFS_SHAPE(64, 2) TFS_SHAPE(128, 1) TFS_SHAPE(128, 2) TFS_SHAPE(256, 1)
     TFS_SHAPE(256, 2)
 #undef TFS_SHAPE
@@ -361,6 +397,7 @@ void build_worklist(const std::vector<BodyRange>& bodies, int tile_targets, int 
                 tl.t_end = std::min(t0 + tile_targets, t_hi);
                 tl.first_item = (int)items.size();
                 tl.write_mode = write;
+                tl.body = (int32_t)(&body - bodies.data());
                 const int tile_id = (int)tiles.size();
                 auto add_src = [&](int src_cls, int lo, int hi) {
                     for (int s0 = lo; s0 < hi; s0 += chunk) items.push_back({tile_id, src_cls, s0, std::min(s0 + chunk, hi)});
@@ -400,17 +437,17 @@ hipError_t build_near_list(int precision, const MatvecArgs& a, const MatvecItem*
 hipError_t launch_matvec(int precision, int seg_dof, const TileShape& shape, const MatvecArgs& a,
                          const OutArgs& o, const MatvecTile* d_tiles, int n_tiles,
                          const MatvecItem* d_items, int n_items, double* partial, const NearList& near,
-                         hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end) {
+                         const PersistState& ps, hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end) {
     if (n_tiles == 0) return hipSuccess;
     if (n_items > 0) {
         hipError_t e;
         if (ev_begin) TFS_HIP_CHECK(hipEventRecord(ev_begin, stream));
         if (precision == 0) {
-            e = seg_dof == 6 ? dispatch_shape<double, 6>(shape, a, d_items, n_items, d_tiles, partial, stream)
-                             : dispatch_shape<double, 3>(shape, a, d_items, n_items, d_tiles, partial, stream);
+            e = seg_dof == 6 ? dispatch_shape<double, 6>(shape, a, d_items, n_items, d_tiles, partial, ps, stream)
+                             : dispatch_shape<double, 3>(shape, a, d_items, n_items, d_tiles, partial, ps, stream);
         } else {
-            e = seg_dof == 6 ? dispatch_shape<float, 6>(shape, a, d_items, n_items, d_tiles, partial, stream)
-                             : dispatch_shape<float, 3>(shape, a, d_items, n_items, d_tiles, partial, stream);
+            e = seg_dof == 6 ? dispatch_shape<float, 6>(shape, a, d_items, n_items, d_tiles, partial, ps, stream)
+                             : dispatch_shape<float, 3>(shape, a, d_items, n_items, d_tiles, partial, ps, stream);
         }
         if (e != hipSuccess) return e;
         if (ev_end) TFS_HIP_CHECK(hipEventRecord(ev_end, stream));

@@ -21,6 +21,7 @@ struct Plan {
 
 }  // namespace tfs
 
+
 struct tfs_rpy {
     tfs_rpy_desc desc{};
     int device = 0;
@@ -45,6 +46,12 @@ struct tfs_rpy {
     tfs::NearList near[2];
     bool near_valid[2] = {false, false};
     int32_t* host_int = nullptr;  // pinned scratch
+    // Persistent launches with timing: per-item stamps of the last launch and
+    // the plan they belong to.
+    int64_t* d_item_stamps = nullptr;
+    size_t item_stamps_len = 0;
+    const tfs::Plan* stamped_plan = nullptr;
+    bool body_timing = false;
     // Live kernel timing: event pairs recorded around each pair-kernel launch.
     bool profiling = false;
     struct EvPair {

@@ -47,6 +47,7 @@ struct MatvecTile {
     int32_t first_item;
     int32_t n_items;
     int32_t write_mode;  // 0 overwrite, 1 accumulate into the output
+    int32_t body;        // index into the body list
 };
 
 // Overlapping pairs (r^2 <= (ai+aj)^2 as evaluated by the pair kernel of one
@@ -66,6 +67,15 @@ struct TileShape {
     int block;       // threads per workgroup (64, 128, 256)
     int per_thread;  // targets per thread (1 or 2)
     int chunk;       // sources per work item (a multiple of block)
+    int persistent;  // 0: one workgroup per item; >0: that many workgroups per CU pulling
+                     // items from an atomic queue
+};
+
+// State of a persistent launch: optional per-item start/finish stamps
+// (wall_clock64 ticks), two per item.
+struct PersistState {
+    int64_t* item_stamps = nullptr;
+    int num_cus = 256;
 };
 
 // Everything the pair kernel needs besides the work list.
@@ -101,7 +111,8 @@ hipError_t build_near_list(int precision, const MatvecArgs& a, const MatvecItem*
 hipError_t launch_matvec(int precision, int seg_dof, const TileShape& shape, const MatvecArgs& a,
                          const OutArgs& o, const MatvecTile* d_tiles, int n_tiles,
                          const MatvecItem* d_items, int n_items, double* partial, const NearList& near,
-                         hipStream_t stream, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+                         const PersistState& ps, hipStream_t stream, hipEvent_t ev_begin = nullptr,
+                         hipEvent_t ev_end = nullptr);
 
 // Flow-field evaluation (Stokeslet + rotlet mode and RPY mode).
 struct FlowArgs {

@@ -35,6 +35,7 @@ TileShape default_shape(int precision, int ns, int nb, int num_cus) {
     TileShape s;
     s.block = 256;
     s.per_thread = 1;
+    s.persistent = 0;
     (void)precision;
     const int tt = s.block * s.per_thread;
     const long n_tiles = (ns + tt - 1) / tt + (nb + tt - 1) / tt;
@@ -172,6 +173,35 @@ int ensure_near(tfs_rpy* h, int precision) {
     return fail(TFS_ERR_INVALID_ARGUMENT, "near-pair lists: too many overlapping neighbours");
 }
 
+int ensure_stamps(tfs_rpy* h, size_t n_items) {
+    if (h->item_stamps_len >= 2 * n_items) return TFS_OK;
+    if (h->d_item_stamps) {
+        hipStreamSynchronize(h->stream);
+        hipFree(h->d_item_stamps);
+    }
+    h->d_item_stamps = nullptr;
+    h->item_stamps_len = 0;
+    hipError_t e = hipMalloc(&h->d_item_stamps, sizeof(int64_t) * 2 * n_items);
+    if (e != hipSuccess) return hip_fail(e, "persistent-launch stamps");
+    h->item_stamps_len = 2 * n_items;
+    return TFS_OK;
+}
+
+void free_persist(tfs_rpy* h) {
+    hipFree(h->d_item_stamps);
+    h->d_item_stamps = nullptr;
+    h->item_stamps_len = 0;
+    h->stamped_plan = nullptr;
+}
+
+void free_plans(tfs_rpy* h) {
+    for (auto& kv : h->plans) {
+        hipFree(kv.second.d_tiles);
+        hipFree(kv.second.d_items);
+    }
+    h->plans.clear();
+}
+
 int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, View vs, View vb,
                int seg_out_slots) {
     const TileShape& sh = h->shape[precision];
@@ -203,9 +233,17 @@ int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, V
         ee = ep.end;
         h->ev_used.push_back(ep);
     }
+    PersistState ps;
+    ps.num_cus = h->num_cus;
+    if (sh.persistent > 0 && h->body_timing) {
+        int rc = ensure_stamps(h, p->items.size());
+        if (rc) return rc;
+        ps.item_stamps = h->d_item_stamps;
+        h->stamped_plan = p;
+    }
     hipError_t e = launch_matvec(precision, h->desc.seg_dof, sh, a, o, p->d_tiles, (int)p->tiles.size(),
-                                 p->d_items, (int)p->items.size(), h->partial, h->near[precision], h->stream,
-                                 eb, ee);
+                                 p->d_items, (int)p->items.size(), h->partial, h->near[precision], ps,
+                                 h->stream, eb, ee);
     if (e != hipSuccess) return hip_fail(e, "matvec launch");
     return TFS_OK;
 }
@@ -423,7 +461,7 @@ int tfs_rpy_set_tile_shape(tfs_rpy* h, int32_t precision, int32_t block, int32_t
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_tile_shape: per_thread must be 1 or 2");
     if (chunk < block || chunk % block)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_tile_shape: chunk must be a positive multiple of block");
-    h->shape[precision] = TileShape{block, per_thread, chunk};
+    h->shape[precision] = TileShape{block, per_thread, chunk, h->shape[precision].persistent};
     return TFS_OK;
 }
 
@@ -450,13 +488,171 @@ int tfs_rpy_set_bodies(tfs_rpy* h, int32_t num_bodies, const int32_t* segs_per_b
     if (s != h->desc.num_segs || b != h->desc.num_blobs)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_bodies: counts do not add up to the handle's sizes");
     hipStreamSynchronize(h->stream);
-    for (auto& kv : h->plans) {
-        hipFree(kv.second.d_tiles);
-        hipFree(kv.second.d_items);
-    }
-    h->plans.clear();
+    free_plans(h);
+    free_persist(h);
     h->bodies = bodies;
     invalidate_near(h);
+    return TFS_OK;
+}
+
+struct tfs_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+int tfs_rpy_graph_create(tfs_rpy* h, uint32_t blocks, int32_t evaluations, tfs_graph** out) {
+    if (!h || !out || evaluations < 1 || (blocks & ~TFS_BLOCK_ALL))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_graph_create: bad arguments");
+    *out = nullptr;
+    // Everything that allocates or synchronises (work list, partial buffer,
+    // near-pair lists, queue state) is created by one evaluation outside capture.
+    int rc = tfs_rpy_evaluate(h, blocks);
+    if (rc) return rc;
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(e, "tfs_rpy_graph_create: warm-up");
+    const bool prof = h->profiling;
+    h->profiling = false;
+    tfs_graph* g = new tfs_graph();
+    g->stream = h->stream;
+    e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        h->profiling = prof;
+        delete g;
+        return hip_fail(e, "tfs_rpy_graph_create: begin capture");
+    }
+    for (int i = 0; i < evaluations && rc == TFS_OK; ++i) rc = tfs_rpy_evaluate(h, blocks);
+    hipError_t e2 = hipStreamEndCapture(h->stream, &g->graph);
+    h->profiling = prof;
+    if (rc || e2 != hipSuccess) {
+        if (g->graph) hipGraphDestroy(g->graph);
+        delete g;
+        return rc ? rc : hip_fail(e2, "tfs_rpy_graph_create: end capture");
+    }
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(g->graph);
+        delete g;
+        return hip_fail(e, "tfs_rpy_graph_create: instantiate");
+    }
+    *out = g;
+    return TFS_OK;
+}
+
+int tfs_graph_launch(tfs_graph* g) {
+    if (!g) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_graph_launch: null graph");
+    hipError_t e = hipGraphLaunch(g->exec, g->stream);
+    return e == hipSuccess ? TFS_OK : hip_fail(e, "tfs_graph_launch");
+}
+
+int tfs_graph_destroy(tfs_graph* g) {
+    if (!g) return TFS_OK;
+    hipStreamSynchronize(g->stream);
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    delete g;
+    return TFS_OK;
+}
+
+int tfs_rpy_autotune(tfs_rpy* h, int32_t precision, uint32_t blocks, int32_t reps, double* best_ms) {
+    if (!h || (precision != 0 && precision != 1) || reps < 1 || (blocks & ~TFS_BLOCK_ALL))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_autotune: bad arguments");
+    const int saved_precision = h->desc.precision;
+    h->desc.precision = precision;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return fail(TFS_ERR_HIP, "tfs_rpy_autotune: events");
+    const bool prof = h->profiling;
+    h->profiling = false;
+    TileShape best = h->shape[precision];
+    double best_t = 1e300;
+    int rc = TFS_OK;
+    const int ns = h->desc.num_segs, nb = h->desc.num_blobs;
+    for (int block : {64, 128, 256}) {
+        for (int r : {1, 2}) {
+            const int tt = block * r;
+            const TileShape base = default_shape(precision, ns, nb, h->num_cus);
+            // chunk candidates around the one that gives ~8 workgroups per CU for this tile size
+            const long n_tiles = (ns + tt - 1) / tt + (nb + tt - 1) / tt;
+            long c0 = n_tiles > 0 ? ((long)(ns + nb) * n_tiles + 8L * h->num_cus - 1) / (8L * h->num_cus) : base.chunk;
+            for (long mult : {1L, 2L, 4L}) {
+                long chunk = std::max<long>(block, ((c0 * mult / 2 + block - 1) / block) * block);
+                for (int pers : {0, 2}) {
+                    h->shape[precision] = TileShape{block, r, (int)chunk, pers};
+                    rc = tfs_rpy_evaluate(h, blocks);  // builds the plan; warm-up
+                    if (rc) break;
+                    hipEventRecord(e0, h->stream);
+                    for (int i = 0; i < reps && !rc; ++i) rc = tfs_rpy_evaluate(h, blocks);
+                    hipEventRecord(e1, h->stream);
+                    if (rc) break;
+                    hipEventSynchronize(e1);
+                    float ms = 0.0f;
+                    hipEventElapsedTime(&ms, e0, e1);
+                    if (ms / reps < best_t) {
+                        best_t = ms / reps;
+                        best = h->shape[precision];
+                    }
+                }
+                if (rc) break;
+            }
+            if (rc) break;
+        }
+        if (rc) break;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    h->profiling = prof;
+    h->desc.precision = saved_precision;
+    h->shape[precision] = best;
+    if (best_ms) *best_ms = best_t;
+    return rc;
+}
+
+int tfs_rpy_set_persistent(tfs_rpy* h, int32_t precision, int32_t workgroups_per_cu) {
+    if (!h || (precision != 0 && precision != 1) || workgroups_per_cu < 0 || workgroups_per_cu > 64)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_persistent: bad arguments");
+    h->shape[precision].persistent = workgroups_per_cu;
+    return TFS_OK;
+}
+
+int tfs_rpy_get_tile_shape(tfs_rpy* h, int32_t precision, int32_t shape[4]) {
+    if (!h || !shape || (precision != 0 && precision != 1))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_get_tile_shape: bad arguments");
+    const TileShape& s = h->shape[precision];
+    shape[0] = s.block;
+    shape[1] = s.per_thread;
+    shape[2] = s.chunk;
+    shape[3] = s.persistent;
+    return TFS_OK;
+}
+
+int tfs_rpy_body_timing(tfs_rpy* h, int32_t enable) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_body_timing: null handle");
+    h->body_timing = enable != 0;
+    return TFS_OK;
+}
+
+int tfs_rpy_body_times(tfs_rpy* h, double* ms_per_body) {
+    if (!h || !ms_per_body) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_body_times: null argument");
+    const Plan* p = h->stamped_plan;
+    if (!p) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_body_times: no timed persistent launch yet");
+    const size_t ni = p->items.size(), nb = h->bodies.size();
+    std::vector<int64_t> st(2 * ni);
+    hipError_t e = hipMemcpyAsync(st.data(), h->d_item_stamps, sizeof(int64_t) * 2 * ni, hipMemcpyDeviceToHost,
+                                  h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    int khz = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, h->device);
+    if (e != hipSuccess) return hip_fail(e, "tfs_rpy_body_times");
+    if (khz <= 0) khz = 100000;
+    int64_t t0 = INT64_MAX;
+    for (size_t i = 0; i < ni; ++i) t0 = std::min(t0, st[2 * i]);
+    std::vector<int64_t> done(nb, t0);
+    for (size_t i = 0; i < ni; ++i) {
+        const int b = p->tiles[p->items[i].tile].body;
+        done[b] = std::max(done[b], st[2 * i + 1]);
+    }
+    for (size_t b = 0; b < nb; ++b) ms_per_body[b] = double(done[b] - t0) / double(khz);
     return TFS_OK;
 }
 

@@ -104,10 +104,64 @@ class RpyHandle:
         """{precision: (summed pair-kernel ms, launches)} since the last read."""
         return profile_read(self.h)
 
+    def set_bodies(self, segs_per_body, blobs_per_body):
+        """Independent bodies stored one after another (block-diagonal mobility)."""
+        s = np.ascontiguousarray(segs_per_body, dtype=np.int32)
+        b = np.ascontiguousarray(blobs_per_body, dtype=np.int32)
+        L.check(L.lib().tfs_rpy_set_bodies(self.h, len(s), s.ctypes.data, b.ctypes.data))
+        self.num_bodies = len(s)
+
+    def set_persistent(self, precision: int, workgroups_per_cu: int):
+        L.check(L.lib().tfs_rpy_set_persistent(self.h, precision, workgroups_per_cu))
+
+    def tile_shape(self, precision: int):
+        out = (ctypes.c_int32 * 4)()
+        L.check(L.lib().tfs_rpy_get_tile_shape(self.h, precision, out))
+        return {"block": out[0], "per_thread": out[1], "chunk": out[2], "persistent": out[3]}
+
+    def body_timing(self, enable: bool = True):
+        L.check(L.lib().tfs_rpy_body_timing(self.h, int(enable)))
+
+    def body_times(self, num_bodies: int):
+        out = (ctypes.c_double * num_bodies)()
+        L.check(L.lib().tfs_rpy_body_times(self.h, out))
+        return np.array(out[:])
+
+    def autotune(self, precision: int, blocks: int = L.BLOCK_ALL, reps: int = 3) -> float:
+        best = ctypes.c_double()
+        L.check(L.lib().tfs_rpy_autotune(self.h, precision, blocks, reps, ctypes.byref(best)))
+        return best.value
+
+    def graph(self, blocks: int = L.BLOCK_ALL, evaluations: int = 1) -> "Graph":
+        return Graph(self, blocks, evaluations)
+
     def device_buffer(self, what: int) -> int:
         p = ctypes.c_void_p()
         L.check(L.lib().tfs_rpy_device_buffer(self.h, what, ctypes.byref(p)))
         return p.value
+
+
+class Graph:
+    """A captured sequence of evaluations, replayed on the handle's stream."""
+
+    def __init__(self, handle: RpyHandle, blocks: int, evaluations: int):
+        g = ctypes.c_void_p()
+        L.check(L.lib().tfs_rpy_graph_create(handle.h, blocks, evaluations, ctypes.byref(g)))
+        self.g = g
+
+    def launch(self):
+        L.check(L.lib().tfs_graph_launch(self.g))
+
+    def close(self):
+        if self.g:
+            L.lib().tfs_graph_destroy(self.g)
+            self.g = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class RpyMobilitySolver:
