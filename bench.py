@@ -154,6 +154,61 @@ def flow_leg(torch, stream):
     return out
 
 
+def ensemble_leg(torch, stream, steps: int = 5):
+    """64 independent bodies of uneven size (filaments U[100,1000] x 20 segments,
+    blobs U[2000,9000]) in one block-diagonal 6-DOF matvec launch: autotuned
+    shape, hipGraph replays; per-body completion p50/p99 from one timed
+    persistent launch."""
+    from timfilsimmod_amd import _lib as L
+    from timfilsimmod_amd.ensemble import make_ensemble
+    from timfilsimmod_amd.mobility import Graph, RpyHandle
+    e = make_ensemble(64)
+    flops = 0.0
+    for b in e.bodies:
+        flops += matvec_flops(b.num_segs, b.num_blobs, 6)
+    out = {"bodies": 64, "segments": e.num_segs, "blobs": e.num_blobs,
+           "algorithmic_tflop_per_launch": flops / 1e12}
+    h = RpyHandle(e.num_segs, e.num_blobs, seg_dof=6, precision=L.PRECISION_FP64, bs_accumulate=True,
+                  stream=stream)
+    h.set_bodies(e.segs_per_body, e.blobs_per_body)
+    for what, arr in ((L.SEG_POSITIONS, e.x_segs), (L.BLOB_POSITIONS, e.x_blobs), (L.SEG_FORCES, e.f_segs),
+                      (L.BLOB_FORCES, e.f_blobs)):
+        h.upload(what, arr)
+    h.sync()
+    for prec, name in ((L.PRECISION_FP64, "fp64"), (L.PRECISION_FP32_ACC64, "mixed")):
+        tuned_ms = h.autotune(prec, reps=1)
+        shape = h.tile_shape(prec)
+        h.set_precision(prec)
+        g = Graph(h, L.BLOCK_ALL, 1)
+        g.launch()
+        torch.cuda.synchronize()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(steps):
+            g.launch()
+        t1.record()
+        torch.cuda.synchronize()
+        g.close()
+        ms = t0.elapsed_time(t1) / steps
+        # per-body completion inside one persistent launch
+        keep = shape["persistent"]
+        if keep == 0:
+            h.set_persistent(prec, 2)
+        h.body_timing(True)
+        h.evaluate()
+        bt = h.body_times(64)
+        h.body_timing(False)
+        h.set_persistent(prec, keep)
+        out[name] = {"ms_per_launch": ms, "body_matvecs_per_s": 64 * 1000.0 / ms,
+                     "TFLOP_s": flops / (ms * 1e-3) / 1e12, "autotuned_shape": shape,
+                     "autotune_best_ms": tuned_ms,
+                     "per_body_completion_ms_p50": float(np.percentile(bt, 50)),
+                     "per_body_completion_ms_p99": float(np.percentile(bt, 99)),
+                     "per_body_completion_ms_max": float(bt.max())}
+    h.close()
+    return out
+
+
 def cpu_baseline(body, budget_s: float = 15.0):
     """FP64 oracle direct sum on the host cores (reported, not gating): the
     prescribed-beat <3,3> matvec of the production body, timed."""
@@ -263,6 +318,8 @@ def main():
     print("bench: flow-field leg", file=sys.stderr, flush=True)
     flow = flow_leg(torch, stream)
     torch.cuda.empty_cache()
+    print("bench: ensemble leg", file=sys.stderr, flush=True)
+    ens = ensemble_leg(torch, stream)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -302,6 +359,7 @@ def main():
                    "mixed_rel_l2_err_vs_fp64": mv_err,
                    "algorithmic_gflop": flops6 / 1e9, "compulsory_bytes": bytes6},
         "flow_field": flow,
+        "ensemble": ens,
         "roofline": {"bound": "valu", "achieved": tf32, "peak": PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
                      "frac": tf32 / PEAK_FP32_VALU_TFLOPS, "traffic": None,
                      "kernel": "rpy_pairs_kernel<float,3> (timestep matvec, 17.11 GFLOP per launch)",

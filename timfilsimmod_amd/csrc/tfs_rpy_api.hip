@@ -568,7 +568,9 @@ int tfs_rpy_autotune(tfs_rpy* h, int32_t precision, uint32_t blocks, int32_t rep
     double best_t = 1e300;
     int rc = TFS_OK;
     const int ns = h->desc.num_segs, nb = h->desc.num_blobs;
+    const double pairs = double(ns + nb) * double(ns + nb) / double(h->bodies.size());
     for (int block : {64, 128, 256}) {
+        if (block == 64 && pairs > 1e8) continue;  // large problems: 64-thread groups never win
         for (int r : {1, 2}) {
             const int tt = block * r;
             const TileShape base = default_shape(precision, ns, nb, h->num_cus);
