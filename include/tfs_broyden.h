@@ -1,0 +1,102 @@
+/*
+ * tfs_broyden.h -- C-ABI of the device-resident Broyden solver.
+ *
+ * Replaces broyden_solver (src/cilia/broyden_solver.hpp:18-43,
+ * src/cilia/broyden_solver.cpp:9-129): the limited-memory inverse-Jacobian
+ * update of the implicit timestep, with FP64 storage of the C and D columns
+ * (NBROY x MAX_BROYDEN_ITER each) in HBM.
+ *
+ *   find_update  (broyden_solver.cpp:26-67):
+ *       update = -c J0^{-1} error - sum_{i<iter} (error . D_i) C_i
+ *   end_of_iter  (broyden_solver.cpp:69-129), when the error is still too large:
+ *       C_iter = -c J0^{-1} new_error - sum_{i<iter} (new_error . D_i) C_i
+ *       D_iter = new_error - error;  C_iter, D_iter /= ||D_iter||;  error = new_error
+ *     and always iter += 1.
+ *
+ * c is JACOBIAN_CONFIDENCE_FACTOR (config.hpp:298).  J0^{-1} -- the model's
+ * approximate Jacobian inverse, swimmer::jacobian_inv_mult (swimmer.cpp:915-1070),
+ * per-filament dense blocks of filament::invert_approx_jacobian
+ * (filament.cpp:1761-1894) plus body blocks -- is given as block-diagonal dense
+ * blocks, applied on the device.
+ */
+#ifndef TFS_BROYDEN_H
+#define TFS_BROYDEN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "tfs_rpy.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tfs_broyden_desc {
+    int32_t n;                    /* NBROY                                                  */
+    int32_t max_iter;             /* MAX_BROYDEN_ITER (config.hpp:300)                      */
+    double confidence;            /* JACOBIAN_CONFIDENCE_FACTOR                             */
+    int32_t num_blocks;           /* diagonal blocks of J0^{-1}                             */
+    const int32_t *block_sizes;   /* host array, num_blocks sizes in [1, 2048] summing to n */
+    int32_t memory;               /* TFS_MEMORY_* of the vectors passed later               */
+    void *stream;                 /* hipStream_t or NULL                                     */
+} tfs_broyden_desc;
+/* Device vectors passed to the engine must be ready when a call is made.  With
+ * a caller stream, device outputs are ordered on it; with stream == NULL the
+ * engine runs on a stream of its own and its device outputs are complete when
+ * the call returns.  Host vectors are always complete on return. */
+
+typedef struct tfs_broyden tfs_broyden;
+
+int tfs_broyden_create(const tfs_broyden_desc *desc, tfs_broyden **out);
+int tfs_broyden_destroy(tfs_broyden *b);
+/* J0^{-1}: the blocks one after another, each row-major (sum of size^2 values),
+ * in the desc's memory space (swimmer::prepare_jacobian_inv, swimmer.cpp:535-913). */
+int tfs_broyden_set_jacobian_inverse(tfs_broyden *b, const double *blocks);
+/* J0 itself, same layout: every block is inverted on the device by Gauss-Jordan
+ * elimination with partial pivoting, one workgroup per block (replaces the
+ * per-filament LAPACK getrf/getri of filament::invert_approx_jacobian,
+ * filament.cpp:1761-1894).  TFS_ERR_NON_FINITE if a block is singular. */
+int tfs_broyden_set_jacobian(tfs_broyden *b, const double *blocks);
+/* Start a timestep: iter = 0 and error = the residual of the initial guess
+ * (compute_errors before the loop, cilia_sim_main.cu:394-411).  error == NULL
+ * keeps the residual that tfs_broyden_compute_errors left in slot 0. */
+int tfs_broyden_start(tfs_broyden *b, const double *error);
+int tfs_broyden_find_update(tfs_broyden *b, double *update);
+/* new_error == NULL takes the residual that tfs_broyden_compute_errors left in slot 1. */
+int tfs_broyden_end_of_iter(tfs_broyden *b, const double *new_error, int32_t error_is_too_large);
+/* Current iteration count (broyden_solver::iter) and ||error||. */
+int tfs_broyden_status(tfs_broyden *b, int32_t *iter, double *error_norm);
+
+/* State of the implicit step for mobility_solver::compute_errors
+ * (mobility_solver.cpp:2661-2946, bodies not in the Broyden-for-everything
+ * form).  Pointers live in the desc's memory space; 3 values per segment or
+ * body unless stated. */
+typedef struct tfs_broyden_state {
+    int32_t num_swim, num_fil, num_seg;  /* NSWIM, NFIL, NSEG: n = 6*(NSWIM*NFIL*NSEG + NSWIM) */
+    int32_t bdf2;                        /* nt >= NUM_EULER_STEPS (config.hpp:326)             */
+    double dt;                           /* DT                                                  */
+    double dl;                           /* DL: the position tolerance is DL*TOL                */
+    double tol;                          /* TOL (config.hpp:302)                                */
+    const double *seg_x, *seg_xm1, *seg_xm2;   /* segment positions at n+1, n, n-1            */
+    const double *seg_u, *seg_um1;             /* segment Lie-algebra elements at n+1, n       */
+    const double *v_segs;                      /* 6 per segment: velocity, angular velocity   */
+    const double *body_x, *body_xm1, *body_xm2, *body_u, *body_um1;
+    const double *v_bodies;                    /* 6 per body                                  */
+} tfs_broyden_state;
+
+/* Residuals of the BDF2 (or backward-Euler) equations for positions and
+ * Lie-algebra elements (dexpinv, quaternion.cpp:610-633), written into the
+ * engine: slot 0 = error (before tfs_broyden_start), slot 1 = new_error (before
+ * tfs_broyden_end_of_iter).  *error_is_too_large is the reference's test
+ * (|e| > DL*TOL for positions, > TOL for Lie-algebra elements).  Returns
+ * TFS_ERR_NON_FINITE if a residual is not finite (cilia_sim_main.cu:469-487). */
+int tfs_broyden_compute_errors(tfs_broyden *b, const tfs_broyden_state *st, int32_t slot,
+                               int32_t *error_is_too_large);
+/* Copy of slot 0 or 1 into `out` (desc's memory space). */
+int tfs_broyden_read_error(tfs_broyden *b, int32_t slot, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TFS_BROYDEN_H */

@@ -1,7 +1,8 @@
 """MI355X-native (gfx950) RPY mobility hot path of TimFilSim.
 
 Submodules: mobility (matvec back-end), flow (flow-field evaluator), solver (per-timestep saddle-point
-solve and prescribed-beat driver), geometry and beat (synthetic inputs).
-All compute runs in libtfs_rpy.so (include/tfs_rpy.h, include/tfs_solver.h).
+solve and prescribed-beat driver), broyden (implicit-step Broyden engine), geometry and beat
+(synthetic inputs).
+All compute runs in libtfs_rpy.so (include/tfs_rpy.h, tfs_solver.h, tfs_flow.h, tfs_broyden.h).
 """
-from . import _lib, geometry, beat, mobility, solver, flow, ensemble  # noqa: F401
+from . import _lib, geometry, beat, mobility, solver, flow, ensemble, broyden  # noqa: F401

@@ -102,6 +102,20 @@ def _declare(lib, table):
         fn.argtypes = args
 
 
+def _share_hip_runtime_with_torch():
+    """Load torch (when installed) before the library.
+
+    torch ships its own libamdhip64 (soname libamdhip64.so.7, the soname the
+    library links against).  Loaded first, it is the HIP runtime the library
+    binds to; loaded after the library, torch would bring up a second HIP
+    runtime in the process, which then sees no GPU.  One runtime per process
+    is also what lets torch tensors and streams be passed to the library."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """The loaded library; raises if it was not built."""
     global _lib
@@ -110,6 +124,7 @@ def lib():
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build the gfx950 library first "
                 "(python -m timfilsimmod_amd.build)")
+        _share_hip_runtime_with_torch()
         handle = ctypes.CDLL(LIB_PATH)
         _declare(handle, SIGNATURES)
         _lib = handle

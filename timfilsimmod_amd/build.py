@@ -25,6 +25,7 @@ SOURCES = [
     "tfs_rpy_api.hip",
     "tfs_solver_api.hip",
     "tfs_flow_api.hip",
+    "tfs_broyden_api.hip",
 ]
 
 # Packed FP32 VALU forms are switched off: only scalar FP32 and FP64 VALU.

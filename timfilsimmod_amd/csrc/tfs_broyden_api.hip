@@ -1,0 +1,613 @@
+// tfs_broyden_api.hip -- device-resident Broyden updates behind include/tfs_broyden.h.
+//
+// The reference keeps C and D as host `matrix` columns and forms every dot
+// product and column update in host loops (broyden_solver.cpp:26-129).  Here
+// C and D live in HBM (column-major, FP64); the sums over previous iterations
+// are one transposed GEMV (error . D_i for all i) and one GEMV (sum h_i C_i),
+// shared with the GMRES kernels, and J0^{-1} is applied as block-diagonal dense
+// blocks, one workgroup per block.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/tfs_broyden.h"
+#include "solver_kernels.h"
+#include "tfs_api_internal.h"
+
+using namespace tfs;
+
+namespace {
+
+// y[rows of block b] = alpha * A_b x[rows of block b]; A_b is stored column-major
+// (the transpose of the caller's row-major block) so that the threads of a
+// workgroup, one per row, read consecutive addresses.
+__global__ __launch_bounds__(128) void block_gemv_kernel(const double* __restrict__ A,
+                                                         const int64_t* __restrict__ mat_off,
+                                                         const int32_t* __restrict__ row_off,
+                                                         const int32_t* __restrict__ sizes,
+                                                         const double* __restrict__ x, double* __restrict__ y,
+                                                         double alpha) {
+    const int b = blockIdx.x;
+    const int bs = sizes[b];
+    const double* Ab = A + mat_off[b];
+    const int r0 = row_off[b];
+    for (int r = threadIdx.x; r < bs; r += blockDim.x) {
+        double s = 0.0;
+        for (int c = 0; c < bs; ++c) s += Ab[(int64_t)c * bs + r] * x[r0 + c];
+        y[r0 + r] = alpha * s;
+    }
+}
+
+// dst_b = src_b^T for every block.
+__global__ __launch_bounds__(256) void block_transpose_kernel(const double* __restrict__ src,
+                                                              double* __restrict__ dst,
+                                                              const int64_t* __restrict__ mat_off,
+                                                              const int32_t* __restrict__ sizes) {
+    const int b = blockIdx.x;
+    const int bs = sizes[b];
+    const double* S = src + mat_off[b];
+    double* T = dst + mat_off[b];
+    for (int64_t e = threadIdx.x; e < (int64_t)bs * bs; e += blockDim.x) {
+        const int r = (int)(e / bs), c = (int)(e % bs);
+        T[(int64_t)c * bs + r] = S[e];
+    }
+}
+
+// In-place Gauss-Jordan inverse of every row-major block, partial pivoting,
+// one workgroup per block.  Step k: pivot search over rows k..bs-1, row swap,
+// scale of the pivot row (its pivot entry becomes 1/d), elimination of column k
+// from every other row (column k becomes -A[i][k]/d); the row swaps are undone
+// as column swaps in reverse order at the end.  status[b] = 1 if a pivot is zero
+// or not finite.
+__global__ __launch_bounds__(256) void block_gauss_jordan_kernel(double* __restrict__ A,
+                                                                 const int64_t* __restrict__ mat_off,
+                                                                 const int32_t* __restrict__ sizes,
+                                                                 int32_t* __restrict__ piv_all,
+                                                                 const int32_t* __restrict__ piv_off,
+                                                                 int32_t* __restrict__ status) {
+    extern __shared__ double lds[];
+    __shared__ double red_v[256];
+    __shared__ int red_i[256];
+    __shared__ int bad;
+    const int b = blockIdx.x;
+    const int bs = sizes[b];
+    double* M = A + mat_off[b];
+    int32_t* piv = piv_all + piv_off[b];
+    double* colk = lds;        // column k of the other rows
+    double* rowk = lds + bs;   // the scaled pivot row
+    const int tid = threadIdx.x;
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    for (int k = 0; k < bs; ++k) {
+        // pivot search
+        double best = -1.0;
+        int bi = k;
+        for (int i = k + tid; i < bs; i += blockDim.x) {
+            const double v = fabs(M[(int64_t)i * bs + k]);
+            if (v > best) {
+                best = v;
+                bi = i;
+            }
+        }
+        red_v[tid] = best;
+        red_i[tid] = bi;
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if (tid < w) {
+                const double o = red_v[tid + w];
+                const int oi = red_i[tid + w];
+                if (o > red_v[tid] || (o == red_v[tid] && oi < red_i[tid])) {
+                    red_v[tid] = o;
+                    red_i[tid] = oi;
+                }
+            }
+            __syncthreads();
+        }
+        const int p = red_i[0];
+        // row swap k <-> p, then the pivot row is scaled into LDS
+        if (p != k)
+            for (int j = tid; j < bs; j += blockDim.x) {
+                const double t = M[(int64_t)k * bs + j];
+                M[(int64_t)k * bs + j] = M[(int64_t)p * bs + j];
+                M[(int64_t)p * bs + j] = t;
+            }
+        if (tid == 0) piv[k] = p;
+        __syncthreads();
+        const double d = M[(int64_t)k * bs + k];
+        if (!(fabs(d) > 0.0) || !isfinite(d)) {
+            if (tid == 0) bad = 1;
+            break;   // uniform across the workgroup: every thread read the same d
+        }
+        const double inv = 1.0 / d;
+        for (int j = tid; j < bs; j += blockDim.x) {
+            const double v = j == k ? inv : M[(int64_t)k * bs + j] * inv;
+            rowk[j] = v;
+            M[(int64_t)k * bs + j] = v;
+        }
+        for (int i = tid; i < bs; i += blockDim.x) colk[i] = i == k ? 0.0 : M[(int64_t)i * bs + k];
+        __syncthreads();
+        for (int64_t e = tid; e < (int64_t)bs * bs; e += blockDim.x) {
+            const int i = (int)(e / bs), j = (int)(e % bs);
+            if (i == k) continue;
+            const double f = colk[i];
+            M[e] = j == k ? -f * rowk[k] : M[e] - f * rowk[j];
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (bad) {
+        if (tid == 0) status[b] = 1;
+        return;
+    }
+    for (int k = bs - 1; k >= 0; --k) {
+        const int p = piv[k];
+        if (p != k)
+            for (int i = tid; i < bs; i += blockDim.x) {
+                const double t = M[(int64_t)i * bs + k];
+                M[(int64_t)i * bs + k] = M[(int64_t)i * bs + p];
+                M[(int64_t)i * bs + p] = t;
+            }
+        __syncthreads();
+    }
+    if (tid == 0) status[b] = 0;
+}
+
+// s[1] = 1 / s[0]
+__global__ void reciprocal_kernel(double* s) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) s[1] = s[0] > 0.0 ? 1.0 / s[0] : 0.0;
+}
+
+// out = dexpinv(u, v) = v - u x v / 2 - fac u x (u x v),
+// fac = ((theta/2) cot(theta/2) - 1) / theta^2, -1/12 below theta = 1e-5
+// (quaternion.cpp:8, 610-633).
+__device__ inline void dexpinv(double* out, const double* u, const double* v) {
+#pragma clang fp contract(off)
+    const double t2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const double th = sqrt(t2);
+    const double fac = th < 1e-5 ? -1.0 / 12.0 : (0.5 * th * cos(0.5 * th) / sin(0.5 * th) - 1.0) / t2;
+    const double c[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+    out[0] = v[0] - 0.5 * c[0] - fac * (u[1] * c[2] - u[2] * c[1]);
+    out[1] = v[1] - 0.5 * c[1] - fac * (u[2] * c[0] - u[0] * c[2]);
+    out[2] = v[2] - 0.5 * c[2] - fac * (u[0] * c[1] - u[1] * c[0]);
+}
+
+struct ResidualArgs {
+    int nswim, nfil, nseg, bdf2;
+    double dt, pos_tol, lie_tol;
+    const double *x, *xm1, *xm2, *u, *um1, *v;
+    const double *bx, *bxm1, *bxm2, *bu, *bum1, *bv;
+};
+
+// One thread per segment (then one per body): 3 position and 3 Lie-algebra
+// residuals in the reference's storage order (mobility_solver.cpp:2661-2946),
+// evaluated without contraction in the reference's operation order.
+// partial[2*block] = max |e|/tolerance, partial[2*block+1] = 1 if any residual
+// of the block is not finite.
+__global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a, double* __restrict__ err,
+                                                       double* __restrict__ partial) {
+#pragma clang fp contract(off)
+    __shared__ double sm[256];
+    __shared__ double sn[256];
+    const int64_t nsegs = (int64_t)a.nswim * a.nfil * a.nseg;
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const double c1 = -4.0 / 3.0, c2 = 1.0 / 3.0, c3 = -2.0 / 3.0;
+    double ratio = 0.0, nonfinite = 0.0;
+    if (g < nsegs + a.nswim) {
+        const bool body = g >= nsegs;
+        int64_t pid, lid;
+        const double *x, *xm1, *xm2, *u, *um1, *vel, *om;
+        if (!body) {
+            const int64_t n = g / ((int64_t)a.nfil * a.nseg);
+            const int64_t r = g % ((int64_t)a.nfil * a.nseg);
+            const int64_t i = r / a.nseg, j = r % a.nseg;
+            pid = n * 6 * ((int64_t)a.nfil * a.nseg + 1) + 6 * i * a.nseg + 3 * j;
+            lid = pid + 3 * a.nseg;
+            x = a.x + 3 * g;
+            xm1 = a.xm1 + 3 * g;
+            xm2 = a.xm2 ? a.xm2 + 3 * g : nullptr;
+            u = a.u + 3 * g;
+            um1 = a.um1 ? a.um1 + 3 * g : nullptr;
+            vel = a.v + 6 * g;
+            om = a.v + 6 * g + 3;
+        } else {
+            const int64_t n = g - nsegs;
+            pid = n * 6 * ((int64_t)a.nfil * a.nseg + 1) + 6 * (int64_t)a.nfil * a.nseg;
+            lid = pid + 3;
+            x = a.bx + 3 * n;
+            xm1 = a.bxm1 + 3 * n;
+            xm2 = a.bxm2 ? a.bxm2 + 3 * n : nullptr;
+            u = a.bu + 3 * n;
+            um1 = a.bum1 ? a.bum1 + 3 * n : nullptr;
+            vel = a.bv + 6 * n;
+            om = a.bv + 6 * n + 3;
+        }
+        double W[3];
+        dexpinv(W, u, om);
+        const double c3dt = c3 * a.dt;
+        for (int d = 0; d < 3; ++d) {
+            double ep, el;
+            if (a.bdf2) {
+                ep = x[d] + c1 * xm1[d] + c2 * xm2[d] + c3dt * vel[d];
+                el = u[d] - c2 * um1[d] + c3dt * W[d];
+            } else {
+                ep = x[d] - xm1[d] - a.dt * vel[d];
+                el = u[d] - a.dt * W[d];
+            }
+            err[pid + d] = ep;
+            err[lid + d] = el;
+            if (!isfinite(ep) || !isfinite(el)) nonfinite = 1.0;
+            ratio = fmax(ratio, fmax(fabs(ep) / a.pos_tol, fabs(el) / a.lie_tol));
+        }
+    }
+    sm[threadIdx.x] = ratio;
+    sn[threadIdx.x] = nonfinite;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
+            sn[threadIdx.x] = fmax(sn[threadIdx.x], sn[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = sm[0];
+        partial[2 * blockIdx.x + 1] = sn[0];
+    }
+}
+
+// out[0] = max ratio, out[1] = nonfinite flag over nb partial pairs (one block).
+__global__ __launch_bounds__(256) void residual_finish_kernel(const double* __restrict__ partial, int nb,
+                                                              double* __restrict__ out) {
+    __shared__ double sm[256];
+    __shared__ double sn[256];
+    double m = 0.0, f = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) {
+        m = fmax(m, partial[2 * b]);
+        f = fmax(f, partial[2 * b + 1]);
+    }
+    sm[threadIdx.x] = m;
+    sn[threadIdx.x] = f;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
+            sn[threadIdx.x] = fmax(sn[threadIdx.x], sn[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sm[0];
+        out[1] = sn[0];
+    }
+}
+
+int check(hipError_t e, const char* where) { return e == hipSuccess ? TFS_OK : hip_fail(e, where); }
+
+int64_t round32(int64_t l) { return ((l + 31) / 32) * 32; }
+
+}  // namespace
+
+struct tfs_broyden {
+    tfs_broyden_desc desc{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int n = 0, max_iter = 0, iter = 0, nblocks = 0, G = 1, RB = 1;
+    int64_t ld = 0, jac_len = 0;
+    double* arena = nullptr;
+    double *C = nullptr, *D = nullptr, *slot[2] = {nullptr, nullptr}, *t = nullptr, *h = nullptr,
+           *partial = nullptr, *scal = nullptr, *J = nullptr, *Jstage = nullptr;
+    int64_t* mat_off = nullptr;
+    int32_t* row_off = nullptr;
+    int32_t* sizes = nullptr;
+    int32_t* piv = nullptr;
+    int32_t* status = nullptr;
+    int max_block = 0;
+    double* host_scal = nullptr;
+    std::vector<int32_t> host_status;
+};
+
+namespace {
+
+bool host_mem(const tfs_broyden* b) { return b->desc.memory == TFS_MEMORY_HOST; }
+
+hipMemcpyKind to_device(const tfs_broyden* b) {
+    return host_mem(b) ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+}
+
+int apply_jinv(tfs_broyden* b, const double* x, double* y, double alpha) {
+    hipLaunchKernelGGL(block_gemv_kernel, dim3(b->nblocks), dim3(128), 0, b->stream, b->J, b->mat_off, b->row_off,
+                       b->sizes, x, y, alpha);
+    return check(hipGetLastError(), "J0^{-1}");
+}
+
+// out = -c J0^{-1} e - C(:, 0:iter) (D(:, 0:iter)^T e)
+int inverse_update(tfs_broyden* b, const double* e, double* out) {
+    int rc = apply_jinv(b, e, out, -b->desc.confidence);
+    const int k = std::min(b->iter, b->max_iter - 1);
+    if (rc || k == 0) return rc;
+    rc = check(k_dot_cols(b->D, b->ld, k, e, b->n, b->partial, b->G, b->stream), "D^T e");
+    if (!rc) rc = check(k_reduce_cols(b->partial, b->G, k, b->h, nullptr, b->stream), "reduce");
+    if (!rc) rc = check(k_gemv(b->C, b->ld, k, b->h, out, b->n, true, b->stream), "C h");
+    return rc;
+}
+
+int copy_in(tfs_broyden* b, double* dst, const double* src, int64_t count, const char* where) {
+    hipError_t e = hipMemcpyAsync(dst, src, count * 8, to_device(b), b->stream);
+    if (e == hipSuccess && host_mem(b)) e = hipStreamSynchronize(b->stream);
+    return check(e, where);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tfs_broyden_create(const tfs_broyden_desc* desc, tfs_broyden** out) {
+    g_last_error.clear();
+    if (!desc || !out) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_create: null argument");
+    *out = nullptr;
+    const tfs_broyden_desc& D = *desc;
+    if (D.n < 1 || D.max_iter < 1 || D.num_blocks < 1 || !D.block_sizes || !(D.confidence > 0))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_create: bad sizes or confidence");
+    if (D.memory != TFS_MEMORY_HOST && D.memory != TFS_MEMORY_DEVICE)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_create: unknown memory space");
+    std::vector<int64_t> moff(D.num_blocks);
+    std::vector<int32_t> roff(D.num_blocks), sz(D.block_sizes, D.block_sizes + D.num_blocks);
+    int64_t rows = 0, mats = 0;
+    int max_block = 0;
+    for (int i = 0; i < D.num_blocks; ++i) {
+        if (sz[i] < 1 || sz[i] > 2048)
+            return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_create: block sizes must lie in [1, 2048]");
+        roff[i] = (int32_t)rows;
+        moff[i] = mats;
+        rows += sz[i];
+        mats += (int64_t)sz[i] * sz[i];
+        max_block = std::max(max_block, (int)sz[i]);
+    }
+    if (rows != D.n) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_create: block sizes do not sum to n");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(TFS_ERR_NO_DEVICE, "tfs_broyden_create: no HIP device visible");
+    tfs_broyden* b = new tfs_broyden();
+    b->desc = D;
+    b->desc.block_sizes = nullptr;
+    b->n = D.n;
+    b->max_iter = D.max_iter;
+    b->nblocks = D.num_blocks;
+    b->max_block = max_block;
+    b->ld = round32(D.n);
+    b->jac_len = mats;
+    b->G = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (D.n + 255) / 256));
+    b->RB = (int)((D.n / 6 + 256) / 256 + 1);   // residual blocks: one thread per segment or body
+    b->host_status.resize(D.num_blocks);
+    hipError_t e = hipSuccess;
+    if (D.stream) {
+        b->stream = (hipStream_t)D.stream;
+    } else {
+        e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+        b->own_stream = e == hipSuccess;
+    }
+    const int64_t parts[] = {b->ld * D.max_iter, b->ld * D.max_iter, b->ld, b->ld, b->ld, D.max_iter + 1,
+                             std::max<int64_t>((int64_t)b->G * (D.max_iter + 1), 2LL * b->RB), 8, mats, mats};
+    double** dst[] = {&b->C, &b->D, &b->slot[0], &b->slot[1], &b->t, &b->h, &b->partial, &b->scal, &b->J,
+                      &b->Jstage};
+    int64_t total = 0;
+    for (int64_t l : parts) total += round32(l);
+    if (e == hipSuccess) e = hipMalloc(&b->arena, total * 8);
+    if (e == hipSuccess) e = hipMalloc(&b->mat_off, sizeof(int64_t) * D.num_blocks);
+    if (e == hipSuccess) e = hipMalloc(&b->row_off, sizeof(int32_t) * D.num_blocks);
+    if (e == hipSuccess) e = hipMalloc(&b->sizes, sizeof(int32_t) * D.num_blocks);
+    if (e == hipSuccess) e = hipMalloc(&b->piv, sizeof(int32_t) * D.n);
+    if (e == hipSuccess) e = hipMalloc(&b->status, sizeof(int32_t) * D.num_blocks);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&b->host_scal, 64, hipHostMallocDefault);
+    if (e == hipSuccess) {
+        double* p = b->arena;
+        for (size_t i = 0; i < sizeof(parts) / sizeof(parts[0]); ++i) {
+            *dst[i] = p;
+            p += round32(parts[i]);
+        }
+        e = hipMemsetAsync(b->arena, 0, total * 8, b->stream);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->mat_off, moff.data(), sizeof(int64_t) * D.num_blocks, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->row_off, roff.data(), sizeof(int32_t) * D.num_blocks, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->sizes, sz.data(), sizeof(int32_t) * D.num_blocks, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) {
+        tfs_broyden_destroy(b);
+        return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_broyden_create: device memory");
+    }
+    *out = b;
+    return TFS_OK;
+}
+
+int tfs_broyden_destroy(tfs_broyden* b) {
+    if (!b) return TFS_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    (void)hipFree(b->arena);
+    (void)hipFree(b->mat_off);
+    (void)hipFree(b->row_off);
+    (void)hipFree(b->sizes);
+    (void)hipFree(b->piv);
+    (void)hipFree(b->status);
+    if (b->host_scal) (void)hipHostFree(b->host_scal);
+    if (b->own_stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return TFS_OK;
+}
+
+int tfs_broyden_set_jacobian_inverse(tfs_broyden* b, const double* blocks) {
+    if (!b || !blocks) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_set_jacobian_inverse: null argument");
+    int rc = copy_in(b, b->Jstage, blocks, b->jac_len, "tfs_broyden_set_jacobian_inverse");
+    if (rc) return rc;
+    hipLaunchKernelGGL(block_transpose_kernel, dim3(b->nblocks), dim3(256), 0, b->stream, b->Jstage, b->J,
+                       b->mat_off, b->sizes);
+    return check(hipGetLastError(), "tfs_broyden_set_jacobian_inverse");
+}
+
+int tfs_broyden_set_jacobian(tfs_broyden* b, const double* blocks) {
+    if (!b || !blocks) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_set_jacobian: null argument");
+    int rc = copy_in(b, b->Jstage, blocks, b->jac_len, "tfs_broyden_set_jacobian");
+    if (rc) return rc;
+    const size_t lds = sizeof(double) * 2 * b->max_block;
+    hipLaunchKernelGGL(block_gauss_jordan_kernel, dim3(b->nblocks), dim3(256), lds, b->stream, b->Jstage, b->mat_off,
+                       b->sizes, b->piv, b->row_off, b->status);
+    rc = check(hipGetLastError(), "Gauss-Jordan");
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(b->host_status.data(), b->status, sizeof(int32_t) * b->nblocks,
+                                  hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return hip_fail(e, "tfs_broyden_set_jacobian");
+    for (int i = 0; i < b->nblocks; ++i)
+        if (b->host_status[i])
+            return fail(TFS_ERR_NON_FINITE,
+                        "tfs_broyden_set_jacobian: block " + std::to_string(i) + " is singular or not finite");
+    hipLaunchKernelGGL(block_transpose_kernel, dim3(b->nblocks), dim3(256), 0, b->stream, b->Jstage, b->J,
+                       b->mat_off, b->sizes);
+    return check(hipGetLastError(), "tfs_broyden_set_jacobian");
+}
+
+int tfs_broyden_start(tfs_broyden* b, const double* error) {
+    if (!b) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_start: null handle");
+    b->iter = 0;
+    if (!error) return TFS_OK;
+    return copy_in(b, b->slot[0], error, b->n, "tfs_broyden_start");
+}
+
+int tfs_broyden_find_update(tfs_broyden* b, double* update) {
+    if (!b || !update) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_find_update: null argument");
+    double* out = host_mem(b) ? b->t : update;
+    int rc = inverse_update(b, b->slot[0], out);
+    if (rc) return rc;
+    if (host_mem(b)) {
+        hipError_t e = hipMemcpyAsync(update, out, (size_t)b->n * 8, hipMemcpyDeviceToHost, b->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+        return check(e, "tfs_broyden_find_update");
+    }
+    // device vectors: ordered on the caller's stream, or complete on return when
+    // the engine runs on a stream of its own
+    return b->own_stream ? check(hipStreamSynchronize(b->stream), "tfs_broyden_find_update") : TFS_OK;
+}
+
+int tfs_broyden_end_of_iter(tfs_broyden* b, const double* new_error, int32_t error_is_too_large) {
+    if (!b) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_end_of_iter: null handle");
+    const int n = b->n;
+    if (error_is_too_large && b->iter < b->max_iter - 1) {
+        int rc = new_error ? copy_in(b, b->slot[1], new_error, n, "tfs_broyden_end_of_iter") : TFS_OK;
+        double* Ck = b->C + (int64_t)b->iter * b->ld;
+        double* Dk = b->D + (int64_t)b->iter * b->ld;
+        if (!rc) rc = inverse_update(b, b->slot[1], Ck);
+        if (!rc) rc = check(k_sub(b->slot[1], b->slot[0], Dk, n, b->stream), "D_k");
+        if (!rc) rc = check(k_norm(Dk, n, b->partial, b->G, b->scal, b->stream), "||D_k||");
+        if (!rc) {
+            hipLaunchKernelGGL(reciprocal_kernel, dim3(1), dim3(64), 0, b->stream, b->scal);
+            rc = check(hipGetLastError(), "1/||D_k||");
+        }
+        if (!rc) rc = check(k_scale(Ck, Ck, b->scal, 1, n, b->stream), "C_k /= ||D_k||");
+        if (!rc) rc = check(k_scale(Dk, Dk, b->scal, 1, n, b->stream), "D_k /= ||D_k||");
+        if (rc) return rc;
+        std::swap(b->slot[0], b->slot[1]);   // error <- new_error
+    }
+    b->iter++;
+    return TFS_OK;
+}
+
+int tfs_broyden_status(tfs_broyden* b, int32_t* iter, double* error_norm) {
+    if (!b) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_status: null handle");
+    if (iter) *iter = b->iter;
+    if (error_norm) {
+        int rc = check(k_norm(b->slot[0], b->n, b->partial, b->G, b->scal, b->stream), "||error||");
+        if (rc) return rc;
+        hipError_t e = hipMemcpyAsync(b->host_scal, b->scal, 8, hipMemcpyDeviceToHost, b->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+        if (e != hipSuccess) return hip_fail(e, "tfs_broyden_status");
+        *error_norm = *b->host_scal;
+    }
+    return TFS_OK;
+}
+
+int tfs_broyden_compute_errors(tfs_broyden* b, const tfs_broyden_state* st, int32_t slot,
+                               int32_t* error_is_too_large) {
+    if (!b || !st || (slot != 0 && slot != 1))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_compute_errors: null argument or bad slot");
+    if (st->num_swim < 1 || st->num_fil < 0 || st->num_seg < 0 ||
+        6LL * ((int64_t)st->num_swim * st->num_fil * st->num_seg + st->num_swim) != b->n)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_compute_errors: 6*(NSWIM*NFIL*NSEG + NSWIM) != n");
+    const bool nsegs = (int64_t)st->num_fil * st->num_seg > 0;
+    if (!(st->dt > 0) || !(st->dl > 0) || !(st->tol > 0) || !st->body_x || !st->body_xm1 || !st->body_u ||
+        !st->v_bodies || (st->bdf2 && (!st->body_xm2 || !st->body_um1)) ||
+        (nsegs && (!st->seg_x || !st->seg_xm1 || !st->seg_u || !st->v_segs ||
+                   (st->bdf2 && (!st->seg_xm2 || !st->seg_um1)))))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_compute_errors: missing state array or bad step");
+    const int64_t nseg_tot = (int64_t)st->num_swim * st->num_fil * st->num_seg;
+    const int64_t nb = st->num_swim;
+    ResidualArgs a{};
+    a.nswim = st->num_swim;
+    a.nfil = st->num_fil;
+    a.nseg = st->num_seg;
+    a.bdf2 = st->bdf2 ? 1 : 0;
+    a.dt = st->dt;
+    a.pos_tol = st->dl * st->tol;
+    a.lie_tol = st->tol;
+    std::vector<void*> staged;
+    auto stage = [&](const double* src, int64_t count) -> const double* {
+        if (!src || count == 0) return src;
+        if (!host_mem(b)) return src;
+        void* d = nullptr;
+        if (hipMalloc(&d, count * 8) != hipSuccess) return nullptr;
+        staged.push_back(d);
+        if (hipMemcpyAsync(d, src, count * 8, hipMemcpyHostToDevice, b->stream) != hipSuccess) return nullptr;
+        return (const double*)d;
+    };
+    a.x = stage(st->seg_x, 3 * nseg_tot);
+    a.xm1 = stage(st->seg_xm1, 3 * nseg_tot);
+    a.xm2 = a.bdf2 ? stage(st->seg_xm2, 3 * nseg_tot) : nullptr;
+    a.u = stage(st->seg_u, 3 * nseg_tot);
+    a.um1 = a.bdf2 ? stage(st->seg_um1, 3 * nseg_tot) : nullptr;
+    a.v = stage(st->v_segs, 6 * nseg_tot);
+    a.bx = stage(st->body_x, 3 * nb);
+    a.bxm1 = stage(st->body_xm1, 3 * nb);
+    a.bxm2 = a.bdf2 ? stage(st->body_xm2, 3 * nb) : nullptr;
+    a.bu = stage(st->body_u, 3 * nb);
+    a.bum1 = a.bdf2 ? stage(st->body_um1, 3 * nb) : nullptr;
+    a.bv = stage(st->v_bodies, 6 * nb);
+    int rc = TFS_OK;
+    if ((nsegs && (!a.x || !a.xm1 || !a.u || !a.v || (a.bdf2 && (!a.xm2 || !a.um1)))) || !a.bx || !a.bxm1 ||
+        !a.bu || !a.bv || (a.bdf2 && (!a.bxm2 || !a.bum1)))
+        rc = fail(TFS_ERR_OUT_OF_MEMORY, "tfs_broyden_compute_errors: staging");
+    const int blocks = (int)((nseg_tot + nb + 255) / 256);
+    if (!rc && 2LL * blocks > std::max<int64_t>((int64_t)b->G * (b->max_iter + 1), 2LL * b->RB))
+        rc = fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_compute_errors: state larger than the engine");
+    if (!rc) {
+        hipLaunchKernelGGL(residual_kernel, dim3(blocks), dim3(256), 0, b->stream, a, b->slot[slot], b->partial);
+        hipLaunchKernelGGL(residual_finish_kernel, dim3(1), dim3(256), 0, b->stream, b->partial, blocks,
+                           b->scal + 2);
+        rc = check(hipGetLastError(), "residual");
+    }
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(b->host_scal, b->scal + 2, 16, hipMemcpyDeviceToHost, b->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "tfs_broyden_compute_errors");
+    } else {
+        (void)hipStreamSynchronize(b->stream);
+    }
+    for (void* p : staged) (void)hipFree(p);
+    if (rc) return rc;
+    if (error_is_too_large) *error_is_too_large = b->host_scal[0] > 1.0 ? 1 : 0;
+    if (b->host_scal[1] != 0.0) return fail(TFS_ERR_NON_FINITE, "tfs_broyden_compute_errors: residual not finite");
+    return TFS_OK;
+}
+
+int tfs_broyden_read_error(tfs_broyden* b, int32_t slot, double* out) {
+    if (!b || !out || (slot != 0 && slot != 1))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_read_error: null argument or bad slot");
+    hipError_t e = hipMemcpyAsync(out, b->slot[slot], (size_t)b->n * 8,
+                                  host_mem(b) ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, b->stream);
+    if (e == hipSuccess && (host_mem(b) || b->own_stream)) e = hipStreamSynchronize(b->stream);
+    return check(e, "tfs_broyden_read_error");
+}
+
+}  // extern "C"
