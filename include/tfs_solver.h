@@ -134,6 +134,47 @@ int tfs_beat_solver(tfs_beat *b, tfs_solver **out);
 /* Switch between TFS_SOLVE_FP64 and TFS_SOLVE_MIXED for later steps. */
 int tfs_beat_set_mode(tfs_beat *b, int32_t mode);
 
+/* ---- Elastic-filament velocities (non-prescribed cilia, body velocity prescribed) ----
+ * mobility_solver::compute_velocities without PRESCRIBED_CILIA
+ * (mobility_solver.cpp:2598-2655): the velocities of given segment forces and
+ * torques, with the blob forces that hold the body surfaces to the rigid-body
+ * motion V + Omega x (x_blob - x_body) (assemble_rhs, mobility_solver.cpp:439-570,
+ * PRESCRIBED_BODY_VELOCITIES):
+ *   v_segs  = M_ss f_segs                              (Mss_mult '=')
+ *   v_blobs = M_bs f_segs                              (Mbs_mult '=')
+ *   M_bb f_blobs = -v_blobs + K V_body                 (GMRES on M_bb, preconditioner
+ *                                                       6 pi mu RBLOB, mobility_solver.cpp:1439-1443)
+ *   v_segs += M_sb f_blobs                             (Msb_mult '+=')
+ * Segments carry 6 values (force and torque; velocity and angular velocity). */
+typedef struct tfs_velocities_desc {
+    int32_t num_swim;          /* NSWIM: bodies, each with the same segment and blob counts */
+    int32_t segs_per_swim;     /* NFIL*NSEG                                                 */
+    int32_t blobs_per_swim;    /* NBLOB (>= 1)                                              */
+    int32_t mode;              /* TFS_SOLVE_* of the blob solve; the segment blocks are
+                                  FP64 in TFS_SOLVE_FP64 and FP32-tile in TFS_SOLVE_MIXED   */
+    double seg_radius, blob_radius, mu;
+    int32_t restart;           /* GMRES restart of the blob solve                           */
+    int32_t max_cycles;
+    double tol;                /* relative residual of the blob solve                       */
+    double inner_tol;          /* mixed mode                                                */
+    int32_t max_refinements;   /* mixed mode                                                */
+    int32_t memory;            /* TFS_MEMORY_* of the arrays passed to tfs_velocities_compute */
+    void *stream;              /* hipStream_t or NULL                                        */
+} tfs_velocities_desc;
+
+typedef struct tfs_velocities tfs_velocities;
+
+int tfs_velocities_create(const tfs_velocities_desc *desc, tfs_velocities **out);
+int tfs_velocities_destroy(tfs_velocities *v);
+/* x_segs 3 and f_segs 6 per segment, x_blobs 3 per blob, x_bodies 3 and
+ * v_bodies 6 (V, Omega) per body; out: v_segs 6 per segment.  f_blobs (3 per
+ * blob) holds the warm start on entry and the solved blob forces on return.
+ * Complete on return.  TFS_ERR_NOT_CONVERGED if the blob solve misses its
+ * tolerance (outputs still written). */
+int tfs_velocities_compute(tfs_velocities *v, const double *x_segs, const double *f_segs,
+                           const double *x_blobs, const double *x_bodies, const double *v_bodies,
+                           double *v_segs, double *f_blobs, tfs_solve_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

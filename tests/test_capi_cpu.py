@@ -78,3 +78,28 @@ def test_create_reports_missing_device_on_cpu_host():
     h = ctypes.c_void_p()
     rc = L.lib().tfs_rpy_create(ctypes.byref(_desc()), ctypes.byref(h))
     assert rc == L.TFS_ERR_NO_DEVICE
+
+
+def test_broyden_and_velocities_reject_bad_descriptors():
+    import numpy as np
+    from timfilsimmod_amd.broyden import BroydenDesc
+    from timfilsimmod_amd.solver import VelocitiesDesc
+    lib = L.lib()
+    sizes = np.array([4, 5], dtype=np.int32)
+    d = BroydenDesc()
+    d.n, d.max_iter, d.confidence, d.num_blocks = 10, 5, 0.1, 2   # sizes sum to 9, not 10
+    d.block_sizes = sizes.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    h = ctypes.c_void_p()
+    assert lib.tfs_broyden_create(ctypes.byref(d), ctypes.byref(h)) == L.TFS_ERR_INVALID_ARGUMENT
+    d.n, d.confidence = 9, 0.0
+    assert lib.tfs_broyden_create(ctypes.byref(d), ctypes.byref(h)) == L.TFS_ERR_INVALID_ARGUMENT
+    assert not h.value
+    assert lib.tfs_broyden_destroy(None) == L.TFS_OK
+    assert lib.tfs_broyden_find_update(None, None) == L.TFS_ERR_INVALID_ARGUMENT
+    v = VelocitiesDesc()
+    v.num_swim, v.segs_per_swim, v.blobs_per_swim = 1, 10, 0   # a body needs blobs
+    assert lib.tfs_velocities_create(ctypes.byref(v), ctypes.byref(h)) == L.TFS_ERR_INVALID_ARGUMENT
+    v.blobs_per_swim, v.mode = 10, 7
+    assert lib.tfs_velocities_create(ctypes.byref(v), ctypes.byref(h)) == L.TFS_ERR_INVALID_ARGUMENT
+    assert not h.value
+    assert lib.tfs_velocities_destroy(None) == L.TFS_OK

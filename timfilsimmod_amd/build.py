@@ -26,6 +26,7 @@ SOURCES = [
     "tfs_solver_api.hip",
     "tfs_flow_api.hip",
     "tfs_broyden_api.hip",
+    "tfs_velocities_api.hip",
 ]
 
 # Packed FP32 VALU forms are switched off: only scalar FP32 and FP64 VALU.

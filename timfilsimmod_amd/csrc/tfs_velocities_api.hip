@@ -1,0 +1,194 @@
+// tfs_velocities_api.hip -- mobility_solver::compute_velocities for elastic
+// (non-prescribed) filaments on a body with prescribed velocity
+// (mobility_solver.cpp:2598-2655), behind include/tfs_solver.h.
+//
+// The reference runs Mss_mult and Mbs_mult, copies the blob velocities to the
+// host, assembles -v_blobs + K V on the host, runs GMRES on M_bb with host
+// Arnoldi, copies the blob forces back and runs Msb_mult.  Here every stage
+// stays on the device: two matvec evaluations of one RPY handle, a kernel for
+// the right-hand side, the device GMRES of a blob-only solver handle, and the
+// accumulating segment-blob evaluation, all on one stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/tfs_solver.h"
+#include "tfs_api_internal.h"
+
+using namespace tfs;
+
+namespace {
+
+// rhs[3m + d] = -v_blobs[3m + d] + (V + Omega x (x_blob - x_body))_d, the body
+// of blob m being m / blobs_per_swim (mobility_solver.cpp:442-446, 533-545).
+__global__ __launch_bounds__(256) void blob_rhs_kernel(const double* __restrict__ v_blobs,
+                                                       const double* __restrict__ x_blobs,
+                                                       const double* __restrict__ x_bodies,
+                                                       const double* __restrict__ v_bodies, int nblob,
+                                                       int blobs_per_swim, int have_segs,
+                                                       double* __restrict__ rhs) {
+    const int m = blockIdx.x * 256 + threadIdx.x;
+    if (m >= nblob) return;
+    const int b = m / blobs_per_swim;
+    const double r0 = x_blobs[3 * m] - x_bodies[3 * b];
+    const double r1 = x_blobs[3 * m + 1] - x_bodies[3 * b + 1];
+    const double r2 = x_blobs[3 * m + 2] - x_bodies[3 * b + 2];
+    const double* V = v_bodies + 6 * b;
+    const double k0 = V[0] + V[4] * r2 - V[5] * r1;
+    const double k1 = V[1] + V[5] * r0 - V[3] * r2;
+    const double k2 = V[2] + V[3] * r1 - V[4] * r0;
+    rhs[3 * m] = (have_segs ? -v_blobs[3 * m] : 0.0) + k0;
+    rhs[3 * m + 1] = (have_segs ? -v_blobs[3 * m + 1] : 0.0) + k1;
+    rhs[3 * m + 2] = (have_segs ? -v_blobs[3 * m + 2] : 0.0) + k2;
+}
+
+int check(hipError_t e, const char* where) { return e == hipSuccess ? TFS_OK : hip_fail(e, where); }
+
+}  // namespace
+
+struct tfs_velocities {
+    tfs_velocities_desc desc{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    tfs_rpy* rpy = nullptr;       // segments (6 DOF) and blobs, BS overwriting
+    tfs_solver* blobs = nullptr;  // GMRES on M_bb
+    int64_t ns = 0, nb = 0, nswim = 0;
+    double* d_rhs = nullptr;      // 3*nb
+    double* d_bodies = nullptr;   // 3*nswim positions then 6*nswim velocities
+};
+
+extern "C" {
+
+int tfs_velocities_create(const tfs_velocities_desc* desc, tfs_velocities** out) {
+    g_last_error.clear();
+    if (!desc || !out) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_create: null argument");
+    *out = nullptr;
+    const tfs_velocities_desc& D = *desc;
+    if (D.num_swim < 1 || D.segs_per_swim < 0 || D.blobs_per_swim < 1)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_create: bad sizes");
+    if (D.memory != TFS_MEMORY_HOST && D.memory != TFS_MEMORY_DEVICE)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_create: unknown memory space");
+    if (D.mode != TFS_SOLVE_FP64 && D.mode != TFS_SOLVE_MIXED)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_create: unknown mode");
+    const int64_t ns = (int64_t)D.num_swim * D.segs_per_swim, nb = (int64_t)D.num_swim * D.blobs_per_swim;
+    if (ns > INT32_MAX / 8 || nb > INT32_MAX / 8)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_create: too many particles");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(TFS_ERR_NO_DEVICE, "tfs_velocities_create: no HIP device visible");
+    tfs_velocities* v = new tfs_velocities();
+    v->desc = D;
+    v->ns = ns;
+    v->nb = nb;
+    v->nswim = D.num_swim;
+    hipError_t e = hipSuccess;
+    if (D.stream) {
+        v->stream = (hipStream_t)D.stream;
+    } else {
+        e = hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking);
+        v->own_stream = e == hipSuccess;
+    }
+    if (e != hipSuccess) {
+        delete v;
+        return hip_fail(e, "tfs_velocities_create: stream");
+    }
+    tfs_rpy_desc rd{};
+    rd.num_segs = (int32_t)ns;
+    rd.num_blobs = (int32_t)nb;
+    rd.seg_dof = 6;
+    rd.precision = D.mode == TFS_SOLVE_MIXED ? TFS_PRECISION_FP32_ACC64 : TFS_PRECISION_FP64;
+    rd.bs_mode = TFS_BS_OVERWRITE;
+    rd.memory = TFS_MEMORY_DEVICE;
+    rd.seg_radius = D.seg_radius;
+    rd.blob_radius = D.blob_radius;
+    rd.mu = D.mu;
+    rd.stream = v->stream;
+    int rc = tfs_rpy_create(&rd, &v->rpy);
+    tfs_solver_desc sd{};
+    sd.num_fil = 0;
+    sd.num_seg = 0;
+    sd.num_blobs = (int32_t)nb;
+    sd.mode = D.mode;
+    sd.seg_radius = D.seg_radius;
+    sd.blob_radius = D.blob_radius;
+    sd.mu = D.mu;
+    sd.restart = D.restart;
+    sd.max_cycles = D.max_cycles;
+    sd.tol = D.tol;
+    sd.inner_tol = D.inner_tol;
+    sd.max_refinements = D.max_refinements;
+    sd.stream = v->stream;
+    if (!rc) rc = tfs_solver_create(&sd, &v->blobs);
+    if (!rc) rc = check(hipMalloc(&v->d_rhs, sizeof(double) * 3 * nb), "tfs_velocities_create: rhs");
+    if (!rc) rc = check(hipMalloc(&v->d_bodies, sizeof(double) * 9 * v->nswim), "tfs_velocities_create: bodies");
+    if (rc) {
+        const std::string msg = g_last_error;
+        tfs_velocities_destroy(v);
+        g_last_error = msg;
+        return rc;
+    }
+    *out = v;
+    return TFS_OK;
+}
+
+int tfs_velocities_destroy(tfs_velocities* v) {
+    if (!v) return TFS_OK;
+    if (v->stream) (void)hipStreamSynchronize(v->stream);
+    if (v->blobs) tfs_solver_destroy(v->blobs);
+    if (v->rpy) tfs_rpy_destroy(v->rpy);
+    if (v->d_rhs) (void)hipFree(v->d_rhs);
+    if (v->d_bodies) (void)hipFree(v->d_bodies);
+    if (v->own_stream) (void)hipStreamDestroy(v->stream);
+    delete v;
+    return TFS_OK;
+}
+
+int tfs_velocities_compute(tfs_velocities* v, const double* x_segs, const double* f_segs, const double* x_blobs,
+                           const double* x_bodies, const double* v_bodies, double* v_segs, double* f_blobs,
+                           tfs_solve_stats* stats) {
+    if (!v || !x_blobs || !x_bodies || !v_bodies || !f_blobs || (v->ns && (!x_segs || !f_segs || !v_segs)))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_compute: null argument");
+    const hipMemcpyKind kin = v->desc.memory == TFS_MEMORY_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    const hipMemcpyKind kout = v->desc.memory == TFS_MEMORY_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    tfs_rpy* h = v->rpy;
+    const int64_t ns = v->ns, nb = v->nb;
+    hipStream_t st = v->stream;
+    // read_positions_and_forces (mobility_solver.cpp:176-236)
+    int rc = check(hipMemcpyAsync(h->x_blobs, x_blobs, 24 * nb, kin, st), "blob positions");
+    if (!rc && ns) rc = check(hipMemcpyAsync(h->x_segs, x_segs, 24 * ns, kin, st), "segment positions");
+    if (!rc && ns) rc = check(hipMemcpyAsync(h->f_segs, f_segs, 48 * ns, kin, st), "segment forces");
+    if (!rc) rc = check(hipMemcpyAsync(h->f_blobs, f_blobs, 24 * nb, kin, st), "blob warm start");
+    if (!rc) rc = check(hipMemcpyAsync(v->d_bodies, x_bodies, 24 * v->nswim, kin, st), "body positions");
+    if (!rc) rc = check(hipMemcpyAsync(v->d_bodies + 3 * v->nswim, v_bodies, 48 * v->nswim, kin, st), "body velocities");
+    if (!rc) rc = tfs_rpy_positions_changed(h);
+    // Mss_mult '=', Mbs_mult '='
+    if (!rc && ns) rc = tfs_rpy_apply(h, TFS_BLOCK_SS, h->f_segs, nullptr, h->v_segs, nullptr);
+    if (!rc && ns) rc = tfs_rpy_apply(h, TFS_BLOCK_BS, h->f_segs, nullptr, nullptr, h->v_blobs);
+    if (!rc) {
+        hipLaunchKernelGGL(blob_rhs_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, h->v_blobs,
+                           h->x_blobs, v->d_bodies, v->d_bodies + 3 * v->nswim, (int)nb, v->desc.blobs_per_swim,
+                           ns > 0 ? 1 : 0, v->d_rhs);
+        rc = check(hipGetLastError(), "blob rhs");
+    }
+    // GMRES on M_bb for the blob forces, warm-started from the given ones
+    if (!rc) rc = tfs_solver_set_geometry(v->blobs, nullptr, h->x_blobs, nullptr, nullptr, TFS_MEMORY_DEVICE);
+    int solve_rc = TFS_OK;
+    if (!rc) {
+        solve_rc = tfs_solver_solve(v->blobs, v->d_rhs, h->f_blobs, TFS_MEMORY_DEVICE, stats);
+        if (solve_rc != TFS_OK && solve_rc != TFS_ERR_NOT_CONVERGED) rc = solve_rc;
+    }
+    const std::string solve_msg = solve_rc ? g_last_error : std::string();
+    // Msb_mult '+='
+    if (!rc && ns) rc = tfs_rpy_apply(h, TFS_BLOCK_SB, nullptr, h->f_blobs, h->v_segs, nullptr);
+    if (!rc && ns) rc = check(hipMemcpyAsync(v_segs, h->v_segs, 48 * ns, kout, st), "segment velocities");
+    if (!rc) rc = check(hipMemcpyAsync(f_blobs, h->f_blobs, 24 * nb, kout, st), "blob forces");
+    if (!rc) rc = check(hipStreamSynchronize(st), "tfs_velocities_compute");
+    if (rc) return rc;
+    if (solve_rc) {
+        g_last_error = solve_msg;
+        return solve_rc;
+    }
+    return TFS_OK;
+}
+
+}  // extern "C"

@@ -77,6 +77,23 @@ L.extra_signatures({
 })
 
 
+class VelocitiesDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_swim", ctypes.c_int32), ("segs_per_swim", ctypes.c_int32), ("blobs_per_swim", ctypes.c_int32),
+        ("mode", ctypes.c_int32), ("seg_radius", ctypes.c_double), ("blob_radius", ctypes.c_double),
+        ("mu", ctypes.c_double), ("restart", ctypes.c_int32), ("max_cycles", ctypes.c_int32),
+        ("tol", ctypes.c_double), ("inner_tol", ctypes.c_double), ("max_refinements", ctypes.c_int32),
+        ("memory", ctypes.c_int32), ("stream", ctypes.c_void_p),
+    ]
+
+
+L.extra_signatures({
+    "tfs_velocities_create": (ctypes.c_int, [ctypes.POINTER(VelocitiesDesc), ctypes.POINTER(_dp)]),
+    "tfs_velocities_destroy": (ctypes.c_int, [_dp]),
+    "tfs_velocities_compute": (ctypes.c_int, [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(SolveStats)]),
+})
+
+
 def solver_desc(num_fil, num_seg, num_blobs, *, dynamic_phase=True, dynamic_angle=True, mode=SOLVE_FP64,
                 seg_radius=1.0, blob_radius=1.0, mu=1.0, restart=50, max_cycles=20, tol=1e-8,
                 inner_tol=1e-3, max_refinements=10, stream=None) -> SolverDesc:
@@ -146,6 +163,54 @@ class Solver:
         L.check(L.lib().tfs_solver_residual(self.h, L.ptr(rhs), L.ptr(x), precision, L.MEMORY_HOST,
                                             ctypes.byref(out)))
         return out.value
+
+
+class ElasticVelocities:
+    """mobility_solver::compute_velocities for elastic filaments on bodies with
+    prescribed velocity (mobility_solver.cpp:2598-2655): segment velocities of
+    given segment forces/torques plus the blob forces that keep each body
+    surface in rigid motion.  Host arrays; every stage runs on the GPU."""
+
+    def __init__(self, num_swim, segs_per_swim, blobs_per_swim, *, mode=SOLVE_FP64, seg_radius=1.0,
+                 blob_radius=1.0, mu=1.0, restart=30, max_cycles=50, tol=1e-10, inner_tol=1e-3,
+                 max_refinements=10, stream=None):
+        d = VelocitiesDesc()
+        d.num_swim, d.segs_per_swim, d.blobs_per_swim, d.mode = num_swim, segs_per_swim, blobs_per_swim, mode
+        d.seg_radius, d.blob_radius, d.mu = seg_radius, blob_radius, mu
+        d.restart, d.max_cycles, d.tol = restart, max_cycles, tol
+        d.inner_tol, d.max_refinements = inner_tol, max_refinements
+        d.memory, d.stream = L.MEMORY_HOST, stream
+        self.desc = d
+        self.ns = num_swim * segs_per_swim
+        self.nb = num_swim * blobs_per_swim
+        h = ctypes.c_void_p()
+        L.check(L.lib().tfs_velocities_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        self.f_blobs = np.zeros(3 * self.nb)
+
+    def close(self):
+        if self.h:
+            L.lib().tfs_velocities_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def compute(self, x_segs, f_segs, x_blobs, x_bodies, v_bodies, *, allow_unconverged=False):
+        """Returns (v_segs [6 per segment], f_blobs [3 per blob], solve stats);
+        the blob forces warm-start the next call."""
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (x_segs, f_segs, x_blobs, x_bodies, v_bodies)]
+        v_segs = np.zeros(6 * self.ns)
+        st = SolveStats()
+        rc = L.lib().tfs_velocities_compute(self.h, *[L.ptr(v) if v.size else None for v in a],
+                                            L.ptr(v_segs) if v_segs.size else None, L.ptr(self.f_blobs),
+                                            ctypes.byref(st))
+        if not (allow_unconverged and rc == L.TFS_ERR_NOT_CONVERGED):
+            L.check(rc)
+        return v_segs, self.f_blobs.copy(), st.as_dict()
 
 
 @dataclass
