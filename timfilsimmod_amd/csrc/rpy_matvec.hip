@@ -302,25 +302,56 @@ __device__ void add_near(double* acc, int src, const double* xi, double ai, cons
     else pair_near<VT, 3>(acc, d0, d1, d2, f, ai, aj, a.mu);
 }
 
-// Adds the partial slots of each tile in item order, then the overlapping
-// pairs (sorted by source index), and applies the write rule.
+// Adds the partial slots of each tile, then the overlapping pairs in
+// increasing source order (the lists are sorted when built), and applies the
+// write rule.  A workgroup covers 64 targets of one tile with four groups of
+// 64 threads: group g sums the items g, g+4, g+8, ... (loads of four items
+// issued together), and the four group sums are added in group order, so the
+// result is the same on every run.
 template <int SEGDOF>
-__global__ __launch_bounds__(64) void rpy_reduce_kernel(const MatvecTile* __restrict__ tiles,
-                                                        const double* __restrict__ partial, int tile_targets,
-                                                        OutArgs o, MatvecArgs a, NearList near) {
-    const MatvecTile tl = tiles[blockIdx.x / ((tile_targets + 63) / 64)];
-    const int l = (blockIdx.x % ((tile_targets + 63) / 64)) * 64 + threadIdx.x;
+__global__ __launch_bounds__(256) void rpy_reduce_kernel(const MatvecTile* __restrict__ tiles,
+                                                         const double* __restrict__ partial, int tile_targets,
+                                                         OutArgs o, MatvecArgs a, NearList near) {
+    __shared__ double grp_acc[3][6][64];
+    const int per_tile = (tile_targets + 63) / 64;
+    const MatvecTile tl = tiles[blockIdx.x / per_tile];
+    const int l = (blockIdx.x % per_tile) * 64 + (threadIdx.x & 63);
+    const int grp = threadIdx.x >> 6;
     const int t = tl.t_begin + l;
-    if (l >= tile_targets || t >= tl.t_end) return;
+    const bool active = l < tile_targets && t < tl.t_end;
     const bool seg_t = tl.cls == 0;
-    const int slots = seg_t ? o.seg_out_slots : 3;
-    const View out = seg_t ? o.vs : o.vb;
-    const size_t slot_len = (size_t)kMaxDof * tile_targets;
-    double acc[6] = {0, 0, 0, 0, 0, 0};
     const int vt = seg_t ? SEGDOF : 3;
-    const double* p = partial + (size_t)tl.first_item * slot_len + l;
-    for (int m = 0; m < tl.n_items; ++m)
-        for (int k = 0; k < vt; ++k) acc[k] += p[(size_t)m * slot_len + (size_t)k * tile_targets];
+    const size_t slot_len = (size_t)kMaxDof * tile_targets;
+    const int lc = l < tile_targets ? l : tile_targets - 1;
+    const double* p = partial + (size_t)tl.first_item * slot_len + lc;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int m0 = grp; m0 < tl.n_items; m0 += 16) {
+        double v[4][6];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int m = m0 + 4 * u;
+            const int mm = m < tl.n_items ? m : grp;
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                v[u][k] = k < vt ? p[(size_t)mm * slot_len + (size_t)k * tile_targets] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const bool in = m0 + 4 * u < tl.n_items;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc[k] += in ? v[u][k] : 0.0;
+        }
+    }
+    if (grp > 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) grp_acc[grp - 1][k][threadIdx.x & 63] = acc[k];
+    }
+    __syncthreads();
+    if (grp > 0 || !active) return;
+#pragma unroll
+    for (int g = 0; g < 3; ++g)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[k] += grp_acc[g][k][threadIdx.x];
     // Active source classes for this target class: bit 0 segments, bit 1 blobs.
     const uint32_t cls_mask = seg_t ? ((o.blocks & 1u) ? 1u : 0u) | ((o.blocks & 2u) ? 2u : 0u)
                                     : ((o.blocks & 8u) ? 1u : 0u) | ((o.blocks & 4u) ? 2u : 0u);
@@ -332,23 +363,36 @@ __global__ __launch_bounds__(64) void rpy_reduce_kernel(const MatvecTile* __rest
         const double xi[3] = {xv.p[(int64_t)t * xv.ld_p], xv.p[(int64_t)t * xv.ld_p + xv.ld_c],
                               xv.p[(int64_t)t * xv.ld_p + 2 * xv.ld_c]};
         const double ai = seg_t ? a.a_seg : a.a_blob;
-        // Visit the entries in increasing source order (selection by successive minima).
-        int last = -1;
         for (int q = 0; q < c; ++q) {
-            int best = 0x7fffffff;
-            for (int u = 0; u < c; ++u) {
-                const int v = e[u];
-                if (v > last && v < best) best = v;
-            }
-            last = best;
-            if (seg_t && SEGDOF == 6) add_near<6>(acc, best, xi, ai, a, SEGDOF, cls_mask);
-            else add_near<3>(acc, best, xi, ai, a, SEGDOF, cls_mask);
+            if (seg_t && SEGDOF == 6) add_near<6>(acc, e[q], xi, ai, a, SEGDOF, cls_mask);
+            else add_near<3>(acc, e[q], xi, ai, a, SEGDOF, cls_mask);
         }
     }
+    const int slots = seg_t ? o.seg_out_slots : 3;
+    const View out = seg_t ? o.vs : o.vb;
     for (int k = 0; k < slots; ++k) {
         double* dst = out.p + (int64_t)t * out.ld_p + k * out.ld_c;
         if (tl.write_mode == 0) *dst = k < vt ? acc[k] : 0.0;
         else if (k < vt) *dst += acc[k];
+    }
+}
+
+// Sorts every target's near list into increasing source order (insertion
+// sort of at most `cap` entries, one thread per target).
+__global__ __launch_bounds__(256) void near_sort_kernel(const int32_t* __restrict__ cnt, int32_t* __restrict__ idx,
+                                                        int cap, int64_t n) {
+    const int64_t tg = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (tg >= n) return;
+    const int c = cnt[tg] < cap ? cnt[tg] : cap;
+    int32_t* e = idx + tg * cap;
+    for (int i = 1; i < c; ++i) {
+        const int32_t v = e[i];
+        int j = i - 1;
+        while (j >= 0 && e[j] > v) {
+            e[j + 1] = e[j];
+            --j;
+        }
+        e[j + 1] = v;
     }
 }
 
@@ -428,6 +472,11 @@ hipError_t build_near_list(int precision, const MatvecArgs& a, const MatvecItem*
                                near.cnt, near.idx, near.cap, near.max_cnt);
     }
     TFS_HIP_CHECK(hipGetLastError());
+    if (n > 0) {
+        hipLaunchKernelGGL(near_sort_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, near.cnt,
+                           near.idx, near.cap, n);
+        TFS_HIP_CHECK(hipGetLastError());
+    }
     TFS_HIP_CHECK(hipMemcpyAsync(host_scratch, near.max_cnt, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
     TFS_HIP_CHECK(hipStreamSynchronize(stream));
     *max_count = *host_scratch;
@@ -455,9 +504,9 @@ hipError_t launch_matvec(int precision, int seg_dof, const TileShape& shape, con
     const int tt = shape.block * shape.per_thread;
     const unsigned grid = (unsigned)n_tiles * (unsigned)((tt + 63) / 64);
     if (seg_dof == 6)
-        hipLaunchKernelGGL(rpy_reduce_kernel<6>, dim3(grid), dim3(64), 0, stream, d_tiles, partial, tt, o, a, near);
+        hipLaunchKernelGGL(rpy_reduce_kernel<6>, dim3(grid), dim3(256), 0, stream, d_tiles, partial, tt, o, a, near);
     else
-        hipLaunchKernelGGL(rpy_reduce_kernel<3>, dim3(grid), dim3(64), 0, stream, d_tiles, partial, tt, o, a, near);
+        hipLaunchKernelGGL(rpy_reduce_kernel<3>, dim3(grid), dim3(256), 0, stream, d_tiles, partial, tt, o, a, near);
     return hipGetLastError();
 }
 

@@ -21,6 +21,7 @@ namespace tfs {
 namespace {
 
 constexpr int kWave = 64;
+constexpr int kMaxBasis = 352;   // columns of V an Arnoldi step can take (restart <= 350, plus one)
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -172,6 +173,319 @@ __global__ __launch_bounds__(256) void sub_kernel(const double* __restrict__ x, 
                                                  double* __restrict__ out, int n) {
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r < n) out[r] = x[r] - y[r];
+}
+
+// ---------------------------------------------------------------------------
+// Arnoldi step (CGS2) in four launches.  Workgroup b owns the row slices
+// b, b + G, b + 2G, ... of kSlice rows.  In a dot phase its four waves take
+// the basis columns round robin and every lane sums its rows, so the column
+// reductions need no workgroup barrier.  A launch that needs the previous
+// launch's coefficients re-reduces that launch's per-workgroup partial sums
+// itself -- every workgroup in the same order, so all of them see identical
+// values -- instead of synchronising across workgroups.
+//   dots:          p1 = partial V^T w
+//   update_dots:   h1 = sum p1; w -= V h1; p2 = partial V^T w
+//   update_sumsq:  h2 = sum p2; w -= V h2; p3 = partial ||w||^2
+//   close:         ||w||; Givens; 1/||w||; est[j]
+// Workgroup 0 writes H(:, j) = h1 + h2.  All sums run in a fixed order: the
+// step is deterministic.
+// ---------------------------------------------------------------------------
+constexpr int kSlice = 512;
+
+struct ArnoldiArgs {
+    const double* V;     // basis, column-major
+    int64_t ld;
+    int ncols;           // j + 1
+    double* w;
+    int n;
+};
+
+// partial[c*G + b] = sum over the rows of workgroup b of V[r, c] * w[r].
+// Each wave takes four columns at a time (kSlice/64 rows x 4 columns of loads
+// in flight per lane), with the lane's rows of w held in registers.
+__device__ __forceinline__ void slice_dots(const ArnoldiArgs& a, double* __restrict__ partial, int b, int G) {
+    constexpr int Q = kSlice / kWave;
+    const double* __restrict__ V = a.V;
+    const double* __restrict__ w = a.w;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int64_t last = a.n - 1;
+    for (int c0 = 4 * wid; c0 < a.ncols; c0 += 16) {
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        const int nc = a.ncols - c0 < 4 ? a.ncols - c0 : 4;
+        for (int64_t r0 = (int64_t)b * kSlice; r0 < a.n; r0 += (int64_t)G * kSlice) {
+            // Branch-free tail: rows past n read row n-1 and weigh it by zero.
+            // All loads of the slice are issued before the first product.
+            int64_t idx[Q];
+            double wv[Q], vv[4][Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int64_t r = r0 + lane + q * kWave;
+                idx[q] = r < a.n ? r : last;
+                wv[q] = w[idx[q]];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double* v = V + (int64_t)(c0 + (u < nc ? u : 0)) * a.ld;
+#pragma unroll
+                for (int q = 0; q < Q; ++q) vv[u][q] = v[idx[q]];
+            }
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const double wq = r0 + lane + q * kWave < a.n ? wv[q] : 0.0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) s[u] += vv[u][q] * wq;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double t = wave_sum(s[u]);
+            if (lane == 0 && u < nc) partial[(int64_t)(c0 + u) * G + b] = t;
+        }
+    }
+}
+
+// Sum of p[0 .. G) by one wave (lanes stride over the entries, then a
+// shuffle tree): valid in lane 0.
+__device__ __forceinline__ double wave_sum_strided(const double* p, int G) {
+    const int lane = threadIdx.x & (kWave - 1);
+    double s = 0.0;
+    for (int b = lane; b < G; b += kWave) s += p[b];
+    return wave_sum(s);
+}
+
+// hs[c] = sum_b partial[c*G + b] into LDS; workgroup 0 also adds it to Hcol.
+// Waves take four columns at a time; lanes stride over b with every load of a
+// column group issued before the sums (entries past G are weighed by zero).
+// Ends with a barrier.
+__device__ __forceinline__ void gather_coeffs(const double* __restrict__ partial, int ncols, int G, double* hs,
+                                              double* __restrict__ Hcol, bool first) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    for (int c0 = 4 * wid; c0 < ncols; c0 += 16) {
+        const int nc = ncols - c0 < 4 ? ncols - c0 : 4;
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int b0 = 0; b0 < G; b0 += 4 * kWave) {
+            double pv[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int bb = b0 + lane + q * kWave;
+                    pv[u][q] = partial[(int64_t)(c0 + (u < nc ? u : 0)) * G + (bb < G ? bb : G - 1)];
+                }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const bool in = b0 + lane + q * kWave < G;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) s[u] += in ? pv[u][q] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double t = wave_sum(s[u]);
+            if (lane == 0 && u < nc) {
+                hs[c0 + u] = t;
+                if (blockIdx.x == 0) Hcol[c0 + u] = first ? t : Hcol[c0 + u] + t;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// w[r] -= sum_c V[r, c] h[c] over the rows of workgroup b (columns four at
+// a time, their loads issued together).
+__device__ __forceinline__ void slice_update(const ArnoldiArgs& a, const double* h, int b, int G) {
+    constexpr int Q = kSlice / 256;
+    const double* __restrict__ V = a.V;
+    double* __restrict__ w = a.w;
+    for (int64_t r0 = (int64_t)b * kSlice + threadIdx.x; r0 < a.n; r0 += (int64_t)G * kSlice) {
+        double s[Q];
+        int64_t idx[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            s[q] = 0.0;
+            idx[q] = r0 + q * 256 < a.n ? r0 + q * 256 : a.n - 1;
+        }
+        int c = 0;
+        for (; c + 4 <= a.ncols; c += 4) {
+            double vv[4][Q];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int q = 0; q < Q; ++q) vv[u][q] = V[(int64_t)(c + u) * a.ld + idx[q]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int q = 0; q < Q; ++q) s[q] += vv[u][q] * h[c + u];
+        }
+        for (; c < a.ncols; ++c)
+#pragma unroll
+            for (int q = 0; q < Q; ++q) s[q] += V[(int64_t)c * a.ld + idx[q]] * h[c];
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            if (r0 + q * 256 < a.n) w[r0 + q * 256] -= s[q];
+    }
+}
+
+__global__ __launch_bounds__(256) void arnoldi_dots_kernel(ArnoldiArgs a, double* __restrict__ p_out) {
+    slice_dots(a, p_out, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(256) void arnoldi_update_dots_kernel(ArnoldiArgs a, const double* __restrict__ p_in,
+                                                                  double* __restrict__ p_out,
+                                                                  double* __restrict__ Hcol) {
+    __shared__ double hs[kMaxBasis];
+    gather_coeffs(p_in, a.ncols, gridDim.x, hs, Hcol, true);
+    slice_update(a, hs, blockIdx.x, gridDim.x);
+    __syncthreads();  // the slice's rows of w are final before the dots read them
+    slice_dots(a, p_out, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(256) void arnoldi_update_sumsq_kernel(ArnoldiArgs a, const double* __restrict__ p_in,
+                                                                   double* __restrict__ p_out,
+                                                                   double* __restrict__ Hcol) {
+    __shared__ double hs[kMaxBasis];
+    __shared__ double sh[4];
+    gather_coeffs(p_in, a.ncols, gridDim.x, hs, Hcol, false);
+    slice_update(a, hs, blockIdx.x, gridDim.x);
+    __syncthreads();
+    double s = 0.0;
+    for (int64_t r = (int64_t)blockIdx.x * kSlice + threadIdx.x; r < a.n; r += (int64_t)gridDim.x * kSlice) {
+#pragma unroll
+        for (int q = 0; q < kSlice / 256; ++q) {
+            const int64_t rr = r + q * 256;
+            if (rr < a.n) s += a.w[rr] * a.w[rr];
+        }
+    }
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) p_out[blockIdx.x] = s;
+}
+
+// ||w|| from the G partial sums, then the Givens bookkeeping of givens_kernel;
+// est[j] = |g(j+1)|.  One wave: the column of H and the previous rotations
+// are staged in LDS so that lane 0's sequential rotations read no memory.
+__global__ __launch_bounds__(64) void arnoldi_close_kernel(const double* __restrict__ p_in, int G,
+                                                           double* __restrict__ H, int ldh, int j,
+                                                           double* __restrict__ cs, double* __restrict__ sn,
+                                                           double* __restrict__ g, double* __restrict__ scal,
+                                                           double* __restrict__ est) {
+    __shared__ double col[kMaxBasis + 1], lc[kMaxBasis], ls[kMaxBasis];
+    double* Hc = H + (int64_t)j * ldh;
+    for (int i = threadIdx.x; i <= j; i += kWave) {
+        col[i] = Hc[i];
+        lc[i] = cs[i];
+        ls[i] = sn[i];
+    }
+    const double t = wave_sum_strided(p_in, G);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double hn = __builtin_sqrt(t);
+        col[j + 1] = hn;
+        scal[1] = hn > 0.0 ? 1.0 / hn : 0.0;
+        for (int i = 1; i <= j; ++i) {
+            const double u = lc[i - 1] * col[i - 1] + ls[i - 1] * col[i];
+            col[i] = lc[i - 1] * col[i] - ls[i - 1] * col[i - 1];
+            col[i - 1] = u;
+        }
+        double c, sv;
+        const double x = col[j], y = col[j + 1];
+        if (y == 0.0) {
+            c = 1.0;
+            sv = 0.0;
+        } else if (__builtin_fabs(y) > __builtin_fabs(x)) {
+            const double q = x / y;
+            sv = (y > 0 ? 1.0 : -1.0) / __builtin_sqrt(1.0 + q * q);
+            c = q * sv;
+        } else {
+            const double q = y / x;
+            c = (x > 0 ? 1.0 : -1.0) / __builtin_sqrt(1.0 + q * q);
+            sv = q * c;
+        }
+        cs[j] = c;
+        sn[j] = sv;
+        col[j] = c * x + sv * y;
+        col[j + 1] = 0.0;
+        const double gj = g[j];
+        g[j + 1] = -sv * gj;
+        g[j] = c * gj;
+        scal[0] = __builtin_fabs(g[j + 1]);
+        est[j] = scal[0];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i <= j + 1; i += kWave) Hc[i] = col[i];
+}
+
+// The next basis vector and its preconditioned image in one pass:
+// v = w * scale (scale = *dev_scale, or host_scale when dev_scale is null),
+// z = P^{-1} v, with the filament-wave layout of precond_kernel.
+__global__ __launch_bounds__(256) void scale_precond_kernel(const double* __restrict__ w,
+                                                           const double* __restrict__ dev_scale,
+                                                           double host_scale, double* __restrict__ v,
+                                                           double* __restrict__ out,
+                                                           const double* __restrict__ kphi,
+                                                           const double* __restrict__ kth, SaddleDims d,
+                                                           double gs, double gb) {
+    const double sc = dev_scale ? dev_scale[1] : host_scale;
+    const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int nwaves = gridDim.x * blockDim.x / kWave;
+    for (int m = gw; m < d.nfil; m += nwaves) {
+        const int len = 3 * d.nseg;
+        const int64_t seg0 = (int64_t)m * len;
+        double dp = 0.0, dt = 0.0, c11 = 0.0, c22 = 0.0, c12 = 0.0;
+        for (int k = lane; k < len; k += kWave) {
+            const double kp = kphi[seg0 + k], kt = kth[seg0 + k];
+            const double s = gs * (w[seg0 + k] * sc);
+            dp += kp * s;
+            dt += kt * s;
+            c11 += kp * kp;
+            c22 += kt * kt;
+            c12 += kp * kt;
+        }
+        dp = wave_sum(dp);
+        dt = wave_sum(dt);
+        c11 = wave_sum(c11);
+        c22 = wave_sum(c22);
+        c12 = wave_sum(c12);
+        double p = 0.0, q = 0.0;
+        if (lane == 0) {
+            const double inp = d.dyn_phase ? w[d.off_phase + m] * sc : 0.0;
+            const double ina = d.dyn_angle ? w[d.off_angle + m] * sc : 0.0;
+            if (d.dyn_phase && d.dyn_angle) {
+                const double r1 = -inp - dp;
+                const double r2 = -ina - dt;
+                const double det = c11 * c22 - c12 * c12;
+                p = (c22 * r1 - c12 * r2) / det / gs;
+                q = (-c12 * r1 + c11 * r2) / det / gs;
+            } else if (d.dyn_phase) {
+                p = (-inp - dp) / (c11 * gs);
+            } else if (d.dyn_angle) {
+                q = (-ina - dt) / (c22 * gs);
+            }
+            if (d.dyn_phase) {
+                v[d.off_phase + m] = inp;
+                out[d.off_phase + m] = p;
+            }
+            if (d.dyn_angle) {
+                v[d.off_angle + m] = ina;
+                out[d.off_angle + m] = q;
+            }
+        }
+        p = __shfl(p, 0, kWave);
+        q = __shfl(q, 0, kWave);
+        for (int k = lane; k < len; k += kWave) {
+            const double in = w[seg0 + k] * sc;
+            v[seg0 + k] = in;
+            out[seg0 + k] = gs * (in + p * kphi[seg0 + k] + q * kth[seg0 + k]);
+        }
+    }
+    const int nb3 = 3 * d.nblob;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb3; i += gridDim.x * blockDim.x) {
+        const double in = w[d.off_blob + i] * sc;
+        v[d.off_blob + i] = in;
+        out[d.off_blob + i] = gb * in;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -429,6 +743,31 @@ hipError_t k_trisolve(const double* H, int ldh, const double* g, int k, double* 
 }
 hipError_t k_axpby(double a, const double* x, double b, double* y, int n, hipStream_t s) {
     hipLaunchKernelGGL(axpby_kernel, dim3(blocks_for(n)), dim3(256), 0, s, a, x, b, y, n);
+    return hipGetLastError();
+}
+hipError_t k_arnoldi_step(const double* V, int64_t ld, int j, double* w, int n, double* pa, double* pb, int G,
+                          double* H, int ldh, double* cs, double* sn, double* g, double* scal, double* est,
+                          hipStream_t s) {
+    if (j + 1 > kMaxBasis) return hipErrorInvalidValue;
+    const ArnoldiArgs a{V, ld, j + 1, w, n};
+    double* Hcol = H + (int64_t)j * ldh;
+    hipLaunchKernelGGL(arnoldi_dots_kernel, dim3(G), dim3(256), 0, s, a, pa);
+    hipLaunchKernelGGL(arnoldi_update_dots_kernel, dim3(G), dim3(256), 0, s, a, (const double*)pa, pb, Hcol);
+    hipLaunchKernelGGL(arnoldi_update_sumsq_kernel, dim3(G), dim3(256), 0, s, a, (const double*)pb, pa, Hcol);
+    hipLaunchKernelGGL(arnoldi_close_kernel, dim3(1), dim3(64), 0, s, (const double*)pa, G, H, ldh, j, cs, sn, g,
+                       scal, est);
+    return hipGetLastError();
+}
+int arnoldi_blocks(int64_t n) {
+    const int64_t g = (n + kSlice - 1) / kSlice;
+    return (int)(g < 1 ? 1 : (g > 1024 ? 1024 : g));
+}
+hipError_t k_scale_precond(const double* w, const double* dev_scale, double host_scale, double* v, double* out,
+                           const double* kphi, const double* kth, const SaddleDims& d, double gs, double gb,
+                           hipStream_t s) {
+    const int64_t work = (int64_t)d.nfil * kWave > 3LL * d.nblob ? (int64_t)d.nfil * kWave : 3LL * d.nblob;
+    hipLaunchKernelGGL(scale_precond_kernel, dim3(blocks_for(work > 0 ? work : 1)), dim3(256), 0, s, w, dev_scale,
+                       host_scale, v, out, kphi, kth, d, gs, gb);
     return hipGetLastError();
 }
 hipError_t k_sub(const double* x, const double* y, double* out, int n, hipStream_t s) {

@@ -25,6 +25,9 @@ using namespace tfs;
 namespace {
 
 constexpr int kMaxRestart = 350;
+// GMRES reads its residual estimates back every kEstBatch iterations; up to
+// kEstBatch - 1 iterations past convergence are computed and discarded.
+constexpr int kEstBatch = 4;
 
 __global__ void fill_kernel(double* p, double v, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -55,8 +58,10 @@ struct tfs_solver {
     double* V = nullptr;          // n x (m+1), column-major, ld = ldv
     int64_t ldv = 0;
     double *H = nullptr, *cs = nullptr, *sn = nullptr, *g = nullptr, *y = nullptr, *h = nullptr;
-    double *partial = nullptr, *scal = nullptr, *nrm = nullptr;
-    double* host_scal = nullptr;  // pinned
+    double *partial = nullptr, *scal = nullptr, *nrm = nullptr, *est = nullptr;
+    double *pa = nullptr, *pb = nullptr;  // per-workgroup partial sums of the Arnoldi step
+    int GA = 1;                           // workgroups of the Arnoldi step
+    double* host_scal = nullptr;  // pinned, kEstBatch + 8 doubles
     double gs = 0, gb = 0;        // 6 pi mu a for segments / blobs
 };
 
@@ -138,28 +143,42 @@ int gmres(tfs_solver* s, int precision, const double* b, double* x, double* r, d
         if (rc) return rc;
         int k = 0;
         double est = rnorm;
-        for (int j = 0; j < m; ++j) {
+        bool done = false;
+        int checked = 0;  // estimates est[0 .. checked) have been examined
+        for (int j = 0; j < m && !done; ++j) {
             double* vj = s->V + (int64_t)j * s->ldv;
-            double* vn = s->V + (int64_t)(j + 1) * s->ldv;
-            double* Hj = s->H + (int64_t)j * ldh;
-            rc = apply_P(s, vj, s->z);
+            // v_j = w_{j-1} / ||w_{j-1}|| (v_0 = r / ||r||), z = P^{-1} v_j
+            if (j == 0)
+                rc = check(k_scale_precond(r, nullptr, 1.0 / rnorm, vj, s->z, s->kphi, s->kth, s->dims, s->gs, s->gb,
+                                           s->stream), "v0");
+            else
+                rc = check(k_scale_precond(s->w, s->scal, 0.0, vj, s->z, s->kphi, s->kth, s->dims, s->gs, s->gb,
+                                           s->stream), "normalise");
             if (!rc) rc = apply_A(s, s->z, s->w, precision, st);
-            // CGS2: two passes of classical Gram-Schmidt against v_0..v_j.
-            for (int pass = 0; pass < 2 && !rc; ++pass) {
-                rc = check(k_dot_cols(s->V, s->ldv, j + 1, s->w, n, s->partial, s->G, s->stream), "dots");
-                if (!rc) rc = check(k_reduce_cols(s->partial, s->G, j + 1, s->h, Hj, s->stream), "reduce");
-                if (!rc) rc = check(k_gemv(s->V, s->ldv, j + 1, s->h, s->w, n, true, s->stream), "orth");
-            }
-            if (!rc) rc = check(k_norm(s->w, n, s->partial, s->G, s->nrm, s->stream), "wnorm");
-            if (!rc) rc = check(k_givens(s->H, ldh, j, s->cs, s->sn, s->g, s->nrm, s->scal, s->stream), "givens");
-            if (!rc) rc = check(k_scale(vn, s->w, s->scal, 1, n, s->stream), "normalise");
-            if (!rc) rc = read_scalar(s, s->scal, &est);
+            if (!rc)
+                rc = check(k_arnoldi_step(s->V, s->ldv, j, s->w, n, s->pa, s->pb, s->GA, s->H, ldh, s->cs, s->sn,
+                                          s->g, s->scal, s->est, s->stream),
+                           "arnoldi");
             if (rc) return rc;
-            st->iterations++;
-            k = j + 1;
-            if (!std::isfinite(est)) return fail(TFS_ERR_NON_FINITE, "GMRES: non-finite residual estimate");
-            if (est <= abs_tol) break;
+            if ((j + 1 - checked) < kEstBatch && j + 1 < m) continue;
+            // examine est[checked .. j]
+            const int cnt = j + 1 - checked;
+            hipError_t e = hipMemcpyAsync(s->host_scal, s->est + checked, sizeof(double) * cnt,
+                                          hipMemcpyDeviceToHost, s->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+            if (e != hipSuccess) return hip_fail(e, "GMRES estimates");
+            for (int q = 0; q < cnt; ++q) {
+                est = s->host_scal[q];
+                k = checked + q + 1;
+                if (!std::isfinite(est)) return fail(TFS_ERR_NON_FINITE, "GMRES: non-finite residual estimate");
+                if (est <= abs_tol) {
+                    done = true;
+                    break;
+                }
+            }
+            checked = j + 1;
         }
+        st->iterations += k;
         // x += P^{-1} V(:, 0:k) y
         rc = check(k_trisolve(s->H, ldh, s->g, k, s->y, s->stream), "trisolve");
         if (!rc) rc = check(k_gemv(s->V, s->ldv, k, s->y, s->t, n, false, s->stream), "Vy");
@@ -177,13 +196,15 @@ int alloc_solver(tfs_solver* s) {
     s->ldv = ((n + 31) / 32) * 32;
     const int m = s->m, ldh = m + 1;
     s->G = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256));
+    s->GA = arnoldi_blocks(n);
     const int64_t ns3 = 3LL * s->desc.num_fil * s->desc.num_seg;
     std::vector<std::pair<double**, int64_t>> parts = {
         {&s->kphi, std::max<int64_t>(ns3, 1)}, {&s->kth, std::max<int64_t>(ns3, 1)},
         {&s->b, s->ldv}, {&s->x, s->ldv}, {&s->r, s->ldv}, {&s->w, s->ldv}, {&s->z, s->ldv},
         {&s->t, s->ldv}, {&s->d, s->ldv}, {&s->rin, s->ldv}, {&s->V, s->ldv * (m + 1)}, {&s->H, (int64_t)ldh * m},
         {&s->cs, m}, {&s->sn, m}, {&s->g, ldh}, {&s->y, ldh}, {&s->h, ldh},
-        {&s->partial, (int64_t)s->G * ldh}, {&s->scal, 8}, {&s->nrm, 8}};
+        {&s->partial, (int64_t)s->G * ldh}, {&s->scal, 8}, {&s->nrm, 8}, {&s->est, ldh},
+        {&s->pa, (int64_t)s->GA * ldh}, {&s->pb, (int64_t)s->GA * ldh}};
     int64_t total = 0;
     for (auto& p : parts) total += ((p.second + 31) / 32) * 32;
     hipError_t e = hipMalloc(&s->arena, total * sizeof(double));
@@ -196,7 +217,7 @@ int alloc_solver(tfs_solver* s) {
     e = hipMemsetAsync(s->arena, 0, total * sizeof(double), s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return hip_fail(e, "solver: clear");
-    e = hipHostMalloc((void**)&s->host_scal, 64, hipHostMallocDefault);
+    e = hipHostMalloc((void**)&s->host_scal, sizeof(double) * (kEstBatch + 8), hipHostMallocDefault);
     if (e != hipSuccess) return hip_fail(e, "solver: pinned scalar");
     return TFS_OK;
 }
