@@ -63,7 +63,8 @@ __device__ __forceinline__ T separation(const double* xi, const T* xh, const T* 
         dy = xi[1] - rec[1];
         dz = xi[2] - rec[2];
     }
-    return __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+    if constexpr (sizeof(T) == 4) return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    else return __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
 }
 
 template <typename T>
@@ -77,8 +78,11 @@ __device__ __forceinline__ void split_position(double x, T& h, T& l) {
     }
 }
 
+// One source into its LDS record: position (double-float split for FP32) and,
+// if with_force, its strengths multiplied by `scale`.
 template <typename T, int FS>
-__device__ __forceinline__ void stage_source(T* rec, const CView& xsrc, const CView& fsrc, int s, bool with_force) {
+__device__ __forceinline__ void stage_source(T* rec, const CView& xsrc, const CView& fsrc, int s, bool with_force,
+                                             double scale = 1.0) {
     const double p0 = xsrc.p[(int64_t)s * xsrc.ld_p];
     const double p1 = xsrc.p[(int64_t)s * xsrc.ld_p + xsrc.ld_c];
     const double p2 = xsrc.p[(int64_t)s * xsrc.ld_p + 2 * xsrc.ld_c];
@@ -94,7 +98,7 @@ __device__ __forceinline__ void stage_source(T* rec, const CView& xsrc, const CV
     }
     if (with_force) {
 #pragma unroll
-        for (int k = 0; k < FS; ++k) rec[F0 + k] = T(fsrc.p[(int64_t)s * fsrc.ld_p + k * fsrc.ld_c]);
+        for (int k = 0; k < FS; ++k) rec[F0 + k] = T(scale * fsrc.p[(int64_t)s * fsrc.ld_p + k * fsrc.ld_c]);
     }
 }
 
@@ -114,7 +118,8 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
     const double PI = 3.14159265358979323846;
     const T sum_a2 = T((ai + aj) * (ai + aj));
     const T s3 = T((ai * ai + aj * aj) / 3.0);
-    const T c8 = T(1.0 / (8.0 * PI * a.mu));
+    const T m3s3 = T(-(ai * ai + aj * aj));
+    const double c8 = 1.0 / (8.0 * PI * a.mu);  // folded into the staged strengths
 
     // Targets of this lane (clamped into the tile; surplus lanes compute a
     // duplicate and their slot entries are never read).
@@ -144,7 +149,7 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
     for (int s0 = it.s_begin; s0 < it.s_end; s0 += BLOCK) {
         const int n = (it.s_end - s0) < BLOCK ? (it.s_end - s0) : BLOCK;
         __syncthreads();  // the previous batch has been consumed
-        if (tid < n) stage_source<T, FS>(lds + tid * REC, xsrc, fsrc, s0 + tid, true);
+        if (tid < n) stage_source<T, FS>(lds + tid * REC, xsrc, fsrc, s0 + tid, true, c8);
         __syncthreads();
         if constexpr (F32) {
 #pragma unroll
@@ -166,8 +171,8 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
                 // Overlapping pairs (and the self pair) are added by the reduce
                 // kernel; a zero 1/r removes them here without a branch.
                 const T rm1 = r2 > sum_a2 ? rsqrt_refined(r2) : T(0);
-                if constexpr (F32) pair_far<T, VT, FS>(part[r], dx, dy, dz, rm1, f, s3, c8);
-                else pair_far<T, VT, FS>(acc[r], dx, dy, dz, rm1, f, s3, c8);
+                if constexpr (F32) pair_far_scaled<T, VT, FS>(part[r], dx, dy, dz, rm1, f, s3, m3s3);
+                else pair_far_scaled<T, VT, FS>(acc[r], dx, dy, dz, rm1, f, s3, m3s3);
             }
         }
         if constexpr (F32) {

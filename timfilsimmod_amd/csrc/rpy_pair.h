@@ -83,6 +83,46 @@ __device__ __forceinline__ void pair_far(T* __restrict__ v, T dx, T dy, T dz, T 
     }
 }
 
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Far field with the source strengths already multiplied by 1/(8 pi mu), in
+// the fewest VALU operations: A = 1/r + s3/r^3, B = (1 - 3 s3/r^2)/r^3,
+// C = 1/r^3, and every term is fused into the accumulators.  The same tensor
+// as pair_far.
+template <typename T, int VT, int FS>
+__device__ __forceinline__ void pair_far_scaled(T* __restrict__ v, T dx, T dy, T dz, T rm1,
+                                                const T* __restrict__ f, T s3, T m3s3) {
+    const T rm2 = rm1 * rm1;
+    const T rm3 = rm1 * rm2;
+    const T A = fma_t(s3, rm3, rm1);
+    const T B = rm3 * fma_t(m3s3, rm2, T(1));
+    const T Bd = B * fma_t(dz, f[2], fma_t(dy, f[1], dx * f[0]));
+    v[0] = fma_t(Bd, dx, fma_t(A, f[0], v[0]));
+    v[1] = fma_t(Bd, dy, fma_t(A, f[1], v[1]));
+    v[2] = fma_t(Bd, dz, fma_t(A, f[2], v[2]));
+    if constexpr (FS == 6) {
+        // translation-rotation: v += C (tau x d)
+        v[0] = fma_t(rm3, f[4] * dz - f[5] * dy, v[0]);
+        v[1] = fma_t(rm3, f[5] * dx - f[3] * dz, v[1]);
+        v[2] = fma_t(rm3, f[3] * dy - f[4] * dx, v[2]);
+    }
+    if constexpr (VT == 6) {
+        // rotation-translation: w += C (F x d)
+        v[3] = fma_t(rm3, f[1] * dz - f[2] * dy, v[3]);
+        v[4] = fma_t(rm3, f[2] * dx - f[0] * dz, v[4]);
+        v[5] = fma_t(rm3, f[0] * dy - f[1] * dx, v[5]);
+        if constexpr (FS == 6) {
+            // rotation-rotation: w += -C/2 tau + (3C/2) (tau.d) d / r^2
+            const T Ar = T(-0.5) * rm3;
+            const T Br = T(1.5) * rm3 * rm2 * fma_t(dz, f[5], fma_t(dy, f[4], dx * f[3]));
+            v[3] = fma_t(Br, dx, fma_t(Ar, f[3], v[3]));
+            v[4] = fma_t(Br, dy, fma_t(Ar, f[4], v[4]));
+            v[5] = fma_t(Br, dz, fma_t(Ar, f[5], v[5]));
+        }
+    }
+}
+
 // Overlap and inner branches: a term-by-term restatement of
 // cuda_functions.cu:207-354, in double.  Accumulates into v.
 template <int VT, int FS>
