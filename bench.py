@@ -80,7 +80,7 @@ def timestep_leg(torch, body, mode: int, steps: int, warmup: int, stream):
     from timfilsimmod_amd import _lib as L
     from timfilsimmod_amd.mobility import profile_read
     from timfilsimmod_amd.solver import BeatSimulation
-    sim = BeatSimulation(body, mode=mode, tol=1e-8, restart=50, inner_tol=1e-3, max_cycles=40,
+    sim = BeatSimulation(body, mode=mode, tol=1e-8, restart=50, inner_tol=1e-4, max_cycles=40,
                          max_refinements=20, stream=stream)
     rpy = sim.rpy_handle()
     for _ in range(warmup):

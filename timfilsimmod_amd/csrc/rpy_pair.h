@@ -92,10 +92,14 @@ __device__ __forceinline__ double fma_t(double a, double b, double c) { return _
 template <typename T, int VT, int FS>
 __device__ __forceinline__ void pair_far_scaled(T* __restrict__ v, T dx, T dy, T dz, T rm1,
                                                 const T* __restrict__ f, T s3, T m3s3) {
+    // Written so that every operation is a two-source multiply or an
+    // accumulate into a dying value (VOP2 forms, the double-rate FP32 issue on
+    // gfx950): A accumulates into 1/r, B into 1/r^3.
     const T rm2 = rm1 * rm1;
     const T rm3 = rm1 * rm2;
+    const T rm5 = rm3 * rm2;
     const T A = fma_t(s3, rm3, rm1);
-    const T B = rm3 * fma_t(m3s3, rm2, T(1));
+    const T B = fma_t(m3s3, rm5, rm3);
     const T Bd = B * fma_t(dz, f[2], fma_t(dy, f[1], dx * f[0]));
     v[0] = fma_t(Bd, dx, fma_t(A, f[0], v[0]));
     v[1] = fma_t(Bd, dy, fma_t(A, f[1], v[1]));

@@ -34,14 +34,15 @@ int hip_fail(hipError_t e, const char* where) {
 TileShape default_shape(int precision, int ns, int nb, int num_cus) {
     TileShape s;
     s.block = 256;
-    s.per_thread = 1;
+    s.per_thread = 2;
     s.persistent = 0;
     (void)precision;
     const int tt = s.block * s.per_thread;
     const long n_tiles = (ns + tt - 1) / tt + (nb + tt - 1) / tt;
     const long n_src = (long)ns + nb;
-    // Aim for ~8 workgroups per CU over the whole work list.
-    const long want_items = 8L * (num_cus > 0 ? num_cus : 256);
+    // Aim for ~16 workgroups per CU over the whole work list (measured best at
+    // the production size for both precisions: short items even out the tail).
+    const long want_items = 16L * (num_cus > 0 ? num_cus : 256);
     long chunk = n_tiles > 0 ? (n_src * n_tiles + want_items - 1) / want_items : n_src;
     chunk = ((chunk + s.block - 1) / s.block) * s.block;
     if (chunk < s.block) chunk = s.block;
