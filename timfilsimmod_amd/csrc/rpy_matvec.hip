@@ -104,7 +104,7 @@ __device__ __forceinline__ void stage_source(T* rec, const CView& xsrc, const CV
 
 template <int VT, int FS, typename T, int BLOCK, int R>
 __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile tl, const MatvecArgs& a,
-                                         double* __restrict__ slot, T* __restrict__ lds) {
+                                         T* __restrict__ slot, T* __restrict__ lds) {
     constexpr int REC = rec_len<T, FS>();
     constexpr int TILE = BLOCK * R;
     constexpr int F0 = sizeof(T) == 4 ? 6 : 3;
@@ -183,21 +183,25 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
         }
     }
 
+    // FP32 tiles store their item sums in FP32: with one LDS batch per item the
+    // sum is an FP32 batch sum already, so nothing is lost, and the reduce
+    // kernel reads half the bytes.
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
-        for (int k = 0; k < VT; ++k) slot[k * TILE + tid + r * BLOCK] = acc[r][k];
+        for (int k = 0; k < VT; ++k) slot[k * TILE + tid + r * BLOCK] = T(acc[r][k]);
 }
 
 template <typename T, int SEGDOF, int BLOCK, int R>
 __global__ __launch_bounds__(BLOCK) void rpy_pairs_kernel(const MatvecItem* __restrict__ items,
                                                           const MatvecTile* __restrict__ tiles,
-                                                          MatvecArgs a, double* __restrict__ partial) {
+                                                          MatvecArgs a, T* __restrict__ partial) {
     constexpr int REC = rec_len<T, 6>() > rec_len<T, 3>() ? rec_len<T, 6>() : rec_len<T, 3>();
     __shared__ __align__(16) T lds[BLOCK * REC];
+    if (a.skip && *a.skip) return;
     const MatvecItem it = items[blockIdx.x];
     const MatvecTile tl = tiles[it.tile];
-    double* slot = partial + (size_t)blockIdx.x * (kMaxDof * BLOCK * R);
+    T* slot = partial + (size_t)blockIdx.x * (kMaxDof * BLOCK * R);
     if (tl.cls == 0) {
         if (it.src_cls == 0) run_item<SEGDOF, SEGDOF, T, BLOCK, R>(it, tl, a, slot, lds);
         else run_item<SEGDOF, 3, T, BLOCK, R>(it, tl, a, slot, lds);
@@ -215,15 +219,16 @@ __global__ __launch_bounds__(BLOCK) void rpy_pairs_kernel(const MatvecItem* __re
 template <typename T, int SEGDOF, int BLOCK, int R>
 __global__ __launch_bounds__(BLOCK) void rpy_pairs_persistent_kernel(const MatvecItem* __restrict__ items,
                                                                      const MatvecTile* __restrict__ tiles,
-                                                                     MatvecArgs a, double* __restrict__ partial,
+                                                                     MatvecArgs a, T* __restrict__ partial,
                                                                      int n_items, PersistState ps) {
     constexpr int REC = rec_len<T, 6>() > rec_len<T, 3>() ? rec_len<T, 6>() : rec_len<T, 3>();
     __shared__ __align__(16) T lds[BLOCK * REC];
+    if (a.skip && *a.skip) return;
     for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
         if (ps.item_stamps) ps.item_stamps[2 * item] = (int64_t)wall_clock64();
         const MatvecItem it = items[item];
         const MatvecTile tl = tiles[it.tile];
-        double* slot = partial + (size_t)item * (kMaxDof * BLOCK * R);
+        T* slot = partial + (size_t)item * (kMaxDof * BLOCK * R);
         if (tl.cls == 0) {
             if (it.src_cls == 0) run_item<SEGDOF, SEGDOF, T, BLOCK, R>(it, tl, a, slot, lds);
             else run_item<SEGDOF, 3, T, BLOCK, R>(it, tl, a, slot, lds);
@@ -313,11 +318,12 @@ __device__ void add_near(double* acc, int src, const double* xi, double ai, cons
 // 64 threads: group g sums the items g, g+4, g+8, ... (loads of four items
 // issued together), and the four group sums are added in group order, so the
 // result is the same on every run.
-template <int SEGDOF>
+template <int SEGDOF, typename P>
 __global__ __launch_bounds__(256) void rpy_reduce_kernel(const MatvecTile* __restrict__ tiles,
-                                                         const double* __restrict__ partial, int tile_targets,
+                                                         const P* __restrict__ partial, int tile_targets,
                                                          OutArgs o, MatvecArgs a, NearList near) {
     __shared__ double grp_acc[3][6][64];
+    if (a.skip && *a.skip) return;
     const int per_tile = (tile_targets + 63) / 64;
     const MatvecTile tl = tiles[blockIdx.x / per_tile];
     const int l = (blockIdx.x % per_tile) * 64 + (threadIdx.x & 63);
@@ -328,7 +334,7 @@ __global__ __launch_bounds__(256) void rpy_reduce_kernel(const MatvecTile* __res
     const int vt = seg_t ? SEGDOF : 3;
     const size_t slot_len = (size_t)kMaxDof * tile_targets;
     const int lc = l < tile_targets ? l : tile_targets - 1;
-    const double* p = partial + (size_t)tl.first_item * slot_len + lc;
+    const P* p = partial + (size_t)tl.first_item * slot_len + lc;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     for (int m0 = grp; m0 < tl.n_items; m0 += 16) {
         double v[4][6];
@@ -338,7 +344,7 @@ __global__ __launch_bounds__(256) void rpy_reduce_kernel(const MatvecTile* __res
             const int mm = m < tl.n_items ? m : grp;
 #pragma unroll
             for (int k = 0; k < 6; ++k)
-                v[u][k] = k < vt ? p[(size_t)mm * slot_len + (size_t)k * tile_targets] : 0.0;
+                v[u][k] = k < vt ? double(p[(size_t)mm * slot_len + (size_t)k * tile_targets]) : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -408,10 +414,10 @@ hipError_t launch_pairs(const TileShape& sh, const MatvecArgs& a, const MatvecIt
         int grid = sh.persistent * ps.num_cus;
         if (grid > n_items) grid = n_items;
         hipLaunchKernelGGL((rpy_pairs_persistent_kernel<T, SEGDOF, BLOCK, R>), dim3(grid), dim3(BLOCK), 0, stream,
-                           d_items, d_tiles, a, partial, n_items, ps);
+                           d_items, d_tiles, a, reinterpret_cast<T*>(partial), n_items, ps);
     } else {
         hipLaunchKernelGGL((rpy_pairs_kernel<T, SEGDOF, BLOCK, R>), dim3(n_items), dim3(BLOCK), 0, stream,
-                           d_items, d_tiles, a, partial);
+                           d_items, d_tiles, a, reinterpret_cast<T*>(partial));
     }
     return hipGetLastError();
 }
@@ -508,10 +514,23 @@ hipError_t launch_matvec(int precision, int seg_dof, const TileShape& shape, con
     }
     const int tt = shape.block * shape.per_thread;
     const unsigned grid = (unsigned)n_tiles * (unsigned)((tt + 63) / 64);
-    if (seg_dof == 6)
-        hipLaunchKernelGGL(rpy_reduce_kernel<6>, dim3(grid), dim3(256), 0, stream, d_tiles, partial, tt, o, a, near);
-    else
-        hipLaunchKernelGGL(rpy_reduce_kernel<3>, dim3(grid), dim3(256), 0, stream, d_tiles, partial, tt, o, a, near);
+    // The partial slots hold the evaluation type (FP32 sums for FP32 tiles).
+    const float* pf = reinterpret_cast<const float*>(partial);
+    if (precision == 0) {
+        if (seg_dof == 6)
+            hipLaunchKernelGGL((rpy_reduce_kernel<6, double>), dim3(grid), dim3(256), 0, stream, d_tiles,
+                               (const double*)partial, tt, o, a, near);
+        else
+            hipLaunchKernelGGL((rpy_reduce_kernel<3, double>), dim3(grid), dim3(256), 0, stream, d_tiles,
+                               (const double*)partial, tt, o, a, near);
+    } else {
+        if (seg_dof == 6)
+            hipLaunchKernelGGL((rpy_reduce_kernel<6, float>), dim3(grid), dim3(256), 0, stream, d_tiles, pf, tt, o,
+                               a, near);
+        else
+            hipLaunchKernelGGL((rpy_reduce_kernel<3, float>), dim3(grid), dim3(256), 0, stream, d_tiles, pf, tt, o,
+                               a, near);
+    }
     return hipGetLastError();
 }
 

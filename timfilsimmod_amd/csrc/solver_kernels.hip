@@ -198,6 +198,7 @@ struct ArnoldiArgs {
     int ncols;           // j + 1
     double* w;
     int n;
+    const int32_t* skip; // stop flag: nonzero -> the launch does nothing
 };
 
 // partial[c*G + b] = sum over the rows of workgroup b of V[r, c] * w[r].
@@ -329,6 +330,7 @@ __device__ __forceinline__ void slice_update(const ArnoldiArgs& a, const double*
 }
 
 __global__ __launch_bounds__(256) void arnoldi_dots_kernel(ArnoldiArgs a, double* __restrict__ p_out) {
+    if (*a.skip) return;
     slice_dots(a, p_out, blockIdx.x, gridDim.x);
 }
 
@@ -336,6 +338,7 @@ __global__ __launch_bounds__(256) void arnoldi_update_dots_kernel(ArnoldiArgs a,
                                                                   double* __restrict__ p_out,
                                                                   double* __restrict__ Hcol) {
     __shared__ double hs[kMaxBasis];
+    if (*a.skip) return;
     gather_coeffs(p_in, a.ncols, gridDim.x, hs, Hcol, true);
     slice_update(a, hs, blockIdx.x, gridDim.x);
     __syncthreads();  // the slice's rows of w are final before the dots read them
@@ -347,6 +350,7 @@ __global__ __launch_bounds__(256) void arnoldi_update_sumsq_kernel(ArnoldiArgs a
                                                                    double* __restrict__ Hcol) {
     __shared__ double hs[kMaxBasis];
     __shared__ double sh[4];
+    if (*a.skip) return;
     gather_coeffs(p_in, a.ncols, gridDim.x, hs, Hcol, false);
     slice_update(a, hs, blockIdx.x, gridDim.x);
     __syncthreads();
@@ -365,12 +369,16 @@ __global__ __launch_bounds__(256) void arnoldi_update_sumsq_kernel(ArnoldiArgs a
 // ||w|| from the G partial sums, then the Givens bookkeeping of givens_kernel;
 // est[j] = |g(j+1)|.  One wave: the column of H and the previous rotations
 // are staged in LDS so that lane 0's sequential rotations read no memory.
+// The first step whose estimate meets abs_tol sets *stop = j + 1 (a
+// non-finite estimate sets -(j + 1)); later launches of the solve then skip.
 __global__ __launch_bounds__(64) void arnoldi_close_kernel(const double* __restrict__ p_in, int G,
                                                            double* __restrict__ H, int ldh, int j,
                                                            double* __restrict__ cs, double* __restrict__ sn,
                                                            double* __restrict__ g, double* __restrict__ scal,
-                                                           double* __restrict__ est) {
+                                                           double* __restrict__ est, int32_t* __restrict__ stop,
+                                                           double abs_tol) {
     __shared__ double col[kMaxBasis + 1], lc[kMaxBasis], ls[kMaxBasis];
+    if (*stop) return;
     double* Hc = H + (int64_t)j * ldh;
     for (int i = threadIdx.x; i <= j; i += kWave) {
         col[i] = Hc[i];
@@ -411,6 +419,8 @@ __global__ __launch_bounds__(64) void arnoldi_close_kernel(const double* __restr
         g[j] = c * gj;
         scal[0] = __builtin_fabs(g[j + 1]);
         est[j] = scal[0];
+        if (!__builtin_isfinite(scal[0])) *stop = -(j + 1);
+        else if (scal[0] <= abs_tol) *stop = j + 1;
     }
     __syncthreads();
     for (int i = threadIdx.x; i <= j + 1; i += kWave) Hc[i] = col[i];
@@ -425,7 +435,8 @@ __global__ __launch_bounds__(256) void scale_precond_kernel(const double* __rest
                                                            double* __restrict__ out,
                                                            const double* __restrict__ kphi,
                                                            const double* __restrict__ kth, SaddleDims d,
-                                                           double gs, double gb) {
+                                                           double gs, double gb, const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
     const double sc = dev_scale ? dev_scale[1] : host_scale;
     const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int lane = threadIdx.x & (kWave - 1);
@@ -495,7 +506,9 @@ __global__ __launch_bounds__(256) void scale_precond_kernel(const double* __rest
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void saddle_k_kernel(const double* __restrict__ in, double* __restrict__ out,
                                                       const double* __restrict__ kphi,
-                                                      const double* __restrict__ kth, SaddleDims d) {
+                                                      const double* __restrict__ kth, SaddleDims d,
+                                                      const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
     const int m = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     if (m >= d.nfil) return;
@@ -747,15 +760,15 @@ hipError_t k_axpby(double a, const double* x, double b, double* y, int n, hipStr
 }
 hipError_t k_arnoldi_step(const double* V, int64_t ld, int j, double* w, int n, double* pa, double* pb, int G,
                           double* H, int ldh, double* cs, double* sn, double* g, double* scal, double* est,
-                          hipStream_t s) {
+                          int32_t* stop, double abs_tol, hipStream_t s) {
     if (j + 1 > kMaxBasis) return hipErrorInvalidValue;
-    const ArnoldiArgs a{V, ld, j + 1, w, n};
+    const ArnoldiArgs a{V, ld, j + 1, w, n, stop};
     double* Hcol = H + (int64_t)j * ldh;
     hipLaunchKernelGGL(arnoldi_dots_kernel, dim3(G), dim3(256), 0, s, a, pa);
     hipLaunchKernelGGL(arnoldi_update_dots_kernel, dim3(G), dim3(256), 0, s, a, (const double*)pa, pb, Hcol);
     hipLaunchKernelGGL(arnoldi_update_sumsq_kernel, dim3(G), dim3(256), 0, s, a, (const double*)pb, pa, Hcol);
     hipLaunchKernelGGL(arnoldi_close_kernel, dim3(1), dim3(64), 0, s, (const double*)pa, G, H, ldh, j, cs, sn, g,
-                       scal, est);
+                       scal, est, stop, abs_tol);
     return hipGetLastError();
 }
 int arnoldi_blocks(int64_t n) {
@@ -764,10 +777,10 @@ int arnoldi_blocks(int64_t n) {
 }
 hipError_t k_scale_precond(const double* w, const double* dev_scale, double host_scale, double* v, double* out,
                            const double* kphi, const double* kth, const SaddleDims& d, double gs, double gb,
-                           hipStream_t s) {
+                           const int32_t* skip, hipStream_t s) {
     const int64_t work = (int64_t)d.nfil * kWave > 3LL * d.nblob ? (int64_t)d.nfil * kWave : 3LL * d.nblob;
     hipLaunchKernelGGL(scale_precond_kernel, dim3(blocks_for(work > 0 ? work : 1)), dim3(256), 0, s, w, dev_scale,
-                       host_scale, v, out, kphi, kth, d, gs, gb);
+                       host_scale, v, out, kphi, kth, d, gs, gb, skip);
     return hipGetLastError();
 }
 hipError_t k_sub(const double* x, const double* y, double* out, int n, hipStream_t s) {
@@ -775,10 +788,10 @@ hipError_t k_sub(const double* x, const double* y, double* out, int n, hipStream
     return hipGetLastError();
 }
 hipError_t k_saddle(const double* in, double* out, const double* kphi, const double* kth, const SaddleDims& d,
-                    hipStream_t s) {
+                    hipStream_t s, const int32_t* skip) {
     if (d.nfil == 0 || !(d.dyn_phase || d.dyn_angle)) return hipSuccess;
     hipLaunchKernelGGL(saddle_k_kernel, dim3(blocks_for((int64_t)d.nfil * kWave)), dim3(256), 0, s, in, out,
-                       kphi, kth, d);
+                       kphi, kth, d, skip);
     return hipGetLastError();
 }
 hipError_t k_precond(const double* in, double* out, const double* kphi, const double* kth, const SaddleDims& d,

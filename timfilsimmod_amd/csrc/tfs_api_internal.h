@@ -52,6 +52,9 @@ struct tfs_rpy {
     size_t item_stamps_len = 0;
     const tfs::Plan* stamped_plan = nullptr;
     bool body_timing = false;
+    // Stop flag passed to the kernels of later evaluations (set by a solver
+    // around its speculative Krylov iterations; null otherwise).
+    const int32_t* skip_flag = nullptr;
     // Live kernel timing: event pairs recorded around each pair-kernel launch.
     bool profiling = false;
     struct EvPair {

@@ -84,6 +84,9 @@ struct MatvecArgs {
     CView fs, fb;      // source strengths (seg_dof or 3 components)
     int ns, nb;
     double a_seg, a_blob, mu;
+    // Device stop flag (may be null): when nonzero the pair and reduce kernels
+    // return at once (iterations a Krylov solver enqueued past convergence).
+    const int32_t* skip = nullptr;
 };
 
 struct OutArgs {

@@ -128,6 +128,7 @@ MatvecArgs matvec_args(tfs_rpy* h, CView fs, CView fb) {
     a.a_seg = h->desc.seg_radius;
     a.a_blob = h->desc.blob_radius;
     a.mu = h->desc.mu;
+    a.skip = h->skip_flag;
     return a;
 }
 

@@ -25,13 +25,24 @@ using namespace tfs;
 namespace {
 
 constexpr int kMaxRestart = 350;
-// GMRES reads its residual estimates back every kEstBatch iterations; up to
-// kEstBatch - 1 iterations past convergence are computed and discarded.
+// GMRES copies its device stop flag back every kEstBatch iterations and the
+// host looks one batch behind; up to 2*kEstBatch - 1 iterations enqueued past
+// convergence run as empty launches.
 constexpr int kEstBatch = 4;
 
 __global__ void fill_kernel(double* p, double v, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+__global__ void fill_i32_kernel(int32_t* p, int32_t v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t k_fill_int(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
 }
 
 hipError_t k_fill(double* p, double v, int64_t n, hipStream_t s) {
@@ -61,6 +72,9 @@ struct tfs_solver {
     double *partial = nullptr, *scal = nullptr, *nrm = nullptr, *est = nullptr;
     double *pa = nullptr, *pb = nullptr;  // per-workgroup partial sums of the Arnoldi step
     int GA = 1;                           // workgroups of the Arnoldi step
+    int32_t* stop = nullptr;              // device stop flag of the current GMRES cycle
+    int32_t* host_flag = nullptr;         // pinned, 3 slots
+    hipEvent_t flag_ev[2] = {nullptr, nullptr};
     double* host_scal = nullptr;  // pinned, kEstBatch + 8 doubles
     double gs = 0, gb = 0;        // 6 pi mu a for segments / blobs
 };
@@ -84,17 +98,20 @@ int norm(tfs_solver* s, const double* v, double* out) {
     return read_scalar(s, s->nrm, out);
 }
 
-// out = A in.
-int apply_A(tfs_solver* s, const double* in, double* out, int precision, tfs_solve_stats* st) {
+// out = A in.  With `skip`, every launch does nothing while *skip != 0.
+int apply_A(tfs_solver* s, const double* in, double* out, int precision, tfs_solve_stats* st,
+            const int32_t* skip = nullptr) {
     const SaddleDims& dm = s->dims;
+    s->rpy->skip_flag = skip;
     int rc = run_matvec(s->rpy, TFS_BLOCK_ALL, precision, CView{in, 3, 1}, CView{in + dm.off_blob, 3, 1},
                         View{out, 3, 1}, View{out + dm.off_blob, 3, 1}, 3);
+    s->rpy->skip_flag = nullptr;
     if (rc) return rc;
     if (st) {
         if (precision == TFS_PRECISION_FP64) st->fp64_matvecs++;
         else st->fp32_matvecs++;
     }
-    return check(k_saddle(in, out, s->kphi, s->kth, dm, s->stream), "saddle couplings");
+    return check(k_saddle(in, out, s->kphi, s->kth, dm, s->stream, skip), "saddle couplings");
 }
 
 int apply_P(tfs_solver* s, const double* in, double* out) {
@@ -141,44 +158,59 @@ int gmres(tfs_solver* s, int precision, const double* b, double* x, double* r, d
         // v0 = r / ||r||
         if (!rc) rc = check(k_axpby(1.0 / rnorm, r, 0.0, s->V, n, s->stream), "v0");
         if (rc) return rc;
-        int k = 0;
-        double est = rnorm;
-        bool done = false;
-        int checked = 0;  // estimates est[0 .. checked) have been examined
-        for (int j = 0; j < m && !done; ++j) {
+        // The iterations are enqueued ahead of the host: every kEstBatch of
+        // them the stop flag is copied back asynchronously, and the host looks
+        // at the copy of the batch before (so one batch is always in flight).
+        // Launches enqueued after the step that met the tolerance see the flag
+        // and do nothing.
+        if (!rc) rc = check(k_fill_int(s->stop, 0, 1, s->stream), "stop flag");
+        if (rc) return rc;
+        int enq = 0, batches = 0;
+        bool seen = false;
+        while (enq < m && !seen) {
+            const int j = enq;
             double* vj = s->V + (int64_t)j * s->ldv;
             // v_j = w_{j-1} / ||w_{j-1}|| (v_0 = r / ||r||), z = P^{-1} v_j
             if (j == 0)
                 rc = check(k_scale_precond(r, nullptr, 1.0 / rnorm, vj, s->z, s->kphi, s->kth, s->dims, s->gs, s->gb,
-                                           s->stream), "v0");
+                                           nullptr, s->stream), "v0");
             else
                 rc = check(k_scale_precond(s->w, s->scal, 0.0, vj, s->z, s->kphi, s->kth, s->dims, s->gs, s->gb,
-                                           s->stream), "normalise");
-            if (!rc) rc = apply_A(s, s->z, s->w, precision, st);
+                                           s->stop, s->stream), "normalise");
+            if (!rc) rc = apply_A(s, s->z, s->w, precision, nullptr, s->stop);
             if (!rc)
                 rc = check(k_arnoldi_step(s->V, s->ldv, j, s->w, n, s->pa, s->pb, s->GA, s->H, ldh, s->cs, s->sn,
-                                          s->g, s->scal, s->est, s->stream),
+                                          s->g, s->scal, s->est, s->stop, abs_tol, s->stream),
                            "arnoldi");
             if (rc) return rc;
-            if ((j + 1 - checked) < kEstBatch && j + 1 < m) continue;
-            // examine est[checked .. j]
-            const int cnt = j + 1 - checked;
-            hipError_t e = hipMemcpyAsync(s->host_scal, s->est + checked, sizeof(double) * cnt,
-                                          hipMemcpyDeviceToHost, s->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-            if (e != hipSuccess) return hip_fail(e, "GMRES estimates");
-            for (int q = 0; q < cnt; ++q) {
-                est = s->host_scal[q];
-                k = checked + q + 1;
-                if (!std::isfinite(est)) return fail(TFS_ERR_NON_FINITE, "GMRES: non-finite residual estimate");
-                if (est <= abs_tol) {
-                    done = true;
-                    break;
-                }
+            ++enq;
+            if (enq % kEstBatch != 0 && enq < m) continue;
+            const int slot = batches & 1;
+            hipError_t e = hipMemcpyAsync(s->host_flag + slot, s->stop, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                          s->stream);
+            if (e == hipSuccess) e = hipEventRecord(s->flag_ev[slot], s->stream);
+            if (e == hipSuccess && batches > 0) {
+                e = hipEventSynchronize(s->flag_ev[slot ^ 1]);
+                if (e == hipSuccess && s->host_flag[slot ^ 1] != 0) seen = true;
             }
-            checked = j + 1;
+            if (e != hipSuccess) return hip_fail(e, "GMRES stop flag");
+            ++batches;
         }
+        int32_t flag = 0;
+        double est = rnorm;
+        {
+            hipError_t e = hipMemcpyAsync(s->host_flag + 2, s->stop, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+            if (e != hipSuccess) return hip_fail(e, "GMRES stop flag");
+            flag = s->host_flag[2];
+        }
+        if (flag < 0) return fail(TFS_ERR_NON_FINITE, "GMRES: non-finite residual estimate");
+        const int k = flag > 0 ? flag : enq;
+        rc = read_scalar(s, s->est + (k - 1), &est);
+        if (rc) return rc;
         st->iterations += k;
+        if (precision == TFS_PRECISION_FP64) st->fp64_matvecs += k;
+        else st->fp32_matvecs += k;
         // x += P^{-1} V(:, 0:k) y
         rc = check(k_trisolve(s->H, ldh, s->g, k, s->y, s->stream), "trisolve");
         if (!rc) rc = check(k_gemv(s->V, s->ldv, k, s->y, s->t, n, false, s->stream), "Vy");
@@ -204,7 +236,7 @@ int alloc_solver(tfs_solver* s) {
         {&s->t, s->ldv}, {&s->d, s->ldv}, {&s->rin, s->ldv}, {&s->V, s->ldv * (m + 1)}, {&s->H, (int64_t)ldh * m},
         {&s->cs, m}, {&s->sn, m}, {&s->g, ldh}, {&s->y, ldh}, {&s->h, ldh},
         {&s->partial, (int64_t)s->G * ldh}, {&s->scal, 8}, {&s->nrm, 8}, {&s->est, ldh},
-        {&s->pa, (int64_t)s->GA * ldh}, {&s->pb, (int64_t)s->GA * ldh}};
+        {&s->pa, (int64_t)s->GA * ldh}, {&s->pb, (int64_t)s->GA * ldh}, {(double**)&s->stop, 1}};
     int64_t total = 0;
     for (auto& p : parts) total += ((p.second + 31) / 32) * 32;
     hipError_t e = hipMalloc(&s->arena, total * sizeof(double));
@@ -218,6 +250,9 @@ int alloc_solver(tfs_solver* s) {
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return hip_fail(e, "solver: clear");
     e = hipHostMalloc((void**)&s->host_scal, sizeof(double) * (kEstBatch + 8), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->host_flag, sizeof(int32_t) * 4, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->flag_ev[0], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->flag_ev[1], hipEventDisableTiming);
     if (e != hipSuccess) return hip_fail(e, "solver: pinned scalar");
     return TFS_OK;
 }
@@ -372,6 +407,9 @@ int tfs_solver_destroy(tfs_solver* s) {
     if (s->stream) hipStreamSynchronize(s->stream);
     if (s->arena) hipFree(s->arena);
     if (s->host_scal) hipHostFree(s->host_scal);
+    if (s->host_flag) hipHostFree(s->host_flag);
+    for (hipEvent_t ev : s->flag_ev)
+        if (ev) hipEventDestroy(ev);
     if (s->rpy) tfs_rpy_destroy(s->rpy);
     delete s;
     return TFS_OK;
