@@ -25,7 +25,35 @@ namespace {
 
 template <typename T>
 constexpr int flow_rec() {
-    return sizeof(T) == 8 ? 10 : 12;  // x y z f0..f5 (+pad) | xh yh zh xl yl zl f0..f5
+    // FP64: x y z f0..f5 (+pad).  FP32: xh yh zh xl yl zl f0..f5 -f3 -f4 -f5 (+pad):
+    // the negated torque lets every rotlet term be an accumulate (VOP2 form).
+    return sizeof(T) == 8 ? 10 : 16;
+}
+
+// Image system of a Stokeslet above the no-slip wall z = 0 (Blake 1971):
+// subtracts the image Stokeslet and adds the Stokeslet doublet and source
+// dipole terms of flow_field_evaluator.cu:83-116.  acc += image velocity.
+__device__ __forceinline__ void blakelet(double* acc, double zi, double zj, double ddx, double ddy, double fx,
+                                         double fy, double fz) {
+    const double Rz = zi + zj;
+    const double R2 = ddx * ddx + ddy * ddy + Rz * Rz;
+    const double Rm1 = __builtin_sqrt(1.0 / R2);
+    const double kk = (ddx * fx + ddy * fy + Rz * fz) * Rm1 * Rm1;
+    acc[0] -= Rm1 * (fx + ddx * kk);
+    acc[1] -= Rm1 * (fy + ddy * kk);
+    acc[2] -= Rm1 * (fz + Rz * kk);
+    const double Dxx = (3.0 * ddx * ddx - R2) * zi;
+    const double Dxy = 3.0 * ddx * ddy * zi;
+    const double Dxz = (R2 - 3.0 * Rz * zi) * ddx;
+    const double Dyy = (3.0 * ddy * ddy - R2) * zi;
+    const double Dyz = (R2 - 3.0 * Rz * zi) * ddy;
+    const double Dzx = (3.0 * Rz * zi + R2) * ddx;
+    const double Dzy = (3.0 * Rz * zi + R2) * ddy;
+    const double Dzz = (R2 - 3.0 * Rz * Rz) * zi;
+    const double g = 2.0 * zj * Rm1 * Rm1 * Rm1 * Rm1 * Rm1;
+    acc[0] += g * (Dxx * fx + Dxy * fy + Dxz * fz);
+    acc[1] += g * (Dxy * fx + Dyy * fy + Dyz * fz);
+    acc[2] += g * (Dzx * fx + Dzy * fy + Dzz * fz);
 }
 
 template <typename T, int MODE, bool BLAKE, int FS, int BLOCK, int R>
@@ -54,6 +82,10 @@ __device__ __forceinline__ void flow_sources(const FlowArgs& a, CView xsrc, CVie
                 rec[3] = float(p0 - double(h0)); rec[4] = float(p1 - double(h1)); rec[5] = float(p2 - double(h2));
 #pragma unroll
                 for (int k = 0; k < FS; ++k) rec[6 + k] = float(fsrc.p[s * fsrc.ld_p + k * fsrc.ld_c]);
+                if constexpr (FS == 6) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) rec[12 + k] = -rec[9 + k];
+                }
             } else {
                 rec[0] = p0; rec[1] = p1; rec[2] = p2;
 #pragma unroll
@@ -82,7 +114,29 @@ __device__ __forceinline__ void flow_sources(const FlowArgs& a, CView xsrc, CVie
                     dz = T(xi[r][2]) - rec[2];
                 }
                 const T r2 = dx * dx + dy * dy + dz * dz;
-                if constexpr (MODE == 0) {
+                if constexpr (MODE == 0 && F32) {
+                    // Stokeslet (+ rotlet): flow_field_evaluator.cu:66-78, 119-124, written
+                    // as accumulates only: v += f/r + (d.f) d / r^3 + (tau x d) / r^3.
+                    const T rm1 = rsqrt_refined(r2);
+                    const T rm3 = rm1 * (rm1 * rm1);
+                    const T sd = rm3 * fma_t(dz, f[2], fma_t(dy, f[1], dx * f[0]));
+                    part[r][0] = fma_t(sd, dx, fma_t(rm1, f[0], part[r][0]));
+                    part[r][1] = fma_t(sd, dy, fma_t(rm1, f[1], part[r][1]));
+                    part[r][2] = fma_t(sd, dz, fma_t(rm1, f[2], part[r][2]));
+                    if constexpr (FS == 6) {
+                        const T* tn = rec + 12;   // -tau
+                        part[r][0] = fma_t(rm3, fma_t(tn[2], dy, f[4] * dz), part[r][0]);
+                        part[r][1] = fma_t(rm3, fma_t(tn[0], dz, f[5] * dx), part[r][1]);
+                        part[r][2] = fma_t(rm3, fma_t(tn[1], dx, f[3] * dy), part[r][2]);
+                    }
+                    if constexpr (BLAKE) {
+                        // Blakelet image system, flow_field_evaluator.cu:83-116, evaluated in
+                        // double (the image distance of a wall-adjacent pair is small).
+                        const double zi = xi[r][2];
+                        const double zj = double(rec[2]) + double(rec[5]);
+                        blakelet(acc[r], zi, zj, double(dx), double(dy), double(f[0]), double(f[1]), double(f[2]));
+                    }
+                } else if constexpr (MODE == 0) {
                     // Stokeslet (+ rotlet): flow_field_evaluator.cu:66-78, 119-124
                     const T rm1 = rsqrt_refined(r2);
                     const T rm2 = rm1 * rm1;
@@ -103,27 +157,7 @@ __device__ __forceinline__ void flow_sources(const FlowArgs& a, CView xsrc, CVie
                         double zj;
                         if constexpr (F32) zj = double(rec[2]) + double(rec[5]);
                         else zj = rec[2];
-                        const double ddx = double(dx), ddy = double(dy);
-                        const double Rz = zi + zj;
-                        const double R2 = ddx * ddx + ddy * ddy + Rz * Rz;
-                        const double Rm1 = __builtin_sqrt(1.0 / R2);
-                        const double fx = double(f[0]), fy = double(f[1]), fz = double(f[2]);
-                        const double kk = (ddx * fx + ddy * fy + Rz * fz) * Rm1 * Rm1;
-                        acc[r][0] -= Rm1 * (fx + ddx * kk);
-                        acc[r][1] -= Rm1 * (fy + ddy * kk);
-                        acc[r][2] -= Rm1 * (fz + Rz * kk);
-                        const double Dxx = (3.0 * ddx * ddx - R2) * zi;
-                        const double Dxy = 3.0 * ddx * ddy * zi;
-                        const double Dxz = (R2 - 3.0 * Rz * zi) * ddx;
-                        const double Dyy = (3.0 * ddy * ddy - R2) * zi;
-                        const double Dyz = (R2 - 3.0 * Rz * zi) * ddy;
-                        const double Dzx = (3.0 * Rz * zi + R2) * ddx;
-                        const double Dzy = (3.0 * Rz * zi + R2) * ddy;
-                        const double Dzz = (R2 - 3.0 * Rz * Rz) * zi;
-                        const double g = 2.0 * zj * Rm1 * Rm1 * Rm1 * Rm1 * Rm1;
-                        acc[r][0] += g * (Dxx * fx + Dxy * fy + Dxz * fz);
-                        acc[r][1] += g * (Dxy * fx + Dyy * fy + Dyz * fz);
-                        acc[r][2] += g * (Dzx * fx + Dzy * fy + Dzz * fz);
+                        blakelet(acc[r], zi, zj, double(dx), double(dy), double(f[0]), double(f[1]), double(f[2]));
                     }
                 } else {
                     // RPY point tracer: the pair tensor with ai = 0.

@@ -39,14 +39,25 @@ namespace {
 
 constexpr int kMaxDof = 6;
 
-// LDS record length (in T) of one staged source with FS strength components.
-template <typename T, int FS>
+// LDS record length (in T) of one staged source with FS strength components
+// for targets with VT velocity components.  FP64: x y z f0..f(FS-1) [pad].
+// FP32: xh yh zh xl yl zl f0..f(FS-1), then the negated torque (FS = 6) and
+// the negated force (VT = 6), padded to 16 B: with the negations staged, every
+// cross-product term of the pair loop is an accumulate (the double-rate VOP2
+// form on gfx950) instead of a negated fma.
+template <typename T, int FS, int VT = 3>
 constexpr int rec_len() {
     if constexpr (sizeof(T) == 8) {
-        return FS == 6 ? 10 : 6;   // x y z f0..f(FS-1) [pad]
+        return FS == 6 ? 10 : 6;
     } else {
-        return 12;                 // xh yh zh xl yl zl f0..f(FS-1) [pad to 16 B]
+        return (6 + FS + (FS == 6 ? 3 : 0) + (VT == 6 ? 3 : 0) + 3) / 4 * 4;
     }
+}
+
+// Largest record over the class combinations a kernel can stage.
+template <typename T>
+constexpr int rec_len_max() {
+    return rec_len<T, 6, 6>();
 }
 
 // Separation and squared distance exactly as the pair loop evaluates them;
@@ -80,7 +91,7 @@ __device__ __forceinline__ void split_position(double x, T& h, T& l) {
 
 // One source into its LDS record: position (double-float split for FP32) and,
 // if with_force, its strengths multiplied by `scale`.
-template <typename T, int FS>
+template <typename T, int FS, int VT = 3>
 __device__ __forceinline__ void stage_source(T* rec, const CView& xsrc, const CView& fsrc, int s, bool with_force,
                                              double scale = 1.0) {
     const double p0 = xsrc.p[(int64_t)s * xsrc.ld_p];
@@ -99,13 +110,25 @@ __device__ __forceinline__ void stage_source(T* rec, const CView& xsrc, const CV
     if (with_force) {
 #pragma unroll
         for (int k = 0; k < FS; ++k) rec[F0 + k] = T(scale * fsrc.p[(int64_t)s * fsrc.ld_p + k * fsrc.ld_c]);
+        if constexpr (sizeof(T) == 4) {
+            int o = F0 + FS;
+            if constexpr (FS == 6) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) rec[o + k] = -rec[F0 + 3 + k];
+                o += 3;
+            }
+            if constexpr (VT == 6) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) rec[o + k] = -rec[F0 + k];
+            }
+        }
     }
 }
 
 template <int VT, int FS, typename T, int BLOCK, int R>
 __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile tl, const MatvecArgs& a,
                                          T* __restrict__ slot, T* __restrict__ lds) {
-    constexpr int REC = rec_len<T, FS>();
+    constexpr int REC = rec_len<T, FS, VT>();
     constexpr int TILE = BLOCK * R;
     constexpr int F0 = sizeof(T) == 4 ? 6 : 3;
     const int tid = threadIdx.x;
@@ -149,7 +172,7 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
     for (int s0 = it.s_begin; s0 < it.s_end; s0 += BLOCK) {
         const int n = (it.s_end - s0) < BLOCK ? (it.s_end - s0) : BLOCK;
         __syncthreads();  // the previous batch has been consumed
-        if (tid < n) stage_source<T, FS>(lds + tid * REC, xsrc, fsrc, s0 + tid, true, c8);
+        if (tid < n) stage_source<T, FS, VT>(lds + tid * REC, xsrc, fsrc, s0 + tid, true, c8);
         __syncthreads();
         if constexpr (F32) {
 #pragma unroll
@@ -171,8 +194,8 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
                 // Overlapping pairs (and the self pair) are added by the reduce
                 // kernel; a zero 1/r removes them here without a branch.
                 const T rm1 = r2 > sum_a2 ? rsqrt_refined(r2) : T(0);
-                if constexpr (F32) pair_far_scaled<T, VT, FS>(part[r], dx, dy, dz, rm1, f, s3, m3s3);
-                else pair_far_scaled<T, VT, FS>(acc[r], dx, dy, dz, rm1, f, s3, m3s3);
+                if constexpr (F32) pair_far_scaled<T, VT, FS>(part[r], dx, dy, dz, rm1, f, rec + F0 + FS, s3, m3s3);
+                else pair_far_scaled<T, VT, FS>(acc[r], dx, dy, dz, rm1, f, nullptr, s3, m3s3);
             }
         }
         if constexpr (F32) {
@@ -196,7 +219,7 @@ template <typename T, int SEGDOF, int BLOCK, int R>
 __global__ __launch_bounds__(BLOCK) void rpy_pairs_kernel(const MatvecItem* __restrict__ items,
                                                           const MatvecTile* __restrict__ tiles,
                                                           MatvecArgs a, T* __restrict__ partial) {
-    constexpr int REC = rec_len<T, 6>() > rec_len<T, 3>() ? rec_len<T, 6>() : rec_len<T, 3>();
+    constexpr int REC = rec_len_max<T>();
     __shared__ __align__(16) T lds[BLOCK * REC];
     if (a.skip && *a.skip) return;
     const MatvecItem it = items[blockIdx.x];
@@ -221,7 +244,7 @@ __global__ __launch_bounds__(BLOCK) void rpy_pairs_persistent_kernel(const Matve
                                                                      const MatvecTile* __restrict__ tiles,
                                                                      MatvecArgs a, T* __restrict__ partial,
                                                                      int n_items, PersistState ps) {
-    constexpr int REC = rec_len<T, 6>() > rec_len<T, 3>() ? rec_len<T, 6>() : rec_len<T, 3>();
+    constexpr int REC = rec_len_max<T>();
     __shared__ __align__(16) T lds[BLOCK * REC];
     if (a.skip && *a.skip) return;
     for (int item = blockIdx.x; item < n_items; item += gridDim.x) {

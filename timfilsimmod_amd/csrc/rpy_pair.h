@@ -89,9 +89,13 @@ __device__ __forceinline__ double fma_t(double a, double b, double c) { return _
 // the fewest VALU operations: A = 1/r + s3/r^3, B = (1 - 3 s3/r^2)/r^3,
 // C = 1/r^3, and every term is fused into the accumulators.  The same tensor
 // as pair_far.
+// fneg (FP32 only, else unused): the negated torque (FS = 6) followed by the
+// negated force (VT = 6) of the source, so that the cross products are
+// accumulates.
 template <typename T, int VT, int FS>
 __device__ __forceinline__ void pair_far_scaled(T* __restrict__ v, T dx, T dy, T dz, T rm1,
-                                                const T* __restrict__ f, T s3, T m3s3) {
+                                                const T* __restrict__ f, const T* __restrict__ fneg, T s3,
+                                                T m3s3) {
     // Written so that every operation is a two-source multiply or an
     // accumulate into a dying value (VOP2 forms, the double-rate FP32 issue on
     // gfx950): A accumulates into 1/r, B into 1/r^3.
@@ -104,17 +108,31 @@ __device__ __forceinline__ void pair_far_scaled(T* __restrict__ v, T dx, T dy, T
     v[0] = fma_t(Bd, dx, fma_t(A, f[0], v[0]));
     v[1] = fma_t(Bd, dy, fma_t(A, f[1], v[1]));
     v[2] = fma_t(Bd, dz, fma_t(A, f[2], v[2]));
+    constexpr bool F32 = sizeof(T) == 4;
     if constexpr (FS == 6) {
         // translation-rotation: v += C (tau x d)
-        v[0] = fma_t(rm3, f[4] * dz - f[5] * dy, v[0]);
-        v[1] = fma_t(rm3, f[5] * dx - f[3] * dz, v[1]);
-        v[2] = fma_t(rm3, f[3] * dy - f[4] * dx, v[2]);
+        if constexpr (F32) {
+            v[0] = fma_t(rm3, fma_t(fneg[2], dy, f[4] * dz), v[0]);
+            v[1] = fma_t(rm3, fma_t(fneg[0], dz, f[5] * dx), v[1]);
+            v[2] = fma_t(rm3, fma_t(fneg[1], dx, f[3] * dy), v[2]);
+        } else {
+            v[0] = fma_t(rm3, f[4] * dz - f[5] * dy, v[0]);
+            v[1] = fma_t(rm3, f[5] * dx - f[3] * dz, v[1]);
+            v[2] = fma_t(rm3, f[3] * dy - f[4] * dx, v[2]);
+        }
     }
     if constexpr (VT == 6) {
         // rotation-translation: w += C (F x d)
-        v[3] = fma_t(rm3, f[1] * dz - f[2] * dy, v[3]);
-        v[4] = fma_t(rm3, f[2] * dx - f[0] * dz, v[4]);
-        v[5] = fma_t(rm3, f[0] * dy - f[1] * dx, v[5]);
+        if constexpr (F32) {
+            const T* nf = fneg + (FS == 6 ? 3 : 0);
+            v[3] = fma_t(rm3, fma_t(nf[2], dy, f[1] * dz), v[3]);
+            v[4] = fma_t(rm3, fma_t(nf[0], dz, f[2] * dx), v[4]);
+            v[5] = fma_t(rm3, fma_t(nf[1], dx, f[0] * dy), v[5]);
+        } else {
+            v[3] = fma_t(rm3, f[1] * dz - f[2] * dy, v[3]);
+            v[4] = fma_t(rm3, f[2] * dx - f[0] * dz, v[4]);
+            v[5] = fma_t(rm3, f[0] * dy - f[1] * dx, v[5]);
+        }
         if constexpr (FS == 6) {
             // rotation-rotation: w += -C/2 tau + (3C/2) (tau.d) d / r^2
             const T Ar = T(-0.5) * rm3;
