@@ -13,9 +13,8 @@
 //   RPY                 -- the point-tracer limit (target radius 0) of the RPY
 //                          pair tensor, <3,6> from segments and <3,3> from blobs.
 //
-// One target per lane (PER_THREAD of them), all sources streamed through LDS
-// in batches of BLOCK; the 4.2M-point plane is 16k workgroups, so no source
-// splitting is needed.  FP32 evaluation uses double-float positions and
+// Two targets per lane, all sources streamed through LDS in batches of BLOCK;
+// the 4.2M-point plane is 8k workgroups, so no source splitting is needed.  FP32 evaluation uses double-float positions and
 // flushes FP32 partial sums into FP64 accumulators after every batch.
 #include "rpy_pair.h"
 #include "tfs_internal.h"
@@ -25,9 +24,10 @@ namespace {
 
 template <typename T>
 constexpr int flow_rec() {
-    // FP64: x y z f0..f5 (+pad).  FP32: xh yh zh xl yl zl f0..f5 -f3 -f4 -f5 (+pad):
-    // the negated torque lets every rotlet term be an accumulate (VOP2 form).
-    return sizeof(T) == 8 ? 10 : 16;
+    // FP64: x y z f0..f5 (+pad).  FP32: xh yh zh xl yl zl f0..f5.  (Staging the
+    // negated torque too, so that the rotlet terms become accumulates, measured
+    // slower: the extra 16 B per source outweighs the saved issue slots.)
+    return sizeof(T) == 8 ? 10 : 12;
 }
 
 // Image system of a Stokeslet above the no-slip wall z = 0 (Blake 1971):
@@ -82,10 +82,7 @@ __device__ __forceinline__ void flow_sources(const FlowArgs& a, CView xsrc, CVie
                 rec[3] = float(p0 - double(h0)); rec[4] = float(p1 - double(h1)); rec[5] = float(p2 - double(h2));
 #pragma unroll
                 for (int k = 0; k < FS; ++k) rec[6 + k] = float(fsrc.p[s * fsrc.ld_p + k * fsrc.ld_c]);
-                if constexpr (FS == 6) {
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) rec[12 + k] = -rec[9 + k];
-                }
+
             } else {
                 rec[0] = p0; rec[1] = p1; rec[2] = p2;
 #pragma unroll
@@ -124,10 +121,9 @@ __device__ __forceinline__ void flow_sources(const FlowArgs& a, CView xsrc, CVie
                     part[r][1] = fma_t(sd, dy, fma_t(rm1, f[1], part[r][1]));
                     part[r][2] = fma_t(sd, dz, fma_t(rm1, f[2], part[r][2]));
                     if constexpr (FS == 6) {
-                        const T* tn = rec + 12;   // -tau
-                        part[r][0] = fma_t(rm3, fma_t(tn[2], dy, f[4] * dz), part[r][0]);
-                        part[r][1] = fma_t(rm3, fma_t(tn[0], dz, f[5] * dx), part[r][1]);
-                        part[r][2] = fma_t(rm3, fma_t(tn[1], dx, f[3] * dy), part[r][2]);
+                        part[r][0] = fma_t(rm3, f[4] * dz - f[5] * dy, part[r][0]);
+                        part[r][1] = fma_t(rm3, f[5] * dx - f[3] * dz, part[r][1]);
+                        part[r][2] = fma_t(rm3, f[3] * dy - f[4] * dx, part[r][2]);
                     }
                     if constexpr (BLAKE) {
                         // Blakelet image system, flow_field_evaluator.cu:83-116, evaluated in
@@ -255,13 +251,13 @@ __global__ void net_force_kernel(FlowArgs a, double* __restrict__ net, int nbod)
 
 template <typename T, int MODE, bool BLAKE>
 hipError_t launch_typed(const FlowArgs& a, const double* net, int block, hipStream_t s) {
-    const int R = 1;
+    const int R = 2;  // two targets per lane share every LDS source read
     const int64_t tile = (int64_t)block * R;
     const dim3 grid((unsigned)((a.nt + tile - 1) / tile));
     switch (block) {
-        case 64: hipLaunchKernelGGL((flow_kernel<T, MODE, BLAKE, 64, 1>), grid, dim3(64), 0, s, a, net); break;
-        case 128: hipLaunchKernelGGL((flow_kernel<T, MODE, BLAKE, 128, 1>), grid, dim3(128), 0, s, a, net); break;
-        case 256: hipLaunchKernelGGL((flow_kernel<T, MODE, BLAKE, 256, 1>), grid, dim3(256), 0, s, a, net); break;
+        case 64: hipLaunchKernelGGL((flow_kernel<T, MODE, BLAKE, 64, 2>), grid, dim3(64), 0, s, a, net); break;
+        case 128: hipLaunchKernelGGL((flow_kernel<T, MODE, BLAKE, 128, 2>), grid, dim3(128), 0, s, a, net); break;
+        case 256: hipLaunchKernelGGL((flow_kernel<T, MODE, BLAKE, 256, 2>), grid, dim3(256), 0, s, a, net); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
