@@ -12,12 +12,13 @@
 // applies the reference's write rule (= or +=), so results are deterministic
 // run to run, unlike atomics.
 //
-// The pair loop is branch-free: it evaluates the far-field RPY forms and
-// drops (by a select) every pair with r^2 <= (ai+aj)^2, which includes the
-// self pair.  Those overlapping pairs -- a handful per particle -- are listed
-// once per geometry by build_near_list, using the pair loop's own arithmetic
-// for r^2, and the reduce kernel adds them with the reference's overlap and
-// inner-branch formulas in FP64.  Every pair is counted exactly once.
+// The pair loop is branch-free: it evaluates the far-field RPY forms for
+// every pair, with r^2 clamped below at (ai+aj)^2 (the self pair included).
+// The pairs that clamp touches -- r^2 <= (ai+aj)^2, a handful per particle --
+// are listed once per geometry by build_near_list, using the pair loop's own
+// arithmetic for r^2, and the reduce kernel takes out their clamped far-field
+// value and adds the reference's overlap and inner-branch formulas in FP64.
+// Every pair is counted exactly once.
 //
 // Splitting the source range into chunks is what fills the chip: 21,800
 // targets are only 341 wave64s, far fewer than the 1,024 SIMDs of an MI355X;
@@ -141,7 +142,8 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
     const double PI = 3.14159265358979323846;
     const T sum_a2 = T((ai + aj) * (ai + aj));
     const T s3 = T((ai * ai + aj * aj) / 3.0);
-    const T m3s3 = T(-(ai * ai + aj * aj));
+    // -3 s3 as its own value (a negated operand would force the VOP3 form of the fma)
+    const T m3s3 = T(-3) * s3;
     const double c8 = 1.0 / (8.0 * PI * a.mu);  // folded into the staged strengths
 
     // Targets of this lane (clamped into the tile; surplus lanes compute a
@@ -191,9 +193,10 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
             for (int r = 0; r < R; ++r) {
                 T dx, dy, dz;
                 const T r2 = separation<T>(xi[r], xh[r], xl[r], rec, dx, dy, dz);
-                // Overlapping pairs (and the self pair) are added by the reduce
-                // kernel; a zero 1/r removes them here without a branch.
-                const T rm1 = r2 > sum_a2 ? rsqrt_refined(r2) : T(0);
+                // Overlapping pairs (and the self pair) are evaluated here with r
+                // clamped to ai + aj -- one max instead of a compare and select --
+                // and the reduce kernel swaps that value for the exact one.
+                const T rm1 = rsqrt_refined(fmax_t(r2, sum_a2));
                 if constexpr (F32) pair_far_scaled<T, VT, FS>(part[r], dx, dy, dz, rm1, f, rec + F0 + FS, s3, m3s3);
                 else pair_far_scaled<T, VT, FS>(acc[r], dx, dy, dz, rm1, f, nullptr, s3, m3s3);
             }
@@ -331,6 +334,18 @@ __device__ void add_near(double* acc, int src, const double* xi, double ai, cons
     const int fs = seg ? seg_dof : 3;
     for (int k = 0; k < 6; ++k) f[k] = k < fs ? fv.p[(int64_t)s * fv.ld_p + k * fv.ld_c] : 0.0;
     const double aj = seg ? a.a_seg : a.a_blob;
+    // Take out what the pair loop added for this pair -- the far-field form at
+    // r clamped to ai + aj, with the strengths scaled by 1/(8 pi mu) -- then add
+    // the reference's overlap / inner-branch value.
+    const double PI = 3.14159265358979323846;
+    const double c8 = 1.0 / (8.0 * PI * a.mu);
+    const double s3 = (ai * ai + aj * aj) / 3.0;
+    double fsc[6], clamp[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 6; ++k) fsc[k] = c8 * f[k];
+    const double rm1c = rsqrt_refined((ai + aj) * (ai + aj));
+    if (fs == 6) pair_far_scaled<double, VT, 6>(clamp, d0, d1, d2, rm1c, fsc, nullptr, s3, -3.0 * s3);
+    else pair_far_scaled<double, VT, 3>(clamp, d0, d1, d2, rm1c, fsc, nullptr, s3, -3.0 * s3);
+    for (int k = 0; k < VT; ++k) acc[k] -= clamp[k];
     if (fs == 6) pair_near<VT, 6>(acc, d0, d1, d2, f, ai, aj, a.mu);
     else pair_near<VT, 3>(acc, d0, d1, d2, f, ai, aj, a.mu);
 }
