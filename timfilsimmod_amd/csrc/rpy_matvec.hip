@@ -350,22 +350,50 @@ __device__ void add_near(double* acc, int src, const double* xi, double ai, cons
     else pair_near<VT, 3>(acc, d0, d1, d2, f, ai, aj, a.mu);
 }
 
-// Adds the partial slots of each tile, then the overlapping pairs in
-// increasing source order (the lists are sorted when built), and applies the
-// write rule.  A workgroup covers 64 targets of one tile with four groups of
-// 64 threads: group g sums the items g, g+4, g+8, ... (loads of four items
-// issued together), and the four group sums are added in group order, so the
-// result is the same on every run.
+// Filament rows of the saddle-point operator: -K_phi^T lambda and
+// -K_theta^T lambda over the filament's 3*nseg entries, by one wave.
+__device__ void saddle_filament_rows(const SaddleEpilogue& e, int m, int lane) {
+    const int len = 3 * e.nseg;
+    const int64_t seg0 = (int64_t)m * len;
+    double sp = 0.0, st = 0.0;
+    for (int k = lane; k < len; k += 64) {
+        const double lam = e.in[seg0 + k];
+        sp += lam * e.kphi[seg0 + k];
+        st += lam * e.kth[seg0 + k];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        sp += __shfl_down(sp, off, 64);
+        st += __shfl_down(st, off, 64);
+    }
+    if (lane == 0) {
+        if (e.dyn_phase) e.out[e.off_phase + m] = -sp;
+        if (e.dyn_angle) e.out[e.off_angle + m] = -st;
+    }
+}
+
+// Adds the partial slots of each tile and the overlapping pairs, and applies
+// the write rule.  A workgroup covers 64 targets of one tile with four groups
+// of 64 threads: group g sums the items g, g+4, g+8, ... (loads of four items
+// issued together) and then the near-list entries g, g+4, ... of its target
+// (the lists are sorted when built); the four group sums are added in group
+// order, so the result is the same on every run.  Workgroups past
+// `tile_blocks` write the filament rows of the saddle-point operator.
 template <int SEGDOF, typename P>
 __global__ __launch_bounds__(256) void rpy_reduce_kernel(const MatvecTile* __restrict__ tiles,
                                                          const P* __restrict__ partial, int tile_targets,
-                                                         OutArgs o, MatvecArgs a, NearList near) {
+                                                         int tile_blocks, OutArgs o, MatvecArgs a, NearList near) {
     __shared__ double grp_acc[3][6][64];
     if (a.skip && *a.skip) return;
+    const int grp = threadIdx.x >> 6;
+    if ((int)blockIdx.x >= tile_blocks) {
+        const int m = ((int)blockIdx.x - tile_blocks) * 4 + grp;
+        if (m < o.sad.nfil) saddle_filament_rows(o.sad, m, threadIdx.x & 63);
+        return;
+    }
     const int per_tile = (tile_targets + 63) / 64;
     const MatvecTile tl = tiles[blockIdx.x / per_tile];
     const int l = (blockIdx.x % per_tile) * 64 + (threadIdx.x & 63);
-    const int grp = threadIdx.x >> 6;
     const int t = tl.t_begin + l;
     const bool active = l < tile_targets && t < tl.t_end;
     const bool seg_t = tl.cls == 0;
@@ -391,6 +419,22 @@ __global__ __launch_bounds__(256) void rpy_reduce_kernel(const MatvecTile* __res
             for (int k = 0; k < 6; ++k) acc[k] += in ? v[u][k] : 0.0;
         }
     }
+    // Active source classes for this target class: bit 0 segments, bit 1 blobs.
+    const uint32_t cls_mask = seg_t ? ((o.blocks & 1u) ? 1u : 0u) | ((o.blocks & 2u) ? 2u : 0u)
+                                    : ((o.blocks & 8u) ? 1u : 0u) | ((o.blocks & 4u) ? 2u : 0u);
+    if (near.cnt && cls_mask && active) {
+        const int tg = (seg_t ? 0 : a.ns) + t;
+        const int c = near.cnt[tg] < near.cap ? near.cnt[tg] : near.cap;
+        const int32_t* e = near.idx + (int64_t)tg * near.cap;
+        const CView xv = seg_t ? a.xs : a.xb;
+        const double xi[3] = {xv.p[(int64_t)t * xv.ld_p], xv.p[(int64_t)t * xv.ld_p + xv.ld_c],
+                              xv.p[(int64_t)t * xv.ld_p + 2 * xv.ld_c]};
+        const double ai = seg_t ? a.a_seg : a.a_blob;
+        for (int q = grp; q < c; q += 4) {
+            if (seg_t && SEGDOF == 6) add_near<6>(acc, e[q], xi, ai, a, SEGDOF, cls_mask);
+            else add_near<3>(acc, e[q], xi, ai, a, SEGDOF, cls_mask);
+        }
+    }
     if (grp > 0) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) grp_acc[grp - 1][k][threadIdx.x & 63] = acc[k];
@@ -401,21 +445,11 @@ __global__ __launch_bounds__(256) void rpy_reduce_kernel(const MatvecTile* __res
     for (int g = 0; g < 3; ++g)
 #pragma unroll
         for (int k = 0; k < 6; ++k) acc[k] += grp_acc[g][k][threadIdx.x];
-    // Active source classes for this target class: bit 0 segments, bit 1 blobs.
-    const uint32_t cls_mask = seg_t ? ((o.blocks & 1u) ? 1u : 0u) | ((o.blocks & 2u) ? 2u : 0u)
-                                    : ((o.blocks & 8u) ? 1u : 0u) | ((o.blocks & 4u) ? 2u : 0u);
-    if (near.cnt && cls_mask) {
-        const int tg = (seg_t ? 0 : a.ns) + t;
-        const int c = near.cnt[tg] < near.cap ? near.cnt[tg] : near.cap;
-        const int32_t* e = near.idx + (int64_t)tg * near.cap;
-        const CView xv = seg_t ? a.xs : a.xb;
-        const double xi[3] = {xv.p[(int64_t)t * xv.ld_p], xv.p[(int64_t)t * xv.ld_p + xv.ld_c],
-                              xv.p[(int64_t)t * xv.ld_p + 2 * xv.ld_c]};
-        const double ai = seg_t ? a.a_seg : a.a_blob;
-        for (int q = 0; q < c; ++q) {
-            if (seg_t && SEGDOF == 6) add_near<6>(acc, e[q], xi, ai, a, SEGDOF, cls_mask);
-            else add_near<3>(acc, e[q], xi, ai, a, SEGDOF, cls_mask);
-        }
+    if (seg_t && o.sad.in) {
+        const int m = t / o.sad.nseg;
+        const double pd = o.sad.dyn_phase ? o.sad.in[o.sad.off_phase + m] : 0.0;
+        const double td = o.sad.dyn_angle ? o.sad.in[o.sad.off_angle + m] : 0.0;
+        for (int k = 0; k < 3; ++k) acc[k] -= pd * o.sad.kphi[3 * (int64_t)t + k] + td * o.sad.kth[3 * (int64_t)t + k];
     }
     const int slots = seg_t ? o.seg_out_slots : 3;
     const View out = seg_t ? o.vs : o.vb;
@@ -551,23 +585,25 @@ hipError_t launch_matvec(int precision, int seg_dof, const TileShape& shape, con
         if (ev_end) TFS_HIP_CHECK(hipEventRecord(ev_end, stream));
     }
     const int tt = shape.block * shape.per_thread;
-    const unsigned grid = (unsigned)n_tiles * (unsigned)((tt + 63) / 64);
+    const int tile_blocks = n_tiles * ((tt + 63) / 64);
+    const int fil_blocks = o.sad.in ? (o.sad.nfil + 3) / 4 : 0;
+    const dim3 grid((unsigned)(tile_blocks + fil_blocks));
     // The partial slots hold the evaluation type (FP32 sums for FP32 tiles).
     const float* pf = reinterpret_cast<const float*>(partial);
     if (precision == 0) {
         if (seg_dof == 6)
-            hipLaunchKernelGGL((rpy_reduce_kernel<6, double>), dim3(grid), dim3(256), 0, stream, d_tiles,
-                               (const double*)partial, tt, o, a, near);
+            hipLaunchKernelGGL((rpy_reduce_kernel<6, double>), grid, dim3(256), 0, stream, d_tiles,
+                               (const double*)partial, tt, tile_blocks, o, a, near);
         else
-            hipLaunchKernelGGL((rpy_reduce_kernel<3, double>), dim3(grid), dim3(256), 0, stream, d_tiles,
-                               (const double*)partial, tt, o, a, near);
+            hipLaunchKernelGGL((rpy_reduce_kernel<3, double>), grid, dim3(256), 0, stream, d_tiles,
+                               (const double*)partial, tt, tile_blocks, o, a, near);
     } else {
         if (seg_dof == 6)
-            hipLaunchKernelGGL((rpy_reduce_kernel<6, float>), dim3(grid), dim3(256), 0, stream, d_tiles, pf, tt, o,
-                               a, near);
+            hipLaunchKernelGGL((rpy_reduce_kernel<6, float>), grid, dim3(256), 0, stream, d_tiles, pf, tt,
+                               tile_blocks, o, a, near);
         else
-            hipLaunchKernelGGL((rpy_reduce_kernel<3, float>), dim3(grid), dim3(256), 0, stream, d_tiles, pf, tt, o,
-                               a, near);
+            hipLaunchKernelGGL((rpy_reduce_kernel<3, float>), grid, dim3(256), 0, stream, d_tiles, pf, tt,
+                               tile_blocks, o, a, near);
     }
     return hipGetLastError();
 }

@@ -54,23 +54,29 @@ hipError_t k_scale(double* v, const double* w, const double* scal, int idx, int 
 hipError_t k_trisolve(const double* H, int ldh, const double* g, int k, double* y, hipStream_t s);
 hipError_t k_axpby(double a, const double* x, double b, double* y, int n, hipStream_t s);
 hipError_t k_sub(const double* x, const double* y, double* out, int n, hipStream_t s);
-// One CGS2 Arnoldi step on w against V(:, 0:j] (four launches of G =
+// One CGS2 Arnoldi step on w against V(:, 0:j] (three launches of G =
 // arnoldi_blocks(n) workgroups; pa and pb hold G x (j+1) partial sums each):
-// H(:, j) = h1 + h2 (and H(j+1, j) rotated), w orthogonalised, Givens
-// rotations updated, scal[0] = est[j] = |g(j+1)|, scal[1] = 1/||w||.  Every
-// launch does nothing while *stop is nonzero; the step sets *stop = j + 1 when
-// est[j] <= abs_tol (-(j + 1) if it is not finite).
+// H(:, j) = h1 + h2, w orthogonalised, and G partial sums of ||w||^2 left in
+// pa for k_arnoldi_close_next.  Every launch does nothing while *stop is
+// nonzero.
 hipError_t k_arnoldi_step(const double* V, int64_t ld, int j, double* w, int n, double* pa, double* pb, int G,
                           double* H, int ldh, double* cs, double* sn, double* g, double* scal, double* est,
                           int32_t* stop, double abs_tol, hipStream_t s);
+// Closes step j -- ||w|| from pa, H(j+1, j) rotated, Givens rotations
+// updated, scal[0] = est[j] = |g(j+1)|, scal[1] = 1/||w||, *stop = j + 1 when
+// est[j] <= abs_tol (-(j + 1) if it is not finite) -- and, when v is not
+// null, writes v = w / ||w|| and out = P^{-1} v in the same launch.  Nothing
+// happens while *stop is nonzero.
+hipError_t k_arnoldi_close_next(const double* pa, int G, double* H, int ldh, int j, double* cs, double* sn,
+                                double* g, double* scal, double* est, int32_t* stop, double abs_tol,
+                                const double* w, double* v, double* out, const double* kphi, const double* kth,
+                                const SaddleDims& d, double gs, double gb, hipStream_t s);
 int arnoldi_blocks(int64_t n);
 // v = w * scale; out = P^{-1} v.  scale = dev_scale[1] if dev_scale, else host_scale.
 // Nothing happens while *skip (if given) is nonzero.
 hipError_t k_scale_precond(const double* w, const double* dev_scale, double host_scale, double* v, double* out,
                            const double* kphi, const double* kth, const SaddleDims& d, double gs, double gb,
                            const int32_t* skip, hipStream_t s);
-hipError_t k_saddle(const double* in, double* out, const double* kphi, const double* kth, const SaddleDims& d,
-                    hipStream_t s, const int32_t* skip = nullptr);
 hipError_t k_precond(const double* in, double* out, const double* kphi, const double* kth, const SaddleDims& d,
                      double gs, double gb, hipStream_t s);
 hipError_t k_beat_coeffs(const double* A, const double* B);

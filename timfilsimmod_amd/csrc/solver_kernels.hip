@@ -294,7 +294,7 @@ __device__ __forceinline__ void gather_coeffs(const double* __restrict__ partial
     __syncthreads();
 }
 
-// w[r] -= sum_c V[r, c] h[c] over the rows of workgroup b (columns four at
+// w[r] -= sum_c V[r, c] h[c] over the rows of workgroup b (columns eight at
 // a time, their loads issued together).
 __device__ __forceinline__ void slice_update(const ArnoldiArgs& a, const double* h, int b, int G) {
     constexpr int Q = kSlice / 256;
@@ -309,6 +309,17 @@ __device__ __forceinline__ void slice_update(const ArnoldiArgs& a, const double*
             idx[q] = r0 + q * 256 < a.n ? r0 + q * 256 : a.n - 1;
         }
         int c = 0;
+        for (; c + 8 <= a.ncols; c += 8) {
+            double vv[8][Q];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int q = 0; q < Q; ++q) vv[u][q] = V[(int64_t)(c + u) * a.ld + idx[q]];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int q = 0; q < Q; ++q) s[q] += vv[u][q] * h[c + u];
+        }
         for (; c + 4 <= a.ncols; c += 4) {
             double vv[4][Q];
 #pragma unroll
@@ -366,29 +377,24 @@ __global__ __launch_bounds__(256) void arnoldi_update_sumsq_kernel(ArnoldiArgs a
     if (threadIdx.x == 0) p_out[blockIdx.x] = s;
 }
 
-// ||w|| from the G partial sums, then the Givens bookkeeping of givens_kernel;
-// est[j] = |g(j+1)|.  One wave: the column of H and the previous rotations
-// are staged in LDS so that lane 0's sequential rotations read no memory.
-// The first step whose estimate meets abs_tol sets *stop = j + 1 (a
-// non-finite estimate sets -(j + 1)); later launches of the solve then skip.
-__global__ __launch_bounds__(64) void arnoldi_close_kernel(const double* __restrict__ p_in, int G,
-                                                           double* __restrict__ H, int ldh, int j,
-                                                           double* __restrict__ cs, double* __restrict__ sn,
-                                                           double* __restrict__ g, double* __restrict__ scal,
-                                                           double* __restrict__ est, int32_t* __restrict__ stop,
-                                                           double abs_tol) {
+// Givens bookkeeping of step j given ||w|| = hn (givens_kernel's rotations):
+// the column of H and the previous rotations are staged in LDS so that
+// thread 0's sequential rotations read no memory; est[j] = |g(j+1)|,
+// scal[1] = 1/||w||.  The first step whose estimate meets abs_tol sets
+// *stop = j + 1 (a non-finite estimate sets -(j + 1)); later launches of the
+// solve then skip.  Called by every thread of one workgroup.
+__device__ void arnoldi_close(double hn, double* __restrict__ H, int ldh, int j, double* __restrict__ cs,
+                              double* __restrict__ sn, double* __restrict__ g, double* __restrict__ scal,
+                              double* __restrict__ est, int32_t* __restrict__ stop, double abs_tol) {
     __shared__ double col[kMaxBasis + 1], lc[kMaxBasis], ls[kMaxBasis];
-    if (*stop) return;
     double* Hc = H + (int64_t)j * ldh;
-    for (int i = threadIdx.x; i <= j; i += kWave) {
+    for (int i = threadIdx.x; i <= j; i += blockDim.x) {
         col[i] = Hc[i];
         lc[i] = cs[i];
         ls[i] = sn[i];
     }
-    const double t = wave_sum_strided(p_in, G);
     __syncthreads();
     if (threadIdx.x == 0) {
-        const double hn = __builtin_sqrt(t);
         col[j + 1] = hn;
         scal[1] = hn > 0.0 ? 1.0 / hn : 0.0;
         for (int i = 1; i <= j; ++i) {
@@ -423,21 +429,14 @@ __global__ __launch_bounds__(64) void arnoldi_close_kernel(const double* __restr
         else if (scal[0] <= abs_tol) *stop = j + 1;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i <= j + 1; i += kWave) Hc[i] = col[i];
+    for (int i = threadIdx.x; i <= j + 1; i += blockDim.x) Hc[i] = col[i];
 }
 
 // The next basis vector and its preconditioned image in one pass:
-// v = w * scale (scale = *dev_scale, or host_scale when dev_scale is null),
-// z = P^{-1} v, with the filament-wave layout of precond_kernel.
-__global__ __launch_bounds__(256) void scale_precond_kernel(const double* __restrict__ w,
-                                                           const double* __restrict__ dev_scale,
-                                                           double host_scale, double* __restrict__ v,
-                                                           double* __restrict__ out,
-                                                           const double* __restrict__ kphi,
-                                                           const double* __restrict__ kth, SaddleDims d,
-                                                           double gs, double gb, const int32_t* __restrict__ skip) {
-    if (skip && *skip) return;
-    const double sc = dev_scale ? dev_scale[1] : host_scale;
+// v = w * sc, out = P^{-1} v, with the filament-wave layout of precond_kernel.
+__device__ void scale_precond_body(const double* __restrict__ w, double sc, double* __restrict__ v,
+                                   double* __restrict__ out, const double* __restrict__ kphi,
+                                   const double* __restrict__ kth, const SaddleDims& d, double gs, double gb) {
     const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     const int nwaves = gridDim.x * blockDim.x / kWave;
@@ -499,37 +498,41 @@ __global__ __launch_bounds__(256) void scale_precond_kernel(const double* __rest
     }
 }
 
-// ---------------------------------------------------------------------------
-// Saddle-point couplings, one wave per filament.
-// Vector layout (mobility_solver.cpp:107-140, prescribed body velocities):
-//   [ lambda_s (3*Ns, xyz per segment) | lambda_b (3*Nb) | phi_dot (Nfil) | theta_dot (Nfil) ]
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void saddle_k_kernel(const double* __restrict__ in, double* __restrict__ out,
-                                                      const double* __restrict__ kphi,
-                                                      const double* __restrict__ kth, SaddleDims d,
-                                                      const int32_t* __restrict__ skip) {
+// scale = *dev_scale[1], or host_scale when dev_scale is null.
+__global__ __launch_bounds__(256) void scale_precond_kernel(const double* __restrict__ w,
+                                                           const double* __restrict__ dev_scale,
+                                                           double host_scale, double* __restrict__ v,
+                                                           double* __restrict__ out,
+                                                           const double* __restrict__ kphi,
+                                                           const double* __restrict__ kth, SaddleDims d,
+                                                           double gs, double gb, const int32_t* __restrict__ skip) {
     if (skip && *skip) return;
-    const int m = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const int lane = threadIdx.x & (kWave - 1);
-    if (m >= d.nfil) return;
-    const int len = 3 * d.nseg;
-    const int64_t seg0 = (int64_t)m * len;
-    const double pd = d.dyn_phase ? in[d.off_phase + m] : 0.0;
-    const double td = d.dyn_angle ? in[d.off_angle + m] : 0.0;
-    double sp = 0.0, st = 0.0;
-    for (int k = lane; k < len; k += kWave) {
-        const double kp = kphi[seg0 + k], kt = kth[seg0 + k];
-        const double lam = in[seg0 + k];
-        out[seg0 + k] -= pd * kp + td * kt;
-        sp += lam * kp;
-        st += lam * kt;
+    scale_precond_body(w, dev_scale ? dev_scale[1] : host_scale, v, out, kphi, kth, d, gs, gb);
+}
+
+// Closes Arnoldi step j and opens step j + 1 in one launch.  Every workgroup
+// sums the G partial sums of ||w||^2 in the same order (so all of them hold
+// the same 1/||w||); workgroup 0 does the Givens bookkeeping of step j; all
+// of them write v_{j+1} = w / ||w|| and z = P^{-1} v_{j+1} (skipped when v is
+// null: the close of the last step of a cycle).  Workgroups that start after
+// step j met the tolerance may still write v_{j+1} and z: neither is read
+// after convergence.
+__global__ __launch_bounds__(256) void arnoldi_close_next_kernel(
+    const double* __restrict__ p_in, int G, double* __restrict__ H, int ldh, int j, double* __restrict__ cs,
+    double* __restrict__ sn, double* __restrict__ g, double* __restrict__ scal, double* __restrict__ est,
+    int32_t* __restrict__ stop, double abs_tol, const double* __restrict__ w, double* __restrict__ v,
+    double* __restrict__ out, const double* __restrict__ kphi, const double* __restrict__ kth, SaddleDims d,
+    double gs, double gb) {
+    __shared__ double nrm;
+    if (*stop) return;
+    if (threadIdx.x < kWave) {
+        const double t = wave_sum_strided(p_in, G);
+        if (threadIdx.x == 0) nrm = __builtin_sqrt(t);
     }
-    sp = wave_sum(sp);
-    st = wave_sum(st);
-    if (lane == 0) {
-        if (d.dyn_phase) out[d.off_phase + m] = -sp;
-        if (d.dyn_angle) out[d.off_angle + m] = -st;
-    }
+    __syncthreads();
+    const double hn = nrm;
+    if (blockIdx.x == 0) arnoldi_close(hn, H, ldh, j, cs, sn, g, scal, est, stop, abs_tol);
+    if (v) scale_precond_body(w, hn > 0.0 ? 1.0 / hn : 0.0, v, out, kphi, kth, d, gs, gb);
 }
 
 // Right preconditioner P^{-1}: the exact inverse of [D -K; -K^T 0] with
@@ -767,8 +770,17 @@ hipError_t k_arnoldi_step(const double* V, int64_t ld, int j, double* w, int n, 
     hipLaunchKernelGGL(arnoldi_dots_kernel, dim3(G), dim3(256), 0, s, a, pa);
     hipLaunchKernelGGL(arnoldi_update_dots_kernel, dim3(G), dim3(256), 0, s, a, (const double*)pa, pb, Hcol);
     hipLaunchKernelGGL(arnoldi_update_sumsq_kernel, dim3(G), dim3(256), 0, s, a, (const double*)pb, pa, Hcol);
-    hipLaunchKernelGGL(arnoldi_close_kernel, dim3(1), dim3(64), 0, s, (const double*)pa, G, H, ldh, j, cs, sn, g,
-                       scal, est, stop, abs_tol);
+    return hipGetLastError();
+}
+hipError_t k_arnoldi_close_next(const double* pa, int G, double* H, int ldh, int j, double* cs, double* sn,
+                                double* g, double* scal, double* est, int32_t* stop, double abs_tol,
+                                const double* w, double* v, double* out, const double* kphi, const double* kth,
+                                const SaddleDims& d, double gs, double gb, hipStream_t s) {
+    if (j + 1 > kMaxBasis) return hipErrorInvalidValue;
+    const int64_t work = (int64_t)d.nfil * kWave > 3LL * d.nblob ? (int64_t)d.nfil * kWave : 3LL * d.nblob;
+    const int grid = v ? blocks_for(work > 0 ? work : 1) : 1;
+    hipLaunchKernelGGL(arnoldi_close_next_kernel, dim3(grid), dim3(256), 0, s, pa, G, H, ldh, j, cs, sn, g, scal,
+                       est, stop, abs_tol, w, v, out, kphi, kth, d, gs, gb);
     return hipGetLastError();
 }
 int arnoldi_blocks(int64_t n) {
@@ -785,13 +797,6 @@ hipError_t k_scale_precond(const double* w, const double* dev_scale, double host
 }
 hipError_t k_sub(const double* x, const double* y, double* out, int n, hipStream_t s) {
     hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(n)), dim3(256), 0, s, x, y, out, n);
-    return hipGetLastError();
-}
-hipError_t k_saddle(const double* in, double* out, const double* kphi, const double* kth, const SaddleDims& d,
-                    hipStream_t s, const int32_t* skip) {
-    if (d.nfil == 0 || !(d.dyn_phase || d.dyn_angle)) return hipSuccess;
-    hipLaunchKernelGGL(saddle_k_kernel, dim3(blocks_for((int64_t)d.nfil * kWave)), dim3(256), 0, s, in, out,
-                       kphi, kth, d, skip);
     return hipGetLastError();
 }
 hipError_t k_precond(const double* in, double* out, const double* kphi, const double* kth, const SaddleDims& d,

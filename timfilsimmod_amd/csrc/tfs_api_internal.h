@@ -78,8 +78,9 @@ void invalidate_near(tfs_rpy* h);
 
 // Matvec with explicit views: forces fs (seg_dof comps) / fb (3 comps) and
 // outputs vs (seg_out_slots comps) / vb (3 comps), with the write rules of
-// `blocks` and the handle's positions.
+// `blocks` and the handle's positions.  `sad` (may be null) folds the
+// saddle-point couplings into the reduce kernel.
 int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, View vs, View vb,
-               int seg_out_slots);
+               int seg_out_slots, const SaddleEpilogue* sad = nullptr);
 
 }  // namespace tfs

@@ -89,10 +89,27 @@ struct MatvecArgs {
     const int32_t* skip = nullptr;
 };
 
+// Saddle-point couplings of the prescribed-cilia operator
+// (mobility_solver.cpp:1806-2090), applied by the reduce kernel when `in` is
+// set: segment rows 3t..3t+2 (filament m = t / nseg) get
+// -(phi_dot_m K_phi + theta_dot_m K_theta), and extra workgroups (one wave per
+// filament) write out[off_phase + m] = -K_phi^T lambda_m and
+// out[off_angle + m] = -K_theta^T lambda_m.  Layout of in/out:
+// [lambda_s (3 Ns) | lambda_b (3 Nb) | phi_dot (Nfil) | theta_dot (Nfil)].
+struct SaddleEpilogue {
+    const double* in = nullptr;
+    double* out = nullptr;
+    const double* kphi = nullptr;
+    const double* kth = nullptr;
+    int nfil = 0, nseg = 0, dyn_phase = 0, dyn_angle = 0;
+    int64_t off_phase = 0, off_angle = 0;
+};
+
 struct OutArgs {
     View vs, vb;       // outputs; vs has seg_out_slots components
     int seg_out_slots; // 6 in the reference layout, 3 for a packed 3-dof vector
     uint32_t blocks;   // active blocks (bit 0 SS, 1 SB, 2 BB, 3 BS) for the near-pair terms
+    SaddleEpilogue sad;
 };
 
 // Builds the work list for one evaluation of independent bodies.

@@ -205,7 +205,7 @@ void free_plans(tfs_rpy* h) {
 }
 
 int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, View vs, View vb,
-               int seg_out_slots) {
+               int seg_out_slots, const SaddleEpilogue* sad) {
     const TileShape& sh = h->shape[precision];
     int err = TFS_OK;
     Plan* p = get_plan(h, blocks, sh, &err);
@@ -219,6 +219,7 @@ int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, V
     o.vb = vb;
     o.seg_out_slots = seg_out_slots;
     o.blocks = p->blocks;
+    if (sad) o.sad = *sad;
     hipEvent_t eb = nullptr, ee = nullptr;
     if (h->profiling && !p->items.empty()) {
         if (h->ev_free.empty()) {

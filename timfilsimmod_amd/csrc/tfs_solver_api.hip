@@ -102,16 +102,31 @@ int norm(tfs_solver* s, const double* v, double* out) {
 int apply_A(tfs_solver* s, const double* in, double* out, int precision, tfs_solve_stats* st,
             const int32_t* skip = nullptr) {
     const SaddleDims& dm = s->dims;
+    // The K_phi / K_theta couplings are applied by the matvec's reduce kernel.
+    SaddleEpilogue sad;
+    const bool couplings = dm.nfil > 0 && (dm.dyn_phase || dm.dyn_angle);
+    if (couplings) {
+        sad.in = in;
+        sad.out = out;
+        sad.kphi = s->kphi;
+        sad.kth = s->kth;
+        sad.nfil = dm.nfil;
+        sad.nseg = dm.nseg;
+        sad.dyn_phase = dm.dyn_phase;
+        sad.dyn_angle = dm.dyn_angle;
+        sad.off_phase = dm.off_phase;
+        sad.off_angle = dm.off_angle;
+    }
     s->rpy->skip_flag = skip;
     int rc = run_matvec(s->rpy, TFS_BLOCK_ALL, precision, CView{in, 3, 1}, CView{in + dm.off_blob, 3, 1},
-                        View{out, 3, 1}, View{out + dm.off_blob, 3, 1}, 3);
+                        View{out, 3, 1}, View{out + dm.off_blob, 3, 1}, 3, couplings ? &sad : nullptr);
     s->rpy->skip_flag = nullptr;
     if (rc) return rc;
     if (st) {
         if (precision == TFS_PRECISION_FP64) st->fp64_matvecs++;
         else st->fp32_matvecs++;
     }
-    return check(k_saddle(in, out, s->kphi, s->kth, dm, s->stream, skip), "saddle couplings");
+    return TFS_OK;
 }
 
 int apply_P(tfs_solver* s, const double* in, double* out) {
@@ -170,13 +185,16 @@ int gmres(tfs_solver* s, int precision, const double* b, double* x, double* r, d
         while (enq < m && !seen) {
             const int j = enq;
             double* vj = s->V + (int64_t)j * s->ldv;
-            // v_j = w_{j-1} / ||w_{j-1}|| (v_0 = r / ||r||), z = P^{-1} v_j
+            // v_j = w_{j-1} / ||w_{j-1}|| (v_0 = r / ||r||), z = P^{-1} v_j; for
+            // j > 0 the same launch closes step j - 1.
             if (j == 0)
                 rc = check(k_scale_precond(r, nullptr, 1.0 / rnorm, vj, s->z, s->kphi, s->kth, s->dims, s->gs, s->gb,
                                            nullptr, s->stream), "v0");
             else
-                rc = check(k_scale_precond(s->w, s->scal, 0.0, vj, s->z, s->kphi, s->kth, s->dims, s->gs, s->gb,
-                                           s->stop, s->stream), "normalise");
+                rc = check(k_arnoldi_close_next(s->pa, s->GA, s->H, ldh, j - 1, s->cs, s->sn, s->g, s->scal, s->est,
+                                                s->stop, abs_tol, s->w, vj, s->z, s->kphi, s->kth, s->dims, s->gs,
+                                                s->gb, s->stream),
+                           "close and normalise");
             if (!rc) rc = apply_A(s, s->z, s->w, precision, nullptr, s->stop);
             if (!rc)
                 rc = check(k_arnoldi_step(s->V, s->ldv, j, s->w, n, s->pa, s->pb, s->GA, s->H, ldh, s->cs, s->sn,
@@ -196,6 +214,12 @@ int gmres(tfs_solver* s, int precision, const double* b, double* x, double* r, d
             if (e != hipSuccess) return hip_fail(e, "GMRES stop flag");
             ++batches;
         }
+        // close the last enqueued step
+        rc = check(k_arnoldi_close_next(s->pa, s->GA, s->H, ldh, enq - 1, s->cs, s->sn, s->g, s->scal, s->est, s->stop,
+                                        abs_tol, s->w, nullptr, nullptr, s->kphi, s->kth, s->dims, s->gs, s->gb,
+                                        s->stream),
+                   "close");
+        if (rc) return rc;
         int32_t flag = 0;
         double est = rnorm;
         {
