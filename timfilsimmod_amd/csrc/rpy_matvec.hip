@@ -26,11 +26,14 @@
 //
 // Two evaluation types:
 //   double : the FP64 path (parity with the FP64 oracle at <= 1e-12).
-//   float  : pair tensors in FP32 from double-float positions (x = hi + lo,
-//            so near pairs -- the ones that dominate -- get an exact hi
-//            difference, by Sterbenz's lemma); per-batch FP32 partial sums are
-//            flushed into FP64 accumulators after every LDS batch.
+//   float  : pair tensors in FP32 from fixed-point positions (per body,
+//            X = rint(2^k (x - centre)) with |X| < 2^30): the separation is an
+//            exact integer difference rounded once to FP32, so near pairs --
+//            the ones that dominate -- lose nothing to cancellation, in two
+//            full-rate instructions per component; per-batch FP32 partial sums
+//            are flushed into FP64 accumulators after every LDS batch.
 #include <algorithm>
+#include <type_traits>
 
 #include "rpy_pair.h"
 #include "tfs_internal.h"
@@ -42,16 +45,16 @@ constexpr int kMaxDof = 6;
 
 // LDS record length (in T) of one staged source with FS strength components
 // for targets with VT velocity components.  FP64: x y z f0..f(FS-1) [pad].
-// FP32: xh yh zh xl yl zl f0..f(FS-1), then the negated torque (FS = 6) and
-// the negated force (VT = 6), padded to 16 B: with the negations staged, every
-// cross-product term of the pair loop is an accumulate (the double-rate VOP2
-// form on gfx950) instead of a negated fma.
+// FP32: X Y Z (fixed-point integers, stored as their bits) f0..f(FS-1), then
+// the negated torque (FS = 6) and the negated force (VT = 6), padded to 16 B:
+// with the negations staged, every cross-product term of the pair loop is an
+// accumulate (the double-rate VOP2 form on gfx950) instead of a negated fma.
 template <typename T, int FS, int VT = 3>
 constexpr int rec_len() {
     if constexpr (sizeof(T) == 8) {
         return FS == 6 ? 10 : 6;
     } else {
-        return (6 + FS + (FS == 6 ? 3 : 0) + (VT == 6 ? 3 : 0) + 3) / 4 * 4;
+        return (3 + FS + (FS == 6 ? 3 : 0) + (VT == 6 ? 3 : 0) + 3) / 4 * 4;
     }
 }
 
@@ -62,55 +65,58 @@ constexpr int rec_len_max() {
 }
 
 // Separation and squared distance exactly as the pair loop evaluates them;
-// build_near_list calls the same function so the two agree bit for bit.
-template <typename T>
-__device__ __forceinline__ T separation(const double* xi, const T* xh, const T* xl, const T* rec, T& dx, T& dy,
-                                        T& dz) {
-    if constexpr (sizeof(T) == 4) {
-        dx = (xh[0] - rec[0]) + (xl[0] - rec[3]);
-        dy = (xh[1] - rec[1]) + (xl[1] - rec[4]);
-        dz = (xh[2] - rec[2]) + (xl[2] - rec[5]);
-    } else {
-        dx = xi[0] - rec[0];
-        dy = xi[1] - rec[1];
-        dz = xi[2] - rec[2];
-    }
-    if constexpr (sizeof(T) == 4) return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-    else return __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+// build_near_list calls the same functions so the two agree bit for bit.
+// FP64: d = x_i - x_j.  FP32: d = lambda (x_i - x_j) from the fixed-point
+// coordinates -- an exact integer difference (|X| < 2^30) rounded once to
+// FP32, two full-rate instructions per component.
+__device__ __forceinline__ double separation(const double* xi, const double* rec, double& dx, double& dy,
+                                             double& dz) {
+    dx = xi[0] - rec[0];
+    dy = xi[1] - rec[1];
+    dz = xi[2] - rec[2];
+    return __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+}
+__device__ __forceinline__ float separation(const int32_t* Xi, const float* rec, float& dx, float& dy, float& dz) {
+    dx = float(Xi[0] - __float_as_int(rec[0]));
+    dy = float(Xi[1] - __float_as_int(rec[1]));
+    dz = float(Xi[2] - __float_as_int(rec[2]));
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
-template <typename T>
-__device__ __forceinline__ void split_position(double x, T& h, T& l) {
-    if constexpr (sizeof(T) == 4) {
-        h = T(x);
-        l = T(x - double(h));
-    } else {
-        h = T(x);
-        l = T(0);
-    }
+// Target coordinates in the evaluation type's form.
+__device__ __forceinline__ void load_target(const MatvecArgs& a, int cls, int t, double* xi) {
+    const CView x = cls == 0 ? a.xs : a.xb;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) xi[k] = x.p[(int64_t)t * x.ld_p + k * x.ld_c];
+}
+__device__ __forceinline__ void load_target(const MatvecArgs& a, int cls, int t, int32_t* Xi) {
+    const int4 q = a.xfx[(cls == 0 ? 0 : a.ns) + t];
+    Xi[0] = q.x;
+    Xi[1] = q.y;
+    Xi[2] = q.z;
 }
 
-// One source into its LDS record: position (double-float split for FP32) and,
-// if with_force, its strengths multiplied by `scale`.
+// One source of class src_cls into its LDS record: its position and, if
+// with_force, its strengths -- forces multiplied by fscale, torques by
+// tscale.
 template <typename T, int FS, int VT = 3>
-__device__ __forceinline__ void stage_source(T* rec, const CView& xsrc, const CView& fsrc, int s, bool with_force,
-                                             double scale = 1.0) {
-    const double p0 = xsrc.p[(int64_t)s * xsrc.ld_p];
-    const double p1 = xsrc.p[(int64_t)s * xsrc.ld_p + xsrc.ld_c];
-    const double p2 = xsrc.p[(int64_t)s * xsrc.ld_p + 2 * xsrc.ld_c];
-    constexpr int F0 = sizeof(T) == 4 ? 6 : 3;
+__device__ __forceinline__ void stage_source(T* rec, const MatvecArgs& a, int src_cls, int s, bool with_force,
+                                             double fscale = 1.0, double tscale = 1.0) {
+    constexpr int F0 = 3;
     if constexpr (sizeof(T) == 4) {
-        split_position(p0, rec[0], rec[3]);
-        split_position(p1, rec[1], rec[4]);
-        split_position(p2, rec[2], rec[5]);
+        const int4 q = a.xfx[(src_cls == 0 ? 0 : a.ns) + s];
+        rec[0] = __int_as_float(q.x);
+        rec[1] = __int_as_float(q.y);
+        rec[2] = __int_as_float(q.z);
     } else {
-        rec[0] = p0;
-        rec[1] = p1;
-        rec[2] = p2;
+        const CView x = src_cls == 0 ? a.xs : a.xb;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) rec[k] = x.p[(int64_t)s * x.ld_p + k * x.ld_c];
     }
     if (with_force) {
+        const CView f = src_cls == 0 ? a.fs : a.fb;
 #pragma unroll
-        for (int k = 0; k < FS; ++k) rec[F0 + k] = T(scale * fsrc.p[(int64_t)s * fsrc.ld_p + k * fsrc.ld_c]);
+        for (int k = 0; k < FS; ++k) rec[F0 + k] = T((k < 3 ? fscale : tscale) * f.p[(int64_t)s * f.ld_p + k * f.ld_c]);
         if constexpr (sizeof(T) == 4) {
             int o = F0 + FS;
             if constexpr (FS == 6) {
@@ -126,44 +132,43 @@ __device__ __forceinline__ void stage_source(T* rec, const CView& xsrc, const CV
     }
 }
 
+// The pairs of one work item.  FP32 tiles work in the fixed-point frame
+// d' = lambda d (lambda = 2^k, exact): with forces staged as lambda c8 F and
+// torques as lambda^2 c8 tau, the translational sums come out in physical
+// units and the rotational ones divided by lambda, which the FP64 flush
+// multiplies back.
 template <int VT, int FS, typename T, int BLOCK, int R>
 __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile tl, const MatvecArgs& a,
                                          T* __restrict__ slot, T* __restrict__ lds) {
     constexpr int REC = rec_len<T, FS, VT>();
     constexpr int TILE = BLOCK * R;
-    constexpr int F0 = sizeof(T) == 4 ? 6 : 3;
+    constexpr int F0 = 3;
+    constexpr bool F32 = sizeof(T) == 4;
     const int tid = threadIdx.x;
 
-    const CView xt = tl.cls == 0 ? a.xs : a.xb;
-    const CView xsrc = it.src_cls == 0 ? a.xs : a.xb;
-    const CView fsrc = it.src_cls == 0 ? a.fs : a.fb;
     const double ai = tl.cls == 0 ? a.a_seg : a.a_blob;
     const double aj = it.src_cls == 0 ? a.a_seg : a.a_blob;
     const double PI = 3.14159265358979323846;
-    const T sum_a2 = T((ai + aj) * (ai + aj));
-    const T s3 = T((ai * ai + aj * aj) / 3.0);
+    const double lam = F32 ? a.lam : 1.0;
+    const T sum_a2 = T(lam * lam * ((ai + aj) * (ai + aj)));
+    const T s3 = T(lam * lam * ((ai * ai + aj * aj) / 3.0));
     // -3 s3 as its own value (a negated operand would force the VOP3 form of the fma)
     const T m3s3 = T(-3) * s3;
     const double c8 = 1.0 / (8.0 * PI * a.mu);  // folded into the staged strengths
 
     // Targets of this lane (clamped into the tile; surplus lanes compute a
     // duplicate and their slot entries are never read).
-    double xi[R][3];
-    T xh[R][3], xl[R][3];
+    using XT = typename std::conditional<F32, int32_t, double>::type;
+    XT xi[R][3];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int t = tl.t_begin + tid + r * BLOCK;
         t = t < tl.t_end ? t : tl.t_end - 1;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            xi[r][k] = xt.p[(int64_t)t * xt.ld_p + k * xt.ld_c];
-            split_position(xi[r][k], xh[r][k], xl[r][k]);
-        }
+        load_target(a, tl.cls, t, xi[r]);
     }
 
     // FP64 accumulates straight into acc; FP32 sums one LDS batch in `part`
     // and flushes it into the FP64 accumulators.
-    constexpr bool F32 = sizeof(T) == 4;
     double acc[R][VT];
     T part[R][F32 ? VT : 1];
 #pragma unroll
@@ -174,7 +179,7 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
     for (int s0 = it.s_begin; s0 < it.s_end; s0 += BLOCK) {
         const int n = (it.s_end - s0) < BLOCK ? (it.s_end - s0) : BLOCK;
         __syncthreads();  // the previous batch has been consumed
-        if (tid < n) stage_source<T, FS, VT>(lds + tid * REC, xsrc, fsrc, s0 + tid, true, c8);
+        if (tid < n) stage_source<T, FS, VT>(lds + tid * REC, a, it.src_cls, s0 + tid, true, lam * c8, lam * lam * c8);
         __syncthreads();
         if constexpr (F32) {
 #pragma unroll
@@ -192,7 +197,7 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 T dx, dy, dz;
-                const T r2 = separation<T>(xi[r], xh[r], xl[r], rec, dx, dy, dz);
+                const T r2 = separation(xi[r], rec, dx, dy, dz);
                 // Overlapping pairs (and the self pair) are evaluated here with r
                 // clamped to ai + aj -- one max instead of a compare and select --
                 // and the reduce kernel swaps that value for the exact one.
@@ -215,7 +220,7 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
-        for (int k = 0; k < VT; ++k) slot[k * TILE + tid + r * BLOCK] = T(acc[r][k]);
+        for (int k = 0; k < VT; ++k) slot[k * TILE + tid + r * BLOCK] = T(k < 3 ? acc[r][k] : lam * acc[r][k]);
 }
 
 template <typename T, int SEGDOF, int BLOCK, int R>
@@ -276,35 +281,32 @@ __global__ __launch_bounds__(256) void near_build_kernel(const MatvecItem* __res
                                                          int32_t* __restrict__ cnt, int32_t* __restrict__ idx,
                                                          int cap, int32_t* __restrict__ max_cnt) {
     constexpr int REC = rec_len<T, 3>();
+    constexpr bool F32 = sizeof(T) == 4;
     __shared__ __align__(16) T lds[256 * REC];
     const MatvecItem it = items[blockIdx.x];
     const MatvecTile tl = tiles[it.tile];
     const int tid = threadIdx.x;
-    const CView xt = tl.cls == 0 ? a.xs : a.xb;
-    const CView xsrc = it.src_cls == 0 ? a.xs : a.xb;
     const double ai = tl.cls == 0 ? a.a_seg : a.a_blob;
     const double aj = it.src_cls == 0 ? a.a_seg : a.a_blob;
-    const T sum_a2 = T((ai + aj) * (ai + aj));
+    const double lam = F32 ? a.lam : 1.0;
+    const T sum_a2 = T(lam * lam * ((ai + aj) * (ai + aj)));
     const int t_local = tl.t_begin + tid;
     const bool active = t_local < tl.t_end;
     const int t = active ? t_local : tl.t_end - 1;
-    double xi[3];
-    T xh[3], xl[3];
-    for (int k = 0; k < 3; ++k) {
-        xi[k] = xt.p[(int64_t)t * xt.ld_p + k * xt.ld_c];
-        split_position(xi[k], xh[k], xl[k]);
-    }
+    using XT = typename std::conditional<F32, int32_t, double>::type;
+    XT xi[3];
+    load_target(a, tl.cls, t, xi);
     const int tg = (tl.cls == 0 ? 0 : a.ns) + t;  // global target index
     const int src_base = it.src_cls == 0 ? 0 : a.ns;
     for (int s0 = it.s_begin; s0 < it.s_end; s0 += 256) {
         const int n = (it.s_end - s0) < 256 ? (it.s_end - s0) : 256;
         __syncthreads();
-        if (tid < n) stage_source<T, 3>(lds + tid * REC, xsrc, a.fb, s0 + tid, false);
+        if (tid < n) stage_source<T, 3>(lds + tid * REC, a, it.src_cls, s0 + tid, false);
         __syncthreads();
         if (!active) continue;
         for (int j = 0; j < n; ++j) {
             T dx, dy, dz;
-            const T r2 = separation<T>(xi, xh, xl, lds + j * REC, dx, dy, dz);
+            const T r2 = separation(xi, lds + j * REC, dx, dy, dz);
             if (!(r2 > sum_a2)) {
                 const int k = atomicAdd(&cnt[tg], 1);
                 if (k < cap) idx[(int64_t)tg * cap + k] = src_base + s0 + j;

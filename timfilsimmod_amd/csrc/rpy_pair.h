@@ -100,17 +100,26 @@ __device__ __forceinline__ void pair_far_scaled(T* __restrict__ v, T dx, T dy, T
                                                 T m3s3) {
     // Written so that every operation is a two-source multiply or an
     // accumulate into a dying value (VOP2 forms, the double-rate FP32 issue on
-    // gfx950): A accumulates into 1/r, B into 1/r^3.
+    // gfx950): A accumulates into 1/r, B into 1/r^3.  FP32 callers pass
+    // d, 1/r, s3 in a frame scaled by up to 2^31 (fixed-point coordinates),
+    // where 1/r^5 would underflow: there B = r^-3 + (m3s3 r^-2) r^-3, and the
+    // rotation-rotation term multiplies (tau.d) r^-2 before r^-3.
+    constexpr bool F32 = sizeof(T) == 4;
     const T rm2 = rm1 * rm1;
     const T rm3 = rm1 * rm2;
-    const T rm5 = rm3 * rm2;
     const T A = fma_t(s3, rm3, rm1);
-    const T B = fma_t(m3s3, rm5, rm3);
+    T B;
+    if constexpr (F32) {
+        const T u = m3s3 * rm2;
+        B = fma_t(u, rm3, rm3);
+    } else {
+        const T rm5 = rm3 * rm2;
+        B = fma_t(m3s3, rm5, rm3);
+    }
     const T Bd = B * fma_t(dz, f[2], fma_t(dy, f[1], dx * f[0]));
     v[0] = fma_t(Bd, dx, fma_t(A, f[0], v[0]));
     v[1] = fma_t(Bd, dy, fma_t(A, f[1], v[1]));
     v[2] = fma_t(Bd, dz, fma_t(A, f[2], v[2]));
-    constexpr bool F32 = sizeof(T) == 4;
     if constexpr (FS == 6) {
         // translation-rotation: v += C (tau x d)
         if constexpr (F32) {
@@ -138,7 +147,7 @@ __device__ __forceinline__ void pair_far_scaled(T* __restrict__ v, T dx, T dy, T
         if constexpr (FS == 6) {
             // rotation-rotation: w += -C/2 tau + (3C/2) (tau.d) d / r^2
             const T Ar = T(-0.5) * rm3;
-            const T Br = T(1.5) * rm3 * rm2 * fma_t(dz, f[5], fma_t(dy, f[4], dx * f[3]));
+            const T Br = T(1.5) * rm3 * (rm2 * fma_t(dz, f[5], fma_t(dy, f[4], dx * f[3])));
             v[3] = fma_t(Br, dx, fma_t(Ar, f[3], v[3]));
             v[4] = fma_t(Br, dy, fma_t(Ar, f[4], v[4]));
             v[5] = fma_t(Br, dz, fma_t(Ar, f[5], v[5]));

@@ -45,6 +45,15 @@ struct tfs_rpy {
     std::vector<tfs::BodyRange> bodies;
     tfs::NearList near[2];
     bool near_valid[2] = {false, false};
+    // Fixed-point positions for FP32 tiles (valid with the near lists):
+    // per-body centres c_b (bounding-box centres) and one lambda = 2^k.
+    int4* x_fx = nullptr;
+    double fx_lambda = 1.0;
+    bool fx_valid = false;
+    double* d_body_box = nullptr;    // 6 per body: min xyz, max xyz
+    int32_t* d_body_range = nullptr; // 4 per body (BodyRange), for the box and conversion kernels
+    size_t body_cap = 0;
+    std::vector<double> fx_centre;   // 3 per body
     int32_t* host_int = nullptr;  // pinned scratch
     // Persistent launches with timing: per-item stamps of the last launch and
     // the plan they belong to.

@@ -87,6 +87,10 @@ struct MatvecArgs {
     // Device stop flag (may be null): when nonzero the pair and reduce kernels
     // return at once (iterations a Krylov solver enqueued past convergence).
     const int32_t* skip = nullptr;
+    // FP32 tiles: fixed-point positions X = rint(lambda (x - c_body)), segments
+    // then blobs (w unused), with lambda = 2^k and |X| < 2^30.
+    const int4* xfx = nullptr;
+    double lam = 1.0;
 };
 
 // Saddle-point couplings of the prescribed-cilia operator

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -108,7 +109,147 @@ Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err, bool 
     return &res.first->second;
 }
 
-void invalidate_near(tfs_rpy* h) { h->near_valid[0] = h->near_valid[1] = false; }
+void invalidate_near(tfs_rpy* h) {
+    h->near_valid[0] = h->near_valid[1] = false;
+    h->fx_valid = false;
+}
+
+namespace {
+
+// Bounding box of each body's segments and blobs (min xyz, max xyz); a
+// non-finite coordinate makes the box NaN.  One workgroup per body.
+__global__ __launch_bounds__(256) void body_box_kernel(const double* __restrict__ xs, const double* __restrict__ xb,
+                                                       const int4* __restrict__ ranges, double* __restrict__ box) {
+    __shared__ double red[6][256];
+    __shared__ int bad_any;
+    const int4 r = ranges[blockIdx.x];
+    const int tid = threadIdx.x;
+    if (tid == 0) bad_any = 0;
+    double lo[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+    double hi[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+    bool bad = false;
+    for (int i = r.x + tid; i < r.y; i += 256)
+        for (int k = 0; k < 3; ++k) {
+            const double v = xs[3 * (int64_t)i + k];
+            bad |= !__builtin_isfinite(v);
+            lo[k] = __builtin_fmin(lo[k], v);
+            hi[k] = __builtin_fmax(hi[k], v);
+        }
+    for (int i = r.z + tid; i < r.w; i += 256)
+        for (int k = 0; k < 3; ++k) {
+            const double v = xb[3 * (int64_t)i + k];
+            bad |= !__builtin_isfinite(v);
+            lo[k] = __builtin_fmin(lo[k], v);
+            hi[k] = __builtin_fmax(hi[k], v);
+        }
+    for (int k = 0; k < 3; ++k) {
+        red[k][tid] = lo[k];
+        red[3 + k][tid] = hi[k];
+    }
+    __syncthreads();
+    if (bad) bad_any = 1;
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w)
+            for (int k = 0; k < 3; ++k) {
+                red[k][tid] = __builtin_fmin(red[k][tid], red[k][tid + w]);
+                red[3 + k][tid] = __builtin_fmax(red[3 + k][tid], red[3 + k][tid + w]);
+            }
+        __syncthreads();
+    }
+    if (tid < 6) box[6 * blockIdx.x + tid] = bad_any ? __builtin_nan("") : red[tid][0];
+}
+
+// X = rint(lambda (x - c_body)) for every segment and blob of body blockIdx.x.
+__global__ __launch_bounds__(256) void to_fixed_kernel(const double* __restrict__ xs, const double* __restrict__ xb,
+                                                       const int4* __restrict__ ranges,
+                                                       const double* __restrict__ centre, double lam, int ns,
+                                                       int4* __restrict__ out) {
+    const int4 r = ranges[blockIdx.x];
+    const double c0 = centre[3 * blockIdx.x], c1 = centre[3 * blockIdx.x + 1], c2 = centre[3 * blockIdx.x + 2];
+    for (int i = r.x + threadIdx.x; i < r.y; i += 256) {
+        const double* x = xs + 3 * (int64_t)i;
+        out[i] = make_int4((int)__builtin_rint((x[0] - c0) * lam), (int)__builtin_rint((x[1] - c1) * lam),
+                           (int)__builtin_rint((x[2] - c2) * lam), 0);
+    }
+    for (int i = r.z + threadIdx.x; i < r.w; i += 256) {
+        const double* x = xb + 3 * (int64_t)i;
+        out[ns + i] = make_int4((int)__builtin_rint((x[0] - c0) * lam), (int)__builtin_rint((x[1] - c1) * lam),
+                                (int)__builtin_rint((x[2] - c2) * lam), 0);
+    }
+}
+
+}  // namespace
+
+// Fixed-point coordinates for FP32 tiles.  Each body is centred on its
+// bounding box; lambda = 2^k is the largest power of two with
+// lambda * (largest half-extent) <= 2^30 - 2, so every coordinate and every
+// difference of two fits an int32.  The quantisation step 1/lambda must stay
+// below 2^-19 of the smallest radius (a body within ~1024 radii of its
+// centre); larger bodies are refused in FP32.  Synchronises the stream.
+int ensure_fixed(tfs_rpy* h) {
+    if (h->fx_valid) return TFS_OK;
+    const int ns = h->desc.num_segs, nb = h->desc.num_blobs;
+    const size_t nbody = h->bodies.size();
+    hipError_t e = hipSuccess;
+    if (!h->x_fx) e = hipMalloc(&h->x_fx, sizeof(int4) * (size_t)std::max(ns + nb, 1));
+    if (e == hipSuccess && h->body_cap < nbody) {
+        if (h->d_body_box) (void)hipFree(h->d_body_box);
+        if (h->d_body_range) (void)hipFree(h->d_body_range);
+        h->d_body_box = nullptr;
+        h->d_body_range = nullptr;
+        e = hipMalloc(&h->d_body_box, sizeof(double) * 6 * nbody);
+        if (e == hipSuccess) e = hipMalloc(&h->d_body_range, sizeof(int4) * nbody);
+        h->body_cap = e == hipSuccess ? nbody : 0;
+    }
+    if (e != hipSuccess) return fail(TFS_ERR_OUT_OF_MEMORY, "fixed-point positions");
+    std::vector<double> box(6 * nbody), centre(3 * nbody, 0.0);
+    e = hipMemcpyAsync(h->d_body_range, h->bodies.data(), sizeof(int4) * nbody, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(body_box_kernel, dim3((unsigned)nbody), dim3(256), 0, h->stream, h->x_segs, h->x_blobs,
+                           (const int4*)h->d_body_range, h->d_body_box);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(box.data(), h->d_body_box, sizeof(double) * 6 * nbody, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(e, "fixed-point positions: bounding boxes");
+    double ext = 0.0;
+    for (size_t b = 0; b < nbody; ++b) {
+        for (int k = 0; k < 3; ++k) {
+            const double lo = box[6 * b + k], hi = box[6 * b + 3 + k];
+            if (std::isnan(lo) || std::isnan(hi)) return fail(TFS_ERR_NON_FINITE, "positions are not finite");
+            if (!(lo <= hi)) continue;  // body without particles
+            const double c = 0.5 * (lo + hi);
+            centre[3 * b + k] = c;
+            ext = std::max(ext, std::max(hi - c, c - lo));
+        }
+    }
+    const double lim = 1073741822.0;  // 2^30 - 2
+    int k = 60;
+    if (ext > 0.0) {
+        k = (int)std::floor(std::log2(lim / ext));
+        while (k > -1000 && std::ldexp(ext, k) > lim) --k;
+        while (k < 60 && std::ldexp(ext, k + 1) <= lim) ++k;
+        k = std::min(k, 60);
+    }
+    const double amin = std::min(h->desc.seg_radius, h->desc.blob_radius);
+    if (std::ldexp(1.0, -k) > std::ldexp(amin, -19))
+        return fail(TFS_ERR_INVALID_ARGUMENT,
+                    "FP32 tiles: a body extends more than about 1024 particle radii from its centre (use FP64)");
+    h->fx_lambda = std::ldexp(1.0, k);
+    h->fx_centre = centre;
+    e = hipMemcpyAsync(h->d_body_box, h->fx_centre.data(), sizeof(double) * 3 * nbody, hipMemcpyHostToDevice,
+                       h->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(to_fixed_kernel, dim3((unsigned)nbody), dim3(256), 0, h->stream, h->x_segs, h->x_blobs,
+                           (const int4*)h->d_body_range, (const double*)h->d_body_box, h->fx_lambda, ns, h->x_fx);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(e, "fixed-point positions");
+    h->fx_valid = true;
+    return TFS_OK;
+}
 
 void free_near(NearList& nl) {
     hipFree(nl.cnt);
@@ -129,19 +270,22 @@ MatvecArgs matvec_args(tfs_rpy* h, CView fs, CView fb) {
     a.a_blob = h->desc.blob_radius;
     a.mu = h->desc.mu;
     a.skip = h->skip_flag;
+    a.xfx = h->x_fx;
+    a.lam = h->fx_lambda;
     return a;
 }
 
 // Lists the overlapping pairs of the current positions for `precision`,
 // growing the per-target capacity until every list fits.
 int ensure_near(tfs_rpy* h, int precision) {
-    if (h->near_valid[precision]) return TFS_OK;
+    if (h->near_valid[precision] && (precision == 0 || h->fx_valid)) return TFS_OK;
     const int nt = h->desc.num_segs + h->desc.num_blobs;
     if (nt == 0) {
         h->near_valid[precision] = true;
         return TFS_OK;
     }
-    int err = TFS_OK;
+    int err = precision == 1 ? ensure_fixed(h) : TFS_OK;
+    if (err) return err;
     const TileShape nsh{256, 1, 1024};
     Plan* p = get_plan(h, TFS_BLOCK_ALL, nsh, &err, true);
     if (!p) return err;
@@ -346,6 +490,9 @@ int tfs_rpy_destroy(tfs_rpy* h) {
     if (h->partial) hipFree(h->partial);
     if (h->arena) hipFree(h->arena);
     for (auto& nl : h->near) free_near(nl);
+    if (h->x_fx) hipFree(h->x_fx);
+    if (h->d_body_box) hipFree(h->d_body_box);
+    if (h->d_body_range) hipFree(h->d_body_range);
     if (h->host_int) hipHostFree(h->host_int);
     if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
     delete h;
