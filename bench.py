@@ -209,6 +209,43 @@ def ensemble_leg(torch, stream, steps: int = 5):
     return out
 
 
+def copy_leg(torch, nbytes: int = 256 << 20, reps: int = 3):
+    """A plain device-to-device copy of `nbytes` (torch's copy kernel), `reps`
+    times: the reference point that scales the FETCH_SIZE / WRITE_SIZE
+    counters to bytes."""
+    dev = torch.device("cuda", 0)
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GB_s": 2 * nbytes / (ms * 1e-3) / 1e9}
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_rpy_pairs_float3_production_body.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the FP32 <3,3> pair kernel, as recorded from the
+    two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; scaled by the copy
+    calibration) in the committed profile summary; None if it is absent."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(body, budget_s: float = 15.0):
     """FP64 oracle direct sum on the host cores (reported, not gating): the
     prescribed-beat <3,3> matvec of the production body, timed."""
@@ -233,7 +270,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--matvec-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--legs", default="timestep,matvec,flow,ensemble",
+                    help="comma list of timestep, matvec, flow, ensemble, copy (copy: a 256 MiB device "
+                         "copy for calibrating HBM byte counters; not part of the default run)")
     args = ap.parse_args()
+    legs = set(args.legs.split(","))
+    unknown = legs - {"timestep", "matvec", "flow", "ensemble", "copy"}
+    if unknown:
+        print(f"bench.py: unknown legs {sorted(unknown)}", file=sys.stderr)
+        sys.exit(2)
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
               "(multi-GPU target sharding is designed, not built)", file=sys.stderr)
@@ -255,17 +300,24 @@ def main():
     body = G.production_body()
     ns, nb = body.num_segs, body.num_blobs
 
+    copy_cal = None
+    if "copy" in legs:
+        copy_cal = copy_leg(torch)
+
     # ---- timestep legs (headline) ------------------------------------------
-    print("bench: timestep leg, mixed precision", file=sys.stderr, flush=True)
-    mixed = timestep_leg(torch, body, SOLVE_MIXED, args.steps, args.warmup, stream)
-    print("bench: timestep leg, FP64", file=sys.stderr, flush=True)
-    fp64 = timestep_leg(torch, body, SOLVE_FP64, args.steps, args.warmup, stream)
-    ts_mixed = 1000.0 / mixed["ms_per_step"]
-    ts_fp64 = 1000.0 / fp64["ms_per_step"]
-    dev = float(np.abs(mixed["state"]["x_segs"] - fp64["state"]["x_segs"]).max() / DL)
-    sol_err = float(np.linalg.norm(mixed["state"]["solution"] - fp64["state"]["solution"]) /
-                    np.linalg.norm(fp64["state"]["solution"]))
-    nsys = mixed["state"]["solution"].size
+    mixed = fp64 = None
+    ts_mixed = ts_fp64 = dev = sol_err = nsys = None
+    if "timestep" in legs:
+        print("bench: timestep leg, mixed precision", file=sys.stderr, flush=True)
+        mixed = timestep_leg(torch, body, SOLVE_MIXED, args.steps, args.warmup, stream)
+        print("bench: timestep leg, FP64", file=sys.stderr, flush=True)
+        fp64 = timestep_leg(torch, body, SOLVE_FP64, args.steps, args.warmup, stream)
+        ts_mixed = 1000.0 / mixed["ms_per_step"]
+        ts_fp64 = 1000.0 / fp64["ms_per_step"]
+        dev = float(np.abs(mixed["state"]["x_segs"] - fp64["state"]["x_segs"]).max() / DL)
+        sol_err = float(np.linalg.norm(mixed["state"]["solution"] - fp64["state"]["solution"]) /
+                        np.linalg.norm(fp64["state"]["solution"]))
+        nsys = mixed["state"]["solution"].size
 
     def summ(leg):
         st = leg["stats"]
@@ -278,51 +330,69 @@ def main():
                 "final_relres_max": float(max(s["relres"] for s in st)),
                 "converged_all": bool(all(s["converged"] for s in st))}
 
-    # Dominant kernel of the headline: the FP32-tile <3,3> pair kernel.
+    # Dominant kernel of the headline: the FP32-tile <3,3> pair kernel.  The
+    # summed event time covers every launch, including the few the solver
+    # enqueued past convergence (they exit at once); the work is counted only
+    # for the matvecs the solver used (its own counters).
     flops3 = matvec_flops(ns, nb, 3)
-    k32_ms, k32_n = mixed["kernels"][L.PRECISION_FP32_ACC64]
-    k64_ms, k64_n = fp64["kernels"][L.PRECISION_FP64]
-    tf32 = flops3 * k32_n / (k32_ms * 1e-3) / 1e12 if k32_n else 0.0
-    tf64 = flops3 * k64_n / (k64_ms * 1e-3) / 1e12 if k64_n else 0.0
+    k32_ms = k32_n = k64_ms = k64_n = 0
+    used32 = used64 = 0
+    if mixed is not None:
+        k32_ms, k32_n = mixed["kernels"][L.PRECISION_FP32_ACC64]
+        k64_ms, k64_n = fp64["kernels"][L.PRECISION_FP64]
+        used32 = int(sum(s["fp32_matvecs"] for s in mixed["stats"]))
+        used64 = int(sum(s["fp64_matvecs"] for s in fp64["stats"]))
+    tf32 = flops3 * used32 / (k32_ms * 1e-3) / 1e12 if used32 else 0.0
+    tf64 = flops3 * used64 / (k64_ms * 1e-3) / 1e12 if used64 else 0.0
 
     # ---- 6-DOF matvec legs ---------------------------------------------------
-    print("bench: 6-DOF matvec legs", file=sys.stderr, flush=True)
-    h = RpyHandle(ns, nb, seg_dof=6, precision=L.PRECISION_FP64, bs_accumulate=True, stream=stream)
-    h.upload(L.SEG_POSITIONS, body.x_segs)
-    h.upload(L.BLOB_POSITIONS, body.x_blobs)
-    h.upload(L.SEG_FORCES, body.f_segs)
-    h.upload(L.BLOB_FORCES, body.f_blobs)
-    h.sync()
-    mv64 = matvec_leg(torch, h, L.PRECISION_FP64, args.matvec_steps, 3)
-    v64s, v64b = np.zeros(6 * ns), np.zeros(3 * nb)
-    h.download(L.SEG_VELOCITIES, v64s)
-    h.download(L.BLOB_VELOCITIES, v64b)
-    h.sync()
-    mv32 = matvec_leg(torch, h, L.PRECISION_FP32_ACC64, args.matvec_steps, 3)
-    v32s, v32b = np.zeros(6 * ns), np.zeros(3 * nb)
-    h.download(L.SEG_VELOCITIES, v32s)
-    h.download(L.BLOB_VELOCITIES, v32b)
-    h.sync()
-    h.close()
-    v64 = np.concatenate([v64s, v64b])
-    mv_err = float(np.linalg.norm(np.concatenate([v32s, v32b]) - v64) / np.linalg.norm(v64))
     flops6 = matvec_flops(ns, nb, 6)
     bytes6 = matvec_bytes(ns, nb, 6)
+    matvec = None
+    if "matvec" in legs:
+        print("bench: 6-DOF matvec legs", file=sys.stderr, flush=True)
+        h = RpyHandle(ns, nb, seg_dof=6, precision=L.PRECISION_FP64, bs_accumulate=True, stream=stream)
+        h.upload(L.SEG_POSITIONS, body.x_segs)
+        h.upload(L.BLOB_POSITIONS, body.x_blobs)
+        h.upload(L.SEG_FORCES, body.f_segs)
+        h.upload(L.BLOB_FORCES, body.f_blobs)
+        h.sync()
+        mv64 = matvec_leg(torch, h, L.PRECISION_FP64, args.matvec_steps, 3)
+        v64s, v64b = np.zeros(6 * ns), np.zeros(3 * nb)
+        h.download(L.SEG_VELOCITIES, v64s)
+        h.download(L.BLOB_VELOCITIES, v64b)
+        h.sync()
+        mv32 = matvec_leg(torch, h, L.PRECISION_FP32_ACC64, args.matvec_steps, 3)
+        v32s, v32b = np.zeros(6 * ns), np.zeros(3 * nb)
+        h.download(L.SEG_VELOCITIES, v32s)
+        h.download(L.BLOB_VELOCITIES, v32b)
+        h.sync()
+        h.close()
+        v64 = np.concatenate([v64s, v64b])
+        mv_err = float(np.linalg.norm(np.concatenate([v32s, v32b]) - v64) / np.linalg.norm(v64))
 
-    def mv(leg, peak):
-        t = leg["pair_kernel_ms"] * 1e-3
-        return {"ms": leg["ms_per_step"], "GB_s": bytes6 / (leg["ms_per_step"] * 1e-3) / 1e9,
-                "pair_kernel_ms": leg["pair_kernel_ms"], "TFLOP_s": flops6 / t / 1e12,
-                "valu_frac": flops6 / t / 1e12 / peak}
+        def mv(leg, peak):
+            t = leg["pair_kernel_ms"] * 1e-3
+            return {"ms": leg["ms_per_step"], "GB_s": bytes6 / (leg["ms_per_step"] * 1e-3) / 1e9,
+                    "pair_kernel_ms": leg["pair_kernel_ms"], "TFLOP_s": flops6 / t / 1e12,
+                    "valu_frac": flops6 / t / 1e12 / peak}
 
-    print("bench: flow-field leg", file=sys.stderr, flush=True)
-    flow = flow_leg(torch, stream)
-    torch.cuda.empty_cache()
-    print("bench: ensemble leg", file=sys.stderr, flush=True)
-    ens = ensemble_leg(torch, stream)
+        matvec = {"workload": "6-DOF grand mobility, dim 6*12800+3*9000",
+                  "fp64": mv(mv64, PEAK_FP64_VALU_TFLOPS), "mixed": mv(mv32, PEAK_FP32_VALU_TFLOPS),
+                  "mixed_rel_l2_err_vs_fp64": mv_err,
+                  "algorithmic_gflop": flops6 / 1e9, "compulsory_bytes": bytes6}
+
+    flow = ens = None
+    if "flow" in legs:
+        print("bench: flow-field leg", file=sys.stderr, flush=True)
+        flow = flow_leg(torch, stream)
+        torch.cuda.empty_cache()
+    if "ensemble" in legs:
+        print("bench: ensemble leg", file=sys.stderr, flush=True)
+        ens = ensemble_leg(torch, stream)
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and fp64 is not None:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         per_step = summ(fp64)["fp64_matvecs_per_step"]
         t_cpu, reps, cores = cpu_baseline(body)
@@ -339,7 +409,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": mixed["ms_per_step"],
+        "ms_per_step": None if mixed is None else mixed["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -349,26 +419,27 @@ def main():
                                "9,000 blobs, R=50a, dt=T/300, GMRES(50) to 1e-8, dynamic phase "
                                "and shape rotation",
                    "num_segs": ns, "num_blobs": nb, "system_size": nsys},
-        "timestep": {"mixed": summ(mixed), "fp64": summ(fp64),
+        "timestep": None if mixed is None else {
+                     "mixed": summ(mixed), "fp64": summ(fp64),
                      "speedup_mixed_over_fp64": ts_mixed / ts_fp64,
                      "trajectory_dev_over_DL_after_steps": dev,
                      "solution_rel_err_mixed_vs_fp64": sol_err,
                      "warmup_plus_timed_steps": args.warmup + args.steps},
-        "matvec": {"workload": "6-DOF grand mobility, dim 6*12800+3*9000",
-                   "fp64": mv(mv64, PEAK_FP64_VALU_TFLOPS), "mixed": mv(mv32, PEAK_FP32_VALU_TFLOPS),
-                   "mixed_rel_l2_err_vs_fp64": mv_err,
-                   "algorithmic_gflop": flops6 / 1e9, "compulsory_bytes": bytes6},
+        "matvec": matvec,
         "flow_field": flow,
         "ensemble": ens,
         "roofline": {"bound": "valu", "achieved": tf32, "peak": PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
-                     "frac": tf32 / PEAK_FP32_VALU_TFLOPS, "traffic": None,
+                     "frac": tf32 / PEAK_FP32_VALU_TFLOPS, "traffic": measured_traffic(),
                      "kernel": "rpy_pairs_kernel<float,3> (timestep matvec, 17.11 GFLOP per launch)",
-                     "launches": k32_n, "avg_ms": k32_ms / max(k32_n, 1),
+                     "launches": k32_n, "matvecs_used": used32, "avg_ms": k32_ms / max(used32, 1),
                      "fp64_kernel": {"achieved": tf64, "peak": PEAK_FP64_VALU_TFLOPS,
                                      "frac": tf64 / PEAK_FP64_VALU_TFLOPS,
-                                     "avg_ms": k64_ms / max(k64_n, 1), "launches": k64_n}},
+                                     "avg_ms": k64_ms / max(used64, 1), "launches": k64_n,
+                                     "matvecs_used": used64}},
         "cpu_baseline": cpu,
     }
+    if copy_cal is not None:
+        out["copy_calibration"] = copy_cal
     print(json.dumps(out), flush=True)
 
 

@@ -151,13 +151,37 @@ __global__ __launch_bounds__(256) void scale_kernel(double* __restrict__ v, cons
 }
 
 // Back substitution H(0:k,0:k) y = g (mobility_solver.cpp:2421-2438).
-__global__ void trisolve_kernel(const double* __restrict__ H, int ldh, const double* __restrict__ g, int k,
-                                double* __restrict__ y) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Back substitution H(0:k, 0:k) y = g for the upper-triangular H (column
+// major, leading dimension ldh), by columns: once y_i is known, every earlier
+// right-hand side loses H(j, i) y_i in parallel.  With STAGE the triangle is
+// first copied into LDS (packed by columns, k(k+1)/2 entries) in one burst of
+// loads, so the k dependent steps read no global memory.
+template <bool STAGE>
+__global__ __launch_bounds__(256) void trisolve_kernel(const double* __restrict__ H, int ldh,
+                                                      const double* __restrict__ g, int k, double* __restrict__ y) {
+    extern __shared__ double tri[];  // STAGE: k(k+1)/2 entries; then k right-hand sides
+    __shared__ double yi;
+    double* rhs = tri + (STAGE ? (int64_t)k * (k + 1) / 2 : 0);
+    for (int j = threadIdx.x; j < k; j += 256) rhs[j] = g[j];
+    if (STAGE) {
+        // flat over the k x k square so that each thread's loads are independent
+#pragma unroll 8
+        for (int e = threadIdx.x; e < k * k; e += 256) {
+            const int i = e / k, j = e - i * k;
+            if (j <= i) tri[i * (i + 1) / 2 + j] = H[(int64_t)i * ldh + j];
+        }
+    }
+    __syncthreads();
     for (int i = k - 1; i >= 0; --i) {
-        double s = g[i];
-        for (int j = i + 1; j < k; ++j) s -= H[(int64_t)j * ldh + i] * y[j];
-        y[i] = s / H[(int64_t)i * ldh + i];
+        const double* col = STAGE ? tri + (int64_t)i * (i + 1) / 2 : H + (int64_t)i * ldh;
+        if (threadIdx.x == 0) {
+            yi = rhs[i] / col[i];
+            y[i] = yi;
+        }
+        __syncthreads();
+        const double v = yi;
+        for (int j = threadIdx.x; j < i; j += 256) rhs[j] -= col[j] * v;
+        __syncthreads();
     }
 }
 
@@ -754,7 +778,11 @@ hipError_t k_scale(double* v, const double* w, const double* scal, int idx, int 
     return hipGetLastError();
 }
 hipError_t k_trisolve(const double* H, int ldh, const double* g, int k, double* y, hipStream_t s) {
-    hipLaunchKernelGGL(trisolve_kernel, dim3(1), dim3(64), 0, s, H, ldh, g, k, y);
+    const size_t staged = sizeof(double) * ((size_t)k * (k + 1) / 2 + k);
+    if (staged <= 60 * 1024)
+        hipLaunchKernelGGL(trisolve_kernel<true>, dim3(1), dim3(256), staged, s, H, ldh, g, k, y);
+    else
+        hipLaunchKernelGGL(trisolve_kernel<false>, dim3(1), dim3(256), sizeof(double) * k, s, H, ldh, g, k, y);
     return hipGetLastError();
 }
 hipError_t k_axpby(double a, const double* x, double b, double* y, int n, hipStream_t s) {
