@@ -8,9 +8,11 @@ One step = one prescribed-beat timestep of that body: phase update, beat
 kinematics, right-hand side and the warm-started saddle-point solve
 (RPY <3,3> mobility + K_phi/K_theta couplings, n = 3*12,800 + 3*9,000 +
 2*640 = 66,680) to a relative residual of 1e-8 with GMRES(50) and FP64
-Arnoldi.  The headline is the mixed-precision form (FP32-tile matvecs inside
-iterative refinement, FP64 residuals outside); the all-FP64 form runs the
-same steps from the same state beside it.
+Arnoldi.  The initial guess of each solve is the quadratic extrapolation of the
+last three solutions (WARM_START).  The headline is the mixed-precision form
+(FP32-tile matvecs in the inner GMRES, to INNER_TOL of the outer residual,
+inside FP64 iterative refinement); the all-FP64 form runs the same steps from
+the same state beside it.
 
 Also timed: the full 6-DOF grand-mobility matvec (dimension 6*12,800 +
 3*9,000) in FP64 and FP32-tile/FP64-accumulate forms -- the "RPY matvec GB/s"
@@ -40,6 +42,8 @@ PEAK_HBM_GBPS = 8000.0
 FLOPS_TT = 36
 FLOPS_SS6, FLOPS_SB6, FLOPS_BS6, FLOPS_BB = 88, 52, 52, 36
 DL = 2.2
+WARM_START = 2      # tfs_beat_set_warm_start order, both precisions
+INNER_TOL = 1e-6    # inner FP32 GMRES: relative reduction of the outer residual
 
 
 def matvec_flops(ns: int, nb: int, seg_dof: int) -> float:
@@ -75,13 +79,14 @@ def matvec_leg(torch, handle, precision: int, steps: int, warmup: int):
     return {"ms_per_step": total_ms / steps, "pair_kernel_ms": ms / max(n, 1), "launches": n}
 
 
-def timestep_leg(torch, body, mode: int, steps: int, warmup: int, stream):
+def timestep_leg(torch, body, mode: int, steps: int, warmup: int, stream, warm_start: int = WARM_START,
+                 inner_tol: float = INNER_TOL):
     """Warm-up steps, then `steps` timed timesteps; returns timings and state."""
     from timfilsimmod_amd import _lib as L
     from timfilsimmod_amd.mobility import profile_read
     from timfilsimmod_amd.solver import BeatSimulation
-    sim = BeatSimulation(body, mode=mode, tol=1e-8, restart=50, inner_tol=1e-4, max_cycles=40,
-                         max_refinements=20, stream=stream)
+    sim = BeatSimulation(body, mode=mode, tol=1e-8, restart=50, inner_tol=inner_tol, max_cycles=40,
+                         max_refinements=20, stream=stream, warm_start=warm_start)
     rpy = sim.rpy_handle()
     for _ in range(warmup):
         sim.step()
@@ -418,6 +423,8 @@ def main():
         "config": {"workload": "prescribed-beat timestep, one body: 640 filaments x 20 segments + "
                                "9,000 blobs, R=50a, dt=T/300, GMRES(50) to 1e-8, dynamic phase "
                                "and shape rotation",
+                   "warm_start": "quadratic extrapolation of the last three solutions",
+                   "inner_tol": INNER_TOL,
                    "num_segs": ns, "num_blobs": nb, "system_size": nsys},
         "timestep": None if mixed is None else {
                      "mixed": summ(mixed), "fp64": summ(fp64),

@@ -133,6 +133,12 @@ int tfs_beat_state(tfs_beat *b, double *phase, double *angle, double *x_segs, do
 int tfs_beat_solver(tfs_beat *b, tfs_solver **out);
 /* Switch between TFS_SOLVE_FP64 and TFS_SOLVE_MIXED for later steps. */
 int tfs_beat_set_mode(tfs_beat *b, int32_t mode);
+/* Initial guess of later solves.  0 (default): the previous step's solution,
+ * as the reference (mobility_solver.cpp:2201-2293).  1 / 2: its linear /
+ * quadratic extrapolation in time from the last two / three solutions (used
+ * once that many exist).  No counterpart in the reference: the solve meets the
+ * same tolerance from a closer start. */
+int tfs_beat_set_warm_start(tfs_beat *b, int32_t order);
 
 /* ---- Elastic-filament velocities (non-prescribed cilia, body velocity prescribed) ----
  * mobility_solver::compute_velocities without PRESCRIBED_CILIA

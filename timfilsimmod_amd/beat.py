@@ -13,12 +13,20 @@ reference transcribes them at filament.cpp:692-750) and are kept identical.
 
 The reference reads its generalised-force tables from files written by a
 separate WRITE_GENERALISED_FORCES run (mobility_solver.cpp:45-103); those files
-are not part of the reference tree.  ``generalised_force_tables`` builds our own
-deterministic stand-in: the drag-only force that drives an isolated filament at
-its natural frequency, 2*pi * 6*pi*mu*a * sum_n |d x_n / d phi|^2, sampled over
-one period (phase table), and a small sinusoidal angle table.
+are not part of the reference tree.  That run drives a filament at its natural
+frequency with the shape angle held, and records the generalised forces of the
+segment forces it needed, q_phi = K_phi . f and q_theta = K_theta . f
+(cilia_sim_main.cu:370-380).  ``generalised_force_tables`` does the same for one
+isolated filament in unbounded fluid: f = M^-1 K_phi * 2 pi with M the RPY
+mobility of its segments, sampled over one period.  The two tables are then
+consistent with each other: the filament alone beats at phi_dot = 2 pi with
+theta_dot = 0 even where K_phi and K_theta are nearly parallel (the power
+stroke, close to a rigid rotation about the base), where an unrelated angle
+table would drive theta_dot without bound.
 """
 from __future__ import annotations
+
+import functools
 
 import numpy as np
 
@@ -78,12 +86,55 @@ def uniform_s(num_seg: int) -> np.ndarray:
     return np.linspace(0.0, 1.0, num_seg)
 
 
+def rpy_translational(x: np.ndarray, a: float = 1.0, mu: float = 1.0) -> np.ndarray:
+    """Dense translational RPY mobility of equal spheres at x (n x 3): far
+    field for r > 2a, the Rotne-Prager overlap form below, 1/(6 pi mu a) on
+    the diagonal."""
+    n = len(x)
+    M = np.zeros((3 * n, 3 * n))
+    I3 = np.eye(3)
+    for i in range(n):
+        M[3 * i:3 * i + 3, 3 * i:3 * i + 3] = I3 / (6.0 * np.pi * mu * a)
+        for j in range(i + 1, n):
+            d = x[i] - x[j]
+            r = np.linalg.norm(d)
+            e = np.outer(d, d) / (r * r)
+            if r > 2.0 * a:
+                c = 1.0 / (8.0 * np.pi * mu * r)
+                blk = c * ((1.0 + 2.0 * a * a / (3.0 * r * r)) * I3 + (1.0 - 2.0 * a * a / (r * r)) * e)
+            else:
+                c = 1.0 / (6.0 * np.pi * mu * a)
+                blk = c * ((1.0 - 9.0 * r / (32.0 * a)) * I3 + (3.0 * r / (32.0 * a)) * e)
+            M[3 * i:3 * i + 3, 3 * j:3 * j + 3] = blk
+            M[3 * j:3 * j + 3, 3 * i:3 * i + 3] = blk
+    return M
+
+
 def generalised_force_tables(num_seg: int, fil_length: float, *, a: float = 1.0, mu: float = 1.0,
                              samples: int = 300):
-    """(phase table, angle table) over one period, see the module docstring."""
+    """(phase table, angle table) over one period, see the module docstring:
+    an isolated filament (base at the origin, beat in the local x-y plane)
+    driven at phi_dot = 2 pi, theta_dot = 0."""
+    qp, qa = _tables(int(num_seg), float(fil_length), float(a), float(mu), int(samples))
+    return qp.copy(), qa.copy()
+
+
+@functools.lru_cache(maxsize=8)
+def _tables(num_seg: int, fil_length: float, a: float, mu: float, samples: int):
     s = uniform_s(num_seg)
-    phases = 2.0 * np.pi * np.arange(samples) / samples
-    c11 = np.array([np.sum(shape(s, p)[1] ** 2) for p in phases]) * fil_length ** 2
-    q_phase = 2.0 * np.pi * 6.0 * np.pi * mu * a * c11
-    q_angle = 0.05 * q_phase.mean() * np.sin(phases)
+    q_phase = np.empty(samples)
+    q_angle = np.empty(samples)
+    for k in range(samples):
+        phase = 2.0 * np.pi * k / samples
+        pos, der = shape(s, phase)
+        x = np.zeros((num_seg, 3))
+        x[:, :2] = fil_length * pos
+        kphi = np.zeros((num_seg, 3))
+        kphi[1:, :2] = fil_length * der[1:]
+        ktheta = np.zeros((num_seg, 3))
+        ktheta[1:, 0] = -x[1:, 1]                 # e_z x ref
+        ktheta[1:, 1] = x[1:, 0]
+        f = np.linalg.solve(rpy_translational(x, a, mu), 2.0 * np.pi * kphi.reshape(-1))
+        q_phase[k] = kphi.reshape(-1) @ f
+        q_angle[k] = ktheta.reshape(-1) @ f
     return q_phase, q_angle

@@ -53,6 +53,8 @@ hipError_t k_givens(double* H, int ldh, int j, double* cs, double* sn, double* g
 hipError_t k_scale(double* v, const double* w, const double* scal, int idx, int n, hipStream_t s);
 hipError_t k_trisolve(const double* H, int ldh, const double* g, int k, double* y, hipStream_t s);
 hipError_t k_axpby(double a, const double* x, double b, double* y, int n, hipStream_t s);
+// x <- extrapolated warm start of the given order from x, h1, h2; h2 <- h1, h1 <- old x.
+hipError_t k_extrapolate(double* x, double* h1, double* h2, int n, int order, hipStream_t s);
 hipError_t k_sub(const double* x, const double* y, double* out, int n, hipStream_t s);
 // One CGS2 Arnoldi step on w against V(:, 0:j] (three launches of G =
 // arnoldi_blocks(n) workgroups; pa and pb hold G x (j+1) partial sums each):

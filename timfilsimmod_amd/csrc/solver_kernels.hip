@@ -185,6 +185,20 @@ __global__ __launch_bounds__(256) void trisolve_kernel(const double* __restrict_
     }
 }
 
+// Warm start from the solution history, and the history shifted by one step:
+// x holds x_{n-1}, h1 x_{n-2}, h2 x_{n-3}; afterwards x is the guess for x_n
+// (order 0: x_{n-1}; 1: 2 x_{n-1} - x_{n-2}; 2: 3 x_{n-1} - 3 x_{n-2} + x_{n-3}),
+// h1 = x_{n-1}, h2 = x_{n-2}.
+__global__ __launch_bounds__(256) void extrapolate_kernel(double* __restrict__ x, double* __restrict__ h1,
+                                                          double* __restrict__ h2, int n, int order) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double a = x[i], b = h1[i], c = h2[i];
+    x[i] = order == 2 ? 3.0 * (a - b) + c : (order == 1 ? 2.0 * a - b : a);
+    h2[i] = b;
+    h1[i] = a;
+}
+
 // y = a*x + b*y
 __global__ __launch_bounds__(256) void axpby_kernel(double a, const double* __restrict__ x, double b,
                                                    double* __restrict__ y, int n) {
@@ -783,6 +797,11 @@ hipError_t k_trisolve(const double* H, int ldh, const double* g, int k, double* 
         hipLaunchKernelGGL(trisolve_kernel<true>, dim3(1), dim3(256), staged, s, H, ldh, g, k, y);
     else
         hipLaunchKernelGGL(trisolve_kernel<false>, dim3(1), dim3(256), sizeof(double) * k, s, H, ldh, g, k, y);
+    return hipGetLastError();
+}
+hipError_t k_extrapolate(double* x, double* h1, double* h2, int n, int order, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(extrapolate_kernel, dim3(blocks_for(n)), dim3(256), 0, s, x, h1, h2, n, order);
     return hipGetLastError();
 }
 hipError_t k_axpby(double a, const double* x, double b, double* y, int n, hipStream_t s) {

@@ -549,6 +549,11 @@ struct tfs_beat {
     double *frames = nullptr, *bases = nullptr, *s_values = nullptr, *phase = nullptr, *angle = nullptr,
            *omega0 = nullptr, *gen_phase = nullptr, *gen_angle = nullptr, *body = nullptr, *rhs = nullptr,
            *sol = nullptr;
+    // Warm-start extrapolation: order 0 = the previous solution (the
+    // reference), 1 = linear, 2 = quadratic in time from the last solutions;
+    // hist1/hist2 hold the two solutions before the one in `sol`.
+    int warm_order = 0;
+    double *hist1 = nullptr, *hist2 = nullptr;
     std::vector<double> beat_A, beat_B;
 };
 
@@ -581,7 +586,8 @@ int tfs_beat_create(const tfs_beat_desc* desc, tfs_beat** out) {
     const int64_t n = s->dims.n;
     std::vector<std::pair<double**, int64_t>> parts = {
         {&b->frames, 9LL * nf}, {&b->bases, 3LL * nf}, {&b->s_values, nseg}, {&b->phase, nf}, {&b->angle, nf},
-        {&b->omega0, nf}, {&b->gen_phase, ngp}, {&b->gen_angle, nga}, {&b->body, 9}, {&b->rhs, n}, {&b->sol, n}};
+        {&b->omega0, nf}, {&b->gen_phase, ngp}, {&b->gen_angle, nga}, {&b->body, 9}, {&b->rhs, n}, {&b->sol, n},
+        {&b->hist1, n}, {&b->hist2, n}};
     int64_t total = 0;
     for (auto& p : parts) total += ((p.second + 31) / 32) * 32;
     if (hipMalloc(&b->arena, total * 8) != hipSuccess) {
@@ -677,10 +683,23 @@ int tfs_beat_step(tfs_beat* b, tfs_solve_stats* stats) {
     if (rc) return rc;
     tfs_solve_stats local{};
     tfs_solve_stats* sst = stats ? stats : &local;
-    // Warm start: the previous step's solution (mobility_solver.cpp:2201-2293).
+    // Warm start: the previous step's solution (mobility_solver.cpp:2201-2293),
+    // or, if asked for, its extrapolation from the solutions before it.
+    if (b->step > 0) {
+        const int order = (int)std::min<int64_t>(b->warm_order, b->step - 1);
+        rc = check(k_extrapolate(b->sol, b->hist1, b->hist2, (int)s->dims.n, order, st), "warm start");
+        if (rc) return rc;
+    }
     const int solve_rc = solve_device(s, b->rhs, b->sol, sst);
     b->step++;
     return solve_rc;
+}
+
+int tfs_beat_set_warm_start(tfs_beat* b, int32_t order) {
+    if (!b) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_set_warm_start: null handle");
+    if (order < 0 || order > 2) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_set_warm_start: order must be 0, 1 or 2");
+    b->warm_order = order;
+    return TFS_OK;
 }
 
 int tfs_beat_state(tfs_beat* b, double* phase, double* angle, double* x_segs, double* solution) {

@@ -74,6 +74,7 @@ L.extra_signatures({
     "tfs_beat_state": (ctypes.c_int, [_dp, _dp, _dp, _dp, _dp]),
     "tfs_beat_solver": (ctypes.c_int, [_dp, ctypes.POINTER(_dp)]),
     "tfs_beat_set_mode": (ctypes.c_int, [_dp, _i32]),
+    "tfs_beat_set_warm_start": (ctypes.c_int, [_dp, _i32]),
 })
 
 
@@ -252,7 +253,8 @@ class BeatSimulation:
 
     def __init__(self, body, *, mode=SOLVE_FP64, dt=1.0 / 300.0, restart=50, tol=1e-8, inner_tol=1e-3,
                  max_cycles=20, max_refinements=10, dynamic_phase=True, dynamic_angle=True,
-                 gen_force_factor=1.0, torsion_factor=0.046, stream=None, setup: BeatSetup | None = None):
+                 gen_force_factor=1.0, torsion_factor=0.046, stream=None, setup: BeatSetup | None = None,
+                 warm_start: int = 0):
         self.setup = setup or beat_setup(body)
         su = self.setup
         d = BeatDesc()
@@ -280,6 +282,8 @@ class BeatSimulation:
         n = ctypes.c_int64()
         L.check(L.lib().tfs_solver_size(sp, ctypes.byref(n)))
         self.n = n.value
+        if warm_start:
+            self.set_warm_start(warm_start)
 
     def close(self):
         if self.h:
@@ -294,6 +298,11 @@ class BeatSimulation:
 
     def set_mode(self, mode: int):
         L.check(L.lib().tfs_beat_set_mode(self.h, mode))
+
+    def set_warm_start(self, order: int):
+        """Initial guess of later solves: 0 the previous solution (the
+        reference's), 1 / 2 its linear / quadratic extrapolation in time."""
+        L.check(L.lib().tfs_beat_set_warm_start(self.h, order))
 
     def rpy_handle(self) -> ctypes.c_void_p:
         """The tfs_rpy handle the driver's solver evaluates with."""
