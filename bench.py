@@ -9,7 +9,8 @@ kinematics, right-hand side and the warm-started saddle-point solve
 (RPY <3,3> mobility + K_phi/K_theta couplings, n = 3*12,800 + 3*9,000 +
 2*640 = 66,680) to a relative residual of 1e-8 with GMRES(50) and FP64
 Arnoldi.  The initial guess of each solve is the quadratic extrapolation of the
-last three solutions (WARM_START).  The headline is the mixed-precision form
+last three solutions (WARM_START); the right preconditioner is the saddle form
+with each filament's own mobility block and 16-blob patches (PRECOND).  The headline is the mixed-precision form
 (FP32-tile matvecs in the inner GMRES, to INNER_TOL of the outer residual,
 inside FP64 iterative refinement); the all-FP64 form runs the same steps from
 the same state beside it.
@@ -44,6 +45,7 @@ FLOPS_SS6, FLOPS_SB6, FLOPS_BS6, FLOPS_BB = 88, 52, 52, 36
 DL = 2.2
 WARM_START = 2      # tfs_beat_set_warm_start order, both precisions
 INNER_TOL = 1e-6    # inner FP32 GMRES: relative reduction of the outer residual
+PRECOND = 1         # TFS_PRECOND_BLOCKS (filament self-mobility blocks + blob patches), both precisions
 
 
 def matvec_flops(ns: int, nb: int, seg_dof: int) -> float:
@@ -86,7 +88,7 @@ def timestep_leg(torch, body, mode: int, steps: int, warmup: int, stream, warm_s
     from timfilsimmod_amd.mobility import profile_read
     from timfilsimmod_amd.solver import BeatSimulation
     sim = BeatSimulation(body, mode=mode, tol=1e-8, restart=50, inner_tol=inner_tol, max_cycles=40,
-                         max_refinements=20, stream=stream, warm_start=warm_start)
+                         max_refinements=20, stream=stream, warm_start=warm_start, precond=PRECOND)
     rpy = sim.rpy_handle()
     for _ in range(warmup):
         sim.step()
@@ -424,6 +426,7 @@ def main():
                                "9,000 blobs, R=50a, dt=T/300, GMRES(50) to 1e-8, dynamic phase "
                                "and shape rotation",
                    "warm_start": "quadratic extrapolation of the last three solutions",
+                   "preconditioner": "saddle form with filament self-mobility blocks and 16-blob patches",
                    "inner_tol": INNER_TOL,
                    "num_segs": ns, "num_blobs": nb, "system_size": nsys},
         "timestep": None if mixed is None else {

@@ -36,6 +36,9 @@ extern "C" {
 #define TFS_SOLVE_FP64 0   /* every matvec in FP64 tiles                               */
 #define TFS_SOLVE_MIXED 1  /* inner GMRES on FP32-tile matvecs, FP64 residuals outside */
 
+#define TFS_PRECOND_REFERENCE 0  /* D = I/(6 pi mu a) in P = [D -K; -K^T 0]              */
+#define TFS_PRECOND_BLOCKS 1     /* D = each filament's own mobility + blob patches     */
+
 typedef struct tfs_solver_desc {
     int32_t num_fil;         /* NSWIM*NFIL                                          */
     int32_t num_seg;         /* NSEG (segments per filament)                        */
@@ -82,6 +85,16 @@ int tfs_solver_system_mult(tfs_solver *s, const double *in, double *out, int32_t
                            int32_t memory);
 /* out = P^{-1} in (apply_preconditioner_new). */
 int tfs_solver_precondition(tfs_solver *s, const double *in, double *out, int32_t memory);
+/* Right preconditioner of later solves.  TFS_PRECOND_REFERENCE (default):
+ * P = [D -K; -K^T 0] with D = I/(6 pi mu a), apply_preconditioner_new
+ * (mobility_solver.cpp:1040-1116).  TFS_PRECOND_BLOCKS: the same saddle form
+ * with D the block-diagonal of every filament's own RPY mobility (3 NSEG
+ * square, rebuilt when the segments move) and of the blobs' RPY mobility
+ * within patches of 16 blobs neighbouring in Morton order (rebuilt when the
+ * blobs move), each block inverted on the device.  No counterpart in the
+ * reference: solves meet the same tolerance in fewer iterations.  Needs
+ * num_seg <= 128. */
+int tfs_solver_set_preconditioner(tfs_solver *s, int32_t kind);
 /* Solve A x = rhs; x holds the warm start on entry (solve_linear_system,
  * including its zero-rhs and rubbish-guess rules, mobility_solver.cpp:2123-2322).
  * Returns TFS_ERR_NOT_CONVERGED (with x updated) if the tolerance is not met,

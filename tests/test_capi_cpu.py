@@ -70,6 +70,7 @@ def test_null_handles_are_errors_not_crashes():
     assert lib.tfs_rpy_evaluate(None, 15) == L.TFS_ERR_INVALID_ARGUMENT
     assert lib.tfs_rpy_destroy(None) == L.TFS_OK
     assert lib.tfs_beat_set_warm_start(None, 1) == L.TFS_ERR_INVALID_ARGUMENT
+    assert lib.tfs_solver_set_preconditioner(None, 1) == L.TFS_ERR_INVALID_ARGUMENT
 
 
 def test_create_reports_missing_device_on_cpu_host():

@@ -22,6 +22,7 @@ SOURCES = [
     "rpy_matvec.hip",
     "flow_field.hip",
     "solver_kernels.hip",
+    "dense_kernels.hip",
     "tfs_rpy_api.hip",
     "tfs_solver_api.hip",
     "tfs_flow_api.hip",

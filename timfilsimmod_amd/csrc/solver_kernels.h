@@ -16,6 +16,31 @@ struct SaddleDims {
     int64_t off_blob, off_phase, off_angle, n;
 };
 
+// Right preconditioner P = [D -K; -K^T 0], applied as its exact inverse.
+// kind 0 (mobility_solver.cpp:1040-1116): D = I/(6 pi mu a), gs / gb = 6 pi mu a
+// for segments / blobs.  kind 1: D = block-diagonal of each filament's own RPY
+// mobility (3 nseg square) and of the blobs' mobility within patches of up to
+// kPatchBlobs blobs (neighbours in Morton order); inverses stored row-major
+// (they are symmetric, so column c is read as row c).  Per filament,
+// fil_dk = D^-1 [K_phi K_theta] (two columns of 3 nseg) and fil_sinv the
+// inverse of the 2 x 2 (or 1 x 1) Schur complement K^T D^-1 K.
+constexpr int kPatchBlobs = 16;
+constexpr int kMaxBlockFilSeg = 128;
+struct PrecondArgs {
+    int kind = 0;
+    double gs = 0, gb = 0;
+    const double* kphi = nullptr;
+    const double* kth = nullptr;
+    const double* fil_inv = nullptr;      // nfil blocks of (3 nseg)^2
+    const double* fil_dk = nullptr;       // nfil x 2 x 3 nseg
+    const double* fil_sinv = nullptr;     // nfil x 4
+    const int32_t* patch_blob = nullptr;  // blob indices, patch after patch
+    const int32_t* patch_start = nullptr; // npatch + 1 offsets into patch_blob
+    const double* patch_inv = nullptr;    // npatch blocks of (3 * patch size)^2 ...
+    const int64_t* patch_off = nullptr;   // ... at these offsets
+    int npatch = 0;
+};
+
 struct BeatArgs {
     int nfil, nseg;
     double length;            // filament length L = DL*(NSEG-1)
@@ -71,16 +96,23 @@ hipError_t k_arnoldi_step(const double* V, int64_t ld, int j, double* w, int n, 
 // happens while *stop is nonzero.
 hipError_t k_arnoldi_close_next(const double* pa, int G, double* H, int ldh, int j, double* cs, double* sn,
                                 double* g, double* scal, double* est, int32_t* stop, double abs_tol,
-                                const double* w, double* v, double* out, const double* kphi, const double* kth,
-                                const SaddleDims& d, double gs, double gb, hipStream_t s);
+                                const double* w, double* v, double* out, const PrecondArgs& P,
+                                const SaddleDims& d, hipStream_t s);
 int arnoldi_blocks(int64_t n);
 // v = w * scale; out = P^{-1} v.  scale = dev_scale[1] if dev_scale, else host_scale.
 // Nothing happens while *skip (if given) is nonzero.
 hipError_t k_scale_precond(const double* w, const double* dev_scale, double host_scale, double* v, double* out,
-                           const double* kphi, const double* kth, const SaddleDims& d, double gs, double gb,
-                           const int32_t* skip, hipStream_t s);
-hipError_t k_precond(const double* in, double* out, const double* kphi, const double* kth, const SaddleDims& d,
-                     double gs, double gb, hipStream_t s);
+                           const PrecondArgs& P, const SaddleDims& d, const int32_t* skip, hipStream_t s);
+hipError_t k_precond(const double* in, double* out, const PrecondArgs& P, const SaddleDims& d, hipStream_t s);
+// Block preconditioner set-up (kind 1).  fil_blk / patch_blk receive the
+// blocks of D (to be inverted in place by k_batched_inverse); k_fil_schur then
+// forms fil_dk and fil_sinv from the inverted filament blocks.
+hipError_t k_fil_blocks(const double* x_segs, int nfil, int nseg, double a_seg, double mu, double* fil_blk,
+                        hipStream_t s);
+hipError_t k_patch_blocks(const double* x_blobs, const int32_t* patch_blob, const int32_t* patch_start,
+                          const int64_t* patch_off, int npatch, double a_blob, double mu, double* patch_blk,
+                          hipStream_t s);
+hipError_t k_fil_schur(const PrecondArgs& P, const SaddleDims& d, double* fil_dk, double* fil_sinv, hipStream_t s);
 hipError_t k_beat_coeffs(const double* A, const double* B);
 hipError_t k_beat(const BeatArgs& a, double* x_segs, double* kphi, double* kth, hipStream_t s);
 hipError_t k_rhs(const RhsArgs& a, const double* x_segs, const double* x_blobs, double* rhs, hipStream_t s);

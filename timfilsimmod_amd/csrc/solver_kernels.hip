@@ -15,6 +15,7 @@
 //   :282-440 assemble_rhs) and the explicit phase update (filament.cpp:1087-1097).
 #include <hip/hip_runtime.h>
 
+#include "rpy_pair.h"
 #include "solver_kernels.h"
 
 namespace tfs {
@@ -536,17 +537,130 @@ __device__ void scale_precond_body(const double* __restrict__ w, double sc, doub
     }
 }
 
+// One filament (segment rows and its phase / angle rows) or one blob patch of
+// the block preconditioner, by one wave: v = w * sc (if v), out = P^{-1} v.
+// Rows are spread over the lanes (row r on lane r % 64, register r / 64); the
+// dense block products take their operands from the owning lanes by shuffle.
+__device__ void block_precond_wave(const PrecondArgs& P, const SaddleDims& d, int item, int lane,
+                                   const double* __restrict__ w, double sc, double* __restrict__ v,
+                                   double* __restrict__ out) {
+    constexpr int RMAX = 3 * kMaxBlockFilSeg / kWave;   // registers per lane
+    if (item < d.nfil) {
+        const int m = item;
+        const int len = 3 * d.nseg;
+        const int64_t seg0 = (int64_t)m * len;
+        const double* Bi = P.fil_inv + (int64_t)m * len * len;
+        double x[RMAX], du[RMAX];
+#pragma unroll
+        for (int q = 0; q < RMAX; ++q) {
+            const int r = q * kWave + lane;
+            x[q] = r < len ? w[seg0 + r] * sc : 0.0;
+            if (v && r < len) v[seg0 + r] = x[q];
+            du[q] = 0.0;
+        }
+        // du = D_m^{-1} x (symmetric: entry (r, c) read as (c, r), coalesced over r)
+#pragma unroll
+        for (int qc = 0; qc < RMAX; ++qc) {
+            if (qc * kWave >= len) break;
+            for (int l = 0; l < kWave; ++l) {
+                const int c = qc * kWave + l;
+                if (c >= len) break;
+                const double xc = __shfl(x[qc], l, kWave);
+#pragma unroll
+                for (int q = 0; q < RMAX; ++q) {
+                    const int r = q * kWave + lane;
+                    if (r < len) du[q] += Bi[(int64_t)c * len + r] * xc;
+                }
+            }
+        }
+        double tp = 0.0, tt = 0.0;
+#pragma unroll
+        for (int q = 0; q < RMAX; ++q) {
+            const int r = q * kWave + lane;
+            if (r < len) {
+                tp += P.kphi[seg0 + r] * du[q];
+                tt += P.kth[seg0 + r] * du[q];
+            }
+        }
+        tp = wave_sum(tp);
+        tt = wave_sum(tt);
+        double yp = 0.0, ya = 0.0;
+        if (lane == 0) {
+            const double* Si = P.fil_sinv + 4 * (int64_t)m;
+            const double vp = d.dyn_phase ? w[d.off_phase + m] * sc : 0.0;
+            const double va = d.dyn_angle ? w[d.off_angle + m] * sc : 0.0;
+            // y = -S^{-1} (v + K^T D^{-1} x)
+            if (d.dyn_phase && d.dyn_angle) {
+                yp = -(Si[0] * (vp + tp) + Si[1] * (va + tt));
+                ya = -(Si[2] * (vp + tp) + Si[3] * (va + tt));
+            } else if (d.dyn_phase) {
+                yp = -Si[0] * (vp + tp);
+            } else if (d.dyn_angle) {
+                ya = -Si[0] * (va + tt);
+            }
+            if (d.dyn_phase) {
+                if (v) v[d.off_phase + m] = vp;
+                out[d.off_phase + m] = yp;
+            }
+            if (d.dyn_angle) {
+                if (v) v[d.off_angle + m] = va;
+                out[d.off_angle + m] = ya;
+            }
+        }
+        yp = __shfl(yp, 0, kWave);
+        ya = __shfl(ya, 0, kWave);
+        // x = D^{-1} (x + K y) = du + (D^{-1} K_phi) yp + (D^{-1} K_theta) ya
+        const double* dkp = P.fil_dk + (int64_t)m * 2 * len;
+        const double* dka = dkp + len;
+#pragma unroll
+        for (int q = 0; q < RMAX; ++q) {
+            const int r = q * kWave + lane;
+            if (r < len) out[seg0 + r] = du[q] + dkp[r] * yp + dka[r] * ya;
+        }
+        return;
+    }
+    // blob patch: at most kPatchBlobs blobs, 3 * kPatchBlobs <= 64 rows
+    const int p = item - d.nfil;
+    const int b0 = P.patch_start[p];
+    const int rows = 3 * (P.patch_start[p + 1] - b0);
+    const double* Bi = P.patch_inv + P.patch_off[p];
+    int64_t idx = 0;
+    double x = 0.0;
+    if (lane < rows) {
+        idx = d.off_blob + 3 * (int64_t)P.patch_blob[b0 + lane / 3] + lane % 3;
+        x = w[idx] * sc;
+        if (v) v[idx] = x;
+    }
+    double o = 0.0;
+    for (int c = 0; c < rows; ++c) {
+        const double xc = __shfl(x, c, kWave);
+        if (lane < rows) o += Bi[(int64_t)c * rows + lane] * xc;
+    }
+    if (lane < rows) out[idx] = o;
+}
+
+// Every wave of the grid takes filaments, then patches, round robin.
+__device__ void block_precond_body(const PrecondArgs& P, const SaddleDims& d, const double* __restrict__ w,
+                                   double sc, double* __restrict__ v, double* __restrict__ out) {
+    const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int nwaves = gridDim.x * blockDim.x / kWave;
+    const int items = d.nfil + P.npatch;
+    for (int it = gw; it < items; it += nwaves) block_precond_wave(P, d, it, lane, w, sc, v, out);
+}
+
 // scale = *dev_scale[1], or host_scale when dev_scale is null.
 __global__ __launch_bounds__(256) void scale_precond_kernel(const double* __restrict__ w,
                                                            const double* __restrict__ dev_scale,
                                                            double host_scale, double* __restrict__ v,
-                                                           double* __restrict__ out,
-                                                           const double* __restrict__ kphi,
-                                                           const double* __restrict__ kth, SaddleDims d,
-                                                           double gs, double gb, const int32_t* __restrict__ skip) {
+                                                           double* __restrict__ out, PrecondArgs P, SaddleDims d,
+                                                           const int32_t* __restrict__ skip) {
     if (skip && *skip) return;
-    scale_precond_body(w, dev_scale ? dev_scale[1] : host_scale, v, out, kphi, kth, d, gs, gb);
+    const double sc = dev_scale ? dev_scale[1] : host_scale;
+    if (P.kind == 1) block_precond_body(P, d, w, sc, v, out);
+    else scale_precond_body(w, sc, v, out, P.kphi, P.kth, d, P.gs, P.gb);
 }
+
 
 // Closes Arnoldi step j and opens step j + 1 in one launch.  Every workgroup
 // sums the G partial sums of ||w||^2 in the same order (so all of them hold
@@ -559,8 +673,7 @@ __global__ __launch_bounds__(256) void arnoldi_close_next_kernel(
     const double* __restrict__ p_in, int G, double* __restrict__ H, int ldh, int j, double* __restrict__ cs,
     double* __restrict__ sn, double* __restrict__ g, double* __restrict__ scal, double* __restrict__ est,
     int32_t* __restrict__ stop, double abs_tol, const double* __restrict__ w, double* __restrict__ v,
-    double* __restrict__ out, const double* __restrict__ kphi, const double* __restrict__ kth, SaddleDims d,
-    double gs, double gb) {
+    double* __restrict__ out, PrecondArgs P, SaddleDims d) {
     __shared__ double nrm;
     if (*stop) return;
     if (threadIdx.x < kWave) {
@@ -570,16 +683,25 @@ __global__ __launch_bounds__(256) void arnoldi_close_next_kernel(
     __syncthreads();
     const double hn = nrm;
     if (blockIdx.x == 0) arnoldi_close(hn, H, ldh, j, cs, sn, g, scal, est, stop, abs_tol);
-    if (v) scale_precond_body(w, hn > 0.0 ? 1.0 / hn : 0.0, v, out, kphi, kth, d, gs, gb);
+    if (v) {
+        const double sc = hn > 0.0 ? 1.0 / hn : 0.0;
+        if (P.kind == 1) block_precond_body(P, d, w, sc, v, out);
+        else scale_precond_body(w, sc, v, out, P.kphi, P.kth, d, P.gs, P.gb);
+    }
 }
 
 // Right preconditioner P^{-1}: the exact inverse of [D -K; -K^T 0] with
 // D = I/(6 pi mu a) (mobility_solver.cpp:1040-1116).  Blob rows are scaled by
 // 6 pi mu a_b.  Filament waves first, then a grid-stride over blob entries.
 __global__ __launch_bounds__(256) void precond_kernel(const double* __restrict__ in, double* __restrict__ out,
-                                                     const double* __restrict__ kphi,
-                                                     const double* __restrict__ kth, SaddleDims d,
-                                                     double gs, double gb) {
+                                                     PrecondArgs P, SaddleDims d) {
+    if (P.kind == 1) {
+        block_precond_body(P, d, in, 1.0, nullptr, out);
+        return;
+    }
+    const double* __restrict__ kphi = P.kphi;
+    const double* __restrict__ kth = P.kth;
+    const double gs = P.gs, gb = P.gb;
     const int gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     const int nwaves = gridDim.x * blockDim.x / kWave;
@@ -625,6 +747,122 @@ __global__ __launch_bounds__(256) void precond_kernel(const double* __restrict__
     const int nb3 = 3 * d.nblob;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb3; i += gridDim.x * blockDim.x)
         out[d.off_blob + i] = gb * in[d.off_blob + i];
+}
+
+// Block preconditioner set-up: the 3 x 3 RPY blocks (the reference's pair
+// tensor, every branch, in FP64) of the pairs inside one filament or one blob
+// patch, row-major; one workgroup per filament / patch.
+__device__ void rpy_block_entries(const double* xi, const double* xj, double a, double mu, double* blk3x3) {
+    const double dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
+    for (int c = 0; c < 3; ++c) {
+        double f[3] = {0.0, 0.0, 0.0}, u[3] = {0.0, 0.0, 0.0};
+        f[c] = 1.0;
+        pair_near<3, 3>(u, dx, dy, dz, f, a, a, mu);
+        for (int r = 0; r < 3; ++r) blk3x3[3 * r + c] = u[r];
+    }
+}
+
+__global__ __launch_bounds__(256) void fil_blocks_kernel(const double* __restrict__ x_segs, int nseg, double a,
+                                                        double mu, double* __restrict__ blk) {
+    const int m = blockIdx.x;
+    const int len = 3 * nseg;
+    const double* x = x_segs + (int64_t)m * len;
+    double* B = blk + (int64_t)m * len * len;
+    for (int e = threadIdx.x; e < nseg * nseg; e += blockDim.x) {
+        const int i = e / nseg, j = e % nseg;
+        double t[9];
+        rpy_block_entries(x + 3 * i, x + 3 * j, a, mu, t);
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) B[(int64_t)(3 * i + r) * len + 3 * j + c] = t[3 * r + c];
+    }
+}
+
+__global__ __launch_bounds__(256) void patch_blocks_kernel(const double* __restrict__ x_blobs,
+                                                          const int32_t* __restrict__ patch_blob,
+                                                          const int32_t* __restrict__ patch_start,
+                                                          const int64_t* __restrict__ patch_off, double a,
+                                                          double mu, double* __restrict__ blk) {
+    const int p = blockIdx.x;
+    const int b0 = patch_start[p], cnt = patch_start[p + 1] - b0;
+    const int rows = 3 * cnt;
+    double* B = blk + patch_off[p];
+    for (int e = threadIdx.x; e < cnt * cnt; e += blockDim.x) {
+        const int i = e / cnt, j = e % cnt;
+        double t[9];
+        rpy_block_entries(x_blobs + 3 * (int64_t)patch_blob[b0 + i], x_blobs + 3 * (int64_t)patch_blob[b0 + j], a,
+                          mu, t);
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) B[(int64_t)(3 * i + r) * rows + 3 * j + c] = t[3 * r + c];
+    }
+}
+
+// fil_dk = D_m^{-1} [K_phi K_theta] and fil_sinv = (K^T D_m^{-1} K)^{-1} (the
+// dynamic couplings only), one wave per filament.
+__global__ __launch_bounds__(256) void fil_schur_kernel(PrecondArgs P, SaddleDims d, double* __restrict__ dk,
+                                                       double* __restrict__ sinv) {
+    constexpr int RMAX = 3 * kMaxBlockFilSeg / kWave;
+    const int m = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (m >= d.nfil) return;
+    const int len = 3 * d.nseg;
+    const int64_t seg0 = (int64_t)m * len;
+    const double* Bi = P.fil_inv + (int64_t)m * len * len;
+    double kp[RMAX], kt[RMAX], up[RMAX], ut[RMAX];
+#pragma unroll
+    for (int q = 0; q < RMAX; ++q) {
+        const int r = q * kWave + lane;
+        kp[q] = r < len ? P.kphi[seg0 + r] : 0.0;
+        kt[q] = r < len ? P.kth[seg0 + r] : 0.0;
+        up[q] = ut[q] = 0.0;
+    }
+#pragma unroll
+    for (int qc = 0; qc < RMAX; ++qc) {
+        if (qc * kWave >= len) break;
+        for (int l = 0; l < kWave; ++l) {
+            const int c = qc * kWave + l;
+            if (c >= len) break;
+            const double pc = __shfl(kp[qc], l, kWave), tc = __shfl(kt[qc], l, kWave);
+#pragma unroll
+            for (int q = 0; q < RMAX; ++q) {
+                const int r = q * kWave + lane;
+                if (r < len) {
+                    const double bv = Bi[(int64_t)c * len + r];
+                    up[q] += bv * pc;
+                    ut[q] += bv * tc;
+                }
+            }
+        }
+    }
+    double s11 = 0.0, s12 = 0.0, s21 = 0.0, s22 = 0.0;
+#pragma unroll
+    for (int q = 0; q < RMAX; ++q) {
+        const int r = q * kWave + lane;
+        if (r < len) {
+            dk[(int64_t)m * 2 * len + r] = up[q];
+            dk[(int64_t)m * 2 * len + len + r] = ut[q];
+            s11 += kp[q] * up[q];
+            s12 += kp[q] * ut[q];
+            s21 += kt[q] * up[q];
+            s22 += kt[q] * ut[q];
+        }
+    }
+    s11 = wave_sum(s11);
+    s12 = wave_sum(s12);
+    s21 = wave_sum(s21);
+    s22 = wave_sum(s22);
+    if (lane == 0) {
+        double* S = sinv + 4 * (int64_t)m;
+        if (d.dyn_phase && d.dyn_angle) {
+            const double det = s11 * s22 - s12 * s21;
+            S[0] = s22 / det;
+            S[1] = -s12 / det;
+            S[2] = -s21 / det;
+            S[3] = s11 / det;
+        } else {
+            S[0] = d.dyn_phase ? 1.0 / s11 : (d.dyn_angle ? 1.0 / s22 : 0.0);
+            S[1] = S[2] = S[3] = 0.0;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -819,15 +1057,21 @@ hipError_t k_arnoldi_step(const double* V, int64_t ld, int j, double* w, int n, 
     hipLaunchKernelGGL(arnoldi_update_sumsq_kernel, dim3(G), dim3(256), 0, s, a, (const double*)pb, pa, Hcol);
     return hipGetLastError();
 }
+// Waves for a preconditioner launch: one per filament (or per patch in the
+// block form), or enough threads for the blob entries.
+static int64_t precond_work(const PrecondArgs& P, const SaddleDims& d) {
+    const int64_t waves = P.kind == 1 ? (int64_t)d.nfil + P.npatch : d.nfil;
+    const int64_t work = waves * kWave > 3LL * d.nblob ? waves * kWave : 3LL * d.nblob;
+    return work > 0 ? work : 1;
+}
 hipError_t k_arnoldi_close_next(const double* pa, int G, double* H, int ldh, int j, double* cs, double* sn,
                                 double* g, double* scal, double* est, int32_t* stop, double abs_tol,
-                                const double* w, double* v, double* out, const double* kphi, const double* kth,
-                                const SaddleDims& d, double gs, double gb, hipStream_t s) {
+                                const double* w, double* v, double* out, const PrecondArgs& P,
+                                const SaddleDims& d, hipStream_t s) {
     if (j + 1 > kMaxBasis) return hipErrorInvalidValue;
-    const int64_t work = (int64_t)d.nfil * kWave > 3LL * d.nblob ? (int64_t)d.nfil * kWave : 3LL * d.nblob;
-    const int grid = v ? blocks_for(work > 0 ? work : 1) : 1;
+    const int grid = v ? blocks_for(precond_work(P, d)) : 1;
     hipLaunchKernelGGL(arnoldi_close_next_kernel, dim3(grid), dim3(256), 0, s, pa, G, H, ldh, j, cs, sn, g, scal,
-                       est, stop, abs_tol, w, v, out, kphi, kth, d, gs, gb);
+                       est, stop, abs_tol, w, v, out, P, d);
     return hipGetLastError();
 }
 int arnoldi_blocks(int64_t n) {
@@ -835,21 +1079,37 @@ int arnoldi_blocks(int64_t n) {
     return (int)(g < 1 ? 1 : (g > 1024 ? 1024 : g));
 }
 hipError_t k_scale_precond(const double* w, const double* dev_scale, double host_scale, double* v, double* out,
-                           const double* kphi, const double* kth, const SaddleDims& d, double gs, double gb,
-                           const int32_t* skip, hipStream_t s) {
-    const int64_t work = (int64_t)d.nfil * kWave > 3LL * d.nblob ? (int64_t)d.nfil * kWave : 3LL * d.nblob;
-    hipLaunchKernelGGL(scale_precond_kernel, dim3(blocks_for(work > 0 ? work : 1)), dim3(256), 0, s, w, dev_scale,
-                       host_scale, v, out, kphi, kth, d, gs, gb, skip);
+                           const PrecondArgs& P, const SaddleDims& d, const int32_t* skip, hipStream_t s) {
+    hipLaunchKernelGGL(scale_precond_kernel, dim3(blocks_for(precond_work(P, d))), dim3(256), 0, s, w, dev_scale,
+                       host_scale, v, out, P, d, skip);
     return hipGetLastError();
 }
 hipError_t k_sub(const double* x, const double* y, double* out, int n, hipStream_t s) {
     hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(n)), dim3(256), 0, s, x, y, out, n);
     return hipGetLastError();
 }
-hipError_t k_precond(const double* in, double* out, const double* kphi, const double* kth, const SaddleDims& d,
-                     double gs, double gb, hipStream_t s) {
-    const int64_t work = (int64_t)d.nfil * kWave > 3LL * d.nblob ? (int64_t)d.nfil * kWave : 3LL * d.nblob;
-    hipLaunchKernelGGL(precond_kernel, dim3(blocks_for(work)), dim3(256), 0, s, in, out, kphi, kth, d, gs, gb);
+hipError_t k_precond(const double* in, double* out, const PrecondArgs& P, const SaddleDims& d, hipStream_t s) {
+    hipLaunchKernelGGL(precond_kernel, dim3(blocks_for(precond_work(P, d))), dim3(256), 0, s, in, out, P, d);
+    return hipGetLastError();
+}
+hipError_t k_fil_blocks(const double* x_segs, int nfil, int nseg, double a_seg, double mu, double* fil_blk,
+                        hipStream_t s) {
+    if (nfil <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fil_blocks_kernel, dim3((unsigned)nfil), dim3(256), 0, s, x_segs, nseg, a_seg, mu, fil_blk);
+    return hipGetLastError();
+}
+hipError_t k_patch_blocks(const double* x_blobs, const int32_t* patch_blob, const int32_t* patch_start,
+                          const int64_t* patch_off, int npatch, double a_blob, double mu, double* patch_blk,
+                          hipStream_t s) {
+    if (npatch <= 0) return hipSuccess;
+    hipLaunchKernelGGL(patch_blocks_kernel, dim3((unsigned)npatch), dim3(256), 0, s, x_blobs, patch_blob,
+                       patch_start, patch_off, a_blob, mu, patch_blk);
+    return hipGetLastError();
+}
+hipError_t k_fil_schur(const PrecondArgs& P, const SaddleDims& d, double* fil_dk, double* fil_sinv, hipStream_t s) {
+    if (d.nfil <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fil_schur_kernel, dim3(blocks_for((int64_t)d.nfil * kWave)), dim3(256), 0, s, P, d, fil_dk,
+                       fil_sinv);
     return hipGetLastError();
 }
 hipError_t k_beat_coeffs(const double* A, const double* B) {

@@ -14,6 +14,7 @@
 
 #include "../../include/tfs_broyden.h"
 #include "solver_kernels.h"
+#include "dense_kernels.h"
 #include "tfs_api_internal.h"
 
 using namespace tfs;
@@ -53,105 +54,6 @@ __global__ __launch_bounds__(256) void block_transpose_kernel(const double* __re
         const int r = (int)(e / bs), c = (int)(e % bs);
         T[(int64_t)c * bs + r] = S[e];
     }
-}
-
-// In-place Gauss-Jordan inverse of every row-major block, partial pivoting,
-// one workgroup per block.  Step k: pivot search over rows k..bs-1, row swap,
-// scale of the pivot row (its pivot entry becomes 1/d), elimination of column k
-// from every other row (column k becomes -A[i][k]/d); the row swaps are undone
-// as column swaps in reverse order at the end.  status[b] = 1 if a pivot is zero
-// or not finite.
-__global__ __launch_bounds__(256) void block_gauss_jordan_kernel(double* __restrict__ A,
-                                                                 const int64_t* __restrict__ mat_off,
-                                                                 const int32_t* __restrict__ sizes,
-                                                                 int32_t* __restrict__ piv_all,
-                                                                 const int32_t* __restrict__ piv_off,
-                                                                 int32_t* __restrict__ status) {
-    extern __shared__ double lds[];
-    __shared__ double red_v[256];
-    __shared__ int red_i[256];
-    __shared__ int bad;
-    const int b = blockIdx.x;
-    const int bs = sizes[b];
-    double* M = A + mat_off[b];
-    int32_t* piv = piv_all + piv_off[b];
-    double* colk = lds;        // column k of the other rows
-    double* rowk = lds + bs;   // the scaled pivot row
-    const int tid = threadIdx.x;
-    if (tid == 0) bad = 0;
-    __syncthreads();
-    for (int k = 0; k < bs; ++k) {
-        // pivot search
-        double best = -1.0;
-        int bi = k;
-        for (int i = k + tid; i < bs; i += blockDim.x) {
-            const double v = fabs(M[(int64_t)i * bs + k]);
-            if (v > best) {
-                best = v;
-                bi = i;
-            }
-        }
-        red_v[tid] = best;
-        red_i[tid] = bi;
-        __syncthreads();
-        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-            if (tid < w) {
-                const double o = red_v[tid + w];
-                const int oi = red_i[tid + w];
-                if (o > red_v[tid] || (o == red_v[tid] && oi < red_i[tid])) {
-                    red_v[tid] = o;
-                    red_i[tid] = oi;
-                }
-            }
-            __syncthreads();
-        }
-        const int p = red_i[0];
-        // row swap k <-> p, then the pivot row is scaled into LDS
-        if (p != k)
-            for (int j = tid; j < bs; j += blockDim.x) {
-                const double t = M[(int64_t)k * bs + j];
-                M[(int64_t)k * bs + j] = M[(int64_t)p * bs + j];
-                M[(int64_t)p * bs + j] = t;
-            }
-        if (tid == 0) piv[k] = p;
-        __syncthreads();
-        const double d = M[(int64_t)k * bs + k];
-        if (!(fabs(d) > 0.0) || !isfinite(d)) {
-            if (tid == 0) bad = 1;
-            break;   // uniform across the workgroup: every thread read the same d
-        }
-        const double inv = 1.0 / d;
-        for (int j = tid; j < bs; j += blockDim.x) {
-            const double v = j == k ? inv : M[(int64_t)k * bs + j] * inv;
-            rowk[j] = v;
-            M[(int64_t)k * bs + j] = v;
-        }
-        for (int i = tid; i < bs; i += blockDim.x) colk[i] = i == k ? 0.0 : M[(int64_t)i * bs + k];
-        __syncthreads();
-        for (int64_t e = tid; e < (int64_t)bs * bs; e += blockDim.x) {
-            const int i = (int)(e / bs), j = (int)(e % bs);
-            if (i == k) continue;
-            const double f = colk[i];
-            M[e] = j == k ? -f * rowk[k] : M[e] - f * rowk[j];
-        }
-        __syncthreads();
-    }
-    __syncthreads();
-    if (bad) {
-        if (tid == 0) status[b] = 1;
-        return;
-    }
-    for (int k = bs - 1; k >= 0; --k) {
-        const int p = piv[k];
-        if (p != k)
-            for (int i = tid; i < bs; i += blockDim.x) {
-                const double t = M[(int64_t)i * bs + k];
-                M[(int64_t)i * bs + k] = M[(int64_t)i * bs + p];
-                M[(int64_t)i * bs + p] = t;
-            }
-        __syncthreads();
-    }
-    if (tid == 0) status[b] = 0;
 }
 
 // s[1] = 1 / s[0]
@@ -453,9 +355,9 @@ int tfs_broyden_set_jacobian(tfs_broyden* b, const double* blocks) {
     int rc = copy_in(b, b->Jstage, blocks, b->jac_len, "tfs_broyden_set_jacobian");
     if (rc) return rc;
     const size_t lds = sizeof(double) * 2 * b->max_block;
-    hipLaunchKernelGGL(block_gauss_jordan_kernel, dim3(b->nblocks), dim3(256), lds, b->stream, b->Jstage, b->mat_off,
-                       b->sizes, b->piv, b->row_off, b->status);
-    rc = check(hipGetLastError(), "Gauss-Jordan");
+    rc = check(k_batched_inverse(b->Jstage, b->mat_off, b->sizes, b->nblocks, b->max_block, b->piv, b->row_off,
+                                 b->status, b->stream),
+               "Gauss-Jordan");
     if (rc) return rc;
     hipError_t e = hipMemcpyAsync(b->host_status.data(), b->status, sizeof(int32_t) * b->nblocks,
                                   hipMemcpyDeviceToHost, b->stream);

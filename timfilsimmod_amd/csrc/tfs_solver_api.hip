@@ -18,6 +18,7 @@
 
 #include "../../include/tfs_solver.h"
 #include "solver_kernels.h"
+#include "dense_kernels.h"
 #include "tfs_api_internal.h"
 
 using namespace tfs;
@@ -77,6 +78,16 @@ struct tfs_solver {
     hipEvent_t flag_ev[2] = {nullptr, nullptr};
     double* host_scal = nullptr;  // pinned, kEstBatch + 8 doubles
     double gs = 0, gb = 0;        // 6 pi mu a for segments / blobs
+    // Block preconditioner (kind 1): filament blocks rebuilt when the
+    // segments move, blob patches when the blobs move.
+    int precond_kind = 0;
+    bool fil_pc_valid = false, blob_pc_valid = false;
+    double *fil_inv = nullptr, *fil_dk = nullptr, *fil_sinv = nullptr, *patch_inv = nullptr;
+    int64_t *fil_off = nullptr, *patch_off = nullptr;
+    int32_t *fil_size = nullptr, *fil_piv = nullptr, *fil_pivoff = nullptr, *fil_status = nullptr;
+    int32_t *patch_blob = nullptr, *patch_start = nullptr, *patch_size = nullptr, *patch_piv = nullptr,
+            *patch_pivoff = nullptr, *patch_status = nullptr;
+    int npatch = 0;
 };
 
 namespace {
@@ -129,8 +140,180 @@ int apply_A(tfs_solver* s, const double* in, double* out, int precision, tfs_sol
     return TFS_OK;
 }
 
+PrecondArgs precond_args(const tfs_solver* s) {
+    PrecondArgs P;
+    P.kind = s->precond_kind;
+    P.gs = s->gs;
+    P.gb = s->gb;
+    P.kphi = s->kphi;
+    P.kth = s->kth;
+    P.fil_inv = s->fil_inv;
+    P.fil_dk = s->fil_dk;
+    P.fil_sinv = s->fil_sinv;
+    P.patch_blob = s->patch_blob;
+    P.patch_start = s->patch_start;
+    P.patch_inv = s->patch_inv;
+    P.patch_off = s->patch_off;
+    P.npatch = s->npatch;
+    return P;
+}
+
+// Device buffer of n elements of T (freed with the handle).
+template <typename T>
+int dev_alloc(T** p, size_t n, const char* what) {
+    if (*p) return TFS_OK;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * std::max<size_t>(n, 1));
+    return e == hipSuccess ? TFS_OK : fail(TFS_ERR_OUT_OF_MEMORY, std::string("block preconditioner: ") + what);
+}
+
+// Status flags of a batched inverse: any singular or non-finite block is an error.
+int check_inverse_status(tfs_solver* s, const int32_t* status, int n, const char* what) {
+    std::vector<int32_t> st(n);
+    hipError_t e = hipMemcpyAsync(st.data(), status, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return hip_fail(e, what);
+    for (int i = 0; i < n; ++i)
+        if (st[i]) return fail(TFS_ERR_NON_FINITE, std::string(what) + ": block " + std::to_string(i) + " is singular");
+    return TFS_OK;
+}
+
+// Morton (Z-order) code of a point quantised to 10 bits per axis in [lo, hi].
+uint32_t morton3(const double* x, const double* lo, const double* span) {
+    uint32_t code = 0;
+    uint32_t q[3];
+    for (int d = 0; d < 3; ++d) {
+        const double t = span[d] > 0 ? (x[d] - lo[d]) / span[d] : 0.0;
+        q[d] = (uint32_t)std::min(1023.0, std::max(0.0, t * 1023.0));
+    }
+    for (int b = 0; b < 10; ++b)
+        for (int d = 0; d < 3; ++d) code |= ((q[d] >> b) & 1u) << (3 * b + d);
+    return code;
+}
+
+// Builds whatever part of the block preconditioner is stale.
+int ensure_block_precond(tfs_solver* s) {
+    if (s->precond_kind != 1) return TFS_OK;
+    const SaddleDims& d = s->dims;
+    const tfs_solver_desc& D = s->desc;
+    int rc = TFS_OK;
+    if (!s->blob_pc_valid && d.nblob > 0) {
+        // patches: consecutive kPatchBlobs blobs in Morton order of the positions
+        std::vector<double> xb(3 * (size_t)d.nblob);
+        hipError_t e = hipMemcpyAsync(xb.data(), s->rpy->x_blobs, xb.size() * 8, hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) return hip_fail(e, "block preconditioner: blob positions");
+        double lo[3], hi[3], span[3];
+        for (int k = 0; k < 3; ++k) lo[k] = hi[k] = xb[k];
+        for (int i = 0; i < d.nblob; ++i)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], xb[3 * i + k]);
+                hi[k] = std::max(hi[k], xb[3 * i + k]);
+            }
+        for (int k = 0; k < 3; ++k) span[k] = hi[k] - lo[k];
+        std::vector<std::pair<uint32_t, int32_t>> key(d.nblob);
+        for (int i = 0; i < d.nblob; ++i) key[i] = {morton3(&xb[3 * i], lo, span), i};
+        std::stable_sort(key.begin(), key.end(),
+                         [](const std::pair<uint32_t, int32_t>& a, const std::pair<uint32_t, int32_t>& b) {
+                             return a.first < b.first;
+                         });
+        const int np = (d.nblob + kPatchBlobs - 1) / kPatchBlobs;
+        std::vector<int32_t> blob(d.nblob), start(np + 1), size(np), pivoff(np);
+        std::vector<int64_t> off(np);
+        int64_t total = 0;
+        for (int i = 0; i < d.nblob; ++i) blob[i] = key[i].second;
+        for (int p = 0; p < np; ++p) {
+            start[p] = p * kPatchBlobs;
+            const int cnt = std::min(kPatchBlobs, d.nblob - start[p]);
+            size[p] = 3 * cnt;
+            off[p] = total;
+            pivoff[p] = 3 * start[p];
+            total += (int64_t)size[p] * size[p];
+        }
+        start[np] = d.nblob;
+        if (s->npatch != np) {
+            for (void* q : {(void*)s->patch_inv, (void*)s->patch_off, (void*)s->patch_blob, (void*)s->patch_start,
+                            (void*)s->patch_size, (void*)s->patch_piv, (void*)s->patch_pivoff, (void*)s->patch_status})
+                if (q) (void)hipFree(q);
+            s->patch_inv = nullptr;
+            s->patch_off = nullptr;
+            s->patch_blob = s->patch_start = s->patch_size = s->patch_piv = s->patch_pivoff = s->patch_status = nullptr;
+        }
+        rc = dev_alloc(&s->patch_inv, (size_t)total, "patch blocks");
+        if (!rc) rc = dev_alloc(&s->patch_off, np, "patch offsets");
+        if (!rc) rc = dev_alloc(&s->patch_blob, d.nblob, "patch blobs");
+        if (!rc) rc = dev_alloc(&s->patch_start, np + 1, "patch starts");
+        if (!rc) rc = dev_alloc(&s->patch_size, np, "patch sizes");
+        if (!rc) rc = dev_alloc(&s->patch_piv, 3 * (size_t)d.nblob, "patch pivots");
+        if (!rc) rc = dev_alloc(&s->patch_pivoff, np, "patch pivot offsets");
+        if (!rc) rc = dev_alloc(&s->patch_status, np, "patch status");
+        if (rc) return rc;
+        s->npatch = np;
+        e = hipMemcpyAsync(s->patch_blob, blob.data(), sizeof(int32_t) * d.nblob, hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(s->patch_start, start.data(), sizeof(int32_t) * (np + 1), hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(s->patch_size, size.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(s->patch_off, off.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(s->patch_pivoff, pivoff.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess)
+            e = k_patch_blocks(s->rpy->x_blobs, s->patch_blob, s->patch_start, s->patch_off, np, D.blob_radius, D.mu,
+                               s->patch_inv, s->stream);
+        if (e == hipSuccess)
+            e = k_batched_inverse(s->patch_inv, s->patch_off, s->patch_size, np, 3 * kPatchBlobs, s->patch_piv,
+                                  s->patch_pivoff, s->patch_status, s->stream);
+        if (e != hipSuccess) return hip_fail(e, "block preconditioner: blob patches");
+        rc = check_inverse_status(s, s->patch_status, np, "block preconditioner: blob patch");
+        if (rc) return rc;
+        s->blob_pc_valid = true;
+    }
+    if (!s->fil_pc_valid && d.nfil > 0) {
+        const int len = 3 * d.nseg;
+        rc = dev_alloc(&s->fil_inv, (size_t)d.nfil * len * len, "filament blocks");
+        if (!rc) rc = dev_alloc(&s->fil_dk, (size_t)d.nfil * 2 * len, "filament couplings");
+        if (!rc) rc = dev_alloc(&s->fil_sinv, (size_t)d.nfil * 4, "filament Schur complements");
+        if (!rc) rc = dev_alloc(&s->fil_piv, (size_t)d.nfil * len, "filament pivots");
+        if (!rc && !s->fil_off) {
+            std::vector<int64_t> off(d.nfil);
+            std::vector<int32_t> size(d.nfil, len), pivoff(d.nfil);
+            for (int m = 0; m < d.nfil; ++m) {
+                off[m] = (int64_t)m * len * len;
+                pivoff[m] = m * len;
+            }
+            rc = dev_alloc(&s->fil_off, d.nfil, "filament offsets");
+            if (!rc) rc = dev_alloc(&s->fil_size, d.nfil, "filament sizes");
+            if (!rc) rc = dev_alloc(&s->fil_pivoff, d.nfil, "filament pivot offsets");
+            if (!rc) rc = dev_alloc(&s->fil_status, d.nfil, "filament status");
+            hipError_t e = rc ? hipSuccess
+                              : hipMemcpyAsync(s->fil_off, off.data(), sizeof(int64_t) * d.nfil,
+                                               hipMemcpyHostToDevice, s->stream);
+            if (!rc && e == hipSuccess)
+                e = hipMemcpyAsync(s->fil_size, size.data(), sizeof(int32_t) * d.nfil, hipMemcpyHostToDevice,
+                                   s->stream);
+            if (!rc && e == hipSuccess)
+                e = hipMemcpyAsync(s->fil_pivoff, pivoff.data(), sizeof(int32_t) * d.nfil, hipMemcpyHostToDevice,
+                                   s->stream);
+            // the host vectors must outlive the copies
+            if (!rc && e == hipSuccess) e = hipStreamSynchronize(s->stream);
+            if (!rc && e != hipSuccess) rc = hip_fail(e, "block preconditioner: filament tables");
+        }
+        if (rc) return rc;
+        hipError_t e = k_fil_blocks(s->rpy->x_segs, d.nfil, d.nseg, D.seg_radius, D.mu, s->fil_inv, s->stream);
+        if (e == hipSuccess)
+            e = k_batched_inverse(s->fil_inv, s->fil_off, s->fil_size, d.nfil, len, s->fil_piv, s->fil_pivoff,
+                                  s->fil_status, s->stream);
+        if (e == hipSuccess) e = k_fil_schur(precond_args(s), d, s->fil_dk, s->fil_sinv, s->stream);
+        if (e != hipSuccess) return hip_fail(e, "block preconditioner: filament blocks");
+        rc = check_inverse_status(s, s->fil_status, d.nfil, "block preconditioner: filament");
+        if (rc) return rc;
+        s->fil_pc_valid = true;
+    }
+    return TFS_OK;
+}
+
 int apply_P(tfs_solver* s, const double* in, double* out) {
-    return check(k_precond(in, out, s->kphi, s->kth, s->dims, s->gs, s->gb, s->stream), "preconditioner");
+    return check(k_precond(in, out, precond_args(s), s->dims, s->stream), "preconditioner");
 }
 
 // r = b - A x (x may be known to be zero: then r = b).
@@ -188,12 +371,12 @@ int gmres(tfs_solver* s, int precision, const double* b, double* x, double* r, d
             // v_j = w_{j-1} / ||w_{j-1}|| (v_0 = r / ||r||), z = P^{-1} v_j; for
             // j > 0 the same launch closes step j - 1.
             if (j == 0)
-                rc = check(k_scale_precond(r, nullptr, 1.0 / rnorm, vj, s->z, s->kphi, s->kth, s->dims, s->gs, s->gb,
-                                           nullptr, s->stream), "v0");
+                rc = check(k_scale_precond(r, nullptr, 1.0 / rnorm, vj, s->z, precond_args(s), s->dims, nullptr,
+                                           s->stream), "v0");
             else
                 rc = check(k_arnoldi_close_next(s->pa, s->GA, s->H, ldh, j - 1, s->cs, s->sn, s->g, s->scal, s->est,
-                                                s->stop, abs_tol, s->w, vj, s->z, s->kphi, s->kth, s->dims, s->gs,
-                                                s->gb, s->stream),
+                                                s->stop, abs_tol, s->w, vj, s->z, precond_args(s), s->dims,
+                                                s->stream),
                            "close and normalise");
             if (!rc) rc = apply_A(s, s->z, s->w, precision, nullptr, s->stop);
             if (!rc)
@@ -216,8 +399,7 @@ int gmres(tfs_solver* s, int precision, const double* b, double* x, double* r, d
         }
         // close the last enqueued step
         rc = check(k_arnoldi_close_next(s->pa, s->GA, s->H, ldh, enq - 1, s->cs, s->sn, s->g, s->scal, s->est, s->stop,
-                                        abs_tol, s->w, nullptr, nullptr, s->kphi, s->kth, s->dims, s->gs, s->gb,
-                                        s->stream),
+                                        abs_tol, s->w, nullptr, nullptr, precond_args(s), s->dims, s->stream),
                    "close");
         if (rc) return rc;
         int32_t flag = 0;
@@ -293,8 +475,10 @@ int solve_device(tfs_solver* s, const double* b, double* x, tfs_solve_stats* st)
     const int n = (int)s->dims.n;
     const tfs_solver_desc& D = s->desc;
     *st = tfs_solve_stats{};
+    int rc = ensure_block_precond(s);
+    if (rc) return rc;
     double bnorm = 0.0;
-    int rc = norm(s, b, &bnorm);
+    rc = norm(s, b, &bnorm);
     if (rc) return rc;
     if (!std::isfinite(bnorm)) return fail(TFS_ERR_NON_FINITE, "solve: non-finite right-hand side");
     if (bnorm == 0.0) {  // mobility_solver.cpp:2127-2194
@@ -435,7 +619,23 @@ int tfs_solver_destroy(tfs_solver* s) {
     for (hipEvent_t ev : s->flag_ev)
         if (ev) hipEventDestroy(ev);
     if (s->rpy) tfs_rpy_destroy(s->rpy);
+    for (void* q : {(void*)s->fil_inv, (void*)s->fil_dk, (void*)s->fil_sinv, (void*)s->patch_inv, (void*)s->fil_off,
+                    (void*)s->patch_off, (void*)s->fil_size, (void*)s->fil_piv, (void*)s->fil_pivoff,
+                    (void*)s->fil_status, (void*)s->patch_blob, (void*)s->patch_start, (void*)s->patch_size,
+                    (void*)s->patch_piv, (void*)s->patch_pivoff, (void*)s->patch_status})
+        if (q) (void)hipFree(q);
     delete s;
+    return TFS_OK;
+}
+
+int tfs_solver_set_preconditioner(tfs_solver* s, int32_t kind) {
+    if (!s) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_preconditioner: null handle");
+    if (kind != TFS_PRECOND_REFERENCE && kind != TFS_PRECOND_BLOCKS)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_preconditioner: unknown kind");
+    if (kind == TFS_PRECOND_BLOCKS && s->desc.num_seg > kMaxBlockFilSeg)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_preconditioner: block form needs num_seg <= " +
+                                                  std::to_string(kMaxBlockFilSeg));
+    s->precond_kind = kind;
     return TFS_OK;
 }
 
@@ -460,6 +660,8 @@ int tfs_solver_set_geometry(tfs_solver* s, const double* x_segs, const double* x
     const hipMemcpyKind k = kind_in(memory);
     hipError_t e = hipSuccess;
     if (x_segs || x_blobs) invalidate_near(s->rpy);
+    if (x_segs || vel_dir_phase || vel_dir_angle) s->fil_pc_valid = false;
+    if (x_blobs) s->blob_pc_valid = false;
     if (x_segs && ns3) e = hipMemcpyAsync(s->rpy->x_segs, x_segs, ns3 * 8, k, s->stream);
     if (e == hipSuccess && x_blobs && nb3) e = hipMemcpyAsync(s->rpy->x_blobs, x_blobs, nb3 * 8, k, s->stream);
     if (e == hipSuccess && vel_dir_phase && ns3) e = hipMemcpyAsync(s->kphi, vel_dir_phase, ns3 * 8, k, s->stream);
@@ -499,6 +701,7 @@ int tfs_solver_precondition(tfs_solver* s, const double* in, double* out, int32_
         din = s->b;
         dout = s->x;
     }
+    if (!rc) rc = ensure_block_precond(s);
     if (!rc) rc = apply_P(s, din, dout);
     if (!rc && memory == TFS_MEMORY_HOST)
         rc = check(hipMemcpyAsync(out, dout, n * 8, hipMemcpyDeviceToHost, s->stream), "download");
@@ -665,6 +868,7 @@ int tfs_beat_step(tfs_beat* b, tfs_solve_stats* stats) {
     rc = check(k_beat(ba, s->rpy->x_segs, s->kphi, s->kth, st), "beat kinematics");
     if (rc) return rc;
     invalidate_near(s->rpy);
+    s->fil_pc_valid = false;
     RhsArgs ra;
     ra.d = s->dims;
     ra.phase = b->phase;

@@ -20,6 +20,8 @@ from . import beat as B
 
 SOLVE_FP64 = 0
 SOLVE_MIXED = 1
+PRECOND_REFERENCE = 0   # D = I/(6 pi mu a), the reference's preconditioner
+PRECOND_BLOCKS = 1      # D = filament self-mobility blocks + blob patches
 
 
 class SolverDesc(ctypes.Structure):
@@ -75,6 +77,7 @@ L.extra_signatures({
     "tfs_beat_solver": (ctypes.c_int, [_dp, ctypes.POINTER(_dp)]),
     "tfs_beat_set_mode": (ctypes.c_int, [_dp, _i32]),
     "tfs_beat_set_warm_start": (ctypes.c_int, [_dp, _i32]),
+    "tfs_solver_set_preconditioner": (ctypes.c_int, [_dp, _i32]),
 })
 
 
@@ -124,6 +127,10 @@ class Solver:
         if self.h:
             L.lib().tfs_solver_destroy(self.h)
             self.h = ctypes.c_void_p()
+
+    def set_preconditioner(self, kind: int):
+        """PRECOND_REFERENCE (default) or PRECOND_BLOCKS for later solves."""
+        L.check(L.lib().tfs_solver_set_preconditioner(self.h, kind))
 
     def __del__(self):
         try:
@@ -254,7 +261,7 @@ class BeatSimulation:
     def __init__(self, body, *, mode=SOLVE_FP64, dt=1.0 / 300.0, restart=50, tol=1e-8, inner_tol=1e-3,
                  max_cycles=20, max_refinements=10, dynamic_phase=True, dynamic_angle=True,
                  gen_force_factor=1.0, torsion_factor=0.046, stream=None, setup: BeatSetup | None = None,
-                 warm_start: int = 0):
+                 warm_start: int = 0, precond: int = PRECOND_REFERENCE):
         self.setup = setup or beat_setup(body)
         su = self.setup
         d = BeatDesc()
@@ -284,6 +291,8 @@ class BeatSimulation:
         self.n = n.value
         if warm_start:
             self.set_warm_start(warm_start)
+        if precond != PRECOND_REFERENCE:
+            L.check(L.lib().tfs_solver_set_preconditioner(sp, precond))
 
     def close(self):
         if self.h:
