@@ -66,7 +66,8 @@ def setup_case(nfil, nseg, nblob, radius, **kw):
     return body, geo, s
 
 
-@pytest.mark.parametrize("nfil,nseg,nblob", [(3, 6, 40), (8, 20, 300), (4, 5, 0), (2, 20, 37)])
+# (2, 24, 20): 72-row filament blocks take the general (pivoting, multi-register) paths
+@pytest.mark.parametrize("nfil,nseg,nblob", [(3, 6, 40), (8, 20, 300), (4, 5, 0), (2, 20, 37), (2, 24, 20)])
 def test_block_preconditioner_inverts_the_block_saddle_matrix(nfil, nseg, nblob):
     print(f"case block precond nfil={nfil} nseg={nseg} nblob={nblob}", flush=True)
     body, geo, s = setup_case(nfil, nseg, nblob, 8.0)

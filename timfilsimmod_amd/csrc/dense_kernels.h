@@ -15,4 +15,11 @@ namespace tfs {
 hipError_t k_batched_inverse(double* A, const int64_t* off, const int32_t* sizes, int nblocks, int max_size,
                              int32_t* piv, const int32_t* piv_off, int32_t* status, hipStream_t stream);
 
+// The same for symmetric positive definite blocks of at most 64 rows, without
+// pivoting, one wave per block (max_size > 64 is an invalid value).  status[b]
+// = 1 if a pivot of block b is not positive: the block is then not inverted
+// and the caller falls back to k_batched_inverse on a fresh copy.
+hipError_t k_batched_inverse_spd(double* A, const int64_t* off, const int32_t* sizes, int nblocks, int max_size,
+                                 int32_t* status, hipStream_t stream);
+
 }  // namespace tfs

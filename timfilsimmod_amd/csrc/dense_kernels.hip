@@ -1,5 +1,6 @@
 // dense_kernels.hip -- batched Gauss-Jordan inverse of small dense blocks.
 #include "dense_kernels.h"
+#include "wave_ops.h"
 
 namespace tfs {
 namespace {
@@ -103,7 +104,65 @@ __global__ __launch_bounds__(256) void block_gauss_jordan_kernel(double* __restr
     if (tid == 0) status[b] = 0;
 }
 
+// In-place inverse of symmetric positive definite row-major blocks of at most
+// 64 rows by Gauss-Jordan elimination without pivoting (SPD: every pivot is
+// positive), one wave per block, four blocks per workgroup.  Lane j keeps
+// column j in registers; the pivot row and column come from the owning lanes
+// by readlane, so no LDS and no barrier is needed.  status[b] = 1 if a pivot
+// is not positive and finite (the block is then left partly eliminated).
+__global__ __launch_bounds__(256) void spd_inverse_wave_kernel(double* __restrict__ A,
+                                                               const int64_t* __restrict__ off,
+                                                               const int32_t* __restrict__ sizes, int nblocks,
+                                                               int32_t* __restrict__ status) {
+    constexpr int W = 64;
+    const int b = blockIdx.x * 4 + (int)(threadIdx.x / W);
+    const int lane = threadIdx.x & (W - 1);
+    if (b >= nblocks) return;
+    const int n = sizes[b];
+    double* M = A + off[b];
+    const bool mine = lane < n;
+    double col[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) col[i] = (i < n && mine) ? M[(int64_t)i * n + lane] : 0.0;
+    int bad = 0;
+    for (int k = 0; k < n; ++k) {
+        double akj = 0.0;
+#pragma unroll
+        for (int i = 0; i < W; ++i)
+            if (i == k) akj = col[i];
+        const double d = lane_bcast(akj, k);
+        if (!(d > 0.0) || !__builtin_isfinite(d)) {
+            bad = 1;
+            break;  // uniform: every lane read the same d
+        }
+        const double inv = 1.0 / d;
+        const double rk = lane == k ? inv : akj * inv;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            if (i < n) {
+                const double f = lane_bcast(col[i], k);
+                col[i] = i == k ? rk : (lane == k ? -f * inv : col[i] - f * rk);
+            }
+        }
+    }
+    if (!bad && mine) {
+#pragma unroll
+        for (int i = 0; i < W; ++i)
+            if (i < n) M[(int64_t)i * n + lane] = col[i];
+    }
+    if (lane == 0) status[b] = bad;
+}
+
 }  // namespace
+
+hipError_t k_batched_inverse_spd(double* A, const int64_t* off, const int32_t* sizes, int nblocks, int max_size,
+                                 int32_t* status, hipStream_t stream) {
+    if (nblocks <= 0) return hipSuccess;
+    if (max_size > 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(spd_inverse_wave_kernel, dim3((unsigned)((nblocks + 3) / 4)), dim3(256), 0, stream, A, off,
+                       sizes, nblocks, status);
+    return hipGetLastError();
+}
 
 hipError_t k_batched_inverse(double* A, const int64_t* off, const int32_t* sizes, int nblocks, int max_size,
                              int32_t* piv, const int32_t* piv_off, int32_t* status, hipStream_t stream) {

@@ -17,6 +17,7 @@
 
 #include "rpy_pair.h"
 #include "solver_kernels.h"
+#include "wave_ops.h"
 
 namespace tfs {
 namespace {
@@ -537,6 +538,26 @@ __device__ void scale_precond_body(const double* __restrict__ w, double sc, doub
     }
 }
 
+// y_lane = sum_c B[c][lane] x_c for a symmetric n x n block (n <= 64), row
+// `lane` of the product on each lane (0 past n); x_c is lane c's x.  The
+// column loads go out sixteen at a time, so a wave keeps many in flight.
+__device__ __forceinline__ double small_symv(const double* __restrict__ B, int n, int lane, double x) {
+    const int r = lane < n ? lane : n - 1;
+    double y = 0.0;
+#pragma unroll
+    for (int c0 = 0; c0 < kWave; c0 += 16) {
+        if (c0 < n) {
+            double b[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) b[u] = c0 + u < n ? B[(int64_t)(c0 + u) * n + r] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+                if (c0 + u < n) y += b[u] * lane_bcast(x, c0 + u);
+        }
+    }
+    return lane < n ? y : 0.0;
+}
+
 // One filament (segment rows and its phase / angle rows) or one blob patch of
 // the block preconditioner, by one wave: v = w * sc (if v), out = P^{-1} v.
 // Rows are spread over the lanes (row r on lane r % 64, register r / 64); the
@@ -545,6 +566,47 @@ __device__ void block_precond_wave(const PrecondArgs& P, const SaddleDims& d, in
                                    const double* __restrict__ w, double sc, double* __restrict__ v,
                                    double* __restrict__ out) {
     constexpr int RMAX = 3 * kMaxBlockFilSeg / kWave;   // registers per lane
+    if (item < d.nfil && 3 * d.nseg <= kWave) {
+        // one row per lane
+        const int m = item;
+        const int len = 3 * d.nseg;
+        const int64_t seg0 = (int64_t)m * len;
+        const bool act = lane < len;
+        const double x = act ? w[seg0 + lane] * sc : 0.0;
+        if (v && act) v[seg0 + lane] = x;
+        const double du = small_symv(P.fil_inv + (int64_t)m * len * len, len, lane, x);
+        double tp = wave_sum(act ? P.kphi[seg0 + lane] * du : 0.0);
+        double tt = wave_sum(act ? P.kth[seg0 + lane] * du : 0.0);
+        double yp = 0.0, ya = 0.0;
+        if (lane == 0) {
+            const double* Si = P.fil_sinv + 4 * (int64_t)m;
+            const double vp = d.dyn_phase ? w[d.off_phase + m] * sc : 0.0;
+            const double va = d.dyn_angle ? w[d.off_angle + m] * sc : 0.0;
+            if (d.dyn_phase && d.dyn_angle) {
+                yp = -(Si[0] * (vp + tp) + Si[1] * (va + tt));
+                ya = -(Si[2] * (vp + tp) + Si[3] * (va + tt));
+            } else if (d.dyn_phase) {
+                yp = -Si[0] * (vp + tp);
+            } else if (d.dyn_angle) {
+                ya = -Si[0] * (va + tt);
+            }
+            if (d.dyn_phase) {
+                if (v) v[d.off_phase + m] = vp;
+                out[d.off_phase + m] = yp;
+            }
+            if (d.dyn_angle) {
+                if (v) v[d.off_angle + m] = va;
+                out[d.off_angle + m] = ya;
+            }
+        }
+        yp = lane_bcast(yp, 0);
+        ya = lane_bcast(ya, 0);
+        if (act) {
+            const double* dkp = P.fil_dk + (int64_t)m * 2 * len;
+            out[seg0 + lane] = du + dkp[lane] * yp + dkp[len + lane] * ya;
+        }
+        return;
+    }
     if (item < d.nfil) {
         const int m = item;
         const int len = 3 * d.nseg;
@@ -631,11 +693,7 @@ __device__ void block_precond_wave(const PrecondArgs& P, const SaddleDims& d, in
         x = w[idx] * sc;
         if (v) v[idx] = x;
     }
-    double o = 0.0;
-    for (int c = 0; c < rows; ++c) {
-        const double xc = __shfl(x, c, kWave);
-        if (lane < rows) o += Bi[(int64_t)c * rows + lane] * xc;
-    }
+    const double o = small_symv(Bi, rows, lane, x);
     if (lane < rows) out[idx] = o;
 }
 
@@ -796,6 +854,21 @@ __global__ __launch_bounds__(256) void patch_blocks_kernel(const double* __restr
     }
 }
 
+// The inverse of the Schur complement [s11 s12; s21 s22] of the dynamic
+// couplings (2 x 2, or 1 x 1 in S[0]).
+__device__ void write_sinv(double* S, const SaddleDims& d, double s11, double s12, double s21, double s22) {
+    if (d.dyn_phase && d.dyn_angle) {
+        const double det = s11 * s22 - s12 * s21;
+        S[0] = s22 / det;
+        S[1] = -s12 / det;
+        S[2] = -s21 / det;
+        S[3] = s11 / det;
+    } else {
+        S[0] = d.dyn_phase ? 1.0 / s11 : (d.dyn_angle ? 1.0 / s22 : 0.0);
+        S[1] = S[2] = S[3] = 0.0;
+    }
+}
+
 // fil_dk = D_m^{-1} [K_phi K_theta] and fil_sinv = (K^T D_m^{-1} K)^{-1} (the
 // dynamic couplings only), one wave per filament.
 __global__ __launch_bounds__(256) void fil_schur_kernel(PrecondArgs P, SaddleDims d, double* __restrict__ dk,
@@ -807,6 +880,19 @@ __global__ __launch_bounds__(256) void fil_schur_kernel(PrecondArgs P, SaddleDim
     const int len = 3 * d.nseg;
     const int64_t seg0 = (int64_t)m * len;
     const double* Bi = P.fil_inv + (int64_t)m * len * len;
+    if (len <= kWave) {
+        const bool act = lane < len;
+        const double kp1 = act ? P.kphi[seg0 + lane] : 0.0, kt1 = act ? P.kth[seg0 + lane] : 0.0;
+        const double up1 = small_symv(Bi, len, lane, kp1), ut1 = small_symv(Bi, len, lane, kt1);
+        if (act) {
+            dk[(int64_t)m * 2 * len + lane] = up1;
+            dk[(int64_t)m * 2 * len + len + lane] = ut1;
+        }
+        const double s11 = wave_sum(kp1 * up1), s12 = wave_sum(kp1 * ut1);
+        const double s21 = wave_sum(kt1 * up1), s22 = wave_sum(kt1 * ut1);
+        if (lane == 0) write_sinv(sinv + 4 * (int64_t)m, d, s11, s12, s21, s22);
+        return;
+    }
     double kp[RMAX], kt[RMAX], up[RMAX], ut[RMAX];
 #pragma unroll
     for (int q = 0; q < RMAX; ++q) {
@@ -850,19 +936,7 @@ __global__ __launch_bounds__(256) void fil_schur_kernel(PrecondArgs P, SaddleDim
     s12 = wave_sum(s12);
     s21 = wave_sum(s21);
     s22 = wave_sum(s22);
-    if (lane == 0) {
-        double* S = sinv + 4 * (int64_t)m;
-        if (d.dyn_phase && d.dyn_angle) {
-            const double det = s11 * s22 - s12 * s21;
-            S[0] = s22 / det;
-            S[1] = -s12 / det;
-            S[2] = -s21 / det;
-            S[3] = s11 / det;
-        } else {
-            S[0] = d.dyn_phase ? 1.0 / s11 : (d.dyn_angle ? 1.0 / s22 : 0.0);
-            S[1] = S[2] = S[3] = 0.0;
-        }
-    }
+    if (lane == 0) write_sinv(sinv + 4 * (int64_t)m, d, s11, s12, s21, s22);
 }
 
 // ---------------------------------------------------------------------------

@@ -190,6 +190,33 @@ uint32_t morton3(const double* x, const double* lo, const double* span) {
     return code;
 }
 
+// Assembles blocks (by `assemble`) and inverts them in place: the one-wave SPD
+// elimination when they have at most 64 rows, else -- or if a pivot was not
+// positive -- re-assembly and Gauss-Jordan with partial pivoting.  Any block
+// still singular is an error.  Synchronises the stream (status read-back).
+template <typename F>
+int invert_blocks(tfs_solver* s, F assemble, double* blk, const int64_t* off, const int32_t* sizes, int n,
+                  int max_size, int32_t* piv, const int32_t* pivoff, int32_t* status, const char* what) {
+    hipError_t e = assemble();
+    if (e != hipSuccess) return hip_fail(e, what);
+    if (max_size <= 64) {
+        e = k_batched_inverse_spd(blk, off, sizes, n, max_size, status, s->stream);
+        if (e != hipSuccess) return hip_fail(e, what);
+        std::vector<int32_t> st(n);
+        e = hipMemcpyAsync(st.data(), status, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) return hip_fail(e, what);
+        bool ok = true;
+        for (int i = 0; i < n && ok; ++i) ok = st[i] == 0;
+        if (ok) return TFS_OK;
+        e = assemble();
+        if (e != hipSuccess) return hip_fail(e, what);
+    }
+    e = k_batched_inverse(blk, off, sizes, n, max_size, piv, pivoff, status, s->stream);
+    if (e != hipSuccess) return hip_fail(e, what);
+    return check_inverse_status(s, status, n, what);
+}
+
 // Builds whatever part of the block preconditioner is stale.
 int ensure_block_precond(tfs_solver* s) {
     if (s->precond_kind != 1) return TFS_OK;
@@ -257,14 +284,12 @@ int ensure_block_precond(tfs_solver* s) {
             e = hipMemcpyAsync(s->patch_off, off.data(), sizeof(int64_t) * np, hipMemcpyHostToDevice, s->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(s->patch_pivoff, pivoff.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s->stream);
-        if (e == hipSuccess)
-            e = k_patch_blocks(s->rpy->x_blobs, s->patch_blob, s->patch_start, s->patch_off, np, D.blob_radius, D.mu,
-                               s->patch_inv, s->stream);
-        if (e == hipSuccess)
-            e = k_batched_inverse(s->patch_inv, s->patch_off, s->patch_size, np, 3 * kPatchBlobs, s->patch_piv,
-                                  s->patch_pivoff, s->patch_status, s->stream);
-        if (e != hipSuccess) return hip_fail(e, "block preconditioner: blob patches");
-        rc = check_inverse_status(s, s->patch_status, np, "block preconditioner: blob patch");
+        if (e != hipSuccess) return hip_fail(e, "block preconditioner: patch tables");
+        rc = invert_blocks(s, [&]() {
+            return k_patch_blocks(s->rpy->x_blobs, s->patch_blob, s->patch_start, s->patch_off, np, D.blob_radius,
+                                  D.mu, s->patch_inv, s->stream);
+        }, s->patch_inv, s->patch_off, s->patch_size, np, 3 * kPatchBlobs, s->patch_piv, s->patch_pivoff,
+                           s->patch_status, "block preconditioner: blob patch");
         if (rc) return rc;
         s->blob_pc_valid = true;
     }
@@ -299,13 +324,12 @@ int ensure_block_precond(tfs_solver* s) {
             if (!rc && e != hipSuccess) rc = hip_fail(e, "block preconditioner: filament tables");
         }
         if (rc) return rc;
-        hipError_t e = k_fil_blocks(s->rpy->x_segs, d.nfil, d.nseg, D.seg_radius, D.mu, s->fil_inv, s->stream);
-        if (e == hipSuccess)
-            e = k_batched_inverse(s->fil_inv, s->fil_off, s->fil_size, d.nfil, len, s->fil_piv, s->fil_pivoff,
-                                  s->fil_status, s->stream);
-        if (e == hipSuccess) e = k_fil_schur(precond_args(s), d, s->fil_dk, s->fil_sinv, s->stream);
-        if (e != hipSuccess) return hip_fail(e, "block preconditioner: filament blocks");
-        rc = check_inverse_status(s, s->fil_status, d.nfil, "block preconditioner: filament");
+        rc = invert_blocks(s, [&]() {
+            return k_fil_blocks(s->rpy->x_segs, d.nfil, d.nseg, D.seg_radius, D.mu, s->fil_inv, s->stream);
+        }, s->fil_inv, s->fil_off, s->fil_size, d.nfil, len, s->fil_piv, s->fil_pivoff, s->fil_status,
+                           "block preconditioner: filament");
+        if (rc) return rc;
+        rc = check(k_fil_schur(precond_args(s), d, s->fil_dk, s->fil_sinv, s->stream), "block preconditioner: Schur");
         if (rc) return rc;
         s->fil_pc_valid = true;
     }
