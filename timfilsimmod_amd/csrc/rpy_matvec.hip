@@ -15,8 +15,8 @@
 // The pair loop is branch-free: it evaluates the far-field RPY forms for
 // every pair, with r^2 clamped below at (ai+aj)^2 (the self pair included).
 // The pairs that clamp touches -- r^2 <= (ai+aj)^2, a handful per particle --
-// are listed once per geometry by build_near_list, using the pair loop's own
-// arithmetic for r^2, and the reduce kernel takes out their clamped far-field
+// are listed once per geometry by build_near_list (a hashed uniform grid),
+// using the pair loop's own arithmetic for r^2, and the reduce kernel takes out their clamped far-field
 // value and adds the reference's overlap and inner-branch formulas in FP64.
 // Every pair is counted exactly once.
 //
@@ -271,49 +271,140 @@ __global__ __launch_bounds__(BLOCK) void rpy_pairs_persistent_kernel(const Matve
     }
 }
 
-// Near-pair listing: the same work decomposition as the pair kernel (one
-// target per lane, BLOCK = 256), testing r^2 <= (ai+aj)^2 with the pair
-// kernel's arithmetic.  Entries are appended with atomics; the reduce kernel
-// sorts them before summing.
-template <typename T>
-__global__ __launch_bounds__(256) void near_build_kernel(const MatvecItem* __restrict__ items,
-                                                         const MatvecTile* __restrict__ tiles, MatvecArgs a,
-                                                         int32_t* __restrict__ cnt, int32_t* __restrict__ idx,
-                                                         int cap, int32_t* __restrict__ max_cnt) {
-    constexpr int REC = rec_len<T, 3>();
-    constexpr bool F32 = sizeof(T) == 4;
-    __shared__ __align__(16) T lds[256 * REC];
-    const MatvecItem it = items[blockIdx.x];
-    const MatvecTile tl = tiles[it.tile];
+// ---------------------------------------------------------------------------
+// Near-pair listing on a hashed uniform grid.  Cells have side h >= (1 + 1e-5)
+// max(ai + aj), so every pair that either arithmetic (FP64, or FP32 fixed
+// point, whose r^2 errs by well under 1e-5 relative) could classify as
+// overlapping lies in neighbouring cells; each candidate is then classified
+// with the pair kernel's own staging and separation functions for the
+// requested precision, so the lists and the pair loop's clamp agree bit for
+// bit.  One thread per target, entries appended in visiting order (the
+// reduce kernel's order comes from near_sort_kernel).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void particle_pos(const MatvecArgs& a, int p, double* x) {
+    const CView v = p < a.ns ? a.xs : a.xb;
+    const int i = p < a.ns ? p : p - a.ns;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) x[k] = v.p[(int64_t)i * v.ld_p + k * v.ld_c];
+}
+
+__device__ __forceinline__ int cell_coord(double x, double inv_h) {
+    const double c = __builtin_floor(x * inv_h);
+    return (int)(c < -1073741824.0 ? -1073741824.0 : (c > 1073741824.0 ? 1073741824.0 : c));
+}
+
+__device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, uint32_t mask) {
+    return ((uint32_t)cx * 73856093u ^ (uint32_t)cy * 19349663u ^ (uint32_t)cz * 83492791u) & mask;
+}
+
+__global__ __launch_bounds__(256) void grid_body_kernel(const int4* __restrict__ ranges, int ns,
+                                                       int32_t* __restrict__ body_of) {
+    const int4 r = ranges[blockIdx.x];
+    for (int i = r.x + threadIdx.x; i < r.y; i += 256) body_of[i] = blockIdx.x;
+    for (int i = r.z + threadIdx.x; i < r.w; i += 256) body_of[ns + i] = blockIdx.x;
+}
+
+__global__ __launch_bounds__(256) void grid_count_kernel(MatvecArgs a, int n, double inv_h, uint32_t mask,
+                                                        int32_t* __restrict__ bucket_of,
+                                                        int32_t* __restrict__ fill) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    double x[3];
+    particle_pos(a, p, x);
+    const uint32_t b = cell_hash(cell_coord(x[0], inv_h), cell_coord(x[1], inv_h), cell_coord(x[2], inv_h), mask);
+    bucket_of[p] = (int32_t)b;
+    atomicAdd(&fill[b], 1);
+}
+
+// start = exclusive prefix sum of fill (table entries, a power of two >= 1024),
+// start[table] = total; fill is cleared for the scatter.  One workgroup.
+__global__ __launch_bounds__(1024) void grid_scan_kernel(int32_t* __restrict__ fill, int32_t* __restrict__ start,
+                                                        int table) {
+    __shared__ int32_t part[1024];
     const int tid = threadIdx.x;
-    const double ai = tl.cls == 0 ? a.a_seg : a.a_blob;
-    const double aj = it.src_cls == 0 ? a.a_seg : a.a_blob;
-    const double lam = F32 ? a.lam : 1.0;
-    const T sum_a2 = T(lam * lam * ((ai + aj) * (ai + aj)));
-    const int t_local = tl.t_begin + tid;
-    const bool active = t_local < tl.t_end;
-    const int t = active ? t_local : tl.t_end - 1;
+    const int per = table / 1024;
+    const int b0 = tid * per;
+    int32_t sum = 0;
+    for (int i = 0; i < per; ++i) sum += fill[b0 + i];
+    part[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int32_t v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int32_t run = part[tid] - sum;  // exclusive prefix of this thread's chunk
+    for (int i = 0; i < per; ++i) {
+        const int32_t c = fill[b0 + i];
+        start[b0 + i] = run;
+        fill[b0 + i] = 0;
+        run += c;
+    }
+    if (tid == 1023) start[table] = part[1023];
+}
+
+__global__ __launch_bounds__(256) void grid_scatter_kernel(int n, const int32_t* __restrict__ bucket_of,
+                                                          const int32_t* __restrict__ start,
+                                                          int32_t* __restrict__ fill, int32_t* __restrict__ order) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    const int32_t b = bucket_of[p];
+    order[start[b] + atomicAdd(&fill[b], 1)] = p;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void near_grid_kernel(MatvecArgs a, int n, double inv_h, uint32_t mask,
+                                                       const int32_t* __restrict__ start,
+                                                       const int32_t* __restrict__ order,
+                                                       const int32_t* __restrict__ body_of,
+                                                       int32_t* __restrict__ cnt, int32_t* __restrict__ idx, int cap,
+                                                       int32_t* __restrict__ max_cnt) {
+    constexpr bool F32 = sizeof(T) == 4;
+    constexpr int REC = rec_len<T, 3>();
+    const int tg = blockIdx.x * 256 + threadIdx.x;
+    if (tg >= n) return;
+    const int cls = tg < a.ns ? 0 : 1;
+    const int t = cls == 0 ? tg : tg - a.ns;
     using XT = typename std::conditional<F32, int32_t, double>::type;
     XT xi[3];
-    load_target(a, tl.cls, t, xi);
-    const int tg = (tl.cls == 0 ? 0 : a.ns) + t;  // global target index
-    const int src_base = it.src_cls == 0 ? 0 : a.ns;
-    for (int s0 = it.s_begin; s0 < it.s_end; s0 += 256) {
-        const int n = (it.s_end - s0) < 256 ? (it.s_end - s0) : 256;
-        __syncthreads();
-        if (tid < n) stage_source<T, 3>(lds + tid * REC, a, it.src_cls, s0 + tid, false);
-        __syncthreads();
-        if (!active) continue;
-        for (int j = 0; j < n; ++j) {
-            T dx, dy, dz;
-            const T r2 = separation(xi, lds + j * REC, dx, dy, dz);
-            if (!(r2 > sum_a2)) {
-                const int k = atomicAdd(&cnt[tg], 1);
-                if (k < cap) idx[(int64_t)tg * cap + k] = src_base + s0 + j;
-                else atomicMax(max_cnt, k + 1);
+    load_target(a, cls, t, xi);
+    double xd[3];
+    particle_pos(a, tg, xd);
+    const int cx = cell_coord(xd[0], inv_h), cy = cell_coord(xd[1], inv_h), cz = cell_coord(xd[2], inv_h);
+    const int body = body_of[tg];
+    const double ai = cls == 0 ? a.a_seg : a.a_blob;
+    const double lam = F32 ? a.lam : 1.0;
+    int k = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int nx = cx + dx, ny = cy + dy, nz = cz + dz;
+                const uint32_t b = cell_hash(nx, ny, nz, mask);
+                for (int q = start[b]; q < start[b + 1]; ++q) {
+                    const int j = order[q];
+                    if (body_of[j] != body) continue;
+                    double xj[3];
+                    particle_pos(a, j, xj);
+                    // the bucket may also hold other cells: visit j from its own cell only
+                    if (cell_coord(xj[0], inv_h) != nx || cell_coord(xj[1], inv_h) != ny ||
+                        cell_coord(xj[2], inv_h) != nz)
+                        continue;
+                    const int jcls = j < a.ns ? 0 : 1;
+                    T rec[REC];
+                    stage_source<T, 3>(rec, a, jcls, jcls == 0 ? j : j - a.ns, false);
+                    const double aj = jcls == 0 ? a.a_seg : a.a_blob;
+                    const T sum_a2 = T(lam * lam * ((ai + aj) * (ai + aj)));
+                    T ddx, ddy, ddz;
+                    const T r2 = separation(xi, rec, ddx, ddy, ddz);
+                    if (!(r2 > sum_a2)) {
+                        if (k < cap) idx[(int64_t)tg * cap + k] = j;
+                        ++k;
+                    }
+                }
             }
-        }
-    }
+    cnt[tg] = k;
+    if (k > cap) atomicMax(max_cnt, k);
 }
 
 __global__ void fill_int_kernel(int32_t* p, int32_t v, int64_t n) {
@@ -542,26 +633,51 @@ void build_worklist(const std::vector<BodyRange>& bodies, int tile_targets, int 
     }
 }
 
-hipError_t build_near_list(int precision, const MatvecArgs& a, const MatvecItem* d_items, int n_items,
-                           const MatvecTile* d_tiles, NearList& near, int n_targets, int* max_count,
-                           int* host_scratch, hipStream_t stream) {
-    const int64_t n = (int64_t)n_targets;
-    hipLaunchKernelGGL(fill_int_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, near.cnt, 0, n);
+void free_grid(NearGrid& g) {
+    for (int32_t* p : {g.start, g.fill, g.bucket_of, g.order, g.body_of})
+        if (p) (void)hipFree(p);
+    g = NearGrid{};
+}
+
+hipError_t build_near_list(int precision, const MatvecArgs& a, NearGrid& grid, const int32_t* body_ranges, int nbody,
+                           NearList& near, int n_targets, int* max_count, int* host_scratch, hipStream_t stream) {
+    const int n = n_targets;
+    *max_count = 0;
+    if (n <= 0) return hipSuccess;
+    int table = 1024;
+    while (table < 2 * n) table <<= 1;
+    if (grid.n != n || grid.table != table) {
+        free_grid(grid);
+        TFS_HIP_CHECK(hipMalloc(&grid.start, sizeof(int32_t) * (table + 1)));
+        TFS_HIP_CHECK(hipMalloc(&grid.fill, sizeof(int32_t) * table));
+        TFS_HIP_CHECK(hipMalloc(&grid.bucket_of, sizeof(int32_t) * n));
+        TFS_HIP_CHECK(hipMalloc(&grid.order, sizeof(int32_t) * n));
+        TFS_HIP_CHECK(hipMalloc(&grid.body_of, sizeof(int32_t) * n));
+        grid.n = n;
+        grid.table = table;
+    }
+    const double h = 2.0 * std::max(a.a_seg, a.a_blob) * (1.0 + 1e-5);
+    const double inv_h = 1.0 / h;
+    const uint32_t mask = (uint32_t)table - 1;
+    const unsigned gn = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(fill_int_kernel, dim3((unsigned)((table + 255) / 256)), dim3(256), 0, stream, grid.fill, 0,
+                       (int64_t)table);
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, stream, near.max_cnt, 0, (int64_t)1);
-    if (n_items > 0) {
-        if (precision == 0)
-            hipLaunchKernelGGL(near_build_kernel<double>, dim3(n_items), dim3(256), 0, stream, d_items, d_tiles, a,
-                               near.cnt, near.idx, near.cap, near.max_cnt);
-        else
-            hipLaunchKernelGGL(near_build_kernel<float>, dim3(n_items), dim3(256), 0, stream, d_items, d_tiles, a,
-                               near.cnt, near.idx, near.cap, near.max_cnt);
-    }
+    hipLaunchKernelGGL(grid_body_kernel, dim3((unsigned)nbody), dim3(256), 0, stream,
+                       reinterpret_cast<const int4*>(body_ranges), a.ns, grid.body_of);
+    hipLaunchKernelGGL(grid_count_kernel, dim3(gn), dim3(256), 0, stream, a, n, inv_h, mask, grid.bucket_of, grid.fill);
+    hipLaunchKernelGGL(grid_scan_kernel, dim3(1), dim3(1024), 0, stream, grid.fill, grid.start, table);
+    hipLaunchKernelGGL(grid_scatter_kernel, dim3(gn), dim3(256), 0, stream, n, grid.bucket_of, grid.start, grid.fill,
+                       grid.order);
+    if (precision == 0)
+        hipLaunchKernelGGL(near_grid_kernel<double>, dim3(gn), dim3(256), 0, stream, a, n, inv_h, mask, grid.start,
+                           grid.order, grid.body_of, near.cnt, near.idx, near.cap, near.max_cnt);
+    else
+        hipLaunchKernelGGL(near_grid_kernel<float>, dim3(gn), dim3(256), 0, stream, a, n, inv_h, mask, grid.start,
+                           grid.order, grid.body_of, near.cnt, near.idx, near.cap, near.max_cnt);
     TFS_HIP_CHECK(hipGetLastError());
-    if (n > 0) {
-        hipLaunchKernelGGL(near_sort_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, near.cnt,
-                           near.idx, near.cap, n);
-        TFS_HIP_CHECK(hipGetLastError());
-    }
+    hipLaunchKernelGGL(near_sort_kernel, dim3(gn), dim3(256), 0, stream, near.cnt, near.idx, near.cap, (int64_t)n);
+    TFS_HIP_CHECK(hipGetLastError());
     TFS_HIP_CHECK(hipMemcpyAsync(host_scratch, near.max_cnt, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
     TFS_HIP_CHECK(hipStreamSynchronize(stream));
     *max_count = *host_scratch;

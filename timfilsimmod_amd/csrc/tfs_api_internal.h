@@ -45,6 +45,7 @@ struct tfs_rpy {
     std::vector<tfs::BodyRange> bodies;
     tfs::NearList near[2];
     bool near_valid[2] = {false, false};
+    tfs::NearGrid grid;
     // Fixed-point positions for FP32 tiles (valid with the near lists):
     // per-body centres c_b (bounding-box centres) and one lambda = 2^k.
     int4* x_fx = nullptr;
@@ -80,8 +81,7 @@ extern thread_local std::string g_last_error;
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* where);
 TileShape default_shape(int precision, int ns, int nb, int num_cus);
-// near_plan: a work list over all four blocks in 256-target tiles, for build_near_list.
-Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err, bool near_plan = false);
+Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err);
 // Positions changed: the near-pair lists must be rebuilt before the next matvec.
 void invalidate_near(tfs_rpy* h);
 

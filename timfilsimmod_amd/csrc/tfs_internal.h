@@ -62,6 +62,19 @@ struct NearList {
     int cap = 0;
 };
 
+// Hashed uniform grid over every particle (segments, then blobs) that
+// build_near_list sorts the particles into: buckets of cells of side
+// >= max(ai + aj), a power-of-two table.  Device arrays, owned by the handle.
+struct NearGrid {
+    int32_t* start = nullptr;      // table + 1: first particle of each bucket in `order`
+    int32_t* fill = nullptr;       // table: per-bucket counters
+    int32_t* bucket_of = nullptr;  // n
+    int32_t* order = nullptr;      // n: particles by bucket
+    int32_t* body_of = nullptr;    // n: body index of each particle
+    int table = 0;
+    int n = 0;
+};
+
 // Launch shape of the pair kernel; chosen by the autotuner.
 struct TileShape {
     int block;       // threads per workgroup (64, 128, 256)
@@ -122,11 +135,14 @@ void build_worklist(const std::vector<BodyRange>& bodies, int tile_targets, int 
                     bool bb, bool bs, int seg_write, int blob_write, std::vector<MatvecTile>& tiles,
                     std::vector<MatvecItem>& items);
 
-// Rebuilds `near` for the given precision from a work list covering all four
-// blocks; returns the largest per-target count in *max_count (synchronises).
-hipError_t build_near_list(int precision, const MatvecArgs& a, const MatvecItem* d_items, int n_items,
-                           const MatvecTile* d_tiles, NearList& near, int n_targets, int* max_count,
-                           int* host_scratch, hipStream_t stream);
+// Rebuilds `near` for the given precision: every pair of particles of one
+// body whose squared separation, in the pair kernel's own arithmetic for that
+// precision, is <= (ai+aj)^2, found through a hashed uniform grid (O(n)).
+// body_ranges: nbody BodyRange records on the device.  Grows `grid` as
+// needed; returns the largest per-target count in *max_count (synchronises).
+hipError_t build_near_list(int precision, const MatvecArgs& a, NearGrid& grid, const int32_t* body_ranges, int nbody,
+                           NearList& near, int n_targets, int* max_count, int* host_scratch, hipStream_t stream);
+void free_grid(NearGrid& g);
 
 // Launches pair kernel + reduce on `stream`.  d_tiles/d_items/partial are device
 // buffers sized for the work list; precision 0 = fp64, 1 = fp32 tiles with fp64

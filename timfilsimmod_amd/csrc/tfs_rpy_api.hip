@@ -51,22 +51,17 @@ TileShape default_shape(int precision, int ns, int nb, int num_cus) {
     return s;
 }
 
-Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err, bool near_plan) {
+Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err) {
     bool ss = blocks & TFS_BLOCK_SS, sb = blocks & TFS_BLOCK_SB;
     bool bb = blocks & TFS_BLOCK_BB, bs = blocks & TFS_BLOCK_BS;
     int seg_write = -1, blob_write = -1;
-    if (near_plan) {
-        ss = sb = bb = bs = true;
-        seg_write = blob_write = 0;
-    } else {
-        // Sequential semantics: with an overwriting BS, an earlier BB is overwritten.
-        if (bs && bb && h->desc.bs_mode == TFS_BS_OVERWRITE) bb = false;
-        seg_write = ss ? 0 : (sb ? 1 : -1);
-        if (bb) blob_write = 0;
-        else if (bs) blob_write = h->desc.bs_mode == TFS_BS_OVERWRITE ? 0 : 1;
-    }
+    // Sequential semantics: with an overwriting BS, an earlier BB is overwritten.
+    if (bs && bb && h->desc.bs_mode == TFS_BS_OVERWRITE) bb = false;
+    seg_write = ss ? 0 : (sb ? 1 : -1);
+    if (bb) blob_write = 0;
+    else if (bs) blob_write = h->desc.bs_mode == TFS_BS_OVERWRITE ? 0 : 1;
     const uint32_t eff = (ss ? 1u : 0u) | (sb ? 2u : 0u) | (bb ? 4u : 0u) | (bs ? 8u : 0u);
-    const auto key = std::make_tuple(eff | (near_plan ? 16u : 0u), sh.block, sh.per_thread, sh.chunk);
+    const auto key = std::make_tuple(eff, sh.block, sh.per_thread, sh.chunk);
     auto it = h->plans.find(key);
     if (it != h->plans.end()) return &it->second;
     Plan p;
@@ -89,7 +84,7 @@ Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err, bool 
             *err = hip_fail(e, "work list upload");
             return nullptr;
         }
-        const size_t need = near_plan ? 0 : p.items.size() * 6 * (size_t)(sh.block * sh.per_thread);
+        const size_t need = p.items.size() * 6 * (size_t)(sh.block * sh.per_thread);
         if (need > h->partial_len) {
             if (h->partial) {
                 hipStreamSynchronize(h->stream);
@@ -186,13 +181,12 @@ __global__ __launch_bounds__(256) void to_fixed_kernel(const double* __restrict_
 // difference of two fits an int32.  The quantisation step 1/lambda must stay
 // below 2^-19 of the smallest radius (a body within ~1024 radii of its
 // centre); larger bodies are refused in FP32.  Synchronises the stream.
-int ensure_fixed(tfs_rpy* h) {
-    if (h->fx_valid) return TFS_OK;
-    const int ns = h->desc.num_segs, nb = h->desc.num_blobs;
+// The body table on the device (BodyRange records, plus room for 6 doubles
+// per body), uploaded on the handle's stream from h->bodies.
+int ensure_body_table(tfs_rpy* h) {
     const size_t nbody = h->bodies.size();
     hipError_t e = hipSuccess;
-    if (!h->x_fx) e = hipMalloc(&h->x_fx, sizeof(int4) * (size_t)std::max(ns + nb, 1));
-    if (e == hipSuccess && h->body_cap < nbody) {
+    if (h->body_cap < nbody) {
         if (h->d_body_box) (void)hipFree(h->d_body_box);
         if (h->d_body_range) (void)hipFree(h->d_body_range);
         h->d_body_box = nullptr;
@@ -201,10 +195,22 @@ int ensure_fixed(tfs_rpy* h) {
         if (e == hipSuccess) e = hipMalloc(&h->d_body_range, sizeof(int4) * nbody);
         h->body_cap = e == hipSuccess ? nbody : 0;
     }
-    if (e != hipSuccess) return fail(TFS_ERR_OUT_OF_MEMORY, "fixed-point positions");
-    std::vector<double> box(6 * nbody), centre(3 * nbody, 0.0);
+    if (e != hipSuccess) return fail(TFS_ERR_OUT_OF_MEMORY, "body table");
     e = hipMemcpyAsync(h->d_body_range, h->bodies.data(), sizeof(int4) * nbody, hipMemcpyHostToDevice, h->stream);
-    if (e == hipSuccess) {
+    return e == hipSuccess ? TFS_OK : hip_fail(e, "body table");
+}
+
+int ensure_fixed(tfs_rpy* h) {
+    if (h->fx_valid) return TFS_OK;
+    const int ns = h->desc.num_segs, nb = h->desc.num_blobs;
+    const size_t nbody = h->bodies.size();
+    hipError_t e = hipSuccess;
+    if (!h->x_fx) e = hipMalloc(&h->x_fx, sizeof(int4) * (size_t)std::max(ns + nb, 1));
+    if (e != hipSuccess) return fail(TFS_ERR_OUT_OF_MEMORY, "fixed-point positions");
+    int rc = ensure_body_table(h);
+    if (rc) return rc;
+    std::vector<double> box(6 * nbody), centre(3 * nbody, 0.0);
+    {
         hipLaunchKernelGGL(body_box_kernel, dim3((unsigned)nbody), dim3(256), 0, h->stream, h->x_segs, h->x_blobs,
                            (const int4*)h->d_body_range, h->d_body_box);
         e = hipGetLastError();
@@ -284,11 +290,8 @@ int ensure_near(tfs_rpy* h, int precision) {
         h->near_valid[precision] = true;
         return TFS_OK;
     }
-    int err = precision == 1 ? ensure_fixed(h) : TFS_OK;
+    int err = precision == 1 ? ensure_fixed(h) : ensure_body_table(h);
     if (err) return err;
-    const TileShape nsh{256, 1, 1024};
-    Plan* p = get_plan(h, TFS_BLOCK_ALL, nsh, &err, true);
-    if (!p) return err;
     NearList& nl = h->near[precision];
     MatvecArgs a = matvec_args(h, CView{h->f_segs, 6, 1}, CView{h->f_blobs, 3, 1});
     for (int attempt = 0; attempt < 8; ++attempt) {
@@ -303,7 +306,7 @@ int ensure_near(tfs_rpy* h, int precision) {
             }
         }
         int max_count = 0;
-        hipError_t e = build_near_list(precision, a, p->d_items, (int)p->items.size(), p->d_tiles, nl, nt,
+        hipError_t e = build_near_list(precision, a, h->grid, h->d_body_range, (int)h->bodies.size(), nl, nt,
                                        &max_count, h->host_int, h->stream);
         if (e != hipSuccess) return hip_fail(e, "near-pair lists");
         if (max_count <= nl.cap) {
@@ -490,6 +493,7 @@ int tfs_rpy_destroy(tfs_rpy* h) {
     if (h->partial) hipFree(h->partial);
     if (h->arena) hipFree(h->arena);
     for (auto& nl : h->near) free_near(nl);
+    free_grid(h->grid);
     if (h->x_fx) hipFree(h->x_fx);
     if (h->d_body_box) hipFree(h->d_body_box);
     if (h->d_body_range) hipFree(h->d_body_range);
