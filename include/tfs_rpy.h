@@ -42,6 +42,11 @@ extern "C" {
 #define TFS_PRECISION_FP64 0       /* FP64 tiles, FP64 sums (reference FP64 build) */
 #define TFS_PRECISION_FP32_ACC64 1 /* FP32 tiles, FP64 accumulation                */
 
+/* Fluid domain of the pair tensor. */
+#define TFS_DOMAIN_UNBOUNDED 0     /* cuda_functions.cu:207-354                     */
+#define TFS_DOMAIN_HALF_SPACE 1    /* no-slip wall at z = 0, INFINITE_PLANE_WALL,
+                                      cuda_functions.cu:12-203                      */
+
 /* Write rule of the blob-segment block (cuda_functions.cu:739-751). */
 #define TFS_BS_OVERWRITE 0         /* GMRES-for-linear-system, elastic filaments   */
 #define TFS_BS_ACCUMULATE 1        /* PRESCRIBED_CILIA || USE_BROYDEN_FOR_EVERYTHING */
@@ -114,6 +119,16 @@ int tfs_rpy_sync(tfs_rpy *h);
 
 /* Switch the pair-tensor precision of later evaluations. */
 int tfs_rpy_set_precision(tfs_rpy *h, int32_t precision);
+
+/* Fluid domain of later evaluations (the reference's compile-time
+ * INFINITE_PLANE_WALL, config.hpp): TFS_DOMAIN_UNBOUNDED (default) or
+ * TFS_DOMAIN_HALF_SPACE, the Swan-Brady wall tensor of
+ * cuda_functions.cu:12-203 for particles above z = 0, which like the
+ * reference assumes radius 1 and viscosity 1 (other values are rejected) and
+ * has no overlap branch.  Half-space evaluations take FP64 tiles only.  The
+ * self term is applied to the same particle (the reference tests the
+ * class-local indices, which would also match segment i with blob i). */
+int tfs_rpy_set_domain(tfs_rpy *h, int32_t domain);
 
 /* Override the launch shape (threads per workgroup 64/128/256, targets per
  * thread 1/2, sources per work item: a multiple of block) for a precision. */

@@ -39,6 +39,9 @@ def _load():
         lib.orc_dense_mobility.argtypes = [_dp, _dp, _dp, i, i, i, d, d, d]
         lib.orc_flow_velocities.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _dp, i, i, i, i, d, i]
         lib.orc_flow_rpy.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, i, i, i, d, d, d]
+        lib.orc_rpy_pair_wall.argtypes = [i, i, _dp, _dp, i, d, d, d, d, d, d]
+        lib.orc_set_domain.argtypes = [i]
+        lib.orc_get_domain.restype = ctypes.c_int
         lib.orc_num_threads.restype = ctypes.c_int
         lib.orc_set_num_threads.argtypes = [i]
         _lib = lib
@@ -73,6 +76,40 @@ def rpy_pair(vel_dof, force_dof, f, xi, ai, xj, aj, mu=1.0):
     ff[: len(f)] = f
     lib.orc_rpy_pair(vel_dof, force_dof, _p(v), _p(ff), float(xi[0]), float(xi[1]), float(xi[2]),
                      float(ai), float(xj[0]), float(xj[1]), float(xj[2]), float(aj), float(mu))
+    return v[:vel_dof]
+
+
+UNBOUNDED, HALF_SPACE = 0, 1
+
+
+def set_domain(d: int) -> None:
+    """Domain of the block sums and the dense mobility: UNBOUNDED or HALF_SPACE
+    (no-slip wall at z = 0, radius 1 and viscosity 1)."""
+    _load().orc_set_domain(int(d))
+
+
+class domain:
+    """Context manager: `with domain(HALF_SPACE): ...` restores the previous domain."""
+
+    def __init__(self, d):
+        self.d = d
+
+    def __enter__(self):
+        self.prev = _load().orc_get_domain()
+        set_domain(self.d)
+
+    def __exit__(self, *exc):
+        set_domain(self.prev)
+
+
+def rpy_pair_wall(vel_dof, force_dof, f, xi, xj, same=False):
+    """Half-space pair tensor applied to f (cuda_functions.cu:12-203)."""
+    lib = _load()
+    ff = np.zeros(6)
+    ff[:len(f)] = f
+    v = np.zeros(6)
+    lib.orc_rpy_pair_wall(vel_dof, force_dof, _p(v), _p(ff), int(bool(same)), float(xi[0]), float(xi[1]),
+                          float(xi[2]), float(xj[0]), float(xj[1]), float(xj[2]))
     return v[:vel_dof]
 
 

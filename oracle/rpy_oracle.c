@@ -26,6 +26,132 @@
 
 #define ORC_PI 3.14159265358979323846264338327950288
 
+/* Domain of the pair tensor used by the block sums and the dense mobility:
+ * 0 unbounded (cuda_functions.cu:207-354), 1 half-space above a no-slip wall
+ * at z = 0 (the INFINITE_PLANE_WALL branch, cuda_functions.cu:12-203). */
+static int orc_domain = 0;
+void orc_set_domain(int d) { orc_domain = d; }
+int orc_get_domain(void) { return orc_domain; }
+
+/* ------------------------------------------------------------------------ */
+/* Half-space pair tensor: src/cilia/cuda_functions.cu:12-203 (Swan & Brady  */
+/* 2007, radius 1 and viscosity 1 as in the reference).  Accumulates into    */
+/* V[0..vel_dof) the velocity of particle i due to F[0..force_dof) on j;      */
+/* same != 0 when i and j are the same particle (the wall self-mobility).   */
+/* The reference evaluates the rotation-translation wall terms with -=, a    */
+/* sign fix of the paper's alternating-tensor convention (:17-21); so here.  */
+/* ------------------------------------------------------------------------ */
+void orc_rpy_pair_wall(int vel_dof, int force_dof, double *V, const double *F, int same,
+                       double xi, double yi, double zi, double xj, double yj, double zj)
+{
+    const double c6 = 1.0 / (6.0 * ORC_PI), c8 = 1.0 / (8.0 * ORC_PI);
+    const int tq = force_dof > 3, rot = vel_dof > 3;
+    if (same) {
+        /* :27-64 */
+        const double g1 = 1.0 / zi, g3 = g1 * g1 * g1, g4 = g3 * g1, g5 = g4 * g1;
+        const double par = c6 - (9.0 * g1 - 2.0 * g3 + g5) / (96.0 * ORC_PI);
+        const double perp = c6 - (9.0 * g1 - 4.0 * g3 + g5) / (48.0 * ORC_PI);
+        const double cpl = g4 / (64.0 * ORC_PI);
+        V[0] += par * F[0];
+        V[1] += par * F[1];
+        V[2] += perp * F[2];
+        if (tq) {
+            V[0] += cpl * F[4];
+            V[1] -= cpl * F[3];
+        }
+        if (rot) {
+            V[3] -= cpl * F[1];
+            V[4] += cpl * F[0];
+            if (tq) {
+                const double rpar = c8 - 5.0 * g3 / (128.0 * ORC_PI);
+                const double rperp = c8 - g3 / (64.0 * ORC_PI);
+                V[3] += rpar * F[3];
+                V[4] += rpar * F[4];
+                V[5] += rperp * F[5];
+            }
+        }
+        return;
+    }
+    /* unbounded far-field part, :68-119 */
+    {
+        const double dx = xi - xj, dy = yi - yj, dz = zi - zj;
+        const double q = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+        const double ux = dx * q, uy = dy * q, uz = dz * q;
+        const double uf = ux * F[0] + uy * F[1] + uz * F[2];
+        const double ca = c8 * q, cb = 1.0 + (2.0 / 3.0) * q * q, cc = 1.0 - 2.0 * q * q;
+        V[0] += ca * (cb * F[0] + cc * uf * ux);
+        V[1] += ca * (cb * F[1] + cc * uf * uy);
+        V[2] += ca * (cb * F[2] + cc * uf * uz);
+        const double c2 = c8 * q * q;
+        if (tq) {
+            V[0] += c2 * (F[4] * uz - F[5] * uy);
+            V[1] += c2 * (F[5] * ux - F[3] * uz);
+            V[2] += c2 * (F[3] * uy - F[4] * ux);
+        }
+        if (rot) {
+            V[3] += c2 * (F[1] * uz - F[2] * uy);
+            V[4] += c2 * (F[2] * ux - F[0] * uz);
+            V[5] += c2 * (F[0] * uy - F[1] * ux);
+            if (tq) {
+                const double c3 = 0.5 * c2 * q, ut = ux * F[3] + uy * F[4] + uz * F[5];
+                V[3] += c3 * (3.0 * ut * ux - F[3]);
+                V[4] += c3 * (3.0 * ut * uy - F[4]);
+                V[5] += c3 * (3.0 * ut * uz - F[5]);
+            }
+        }
+    }
+    /* wall corrections from the image at (xj, yj, -zj), :121-199 */
+    const double Rx = xi - xj, Ry = yi - yj, Rz = zi + zj;
+    const double p1 = 1.0 / sqrt(Rx * Rx + Ry * Ry + Rz * Rz);
+    const double p2 = p1 * p1, p3 = p2 * p1, p4 = p3 * p1, p5 = p4 * p1;
+    const double ex = Rx * p1, ey = Ry * p1, ez = Rz * p1, ez2 = ez * ez;
+    const double h = zj / (zi + zj);
+    const double ef = ex * F[0] + ey * F[1] + ez * F[2];
+    {
+        const double t0 = -(0.75 * p1 * (1.0 + 2.0 * h * (1.0 - h) * ez2) +
+                            0.5 * (p3 * (1.0 - 3.0 * ez2) - p5 * (1.0 - 5.0 * ez2)));
+        const double t1 = -(0.75 * p1 * (1.0 - 6.0 * h * (1.0 - h) * ez2) - 1.5 * p3 * (1.0 - 5.0 * ez2) +
+                            2.5 * p5 * (1.0 - 7.0 * ez2)) * ef;
+        const double t2 = ez * F[2] * (1.5 * h * p1 * (1.0 - 6.0 * (1.0 - h) * ez2) - 3.0 * p3 * (1.0 - 5.0 * ez2) +
+                                      5.0 * p5 * (2.0 - 7.0 * ez2));
+        const double t3 = (1.5 * h * p1 - 5.0 * p5) * ef;
+        const double t4 = -((3.0 * h * h * p1 + 3.0 * p3) * ez2 + (2.0 - 15.0 * ez2) * p5) * F[2];
+        V[0] += c6 * (t0 * F[0] + (t1 + t2) * ex);
+        V[1] += c6 * (t0 * F[1] + (t1 + t2) * ey);
+        V[2] += c6 * (t0 * F[2] + (t1 + t2 + t3) * ez + t4);
+    }
+    if (rot) {
+        const double u0 = 0.75 * p2;
+        const double u12 = F[2] * (9.0 * h * ez2 * p2 + (1.5 - 15.0 * ez2) * p4) - ez * ef * (4.5 * h * p2 - 7.5 * p4);
+        const double u3 = -1.5 * ez * (h * p2 - p4);
+        V[3] -= c6 * (u0 * (F[1] * ez - F[2] * ey) - u12 * ey + u3 * F[1]);
+        V[4] -= c6 * (u0 * (F[2] * ex - F[0] * ez) + u12 * ex - u3 * F[0]);
+        V[5] -= c6 * u0 * (F[0] * ey - F[1] * ex);
+        if (tq) {
+            const double et = ex * F[3] + ey * F[4] + ez * F[5];
+            const double w0 = p3 * (0.375 - 2.25 * ez2), w1 = -1.125 * p3 * et, w2 = 2.25 * ez * p3 * et;
+            const double w3 = 2.25 * p3 * (ex * F[4] - ey * F[3]);
+            V[3] += c6 * (w0 * F[3] + w1 * ex - w3 * ey);
+            V[4] += c6 * (w0 * F[4] + w1 * ey + w3 * ex);
+            V[5] += c6 * (w0 * F[5] + w1 * ez + w2);
+        }
+    }
+    if (tq) {
+        /* translation from torque: the transpose of the rotation-from-force
+         * block of the pair seen from j (image weight 1 - h, e_x and e_y
+         * reversed), :181-199 */
+        const double hh = 1.0 - h, fx = -ex, fy = -ey;
+        const double ext = fx * F[4] - fy * F[3];
+        const double q0 = 0.75 * p2;
+        const double q1 = (9.0 * hh * ez2 * p2 + (1.5 - 15.0 * ez2) * p4) * ext;
+        const double q2 = -ez * (4.5 * hh * p2 - 7.5 * p4) * ext;
+        const double q3 = -1.5 * ez * (hh * p2 - p4);
+        V[0] -= c6 * (q0 * (F[5] * fy - F[4] * ez) + q2 * fx - q3 * F[4]);
+        V[1] -= c6 * (q0 * (F[3] * ez - F[5] * fx) + q2 * fy + q3 * F[3]);
+        V[2] -= c6 * (q0 * (F[4] * fx - F[3] * fy) + q2 * ez + q1);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Pair tensor: src/cilia/cuda_functions.cu:207-354 (unbounded branch).      */
 /* Accumulates into V[0..vel_dof) the velocity of particle i (radius ai) due */
@@ -136,6 +262,14 @@ void orc_rpy_pair(int vel_dof, int force_dof, double *V, const double *F,
     }
 }
 
+static void orc_pair_domain(int vel_dof, int force_dof, double *V, const double *F, int same,
+                            double xi, double yi, double zi, double ai,
+                            double xj, double yj, double zj, double aj, double mu)
+{
+    if (orc_domain == 1) orc_rpy_pair_wall(vel_dof, force_dof, V, F, same, xi, yi, zi, xj, yj, zj);
+    else orc_rpy_pair(vel_dof, force_dof, V, F, xi, yi, zi, ai, xj, yj, zj, aj, mu);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Block sums: src/cilia/cuda_functions.cu:362-757.  The target range is    */
 /* [start, start+count); outputs are indexed from 0 at `start` exactly as   */
@@ -153,8 +287,8 @@ void orc_Mss(double *V, const double *F, const double *X, int start, int count,
         for (int j = 0; j < num_segs; ++j) {
             double f[6] = {F[6 * j], F[6 * j + 1], F[6 * j + 2], 0, 0, 0};
             if (seg_dof == 6) { f[3] = F[6 * j + 3]; f[4] = F[6 * j + 4]; f[5] = F[6 * j + 5]; }
-            orc_rpy_pair(seg_dof, seg_dof, v, f, X[3 * i], X[3 * i + 1], X[3 * i + 2], a_seg,
-                         X[3 * j], X[3 * j + 1], X[3 * j + 2], a_seg, mu);
+            orc_pair_domain(seg_dof, seg_dof, v, f, i == j, X[3 * i], X[3 * i + 1], X[3 * i + 2], a_seg,
+                            X[3 * j], X[3 * j + 1], X[3 * j + 2], a_seg, mu);
         }
         for (int k = 0; k < 6; ++k) V[6 * (i - start) + k] = v[k];
     }
@@ -169,8 +303,8 @@ void orc_Msb(double *V, const double *Fb, const double *Xs, const double *Xb, in
         double v[6] = {0, 0, 0, 0, 0, 0};
         for (int j = 0; j < num_blobs; ++j) {
             const double f[3] = {Fb[3 * j], Fb[3 * j + 1], Fb[3 * j + 2]};
-            orc_rpy_pair(seg_dof, 3, v, f, Xs[3 * i], Xs[3 * i + 1], Xs[3 * i + 2], a_seg,
-                         Xb[3 * j], Xb[3 * j + 1], Xb[3 * j + 2], a_blob, mu);
+            orc_pair_domain(seg_dof, 3, v, f, 0, Xs[3 * i], Xs[3 * i + 1], Xs[3 * i + 2], a_seg,
+                            Xb[3 * j], Xb[3 * j + 1], Xb[3 * j + 2], a_blob, mu);
         }
         for (int k = 0; k < 6; ++k) V[6 * (i - start) + k] += v[k];
     }
@@ -185,8 +319,8 @@ void orc_Mbb(double *V, const double *F, const double *X, int start, int count,
         double v[3] = {0, 0, 0};
         for (int j = 0; j < num_blobs; ++j) {
             const double f[3] = {F[3 * j], F[3 * j + 1], F[3 * j + 2]};
-            orc_rpy_pair(3, 3, v, f, X[3 * i], X[3 * i + 1], X[3 * i + 2], a_blob,
-                         X[3 * j], X[3 * j + 1], X[3 * j + 2], a_blob, mu);
+            orc_pair_domain(3, 3, v, f, i == j, X[3 * i], X[3 * i + 1], X[3 * i + 2], a_blob,
+                            X[3 * j], X[3 * j + 1], X[3 * j + 2], a_blob, mu);
         }
         for (int k = 0; k < 3; ++k) V[3 * (i - start) + k] = v[k];
     }
@@ -205,8 +339,8 @@ void orc_Mbs(double *V, const double *Fs, const double *Xb, const double *Xs, in
         for (int j = 0; j < num_segs; ++j) {
             double f[6] = {Fs[6 * j], Fs[6 * j + 1], Fs[6 * j + 2], 0, 0, 0};
             if (seg_dof == 6) { f[3] = Fs[6 * j + 3]; f[4] = Fs[6 * j + 4]; f[5] = Fs[6 * j + 5]; }
-            orc_rpy_pair(3, seg_dof, v, f, Xb[3 * i], Xb[3 * i + 1], Xb[3 * i + 2], a_blob,
-                         Xs[3 * j], Xs[3 * j + 1], Xs[3 * j + 2], a_seg, mu);
+            orc_pair_domain(3, seg_dof, v, f, 0, Xb[3 * i], Xb[3 * i + 1], Xb[3 * i + 2], a_blob,
+                            Xs[3 * j], Xs[3 * j + 1], Xs[3 * j + 2], a_seg, mu);
         }
         for (int k = 0; k < 3; ++k) {
             if (accumulate) V[3 * (i - start) + k] += v[k];
@@ -260,7 +394,7 @@ void orc_dense_mobility(double *M, const double *x_segs, const double *x_blobs,
                 double f[6] = {0, 0, 0, 0, 0, 0};
                 double v[6] = {0, 0, 0, 0, 0, 0};
                 f[c] = 1.0;
-                orc_rpy_pair(pd, qd, v, f, xp[0], xp[1], xp[2], ap, xq[0], xq[1], xq[2], aq, mu);
+                orc_pair_domain(pd, qd, v, f, p == q, xp[0], xp[1], xp[2], ap, xq[0], xq[1], xq[2], aq, mu);
                 for (int r = 0; r < pd; ++r) M[(size_t)(prow + r) * n + qcol + c] = v[r];
             }
         }

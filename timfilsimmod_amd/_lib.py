@@ -21,6 +21,8 @@ TFS_ERR_OUT_OF_MEMORY = 6
 
 PRECISION_FP64 = 0
 PRECISION_FP32_ACC64 = 1
+DOMAIN_UNBOUNDED = 0
+DOMAIN_HALF_SPACE = 1
 BS_OVERWRITE = 0
 BS_ACCUMULATE = 1
 BLOCK_SS, BLOCK_SB, BLOCK_BB, BLOCK_BS, BLOCK_ALL = 1, 2, 4, 8, 15
@@ -64,6 +66,7 @@ SIGNATURES = {
     "tfs_rpy_apply": (ctypes.c_int, [_vp, _u32, _vp, _vp, _vp, _vp]),
     "tfs_rpy_sync": (ctypes.c_int, [_vp]),
     "tfs_rpy_set_precision": (ctypes.c_int, [_vp, _i32]),
+    "tfs_rpy_set_domain": (ctypes.c_int, [_vp, _i32]),
     "tfs_rpy_set_tile_shape": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32]),
     "tfs_rpy_device_buffer": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_vp)]),
     "tfs_rpy_profile": (ctypes.c_int, [_vp, _i32]),

@@ -242,6 +242,56 @@ __global__ __launch_bounds__(BLOCK) void rpy_pairs_kernel(const MatvecItem* __re
     }
 }
 
+// Half-space domain (no-slip wall at z = 0): every pair of an item through
+// pair_wall in FP64, one target per lane; the self pair (same global index)
+// takes the wall self-mobility.  Partial slots as run_item.
+template <int VT, int FS>
+__device__ void run_item_wall(const MatvecItem it, const MatvecTile tl, const MatvecArgs& a,
+                              double* __restrict__ slot, double* __restrict__ lds) {
+    constexpr int REC = rec_len<double, FS>();
+    constexpr int TILE = kWallBlock;
+    const int tid = threadIdx.x;
+    int t = tl.t_begin + tid;
+    t = t < tl.t_end ? t : tl.t_end - 1;
+    double xi[3];
+    load_target(a, tl.cls, t, xi);
+    const int tg = (tl.cls == 0 ? 0 : a.ns) + t;
+    const int src_base = it.src_cls == 0 ? 0 : a.ns;
+    double acc[VT];
+#pragma unroll
+    for (int k = 0; k < VT; ++k) acc[k] = 0.0;
+    for (int s0 = it.s_begin; s0 < it.s_end; s0 += kWallBlock) {
+        const int n = (it.s_end - s0) < kWallBlock ? (it.s_end - s0) : kWallBlock;
+        __syncthreads();
+        if (tid < n) stage_source<double, FS>(lds + tid * REC, a, it.src_cls, s0 + tid, true);
+        __syncthreads();
+        for (int j = 0; j < n; ++j) {
+            const double* rec = lds + j * REC;
+            pair_wall<VT, FS>(acc, src_base + s0 + j == tg, xi, rec, rec + 3);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < VT; ++k) slot[k * TILE + tid] = acc[k];
+}
+
+template <int SEGDOF>
+__global__ __launch_bounds__(kWallBlock) void rpy_pairs_wall_kernel(const MatvecItem* __restrict__ items,
+                                                                    const MatvecTile* __restrict__ tiles,
+                                                                    MatvecArgs a, double* __restrict__ partial) {
+    __shared__ __align__(16) double lds[kWallBlock * rec_len_max<double>()];
+    if (a.skip && *a.skip) return;
+    const MatvecItem it = items[blockIdx.x];
+    const MatvecTile tl = tiles[it.tile];
+    double* slot = partial + (size_t)blockIdx.x * (kMaxDof * kWallBlock);
+    if (tl.cls == 0) {
+        if (it.src_cls == 0) run_item_wall<SEGDOF, SEGDOF>(it, tl, a, slot, lds);
+        else run_item_wall<SEGDOF, 3>(it, tl, a, slot, lds);
+    } else {
+        if (it.src_cls == 0) run_item_wall<3, SEGDOF>(it, tl, a, slot, lds);
+        else run_item_wall<3, 3>(it, tl, a, slot, lds);
+    }
+}
+
 // Persistent form: a fixed grid of `gridDim.x` workgroups walks the work list
 // with stride gridDim.x (the list is ordered by body, items of similar size),
 // so every workgroup's loop count and exit are uniform -- no atomics and no
@@ -682,6 +732,33 @@ hipError_t build_near_list(int precision, const MatvecArgs& a, NearGrid& grid, c
     TFS_HIP_CHECK(hipStreamSynchronize(stream));
     *max_count = *host_scratch;
     return hipSuccess;
+}
+
+hipError_t launch_matvec_wall(int seg_dof, const MatvecArgs& a, const OutArgs& o, const MatvecTile* d_tiles,
+                              int n_tiles, const MatvecItem* d_items, int n_items, double* partial,
+                              hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end) {
+    if (n_tiles == 0) return hipSuccess;
+    if (n_items > 0) {
+        if (ev_begin) TFS_HIP_CHECK(hipEventRecord(ev_begin, stream));
+        if (seg_dof == 6)
+            hipLaunchKernelGGL(rpy_pairs_wall_kernel<6>, dim3(n_items), dim3(kWallBlock), 0, stream, d_items, d_tiles,
+                               a, partial);
+        else
+            hipLaunchKernelGGL(rpy_pairs_wall_kernel<3>, dim3(n_items), dim3(kWallBlock), 0, stream, d_items, d_tiles,
+                               a, partial);
+        TFS_HIP_CHECK(hipGetLastError());
+        if (ev_end) TFS_HIP_CHECK(hipEventRecord(ev_end, stream));
+    }
+    const int tile_blocks = n_tiles * ((kWallBlock + 63) / 64);
+    const int fil_blocks = o.sad.in ? (o.sad.nfil + 3) / 4 : 0;
+    const NearList none{};
+    if (seg_dof == 6)
+        hipLaunchKernelGGL((rpy_reduce_kernel<6, double>), dim3(tile_blocks + fil_blocks), dim3(256), 0, stream,
+                           d_tiles, (const double*)partial, kWallBlock, tile_blocks, o, a, none);
+    else
+        hipLaunchKernelGGL((rpy_reduce_kernel<3, double>), dim3(tile_blocks + fil_blocks), dim3(256), 0, stream,
+                           d_tiles, (const double*)partial, kWallBlock, tile_blocks, o, a, none);
+    return hipGetLastError();
 }
 
 hipError_t launch_matvec(int precision, int seg_dof, const TileShape& shape, const MatvecArgs& a,

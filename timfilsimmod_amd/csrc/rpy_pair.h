@@ -247,4 +247,123 @@ __device__ __forceinline__ void pair_near(double* __restrict__ v, double dx, dou
     }
 }
 
+// Half-space pair tensor (no-slip wall at z = 0; cuda_functions.cu:12-203,
+// Swan & Brady 2007, radius 1 and viscosity 1 as the reference assumes), in
+// FP64.  Accumulates into v the velocity (VT = 3) or velocity and angular
+// velocity (VT = 6) of particle i at xi due to the force (FS = 3) or force
+// and torque (FS = 6) f on particle j at xj; `same` marks i and j as one
+// particle (the wall self-mobility).  Distinct particles take the unbounded
+// far-field form plus the image corrections; as in the reference, the
+// rotation-translation wall terms enter with the opposite sign to the
+// paper's printed alternating-tensor convention.
+template <int VT, int FS>
+__device__ inline void pair_wall(double* __restrict__ v, bool same, const double* xi, const double* xj,
+                                 const double* __restrict__ f) {
+    const double PI = 3.14159265358979323846;
+    const double k6 = 1.0 / (6.0 * PI), k8 = 1.0 / (8.0 * PI);
+    if (same) {
+        const double g1 = 1.0 / xi[2], g3 = g1 * g1 * g1, g4 = g3 * g1, g5 = g4 * g1;
+        const double tpar = k6 - (9.0 * g1 - 2.0 * g3 + g5) * (k6 / 16.0);
+        const double tperp = k6 - (9.0 * g1 - 4.0 * g3 + g5) * (k6 / 8.0);
+        const double cpl = g4 * (k6 * 3.0 / 32.0);
+        v[0] += tpar * f[0];
+        v[1] += tpar * f[1];
+        v[2] += tperp * f[2];
+        if constexpr (FS == 6) {
+            v[0] += cpl * f[4];
+            v[1] -= cpl * f[3];
+        }
+        if constexpr (VT == 6) {
+            v[3] -= cpl * f[1];
+            v[4] += cpl * f[0];
+            if constexpr (FS == 6) {
+                const double rpar = k8 - g3 * (k6 * 15.0 / 64.0);
+                const double rperp = k8 - g3 * (k6 * 3.0 / 32.0);
+                v[3] += rpar * f[3];
+                v[4] += rpar * f[4];
+                v[5] += rperp * f[5];
+            }
+        }
+        return;
+    }
+    // unbounded far field between the particles
+    {
+        const double dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
+        const double q = rsqrt_refined(dx * dx + dy * dy + dz * dz);
+        const double ux = dx * q, uy = dy * q, uz = dz * q;
+        const double uf = ux * f[0] + uy * f[1] + uz * f[2];
+        const double ca = k8 * q, cb = 1.0 + (2.0 / 3.0) * q * q, cc = (1.0 - 2.0 * q * q) * uf;
+        v[0] += ca * (cb * f[0] + cc * ux);
+        v[1] += ca * (cb * f[1] + cc * uy);
+        v[2] += ca * (cb * f[2] + cc * uz);
+        const double c2 = ca * q;
+        if constexpr (FS == 6) {
+            v[0] += c2 * (f[4] * uz - f[5] * uy);
+            v[1] += c2 * (f[5] * ux - f[3] * uz);
+            v[2] += c2 * (f[3] * uy - f[4] * ux);
+        }
+        if constexpr (VT == 6) {
+            v[3] += c2 * (f[1] * uz - f[2] * uy);
+            v[4] += c2 * (f[2] * ux - f[0] * uz);
+            v[5] += c2 * (f[0] * uy - f[1] * ux);
+            if constexpr (FS == 6) {
+                const double c3 = 0.5 * c2 * q, ut = ux * f[3] + uy * f[4] + uz * f[5];
+                v[3] += c3 * (3.0 * ut * ux - f[3]);
+                v[4] += c3 * (3.0 * ut * uy - f[4]);
+                v[5] += c3 * (3.0 * ut * uz - f[5]);
+            }
+        }
+    }
+    // image corrections: separation from the image of j at (xj, yj, -zj)
+    const double Rx = xi[0] - xj[0], Ry = xi[1] - xj[1], Rz = xi[2] + xj[2];
+    const double p1 = rsqrt_refined(Rx * Rx + Ry * Ry + Rz * Rz);
+    const double p2 = p1 * p1, p3 = p2 * p1, p4 = p3 * p1, p5 = p4 * p1;
+    const double ex = Rx * p1, ey = Ry * p1, ez = Rz * p1, ez2 = ez * ez;
+    const double h = xj[2] / (xi[2] + xj[2]);
+    const double ef = ex * f[0] + ey * f[1] + ez * f[2];
+    {
+        const double t0 = -(0.75 * p1 * (1.0 + 2.0 * h * (1.0 - h) * ez2) +
+                            0.5 * (p3 * (1.0 - 3.0 * ez2) - p5 * (1.0 - 5.0 * ez2)));
+        const double t12 = -(0.75 * p1 * (1.0 - 6.0 * h * (1.0 - h) * ez2) - 1.5 * p3 * (1.0 - 5.0 * ez2) +
+                             2.5 * p5 * (1.0 - 7.0 * ez2)) * ef +
+                           ez * f[2] *
+                               (1.5 * h * p1 * (1.0 - 6.0 * (1.0 - h) * ez2) - 3.0 * p3 * (1.0 - 5.0 * ez2) +
+                                5.0 * p5 * (2.0 - 7.0 * ez2));
+        const double t3 = (1.5 * h * p1 - 5.0 * p5) * ef;
+        const double t4 = -((3.0 * h * h * p1 + 3.0 * p3) * ez2 + (2.0 - 15.0 * ez2) * p5) * f[2];
+        v[0] += k6 * (t0 * f[0] + t12 * ex);
+        v[1] += k6 * (t0 * f[1] + t12 * ey);
+        v[2] += k6 * (t0 * f[2] + (t12 + t3) * ez + t4);
+    }
+    if constexpr (VT == 6) {
+        const double u0 = 0.75 * p2;
+        const double u12 = f[2] * (9.0 * h * ez2 * p2 + (1.5 - 15.0 * ez2) * p4) - ez * ef * (4.5 * h * p2 - 7.5 * p4);
+        const double u3 = -1.5 * ez * (h * p2 - p4);
+        v[3] -= k6 * (u0 * (f[1] * ez - f[2] * ey) - u12 * ey + u3 * f[1]);
+        v[4] -= k6 * (u0 * (f[2] * ex - f[0] * ez) + u12 * ex - u3 * f[0]);
+        v[5] -= k6 * u0 * (f[0] * ey - f[1] * ex);
+        if constexpr (FS == 6) {
+            const double et = ex * f[3] + ey * f[4] + ez * f[5];
+            const double w0 = p3 * (0.375 - 2.25 * ez2), w1 = -1.125 * p3 * et, w2 = 2.25 * ez * p3 * et;
+            const double w3 = 2.25 * p3 * (ex * f[4] - ey * f[3]);
+            v[3] += k6 * (w0 * f[3] + w1 * ex - w3 * ey);
+            v[4] += k6 * (w0 * f[4] + w1 * ey + w3 * ex);
+            v[5] += k6 * (w0 * f[5] + w1 * ez + w2);
+        }
+    }
+    if constexpr (FS == 6) {
+        // translation from torque: transpose of the rotation-from-force block
+        // seen from j (image weight 1 - h, e_x and e_y reversed)
+        const double hh = 1.0 - h, fx = -ex, fy = -ey;
+        const double ext = fx * f[4] - fy * f[3];
+        const double q0 = 0.75 * p2;
+        const double q1 = (9.0 * hh * ez2 * p2 + (1.5 - 15.0 * ez2) * p4) * ext;
+        const double q2 = -ez * (4.5 * hh * p2 - 7.5 * p4) * ext;
+        const double q3 = -1.5 * ez * (hh * p2 - p4);
+        v[0] -= k6 * (q0 * (f[5] * fy - f[4] * ez) + q2 * fx - q3 * f[4]);
+        v[1] -= k6 * (q0 * (f[3] * ez - f[5] * fx) + q2 * fy + q3 * f[3]);
+        v[2] -= k6 * (q0 * (f[4] * fx - f[3] * fy) + q2 * ez + q1);
+    }
+}
+
 }  // namespace tfs

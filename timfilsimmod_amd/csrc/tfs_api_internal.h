@@ -46,6 +46,7 @@ struct tfs_rpy {
     tfs::NearList near[2];
     bool near_valid[2] = {false, false};
     tfs::NearGrid grid;
+    int domain = 0;  // TFS_DOMAIN_UNBOUNDED or TFS_DOMAIN_HALF_SPACE
     // Fixed-point positions for FP32 tiles (valid with the near lists):
     // per-body centres c_b (bounding-box centres) and one lambda = 2^k.
     int4* x_fx = nullptr;

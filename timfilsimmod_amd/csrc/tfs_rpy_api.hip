@@ -353,12 +353,24 @@ void free_plans(tfs_rpy* h) {
 
 int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, View vs, View vb,
                int seg_out_slots, const SaddleEpilogue* sad) {
-    const TileShape& sh = h->shape[precision];
+    const bool wall = h->domain == TFS_DOMAIN_HALF_SPACE;
+    if (wall && precision != TFS_PRECISION_FP64)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "half-space domain: FP64 tiles only");
+    TileShape sh = h->shape[precision];
+    if (wall) {
+        // one target per lane, ~16 workgroups per CU over the work list
+        const int ns = h->desc.num_segs, nb = h->desc.num_blobs;
+        const long n_tiles = (ns + kWallBlock - 1) / kWallBlock + (nb + kWallBlock - 1) / kWallBlock;
+        const long want = 16L * (h->num_cus > 0 ? h->num_cus : 256);
+        long chunk = n_tiles > 0 ? ((long)(ns + nb) * n_tiles + want - 1) / want : (long)(ns + nb);
+        chunk = std::max<long>(kWallBlock, (chunk + kWallBlock - 1) / kWallBlock * kWallBlock);
+        sh = TileShape{kWallBlock, 1, (int)chunk, 0};
+    }
     int err = TFS_OK;
     Plan* p = get_plan(h, blocks, sh, &err);
     if (!p) return err;
     if (p->tiles.empty()) return TFS_OK;
-    err = ensure_near(h, precision);
+    if (!wall) err = ensure_near(h, precision);
     if (err) return err;
     MatvecArgs a = matvec_args(h, fs, fb);
     OutArgs o;
@@ -391,9 +403,11 @@ int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, V
         ps.item_stamps = h->d_item_stamps;
         h->stamped_plan = p;
     }
-    hipError_t e = launch_matvec(precision, h->desc.seg_dof, sh, a, o, p->d_tiles, (int)p->tiles.size(),
-                                 p->d_items, (int)p->items.size(), h->partial, h->near[precision], ps,
-                                 h->stream, eb, ee);
+    hipError_t e = wall ? launch_matvec_wall(h->desc.seg_dof, a, o, p->d_tiles, (int)p->tiles.size(), p->d_items,
+                                             (int)p->items.size(), h->partial, h->stream, eb, ee)
+                        : launch_matvec(precision, h->desc.seg_dof, sh, a, o, p->d_tiles, (int)p->tiles.size(),
+                                        p->d_items, (int)p->items.size(), h->partial, h->near[precision], ps,
+                                        h->stream, eb, ee);
     if (e != hipSuccess) return hip_fail(e, "matvec launch");
     return TFS_OK;
 }
@@ -588,6 +602,18 @@ int tfs_rpy_apply(tfs_rpy* h, uint32_t blocks, const double* f_segs, const doubl
     if (!rc && out_s && ns) rc = tfs_rpy_download(h, TFS_SEG_VELOCITIES, v_segs);
     if (!rc && out_b && nb) rc = tfs_rpy_download(h, TFS_BLOB_VELOCITIES, v_blobs);
     return rc;
+}
+
+int tfs_rpy_set_domain(tfs_rpy* h, int32_t domain) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_domain: null handle");
+    if (domain != TFS_DOMAIN_UNBOUNDED && domain != TFS_DOMAIN_HALF_SPACE)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_domain: unknown domain");
+    if (domain == TFS_DOMAIN_HALF_SPACE &&
+        (h->desc.seg_radius != 1.0 || h->desc.blob_radius != 1.0 || h->desc.mu != 1.0))
+        return fail(TFS_ERR_INVALID_ARGUMENT,
+                    "tfs_rpy_set_domain: the half-space tensor is for radius 1 and viscosity 1");
+    h->domain = domain;
+    return TFS_OK;
 }
 
 int tfs_rpy_sync(tfs_rpy* h) {

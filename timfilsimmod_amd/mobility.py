@@ -94,6 +94,10 @@ class RpyHandle:
     def set_precision(self, precision: int):
         L.check(L.lib().tfs_rpy_set_precision(self.h, precision))
 
+    def set_domain(self, domain: int):
+        """DOMAIN_UNBOUNDED or DOMAIN_HALF_SPACE (no-slip wall at z = 0)."""
+        L.check(L.lib().tfs_rpy_set_domain(self.h, domain))
+
     def set_tile_shape(self, precision: int, block: int, per_thread: int, chunk: int):
         L.check(L.lib().tfs_rpy_set_tile_shape(self.h, precision, block, per_thread, chunk))
 
@@ -271,7 +275,7 @@ class RpyMobilitySolver:
 
 def matvec(x_segs, x_blobs, f_segs, f_blobs, *, seg_dof=6, precision="fp64", bs_accumulate=False,
            blocks=L.BLOCK_ALL, v_segs=None, v_blobs=None, seg_radius=1.0, blob_radius=1.0, mu=1.0,
-           tile_shape=None):
+           tile_shape=None, domain=L.DOMAIN_UNBOUNDED):
     """One-shot matvec on host numpy arrays: returns (v_segs[6*Ns], v_blobs[3*Nb])."""
     xs = np.ascontiguousarray(x_segs, dtype=np.float64).reshape(-1)
     xb = np.ascontiguousarray(x_blobs, dtype=np.float64).reshape(-1)
@@ -286,6 +290,8 @@ def matvec(x_segs, x_blobs, f_segs, f_blobs, *, seg_dof=6, precision="fp64", bs_
     try:
         if tile_shape is not None:
             h.set_tile_shape(prec, *tile_shape)
+        if domain != L.DOMAIN_UNBOUNDED:
+            h.set_domain(domain)
         if ns:
             h.upload(L.SEG_POSITIONS, xs)
         if nb:
