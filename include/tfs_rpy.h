@@ -37,6 +37,8 @@ extern "C" {
 #define TFS_ERR_NOT_CONVERGED 4
 #define TFS_ERR_NO_DEVICE 5
 #define TFS_ERR_OUT_OF_MEMORY 6
+#define TFS_ERR_IO 7     /* a data file cannot be opened, read or written (tfs_io.h) */
+#define TFS_ERR_FORMAT 8 /* a data file record does not parse or has the wrong length */
 
 /* Evaluation precision of the pair tensors. */
 #define TFS_PRECISION_FP64 0       /* FP64 tiles, FP64 sums (reference FP64 build) */

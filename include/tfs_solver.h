@@ -142,6 +142,19 @@ int tfs_beat_destroy(tfs_beat *b);
 int tfs_beat_step(tfs_beat *b, tfs_solve_stats *stats);
 /* Host copies of the current state (any pointer may be NULL). */
 int tfs_beat_state(tfs_beat *b, double *phase, double *angle, double *x_segs, double *solution);
+/* Resume from saved state (READ_INITIAL_CONDITIONS_FROM_BACKUP,
+ * cilia_sim_main.cu:118-160, filament.cpp:85-130): the phases, the angles
+ * (NULL: zero) and the solution vector the next solve starts from (NULL:
+ * zero).  The next tfs_beat_step solves at these phases without advancing
+ * them first, as the reference's first step after a restart re-solves the
+ * saved step. */
+int tfs_beat_restore(tfs_beat *b, const double *phase, const double *angle, const double *solution);
+/* Host velocities of the last solution's forces, M lambda in FP64 at the
+ * current positions: v_segs [6*Ns] (rotational slots 0, as the prescribed
+ * build writes them) and v_blobs [3*Nb] (may be NULL without blobs).  This is
+ * the product the reference makes before saving velocities
+ * (mobility_solver.cpp:3053-3058, written at :3149). */
+int tfs_beat_velocities(tfs_beat *b, double *v_segs, double *v_blobs);
 /* The solver handle inside the driver (to call tfs_solver_* on its state). */
 int tfs_beat_solver(tfs_beat *b, tfs_solver **out);
 /* Switch between TFS_SOLVE_FP64 and TFS_SOLVE_MIXED for later steps. */

@@ -2,7 +2,8 @@
 
 Submodules: mobility (matvec back-end), flow (flow-field evaluator), solver (per-timestep saddle-point
 solve and prescribed-beat driver), broyden (implicit-step Broyden engine), geometry and beat
-(synthetic inputs).
-All compute runs in libtfs_rpy.so (include/tfs_rpy.h, tfs_solver.h, tfs_flow.h, tfs_broyden.h).
+(synthetic inputs), datafiles (the reference's data files and its file-driven flow-field run).
+All compute runs in libtfs_rpy.so (include/tfs_rpy.h, tfs_solver.h, tfs_flow.h, tfs_broyden.h,
+tfs_io.h).
 """
-from . import _lib, geometry, beat, mobility, solver, flow, ensemble, broyden  # noqa: F401
+from . import _lib, geometry, beat, mobility, solver, flow, ensemble, broyden, datafiles  # noqa: F401

@@ -18,6 +18,8 @@ TFS_ERR_NON_FINITE = 3
 TFS_ERR_NOT_CONVERGED = 4
 TFS_ERR_NO_DEVICE = 5
 TFS_ERR_OUT_OF_MEMORY = 6
+TFS_ERR_IO = 7
+TFS_ERR_FORMAT = 8
 
 PRECISION_FP64 = 0
 PRECISION_FP32_ACC64 = 1

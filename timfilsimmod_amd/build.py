@@ -28,6 +28,7 @@ SOURCES = [
     "tfs_flow_api.hip",
     "tfs_broyden_api.hip",
     "tfs_velocities_api.hip",
+    "tfs_io.cpp",
 ]
 
 # Packed FP32 VALU forms are switched off: only scalar FP32 and FP64 VALU.
@@ -45,12 +46,19 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP library cannot be built")
 
 
+# Host-only sources (no device code) go through the host compiler.
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I", INCLUDE, "-I", CSRC]
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    if src.endswith(".cpp"):
+        cmd = ["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    else:
+        cmd = [hipcc(), *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed on {src}:\n{res.stderr}")
+        raise RuntimeError(f"compiling {src} failed:\n{res.stderr}")
     return obj
 
 

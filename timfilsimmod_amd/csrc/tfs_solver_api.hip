@@ -782,6 +782,9 @@ struct tfs_beat {
     int warm_order = 0;
     double *hist1 = nullptr, *hist2 = nullptr;
     std::vector<double> beat_A, beat_B;
+    // Velocities of the last solution's forces (tfs_beat_velocities), made on
+    // first use.
+    double* vel = nullptr;
 };
 
 extern "C" {
@@ -862,6 +865,7 @@ int tfs_beat_destroy(tfs_beat* b) {
     if (!b) return TFS_OK;
     if (b->solver && b->solver->stream) hipStreamSynchronize(b->solver->stream);
     if (b->arena) hipFree(b->arena);
+    if (b->vel) hipFree(b->vel);
     tfs_solver_destroy(b->solver);
     delete b;
     return TFS_OK;
@@ -944,6 +948,47 @@ int tfs_beat_state(tfs_beat* b, double* phase, double* angle, double* x_segs, do
     down(solution, b->sol, s->dims.n);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     return check(e, "tfs_beat_state");
+}
+
+int tfs_beat_restore(tfs_beat* b, const double* phase, const double* angle, const double* solution) {
+    if (!b || !phase) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_restore: null handle or phases");
+    tfs_solver* s = b->solver;
+    const int nf = b->desc.solver.num_fil;
+    const int64_t n = s->dims.n;
+    std::vector<double> zeros(std::max<int64_t>(n, nf), 0.0);
+    hipError_t e = hipMemcpyAsync(b->phase, phase, 8LL * nf, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->angle, angle ? angle : zeros.data(), 8LL * nf, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->sol, solution ? solution : zeros.data(), 8 * n, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return hip_fail(e, "tfs_beat_restore");
+    // The next step solves at the restored phases (no update) and starts from
+    // `solution`; the extrapolation history begins again.
+    b->step = 0;
+    return TFS_OK;
+}
+
+int tfs_beat_velocities(tfs_beat* b, double* v_segs, double* v_blobs) {
+    if (!b || !v_segs) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_velocities: null handle or segment array");
+    tfs_solver* s = b->solver;
+    const SaddleDims& dm = s->dims;
+    const int64_t ns = int64_t(dm.nfil) * dm.nseg, nb = dm.nblob;
+    if (nb > 0 && !v_blobs) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_velocities: null blob array");
+    if (!b->vel && hipMalloc(&b->vel, 8 * std::max<int64_t>(3 * (ns + nb), 1)) != hipSuccess)
+        return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_beat_velocities: device memory");
+    // M lambda in FP64 at the current positions (mobility_solver.cpp:3053-3058).
+    int rc = run_matvec(s->rpy, TFS_BLOCK_ALL, TFS_PRECISION_FP64, CView{b->sol, 3, 1},
+                        CView{b->sol + dm.off_blob, 3, 1}, View{b->vel, 3, 1}, View{b->vel + 3 * ns, 3, 1}, 3);
+    if (rc) return rc;
+    std::vector<double> v(3 * (ns + nb));
+    hipError_t e = hipMemcpyAsync(v.data(), b->vel, 8 * v.size(), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return hip_fail(e, "tfs_beat_velocities");
+    for (int64_t i = 0; i < ns; ++i)
+        for (int c = 0; c < 6; ++c) v_segs[6 * i + c] = c < 3 ? v[3 * i + c] : 0.0;
+    if (nb > 0) std::copy(v.begin() + 3 * ns, v.end(), v_blobs);
+    return TFS_OK;
 }
 
 int tfs_beat_solver(tfs_beat* b, tfs_solver** out) {

@@ -77,6 +77,8 @@ L.extra_signatures({
     "tfs_beat_solver": (ctypes.c_int, [_dp, ctypes.POINTER(_dp)]),
     "tfs_beat_set_mode": (ctypes.c_int, [_dp, _i32]),
     "tfs_beat_set_warm_start": (ctypes.c_int, [_dp, _i32]),
+    "tfs_beat_restore": (ctypes.c_int, [_dp, _dp, _dp, _dp]),
+    "tfs_beat_velocities": (ctypes.c_int, [_dp, _dp, _dp]),
     "tfs_solver_set_preconditioner": (ctypes.c_int, [_dp, _i32]),
 })
 
@@ -330,3 +332,21 @@ class BeatSimulation:
         ph, an, xs, sol = np.zeros(nf), np.zeros(nf), np.zeros(3 * ns), np.zeros(self.n)
         L.check(L.lib().tfs_beat_state(self.h, L.ptr(ph), L.ptr(an), L.ptr(xs), L.ptr(sol)))
         return {"phase": ph, "angle": an, "x_segs": xs, "solution": sol}
+
+    def restore(self, phase, angle=None, solution=None):
+        """Continue from saved state: the next step solves at these phases
+        (no update first) from ``solution`` as its initial guess."""
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                for a in (phase, angle, solution)]
+        if arrs[0].size != self.num_fil or (arrs[1] is not None and arrs[1].size != self.num_fil) or \
+                (arrs[2] is not None and arrs[2].size != self.n):
+            raise ValueError("restore: phase/angle need num_fil values, solution the system size")
+        L.check(L.lib().tfs_beat_restore(self.h, *[L.ptr(a) if a is not None else None for a in arrs]))
+
+    def velocities(self):
+        """(v_segs [6 Ns], v_blobs [3 Nb]): M lambda of the last solution in
+        FP64, rotational slots zero."""
+        ns, nb = self.num_fil * self.num_seg, self.desc.solver.num_blobs
+        vs, vb = np.zeros(6 * ns), np.zeros(3 * nb)
+        L.check(L.lib().tfs_beat_velocities(self.h, L.ptr(vs), L.ptr(vb) if nb else None))
+        return vs, vb
