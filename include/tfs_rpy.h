@@ -195,6 +195,29 @@ int tfs_host_alloc(size_t bytes, void **out);
 int tfs_host_free(void *p);
 
 /* Message of the last failure on this thread ("" if none). */
+/* Barrier (short-range repulsion) forces of the elastic model,
+ * apply_interparticle_forces (rpy_mobility_solver.cu:207-232) and its kernel
+ * barrier_forces (cuda_functions.cu:1240-1436), at the handle's current
+ * positions.  For each particle p and every other particle j closer than
+ * 1.1 (a_p + a_j) -- segments skip their neighbours i-1, i+1 on the same
+ * filament --
+ *   f_p += strength * fac^4 * (x_p - x_j) / |x_p - x_j|,
+ *   fac = min(1, 1 - 10 (|x_p - x_j| - a_p - a_j) / (a_p + a_j));
+ * with `wall`, segments below z = dl/2 first get the normal force
+ * strength * min(1, 1 - (z - a)/(dl/2 - a))^4 (INFINITE_PLANE_WALL).
+ * f_segs [6*Ns]: forces added to slots 0-2 (the reference adds them to the
+ * device forces the next matvec uses).  f_blobs_repulsion [3*Nb] (may be
+ * NULL): overwritten (f_blobs_repulsion_device).  Pointers in the handle's
+ * memory space; complete on return for host memory. */
+typedef struct tfs_barrier_desc {
+    double strength;      /* REPULSIVE_FORCE_FACTOR * END_FORCE_MAGNITUDE */
+    int32_t seg_per_fil;  /* NSEG                                         */
+    int32_t wall;         /* 1: plane wall at z = 0                       */
+    double dl;            /* DL (wall term only)                          */
+} tfs_barrier_desc;
+
+int tfs_rpy_barrier_forces(tfs_rpy *h, const tfs_barrier_desc *desc, double *f_segs, double *f_blobs_repulsion);
+
 const char *tfs_last_error(void);
 
 /* Library version string. */

@@ -40,6 +40,7 @@ def _load():
         lib.orc_flow_velocities.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _dp, i, i, i, i, d, i]
         lib.orc_flow_rpy.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, i, i, i, d, d, d]
         lib.orc_rpy_pair_wall.argtypes = [i, i, _dp, _dp, i, d, d, d, d, d, d]
+        lib.orc_barrier_forces.argtypes = [_dp, _dp, _dp, _dp, i, i, i, d, d, d, i, d]
         lib.orc_set_domain.argtypes = [i]
         lib.orc_get_domain.restype = ctypes.c_int
         lib.orc_num_threads.restype = ctypes.c_int
@@ -192,6 +193,19 @@ def flow_velocities(x_flow, x_segs, f_segs, x_blobs, f_blobs, x_bodies, *, num_b
                             ns_total // num_bodies, num_bodies, nb_total // num_bodies,
                             float(prefac), int(bool(remove_net_force)))
     return v
+
+
+def barrier_forces(x_segs, x_blobs, f_segs, *, nseg, strength, a_seg=1.0, a_blob=1.0, wall=False, dl=2.2):
+    """barrier_forces kernel: (f_segs + barrier on slots 0-2, blob repulsion)."""
+    lib = _load()
+    xs, xb = _arr(x_segs), _arr(x_blobs)
+    ns, nb = xs.size // 3, xb.size // 3
+    fs = _arr(f_segs, 6 * ns).copy()
+    rep = np.zeros(3 * nb)
+    lib.orc_barrier_forces(_p(fs if ns else np.zeros(6)), _p(rep if nb else np.zeros(3)),
+                           _p(xs if ns else np.zeros(3)), _p(xb if nb else np.zeros(3)), ns, nb, int(nseg),
+                           float(a_seg), float(a_blob), float(strength), int(bool(wall)), float(dl))
+    return fs, rep
 
 
 def flow_rpy(x_flow, x_segs, f_segs, x_blobs, f_blobs, *, a_seg=1.0, a_blob=1.0, mu=1.0):

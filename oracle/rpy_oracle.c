@@ -22,6 +22,7 @@
  */
 #include <math.h>
 #include <stddef.h>
+#include <stdlib.h>
 #include <string.h>
 
 #define ORC_PI 3.14159265358979323846264338327950288
@@ -499,6 +500,68 @@ void orc_flow_rpy(double *Vf, const double *Xf, const double *Fs, const double *
         Vf[3 * i] = v[0];
         Vf[3 * i + 1] = v[1];
         Vf[3 * i + 2] = v[2];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Barrier forces: src/cilia/cuda_functions.cu:1240-1436 (barrier_forces),   */
+/* all particles of all bodies, sources in the reference's index order      */
+/* (segments then blobs).  Segment forces are added to F_s slots 0-2; blob  */
+/* forces overwrite F_rep.  strength = REPULSIVE_FORCE_FACTOR *              */
+/* END_FORCE_MAGNITUDE; nseg = NSEG; wall selects INFINITE_PLANE_WALL's     */
+/* normal force on segments below z = dl/2 (:1271-1278).                    */
+static void orc_barrier_sum(double *f, const double *xi, double ai, int i_global, int is_seg, int fili,
+                            const double *Xs, const double *Xb, int ns, int nb, int nseg, double a_seg,
+                            double a_blob, double strength) {
+    for (int j = 0; j < ns + nb; ++j) {
+        const int jseg = j < ns;
+        const double *xj = jseg ? &Xs[3 * j] : &Xb[3 * (j - ns)];
+        const double a_sum = ai + (jseg ? a_seg : a_blob);
+        const double chi_fac = 10.0 / a_sum;
+        const double dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
+        const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+        int excluded;
+        if (is_seg) {
+            const int filj = j / nseg;
+            excluded = (fili == filj) && abs(i_global - j) <= 1;
+        } else {
+            excluded = (i_global == j);
+        }
+        if (!excluded && dist < 1.1 * a_sum) {
+            double fac = fmin(1.0, 1.0 - chi_fac * (dist - a_sum));
+            fac *= strength * fac * fac * fac;
+            const double dm1 = 1.0 / dist;
+            f[0] += fac * dx * dm1;
+            f[1] += fac * dy * dm1;
+            f[2] += fac * dz * dm1;
+        }
+    }
+}
+
+void orc_barrier_forces(double *F_s, double *F_rep, const double *Xs, const double *Xb, int ns, int nb,
+                        int nseg, double a_seg, double a_blob, double strength, int wall, double dl) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < ns; ++i) {
+        double f[3] = {0.0, 0.0, 0.0};
+        const double *xi = &Xs[3 * i];
+        if (wall && xi[2] < 0.5 * dl) {
+            double fz = fmin(1.0, 1.0 - (xi[2] - a_seg) / (0.5 * dl - a_seg));
+            fz *= strength * fz * fz * fz;
+            f[2] = fz;
+        }
+        orc_barrier_sum(f, xi, a_seg, i, 1, i / nseg, Xs, Xb, ns, nb, nseg, a_seg, a_blob, strength);
+        F_s[6 * i] += f[0];
+        F_s[6 * i + 1] += f[1];
+        F_s[6 * i + 2] += f[2];
+    }
+    if (!F_rep) return;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nb; ++k) {
+        double f[3] = {0.0, 0.0, 0.0};
+        orc_barrier_sum(f, &Xb[3 * k], a_blob, ns + k, 0, -1, Xs, Xb, ns, nb, nseg, a_seg, a_blob, strength);
+        F_rep[3 * k] = f[0];
+        F_rep[3 * k + 1] = f[1];
+        F_rep[3 * k + 2] = f[2];
     }
 }
 

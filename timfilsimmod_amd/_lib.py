@@ -58,6 +58,11 @@ _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _u32 = ctypes.c_uint32
 
+class BarrierDesc(ctypes.Structure):
+    _fields_ = [("strength", ctypes.c_double), ("seg_per_fil", ctypes.c_int32), ("wall", ctypes.c_int32),
+                ("dl", ctypes.c_double)]
+
+
 # name -> (restype, argtypes); the exported surface of include/*.h.
 SIGNATURES = {
     "tfs_rpy_create": (ctypes.c_int, [ctypes.POINTER(RpyDesc), ctypes.POINTER(_vp)]),
@@ -69,6 +74,7 @@ SIGNATURES = {
     "tfs_rpy_sync": (ctypes.c_int, [_vp]),
     "tfs_rpy_set_precision": (ctypes.c_int, [_vp, _i32]),
     "tfs_rpy_set_domain": (ctypes.c_int, [_vp, _i32]),
+    "tfs_rpy_barrier_forces": (ctypes.c_int, [_vp, ctypes.POINTER(BarrierDesc), _vp, _vp]),
     "tfs_rpy_set_tile_shape": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32]),
     "tfs_rpy_device_buffer": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_vp)]),
     "tfs_rpy_profile": (ctypes.c_int, [_vp, _i32]),

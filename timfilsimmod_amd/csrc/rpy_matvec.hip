@@ -462,6 +462,71 @@ __global__ void fill_int_kernel(int32_t* p, int32_t v, int64_t n) {
     if (i < n) p[i] = v;
 }
 
+// Barrier (short-range repulsion) forces of the elastic model
+// (cuda_functions.cu:1240-1436).  Per particle p, over every other particle j
+// within 1.1 (ai + aj) -- except, for segments, j itself and its neighbours
+// on the same filament --
+//   f_p += S fac^4 (x_p - x_j)/|x_p - x_j|,  fac = min(1, 1 - 10 (r - ai - aj)/(ai + aj)),
+// and for segments below z = DL/2 with the plane wall a normal force
+// S min(1, 1 - (z - a)/(DL/2 - a))^4 first.  Segment forces are added to
+// slots 0-2 of fs (stride 6); blob forces overwrite fb (stride 3, may be
+// null).  Candidates come from the 27 neighbouring cells of side
+// >= 1.1 * 2 max(a); each is visited from its own cell only.
+__global__ __launch_bounds__(256) void barrier_kernel(MatvecArgs a, int n, double inv_h, uint32_t mask,
+                                                     const int32_t* __restrict__ start,
+                                                     const int32_t* __restrict__ order, BarrierParams bp,
+                                                     double* __restrict__ fs, double* __restrict__ fb) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    double xi[3];
+    particle_pos(a, p, xi);
+    const bool seg = p < a.ns;
+    const double ai = seg ? a.a_seg : a.a_blob;
+    const int fili = seg ? p / bp.nseg : -1;
+    double f[3] = {0.0, 0.0, 0.0};
+    if (seg && bp.wall && xi[2] < 0.5 * bp.dl) {
+        double fz = fmin(1.0, 1.0 - (xi[2] - a.a_seg) / (0.5 * bp.dl - a.a_seg));
+        fz *= bp.strength * fz * fz * fz;
+        f[2] = fz;
+    }
+    const int cx = cell_coord(xi[0], inv_h), cy = cell_coord(xi[1], inv_h), cz = cell_coord(xi[2], inv_h);
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int nx = cx + dx, ny = cy + dy, nz = cz + dz;
+                const uint32_t b = cell_hash(nx, ny, nz, mask);
+                for (int q = start[b]; q < start[b + 1]; ++q) {
+                    const int j = order[q];
+                    if (j == p) continue;
+                    const bool jseg = j < a.ns;
+                    if (seg && jseg && j / bp.nseg == fili && (j - p <= 1 && p - j <= 1)) continue;
+                    double xj[3];
+                    particle_pos(a, j, xj);
+                    if (cell_coord(xj[0], inv_h) != nx || cell_coord(xj[1], inv_h) != ny ||
+                        cell_coord(xj[2], inv_h) != nz)
+                        continue;
+                    const double asum = ai + (jseg ? a.a_seg : a.a_blob);
+                    const double ddx = xi[0] - xj[0], ddy = xi[1] - xj[1], ddz = xi[2] - xj[2];
+                    const double dist = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+                    if (dist < 1.1 * asum) {
+                        double fac = fmin(1.0, 1.0 - (10.0 / asum) * (dist - asum));
+                        fac *= bp.strength * fac * fac * fac;
+                        const double dm1 = 1.0 / dist;
+                        f[0] += fac * ddx * dm1;
+                        f[1] += fac * ddy * dm1;
+                        f[2] += fac * ddz * dm1;
+                    }
+                }
+            }
+    if (seg) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fs[(int64_t)p * 6 + k] += f[k];
+    } else if (fb) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fb[(int64_t)(p - a.ns) * 3 + k] = f[k];
+    }
+}
+
 template <int VT>
 __device__ void add_near(double* acc, int src, const double* xi, double ai, const MatvecArgs& a, int seg_dof,
                          uint32_t cls_mask) {
@@ -689,11 +754,9 @@ void free_grid(NearGrid& g) {
     g = NearGrid{};
 }
 
-hipError_t build_near_list(int precision, const MatvecArgs& a, NearGrid& grid, const int32_t* body_ranges, int nbody,
-                           NearList& near, int n_targets, int* max_count, int* host_scratch, hipStream_t stream) {
-    const int n = n_targets;
-    *max_count = 0;
-    if (n <= 0) return hipSuccess;
+// Sorts the n particles (segments, then blobs) into the grid's buckets for
+// cells of side 1 / inv_h: start (exclusive bucket offsets) and order.
+static hipError_t sort_into_grid(const MatvecArgs& a, NearGrid& grid, int n, double inv_h, hipStream_t stream) {
     int table = 1024;
     while (table < 2 * n) table <<= 1;
     if (grid.n != n || grid.table != table) {
@@ -706,19 +769,30 @@ hipError_t build_near_list(int precision, const MatvecArgs& a, NearGrid& grid, c
         grid.n = n;
         grid.table = table;
     }
-    const double h = 2.0 * std::max(a.a_seg, a.a_blob) * (1.0 + 1e-5);
-    const double inv_h = 1.0 / h;
     const uint32_t mask = (uint32_t)table - 1;
     const unsigned gn = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(fill_int_kernel, dim3((unsigned)((table + 255) / 256)), dim3(256), 0, stream, grid.fill, 0,
                        (int64_t)table);
-    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, stream, near.max_cnt, 0, (int64_t)1);
-    hipLaunchKernelGGL(grid_body_kernel, dim3((unsigned)nbody), dim3(256), 0, stream,
-                       reinterpret_cast<const int4*>(body_ranges), a.ns, grid.body_of);
     hipLaunchKernelGGL(grid_count_kernel, dim3(gn), dim3(256), 0, stream, a, n, inv_h, mask, grid.bucket_of, grid.fill);
     hipLaunchKernelGGL(grid_scan_kernel, dim3(1), dim3(1024), 0, stream, grid.fill, grid.start, table);
     hipLaunchKernelGGL(grid_scatter_kernel, dim3(gn), dim3(256), 0, stream, n, grid.bucket_of, grid.start, grid.fill,
                        grid.order);
+    return hipGetLastError();
+}
+
+hipError_t build_near_list(int precision, const MatvecArgs& a, NearGrid& grid, const int32_t* body_ranges, int nbody,
+                           NearList& near, int n_targets, int* max_count, int* host_scratch, hipStream_t stream) {
+    const int n = n_targets;
+    *max_count = 0;
+    if (n <= 0) return hipSuccess;
+    const double h = 2.0 * std::max(a.a_seg, a.a_blob) * (1.0 + 1e-5);
+    const double inv_h = 1.0 / h;
+    TFS_HIP_CHECK(sort_into_grid(a, grid, n, inv_h, stream));
+    const uint32_t mask = (uint32_t)grid.table - 1;
+    const unsigned gn = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, stream, near.max_cnt, 0, (int64_t)1);
+    hipLaunchKernelGGL(grid_body_kernel, dim3((unsigned)nbody), dim3(256), 0, stream,
+                       reinterpret_cast<const int4*>(body_ranges), a.ns, grid.body_of);
     if (precision == 0)
         hipLaunchKernelGGL(near_grid_kernel<double>, dim3(gn), dim3(256), 0, stream, a, n, inv_h, mask, grid.start,
                            grid.order, grid.body_of, near.cnt, near.idx, near.cap, near.max_cnt);
@@ -732,6 +806,18 @@ hipError_t build_near_list(int precision, const MatvecArgs& a, NearGrid& grid, c
     TFS_HIP_CHECK(hipStreamSynchronize(stream));
     *max_count = *host_scratch;
     return hipSuccess;
+}
+
+hipError_t launch_barrier(const MatvecArgs& a, NearGrid& grid, const BarrierParams& bp, double* fs, double* fb,
+                          hipStream_t stream) {
+    const int n = a.ns + a.nb;
+    if (n <= 0) return hipSuccess;
+    const double h = 2.2 * std::max(a.a_seg, a.a_blob) * (1.0 + 1e-6);
+    const double inv_h = 1.0 / h;
+    TFS_HIP_CHECK(sort_into_grid(a, grid, n, inv_h, stream));
+    hipLaunchKernelGGL(barrier_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, a, n, inv_h,
+                       (uint32_t)grid.table - 1, grid.start, grid.order, bp, fs, fb);
+    return hipGetLastError();
 }
 
 hipError_t launch_matvec_wall(int seg_dof, const MatvecArgs& a, const OutArgs& o, const MatvecTile* d_tiles,

@@ -57,6 +57,7 @@ struct tfs_rpy {
     size_t body_cap = 0;
     std::vector<double> fx_centre;   // 3 per body
     int32_t* host_int = nullptr;  // pinned scratch
+    double* blob_rep = nullptr;   // device blob repulsion staging (tfs_rpy_barrier_forces, host memory)
     // Persistent launches with timing: per-item stamps of the last launch and
     // the plan they belong to.
     int64_t* d_item_stamps = nullptr;

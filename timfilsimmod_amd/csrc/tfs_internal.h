@@ -75,6 +75,16 @@ struct NearGrid {
     int n = 0;
 };
 
+// Barrier forces (cuda_functions.cu:1240-1436): strength
+// REPULSIVE_FORCE_FACTOR * END_FORCE_MAGNITUDE, NSEG for the neighbour
+// exclusion, and the plane-wall term below z = DL/2.
+struct BarrierParams {
+    double strength = 0.0;
+    int nseg = 1;
+    int wall = 0;
+    double dl = 0.0;
+};
+
 // Launch shape of the pair kernel; chosen by the autotuner.
 struct TileShape {
     int block;       // threads per workgroup (64, 128, 256)
@@ -143,6 +153,11 @@ void build_worklist(const std::vector<BodyRange>& bodies, int tile_targets, int 
 hipError_t build_near_list(int precision, const MatvecArgs& a, NearGrid& grid, const int32_t* body_ranges, int nbody,
                            NearList& near, int n_targets, int* max_count, int* host_scratch, hipStream_t stream);
 void free_grid(NearGrid& g);
+// Barrier forces of the positions in `a` (xs, xb, ns, nb, radii): fs += on
+// segment slots 0-2 (stride 6), fb = (stride 3, may be null).  Uses `grid`
+// as scratch.
+hipError_t launch_barrier(const MatvecArgs& a, NearGrid& grid, const BarrierParams& bp, double* fs, double* fb,
+                          hipStream_t stream);
 
 // Launches pair kernel + reduce on `stream`.  d_tiles/d_items/partial are device
 // buffers sized for the work list; precision 0 = fp64, 1 = fp32 tiles with fp64

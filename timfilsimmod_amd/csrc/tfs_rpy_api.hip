@@ -512,6 +512,7 @@ int tfs_rpy_destroy(tfs_rpy* h) {
     if (h->d_body_box) hipFree(h->d_body_box);
     if (h->d_body_range) hipFree(h->d_body_range);
     if (h->host_int) hipHostFree(h->host_int);
+    if (h->blob_rep) hipFree(h->blob_rep);
     if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
     delete h;
     return TFS_OK;
@@ -602,6 +603,48 @@ int tfs_rpy_apply(tfs_rpy* h, uint32_t blocks, const double* f_segs, const doubl
     if (!rc && out_s && ns) rc = tfs_rpy_download(h, TFS_SEG_VELOCITIES, v_segs);
     if (!rc && out_b && nb) rc = tfs_rpy_download(h, TFS_BLOB_VELOCITIES, v_blobs);
     return rc;
+}
+
+int tfs_rpy_barrier_forces(tfs_rpy* h, const tfs_barrier_desc* desc, double* f_segs, double* f_blobs_repulsion) {
+    if (!h || !desc) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_barrier_forces: null handle or descriptor");
+    const int64_t ns = h->desc.num_segs, nb = h->desc.num_blobs;
+    if (ns > 0 && !f_segs) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_barrier_forces: null segment forces");
+    if (!(desc->strength >= 0) || !std::isfinite(desc->strength))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_barrier_forces: strength must be finite and >= 0");
+    if (desc->seg_per_fil < 1 || (ns > 0 && ns % desc->seg_per_fil != 0))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_barrier_forces: seg_per_fil must divide the segment count");
+    if (desc->wall && !(0.5 * desc->dl > h->desc.seg_radius))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_barrier_forces: the wall term needs dl/2 > segment radius");
+    BarrierParams bp;
+    bp.strength = desc->strength;
+    bp.nseg = desc->seg_per_fil;
+    bp.wall = desc->wall ? 1 : 0;
+    bp.dl = desc->dl;
+    const bool host = h->desc.memory == TFS_MEMORY_HOST;
+    double* fs = f_segs;
+    double* fb = f_blobs_repulsion;
+    hipError_t e = hipSuccess;
+    if (host) {
+        fs = h->f_segs;
+        if (ns > 0) e = hipMemcpyAsync(fs, f_segs, 48 * ns, hipMemcpyHostToDevice, h->stream);
+        if (f_blobs_repulsion && nb > 0) {
+            if (!h->blob_rep && hipMalloc(&h->blob_rep, 24 * nb) != hipSuccess)
+                return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_rpy_barrier_forces: device memory");
+            fb = h->blob_rep;
+        }
+    }
+    if (e != hipSuccess) return hip_fail(e, "tfs_rpy_barrier_forces: upload");
+    const MatvecArgs a = matvec_args(h, CView{h->f_segs, 6, 1}, CView{h->f_blobs, 3, 1});
+    e = launch_barrier(a, h->grid, bp, fs, nb > 0 ? fb : nullptr, h->stream);
+    if (e != hipSuccess) return hip_fail(e, "tfs_rpy_barrier_forces");
+    if (host) {
+        if (ns > 0) e = hipMemcpyAsync(f_segs, fs, 48 * ns, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess && f_blobs_repulsion && nb > 0)
+            e = hipMemcpyAsync(f_blobs_repulsion, fb, 24 * nb, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return hip_fail(e, "tfs_rpy_barrier_forces: download");
+    }
+    return TFS_OK;
 }
 
 int tfs_rpy_set_domain(tfs_rpy* h, int32_t domain) {

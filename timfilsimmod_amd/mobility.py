@@ -98,6 +98,14 @@ class RpyHandle:
         """DOMAIN_UNBOUNDED or DOMAIN_HALF_SPACE (no-slip wall at z = 0)."""
         L.check(L.lib().tfs_rpy_set_domain(self.h, domain))
 
+    def barrier_forces(self, f_segs, f_blobs_repulsion=None, *, strength: float, seg_per_fil: int,
+                       wall: bool = False, dl: float = 2.2):
+        """Adds the barrier forces of the current positions to f_segs (slots
+        0-2) and writes the blobs' into f_blobs_repulsion (arrays in the
+        handle's memory space)."""
+        d = L.BarrierDesc(float(strength), int(seg_per_fil), int(bool(wall)), float(dl))
+        L.check(L.lib().tfs_rpy_barrier_forces(self.h, ctypes.byref(d), L.ptr(f_segs), L.ptr(f_blobs_repulsion)))
+
     def set_tile_shape(self, precision: int, block: int, per_thread: int, chunk: int):
         L.check(L.lib().tfs_rpy_set_tile_shape(self.h, precision, block, per_thread, chunk))
 
@@ -243,12 +251,26 @@ class RpyMobilitySolver:
     def copy_blob_velocities_to_host(self):
         self.handle.download(L.BLOB_VELOCITIES, self.v_blobs_host)
 
+    def set_barrier(self, *, strength: float, seg_per_fil: int, wall: bool = False, dl: float = 2.2):
+        """Barrier-force parameters of a non-prescribed build (REPULSIVE_FORCE_FACTOR *
+        END_FORCE_MAGNITUDE, NSEG, INFINITE_PLANE_WALL, DL); without them
+        apply_interparticle_forces adds nothing, as in a prescribed-cilia build."""
+        self.barrier = dict(strength=strength, seg_per_fil=seg_per_fil, wall=wall, dl=dl)
+
     def copy_interparticle_blob_forces_to_host(self):
-        # Barrier forces are not part of this back-end (prescribed cilia have none).
-        self.f_blobs_repulsion_host[:] = 0.0
+        rep = getattr(self, "_blob_repulsion", None)
+        self.f_blobs_repulsion_host[:] = 0.0 if rep is None else rep
 
     def apply_interparticle_forces(self):
-        pass
+        """rpy_mobility_solver.cu:207-232: the device segment forces gain the
+        barrier forces (the host array keeps the model's), and the blob
+        repulsion is computed for copy_interparticle_blob_forces_to_host."""
+        nb = len(self.f_blobs_repulsion_host)
+        self._blob_repulsion = np.zeros(nb)
+        if getattr(self, "barrier", None) is None:
+            return
+        staged = np.array(self.f_segs_host, dtype=np.float64)
+        self.handle.barrier_forces(staged, self._blob_repulsion if nb else None, **self.barrier)
 
     def wait_for_device(self):
         self.handle.sync()
