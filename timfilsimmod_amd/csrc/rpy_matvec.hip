@@ -403,6 +403,27 @@ __global__ __launch_bounds__(256) void grid_scatter_kernel(int n, const int32_t*
     order[start[b] + atomicAdd(&fill[b], 1)] = p;
 }
 
+// Puts each bucket's particles in increasing index order (the scatter's
+// atomics leave them in arrival order), so that sums over a cell's particles
+// are reproducible.  Insertion sort per bucket; buckets hold a few particles
+// (cells of side ~2a hold at most a handful), larger ones are left as they are.
+__global__ __launch_bounds__(256) void grid_sort_buckets_kernel(int table, const int32_t* __restrict__ start,
+                                                               int32_t* __restrict__ order) {
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= table) return;
+    const int lo = start[b], hi = start[b + 1];
+    if (hi - lo < 2 || hi - lo > 256) return;
+    for (int i = lo + 1; i < hi; ++i) {
+        const int32_t v = order[i];
+        int k = i - 1;
+        while (k >= lo && order[k] > v) {
+            order[k + 1] = order[k];
+            --k;
+        }
+        order[k + 1] = v;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void near_grid_kernel(MatvecArgs a, int n, double inv_h, uint32_t mask,
                                                        const int32_t* __restrict__ start,
@@ -471,7 +492,8 @@ __global__ void fill_int_kernel(int32_t* p, int32_t v, int64_t n) {
 // S min(1, 1 - (z - a)/(DL/2 - a))^4 first.  Segment forces are added to
 // slots 0-2 of fs (stride 6); blob forces overwrite fb (stride 3, may be
 // null).  Candidates come from the 27 neighbouring cells of side
-// >= 1.1 * 2 max(a); each is visited from its own cell only.
+// >= 1.1 * 2 max(a), in a fixed order (cells, then indices within a bucket),
+// so results are reproducible; each is visited from its own cell only.
 __global__ __launch_bounds__(256) void barrier_kernel(MatvecArgs a, int n, double inv_h, uint32_t mask,
                                                      const int32_t* __restrict__ start,
                                                      const int32_t* __restrict__ order, BarrierParams bp,
@@ -777,6 +799,8 @@ static hipError_t sort_into_grid(const MatvecArgs& a, NearGrid& grid, int n, dou
     hipLaunchKernelGGL(grid_scan_kernel, dim3(1), dim3(1024), 0, stream, grid.fill, grid.start, table);
     hipLaunchKernelGGL(grid_scatter_kernel, dim3(gn), dim3(256), 0, stream, n, grid.bucket_of, grid.start, grid.fill,
                        grid.order);
+    hipLaunchKernelGGL(grid_sort_buckets_kernel, dim3((unsigned)((table + 255) / 256)), dim3(256), 0, stream, table,
+                       grid.start, grid.order);
     return hipGetLastError();
 }
 
