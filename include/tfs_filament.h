@@ -1,0 +1,99 @@
+/*
+ * tfs_filament.h -- C-ABI of the Cosserat filament model on the device: the
+ * per-filament work of the elastic implicit timestep around the Broyden solver
+ * (tfs_broyden.h) and the mobility (tfs_solver.h tfs_velocities_*).
+ *
+ * Replaces, for every filament of every swimmer at once:
+ *   filament::initial_guess        filament.cpp:1083-1385 (with segment::initial_guess,
+ *                                  and accept_state_from_rigid_body :649-672 for a body
+ *                                  held still)
+ *   filament::internal_forces_and_torques   filament.cpp:1467-1575
+ *   filament::invert_approx_jacobian        filament.cpp:1761-1894 (the matrix; its
+ *                                  inversion is tfs_broyden_set_jacobian) and
+ *   filament::jacobian_lie_algebra_block    filament.cpp:1577-1759
+ *   filament::update, robot_arm    filament.cpp:1896-1930, 625-647
+ * for the reference's clamped-base build with a follower end force
+ * (INSTABILITY_CILIA): the base segment is clamped by a multiplier, a force
+ * -END_FORCE_MAGNITUDE t acts on the last segment, and an optional kick acts
+ * on two segments during the first step (filament.cpp:1533-1552).
+ *
+ * Layouts: segments body -> filament -> segment (as x_segs, f_segs of
+ * tfs_rpy.h); quaternions (w, x, y, z); per filament the 3 (NSEG + 1)
+ * multipliers [tether, lambda_0 .. lambda_{NSEG-2}, clamp].  The Broyden
+ * vector and J0 blocks follow swimmer ordering: per swimmer its NFIL filament
+ * blocks of 6 NSEG unknowns ([tether, lambda..., clamp, u_1 ...]) and then 6
+ * body unknowns; J0 as tfs_broyden_set_jacobian takes it, per swimmer NFIL
+ * (6 NSEG)^2 row-major blocks then a 6 x 6 body block (left to the caller).
+ */
+#ifndef TFS_FILAMENT_H
+#define TFS_FILAMENT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "tfs_rpy.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tfs_filament_desc {
+    int32_t num_swim;          /* NSWIM                                        */
+    int32_t fil_per_swim;      /* NFIL                                         */
+    int32_t num_seg;           /* NSEG (>= 2)                                  */
+    double dl;                 /* DL                                           */
+    double kb, kt;             /* KB, KT                                       */
+    double mu, radius;         /* MU, RSEG                                     */
+    double dt;                 /* DT                                           */
+    double end_force;          /* END_FORCE_MAGNITUDE                          */
+    double strain_twist[3];    /* intrinsic strain and twist (zero: straight)  */
+    int32_t memory;            /* TFS_MEMORY_* of the arrays passed later      */
+    void *stream;              /* hipStream_t or NULL                          */
+} tfs_filament_desc;
+
+typedef struct tfs_filament tfs_filament;
+
+int tfs_filament_create(const tfs_filament_desc *desc, tfs_filament **out);
+int tfs_filament_destroy(tfs_filament *h);
+/* Initial state (filament::initial_setup): segment-0 positions [3 per
+ * filament] and quaternions [4 per segment]; multipliers and Lie-algebra
+ * elements zero, the positions from the robot arm, their history equal. */
+int tfs_filament_set_state(tfs_filament *h, const double *bases, const double *q);
+/* First-step kick: forces [6 per filament: on seg_a, then on seg_b] added
+ * while nt == 0 (the reference's perturbation1/2 on segments 4 and 12). */
+int tfs_filament_set_perturbation(tfs_filament *h, int32_t seg_a, int32_t seg_b, const double *forces);
+/* Start of step nt: history shift, extrapolated Lie-algebra elements and
+ * multipliers, robot arm. */
+int tfs_filament_initial_guess(tfs_filament *h, int32_t nt);
+/* Internal forces and torques at the current state, f_segs [6 per segment]
+ * (overwritten). */
+int tfs_filament_forces(tfs_filament *h, int32_t nt, double *f_segs);
+/* J0 filament blocks of step nt at the current state, with f_segs the
+ * internal forces there; written into `blocks` at the swimmer layout above
+ * (body blocks untouched). */
+int tfs_filament_jacobian(tfs_filament *h, int32_t nt, const double *f_segs, double *blocks);
+/* Adds a Broyden update (the whole vector, swimmer layout) and re-runs the
+ * robot arm. */
+int tfs_filament_update(tfs_filament *h, const double *update);
+
+/* Waits for the handle's work (device-memory outputs are then complete). */
+int tfs_filament_sync(tfs_filament *h);
+
+/* Device arrays of the state (owned by the handle), e.g. for
+ * tfs_broyden_compute_errors: */
+#define TFS_FIL_X 0     /* 3 per segment, positions at n+1 */
+#define TFS_FIL_XM1 1   /* at n                            */
+#define TFS_FIL_XM2 2   /* at n-1                          */
+#define TFS_FIL_U 3     /* 3 per segment, Lie algebra at n+1 */
+#define TFS_FIL_UM1 4
+#define TFS_FIL_Q 5     /* 4 per segment                   */
+#define TFS_FIL_MULT 6  /* 3 (NSEG+1) per filament          */
+int tfs_filament_device_array(tfs_filament *h, int32_t which, double **ptr);
+/* Host copy of one of those arrays (synchronises). */
+int tfs_filament_read(tfs_filament *h, int32_t which, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TFS_FILAMENT_H */

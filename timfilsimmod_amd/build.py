@@ -28,6 +28,7 @@ SOURCES = [
     "tfs_flow_api.hip",
     "tfs_broyden_api.hip",
     "tfs_velocities_api.hip",
+    "tfs_filament_api.hip",
     "tfs_io.cpp",
 ]
 

@@ -1,0 +1,772 @@
+// tfs_filament_api.hip -- the Cosserat filament model on the device
+// (include/tfs_filament.h).
+//
+// The reference runs these per filament in host loops over `matrix` objects,
+// between every pair of mobility solves of a Broyden iteration
+// (cilia_sim_main.cu:394-502).  Here all filaments of all swimmers go at once,
+// in FP64: forces one thread per segment (each adds the two links it touches,
+// so no atomics and a fixed summation order), the approximate Jacobian one
+// workgroup per filament (each thread owns the position and Lie-algebra rows
+// of one segment), the sequential robot arm one thread per filament.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/tfs_filament.h"
+#include "tfs_api_internal.h"
+
+using namespace tfs;
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+struct Q {
+    double w, x, y, z;
+};
+
+__device__ __forceinline__ Q qmul(const Q& p, const Q& q) {
+    return Q{p.w * q.w - (p.x * q.x + p.y * q.y + p.z * q.z), p.w * q.x + q.w * p.x + p.y * q.z - p.z * q.y,
+             p.w * q.y + q.w * p.y + p.z * q.x - p.x * q.z, p.w * q.z + q.w * p.z + p.x * q.y - p.y * q.x};
+}
+
+__device__ __forceinline__ Q qconj(const Q& q) { return Q{q.w, -q.x, -q.y, -q.z}; }
+
+// Principal square root (quaternion.cpp:245-270).
+__device__ __forceinline__ Q qsqrt(const Q& q) {
+    if (q.w < -0.999999999) return Q{0.0, 0.0, 0.0, 1.0};
+    const double s = sqrt(0.5 * (1.0 + q.w));
+    const double d = 2.0 * s;
+    return Q{s, q.x / d, q.y / d, q.z / d};
+}
+
+// (q2 q1*)^(1/2) q1 (quaternion.cpp:555-564).
+__device__ __forceinline__ Q midpoint(const Q& q1, const Q& q2) { return qmul(qsqrt(qmul(q2, qconj(q1))), q1); }
+
+// exp map of the Lie algebra (quaternion.cpp:522-545).
+__device__ __forceinline__ Q lie_exp(const double* u) {
+    const double th = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    if (th < 1e-6) return Q{1.0, 0.0, 0.0, 0.0};
+    const double sn = sin(0.5 * th) / th;
+    return Q{cos(0.5 * th), sn * u[0], sn * u[1], sn * u[2]};
+}
+
+__device__ __forceinline__ Q load_q(const double* p) { return Q{p[0], p[1], p[2], p[3]}; }
+__device__ __forceinline__ void store_q(double* p, const Q& q) {
+    p[0] = q.w;
+    p[1] = q.x;
+    p[2] = q.y;
+    p[3] = q.z;
+}
+
+// Columns of the rotation matrix: tangent, normal, binormal (quaternion.cpp:285-307).
+__device__ __forceinline__ void tangent(const Q& q, double* t) {
+    t[0] = 1.0 - 2.0 * (q.y * q.y + q.z * q.z);
+    t[1] = 2.0 * (q.x * q.y + q.w * q.z);
+    t[2] = 2.0 * (q.x * q.z - q.w * q.y);
+}
+__device__ __forceinline__ void normal(const Q& q, double* n) {
+    n[0] = 2.0 * (q.x * q.y - q.w * q.z);
+    n[1] = 1.0 - 2.0 * (q.x * q.x + q.z * q.z);
+    n[2] = 2.0 * (q.y * q.z + q.w * q.x);
+}
+__device__ __forceinline__ void binormal(const Q& q, double* b) {
+    b[0] = 2.0 * (q.x * q.z + q.w * q.y);
+    b[1] = 2.0 * (q.y * q.z - q.w * q.x);
+    b[2] = 1.0 - 2.0 * (q.x * q.x + q.y * q.y);
+}
+
+// 3 x 3 row-major helpers.
+struct M3 {
+    double a[9];
+};
+__device__ __forceinline__ M3 zero3() {
+    M3 m;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) m.a[i] = 0.0;
+    return m;
+}
+// rcross(v): M x = x cross v (matrix.cpp:665-679).
+__device__ __forceinline__ M3 rcross(const double* v) {
+    M3 m;
+    m.a[0] = 0.0;
+    m.a[1] = v[2];
+    m.a[2] = -v[1];
+    m.a[3] = -v[2];
+    m.a[4] = 0.0;
+    m.a[5] = v[0];
+    m.a[6] = v[1];
+    m.a[7] = -v[0];
+    m.a[8] = 0.0;
+    return m;
+}
+__device__ __forceinline__ M3 mm(const M3& A, const M3& B) {
+    M3 C;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            C.a[3 * i + j] = A.a[3 * i] * B.a[j] + A.a[3 * i + 1] * B.a[3 + j] + A.a[3 * i + 2] * B.a[6 + j];
+    return C;
+}
+__device__ __forceinline__ M3 axpy(double s, const M3& A, const M3& B) {  // s A + B
+    M3 C;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) C.a[i] = s * A.a[i] + B.a[i];
+    return C;
+}
+__device__ __forceinline__ M3 scale(double s, const M3& A) {
+    M3 C;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) C.a[i] = s * A.a[i];
+    return C;
+}
+__device__ __forceinline__ void mv(const M3& A, const double* v, double* out) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[i] = A.a[3 * i] * v[0] + A.a[3 * i + 1] * v[1] + A.a[3 * i + 2] * v[2];
+}
+// Rotation matrix of q (columns t, n, b).
+__device__ __forceinline__ M3 rot(const Q& q) {
+    double t[3], n[3], b[3];
+    tangent(q, t);
+    normal(q, n);
+    binormal(q, b);
+    M3 R;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        R.a[3 * i] = t[i];
+        R.a[3 * i + 1] = n[i];
+        R.a[3 * i + 2] = b[i];
+    }
+    return R;
+}
+
+__device__ __forceinline__ double dexpinv_fac(const double* u) {
+    const double th2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const double th = sqrt(th2);
+    if (th < 1e-5) return -1.0 / 12.0;
+    return (0.5 * th * cos(0.5 * th) / sin(0.5 * th) - 1.0) / th2;
+}
+
+__device__ __forceinline__ void cross(const double* a, const double* b, double* c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// dexpinv(u)^T v (quaternion.cpp:635-660).
+__device__ __forceinline__ void dexpinv_transpose(const double* u, const double* v, double* out) {
+    const double fac = dexpinv_fac(u);
+    double c[3], cc[3];
+    cross(u, v, c);
+    cross(u, c, cc);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[i] = v[i] + 0.5 * c[i] - fac * cc[i];
+}
+
+struct FilParams {
+    int nfil, nseg, fil_per_swim;
+    double dl, kb, kt, mu, a, dt, end_force;
+    double st[3];
+    int pert_a, pert_b;
+};
+
+// Body-frame elastic moment of the link q1 -> q2 (filament.cpp:1451-1465).
+__device__ void body_frame_moment(const Q& q1, const Q& q2, const FilParams& P, double* M) {
+    const Q mid = qconj(midpoint(q1, q2));
+    const double inv = 1.0 / P.dl;
+    const Q d{(q2.w - q1.w) * inv, (q2.x - q1.x) * inv, (q2.y - q1.y) * inv, (q2.z - q1.z) * inv};
+    const Q m = qmul(mid, d);
+    M[0] = P.kt * (2.0 * m.x - P.st[0]);
+    M[1] = P.kb * (2.0 * m.y - P.st[1]);
+    M[2] = P.kb * (2.0 * m.z - P.st[2]);
+}
+
+// d(body-frame moment)/d(rotation of q2 (upper) or q1) (filament.cpp:9-55):
+// column c from the quaternion A_c = +-psi(q) e_c / DL.
+__device__ M3 moment_lie_derivative(const Q& q1, const Q& q2, bool upper, const FilParams& P) {
+    const Q mid = qconj(midpoint(q1, q2));
+    const double inv = 1.0 / P.dl;
+    const Q dq{(q2.w - q1.w) * inv, (q2.x - q1.x) * inv, (q2.y - q1.y) * inv, (q2.z - q1.z) * inv};
+    const Q qs = upper ? q2 : q1;
+    const double sa = (upper ? 0.5 : -0.5) * inv;     // psi carries the factor 1/2
+    const double sb = upper ? 0.5 * P.dl : -0.5 * P.dl;
+    const double kscale[3] = {P.kt, P.kb, P.kb};
+    M3 out;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Q e{0.0, c == 0 ? 1.0 : 0.0, c == 1 ? 1.0 : 0.0, c == 2 ? 1.0 : 0.0};
+        Q A = qmul(e, qs);
+        A = Q{sa * A.w, sa * A.x, sa * A.y, sa * A.z};
+        const Q t1 = qmul(mid, A);
+        const Q t2 = qmul(dq, Q{sb * A.w, sb * A.x, sb * A.y, sb * A.z});
+        out.a[c] = 2.0 * (t1.x + t2.x) * kscale[0];
+        out.a[3 + c] = 2.0 * (t1.y + t2.y) * kscale[1];
+        out.a[6 + c] = 2.0 * (t1.z + t2.z) * kscale[2];
+    }
+    return out;
+}
+
+// K0 rcross(t) + K1 rcross(n) + K2 rcross(b) of q.
+__device__ M3 frame_cross(const Q& q, const double* K) {
+    double t[3], n[3], b[3];
+    tangent(q, t);
+    normal(q, n);
+    binormal(q, b);
+    double v[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) v[i] = K[0] * t[i];
+    M3 m = rcross(v);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) v[i] = K[1] * n[i];
+    M3 m2 = rcross(v);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) v[i] = K[2] * b[i];
+    M3 m3 = rcross(v);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) m.a[i] += m2.a[i] + m3.a[i];
+    return m;
+}
+
+struct FilState {
+    double *q, *qm1, *u, *um1, *x, *xm1, *xm2;
+    double *mult, *mult_m1, *mult_m2;  // 3 (nseg + 1) per filament
+    double *bases;                     // 3 per filament
+    double *pert;                      // 6 per filament (may be null)
+};
+
+// ---- kernels ---------------------------------------------------------------
+__device__ void robot_arm(const FilState& S, const FilParams& P, int f) {
+    const int64_t s0 = (int64_t)f * P.nseg;
+    double t1[3], t2[3];
+    tangent(load_q(S.q + 4 * s0), t1);
+    for (int i = 1; i < P.nseg; ++i) {
+        tangent(load_q(S.q + 4 * (s0 + i)), t2);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            S.x[3 * (s0 + i) + k] = S.x[3 * (s0 + i - 1) + k] + 0.5 * P.dl * (t1[k] + t2[k]);
+            t1[k] = t2[k];
+        }
+    }
+}
+
+__global__ void set_state_kernel(FilState S, FilParams P) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= P.nfil) return;
+    const int64_t s0 = (int64_t)f * P.nseg;
+    for (int i = 0; i < P.nseg; ++i) {
+        const int64_t g = s0 + i;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) S.qm1[4 * g + k] = S.q[4 * g + k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) S.u[3 * g + k] = S.um1[3 * g + k] = 0.0;
+    }
+    for (int k = 0; k < 3; ++k) S.x[3 * s0 + k] = S.bases[3 * f + k];
+    robot_arm(S, P, f);
+    for (int i = 0; i < 3 * P.nseg; ++i) S.xm1[3 * s0 + i] = S.xm2[3 * s0 + i] = S.x[3 * s0 + i];
+    const int nm = 3 * (P.nseg + 1);
+    for (int i = 0; i < nm; ++i) S.mult[(int64_t)f * nm + i] = S.mult_m1[(int64_t)f * nm + i] = S.mult_m2[(int64_t)f * nm + i] = 0.0;
+}
+
+// filament::initial_guess for a body held still.
+__global__ void initial_guess_kernel(FilState S, FilParams P, int nt) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= P.nfil) return;
+    const int64_t s0 = (int64_t)f * P.nseg;
+    for (int i = 0; i < P.nseg; ++i) {
+        const int64_t g = s0 + i;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            S.xm2[3 * g + k] = S.xm1[3 * g + k];
+            S.xm1[3 * g + k] = S.x[3 * g + k];
+        }
+        if (nt <= 1) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) S.um1[3 * g + k] = S.u[3 * g + k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double t = 2.0 * S.u[3 * g + k] - S.um1[3 * g + k];
+                S.um1[3 * g + k] = S.u[3 * g + k];
+                S.u[3 * g + k] = t;
+            }
+        }
+        const Q q = load_q(S.q + 4 * g);
+        store_q(S.qm1 + 4 * g, q);
+        if (i == 0) {
+            // clamped to a body that does not turn: u_0 = 0, q_0 unchanged
+#pragma unroll
+            for (int k = 0; k < 3; ++k) S.u[3 * g + k] = 0.0;
+        } else {
+            store_q(S.q + 4 * g, qmul(lie_exp(S.u + 3 * g), q));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) S.x[3 * s0 + k] = S.bases[3 * f + k];
+    const int nm = 3 * (P.nseg + 1);
+    double* m = S.mult + (int64_t)f * nm;
+    double* m1 = S.mult_m1 + (int64_t)f * nm;
+    double* m2 = S.mult_m2 + (int64_t)f * nm;
+    for (int i = 0; i < nm; ++i) {
+        if (nt == 1) {
+            m1[i] = m[i];
+        } else if (nt == 2) {
+            m2[i] = m1[i];
+            m1[i] = m[i];
+            m[i] = 2.0 * m1[i] - m2[i];
+        } else if (nt > 2) {
+            const double t = 3.0 * (m[i] - m1[i]) + m2[i];
+            m2[i] = m1[i];
+            m1[i] = m[i];
+            m[i] = t;
+        }
+    }
+    robot_arm(S, P, f);
+}
+
+// filament::internal_forces_and_torques, one thread per segment.
+__global__ void forces_kernel(FilState S, FilParams P, int nt, double* __restrict__ out) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)P.nfil * P.nseg) return;
+    const int f = (int)(g / P.nseg), i = (int)(g % P.nseg), N = P.nseg;
+    const int nm = 3 * (N + 1);
+    const double* mult = S.mult + (int64_t)f * nm;
+    const Q qi = load_q(S.q + 4 * g);
+    double t[3];
+    tangent(qi, t);
+    double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+    if (i > 0) {  // link i-1 -> i
+        const double* lam = mult + 3 * i;
+        const Q qp = load_q(S.q + 4 * (g - 1));
+        double c[3], Mb[3], M[3];
+        cross(t, lam, c);
+        body_frame_moment(qp, qi, P, Mb);
+        mv(rot(midpoint(qp, qi)), Mb, M);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            F[k] += lam[k];
+            T[k] += -0.5 * P.dl * c[k] - M[k];
+        }
+    }
+    if (i + 1 < N) {  // link i -> i+1
+        const double* lam = mult + 3 * (i + 1);
+        const Q qn = load_q(S.q + 4 * (g + 1));
+        double c[3], Mb[3], M[3];
+        cross(t, lam, c);
+        body_frame_moment(qi, qn, P, Mb);
+        mv(rot(midpoint(qi, qn)), Mb, M);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            F[k] -= lam[k];
+            T[k] += -0.5 * P.dl * c[k] + M[k];
+        }
+    }
+    if (i == 0) {
+        double ct[3];
+        dexpinv_transpose(S.u + 3 * g, mult + 3 * N, ct);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            F[k] += mult[k];
+            T[k] += ct[k];
+        }
+    }
+    if (i == N - 1) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) F[k] -= P.end_force * t[k];
+    }
+    if (nt == 0 && S.pert) {
+        if (i == P.pert_a)
+            for (int k = 0; k < 3; ++k) F[k] += S.pert[6 * f + k];
+        if (i == P.pert_b)
+            for (int k = 0; k < 3; ++k) F[k] += S.pert[6 * f + 3 + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        out[6 * g + k] = F[k];
+        out[6 * g + 3 + k] = T[k];
+    }
+}
+
+__device__ __forceinline__ void put3(double* J, int64_t ld, int r, int c, const M3& B) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) J[(int64_t)(r + i) * ld + c + j] = B.a[3 * i + j];
+}
+
+// Approximate Jacobian of one filament (filament.cpp:1577-1894), one
+// workgroup per filament; thread k owns rows 3k.. (positions) and 3N+3k..
+// (Lie algebra) of segment k.
+__global__ __launch_bounds__(64) void jacobian_kernel(FilState S, FilParams P, int bdf2,
+                                                      const double* __restrict__ forces, double* __restrict__ blocks) {
+    const int f = blockIdx.x;
+    const int N = P.nseg, n6 = 6 * N;
+    const int swim = f / P.fil_per_swim, fs = f % P.fil_per_swim;
+    double* J = blocks + (int64_t)swim * ((int64_t)P.fil_per_swim * n6 * n6 + 36) + (int64_t)fs * n6 * n6;
+    for (int64_t e = threadIdx.x; e < (int64_t)n6 * n6; e += blockDim.x) J[e] = 0.0;
+    __syncthreads();
+    const int64_t s0 = (int64_t)f * N;
+    const int nm = 3 * (N + 1);
+    const double* mult = S.mult + (int64_t)f * nm;
+    const double pi = kPi, mu = P.mu, a = P.a;
+    const double fac = bdf2 ? -P.dt / (9.0 * pi * mu * a) : -P.dt / (6.0 * pi * mu * a);
+    const double fac2 = bdf2 ? P.dl * P.dt / (24.0 * pi * mu * a * a * a) : P.dl * P.dt / (16.0 * pi * mu * a * a * a);
+    const double fac3 = bdf2 ? -P.dt / (12.0 * pi * mu * a * a * a) : -P.dt / (8.0 * pi * mu * a * a * a);
+    const double tsf = bdf2 ? 2.0 * P.dt / 3.0 : P.dt;
+    const double Tfac = 1.0 / (8.0 * pi * mu * a * a * a);
+    M3 I3 = zero3();
+    I3.a[0] = I3.a[4] = I3.a[8] = 1.0;
+    for (int k = threadIdx.x; k < N; k += blockDim.x) {
+        const int64_t g = s0 + k;
+        const Q qk = load_q(S.q + 4 * g);
+        double t[3];
+        tangent(qk, t);
+        // position rows: multipliers (diagonal fac, next -fac), then the
+        // rotations of segments 1..k
+        for (int c = 0; c < 3; ++c) {
+            J[(int64_t)(3 * k + c) * n6 + 3 * k + c] = fac;
+            if (k < N - 1) J[(int64_t)(3 * k + c) * n6 + 3 * k + 3 + c] = -fac;
+        }
+        for (int j = 1; j <= k; ++j) {
+            double tj[3];
+            tangent(load_q(S.q + 4 * (s0 + j)), tj);
+            M3 B = scale(j == k ? 0.5 * P.dl : P.dl, rcross(tj));
+            if (k == N - 1 && j == N - 1) B = axpy(-fac * P.end_force, rcross(tj), B);
+            put3(J, n6, 3 * k, 3 * (N + j), B);
+        }
+        // Lie rows, multiplier columns 3k (k >= 1) and 3(k+1) (k+1 <= N-1)
+        const M3 um = rcross(S.u + 3 * g);
+        const M3 tm = scale(-1.0, rcross(t));
+        const M3 ut = mm(um, tm);
+        M3 LB = axpy(0.5, ut, tm);
+        LB = axpy(1.0 / 12.0, mm(um, ut), LB);
+        LB = scale(fac2, LB);
+        if (k >= 1) put3(J, n6, 3 * N + 3 * k, 3 * k, LB);
+        if (k + 1 <= N - 1) put3(J, n6, 3 * N + 3 * k, 3 * (k + 1), LB);
+        // Lie-Lie blocks (jacobian_lie_algebra_block)
+        const M3 L = scale(-1.0, rcross(S.u + 3 * g));
+        double om[3];
+        for (int c = 0; c < 3; ++c) om[c] = Tfac * forces[6 * g + 3 + c];
+        const M3 Om = rcross(om);
+        M3 elastic = zero3();
+        double lamsum[3] = {0.0, 0.0, 0.0};
+        if (k + 1 < N) {
+            const Q qn = load_q(S.q + 4 * (g + 1));
+            double Kph[3];
+            body_frame_moment(qk, qn, P, Kph);
+            const M3 R = rot(midpoint(qk, qn));
+            elastic = axpy(0.5, frame_cross(qk, Kph), mm(R, moment_lie_derivative(qk, qn, false, P)));
+            const M3 right = scale(Tfac, axpy(0.5, frame_cross(qn, Kph), mm(R, moment_lie_derivative(qk, qn, true, P))));
+            M3 B = axpy(-0.5, mm(L, right), right);
+            B = axpy(1.0 / 12.0, mm(L, mm(L, right)), B);
+            put3(J, n6, 3 * N + 3 * k, 3 * N + 3 * k + 3, scale(-tsf, B));
+            for (int c = 0; c < 3; ++c) lamsum[c] += mult[3 * (k + 1) + c];
+        }
+        if (k > 0) {
+            const Q qp = load_q(S.q + 4 * (g - 1));
+            double Kmh[3];
+            body_frame_moment(qp, qk, P, Kmh);
+            const M3 R = rot(midpoint(qp, qk));
+            elastic = axpy(-1.0, axpy(0.5, frame_cross(qk, Kmh), mm(R, moment_lie_derivative(qp, qk, true, P))),
+                           elastic);
+            const M3 left =
+                scale(-Tfac, axpy(0.5, frame_cross(qp, Kmh), mm(R, moment_lie_derivative(qp, qk, false, P))));
+            M3 B = axpy(-0.5, mm(L, left), left);
+            B = axpy(1.0 / 12.0, mm(L, mm(L, left)), B);
+            // segment 1's block on the clamp column is zero (filament.cpp:1887)
+            if (k != 1) put3(J, n6, 3 * N + 3 * k, 3 * N + 3 * k - 3, scale(-tsf, B));
+            for (int c = 0; c < 3; ++c) lamsum[c] += mult[3 * k + c];
+        }
+        if (k == 0) {
+            // the clamp multiplier replaces u_0 (filament.cpp:1866-1888)
+            const M3 blk = axpy(1.0 / 12.0, mm(um, um), axpy(0.5, um, I3));
+            const M3 blkT = axpy(1.0 / 12.0, mm(um, um), axpy(-0.5, um, I3));
+            put3(J, n6, 3 * N, 3 * N, scale(fac3, mm(blk, blkT)));
+        } else {
+            const M3 constraint = scale(-0.5 * P.dl, mm(rcross(lamsum), rcross(t)));
+            const M3 dOm = scale(Tfac, axpy(1.0, elastic, constraint));
+            double ou[3];
+            mv(Om, S.u + 3 * g, ou);
+            M3 B = axpy(-0.5, axpy(1.0, Om, mm(L, dOm)), dOm);
+            const M3 hi = axpy(1.0, rcross(ou), axpy(1.0, mm(L, Om), mm(L, mm(L, dOm))));
+            B = axpy(1.0 / 12.0, hi, B);
+            put3(J, n6, 3 * N + 3 * k, 3 * N + 3 * k, axpy(-tsf, B, I3));
+        }
+    }
+}
+
+// filament::update for every filament, then the robot arm.
+__global__ void update_kernel(FilState S, FilParams P, const double* __restrict__ du) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= P.nfil) return;
+    const int N = P.nseg, n6 = 6 * N;
+    const int swim = f / P.fil_per_swim, fs = f % P.fil_per_swim;
+    const double* d = du + (int64_t)swim * ((int64_t)P.fil_per_swim * n6 + 6) + (int64_t)fs * n6;
+    const int nm = 3 * (N + 1);
+    double* m = S.mult + (int64_t)f * nm;
+    for (int i = 0; i < nm; ++i) m[i] += d[i];   // tether, lambda..., clamp
+    const int64_t s0 = (int64_t)f * N;
+    for (int i = 1; i < N; ++i) {
+        const int64_t g = s0 + i;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) S.u[3 * g + k] += d[3 * (N + i) + k];
+        store_q(S.q + 4 * g, qmul(lie_exp(S.u + 3 * g), load_q(S.qm1 + 4 * g)));
+    }
+    robot_arm(S, P, f);
+}
+
+}  // namespace
+
+struct tfs_filament {
+    tfs_filament_desc desc{};
+    FilParams P{};
+    FilState S{};
+    double* arena = nullptr;
+    double* stage = nullptr;  // host-memory staging (forces, blocks, updates)
+    size_t stage_len = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+};
+
+namespace {
+
+int check(hipError_t e, const char* where) { return e == hipSuccess ? TFS_OK : hip_fail(e, where); }
+
+int64_t num_segs(const tfs_filament* h) { return (int64_t)h->P.nfil * h->P.nseg; }
+int64_t update_len(const tfs_filament* h) {
+    return (int64_t)h->desc.num_swim * ((int64_t)h->desc.fil_per_swim * 6 * h->P.nseg + 6);
+}
+int64_t blocks_len(const tfs_filament* h) {
+    const int64_t n6 = 6 * h->P.nseg;
+    return (int64_t)h->desc.num_swim * ((int64_t)h->desc.fil_per_swim * n6 * n6 + 36);
+}
+
+double* staging(tfs_filament* h, size_t n) {
+    if (h->stage_len < n) {
+        if (h->stage) (void)hipFree(h->stage);
+        h->stage = nullptr;
+        h->stage_len = 0;
+        if (hipMalloc(&h->stage, n * 8) != hipSuccess) return nullptr;
+        h->stage_len = n;
+    }
+    return h->stage;
+}
+
+unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+extern "C" {
+
+int tfs_filament_create(const tfs_filament_desc* desc, tfs_filament** out) {
+    g_last_error.clear();
+    if (!desc || !out) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_create: null argument");
+    *out = nullptr;
+    const tfs_filament_desc& D = *desc;
+    if (D.num_swim < 1 || D.fil_per_swim < 1 || D.num_seg < 2)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_create: need swimmers, filaments and >= 2 segments");
+    if (!(D.dl > 0) || !(D.mu > 0) || !(D.radius > 0) || !(D.dt > 0) || !std::isfinite(D.kb) ||
+        !std::isfinite(D.kt) || !std::isfinite(D.end_force))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_create: dl, mu, radius, dt must be positive");
+    if (D.memory != TFS_MEMORY_HOST && D.memory != TFS_MEMORY_DEVICE)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_create: unknown memory space");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(TFS_ERR_NO_DEVICE, "tfs_filament_create: no GPU");
+    tfs_filament* h = new tfs_filament();
+    h->desc = D;
+    FilParams& P = h->P;
+    P.nfil = D.num_swim * D.fil_per_swim;
+    P.nseg = D.num_seg;
+    P.fil_per_swim = D.fil_per_swim;
+    P.dl = D.dl;
+    P.kb = D.kb;
+    P.kt = D.kt;
+    P.mu = D.mu;
+    P.a = D.radius;
+    P.dt = D.dt;
+    P.end_force = D.end_force;
+    for (int k = 0; k < 3; ++k) P.st[k] = D.strain_twist[k];
+    P.pert_a = P.pert_b = -1;
+    const int64_t ns = num_segs(h), nf = P.nfil, nm = 3LL * (P.nseg + 1) * nf;
+    std::vector<std::pair<double**, int64_t>> parts = {
+        {&h->S.q, 4 * ns},   {&h->S.qm1, 4 * ns},  {&h->S.u, 3 * ns},      {&h->S.um1, 3 * ns},
+        {&h->S.x, 3 * ns},   {&h->S.xm1, 3 * ns},  {&h->S.xm2, 3 * ns},    {&h->S.mult, nm},
+        {&h->S.mult_m1, nm}, {&h->S.mult_m2, nm},  {&h->S.bases, 3 * nf},  {&h->S.pert, 6 * nf}};
+    int64_t total = 0;
+    for (auto& p : parts) total += (p.second + 31) / 32 * 32;
+    if (hipMalloc(&h->arena, total * 8) != hipSuccess) {
+        delete h;
+        return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_filament_create: device memory");
+    }
+    double* p = h->arena;
+    for (auto& q : parts) {
+        *q.first = p;
+        p += (q.second + 31) / 32 * 32;
+    }
+    h->S.pert = nullptr;  // set by tfs_filament_set_perturbation
+    if (D.stream) {
+        h->stream = (hipStream_t)D.stream;
+    } else {
+        if (hipStreamCreate(&h->stream) != hipSuccess) {
+            (void)hipFree(h->arena);
+            delete h;
+            return fail(TFS_ERR_HIP, "tfs_filament_create: stream");
+        }
+        h->own_stream = true;
+    }
+    if (hipMemsetAsync(h->arena, 0, total * 8, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) {
+        tfs_filament_destroy(h);
+        return fail(TFS_ERR_HIP, "tfs_filament_create: clear");
+    }
+    *out = h;
+    return TFS_OK;
+}
+
+int tfs_filament_destroy(tfs_filament* h) {
+    if (!h) return TFS_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->arena) (void)hipFree(h->arena);
+    if (h->stage) (void)hipFree(h->stage);
+    if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return TFS_OK;
+}
+
+int tfs_filament_set_state(tfs_filament* h, const double* bases, const double* q) {
+    if (!h || !bases || !q) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_set_state: null argument");
+    const hipMemcpyKind k = h->desc.memory == TFS_MEMORY_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    hipError_t e = hipMemcpyAsync(h->S.bases, bases, 24LL * h->P.nfil, k, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h->S.q, q, 32 * num_segs(h), k, h->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(set_state_kernel, dim3(grid_for(h->P.nfil, 64)), dim3(64), 0, h->stream, h->S, h->P);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    return check(e, "tfs_filament_set_state");
+}
+
+int tfs_filament_set_perturbation(tfs_filament* h, int32_t seg_a, int32_t seg_b, const double* forces) {
+    if (!h || !forces) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_set_perturbation: null argument");
+    if (seg_a < 0 || seg_a >= h->P.nseg || seg_b < 0 || seg_b >= h->P.nseg)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_set_perturbation: segment index out of range");
+    // the perturbation slab follows the bases in the arena
+    double* pert = h->S.bases + (3LL * h->P.nfil + 31) / 32 * 32;
+    const hipMemcpyKind k = h->desc.memory == TFS_MEMORY_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    hipError_t e = hipMemcpyAsync(pert, forces, 48LL * h->P.nfil, k, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(e, "tfs_filament_set_perturbation");
+    h->S.pert = pert;
+    h->P.pert_a = seg_a;
+    h->P.pert_b = seg_b;
+    return TFS_OK;
+}
+
+int tfs_filament_initial_guess(tfs_filament* h, int32_t nt) {
+    if (!h || nt < 0) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_initial_guess: null handle or nt < 0");
+    hipLaunchKernelGGL(initial_guess_kernel, dim3(grid_for(h->P.nfil, 64)), dim3(64), 0, h->stream, h->S, h->P,
+                       (int)nt);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && h->desc.memory == TFS_MEMORY_HOST) e = hipStreamSynchronize(h->stream);
+    return check(e, "tfs_filament_initial_guess");
+}
+
+int tfs_filament_forces(tfs_filament* h, int32_t nt, double* f_segs) {
+    if (!h || !f_segs) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_forces: null argument");
+    const int64_t ns = num_segs(h);
+    const bool host = h->desc.memory == TFS_MEMORY_HOST;
+    double* out = host ? staging(h, 6 * ns) : f_segs;
+    if (!out) return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_filament_forces: staging memory");
+    hipLaunchKernelGGL(forces_kernel, dim3(grid_for(ns, 256)), dim3(256), 0, h->stream, h->S, h->P, (int)nt, out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && host) e = hipMemcpyAsync(f_segs, out, 48 * ns, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && host) e = hipStreamSynchronize(h->stream);
+    return check(e, "tfs_filament_forces");
+}
+
+int tfs_filament_jacobian(tfs_filament* h, int32_t nt, const double* f_segs, double* blocks) {
+    if (!h || !f_segs || !blocks) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_jacobian: null argument");
+    const int64_t ns = num_segs(h), nb = blocks_len(h);
+    const bool host = h->desc.memory == TFS_MEMORY_HOST;
+    const double* fd = f_segs;
+    double* out = blocks;
+    hipError_t e = hipSuccess;
+    if (host) {
+        double* st = staging(h, 6 * ns + nb);
+        if (!st) return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_filament_jacobian: staging memory");
+        e = hipMemcpyAsync(st, f_segs, 48 * ns, hipMemcpyHostToDevice, h->stream);
+        // body blocks are the caller's: start from the caller's array
+        if (e == hipSuccess) e = hipMemcpyAsync(st + 6 * ns, blocks, 8 * nb, hipMemcpyHostToDevice, h->stream);
+        fd = st;
+        out = st + 6 * ns;
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(jacobian_kernel, dim3((unsigned)h->P.nfil), dim3(64), 0, h->stream, h->S, h->P,
+                           nt >= 1 ? 1 : 0, fd, out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && host) e = hipMemcpyAsync(blocks, out, 8 * nb, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && host) e = hipStreamSynchronize(h->stream);
+    return check(e, "tfs_filament_jacobian");
+}
+
+int tfs_filament_update(tfs_filament* h, const double* update) {
+    if (!h || !update) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_update: null argument");
+    const int64_t n = update_len(h);
+    const bool host = h->desc.memory == TFS_MEMORY_HOST;
+    const double* du = update;
+    hipError_t e = hipSuccess;
+    if (host) {
+        double* st = staging(h, n);
+        if (!st) return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_filament_update: staging memory");
+        e = hipMemcpyAsync(st, update, 8 * n, hipMemcpyHostToDevice, h->stream);
+        du = st;
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(update_kernel, dim3(grid_for(h->P.nfil, 64)), dim3(64), 0, h->stream, h->S, h->P, du);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && host) e = hipStreamSynchronize(h->stream);
+    return check(e, "tfs_filament_update");
+}
+
+int tfs_filament_sync(tfs_filament* h) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_sync: null handle");
+    return check(hipStreamSynchronize(h->stream), "tfs_filament_sync");
+}
+
+static int array_of(tfs_filament* h, int32_t which, double** p, int64_t* len) {
+    const int64_t ns = num_segs(h);
+    switch (which) {
+        case TFS_FIL_X: *p = h->S.x; *len = 3 * ns; return TFS_OK;
+        case TFS_FIL_XM1: *p = h->S.xm1; *len = 3 * ns; return TFS_OK;
+        case TFS_FIL_XM2: *p = h->S.xm2; *len = 3 * ns; return TFS_OK;
+        case TFS_FIL_U: *p = h->S.u; *len = 3 * ns; return TFS_OK;
+        case TFS_FIL_UM1: *p = h->S.um1; *len = 3 * ns; return TFS_OK;
+        case TFS_FIL_Q: *p = h->S.q; *len = 4 * ns; return TFS_OK;
+        case TFS_FIL_MULT: *p = h->S.mult; *len = 3LL * (h->P.nseg + 1) * h->P.nfil; return TFS_OK;
+        default: return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament: unknown array");
+    }
+}
+
+int tfs_filament_device_array(tfs_filament* h, int32_t which, double** ptr) {
+    if (!h || !ptr) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_device_array: null argument");
+    int64_t len;
+    return array_of(h, which, ptr, &len);
+}
+
+int tfs_filament_read(tfs_filament* h, int32_t which, double* out) {
+    if (!h || !out) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_read: null argument");
+    double* p;
+    int64_t len;
+    int rc = array_of(h, which, &p, &len);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(out, p, 8 * len, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    return check(e, "tfs_filament_read");
+}
+
+}  // extern "C"

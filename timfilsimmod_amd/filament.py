@@ -1,0 +1,125 @@
+"""Cosserat filament model on the device, over include/tfs_filament.h.
+
+``FilamentModel`` holds every filament of every swimmer and mirrors the
+filament methods the elastic implicit timestep calls (filament.cpp):
+``initial_guess`` (:1083-1385), ``forces`` (internal_forces_and_torques,
+:1467-1575), ``jacobian`` (the J0 blocks of invert_approx_jacobian,
+:1761-1894, inverted by broyden.BroydenSolver.set_jacobian) and ``update``
+(:1896-1930).  All of it runs in libtfs_rpy.so.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+X, XM1, XM2, U, UM1, Q, MULT = range(7)
+
+
+class FilamentDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_swim", ctypes.c_int32), ("fil_per_swim", ctypes.c_int32), ("num_seg", ctypes.c_int32),
+        ("dl", ctypes.c_double), ("kb", ctypes.c_double), ("kt", ctypes.c_double), ("mu", ctypes.c_double),
+        ("radius", ctypes.c_double), ("dt", ctypes.c_double), ("end_force", ctypes.c_double),
+        ("strain_twist", ctypes.c_double * 3), ("memory", ctypes.c_int32), ("stream", ctypes.c_void_p),
+    ]
+
+
+_vp, _i32 = ctypes.c_void_p, ctypes.c_int32
+L.extra_signatures({
+    "tfs_filament_create": (ctypes.c_int, [ctypes.POINTER(FilamentDesc), ctypes.POINTER(_vp)]),
+    "tfs_filament_destroy": (ctypes.c_int, [_vp]),
+    "tfs_filament_set_state": (ctypes.c_int, [_vp, _vp, _vp]),
+    "tfs_filament_set_perturbation": (ctypes.c_int, [_vp, _i32, _i32, _vp]),
+    "tfs_filament_initial_guess": (ctypes.c_int, [_vp, _i32]),
+    "tfs_filament_forces": (ctypes.c_int, [_vp, _i32, _vp]),
+    "tfs_filament_jacobian": (ctypes.c_int, [_vp, _i32, _vp, _vp]),
+    "tfs_filament_update": (ctypes.c_int, [_vp, _vp]),
+    "tfs_filament_sync": (ctypes.c_int, [_vp]),
+    "tfs_filament_device_array": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_vp)]),
+    "tfs_filament_read": (ctypes.c_int, [_vp, _i32, _vp]),
+})
+
+
+class FilamentModel:
+    def __init__(self, num_swim, fil_per_swim, num_seg, *, dl=2.2, kb=1800.0, kt=1800.0, mu=1.0, radius=1.0,
+                 dt=1.0, end_force=0.0, strain_twist=(0.0, 0.0, 0.0), memory=L.MEMORY_HOST, stream=None):
+        d = FilamentDesc()
+        d.num_swim, d.fil_per_swim, d.num_seg = num_swim, fil_per_swim, num_seg
+        d.dl, d.kb, d.kt, d.mu, d.radius, d.dt, d.end_force = dl, kb, kt, mu, radius, dt, end_force
+        for i in range(3):
+            d.strain_twist[i] = strain_twist[i]
+        d.memory, d.stream = memory, stream
+        self.desc = d
+        self.nfil = num_swim * fil_per_swim
+        self.ns = self.nfil * num_seg
+        h = ctypes.c_void_p()
+        L.check(L.lib().tfs_filament_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            L.lib().tfs_filament_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def update_size(self):
+        d = self.desc
+        return d.num_swim * (d.fil_per_swim * 6 * d.num_seg + 6)
+
+    def block_sizes(self):
+        """Sizes of the J0 blocks in the Broyden layout (filaments, then the body)."""
+        d = self.desc
+        return ([6 * d.num_seg] * d.fil_per_swim + [6]) * d.num_swim
+
+    @staticmethod
+    def _arr(a):
+        """Device tensors pass through; anything else becomes a float64 host array."""
+        return a if hasattr(a, "data_ptr") else np.ascontiguousarray(a, dtype=np.float64)
+
+    def set_state(self, bases, q):
+        b, qq = self._arr(bases), self._arr(q)
+        L.check(L.lib().tfs_filament_set_state(self.h, L.ptr(b), L.ptr(qq)))
+
+    def set_perturbation(self, seg_a, seg_b, forces):
+        self._pert = self._arr(forces)
+        L.check(L.lib().tfs_filament_set_perturbation(self.h, int(seg_a), int(seg_b), L.ptr(self._pert)))
+
+    def initial_guess(self, nt):
+        L.check(L.lib().tfs_filament_initial_guess(self.h, int(nt)))
+
+    def forces(self, nt, out=None):
+        out = np.zeros(6 * self.ns) if out is None else out
+        L.check(L.lib().tfs_filament_forces(self.h, int(nt), L.ptr(out)))
+        return out
+
+    def jacobian(self, nt, f_segs, blocks):
+        L.check(L.lib().tfs_filament_jacobian(self.h, int(nt), L.ptr(f_segs), L.ptr(blocks)))
+        return blocks
+
+    def update(self, du):
+        d = self._arr(du)
+        L.check(L.lib().tfs_filament_update(self.h, L.ptr(d)))
+
+    def sync(self):
+        L.check(L.lib().tfs_filament_sync(self.h))
+
+    def device_array(self, which) -> int:
+        p = ctypes.c_void_p()
+        L.check(L.lib().tfs_filament_device_array(self.h, which, ctypes.byref(p)))
+        return p.value
+
+    def read(self, which):
+        n = {X: 3, XM1: 3, XM2: 3, U: 3, UM1: 3, Q: 4}.get(which)
+        size = n * self.ns if n else 3 * (self.desc.num_seg + 1) * self.nfil
+        out = np.zeros(size)
+        L.check(L.lib().tfs_filament_read(self.h, which, L.ptr(out)))
+        return out
