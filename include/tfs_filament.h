@@ -92,6 +92,46 @@ int tfs_filament_device_array(tfs_filament *h, int32_t which, double **ptr);
 /* Host copy of one of those arrays (synchronises). */
 int tfs_filament_read(tfs_filament *h, int32_t which, double *out);
 
+/* ---- One elastic implicit timestep on the device ----
+ * The time loop body of cilia_sim_main.cu:353-502 for filaments on bodies
+ * held still (PRESCRIBED_BODY_VELOCITIES, GMRES for the blob forces):
+ *   initial_guess; forces; velocities; errors;
+ *   if too large: J0 = approximate Jacobian (body blocks I), inverted on the device;
+ *   Broyden: update -> forces -> velocities -> errors -> end_of_iter, until the
+ *   residual meets TOL or max_iter iterations.
+ * The three handles are the caller's, created with TFS_MEMORY_DEVICE and sizes
+ * that agree (tfs_velocities: num_swim bodies of fil_per_swim*num_seg segments;
+ * tfs_broyden: the block sizes of the Broyden layout above).  Only the
+ * too-large flag of each residual crosses to the host. */
+struct tfs_velocities;
+struct tfs_broyden;
+typedef struct tfs_elastic_desc {
+    tfs_filament *filaments;
+    struct tfs_velocities *velocities;
+    struct tfs_broyden *broyden;
+    const double *x_blobs;     /* device, 3 per blob (fixed: the bodies do not move)  */
+    const double *x_bodies;    /* device, 3 per body                                  */
+    const double *v_bodies;    /* device, 6 per body (zero: held still)               */
+    double *f_blobs;           /* device, 3 per blob: warm start in, blob forces out  */
+    double tol;                /* TOL (config.hpp:302)                                 */
+    int32_t max_iter;          /* MAX_BROYDEN_ITER                                     */
+} tfs_elastic_desc;
+
+typedef struct tfs_elastic_stats {
+    int32_t broyden_iterations;
+    int32_t gmres_iterations;  /* summed over the blob solves of the step             */
+    int32_t converged;         /* 1 if the residual met TOL                            */
+    int32_t velocity_solves;
+    double error_norm;         /* ||error|| at the end                                 */
+} tfs_elastic_stats;
+
+typedef struct tfs_elastic tfs_elastic;
+int tfs_elastic_create(const tfs_elastic_desc *desc, tfs_elastic **out);
+int tfs_elastic_destroy(tfs_elastic *e);
+/* Step nt.  TFS_ERR_NOT_CONVERGED if max_iter is reached (state kept),
+ * TFS_ERR_NON_FINITE if a residual is not finite (cilia_sim_main.cu:469-487). */
+int tfs_elastic_step(tfs_elastic *e, int32_t nt, tfs_elastic_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

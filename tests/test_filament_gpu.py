@@ -153,66 +153,37 @@ def test_initial_guess_sequence_matches_oracle():
         model.close()
 
 
-def test_elastic_implicit_step_matches_oracle_loop():
-    print("case elastic implicit step 16x20 + 2000 blobs", flush=True)
+STEPS = 2
+
+
+def elastic_case():
     body = G.small_body()
-    nfil, nseg, nb = body.num_fil, body.num_seg, body.num_blobs
-    ns = nfil * nseg
+    nfil, nseg = body.num_fil, body.num_seg
     dl, kb, dt, tol = 2.2, 1800.0, 0.1, 1e-8
     efm = 220.0 * kb / (dl * dl * nseg * nseg)
-    p = F.Params(nseg=nseg, dl=dl, kb=kb, kt=kb, dt=dt, end_force=efm)
     ex = np.array([1.0, 0.0, 0.0])
     q0 = np.array([np.roll(Rotation.align_vectors([nrm], [ex])[0].as_quat(), 1) for nrm in body.fil_normal])
     qs = np.repeat(q0[:, None, :], nseg, axis=1)
-    rng = np.random.default_rng(0)
-    pert = rng.standard_normal((nfil, 6)) * 0.1 * efm
-    zero3, zero6 = np.zeros(3), np.zeros(6)
+    pert = np.random.default_rng(0).standard_normal((nfil, 6)) * 0.1 * efm
+    return dict(body=body, nfil=nfil, nseg=nseg, nb=body.num_blobs, dl=dl, kb=kb, dt=dt, tol=tol, efm=efm, qs=qs,
+                pert=pert)
 
-    def state(x, xm1, xm2, u, um1, v):
-        return {"seg_x": x, "seg_xm1": xm1, "seg_xm2": xm2, "seg_u": u, "seg_um1": um1, "v_segs": v,
-                "body_x": zero3, "body_xm1": zero3, "body_xm2": zero3, "body_u": zero3, "body_um1": zero3,
-                "v_bodies": zero6}
 
-    steps = 2
-    # device loop
-    model = FM.FilamentModel(1, nfil, nseg, dl=dl, kb=kb, kt=kb, dt=dt, end_force=efm)
-    ev = ElasticVelocities(1, ns, nb, mode=SOLVE_FP64, restart=30, max_cycles=200, tol=1e-10)
-    eng = BroydenSolver(model.block_sizes())
-    gpu_iters, gpu_x = [], []
-    try:
-        model.set_state(body.fil_base.reshape(-1), qs.reshape(-1))
-        model.set_perturbation(4, 12, pert.reshape(-1))
-        n6 = 6 * nseg
-        blocks = np.zeros(nfil * n6 * n6 + 36)
-        blocks[nfil * n6 * n6:] = np.eye(6).reshape(-1)
-        for nt in range(steps):
-            model.initial_guess(nt)
+def residual_state(x, xm1, xm2, u, um1, v):
+    z3, z6 = np.zeros(3), np.zeros(6)
+    return {"seg_x": x, "seg_xm1": xm1, "seg_xm2": xm2, "seg_u": u, "seg_um1": um1, "v_segs": v,
+            "body_x": z3, "body_xm1": z3, "body_xm2": z3, "body_u": z3, "body_um1": z3, "v_bodies": z6}
 
-            def evaluate():
-                f = model.forces(nt)
-                v, _, _ = ev.compute(model.read(FM.X), f, body.x_blobs, zero3, zero6)
-                big = eng.compute_errors(state(*[model.read(w) for w in (FM.X, FM.XM1, FM.XM2, FM.U, FM.UM1)], v),
-                                         1, num_swim=1, num_fil=nfil, num_seg=nseg, bdf2=nt >= 1, dt=dt, dl=dl,
-                                         tol=tol)
-                return f, big
 
-            f, big = evaluate()
-            model.jacobian(nt, f, blocks)
-            eng.set_jacobian(blocks)
-            eng.start(eng.read_error(1))
-            while big and eng.iter < 400:
-                model.update(eng.find_update())
-                _, big = evaluate()
-                eng.end_of_iter(eng.read_error(1), big)
-            gpu_iters.append(eng.iter)
-            gpu_x.append(model.read(FM.X))
-    finally:
-        model.close()
-        ev.close()
-        eng.close()
-
-    # oracle loop (cilia_sim_main.cu:353-502 for a body held still)
-    fils = [F.Filament.at_rest(body.fil_base[i], qs[i], nseg, dl) for i in range(nfil)]
+@pytest.fixture(scope="module")
+def oracle_run():
+    """The elastic time loop (cilia_sim_main.cu:353-502, body held still) on the oracle."""
+    c = elastic_case()
+    body, nfil, nseg, nb, dl, dt, tol = c["body"], c["nfil"], c["nseg"], c["nb"], c["dl"], c["dt"], c["tol"]
+    ns = nfil * nseg
+    p = F.Params(nseg=nseg, dl=dl, kb=c["kb"], kt=c["kb"], dt=dt, end_force=c["efm"])
+    fils = [F.Filament.at_rest(body.fil_base[i], c["qs"][i], nseg, dl) for i in range(nfil)]
+    x_start = np.concatenate([f.x.reshape(-1) for f in fils])
     lu = sla.lu_factor(O.dense_mobility(np.zeros(0), body.x_blobs))
 
     def vel(x, f):
@@ -222,17 +193,17 @@ def test_elastic_implicit_step_matches_oracle_loop():
         return O.block("sb", x, body.x_blobs, f, lam, vs, seg_dof=6)
 
     cat = lambda name: np.concatenate([getattr(fl, name).reshape(-1) for fl in fils])
-    ref_iters, ref_x = [], []
-    for nt in range(steps):
+    iters, xs = [], []
+    for nt in range(STEPS):
         for fl in fils:
             F.initial_guess(fl, nt, p)
 
         def evaluate():
-            f = np.concatenate([F.internal_forces(fl, p, {4: pert[i, :3], 12: pert[i, 3:]} if nt == 0 else None)
-                                .reshape(-1) for i, fl in enumerate(fils)])
+            f = np.concatenate([F.internal_forces(fl, p, {4: c["pert"][i, :3], 12: c["pert"][i, 3:]}
+                                                  if nt == 0 else None).reshape(-1) for i, fl in enumerate(fils)])
             v = vel(cat("x"), f)
-            e, big = BO.compute_errors(state(cat("x"), cat("xm1"), cat("xm2"), cat("u"), cat("um1"), v), 1, nfil,
-                                       nseg, nt >= 1, dt, dl, tol)
+            e, big = BO.compute_errors(residual_state(cat("x"), cat("xm1"), cat("xm2"), cat("u"), cat("um1"), v),
+                                       1, nfil, nseg, nt >= 1, dt, dl, tol)
             return f, e, big
 
         f, e, big = evaluate()
@@ -246,13 +217,94 @@ def test_elastic_implicit_step_matches_oracle_loop():
                 F.update(fl, du[6 * nseg * i:6 * nseg * (i + 1)], p)
             f, e, big = evaluate()
             ref.end_of_iter(e, big)
-        ref_iters.append(ref.iter)
-        ref_x.append(cat("x"))
-    print("broyden iterations gpu", gpu_iters, "oracle", ref_iters,
-          "max |dx|", [float(np.abs(a - b).max()) for a, b in zip(gpu_x, ref_x)])
-    for s in range(steps):
-        assert ref_iters[s] >= 10 and abs(gpu_iters[s] - ref_iters[s]) <= max(2, ref_iters[s] // 10)
-        assert np.abs(gpu_x[s] - ref_x[s]).max() <= 1e-6
+        iters.append(ref.iter)
+        xs.append(cat("x"))
+    return c, iters, xs, x_start
+
+
+def check_against_oracle(oracle_run, iters, xs):
+    _, ref_iters, ref_x, x_start = oracle_run
+    print("broyden iterations", iters, "oracle", ref_iters,
+          "max |dx|", [float(np.abs(a - b).max()) for a, b in zip(xs, ref_x)], flush=True)
+    for s in range(STEPS):
+        assert ref_iters[s] >= 10 and abs(iters[s] - ref_iters[s]) <= max(2, ref_iters[s] // 10)
+        assert np.abs(xs[s] - ref_x[s]).max() <= 1e-6
         # the filaments moved (the end force and the kick act)
-        assert np.abs(ref_x[s] - (ref_x[s - 1] if s else np.concatenate(
-            [F.Filament.at_rest(body.fil_base[i], qs[i], nseg, dl).x.reshape(-1) for i in range(nfil)]))).max() > 1e-3
+        assert np.abs(ref_x[s] - (ref_x[s - 1] if s else x_start)).max() > 1e-3
+
+
+def test_elastic_implicit_step_matches_oracle_loop(oracle_run):
+    print("case elastic implicit step 16x20 + 2000 blobs, host loop", flush=True)
+    c = oracle_run[0]
+    body, nfil, nseg, nb, dl, dt, tol = c["body"], c["nfil"], c["nseg"], c["nb"], c["dl"], c["dt"], c["tol"]
+    ns = nfil * nseg
+    zero3, zero6 = np.zeros(3), np.zeros(6)
+    model = FM.FilamentModel(1, nfil, nseg, dl=dl, kb=c["kb"], kt=c["kb"], dt=dt, end_force=c["efm"])
+    ev = ElasticVelocities(1, ns, nb, mode=SOLVE_FP64, restart=30, max_cycles=200, tol=1e-10)
+    eng = BroydenSolver(model.block_sizes())
+    iters, xs = [], []
+    try:
+        model.set_state(body.fil_base.reshape(-1), c["qs"].reshape(-1))
+        model.set_perturbation(4, 12, c["pert"].reshape(-1))
+        n6 = 6 * nseg
+        blocks = np.zeros(nfil * n6 * n6 + 36)
+        blocks[nfil * n6 * n6:] = np.eye(6).reshape(-1)
+        for nt in range(STEPS):
+            model.initial_guess(nt)
+
+            def evaluate():
+                f = model.forces(nt)
+                v, _, _ = ev.compute(model.read(FM.X), f, body.x_blobs, zero3, zero6)
+                big = eng.compute_errors(residual_state(*[model.read(w) for w in (FM.X, FM.XM1, FM.XM2, FM.U,
+                                                                                   FM.UM1)], v),
+                                         1, num_swim=1, num_fil=nfil, num_seg=nseg, bdf2=nt >= 1, dt=dt, dl=dl,
+                                         tol=tol)
+                return f, big
+
+            f, big = evaluate()
+            model.jacobian(nt, f, blocks)
+            eng.set_jacobian(blocks)
+            eng.start(eng.read_error(1))
+            while big and eng.iter < 400:
+                model.update(eng.find_update())
+                _, big = evaluate()
+                eng.end_of_iter(eng.read_error(1), big)
+            iters.append(eng.iter)
+            xs.append(model.read(FM.X))
+    finally:
+        model.close()
+        ev.close()
+        eng.close()
+    check_against_oracle(oracle_run, iters, xs)
+
+
+def test_device_resident_elastic_step(oracle_run):
+    print("case elastic implicit step 16x20 + 2000 blobs, tfs_elastic_step", flush=True)
+    c = oracle_run[0]
+    body, nfil, nseg, nb, dl, dt, tol = c["body"], c["nfil"], c["nseg"], c["nb"], c["dl"], c["dt"], c["tol"]
+    ns = nfil * nseg
+    dev = lambda a: torch.tensor(np.asarray(a, dtype=np.float64), device="cuda")
+    model = FM.FilamentModel(1, nfil, nseg, dl=dl, kb=c["kb"], kt=c["kb"], dt=dt, end_force=c["efm"],
+                             memory=L.MEMORY_DEVICE)
+    ev = ElasticVelocities(1, ns, nb, mode=SOLVE_FP64, restart=30, max_cycles=200, tol=1e-10,
+                           memory=L.MEMORY_DEVICE)
+    eng = BroydenSolver(model.block_sizes(), memory=L.MEMORY_DEVICE)
+    x_blobs, x_bodies, v_bodies, f_blobs = dev(body.x_blobs), dev(np.zeros(3)), dev(np.zeros(6)), dev(np.zeros(3 * nb))
+    step = None
+    iters, xs = [], []
+    try:
+        model.set_state(dev(body.fil_base.reshape(-1)), dev(c["qs"].reshape(-1)))
+        model.set_perturbation(4, 12, dev(c["pert"].reshape(-1)))
+        step = FM.ElasticStep(model, ev, eng, x_blobs, x_bodies, v_bodies, f_blobs, tol=tol, max_iter=400)
+        for nt in range(STEPS):
+            st = step.step(nt)
+            assert st["converged"] == 1 and st["velocity_solves"] == st["broyden_iterations"] + 1
+            iters.append(st["broyden_iterations"])
+            xs.append(model.read(FM.X))
+    finally:
+        if step is not None:
+            step.close()
+        model.close()
+        ev.close()
+        eng.close()
+    check_against_oracle(oracle_run, iters, xs)

@@ -14,7 +14,9 @@
 #include <cmath>
 #include <vector>
 
+#include "../../include/tfs_broyden.h"
 #include "../../include/tfs_filament.h"
+#include "../../include/tfs_solver.h"
 #include "tfs_api_internal.h"
 
 using namespace tfs;
@@ -767,6 +769,129 @@ int tfs_filament_read(tfs_filament* h, int32_t which, double* out) {
     hipError_t e = hipMemcpyAsync(out, p, 8 * len, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     return check(e, "tfs_filament_read");
+}
+
+// ---- elastic implicit timestep --------------------------------------------
+
+struct tfs_elastic {
+    tfs_elastic_desc desc{};
+    double* arena = nullptr;
+    double *f_segs = nullptr, *v_segs = nullptr, *du = nullptr, *blocks = nullptr, *zeros = nullptr;
+    int64_t n = 0;
+};
+
+int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
+    g_last_error.clear();
+    if (!desc || !out || !desc->filaments || !desc->velocities || !desc->broyden || !desc->x_bodies ||
+        !desc->v_bodies || !desc->f_blobs || !desc->x_blobs)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_create: null handle or array");
+    *out = nullptr;
+    const tfs_filament* fh = desc->filaments;
+    if (fh->desc.memory != TFS_MEMORY_DEVICE)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_create: the filament handle must use device memory");
+    if (!(desc->tol > 0) || desc->max_iter < 1)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_create: tol > 0 and max_iter >= 1");
+    tfs_elastic* e = new tfs_elastic();
+    e->desc = *desc;
+    const int64_t ns = num_segs(fh), nswim = fh->desc.num_swim;
+    e->n = update_len(fh);
+    const int64_t nb = blocks_len(fh);
+    std::vector<std::pair<double**, int64_t>> parts = {{&e->f_segs, 6 * ns}, {&e->v_segs, 6 * ns},
+                                                       {&e->du, e->n},       {&e->blocks, nb},
+                                                       {&e->zeros, 6 * nswim}};
+    int64_t total = 0;
+    for (auto& p : parts) total += (p.second + 31) / 32 * 32;
+    if (hipMalloc(&e->arena, total * 8) != hipSuccess) {
+        delete e;
+        return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_elastic_create: device memory");
+    }
+    double* p = e->arena;
+    for (auto& q : parts) {
+        *q.first = p;
+        p += (q.second + 31) / 32 * 32;
+    }
+    // J0's body blocks: the identity (the body rows' residuals vanish for a
+    // body held still); the filament blocks are rewritten every step.
+    std::vector<double> host(nb, 0.0);
+    const int64_t n6 = 6 * fh->P.nseg, per = (int64_t)fh->desc.fil_per_swim * n6 * n6 + 36;
+    for (int64_t s = 0; s < nswim; ++s)
+        for (int k = 0; k < 6; ++k) host[s * per + fh->desc.fil_per_swim * n6 * n6 + 7 * k] = 1.0;
+    hipError_t err = hipMemcpy(e->blocks, host.data(), 8 * nb, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemset(e->zeros, 0, 48 * nswim);
+    if (err != hipSuccess) {
+        tfs_elastic_destroy(e);
+        return hip_fail(err, "tfs_elastic_create");
+    }
+    *out = e;
+    return TFS_OK;
+}
+
+int tfs_elastic_destroy(tfs_elastic* e) {
+    if (!e) return TFS_OK;
+    if (e->arena) (void)hipFree(e->arena);
+    delete e;
+    return TFS_OK;
+}
+
+int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
+    if (!e || nt < 0) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_step: null handle or nt < 0");
+    const tfs_elastic_desc& D = e->desc;
+    tfs_filament* fh = D.filaments;
+    tfs_elastic_stats local{};
+    tfs_elastic_stats& S = stats ? *stats : local;
+    S = tfs_elastic_stats{};
+    tfs_broyden_state st{};
+    st.num_swim = fh->desc.num_swim;
+    st.num_fil = fh->desc.fil_per_swim;
+    st.num_seg = fh->P.nseg;
+    st.bdf2 = nt >= 1 ? 1 : 0;  // NUM_EULER_STEPS = 1
+    st.dt = fh->P.dt;
+    st.dl = fh->P.dl;
+    st.tol = D.tol;
+    st.seg_x = fh->S.x;
+    st.seg_xm1 = fh->S.xm1;
+    st.seg_xm2 = fh->S.xm2;
+    st.seg_u = fh->S.u;
+    st.seg_um1 = fh->S.um1;
+    st.v_segs = e->v_segs;
+    st.body_x = st.body_xm1 = st.body_xm2 = st.body_u = st.body_um1 = e->zeros;
+    st.v_bodies = e->zeros;
+    // forces -> velocities -> residual in `slot`
+    auto evaluate = [&](int slot, int32_t* too_large) -> int {
+        int rc = tfs_filament_forces(fh, nt, e->f_segs);
+        if (!rc) rc = tfs_filament_sync(fh);
+        tfs_solve_stats vs{};
+        if (!rc) rc = tfs_velocities_compute(D.velocities, fh->S.x, e->f_segs, D.x_blobs, D.x_bodies, D.v_bodies,
+                                             e->v_segs, D.f_blobs, &vs);
+        S.gmres_iterations += vs.iterations;
+        S.velocity_solves++;
+        if (!rc) rc = tfs_broyden_compute_errors(D.broyden, &st, slot, too_large);
+        return rc;
+    };
+    int rc = tfs_filament_initial_guess(fh, nt);
+    int32_t big = 0;
+    if (!rc) rc = evaluate(0, &big);
+    if (!rc && big) {
+        rc = tfs_filament_jacobian(fh, nt, e->f_segs, e->blocks);
+        if (!rc) rc = tfs_filament_sync(fh);
+        if (!rc) rc = tfs_broyden_set_jacobian(D.broyden, e->blocks);
+    }
+    if (!rc) rc = tfs_broyden_start(D.broyden, nullptr);
+    int32_t iter = 0;
+    while (!rc && big && iter < D.max_iter) {
+        rc = tfs_broyden_find_update(D.broyden, e->du);
+        if (!rc) rc = tfs_filament_update(fh, e->du);
+        if (!rc) rc = evaluate(1, &big);
+        if (!rc) rc = tfs_broyden_end_of_iter(D.broyden, nullptr, big);
+        if (!rc) rc = tfs_broyden_status(D.broyden, &iter, nullptr);
+    }
+    double norm = 0.0;
+    if (!rc) rc = tfs_broyden_status(D.broyden, nullptr, &norm);
+    S.broyden_iterations = iter;
+    S.error_norm = norm;
+    S.converged = (!rc && !big) ? 1 : 0;
+    if (rc) return rc;
+    return big ? fail(TFS_ERR_NOT_CONVERGED, "tfs_elastic_step: Broyden iterations exhausted") : TFS_OK;
 }
 
 }  // extern "C"

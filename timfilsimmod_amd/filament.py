@@ -38,6 +38,9 @@ L.extra_signatures({
     "tfs_filament_jacobian": (ctypes.c_int, [_vp, _i32, _vp, _vp]),
     "tfs_filament_update": (ctypes.c_int, [_vp, _vp]),
     "tfs_filament_sync": (ctypes.c_int, [_vp]),
+    "tfs_elastic_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_vp)]),
+    "tfs_elastic_destroy": (ctypes.c_int, [_vp]),
+    "tfs_elastic_step": (ctypes.c_int, [_vp, _i32, ctypes.c_void_p]),
     "tfs_filament_device_array": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_vp)]),
     "tfs_filament_read": (ctypes.c_int, [_vp, _i32, _vp]),
 })
@@ -123,3 +126,51 @@ class FilamentModel:
         out = np.zeros(size)
         L.check(L.lib().tfs_filament_read(self.h, which, L.ptr(out)))
         return out
+
+
+class ElasticDesc(ctypes.Structure):
+    _fields_ = [("filaments", ctypes.c_void_p), ("velocities", ctypes.c_void_p), ("broyden", ctypes.c_void_p),
+                ("x_blobs", ctypes.c_void_p), ("x_bodies", ctypes.c_void_p), ("v_bodies", ctypes.c_void_p),
+                ("f_blobs", ctypes.c_void_p), ("tol", ctypes.c_double), ("max_iter", ctypes.c_int32)]
+
+
+class ElasticStats(ctypes.Structure):
+    _fields_ = [("broyden_iterations", ctypes.c_int32), ("gmres_iterations", ctypes.c_int32),
+                ("converged", ctypes.c_int32), ("velocity_solves", ctypes.c_int32), ("error_norm", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ElasticStep:
+    """One elastic implicit timestep on the device (tfs_elastic_step): the
+    time-loop body of cilia_sim_main.cu:353-502 for bodies held still.  The
+    handles must use device memory; x_blobs, x_bodies, v_bodies, f_blobs are
+    device tensors (f_blobs is the blob-force warm start and result)."""
+
+    def __init__(self, filaments, velocities, broyden, x_blobs, x_bodies, v_bodies, f_blobs, *, tol=1e-4,
+                 max_iter=400):
+        self._keep = (filaments, velocities, broyden, x_blobs, x_bodies, v_bodies, f_blobs)
+        d = ElasticDesc(filaments.h.value, velocities.h.value, broyden.h.value, L.ptr(x_blobs), L.ptr(x_bodies),
+                        L.ptr(v_bodies), L.ptr(f_blobs), float(tol), int(max_iter))
+        h = ctypes.c_void_p()
+        L.check(L.lib().tfs_elastic_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            L.lib().tfs_elastic_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, nt, *, allow_unconverged=False):
+        st = ElasticStats()
+        rc = L.lib().tfs_elastic_step(self.h, int(nt), ctypes.byref(st))
+        if not (allow_unconverged and rc == L.TFS_ERR_NOT_CONVERGED):
+            L.check(rc)
+        return st.as_dict()

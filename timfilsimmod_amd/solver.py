@@ -183,13 +183,13 @@ class ElasticVelocities:
 
     def __init__(self, num_swim, segs_per_swim, blobs_per_swim, *, mode=SOLVE_FP64, seg_radius=1.0,
                  blob_radius=1.0, mu=1.0, restart=30, max_cycles=50, tol=1e-10, inner_tol=1e-3,
-                 max_refinements=10, stream=None):
+                 max_refinements=10, stream=None, memory=L.MEMORY_HOST):
         d = VelocitiesDesc()
         d.num_swim, d.segs_per_swim, d.blobs_per_swim, d.mode = num_swim, segs_per_swim, blobs_per_swim, mode
         d.seg_radius, d.blob_radius, d.mu = seg_radius, blob_radius, mu
         d.restart, d.max_cycles, d.tol = restart, max_cycles, tol
         d.inner_tol, d.max_refinements = inner_tol, max_refinements
-        d.memory, d.stream = L.MEMORY_HOST, stream
+        d.memory, d.stream = memory, stream
         self.desc = d
         self.ns = num_swim * segs_per_swim
         self.nb = num_swim * blobs_per_swim
@@ -211,7 +211,11 @@ class ElasticVelocities:
 
     def compute(self, x_segs, f_segs, x_blobs, x_bodies, v_bodies, *, allow_unconverged=False):
         """Returns (v_segs [6 per segment], f_blobs [3 per blob], solve stats);
-        the blob forces warm-start the next call."""
+        the blob forces warm-start the next call.  Host memory only (a
+        device-memory handle is driven through the C-ABI, e.g. by
+        filament.ElasticStep)."""
+        if self.desc.memory != L.MEMORY_HOST:
+            raise ValueError("compute() takes host arrays; this handle was made for device memory")
         a = [np.ascontiguousarray(v, dtype=np.float64) for v in (x_segs, f_segs, x_blobs, x_bodies, v_bodies)]
         v_segs = np.zeros(6 * self.ns)
         st = SolveStats()
