@@ -216,6 +216,55 @@ def ensemble_leg(torch, stream, steps: int = 5):
     return out
 
 
+def elastic_leg(torch, steps: int = 3):
+    """The small elastic configuration on the device: one rigid sphere of
+    radius 20a with 2,000 blobs and 16 filaments x 20 segments of the
+    reference's Cosserat model (clamped, follower end force of dimensionless
+    strength 220, a first-step kick), each timestep through tfs_elastic_step:
+    blob forces by GMRES(30) to 1e-10, Broyden to 1e-8."""
+    from scipy.spatial.transform import Rotation
+
+    from timfilsimmod_amd import _lib as L
+    from timfilsimmod_amd import filament as FM
+    from timfilsimmod_amd import geometry as G
+    from timfilsimmod_amd.broyden import BroydenSolver
+    from timfilsimmod_amd.solver import PRECOND_BLOCKS, SOLVE_FP64, ElasticVelocities
+    body = G.small_body()
+    nfil, nseg, nb = body.num_fil, body.num_seg, body.num_blobs
+    dl, kb, dt = 2.2, 1800.0, 0.1
+    efm = 220.0 * kb / (dl * dl * nseg * nseg)
+    ex = np.array([1.0, 0.0, 0.0])
+    q0 = np.array([np.roll(Rotation.align_vectors([n], [ex])[0].as_quat(), 1) for n in body.fil_normal])
+    qs = np.repeat(q0[:, None, :], nseg, axis=1)
+    pert = np.random.default_rng(0).standard_normal((nfil, 6)) * 0.1 * efm
+    dev = lambda a: torch.tensor(np.asarray(a, dtype=np.float64), device="cuda")
+    model = FM.FilamentModel(1, nfil, nseg, dl=dl, kb=kb, kt=kb, dt=dt, end_force=efm, memory=L.MEMORY_DEVICE)
+    ev = ElasticVelocities(1, nfil * nseg, nb, mode=SOLVE_FP64, restart=30, max_cycles=200, tol=1e-10,
+                           memory=L.MEMORY_DEVICE)
+    ev.set_preconditioner(PRECOND_BLOCKS)
+    eng = BroydenSolver(model.block_sizes(), memory=L.MEMORY_DEVICE)
+    keep = [dev(body.x_blobs), dev(np.zeros(3)), dev(np.zeros(6)), dev(np.zeros(3 * nb))]
+    model.set_state(dev(body.fil_base.reshape(-1)), dev(qs.reshape(-1)))
+    model.set_perturbation(4, 12, dev(pert.reshape(-1)))
+    step = FM.ElasticStep(model, ev, eng, *keep, tol=1e-8, max_iter=400)
+    per = []
+    for nt in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = step.step(nt)
+        torch.cuda.synchronize()
+        per.append({"nt": nt, "ms": (time.perf_counter() - t0) * 1e3, "broyden_iterations": st["broyden_iterations"],
+                    "gmres_iterations": st["gmres_iterations"]})
+    step.close()
+    model.close()
+    ev.close()
+    eng.close()
+    return {"workload": "16 filaments x 20 segments (Cosserat, clamped, end force 220) + 2,000 blobs, "
+                        "dt 0.1, Broyden to 1e-8, blob forces by GMRES(30) to 1e-10 with 16-blob patch "
+                        "blocks", "steps": per,
+            "ms_per_step_mean": float(np.mean([p["ms"] for p in per]))}
+
+
 def copy_leg(torch, nbytes: int = 256 << 20, reps: int = 3):
     """A plain device-to-device copy of `nbytes` (torch's copy kernel), `reps`
     times: the reference point that scales the FETCH_SIZE / WRITE_SIZE
@@ -277,12 +326,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--matvec-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="timestep,matvec,flow,ensemble",
+    ap.add_argument("--legs", default="timestep,matvec,flow,ensemble,elastic",
                     help="comma list of timestep, matvec, flow, ensemble, copy (copy: a 256 MiB device "
                          "copy for calibrating HBM byte counters; not part of the default run)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
-    unknown = legs - {"timestep", "matvec", "flow", "ensemble", "copy"}
+    unknown = legs - {"timestep", "matvec", "flow", "ensemble", "copy", "elastic"}
     if unknown:
         print(f"bench.py: unknown legs {sorted(unknown)}", file=sys.stderr)
         sys.exit(2)
@@ -397,6 +446,10 @@ def main():
     if "ensemble" in legs:
         print("bench: ensemble leg", file=sys.stderr, flush=True)
         ens = ensemble_leg(torch, stream)
+    elastic = None
+    if "elastic" in legs:
+        print("bench: elastic-step leg", file=sys.stderr, flush=True)
+        elastic = elastic_leg(torch)
 
     cpu = None
     if not args.no_cpu_baseline and fp64 is not None:
@@ -438,6 +491,7 @@ def main():
         "matvec": matvec,
         "flow_field": flow,
         "ensemble": ens,
+        "elastic_step": elastic,
         "roofline": {"bound": "valu", "achieved": tf32, "peak": PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
                      "frac": tf32 / PEAK_FP32_VALU_TFLOPS, "traffic": measured_traffic(),
                      "kernel": "rpy_pairs_kernel<float,3> (timestep matvec, 17.11 GFLOP per launch)",

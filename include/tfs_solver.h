@@ -203,6 +203,10 @@ int tfs_velocities_destroy(tfs_velocities *v);
  * blob) holds the warm start on entry and the solved blob forces on return.
  * Complete on return.  TFS_ERR_NOT_CONVERGED if the blob solve misses its
  * tolerance (outputs still written). */
+/* Preconditioner of the blob solve: TFS_PRECOND_REFERENCE (6 pi mu a, the
+ * reference's, default) or TFS_PRECOND_BLOCKS (the blobs' mobility inverted
+ * within 16-blob patches; rebuilt only when the blob positions change). */
+int tfs_velocities_set_preconditioner(tfs_velocities *v, int32_t kind);
 int tfs_velocities_compute(tfs_velocities *v, const double *x_segs, const double *f_segs,
                            const double *x_blobs, const double *x_bodies, const double *v_bodies,
                            double *v_segs, double *f_blobs, tfs_solve_stats *stats);

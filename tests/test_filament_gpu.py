@@ -23,7 +23,7 @@ from timfilsimmod_amd import _lib as L
 from timfilsimmod_amd import filament as FM
 from timfilsimmod_amd import geometry as G
 from timfilsimmod_amd.broyden import BroydenSolver
-from timfilsimmod_amd.solver import SOLVE_FP64, ElasticVelocities
+from timfilsimmod_amd.solver import PRECOND_BLOCKS, PRECOND_REFERENCE, SOLVE_FP64, ElasticVelocities
 
 pytestmark = pytest.mark.gpu
 
@@ -278,8 +278,9 @@ def test_elastic_implicit_step_matches_oracle_loop(oracle_run):
     check_against_oracle(oracle_run, iters, xs)
 
 
-def test_device_resident_elastic_step(oracle_run):
-    print("case elastic implicit step 16x20 + 2000 blobs, tfs_elastic_step", flush=True)
+@pytest.mark.parametrize("precond", [PRECOND_REFERENCE, PRECOND_BLOCKS])
+def test_device_resident_elastic_step(oracle_run, precond):
+    print(f"case elastic implicit step 16x20 + 2000 blobs, tfs_elastic_step, precond {precond}", flush=True)
     c = oracle_run[0]
     body, nfil, nseg, nb, dl, dt, tol = c["body"], c["nfil"], c["nseg"], c["nb"], c["dl"], c["dt"], c["tol"]
     ns = nfil * nseg
@@ -288,6 +289,7 @@ def test_device_resident_elastic_step(oracle_run):
                              memory=L.MEMORY_DEVICE)
     ev = ElasticVelocities(1, ns, nb, mode=SOLVE_FP64, restart=30, max_cycles=200, tol=1e-10,
                            memory=L.MEMORY_DEVICE)
+    ev.set_preconditioner(precond)
     eng = BroydenSolver(model.block_sizes(), memory=L.MEMORY_DEVICE)
     x_blobs, x_bodies, v_bodies, f_blobs = dev(body.x_blobs), dev(np.zeros(3)), dev(np.zeros(6)), dev(np.zeros(3 * nb))
     step = None
@@ -298,6 +300,7 @@ def test_device_resident_elastic_step(oracle_run):
         step = FM.ElasticStep(model, ev, eng, x_blobs, x_bodies, v_bodies, f_blobs, tol=tol, max_iter=400)
         for nt in range(STEPS):
             st = step.step(nt)
+            print("step", nt, st, flush=True)
             assert st["converged"] == 1 and st["velocity_solves"] == st["broyden_iterations"] + 1
             iters.append(st["broyden_iterations"])
             xs.append(model.read(FM.X))

@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
+#include <vector>
 
 #include "../../include/tfs_solver.h"
 #include "tfs_api_internal.h"
@@ -42,6 +44,13 @@ __global__ __launch_bounds__(256) void blob_rhs_kernel(const double* __restrict_
     rhs[3 * m + 2] = (have_segs ? -v_blobs[3 * m + 2] : 0.0) + k2;
 }
 
+// flag = 1 if a[i] != b[i] for some i (every writer stores the same value).
+__global__ __launch_bounds__(256) void differs_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                                                     int64_t n, int32_t* __restrict__ flag) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n && a[i] != b[i]) *flag = 1;
+}
+
 int check(hipError_t e, const char* where) { return e == hipSuccess ? TFS_OK : hip_fail(e, where); }
 
 }  // namespace
@@ -55,6 +64,12 @@ struct tfs_velocities {
     int64_t ns = 0, nb = 0, nswim = 0;
     double* d_rhs = nullptr;      // 3*nb
     double* d_bodies = nullptr;   // 3*nswim positions then 6*nswim velocities
+    // The blob solve's geometry (and so its preconditioner blocks) is renewed
+    // only when the blob positions change: flag in device memory, its host
+    // copy in pinned memory.
+    int32_t* d_flag = nullptr;
+    int32_t* h_flag = nullptr;
+    bool have_blobs = false;
 };
 
 extern "C" {
@@ -121,6 +136,8 @@ int tfs_velocities_create(const tfs_velocities_desc* desc, tfs_velocities** out)
     if (!rc) rc = tfs_solver_create(&sd, &v->blobs);
     if (!rc) rc = check(hipMalloc(&v->d_rhs, sizeof(double) * 3 * nb), "tfs_velocities_create: rhs");
     if (!rc) rc = check(hipMalloc(&v->d_bodies, sizeof(double) * 9 * v->nswim), "tfs_velocities_create: bodies");
+    if (!rc) rc = check(hipMalloc(&v->d_flag, sizeof(int32_t)), "tfs_velocities_create: flag");
+    if (!rc) rc = check(hipHostMalloc(&v->h_flag, sizeof(int32_t)), "tfs_velocities_create: flag");
     if (rc) {
         const std::string msg = g_last_error;
         tfs_velocities_destroy(v);
@@ -138,6 +155,8 @@ int tfs_velocities_destroy(tfs_velocities* v) {
     if (v->rpy) tfs_rpy_destroy(v->rpy);
     if (v->d_rhs) (void)hipFree(v->d_rhs);
     if (v->d_bodies) (void)hipFree(v->d_bodies);
+    if (v->d_flag) (void)hipFree(v->d_flag);
+    if (v->h_flag) (void)hipHostFree(v->h_flag);
     if (v->own_stream) (void)hipStreamDestroy(v->stream);
     delete v;
     return TFS_OK;
@@ -153,8 +172,30 @@ int tfs_velocities_compute(tfs_velocities* v, const double* x_segs, const double
     tfs_rpy* h = v->rpy;
     const int64_t ns = v->ns, nb = v->nb;
     hipStream_t st = v->stream;
-    // read_positions_and_forces (mobility_solver.cpp:176-236)
-    int rc = check(hipMemcpyAsync(h->x_blobs, x_blobs, 24 * nb, kin, st), "blob positions");
+    // read_positions_and_forces (mobility_solver.cpp:176-236).  Blob positions
+    // that did not change keep the blob solve's geometry.
+    int rc = TFS_OK;
+    bool blobs_moved = !v->have_blobs;
+    if (v->have_blobs) {
+        if (v->desc.memory == TFS_MEMORY_HOST) {
+            std::vector<double> old(3 * nb);
+            rc = check(hipMemcpyAsync(old.data(), h->x_blobs, 24 * nb, hipMemcpyDeviceToHost, st), "blob positions");
+            if (!rc) rc = check(hipStreamSynchronize(st), "blob positions");
+            blobs_moved = !rc && std::memcmp(old.data(), x_blobs, 24 * nb) != 0;
+        } else {
+            *v->h_flag = 0;
+            rc = check(hipMemsetAsync(v->d_flag, 0, sizeof(int32_t), st), "blob flag");
+            if (!rc) {
+                hipLaunchKernelGGL(differs_kernel, dim3((unsigned)((3 * nb + 255) / 256)), dim3(256), 0, st, x_blobs,
+                                   h->x_blobs, 3 * nb, v->d_flag);
+                rc = check(hipGetLastError(), "blob flag");
+            }
+            if (!rc) rc = check(hipMemcpyAsync(v->h_flag, v->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, st), "blob flag");
+            if (!rc) rc = check(hipStreamSynchronize(st), "blob flag");
+            blobs_moved = *v->h_flag != 0;
+        }
+    }
+    if (!rc) rc = check(hipMemcpyAsync(h->x_blobs, x_blobs, 24 * nb, kin, st), "blob positions");
     if (!rc && ns) rc = check(hipMemcpyAsync(h->x_segs, x_segs, 24 * ns, kin, st), "segment positions");
     if (!rc && ns) rc = check(hipMemcpyAsync(h->f_segs, f_segs, 48 * ns, kin, st), "segment forces");
     if (!rc) rc = check(hipMemcpyAsync(h->f_blobs, f_blobs, 24 * nb, kin, st), "blob warm start");
@@ -171,7 +212,8 @@ int tfs_velocities_compute(tfs_velocities* v, const double* x_segs, const double
         rc = check(hipGetLastError(), "blob rhs");
     }
     // GMRES on M_bb for the blob forces, warm-started from the given ones
-    if (!rc) rc = tfs_solver_set_geometry(v->blobs, nullptr, h->x_blobs, nullptr, nullptr, TFS_MEMORY_DEVICE);
+    if (!rc && blobs_moved) rc = tfs_solver_set_geometry(v->blobs, nullptr, h->x_blobs, nullptr, nullptr, TFS_MEMORY_DEVICE);
+    if (!rc) v->have_blobs = true;
     int solve_rc = TFS_OK;
     if (!rc) {
         solve_rc = tfs_solver_solve(v->blobs, v->d_rhs, h->f_blobs, TFS_MEMORY_DEVICE, stats);
@@ -189,6 +231,11 @@ int tfs_velocities_compute(tfs_velocities* v, const double* x_segs, const double
         return solve_rc;
     }
     return TFS_OK;
+}
+
+int tfs_velocities_set_preconditioner(tfs_velocities* v, int32_t kind) {
+    if (!v) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_set_preconditioner: null handle");
+    return tfs_solver_set_preconditioner(v->blobs, kind);
 }
 
 }  // extern "C"

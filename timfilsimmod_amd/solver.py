@@ -78,6 +78,7 @@ L.extra_signatures({
     "tfs_beat_set_mode": (ctypes.c_int, [_dp, _i32]),
     "tfs_beat_set_warm_start": (ctypes.c_int, [_dp, _i32]),
     "tfs_beat_restore": (ctypes.c_int, [_dp, _dp, _dp, _dp]),
+    "tfs_velocities_set_preconditioner": (ctypes.c_int, [_dp, _i32]),
     "tfs_beat_velocities": (ctypes.c_int, [_dp, _dp, _dp]),
     "tfs_solver_set_preconditioner": (ctypes.c_int, [_dp, _i32]),
 })
@@ -208,6 +209,10 @@ class ElasticVelocities:
             self.close()
         except Exception:
             pass
+
+    def set_preconditioner(self, kind: int):
+        """PRECOND_REFERENCE (default) or PRECOND_BLOCKS for the blob solve."""
+        L.check(L.lib().tfs_velocities_set_preconditioner(self.h, kind))
 
     def compute(self, x_segs, f_segs, x_blobs, x_bodies, v_bodies, *, allow_unconverged=False):
         """Returns (v_segs [6 per segment], f_blobs [3 per blob], solve stats);
