@@ -53,6 +53,12 @@ typedef struct tfs_solver_desc {
     double inner_tol;        /* mixed: reduction asked of each inner solve           */
     int32_t max_refinements; /* mixed: iterative-refinement steps                    */
     void *stream;            /* hipStream_t or NULL                                   */
+    int32_t num_swim;        /* NSWIM (0 is taken as 1): filaments, segments and
+                                blobs split evenly between the swimmers             */
+    int32_t free_bodies;     /* 1: each swimmer's velocity V (6) is an unknown, with
+                                force- and torque-balance rows (not
+                                PRESCRIBED_BODY_VELOCITIES); body centres from
+                                tfs_solver_set_bodies                                */
 } tfs_solver_desc;
 
 typedef struct tfs_solve_stats {
@@ -80,10 +86,23 @@ int tfs_solver_size(tfs_solver *s, int64_t *n);
 int tfs_solver_set_geometry(tfs_solver *s, const double *x_segs, const double *x_blobs,
                             const double *vel_dir_phase, const double *vel_dir_angle,
                             int32_t memory);
-/* out = A in (system_matrix_mult_new); precision is TFS_PRECISION_*. */
+/* Body centres [3 per swimmer] of a free-body system (the x_body of
+ * mobility_solver.cpp:1858-1870 and the blob displacements R b_ref = x_blob -
+ * x_body).  Needed before the first product when free_bodies is set. */
+int tfs_solver_set_bodies(tfs_solver *s, const double *x_bodies, int32_t memory);
+/* out = A in (system_matrix_mult_new); precision is TFS_PRECISION_*.  With
+ * free bodies the vector is [lambda_s | lambda_b | V | phi_dot | theta_dot]
+ * (mobility_solver.cpp:107-140) and
+ *   particle rows  M lambda - (V + Omega x (x - x_body)) - K_phi phi_dot - K_theta theta_dot,
+ *   body rows      -sum lambda,  -sum (x - x_body) x lambda   (:1850-1908). */
 int tfs_solver_system_mult(tfs_solver *s, const double *in, double *out, int32_t precision,
                            int32_t memory);
-/* out = P^{-1} in (apply_preconditioner_new). */
+/* out = P^{-1} in (apply_preconditioner_new).  With free bodies, the exact
+ * inverse of [D -K; -K^T 0] with K = [K_body K_phi K_theta]: the
+ * preconditioner without body columns, P0, then a rank-6-per-swimmer
+ * correction, y = -(E^T P0^-1 E)^-1 (r_body + E^T P0^-1 r), with E the body
+ * columns; the 6 x 6 matrices play the part of the reference's KTMinvK_inv
+ * (mobility_solver.cpp:1186-1330, swimmer.cpp:1154-1326). */
 int tfs_solver_precondition(tfs_solver *s, const double *in, double *out, int32_t memory);
 /* Right preconditioner of later solves.  TFS_PRECOND_REFERENCE (default):
  * P = [D -K; -K^T 0] with D = I/(6 pi mu a), apply_preconditioner_new

@@ -73,24 +73,54 @@ def k_matrix(kphi, ktheta, nfil, nseg, nblob, dyn_phase=True, dyn_angle=True):
     return np.concatenate(cols, axis=1) if cols else np.zeros((ns3 + 3 * nblob, 0))
 
 
+def k_body_matrix(x_segs, x_blobs, x_bodies, num_swim):
+    """Body columns K_b (mobility_solver.cpp:1850-1908): for swimmer s, the
+    velocity V + Omega x (x - x_body_s) of its particles; segments and blobs
+    split evenly between the swimmers."""
+    xs = np.asarray(x_segs, dtype=float).reshape(-1, 3)
+    xb = np.asarray(x_blobs, dtype=float).reshape(-1, 3)
+    xbod = np.asarray(x_bodies, dtype=float).reshape(num_swim, 3)
+    ns, nb = len(xs), len(xb)
+    K = np.zeros((3 * (ns + nb), 6 * num_swim))
+    for rows, x, per in ((0, xs, ns // num_swim), (3 * ns, xb, nb // num_swim)):
+        for i in range(len(x)):
+            s = i // per
+            r = x[i] - xbod[s]
+            R = rows + 3 * i
+            K[R:R + 3, 6 * s:6 * s + 3] = np.eye(3)
+            # Omega x r = -[r]x Omega
+            K[R:R + 3, 6 * s + 3:6 * s + 6] = np.array([[0.0, r[2], -r[1]], [-r[2], 0.0, r[0]], [r[1], -r[0], 0.0]])
+    return K
+
+
+def _with_bodies(K, x_segs, x_blobs, x_bodies, num_swim):
+    if x_bodies is None:
+        return K
+    return np.concatenate([k_body_matrix(x_segs, x_blobs, x_bodies, num_swim), K], axis=1)
+
+
 def saddle_matrix(x_segs, x_blobs, kphi, ktheta, nfil, nseg, *, dyn_phase=True, dyn_angle=True,
-                  a_seg=1.0, a_blob=1.0, mu=1.0):
-    """Dense A = [[M, -K], [-K^T, 0]] with M the <3,3> grand mobility."""
+                  a_seg=1.0, a_blob=1.0, mu=1.0, x_bodies=None, num_swim=1):
+    """Dense A = [[M, -K], [-K^T, 0]] with M the <3,3> grand mobility; with
+    x_bodies, K = [K_b, K_phi, K_theta] (free bodies)."""
     nblob = len(x_blobs) // 3
     M = O.dense_mobility(x_segs, x_blobs, seg_dof=3, a_seg=a_seg, a_blob=a_blob, mu=mu)
-    K = k_matrix(kphi, ktheta, nfil, nseg, nblob, dyn_phase, dyn_angle)
+    K = _with_bodies(k_matrix(kphi, ktheta, nfil, nseg, nblob, dyn_phase, dyn_angle), x_segs, x_blobs, x_bodies,
+                     num_swim)
     g = K.shape[1]
     return np.block([[M, -K], [-K.T, np.zeros((g, g))]])
 
 
 def precond_matrix(kphi, ktheta, nfil, nseg, nblob, *, dyn_phase=True, dyn_angle=True, a_seg=1.0,
-                   a_blob=1.0, mu=1.0):
+                   a_blob=1.0, mu=1.0, x_segs=None, x_blobs=None, x_bodies=None, num_swim=1):
     """P = [[D, -K], [-K^T, 0]], D = diag(1/(6 pi mu a)): the approximation
-    whose exact inverse apply_preconditioner_new applies."""
+    whose exact inverse apply_preconditioner_new applies (with x_bodies, K
+    carries the body columns too)."""
     ns3 = 3 * nfil * nseg
     D = np.diag(np.concatenate([np.full(ns3, 1.0 / (6 * np.pi * mu * a_seg)),
                                 np.full(3 * nblob, 1.0 / (6 * np.pi * mu * a_blob))]))
-    K = k_matrix(kphi, ktheta, nfil, nseg, nblob, dyn_phase, dyn_angle)
+    K = _with_bodies(k_matrix(kphi, ktheta, nfil, nseg, nblob, dyn_phase, dyn_angle), x_segs, x_blobs, x_bodies,
+                     num_swim)
     g = K.shape[1]
     return np.block([[D, -K], [-K.T, np.zeros((g, g))]])
 

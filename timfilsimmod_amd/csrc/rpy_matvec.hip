@@ -675,11 +675,23 @@ __global__ __launch_bounds__(256) void rpy_reduce_kernel(const MatvecTile* __res
     for (int g = 0; g < 3; ++g)
 #pragma unroll
         for (int k = 0; k < 6; ++k) acc[k] += grp_acc[g][k][threadIdx.x];
-    if (seg_t && o.sad.in) {
+    if (seg_t && o.sad.in && (o.sad.dyn_phase || o.sad.dyn_angle)) {
         const int m = t / o.sad.nseg;
         const double pd = o.sad.dyn_phase ? o.sad.in[o.sad.off_phase + m] : 0.0;
         const double td = o.sad.dyn_angle ? o.sad.in[o.sad.off_angle + m] : 0.0;
         for (int k = 0; k < 3; ++k) acc[k] -= pd * o.sad.kphi[3 * (int64_t)t + k] + td * o.sad.kth[3 * (int64_t)t + k];
+    }
+    if (o.sad.free_body) {
+        // -(V + Omega x r) of the target's swimmer (mobility_solver.cpp:1856-1870, 1885-1894)
+        const int sw = seg_t ? t / o.sad.segs_per_swim : t / o.sad.blobs_per_swim;
+        const double* V = o.sad.in + o.sad.off_body + 6 * (int64_t)sw;
+        const CView xv = seg_t ? a.xs : a.xb;
+        const double r0 = xv.p[(int64_t)t * xv.ld_p] - o.sad.x_bodies[3 * sw];
+        const double r1 = xv.p[(int64_t)t * xv.ld_p + xv.ld_c] - o.sad.x_bodies[3 * sw + 1];
+        const double r2 = xv.p[(int64_t)t * xv.ld_p + 2 * xv.ld_c] - o.sad.x_bodies[3 * sw + 2];
+        acc[0] -= V[0] + (V[4] * r2 - V[5] * r1);
+        acc[1] -= V[1] + (V[5] * r0 - V[3] * r2);
+        acc[2] -= V[2] + (V[3] * r1 - V[4] * r0);
     }
     const int slots = seg_t ? o.seg_out_slots : 3;
     const View out = seg_t ? o.vs : o.vb;

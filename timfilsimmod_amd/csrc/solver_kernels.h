@@ -9,11 +9,18 @@
 namespace tfs {
 
 // Offsets into the saddle-point vector
-//   [ lambda_s (3*nfil*nseg) | lambda_b (3*nblob) | phi_dot (nfil) | theta_dot (nfil) ].
+//   [ lambda_s (3*nfil*nseg) | lambda_b (3*nblob) | V (6*nswim, free bodies) |
+//     phi_dot (nfil) | theta_dot (nfil) ]
+// (mobility_solver.cpp:107-140).  Swimmer s owns segments
+// [s*segs_per_swim, ...), blobs [s*blobs_per_swim, ...), filaments
+// [s*fil_per_swim, ...).
 struct SaddleDims {
-    int nfil, nseg, nblob;
-    int dyn_phase, dyn_angle;
-    int64_t off_blob, off_phase, off_angle, n;
+    int nfil = 0, nseg = 0, nblob = 0;
+    int dyn_phase = 0, dyn_angle = 0;
+    int64_t off_blob = 0, off_phase = 0, off_angle = 0, n = 0;
+    int free_body = 0, nswim = 1;
+    int segs_per_swim = 0, blobs_per_swim = 0, fil_per_swim = 0;
+    int64_t off_body = 0;
 };
 
 // Right preconditioner P = [D -K; -K^T 0], applied as its exact inverse.
@@ -118,5 +125,25 @@ hipError_t k_beat(const BeatArgs& a, double* x_segs, double* kphi, double* kth, 
 hipError_t k_rhs(const RhsArgs& a, const double* x_segs, const double* x_blobs, double* rhs, hipStream_t s);
 hipError_t k_phase_update(const double* x, const SaddleDims& d, double dt, double* phase, double* angle,
                           hipStream_t s);
+
+// ---- free bodies ----------------------------------------------------------
+// out[6 s + c] = sign * (sum over the particles i of swimmer s of
+// (vec_i, (x_i - x_body_s) x vec_i))_c: the body rows K_b^T vec (one
+// workgroup per swimmer).
+hipError_t k_body_moments(const double* vec, const SaddleDims& d, const double* x_segs, const double* x_blobs,
+                          const double* x_bodies, double sign, double* out, const int32_t* skip, hipStream_t s);
+// Column c of E = [K_b; 0] for every swimmer: particle rows e_c (c < 3) or
+// e_{c-3} x (x_i - x_body), zero elsewhere.
+hipError_t k_body_column(double* col, int c, const SaddleDims& d, const double* x_segs, const double* x_blobs,
+                         const double* x_bodies, hipStream_t s);
+// The body part of P^{-1} on top of z0 = P0^{-1} r (P0: the preconditioner
+// without body columns): y_s = -Sinv_s (r_body_s + t_s) with t = E^T z0,
+// z body rows = y, other rows z += Z y (Z = P0^{-1} E, 6 columns of ldz).
+// r is in * sc with sc = 1 / sqrt(sum of the G partials p_in) (as the Arnoldi
+// close computes it) or host_sc when p_in is null; v (may be null) gets the
+// body rows of r.
+hipError_t k_body_correct(const double* in, double host_sc, const double* p_in, int G, double* v, double* z,
+                          const double* Z, int64_t ldz, const double* t, const double* sinv, const SaddleDims& d,
+                          const int32_t* skip, hipStream_t s);
 
 }  // namespace tfs

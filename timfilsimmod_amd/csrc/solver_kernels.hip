@@ -1071,6 +1071,120 @@ __global__ void phase_update_kernel(const double* __restrict__ x, SaddleDims d, 
     if (d.dyn_angle) angle[m] += x[d.off_angle + m] * dt;
 }
 
+// ---- free bodies ------------------------------------------------------------
+__device__ __forceinline__ int swimmer_of(const SaddleDims& d, int64_t e) {
+    if (e < d.off_blob) return (int)((e / 3) / d.segs_per_swim);
+    if (e < d.off_body) return (int)(((e - d.off_blob) / 3) / d.blobs_per_swim);
+    if (e < d.off_phase) return (int)((e - d.off_body) / 6);
+    if (e < d.off_angle) return (int)((e - d.off_phase) / d.fil_per_swim);
+    return (int)((e - d.off_angle) / d.fil_per_swim);
+}
+
+__global__ __launch_bounds__(256) void body_moments_kernel(const double* __restrict__ vec, SaddleDims d,
+                                                          const double* __restrict__ xs,
+                                                          const double* __restrict__ xb,
+                                                          const double* __restrict__ xbod, double sign,
+                                                          double* __restrict__ out,
+                                                          const int32_t* __restrict__ skip) {
+    __shared__ double sh[256];
+    if (skip && *skip) return;
+    const int sw = blockIdx.x;
+    const double X0 = xbod[3 * sw], X1 = xbod[3 * sw + 1], X2 = xbod[3 * sw + 2];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const int nparts = d.segs_per_swim + d.blobs_per_swim;
+    for (int q = threadIdx.x; q < nparts; q += 256) {
+        const bool seg = q < d.segs_per_swim;
+        const int64_t i = seg ? (int64_t)sw * d.segs_per_swim + q : (int64_t)sw * d.blobs_per_swim + (q - d.segs_per_swim);
+        const double* x = seg ? xs + 3 * i : xb + 3 * i;
+        const double* f = seg ? vec + 3 * i : vec + d.off_blob + 3 * i;
+        const double r0 = x[0] - X0, r1 = x[1] - X1, r2 = x[2] - X2;
+        const double f0 = f[0], f1 = f[1], f2 = f[2];
+        acc[0] += f0;
+        acc[1] += f1;
+        acc[2] += f2;
+        acc[3] += r1 * f2 - r2 * f1;
+        acc[4] += r2 * f0 - r0 * f2;
+        acc[5] += r0 * f1 - r1 * f0;
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        const double tot = block_sum256(acc[c], sh);
+        if (threadIdx.x == 0) out[6 * sw + c] = sign * tot;
+    }
+}
+
+__global__ __launch_bounds__(256) void body_column_kernel(double* __restrict__ col, int c, SaddleDims d,
+                                                         const double* __restrict__ xs,
+                                                         const double* __restrict__ xb,
+                                                         const double* __restrict__ xbod) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= d.n) return;
+    double val = 0.0;
+    if (e < d.off_body) {
+        const bool seg = e < d.off_blob;
+        const int64_t i = seg ? e / 3 : (e - d.off_blob) / 3;
+        const int k = (int)((seg ? e : e - d.off_blob) % 3);
+        const int sw = swimmer_of(d, e);
+        if (c < 3) {
+            val = k == c ? 1.0 : 0.0;
+        } else {
+            const double* x = seg ? xs + 3 * i : xb + 3 * i;
+            const double r[3] = {x[0] - xbod[3 * sw], x[1] - xbod[3 * sw + 1], x[2] - xbod[3 * sw + 2]};
+            // (e_a x r)_k with a = c - 3: -r[a+2] at k = a+1, r[a+1] at k = a+2
+            const int a = c - 3;
+            const int k1 = (a + 1) % 3, k2 = (a + 2) % 3;
+            val = k == k1 ? -r[k2] : (k == k2 ? r[k1] : 0.0);
+        }
+    }
+    col[e] = val;
+}
+
+__global__ __launch_bounds__(256) void body_correct_kernel(const double* __restrict__ in, double host_sc,
+                                                          const double* __restrict__ p_in, int G,
+                                                          double* __restrict__ v, double* __restrict__ z,
+                                                          const double* __restrict__ Z, int64_t ldz,
+                                                          const double* __restrict__ t,
+                                                          const double* __restrict__ sinv, SaddleDims d,
+                                                          const int32_t* __restrict__ skip) {
+    __shared__ double scs;
+    if (skip && *skip) return;
+    if (p_in) {
+        if (threadIdx.x < kWave) {
+            const double tt = wave_sum_strided(p_in, G);
+            if (threadIdx.x == 0) {
+                const double hn = __builtin_sqrt(tt);
+                scs = hn > 0.0 ? 1.0 / hn : 0.0;
+            }
+        }
+        __syncthreads();
+    }
+    const double sc = p_in ? scs : host_sc;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < d.n; e += (int64_t)gridDim.x * 256) {
+        const int sw = swimmer_of(d, e);
+        double rb[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) rb[c] = in[d.off_body + 6 * sw + c] * sc + t[6 * sw + c];
+        double y[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc += sinv[36 * sw + 6 * c + k] * rb[k];
+            y[c] = -acc;
+        }
+        if (e >= d.off_body && e < d.off_phase) {
+            const int c = (int)((e - d.off_body) % 6);
+            z[e] = y[c];
+            if (v) v[e] = in[e] * sc;
+        } else {
+            double acc = z[e];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc += Z[(int64_t)c * ldz + e] * y[c];
+            z[e] = acc;
+        }
+    }
+}
+
 }  // namespace
 
 static inline int blocks_for(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
@@ -1208,6 +1322,29 @@ hipError_t k_phase_update(const double* x, const SaddleDims& d, double dt, doubl
                           hipStream_t s) {
     if (d.nfil == 0) return hipSuccess;
     hipLaunchKernelGGL(phase_update_kernel, dim3(blocks_for(d.nfil)), dim3(256), 0, s, x, d, dt, phase, angle);
+    return hipGetLastError();
+}
+
+hipError_t k_body_moments(const double* vec, const SaddleDims& d, const double* x_segs, const double* x_blobs,
+                          const double* x_bodies, double sign, double* out, const int32_t* skip, hipStream_t s) {
+    hipLaunchKernelGGL(body_moments_kernel, dim3((unsigned)d.nswim), dim3(256), 0, s, vec, d, x_segs, x_blobs, x_bodies,
+                       sign, out, skip);
+    return hipGetLastError();
+}
+
+hipError_t k_body_column(double* col, int c, const SaddleDims& d, const double* x_segs, const double* x_blobs,
+                         const double* x_bodies, hipStream_t s) {
+    hipLaunchKernelGGL(body_column_kernel, dim3(blocks_for(d.n)), dim3(256), 0, s, col, c, d, x_segs, x_blobs,
+                       x_bodies);
+    return hipGetLastError();
+}
+
+hipError_t k_body_correct(const double* in, double host_sc, const double* p_in, int G, double* v, double* z,
+                          const double* Z, int64_t ldz, const double* t, const double* sinv, const SaddleDims& d,
+                          const int32_t* skip, hipStream_t s) {
+    const int grid = std::min(blocks_for(d.n), 1024);
+    hipLaunchKernelGGL(body_correct_kernel, dim3(grid), dim3(256), 0, s, in, host_sc, p_in, G, v, z, Z, ldz, t, sinv,
+                       d, skip);
     return hipGetLastError();
 }
 

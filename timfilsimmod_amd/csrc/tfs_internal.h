@@ -130,6 +130,11 @@ struct SaddleEpilogue {
     const double* kth = nullptr;
     int nfil = 0, nseg = 0, dyn_phase = 0, dyn_angle = 0;
     int64_t off_phase = 0, off_angle = 0;
+    // free bodies: particle rows also get -(V + Omega x (x - x_body)) of
+    // their swimmer, V at in[off_body + 6 s]
+    const double* x_bodies = nullptr;
+    int free_body = 0, segs_per_swim = 0, blobs_per_swim = 0;
+    int64_t off_body = 0;
 };
 
 struct OutArgs {

@@ -88,6 +88,10 @@ struct tfs_solver {
     int32_t *patch_blob = nullptr, *patch_start = nullptr, *patch_size = nullptr, *patch_piv = nullptr,
             *patch_pivoff = nullptr, *patch_status = nullptr;
     int npatch = 0;
+    // Free bodies: centres, Z = P0^{-1} E (6 columns of ldv), the 6 x 6
+    // inverses of E^T Z per swimmer, and a 6-per-swimmer work vector.
+    double *x_bodies = nullptr, *bodyZ = nullptr, *body_sinv = nullptr, *body_t = nullptr;
+    bool bodies_set = false, body_pc_valid = false;
 };
 
 namespace {
@@ -113,9 +117,10 @@ int norm(tfs_solver* s, const double* v, double* out) {
 int apply_A(tfs_solver* s, const double* in, double* out, int precision, tfs_solve_stats* st,
             const int32_t* skip = nullptr) {
     const SaddleDims& dm = s->dims;
+    if (dm.free_body && !s->bodies_set) return fail(TFS_ERR_INVALID_ARGUMENT, "free bodies: tfs_solver_set_bodies not called");
     // The K_phi / K_theta couplings are applied by the matvec's reduce kernel.
     SaddleEpilogue sad;
-    const bool couplings = dm.nfil > 0 && (dm.dyn_phase || dm.dyn_angle);
+    const bool couplings = (dm.nfil > 0 && (dm.dyn_phase || dm.dyn_angle)) || dm.free_body;
     if (couplings) {
         sad.in = in;
         sad.out = out;
@@ -127,6 +132,20 @@ int apply_A(tfs_solver* s, const double* in, double* out, int precision, tfs_sol
         sad.dyn_angle = dm.dyn_angle;
         sad.off_phase = dm.off_phase;
         sad.off_angle = dm.off_angle;
+        if (dm.free_body) {
+            sad.x_bodies = s->x_bodies;
+            sad.free_body = 1;
+            sad.segs_per_swim = dm.segs_per_swim;
+            sad.blobs_per_swim = dm.blobs_per_swim;
+            sad.off_body = dm.off_body;
+        }
+    }
+    if (dm.free_body) {
+        // body rows: -K_b^T lambda (mobility_solver.cpp:1850-1908)
+        int rc0 = check(k_body_moments(in, dm, s->rpy->x_segs, s->rpy->x_blobs, s->x_bodies, -1.0, out + dm.off_body,
+                                       skip, s->stream),
+                        "body rows");
+        if (rc0) return rc0;
     }
     s->rpy->skip_flag = skip;
     int rc = run_matvec(s->rpy, TFS_BLOCK_ALL, precision, CView{in, 3, 1}, CView{in + dm.off_blob, 3, 1},
@@ -336,8 +355,81 @@ int ensure_block_precond(tfs_solver* s) {
     return TFS_OK;
 }
 
+// The free-body part of P^{-1} after P0^{-1} wrote z (see k_body_correct).
+int body_correct(tfs_solver* s, const double* in, double host_sc, const double* p_in, int G, double* v, double* z,
+                 const int32_t* skip) {
+    const SaddleDims& d = s->dims;
+    if (!d.free_body) return TFS_OK;
+    int rc = check(k_body_moments(z, d, s->rpy->x_segs, s->rpy->x_blobs, s->x_bodies, 1.0, s->body_t, skip, s->stream),
+                   "body moments");
+    if (!rc)
+        rc = check(k_body_correct(in, host_sc, p_in, G, v, z, s->bodyZ, s->ldv, s->body_t, s->body_sinv, d, skip,
+                                  s->stream),
+                   "body correction");
+    return rc;
+}
+
 int apply_P(tfs_solver* s, const double* in, double* out) {
-    return check(k_precond(in, out, precond_args(s), s->dims, s->stream), "preconditioner");
+    int rc = check(k_precond(in, out, precond_args(s), s->dims, s->stream), "preconditioner");
+    if (!rc) rc = body_correct(s, in, 1.0, nullptr, 0, nullptr, out, nullptr);
+    return rc;
+}
+
+// Z = P0^{-1} E and the inverses of E^T Z (one 6 x 6 per swimmer), rebuilt
+// when the geometry or P0 changed.
+int ensure_body_precond(tfs_solver* s) {
+    const SaddleDims& d = s->dims;
+    if (!d.free_body || s->body_pc_valid) return TFS_OK;
+    if (!s->bodies_set) return fail(TFS_ERR_INVALID_ARGUMENT, "free bodies: tfs_solver_set_bodies not called");
+    const int nsw = d.nswim;
+    std::vector<double> sig(36 * (size_t)nsw);
+    int rc = TFS_OK;
+    for (int c = 0; c < 6 && !rc; ++c) {
+        double* Zc = s->bodyZ + (int64_t)c * s->ldv;
+        rc = check(k_body_column(s->t, c, d, s->rpy->x_segs, s->rpy->x_blobs, s->x_bodies, s->stream), "body column");
+        if (!rc) rc = check(k_fill(Zc, 0.0, d.n, s->stream), "body Z");
+        if (!rc) rc = check(k_precond(s->t, Zc, precond_args(s), d, s->stream), "body Z");
+        if (!rc) rc = check(k_body_moments(Zc, d, s->rpy->x_segs, s->rpy->x_blobs, s->x_bodies, 1.0, s->body_t, nullptr,
+                                           s->stream),
+                            "body Schur");
+        std::vector<double> col(6 * (size_t)nsw);
+        if (!rc) rc = check(hipMemcpyAsync(col.data(), s->body_t, 48 * (size_t)nsw, hipMemcpyDeviceToHost, s->stream),
+                            "body Schur");
+        if (!rc) rc = check(hipStreamSynchronize(s->stream), "body Schur");
+        for (int w = 0; w < nsw && !rc; ++w)
+            for (int r = 0; r < 6; ++r) sig[36 * (size_t)w + 6 * r + c] = col[6 * (size_t)w + r];
+    }
+    if (rc) return rc;
+    // invert each 6 x 6 (Gauss-Jordan, partial pivoting)
+    std::vector<double> inv(36 * (size_t)nsw);
+    for (int w = 0; w < nsw; ++w) {
+        double a[6][12];
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 12; ++c) a[r][c] = c < 6 ? sig[36 * (size_t)w + 6 * r + c] : (c - 6 == r ? 1.0 : 0.0);
+        for (int k = 0; k < 6; ++k) {
+            int piv = k;
+            for (int r = k + 1; r < 6; ++r)
+                if (std::fabs(a[r][k]) > std::fabs(a[piv][k])) piv = r;
+            if (!(std::fabs(a[piv][k]) > 0.0) || !std::isfinite(a[piv][k]))
+                return fail(TFS_ERR_NON_FINITE, "free bodies: singular body Schur complement");
+            if (piv != k)
+                for (int c = 0; c < 12; ++c) std::swap(a[k][c], a[piv][c]);
+            const double inv_p = 1.0 / a[k][k];
+            for (int c = 0; c < 12; ++c) a[k][c] *= inv_p;
+            for (int r = 0; r < 6; ++r)
+                if (r != k) {
+                    const double f = a[r][k];
+                    for (int c = 0; c < 12; ++c) a[r][c] -= f * a[k][c];
+                }
+        }
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) inv[36 * (size_t)w + 6 * r + c] = a[r][6 + c];
+    }
+    rc = check(hipMemcpyAsync(s->body_sinv, inv.data(), 288 * (size_t)nsw, hipMemcpyHostToDevice, s->stream),
+               "body Schur inverse");
+    if (!rc) rc = check(hipStreamSynchronize(s->stream), "body Schur inverse");
+    if (!rc) s->body_pc_valid = true;
+    return rc;
 }
 
 // r = b - A x (x may be known to be zero: then r = b).
@@ -394,14 +486,17 @@ int gmres(tfs_solver* s, int precision, const double* b, double* x, double* r, d
             double* vj = s->V + (int64_t)j * s->ldv;
             // v_j = w_{j-1} / ||w_{j-1}|| (v_0 = r / ||r||), z = P^{-1} v_j; for
             // j > 0 the same launch closes step j - 1.
-            if (j == 0)
+            if (j == 0) {
                 rc = check(k_scale_precond(r, nullptr, 1.0 / rnorm, vj, s->z, precond_args(s), s->dims, nullptr,
                                            s->stream), "v0");
-            else
+                if (!rc) rc = body_correct(s, r, 1.0 / rnorm, nullptr, 0, vj, s->z, nullptr);
+            } else {
                 rc = check(k_arnoldi_close_next(s->pa, s->GA, s->H, ldh, j - 1, s->cs, s->sn, s->g, s->scal, s->est,
                                                 s->stop, abs_tol, s->w, vj, s->z, precond_args(s), s->dims,
                                                 s->stream),
                            "close and normalise");
+                if (!rc) rc = body_correct(s, s->w, 1.0, s->pa, s->GA, vj, s->z, s->stop);
+            }
             if (!rc) rc = apply_A(s, s->z, s->w, precision, nullptr, s->stop);
             if (!rc)
                 rc = check(k_arnoldi_step(s->V, s->ldv, j, s->w, n, s->pa, s->pb, s->GA, s->H, ldh, s->cs, s->sn,
@@ -467,6 +562,13 @@ int alloc_solver(tfs_solver* s) {
         {&s->cs, m}, {&s->sn, m}, {&s->g, ldh}, {&s->y, ldh}, {&s->h, ldh},
         {&s->partial, (int64_t)s->G * ldh}, {&s->scal, 8}, {&s->nrm, 8}, {&s->est, ldh},
         {&s->pa, (int64_t)s->GA * ldh}, {&s->pb, (int64_t)s->GA * ldh}, {(double**)&s->stop, 1}};
+    if (s->dims.free_body) {
+        const int64_t nsw = s->dims.nswim;
+        parts.push_back({&s->x_bodies, 3 * nsw});
+        parts.push_back({&s->bodyZ, 6 * s->ldv});
+        parts.push_back({&s->body_sinv, 36 * nsw});
+        parts.push_back({&s->body_t, 6 * nsw});
+    }
     int64_t total = 0;
     for (auto& p : parts) total += ((p.second + 31) / 32) * 32;
     hipError_t e = hipMalloc(&s->arena, total * sizeof(double));
@@ -500,6 +602,7 @@ int solve_device(tfs_solver* s, const double* b, double* x, tfs_solve_stats* st)
     const tfs_solver_desc& D = s->desc;
     *st = tfs_solve_stats{};
     int rc = ensure_block_precond(s);
+    if (!rc) rc = ensure_body_precond(s);
     if (rc) return rc;
     double bnorm = 0.0;
     rc = norm(s, b, &bnorm);
@@ -596,8 +699,18 @@ int tfs_solver_create(const tfs_solver_desc* desc, tfs_solver** out) {
     dm.dyn_phase = D.dynamic_phase ? 1 : 0;
     dm.dyn_angle = D.dynamic_angle ? 1 : 0;
     const int64_t ns = (int64_t)D.num_fil * D.num_seg;
+    dm.nswim = D.num_swim > 0 ? D.num_swim : 1;
+    dm.free_body = D.free_bodies ? 1 : 0;
+    if (D.num_fil % dm.nswim != 0 || D.num_blobs % dm.nswim != 0) {
+        delete s;
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_create: filaments and blobs must split evenly between swimmers");
+    }
+    dm.fil_per_swim = D.num_fil / dm.nswim;
+    dm.segs_per_swim = (int)(ns / dm.nswim);
+    dm.blobs_per_swim = D.num_blobs / dm.nswim;
     dm.off_blob = 3 * ns;
-    dm.off_phase = dm.off_blob + 3LL * D.num_blobs;
+    dm.off_body = dm.off_blob + 3LL * D.num_blobs;
+    dm.off_phase = dm.off_body + (dm.free_body ? 6LL * dm.nswim : 0);
     dm.off_angle = dm.off_phase + (dm.dyn_phase ? D.num_fil : 0);
     dm.n = dm.off_angle + (dm.dyn_angle ? D.num_fil : 0);
     if (dm.n > INT32_MAX / 2) {
@@ -652,6 +765,19 @@ int tfs_solver_destroy(tfs_solver* s) {
     return TFS_OK;
 }
 
+int tfs_solver_set_bodies(tfs_solver* s, const double* x_bodies, int32_t memory) {
+    if (!s || !x_bodies) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_bodies: null argument");
+    if (!s->dims.free_body) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_bodies: the system has no free bodies");
+    if (memory != TFS_MEMORY_HOST && memory != TFS_MEMORY_DEVICE)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_bodies: unknown memory space");
+    hipError_t e = hipMemcpyAsync(s->x_bodies, x_bodies, 24LL * s->dims.nswim, kind_in(memory), s->stream);
+    if (e == hipSuccess && memory == TFS_MEMORY_HOST) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return hip_fail(e, "tfs_solver_set_bodies");
+    s->bodies_set = true;
+    s->body_pc_valid = false;
+    return TFS_OK;
+}
+
 int tfs_solver_set_preconditioner(tfs_solver* s, int32_t kind) {
     if (!s) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_preconditioner: null handle");
     if (kind != TFS_PRECOND_REFERENCE && kind != TFS_PRECOND_BLOCKS)
@@ -660,6 +786,7 @@ int tfs_solver_set_preconditioner(tfs_solver* s, int32_t kind) {
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_preconditioner: block form needs num_seg <= " +
                                                   std::to_string(kMaxBlockFilSeg));
     s->precond_kind = kind;
+    s->body_pc_valid = false;
     return TFS_OK;
 }
 
@@ -686,6 +813,7 @@ int tfs_solver_set_geometry(tfs_solver* s, const double* x_segs, const double* x
     if (x_segs || x_blobs) invalidate_near(s->rpy);
     if (x_segs || vel_dir_phase || vel_dir_angle) s->fil_pc_valid = false;
     if (x_blobs) s->blob_pc_valid = false;
+    if (x_segs || x_blobs || vel_dir_phase || vel_dir_angle) s->body_pc_valid = false;
     if (x_segs && ns3) e = hipMemcpyAsync(s->rpy->x_segs, x_segs, ns3 * 8, k, s->stream);
     if (e == hipSuccess && x_blobs && nb3) e = hipMemcpyAsync(s->rpy->x_blobs, x_blobs, nb3 * 8, k, s->stream);
     if (e == hipSuccess && vel_dir_phase && ns3) e = hipMemcpyAsync(s->kphi, vel_dir_phase, ns3 * 8, k, s->stream);
@@ -726,6 +854,7 @@ int tfs_solver_precondition(tfs_solver* s, const double* in, double* out, int32_
         dout = s->x;
     }
     if (!rc) rc = ensure_block_precond(s);
+    if (!rc) rc = ensure_body_precond(s);
     if (!rc) rc = apply_P(s, din, dout);
     if (!rc && memory == TFS_MEMORY_HOST)
         rc = check(hipMemcpyAsync(out, dout, n * 8, hipMemcpyDeviceToHost, s->stream), "download");
@@ -799,6 +928,8 @@ int tfs_beat_create(const tfs_beat_desc* desc, tfs_beat** out) {
     if (!D.frames || !D.bases || !D.omega0 || !D.phase0 || !D.beat_A || !D.beat_B)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_create: missing geometry or beat tables");
     if (D.solver.num_blobs > 0 && !D.x_blobs) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_create: missing blobs");
+    if (D.solver.free_bodies || D.solver.num_swim > 1)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_create: one body held still (free bodies: tfs_solver_*)");
     if (D.solver.dynamic_phase && (!D.gen_phase_forces || D.n_gen_phase < 2))
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_create: missing generalised phase forces");
     if (D.solver.dynamic_angle && (!D.gen_angle_forces || D.n_gen_angle < 2))

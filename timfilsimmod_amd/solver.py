@@ -31,6 +31,7 @@ class SolverDesc(ctypes.Structure):
         ("seg_radius", ctypes.c_double), ("blob_radius", ctypes.c_double), ("mu", ctypes.c_double),
         ("restart", ctypes.c_int32), ("max_cycles", ctypes.c_int32), ("tol", ctypes.c_double),
         ("inner_tol", ctypes.c_double), ("max_refinements", ctypes.c_int32), ("stream", ctypes.c_void_p),
+        ("num_swim", ctypes.c_int32), ("free_bodies", ctypes.c_int32),
     ]
 
 
@@ -78,6 +79,7 @@ L.extra_signatures({
     "tfs_beat_set_mode": (ctypes.c_int, [_dp, _i32]),
     "tfs_beat_set_warm_start": (ctypes.c_int, [_dp, _i32]),
     "tfs_beat_restore": (ctypes.c_int, [_dp, _dp, _dp, _dp]),
+    "tfs_solver_set_bodies": (ctypes.c_int, [_dp, _dp, _i32]),
     "tfs_velocities_set_preconditioner": (ctypes.c_int, [_dp, _i32]),
     "tfs_beat_velocities": (ctypes.c_int, [_dp, _dp, _dp]),
     "tfs_solver_set_preconditioner": (ctypes.c_int, [_dp, _i32]),
@@ -103,7 +105,7 @@ L.extra_signatures({
 
 def solver_desc(num_fil, num_seg, num_blobs, *, dynamic_phase=True, dynamic_angle=True, mode=SOLVE_FP64,
                 seg_radius=1.0, blob_radius=1.0, mu=1.0, restart=50, max_cycles=20, tol=1e-8,
-                inner_tol=1e-3, max_refinements=10, stream=None) -> SolverDesc:
+                inner_tol=1e-3, max_refinements=10, stream=None, num_swim=1, free_bodies=False) -> SolverDesc:
     d = SolverDesc()
     d.num_fil, d.num_seg, d.num_blobs = num_fil, num_seg, num_blobs
     d.dynamic_phase, d.dynamic_angle, d.mode = int(dynamic_phase), int(dynamic_angle), mode
@@ -111,6 +113,7 @@ def solver_desc(num_fil, num_seg, num_blobs, *, dynamic_phase=True, dynamic_angl
     d.restart, d.max_cycles, d.tol = restart, max_cycles, tol
     d.inner_tol, d.max_refinements = inner_tol, max_refinements
     d.stream = stream
+    d.num_swim, d.free_bodies = num_swim, int(bool(free_bodies))
     return d
 
 
@@ -140,6 +143,11 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+    def set_bodies(self, x_bodies):
+        """Body centres of a free-body system (3 per swimmer)."""
+        xb = np.ascontiguousarray(x_bodies, dtype=np.float64)
+        L.check(L.lib().tfs_solver_set_bodies(self.h, L.ptr(xb), L.MEMORY_HOST))
 
     def set_geometry(self, x_segs=None, x_blobs=None, vel_dir_phase=None, vel_dir_angle=None):
         a = [None if v is None else np.ascontiguousarray(v, dtype=np.float64)
