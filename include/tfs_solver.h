@@ -222,9 +222,16 @@ int tfs_velocities_destroy(tfs_velocities *v);
  * blob) holds the warm start on entry and the solved blob forces on return.
  * Complete on return.  TFS_ERR_NOT_CONVERGED if the blob solve misses its
  * tolerance (outputs still written). */
-/* Preconditioner of the blob solve: TFS_PRECOND_REFERENCE (6 pi mu a, the
- * reference's, default) or TFS_PRECOND_BLOCKS (the blobs' mobility inverted
- * within 16-blob patches; rebuilt only when the blob positions change). */
+/* How the blob forces are solved for: GMRES preconditioned with
+ * TFS_PRECOND_REFERENCE (6 pi mu a, the reference's, default) or
+ * TFS_PRECOND_BLOCKS (the blobs' mobility inverted within 16-blob patches),
+ * or TFS_BLOB_DIRECT: the dense blob mobility (3 NBLOB square, equal blob radii)
+ * inverted on the device by blocked Gauss-Jordan, then one dense product per
+ * solve.  Both the patches and the inverse are rebuilt only when the blob
+ * positions change; the inverse holds 8 (3 NBLOB)^2 bytes of device memory
+ * (0.29 GB at 2,000 blobs, 5.8 GB at 9,000).  No counterpart in the
+ * reference, whose GMRES runs on every call. */
+#define TFS_BLOB_DIRECT 2
 int tfs_velocities_set_preconditioner(tfs_velocities *v, int32_t kind);
 int tfs_velocities_compute(tfs_velocities *v, const double *x_segs, const double *f_segs,
                            const double *x_blobs, const double *x_bodies, const double *v_bodies,

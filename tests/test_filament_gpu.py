@@ -23,7 +23,7 @@ from timfilsimmod_amd import _lib as L
 from timfilsimmod_amd import filament as FM
 from timfilsimmod_amd import geometry as G
 from timfilsimmod_amd.broyden import BroydenSolver
-from timfilsimmod_amd.solver import PRECOND_BLOCKS, PRECOND_REFERENCE, SOLVE_FP64, ElasticVelocities
+from timfilsimmod_amd.solver import BLOB_DIRECT, PRECOND_BLOCKS, PRECOND_REFERENCE, SOLVE_FP64, ElasticVelocities
 
 pytestmark = pytest.mark.gpu
 
@@ -278,7 +278,7 @@ def test_elastic_implicit_step_matches_oracle_loop(oracle_run):
     check_against_oracle(oracle_run, iters, xs)
 
 
-@pytest.mark.parametrize("precond", [PRECOND_REFERENCE, PRECOND_BLOCKS])
+@pytest.mark.parametrize("precond", [PRECOND_REFERENCE, PRECOND_BLOCKS, BLOB_DIRECT])
 def test_device_resident_elastic_step(oracle_run, precond):
     print(f"case elastic implicit step 16x20 + 2000 blobs, tfs_elastic_step, precond {precond}", flush=True)
     c = oracle_run[0]

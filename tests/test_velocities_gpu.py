@@ -98,3 +98,33 @@ def test_rigid_sphere_drag_and_torque():
         assert abs(r_drag - r_rot) <= 0.01 * R
     finally:
         ev.close()
+
+
+@pytest.mark.parametrize("nfil,nb", [(1, 10), (4, 200), (16, 2000)])
+def test_direct_blob_solve_matches_dense_oracle(nfil, nb):
+    """TFS_BLOB_DIRECT (dense inverse of the blob mobility by blocked
+    Gauss-Jordan on the device, one product per solve) against an LU solve
+    of the oracle's dense M_bb; sizes below, at and above one 64-row tile."""
+    import scipy.linalg as sla
+    from timfilsimmod_amd.solver import BLOB_DIRECT
+    print(f"case direct blob solve nfil={nfil} nb={nb}", flush=True)
+    body = G.make_body(nfil, 8, nb, 6.0 + nb ** 0.5 * 0.45)
+    ns = nfil * 8
+    ev = ElasticVelocities(1, ns, nb)
+    try:
+        ev.set_preconditioner(BLOB_DIRECT)
+        f = body.f_segs
+        V = np.array([0.1, -0.2, 0.05, 0.01, 0.02, -0.03])
+        for rep in range(2):          # the second call reuses the stored inverse
+            vs, lam, st = ev.compute(body.x_segs, f, body.x_blobs, body.centre, V)
+        vb = O.block("bs", body.x_segs, body.x_blobs, f, np.zeros(3 * nb), seg_dof=6, bs_accumulate=False)
+        r = body.x_blobs.reshape(-1, 3) - body.centre
+        kv = V[:3] + np.cross(V[3:], r)
+        want_lam = sla.solve(O.dense_mobility(np.zeros(0), body.x_blobs), -vb + kv.reshape(-1), assume_a="pos")
+        vs_o = O.block("ss", body.x_segs, body.x_blobs, f, want_lam, seg_dof=6)
+        vs_o = O.block("sb", body.x_segs, body.x_blobs, f, want_lam, vs_o, seg_dof=6)
+        assert np.linalg.norm(lam - want_lam) <= 1e-10 * np.linalg.norm(want_lam)
+        assert np.linalg.norm(vs - vs_o) <= 1e-10 * np.linalg.norm(vs_o)
+        assert st["converged"] == 1 and st["iterations"] == 0
+    finally:
+        ev.close()

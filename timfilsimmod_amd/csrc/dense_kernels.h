@@ -22,4 +22,28 @@ hipError_t k_batched_inverse(double* A, const int64_t* off, const int32_t* sizes
 hipError_t k_batched_inverse_spd(double* A, const int64_t* off, const int32_t* sizes, int nblocks, int max_size,
                                  int32_t* status, hipStream_t stream);
 
+// ---- dense SPD inverse of a large matrix (one body's blob mobility) ----------
+// Work space of k_spd_inverse_blocked for an n_pad x n_pad matrix (n_pad a
+// multiple of kDenseTile): 64 x 64 pivot block, its pivots, status flags.
+constexpr int kDenseTile = 64;
+struct DenseWork {
+    double* pivot = nullptr;    // kDenseTile^2
+    int64_t* off = nullptr;     // 1 (zero)
+    int32_t* size = nullptr;    // 1 (kDenseTile)
+    int32_t* piv = nullptr;     // kDenseTile
+    int32_t* piv_off = nullptr; // 1 (zero)
+    int32_t* status = nullptr;  // n_pad / kDenseTile, one per pivot step
+};
+// A (row-major n_pad x n_pad, symmetric positive definite) := A^{-1} by
+// blocked Gauss-Jordan elimination in 64 x 64 tiles: per step k the pivot
+// block is inverted (partial pivoting inside it), then the row panel, the
+// trailing update and the column panel are 64^3-FMA tile products, one
+// workgroup per tile.  work.status[k] = 1 if step k met a singular pivot.
+hipError_t k_spd_inverse_blocked(double* A, int n_pad, const DenseWork& work, hipStream_t stream);
+// y[0..n) = A x for a symmetric n_pad x n_pad A (x and y of n_pad entries, x
+// zero past n): column chunks into partial (chunks x n_pad), then summed in
+// chunk order.
+hipError_t k_symv_dense(const double* A, int n_pad, const double* x, double* y, double* partial, int chunks,
+                        hipStream_t stream);
+
 }  // namespace tfs

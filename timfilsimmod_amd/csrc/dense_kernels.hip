@@ -153,6 +153,106 @@ __global__ __launch_bounds__(256) void spd_inverse_wave_kernel(double* __restric
     if (lane == 0) status[b] = bad;
 }
 
+// ---- blocked Gauss-Jordan on 64 x 64 tiles ------------------------------------
+constexpr int T = kDenseTile;
+
+// C (64 x 64 at c, ld ldc) = alpha * Aa * Bb (+ C if accumulate), the tiles
+// staged in LDS; 256 threads, 4 x 4 outputs each.  Aa, Bb may alias C: both
+// are read completely before C is written.
+__device__ void tile_mm(const double* __restrict__ a, int64_t lda, const double* __restrict__ b, int64_t ldb,
+                        double* c, int64_t ldc, double alpha, bool accumulate) {
+    __shared__ double As[T][T + 1];
+    __shared__ double Bs[T][T + 1];
+    for (int e = threadIdx.x; e < T * T; e += 256) {
+        const int r = e / T, q = e % T;
+        As[r][q] = a[r * lda + q];
+        Bs[r][q] = b[r * ldb + q];
+    }
+    __syncthreads();
+    const int r0 = (threadIdx.x / 16) * 4, c0 = (threadIdx.x % 16) * 4;
+    double acc[4][4] = {};
+    for (int k = 0; k < T; ++k) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) av[i] = As[r0 + i][k];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bv[j] = Bs[k][c0 + j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            double* dst = c + (int64_t)(r0 + i) * ldc + c0 + j;
+            *dst = accumulate ? *dst + alpha * acc[i][j] : alpha * acc[i][j];
+        }
+}
+
+__global__ __launch_bounds__(256) void gj_copy_pivot_kernel(const double* __restrict__ A, int n_pad, int k,
+                                                           double* __restrict__ P) {
+    for (int e = threadIdx.x; e < T * T; e += 256) {
+        const int r = e / T, q = e % T;
+        P[e] = A[(int64_t)(k * T + r) * n_pad + k * T + q];
+    }
+}
+
+// Row panel: A_kj := P A_kj for every column tile j != k.
+__global__ __launch_bounds__(256) void gj_row_panel_kernel(double* A, int n_pad, int k, const double* P) {
+    const int j = blockIdx.x < (unsigned)k ? blockIdx.x : blockIdx.x + 1;
+    double* tile = A + (int64_t)k * T * n_pad + (int64_t)j * T;
+    tile_mm(P, T, tile, n_pad, tile, n_pad, 1.0, false);
+}
+
+// Trailing update: A_ij -= A_ik A_kj (A_kj already P A_kj) for i, j != k.
+__global__ __launch_bounds__(256) void gj_update_kernel(double* A, int n_pad, int k) {
+    const int nt = n_pad / T - 1;
+    const int bi = blockIdx.x / nt, bj = blockIdx.x % nt;
+    const int i = bi < k ? bi : bi + 1, j = bj < k ? bj : bj + 1;
+    tile_mm(A + (int64_t)i * T * n_pad + (int64_t)k * T, n_pad, A + (int64_t)k * T * n_pad + (int64_t)j * T, n_pad,
+            A + (int64_t)i * T * n_pad + (int64_t)j * T, n_pad, -1.0, true);
+}
+
+// Column panel: A_ik := -A_ik P for i != k, and A_kk := P.
+__global__ __launch_bounds__(256) void gj_col_panel_kernel(double* A, int n_pad, int k, const double* P) {
+    const int nt = n_pad / T;
+    if ((int)blockIdx.x == nt - 1) {
+        for (int e = threadIdx.x; e < T * T; e += 256)
+            A[(int64_t)(k * T + e / T) * n_pad + k * T + e % T] = P[e];
+        return;
+    }
+    const int i = blockIdx.x < (unsigned)k ? blockIdx.x : blockIdx.x + 1;
+    double* tile = A + (int64_t)i * T * n_pad + (int64_t)k * T;
+    tile_mm(tile, n_pad, P, T, tile, n_pad, -1.0, false);
+}
+
+__global__ __launch_bounds__(256) void gj_status_kernel(const int32_t* __restrict__ st, int32_t* __restrict__ out, int k) {
+    if (threadIdx.x == 0) out[k] = st[0];
+}
+
+__global__ __launch_bounds__(256) void symv_partial_kernel(const double* __restrict__ A, int n_pad,
+                                                          const double* __restrict__ x, double* __restrict__ partial,
+                                                          int chunk) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int c0 = blockIdx.y * chunk, c1 = min(c0 + chunk, n_pad);
+    if (r >= n_pad) return;
+    double acc = 0.0;
+    for (int c = c0; c < c1; ++c) acc += A[(int64_t)c * n_pad + r] * x[c];   // symmetric: row r = column r
+    partial[(int64_t)blockIdx.y * n_pad + r] = acc;
+}
+
+__global__ __launch_bounds__(256) void symv_sum_kernel(const double* __restrict__ partial, int n_pad, int chunks,
+                                                      double* __restrict__ y) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= n_pad) return;
+    double acc = 0.0;
+    for (int q = 0; q < chunks; ++q) acc += partial[(int64_t)q * n_pad + r];
+    y[r] = acc;
+}
+
 }  // namespace
 
 hipError_t k_batched_inverse_spd(double* A, const int64_t* off, const int32_t* sizes, int nblocks, int max_size,
@@ -170,6 +270,37 @@ hipError_t k_batched_inverse(double* A, const int64_t* off, const int32_t* sizes
     const size_t lds = sizeof(double) * 2 * (size_t)max_size;
     hipLaunchKernelGGL(block_gauss_jordan_kernel, dim3((unsigned)nblocks), dim3(256), lds, stream, A, off, sizes, piv,
                        piv_off, status);
+    return hipGetLastError();
+}
+
+hipError_t k_spd_inverse_blocked(double* A, int n_pad, const DenseWork& w, hipStream_t stream) {
+    const int nt = n_pad / T;
+    for (int k = 0; k < nt; ++k) {
+        hipLaunchKernelGGL(gj_copy_pivot_kernel, dim3(1), dim3(256), 0, stream, A, n_pad, k, w.pivot);
+        hipError_t e = k_batched_inverse(w.pivot, w.off, w.size, 1, T, w.piv, w.piv_off, w.status + nt, stream);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(gj_status_kernel, dim3(1), dim3(64), 0, stream, w.status + nt, w.status, k);
+        if (nt > 1) {
+            hipLaunchKernelGGL(gj_row_panel_kernel, dim3((unsigned)(nt - 1)), dim3(256), 0, stream, A, n_pad, k,
+                               (const double*)w.pivot);
+            hipLaunchKernelGGL(gj_update_kernel, dim3((unsigned)((nt - 1) * (nt - 1))), dim3(256), 0, stream, A,
+                               n_pad, k);
+        }
+        hipLaunchKernelGGL(gj_col_panel_kernel, dim3((unsigned)nt), dim3(256), 0, stream, A, n_pad, k,
+                           (const double*)w.pivot);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t k_symv_dense(const double* A, int n_pad, const double* x, double* y, double* partial, int chunks,
+                        hipStream_t stream) {
+    const int chunk = (n_pad + chunks - 1) / chunks;
+    hipLaunchKernelGGL(symv_partial_kernel, dim3((unsigned)((n_pad + 255) / 256), (unsigned)chunks), dim3(256), 0,
+                       stream, A, n_pad, x, partial, chunk);
+    hipLaunchKernelGGL(symv_sum_kernel, dim3((unsigned)((n_pad + 255) / 256)), dim3(256), 0, stream, partial, n_pad,
+                       chunks, y);
     return hipGetLastError();
 }
 

@@ -146,4 +146,9 @@ hipError_t k_body_correct(const double* in, double host_sc, const double* p_in, 
                           const double* Z, int64_t ldz, const double* t, const double* sinv, const SaddleDims& d,
                           const int32_t* skip, hipStream_t s);
 
+// Dense blob-blob mobility of equal spheres into a row-major n_pad x n_pad
+// matrix (identity past 3 nb).
+hipError_t k_blob_mobility_dense(const double* x_blobs, int nb, double a, double mu, double* A, int n_pad,
+                                 hipStream_t s);
+
 }  // namespace tfs

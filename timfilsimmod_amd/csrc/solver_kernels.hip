@@ -1185,6 +1185,32 @@ __global__ __launch_bounds__(256) void body_correct_kernel(const double* __restr
     }
 }
 
+// Dense blob mobility (row-major n_pad x n_pad, n = 3 nb): 3 x 3 RPY blocks of
+// every blob pair (the pair tensor's every branch, self included); the
+// padding past n is the identity.  One thread per (i, j) block.
+__global__ __launch_bounds__(256) void blob_mobility_dense_kernel(const double* __restrict__ xb, int nb, double a,
+                                                                 double mu, double* __restrict__ A, int n_pad) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t nbb = (int64_t)nb * nb;
+    if (idx < nbb) {
+        const int i = (int)(idx / nb), j = (int)(idx % nb);
+        double blk[9];
+        rpy_block_entries(xb + 3 * i, xb + 3 * j, a, mu, blk);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) A[(int64_t)(3 * i + r) * n_pad + 3 * j + c] = blk[3 * r + c];
+        return;
+    }
+    // padding: identity rows and columns
+    const int64_t p = idx - nbb;
+    const int n = 3 * nb, extra = n_pad - n;
+    if (p >= (int64_t)extra * n_pad) return;
+    const int r = n + (int)(p / n_pad), c = (int)(p % n_pad);
+    A[(int64_t)r * n_pad + c] = r == c ? 1.0 : 0.0;
+    A[(int64_t)c * n_pad + r] = r == c ? 1.0 : 0.0;
+}
+
 }  // namespace
 
 static inline int blocks_for(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
@@ -1345,6 +1371,14 @@ hipError_t k_body_correct(const double* in, double host_sc, const double* p_in, 
     const int grid = std::min(blocks_for(d.n), 1024);
     hipLaunchKernelGGL(body_correct_kernel, dim3(grid), dim3(256), 0, s, in, host_sc, p_in, G, v, z, Z, ldz, t, sinv,
                        d, skip);
+    return hipGetLastError();
+}
+
+hipError_t k_blob_mobility_dense(const double* x_blobs, int nb, double a, double mu, double* A, int n_pad,
+                                 hipStream_t s) {
+    const int64_t work = (int64_t)nb * nb + (int64_t)(n_pad - 3 * nb) * n_pad;
+    hipLaunchKernelGGL(blob_mobility_dense_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, x_blobs, nb,
+                       a, mu, A, n_pad);
     return hipGetLastError();
 }
 

@@ -15,6 +15,8 @@
 #include <vector>
 
 #include "../../include/tfs_solver.h"
+#include "dense_kernels.h"
+#include "solver_kernels.h"
 #include "tfs_api_internal.h"
 
 using namespace tfs;
@@ -70,7 +72,66 @@ struct tfs_velocities {
     int32_t* d_flag = nullptr;
     int32_t* h_flag = nullptr;
     bool have_blobs = false;
+    // Direct blob solve (TFS_BLOB_DIRECT): the inverse of the dense blob
+    // mobility, formed once per blob geometry, applied as one dense product.
+    bool direct = false, direct_valid = false;
+    int n_pad = 0, chunks = 32;
+    double *Ainv = nullptr, *xpad = nullptr, *ypad = nullptr, *sym_partial = nullptr;
+    void* dense_arena = nullptr;
+    tfs::DenseWork dw;
 };
+
+namespace {
+
+// Forms v->Ainv = M_bb^{-1} for the current blob positions.
+int build_direct(tfs_velocities* v) {
+    const int nb = (int)v->nb;
+    const int n = 3 * nb;
+    const int n_pad = (n + kDenseTile - 1) / kDenseTile * kDenseTile;
+    const int nt = n_pad / kDenseTile;
+    hipStream_t st = v->stream;
+    if (!v->Ainv) {
+        v->n_pad = n_pad;
+        if (hipMalloc(&v->Ainv, 8 * (size_t)n_pad * n_pad) != hipSuccess)
+            return fail(TFS_ERR_OUT_OF_MEMORY, "direct blob solve: " + std::to_string(8.0 * n_pad * n_pad / 1e9) +
+                                                   " GB for the dense blob mobility");
+        const size_t bytes = 8 * ((size_t)2 * n_pad + (size_t)v->chunks * n_pad + kDenseTile * kDenseTile) + 8 + 4 +
+                             4 * kDenseTile + 4 + 4 * (size_t)(nt + 1) + 64;
+        if (hipMalloc(&v->dense_arena, bytes) != hipSuccess) return fail(TFS_ERR_OUT_OF_MEMORY, "direct blob solve: work space");
+        char* p = (char*)v->dense_arena;
+        auto take = [&](size_t b) { char* q = p; p += (b + 7) / 8 * 8; return q; };
+        v->xpad = (double*)take(8 * (size_t)n_pad);
+        v->ypad = (double*)take(8 * (size_t)n_pad);
+        v->sym_partial = (double*)take(8 * (size_t)v->chunks * n_pad);
+        v->dw.pivot = (double*)take(8 * kDenseTile * kDenseTile);
+        v->dw.off = (int64_t*)take(8);
+        v->dw.size = (int32_t*)take(4);
+        v->dw.piv = (int32_t*)take(4 * kDenseTile);
+        v->dw.piv_off = (int32_t*)take(4);
+        v->dw.status = (int32_t*)take(4 * (size_t)(nt + 1));
+        const int64_t zero = 0;
+        const int32_t sz = kDenseTile, zero32 = 0;
+        int rc = check(hipMemsetAsync(v->xpad, 0, 8 * (size_t)n_pad, st), "direct blob solve");
+        if (!rc) rc = check(hipMemcpyAsync(v->dw.off, &zero, 8, hipMemcpyHostToDevice, st), "direct blob solve");
+        if (!rc) rc = check(hipMemcpyAsync(v->dw.size, &sz, 4, hipMemcpyHostToDevice, st), "direct blob solve");
+        if (!rc) rc = check(hipMemcpyAsync(v->dw.piv_off, &zero32, 4, hipMemcpyHostToDevice, st), "direct blob solve");
+        if (!rc) rc = check(hipStreamSynchronize(st), "direct blob solve");
+        if (rc) return rc;
+    }
+    int rc = check(k_blob_mobility_dense(v->rpy->x_blobs, nb, v->desc.blob_radius, v->desc.mu, v->Ainv, n_pad, st),
+                   "dense blob mobility");
+    if (!rc) rc = check(k_spd_inverse_blocked(v->Ainv, n_pad, v->dw, st), "dense blob inverse");
+    std::vector<int32_t> status(nt);
+    if (!rc) rc = check(hipMemcpyAsync(status.data(), v->dw.status, 4 * (size_t)nt, hipMemcpyDeviceToHost, st), "status");
+    if (!rc) rc = check(hipStreamSynchronize(st), "dense blob inverse");
+    if (rc) return rc;
+    for (int k = 0; k < nt; ++k)
+        if (status[k]) return fail(TFS_ERR_NON_FINITE, "direct blob solve: singular pivot block " + std::to_string(k));
+    v->direct_valid = true;
+    return TFS_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -156,6 +217,8 @@ int tfs_velocities_destroy(tfs_velocities* v) {
     if (v->d_rhs) (void)hipFree(v->d_rhs);
     if (v->d_bodies) (void)hipFree(v->d_bodies);
     if (v->d_flag) (void)hipFree(v->d_flag);
+    if (v->Ainv) (void)hipFree(v->Ainv);
+    if (v->dense_arena) (void)hipFree(v->dense_arena);
     if (v->h_flag) (void)hipHostFree(v->h_flag);
     if (v->own_stream) (void)hipStreamDestroy(v->stream);
     delete v;
@@ -212,10 +275,21 @@ int tfs_velocities_compute(tfs_velocities* v, const double* x_segs, const double
         rc = check(hipGetLastError(), "blob rhs");
     }
     // GMRES on M_bb for the blob forces, warm-started from the given ones
+    // (or, direct, one product with the stored inverse)
     if (!rc && blobs_moved) rc = tfs_solver_set_geometry(v->blobs, nullptr, h->x_blobs, nullptr, nullptr, TFS_MEMORY_DEVICE);
     if (!rc) v->have_blobs = true;
+    if (!rc && blobs_moved) v->direct_valid = false;
     int solve_rc = TFS_OK;
-    if (!rc) {
+    if (!rc && v->direct) {
+        if (!v->direct_valid) rc = build_direct(v);
+        if (!rc) rc = check(hipMemcpyAsync(v->xpad, v->d_rhs, 24 * nb, hipMemcpyDeviceToDevice, st), "direct rhs");
+        if (!rc) rc = check(k_symv_dense(v->Ainv, v->n_pad, v->xpad, v->ypad, v->sym_partial, v->chunks, st), "direct solve");
+        if (!rc) rc = check(hipMemcpyAsync(h->f_blobs, v->ypad, 24 * nb, hipMemcpyDeviceToDevice, st), "direct result");
+        if (stats) {
+            *stats = tfs_solve_stats{};
+            stats->converged = 1;
+        }
+    } else if (!rc) {
         solve_rc = tfs_solver_solve(v->blobs, v->d_rhs, h->f_blobs, TFS_MEMORY_DEVICE, stats);
         if (solve_rc != TFS_OK && solve_rc != TFS_ERR_NOT_CONVERGED) rc = solve_rc;
     }
@@ -235,7 +309,13 @@ int tfs_velocities_compute(tfs_velocities* v, const double* x_segs, const double
 
 int tfs_velocities_set_preconditioner(tfs_velocities* v, int32_t kind) {
     if (!v) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_set_preconditioner: null handle");
-    return tfs_solver_set_preconditioner(v->blobs, kind);
+    if (kind == TFS_BLOB_DIRECT) {
+        v->direct = true;
+        return TFS_OK;
+    }
+    const int rc = tfs_solver_set_preconditioner(v->blobs, kind);
+    if (!rc) v->direct = false;
+    return rc;
 }
 
 }  // extern "C"

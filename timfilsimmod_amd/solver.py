@@ -22,6 +22,7 @@ SOLVE_FP64 = 0
 SOLVE_MIXED = 1
 PRECOND_REFERENCE = 0   # D = I/(6 pi mu a), the reference's preconditioner
 PRECOND_BLOCKS = 1      # D = filament self-mobility blocks + blob patches
+BLOB_DIRECT = 2         # tfs_velocities: dense inverse of the blob mobility
 
 
 class SolverDesc(ctypes.Structure):
