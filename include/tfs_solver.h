@@ -174,6 +174,15 @@ int tfs_beat_restore(tfs_beat *b, const double *phase, const double *angle, cons
  * the product the reference makes before saving velocities
  * (mobility_solver.cpp:3053-3058, written at :3149). */
 int tfs_beat_velocities(tfs_beat *b, double *v_segs, double *v_blobs);
+/* Body state: centre x [3], orientation q [4] (w, x, y, z) and velocity v [6]
+ * (V, Omega; the last solve's for a swimming body).  With
+ * desc.solver.free_bodies the body swims: its velocity is solved for with
+ * force- and torque-free rows, and each step first moves it explicitly,
+ * x += dt V, q = exp(dt Omega) q (cilia_sim_main.cu:677-693), carrying the
+ * filament bases, frames and blobs with it (rigid_body::blob_positions,
+ * rigid_body.cpp:482-500); the positions given at creation are the body frame
+ * at orientation 1. */
+int tfs_beat_body(tfs_beat *b, double *x, double *q, double *v);
 /* The solver handle inside the driver (to call tfs_solver_* on its state). */
 int tfs_beat_solver(tfs_beat *b, tfs_solver **out);
 /* Switch between TFS_SOLVE_FP64 and TFS_SOLVE_MIXED for later steps. */

@@ -113,3 +113,63 @@ def test_rigid_sphere_under_force_and_torque():
         assert abs(rh_drag - rh_rot) <= 0.02 * R
     finally:
         s.close()
+
+
+def test_swimming_beat_matches_oracle_loop():
+    """A swimming body in the prescribed-beat loop (free_bodies): each step the
+    body moves by the last solve's V, Omega (cilia_sim_main.cu:677-693), its
+    filaments and blobs with it; against the same loop on the oracle (dense
+    free-body solves, the beat geometry of the moved frames)."""
+    from scipy.spatial.transform import Rotation
+    from timfilsimmod_amd.solver import BeatSimulation
+    print("case swimming beat", flush=True)
+    body = G.make_body(6, 20, 200, 8.0, centre=(1.0, -2.0, 0.5))
+    steps, dt = 4, 1.0 / 300.0
+    sim = BeatSimulation(body, tol=1e-11, restart=60, max_cycles=50, free_body=True)
+    su = sim.setup
+    try:
+        gpu = []
+        for _ in range(steps):
+            st = sim.step()
+            assert st["converged"] == 1
+            x, q, v = sim.body()
+            gpu.append((sim.state()["phase"].copy(), x, q, v))
+    finally:
+        sim.close()
+    # oracle loop
+    nf, nseg, nb = body.num_fil, body.num_seg, body.num_blobs
+    X0 = np.array(body.centre)
+    X, qb = X0.copy(), np.array([1.0, 0.0, 0.0, 0.0])
+    phase, angle = su.phase0.copy(), su.angle0.copy()
+    base_ref = su.bases.reshape(nf, 3) - X0
+    frames_ref = su.frames.reshape(nf, 3, 3)
+    blob_ref = su.x_blobs.reshape(nb, 3) - X0
+    sol = None
+    for step in range(steps):
+        if step > 0:
+            off = 3 * (nf * nseg + nb)
+            V, W = sol[off:off + 3], sol[off + 3:off + 6]
+            phase = phase + dt * sol[off + 6:off + 6 + nf]
+            angle = angle + dt * sol[off + 6 + nf:off + 6 + 2 * nf]
+            X = X + dt * V
+            e = Rotation.from_rotvec(dt * W)
+            qb = np.roll((e * Rotation.from_quat(np.roll(qb, -1))).as_quat(), 1)
+            if qb[0] * gpu[step][2][0] < 0:
+                qb = -qb
+        R = Rotation.from_quat(np.roll(qb, -1)).as_matrix()
+        moved = type(su)(**{**su.__dict__, "bases": (X + base_ref @ R.T).reshape(-1),
+                            "frames": (frames_ref @ R.T).reshape(-1), "x_blobs": (X + blob_ref @ R.T).reshape(-1)})
+        geo = SO.beat_geometry(moved, nseg, B.BEAT_A, B.BEAT_B, phase=phase, angle=angle)
+        xb = moved.x_blobs
+        A = SO.saddle_matrix(geo["x_segs"], xb, geo["kphi"], geo["ktheta"], nf, nseg, x_bodies=X)
+        r0 = SO.rhs_ref(moved, nseg, nb, geo["x_segs"], xb, X, np.zeros(6), phase=phase, angle=angle)
+        off = 3 * (nf * nseg + nb)
+        rhs = np.concatenate([r0[:off], np.zeros(6), r0[off:]])
+        sol = np.linalg.solve(A, rhs)
+        gp, gx, gq, gv = gpu[step]
+        np.testing.assert_allclose(gp, phase, rtol=0, atol=1e-9)
+        np.testing.assert_allclose(gx, X, rtol=0, atol=1e-9)
+        np.testing.assert_allclose(gq, qb, rtol=0, atol=1e-9)
+        off = 3 * (nf * nseg + nb)
+        assert np.linalg.norm(gv - sol[off:off + 6]) <= 1e-7 * np.linalg.norm(sol[off:off + 6])
+    assert np.linalg.norm(gpu[-1][1] - X0) > 0   # it swam

@@ -151,4 +151,12 @@ hipError_t k_body_correct(const double* in, double host_sc, const double* p_in, 
 hipError_t k_blob_mobility_dense(const double* x_blobs, int nb, double a, double mu, double* A, int n_pad,
                                  hipStream_t s);
 
+// Swimming body: pose = (X[3], q[4]) on the device.  Lab-frame bases,
+// frames, blob positions and body centre from body-frame references; and the
+// explicit pose update from the solution's body velocity.
+hipError_t k_body_pose(const double* pose, const double* base_ref, const double* frame_ref, const double* blob_ref,
+                       int nfil, int nblob, double* bases, double* frames, double* x_blobs, double* body_x,
+                       hipStream_t s);
+hipError_t k_body_advance(const double* sol, int64_t off_body, double dt, double* pose, hipStream_t s);
+
 }  // namespace tfs

@@ -15,6 +15,8 @@
 //   :282-440 assemble_rhs) and the explicit phase update (filament.cpp:1087-1097).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "rpy_pair.h"
 #include "solver_kernels.h"
 #include "wave_ops.h"
@@ -1047,6 +1049,7 @@ __global__ __launch_bounds__(256) void rhs_kernel(RhsArgs a, const double* __res
         o[1] = V[1] + V[5] * r0 - V[3] * r2;
         o[2] = V[2] + V[3] * r1 - V[4] * r0;
     }
+    if (a.d.free_body && t < 6 * a.d.nswim) rhs[a.d.off_body + t] = 0.0;  // force- and torque-free
     if (t < a.d.nfil) {
         const double w = a.omega0[t] * 0.5 / PI;
         if (a.d.dyn_phase) {
@@ -1209,6 +1212,57 @@ __global__ __launch_bounds__(256) void blob_mobility_dense_kernel(const double* 
     const int r = n + (int)(p / n_pad), c = (int)(p % n_pad);
     A[(int64_t)r * n_pad + c] = r == c ? 1.0 : 0.0;
     A[(int64_t)c * n_pad + r] = r == c ? 1.0 : 0.0;
+}
+
+// Pose of a swimming body (X, q): filament bases X + R b_ref, frames R e_ref,
+// blob positions X + R x_ref (rigid_body::blob_positions, rigid_body.cpp:482-500);
+// thread 0 also writes body_x = X.
+__global__ __launch_bounds__(256) void body_pose_kernel(const double* __restrict__ pose,
+                                                       const double* __restrict__ base_ref,
+                                                       const double* __restrict__ frame_ref,
+                                                       const double* __restrict__ blob_ref, int nfil, int nblob,
+                                                       double* __restrict__ bases, double* __restrict__ frames,
+                                                       double* __restrict__ x_blobs, double* __restrict__ body_x) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const double X[3] = {pose[0], pose[1], pose[2]};
+    const double w = pose[3], x = pose[4], y = pose[5], z = pose[6];
+    const double R[9] = {1.0 - 2.0 * (y * y + z * z), 2.0 * (x * y - w * z),       2.0 * (x * z + w * y),
+                         2.0 * (x * y + w * z),       1.0 - 2.0 * (x * x + z * z), 2.0 * (y * z - w * x),
+                         2.0 * (x * z - w * y),       2.0 * (y * z + w * x),       1.0 - 2.0 * (x * x + y * y)};
+    auto rot = [&](const double* v, double* o, bool shift) {
+        for (int r = 0; r < 3; ++r) o[r] = (shift ? X[r] : 0.0) + R[3 * r] * v[0] + R[3 * r + 1] * v[1] + R[3 * r + 2] * v[2];
+    };
+    if (t == 0)
+        for (int r = 0; r < 3; ++r) body_x[r] = X[r];
+    if (t < nfil) {
+        rot(base_ref + 3 * t, bases + 3 * t, true);
+        for (int e = 0; e < 3; ++e) rot(frame_ref + 9 * t + 3 * e, frames + 9 * t + 3 * e, false);
+    }
+    if (t < nblob) rot(blob_ref + 3 * t, x_blobs + 3 * t, true);
+}
+
+// Explicit body update after a solve (cilia_sim_main.cu:677-693):
+// X += dt V, q = exp(dt Omega) q.
+__global__ void body_advance_kernel(const double* __restrict__ sol, int64_t off_body, double dt,
+                                    double* __restrict__ pose) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double* V = sol + off_body;
+    for (int r = 0; r < 3; ++r) pose[r] += dt * V[r];
+    const double u[3] = {dt * V[3], dt * V[4], dt * V[5]};
+    const double th = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    double e0 = 1.0, e1 = 0.0, e2 = 0.0, e3 = 0.0;
+    if (th >= 1e-6) {  // lie_exp (quaternion.cpp:522-545)
+        const double sn = sin(0.5 * th) / th;
+        e0 = cos(0.5 * th);
+        e1 = sn * u[0];
+        e2 = sn * u[1];
+        e3 = sn * u[2];
+    }
+    const double q0 = pose[3], q1 = pose[4], q2 = pose[5], q3 = pose[6];
+    pose[3] = e0 * q0 - (e1 * q1 + e2 * q2 + e3 * q3);
+    pose[4] = e0 * q1 + q0 * e1 + e2 * q3 - e3 * q2;
+    pose[5] = e0 * q2 + q0 * e2 + e3 * q1 - e1 * q3;
+    pose[6] = e0 * q3 + q0 * e3 + e1 * q2 - e2 * q1;
 }
 
 }  // namespace
@@ -1379,6 +1433,20 @@ hipError_t k_blob_mobility_dense(const double* x_blobs, int nb, double a, double
     const int64_t work = (int64_t)nb * nb + (int64_t)(n_pad - 3 * nb) * n_pad;
     hipLaunchKernelGGL(blob_mobility_dense_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, x_blobs, nb,
                        a, mu, A, n_pad);
+    return hipGetLastError();
+}
+
+hipError_t k_body_pose(const double* pose, const double* base_ref, const double* frame_ref, const double* blob_ref,
+                       int nfil, int nblob, double* bases, double* frames, double* x_blobs, double* body_x,
+                       hipStream_t s) {
+    const int n = std::max(std::max(nfil, nblob), 1);
+    hipLaunchKernelGGL(body_pose_kernel, dim3(blocks_for(n)), dim3(256), 0, s, pose, base_ref, frame_ref, blob_ref,
+                       nfil, nblob, bases, frames, x_blobs, body_x);
+    return hipGetLastError();
+}
+
+hipError_t k_body_advance(const double* sol, int64_t off_body, double dt, double* pose, hipStream_t s) {
+    hipLaunchKernelGGL(body_advance_kernel, dim3(1), dim3(64), 0, s, sol, off_body, dt, pose);
     return hipGetLastError();
 }
 

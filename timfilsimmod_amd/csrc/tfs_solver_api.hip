@@ -914,6 +914,10 @@ struct tfs_beat {
     // Velocities of the last solution's forces (tfs_beat_velocities), made on
     // first use.
     double* vel = nullptr;
+    // Swimming body (solver.free_bodies): pose (X, q) and body-frame
+    // references of the filament bases, frames and blobs.
+    bool free_body = false;
+    double *pose = nullptr, *base_ref = nullptr, *frame_ref = nullptr, *blob_ref = nullptr;
 };
 
 extern "C" {
@@ -928,8 +932,7 @@ int tfs_beat_create(const tfs_beat_desc* desc, tfs_beat** out) {
     if (!D.frames || !D.bases || !D.omega0 || !D.phase0 || !D.beat_A || !D.beat_B)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_create: missing geometry or beat tables");
     if (D.solver.num_blobs > 0 && !D.x_blobs) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_create: missing blobs");
-    if (D.solver.free_bodies || D.solver.num_swim > 1)
-        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_create: one body held still (free bodies: tfs_solver_*)");
+    if (D.solver.num_swim > 1) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_create: one body");
     if (D.solver.dynamic_phase && (!D.gen_phase_forces || D.n_gen_phase < 2))
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_create: missing generalised phase forces");
     if (D.solver.dynamic_angle && (!D.gen_angle_forces || D.n_gen_angle < 2))
@@ -949,6 +952,12 @@ int tfs_beat_create(const tfs_beat_desc* desc, tfs_beat** out) {
         {&b->frames, 9LL * nf}, {&b->bases, 3LL * nf}, {&b->s_values, nseg}, {&b->phase, nf}, {&b->angle, nf},
         {&b->omega0, nf}, {&b->gen_phase, ngp}, {&b->gen_angle, nga}, {&b->body, 9}, {&b->rhs, n}, {&b->sol, n},
         {&b->hist1, n}, {&b->hist2, n}};
+    if (D.solver.free_bodies) {
+        parts.push_back({&b->pose, 8});
+        parts.push_back({&b->base_ref, 3LL * nf});
+        parts.push_back({&b->frame_ref, 9LL * nf});
+        parts.push_back({&b->blob_ref, std::max<int64_t>(3LL * D.solver.num_blobs, 1)});
+    }
     int64_t total = 0;
     for (auto& p : parts) total += ((p.second + 31) / 32) * 32;
     if (hipMalloc(&b->arena, total * 8) != hipSuccess) {
@@ -963,7 +972,9 @@ int tfs_beat_create(const tfs_beat_desc* desc, tfs_beat** out) {
     std::vector<double> sv(nseg), zeros(std::max(nf, 1), 0.0), body(9, 0.0);
     for (int i = 0; i < nseg; ++i) sv[i] = D.s_values ? D.s_values[i] : double(i) / double(nseg - 1);
     for (int i = 0; i < 3; ++i) body[i] = D.body_x[i];
-    for (int i = 0; i < 6; ++i) body[3 + i] = D.body_v[i];
+    // a swimming body's velocity is solved for: no prescribed motion in the rhs
+    for (int i = 0; i < 6; ++i) body[3 + i] = D.solver.free_bodies ? 0.0 : D.body_v[i];
+    b->free_body = D.solver.free_bodies != 0;
     hipStream_t st = s->stream;
     hipError_t e = hipMemsetAsync(b->arena, 0, total * 8, st);
     auto up = [&](double* dst, const double* src, int64_t cnt) {
@@ -980,6 +991,22 @@ int tfs_beat_create(const tfs_beat_desc* desc, tfs_beat** out) {
     up(b->body, body.data(), 9);
     if (e == hipSuccess && D.solver.num_blobs > 0)
         e = hipMemcpyAsync(s->rpy->x_blobs, D.x_blobs, 24LL * D.solver.num_blobs, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && b->free_body) {
+        // body-frame references: the given lab positions relative to the
+        // initial centre, initial orientation the identity
+        std::vector<double> pose = {D.body_x[0], D.body_x[1], D.body_x[2], 1.0, 0.0, 0.0, 0.0, 0.0};
+        std::vector<double> bref(3 * (size_t)nf), xref(3 * (size_t)std::max(D.solver.num_blobs, 0));
+        for (int m = 0; m < nf; ++m)
+            for (int k = 0; k < 3; ++k) bref[3 * m + k] = D.bases[3 * m + k] - D.body_x[k];
+        for (int i = 0; i < D.solver.num_blobs; ++i)
+            for (int k = 0; k < 3; ++k) xref[3 * i + k] = D.x_blobs[3 * i + k] - D.body_x[k];
+        e = hipMemcpyAsync(b->pose, pose.data(), 64, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(b->base_ref, bref.data(), 24LL * nf, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(b->frame_ref, D.frames, 72LL * nf, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && D.solver.num_blobs > 0)
+            e = hipMemcpyAsync(b->blob_ref, xref.data(), 24LL * D.solver.num_blobs, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
     invalidate_near(s->rpy);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     b->beat_A.assign(D.beat_A, D.beat_A + 24);
@@ -1010,10 +1037,20 @@ int tfs_beat_step(tfs_beat* b, tfs_solve_stats* stats) {
     // The beat coefficients live in constant memory shared by every handle.
     int rc = check(k_beat_coeffs(b->beat_A.data(), b->beat_B.data()), "beat coefficients");
     if (rc) return rc;
-    // filament::initial_guess: explicit phase/angle update from the last solve.
+    // filament::initial_guess: explicit phase/angle update from the last solve
+    // (and, for a swimming body, its position and orientation).
     if (b->step > 0) {
         rc = check(k_phase_update(b->sol, s->dims, D.dt, b->phase, b->angle, st), "phase update");
+        if (!rc && b->free_body) rc = check(k_body_advance(b->sol, s->dims.off_body, D.dt, b->pose, st), "body update");
         if (rc) return rc;
+    }
+    if (b->free_body) {
+        rc = check(k_body_pose(b->pose, b->base_ref, b->frame_ref, b->blob_ref, D.solver.num_fil, D.solver.num_blobs,
+                               b->bases, b->frames, s->rpy->x_blobs, b->body, st),
+                   "body pose");
+        if (!rc) rc = tfs_solver_set_bodies(s, b->body, TFS_MEMORY_DEVICE);
+        if (rc) return rc;
+        s->blob_pc_valid = false;
     }
     BeatArgs ba;
     ba.nfil = D.solver.num_fil;
@@ -1119,6 +1156,26 @@ int tfs_beat_velocities(tfs_beat* b, double* v_segs, double* v_blobs) {
     for (int64_t i = 0; i < ns; ++i)
         for (int c = 0; c < 6; ++c) v_segs[6 * i + c] = c < 3 ? v[3 * i + c] : 0.0;
     if (nb > 0) std::copy(v.begin() + 3 * ns, v.end(), v_blobs);
+    return TFS_OK;
+}
+
+int tfs_beat_body(tfs_beat* b, double* x, double* q, double* v) {
+    if (!b) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_beat_body: null handle");
+    tfs_solver* s = b->solver;
+    double pose[8] = {b->desc.body_x[0], b->desc.body_x[1], b->desc.body_x[2], 1.0, 0.0, 0.0, 0.0, 0.0};
+    double vel[6] = {0, 0, 0, 0, 0, 0};
+    hipError_t e = hipSuccess;
+    if (b->free_body) {
+        e = hipMemcpyAsync(pose, b->pose, 64, hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(vel, b->sol + s->dims.off_body, 48, hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    } else {
+        for (int k = 0; k < 6; ++k) vel[k] = b->desc.body_v[k];
+    }
+    if (e != hipSuccess) return hip_fail(e, "tfs_beat_body");
+    if (x) std::copy(pose, pose + 3, x);
+    if (q) std::copy(pose + 3, pose + 7, q);
+    if (v) std::copy(vel, vel + 6, v);
     return TFS_OK;
 }
 

@@ -81,6 +81,7 @@ L.extra_signatures({
     "tfs_beat_set_warm_start": (ctypes.c_int, [_dp, _i32]),
     "tfs_beat_restore": (ctypes.c_int, [_dp, _dp, _dp, _dp]),
     "tfs_solver_set_bodies": (ctypes.c_int, [_dp, _dp, _i32]),
+    "tfs_beat_body": (ctypes.c_int, [_dp, _dp, _dp, _dp]),
     "tfs_velocities_set_preconditioner": (ctypes.c_int, [_dp, _i32]),
     "tfs_beat_velocities": (ctypes.c_int, [_dp, _dp, _dp]),
     "tfs_solver_set_preconditioner": (ctypes.c_int, [_dp, _i32]),
@@ -281,14 +282,15 @@ class BeatSimulation:
     def __init__(self, body, *, mode=SOLVE_FP64, dt=1.0 / 300.0, restart=50, tol=1e-8, inner_tol=1e-3,
                  max_cycles=20, max_refinements=10, dynamic_phase=True, dynamic_angle=True,
                  gen_force_factor=1.0, torsion_factor=0.046, stream=None, setup: BeatSetup | None = None,
-                 warm_start: int = 0, precond: int = PRECOND_REFERENCE):
+                 warm_start: int = 0, precond: int = PRECOND_REFERENCE, free_body: bool = False):
         self.setup = setup or beat_setup(body)
         su = self.setup
         d = BeatDesc()
         d.solver = solver_desc(body.num_fil, body.num_seg, body.num_blobs, dynamic_phase=dynamic_phase,
                                dynamic_angle=dynamic_angle, mode=mode, seg_radius=body.a,
                                blob_radius=body.a, restart=restart, max_cycles=max_cycles, tol=tol,
-                               inner_tol=inner_tol, max_refinements=max_refinements, stream=stream)
+                               inner_tol=inner_tol, max_refinements=max_refinements, stream=stream,
+                               free_bodies=free_body)
         d.dt, d.fil_length = dt, su.fil_length
         self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
                       (su.frames, su.bases, su.x_blobs, su.s_values, su.omega0, su.phase0, su.angle0,
@@ -350,6 +352,12 @@ class BeatSimulation:
         ph, an, xs, sol = np.zeros(nf), np.zeros(nf), np.zeros(3 * ns), np.zeros(self.n)
         L.check(L.lib().tfs_beat_state(self.h, L.ptr(ph), L.ptr(an), L.ptr(xs), L.ptr(sol)))
         return {"phase": ph, "angle": an, "x_segs": xs, "solution": sol}
+
+    def body(self):
+        """(centre [3], orientation quaternion [4] (w, x, y, z), velocity [6])."""
+        x, q, v = np.zeros(3), np.zeros(4), np.zeros(6)
+        L.check(L.lib().tfs_beat_body(self.h, L.ptr(x), L.ptr(q), L.ptr(v)))
+        return x, q, v
 
     def restore(self, phase, angle=None, solution=None):
         """Continue from saved state: the next step solves at these phases
