@@ -105,3 +105,31 @@ def test_broyden_and_velocities_reject_bad_descriptors():
     assert lib.tfs_velocities_create(ctypes.byref(v), ctypes.byref(h)) == L.TFS_ERR_INVALID_ARGUMENT
     assert not h.value
     assert lib.tfs_velocities_destroy(None) == L.TFS_OK
+
+
+def test_newer_entry_points_reject_null_handles_and_bad_descriptors():
+    import timfilsimmod_amd  # noqa: F401  (registers every signature)
+    from timfilsimmod_amd.filament import FilamentDesc
+    lib = L.lib()
+    assert lib.tfs_rpy_barrier_forces(None, None, None, None) == L.TFS_ERR_INVALID_ARGUMENT
+    assert lib.tfs_solver_set_bodies(None, None, 0) == L.TFS_ERR_INVALID_ARGUMENT
+    assert lib.tfs_beat_body(None, None, None, None) == L.TFS_ERR_INVALID_ARGUMENT
+    assert lib.tfs_beat_restore(None, None, None, None) == L.TFS_ERR_INVALID_ARGUMENT
+    assert lib.tfs_beat_velocities(None, None, None) == L.TFS_ERR_INVALID_ARGUMENT
+    assert lib.tfs_velocities_set_preconditioner(None, 2) == L.TFS_ERR_INVALID_ARGUMENT
+    for fn, args in (("tfs_filament_initial_guess", (None, 0)), ("tfs_filament_forces", (None, 0, None)),
+                     ("tfs_filament_jacobian", (None, 0, None, None)), ("tfs_filament_update", (None, None)),
+                     ("tfs_filament_sync", (None,)), ("tfs_filament_read", (None, 0, None)),
+                     ("tfs_elastic_step", (None, 0, None)), ("tfs_io_read_record", (None, 1, 1, None, None)),
+                     ("tfs_io_peek_count", (None, None)), ("tfs_io_rewind", (None,))):
+        assert getattr(lib, fn)(*args) == L.TFS_ERR_INVALID_ARGUMENT, fn
+    for fn in ("tfs_filament_destroy", "tfs_elastic_destroy", "tfs_io_close"):
+        assert getattr(lib, fn)(None) == L.TFS_OK, fn
+    h = ctypes.c_void_p()
+    d = FilamentDesc()
+    d.num_swim, d.fil_per_swim, d.num_seg = 1, 2, 1          # a filament needs two segments
+    d.dl = d.mu = d.radius = d.dt = 1.0
+    assert lib.tfs_filament_create(ctypes.byref(d), ctypes.byref(h)) == L.TFS_ERR_INVALID_ARGUMENT
+    d.num_seg, d.dt = 4, 0.0
+    assert lib.tfs_filament_create(ctypes.byref(d), ctypes.byref(h)) == L.TFS_ERR_INVALID_ARGUMENT
+    assert not h.value
