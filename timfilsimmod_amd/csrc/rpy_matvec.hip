@@ -367,31 +367,49 @@ __global__ __launch_bounds__(256) void grid_count_kernel(MatvecArgs a, int n, do
 }
 
 // start = exclusive prefix sum of fill (table entries, a power of two >= 1024),
-// start[table] = total; fill is cleared for the scatter.  One workgroup.
+// start[table] = total; fill is cleared for the scatter.  One workgroup, in
+// chunks of 16384 entries staged through LDS with coalesced loads and stores:
+// each thread scans 16 consecutive entries, the 1024 thread sums are scanned
+// in LDS, and a running carry joins the chunks.
 __global__ __launch_bounds__(1024) void grid_scan_kernel(int32_t* __restrict__ fill, int32_t* __restrict__ start,
                                                         int table) {
+    constexpr int kChunk = 16384, kPer = kChunk / 1024;
+    __shared__ int32_t buf[kChunk];
     __shared__ int32_t part[1024];
     const int tid = threadIdx.x;
-    const int per = table / 1024;
-    const int b0 = tid * per;
-    int32_t sum = 0;
-    for (int i = 0; i < per; ++i) sum += fill[b0 + i];
-    part[tid] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int32_t v = tid >= off ? part[tid - off] : 0;
+    int32_t carry = 0;
+    for (int c0 = 0; c0 < table; c0 += kChunk) {
+        const int len = table - c0 < kChunk ? table - c0 : kChunk;   // table >= 1024: len is a multiple of 1024
+        for (int i = tid; i < len; i += 1024) buf[i] = fill[c0 + i];
         __syncthreads();
-        part[tid] += v;
+        const int per = len / 1024;
+        int32_t sum = 0;
+        for (int i = 0; i < per; ++i) sum += buf[tid * per + i];
+        part[tid] = sum;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int32_t v = tid >= off ? part[tid - off] : 0;
+            __syncthreads();
+            part[tid] += v;
+            __syncthreads();
+        }
+        int32_t run = carry + part[tid] - sum;  // exclusive prefix of this thread's entries
+        for (int i = 0; i < per; ++i) {
+            const int32_t cnt = buf[tid * per + i];
+            buf[tid * per + i] = run;
+            run += cnt;
+        }
+        const int32_t total = part[1023];
+        __syncthreads();
+        for (int i = tid; i < len; i += 1024) {
+            start[c0 + i] = buf[i];
+            fill[c0 + i] = 0;
+        }
+        carry += total;
         __syncthreads();
     }
-    int32_t run = part[tid] - sum;  // exclusive prefix of this thread's chunk
-    for (int i = 0; i < per; ++i) {
-        const int32_t c = fill[b0 + i];
-        start[b0 + i] = run;
-        fill[b0 + i] = 0;
-        run += c;
-    }
-    if (tid == 1023) start[table] = part[1023];
+    if (tid == 0) start[table] = carry;
+    (void)kPer;
 }
 
 __global__ __launch_bounds__(256) void grid_scatter_kernel(int n, const int32_t* __restrict__ bucket_of,
