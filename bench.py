@@ -265,6 +265,64 @@ def elastic_leg(torch, steps: int = 3):
             "ms_per_step_mean": float(np.mean([p["ms"] for p in per]))}
 
 
+def elastic_cpu_baseline():
+    """The first elastic step of the small body on the host cores: the same
+    loop on the FP64 oracle (C block sums with OpenMP, an LU of the blob
+    mobility factored once, the numpy Cosserat restatement, the numpy Broyden
+    restatement).  Reported, not gating."""
+    import scipy.linalg as sla
+    from scipy.spatial.transform import Rotation
+
+    from oracle import broyden_oracle as BO
+    from oracle import filament_oracle as F
+    from oracle import oracle as O
+    from timfilsimmod_amd import geometry as G
+    body = G.small_body()
+    nfil, nseg, nb = body.num_fil, body.num_seg, body.num_blobs
+    dl, kb, dt, tol = 2.2, 1800.0, 0.1, 1e-8
+    efm = 220.0 * kb / (dl * dl * nseg * nseg)
+    p = F.Params(nseg=nseg, dl=dl, kb=kb, kt=kb, dt=dt, end_force=efm)
+    ex = np.array([1.0, 0.0, 0.0])
+    q0 = np.array([np.roll(Rotation.align_vectors([n], [ex])[0].as_quat(), 1) for n in body.fil_normal])
+    pert = np.random.default_rng(0).standard_normal((nfil, 6)) * 0.1 * efm
+    fils = [F.Filament.at_rest(body.fil_base[i], np.tile(q0[i], (nseg, 1)), nseg, dl) for i in range(nfil)]
+    t0 = time.perf_counter()
+    lu = sla.lu_factor(O.dense_mobility(np.zeros(0), body.x_blobs))
+    z3, z6 = np.zeros(3), np.zeros(6)
+    cat = lambda name: np.concatenate([getattr(f, name).reshape(-1) for f in fils])
+
+    def evaluate():
+        f = np.concatenate([F.internal_forces(fl, p, {4: pert[i, :3], 12: pert[i, 3:]}).reshape(-1)
+                            for i, fl in enumerate(fils)])
+        x = cat("x")
+        vb = O.block("bs", x, body.x_blobs, f, np.zeros(3 * nb), seg_dof=6, bs_accumulate=False)
+        lam = sla.lu_solve(lu, -vb)
+        v = O.block("sb", x, body.x_blobs, f, lam, O.block("ss", x, body.x_blobs, f, lam, seg_dof=6), seg_dof=6)
+        st = {"seg_x": x, "seg_xm1": cat("xm1"), "seg_xm2": cat("xm2"), "seg_u": cat("u"), "seg_um1": cat("um1"),
+              "v_segs": v, "body_x": z3, "body_xm1": z3, "body_xm2": z3, "body_u": z3, "body_um1": z3,
+              "v_bodies": z6}
+        e, big = BO.compute_errors(st, 1, nfil, nseg, False, dt, dl, tol)
+        return f, e, big
+
+    for fl in fils:
+        F.initial_guess(fl, 0, p)
+    f, e, big = evaluate()
+    inv = [np.linalg.inv(F.approx_jacobian(fl, f.reshape(nfil, nseg, 6)[i], False, p))
+           for i, fl in enumerate(fils)] + [np.eye(6)]
+    ref = BO.BroydenRef(6 * (nfil * nseg + 1), 400, 0.1, inv)
+    ref.start(e)
+    while big and ref.iter < 400:
+        du = ref.find_update()
+        for i, fl in enumerate(fils):
+            F.update(fl, du[6 * nseg * i:6 * nseg * (i + 1)], p)
+        f, e, big = evaluate()
+        ref.end_of_iter(e, big)
+    secs = time.perf_counter() - t0
+    return {"value": 1.0 / secs, "unit": "timesteps/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"the first elastic step (nt = 0, {ref.iter} Broyden iterations) of the same workload on "
+                      f"the oracle: {secs:.2f} s including the blob mobility's LU"}
+
+
 def copy_leg(torch, nbytes: int = 256 << 20, reps: int = 3):
     """A plain device-to-device copy of `nbytes` (torch's copy kernel), `reps`
     times: the reference point that scales the FETCH_SIZE / WRITE_SIZE
@@ -450,6 +508,9 @@ def main():
     if "elastic" in legs:
         print("bench: elastic-step leg", file=sys.stderr, flush=True)
         elastic = elastic_leg(torch)
+        if not args.no_cpu_baseline:
+            print("bench: elastic-step CPU baseline", file=sys.stderr, flush=True)
+            elastic["cpu_baseline"] = elastic_cpu_baseline()
 
     cpu = None
     if not args.no_cpu_baseline and fp64 is not None:
