@@ -57,6 +57,15 @@ int tfs_broyden_set_jacobian_inverse(tfs_broyden *b, const double *blocks);
  * per-filament LAPACK getrf/getri of filament::invert_approx_jacobian,
  * filament.cpp:1761-1894).  TFS_ERR_NON_FINITE if a block is singular. */
 int tfs_broyden_set_jacobian(tfs_broyden *b, const double *blocks);
+/* Free bodies (swimmer::jacobian_inv_mult, swimmer.cpp:1012-1066): after
+ * tfs_broyden_set_jacobian, couples each swimmer's filament blocks J_i with
+ * its 6 x 6 body block through the body's Schur complement, so that J0^{-1} is
+ * the inverse of [[diag J_i, A], [B, S0]].  The blocks must be, per swimmer,
+ * fil_per_swim filament blocks of one size and then one block of 6.  A, B, S0
+ * as tfs_filament_body_coupling writes them (desc's memory space).  A later
+ * tfs_broyden_set_jacobian or _inverse removes the coupling. */
+int tfs_broyden_set_body_coupling(tfs_broyden *b, int32_t num_swim, int32_t fil_per_swim, const double *A,
+                                  const double *B, const double *S0);
 /* Start a timestep: iter = 0 and error = the residual of the initial guess
  * (compute_errors before the loop, cilia_sim_main.cu:394-411).  error == NULL
  * keeps the residual that tfs_broyden_compute_errors left in slot 0. */

@@ -76,6 +76,35 @@ int tfs_filament_jacobian(tfs_filament *h, int32_t nt, const double *f_segs, dou
  * robot arm. */
 int tfs_filament_update(tfs_filament *h, const double *update);
 
+/* ---- Free bodies (PRESCRIBED_BODY_VELOCITIES false) ----
+ * Makes each swimmer's body a state of its own: position, Lie-algebra element
+ * and orientation (rigid_body.cpp:395-447, 506-535).  Filament f is then
+ * anchored at X + R(q) fil_refs[f] and its base segment turns with the body
+ * (swimmer::initial_guess, swimmer::update and
+ * filament::accept_state_from_rigid_body, swimmer.cpp:433-467, 1072-1098,
+ * filament.cpp:649-672): tfs_filament_initial_guess advances the body first,
+ * tfs_filament_update applies the body's 6 entries of the update first.
+ * x_bodies [3], q_bodies [4] per swimmer (the starting pose, at rest);
+ * fil_refs [3 per filament] in the body frame; blob_refs [3 per blob],
+ * blobs_per_swim per swimmer, for tfs_filament_blob_positions. */
+int tfs_filament_set_body(tfs_filament *h, const double *x_bodies, const double *q_bodies, const double *fil_refs,
+                          int32_t blobs_per_swim, const double *blob_refs);
+/* rigid_body::blob_positions (rigid_body.cpp:482-500) at the current pose:
+ * x_blobs [3 per blob], swimmer by swimmer. */
+int tfs_filament_blob_positions(tfs_filament *h, double *x_blobs);
+/* The body's part of J0 (swimmer::prepare_jacobian_inv, GMRES branch with a
+ * free body, swimmer.cpp:688-880), after tfs_filament_jacobian at the same
+ * state: per filament A (6 NSEG x 6, its rows and the body's columns) and B
+ * (6 x 6 NSEG, the body's rows and its columns), row-major, swimmer by
+ * swimmer; per swimmer S0 (6 x 6), so that J0 = [[diag J_i, A], [B, S0]] (see
+ * tfs_broyden_set_body_coupling).  body_mobility: per swimmer the 6 x 6 body
+ * mobility in the body frame (make_body_reference_matrices,
+ * mobility_solver.cpp:2950-3047); sphere_radius > 0 adds the RPY couplings of
+ * the segments with the body as a sphere of that radius (the reference does
+ * so for spherical bodies, swimmer.cpp:818-843); 0 leaves them out. */
+int tfs_filament_body_coupling(tfs_filament *h, int32_t nt, const double *body_mobility, double sphere_radius,
+                               double *A, double *B, double *S0);
+
 /* Waits for the handle's work (device-memory outputs are then complete). */
 int tfs_filament_sync(tfs_filament *h);
 
@@ -88,17 +117,31 @@ int tfs_filament_sync(tfs_filament *h);
 #define TFS_FIL_UM1 4
 #define TFS_FIL_Q 5     /* 4 per segment                   */
 #define TFS_FIL_MULT 6  /* 3 (NSEG+1) per filament          */
+/* free bodies only (tfs_filament_set_body), per swimmer:          */
+#define TFS_FIL_BODY_X 7    /* 3, position at n+1                 */
+#define TFS_FIL_BODY_XM1 8  /* at n                               */
+#define TFS_FIL_BODY_XM2 9  /* at n-1                             */
+#define TFS_FIL_BODY_U 10   /* 3, Lie-algebra element at n+1      */
+#define TFS_FIL_BODY_UM1 11
+#define TFS_FIL_BODY_Q 12   /* 4, orientation                     */
 int tfs_filament_device_array(tfs_filament *h, int32_t which, double **ptr);
 /* Host copy of one of those arrays (synchronises). */
 int tfs_filament_read(tfs_filament *h, int32_t which, double *out);
 
 /* ---- One elastic implicit timestep on the device ----
  * The time loop body of cilia_sim_main.cu:353-502 for filaments on bodies
- * held still (PRESCRIBED_BODY_VELOCITIES, GMRES for the blob forces):
+ * held still (PRESCRIBED_BODY_VELOCITIES, GMRES for the blob forces) or, when
+ * the filament handle has free bodies (tfs_filament_set_body), on free bodies:
  *   initial_guess; forces; velocities; errors;
  *   if too large: J0 = approximate Jacobian (body blocks I), inverted on the device;
  *   Broyden: update -> forces -> velocities -> errors -> end_of_iter, until the
  *   residual meets TOL or max_iter iterations.
+ * Free bodies: the velocities handle is created with free_bodies; blob
+ * positions follow the bodies (x_blobs, x_bodies and v_bodies of the
+ * descriptor are not used), the blob forces and body velocities come from
+ * tfs_velocities_compute_free, the residual has the body rows, and J0^{-1}
+ * takes the body's Schur complement (tfs_filament_body_coupling,
+ * tfs_broyden_set_body_coupling) with body_mobility and sphere_radius below.
  * The three handles are the caller's, created with TFS_MEMORY_DEVICE and sizes
  * that agree (tfs_velocities: num_swim bodies of fil_per_swim*num_seg segments;
  * tfs_broyden: the block sizes of the Broyden layout above).  Only the
@@ -115,6 +158,8 @@ typedef struct tfs_elastic_desc {
     double *f_blobs;           /* device, 3 per blob: warm start in, blob forces out  */
     double tol;                /* TOL (config.hpp:302)                                 */
     int32_t max_iter;          /* MAX_BROYDEN_ITER                                     */
+    const double *body_mobility; /* free bodies: device, 36 per swimmer (body frame)   */
+    double sphere_radius;      /* free bodies: see tfs_filament_body_coupling          */
 } tfs_elastic_desc;
 
 typedef struct tfs_elastic_stats {

@@ -220,6 +220,7 @@ typedef struct tfs_velocities_desc {
     int32_t max_refinements;   /* mixed mode                                                */
     int32_t memory;            /* TFS_MEMORY_* of the arrays passed to tfs_velocities_compute */
     void *stream;              /* hipStream_t or NULL                                        */
+    int32_t free_bodies;       /* 1: bodies free, use tfs_velocities_compute_free           */
 } tfs_velocities_desc;
 
 typedef struct tfs_velocities tfs_velocities;
@@ -245,6 +246,17 @@ int tfs_velocities_set_preconditioner(tfs_velocities *v, int32_t kind);
 int tfs_velocities_compute(tfs_velocities *v, const double *x_segs, const double *f_segs,
                            const double *x_blobs, const double *x_bodies, const double *v_bodies,
                            double *v_segs, double *f_blobs, tfs_solve_stats *stats);
+/* Free bodies (a handle created with free_bodies; GMRES blob solves only):
+ * the blob forces and each body's velocity V, Omega solve the blob-only
+ * system with body rows (mobility_solver.cpp:544-560 and the operator of
+ * :1806-2090): M_bb f_b - K V = -v_blobs, -K^T f_b = -(f_ext - sum over the
+ * body's segments of (F, T + (x - X) x F)), the force- and torque-free
+ * condition (swimmer.cpp:469-533).  f_bodies: external force and torque per
+ * body (6 each) or NULL for none; v_bodies (6 per body) holds the warm start
+ * on entry and the solved velocities on return; the rest as above. */
+int tfs_velocities_compute_free(tfs_velocities *v, const double *x_segs, const double *f_segs,
+                                const double *x_blobs, const double *x_bodies, const double *f_bodies,
+                                double *v_segs, double *f_blobs, double *v_bodies, tfs_solve_stats *stats);
 
 #ifdef __cplusplus
 }

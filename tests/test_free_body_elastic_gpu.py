@@ -1,0 +1,245 @@
+"""Free bodies in the elastic implicit timestep on the GPU against the oracle's
+free swimmer (oracle/swimmer_oracle.py, pinned on the CPU):
+
+* the blob-only solve with body rows (tfs_velocities_compute_free): segment
+  velocities, blob forces and body velocities against the dense solve, with
+  and without an external force on the body; the body mobility from six such
+  solves against (K^T M_bb^{-1} K)^{-1};
+* the swimmer's state kernels (body initial guess and update, anchored
+  filaments, blob positions) and the body's part of J0 (A_i, B_i, S0 with and
+  without the sphere correction) against the restatement, <= 1e-11 relative;
+* the Broyden engine's coupled J0^{-1} (tfs_broyden_set_body_coupling)
+  against the inverse of the assembled block Jacobian;
+* two elastic implicit timesteps of a free swimmer (4 filaments x 20
+  segments on 400 blobs) through tfs_elastic_step against the oracle's loop:
+  the same Broyden iteration counts (within 10 %) and states within 1e-6."""
+import numpy as np
+import pytest
+import torch
+from scipy.spatial.transform import Rotation
+
+from oracle import filament_oracle as F
+from oracle import oracle as O
+from oracle import swimmer_oracle as S
+from timfilsimmod_amd import _lib as L
+from timfilsimmod_amd import filament as FM
+from timfilsimmod_amd import geometry as G
+from timfilsimmod_amd.broyden import BroydenSolver
+from timfilsimmod_amd.solver import SOLVE_FP64, ElasticVelocities
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("external", [False, True])
+def test_free_blob_solve_matches_dense_oracle(external):
+    print(f"case free blob solve 3x6 segments + 150 blobs, external force {external}", flush=True)
+    body = G.make_body(3, 6, 150, 6.0)
+    X = np.array([0.3, -0.2, 0.1])
+    fe = np.array([1.0, -2.0, 0.5, 3.0, 0.2, -1.0]) if external else None
+    ev = ElasticVelocities(1, 18, 150, mode=SOLVE_FP64, restart=60, max_cycles=100, tol=1e-12, free_bodies=True)
+    try:
+        vs, fb, vb, st = ev.compute_free(body.x_segs, body.f_segs, body.x_blobs, X, fe)
+    finally:
+        ev.close()
+    xs, xb, fs = body.x_segs, body.x_blobs, body.f_segs
+    fils = [F.Filament.at_rest(np.zeros(3), np.tile([1.0, 0, 0, 0], (6, 1)), 6, 2.2) for _ in range(3)]
+    for i, f in enumerate(fils):
+        f.x = xs.reshape(3, 6, 3)[i]
+    sf = S.body_force(S.Body.at_rest(X), fils, [fs.reshape(3, 6, 6)[i] for i in range(3)])
+    if external:
+        sf = sf + fe
+    v_b = O.block("bs", xs, xb, fs, np.zeros(xb.size), seg_dof=6, bs_accumulate=False)
+    lam, V = S.free_blob_solve(O.dense_mobility(np.zeros(0), xb), xb, X, v_b, sf)
+    v_s = O.block("sb", xs, xb, fs, lam, O.block("ss", xs, xb, fs, lam, seg_dof=6), seg_dof=6)
+    print("rel err f_blobs", rel(fb, lam), "V", rel(vb, V), "v_segs", rel(vs, v_s), st, flush=True)
+    assert st["converged"]
+    # GMRES stops at a relative residual of 1e-12; the blob forces carry it
+    # times the condition number of the blob mobility (overlapping blobs,
+    # ~1e3 here), the velocities do not
+    assert rel(fb, lam) < 1e-8 and rel(vb, V) < 1e-10 and rel(vs, v_s) < 1e-10
+
+
+def test_body_mobility_matches_oracle():
+    print("case body mobility, 300 blobs", flush=True)
+    refs = G.fibonacci_sphere(300, 8.0)
+    X = np.array([1.0, 2.0, -0.5])
+    ev = ElasticVelocities(1, 2, 300, mode=SOLVE_FP64, restart=60, max_cycles=100, tol=1e-12, free_bodies=True)
+    try:
+        N = ev.body_mobility((refs + X).reshape(-1), X)[0]
+    finally:
+        ev.close()
+    ref = S.body_mobility_reference(O.dense_mobility(np.zeros(0), refs.reshape(-1)), refs)
+    print("rel err", rel(N, ref), flush=True)
+    assert rel(N, ref) < 1e-9
+
+
+def swimmer_pair(nfil=3, nseg=8, nb=40, seed=0, memory=L.MEMORY_HOST):
+    """The same free swimmer on the device and on the oracle."""
+    rng = np.random.default_rng(seed)
+    p = F.Params(nseg=nseg, dl=2.2, kb=1800.0, kt=1300.0, dt=0.2, end_force=150.0)
+    xb0 = np.array([0.5, -1.0, 2.0])
+    qb0 = np.roll(Rotation.random(random_state=seed).as_quat(), 1)
+    refs = [7.0 * v / np.linalg.norm(v) for v in rng.standard_normal((nfil, 3))]
+    blob_refs = G.fibonacci_sphere(nb, 5.0)
+    qs = []
+    for i in range(nfil):
+        q = [np.roll(Rotation.random(random_state=seed + 20 + i).as_quat(), 1)]
+        for _ in range(nseg - 1):
+            q.append(F.qmul(F.lie_exp(0.1 * rng.standard_normal(3)), q[-1]))
+        qs.append(np.array(q))
+    R0 = F.rot_mat(qb0)
+    bases = np.array([xb0 + R0 @ r for r in refs])
+    model = FM.FilamentModel(1, nfil, nseg, dl=p.dl, kb=p.kb, kt=p.kt, dt=p.dt, end_force=p.end_force)
+    model.set_state(bases.reshape(-1), np.array(qs).reshape(-1))
+    model.set_body(xb0, qb0, np.array(refs).reshape(-1), nb, blob_refs.reshape(-1))
+    body = S.Body.at_rest(xb0, qb0)
+    fils = [F.Filament.at_rest(bases[i], qs[i], nseg, p.dl) for i in range(nfil)]
+    return p, model, body, fils, refs, blob_refs, rng
+
+
+@pytest.mark.parametrize("sphere", [0.0, 5.0])
+def test_swimmer_state_and_body_coupling_match_oracle(sphere):
+    print(f"case swimmer state and J0 coupling 3x8, sphere {sphere}", flush=True)
+    p, model, body, fils, refs, blob_refs, rng = swimmer_pair(seed=3)
+    nfil, N = len(fils), p.nseg
+    n = 6 * (nfil * N + 1)
+    N_ref = np.diag([0.02, 0.021, 0.019, 0.001, 0.0011, 0.0009]) + 1e-4 * rng.standard_normal((6, 6))
+    try:
+        for nt in range(3):
+            model.initial_guess(nt)
+            S.swimmer_initial_guess(body, fils, refs, nt, p)
+            du = rng.standard_normal(n) * 0.02
+            du[-6:] *= 0.5
+            model.update(du)
+            S.swimmer_update(body, fils, refs, du, p)
+            assert rel(model.read(FM.X), np.concatenate([f.x.reshape(-1) for f in fils])) < 1e-12
+            assert rel(model.read(FM.Q), np.concatenate([f.q.reshape(-1) for f in fils])) < 1e-12
+            assert rel(model.read(FM.BODY_X), body.x) < 1e-14 and rel(model.read(FM.BODY_Q), body.q) < 1e-14
+            assert rel(model.read(FM.BODY_U), body.u) < 1e-14
+            assert rel(model.blob_positions(), S.blob_positions(body, blob_refs)) < 1e-13
+        nt = 2
+        f = model.forces(nt)
+        forces = [F.internal_forces(fl, p) for fl in fils]
+        assert rel(f, np.concatenate([fc.reshape(-1) for fc in forces])) < 1e-12
+        blocks = np.zeros(nfil * (6 * N) ** 2 + 36)
+        model.jacobian(nt, f, blocks)
+        A, B, S0 = model.body_coupling(nt, N_ref.reshape(-1), sphere)
+    finally:
+        model.close()
+    S0r, As, Bs = S.body_coupling(body, fils, refs, forces, nt, N_ref, p, sphere)
+    n6 = 6 * N
+    print("rel err A", rel(A, np.concatenate([a.reshape(-1) for a in As])), "B",
+          rel(B, np.concatenate([b.reshape(-1) for b in Bs])), "S0", rel(S0.reshape(6, 6), S0r), flush=True)
+    assert rel(A, np.concatenate([a.reshape(-1) for a in As])) < 1e-11
+    assert rel(B, np.concatenate([b.reshape(-1) for b in Bs])) < 1e-11
+    assert rel(S0.reshape(6, 6), S0r) < 1e-11
+    assert A.size == nfil * n6 * 6
+
+
+def test_coupled_broyden_inverse_matches_assembled_jacobian():
+    print("case coupled J0^{-1}, 2 swimmers x 3 filaments x 6 segments", flush=True)
+    rng = np.random.default_rng(4)
+    nsw, nfil, N = 2, 3, 6
+    n6 = 6 * N
+    Js, As, Bs, S0s = [], [], [], []
+    for s in range(nsw):
+        Js.append([np.eye(n6) * 3 + rng.standard_normal((n6, n6)) * 0.2 for _ in range(nfil)])
+        As.append([rng.standard_normal((n6, 6)) * 0.3 for _ in range(nfil)])
+        Bs.append([rng.standard_normal((6, n6)) * 0.3 for _ in range(nfil)])
+        S0s.append(np.eye(6) * 4 + rng.standard_normal((6, 6)) * 0.2)
+    sizes = ([n6] * nfil + [6]) * nsw
+    eng = BroydenSolver(sizes, confidence=0.1)
+    try:
+        eng.set_jacobian([b for s in range(nsw) for b in Js[s] + [np.eye(6)]])
+        eng.set_body_coupling(nsw, nfil, np.concatenate([a.reshape(-1) for s in range(nsw) for a in As[s]]),
+                              np.concatenate([b.reshape(-1) for s in range(nsw) for b in Bs[s]]),
+                              np.concatenate([m.reshape(-1) for m in S0s]))
+        e = rng.standard_normal(sum(sizes))
+        eng.start(e)
+        up = eng.find_update()
+    finally:
+        eng.close()
+    want = np.concatenate([-0.1 * np.linalg.solve(S.assembled_jacobian(Js[s], S0s[s], As[s], Bs[s]),
+                                                  e[s * (nfil * n6 + 6):(s + 1) * (nfil * n6 + 6)])
+                           for s in range(nsw)])
+    print("rel err", rel(up, want), flush=True)
+    assert rel(up, want) < 1e-12
+
+
+STEPS = 2
+
+
+@pytest.fixture(scope="module")
+def free_case():
+    nfil, nseg, nb, R = 4, 20, 400, 10.0
+    body = G.make_body(nfil, nseg, nb, R)
+    dl, kb = 2.2, 1800.0
+    efm = 220.0 * kb / (dl * dl * nseg * nseg)
+    p = F.Params(nseg=nseg, dl=dl, kb=kb, kt=kb, dt=0.1, end_force=efm)
+    ex = np.array([1.0, 0.0, 0.0])
+    q0 = np.array([np.roll(Rotation.align_vectors([nrm], [ex])[0].as_quat(), 1) for nrm in body.fil_normal])
+    pert = np.random.default_rng(0).standard_normal((nfil, 6)) * 0.1 * efm
+    refs = [body.fil_base[i] - body.centre for i in range(nfil)]
+    N_ref = S.body_mobility_reference(O.dense_mobility(np.zeros(0), body.x_blobs), body.blob_refs)
+    b = S.Body.at_rest(body.centre)
+    fils = [F.Filament.at_rest(body.fil_base[i], np.tile(q0[i], (nseg, 1)), nseg, dl) for i in range(nfil)]
+    runs = S.free_swimmer_loop(b, fils, refs, body.blob_refs, p, STEPS, 1e-8, pert, N_ref, sphere_radius=R)
+    return dict(body=body, p=p, pert=pert, refs=refs, N_ref=N_ref, q0=q0, R=R, runs=runs)
+
+
+def test_free_swimmer_elastic_step_matches_oracle_loop(free_case):
+    print("case free swimmer elastic step 4x20 + 400 blobs, tfs_elastic_step", flush=True)
+    c = free_case
+    body, p, nb = c["body"], c["p"], c["body"].num_blobs
+    nfil, nseg = body.num_fil, p.nseg
+    dev = lambda a: torch.tensor(np.asarray(a, dtype=np.float64), device="cuda")
+    # the body mobility on the device (six free-body solves) agrees with the oracle's
+    evh = ElasticVelocities(1, nfil * nseg, nb, mode=SOLVE_FP64, restart=60, max_cycles=100, tol=1e-13,
+                            free_bodies=True)
+    try:
+        N_dev = evh.body_mobility(body.x_blobs, body.centre)[0]
+    finally:
+        evh.close()
+    print("body mobility rel err", rel(N_dev, c["N_ref"]), flush=True)
+    assert rel(N_dev, c["N_ref"]) < 1e-9
+    model = FM.FilamentModel(1, nfil, nseg, dl=p.dl, kb=p.kb, kt=p.kt, dt=p.dt, end_force=p.end_force,
+                             memory=L.MEMORY_DEVICE)
+    ev = ElasticVelocities(1, nfil * nseg, nb, mode=SOLVE_FP64, restart=30, max_cycles=200, tol=1e-10,
+                           memory=L.MEMORY_DEVICE, free_bodies=True)
+    eng = BroydenSolver(model.block_sizes(), memory=L.MEMORY_DEVICE)
+    f_blobs, nmob = dev(np.zeros(3 * nb)), dev(N_dev.reshape(-1))
+    step = None
+    iters, xs, bx = [], [], []
+    try:
+        model.set_state(dev(body.fil_base.reshape(-1)), dev(np.repeat(c["q0"][:, None, :], nseg, axis=1).reshape(-1)))
+        model.set_body(dev(body.centre), dev([1.0, 0.0, 0.0, 0.0]), dev(np.array(c["refs"]).reshape(-1)), nb,
+                       dev(body.blob_refs.reshape(-1)))
+        model.set_perturbation(4, 12, dev(c["pert"].reshape(-1)))
+        step = FM.ElasticStep(model, ev, eng, None, None, None, f_blobs, tol=1e-8, max_iter=400,
+                              body_mobility=nmob, sphere_radius=c["R"])
+        for nt in range(STEPS):
+            st = step.step(nt)
+            print("step", nt, st, flush=True)
+            assert st["converged"]
+            iters.append(st["broyden_iterations"])
+            xs.append(model.read(FM.X))
+            bx.append(model.read(FM.BODY_X))
+    finally:
+        if step is not None:
+            step.close()
+        model.close()
+        ev.close()
+        eng.close()
+    ref = c["runs"]
+    print("broyden iterations", iters, "oracle", [r["iters"] for r in ref],
+          "max |dx|", [float(np.abs(a - r["x"]).max()) for a, r in zip(xs, ref)],
+          "body", [float(np.abs(a - r["body"].x).max()) for a, r in zip(bx, ref)], flush=True)
+    for s in range(STEPS):
+        r = ref[s]
+        assert abs(iters[s] - r["iters"]) <= max(2, r["iters"] // 10)
+        assert np.abs(xs[s] - r["x"]).max() <= 1e-6
+        assert np.abs(bx[s] - r["body"].x).max() <= 1e-8 + 1e-3 * np.abs(r["body"].x - body.centre).max()

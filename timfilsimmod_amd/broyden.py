@@ -43,6 +43,7 @@ L.extra_signatures({
     "tfs_broyden_destroy": (ctypes.c_int, [_vp]),
     "tfs_broyden_set_jacobian_inverse": (ctypes.c_int, [_vp, _vp]),
     "tfs_broyden_set_jacobian": (ctypes.c_int, [_vp, _vp]),
+    "tfs_broyden_set_body_coupling": (ctypes.c_int, [_vp, _i32, _i32, _vp, _vp, _vp]),
     "tfs_broyden_start": (ctypes.c_int, [_vp, _vp]),
     "tfs_broyden_find_update": (ctypes.c_int, [_vp, _vp]),
     "tfs_broyden_end_of_iter": (ctypes.c_int, [_vp, _vp, _i32]),
@@ -109,6 +110,14 @@ class BroydenSolver:
         """J0 as dense blocks; inverted on the device."""
         flat = _concat(blocks, self.memory)
         L.check(L.lib().tfs_broyden_set_jacobian(self.h, L.ptr(flat)))
+
+    def set_body_coupling(self, num_swim, fil_per_swim, A, B, S0):
+        """Free bodies: J0^{-1} through each body's Schur complement
+        (swimmer::jacobian_inv_mult, swimmer.cpp:1012-1066), after set_jacobian;
+        A, B, S0 as FilamentModel.body_coupling writes them."""
+        a, b, s = _f64(A), _f64(B), _f64(S0)
+        L.check(L.lib().tfs_broyden_set_body_coupling(self.h, int(num_swim), int(fil_per_swim), L.ptr(a), L.ptr(b),
+                                                      L.ptr(s)))
 
     def start(self, error=None):
         e = _f64(error)

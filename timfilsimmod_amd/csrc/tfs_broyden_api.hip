@@ -56,6 +56,105 @@ __global__ __launch_bounds__(256) void block_transpose_kernel(const double* __re
     }
 }
 
+// Free bodies, once per J0: per filament W = J_i^{-1} A_i (6N x 6) and
+// BJ = B_i J_i^{-1} (6 x 6N), J_i^{-1} stored column-major at mat_off.  One
+// workgroup per filament.
+__global__ __launch_bounds__(256) void coupling_prep_kernel(const double* __restrict__ J,
+                                                            const int64_t* __restrict__ mat_off, int fil_per_swim,
+                                                            int n6, const double* __restrict__ A,
+                                                            const double* __restrict__ B, double* __restrict__ W,
+                                                            double* __restrict__ BJ) {
+    const int f = blockIdx.x;
+    const int s = f / fil_per_swim, i = f % fil_per_swim;
+    const double* Ji = J + mat_off[s * (fil_per_swim + 1) + i];
+    const int64_t o = (int64_t)f * 6 * n6;
+    for (int r = threadIdx.x; r < n6; r += blockDim.x) {
+        double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int c = 0; c < n6; ++c) {
+            const double jv = Ji[(int64_t)c * n6 + r];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) w[k] += jv * A[o + (int64_t)c * 6 + k];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) W[o + (int64_t)r * 6 + k] = w[k];
+    }
+    for (int c = threadIdx.x; c < n6; c += blockDim.x) {
+        double bj[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int r = 0; r < n6; ++r) {
+            const double jv = Ji[(int64_t)c * n6 + r];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) bj[k] += B[o + (int64_t)k * n6 + r] * jv;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) BJ[o + (int64_t)k * n6 + c] = bj[k];
+    }
+}
+
+// S = S0 - sum_i BJ_i A_i, one workgroup per swimmer, thread (k, j) < 36.
+__global__ __launch_bounds__(64) void coupling_schur_kernel(const double* __restrict__ S0,
+                                                            const double* __restrict__ BJ,
+                                                            const double* __restrict__ A, int fil_per_swim, int n6,
+                                                            double* __restrict__ S) {
+    const int s = blockIdx.x, t = threadIdx.x;
+    if (t >= 36) return;
+    const int k = t / 6, j = t % 6;
+    double acc = S0[36 * s + t];
+    for (int i = 0; i < fil_per_swim; ++i) {
+        const int64_t o = ((int64_t)s * fil_per_swim + i) * 6 * n6;
+        for (int c = 0; c < n6; ++c) acc -= BJ[o + (int64_t)k * n6 + c] * A[o + (int64_t)c * 6 + j];
+    }
+    S[36 * s + t] = acc;
+}
+
+// After the block-diagonal product y = alpha diag(J_i^{-1}, S^{-1}) x: the
+// body rows become alpha S^{-1} (x_b - sum_i BJ_i x_i) and the filament rows
+// y_i - W_i y_b (swimmer.cpp:1012-1066).  One workgroup per swimmer.
+__global__ __launch_bounds__(256) void coupling_apply_kernel(const double* __restrict__ W,
+                                                             const double* __restrict__ BJ,
+                                                             const double* __restrict__ Sinv, int fil_per_swim,
+                                                             int n6, const double* __restrict__ x,
+                                                             double* __restrict__ y, double alpha) {
+    const int s = blockIdx.x;
+    const int64_t rows = (int64_t)fil_per_swim * n6;
+    const int64_t r0 = (int64_t)s * (rows + 6);
+    const int64_t o = (int64_t)s * fil_per_swim * 6 * n6;
+    double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int64_t e = threadIdx.x; e < rows; e += blockDim.x) {
+        const int64_t i = e / n6, c = e % n6;
+        const double xv = x[r0 + e];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) g[k] += BJ[o + i * 6 * n6 + (int64_t)k * n6 + c] * xv;
+    }
+    __shared__ double sm[6][256];
+    __shared__ double ob[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sm[k][threadIdx.x] = g[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sm[k][threadIdx.x] += sm[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        double a = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) a += Sinv[36 * s + 6 * k + j] * sm[j][0];
+        const double v = y[r0 + rows + k] - alpha * a;
+        ob[k] = v;
+        y[r0 + rows + k] = v;
+    }
+    __syncthreads();
+    for (int64_t e = threadIdx.x; e < rows; e += blockDim.x) {
+        const int64_t i = e / n6, r = e % n6;
+        double a = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) a += W[o + i * 6 * n6 + r * 6 + k] * ob[k];
+        y[r0 + e] -= a;
+    }
+}
+
 // s[1] = 1 / s[0]
 __global__ void reciprocal_kernel(double* s) {
     if (threadIdx.x == 0 && blockIdx.x == 0) s[1] = s[0] > 0.0 ? 1.0 / s[0] : 0.0;
@@ -208,6 +307,13 @@ struct tfs_broyden {
     int max_block = 0;
     double* host_scal = nullptr;
     std::vector<int32_t> host_status;
+    std::vector<int32_t> host_sizes;
+    std::vector<int64_t> host_moff;
+    // free-body coupling (tfs_broyden_set_body_coupling)
+    bool coupled = false;
+    int cpl_swim = 0, cpl_fil = 0, cpl_n6 = 0;
+    double* cpl_arena = nullptr;
+    double *W = nullptr, *BJ = nullptr, *Sinv = nullptr, *Sbuf = nullptr, *cpl_in = nullptr;
 };
 
 namespace {
@@ -221,7 +327,13 @@ hipMemcpyKind to_device(const tfs_broyden* b) {
 int apply_jinv(tfs_broyden* b, const double* x, double* y, double alpha) {
     hipLaunchKernelGGL(block_gemv_kernel, dim3(b->nblocks), dim3(128), 0, b->stream, b->J, b->mat_off, b->row_off,
                        b->sizes, x, y, alpha);
-    return check(hipGetLastError(), "J0^{-1}");
+    int rc = check(hipGetLastError(), "J0^{-1}");
+    if (!rc && b->coupled) {
+        hipLaunchKernelGGL(coupling_apply_kernel, dim3(b->cpl_swim), dim3(256), 0, b->stream, (const double*)b->W,
+                           (const double*)b->BJ, (const double*)b->Sinv, b->cpl_fil, b->cpl_n6, x, y, alpha);
+        rc = check(hipGetLastError(), "J0^{-1} body coupling");
+    }
+    return rc;
 }
 
 // out = -c J0^{-1} e - C(:, 0:iter) (D(:, 0:iter)^T e)
@@ -283,6 +395,8 @@ int tfs_broyden_create(const tfs_broyden_desc* desc, tfs_broyden** out) {
     b->G = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (D.n + 255) / 256));
     b->RB = (int)((D.n / 6 + 256) / 256 + 1);   // residual blocks: one thread per segment or body
     b->host_status.resize(D.num_blocks);
+    b->host_sizes = sz;
+    b->host_moff = moff;
     hipError_t e = hipSuccess;
     if (D.stream) {
         b->stream = (hipStream_t)D.stream;
@@ -335,6 +449,7 @@ int tfs_broyden_destroy(tfs_broyden* b) {
     (void)hipFree(b->sizes);
     (void)hipFree(b->piv);
     (void)hipFree(b->status);
+    if (b->cpl_arena) (void)hipFree(b->cpl_arena);
     if (b->host_scal) (void)hipHostFree(b->host_scal);
     if (b->own_stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -343,6 +458,7 @@ int tfs_broyden_destroy(tfs_broyden* b) {
 
 int tfs_broyden_set_jacobian_inverse(tfs_broyden* b, const double* blocks) {
     if (!b || !blocks) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_set_jacobian_inverse: null argument");
+    b->coupled = false;
     int rc = copy_in(b, b->Jstage, blocks, b->jac_len, "tfs_broyden_set_jacobian_inverse");
     if (rc) return rc;
     hipLaunchKernelGGL(block_transpose_kernel, dim3(b->nblocks), dim3(256), 0, b->stream, b->Jstage, b->J,
@@ -352,6 +468,7 @@ int tfs_broyden_set_jacobian_inverse(tfs_broyden* b, const double* blocks) {
 
 int tfs_broyden_set_jacobian(tfs_broyden* b, const double* blocks) {
     if (!b || !blocks) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_set_jacobian: null argument");
+    b->coupled = false;
     int rc = copy_in(b, b->Jstage, blocks, b->jac_len, "tfs_broyden_set_jacobian");
     if (rc) return rc;
     const size_t lds = sizeof(double) * 2 * b->max_block;
@@ -370,6 +487,103 @@ int tfs_broyden_set_jacobian(tfs_broyden* b, const double* blocks) {
     hipLaunchKernelGGL(block_transpose_kernel, dim3(b->nblocks), dim3(256), 0, b->stream, b->Jstage, b->J,
                        b->mat_off, b->sizes);
     return check(hipGetLastError(), "tfs_broyden_set_jacobian");
+}
+
+int tfs_broyden_set_body_coupling(tfs_broyden* b, int32_t num_swim, int32_t fil_per_swim, const double* A,
+                                  const double* B, const double* S0) {
+    if (!b || !A || !B || !S0) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_set_body_coupling: null argument");
+    if (num_swim < 1 || fil_per_swim < 1 || (int64_t)num_swim * (fil_per_swim + 1) != b->nblocks)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_set_body_coupling: block count is not num_swim * (fil_per_swim + 1)");
+    const int n6 = b->host_sizes[0];
+    for (int s = 0; s < num_swim; ++s)
+        for (int i = 0; i <= fil_per_swim; ++i)
+            if (b->host_sizes[(size_t)s * (fil_per_swim + 1) + i] != (i < fil_per_swim ? n6 : 6))
+                return fail(TFS_ERR_INVALID_ARGUMENT,
+                            "tfs_broyden_set_body_coupling: blocks are not fil_per_swim equal blocks and a 6 x 6 per swimmer");
+    const int64_t lab = (int64_t)num_swim * fil_per_swim * 6 * n6;
+    if (b->cpl_arena && (b->cpl_swim != num_swim || b->cpl_fil != fil_per_swim)) {
+        (void)hipFree(b->cpl_arena);
+        b->cpl_arena = nullptr;
+    }
+    if (!b->cpl_arena) {
+        const int64_t parts[] = {lab, lab, 36LL * num_swim, 36LL * num_swim, 2 * lab + 36LL * num_swim};
+        double** dst[] = {&b->W, &b->BJ, &b->Sinv, &b->Sbuf, &b->cpl_in};
+        int64_t total = 0;
+        for (int64_t l : parts) total += round32(l);
+        if (hipMalloc(&b->cpl_arena, total * 8) != hipSuccess)
+            return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_broyden_set_body_coupling: device memory");
+        double* p = b->cpl_arena;
+        for (size_t i = 0; i < sizeof(parts) / sizeof(parts[0]); ++i) {
+            *dst[i] = p;
+            p += round32(parts[i]);
+        }
+        b->cpl_swim = num_swim;
+        b->cpl_fil = fil_per_swim;
+    }
+    b->cpl_n6 = n6;
+    b->coupled = false;
+    const double *dA = A, *dB = B, *dS0 = S0;
+    int rc = TFS_OK;
+    if (host_mem(b)) {
+        rc = copy_in(b, b->cpl_in, A, lab, "tfs_broyden_set_body_coupling");
+        if (!rc) rc = copy_in(b, b->cpl_in + lab, B, lab, "tfs_broyden_set_body_coupling");
+        if (!rc) rc = copy_in(b, b->cpl_in + 2 * lab, S0, 36LL * num_swim, "tfs_broyden_set_body_coupling");
+        dA = b->cpl_in;
+        dB = b->cpl_in + lab;
+        dS0 = b->cpl_in + 2 * lab;
+    }
+    if (rc) return rc;
+    hipLaunchKernelGGL(coupling_prep_kernel, dim3((unsigned)(num_swim * fil_per_swim)), dim3(256), 0, b->stream,
+                       (const double*)b->J, (const int64_t*)b->mat_off, fil_per_swim, n6, dA, dB, b->W, b->BJ);
+    rc = check(hipGetLastError(), "body coupling");
+    if (!rc) {
+        hipLaunchKernelGGL(coupling_schur_kernel, dim3((unsigned)num_swim), dim3(64), 0, b->stream, dS0,
+                           (const double*)b->BJ, dA, fil_per_swim, n6, b->Sbuf);
+        rc = check(hipGetLastError(), "body Schur complement");
+    }
+    std::vector<double> S(36 * (size_t)num_swim), Si(36 * (size_t)num_swim);
+    if (!rc) rc = check(hipMemcpyAsync(S.data(), b->Sbuf, 288LL * num_swim, hipMemcpyDeviceToHost, b->stream), "Schur");
+    if (!rc) rc = check(hipStreamSynchronize(b->stream), "Schur");
+    if (rc) return rc;
+    // invert each 6 x 6 (Gauss-Jordan, partial pivoting)
+    for (int s = 0; s < num_swim; ++s) {
+        double a[6][12];
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 12; ++c) a[r][c] = c < 6 ? S[36 * (size_t)s + 6 * r + c] : (c - 6 == r ? 1.0 : 0.0);
+        for (int k = 0; k < 6; ++k) {
+            int piv = k;
+            for (int r = k + 1; r < 6; ++r)
+                if (std::fabs(a[r][k]) > std::fabs(a[piv][k])) piv = r;
+            if (!(std::fabs(a[piv][k]) > 0.0) || !std::isfinite(a[piv][k]))
+                return fail(TFS_ERR_NON_FINITE, "tfs_broyden_set_body_coupling: singular body Schur complement");
+            if (piv != k)
+                for (int c = 0; c < 12; ++c) std::swap(a[k][c], a[piv][c]);
+            const double inv_p = 1.0 / a[k][k];
+            for (int c = 0; c < 12; ++c) a[k][c] *= inv_p;
+            for (int r = 0; r < 6; ++r)
+                if (r != k) {
+                    const double f = a[r][k];
+                    for (int c = 0; c < 12; ++c) a[r][c] -= f * a[k][c];
+                }
+        }
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) Si[36 * (size_t)s + 6 * r + c] = a[r][6 + c];
+    }
+    rc = check(hipMemcpyAsync(b->Sinv, Si.data(), 288LL * num_swim, hipMemcpyHostToDevice, b->stream), "S^{-1}");
+    // the body blocks of J0^{-1}: S^{-1}, column-major like every block
+    std::vector<double> colmajor(36 * (size_t)num_swim);
+    for (int s = 0; s < num_swim; ++s)
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) colmajor[36 * (size_t)s + 6 * c + r] = Si[36 * (size_t)s + 6 * r + c];
+    for (int s = 0; s < num_swim && !rc; ++s) {
+        const size_t blk = (size_t)s * (fil_per_swim + 1) + fil_per_swim;
+        rc = check(hipMemcpyAsync(b->J + b->host_moff[blk], colmajor.data() + 36 * (size_t)s, 288,
+                                  hipMemcpyHostToDevice, b->stream),
+                   "S^{-1} block");
+    }
+    if (!rc) rc = check(hipStreamSynchronize(b->stream), "tfs_broyden_set_body_coupling");
+    if (!rc) b->coupled = true;
+    return rc;
 }
 
 int tfs_broyden_start(tfs_broyden* b, const double* error) {

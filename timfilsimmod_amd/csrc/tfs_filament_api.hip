@@ -237,7 +237,22 @@ struct FilState {
     double *mult, *mult_m1, *mult_m2;  // 3 (nseg + 1) per filament
     double *bases;                     // 3 per filament
     double *pert;                      // 6 per filament (may be null)
+    double *clampblk;                  // 18 per filament: elastic clamping blocks 1, 2 (row-major)
+    // A free body per swimmer (all null while the bodies are held still):
+    // position at n+1, n, n-1; Lie-algebra element at n+1, n; quaternion at
+    // n+1, n; Q_init = R(q) of the initial guess (3, 3, 3, 3, 3, 4, 4, 9 each).
+    double *bx, *bxm1, *bxm2, *bu, *bum1, *bq, *bqm1, *bQ;
+    double *brefs;                     // 3 per filament: its anchor in the body frame
 };
+
+// The anchor of filament f on its (free) body: X + R(q) ref.
+__device__ void body_anchor(const FilState& S, const FilParams& P, int f, double* base) {
+    const int sw = f / P.fil_per_swim;
+    const M3 R = rot(load_q(S.bq + 4 * sw));
+    mv(R, S.brefs + 3 * f, base);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) base[k] += S.bx[3 * sw + k];
+}
 
 // ---- kernels ---------------------------------------------------------------
 __device__ void robot_arm(const FilState& S, const FilParams& P, int f) {
@@ -297,16 +312,27 @@ __global__ void initial_guess_kernel(FilState S, FilParams P, int nt) {
         }
         const Q q = load_q(S.q + 4 * g);
         store_q(S.qm1 + 4 * g, q);
-        if (i == 0) {
+        if (i == 0 && !S.bx) {
             // clamped to a body that does not turn: u_0 = 0, q_0 unchanged
 #pragma unroll
             for (int k = 0; k < 3; ++k) S.u[3 * g + k] = 0.0;
+        } else if (i == 0) {
+            // accept_state_from_rigid_body (filament.cpp:649-672): the base
+            // segment turns with the body, u_0 = u_body
+            const int sw = f / P.fil_per_swim;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) S.u[3 * g + k] = S.bu[3 * sw + k];
+            store_q(S.q + 4 * g, qmul(lie_exp(S.u + 3 * g), q));
         } else {
             store_q(S.q + 4 * g, qmul(lie_exp(S.u + 3 * g), q));
         }
     }
+    if (S.bx) {
+        body_anchor(S, P, f, S.x + 3 * s0);
+    } else {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) S.x[3 * s0 + k] = S.bases[3 * f + k];
+        for (int k = 0; k < 3; ++k) S.x[3 * s0 + k] = S.bases[3 * f + k];
+    }
     const int nm = 3 * (P.nseg + 1);
     double* m = S.mult + (int64_t)f * nm;
     double* m1 = S.mult_m1 + (int64_t)f * nm;
@@ -477,8 +503,15 @@ __global__ __launch_bounds__(64) void jacobian_kernel(FilState S, FilParams P, i
                 scale(-Tfac, axpy(0.5, frame_cross(qp, Kmh), mm(R, moment_lie_derivative(qp, qk, false, P))));
             M3 B = axpy(-0.5, mm(L, left), left);
             B = axpy(1.0 / 12.0, mm(L, mm(L, left)), B);
-            // segment 1's block on the clamp column is zero (filament.cpp:1887)
-            if (k != 1) put3(J, n6, 3 * N + 3 * k, 3 * N + 3 * k - 3, scale(-tsf, B));
+            // segment 1's block on the clamp column is zero (filament.cpp:1887);
+            // its value before that is elastic_clamping_block2
+            if (k != 1) {
+                put3(J, n6, 3 * N + 3 * k, 3 * N + 3 * k - 3, scale(-tsf, B));
+            } else if (S.clampblk) {
+                const M3 c2 = scale(-tsf, B);
+#pragma unroll
+                for (int e = 0; e < 9; ++e) S.clampblk[18 * (int64_t)f + 9 + e] = c2.a[e];
+            }
             for (int c = 0; c < 3; ++c) lamsum[c] += mult[3 * k + c];
         }
         if (k == 0) {
@@ -486,7 +519,10 @@ __global__ __launch_bounds__(64) void jacobian_kernel(FilState S, FilParams P, i
             const M3 blk = axpy(1.0 / 12.0, mm(um, um), axpy(0.5, um, I3));
             const M3 blkT = axpy(1.0 / 12.0, mm(um, um), axpy(-0.5, um, I3));
             put3(J, n6, 3 * N, 3 * N, scale(fac3, mm(blk, blkT)));
-        } else {
+        }
+        if (k > 0 || S.clampblk) {
+            // the Lie-algebra diagonal block; for segment 0 it is overwritten
+            // above and kept as elastic_clamping_block1
             const M3 constraint = scale(-0.5 * P.dl, mm(rcross(lamsum), rcross(t)));
             const M3 dOm = scale(Tfac, axpy(1.0, elastic, constraint));
             double ou[3];
@@ -494,7 +530,13 @@ __global__ __launch_bounds__(64) void jacobian_kernel(FilState S, FilParams P, i
             M3 B = axpy(-0.5, axpy(1.0, Om, mm(L, dOm)), dOm);
             const M3 hi = axpy(1.0, rcross(ou), axpy(1.0, mm(L, Om), mm(L, mm(L, dOm))));
             B = axpy(1.0 / 12.0, hi, B);
-            put3(J, n6, 3 * N + 3 * k, 3 * N + 3 * k, axpy(-tsf, B, I3));
+            const M3 diag = axpy(-tsf, B, I3);
+            if (k > 0) {
+                put3(J, n6, 3 * N + 3 * k, 3 * N + 3 * k, diag);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 9; ++e) S.clampblk[18 * (int64_t)f + e] = diag.a[e];
+            }
         }
     }
 }
@@ -510,6 +552,15 @@ __global__ void update_kernel(FilState S, FilParams P, const double* __restrict_
     double* m = S.mult + (int64_t)f * nm;
     for (int i = 0; i < nm; ++i) m[i] += d[i];   // tether, lambda..., clamp
     const int64_t s0 = (int64_t)f * N;
+    if (S.bx) {
+        // swimmer::update: the anchor and base rotation follow the body
+        // updated just before (accept_state_from_rigid_body)
+        const int sw = f / P.fil_per_swim;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) S.u[3 * s0 + k] = S.bu[3 * sw + k];
+        store_q(S.q + 4 * s0, qmul(lie_exp(S.u + 3 * s0), load_q(S.qm1 + 4 * s0)));
+        body_anchor(S, P, f, S.x + 3 * s0);
+    }
     for (int i = 1; i < N; ++i) {
         const int64_t g = s0 + i;
 #pragma unroll
@@ -517,6 +568,293 @@ __global__ void update_kernel(FilState S, FilParams P, const double* __restrict_
         store_q(S.q + 4 * g, qmul(lie_exp(S.u + 3 * g), load_q(S.qm1 + 4 * g)));
     }
     robot_arm(S, P, f);
+}
+
+// rigid_body::initial_guess (rigid_body.cpp:395-447), one thread per body.
+__global__ void body_guess_kernel(FilState S, int nswim, int nt) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nswim) return;
+    double* x = S.bx + 3 * b;
+    double* xm1 = S.bxm1 + 3 * b;
+    double* xm2 = S.bxm2 + 3 * b;
+    double* u = S.bu + 3 * b;
+    double* um1 = S.bum1 + 3 * b;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double gx = nt <= 1 ? 2.0 * x[k] - xm1[k] : 3.0 * (x[k] - xm1[k]) + xm2[k];
+        const double gu = nt <= 1 ? u[k] : 2.0 * u[k] - um1[k];
+        xm2[k] = xm1[k];
+        xm1[k] = x[k];
+        x[k] = gx;
+        um1[k] = u[k];
+        u[k] = gu;
+    }
+    const Q q = load_q(S.bq + 4 * b);
+    store_q(S.bqm1 + 4 * b, q);
+    const Q qn = qmul(lie_exp(u), q);
+    store_q(S.bq + 4 * b, qn);
+    const M3 R = rot(qn);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) S.bQ[9 * b + e] = R.a[e];
+}
+
+// Q_init = R(q) of the current pose, one thread per body.
+__global__ void body_rot_kernel(FilState S, int nswim) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nswim) return;
+    const M3 R = rot(load_q(S.bq + 4 * b));
+#pragma unroll
+    for (int e = 0; e < 9; ++e) S.bQ[9 * b + e] = R.a[e];
+}
+
+// rigid_body::update (rigid_body.cpp:506-535) with the body's 6 entries of the
+// Broyden update (after its filaments' blocks).
+__global__ void body_update_kernel(FilState S, FilParams P, int nswim, const double* __restrict__ du) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nswim) return;
+    const int64_t n6 = 6LL * P.nseg;
+    const double* d = du + (int64_t)b * (P.fil_per_swim * n6 + 6) + P.fil_per_swim * n6;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        S.bx[3 * b + k] += d[k];
+        S.bu[3 * b + k] += d[3 + k];
+    }
+    store_q(S.bq + 4 * b, qmul(lie_exp(S.bu + 3 * b), load_q(S.bqm1 + 4 * b)));
+}
+
+// rigid_body::blob_positions (rigid_body.cpp:482-500): X + R(q) ref per blob.
+__global__ void blob_positions_kernel(FilState S, const double* __restrict__ refs, int64_t nblob, int blobs_per_swim,
+                                      double* __restrict__ xb) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nblob) return;
+    const int b = (int)(m / blobs_per_swim);
+    const M3 R = rot(load_q(S.bq + 4 * b));
+    double p[3];
+    mv(R, refs + 3 * m, p);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) xb[3 * m + k] = S.bx[3 * b + k] + p[k];
+}
+
+// The body mobility at the pose of the initial guess: Q N_ref Q^T per 3 x 3
+// block (swimmer.cpp:808-813), one thread per body.
+__global__ void rotate_mobility_kernel(FilState S, int nswim, const double* __restrict__ n_ref,
+                                       double* __restrict__ n_rot) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nswim) return;
+    M3 Q;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) Q.a[e] = S.bQ[9 * b + e];
+    M3 Qt;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Qt.a[3 * r + c] = Q.a[3 * c + r];
+    for (int br = 0; br < 2; ++br)
+        for (int bc = 0; bc < 2; ++bc) {
+            M3 blk;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) blk.a[3 * r + c] = n_ref[36 * b + 6 * (3 * br + r) + 3 * bc + c];
+            const M3 o = mm(Q, mm(blk, Qt));
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) n_rot[36 * b + 6 * (3 * br + r) + 3 * bc + c] = o.a[3 * r + c];
+        }
+}
+
+__device__ __forceinline__ M3 block_of(const double* n6x6, int br, int bc) {
+    M3 m;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) m.a[3 * r + c] = n6x6[6 * (3 * br + r) + 3 * bc + c];
+    return m;
+}
+
+__device__ __forceinline__ M3 identity3() {
+    M3 m = zero3();
+    m.a[0] = m.a[4] = m.a[8] = 1.0;
+    return m;
+}
+
+// RPY couplings between segment n and the body taken as a sphere of radius
+// R (swimmer.cpp:818-843).
+__device__ void sphere_terms(const FilState& S, const FilParams& P, int64_t g, int sw, double R, M3& Mtt, M3& Mtr,
+                             M3& Mrr, M3& tx) {
+    double rv[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) rv[k] = S.x[3 * g + k] - S.bx[3 * sw + k];
+    const double r = sqrt(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
+    double rh[3] = {rv[0] / r, rv[1] / r, rv[2] / r};
+    const double a2 = P.a * P.a, R2 = R * R;
+    const double ct = 1.0 / (8.0 * kPi * P.mu * r), cr = 1.0 / (8.0 * kPi * P.mu * r * r),
+                 crr = 1.0 / (16.0 * kPi * P.mu * r * r * r);
+    const double d1 = 1.0 + (R2 + a2) / (3.0 * r * r), d2 = 1.0 - (R2 + a2) / (r * r);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double rr = rh[i] * rh[j], id = i == j ? 1.0 : 0.0;
+            Mtt.a[3 * i + j] = (d1 * id + d2 * rr) * ct;
+            Mrr.a[3 * i + j] = (3.0 * rr - id) * crr;
+        }
+    Mtr = scale(cr, rcross(rh));
+    double t[3];
+    tangent(load_q(S.q + 4 * g), t);
+    tx = rcross(t);
+}
+
+struct CouplingOut {
+    const double* n_rot;   // 36 per body
+    double* A;             // per filament 6N x 6, swimmer layout
+    double* B;             // per filament 6 x 6N
+    double* part;          // 9 per filament: its term of omega_lie_deriv
+    double sphere;         // radius of the sphere correction (0: none)
+    int bdf2;
+};
+
+// swimmer::prepare_jacobian_inv, GMRES branch with a free body
+// (swimmer.cpp:688-880): the coupling blocks of one filament per workgroup;
+// thread c owns column block c of B_i (3 columns) and row block c of A_i.
+__global__ __launch_bounds__(64) void coupling_kernel(FilState S, FilParams P, CouplingOut O) {
+    const int f = blockIdx.x;
+    const int N = P.nseg, n6 = 6 * N;
+    const int sw = f / P.fil_per_swim;
+    const int64_t s0 = (int64_t)f * N;
+    const double dt_fac = O.bdf2 ? -2.0 * P.dt / 3.0 : -P.dt;
+    M3 Qi;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) Qi.a[e] = S.bQ[9 * sw + e];
+    double d[3];
+    mv(Qi, S.brefs + 3 * f, d);
+    const double nd = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    double dd[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dd[k] = (0.5 * P.dl + nd) * d[k] / nd;
+    const M3 Dm = rcross(dd);
+    const M3 I3 = identity3();
+    const M3 L = scale(-1.0, rcross(S.bu + 3 * sw));
+    const M3 LL = mm(L, L);
+    const M3 Dinv = axpy(-0.5, L, axpy(1.0 / 12.0, LL, I3));
+    const M3 DinvT = axpy(0.5, L, axpy(1.0 / 12.0, LL, I3));
+    double tN[3], efN[3];
+    tangent(load_q(S.q + 4 * (s0 + N - 1)), tN);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) efN[k] = P.end_force * tN[k];
+    const M3 T = rcross(efN);
+    const double* nr = O.n_rot + 36 * sw;
+    const M3 N00 = block_of(nr, 0, 0), N01 = block_of(nr, 0, 1), N10 = block_of(nr, 1, 0), N11 = block_of(nr, 1, 1);
+    const int swim_f = f % P.fil_per_swim;
+    double* Bf = O.B + (int64_t)sw * P.fil_per_swim * 6 * n6 + (int64_t)swim_f * 6 * n6;
+    double* Af = O.A + (int64_t)sw * P.fil_per_swim * 6 * n6 + (int64_t)swim_f * 6 * n6;
+    const double* mult = S.mult + (int64_t)f * 3 * (N + 1);
+    for (int c = threadIdx.x; c < 2 * N; c += blockDim.x) {
+        M3 top = zero3(), bot = zero3();
+        if (c == 0) {
+            top = scale(-1.0, I3);
+            bot = rcross(d);
+        } else if (c == N) {
+            bot = scale(-1.0, DinvT);
+        } else if (c == 2 * N - 1) {
+            top = T;
+            double v[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v[k] = 0.5 * P.dl * tN[k] - (S.x[3 * (s0 + N - 1) + k] - S.bx[3 * sw + k]);
+            bot = mm(rcross(v), T);
+        } else if (c > N) {
+            double tm[3];
+            tangent(load_q(S.q + 4 * (s0 + c - N)), tm);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) tm[k] *= P.dl;
+            bot = mm(T, rcross(tm));
+        }
+        // body_mobility * B
+        M3 nt_ = axpy(1.0, mm(N01, bot), mm(N00, top));
+        M3 nb_ = axpy(1.0, mm(N11, bot), mm(N10, top));
+        if (O.sphere > 0.0) {
+            M3 Mtt, Mtr, Mrr, tx;
+            if (c == 0) {
+                sphere_terms(S, P, s0, sw, O.sphere, Mtt, Mtr, Mrr, tx);
+                nt_ = axpy(1.0, Mtt, nt_);
+                nb_ = axpy(1.0, Mtr, nb_);
+            } else if (c == N) {
+                sphere_terms(S, P, s0, sw, O.sphere, Mtt, Mtr, Mrr, tx);
+                nt_ = axpy(1.0, mm(Mtr, DinvT), nt_);
+                nb_ = axpy(1.0, mm(Mrr, DinvT), nb_);
+            } else if (c < N) {
+                // lambda_{c-1}: segment c-1 from the left, segment c from the right
+                sphere_terms(S, P, s0 + c - 1, sw, O.sphere, Mtt, Mtr, Mrr, tx);
+                nt_ = axpy(-1.0, Mtt, axpy(0.5 * P.dl, mm(Mtr, tx), nt_));
+                nb_ = axpy(-1.0, Mtr, axpy(0.5 * P.dl, mm(Mrr, tx), nb_));
+                sphere_terms(S, P, s0 + c, sw, O.sphere, Mtt, Mtr, Mrr, tx);
+                nt_ = axpy(1.0, Mtt, axpy(0.5 * P.dl, mm(Mtr, tx), nt_));
+                nb_ = axpy(1.0, Mtr, axpy(0.5 * P.dl, mm(Mrr, tx), nb_));
+            } else if (c == 2 * N - 1) {
+                sphere_terms(S, P, s0 + N - 1, sw, O.sphere, Mtt, Mtr, Mrr, tx);
+                nt_ = axpy(-P.end_force, mm(Mtt, tx), nt_);
+                nb_ = axpy(-P.end_force, mm(Mtr, tx), nb_);
+            }
+        }
+        nb_ = mm(Dinv, nb_);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                Bf[(int64_t)r * n6 + 3 * c + j] = dt_fac * nt_.a[3 * r + j];
+                Bf[(int64_t)(3 + r) * n6 + 3 * c + j] = dt_fac * nb_.a[3 * r + j];
+            }
+        // A: [I | Dm] on the position rows, the clamping blocks on the Lie
+        // rows of segments 0 and 1
+        M3 al = zero3(), ar = zero3();
+        if (c < N) {
+            al = I3;
+            ar = Dm;
+        } else if (c == N || c == N + 1) {
+#pragma unroll
+            for (int e = 0; e < 9; ++e) ar.a[e] = S.clampblk[18 * (int64_t)f + 9 * (c - N) + e];
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                Af[(int64_t)(3 * c + r) * 6 + j] = al.a[3 * r + j];
+                Af[(int64_t)(3 * c + r) * 6 + 3 + j] = ar.a[3 * r + j];
+            }
+    }
+    if (threadIdx.x == 0) {
+        const double* tether = mult;
+        const double* clamp = mult + 3 * N;
+        M3 part = scale(-1.0, mm(rcross(tether), rcross(d)));
+        double uxl[3];
+        cross(S.bu + 3 * sw, clamp, uxl);
+        const M3 rc = rcross(clamp);
+        part = axpy(0.5, rc, part);
+        part = axpy(-1.0 / 12.0, axpy(1.0, rcross(uxl), mm(L, rc)), part);
+        part = axpy(1.0, mm(T, Dm), part);
+#pragma unroll
+        for (int e = 0; e < 9; ++e) O.part[9 * (int64_t)f + e] = part.a[e];
+    }
+}
+
+// The body block S0 (swimmer.cpp:692-780), one thread per body.
+__global__ void body_block_kernel(FilState S, FilParams P, int nswim, const double* __restrict__ n_rot,
+                                  const double* __restrict__ part, int bdf2, double* __restrict__ S0) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nswim) return;
+    const double dt_fac = bdf2 ? -2.0 * P.dt / 3.0 : -P.dt;
+    const M3 L = scale(-1.0, rcross(S.bu + 3 * b));
+    const M3 cbo = scale(-1.0 / dt_fac, L);
+    M3 old = zero3();
+    for (int i = 0; i < P.fil_per_swim; ++i)
+#pragma unroll
+        for (int e = 0; e < 9; ++e) old.a[e] += part[9 * ((int64_t)b * P.fil_per_swim + i) + e];
+    old = mm(block_of(n_rot + 36 * b, 1, 1), old);
+    M3 lie = axpy(-0.5, cbo, scale(1.0 / 12.0, mm(L, cbo)));
+    M3 om = axpy(1.0 / 12.0, mm(L, mm(L, old)), axpy(-0.5, mm(L, old), old));
+    lie = axpy(1.0, om, lie);
+    double* s = S0 + 36 * b;
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) s[6 * r + c] = r == c ? 1.0 : 0.0;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) s[6 * (3 + r) + 3 + c] += dt_fac * lie.a[3 * r + c];
 }
 
 }  // namespace
@@ -530,6 +868,12 @@ struct tfs_filament {
     size_t stage_len = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    // free bodies (tfs_filament_set_body)
+    double* body_arena = nullptr;
+    double *blob_refs = nullptr, *n_ref = nullptr, *n_rot = nullptr, *part = nullptr;
+    int64_t nblob = 0;
+    int32_t blobs_per_swim = 0;
+    bool jacobian_done = false;
 };
 
 namespace {
@@ -596,7 +940,8 @@ int tfs_filament_create(const tfs_filament_desc* desc, tfs_filament** out) {
     std::vector<std::pair<double**, int64_t>> parts = {
         {&h->S.q, 4 * ns},   {&h->S.qm1, 4 * ns},  {&h->S.u, 3 * ns},      {&h->S.um1, 3 * ns},
         {&h->S.x, 3 * ns},   {&h->S.xm1, 3 * ns},  {&h->S.xm2, 3 * ns},    {&h->S.mult, nm},
-        {&h->S.mult_m1, nm}, {&h->S.mult_m2, nm},  {&h->S.bases, 3 * nf},  {&h->S.pert, 6 * nf}};
+        {&h->S.mult_m1, nm}, {&h->S.mult_m2, nm},  {&h->S.bases, 3 * nf},  {&h->S.pert, 6 * nf},
+        {&h->S.clampblk, 18 * nf}};
     int64_t total = 0;
     for (auto& p : parts) total += (p.second + 31) / 32 * 32;
     if (hipMalloc(&h->arena, total * 8) != hipSuccess) {
@@ -632,6 +977,7 @@ int tfs_filament_destroy(tfs_filament* h) {
     if (!h) return TFS_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->arena) (void)hipFree(h->arena);
+    if (h->body_arena) (void)hipFree(h->body_arena);
     if (h->stage) (void)hipFree(h->stage);
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -669,6 +1015,14 @@ int tfs_filament_set_perturbation(tfs_filament* h, int32_t seg_a, int32_t seg_b,
 
 int tfs_filament_initial_guess(tfs_filament* h, int32_t nt) {
     if (!h || nt < 0) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_initial_guess: null handle or nt < 0");
+    h->jacobian_done = false;
+    if (h->S.bx) {
+        // swimmer::initial_guess: the body, then its filaments from the new pose
+        hipLaunchKernelGGL(body_guess_kernel, dim3(grid_for(h->desc.num_swim, 64)), dim3(64), 0, h->stream, h->S,
+                           (int)h->desc.num_swim, (int)nt);
+        const hipError_t e0 = hipGetLastError();
+        if (e0 != hipSuccess) return hip_fail(e0, "tfs_filament_initial_guess: body");
+    }
     hipLaunchKernelGGL(initial_guess_kernel, dim3(grid_for(h->P.nfil, 64)), dim3(64), 0, h->stream, h->S, h->P,
                        (int)nt);
     hipError_t e = hipGetLastError();
@@ -710,6 +1064,7 @@ int tfs_filament_jacobian(tfs_filament* h, int32_t nt, const double* f_segs, dou
                            nt >= 1 ? 1 : 0, fd, out);
         e = hipGetLastError();
     }
+    h->jacobian_done = e == hipSuccess;
     if (e == hipSuccess && host) e = hipMemcpyAsync(blocks, out, 8 * nb, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess && host) e = hipStreamSynchronize(h->stream);
     return check(e, "tfs_filament_jacobian");
@@ -726,6 +1081,11 @@ int tfs_filament_update(tfs_filament* h, const double* update) {
         if (!st) return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_filament_update: staging memory");
         e = hipMemcpyAsync(st, update, 8 * n, hipMemcpyHostToDevice, h->stream);
         du = st;
+    }
+    if (e == hipSuccess && h->S.bx) {
+        hipLaunchKernelGGL(body_update_kernel, dim3(grid_for(h->desc.num_swim, 64)), dim3(64), 0, h->stream, h->S,
+                           h->P, (int)h->desc.num_swim, du);
+        e = hipGetLastError();
     }
     if (e == hipSuccess) {
         hipLaunchKernelGGL(update_kernel, dim3(grid_for(h->P.nfil, 64)), dim3(64), 0, h->stream, h->S, h->P, du);
@@ -750,6 +1110,14 @@ static int array_of(tfs_filament* h, int32_t which, double** p, int64_t* len) {
         case TFS_FIL_UM1: *p = h->S.um1; *len = 3 * ns; return TFS_OK;
         case TFS_FIL_Q: *p = h->S.q; *len = 4 * ns; return TFS_OK;
         case TFS_FIL_MULT: *p = h->S.mult; *len = 3LL * (h->P.nseg + 1) * h->P.nfil; return TFS_OK;
+        case TFS_FIL_BODY_X: case TFS_FIL_BODY_XM1: case TFS_FIL_BODY_XM2: case TFS_FIL_BODY_U:
+        case TFS_FIL_BODY_UM1: case TFS_FIL_BODY_Q: {
+            if (!h->S.bx) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament: no free body (tfs_filament_set_body)");
+            double* const arr[] = {h->S.bx, h->S.bxm1, h->S.bxm2, h->S.bu, h->S.bum1, h->S.bq};
+            *p = arr[which - TFS_FIL_BODY_X];
+            *len = (which == TFS_FIL_BODY_Q ? 4LL : 3LL) * h->desc.num_swim;
+            return TFS_OK;
+        }
         default: return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament: unknown array");
     }
 }
@@ -771,6 +1139,108 @@ int tfs_filament_read(tfs_filament* h, int32_t which, double* out) {
     return check(e, "tfs_filament_read");
 }
 
+int tfs_filament_set_body(tfs_filament* h, const double* x_bodies, const double* q_bodies, const double* fil_refs,
+                          int32_t blobs_per_swim, const double* blob_refs) {
+    if (!h || !x_bodies || !q_bodies || !fil_refs || (blobs_per_swim > 0 && !blob_refs) || blobs_per_swim < 0)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_set_body: null argument or negative blob count");
+    const int64_t nsw = h->desc.num_swim, nf = h->P.nfil, nb = nsw * blobs_per_swim;
+    if (h->body_arena && nb != h->nblob)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_set_body: the blob count cannot change");
+    if (!h->body_arena) {
+        const int64_t parts[] = {3 * nsw, 3 * nsw, 3 * nsw, 3 * nsw, 3 * nsw, 4 * nsw, 4 * nsw, 9 * nsw, 3 * nf,
+                                 3 * nb,  36 * nsw, 36 * nsw, 9 * nf};
+        int64_t total = 0;
+        for (int64_t l : parts) total += (l + 31) / 32 * 32;
+        if (hipMalloc(&h->body_arena, total * 8) != hipSuccess)
+            return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_filament_set_body: device memory");
+        double** dst[] = {&h->S.bx, &h->S.bxm1, &h->S.bxm2, &h->S.bu, &h->S.bum1, &h->S.bq, &h->S.bqm1,
+                          &h->S.bQ, &h->S.brefs, &h->blob_refs, &h->n_ref, &h->n_rot, &h->part};
+        double* p = h->body_arena;
+        for (size_t i = 0; i < sizeof(parts) / sizeof(parts[0]); ++i) {
+            *dst[i] = p;
+            p += (parts[i] + 31) / 32 * 32;
+        }
+        h->nblob = nb;
+        h->blobs_per_swim = blobs_per_swim;
+    }
+    const hipMemcpyKind k = h->desc.memory == TFS_MEMORY_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    hipError_t e = hipSuccess;
+    for (double* d : {h->S.bx, h->S.bxm1, h->S.bxm2})
+        if (e == hipSuccess) e = hipMemcpyAsync(d, x_bodies, 24 * nsw, k, h->stream);
+    for (double* d : {h->S.bq, h->S.bqm1})
+        if (e == hipSuccess) e = hipMemcpyAsync(d, q_bodies, 32 * nsw, k, h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->S.bu, 0, 24 * nsw, h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->S.bum1, 0, 24 * nsw, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h->S.brefs, fil_refs, 24 * nf, k, h->stream);
+    if (e == hipSuccess && nb) e = hipMemcpyAsync(h->blob_refs, blob_refs, 24 * nb, k, h->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(body_rot_kernel, dim3(grid_for(nsw, 64)), dim3(64), 0, h->stream, h->S, (int)nsw);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    return check(e, "tfs_filament_set_body");
+}
+
+int tfs_filament_blob_positions(tfs_filament* h, double* x_blobs) {
+    if (!h || !x_blobs) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_blob_positions: null argument");
+    if (!h->S.bx) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_blob_positions: no free body");
+    if (!h->nblob) return TFS_OK;
+    const bool host = h->desc.memory == TFS_MEMORY_HOST;
+    double* out = host ? staging(h, 3 * h->nblob) : x_blobs;
+    if (!out) return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_filament_blob_positions: staging memory");
+    hipLaunchKernelGGL(blob_positions_kernel, dim3(grid_for(h->nblob, 256)), dim3(256), 0, h->stream, h->S,
+                       (const double*)h->blob_refs, h->nblob, (int)h->blobs_per_swim, out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && host) e = hipMemcpyAsync(x_blobs, out, 24 * h->nblob, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && host) e = hipStreamSynchronize(h->stream);
+    return check(e, "tfs_filament_blob_positions");
+}
+
+int tfs_filament_body_coupling(tfs_filament* h, int32_t nt, const double* body_mobility, double sphere_radius,
+                               double* A, double* B, double* S0) {
+    if (!h || !body_mobility || !A || !B || !S0 || nt < 0)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_body_coupling: null argument or nt < 0");
+    if (!h->S.bx) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_body_coupling: no free body");
+    if (!h->jacobian_done)
+        return fail(TFS_ERR_INVALID_ARGUMENT,
+                    "tfs_filament_body_coupling: call tfs_filament_jacobian at this state first");
+    if (!(sphere_radius >= 0.0) || !std::isfinite(sphere_radius))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_body_coupling: sphere radius must be >= 0");
+    const int64_t nsw = h->desc.num_swim, n6 = 6LL * h->P.nseg, lab = (int64_t)h->P.nfil * 6 * n6;
+    const bool host = h->desc.memory == TFS_MEMORY_HOST;
+    const hipMemcpyKind k = host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    double *dA = A, *dB = B, *dS = S0;
+    if (host) {
+        double* st = staging(h, 2 * lab + 36 * nsw);
+        if (!st) return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_filament_body_coupling: staging memory");
+        dA = st;
+        dB = st + lab;
+        dS = st + 2 * lab;
+    }
+    hipError_t e = hipMemcpyAsync(h->n_ref, body_mobility, 288 * nsw, k, h->stream);
+    const int bdf2 = nt >= 1 ? 1 : 0;  // NUM_EULER_STEPS = 1
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(rotate_mobility_kernel, dim3(grid_for(nsw, 64)), dim3(64), 0, h->stream, h->S, (int)nsw,
+                           (const double*)h->n_ref, h->n_rot);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        CouplingOut o{h->n_rot, dA, dB, h->part, sphere_radius, bdf2};
+        hipLaunchKernelGGL(coupling_kernel, dim3((unsigned)h->P.nfil), dim3(64), 0, h->stream, h->S, h->P, o);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(body_block_kernel, dim3(grid_for(nsw, 64)), dim3(64), 0, h->stream, h->S, h->P, (int)nsw,
+                           (const double*)h->n_rot, (const double*)h->part, bdf2, dS);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && host) e = hipMemcpyAsync(A, dA, 8 * lab, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && host) e = hipMemcpyAsync(B, dB, 8 * lab, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && host) e = hipMemcpyAsync(S0, dS, 288 * nsw, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && host) e = hipStreamSynchronize(h->stream);
+    return check(e, "tfs_filament_body_coupling");
+}
+
 // ---- elastic implicit timestep --------------------------------------------
 
 struct tfs_elastic {
@@ -778,15 +1248,22 @@ struct tfs_elastic {
     double* arena = nullptr;
     double *f_segs = nullptr, *v_segs = nullptr, *du = nullptr, *blocks = nullptr, *zeros = nullptr;
     int64_t n = 0;
+    // free bodies: blob positions, body velocities, the coupling blocks of J0
+    bool free_bodies = false;
+    double *x_blobs = nullptr, *v_bodies = nullptr, *A = nullptr, *B = nullptr, *S0 = nullptr;
 };
 
 int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
     g_last_error.clear();
-    if (!desc || !out || !desc->filaments || !desc->velocities || !desc->broyden || !desc->x_bodies ||
-        !desc->v_bodies || !desc->f_blobs || !desc->x_blobs)
+    if (!desc || !out || !desc->filaments || !desc->velocities || !desc->broyden || !desc->f_blobs)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_create: null handle or array");
     *out = nullptr;
     const tfs_filament* fh = desc->filaments;
+    const bool free_bodies = fh->S.bx != nullptr;
+    if (!free_bodies && (!desc->x_bodies || !desc->v_bodies || !desc->x_blobs))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_create: null body or blob array");
+    if (free_bodies && (!desc->body_mobility || !(desc->sphere_radius >= 0.0)))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_create: free bodies need the body mobility (and radius >= 0)");
     if (fh->desc.memory != TFS_MEMORY_DEVICE)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_create: the filament handle must use device memory");
     if (!(desc->tol > 0) || desc->max_iter < 1)
@@ -796,9 +1273,12 @@ int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
     const int64_t ns = num_segs(fh), nswim = fh->desc.num_swim;
     e->n = update_len(fh);
     const int64_t nb = blocks_len(fh);
-    std::vector<std::pair<double**, int64_t>> parts = {{&e->f_segs, 6 * ns}, {&e->v_segs, 6 * ns},
-                                                       {&e->du, e->n},       {&e->blocks, nb},
-                                                       {&e->zeros, 6 * nswim}};
+    e->free_bodies = free_bodies;
+    const int64_t lab = free_bodies ? (int64_t)fh->P.nfil * 36 * fh->P.nseg : 0;
+    std::vector<std::pair<double**, int64_t>> parts = {
+        {&e->f_segs, 6 * ns}, {&e->v_segs, 6 * ns},        {&e->du, e->n},  {&e->blocks, nb},
+        {&e->zeros, 6 * nswim}, {&e->x_blobs, 3 * fh->nblob}, {&e->v_bodies, 6 * nswim}, {&e->A, lab},
+        {&e->B, lab},          {&e->S0, 36 * nswim}};
     int64_t total = 0;
     for (auto& p : parts) total += (p.second + 31) / 32 * 32;
     if (hipMalloc(&e->arena, total * 8) != hipSuccess) {
@@ -818,6 +1298,7 @@ int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
         for (int k = 0; k < 6; ++k) host[s * per + fh->desc.fil_per_swim * n6 * n6 + 7 * k] = 1.0;
     hipError_t err = hipMemcpy(e->blocks, host.data(), 8 * nb, hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemset(e->zeros, 0, 48 * nswim);
+    if (err == hipSuccess) err = hipMemset(e->v_bodies, 0, 48 * nswim);
     if (err != hipSuccess) {
         tfs_elastic_destroy(e);
         return hip_fail(err, "tfs_elastic_create");
@@ -856,13 +1337,26 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
     st.v_segs = e->v_segs;
     st.body_x = st.body_xm1 = st.body_xm2 = st.body_u = st.body_um1 = e->zeros;
     st.v_bodies = e->zeros;
+    if (e->free_bodies) {
+        st.body_x = fh->S.bx;
+        st.body_xm1 = fh->S.bxm1;
+        st.body_xm2 = fh->S.bxm2;
+        st.body_u = fh->S.bu;
+        st.body_um1 = fh->S.bum1;
+        st.v_bodies = e->v_bodies;
+    }
     // forces -> velocities -> residual in `slot`
     auto evaluate = [&](int slot, int32_t* too_large) -> int {
-        int rc = tfs_filament_forces(fh, nt, e->f_segs);
+        int rc = e->free_bodies ? tfs_filament_blob_positions(fh, e->x_blobs) : TFS_OK;
+        if (!rc) rc = tfs_filament_forces(fh, nt, e->f_segs);
         if (!rc) rc = tfs_filament_sync(fh);
         tfs_solve_stats vs{};
-        if (!rc) rc = tfs_velocities_compute(D.velocities, fh->S.x, e->f_segs, D.x_blobs, D.x_bodies, D.v_bodies,
-                                             e->v_segs, D.f_blobs, &vs);
+        if (!rc && e->free_bodies)
+            rc = tfs_velocities_compute_free(D.velocities, fh->S.x, e->f_segs, e->x_blobs, fh->S.bx, nullptr,
+                                             e->v_segs, D.f_blobs, e->v_bodies, &vs);
+        else if (!rc)
+            rc = tfs_velocities_compute(D.velocities, fh->S.x, e->f_segs, D.x_blobs, D.x_bodies, D.v_bodies,
+                                        e->v_segs, D.f_blobs, &vs);
         S.gmres_iterations += vs.iterations;
         S.velocity_solves++;
         if (!rc) rc = tfs_broyden_compute_errors(D.broyden, &st, slot, too_large);
@@ -875,6 +1369,14 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
         rc = tfs_filament_jacobian(fh, nt, e->f_segs, e->blocks);
         if (!rc) rc = tfs_filament_sync(fh);
         if (!rc) rc = tfs_broyden_set_jacobian(D.broyden, e->blocks);
+        if (!rc && e->free_bodies) {
+            // swimmer::prepare_jacobian_inv: the body's Schur complement
+            rc = tfs_filament_body_coupling(fh, nt, D.body_mobility, D.sphere_radius, e->A, e->B, e->S0);
+            if (!rc) rc = tfs_filament_sync(fh);
+            if (!rc)
+                rc = tfs_broyden_set_body_coupling(D.broyden, fh->desc.num_swim, fh->desc.fil_per_swim, e->A, e->B,
+                                                   e->S0);
+        }
     }
     if (!rc) rc = tfs_broyden_start(D.broyden, nullptr);
     int32_t iter = 0;

@@ -46,6 +46,45 @@ __global__ __launch_bounds__(256) void blob_rhs_kernel(const double* __restrict_
     rhs[3 * m + 2] = (have_segs ? -v_blobs[3 * m + 2] : 0.0) + k2;
 }
 
+// Free bodies: rhs[3 nb + 6 b + k] = -swimmer.f = sum over the body's segments
+// of (F, T + (x - X) x F) minus the external force and torque
+// (swimmer.cpp:469-533, mobility_solver.cpp:551-556); blob rows -v_blobs.
+// One workgroup per body.
+__global__ __launch_bounds__(256) void free_rhs_kernel(const double* __restrict__ x_segs,
+                                                       const double* __restrict__ f_segs,
+                                                       const double* __restrict__ v_blobs, int64_t nblob,
+                                                       int segs_per_swim, const double* __restrict__ x_bodies,
+                                                       const double* __restrict__ f_ext, double* __restrict__ rhs) {
+    const int b = blockIdx.x;
+    // blob rows, spread over every workgroup
+    for (int64_t i = (int64_t)b * 256 + threadIdx.x; i < 3 * nblob; i += (int64_t)gridDim.x * 256)
+        rhs[i] = segs_per_swim > 0 ? -v_blobs[i] : 0.0;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double X0 = x_bodies[3 * b], X1 = x_bodies[3 * b + 1], X2 = x_bodies[3 * b + 2];
+    for (int i = threadIdx.x; i < segs_per_swim; i += 256) {
+        const int64_t g = (int64_t)b * segs_per_swim + i;
+        const double* f = f_segs + 6 * g;
+        const double r0 = x_segs[3 * g] - X0, r1 = x_segs[3 * g + 1] - X1, r2 = x_segs[3 * g + 2] - X2;
+        acc[0] += f[0];
+        acc[1] += f[1];
+        acc[2] += f[2];
+        acc[3] += f[3] + r1 * f[2] - r2 * f[1];
+        acc[4] += f[4] + r2 * f[0] - r0 * f[2];
+        acc[5] += f[5] + r0 * f[1] - r1 * f[0];
+    }
+    __shared__ double sm[6][256];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sm[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sm[k][threadIdx.x] += sm[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) rhs[3 * nblob + 6 * b + threadIdx.x] = sm[threadIdx.x][0] - (f_ext ? f_ext[6 * b + threadIdx.x] : 0.0);
+}
+
 // flag = 1 if a[i] != b[i] for some i (every writer stores the same value).
 __global__ __launch_bounds__(256) void differs_kernel(const double* __restrict__ a, const double* __restrict__ b,
                                                      int64_t n, int32_t* __restrict__ flag) {
@@ -65,7 +104,8 @@ struct tfs_velocities {
     tfs_solver* blobs = nullptr;  // GMRES on M_bb
     int64_t ns = 0, nb = 0, nswim = 0;
     double* d_rhs = nullptr;      // 3*nb
-    double* d_bodies = nullptr;   // 3*nswim positions then 6*nswim velocities
+    double* d_bodies = nullptr;   // 3*nswim positions then 6*nswim velocities (or external forces)
+    double* d_sol = nullptr;      // free bodies: [blob forces | body velocities]
     // The blob solve's geometry (and so its preconditioner blocks) is renewed
     // only when the blob positions change: flag in device memory, its host
     // copy in pinned memory.
@@ -144,6 +184,8 @@ int tfs_velocities_create(const tfs_velocities_desc* desc, tfs_velocities** out)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_create: bad sizes");
     if (D.memory != TFS_MEMORY_HOST && D.memory != TFS_MEMORY_DEVICE)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_create: unknown memory space");
+    if (D.free_bodies != 0 && D.free_bodies != 1)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_create: free_bodies is 0 or 1");
     if (D.mode != TFS_SOLVE_FP64 && D.mode != TFS_SOLVE_MIXED)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_create: unknown mode");
     const int64_t ns = (int64_t)D.num_swim * D.segs_per_swim, nb = (int64_t)D.num_swim * D.blobs_per_swim;
@@ -194,8 +236,12 @@ int tfs_velocities_create(const tfs_velocities_desc* desc, tfs_velocities** out)
     sd.inner_tol = D.inner_tol;
     sd.max_refinements = D.max_refinements;
     sd.stream = v->stream;
+    sd.num_swim = D.num_swim;
+    sd.free_bodies = D.free_bodies;
+    const int64_t nsys = 3 * nb + (D.free_bodies ? 6LL * D.num_swim : 0);
     if (!rc) rc = tfs_solver_create(&sd, &v->blobs);
-    if (!rc) rc = check(hipMalloc(&v->d_rhs, sizeof(double) * 3 * nb), "tfs_velocities_create: rhs");
+    if (!rc) rc = check(hipMalloc(&v->d_rhs, sizeof(double) * nsys), "tfs_velocities_create: rhs");
+    if (!rc && D.free_bodies) rc = check(hipMalloc(&v->d_sol, sizeof(double) * nsys), "tfs_velocities_create: solution");
     if (!rc) rc = check(hipMalloc(&v->d_bodies, sizeof(double) * 9 * v->nswim), "tfs_velocities_create: bodies");
     if (!rc) rc = check(hipMalloc(&v->d_flag, sizeof(int32_t)), "tfs_velocities_create: flag");
     if (!rc) rc = check(hipHostMalloc(&v->h_flag, sizeof(int32_t)), "tfs_velocities_create: flag");
@@ -216,6 +262,7 @@ int tfs_velocities_destroy(tfs_velocities* v) {
     if (v->rpy) tfs_rpy_destroy(v->rpy);
     if (v->d_rhs) (void)hipFree(v->d_rhs);
     if (v->d_bodies) (void)hipFree(v->d_bodies);
+    if (v->d_sol) (void)hipFree(v->d_sol);
     if (v->d_flag) (void)hipFree(v->d_flag);
     if (v->Ainv) (void)hipFree(v->Ainv);
     if (v->dense_arena) (void)hipFree(v->dense_arena);
@@ -225,11 +272,17 @@ int tfs_velocities_destroy(tfs_velocities* v) {
     return TFS_OK;
 }
 
-int tfs_velocities_compute(tfs_velocities* v, const double* x_segs, const double* f_segs, const double* x_blobs,
-                           const double* x_bodies, const double* v_bodies, double* v_segs, double* f_blobs,
-                           tfs_solve_stats* stats) {
-    if (!v || !x_blobs || !x_bodies || !v_bodies || !f_blobs || (v->ns && (!x_segs || !f_segs || !v_segs)))
-        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_compute: null argument");
+}  // extern "C"
+
+namespace {
+
+// The shared body of tfs_velocities_compute (prescribed body velocities:
+// `bodies` = V, Omega) and tfs_velocities_compute_free (`bodies` = external
+// force and torque or NULL; v_bodies_io the body velocities' warm start and
+// result).
+int compute(tfs_velocities* v, const double* x_segs, const double* f_segs, const double* x_blobs,
+            const double* x_bodies, const double* bodies, double* v_segs, double* f_blobs, double* v_bodies_io,
+            tfs_solve_stats* stats, bool free_bodies) {
     const hipMemcpyKind kin = v->desc.memory == TFS_MEMORY_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
     const hipMemcpyKind kout = v->desc.memory == TFS_MEMORY_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
     tfs_rpy* h = v->rpy;
@@ -261,14 +314,24 @@ int tfs_velocities_compute(tfs_velocities* v, const double* x_segs, const double
     if (!rc) rc = check(hipMemcpyAsync(h->x_blobs, x_blobs, 24 * nb, kin, st), "blob positions");
     if (!rc && ns) rc = check(hipMemcpyAsync(h->x_segs, x_segs, 24 * ns, kin, st), "segment positions");
     if (!rc && ns) rc = check(hipMemcpyAsync(h->f_segs, f_segs, 48 * ns, kin, st), "segment forces");
-    if (!rc) rc = check(hipMemcpyAsync(h->f_blobs, f_blobs, 24 * nb, kin, st), "blob warm start");
+    double* sol = free_bodies ? v->d_sol : h->f_blobs;
+    if (!rc) rc = check(hipMemcpyAsync(sol, f_blobs, 24 * nb, kin, st), "blob warm start");
+    if (!rc && free_bodies)
+        rc = check(hipMemcpyAsync(sol + 3 * nb, v_bodies_io, 48 * v->nswim, kin, st), "body warm start");
     if (!rc) rc = check(hipMemcpyAsync(v->d_bodies, x_bodies, 24 * v->nswim, kin, st), "body positions");
-    if (!rc) rc = check(hipMemcpyAsync(v->d_bodies + 3 * v->nswim, v_bodies, 48 * v->nswim, kin, st), "body velocities");
+    if (!rc && bodies)
+        rc = check(hipMemcpyAsync(v->d_bodies + 3 * v->nswim, bodies, 48 * v->nswim, kin, st), "body velocities");
     if (!rc) rc = tfs_rpy_positions_changed(h);
     // Mss_mult '=', Mbs_mult '='
     if (!rc && ns) rc = tfs_rpy_apply(h, TFS_BLOCK_SS, h->f_segs, nullptr, h->v_segs, nullptr);
     if (!rc && ns) rc = tfs_rpy_apply(h, TFS_BLOCK_BS, h->f_segs, nullptr, nullptr, h->v_blobs);
-    if (!rc) {
+    if (!rc && free_bodies) {
+        hipLaunchKernelGGL(free_rhs_kernel, dim3((unsigned)v->nswim), dim3(256), 0, st, h->x_segs, h->f_segs,
+                           h->v_blobs, nb, (int)v->desc.segs_per_swim, v->d_bodies,
+                           bodies ? v->d_bodies + 3 * v->nswim : nullptr, v->d_rhs);
+        rc = check(hipGetLastError(), "free-body rhs");
+        if (!rc) rc = tfs_solver_set_bodies(v->blobs, v->d_bodies, TFS_MEMORY_DEVICE);
+    } else if (!rc) {
         hipLaunchKernelGGL(blob_rhs_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, h->v_blobs,
                            h->x_blobs, v->d_bodies, v->d_bodies + 3 * v->nswim, (int)nb, v->desc.blobs_per_swim,
                            ns > 0 ? 1 : 0, v->d_rhs);
@@ -290,14 +353,18 @@ int tfs_velocities_compute(tfs_velocities* v, const double* x_segs, const double
             stats->converged = 1;
         }
     } else if (!rc) {
-        solve_rc = tfs_solver_solve(v->blobs, v->d_rhs, h->f_blobs, TFS_MEMORY_DEVICE, stats);
+        solve_rc = tfs_solver_solve(v->blobs, v->d_rhs, sol, TFS_MEMORY_DEVICE, stats);
         if (solve_rc != TFS_OK && solve_rc != TFS_ERR_NOT_CONVERGED) rc = solve_rc;
+        if (!rc && free_bodies)
+            rc = check(hipMemcpyAsync(h->f_blobs, sol, 24 * nb, hipMemcpyDeviceToDevice, st), "blob forces");
     }
     const std::string solve_msg = solve_rc ? g_last_error : std::string();
     // Msb_mult '+='
     if (!rc && ns) rc = tfs_rpy_apply(h, TFS_BLOCK_SB, nullptr, h->f_blobs, h->v_segs, nullptr);
     if (!rc && ns) rc = check(hipMemcpyAsync(v_segs, h->v_segs, 48 * ns, kout, st), "segment velocities");
     if (!rc) rc = check(hipMemcpyAsync(f_blobs, h->f_blobs, 24 * nb, kout, st), "blob forces");
+    if (!rc && free_bodies)
+        rc = check(hipMemcpyAsync(v_bodies_io, sol + 3 * nb, 48 * v->nswim, kout, st), "body velocities");
     if (!rc) rc = check(hipStreamSynchronize(st), "tfs_velocities_compute");
     if (rc) return rc;
     if (solve_rc) {
@@ -307,9 +374,39 @@ int tfs_velocities_compute(tfs_velocities* v, const double* x_segs, const double
     return TFS_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int tfs_velocities_compute(tfs_velocities* v, const double* x_segs, const double* f_segs, const double* x_blobs,
+                           const double* x_bodies, const double* v_bodies, double* v_segs, double* f_blobs,
+                           tfs_solve_stats* stats) {
+    if (!v || !x_blobs || !x_bodies || !v_bodies || !f_blobs || (v->ns && (!x_segs || !f_segs || !v_segs)))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_compute: null argument");
+    if (v->desc.free_bodies)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_compute: free bodies: use tfs_velocities_compute_free");
+    return compute(v, x_segs, f_segs, x_blobs, x_bodies, v_bodies, v_segs, f_blobs, nullptr, stats, false);
+}
+
+int tfs_velocities_compute_free(tfs_velocities* v, const double* x_segs, const double* f_segs, const double* x_blobs,
+                                const double* x_bodies, const double* f_bodies, double* v_segs, double* f_blobs,
+                                double* v_bodies, tfs_solve_stats* stats) {
+    if (!v || !x_blobs || !x_bodies || !v_bodies || !f_blobs || (v->ns && (!x_segs || !f_segs || !v_segs)))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_compute_free: null argument");
+    if (!v->desc.free_bodies)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_compute_free: the handle has no free bodies");
+    if (v->direct)
+        return fail(TFS_ERR_INVALID_ARGUMENT,
+                    "tfs_velocities_compute_free: free bodies take the GMRES blob solve (not TFS_BLOB_DIRECT)");
+    return compute(v, x_segs, f_segs, x_blobs, x_bodies, f_bodies, v_segs, f_blobs, v_bodies, stats, true);
+}
+
 int tfs_velocities_set_preconditioner(tfs_velocities* v, int32_t kind) {
     if (!v) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_set_preconditioner: null handle");
     if (kind == TFS_BLOB_DIRECT) {
+        if (v->desc.free_bodies)
+            return fail(TFS_ERR_INVALID_ARGUMENT,
+                        "tfs_velocities_set_preconditioner: free bodies take the GMRES blob solve");
         v->direct = true;
         return TFS_OK;
     }

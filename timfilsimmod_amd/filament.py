@@ -16,6 +16,7 @@ import numpy as np
 from . import _lib as L
 
 X, XM1, XM2, U, UM1, Q, MULT = range(7)
+BODY_X, BODY_XM1, BODY_XM2, BODY_U, BODY_UM1, BODY_Q = range(7, 13)
 
 
 class FilamentDesc(ctypes.Structure):
@@ -43,6 +44,9 @@ L.extra_signatures({
     "tfs_elastic_step": (ctypes.c_int, [_vp, _i32, ctypes.c_void_p]),
     "tfs_filament_device_array": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_vp)]),
     "tfs_filament_read": (ctypes.c_int, [_vp, _i32, _vp]),
+    "tfs_filament_set_body": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _vp]),
+    "tfs_filament_blob_positions": (ctypes.c_int, [_vp, _vp]),
+    "tfs_filament_body_coupling": (ctypes.c_int, [_vp, _i32, _vp, ctypes.c_double, _vp, _vp, _vp]),
 })
 
 
@@ -96,6 +100,33 @@ class FilamentModel:
         self._pert = self._arr(forces)
         L.check(L.lib().tfs_filament_set_perturbation(self.h, int(seg_a), int(seg_b), L.ptr(self._pert)))
 
+    def set_body(self, x_bodies, q_bodies, fil_refs, blobs_per_swim=0, blob_refs=None):
+        """Free bodies (swimmer.cpp:433-467, 1072-1098): each swimmer's body
+        starts at x_bodies [3], q_bodies [4] at rest; filament f is anchored at
+        X + R(q) fil_refs[f]; blob_refs [3 per blob] for blob_positions."""
+        self._body = [self._arr(a) for a in (x_bodies, q_bodies, fil_refs)]
+        self._blob_refs = None if blob_refs is None else self._arr(blob_refs)
+        self.nblob = self.desc.num_swim * int(blobs_per_swim)
+        L.check(L.lib().tfs_filament_set_body(self.h, *[L.ptr(a) for a in self._body], int(blobs_per_swim),
+                                              L.ptr(self._blob_refs)))
+
+    def blob_positions(self, out=None):
+        out = np.zeros(3 * self.nblob) if out is None else out
+        L.check(L.lib().tfs_filament_blob_positions(self.h, L.ptr(out)))
+        return out
+
+    def body_coupling(self, nt, body_mobility, sphere_radius, A=None, B=None, S0=None):
+        """J0's body coupling (swimmer.cpp:688-880) after jacobian() at the same
+        state: per filament A (6N x 6) and B (6 x 6N), per swimmer S0 (6 x 6)."""
+        n6, nf, ns = 6 * self.desc.num_seg, self.nfil, self.desc.num_swim
+        A = np.zeros(nf * 6 * n6) if A is None else A
+        B = np.zeros(nf * 6 * n6) if B is None else B
+        S0 = np.zeros(36 * ns) if S0 is None else S0
+        nm = self._arr(body_mobility)
+        L.check(L.lib().tfs_filament_body_coupling(self.h, int(nt), L.ptr(nm), float(sphere_radius), L.ptr(A),
+                                                   L.ptr(B), L.ptr(S0)))
+        return A, B, S0
+
     def initial_guess(self, nt):
         L.check(L.lib().tfs_filament_initial_guess(self.h, int(nt)))
 
@@ -123,6 +154,8 @@ class FilamentModel:
     def read(self, which):
         n = {X: 3, XM1: 3, XM2: 3, U: 3, UM1: 3, Q: 4}.get(which)
         size = n * self.ns if n else 3 * (self.desc.num_seg + 1) * self.nfil
+        if which >= BODY_X:
+            size = (4 if which == BODY_Q else 3) * self.desc.num_swim
         out = np.zeros(size)
         L.check(L.lib().tfs_filament_read(self.h, which, L.ptr(out)))
         return out
@@ -131,7 +164,8 @@ class FilamentModel:
 class ElasticDesc(ctypes.Structure):
     _fields_ = [("filaments", ctypes.c_void_p), ("velocities", ctypes.c_void_p), ("broyden", ctypes.c_void_p),
                 ("x_blobs", ctypes.c_void_p), ("x_bodies", ctypes.c_void_p), ("v_bodies", ctypes.c_void_p),
-                ("f_blobs", ctypes.c_void_p), ("tol", ctypes.c_double), ("max_iter", ctypes.c_int32)]
+                ("f_blobs", ctypes.c_void_p), ("tol", ctypes.c_double), ("max_iter", ctypes.c_int32),
+                ("body_mobility", ctypes.c_void_p), ("sphere_radius", ctypes.c_double)]
 
 
 class ElasticStats(ctypes.Structure):
@@ -144,15 +178,19 @@ class ElasticStats(ctypes.Structure):
 
 class ElasticStep:
     """One elastic implicit timestep on the device (tfs_elastic_step): the
-    time-loop body of cilia_sim_main.cu:353-502 for bodies held still.  The
-    handles must use device memory; x_blobs, x_bodies, v_bodies, f_blobs are
-    device tensors (f_blobs is the blob-force warm start and result)."""
+    time-loop body of cilia_sim_main.cu:353-502 for bodies held still, or free
+    bodies when the filament model has them (set_body; then x_blobs, x_bodies,
+    v_bodies may be None and body_mobility is a device tensor of 36 per
+    swimmer).  The handles must use device memory; x_blobs, x_bodies,
+    v_bodies, f_blobs are device tensors (f_blobs is the blob-force warm start
+    and result)."""
 
     def __init__(self, filaments, velocities, broyden, x_blobs, x_bodies, v_bodies, f_blobs, *, tol=1e-4,
-                 max_iter=400):
-        self._keep = (filaments, velocities, broyden, x_blobs, x_bodies, v_bodies, f_blobs)
+                 max_iter=400, body_mobility=None, sphere_radius=0.0):
+        self._keep = (filaments, velocities, broyden, x_blobs, x_bodies, v_bodies, f_blobs, body_mobility)
         d = ElasticDesc(filaments.h.value, velocities.h.value, broyden.h.value, L.ptr(x_blobs), L.ptr(x_bodies),
-                        L.ptr(v_bodies), L.ptr(f_blobs), float(tol), int(max_iter))
+                        L.ptr(v_bodies), L.ptr(f_blobs), float(tol), int(max_iter), L.ptr(body_mobility),
+                        float(sphere_radius))
         h = ctypes.c_void_p()
         L.check(L.lib().tfs_elastic_create(ctypes.byref(d), ctypes.byref(h)))
         self.h = h

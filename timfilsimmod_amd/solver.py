@@ -94,11 +94,12 @@ class VelocitiesDesc(ctypes.Structure):
         ("mode", ctypes.c_int32), ("seg_radius", ctypes.c_double), ("blob_radius", ctypes.c_double),
         ("mu", ctypes.c_double), ("restart", ctypes.c_int32), ("max_cycles", ctypes.c_int32),
         ("tol", ctypes.c_double), ("inner_tol", ctypes.c_double), ("max_refinements", ctypes.c_int32),
-        ("memory", ctypes.c_int32), ("stream", ctypes.c_void_p),
+        ("memory", ctypes.c_int32), ("stream", ctypes.c_void_p), ("free_bodies", ctypes.c_int32),
     ]
 
 
 L.extra_signatures({
+    "tfs_velocities_compute_free": (ctypes.c_int, [_dp] * 9 + [ctypes.POINTER(SolveStats)]),
     "tfs_velocities_create": (ctypes.c_int, [ctypes.POINTER(VelocitiesDesc), ctypes.POINTER(_dp)]),
     "tfs_velocities_destroy": (ctypes.c_int, [_dp]),
     "tfs_velocities_compute": (ctypes.c_int, [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(SolveStats)]),
@@ -194,8 +195,9 @@ class ElasticVelocities:
 
     def __init__(self, num_swim, segs_per_swim, blobs_per_swim, *, mode=SOLVE_FP64, seg_radius=1.0,
                  blob_radius=1.0, mu=1.0, restart=30, max_cycles=50, tol=1e-10, inner_tol=1e-3,
-                 max_refinements=10, stream=None, memory=L.MEMORY_HOST):
+                 max_refinements=10, stream=None, memory=L.MEMORY_HOST, free_bodies=False):
         d = VelocitiesDesc()
+        d.free_bodies = 1 if free_bodies else 0
         d.num_swim, d.segs_per_swim, d.blobs_per_swim, d.mode = num_swim, segs_per_swim, blobs_per_swim, mode
         d.seg_radius, d.blob_radius, d.mu = seg_radius, blob_radius, mu
         d.restart, d.max_cycles, d.tol = restart, max_cycles, tol
@@ -208,6 +210,8 @@ class ElasticVelocities:
         L.check(L.lib().tfs_velocities_create(ctypes.byref(d), ctypes.byref(h)))
         self.h = h
         self.f_blobs = np.zeros(3 * self.nb)
+        self.v_bodies = np.zeros(6 * num_swim)
+        self.num_swim = num_swim
 
     def close(self):
         if self.h:
@@ -240,6 +244,45 @@ class ElasticVelocities:
         if not (allow_unconverged and rc == L.TFS_ERR_NOT_CONVERGED):
             L.check(rc)
         return v_segs, self.f_blobs.copy(), st.as_dict()
+
+    def compute_free(self, x_segs, f_segs, x_blobs, x_bodies, f_bodies=None, *, allow_unconverged=False):
+        """Free bodies (a handle made with free_bodies=True): returns (v_segs,
+        f_blobs, v_bodies [V, Omega per body], solve stats); the blob forces and
+        body velocities warm-start the next call.  f_bodies: external force and
+        torque per body, or None.  Host memory only."""
+        if self.desc.memory != L.MEMORY_HOST:
+            raise ValueError("compute_free() takes host arrays; this handle was made for device memory")
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (x_segs, f_segs, x_blobs, x_bodies)]
+        fb = None if f_bodies is None else np.ascontiguousarray(f_bodies, dtype=np.float64)
+        v_segs = np.zeros(6 * self.ns)
+        st = SolveStats()
+        rc = L.lib().tfs_velocities_compute_free(self.h, *[L.ptr(v) if v.size else None for v in a], L.ptr(fb),
+                                                 L.ptr(v_segs) if v_segs.size else None, L.ptr(self.f_blobs),
+                                                 L.ptr(self.v_bodies), ctypes.byref(st))
+        if not (allow_unconverged and rc == L.TFS_ERR_NOT_CONVERGED):
+            L.check(rc)
+        return v_segs, self.f_blobs.copy(), self.v_bodies.copy(), st.as_dict()
+
+    def body_mobility(self, x_blobs, x_bodies):
+        """make_body_reference_matrices (mobility_solver.cpp:2950-3047): per body
+        the 6 x 6 mobility, column k the velocities V, Omega under a unit force
+        (k < 3) or torque on the body with no filament forces, by six free-body
+        solves at the given pose.  With several bodies each column loads every
+        body at once, so the matrices include the bodies' mutual flow (the
+        reference moves the other bodies far away first)."""
+        xs = np.zeros(3 * self.ns)
+        fs = np.zeros(6 * self.ns)
+        out = np.zeros((self.num_swim, 6, 6))
+        for k in range(6):
+            fe = np.zeros((self.num_swim, 6))
+            fe[:, k] = 1.0
+            self.f_blobs[:] = 0.0
+            self.v_bodies[:] = 0.0
+            _, _, v, _ = self.compute_free(xs, fs, x_blobs, x_bodies, fe.reshape(-1))
+            out[:, :, k] = v.reshape(self.num_swim, 6)
+        self.f_blobs[:] = 0.0
+        self.v_bodies[:] = 0.0
+        return out
 
 
 @dataclass
