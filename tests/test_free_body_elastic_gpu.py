@@ -77,67 +77,80 @@ def test_body_mobility_matches_oracle():
     assert rel(N, ref) < 1e-9
 
 
-def swimmer_pair(nfil=3, nseg=8, nb=40, seed=0, memory=L.MEMORY_HOST):
-    """The same free swimmer on the device and on the oracle."""
+def swimmer_pair(nswim=1, nfil=3, nseg=8, nb=40, seed=0):
+    """The same free swimmers on the device (one handle) and on the oracle (one
+    restatement per swimmer)."""
     rng = np.random.default_rng(seed)
     p = F.Params(nseg=nseg, dl=2.2, kb=1800.0, kt=1300.0, dt=0.2, end_force=150.0)
-    xb0 = np.array([0.5, -1.0, 2.0])
-    qb0 = np.roll(Rotation.random(random_state=seed).as_quat(), 1)
-    refs = [7.0 * v / np.linalg.norm(v) for v in rng.standard_normal((nfil, 3))]
-    blob_refs = G.fibonacci_sphere(nb, 5.0)
-    qs = []
-    for i in range(nfil):
-        q = [np.roll(Rotation.random(random_state=seed + 20 + i).as_quat(), 1)]
-        for _ in range(nseg - 1):
-            q.append(F.qmul(F.lie_exp(0.1 * rng.standard_normal(3)), q[-1]))
-        qs.append(np.array(q))
-    R0 = F.rot_mat(qb0)
-    bases = np.array([xb0 + R0 @ r for r in refs])
-    model = FM.FilamentModel(1, nfil, nseg, dl=p.dl, kb=p.kb, kt=p.kt, dt=p.dt, end_force=p.end_force)
-    model.set_state(bases.reshape(-1), np.array(qs).reshape(-1))
-    model.set_body(xb0, qb0, np.array(refs).reshape(-1), nb, blob_refs.reshape(-1))
-    body = S.Body.at_rest(xb0, qb0)
-    fils = [F.Filament.at_rest(bases[i], qs[i], nseg, p.dl) for i in range(nfil)]
-    return p, model, body, fils, refs, blob_refs, rng
+    xb0 = np.array([[0.5, -1.0, 2.0], [30.0, 4.0, -3.0]])[:nswim]
+    qb0 = np.array([np.roll(Rotation.random(random_state=seed + s).as_quat(), 1) for s in range(nswim)])
+    refs = [[7.0 * v / np.linalg.norm(v) for v in rng.standard_normal((nfil, 3))] for _ in range(nswim)]
+    blob_refs = np.concatenate([G.fibonacci_sphere(nb, 5.0 + s) for s in range(nswim)])
+    qs, bases = [], []
+    for s in range(nswim):
+        R0 = F.rot_mat(qb0[s])
+        for i in range(nfil):
+            q = [np.roll(Rotation.random(random_state=seed + 20 + 10 * s + i).as_quat(), 1)]
+            for _ in range(nseg - 1):
+                q.append(F.qmul(F.lie_exp(0.1 * rng.standard_normal(3)), q[-1]))
+            qs.append(np.array(q))
+            bases.append(xb0[s] + R0 @ refs[s][i])
+    model = FM.FilamentModel(nswim, nfil, nseg, dl=p.dl, kb=p.kb, kt=p.kt, dt=p.dt, end_force=p.end_force)
+    model.set_state(np.array(bases).reshape(-1), np.array(qs).reshape(-1))
+    model.set_body(xb0.reshape(-1), qb0.reshape(-1), np.array(refs).reshape(-1), nb, blob_refs.reshape(-1))
+    swimmers = []
+    for s in range(nswim):
+        fils = [F.Filament.at_rest(bases[s * nfil + i], qs[s * nfil + i], nseg, p.dl) for i in range(nfil)]
+        swimmers.append((S.Body.at_rest(xb0[s], qb0[s]), fils, refs[s], blob_refs[s * nb:(s + 1) * nb]))
+    return p, model, swimmers, rng
 
 
-@pytest.mark.parametrize("sphere", [0.0, 5.0])
-def test_swimmer_state_and_body_coupling_match_oracle(sphere):
-    print(f"case swimmer state and J0 coupling 3x8, sphere {sphere}", flush=True)
-    p, model, body, fils, refs, blob_refs, rng = swimmer_pair(seed=3)
-    nfil, N = len(fils), p.nseg
+@pytest.mark.parametrize("nswim,sphere", [(1, 0.0), (1, 5.0), (2, 5.0)])
+def test_swimmer_state_and_body_coupling_match_oracle(nswim, sphere):
+    print(f"case swimmer state and J0 coupling {nswim} x 3 x 8, sphere {sphere}", flush=True)
+    p, model, swimmers, rng = swimmer_pair(nswim=nswim, seed=3)
+    nfil, N = 3, p.nseg
     n = 6 * (nfil * N + 1)
-    N_ref = np.diag([0.02, 0.021, 0.019, 0.001, 0.0011, 0.0009]) + 1e-4 * rng.standard_normal((6, 6))
+    N_ref = [np.diag([0.02, 0.021, 0.019, 0.001, 0.0011, 0.0009]) + 1e-4 * rng.standard_normal((6, 6))
+             for _ in range(nswim)]
+    cat = lambda name: np.concatenate([getattr(f, name).reshape(-1) for _, fils, _, _ in swimmers for f in fils])
     try:
         for nt in range(3):
             model.initial_guess(nt)
-            S.swimmer_initial_guess(body, fils, refs, nt, p)
-            du = rng.standard_normal(n) * 0.02
-            du[-6:] *= 0.5
+            du = rng.standard_normal(n * nswim) * 0.02
+            for s, (body, fils, refs, _) in enumerate(swimmers):
+                du[n * s + n - 6:n * (s + 1)] *= 0.5
+                S.swimmer_initial_guess(body, fils, refs, nt, p)
+                S.swimmer_update(body, fils, refs, du[n * s:n * (s + 1)], p)
             model.update(du)
-            S.swimmer_update(body, fils, refs, du, p)
-            assert rel(model.read(FM.X), np.concatenate([f.x.reshape(-1) for f in fils])) < 1e-12
-            assert rel(model.read(FM.Q), np.concatenate([f.q.reshape(-1) for f in fils])) < 1e-12
-            assert rel(model.read(FM.BODY_X), body.x) < 1e-14 and rel(model.read(FM.BODY_Q), body.q) < 1e-14
-            assert rel(model.read(FM.BODY_U), body.u) < 1e-14
-            assert rel(model.blob_positions(), S.blob_positions(body, blob_refs)) < 1e-13
+            assert rel(model.read(FM.X), cat("x")) < 1e-12 and rel(model.read(FM.Q), cat("q")) < 1e-12
+            bodies = [sw[0] for sw in swimmers]
+            assert rel(model.read(FM.BODY_X), np.concatenate([b.x for b in bodies])) < 1e-14
+            assert rel(model.read(FM.BODY_Q), np.concatenate([b.q for b in bodies])) < 1e-14
+            assert rel(model.read(FM.BODY_U), np.concatenate([b.u for b in bodies])) < 1e-14
+            want = np.concatenate([S.blob_positions(b, br) for b, _, _, br in swimmers])
+            assert rel(model.blob_positions(), want) < 1e-13
         nt = 2
         f = model.forces(nt)
-        forces = [F.internal_forces(fl, p) for fl in fils]
-        assert rel(f, np.concatenate([fc.reshape(-1) for fc in forces])) < 1e-12
-        blocks = np.zeros(nfil * (6 * N) ** 2 + 36)
+        forces = [[F.internal_forces(fl, p) for fl in fils] for _, fils, _, _ in swimmers]
+        assert rel(f, np.concatenate([fc.reshape(-1) for fs in forces for fc in fs])) < 1e-12
+        blocks = np.zeros(nswim * (nfil * (6 * N) ** 2 + 36))
         model.jacobian(nt, f, blocks)
-        A, B, S0 = model.body_coupling(nt, N_ref.reshape(-1), sphere)
+        A, B, S0 = model.body_coupling(nt, np.array(N_ref).reshape(-1), sphere)
     finally:
         model.close()
-    S0r, As, Bs = S.body_coupling(body, fils, refs, forces, nt, N_ref, p, sphere)
-    n6 = 6 * N
-    print("rel err A", rel(A, np.concatenate([a.reshape(-1) for a in As])), "B",
-          rel(B, np.concatenate([b.reshape(-1) for b in Bs])), "S0", rel(S0.reshape(6, 6), S0r), flush=True)
-    assert rel(A, np.concatenate([a.reshape(-1) for a in As])) < 1e-11
-    assert rel(B, np.concatenate([b.reshape(-1) for b in Bs])) < 1e-11
-    assert rel(S0.reshape(6, 6), S0r) < 1e-11
-    assert A.size == nfil * n6 * 6
+    As, Bs, S0s = [], [], []
+    for s, (body, fils, refs, _) in enumerate(swimmers):
+        S0r, Ar, Br = S.body_coupling(body, fils, refs, forces[s], nt, N_ref[s], p, sphere)
+        As += Ar
+        Bs += Br
+        S0s.append(S0r)
+    want_A = np.concatenate([a.reshape(-1) for a in As])
+    want_B = np.concatenate([b.reshape(-1) for b in Bs])
+    want_S = np.concatenate([m.reshape(-1) for m in S0s])
+    print("rel err A", rel(A, want_A), "B", rel(B, want_B), "S0", rel(S0, want_S), flush=True)
+    assert rel(A, want_A) < 1e-11 and rel(B, want_B) < 1e-11 and rel(S0, want_S) < 1e-11
+    assert A.size == nswim * nfil * 6 * N * 6
 
 
 def test_coupled_broyden_inverse_matches_assembled_jacobian():
