@@ -77,8 +77,8 @@ int tfs_broyden_end_of_iter(tfs_broyden *b, const double *new_error, int32_t err
 int tfs_broyden_status(tfs_broyden *b, int32_t *iter, double *error_norm);
 
 /* State of the implicit step for mobility_solver::compute_errors
- * (mobility_solver.cpp:2661-2946, bodies not in the Broyden-for-everything
- * form).  Pointers live in the desc's memory space; 3 values per segment or
+ * (mobility_solver.cpp:2661-2946; free bodies only outside the
+ * Broyden-for-everything form).  Pointers live in the desc's memory space; 3 values per segment or
  * body unless stated. */
 typedef struct tfs_broyden_state {
     int32_t num_swim, num_fil, num_seg;  /* NSWIM, NFIL, NSEG: n = 6*(NSWIM*NFIL*NSEG + NSWIM) */
@@ -91,6 +91,13 @@ typedef struct tfs_broyden_state {
     const double *v_segs;                      /* 6 per segment: velocity, angular velocity   */
     const double *body_x, *body_xm1, *body_xm2, *body_u, *body_um1;
     const double *v_bodies;                    /* 6 per body                                  */
+    /* blobs_per_swim > 0: the Broyden-for-everything form (USE_BROYDEN_FOR_EVERYTHING,
+     * bodies of prescribed velocity, mobility_solver.cpp:2674-2684, 2774-2850): per
+     * swimmer its 3 NBLOB rows V + Omega x (x_blob - x_body) - v_blob follow the body's
+     * rows (tolerance TOL) and n = 6*(NSWIM*NFIL*NSEG + NSWIM) + 3*NSWIM*NBLOB.  0: the
+     * form above. */
+    int32_t blobs_per_swim;
+    const double *x_blobs, *v_blobs;           /* 3 per blob                                  */
 } tfs_broyden_state;
 
 /* Residuals of the BDF2 (or backward-Euler) equations for positions and

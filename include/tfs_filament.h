@@ -142,6 +142,15 @@ int tfs_filament_read(tfs_filament *h, int32_t which, double *out);
  * tfs_velocities_compute_free, the residual has the body rows, and J0^{-1}
  * takes the body's Schur complement (tfs_filament_body_coupling,
  * tfs_broyden_set_body_coupling) with body_mobility and sphere_radius below.
+ * Broyden for everything (USE_BROYDEN_FOR_EVERYTHING, bodies held still):
+ * broyden_blobs = NBLOB makes the blob forces Broyden unknowns, 3 NBLOB per
+ * swimmer after its body's 6 (the tfs_broyden handle is made with the blocks
+ * [6 NSEG] * NFIL, [6], [3] * NBLOB per swimmer).  The velocities are one
+ * plain matvec (Mss, Msb, Mbb, Mbs '+=', mobility_solver.cpp:2603-2640; the
+ * velocities handle is not used and may be NULL), the residual has the blob
+ * rows, f_blobs holds the blob forces, extrapolated at each step's start and
+ * updated with the Broyden update (rigid_body.cpp:447-478, 506-520), and
+ * J0^{-1} scales the blob rows by -6 pi mu RBLOB (swimmer.cpp:931-945).
  * The three handles are the caller's, created with TFS_MEMORY_DEVICE and sizes
  * that agree (tfs_velocities: num_swim bodies of fil_per_swim*num_seg segments;
  * tfs_broyden: the block sizes of the Broyden layout above).  Only the
@@ -160,6 +169,8 @@ typedef struct tfs_elastic_desc {
     int32_t max_iter;          /* MAX_BROYDEN_ITER                                     */
     const double *body_mobility; /* free bodies: device, 36 per swimmer (body frame)   */
     double sphere_radius;      /* free bodies: see tfs_filament_body_coupling          */
+    int32_t broyden_blobs;     /* > 0: NBLOB, the Broyden-for-everything form (below)  */
+    double blob_radius;        /* RBLOB, for that form's matvec                         */
 } tfs_elastic_desc;
 
 typedef struct tfs_elastic_stats {

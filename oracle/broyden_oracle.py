@@ -134,14 +134,19 @@ def dexpinv(u, v):
     return (v - 0.5 * c) - fac * w
 
 
-def compute_errors(st, num_swim, num_fil, num_seg, bdf2, dt, dl, tol):
+def compute_errors(st, num_swim, num_fil, num_seg, bdf2, dt, dl, tol, blobs_per_swim=0):
     """(error, error_is_too_large) of mobility_solver.cpp:2661-2946.
 
     ``st`` holds seg_x, seg_xm1, seg_xm2, seg_u, seg_um1 (3 per segment),
     v_segs (6 per segment) and body_x, body_xm1, body_xm2, body_u, body_um1
-    (3 per body), v_bodies (6 per body)."""
+    (3 per body), v_bodies (6 per body).  blobs_per_swim > 0 is the
+    Broyden-for-everything form with bodies of prescribed velocity
+    (USE_BROYDEN_FOR_EVERYTHING, :2674-2684, 2774-2850): per swimmer the 3 NBLOB
+    blob rows V + Omega x (x_blob - x_body) - v_blob follow the body's rows;
+    st then also holds x_blobs and v_blobs (3 per blob)."""
     c1, c2, c3 = -4.0 / 3.0, 1.0 / 3.0, -2.0 / 3.0
-    n = 6 * (num_swim * num_fil * num_seg + num_swim)
+    per = 6 * (num_fil * num_seg + 1) + 3 * blobs_per_swim
+    n = num_swim * per
     err = np.zeros(n)
     too_large = False
 
@@ -164,7 +169,7 @@ def compute_errors(st, num_swim, num_fil, num_seg, bdf2, dt, dl, tol):
     vs = np.asarray(st.get("v_segs", np.zeros(0)), dtype=np.float64)
     vb = np.asarray(st["v_bodies"], dtype=np.float64)
     for s in range(num_swim):
-        base = s * 6 * (num_fil * num_seg + 1)
+        base = s * per
         for i in range(num_fil):
             for j in range(num_seg):
                 g = j + num_seg * (i + s * num_fil)
@@ -175,4 +180,12 @@ def compute_errors(st, num_swim, num_fil, num_seg, bdf2, dt, dl, tol):
         pid = base + 6 * num_fil * num_seg
         too_large |= one(g3("body_x", s), g3("body_xm1", s), g3("body_xm2", s), g3("body_u", s),
                          g3("body_um1", s), vb[6 * s:6 * s + 3], vb[6 * s + 3:6 * s + 6], pid, pid + 3)
+        for m in range(blobs_per_swim):
+            g = s * blobs_per_swim + m
+            diff = g3("x_blobs", g) - g3("body_x", s)
+            w = np.cross(vb[6 * s + 3:6 * s + 6], diff)
+            e = vb[6 * s:6 * s + 3] + w - g3("v_blobs", g)
+            rid = base + 6 * (num_fil * num_seg + 1) + 3 * m
+            err[rid:rid + 3] = e
+            too_large |= bool(np.any(np.abs(e) > tol))
     return err, too_large

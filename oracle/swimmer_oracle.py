@@ -389,3 +389,85 @@ def free_swimmer_loop(b: Body, fils, refs, blob_refs, p: F.Params, steps, tol, p
         out.append({"iters": ref.iter, "converged": not big, "x": cat("x"), "body": b.copy(),
                     "forces": [fc.copy() for fc in forces], "f_blobs": lam, "v_body": V, "error": e})
     return out
+
+
+def blob_force_guess(fb, fbm1, fbm2, nt):
+    """The blob forces' extrapolation of rigid_body::initial_guess in the
+    Broyden-for-everything form (rigid_body.cpp:447-478)."""
+    if nt == 1:
+        fbm1 = fb.copy()
+    elif nt == 2:
+        fbm2, fbm1 = fbm1.copy(), fb.copy()
+        fb = 2.0 * fbm1 - fbm2
+    elif nt > 2:
+        t = 3.0 * (fb - fbm1) + fbm2
+        fbm2, fbm1, fb = fbm1.copy(), fb.copy(), t
+    return fb, fbm1, fbm2
+
+
+def held_still_loop(fils, x_blobs, p: F.Params, steps, tol, perturbation, broyden_blobs, blob_radius=1.0,
+                    max_iter=400, confidence=0.1):
+    """The elastic time loop (cilia_sim_main.cu:353-502) for one body held
+    still, in either solver form: broyden_blobs=False solves the blob forces
+    each evaluation (SOLVER_TYPE 1; a dense solve here), True makes them
+    Broyden unknowns (USE_BROYDEN_FOR_EVERYTHING): the velocities are then one
+    plain matvec (mobility_solver.cpp:2603-2640), the residual has the blob
+    rows, the blob forces are extrapolated at each step's start and updated by
+    the Broyden update (rigid_body.cpp:506-520), and J0^{-1} scales the blob
+    rows by -6 pi mu a (swimmer.cpp:931-945)."""
+    import scipy.linalg as sla
+
+    from oracle import broyden_oracle as BO
+    from oracle import oracle as O
+
+    N, nfil = p.nseg, len(fils)
+    xb = np.asarray(x_blobs, dtype=float).reshape(-1)
+    nb3 = xb.size
+    n = 6 * (nfil * N + 1) + (nb3 if broyden_blobs else 0)
+    cat = lambda name: np.concatenate([getattr(fl, name).reshape(-1) for fl in fils])
+    lu = None if broyden_blobs else sla.lu_factor(O.dense_mobility(np.zeros(0), xb, a_blob=blob_radius))
+    fb, fbm1, fbm2 = np.zeros(nb3), np.zeros(nb3), np.zeros(nb3)
+    z3, z6 = np.zeros(3), np.zeros(6)
+    out = []
+    for nt in range(steps):
+        for fl in fils:
+            F.initial_guess(fl, nt, p)
+        if broyden_blobs:
+            fb, fbm1, fbm2 = blob_force_guess(fb, fbm1, fbm2, nt)
+
+        def evaluate(fb):
+            kicks = [{4: perturbation[i, :3], 12: perturbation[i, 3:]} if nt == 0 else None for i in range(nfil)]
+            f = np.concatenate([F.internal_forces(fl, p, kicks[i]).reshape(-1) for i, fl in enumerate(fils)])
+            x = cat("x")
+            kw = dict(seg_dof=6, a_blob=blob_radius)
+            if broyden_blobs:
+                vb = O.block("bb", x, xb, f, fb, **kw)
+                vb = O.block("bs", x, xb, f, fb, vb, bs_accumulate=True, **kw)
+            else:
+                fb = sla.lu_solve(lu, -O.block("bs", x, xb, f, np.zeros(nb3), bs_accumulate=False, **kw))
+                vb = None
+            vs = O.block("sb", x, xb, f, fb, O.block("ss", x, xb, f, fb, **kw), **kw)
+            st = {"seg_x": x, "seg_xm1": cat("xm1"), "seg_xm2": cat("xm2"), "seg_u": cat("u"),
+                  "seg_um1": cat("um1"), "v_segs": vs, "body_x": z3, "body_xm1": z3, "body_xm2": z3,
+                  "body_u": z3, "body_um1": z3, "v_bodies": z6, "x_blobs": xb, "v_blobs": vb}
+            e, big = BO.compute_errors(st, 1, nfil, N, nt >= 1, p.dt, p.dl, tol,
+                                       blobs_per_swim=nb3 // 3 if broyden_blobs else 0)
+            return f, e, big, fb
+
+        f, e, big, fb_eval = evaluate(fb)
+        blocks = [np.linalg.inv(F.approx_jacobian(fl, f.reshape(nfil, N, 6)[i], nt >= 1, p))
+                  for i, fl in enumerate(fils)] + [np.eye(6)]
+        if broyden_blobs:
+            blocks += [-6.0 * PI * p.mu * blob_radius * np.eye(3)] * (nb3 // 3)
+        ref = BO.BroydenRef(n, max_iter, confidence, blocks)
+        ref.start(e)
+        while big and ref.iter < max_iter:
+            du = ref.find_update()
+            for i, fl in enumerate(fils):
+                F.update(fl, du[6 * N * i:6 * N * (i + 1)], p)
+            if broyden_blobs:
+                fb = fb + du[6 * (nfil * N + 1):]
+            f, e, big, fb_eval = evaluate(fb)
+            ref.end_of_iter(e, big)
+        out.append({"iters": ref.iter, "converged": not big, "x": cat("x"), "f_blobs": fb_eval.copy()})
+    return out

@@ -10,7 +10,9 @@
   own self-check, mobility_solver.cpp:3017-3033);
 * a free swimmer's implicit step converges, fewer Broyden iterations with the
   body's Schur complement than with an identity body block, and its converged
-  state is force-free."""
+  state is force-free;
+* the Broyden-for-everything form (blob forces as Broyden unknowns) reaches
+  the trajectory of the form that solves for them at every evaluation."""
 import numpy as np
 import pytest
 from scipy.spatial.transform import Rotation
@@ -167,3 +169,23 @@ def test_free_swimmer_step_converges_and_is_force_free():
     # force-free: the blob forces balance the segments' forces
     Fseg = np.concatenate([f[:, :3] for f in last["forces"]]).sum(0)
     assert np.abs(Fseg + last["f_blobs"].reshape(-1, 3).sum(0)).max() <= 1e-8 * np.abs(Fseg).max() + 1e-9
+
+
+def test_broyden_for_everything_reaches_the_gmres_form_trajectory():
+    """Two solver forms of the same implicit step (blob forces solved for each
+    evaluation, or carried as Broyden unknowns) converge to the same state."""
+    c = free_case()
+    body, p = c["body"], c["p"]
+    runs = {}
+    for mode in (False, True):
+        q0 = [f.q[0] for f in c["fresh"]()[1]]
+        fils = [F.Filament.at_rest(body.fil_base[i], np.tile(q0[i], (p.nseg, 1)), p.nseg, p.dl)
+                for i in range(body.num_fil)]
+        runs[mode] = S.held_still_loop(fils, body.x_blobs, p, 2, c["tol"], c["pert"], mode)
+    print("Broyden iterations, GMRES form", [r["iters"] for r in runs[False]], "for everything",
+          [r["iters"] for r in runs[True]])
+    for a, b in zip(runs[False], runs[True]):
+        assert a["converged"] and b["converged"]
+        assert np.abs(a["x"] - b["x"]).max() < 1e-7
+        assert np.linalg.norm(a["f_blobs"] - b["f_blobs"]) < 1e-2 * np.linalg.norm(a["f_blobs"])
+    assert all(b["iters"] > a["iters"] for a, b in zip(runs[False], runs[True]))

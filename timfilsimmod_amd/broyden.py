@@ -33,6 +33,7 @@ class BroydenState(ctypes.Structure):
         ("seg_u", ctypes.c_void_p), ("seg_um1", ctypes.c_void_p), ("v_segs", ctypes.c_void_p),
         ("body_x", ctypes.c_void_p), ("body_xm1", ctypes.c_void_p), ("body_xm2", ctypes.c_void_p),
         ("body_u", ctypes.c_void_p), ("body_um1", ctypes.c_void_p), ("v_bodies", ctypes.c_void_p),
+        ("blobs_per_swim", ctypes.c_int32), ("x_blobs", ctypes.c_void_p), ("v_blobs", ctypes.c_void_p),
     ]
 
 
@@ -143,14 +144,19 @@ class BroydenSolver:
         L.check(L.lib().tfs_broyden_status(self.h, None, ctypes.byref(v)))
         return v.value
 
-    def compute_errors(self, state: dict, slot: int, *, num_swim, num_fil, num_seg, bdf2, dt, dl, tol=TOL):
+    def compute_errors(self, state: dict, slot: int, *, num_swim, num_fil, num_seg, bdf2, dt, dl, tol=TOL,
+                       blobs_per_swim=0):
         """Residuals into slot 0 (error) or 1 (new_error); returns error_is_too_large.
-        ``state`` maps the tfs_broyden_state array names to arrays."""
+        ``state`` maps the tfs_broyden_state array names to arrays; blobs_per_swim > 0
+        is the Broyden-for-everything form (x_blobs, v_blobs in ``state``)."""
         st = BroydenState()
         st.num_swim, st.num_fil, st.num_seg, st.bdf2 = num_swim, num_fil, num_seg, int(bool(bdf2))
         st.dt, st.dl, st.tol = dt, dl, tol
+        st.blobs_per_swim = int(blobs_per_swim)
         keep = []
         for name, _ in BroydenState._fields_[7:]:
+            if name == "blobs_per_swim":
+                continue
             a = _f64(state.get(name))
             keep.append(a)
             setattr(st, name, L.ptr(a) if a is not None and (hasattr(a, "data_ptr") or a.size) else None)

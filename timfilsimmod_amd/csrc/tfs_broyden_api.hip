@@ -179,6 +179,8 @@ struct ResidualArgs {
     double dt, pos_tol, lie_tol;
     const double *x, *xm1, *xm2, *u, *um1, *v;
     const double *bx, *bxm1, *bxm2, *bu, *bum1, *bv;
+    int nblob;                        // blobs per swimmer (Broyden-for-everything form) or 0
+    const double *xb, *vb;            // 3 per blob
 };
 
 // One thread per segment (then one per body): 3 position and 3 Lie-algebra
@@ -194,8 +196,25 @@ __global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a, double* _
     const int64_t nsegs = (int64_t)a.nswim * a.nfil * a.nseg;
     const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const double c1 = -4.0 / 3.0, c2 = 1.0 / 3.0, c3 = -2.0 / 3.0;
+    // per-swimmer stride of the Broyden vector
+    const int64_t per = 6 * ((int64_t)a.nfil * a.nseg + 1) + 3LL * a.nblob;
     double ratio = 0.0, nonfinite = 0.0;
-    if (g < nsegs + a.nswim) {
+    if (g >= nsegs + a.nswim && g < nsegs + a.nswim + (int64_t)a.nswim * a.nblob) {
+        // blob rows (mobility_solver.cpp:2812-2835): V + Omega x (x_blob - X) - v_blob
+        const int64_t m = g - nsegs - a.nswim;
+        const int64_t n = m / a.nblob, k = m % a.nblob;
+        const double* V = a.bv + 6 * n;
+        double d[3];
+        for (int c = 0; c < 3; ++c) d[c] = a.xb[3 * m + c] - a.bx[3 * n + c];
+        const double w[3] = {V[4] * d[2] - V[5] * d[1], V[5] * d[0] - V[3] * d[2], V[3] * d[1] - V[4] * d[0]};
+        const int64_t rid = n * per + 6 * ((int64_t)a.nfil * a.nseg + 1) + 3 * k;
+        for (int c = 0; c < 3; ++c) {
+            const double e = V[c] + w[c] - a.vb[3 * m + c];
+            err[rid + c] = e;
+            if (!isfinite(e)) nonfinite = 1.0;
+            ratio = fmax(ratio, fabs(e) / a.lie_tol);
+        }
+    } else if (g < nsegs + a.nswim) {
         const bool body = g >= nsegs;
         int64_t pid, lid;
         const double *x, *xm1, *xm2, *u, *um1, *vel, *om;
@@ -203,7 +222,7 @@ __global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a, double* _
             const int64_t n = g / ((int64_t)a.nfil * a.nseg);
             const int64_t r = g % ((int64_t)a.nfil * a.nseg);
             const int64_t i = r / a.nseg, j = r % a.nseg;
-            pid = n * 6 * ((int64_t)a.nfil * a.nseg + 1) + 6 * i * a.nseg + 3 * j;
+            pid = n * per + 6 * i * a.nseg + 3 * j;
             lid = pid + 3 * a.nseg;
             x = a.x + 3 * g;
             xm1 = a.xm1 + 3 * g;
@@ -214,7 +233,7 @@ __global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a, double* _
             om = a.v + 6 * g + 3;
         } else {
             const int64_t n = g - nsegs;
-            pid = n * 6 * ((int64_t)a.nfil * a.nseg + 1) + 6 * (int64_t)a.nfil * a.nseg;
+            pid = n * per + 6 * (int64_t)a.nfil * a.nseg;
             lid = pid + 3;
             x = a.bx + 3 * n;
             xm1 = a.bxm1 + 3 * n;
@@ -393,7 +412,7 @@ int tfs_broyden_create(const tfs_broyden_desc* desc, tfs_broyden** out) {
     b->ld = round32(D.n);
     b->jac_len = mats;
     b->G = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (D.n + 255) / 256));
-    b->RB = (int)((D.n / 6 + 256) / 256 + 1);   // residual blocks: one thread per segment or body
+    b->RB = (int)((D.n / 3 + 256) / 256 + 1);   // residual blocks: one thread per segment, body or blob
     b->host_status.resize(D.num_blocks);
     b->host_sizes = sz;
     b->host_moff = moff;
@@ -649,9 +668,13 @@ int tfs_broyden_compute_errors(tfs_broyden* b, const tfs_broyden_state* st, int3
                                int32_t* error_is_too_large) {
     if (!b || !st || (slot != 0 && slot != 1))
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_compute_errors: null argument or bad slot");
-    if (st->num_swim < 1 || st->num_fil < 0 || st->num_seg < 0 ||
-        6LL * ((int64_t)st->num_swim * st->num_fil * st->num_seg + st->num_swim) != b->n)
-        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_compute_errors: 6*(NSWIM*NFIL*NSEG + NSWIM) != n");
+    if (st->num_swim < 1 || st->num_fil < 0 || st->num_seg < 0 || st->blobs_per_swim < 0 ||
+        6LL * ((int64_t)st->num_swim * st->num_fil * st->num_seg + st->num_swim) +
+                3LL * st->num_swim * st->blobs_per_swim != b->n)
+        return fail(TFS_ERR_INVALID_ARGUMENT,
+                    "tfs_broyden_compute_errors: 6*(NSWIM*NFIL*NSEG + NSWIM) + 3*NSWIM*blobs_per_swim != n");
+    if (st->blobs_per_swim > 0 && (!st->x_blobs || !st->v_blobs))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_compute_errors: blob rows need x_blobs and v_blobs");
     const bool nsegs = (int64_t)st->num_fil * st->num_seg > 0;
     if (!(st->dt > 0) || !(st->dl > 0) || !(st->tol > 0) || !st->body_x || !st->body_xm1 || !st->body_u ||
         !st->v_bodies || (st->bdf2 && (!st->body_xm2 || !st->body_um1)) ||
@@ -690,11 +713,16 @@ int tfs_broyden_compute_errors(tfs_broyden* b, const tfs_broyden_state* st, int3
     a.bu = stage(st->body_u, 3 * nb);
     a.bum1 = a.bdf2 ? stage(st->body_um1, 3 * nb) : nullptr;
     a.bv = stage(st->v_bodies, 6 * nb);
+    const int64_t nblob_tot = (int64_t)st->num_swim * st->blobs_per_swim;
+    a.nblob = st->blobs_per_swim;
+    a.xb = stage(st->x_blobs, 3 * nblob_tot);
+    a.vb = stage(st->v_blobs, 3 * nblob_tot);
     int rc = TFS_OK;
-    if ((nsegs && (!a.x || !a.xm1 || !a.u || !a.v || (a.bdf2 && (!a.xm2 || !a.um1)))) || !a.bx || !a.bxm1 ||
-        !a.bu || !a.bv || (a.bdf2 && (!a.bxm2 || !a.bum1)))
+    if (nblob_tot && (!a.xb || !a.vb)) rc = fail(TFS_ERR_OUT_OF_MEMORY, "tfs_broyden_compute_errors: staging");
+    if (!rc && ((nsegs && (!a.x || !a.xm1 || !a.u || !a.v || (a.bdf2 && (!a.xm2 || !a.um1)))) || !a.bx ||
+                !a.bxm1 || !a.bu || !a.bv || (a.bdf2 && (!a.bxm2 || !a.bum1))))
         rc = fail(TFS_ERR_OUT_OF_MEMORY, "tfs_broyden_compute_errors: staging");
-    const int blocks = (int)((nseg_tot + nb + 255) / 256);
+    const int blocks = (int)((nseg_tot + nb + nblob_tot + 255) / 256);
     if (!rc && 2LL * blocks > std::max<int64_t>((int64_t)b->G * (b->max_iter + 1), 2LL * b->RB))
         rc = fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_compute_errors: state larger than the engine");
     if (!rc) {

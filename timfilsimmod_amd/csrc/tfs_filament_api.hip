@@ -598,6 +598,35 @@ __global__ void body_guess_kernel(FilState S, int nswim, int nt) {
     for (int e = 0; e < 9; ++e) S.bQ[9 * b + e] = R.a[e];
 }
 
+// The blob forces' extrapolation at a step's start (rigid_body.cpp:447-478).
+__global__ void blob_force_guess_kernel(double* __restrict__ fb, double* __restrict__ m1, double* __restrict__ m2,
+                                        int64_t n, int nt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (nt == 1) {
+        m1[i] = fb[i];
+    } else if (nt == 2) {
+        m2[i] = m1[i];
+        m1[i] = fb[i];
+        fb[i] = 2.0 * m1[i] - m2[i];
+    } else if (nt > 2) {
+        const double t = 3.0 * (fb[i] - m1[i]) + m2[i];
+        m2[i] = m1[i];
+        m1[i] = fb[i];
+        fb[i] = t;
+    }
+}
+
+// rigid_body::update's blob part (rigid_body.cpp:506-520): swimmer s's blob
+// forces take the entries after its body's 6 in the Broyden update.
+__global__ void blob_force_update_kernel(double* __restrict__ fb, const double* __restrict__ du, int64_t nb3,
+                                         int nswim, int64_t per, int64_t off) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb3 * nswim) return;
+    const int64_t s = i / nb3, k = i % nb3;
+    fb[i] += du[s * per + off + k];
+}
+
 // Q_init = R(q) of the current pose, one thread per body.
 __global__ void body_rot_kernel(FilState S, int nswim) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1251,15 +1280,25 @@ struct tfs_elastic {
     // free bodies: blob positions, body velocities, the coupling blocks of J0
     bool free_bodies = false;
     double *x_blobs = nullptr, *v_bodies = nullptr, *A = nullptr, *B = nullptr, *S0 = nullptr;
+    // Broyden for everything: the plain matvec, blob velocities and force
+    // history, J0 and the update in the Broyden layout (blobs after each body)
+    int64_t nb_swim = 0, per = 0;
+    tfs_rpy* rpy = nullptr;
+    double *v_blobs = nullptr, *fbm1 = nullptr, *fbm2 = nullptr, *bfe_blocks = nullptr, *bfe_du = nullptr;
 };
 
 int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
     g_last_error.clear();
-    if (!desc || !out || !desc->filaments || !desc->velocities || !desc->broyden || !desc->f_blobs)
+    if (!desc || !out || !desc->filaments || !desc->broyden || !desc->f_blobs ||
+        (!desc->velocities && desc->broyden_blobs <= 0))
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_create: null handle or array");
     *out = nullptr;
     const tfs_filament* fh = desc->filaments;
     const bool free_bodies = fh->S.bx != nullptr;
+    const bool bfe = desc->broyden_blobs > 0;
+    if (bfe && (free_bodies || !(desc->blob_radius > 0)))
+        return fail(TFS_ERR_INVALID_ARGUMENT,
+                    "tfs_elastic_create: the Broyden-for-everything form takes bodies held still and blob_radius > 0");
     if (!free_bodies && (!desc->x_bodies || !desc->v_bodies || !desc->x_blobs))
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_create: null body or blob array");
     if (free_bodies && (!desc->body_mobility || !(desc->sphere_radius >= 0.0)))
@@ -1275,10 +1314,16 @@ int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
     const int64_t nb = blocks_len(fh);
     e->free_bodies = free_bodies;
     const int64_t lab = free_bodies ? (int64_t)fh->P.nfil * 36 * fh->P.nseg : 0;
+    const int64_t n6_ = 6LL * fh->P.nseg, fil_ = fh->desc.fil_per_swim;
+    e->nb_swim = bfe ? desc->broyden_blobs : 0;
+    e->per = fil_ * n6_ + 6 + 3 * e->nb_swim;
+    const int64_t nbt = nswim * e->nb_swim;
+    const int64_t bfe_len = bfe ? nswim * (fil_ * n6_ * n6_ + 36 + 9 * e->nb_swim) : 0;
     std::vector<std::pair<double**, int64_t>> parts = {
         {&e->f_segs, 6 * ns}, {&e->v_segs, 6 * ns},        {&e->du, e->n},  {&e->blocks, nb},
         {&e->zeros, 6 * nswim}, {&e->x_blobs, 3 * fh->nblob}, {&e->v_bodies, 6 * nswim}, {&e->A, lab},
-        {&e->B, lab},          {&e->S0, 36 * nswim}};
+        {&e->B, lab},          {&e->S0, 36 * nswim},        {&e->v_blobs, 3 * nbt},   {&e->fbm1, 3 * nbt},
+        {&e->fbm2, 3 * nbt},   {&e->bfe_blocks, bfe_len},   {&e->bfe_du, bfe ? nswim * e->per : 0}};
     int64_t total = 0;
     for (auto& p : parts) total += (p.second + 31) / 32 * 32;
     if (hipMalloc(&e->arena, total * 8) != hipSuccess) {
@@ -1299,6 +1344,41 @@ int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
     hipError_t err = hipMemcpy(e->blocks, host.data(), 8 * nb, hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemset(e->zeros, 0, 48 * nswim);
     if (err == hipSuccess) err = hipMemset(e->v_bodies, 0, 48 * nswim);
+    if (err == hipSuccess && bfe) {
+        // J0 in the Broyden layout: filament blocks copied in every step; the
+        // body's identity and the blobs' 3 x 3 blocks (1 / (-6 pi mu a)) I fixed
+        std::vector<double> hb(bfe_len, 0.0);
+        const double inv_fac = -1.0 / (6.0 * kPi * fh->P.mu * desc->blob_radius);
+        const int64_t sblk = fil_ * n6_ * n6_ + 36 + 9 * e->nb_swim;
+        for (int64_t s = 0; s < nswim; ++s) {
+            double* b0 = hb.data() + s * sblk + fil_ * n6_ * n6_;
+            for (int k = 0; k < 6; ++k) b0[7 * k] = 1.0;
+            for (int64_t m = 0; m < e->nb_swim; ++m)
+                for (int k = 0; k < 3; ++k) b0[36 + 9 * m + 4 * k] = inv_fac;
+        }
+        err = hipMemcpy(e->bfe_blocks, hb.data(), 8 * bfe_len, hipMemcpyHostToDevice);
+        if (err == hipSuccess) err = hipMemset(e->fbm1, 0, 24 * nbt);
+        if (err == hipSuccess) err = hipMemset(e->fbm2, 0, 24 * nbt);
+        if (err == hipSuccess) {
+            tfs_rpy_desc rd{};
+            rd.num_segs = (int32_t)ns;
+            rd.num_blobs = (int32_t)nbt;
+            rd.seg_dof = 6;
+            rd.precision = TFS_PRECISION_FP64;
+            rd.bs_mode = TFS_BS_ACCUMULATE;
+            rd.memory = TFS_MEMORY_DEVICE;
+            rd.seg_radius = fh->P.a;
+            rd.blob_radius = desc->blob_radius;
+            rd.mu = fh->P.mu;
+            rd.stream = fh->stream;
+            if (tfs_rpy_create(&rd, &e->rpy) != TFS_OK) {
+                const std::string msg = g_last_error;
+                tfs_elastic_destroy(e);
+                g_last_error = msg;
+                return TFS_ERR_HIP;
+            }
+        }
+    }
     if (err != hipSuccess) {
         tfs_elastic_destroy(e);
         return hip_fail(err, "tfs_elastic_create");
@@ -1309,6 +1389,7 @@ int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
 
 int tfs_elastic_destroy(tfs_elastic* e) {
     if (!e) return TFS_OK;
+    if (e->rpy) tfs_rpy_destroy(e->rpy);
     if (e->arena) (void)hipFree(e->arena);
     delete e;
     return TFS_OK;
@@ -1345,13 +1426,26 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
         st.body_um1 = fh->S.bum1;
         st.v_bodies = e->v_bodies;
     }
+    if (e->rpy) {
+        st.blobs_per_swim = (int32_t)e->nb_swim;
+        st.x_blobs = D.x_blobs;
+        st.v_blobs = e->v_blobs;
+    }
+    const int64_t nswim = fh->desc.num_swim, fil = fh->desc.fil_per_swim, n6 = 6LL * fh->P.nseg;
+    const int64_t nb3 = 3 * e->nb_swim;
     // forces -> velocities -> residual in `slot`
     auto evaluate = [&](int slot, int32_t* too_large) -> int {
         int rc = e->free_bodies ? tfs_filament_blob_positions(fh, e->x_blobs) : TFS_OK;
         if (!rc) rc = tfs_filament_forces(fh, nt, e->f_segs);
         if (!rc) rc = tfs_filament_sync(fh);
         tfs_solve_stats vs{};
-        if (!rc && e->free_bodies)
+        if (!rc && e->rpy) {
+            // USE_BROYDEN_FOR_EVERYTHING: one plain matvec with the blob forces given
+            rc = tfs_rpy_upload(e->rpy, TFS_SEG_POSITIONS, fh->S.x);
+            if (!rc) rc = tfs_rpy_upload(e->rpy, TFS_BLOB_POSITIONS, D.x_blobs);
+            if (!rc) rc = tfs_rpy_apply(e->rpy, TFS_BLOCK_ALL, e->f_segs, D.f_blobs, e->v_segs, e->v_blobs);
+            if (!rc) rc = tfs_filament_sync(fh);
+        } else if (!rc && e->free_bodies)
             rc = tfs_velocities_compute_free(D.velocities, fh->S.x, e->f_segs, e->x_blobs, fh->S.bx, nullptr,
                                              e->v_segs, D.f_blobs, e->v_bodies, &vs);
         else if (!rc)
@@ -1363,12 +1457,26 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
         return rc;
     };
     int rc = tfs_filament_initial_guess(fh, nt);
+    if (!rc && e->rpy && nb3) {
+        hipLaunchKernelGGL(blob_force_guess_kernel, dim3(grid_for(nswim * nb3, 256)), dim3(256), 0, fh->stream,
+                           D.f_blobs, e->fbm1, e->fbm2, nswim * nb3, (int)nt);
+        rc = check(hipGetLastError(), "blob force guess");
+    }
     int32_t big = 0;
     if (!rc) rc = evaluate(0, &big);
     if (!rc && big) {
         rc = tfs_filament_jacobian(fh, nt, e->f_segs, e->blocks);
         if (!rc) rc = tfs_filament_sync(fh);
-        if (!rc) rc = tfs_broyden_set_jacobian(D.broyden, e->blocks);
+        if (!rc && e->rpy) {
+            // the filament blocks into the Broyden layout (blob blocks fixed)
+            const int64_t src = fil * n6 * n6 + 36, dst = src + 3 * nb3;
+            for (int64_t s = 0; s < nswim && !rc; ++s)
+                rc = check(hipMemcpyAsync(e->bfe_blocks + s * dst, e->blocks + s * src, 8 * fil * n6 * n6,
+                                          hipMemcpyDeviceToDevice, fh->stream),
+                           "J0 blocks");
+            if (!rc) rc = tfs_filament_sync(fh);
+        }
+        if (!rc) rc = tfs_broyden_set_jacobian(D.broyden, e->rpy ? e->bfe_blocks : e->blocks);
         if (!rc && e->free_bodies) {
             // swimmer::prepare_jacobian_inv: the body's Schur complement
             rc = tfs_filament_body_coupling(fh, nt, D.body_mobility, D.sphere_radius, e->A, e->B, e->S0);
@@ -1381,7 +1489,23 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
     if (!rc) rc = tfs_broyden_start(D.broyden, nullptr);
     int32_t iter = 0;
     while (!rc && big && iter < D.max_iter) {
-        rc = tfs_broyden_find_update(D.broyden, e->du);
+        if (e->rpy) {
+            // the filament and body entries to the filament layout, the blob
+            // entries onto the blob forces
+            rc = tfs_broyden_find_update(D.broyden, e->bfe_du);
+            for (int64_t s = 0; s < nswim && !rc; ++s)
+                rc = check(hipMemcpyAsync(e->du + s * (fil * n6 + 6), e->bfe_du + s * e->per, 8 * (fil * n6 + 6),
+                                          hipMemcpyDeviceToDevice, fh->stream),
+                           "update");
+            if (!rc && nb3) {
+                hipLaunchKernelGGL(blob_force_update_kernel, dim3(grid_for(nswim * nb3, 256)), dim3(256), 0,
+                                   fh->stream, D.f_blobs, (const double*)e->bfe_du, nb3, (int)nswim, e->per,
+                                   fil * n6 + 6);
+                rc = check(hipGetLastError(), "blob force update");
+            }
+        } else {
+            rc = tfs_broyden_find_update(D.broyden, e->du);
+        }
         if (!rc) rc = tfs_filament_update(fh, e->du);
         if (!rc) rc = evaluate(1, &big);
         if (!rc) rc = tfs_broyden_end_of_iter(D.broyden, nullptr, big);

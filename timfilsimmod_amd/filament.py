@@ -82,10 +82,11 @@ class FilamentModel:
         d = self.desc
         return d.num_swim * (d.fil_per_swim * 6 * d.num_seg + 6)
 
-    def block_sizes(self):
-        """Sizes of the J0 blocks in the Broyden layout (filaments, then the body)."""
+    def block_sizes(self, blobs_per_swim=0):
+        """Sizes of the J0 blocks in the Broyden layout (filaments, then the body,
+        then, in the Broyden-for-everything form, one 3 x 3 block per blob)."""
         d = self.desc
-        return ([6 * d.num_seg] * d.fil_per_swim + [6]) * d.num_swim
+        return ([6 * d.num_seg] * d.fil_per_swim + [6] + [3] * int(blobs_per_swim)) * d.num_swim
 
     @staticmethod
     def _arr(a):
@@ -165,7 +166,8 @@ class ElasticDesc(ctypes.Structure):
     _fields_ = [("filaments", ctypes.c_void_p), ("velocities", ctypes.c_void_p), ("broyden", ctypes.c_void_p),
                 ("x_blobs", ctypes.c_void_p), ("x_bodies", ctypes.c_void_p), ("v_bodies", ctypes.c_void_p),
                 ("f_blobs", ctypes.c_void_p), ("tol", ctypes.c_double), ("max_iter", ctypes.c_int32),
-                ("body_mobility", ctypes.c_void_p), ("sphere_radius", ctypes.c_double)]
+                ("body_mobility", ctypes.c_void_p), ("sphere_radius", ctypes.c_double),
+                ("broyden_blobs", ctypes.c_int32), ("blob_radius", ctypes.c_double)]
 
 
 class ElasticStats(ctypes.Structure):
@@ -186,11 +188,13 @@ class ElasticStep:
     and result)."""
 
     def __init__(self, filaments, velocities, broyden, x_blobs, x_bodies, v_bodies, f_blobs, *, tol=1e-4,
-                 max_iter=400, body_mobility=None, sphere_radius=0.0):
+                 max_iter=400, body_mobility=None, sphere_radius=0.0, broyden_blobs=0, blob_radius=1.0):
+        """broyden_blobs = NBLOB: the Broyden-for-everything form (blob forces in
+        f_blobs are Broyden unknowns; velocities may be None)."""
         self._keep = (filaments, velocities, broyden, x_blobs, x_bodies, v_bodies, f_blobs, body_mobility)
-        d = ElasticDesc(filaments.h.value, velocities.h.value, broyden.h.value, L.ptr(x_blobs), L.ptr(x_bodies),
-                        L.ptr(v_bodies), L.ptr(f_blobs), float(tol), int(max_iter), L.ptr(body_mobility),
-                        float(sphere_radius))
+        d = ElasticDesc(filaments.h.value, velocities.h.value if velocities is not None else None, broyden.h.value,
+                        L.ptr(x_blobs), L.ptr(x_bodies), L.ptr(v_bodies), L.ptr(f_blobs), float(tol), int(max_iter),
+                        L.ptr(body_mobility), float(sphere_radius), int(broyden_blobs), float(blob_radius))
         h = ctypes.c_void_p()
         L.check(L.lib().tfs_elastic_create(ctypes.byref(d), ctypes.byref(h)))
         self.h = h
