@@ -188,6 +188,24 @@ int tfs_elastic_destroy(tfs_elastic *e);
  * TFS_ERR_NON_FINITE if a residual is not finite (cilia_sim_main.cu:469-487). */
 int tfs_elastic_step(tfs_elastic *e, int32_t nt, tfs_elastic_stats *stats);
 
+/* The state of the step in the reference's backup record, after its step
+ * number (swimmer::write_backup, filament::write_backup and
+ * rigid_body::write_backup, swimmer.cpp:1139-1150, filament.cpp:1952-2020,
+ * rigid_body.cpp:547-572; read back by the initial_setup functions): per
+ * swimmer, per filament per segment q (4), x, xm1, xm2, u, um1 (3 each), then
+ * the tether multiplier and its two previous values, the clamp's, and per
+ * link lambda and its two previous values (28 NSEG + 9 values), then the
+ * body: q (4), x, xm1, xm2, u, um1 (19), then in the Broyden-for-everything
+ * form per blob component the force and its two previous values (9 NBLOB).
+ * Host memory, *count values (tfs_elastic_backup_size).  held_bodies: for
+ * bodies held still, x (3) and q (4) per swimmer to record, or NULL for the
+ * origin and no rotation; free bodies record their own state.  Restoring
+ * sets each previous orientation to the current one, as the reference's
+ * reader does, and leaves the velocity solve's warm start alone. */
+int tfs_elastic_backup_size(tfs_elastic *e, int64_t *count);
+int tfs_elastic_backup(tfs_elastic *e, const double *held_bodies, double *out);
+int tfs_elastic_restore(tfs_elastic *e, const double *in);
+
 #ifdef __cplusplus
 }
 #endif

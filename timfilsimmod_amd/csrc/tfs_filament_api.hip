@@ -1521,3 +1521,163 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
 }
 
 }  // extern "C"
+
+namespace {
+
+// Host images of the device state, packed in the backup order or unpacked from it.
+struct Backup {
+    std::vector<double> q, x, xm1, xm2, u, um1, m, m1, m2, bx, bxm1, bxm2, bu, bum1, bq, fb, fb1, fb2;
+};
+
+int64_t backup_len(const tfs_elastic* e) {
+    const tfs_filament* fh = e->desc.filaments;
+    const int64_t N = fh->P.nseg, nsw = fh->desc.num_swim, fil = fh->desc.fil_per_swim;
+    return nsw * (fil * (28 * N + 9) + 19 + 9 * e->nb_swim);
+}
+
+int transfer(tfs_elastic* e, Backup& B, bool to_host) {
+    tfs_filament* fh = e->desc.filaments;
+    const int64_t ns = num_segs(fh), nf = fh->P.nfil, nm = 3LL * (fh->P.nseg + 1) * nf, nsw = fh->desc.num_swim;
+    const int64_t nb3 = 3 * nsw * e->nb_swim;
+    struct Item {
+        std::vector<double>* v;
+        double* d;
+        int64_t n;
+    };
+    std::vector<Item> items = {{&B.q, fh->S.q, 4 * ns},     {&B.x, fh->S.x, 3 * ns},     {&B.xm1, fh->S.xm1, 3 * ns},
+                               {&B.xm2, fh->S.xm2, 3 * ns}, {&B.u, fh->S.u, 3 * ns},     {&B.um1, fh->S.um1, 3 * ns},
+                               {&B.m, fh->S.mult, nm},      {&B.m1, fh->S.mult_m1, nm},  {&B.m2, fh->S.mult_m2, nm}};
+    if (fh->S.bx) {
+        items.insert(items.end(), {{&B.bx, fh->S.bx, 3 * nsw},
+                                   {&B.bxm1, fh->S.bxm1, 3 * nsw},
+                                   {&B.bxm2, fh->S.bxm2, 3 * nsw},
+                                   {&B.bu, fh->S.bu, 3 * nsw},
+                                   {&B.bum1, fh->S.bum1, 3 * nsw},
+                                   {&B.bq, fh->S.bq, 4 * nsw}});
+    }
+    if (e->rpy && nb3) {
+        items.insert(items.end(), {{&B.fb, e->desc.f_blobs, nb3}, {&B.fb1, e->fbm1, nb3}, {&B.fb2, e->fbm2, nb3}});
+    }
+    hipError_t err = hipStreamSynchronize(fh->stream);
+    for (auto& it : items) {
+        if (err != hipSuccess) break;
+        if (to_host) {
+            it.v->resize(it.n);
+            err = hipMemcpy(it.v->data(), it.d, 8 * it.n, hipMemcpyDeviceToHost);
+        } else {
+            err = hipMemcpy(it.d, it.v->data(), 8 * it.n, hipMemcpyHostToDevice);
+        }
+    }
+    return check(err, "tfs_elastic backup transfer");
+}
+
+}  // namespace
+
+extern "C" {
+
+int tfs_elastic_backup_size(tfs_elastic* e, int64_t* count) {
+    if (!e || !count) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_backup_size: null argument");
+    *count = backup_len(e);
+    return TFS_OK;
+}
+
+int tfs_elastic_backup(tfs_elastic* e, const double* held_bodies, double* out) {
+    if (!e || !out) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_backup: null argument");
+    tfs_filament* fh = e->desc.filaments;
+    Backup B;
+    int rc = transfer(e, B, true);
+    if (rc) return rc;
+    const int64_t N = fh->P.nseg, nsw = fh->desc.num_swim, fil = fh->desc.fil_per_swim, nm = 3 * (N + 1);
+    int64_t p = 0;
+    for (int64_t s = 0; s < nsw; ++s) {
+        for (int64_t i = 0; i < fil; ++i) {
+            const int64_t f = s * fil + i, s0 = f * N;
+            for (int64_t n = 0; n < N; ++n) {
+                const int64_t g = s0 + n;
+                for (int k = 0; k < 4; ++k) out[p++] = B.q[4 * g + k];
+                for (auto* v : {&B.x, &B.xm1, &B.xm2, &B.u, &B.um1})
+                    for (int k = 0; k < 3; ++k) out[p++] = (*v)[3 * g + k];
+            }
+            // [tether, lambda_0 .. lambda_{N-2}, clamp] in the device layout
+            for (int64_t slot : {(int64_t)0, 3 * N})
+                for (auto* v : {&B.m, &B.m1, &B.m2})
+                    for (int k = 0; k < 3; ++k) out[p++] = (*v)[f * nm + slot + k];
+            for (int64_t l = 0; l < N - 1; ++l)
+                for (auto* v : {&B.m, &B.m1, &B.m2})
+                    for (int k = 0; k < 3; ++k) out[p++] = (*v)[f * nm + 3 * (l + 1) + k];
+        }
+        if (fh->S.bx) {
+            for (int k = 0; k < 4; ++k) out[p++] = B.bq[4 * s + k];
+            for (auto* v : {&B.bx, &B.bxm1, &B.bxm2, &B.bu, &B.bum1})
+                for (int k = 0; k < 3; ++k) out[p++] = (*v)[3 * s + k];
+        } else {
+            const double* hb = held_bodies ? held_bodies + 7 * s : nullptr;
+            const double q[4] = {hb ? hb[3] : 1.0, hb ? hb[4] : 0.0, hb ? hb[5] : 0.0, hb ? hb[6] : 0.0};
+            for (int k = 0; k < 4; ++k) out[p++] = q[k];
+            for (int r = 0; r < 3; ++r)
+                for (int k = 0; k < 3; ++k) out[p++] = hb ? hb[k] : 0.0;
+            for (int k = 0; k < 6; ++k) out[p++] = 0.0;
+        }
+        for (int64_t c = 0; c < 3 * e->nb_swim; ++c) {
+            const int64_t g = 3 * s * e->nb_swim + c;
+            out[p++] = B.fb[g];
+            out[p++] = B.fb1[g];
+            out[p++] = B.fb2[g];
+        }
+    }
+    return p == backup_len(e) ? TFS_OK : fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_backup: layout mismatch");
+}
+
+int tfs_elastic_restore(tfs_elastic* e, const double* in) {
+    if (!e || !in) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_restore: null argument");
+    tfs_filament* fh = e->desc.filaments;
+    Backup B;
+    int rc = transfer(e, B, true);  // sizes, and the fields a backup does not hold
+    if (rc) return rc;
+    const int64_t N = fh->P.nseg, nsw = fh->desc.num_swim, fil = fh->desc.fil_per_swim, nm = 3 * (N + 1);
+    int64_t p = 0;
+    for (int64_t s = 0; s < nsw; ++s) {
+        for (int64_t i = 0; i < fil; ++i) {
+            const int64_t f = s * fil + i, s0 = f * N;
+            for (int64_t n = 0; n < N; ++n) {
+                const int64_t g = s0 + n;
+                for (int k = 0; k < 4; ++k) B.q[4 * g + k] = in[p++];
+                for (auto* v : {&B.x, &B.xm1, &B.xm2, &B.u, &B.um1})
+                    for (int k = 0; k < 3; ++k) (*v)[3 * g + k] = in[p++];
+            }
+            for (int64_t slot : {(int64_t)0, 3 * N})
+                for (auto* v : {&B.m, &B.m1, &B.m2})
+                    for (int k = 0; k < 3; ++k) (*v)[f * nm + slot + k] = in[p++];
+            for (int64_t l = 0; l < N - 1; ++l)
+                for (auto* v : {&B.m, &B.m1, &B.m2})
+                    for (int k = 0; k < 3; ++k) (*v)[f * nm + 3 * (l + 1) + k] = in[p++];
+        }
+        if (fh->S.bx) {
+            for (int k = 0; k < 4; ++k) B.bq[4 * s + k] = in[p++];
+            for (auto* v : {&B.bx, &B.bxm1, &B.bxm2, &B.bu, &B.bum1})
+                for (int k = 0; k < 3; ++k) (*v)[3 * s + k] = in[p++];
+        } else {
+            p += 19;  // a body held still keeps the pose it was given
+        }
+        for (int64_t c = 0; c < 3 * e->nb_swim; ++c) {
+            const int64_t g = 3 * s * e->nb_swim + c;
+            B.fb[g] = in[p++];
+            B.fb1[g] = in[p++];
+            B.fb2[g] = in[p++];
+        }
+    }
+    rc = transfer(e, B, false);
+    // previous orientations = current ones (filament.cpp:140-146, rigid_body.cpp:37-44)
+    if (!rc) rc = check(hipMemcpy(fh->S.qm1, fh->S.q, 32 * num_segs(fh), hipMemcpyDeviceToDevice), "restore");
+    if (!rc && fh->S.bx) {
+        rc = check(hipMemcpy(fh->S.bqm1, fh->S.bq, 32 * nsw, hipMemcpyDeviceToDevice), "restore");
+        if (!rc) {
+            hipLaunchKernelGGL(body_rot_kernel, dim3(grid_for(nsw, 64)), dim3(64), 0, fh->stream, fh->S, (int)nsw);
+            rc = check(hipGetLastError(), "restore");
+        }
+        if (!rc) rc = check(hipStreamSynchronize(fh->stream), "restore");
+    }
+    return rc;
+}
+
+}  // extern "C"

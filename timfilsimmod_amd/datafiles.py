@@ -371,3 +371,23 @@ def resume_beat(sim, name) -> int:
         sol[off + nf * dyn_phase:off + nf * dyn_phase + nf] = b["angle_dot"][0]
     sim.restore(b["phase"][0], b["angle"][0], sol)
     return step
+
+
+def write_elastic_backup(path, step: int, elastic, held_bodies=None) -> None:
+    """The elastic run's backup file (cilia_sim_main.cu:611-625): one record,
+    the step then every swimmer's state (filament.ElasticStep.backup),
+    overwritten each time."""
+    write_record(path, elastic.backup(held_bodies), step=step, append=False)
+
+
+def resume_elastic(path, elastic) -> int:
+    """Reads a backup written by write_elastic_backup (or by the reference, for
+    the same sizes) into the step's state; returns the step it was taken at,
+    so the run continues with nt = step + 1 (cilia_sim_main.cu:140-166)."""
+    with RecordReader(path) as r:
+        count = r.peek_count()
+        if count is None:
+            raise ValueError(f"{path}: empty backup")
+        step, values = r.read(count - 1)
+    elastic.restore(values)
+    return int(step)

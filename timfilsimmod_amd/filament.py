@@ -47,6 +47,9 @@ L.extra_signatures({
     "tfs_filament_set_body": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _vp]),
     "tfs_filament_blob_positions": (ctypes.c_int, [_vp, _vp]),
     "tfs_filament_body_coupling": (ctypes.c_int, [_vp, _i32, _vp, ctypes.c_double, _vp, _vp, _vp]),
+    "tfs_elastic_backup_size": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    "tfs_elastic_backup": (ctypes.c_int, [_vp, _vp, _vp]),
+    "tfs_elastic_restore": (ctypes.c_int, [_vp, _vp]),
 })
 
 
@@ -209,6 +212,24 @@ class ElasticStep:
             self.close()
         except Exception:
             pass
+
+    def backup(self, held_bodies=None):
+        """The state in the reference's backup layout (tfs_elastic_backup):
+        a host array; held_bodies (x, q per swimmer) for bodies held still."""
+        n = ctypes.c_int64()
+        L.check(L.lib().tfs_elastic_backup_size(self.h, ctypes.byref(n)))
+        out = np.zeros(n.value)
+        hb = None if held_bodies is None else np.ascontiguousarray(held_bodies, dtype=np.float64)
+        L.check(L.lib().tfs_elastic_backup(self.h, L.ptr(hb), L.ptr(out)))
+        return out
+
+    def restore(self, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        n = ctypes.c_int64()
+        L.check(L.lib().tfs_elastic_backup_size(self.h, ctypes.byref(n)))
+        if v.size != n.value:
+            raise ValueError(f"backup holds {v.size} values, this step needs {n.value}")
+        L.check(L.lib().tfs_elastic_restore(self.h, L.ptr(v)))
 
     def step(self, nt, *, allow_unconverged=False):
         st = ElasticStats()
