@@ -259,10 +259,62 @@ def elastic_leg(torch, steps: int = 3):
     model.close()
     ev.close()
     eng.close()
-    return {"workload": "16 filaments x 20 segments (Cosserat, clamped, end force 220) + 2,000 blobs, "
-                        "dt 0.1, Broyden to 1e-8, blob forces from the dense inverse of the blob "
-                        "mobility (formed once, blocked Gauss-Jordan on the device)", "steps": per,
-            "ms_per_step_mean": float(np.mean([p["ms"] for p in per]))}
+    out = {"workload": "16 filaments x 20 segments (Cosserat, clamped, end force 220) + 2,000 blobs, "
+                       "dt 0.1, Broyden to 1e-8, blob forces from the dense inverse of the blob "
+                       "mobility (formed once, blocked Gauss-Jordan on the device)", "steps": per,
+           "ms_per_step_mean": float(np.mean([p["ms"] for p in per]))}
+
+    def timed(step_obj):
+        rows = []
+        for nt in range(steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = step_obj.step(nt)
+            torch.cuda.synchronize()
+            rows.append({"nt": nt, "ms": (time.perf_counter() - t0) * 1e3,
+                         "broyden_iterations": st["broyden_iterations"], "gmres_iterations": st["gmres_iterations"]})
+        return rows
+
+    # the same body swimming freely: body state on the device, the blob-only
+    # solve with body rows by GMRES(30) to 1e-10, J0 with the body's Schur complement
+    fil_refs = dev((body.fil_base - body.centre).reshape(-1))
+    model = FM.FilamentModel(1, nfil, nseg, dl=dl, kb=kb, kt=kb, dt=dt, end_force=efm, memory=L.MEMORY_DEVICE)
+    model.set_state(dev(body.fil_base.reshape(-1)), dev(qs.reshape(-1)))
+    model.set_body(dev(body.centre), dev([1.0, 0.0, 0.0, 0.0]), fil_refs, nb, dev(body.blob_refs.reshape(-1)))
+    model.set_perturbation(4, 12, dev(pert.reshape(-1)))
+    evh = ElasticVelocities(1, nfil * nseg, nb, mode=SOLVE_FP64, restart=60, max_cycles=100, tol=1e-12,
+                            free_bodies=True)
+    nmob = dev(evh.body_mobility(body.x_blobs, body.centre)[0].reshape(-1))
+    evh.close()
+    ev = ElasticVelocities(1, nfil * nseg, nb, mode=SOLVE_FP64, restart=30, max_cycles=200, tol=1e-10,
+                           memory=L.MEMORY_DEVICE, free_bodies=True)
+    eng = BroydenSolver(model.block_sizes(), memory=L.MEMORY_DEVICE)
+    fb = dev(np.zeros(3 * nb))
+    step = FM.ElasticStep(model, ev, eng, None, None, None, fb, tol=1e-8, max_iter=400, body_mobility=nmob,
+                          sphere_radius=float(body.radius))
+    rows = timed(step)
+    step.close()
+    model.close()
+    ev.close()
+    eng.close()
+    out["free_body"] = {"workload": "the same body swimming freely (blob forces and body velocity by GMRES(30) "
+                                    "to 1e-10, J0 with the body's Schur complement)", "steps": rows,
+                        "ms_per_step_mean": float(np.mean([r["ms"] for r in rows]))}
+    # the body held still with the blob forces as Broyden unknowns
+    model = FM.FilamentModel(1, nfil, nseg, dl=dl, kb=kb, kt=kb, dt=dt, end_force=efm, memory=L.MEMORY_DEVICE)
+    model.set_state(dev(body.fil_base.reshape(-1)), dev(qs.reshape(-1)))
+    model.set_perturbation(4, 12, dev(pert.reshape(-1)))
+    eng = BroydenSolver(model.block_sizes(blobs_per_swim=nb), memory=L.MEMORY_DEVICE)
+    keep = [dev(body.x_blobs), dev(np.zeros(3)), dev(np.zeros(6)), dev(np.zeros(3 * nb))]
+    step = FM.ElasticStep(model, None, eng, *keep, tol=1e-8, max_iter=400, broyden_blobs=nb, blob_radius=1.0)
+    rows = timed(step)
+    step.close()
+    model.close()
+    eng.close()
+    out["broyden_for_everything"] = {"workload": "the body held still, blob forces as Broyden unknowns "
+                                                 "(one plain matvec per evaluation)", "steps": rows,
+                                     "ms_per_step_mean": float(np.mean([r["ms"] for r in rows]))}
+    return out
 
 
 def elastic_cpu_baseline():

@@ -1399,6 +1399,9 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
     if (!e || nt < 0) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_step: null handle or nt < 0");
     const tfs_elastic_desc& D = e->desc;
     tfs_filament* fh = D.filaments;
+    if ((fh->S.bx != nullptr) != e->free_bodies)
+        return fail(TFS_ERR_INVALID_ARGUMENT,
+                    "tfs_elastic_step: the filament handle's bodies were set after tfs_elastic_create");
     tfs_elastic_stats local{};
     tfs_elastic_stats& S = stats ? *stats : local;
     S = tfs_elastic_stats{};
