@@ -253,10 +253,23 @@ int tfs_velocities_compute(tfs_velocities *v, const double *x_segs, const double
  * body's segments of (F, T + (x - X) x F)), the force- and torque-free
  * condition (swimmer.cpp:469-533).  f_bodies: external force and torque per
  * body (6 each) or NULL for none; v_bodies (6 per body) holds the warm start
- * on entry and the solved velocities on return; the rest as above. */
+ * on entry and the solved velocities on return; the rest as above.
+ * q_bodies (4 per body, the orientation of the pose the blobs are at) is used
+ * by the direct solve only and may be NULL otherwise.
+ * Direct solve with one free body (TFS_BLOB_DIRECT, num_swim 1, after
+ * tfs_velocities_set_body_frame): the blob mobility at pose (X, R) is
+ * (I x R) M_ref (I x R^T) with M_ref that of the body-frame positions, so its
+ * inverse A, Z = A K_ref and the body mobility N = (K_ref^T Z)^{-1} are
+ * formed once; each call is then one dense product: y = A R^T r,
+ * V_ref = N (R^T f - K_ref^T y), f_b = R (y + Z V_ref).  No counterpart in
+ * the reference, which runs GMRES on every call. */
 int tfs_velocities_compute_free(tfs_velocities *v, const double *x_segs, const double *f_segs,
                                 const double *x_blobs, const double *x_bodies, const double *f_bodies,
-                                double *v_segs, double *f_blobs, double *v_bodies, tfs_solve_stats *stats);
+                                const double *q_bodies, double *v_segs, double *f_blobs, double *v_bodies,
+                                tfs_solve_stats *stats);
+/* The body-frame blob positions (3 per blob, relative to the centre, at
+ * orientation 1) for the direct free-body solve. */
+int tfs_velocities_set_body_frame(tfs_velocities *v, const double *blob_refs);
 
 #ifdef __cplusplus
 }

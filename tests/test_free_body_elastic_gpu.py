@@ -25,7 +25,7 @@ from timfilsimmod_amd import _lib as L
 from timfilsimmod_amd import filament as FM
 from timfilsimmod_amd import geometry as G
 from timfilsimmod_amd.broyden import BroydenSolver
-from timfilsimmod_amd.solver import SOLVE_FP64, ElasticVelocities
+from timfilsimmod_amd.solver import BLOB_DIRECT, SOLVE_FP64, ElasticVelocities
 
 pytestmark = pytest.mark.gpu
 
@@ -61,6 +61,34 @@ def test_free_blob_solve_matches_dense_oracle(external):
     # times the condition number of the blob mobility (overlapping blobs,
     # ~1e3 here), the velocities do not
     assert rel(fb, lam) < 1e-8 and rel(vb, V) < 1e-10 and rel(vs, v_s) < 1e-10
+
+
+def test_direct_free_blob_solve_matches_dense_oracle():
+    print("case direct free-body solve 3x6 segments + 150 blobs, turned body", flush=True)
+    body = G.make_body(3, 6, 150, 6.0)
+    X = np.array([0.3, -0.2, 0.1])
+    q = np.roll(Rotation.random(random_state=11).as_quat(), 1)
+    R = F.rot_mat(q)
+    xb = (X + body.blob_refs @ R.T).reshape(-1)
+    fe = np.array([1.0, -2.0, 0.5, 3.0, 0.2, -1.0])
+    ev = ElasticVelocities(1, 18, 150, mode=SOLVE_FP64, restart=60, max_cycles=100, tol=1e-12, free_bodies=True)
+    try:
+        ev.set_body_frame(body.blob_refs.reshape(-1))
+        ev.set_preconditioner(BLOB_DIRECT)
+        out = [ev.compute_free(body.x_segs, body.f_segs, xb, X, fe, q) for _ in range(2)]
+    finally:
+        ev.close()
+    xs, fs = body.x_segs, body.f_segs
+    fils = [F.Filament.at_rest(np.zeros(3), np.tile([1.0, 0, 0, 0], (6, 1)), 6, 2.2) for _ in range(3)]
+    for i, f in enumerate(fils):
+        f.x = xs.reshape(3, 6, 3)[i]
+    sf = S.body_force(S.Body.at_rest(X), fils, [fs.reshape(3, 6, 6)[i] for i in range(3)]) + fe
+    v_b = O.block("bs", xs, xb, fs, np.zeros(xb.size), seg_dof=6, bs_accumulate=False)
+    lam, V = S.free_blob_solve(O.dense_mobility(np.zeros(0), xb), xb, X, v_b, sf)
+    v_s = O.block("sb", xs, xb, fs, lam, O.block("ss", xs, xb, fs, lam, seg_dof=6), seg_dof=6)
+    for vs, fb, vb, st in out:
+        print("rel err f_blobs", rel(fb, lam), "V", rel(vb, V), "v_segs", rel(vs, v_s), flush=True)
+        assert rel(fb, lam) < 1e-10 and rel(vb, V) < 1e-11 and rel(vs, v_s) < 1e-11
 
 
 def test_body_mobility_matches_oracle():
@@ -204,8 +232,9 @@ def free_case():
     return dict(body=body, p=p, pert=pert, refs=refs, N_ref=N_ref, q0=q0, R=R, runs=runs)
 
 
-def test_free_swimmer_elastic_step_matches_oracle_loop(free_case):
-    print("case free swimmer elastic step 4x20 + 400 blobs, tfs_elastic_step", flush=True)
+@pytest.mark.parametrize("direct", [False, True])
+def test_free_swimmer_elastic_step_matches_oracle_loop(free_case, direct):
+    print(f"case free swimmer elastic step 4x20 + 400 blobs, tfs_elastic_step, direct {direct}", flush=True)
     c = free_case
     body, p, nb = c["body"], c["p"], c["body"].num_blobs
     nfil, nseg = body.num_fil, p.nseg
@@ -223,6 +252,8 @@ def test_free_swimmer_elastic_step_matches_oracle_loop(free_case):
                              memory=L.MEMORY_DEVICE)
     ev = ElasticVelocities(1, nfil * nseg, nb, mode=SOLVE_FP64, restart=30, max_cycles=200, tol=1e-10,
                            memory=L.MEMORY_DEVICE, free_bodies=True)
+    if direct:
+        ev.set_preconditioner(BLOB_DIRECT)
     eng = BroydenSolver(model.block_sizes(), memory=L.MEMORY_DEVICE)
     f_blobs, nmob = dev(np.zeros(3 * nb)), dev(N_dev.reshape(-1))
     step = None

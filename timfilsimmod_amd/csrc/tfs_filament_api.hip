@@ -1344,6 +1344,13 @@ int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
     hipError_t err = hipMemcpy(e->blocks, host.data(), 8 * nb, hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemset(e->zeros, 0, 48 * nswim);
     if (err == hipSuccess) err = hipMemset(e->v_bodies, 0, 48 * nswim);
+    if (err == hipSuccess && free_bodies && fh->nblob &&
+        tfs_velocities_set_body_frame(desc->velocities, fh->blob_refs) != TFS_OK) {
+        const std::string msg = g_last_error;
+        tfs_elastic_destroy(e);
+        g_last_error = msg;
+        return TFS_ERR_INVALID_ARGUMENT;
+    }
     if (err == hipSuccess && bfe) {
         // J0 in the Broyden layout: filament blocks copied in every step; the
         // body's identity and the blobs' 3 x 3 blocks (1 / (-6 pi mu a)) I fixed
@@ -1450,7 +1457,7 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
             if (!rc) rc = tfs_filament_sync(fh);
         } else if (!rc && e->free_bodies)
             rc = tfs_velocities_compute_free(D.velocities, fh->S.x, e->f_segs, e->x_blobs, fh->S.bx, nullptr,
-                                             e->v_segs, D.f_blobs, e->v_bodies, &vs);
+                                             fh->S.bq, e->v_segs, D.f_blobs, e->v_bodies, &vs);
         else if (!rc)
             rc = tfs_velocities_compute(D.velocities, fh->S.x, e->f_segs, D.x_blobs, D.x_bodies, D.v_bodies,
                                         e->v_segs, D.f_blobs, &vs);

@@ -85,6 +85,142 @@ __global__ __launch_bounds__(256) void free_rhs_kernel(const double* __restrict_
     if (threadIdx.x < 6) rhs[3 * nblob + 6 * b + threadIdx.x] = sm[threadIdx.x][0] - (f_ext ? f_ext[6 * b + threadIdx.x] : 0.0);
 }
 
+// ---- direct solve with one free body --------------------------------------
+// The blob mobility at pose (X, R) is (I x R) M_ref (I x R^T), M_ref that of
+// the body-frame reference positions; with A = M_ref^{-1}, Z = A K_ref and
+// N = (K_ref^T Z)^{-1} (the body mobility) the free-body system solves in the
+// body frame as y = A r_ref, V_ref = N (f_ref - K_ref^T y), f_b = R (y + Z V_ref).
+__device__ __forceinline__ void rot_of(const double* q, double* R) {
+    const double w = q[0], x = q[1], y = q[2], z = q[3];
+    R[0] = 1.0 - 2.0 * (y * y + z * z);
+    R[1] = 2.0 * (x * y - w * z);
+    R[2] = 2.0 * (x * z + w * y);
+    R[3] = 2.0 * (x * y + w * z);
+    R[4] = 1.0 - 2.0 * (x * x + z * z);
+    R[5] = 2.0 * (y * z - w * x);
+    R[6] = 2.0 * (x * z - w * y);
+    R[7] = 2.0 * (y * z + w * x);
+    R[8] = 1.0 - 2.0 * (x * x + y * y);
+}
+
+// Column k of K_ref at every blob: e_k (k < 3) or e_{k-3} x ref.
+__global__ __launch_bounds__(256) void kref_column_kernel(const double* __restrict__ refs, int nb, int k,
+                                                          double* __restrict__ x) {
+    const int m = blockIdx.x * 256 + threadIdx.x;
+    if (m >= nb) return;
+    const double* r = refs + 3 * m;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (k < 3) {
+        v[k] = 1.0;
+    } else {
+        const int a = k - 3;
+        // e_a x r
+        v[0] = a == 1 ? r[2] : (a == 2 ? -r[1] : 0.0);
+        v[1] = a == 2 ? r[0] : (a == 0 ? -r[2] : 0.0);
+        v[2] = a == 0 ? r[1] : (a == 1 ? -r[0] : 0.0);
+    }
+    for (int c = 0; c < 3; ++c) x[3 * m + c] = v[c];
+}
+
+// out = K_ref^T y = (sum y_m, sum ref_m x y_m), one workgroup.
+__device__ void kref_transpose(const double* refs, int nb, const double* y, double* out, double (*sm)[256]) {
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int m = threadIdx.x; m < nb; m += 256) {
+        const double* r = refs + 3 * m;
+        const double* f = y + 3 * m;
+        acc[0] += f[0];
+        acc[1] += f[1];
+        acc[2] += f[2];
+        acc[3] += r[1] * f[2] - r[2] * f[1];
+        acc[4] += r[2] * f[0] - r[0] * f[2];
+        acc[5] += r[0] * f[1] - r[1] * f[0];
+    }
+    for (int k = 0; k < 6; ++k) sm[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 6; ++k) sm[k][threadIdx.x] += sm[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[threadIdx.x] = sm[threadIdx.x][0];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void kref_transpose_kernel(const double* __restrict__ refs, int nb,
+                                                             const double* __restrict__ y, double* __restrict__ out) {
+    __shared__ double sm[6][256];
+    kref_transpose(refs, nb, y, out, sm);
+}
+
+// Blob rows into the body frame, r_ref = R^T r; f_ref = R^T f from the body
+// rows (which hold -f).
+__global__ __launch_bounds__(256) void to_body_frame_kernel(const double* __restrict__ rhs, int nb,
+                                                            const double* __restrict__ q, double* __restrict__ xpad,
+                                                            double* __restrict__ fref) {
+    const int m = blockIdx.x * 256 + threadIdx.x;
+    double R[9];
+    rot_of(q, R);
+    if (m < nb) {
+        const double* r = rhs + 3 * m;
+        for (int c = 0; c < 3; ++c) xpad[3 * m + c] = R[c] * r[0] + R[3 + c] * r[1] + R[6 + c] * r[2];
+    }
+    if (m == 0) {
+        const double* b = rhs + 3 * nb;
+        for (int c = 0; c < 3; ++c) {
+            fref[c] = -(R[c] * b[0] + R[3 + c] * b[1] + R[6 + c] * b[2]);
+            fref[3 + c] = -(R[c] * b[3] + R[3 + c] * b[4] + R[6 + c] * b[5]);
+        }
+    }
+}
+
+// V_ref = N (f_ref - K_ref^T y); the body velocity R V_ref into sol_body.
+__global__ __launch_bounds__(256) void free_direct_velocity_kernel(const double* __restrict__ refs, int nb,
+                                                                   const double* __restrict__ y,
+                                                                   const double* __restrict__ fref,
+                                                                   const double* __restrict__ nref,
+                                                                   const double* __restrict__ q,
+                                                                   double* __restrict__ vref,
+                                                                   double* __restrict__ sol_body) {
+    __shared__ double sm[6][256];
+    __shared__ double kt[6];
+    kref_transpose(refs, nb, y, kt, sm);
+    if (threadIdx.x == 0) {
+        double g[6], v[6];
+        for (int k = 0; k < 6; ++k) g[k] = fref[k] - kt[k];
+        for (int k = 0; k < 6; ++k) {
+            double a = 0.0;
+            for (int j = 0; j < 6; ++j) a += nref[6 * k + j] * g[j];
+            v[k] = a;
+            vref[k] = a;
+        }
+        double R[9];
+        rot_of(q, R);
+        for (int c = 0; c < 3; ++c) {
+            sol_body[c] = R[3 * c] * v[0] + R[3 * c + 1] * v[1] + R[3 * c + 2] * v[2];
+            sol_body[3 + c] = R[3 * c] * v[3] + R[3 * c + 1] * v[4] + R[3 * c + 2] * v[5];
+        }
+    }
+}
+
+// f_b = R (y + Z V_ref) per blob.
+__global__ __launch_bounds__(256) void free_direct_forces_kernel(const double* __restrict__ y,
+                                                                 const double* __restrict__ Z, int64_t n_pad,
+                                                                 const double* __restrict__ vref,
+                                                                 const double* __restrict__ q, int nb,
+                                                                 double* __restrict__ out) {
+    const int m = blockIdx.x * 256 + threadIdx.x;
+    if (m >= nb) return;
+    double R[9];
+    rot_of(q, R);
+    double b[3];
+    for (int c = 0; c < 3; ++c) {
+        double a = y[3 * m + c];
+        for (int k = 0; k < 6; ++k) a += Z[k * n_pad + 3 * m + c] * vref[k];
+        b[c] = a;
+    }
+    for (int c = 0; c < 3; ++c) out[3 * m + c] = R[3 * c] * b[0] + R[3 * c + 1] * b[1] + R[3 * c + 2] * b[2];
+}
+
 // flag = 1 if a[i] != b[i] for some i (every writer stores the same value).
 __global__ __launch_bounds__(256) void differs_kernel(const double* __restrict__ a, const double* __restrict__ b,
                                                      int64_t n, int32_t* __restrict__ flag) {
@@ -119,12 +255,16 @@ struct tfs_velocities {
     double *Ainv = nullptr, *xpad = nullptr, *ypad = nullptr, *sym_partial = nullptr;
     void* dense_arena = nullptr;
     tfs::DenseWork dw;
+    // one free body, direct: body-frame blob positions, Z = A K_ref (6
+    // columns of n_pad), N (6 x 6), and V_ref, f_ref, K^T Z scratch
+    double *body_refs = nullptr, *Z = nullptr, *nref = nullptr, *small = nullptr, *d_q = nullptr;
+    bool free_direct_valid = false;
 };
 
 namespace {
 
-// Forms v->Ainv = M_bb^{-1} for the current blob positions.
-int build_direct(tfs_velocities* v) {
+// Forms v->Ainv = M_bb^{-1} for the blob positions x (device, 3 per blob).
+int build_direct(tfs_velocities* v, const double* x) {
     const int nb = (int)v->nb;
     const int n = 3 * nb;
     const int n_pad = (n + kDenseTile - 1) / kDenseTile * kDenseTile;
@@ -158,7 +298,7 @@ int build_direct(tfs_velocities* v) {
         if (!rc) rc = check(hipStreamSynchronize(st), "direct blob solve");
         if (rc) return rc;
     }
-    int rc = check(k_blob_mobility_dense(v->rpy->x_blobs, nb, v->desc.blob_radius, v->desc.mu, v->Ainv, n_pad, st),
+    int rc = check(k_blob_mobility_dense(x, nb, v->desc.blob_radius, v->desc.mu, v->Ainv, n_pad, st),
                    "dense blob mobility");
     if (!rc) rc = check(k_spd_inverse_blocked(v->Ainv, n_pad, v->dw, st), "dense blob inverse");
     std::vector<int32_t> status(nt);
@@ -169,6 +309,66 @@ int build_direct(tfs_velocities* v) {
         if (status[k]) return fail(TFS_ERR_NON_FINITE, "direct blob solve: singular pivot block " + std::to_string(k));
     v->direct_valid = true;
     return TFS_OK;
+}
+
+// The inverse at the body-frame reference positions, Z = A K_ref and N.
+int build_free_direct(tfs_velocities* v) {
+    const int nb = (int)v->nb;
+    hipStream_t st = v->stream;
+    int rc = build_direct(v, v->body_refs);
+    v->direct_valid = false;  // not the lab-frame inverse
+    if (rc) return rc;
+    const int64_t n_pad = v->n_pad;
+    if (!v->Z) {
+        if (hipMalloc(&v->Z, 8 * (6 * n_pad + 36 + 32)) != hipSuccess)
+            return fail(TFS_ERR_OUT_OF_MEMORY, "direct free-body solve: work space");
+        v->nref = v->Z + 6 * n_pad;
+        v->small = v->nref + 36;
+    }
+    double kz[36];
+    for (int k = 0; k < 6 && !rc; ++k) {
+        hipLaunchKernelGGL(kref_column_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st,
+                           (const double*)v->body_refs, nb, k, v->xpad);
+        rc = check(hipGetLastError(), "K_ref column");
+        if (!rc) rc = check(k_symv_dense(v->Ainv, v->n_pad, v->xpad, v->ypad, v->sym_partial, v->chunks, st), "A K_ref");
+        if (!rc) rc = check(hipMemcpyAsync(v->Z + k * n_pad, v->ypad, 8 * n_pad, hipMemcpyDeviceToDevice, st), "Z");
+        if (!rc) {
+            hipLaunchKernelGGL(kref_transpose_kernel, dim3(1), dim3(256), 0, st, (const double*)v->body_refs, nb,
+                               (const double*)v->ypad, v->small);
+            rc = check(hipGetLastError(), "K_ref^T Z");
+        }
+        double col[6];
+        if (!rc) rc = check(hipMemcpyAsync(col, v->small, 48, hipMemcpyDeviceToHost, st), "K_ref^T Z");
+        if (!rc) rc = check(hipStreamSynchronize(st), "K_ref^T Z");
+        for (int r = 0; r < 6; ++r) kz[6 * r + k] = col[r];
+    }
+    if (rc) return rc;
+    double a[6][12];
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 12; ++c) a[r][c] = c < 6 ? kz[6 * r + c] : (c - 6 == r ? 1.0 : 0.0);
+    for (int k = 0; k < 6; ++k) {
+        int piv = k;
+        for (int r = k + 1; r < 6; ++r)
+            if (std::fabs(a[r][k]) > std::fabs(a[piv][k])) piv = r;
+        if (!(std::fabs(a[piv][k]) > 0.0) || !std::isfinite(a[piv][k]))
+            return fail(TFS_ERR_NON_FINITE, "direct free-body solve: singular body resistance");
+        if (piv != k)
+            for (int c = 0; c < 12; ++c) std::swap(a[k][c], a[piv][c]);
+        const double inv_p = 1.0 / a[k][k];
+        for (int c = 0; c < 12; ++c) a[k][c] *= inv_p;
+        for (int r = 0; r < 6; ++r)
+            if (r != k) {
+                const double f = a[r][k];
+                for (int c = 0; c < 12; ++c) a[r][c] -= f * a[k][c];
+            }
+    }
+    double nr[36];
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) nr[6 * r + c] = a[r][6 + c];
+    rc = check(hipMemcpyAsync(v->nref, nr, 288, hipMemcpyHostToDevice, st), "N");
+    if (!rc) rc = check(hipStreamSynchronize(st), "N");
+    if (!rc) v->free_direct_valid = true;
+    return rc;
 }
 
 }  // namespace
@@ -242,6 +442,7 @@ int tfs_velocities_create(const tfs_velocities_desc* desc, tfs_velocities** out)
     if (!rc) rc = tfs_solver_create(&sd, &v->blobs);
     if (!rc) rc = check(hipMalloc(&v->d_rhs, sizeof(double) * nsys), "tfs_velocities_create: rhs");
     if (!rc && D.free_bodies) rc = check(hipMalloc(&v->d_sol, sizeof(double) * nsys), "tfs_velocities_create: solution");
+    if (!rc && D.free_bodies) rc = check(hipMalloc(&v->d_q, 32LL * D.num_swim), "tfs_velocities_create: orientations");
     if (!rc) rc = check(hipMalloc(&v->d_bodies, sizeof(double) * 9 * v->nswim), "tfs_velocities_create: bodies");
     if (!rc) rc = check(hipMalloc(&v->d_flag, sizeof(int32_t)), "tfs_velocities_create: flag");
     if (!rc) rc = check(hipHostMalloc(&v->h_flag, sizeof(int32_t)), "tfs_velocities_create: flag");
@@ -263,6 +464,9 @@ int tfs_velocities_destroy(tfs_velocities* v) {
     if (v->d_rhs) (void)hipFree(v->d_rhs);
     if (v->d_bodies) (void)hipFree(v->d_bodies);
     if (v->d_sol) (void)hipFree(v->d_sol);
+    if (v->body_refs) (void)hipFree(v->body_refs);
+    if (v->Z) (void)hipFree(v->Z);
+    if (v->d_q) (void)hipFree(v->d_q);
     if (v->d_flag) (void)hipFree(v->d_flag);
     if (v->Ainv) (void)hipFree(v->Ainv);
     if (v->dense_arena) (void)hipFree(v->dense_arena);
@@ -281,8 +485,8 @@ namespace {
 // force and torque or NULL; v_bodies_io the body velocities' warm start and
 // result).
 int compute(tfs_velocities* v, const double* x_segs, const double* f_segs, const double* x_blobs,
-            const double* x_bodies, const double* bodies, double* v_segs, double* f_blobs, double* v_bodies_io,
-            tfs_solve_stats* stats, bool free_bodies) {
+            const double* x_bodies, const double* bodies, const double* q_bodies, double* v_segs, double* f_blobs,
+            double* v_bodies_io, tfs_solve_stats* stats, bool free_bodies) {
     const hipMemcpyKind kin = v->desc.memory == TFS_MEMORY_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
     const hipMemcpyKind kout = v->desc.memory == TFS_MEMORY_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
     tfs_rpy* h = v->rpy;
@@ -321,6 +525,7 @@ int compute(tfs_velocities* v, const double* x_segs, const double* f_segs, const
     if (!rc) rc = check(hipMemcpyAsync(v->d_bodies, x_bodies, 24 * v->nswim, kin, st), "body positions");
     if (!rc && bodies)
         rc = check(hipMemcpyAsync(v->d_bodies + 3 * v->nswim, bodies, 48 * v->nswim, kin, st), "body velocities");
+    if (!rc && q_bodies) rc = check(hipMemcpyAsync(v->d_q, q_bodies, 32 * v->nswim, kin, st), "body orientations");
     if (!rc) rc = tfs_rpy_positions_changed(h);
     // Mss_mult '=', Mbs_mult '='
     if (!rc && ns) rc = tfs_rpy_apply(h, TFS_BLOCK_SS, h->f_segs, nullptr, h->v_segs, nullptr);
@@ -343,8 +548,32 @@ int compute(tfs_velocities* v, const double* x_segs, const double* f_segs, const
     if (!rc) v->have_blobs = true;
     if (!rc && blobs_moved) v->direct_valid = false;
     int solve_rc = TFS_OK;
-    if (!rc && v->direct) {
-        if (!v->direct_valid) rc = build_direct(v);
+    if (!rc && v->direct && free_bodies) {
+        // one free body: in its frame with the stored reference inverse
+        if (!v->free_direct_valid) rc = build_free_direct(v);
+        const unsigned g = (unsigned)((nb + 255) / 256);
+        if (!rc) {
+            hipLaunchKernelGGL(to_body_frame_kernel, dim3(g), dim3(256), 0, st, (const double*)v->d_rhs, (int)nb,
+                               (const double*)v->d_q, v->xpad, v->small + 8);
+            rc = check(hipGetLastError(), "body frame");
+        }
+        if (!rc) rc = check(k_symv_dense(v->Ainv, v->n_pad, v->xpad, v->ypad, v->sym_partial, v->chunks, st), "direct solve");
+        if (!rc) {
+            hipLaunchKernelGGL(free_direct_velocity_kernel, dim3(1), dim3(256), 0, st, (const double*)v->body_refs,
+                               (int)nb, (const double*)v->ypad, (const double*)(v->small + 8),
+                               (const double*)v->nref, (const double*)v->d_q, v->small, sol + 3 * nb);
+            hipLaunchKernelGGL(free_direct_forces_kernel, dim3(g), dim3(256), 0, st, (const double*)v->ypad,
+                               (const double*)v->Z, (int64_t)v->n_pad, (const double*)v->small,
+                               (const double*)v->d_q, (int)nb, h->f_blobs);
+            rc = check(hipGetLastError(), "direct free-body solve");
+        }
+        if (!rc) rc = check(hipMemcpyAsync(sol, h->f_blobs, 24 * nb, hipMemcpyDeviceToDevice, st), "blob forces");
+        if (stats) {
+            *stats = tfs_solve_stats{};
+            stats->converged = 1;
+        }
+    } else if (!rc && v->direct) {
+        if (!v->direct_valid) rc = build_direct(v, v->rpy->x_blobs);
         if (!rc) rc = check(hipMemcpyAsync(v->xpad, v->d_rhs, 24 * nb, hipMemcpyDeviceToDevice, st), "direct rhs");
         if (!rc) rc = check(k_symv_dense(v->Ainv, v->n_pad, v->xpad, v->ypad, v->sym_partial, v->chunks, st), "direct solve");
         if (!rc) rc = check(hipMemcpyAsync(h->f_blobs, v->ypad, 24 * nb, hipMemcpyDeviceToDevice, st), "direct result");
@@ -385,28 +614,41 @@ int tfs_velocities_compute(tfs_velocities* v, const double* x_segs, const double
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_compute: null argument");
     if (v->desc.free_bodies)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_compute: free bodies: use tfs_velocities_compute_free");
-    return compute(v, x_segs, f_segs, x_blobs, x_bodies, v_bodies, v_segs, f_blobs, nullptr, stats, false);
+    return compute(v, x_segs, f_segs, x_blobs, x_bodies, v_bodies, nullptr, v_segs, f_blobs, nullptr, stats, false);
 }
 
 int tfs_velocities_compute_free(tfs_velocities* v, const double* x_segs, const double* f_segs, const double* x_blobs,
-                                const double* x_bodies, const double* f_bodies, double* v_segs, double* f_blobs,
-                                double* v_bodies, tfs_solve_stats* stats) {
+                                const double* x_bodies, const double* f_bodies, const double* q_bodies, double* v_segs,
+                                double* f_blobs, double* v_bodies, tfs_solve_stats* stats) {
     if (!v || !x_blobs || !x_bodies || !v_bodies || !f_blobs || (v->ns && (!x_segs || !f_segs || !v_segs)))
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_compute_free: null argument");
     if (!v->desc.free_bodies)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_compute_free: the handle has no free bodies");
-    if (v->direct)
+    if (v->direct && (!q_bodies || !v->body_refs))
         return fail(TFS_ERR_INVALID_ARGUMENT,
-                    "tfs_velocities_compute_free: free bodies take the GMRES blob solve (not TFS_BLOB_DIRECT)");
-    return compute(v, x_segs, f_segs, x_blobs, x_bodies, f_bodies, v_segs, f_blobs, v_bodies, stats, true);
+                    "tfs_velocities_compute_free: the direct solve needs the orientation and "
+                    "tfs_velocities_set_body_frame");
+    return compute(v, x_segs, f_segs, x_blobs, x_bodies, f_bodies, q_bodies, v_segs, f_blobs, v_bodies, stats, true);
+}
+
+int tfs_velocities_set_body_frame(tfs_velocities* v, const double* blob_refs) {
+    if (!v || !blob_refs) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_set_body_frame: null argument");
+    if (!v->desc.free_bodies) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_set_body_frame: no free bodies");
+    if (!v->body_refs && hipMalloc(&v->body_refs, 24 * v->nb) != hipSuccess)
+        return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_velocities_set_body_frame: device memory");
+    const hipMemcpyKind kin = v->desc.memory == TFS_MEMORY_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    int rc = check(hipMemcpyAsync(v->body_refs, blob_refs, 24 * v->nb, kin, v->stream), "tfs_velocities_set_body_frame");
+    if (!rc) rc = check(hipStreamSynchronize(v->stream), "tfs_velocities_set_body_frame");
+    v->free_direct_valid = false;
+    return rc;
 }
 
 int tfs_velocities_set_preconditioner(tfs_velocities* v, int32_t kind) {
     if (!v) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_set_preconditioner: null handle");
     if (kind == TFS_BLOB_DIRECT) {
-        if (v->desc.free_bodies)
+        if (v->desc.free_bodies && v->nswim != 1)
             return fail(TFS_ERR_INVALID_ARGUMENT,
-                        "tfs_velocities_set_preconditioner: free bodies take the GMRES blob solve");
+                        "tfs_velocities_set_preconditioner: the direct solve takes one free body");
         v->direct = true;
         return TFS_OK;
     }

@@ -99,7 +99,8 @@ class VelocitiesDesc(ctypes.Structure):
 
 
 L.extra_signatures({
-    "tfs_velocities_compute_free": (ctypes.c_int, [_dp] * 9 + [ctypes.POINTER(SolveStats)]),
+    "tfs_velocities_compute_free": (ctypes.c_int, [_dp] * 10 + [ctypes.POINTER(SolveStats)]),
+    "tfs_velocities_set_body_frame": (ctypes.c_int, [_dp, _dp]),
     "tfs_velocities_create": (ctypes.c_int, [ctypes.POINTER(VelocitiesDesc), ctypes.POINTER(_dp)]),
     "tfs_velocities_destroy": (ctypes.c_int, [_dp]),
     "tfs_velocities_compute": (ctypes.c_int, [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(SolveStats)]),
@@ -245,19 +246,28 @@ class ElasticVelocities:
             L.check(rc)
         return v_segs, self.f_blobs.copy(), st.as_dict()
 
-    def compute_free(self, x_segs, f_segs, x_blobs, x_bodies, f_bodies=None, *, allow_unconverged=False):
+    def set_body_frame(self, blob_refs):
+        """Body-frame blob positions for the direct free-body solve (one body)."""
+        self._refs = np.ascontiguousarray(blob_refs, dtype=np.float64)
+        L.check(L.lib().tfs_velocities_set_body_frame(self.h, L.ptr(self._refs)))
+
+    def compute_free(self, x_segs, f_segs, x_blobs, x_bodies, f_bodies=None, q_bodies=None, *,
+                     allow_unconverged=False):
         """Free bodies (a handle made with free_bodies=True): returns (v_segs,
         f_blobs, v_bodies [V, Omega per body], solve stats); the blob forces and
         body velocities warm-start the next call.  f_bodies: external force and
-        torque per body, or None.  Host memory only."""
+        torque per body, or None; q_bodies: the orientation (4 per body), needed
+        by the direct solve.  Host memory only."""
         if self.desc.memory != L.MEMORY_HOST:
             raise ValueError("compute_free() takes host arrays; this handle was made for device memory")
         a = [np.ascontiguousarray(v, dtype=np.float64) for v in (x_segs, f_segs, x_blobs, x_bodies)]
         fb = None if f_bodies is None else np.ascontiguousarray(f_bodies, dtype=np.float64)
+        qb = None if q_bodies is None else np.ascontiguousarray(q_bodies, dtype=np.float64)
         v_segs = np.zeros(6 * self.ns)
         st = SolveStats()
         rc = L.lib().tfs_velocities_compute_free(self.h, *[L.ptr(v) if v.size else None for v in a], L.ptr(fb),
-                                                 L.ptr(v_segs) if v_segs.size else None, L.ptr(self.f_blobs),
+                                                 L.ptr(qb), L.ptr(v_segs) if v_segs.size else None,
+                                                 L.ptr(self.f_blobs),
                                                  L.ptr(self.v_bodies), ctypes.byref(st))
         if not (allow_unconverged and rc == L.TFS_ERR_NOT_CONVERGED):
             L.check(rc)
