@@ -447,12 +447,16 @@ __global__ __launch_bounds__(256) void near_grid_kernel(MatvecArgs a, int n, dou
                                                        const int32_t* __restrict__ start,
                                                        const int32_t* __restrict__ order,
                                                        const int32_t* __restrict__ body_of,
-                                                       int32_t* __restrict__ cnt, int32_t* __restrict__ idx, int cap,
-                                                       int32_t* __restrict__ max_cnt) {
+                                                       int32_t* __restrict__ cnt, int32_t* __restrict__ idx,
+                                                       int cap) {
+    // 32 lanes per target, lane l < 27 visiting neighbour cell l; near sources
+    // are appended in any order (near_rank_sort_kernel orders them).  cnt must
+    // start at zero and ends as the full count, entries past cap dropped.
     constexpr bool F32 = sizeof(T) == 4;
     constexpr int REC = rec_len<T, 3>();
-    const int tg = blockIdx.x * 256 + threadIdx.x;
-    if (tg >= n) return;
+    const int tg = blockIdx.x * 8 + (int)(threadIdx.x >> 5);
+    const int lane = (int)(threadIdx.x & 31);
+    if (tg >= n || lane >= 27) return;
     const int cls = tg < a.ns ? 0 : 1;
     const int t = cls == 0 ? tg : tg - a.ns;
     using XT = typename std::conditional<F32, int32_t, double>::type;
@@ -460,40 +464,33 @@ __global__ __launch_bounds__(256) void near_grid_kernel(MatvecArgs a, int n, dou
     load_target(a, cls, t, xi);
     double xd[3];
     particle_pos(a, tg, xd);
-    const int cx = cell_coord(xd[0], inv_h), cy = cell_coord(xd[1], inv_h), cz = cell_coord(xd[2], inv_h);
+    const int nx = cell_coord(xd[0], inv_h) + lane % 3 - 1;
+    const int ny = cell_coord(xd[1], inv_h) + (lane / 3) % 3 - 1;
+    const int nz = cell_coord(xd[2], inv_h) + lane / 9 - 1;
     const int body = body_of[tg];
     const double ai = cls == 0 ? a.a_seg : a.a_blob;
     const double lam = F32 ? a.lam : 1.0;
-    int k = 0;
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int nx = cx + dx, ny = cy + dy, nz = cz + dz;
-                const uint32_t b = cell_hash(nx, ny, nz, mask);
-                for (int q = start[b]; q < start[b + 1]; ++q) {
-                    const int j = order[q];
-                    if (body_of[j] != body) continue;
-                    double xj[3];
-                    particle_pos(a, j, xj);
-                    // the bucket may also hold other cells: visit j from its own cell only
-                    if (cell_coord(xj[0], inv_h) != nx || cell_coord(xj[1], inv_h) != ny ||
-                        cell_coord(xj[2], inv_h) != nz)
-                        continue;
-                    const int jcls = j < a.ns ? 0 : 1;
-                    T rec[REC];
-                    stage_source<T, 3>(rec, a, jcls, jcls == 0 ? j : j - a.ns, false);
-                    const double aj = jcls == 0 ? a.a_seg : a.a_blob;
-                    const T sum_a2 = T(lam * lam * ((ai + aj) * (ai + aj)));
-                    T ddx, ddy, ddz;
-                    const T r2 = separation(xi, rec, ddx, ddy, ddz);
-                    if (!(r2 > sum_a2)) {
-                        if (k < cap) idx[(int64_t)tg * cap + k] = j;
-                        ++k;
-                    }
-                }
-            }
-    cnt[tg] = k;
-    if (k > cap) atomicMax(max_cnt, k);
+    const uint32_t b = cell_hash(nx, ny, nz, mask);
+    for (int q = start[b]; q < start[b + 1]; ++q) {
+        const int j = order[q];
+        if (body_of[j] != body) continue;
+        double xj[3];
+        particle_pos(a, j, xj);
+        // the bucket may also hold other cells: visit j from its own cell only
+        if (cell_coord(xj[0], inv_h) != nx || cell_coord(xj[1], inv_h) != ny || cell_coord(xj[2], inv_h) != nz)
+            continue;
+        const int jcls = j < a.ns ? 0 : 1;
+        T rec[REC];
+        stage_source<T, 3>(rec, a, jcls, jcls == 0 ? j : j - a.ns, false);
+        const double aj = jcls == 0 ? a.a_seg : a.a_blob;
+        const T sum_a2 = T(lam * lam * ((ai + aj) * (ai + aj)));
+        T ddx, ddy, ddz;
+        const T r2 = separation(xi, rec, ddx, ddy, ddz);
+        if (!(r2 > sum_a2)) {
+            const int k = atomicAdd(cnt + tg, 1);
+            if (k < cap) idx[(int64_t)tg * cap + k] = j;
+        }
+    }
 }
 
 __global__ void fill_int_kernel(int32_t* p, int32_t v, int64_t n) {
@@ -720,12 +717,51 @@ __global__ __launch_bounds__(256) void rpy_reduce_kernel(const MatvecTile* __res
     }
 }
 
-// Sorts every target's near list into increasing source order (insertion
-// sort of at most `cap` entries, one thread per target).
+// Sorts every target's near list into increasing source order, 32 lanes per
+// target: each lane ranks up to kRankPer entries against the whole list (the
+// entries are distinct source indices), then all write after a barrier.
+// Lists up to 32 kRankPer long; *max_cnt gets any count above cap.
+constexpr int kRankPer = 8;
+__global__ __launch_bounds__(256) void near_rank_sort_kernel(const int32_t* __restrict__ cnt,
+                                                             int32_t* __restrict__ idx, int cap, int64_t n,
+                                                             int32_t* __restrict__ max_cnt) {
+    const int64_t tg = (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);
+    const int lane = (int)(threadIdx.x & 31);
+    int c = 0;
+    int32_t* e = idx;
+    if (tg < n) {
+        const int k = cnt[tg];
+        c = k < cap ? k : cap;
+        e = idx + tg * cap;
+        if (lane == 0 && k > cap) atomicMax(max_cnt, k);
+    }
+    int32_t val[kRankPer];
+    int rank[kRankPer];
+#pragma unroll
+    for (int s = 0; s < kRankPer; ++s) {
+        const int i = lane + 32 * s;
+        val[s] = 0;
+        rank[s] = -1;
+        if (i < c) {
+            const int32_t v = e[i];
+            int r = 0;
+            for (int j = 0; j < c; ++j) r += e[j] < v ? 1 : 0;
+            val[s] = v;
+            rank[s] = r;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kRankPer; ++s)
+        if (rank[s] >= 0) e[rank[s]] = val[s];
+}
+
+// The same for longer lists (insertion sort, one thread per target).
 __global__ __launch_bounds__(256) void near_sort_kernel(const int32_t* __restrict__ cnt, int32_t* __restrict__ idx,
-                                                        int cap, int64_t n) {
+                                                        int cap, int64_t n, int32_t* __restrict__ max_cnt) {
     const int64_t tg = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (tg >= n) return;
+    if (cnt[tg] > cap) atomicMax(max_cnt, cnt[tg]);
     const int c = cnt[tg] < cap ? cnt[tg] : cap;
     int32_t* e = idx + tg * cap;
     for (int i = 1; i < c; ++i) {
@@ -844,17 +880,24 @@ hipError_t build_near_list(int precision, const MatvecArgs& a, NearGrid& grid, c
     TFS_HIP_CHECK(sort_into_grid(a, grid, n, inv_h, stream));
     const uint32_t mask = (uint32_t)grid.table - 1;
     const unsigned gn = (unsigned)((n + 255) / 256);
+    const unsigned gl = (unsigned)((n + 7) / 8);   // 8 targets of 32 lanes per workgroup
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, stream, near.max_cnt, 0, (int64_t)1);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(gn), dim3(256), 0, stream, near.cnt, 0, (int64_t)n);
     hipLaunchKernelGGL(grid_body_kernel, dim3((unsigned)nbody), dim3(256), 0, stream,
                        reinterpret_cast<const int4*>(body_ranges), a.ns, grid.body_of);
     if (precision == 0)
-        hipLaunchKernelGGL(near_grid_kernel<double>, dim3(gn), dim3(256), 0, stream, a, n, inv_h, mask, grid.start,
-                           grid.order, grid.body_of, near.cnt, near.idx, near.cap, near.max_cnt);
+        hipLaunchKernelGGL(near_grid_kernel<double>, dim3(gl), dim3(256), 0, stream, a, n, inv_h, mask, grid.start,
+                           grid.order, grid.body_of, near.cnt, near.idx, near.cap);
     else
-        hipLaunchKernelGGL(near_grid_kernel<float>, dim3(gn), dim3(256), 0, stream, a, n, inv_h, mask, grid.start,
-                           grid.order, grid.body_of, near.cnt, near.idx, near.cap, near.max_cnt);
+        hipLaunchKernelGGL(near_grid_kernel<float>, dim3(gl), dim3(256), 0, stream, a, n, inv_h, mask, grid.start,
+                           grid.order, grid.body_of, near.cnt, near.idx, near.cap);
     TFS_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(near_sort_kernel, dim3(gn), dim3(256), 0, stream, near.cnt, near.idx, near.cap, (int64_t)n);
+    if (near.cap <= 32 * kRankPer)
+        hipLaunchKernelGGL(near_rank_sort_kernel, dim3(gl), dim3(256), 0, stream, near.cnt, near.idx, near.cap,
+                           (int64_t)n, near.max_cnt);
+    else
+        hipLaunchKernelGGL(near_sort_kernel, dim3(gn), dim3(256), 0, stream, near.cnt, near.idx, near.cap,
+                           (int64_t)n, near.max_cnt);
     TFS_HIP_CHECK(hipGetLastError());
     TFS_HIP_CHECK(hipMemcpyAsync(host_scratch, near.max_cnt, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
     TFS_HIP_CHECK(hipStreamSynchronize(stream));
