@@ -448,7 +448,7 @@ def main():
         print(f"bench.py: unknown legs {sorted(unknown)}", file=sys.stderr)
         sys.exit(2)
     if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
+        print(f"bench.py: --gpus {args.gpus} is not supported: the library runs on one GPU "
               "(multi-GPU target sharding is designed, not built)", file=sys.stderr)
         sys.exit(2)
 
