@@ -22,6 +22,17 @@ reference solves it by GMRES; a dense solve here):
 
 Broyden vector of the swimmer: NFIL filament blocks of 6 NSEG, then the body's
 6 unknowns [position (3), Lie-algebra element (3)].
+
+What pins it (tests/test_oracle_swimmer.py): the coupled inverse against a
+dense solve of the assembled block Jacobian; anchoring and quadratic
+extrapolation on closed forms; the free blob solve's force and torque balance
+and rigid blob motion; the body mobility of a sphere of blobs against Stokes
+drag; convergence of the free swimmer, with fewer iterations than an identity
+body block; the Broyden-for-everything loop reaching the trajectory of the
+form that solves for the blob forces.  The individual entries of A, B and S0
+in body_coupling (swimmer.cpp:688-880) and the blob forces' extrapolation in
+blob_force_guess are restated without a check of their own: parity unpinned
+beyond the structural and convergence checks above.
 """
 from __future__ import annotations
 
