@@ -154,7 +154,9 @@ int tfs_filament_read(tfs_filament *h, int32_t which, double *out);
  * The three handles are the caller's, created with TFS_MEMORY_DEVICE and sizes
  * that agree (tfs_velocities: num_swim bodies of fil_per_swim*num_seg segments;
  * tfs_broyden: the block sizes of the Broyden layout above).  Only the
- * too-large flag of each residual crosses to the host. */
+ * too-large flag of each residual crosses to the host.  tfs_elastic_create
+ * sets the velocities handle's blob motion (tfs_velocities_set_blob_motion):
+ * TFS_BLOBS_FIXED for bodies held still, TFS_BLOBS_MOVING for free bodies. */
 struct tfs_velocities;
 struct tfs_broyden;
 typedef struct tfs_elastic_desc {

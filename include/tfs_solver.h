@@ -270,6 +270,16 @@ int tfs_velocities_compute_free(tfs_velocities *v, const double *x_segs, const d
 /* The body-frame blob positions (3 per blob, relative to the centre, at
  * orientation 1) for the direct free-body solve. */
 int tfs_velocities_set_body_frame(tfs_velocities *v, const double *blob_refs);
+/* How a call learns whether the blob positions changed since the last one
+ * (the blob solve's geometry, or the dense inverse, is renewed when they did):
+ * TFS_BLOBS_CHECK compares them on the device (one flag read back per call,
+ * the default), TFS_BLOBS_FIXED takes them as unchanged after the first call
+ * (a body held still), TFS_BLOBS_MOVING as changed on every call (a free
+ * body).  No counterpart in the reference. */
+#define TFS_BLOBS_CHECK 0
+#define TFS_BLOBS_FIXED 1
+#define TFS_BLOBS_MOVING 2
+int tfs_velocities_set_blob_motion(tfs_velocities *v, int32_t motion);
 
 #ifdef __cplusplus
 }

@@ -128,6 +128,7 @@ def test_newer_entry_points_reject_null_handles_and_bad_descriptors():
                      ("tfs_broyden_set_body_coupling", (None, 1, 1, None, None, None)),
                      ("tfs_velocities_compute_free", (None,) * 10 + (None,)),
                      ("tfs_velocities_set_body_frame", (None, None)),
+                     ("tfs_velocities_set_blob_motion", (None, 0)),
                      ("tfs_elastic_backup_size", (None, None)), ("tfs_elastic_backup", (None, None, None)),
                      ("tfs_elastic_restore", (None, None))):
         assert getattr(lib, fn)(*args) == L.TFS_ERR_INVALID_ARGUMENT, fn

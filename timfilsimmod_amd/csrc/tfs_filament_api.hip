@@ -1344,6 +1344,10 @@ int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
     hipError_t err = hipMemcpy(e->blocks, host.data(), 8 * nb, hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemset(e->zeros, 0, 48 * nswim);
     if (err == hipSuccess) err = hipMemset(e->v_bodies, 0, 48 * nswim);
+    // the step knows how its blobs move: held still, or with a free body
+    if (err == hipSuccess && desc->velocities &&
+        tfs_velocities_set_blob_motion(desc->velocities, free_bodies ? TFS_BLOBS_MOVING : TFS_BLOBS_FIXED) != TFS_OK)
+        err = hipErrorInvalidValue;
     if (err == hipSuccess && free_bodies && fh->nblob &&
         tfs_velocities_set_body_frame(desc->velocities, fh->blob_refs) != TFS_OK) {
         const std::string msg = g_last_error;

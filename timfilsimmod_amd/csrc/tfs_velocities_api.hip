@@ -259,6 +259,7 @@ struct tfs_velocities {
     // columns of n_pad), N (6 x 6), and V_ref, f_ref, K^T Z scratch
     double *body_refs = nullptr, *Z = nullptr, *nref = nullptr, *small = nullptr, *d_q = nullptr;
     bool free_direct_valid = false;
+    int32_t blob_motion = TFS_BLOBS_CHECK;  // tfs_velocities_set_blob_motion
 };
 
 namespace {
@@ -496,7 +497,9 @@ int compute(tfs_velocities* v, const double* x_segs, const double* f_segs, const
     // that did not change keep the blob solve's geometry.
     int rc = TFS_OK;
     bool blobs_moved = !v->have_blobs;
-    if (v->have_blobs) {
+    if (v->have_blobs && v->blob_motion == TFS_BLOBS_MOVING) {
+        blobs_moved = true;
+    } else if (v->have_blobs && v->blob_motion == TFS_BLOBS_CHECK) {
         if (v->desc.memory == TFS_MEMORY_HOST) {
             std::vector<double> old(3 * nb);
             rc = check(hipMemcpyAsync(old.data(), h->x_blobs, 24 * nb, hipMemcpyDeviceToHost, st), "blob positions");
@@ -544,7 +547,9 @@ int compute(tfs_velocities* v, const double* x_segs, const double* f_segs, const
     }
     // GMRES on M_bb for the blob forces, warm-started from the given ones
     // (or, direct, one product with the stored inverse)
-    if (!rc && blobs_moved) rc = tfs_solver_set_geometry(v->blobs, nullptr, h->x_blobs, nullptr, nullptr, TFS_MEMORY_DEVICE);
+    // (the direct free-body solve works in the body frame and needs no geometry)
+    if (!rc && blobs_moved && !(free_bodies && v->direct))
+        rc = tfs_solver_set_geometry(v->blobs, nullptr, h->x_blobs, nullptr, nullptr, TFS_MEMORY_DEVICE);
     if (!rc) v->have_blobs = true;
     if (!rc && blobs_moved) v->direct_valid = false;
     int solve_rc = TFS_OK;
@@ -629,6 +634,13 @@ int tfs_velocities_compute_free(tfs_velocities* v, const double* x_segs, const d
                     "tfs_velocities_compute_free: the direct solve needs the orientation and "
                     "tfs_velocities_set_body_frame");
     return compute(v, x_segs, f_segs, x_blobs, x_bodies, f_bodies, q_bodies, v_segs, f_blobs, v_bodies, stats, true);
+}
+
+int tfs_velocities_set_blob_motion(tfs_velocities* v, int32_t motion) {
+    if (!v || (motion != TFS_BLOBS_CHECK && motion != TFS_BLOBS_FIXED && motion != TFS_BLOBS_MOVING))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_set_blob_motion: null handle or unknown mode");
+    v->blob_motion = motion;
+    return TFS_OK;
 }
 
 int tfs_velocities_set_body_frame(tfs_velocities* v, const double* blob_refs) {

@@ -101,6 +101,7 @@ class VelocitiesDesc(ctypes.Structure):
 L.extra_signatures({
     "tfs_velocities_compute_free": (ctypes.c_int, [_dp] * 10 + [ctypes.POINTER(SolveStats)]),
     "tfs_velocities_set_body_frame": (ctypes.c_int, [_dp, _dp]),
+    "tfs_velocities_set_blob_motion": (ctypes.c_int, [_dp, ctypes.c_int32]),
     "tfs_velocities_create": (ctypes.c_int, [ctypes.POINTER(VelocitiesDesc), ctypes.POINTER(_dp)]),
     "tfs_velocities_destroy": (ctypes.c_int, [_dp]),
     "tfs_velocities_compute": (ctypes.c_int, [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(SolveStats)]),
