@@ -281,7 +281,13 @@ class ElasticVelocities:
         solves at the given pose.  With several bodies each column loads every
         body at once, so the matrices include the bodies' mutual flow (the
         reference moves the other bodies far away first)."""
-        xs = np.zeros(3 * self.ns)
+        # the segments carry no force; placed apart and far away they add no
+        # near pairs (their positions do not enter the result)
+        xb = np.asarray(x_blobs, dtype=np.float64).reshape(-1, 3)
+        far = np.abs(xb).max() + 1e4 if xb.size else 1e4
+        xs = np.zeros((self.ns, 3))
+        xs[:, 0] = far + 10.0 * np.arange(self.ns)
+        xs = xs.reshape(-1)
         fs = np.zeros(6 * self.ns)
         out = np.zeros((self.num_swim, 6, 6))
         for k in range(6):
