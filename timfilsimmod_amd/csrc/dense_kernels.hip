@@ -239,9 +239,17 @@ __global__ __launch_bounds__(256) void symv_partial_kernel(const double* __restr
     const int r = blockIdx.x * 256 + threadIdx.x;
     const int c0 = blockIdx.y * chunk, c1 = min(c0 + chunk, n_pad);
     if (r >= n_pad) return;
-    double acc = 0.0;
-    for (int c = c0; c < c1; ++c) acc += A[(int64_t)c * n_pad + r] * x[c];   // symmetric: row r = column r
-    partial[(int64_t)blockIdx.y * n_pad + r] = acc;
+    // symmetric: row r = column r; four independent chains keep more loads in flight
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int c = c0;
+    for (; c + 4 <= c1; c += 4) {
+        a0 += A[(int64_t)c * n_pad + r] * x[c];
+        a1 += A[(int64_t)(c + 1) * n_pad + r] * x[c + 1];
+        a2 += A[(int64_t)(c + 2) * n_pad + r] * x[c + 2];
+        a3 += A[(int64_t)(c + 3) * n_pad + r] * x[c + 3];
+    }
+    for (; c < c1; ++c) a0 += A[(int64_t)c * n_pad + r] * x[c];
+    partial[(int64_t)blockIdx.y * n_pad + r] = (a0 + a1) + (a2 + a3);
 }
 
 __global__ __launch_bounds__(256) void symv_sum_kernel(const double* __restrict__ partial, int n_pad, int chunks,
