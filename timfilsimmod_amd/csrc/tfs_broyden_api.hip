@@ -35,9 +35,17 @@ __global__ __launch_bounds__(128) void block_gemv_kernel(const double* __restric
     const double* Ab = A + mat_off[b];
     const int r0 = row_off[b];
     for (int r = threadIdx.x; r < bs; r += blockDim.x) {
-        double s = 0.0;
-        for (int c = 0; c < bs; ++c) s += Ab[(int64_t)c * bs + r] * x[r0 + c];
-        y[r0 + r] = alpha * s;
+        // four independent chains keep more loads in flight
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int c = 0;
+        for (; c + 4 <= bs; c += 4) {
+            s0 += Ab[(int64_t)c * bs + r] * x[r0 + c];
+            s1 += Ab[(int64_t)(c + 1) * bs + r] * x[r0 + c + 1];
+            s2 += Ab[(int64_t)(c + 2) * bs + r] * x[r0 + c + 2];
+            s3 += Ab[(int64_t)(c + 3) * bs + r] * x[r0 + c + 3];
+        }
+        for (; c < bs; ++c) s0 += Ab[(int64_t)c * bs + r] * x[r0 + c];
+        y[r0 + r] = alpha * ((s0 + s1) + (s2 + s3));
     }
 }
 
