@@ -541,32 +541,43 @@ __global__ __launch_bounds__(64) void jacobian_kernel(FilState S, FilParams P, i
     }
 }
 
-// filament::update for every filament, then the robot arm.
+// filament::update (filament.cpp:1896-1930), one thread per segment: the
+// segment's Lie-algebra element and orientation, and its three multiplier
+// entries (segment 0 also the clamp's); the robot arm follows in
+// robot_arm_kernel.
 __global__ void update_kernel(FilState S, FilParams P, const double* __restrict__ du) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= P.nfil) return;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)P.nfil * P.nseg) return;
     const int N = P.nseg, n6 = 6 * N;
+    const int f = (int)(g / N), i = (int)(g % N);
     const int swim = f / P.fil_per_swim, fs = f % P.fil_per_swim;
     const double* d = du + (int64_t)swim * ((int64_t)P.fil_per_swim * n6 + 6) + (int64_t)fs * n6;
     const int nm = 3 * (N + 1);
     double* m = S.mult + (int64_t)f * nm;
-    for (int i = 0; i < nm; ++i) m[i] += d[i];   // tether, lambda..., clamp
-    const int64_t s0 = (int64_t)f * N;
-    if (S.bx) {
-        // swimmer::update: the anchor and base rotation follow the body
-        // updated just before (accept_state_from_rigid_body)
-        const int sw = f / P.fil_per_swim;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) S.u[3 * s0 + k] = S.bu[3 * sw + k];
-        store_q(S.q + 4 * s0, qmul(lie_exp(S.u + 3 * s0), load_q(S.qm1 + 4 * s0)));
-        body_anchor(S, P, f, S.x + 3 * s0);
-    }
-    for (int i = 1; i < N; ++i) {
-        const int64_t g = s0 + i;
+    for (int k = 0; k < 3; ++k) m[3 * i + k] += d[3 * i + k];   // tether (i = 0) or lambda_{i-1}
+    if (i == 0) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) S.u[3 * g + k] += d[3 * (N + i) + k];
-        store_q(S.q + 4 * g, qmul(lie_exp(S.u + 3 * g), load_q(S.qm1 + 4 * g)));
+        for (int k = 0; k < 3; ++k) m[3 * N + k] += d[3 * N + k];   // clamp
+        if (S.bx) {
+            // swimmer::update: the anchor and base rotation follow the body
+            // updated just before (accept_state_from_rigid_body)
+            const int sw = f / P.fil_per_swim;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) S.u[3 * g + k] = S.bu[3 * sw + k];
+            store_q(S.q + 4 * g, qmul(lie_exp(S.u + 3 * g), load_q(S.qm1 + 4 * g)));
+            body_anchor(S, P, f, S.x + 3 * g);
+        }
+        return;
     }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) S.u[3 * g + k] += d[3 * (N + i) + k];
+    store_q(S.q + 4 * g, qmul(lie_exp(S.u + 3 * g), load_q(S.qm1 + 4 * g)));
+}
+
+__global__ void robot_arm_kernel(FilState S, FilParams P) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= P.nfil) return;
     robot_arm(S, P, f);
 }
 
@@ -1117,7 +1128,8 @@ int tfs_filament_update(tfs_filament* h, const double* update) {
         e = hipGetLastError();
     }
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(update_kernel, dim3(grid_for(h->P.nfil, 64)), dim3(64), 0, h->stream, h->S, h->P, du);
+        hipLaunchKernelGGL(update_kernel, dim3(grid_for(num_segs(h), 256)), dim3(256), 0, h->stream, h->S, h->P, du);
+        hipLaunchKernelGGL(robot_arm_kernel, dim3(grid_for(h->P.nfil, 64)), dim3(64), 0, h->stream, h->S, h->P);
         e = hipGetLastError();
     }
     if (e == hipSuccess && host) e = hipStreamSynchronize(h->stream);
