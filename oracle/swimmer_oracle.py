@@ -29,10 +29,13 @@ extrapolation on closed forms; the free blob solve's force and torque balance
 and rigid blob motion; the body mobility of a sphere of blobs against Stokes
 drag; convergence of the free swimmer, with fewer iterations than an identity
 body block; the Broyden-for-everything loop reaching the trajectory of the
-form that solves for the blob forces.  The individual entries of A, B and S0
-in body_coupling (swimmer.cpp:688-880) and the blob forces' extrapolation in
-blob_force_guess are restated without a check of their own: parity unpinned
-beyond the structural and convergence checks above.
+form that solves for the blob forces.  tests/test_oracle_body_coupling.py pins
+body_coupling entry by entry: A, B and S0 against central differences of the
+swimmer residual they linearise (self-mobility segments, the body's 6 x 6
+mobility, the sphere term from the pinned pair tensor), Euler and BDF2, with
+and without the sphere, the reference's own simplifications checked as their
+exact closed-form differences; and blob_force_guess as exact on linear and
+quadratic force histories.
 """
 from __future__ import annotations
 
