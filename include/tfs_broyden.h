@@ -73,7 +73,10 @@ int tfs_broyden_start(tfs_broyden *b, const double *error);
 int tfs_broyden_find_update(tfs_broyden *b, double *update);
 /* new_error == NULL takes the residual that tfs_broyden_compute_errors left in slot 1. */
 int tfs_broyden_end_of_iter(tfs_broyden *b, const double *new_error, int32_t error_is_too_large);
-/* Current iteration count (broyden_solver::iter) and ||error||. */
+/* Current iteration count (broyden_solver::iter) and ||error||, the engine's
+ * error (slot 0): like broyden_solver::error it is not replaced when
+ * end_of_iter finds the new error small enough or the memory full (that
+ * residual stays in slot 1). */
 int tfs_broyden_status(tfs_broyden *b, int32_t *iter, double *error_norm);
 
 /* State of the implicit step for mobility_solver::compute_errors

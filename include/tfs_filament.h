@@ -154,9 +154,11 @@ int tfs_filament_read(tfs_filament *h, int32_t which, double *out);
  * The three handles are the caller's, created with TFS_MEMORY_DEVICE and sizes
  * that agree (tfs_velocities: num_swim bodies of fil_per_swim*num_seg segments;
  * tfs_broyden: the block sizes of the Broyden layout above).  Only the
- * too-large flag of each residual crosses to the host.  tfs_elastic_create
- * sets the velocities handle's blob motion (tfs_velocities_set_blob_motion):
- * TFS_BLOBS_FIXED for bodies held still, TFS_BLOBS_MOVING for free bodies. */
+ * too-large flag of each residual crosses to the host.  During each
+ * tfs_elastic_step the velocities handle's blob motion
+ * (tfs_velocities_set_blob_motion) is TFS_BLOBS_MOVING for free bodies and,
+ * for bodies held still, TFS_BLOBS_CHECK at the step's first evaluation and
+ * TFS_BLOBS_FIXED after it; the handle's own mode is restored on return. */
 struct tfs_velocities;
 struct tfs_broyden;
 typedef struct tfs_elastic_desc {
@@ -180,7 +182,7 @@ typedef struct tfs_elastic_stats {
     int32_t gmres_iterations;  /* summed over the blob solves of the step             */
     int32_t converged;         /* 1 if the residual met TOL                            */
     int32_t velocity_solves;
-    double error_norm;         /* ||error|| at the end                                 */
+    double error_norm;         /* ||residual|| of the last evaluation                  */
 } tfs_elastic_stats;
 
 typedef struct tfs_elastic tfs_elastic;

@@ -51,6 +51,12 @@ int tfs_flow_set_sources(tfs_flow *h, const double *x_segs, const double *f_segs
 /* Velocities at num_points tracer points (velocities()).  Asynchronous for
  * device memory; with host memory it returns when v_points is written. */
 int tfs_flow_evaluate(tfs_flow *h, int64_t num_points, const double *x_points, double *v_points);
+/* The same with float arrays (flow_field_evaluator of a Real = float build,
+ * flow_field_evaluator.hpp:8-63): widened to FP64 on the device, velocities
+ * rounded once to float. */
+int tfs_flow_set_sources_f32(tfs_flow *h, const float *x_segs, const float *f_segs,
+                             const float *x_blobs, const float *f_blobs, const float *x_bodies);
+int tfs_flow_evaluate_f32(tfs_flow *h, int64_t num_points, const float *x_points, float *v_points);
 int tfs_flow_sync(tfs_flow *h);
 /* Threads per workgroup (64, 128 or 256) of the evaluation kernel. */
 int tfs_flow_set_block(tfs_flow *h, int32_t block);

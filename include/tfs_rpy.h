@@ -218,6 +218,25 @@ typedef struct tfs_barrier_desc {
 
 int tfs_rpy_barrier_forces(tfs_rpy *h, const tfs_barrier_desc *desc, double *f_segs, double *f_blobs_repulsion);
 
+/* ---- FP32 host arrays ------------------------------------------------------
+ * The reference's host arrays are of its build's Real, float as shipped
+ * (config.hpp:36,52; mobility_solver.hpp:57-63).  These take float arrays
+ * (host or device memory, as the handle's desc says) with the same layouts
+ * and meaning as the double entry points above: floats are widened exactly to
+ * FP64 on the device, every evaluation stays FP64 (or FP32 tiles with FP64
+ * sums, per the handle's precision), and results are rounded once to float.
+ * Asynchronous as their double twins. */
+/* copy_*_to_device of a Real = float build (rpy_mobility_solver.cu:119-161). */
+int tfs_rpy_upload_f32(tfs_rpy *h, int32_t what, const float *src);
+/* copy_*_velocities_to_host of a Real = float build (rpy_mobility_solver.cu:177-205). */
+int tfs_rpy_download_f32(tfs_rpy *h, int32_t what, float *dst);
+/* tfs_rpy_apply with float arrays. */
+int tfs_rpy_apply_f32(tfs_rpy *h, uint32_t blocks, const float *f_segs, const float *f_blobs,
+                      float *v_segs, float *v_blobs);
+/* apply_interparticle_forces of a Real = float build (rpy_mobility_solver.cu:207-232). */
+int tfs_rpy_barrier_forces_f32(tfs_rpy *h, const tfs_barrier_desc *desc, float *f_segs,
+                               float *f_blobs_repulsion);
+
 const char *tfs_last_error(void);
 
 /* Library version string. */

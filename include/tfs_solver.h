@@ -124,6 +124,22 @@ int tfs_solver_solve(tfs_solver *s, const double *rhs, double *x, int32_t memory
 int tfs_solver_residual(tfs_solver *s, const double *rhs, const double *x, int32_t precision,
                         int32_t memory, double *relres);
 
+/* Float twins of the array entry points above, for a Real = float build
+ * whose matrix class holds floats (mobility_solver.cpp works on Real): the
+ * arrays are widened exactly to FP64 on the device, the solve runs as its
+ * double twin (FP64 Arnoldi, residuals and solution), and outputs are rounded
+ * once to float.  Complete on return. */
+int tfs_solver_set_geometry_f32(tfs_solver *s, const float *x_segs, const float *x_blobs,
+                                const float *vel_dir_phase, const float *vel_dir_angle, int32_t memory);
+int tfs_solver_set_bodies_f32(tfs_solver *s, const float *x_bodies, int32_t memory);
+int tfs_solver_system_mult_f32(tfs_solver *s, const float *in, float *out, int32_t precision,
+                               int32_t memory);
+int tfs_solver_precondition_f32(tfs_solver *s, const float *in, float *out, int32_t memory);
+int tfs_solver_solve_f32(tfs_solver *s, const float *rhs, float *x, int32_t memory,
+                         tfs_solve_stats *stats);
+int tfs_solver_residual_f32(tfs_solver *s, const float *rhs, const float *x, int32_t precision,
+                            int32_t memory, double *relres);
+
 /* The RPY matvec handle the solver evaluates with (for tfs_rpy_profile,
  * tfs_rpy_set_tile_shape, ...).  Owned by the solver. */
 int tfs_solver_rpy(tfs_solver *s, tfs_rpy **out);
@@ -267,6 +283,17 @@ int tfs_velocities_compute_free(tfs_velocities *v, const double *x_segs, const d
                                 const double *x_blobs, const double *x_bodies, const double *f_bodies,
                                 const double *q_bodies, double *v_segs, double *f_blobs, double *v_bodies,
                                 tfs_solve_stats *stats);
+/* Float twins of tfs_velocities_compute / tfs_velocities_compute_free
+ * (compute_velocities of a Real = float build): inputs widened exactly to
+ * FP64 on the device, the FP64 computation, outputs (and the warm starts they
+ * overwrite) rounded once to float.  Complete on return. */
+int tfs_velocities_compute_f32(tfs_velocities *v, const float *x_segs, const float *f_segs,
+                               const float *x_blobs, const float *x_bodies, const float *v_bodies,
+                               float *v_segs, float *f_blobs, tfs_solve_stats *stats);
+int tfs_velocities_compute_free_f32(tfs_velocities *v, const float *x_segs, const float *f_segs,
+                                    const float *x_blobs, const float *x_bodies, const float *f_bodies,
+                                    const float *q_bodies, float *v_segs, float *f_blobs, float *v_bodies,
+                                    tfs_solve_stats *stats);
 /* The body-frame blob positions (3 per blob, relative to the centre, at
  * orientation 1) for the direct free-body solve. */
 int tfs_velocities_set_body_frame(tfs_velocities *v, const double *blob_refs);

@@ -92,6 +92,10 @@ SIGNATURES = {
     "tfs_rpy_set_bodies": (ctypes.c_int, [_vp, _i32, _vp, _vp]),
     "tfs_host_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "tfs_host_free": (ctypes.c_int, [_vp]),
+    "tfs_rpy_upload_f32": (ctypes.c_int, [_vp, _i32, _vp]),
+    "tfs_rpy_download_f32": (ctypes.c_int, [_vp, _i32, _vp]),
+    "tfs_rpy_apply_f32": (ctypes.c_int, [_vp, _u32, _vp, _vp, _vp, _vp]),
+    "tfs_rpy_barrier_forces_f32": (ctypes.c_int, [_vp, ctypes.POINTER(BarrierDesc), _vp, _vp]),
     "tfs_last_error": (ctypes.c_char_p, []),
     "tfs_version": (ctypes.c_char_p, []),
 }

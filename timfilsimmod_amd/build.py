@@ -29,6 +29,7 @@ SOURCES = [
     "tfs_broyden_api.hip",
     "tfs_velocities_api.hip",
     "tfs_filament_api.hip",
+    "tfs_real32.hip",
     "tfs_io.cpp",
 ]
 

@@ -232,6 +232,7 @@ __global__ __launch_bounds__(BLOCK) void rpy_pairs_kernel(const MatvecItem* __re
     if (a.skip && *a.skip) return;
     const MatvecItem it = items[blockIdx.x];
     const MatvecTile tl = tiles[it.tile];
+    if (a.body_skip && a.body_skip[tl.body]) return;
     T* slot = partial + (size_t)blockIdx.x * (kMaxDof * BLOCK * R);
     if (tl.cls == 0) {
         if (it.src_cls == 0) run_item<SEGDOF, SEGDOF, T, BLOCK, R>(it, tl, a, slot, lds);
@@ -309,6 +310,7 @@ __global__ __launch_bounds__(BLOCK) void rpy_pairs_persistent_kernel(const Matve
         if (ps.item_stamps) ps.item_stamps[2 * item] = (int64_t)wall_clock64();
         const MatvecItem it = items[item];
         const MatvecTile tl = tiles[it.tile];
+        if (a.body_skip && a.body_skip[tl.body]) continue;  // uniform over the workgroup
         T* slot = partial + (size_t)item * (kMaxDof * BLOCK * R);
         if (tl.cls == 0) {
             if (it.src_cls == 0) run_item<SEGDOF, SEGDOF, T, BLOCK, R>(it, tl, a, slot, lds);
@@ -633,11 +635,13 @@ __global__ __launch_bounds__(256) void rpy_reduce_kernel(const MatvecTile* __res
     const int grp = threadIdx.x >> 6;
     if ((int)blockIdx.x >= tile_blocks) {
         const int m = ((int)blockIdx.x - tile_blocks) * 4 + grp;
-        if (m < o.sad.nfil) saddle_filament_rows(o.sad, m, threadIdx.x & 63);
+        if (m < o.sad.nfil && !(a.body_skip && o.sad.fil_body && a.body_skip[o.sad.fil_body[m]]))
+            saddle_filament_rows(o.sad, m, threadIdx.x & 63);
         return;
     }
     const int per_tile = (tile_targets + 63) / 64;
     const MatvecTile tl = tiles[blockIdx.x / per_tile];
+    if (a.body_skip && a.body_skip[tl.body]) return;  // the whole workgroup has this tile
     const int l = (blockIdx.x % per_tile) * 64 + (threadIdx.x & 63);
     const int t = tl.t_begin + l;
     const bool active = l < tile_targets && t < tl.t_end;

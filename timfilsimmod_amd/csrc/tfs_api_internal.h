@@ -10,6 +10,20 @@
 
 namespace tfs {
 
+// Device staging of FP32 caller arrays (the reference's Real = float build):
+// floats cross PCIe and are widened to FP64 on the device, or narrowed from
+// it, so every computation behind the boundary stays FP64.
+struct F32Stage {
+    float* p = nullptr;
+    size_t cap = 0;
+};
+void free_stage(F32Stage& st);
+// dst[0:n] (device, FP64) = src[0:n] (FP32 in host or device memory), exact.
+int widen_in(F32Stage& st, const float* src, double* dst, int64_t n, bool host, hipStream_t s, const char* where);
+// dst[0:n] (FP32 in host or device memory) = src[0:n] (device, FP64) rounded
+// to nearest.  Asynchronous on s like the FP64 copies.
+int narrow_out(F32Stage& st, const double* src, float* dst, int64_t n, bool host, hipStream_t s, const char* where);
+
 // A cached work list for one block mask and launch shape.
 struct Plan {
     std::vector<MatvecTile> tiles;
@@ -67,6 +81,9 @@ struct tfs_rpy {
     // Stop flag passed to the kernels of later evaluations (set by a solver
     // around its speculative Krylov iterations; null otherwise).
     const int32_t* skip_flag = nullptr;
+    // Per-body stop flags passed to the kernels (MatvecArgs::body_skip), set
+    // by a solver around the lock-step iterations of independent bodies.
+    const int32_t* body_skip = nullptr;
     // Live kernel timing: event pairs recorded around each pair-kernel launch.
     bool profiling = false;
     struct EvPair {
@@ -75,13 +92,26 @@ struct tfs_rpy {
     };
     std::vector<EvPair> ev_used, ev_free;
     std::map<std::tuple<uint32_t, int, int, int>, tfs::Plan> plans;
+    tfs::F32Stage f32;  // FP32 entry points (tfs_rpy_*_f32)
 };
 
+struct tfs_velocities;
+struct tfs_broyden;
+
 namespace tfs {
+
+// The blob-motion mode of a velocities handle (tfs_velocities_set_blob_motion).
+int32_t velocities_blob_motion(const tfs_velocities* v);
+// Norm of the latest residual given to a Broyden engine: the new_error of its
+// last end_of_iter (slot 0 after an update, slot 1 otherwise), else the
+// error it started from.  Synchronises.
+int broyden_latest_norm(tfs_broyden* b, double* out);
 
 extern thread_local std::string g_last_error;
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* where);
+// TFS_OK for hipSuccess, else hip_fail(e, where).
+int hip_fail_if(hipError_t e, const char* where);
 TileShape default_shape(int precision, int ns, int nb, int num_cus);
 Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err);
 // Positions changed: the near-pair lists must be rebuilt before the next matvec.

@@ -322,6 +322,7 @@ struct tfs_broyden {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int n = 0, max_iter = 0, iter = 0, nblocks = 0, G = 1, RB = 1;
+    int latest = 0;  // slot holding the latest residual handed to the engine
     int64_t ld = 0, jac_len = 0;
     double* arena = nullptr;
     double *C = nullptr, *D = nullptr, *slot[2] = {nullptr, nullptr}, *t = nullptr, *h = nullptr,
@@ -616,6 +617,7 @@ int tfs_broyden_set_body_coupling(tfs_broyden* b, int32_t num_swim, int32_t fil_
 int tfs_broyden_start(tfs_broyden* b, const double* error) {
     if (!b) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_start: null handle");
     b->iter = 0;
+    b->latest = 0;
     if (!error) return TFS_OK;
     return copy_in(b, b->slot[0], error, b->n, "tfs_broyden_start");
 }
@@ -653,6 +655,15 @@ int tfs_broyden_end_of_iter(tfs_broyden* b, const double* new_error, int32_t err
         if (!rc) rc = check(k_scale(Dk, Dk, b->scal, 1, n, b->stream), "D_k /= ||D_k||");
         if (rc) return rc;
         std::swap(b->slot[0], b->slot[1]);   // error <- new_error
+        b->latest = 0;
+    } else {
+        // no update: the engine's error stays in slot 0 (broyden_solver.cpp:69-129),
+        // the new residual is kept in slot 1
+        if (new_error) {
+            int rc = copy_in(b, b->slot[1], new_error, n, "tfs_broyden_end_of_iter");
+            if (rc) return rc;
+        }
+        b->latest = 1;
     }
     b->iter++;
     return TFS_OK;
@@ -752,6 +763,20 @@ int tfs_broyden_compute_errors(tfs_broyden* b, const tfs_broyden_state* st, int3
     if (b->host_scal[1] != 0.0) return fail(TFS_ERR_NON_FINITE, "tfs_broyden_compute_errors: residual not finite");
     return TFS_OK;
 }
+
+}  // extern "C"
+
+int tfs::broyden_latest_norm(tfs_broyden* b, double* out) {
+    int rc = check(k_norm(b->slot[b->latest], b->n, b->partial, b->G, b->scal, b->stream), "||residual||");
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(b->host_scal, b->scal, 8, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return hip_fail(e, "broyden_latest_norm");
+    *out = *b->host_scal;
+    return TFS_OK;
+}
+
+extern "C" {
 
 int tfs_broyden_read_error(tfs_broyden* b, int32_t slot, double* out) {
     if (!b || !out || (slot != 0 && slot != 1))

@@ -1356,10 +1356,6 @@ int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
     hipError_t err = hipMemcpy(e->blocks, host.data(), 8 * nb, hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemset(e->zeros, 0, 48 * nswim);
     if (err == hipSuccess) err = hipMemset(e->v_bodies, 0, 48 * nswim);
-    // the step knows how its blobs move: held still, or with a free body
-    if (err == hipSuccess && desc->velocities &&
-        tfs_velocities_set_blob_motion(desc->velocities, free_bodies ? TFS_BLOBS_MOVING : TFS_BLOBS_FIXED) != TFS_OK)
-        err = hipErrorInvalidValue;
     if (err == hipSuccess && free_bodies && fh->nblob &&
         tfs_velocities_set_body_frame(desc->velocities, fh->blob_refs) != TFS_OK) {
         const std::string msg = g_last_error;
@@ -1459,6 +1455,24 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
     }
     const int64_t nswim = fh->desc.num_swim, fil = fh->desc.fil_per_swim, n6 = 6LL * fh->P.nseg;
     const int64_t nb3 = 3 * e->nb_swim;
+    // How the velocities handle learns whether the blobs moved, for this step
+    // only: a free body's blobs move at every evaluation; a body held still
+    // has its blobs compared once, at the step's first evaluation (the caller
+    // may have moved them since the last step), and taken as fixed after
+    // that.  The handle's own mode is restored on return.
+    struct MotionScope {
+        tfs_velocities* v = nullptr;
+        int32_t saved = TFS_BLOBS_CHECK;
+        ~MotionScope() {
+            if (v) (void)tfs_velocities_set_blob_motion(v, saved);
+        }
+    } motion;
+    if (D.velocities) {
+        motion.saved = tfs::velocities_blob_motion(D.velocities);
+        const int rc0 = tfs_velocities_set_blob_motion(D.velocities, e->free_bodies ? TFS_BLOBS_MOVING : TFS_BLOBS_CHECK);
+        if (rc0) return rc0;
+        motion.v = D.velocities;
+    }
     // forces -> velocities -> residual in `slot`
     auto evaluate = [&](int slot, int32_t* too_large) -> int {
         int rc = e->free_bodies ? tfs_filament_blob_positions(fh, e->x_blobs) : TFS_OK;
@@ -1490,6 +1504,7 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
     }
     int32_t big = 0;
     if (!rc) rc = evaluate(0, &big);
+    if (!rc && D.velocities && !e->free_bodies) rc = tfs_velocities_set_blob_motion(D.velocities, TFS_BLOBS_FIXED);
     if (!rc && big) {
         rc = tfs_filament_jacobian(fh, nt, e->f_segs, e->blocks);
         if (!rc) rc = tfs_filament_sync(fh);
@@ -1538,7 +1553,7 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
         if (!rc) rc = tfs_broyden_status(D.broyden, &iter, nullptr);
     }
     double norm = 0.0;
-    if (!rc) rc = tfs_broyden_status(D.broyden, nullptr, &norm);
+    if (!rc) rc = tfs::broyden_latest_norm(D.broyden, &norm);
     S.broyden_iterations = iter;
     S.error_norm = norm;
     S.converged = (!rc && !big) ? 1 : 0;

@@ -25,6 +25,7 @@ struct tfs_flow {
     double* pts = nullptr;     // staging for host callers: points | velocities
     int64_t pts_cap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    tfs::F32Stage f32;         // FP32 entry points (tfs_flow_*_f32)
 };
 
 namespace {
@@ -89,6 +90,7 @@ int tfs_flow_destroy(tfs_flow* h) {
     if (h->stream) hipStreamSynchronize(h->stream);
     if (h->src) hipFree(h->src);
     if (h->pts) hipFree(h->pts);
+    free_stage(h->f32);
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
     if (h->own_stream) hipStreamDestroy(h->stream);
@@ -114,30 +116,37 @@ int tfs_flow_set_sources(tfs_flow* h, const double* x_segs, const double* f_segs
     return check(e, "tfs_flow_set_sources");
 }
 
-int tfs_flow_evaluate(tfs_flow* h, int64_t num_points, const double* x_points, double* v_points) {
-    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_flow_evaluate: null handle");
-    if (num_points < 0 || (num_points > 0 && (!x_points || !v_points)))
-        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_flow_evaluate: bad points");
-    if (num_points == 0) return TFS_OK;
-    if (num_points > ((int64_t)1 << 31) / 4)
-        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_flow_evaluate: too many points for one call");
-    const tfs_flow_desc& D = h->desc;
-    const double* xp = x_points;
-    double* vp = v_points;
-    hipError_t e = hipSuccess;
-    if (D.memory == TFS_MEMORY_HOST) {
-        if (num_points > h->pts_cap) {
-            if (h->pts) hipFree(h->pts);
-            h->pts = nullptr;
-            h->pts_cap = 0;
-            e = hipMalloc(&h->pts, 48 * num_points);
-            if (e != hipSuccess) return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_flow_evaluate: staging");
-            h->pts_cap = num_points;
-        }
-        e = hipMemcpyAsync(h->pts, x_points, 24 * num_points, hipMemcpyHostToDevice, h->stream);
-        xp = h->pts;
-        vp = h->pts + 3 * num_points;
+}  // extern "C"
+
+namespace {
+
+// Staging for host callers' points and velocities (grown to the largest request).
+int ensure_points(tfs_flow* h, int64_t num_points) {
+    if (num_points <= h->pts_cap) return TFS_OK;
+    if (h->pts) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipFree(h->pts);
     }
+    h->pts = nullptr;
+    h->pts_cap = 0;
+    if (hipMalloc(&h->pts, 48 * num_points) != hipSuccess)
+        return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_flow_evaluate: staging");
+    h->pts_cap = num_points;
+    return TFS_OK;
+}
+
+int check_points(tfs_flow* h, int64_t num_points, bool have_arrays, const char* where) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, std::string(where) + ": null handle");
+    if (num_points < 0 || (num_points > 0 && !have_arrays))
+        return fail(TFS_ERR_INVALID_ARGUMENT, std::string(where) + ": bad points");
+    if (num_points > ((int64_t)1 << 31) / 4)
+        return fail(TFS_ERR_INVALID_ARGUMENT, std::string(where) + ": too many points for one call");
+    return TFS_OK;
+}
+
+// The velocities() kernels on device points xp, velocities vp.
+hipError_t run_flow(tfs_flow* h, int64_t num_points, const double* xp, double* vp) {
+    const tfs_flow_desc& D = h->desc;
     FlowArgs a;
     a.xt = {xp, 3, 1};
     a.vt = {vp, 3, 1};
@@ -157,15 +166,72 @@ int tfs_flow_evaluate(tfs_flow* h, int64_t num_points, const double* x_points, d
     a.a_seg = D.seg_radius;
     a.a_blob = D.blob_radius;
     a.mu = D.mu;
-    if (e == hipSuccess && a.remove_net_force) e = launch_flow_net(a, h->net, D.num_bodies, h->stream);
+    hipError_t e = hipSuccess;
+    if (a.remove_net_force) e = launch_flow_net(a, h->net, D.num_bodies, h->stream);
     if (e == hipSuccess) e = hipEventRecord(h->ev0, h->stream);
     if (e == hipSuccess) e = launch_flow_eval(D.precision, a, h->net, h->block, h->stream);
     if (e == hipSuccess) e = hipEventRecord(h->ev1, h->stream);
-    if (e == hipSuccess && D.memory == TFS_MEMORY_HOST) {
+    return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tfs_flow_evaluate(tfs_flow* h, int64_t num_points, const double* x_points, double* v_points) {
+    int rc = check_points(h, num_points, x_points && v_points, "tfs_flow_evaluate");
+    if (rc || num_points == 0) return rc;
+    const double* xp = x_points;
+    double* vp = v_points;
+    hipError_t e = hipSuccess;
+    const bool host = h->desc.memory == TFS_MEMORY_HOST;
+    if (host) {
+        rc = ensure_points(h, num_points);
+        if (rc) return rc;
+        e = hipMemcpyAsync(h->pts, x_points, 24 * num_points, hipMemcpyHostToDevice, h->stream);
+        xp = h->pts;
+        vp = h->pts + 3 * num_points;
+    }
+    if (e == hipSuccess) e = run_flow(h, num_points, xp, vp);
+    if (e == hipSuccess && host) {
         e = hipMemcpyAsync(v_points, vp, 24 * num_points, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     }
     return check(e, "tfs_flow_evaluate");
+}
+
+int tfs_flow_set_sources_f32(tfs_flow* h, const float* x_segs, const float* f_segs, const float* x_blobs,
+                             const float* f_blobs, const float* x_bodies) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_flow_set_sources_f32: null handle");
+    const tfs_flow_desc& D = h->desc;
+    const int64_t ns = (int64_t)D.num_bodies * D.segs_per_body, nb = (int64_t)D.num_bodies * D.blobs_per_body;
+    if ((ns && (!x_segs || !f_segs)) || (nb && (!x_blobs || !f_blobs)) || (D.remove_net_force && !x_bodies))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_flow_set_sources_f32: missing array");
+    const bool host = D.memory == TFS_MEMORY_HOST;
+    const char* w = "tfs_flow_set_sources_f32";
+    int rc = TFS_OK;
+    if (ns) rc = widen_in(h->f32, x_segs, h->x_segs, 3 * ns, host, h->stream, w);
+    if (!rc && ns) rc = widen_in(h->f32, f_segs, h->f_segs, 6 * ns, host, h->stream, w);
+    if (!rc && nb) rc = widen_in(h->f32, x_blobs, h->x_blobs, 3 * nb, host, h->stream, w);
+    if (!rc && nb) rc = widen_in(h->f32, f_blobs, h->f_blobs, 3 * nb, host, h->stream, w);
+    if (!rc && x_bodies) rc = widen_in(h->f32, x_bodies, h->x_bodies, 3LL * D.num_bodies, host, h->stream, w);
+    if (!rc && host) rc = check(hipStreamSynchronize(h->stream), w);
+    return rc;
+}
+
+int tfs_flow_evaluate_f32(tfs_flow* h, int64_t num_points, const float* x_points, float* v_points) {
+    int rc = check_points(h, num_points, x_points && v_points, "tfs_flow_evaluate_f32");
+    if (rc || num_points == 0) return rc;
+    const bool host = h->desc.memory == TFS_MEMORY_HOST;
+    rc = ensure_points(h, num_points);
+    if (!rc) rc = widen_in(h->f32, x_points, h->pts, 3 * num_points, host, h->stream, "tfs_flow_evaluate_f32");
+    if (rc) return rc;
+    hipError_t e = run_flow(h, num_points, h->pts, h->pts + 3 * num_points);
+    if (e != hipSuccess) return check(e, "tfs_flow_evaluate_f32");
+    rc = narrow_out(h->f32, h->pts + 3 * num_points, v_points, 3 * num_points, host, h->stream,
+                    "tfs_flow_evaluate_f32");
+    if (!rc && host) rc = check(hipStreamSynchronize(h->stream), "tfs_flow_evaluate_f32");
+    return rc;
 }
 
 int tfs_flow_sync(tfs_flow* h) {

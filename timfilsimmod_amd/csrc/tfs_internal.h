@@ -114,6 +114,10 @@ struct MatvecArgs {
     // then blobs (w unused), with lambda = 2^k and |X| < 2^30.
     const int4* xfx = nullptr;
     double lam = 1.0;
+    // Per-body stop flags (may be null): items and tiles of a body whose flag
+    // is nonzero do nothing (independent bodies solved in lock-step, whose
+    // converged members drop out).
+    const int32_t* body_skip = nullptr;
 };
 
 // Saddle-point couplings of the prescribed-cilia operator
@@ -135,6 +139,9 @@ struct SaddleEpilogue {
     const double* x_bodies = nullptr;
     int free_body = 0, segs_per_swim = 0, blobs_per_swim = 0;
     int64_t off_body = 0;
+    // independent bodies: the body of each filament, for the per-body stop
+    // flags of MatvecArgs::body_skip
+    const int32_t* fil_body = nullptr;
 };
 
 struct OutArgs {

@@ -32,6 +32,8 @@ int hip_fail(hipError_t e, const char* where) {
     return fail(TFS_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
 }
 
+int hip_fail_if(hipError_t e, const char* where) { return e == hipSuccess ? TFS_OK : hip_fail(e, where); }
+
 TileShape default_shape(int precision, int ns, int nb, int num_cus) {
     TileShape s;
     s.block = 256;
@@ -200,6 +202,36 @@ int ensure_body_table(tfs_rpy* h) {
     return e == hipSuccess ? TFS_OK : hip_fail(e, "body table");
 }
 
+// The fixed-point frame of the bodies' bounding boxes (6 per body: min xyz,
+// max xyz): each body's centre and one lambda = 2^k for all of them.
+static int fixed_point_frame(const double* box, size_t nbody, double amin, double* centre, double* lam) {
+    double ext = 0.0;
+    for (size_t b = 0; b < nbody; ++b) {
+        for (int k = 0; k < 3; ++k) {
+            const double lo = box[6 * b + k], hi = box[6 * b + 3 + k];
+            if (std::isnan(lo) || std::isnan(hi)) return fail(TFS_ERR_NON_FINITE, "positions are not finite");
+            centre[3 * b + k] = 0.0;
+            if (!(lo <= hi)) continue;  // body without particles
+            const double c = 0.5 * (lo + hi);
+            centre[3 * b + k] = c;
+            ext = std::max(ext, std::max(hi - c, c - lo));
+        }
+    }
+    const double lim = 1073741822.0;  // 2^30 - 2
+    int k = 60;
+    if (ext > 0.0) {
+        k = (int)std::floor(std::log2(lim / ext));
+        while (k > -1000 && std::ldexp(ext, k) > lim) --k;
+        while (k < 60 && std::ldexp(ext, k + 1) <= lim) ++k;
+        k = std::min(k, 60);
+    }
+    if (std::ldexp(1.0, -k) > std::ldexp(amin, -19))
+        return fail(TFS_ERR_INVALID_ARGUMENT,
+                    "FP32 tiles: a body extends more than about 1024 particle radii from its centre (use FP64)");
+    *lam = std::ldexp(1.0, k);
+    return TFS_OK;
+}
+
 int ensure_fixed(tfs_rpy* h) {
     if (h->fx_valid) return TFS_OK;
     const int ns = h->desc.num_segs, nb = h->desc.num_blobs;
@@ -219,30 +251,9 @@ int ensure_fixed(tfs_rpy* h) {
         e = hipMemcpyAsync(box.data(), h->d_body_box, sizeof(double) * 6 * nbody, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return hip_fail(e, "fixed-point positions: bounding boxes");
-    double ext = 0.0;
-    for (size_t b = 0; b < nbody; ++b) {
-        for (int k = 0; k < 3; ++k) {
-            const double lo = box[6 * b + k], hi = box[6 * b + 3 + k];
-            if (std::isnan(lo) || std::isnan(hi)) return fail(TFS_ERR_NON_FINITE, "positions are not finite");
-            if (!(lo <= hi)) continue;  // body without particles
-            const double c = 0.5 * (lo + hi);
-            centre[3 * b + k] = c;
-            ext = std::max(ext, std::max(hi - c, c - lo));
-        }
-    }
-    const double lim = 1073741822.0;  // 2^30 - 2
-    int k = 60;
-    if (ext > 0.0) {
-        k = (int)std::floor(std::log2(lim / ext));
-        while (k > -1000 && std::ldexp(ext, k) > lim) --k;
-        while (k < 60 && std::ldexp(ext, k + 1) <= lim) ++k;
-        k = std::min(k, 60);
-    }
-    const double amin = std::min(h->desc.seg_radius, h->desc.blob_radius);
-    if (std::ldexp(1.0, -k) > std::ldexp(amin, -19))
-        return fail(TFS_ERR_INVALID_ARGUMENT,
-                    "FP32 tiles: a body extends more than about 1024 particle radii from its centre (use FP64)");
-    h->fx_lambda = std::ldexp(1.0, k);
+    int rc2 = fixed_point_frame(box.data(), nbody, std::min(h->desc.seg_radius, h->desc.blob_radius), centre.data(),
+                                &h->fx_lambda);
+    if (rc2) return rc2;
     h->fx_centre = centre;
     e = hipMemcpyAsync(h->d_body_box, h->fx_centre.data(), sizeof(double) * 3 * nbody, hipMemcpyHostToDevice,
                        h->stream);
@@ -276,6 +287,7 @@ MatvecArgs matvec_args(tfs_rpy* h, CView fs, CView fb) {
     a.a_blob = h->desc.blob_radius;
     a.mu = h->desc.mu;
     a.skip = h->skip_flag;
+    a.body_skip = h->body_skip;
     a.xfx = h->x_fx;
     a.lam = h->fx_lambda;
     return a;
@@ -513,6 +525,7 @@ int tfs_rpy_destroy(tfs_rpy* h) {
     if (h->d_body_range) hipFree(h->d_body_range);
     if (h->host_int) hipHostFree(h->host_int);
     if (h->blob_rep) hipFree(h->blob_rep);
+    free_stage(h->f32);
     if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
     delete h;
     return TFS_OK;
@@ -605,21 +618,34 @@ int tfs_rpy_apply(tfs_rpy* h, uint32_t blocks, const double* f_segs, const doubl
     return rc;
 }
 
-int tfs_rpy_barrier_forces(tfs_rpy* h, const tfs_barrier_desc* desc, double* f_segs, double* f_blobs_repulsion) {
-    if (!h || !desc) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_barrier_forces: null handle or descriptor");
-    const int64_t ns = h->desc.num_segs, nb = h->desc.num_blobs;
-    if (ns > 0 && !f_segs) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_barrier_forces: null segment forces");
+}  // extern "C"
+
+// Checks a barrier descriptor against the handle; fills the kernel's parameters.
+static int barrier_params(tfs_rpy* h, const tfs_barrier_desc* desc, bool have_segs, BarrierParams* bp,
+                          const std::string& where) {
+    if (!h || !desc) return fail(TFS_ERR_INVALID_ARGUMENT, where + ": null handle or descriptor");
+    const int64_t ns = h->desc.num_segs;
+    if (ns > 0 && !have_segs) return fail(TFS_ERR_INVALID_ARGUMENT, where + ": null segment forces");
     if (!(desc->strength >= 0) || !std::isfinite(desc->strength))
-        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_barrier_forces: strength must be finite and >= 0");
+        return fail(TFS_ERR_INVALID_ARGUMENT, where + ": strength must be finite and >= 0");
     if (desc->seg_per_fil < 1 || (ns > 0 && ns % desc->seg_per_fil != 0))
-        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_barrier_forces: seg_per_fil must divide the segment count");
+        return fail(TFS_ERR_INVALID_ARGUMENT, where + ": seg_per_fil must divide the segment count");
     if (desc->wall && !(0.5 * desc->dl > h->desc.seg_radius))
-        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_barrier_forces: the wall term needs dl/2 > segment radius");
+        return fail(TFS_ERR_INVALID_ARGUMENT, where + ": the wall term needs dl/2 > segment radius");
+    bp->strength = desc->strength;
+    bp->nseg = desc->seg_per_fil;
+    bp->wall = desc->wall ? 1 : 0;
+    bp->dl = desc->dl;
+    return TFS_OK;
+}
+
+extern "C" {
+
+int tfs_rpy_barrier_forces(tfs_rpy* h, const tfs_barrier_desc* desc, double* f_segs, double* f_blobs_repulsion) {
     BarrierParams bp;
-    bp.strength = desc->strength;
-    bp.nseg = desc->seg_per_fil;
-    bp.wall = desc->wall ? 1 : 0;
-    bp.dl = desc->dl;
+    int rc0 = barrier_params(h, desc, f_segs != nullptr, &bp, "tfs_rpy_barrier_forces");
+    if (rc0) return rc0;
+    const int64_t ns = h->desc.num_segs, nb = h->desc.num_blobs;
     const bool host = h->desc.memory == TFS_MEMORY_HOST;
     double* fs = f_segs;
     double* fb = f_blobs_repulsion;
@@ -923,6 +949,84 @@ int tfs_rpy_device_buffer(tfs_rpy* h, int32_t what, double** out) {
     if (!h || !out) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_device_buffer: null argument");
     size_t len;
     return array_info(h, what, out, &len);
+}
+
+// ---- FP32 arrays (Real = float builds) -------------------------------------
+
+int tfs_rpy_upload_f32(tfs_rpy* h, int32_t what, const float* src) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_upload_f32: null handle");
+    if (what > TFS_BLOB_FORCES && what != TFS_SEG_VELOCITIES && what != TFS_BLOB_VELOCITIES)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_upload_f32: not an input array");
+    double* dev;
+    size_t len;
+    int rc = array_info(h, what, &dev, &len);
+    if (rc) return rc;
+    if (len == 0) return TFS_OK;
+    if (!src) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_upload_f32: null source");
+    if (what == TFS_SEG_POSITIONS || what == TFS_BLOB_POSITIONS) invalidate_near(h);
+    return widen_in(h->f32, src, dev, (int64_t)len, h->desc.memory == TFS_MEMORY_HOST, h->stream,
+                    "tfs_rpy_upload_f32");
+}
+
+int tfs_rpy_download_f32(tfs_rpy* h, int32_t what, float* dst) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_download_f32: null handle");
+    if (what != TFS_SEG_VELOCITIES && what != TFS_BLOB_VELOCITIES)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_download_f32: not an output array");
+    double* dev;
+    size_t len;
+    int rc = array_info(h, what, &dev, &len);
+    if (rc) return rc;
+    if (len == 0) return TFS_OK;
+    if (!dst) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_download_f32: null destination");
+    return narrow_out(h->f32, dev, dst, (int64_t)len, h->desc.memory == TFS_MEMORY_HOST, h->stream,
+                      "tfs_rpy_download_f32");
+}
+
+int tfs_rpy_apply_f32(tfs_rpy* h, uint32_t blocks, const float* f_segs, const float* f_blobs, float* v_segs,
+                      float* v_blobs) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_apply_f32: null handle");
+    if (blocks & ~TFS_BLOCK_ALL) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_apply_f32: bad block mask");
+    const bool need_fs = blocks & (TFS_BLOCK_SS | TFS_BLOCK_BS);
+    const bool need_fb = blocks & (TFS_BLOCK_SB | TFS_BLOCK_BB);
+    const bool out_s = blocks & (TFS_BLOCK_SS | TFS_BLOCK_SB);
+    const bool out_b = blocks & (TFS_BLOCK_BB | TFS_BLOCK_BS);
+    const bool ns = h->desc.num_segs > 0, nb = h->desc.num_blobs > 0;
+    if ((need_fs && ns && !f_segs) || (need_fb && nb && !f_blobs) || (out_s && ns && !v_segs) ||
+        (out_b && nb && !v_blobs))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_apply_f32: missing array for requested block");
+    int rc = TFS_OK;
+    if (need_fs && ns) rc = tfs_rpy_upload_f32(h, TFS_SEG_FORCES, f_segs);
+    if (!rc && need_fb && nb) rc = tfs_rpy_upload_f32(h, TFS_BLOB_FORCES, f_blobs);
+    // Accumulating blocks read the caller's current velocities.
+    const bool acc_s = (blocks & TFS_BLOCK_SB) && !(blocks & TFS_BLOCK_SS);
+    const bool acc_b = (blocks & TFS_BLOCK_BS) && !(blocks & TFS_BLOCK_BB) && h->desc.bs_mode == TFS_BS_ACCUMULATE;
+    if (!rc && acc_s && ns) rc = tfs_rpy_upload_f32(h, TFS_SEG_VELOCITIES, v_segs);
+    if (!rc && acc_b && nb) rc = tfs_rpy_upload_f32(h, TFS_BLOB_VELOCITIES, v_blobs);
+    if (!rc) rc = tfs_rpy_evaluate(h, blocks);
+    if (!rc && out_s && ns) rc = tfs_rpy_download_f32(h, TFS_SEG_VELOCITIES, v_segs);
+    if (!rc && out_b && nb) rc = tfs_rpy_download_f32(h, TFS_BLOB_VELOCITIES, v_blobs);
+    return rc;
+}
+
+int tfs_rpy_barrier_forces_f32(tfs_rpy* h, const tfs_barrier_desc* desc, float* f_segs, float* f_blobs_repulsion) {
+    BarrierParams bp;
+    int rc = barrier_params(h, desc, f_segs != nullptr, &bp, "tfs_rpy_barrier_forces_f32");
+    if (rc) return rc;
+    const int64_t ns = h->desc.num_segs, nb = h->desc.num_blobs;
+    const bool host = h->desc.memory == TFS_MEMORY_HOST;
+    if (f_blobs_repulsion && nb > 0 && !h->blob_rep && hipMalloc(&h->blob_rep, 24 * nb) != hipSuccess)
+        return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_rpy_barrier_forces_f32: device memory");
+    rc = widen_in(h->f32, f_segs, h->f_segs, 6 * ns, host, h->stream, "tfs_rpy_barrier_forces_f32");
+    if (rc) return rc;
+    const MatvecArgs a = matvec_args(h, CView{h->f_segs, 6, 1}, CView{h->f_blobs, 3, 1});
+    hipError_t e = launch_barrier(a, h->grid, bp, h->f_segs, f_blobs_repulsion && nb > 0 ? h->blob_rep : nullptr,
+                                  h->stream);
+    if (e != hipSuccess) return hip_fail(e, "tfs_rpy_barrier_forces_f32");
+    rc = narrow_out(h->f32, h->f_segs, f_segs, 6 * ns, host, h->stream, "tfs_rpy_barrier_forces_f32");
+    if (!rc && f_blobs_repulsion && nb > 0)
+        rc = narrow_out(h->f32, h->blob_rep, f_blobs_repulsion, 3 * nb, host, h->stream, "tfs_rpy_barrier_forces_f32");
+    if (!rc && host) rc = hip_fail_if(hipStreamSynchronize(h->stream), "tfs_rpy_barrier_forces_f32");
+    return rc;
 }
 
 }  // extern "C"

@@ -92,6 +92,7 @@ struct tfs_solver {
     // inverses of E^T Z per swimmer, and a 6-per-swimmer work vector.
     double *x_bodies = nullptr, *bodyZ = nullptr, *body_sinv = nullptr, *body_t = nullptr;
     bool bodies_set = false, body_pc_valid = false;
+    tfs::F32Stage f32;  // FP32 entry points (tfs_solver_*_f32)
 };
 
 namespace {
@@ -753,6 +754,7 @@ int tfs_solver_destroy(tfs_solver* s) {
     if (s->arena) hipFree(s->arena);
     if (s->host_scal) hipHostFree(s->host_scal);
     if (s->host_flag) hipHostFree(s->host_flag);
+    free_stage(s->f32);
     for (hipEvent_t ev : s->flag_ev)
         if (ev) hipEventDestroy(ev);
     if (s->rpy) tfs_rpy_destroy(s->rpy);
@@ -883,6 +885,118 @@ int tfs_solver_residual(tfs_solver* s, const double* rhs, const double* x, int32
     const size_t n = s->dims.n;
     int rc = check(hipMemcpyAsync(s->b, rhs, n * 8, kind_in(memory), s->stream), "rhs");
     if (!rc) rc = check(hipMemcpyAsync(s->d, x, n * 8, kind_in(memory), s->stream), "x");
+    if (!rc) rc = residual(s, s->b, s->d, false, s->r, precision, nullptr);
+    double rn = 0, bn = 0;
+    if (!rc) rc = norm(s, s->r, &rn);
+    if (!rc) rc = norm(s, s->b, &bn);
+    if (rc) return rc;
+    *relres = bn > 0 ? rn / bn : rn;
+    return TFS_OK;
+}
+
+// ---- FP32 arrays (Real = float builds): widened into the FP64 device
+// vectors the solver works on, results narrowed back. --------------------
+
+static int memory_ok(int32_t memory, const char* where) {
+    return memory == TFS_MEMORY_HOST || memory == TFS_MEMORY_DEVICE
+               ? TFS_OK
+               : fail(TFS_ERR_INVALID_ARGUMENT, std::string(where) + ": unknown memory space");
+}
+
+int tfs_solver_set_geometry_f32(tfs_solver* s, const float* x_segs, const float* x_blobs, const float* vel_dir_phase,
+                                const float* vel_dir_angle, int32_t memory) {
+    const char* w = "tfs_solver_set_geometry_f32";
+    if (!s) return fail(TFS_ERR_INVALID_ARGUMENT, std::string(w) + ": null handle");
+    int rc = memory_ok(memory, w);
+    if (rc) return rc;
+    const int64_t ns3 = 3LL * s->desc.num_fil * s->desc.num_seg, nb3 = 3LL * s->desc.num_blobs;
+    const bool host = memory == TFS_MEMORY_HOST;
+    if (x_segs || x_blobs) invalidate_near(s->rpy);
+    if (x_segs || vel_dir_phase || vel_dir_angle) s->fil_pc_valid = false;
+    if (x_blobs) s->blob_pc_valid = false;
+    if (x_segs || x_blobs || vel_dir_phase || vel_dir_angle) s->body_pc_valid = false;
+    if (x_segs) rc = widen_in(s->f32, x_segs, s->rpy->x_segs, ns3, host, s->stream, w);
+    if (!rc && x_blobs) rc = widen_in(s->f32, x_blobs, s->rpy->x_blobs, nb3, host, s->stream, w);
+    if (!rc && vel_dir_phase) rc = widen_in(s->f32, vel_dir_phase, s->kphi, ns3, host, s->stream, w);
+    if (!rc && vel_dir_angle) rc = widen_in(s->f32, vel_dir_angle, s->kth, ns3, host, s->stream, w);
+    if (!rc && host) rc = check(hipStreamSynchronize(s->stream), w);
+    return rc;
+}
+
+int tfs_solver_set_bodies_f32(tfs_solver* s, const float* x_bodies, int32_t memory) {
+    const char* w = "tfs_solver_set_bodies_f32";
+    if (!s || !x_bodies) return fail(TFS_ERR_INVALID_ARGUMENT, std::string(w) + ": null argument");
+    if (!s->dims.free_body) return fail(TFS_ERR_INVALID_ARGUMENT, std::string(w) + ": the system has no free bodies");
+    int rc = memory_ok(memory, w);
+    if (!rc) rc = widen_in(s->f32, x_bodies, s->x_bodies, 3LL * s->dims.nswim, memory == TFS_MEMORY_HOST, s->stream, w);
+    if (!rc && memory == TFS_MEMORY_HOST) rc = check(hipStreamSynchronize(s->stream), w);
+    if (rc) return rc;
+    s->bodies_set = true;
+    s->body_pc_valid = false;
+    return TFS_OK;
+}
+
+int tfs_solver_system_mult_f32(tfs_solver* s, const float* in, float* out, int32_t precision, int32_t memory) {
+    const char* w = "tfs_solver_system_mult_f32";
+    if (!s || !in || !out) return fail(TFS_ERR_INVALID_ARGUMENT, std::string(w) + ": null argument");
+    if (precision != TFS_PRECISION_FP64 && precision != TFS_PRECISION_FP32_ACC64)
+        return fail(TFS_ERR_INVALID_ARGUMENT, std::string(w) + ": unknown precision");
+    int rc = memory_ok(memory, w);
+    const bool host = memory == TFS_MEMORY_HOST;
+    const int64_t n = s->dims.n;
+    if (!rc) rc = widen_in(s->f32, in, s->b, n, host, s->stream, w);
+    if (!rc) rc = apply_A(s, s->b, s->x, precision, nullptr);
+    if (!rc) rc = narrow_out(s->f32, s->x, out, n, host, s->stream, w);
+    if (!rc) rc = check(hipStreamSynchronize(s->stream), w);
+    return rc;
+}
+
+int tfs_solver_precondition_f32(tfs_solver* s, const float* in, float* out, int32_t memory) {
+    const char* w = "tfs_solver_precondition_f32";
+    if (!s || !in || !out) return fail(TFS_ERR_INVALID_ARGUMENT, std::string(w) + ": null argument");
+    int rc = memory_ok(memory, w);
+    const bool host = memory == TFS_MEMORY_HOST;
+    const int64_t n = s->dims.n;
+    if (!rc) rc = widen_in(s->f32, in, s->b, n, host, s->stream, w);
+    if (!rc) rc = ensure_block_precond(s);
+    if (!rc) rc = ensure_body_precond(s);
+    if (!rc) rc = apply_P(s, s->b, s->x);
+    if (!rc) rc = narrow_out(s->f32, s->x, out, n, host, s->stream, w);
+    if (!rc) rc = check(hipStreamSynchronize(s->stream), w);
+    return rc;
+}
+
+int tfs_solver_solve_f32(tfs_solver* s, const float* rhs, float* x, int32_t memory, tfs_solve_stats* stats) {
+    const char* w = "tfs_solver_solve_f32";
+    if (!s || !rhs || !x) return fail(TFS_ERR_INVALID_ARGUMENT, std::string(w) + ": null argument");
+    int rc = memory_ok(memory, w);
+    if (rc) return rc;
+    const bool host = memory == TFS_MEMORY_HOST;
+    const int64_t n = s->dims.n;
+    tfs_solve_stats local{};
+    tfs_solve_stats* st = stats ? stats : &local;
+    rc = widen_in(s->f32, rhs, s->b, n, host, s->stream, w);
+    if (!rc) rc = widen_in(s->f32, x, s->x, n, host, s->stream, w);
+    if (rc) return rc;
+    const int solve_rc = solve_device(s, s->b, s->x, st);
+    if (solve_rc != TFS_OK && solve_rc != TFS_ERR_NOT_CONVERGED) return solve_rc;
+    const std::string msg = g_last_error;
+    rc = narrow_out(s->f32, s->x, x, n, host, s->stream, w);
+    if (!rc) rc = check(hipStreamSynchronize(s->stream), w);
+    if (rc) return rc;
+    g_last_error = msg;
+    return solve_rc;
+}
+
+int tfs_solver_residual_f32(tfs_solver* s, const float* rhs, const float* x, int32_t precision, int32_t memory,
+                            double* relres) {
+    const char* w = "tfs_solver_residual_f32";
+    if (!s || !rhs || !x || !relres) return fail(TFS_ERR_INVALID_ARGUMENT, std::string(w) + ": null argument");
+    int rc = memory_ok(memory, w);
+    const bool host = memory == TFS_MEMORY_HOST;
+    const int64_t n = s->dims.n;
+    if (!rc) rc = widen_in(s->f32, rhs, s->b, n, host, s->stream, w);
+    if (!rc) rc = widen_in(s->f32, x, s->d, n, host, s->stream, w);
     if (!rc) rc = residual(s, s->b, s->d, false, s->r, precision, nullptr);
     double rn = 0, bn = 0;
     if (!rc) rc = norm(s, s->r, &rn);

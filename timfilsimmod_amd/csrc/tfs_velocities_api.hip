@@ -260,6 +260,9 @@ struct tfs_velocities {
     double *body_refs = nullptr, *Z = nullptr, *nref = nullptr, *small = nullptr, *d_q = nullptr;
     bool free_direct_valid = false;
     int32_t blob_motion = TFS_BLOBS_CHECK;  // tfs_velocities_set_blob_motion
+    // FP32 entry points: staging, and the FP64 images of the caller's arrays
+    tfs::F32Stage f32;
+    double* f32_arena = nullptr;
 };
 
 namespace {
@@ -472,6 +475,8 @@ int tfs_velocities_destroy(tfs_velocities* v) {
     if (v->Ainv) (void)hipFree(v->Ainv);
     if (v->dense_arena) (void)hipFree(v->dense_arena);
     if (v->h_flag) (void)hipHostFree(v->h_flag);
+    if (v->f32_arena) (void)hipFree(v->f32_arena);
+    free_stage(v->f32);
     if (v->own_stream) (void)hipStreamDestroy(v->stream);
     delete v;
     return TFS_OK;
@@ -634,6 +639,85 @@ int tfs_velocities_compute_free(tfs_velocities* v, const double* x_segs, const d
                     "tfs_velocities_compute_free: the direct solve needs the orientation and "
                     "tfs_velocities_set_body_frame");
     return compute(v, x_segs, f_segs, x_blobs, x_bodies, f_bodies, q_bodies, v_segs, f_blobs, v_bodies, stats, true);
+}
+
+}  // extern "C"
+
+namespace {
+
+// compute() on FP32 caller arrays: every input widened into an FP64 device
+// image, the FP64 computation on those (device memory), the outputs narrowed
+// back into the caller's arrays.
+int compute_f32(tfs_velocities* v, const float* x_segs, const float* f_segs, const float* x_blobs,
+                const float* x_bodies, const float* bodies, const float* q_bodies, float* v_segs, float* f_blobs,
+                float* v_bodies_io, tfs_solve_stats* stats, bool free_bodies, const char* w) {
+    const int64_t ns = v->ns, nb = v->nb, nsw = v->nswim;
+    const int64_t lens[9] = {3 * ns, 6 * ns, 3 * nb, 3 * nsw, 6 * nsw, 4 * nsw, 6 * ns, 3 * nb, 6 * nsw};
+    int64_t total = 0;
+    for (int64_t l : lens) total += (std::max<int64_t>(l, 1) + 31) / 32 * 32;
+    if (!v->f32_arena && hipMalloc(&v->f32_arena, 8 * total) != hipSuccess)
+        return fail(TFS_ERR_OUT_OF_MEMORY, std::string(w) + ": device memory");
+    double* img[9];
+    double* p = v->f32_arena;
+    for (int i = 0; i < 9; ++i) {
+        img[i] = p;
+        p += (std::max<int64_t>(lens[i], 1) + 31) / 32 * 32;
+    }
+    const bool host = v->desc.memory == TFS_MEMORY_HOST;
+    hipStream_t st = v->stream;
+    const float* in[9] = {x_segs, f_segs, x_blobs, x_bodies, bodies, q_bodies, nullptr, f_blobs, v_bodies_io};
+    int rc = TFS_OK;
+    for (int i = 0; i < 9 && !rc; ++i)
+        if (in[i] && lens[i] > 0) rc = widen_in(v->f32, in[i], img[i], lens[i], host, st, w);
+    if (rc) return rc;
+    const int32_t memory = v->desc.memory;
+    v->desc.memory = TFS_MEMORY_DEVICE;
+    const int crc = compute(v, ns ? img[0] : nullptr, ns ? img[1] : nullptr, img[2], img[3], bodies ? img[4] : nullptr,
+                            q_bodies ? img[5] : nullptr, ns ? img[6] : nullptr, img[7], free_bodies ? img[8] : nullptr,
+                            stats, free_bodies);
+    v->desc.memory = memory;
+    if (crc != TFS_OK && crc != TFS_ERR_NOT_CONVERGED) return crc;
+    const std::string msg = g_last_error;
+    if (ns) rc = narrow_out(v->f32, img[6], v_segs, 6 * ns, host, st, w);
+    if (!rc) rc = narrow_out(v->f32, img[7], f_blobs, 3 * nb, host, st, w);
+    if (!rc && free_bodies) rc = narrow_out(v->f32, img[8], v_bodies_io, 6 * nsw, host, st, w);
+    if (!rc) rc = check(hipStreamSynchronize(st), w);
+    if (rc) return rc;
+    g_last_error = msg;
+    return crc;
+}
+
+}  // namespace
+
+int32_t tfs::velocities_blob_motion(const tfs_velocities* v) { return v->blob_motion; }
+
+extern "C" {
+
+int tfs_velocities_compute_f32(tfs_velocities* v, const float* x_segs, const float* f_segs, const float* x_blobs,
+                               const float* x_bodies, const float* v_bodies, float* v_segs, float* f_blobs,
+                               tfs_solve_stats* stats) {
+    if (!v || !x_blobs || !x_bodies || !v_bodies || !f_blobs || (v->ns && (!x_segs || !f_segs || !v_segs)))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_compute_f32: null argument");
+    if (v->desc.free_bodies)
+        return fail(TFS_ERR_INVALID_ARGUMENT,
+                    "tfs_velocities_compute_f32: free bodies: use tfs_velocities_compute_free_f32");
+    return compute_f32(v, x_segs, f_segs, x_blobs, x_bodies, v_bodies, nullptr, v_segs, f_blobs, nullptr, stats, false,
+                       "tfs_velocities_compute_f32");
+}
+
+int tfs_velocities_compute_free_f32(tfs_velocities* v, const float* x_segs, const float* f_segs, const float* x_blobs,
+                                    const float* x_bodies, const float* f_bodies, const float* q_bodies, float* v_segs,
+                                    float* f_blobs, float* v_bodies, tfs_solve_stats* stats) {
+    if (!v || !x_blobs || !x_bodies || !v_bodies || !f_blobs || (v->ns && (!x_segs || !f_segs || !v_segs)))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_compute_free_f32: null argument");
+    if (!v->desc.free_bodies)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_velocities_compute_free_f32: the handle has no free bodies");
+    if (v->direct && (!q_bodies || !v->body_refs))
+        return fail(TFS_ERR_INVALID_ARGUMENT,
+                    "tfs_velocities_compute_free_f32: the direct solve needs the orientation and "
+                    "tfs_velocities_set_body_frame");
+    return compute_f32(v, x_segs, f_segs, x_blobs, x_bodies, f_bodies, q_bodies, v_segs, f_blobs, v_bodies, stats, true,
+                       "tfs_velocities_compute_free_f32");
 }
 
 int tfs_velocities_set_blob_motion(tfs_velocities* v, int32_t motion) {

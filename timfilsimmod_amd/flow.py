@@ -34,6 +34,8 @@ L.extra_signatures({
     "tfs_flow_destroy": (ctypes.c_int, [_vp]),
     "tfs_flow_set_sources": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "tfs_flow_evaluate": (ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp]),
+    "tfs_flow_set_sources_f32": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "tfs_flow_evaluate_f32": (ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp]),
     "tfs_flow_sync": (ctypes.c_int, [_vp]),
     "tfs_flow_set_block": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "tfs_flow_last_kernel_ms": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),

@@ -85,6 +85,12 @@ L.extra_signatures({
     "tfs_velocities_set_preconditioner": (ctypes.c_int, [_dp, _i32]),
     "tfs_beat_velocities": (ctypes.c_int, [_dp, _dp, _dp]),
     "tfs_solver_set_preconditioner": (ctypes.c_int, [_dp, _i32]),
+    "tfs_solver_set_geometry_f32": (ctypes.c_int, [_dp, _dp, _dp, _dp, _dp, _i32]),
+    "tfs_solver_set_bodies_f32": (ctypes.c_int, [_dp, _dp, _i32]),
+    "tfs_solver_system_mult_f32": (ctypes.c_int, [_dp, _dp, _dp, _i32, _i32]),
+    "tfs_solver_precondition_f32": (ctypes.c_int, [_dp, _dp, _dp, _i32]),
+    "tfs_solver_solve_f32": (ctypes.c_int, [_dp, _dp, _dp, _i32, ctypes.POINTER(SolveStats)]),
+    "tfs_solver_residual_f32": (ctypes.c_int, [_dp, _dp, _dp, _i32, _i32, ctypes.POINTER(ctypes.c_double)]),
 })
 
 
@@ -105,6 +111,8 @@ L.extra_signatures({
     "tfs_velocities_create": (ctypes.c_int, [ctypes.POINTER(VelocitiesDesc), ctypes.POINTER(_dp)]),
     "tfs_velocities_destroy": (ctypes.c_int, [_dp]),
     "tfs_velocities_compute": (ctypes.c_int, [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(SolveStats)]),
+    "tfs_velocities_compute_f32": (ctypes.c_int, [_dp] * 8 + [ctypes.POINTER(SolveStats)]),
+    "tfs_velocities_compute_free_f32": (ctypes.c_int, [_dp] * 10 + [ctypes.POINTER(SolveStats)]),
 })
 
 
