@@ -237,6 +237,26 @@ int tfs_rpy_apply_f32(tfs_rpy *h, uint32_t blocks, const float *f_segs, const fl
 int tfs_rpy_barrier_forces_f32(tfs_rpy *h, const tfs_barrier_desc *desc, float *f_segs,
                                float *f_blobs_repulsion);
 
+/* The matvec of tfs_rpy_apply without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No handle: `desc` gives the sizes, radii, precision, write rule and stream
+ * (memory must be TFS_MEMORY_DEVICE: every pointer is a device pointer);
+ * `blocks` and the write rules are those of tfs_rpy_apply (v_* are read
+ * first where a block accumulates), for one body in the unbounded domain.
+ * The work list, partial sums, overlapping-pair lists, their grid and the
+ * FP32 tiles' fixed-point positions live in one static device array of about
+ * 22 MiB that each GPU the library runs on holds until the process ends.
+ * Limits: at most 32768 particles (segments + blobs) per call, at most 32
+ * overlapping neighbours per particle (TFS_ERR_INVALID_ARGUMENT beyond
+ * either).  It synchronises its stream once (the neighbour count is read
+ * back) and is otherwise asynchronous.  Every call on one GPU shares that
+ * array: calls that overlap in time on one GPU (two host threads, or two
+ * streams) race and must be serialised by the caller. */
+int tfs_rpy_apply_noalloc(const tfs_rpy_desc *desc, uint32_t blocks, const double *x_segs,
+                          const double *x_blobs, const double *f_segs, const double *f_blobs,
+                          double *v_segs, double *v_blobs);
+
 const char *tfs_last_error(void);
 
 /* Library version string. */

@@ -96,6 +96,7 @@ SIGNATURES = {
     "tfs_rpy_download_f32": (ctypes.c_int, [_vp, _i32, _vp]),
     "tfs_rpy_apply_f32": (ctypes.c_int, [_vp, _u32, _vp, _vp, _vp, _vp]),
     "tfs_rpy_barrier_forces_f32": (ctypes.c_int, [_vp, ctypes.POINTER(BarrierDesc), _vp, _vp]),
+    "tfs_rpy_apply_noalloc": (ctypes.c_int, [ctypes.POINTER(RpyDesc), _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "tfs_last_error": (ctypes.c_char_p, []),
     "tfs_version": (ctypes.c_char_p, []),
 }

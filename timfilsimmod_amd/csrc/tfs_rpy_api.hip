@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -1027,6 +1028,233 @@ int tfs_rpy_barrier_forces_f32(tfs_rpy* h, const tfs_barrier_desc* desc, float* 
         rc = narrow_out(h->f32, h->blob_rep, f_blobs_repulsion, 3 * nb, host, h->stream, "tfs_rpy_barrier_forces_f32");
     if (!rc && host) rc = hip_fail_if(hipStreamSynchronize(h->stream), "tfs_rpy_barrier_forces_f32");
     return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// tfs_rpy_apply_noalloc: the matvec of tfs_rpy_apply for callers that cannot
+// allocate.  Everything it needs beyond the caller's arrays -- the work list,
+// the partial sums, the overlapping-pair lists, the hashed grid they are found
+// with and the fixed-point positions of FP32 tiles -- lives in `scratch`, one
+// file-scope device array of kScratchBytes that each GPU holds for the life of
+// the process.  Host code never passes `scratch` itself to a kernel (in host
+// code the name is a host-side stand-in): its device address is looked up
+// once per device with hipGetSymbolAddress and kept in g_scratch_ptr.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kStaticMaxParticles = 32768;   // segments + blobs per call
+constexpr int kStaticTile = 256;             // targets per tile: 256 threads, one target each
+constexpr int kStaticNearCap = 32;           // overlapping neighbours per particle
+constexpr size_t kStaticPartialBytes = size_t(16) << 20;
+constexpr int kStaticMaxItems = (int)(kStaticPartialBytes / (6 * kStaticTile * sizeof(double)));
+constexpr int kStaticMaxTiles = 2 * (kStaticMaxParticles / kStaticTile + 1);
+constexpr int kStaticTable = 65536;          // grid buckets: a power of two >= 2 kStaticMaxParticles
+
+constexpr size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+constexpr size_t kOffPartial = 0;
+constexpr size_t kOffNearIdx = align256(kOffPartial + kStaticPartialBytes);
+constexpr size_t kOffNearCnt = align256(kOffNearIdx + sizeof(int32_t) * kStaticMaxParticles * kStaticNearCap);
+constexpr size_t kOffNearMax = align256(kOffNearCnt + sizeof(int32_t) * kStaticMaxParticles);
+constexpr size_t kOffStart = align256(kOffNearMax + sizeof(int32_t));
+constexpr size_t kOffFill = align256(kOffStart + sizeof(int32_t) * (kStaticTable + 1));
+constexpr size_t kOffBucket = align256(kOffFill + sizeof(int32_t) * kStaticTable);
+constexpr size_t kOffOrder = align256(kOffBucket + sizeof(int32_t) * kStaticMaxParticles);
+constexpr size_t kOffBodyOf = align256(kOffOrder + sizeof(int32_t) * kStaticMaxParticles);
+constexpr size_t kOffFixed = align256(kOffBodyOf + sizeof(int32_t) * kStaticMaxParticles);
+constexpr size_t kOffBody = align256(kOffFixed + sizeof(int4) * kStaticMaxParticles);
+constexpr size_t kOffBox = align256(kOffBody + sizeof(int4));
+constexpr size_t kOffTiles = align256(kOffBox + 6 * sizeof(double));
+constexpr size_t kOffItems = align256(kOffTiles + sizeof(MatvecTile) * kStaticMaxTiles);
+constexpr size_t kScratchBytes = align256(kOffItems + sizeof(MatvecItem) * kStaticMaxItems);
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on first use).
+int scratch_address(unsigned char** out, const char* where) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, where);
+    if (dev < 0 || dev >= kMaxDevices) return fail(TFS_ERR_NO_DEVICE, std::string(where) + ": device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, where);
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return TFS_OK;
+}
+
+// The work list of one body: ts segment tiles (is items each) then tb blob
+// tiles (ib items each); a tile's items take its sources in chunks, segments
+// first, then blobs -- the order build_worklist gives.  One thread per tile;
+// thread 0 also writes the body record.
+__global__ __launch_bounds__(256) void static_worklist_kernel(int ns, int nb, int chunk, int ts, int tb, int is_,
+                                                              int ib, int seg_write, int blob_write, int ss, int sb,
+                                                              int bs, int bb, MatvecTile* __restrict__ tiles,
+                                                              MatvecItem* __restrict__ items, int4* __restrict__ body) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t == 0) *body = make_int4(0, ns, 0, nb);
+    if (t >= ts + tb) return;
+    const bool seg = t < ts;
+    const int u = seg ? t : t - ts;
+    const int n_t = seg ? ns : nb;
+    MatvecTile tl;
+    tl.cls = seg ? 0 : 1;
+    tl.t_begin = u * kStaticTile;
+    tl.t_end = tl.t_begin + kStaticTile < n_t ? tl.t_begin + kStaticTile : n_t;
+    tl.first_item = seg ? u * is_ : ts * is_ + u * ib;
+    tl.n_items = seg ? is_ : ib;
+    tl.write_mode = seg ? seg_write : blob_write;
+    tl.body = 0;
+    tiles[t] = tl;
+    int k = tl.first_item;
+    if (seg ? ss : bs)
+        for (int s0 = 0; s0 < ns; s0 += chunk) items[k++] = MatvecItem{t, 0, s0, s0 + chunk < ns ? s0 + chunk : ns};
+    if (seg ? sb : bb)
+        for (int s0 = 0; s0 < nb; s0 += chunk) items[k++] = MatvecItem{t, 1, s0, s0 + chunk < nb ? s0 + chunk : nb};
+}
+
+// X = rint(lambda (x - c)) for the one body of the call.
+__global__ __launch_bounds__(256) void static_fixed_kernel(const double* __restrict__ xs, const double* __restrict__ xb,
+                                                           int ns, int nb, double c0, double c1, double c2, double lam,
+                                                           int4* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= ns + nb) return;
+    const double* x = i < ns ? xs + 3 * (int64_t)i : xb + 3 * (int64_t)(i - ns);
+    out[i] = make_int4((int)__builtin_rint((x[0] - c0) * lam), (int)__builtin_rint((x[1] - c1) * lam),
+                       (int)__builtin_rint((x[2] - c2) * lam), 0);
+}
+
+int static_items(int ns, int nb, int chunk, int ts, int tb, bool ss, bool sb, bool bs, bool bb, int* is_, int* ib) {
+    const int cs = (ns + chunk - 1) / chunk, cb = (nb + chunk - 1) / chunk;
+    *is_ = (ss ? cs : 0) + (sb ? cb : 0);
+    *ib = (bs ? cs : 0) + (bb ? cb : 0);
+    return ts * *is_ + tb * *ib;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tfs_rpy_apply_noalloc(const tfs_rpy_desc* desc, uint32_t blocks, const double* x_segs, const double* x_blobs,
+                          const double* f_segs, const double* f_blobs, double* v_segs, double* v_blobs) {
+    const char* w = "tfs_rpy_apply_noalloc";
+    if (!desc) return fail(TFS_ERR_INVALID_ARGUMENT, std::string(w) + ": null descriptor");
+    const tfs_rpy_desc& D = *desc;
+    if (D.num_segs < 0 || D.num_blobs < 0 || (D.seg_dof != 3 && D.seg_dof != 6) ||
+        (D.precision != TFS_PRECISION_FP64 && D.precision != TFS_PRECISION_FP32_ACC64) ||
+        (D.bs_mode != TFS_BS_OVERWRITE && D.bs_mode != TFS_BS_ACCUMULATE) || !(D.seg_radius > 0) ||
+        !(D.blob_radius > 0) || !(D.mu > 0))
+        return fail(TFS_ERR_INVALID_ARGUMENT, std::string(w) + ": bad descriptor");
+    if (D.memory != TFS_MEMORY_DEVICE)
+        return fail(TFS_ERR_INVALID_ARGUMENT, std::string(w) + ": takes device pointers (memory = TFS_MEMORY_DEVICE)");
+    if (blocks & ~TFS_BLOCK_ALL) return fail(TFS_ERR_INVALID_ARGUMENT, std::string(w) + ": bad block mask");
+    const int ns = D.num_segs, nb = D.num_blobs;
+    if ((int64_t)ns + nb > kStaticMaxParticles)
+        return fail(TFS_ERR_INVALID_ARGUMENT, std::string(w) + ": at most " + std::to_string(kStaticMaxParticles) +
+                                                  " particles (segments + blobs) per call; use tfs_rpy_apply");
+    // the blocks evaluated and their write rules, as tfs_rpy_apply
+    bool ss = blocks & TFS_BLOCK_SS, sb = blocks & TFS_BLOCK_SB, bb = blocks & TFS_BLOCK_BB, bs = blocks & TFS_BLOCK_BS;
+    if (bs && bb && D.bs_mode == TFS_BS_OVERWRITE) bb = false;
+    const int seg_write = ss ? 0 : (sb ? 1 : -1);
+    const int blob_write = bb ? 0 : (bs ? (D.bs_mode == TFS_BS_OVERWRITE ? 0 : 1) : -1);
+    const uint32_t eff = (ss ? 1u : 0u) | (sb ? 2u : 0u) | (bb ? 4u : 0u) | (bs ? 8u : 0u);
+    if ((ns && !x_segs) || (nb && !x_blobs) || ((ss || bs) && ns && !f_segs) || ((sb || bb) && nb && !f_blobs) ||
+        (seg_write >= 0 && ns && !v_segs) || (blob_write >= 0 && nb && !v_blobs))
+        return fail(TFS_ERR_INVALID_ARGUMENT, std::string(w) + ": missing array for requested block");
+    const int ts = seg_write >= 0 ? (ns + kStaticTile - 1) / kStaticTile : 0;
+    const int tb = blob_write >= 0 ? (nb + kStaticTile - 1) / kStaticTile : 0;
+    if (ts + tb == 0) return TFS_OK;
+    // the smallest chunk (a multiple of the tile) whose items fit the partial buffer
+    int chunk = kStaticTile, is_ = 0, ib = 0;
+    while (static_items(ns, nb, chunk, ts, tb, ss, sb, bs, bb, &is_, &ib) > kStaticMaxItems) chunk += kStaticTile;
+    const int n_items = ts * is_ + tb * ib;
+    unsigned char* base = nullptr;
+    int rc = scratch_address(&base, w);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)D.stream;
+    MatvecTile* tiles = reinterpret_cast<MatvecTile*>(base + kOffTiles);
+    MatvecItem* items = reinterpret_cast<MatvecItem*>(base + kOffItems);
+    int4* body = reinterpret_cast<int4*>(base + kOffBody);
+    hipLaunchKernelGGL(static_worklist_kernel, dim3((unsigned)((ts + tb + 255) / 256)), dim3(256), 0, st, ns, nb, chunk,
+                       ts, tb, is_, ib, seg_write, blob_write, (int)ss, (int)sb, (int)bs, (int)bb, tiles, items, body);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, w);
+
+    MatvecArgs a;
+    a.xs = {x_segs, 3, 1};
+    a.xb = {x_blobs, 3, 1};
+    a.fs = {f_segs, 6, 1};
+    a.fb = {f_blobs, 3, 1};
+    a.ns = ns;
+    a.nb = nb;
+    a.a_seg = D.seg_radius;
+    a.a_blob = D.blob_radius;
+    a.mu = D.mu;
+    const int n = ns + nb;
+    if (D.precision == TFS_PRECISION_FP32_ACC64) {
+        // fixed-point frame of the body's bounding box (read back to the stack)
+        double* box = reinterpret_cast<double*>(base + kOffBox);
+        hipLaunchKernelGGL(body_box_kernel, dim3(1), dim3(256), 0, st, x_segs, x_blobs, (const int4*)body, box);
+        double hbox[6], centre[3], lam = 1.0;
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(hbox, box, sizeof(hbox), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(e, w);
+        rc = fixed_point_frame(hbox, 1, std::min(D.seg_radius, D.blob_radius), centre, &lam);
+        if (rc) return rc;
+        int4* fx = reinterpret_cast<int4*>(base + kOffFixed);
+        hipLaunchKernelGGL(static_fixed_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x_segs, x_blobs,
+                           ns, nb, centre[0], centre[1], centre[2], lam, fx);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, w);
+        a.xfx = fx;
+        a.lam = lam;
+    }
+    // overlapping pairs, on the grid kept in scratch (sized for n: no allocation)
+    NearList near;
+    near.cnt = reinterpret_cast<int32_t*>(base + kOffNearCnt);
+    near.idx = reinterpret_cast<int32_t*>(base + kOffNearIdx);
+    near.max_cnt = reinterpret_cast<int32_t*>(base + kOffNearMax);
+    near.cap = kStaticNearCap;
+    NearGrid grid;
+    grid.start = reinterpret_cast<int32_t*>(base + kOffStart);
+    grid.fill = reinterpret_cast<int32_t*>(base + kOffFill);
+    grid.bucket_of = reinterpret_cast<int32_t*>(base + kOffBucket);
+    grid.order = reinterpret_cast<int32_t*>(base + kOffOrder);
+    grid.body_of = reinterpret_cast<int32_t*>(base + kOffBodyOf);
+    grid.n = n;
+    grid.table = 1024;
+    while (grid.table < 2 * n) grid.table <<= 1;
+    int max_count = 0, host_count = 0;
+    e = build_near_list(D.precision, a, grid, reinterpret_cast<const int32_t*>(body), 1, near, n, &max_count, &host_count,
+                        st);
+    if (e != hipSuccess) return hip_fail(e, w);
+    if (max_count > kStaticNearCap)
+        return fail(TFS_ERR_INVALID_ARGUMENT, std::string(w) + ": a particle overlaps more than " +
+                                                  std::to_string(kStaticNearCap) + " others; use tfs_rpy_apply");
+    OutArgs o;
+    o.vs = View{v_segs, 6, 1};
+    o.vb = View{v_blobs, 3, 1};
+    o.seg_out_slots = 6;
+    o.blocks = eff;
+    const TileShape sh{kStaticTile, 1, chunk, 0};
+    PersistState ps;
+    e = launch_matvec(D.precision, D.seg_dof, sh, a, o, tiles, ts + tb, items, n_items,
+                      reinterpret_cast<double*>(base + kOffPartial), near, ps, st);
+    return e == hipSuccess ? TFS_OK : hip_fail(e, w);
 }
 
 }  // extern "C"
