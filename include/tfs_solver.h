@@ -144,6 +144,34 @@ int tfs_solver_residual_f32(tfs_solver *s, const float *rhs, const float *x, int
  * tfs_rpy_set_tile_shape, ...).  Owned by the solver. */
 int tfs_solver_rpy(tfs_solver *s, tfs_rpy **out);
 
+/* Independent bodies solved in lock-step.  The solver's arrays hold
+ * num_bodies bodies one after another (fil_per_body[i] filaments and
+ * blobs_per_body[i] blobs each, summing to the solver's sizes); the matvec
+ * couples particles of the same body only (tfs_rpy_set_bodies), and every
+ * tfs_solver_solve runs one GMRES per body: each body has its own Krylov
+ * basis, Hessenberg matrix, tolerance (tol times its own ||rhs||), warm-start
+ * rules and refinement count, and one batched launch serves all bodies that
+ * are still iterating -- a body that has met its tolerance drops out of the
+ * matvec's work list and of the Arnoldi launches.  With graphs on (the
+ * default) each GMRES cycle of `restart` steps is one captured hipGraph,
+ * replayed while the positions' work list and scales are unchanged.  Fixed
+ * bodies only (no free_bodies); restart <= 120.  No counterpart in the
+ * reference, whose NSWIM bodies are all coupled (mobility_solver.cpp:1806-1908);
+ * a solve here equals num_bodies separate one-body solves.  num_bodies = 1
+ * returns to the single-system solve. */
+int tfs_solver_set_independent_bodies(tfs_solver *s, int32_t num_bodies, const int32_t *fil_per_body,
+                                      const int32_t *blobs_per_body);
+/* Captured cycles on (1) or off (0: the cycle's steps are enqueued four at a
+ * time with a device stop count read back between). */
+int tfs_solver_set_graphs(tfs_solver *s, int32_t enable);
+/* Per body, of the last solve: GMRES steps, whether its tolerance was met,
+ * its final ||rhs - A x|| / ||rhs||, and the device time from the solve's
+ * start to the body's last Arnoldi stop in ms (any argument may be null). */
+int tfs_solver_body_stats(tfs_solver *s, int32_t *iterations, int32_t *converged, double *relres,
+                          double *solve_ms);
+/* GMRES cycles run as captured graphs and as eager launches since creation. */
+int tfs_solver_cycle_counts(tfs_solver *s, int64_t *graph_cycles, int64_t *eager_cycles);
+
 /* ---- Prescribed-beat timestep driver (one body, body velocity prescribed) ---- */
 typedef struct tfs_beat_desc {
     tfs_solver_desc solver;

@@ -126,6 +126,52 @@ hipError_t k_rhs(const RhsArgs& a, const double* x_segs, const double* x_blobs, 
 hipError_t k_phase_update(const double* x, const SaddleDims& d, double dt, double* phase, double* angle,
                           hipStream_t s);
 
+// ---- independent bodies in lock-step ----------------------------------------
+// One GMRES per body, every body's Krylov vectors contiguous in the Krylov
+// layout (body b at [koff[b], koff[b+1])).  Per-body arrays: H at b*hstride
+// (ldh x m), cs / sn / g / est / y at b*mstride (m + 1 each), scal at 2b,
+// partial sums at b*pstride (G x (m+1)).  stop[b]: 0 running, j + 1 stopped
+// at step j, -(j + 1) non-finite, kEnsInactive not taking part.
+constexpr int32_t kEnsInactive = 0x40000000;
+struct EnsArgs {
+    const int64_t* koff = nullptr;
+    int nbody = 0;
+    double* V = nullptr;
+    int64_t ld = 0;
+    double* w = nullptr;
+    double* H = nullptr;
+    int ldh = 0;
+    int64_t hstride = 0;
+    double *cs = nullptr, *sn = nullptr, *g = nullptr, *est = nullptr;
+    int mstride = 0;
+    double* scal = nullptr;
+    int32_t* stop = nullptr;
+    int32_t* nstop = nullptr;         // bodies that stopped since it was cleared
+    const double* abs_tol = nullptr;  // per body
+    double *pa = nullptr, *pb = nullptr;
+    int64_t pstride = 0;
+    int64_t* stamps = nullptr;        // per body: wall clock of its last stop (may be null)
+};
+// CGS2 step j of every running body (three launches, G x nbody workgroups).
+hipError_t k_ens_arnoldi_step(const EnsArgs& e, int j, int G, hipStream_t s);
+// Close step j of every running body; with write_next, v_{j+1} into the basis.
+hipError_t k_ens_close(const EnsArgs& e, int j, bool write_next, int G, hipStream_t s);
+// v_0 = r / ||r|| and g = (||r||, 0, ...) for every running body (r in the Krylov layout).
+hipError_t k_ens_open(const EnsArgs& e, const double* r, const double* rnorm, int G, hipStream_t s);
+// Between layouts: map[i] is the operator index of Krylov index i.
+hipError_t k_ens_permute(const double* src, double* dst, const int32_t* map, int64_t n, bool to_operator,
+                         hipStream_t s);
+// out[b] = ||v|| over body b's rows of a Krylov-layout vector.
+hipError_t k_ens_norms(const double* v, const int64_t* koff, int nbody, double* out, hipStream_t s);
+// Per body: the back substitution of the cycle's k_b steps (kdone[b] = k_b)
+// and out_b = V_b y_b (Krylov layout).
+hipError_t k_ens_update(const EnsArgs& e, int m, double* y, int32_t* kdone, double* out, int G, hipStream_t s);
+// x = 0 (and r = rhs) over the operator rows of every body with flag[b] set.
+hipError_t k_ens_reset(const int32_t* flag, const int64_t* koff, int nbody, const int32_t* map, double* x, double* r,
+                       const double* rhs, int G, hipStream_t s);
+// stamps[b] = the device wall clock, every body.
+hipError_t k_ens_stamp(int64_t* stamps, int nbody, hipStream_t s);
+
 // ---- free bodies ----------------------------------------------------------
 // out[6 s + c] = sign * (sum over the particles i of swimmer s of
 // (vec_i, (x_i - x_body_s) x vec_i))_c: the body rows K_b^T vec (one

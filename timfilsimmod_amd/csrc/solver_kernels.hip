@@ -1265,6 +1265,196 @@ __global__ void body_advance_kernel(const double* __restrict__ sol, int64_t off_
     pose[6] = e0 * q3 + q0 * e3 + e1 * q2 - e2 * q1;
 }
 
+
+// ---------------------------------------------------------------------------
+// Independent bodies in lock-step: one GMRES per body.  The Krylov basis is
+// kept in the bodies' own layout (body b's unknowns contiguous at koff[b]), so
+// the per-vector device functions above run per body on a 2-D grid: x over
+// the body's row slices (gridDim.x workgroups), y over the bodies.  A body
+// whose stop flag is set does nothing.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ ArnoldiArgs ens_slice(const EnsArgs& e, int b, int ncols) {
+    return ArnoldiArgs{e.V + e.koff[b], e.ld, ncols, e.w + e.koff[b], (int)(e.koff[b + 1] - e.koff[b]), nullptr};
+}
+
+__global__ __launch_bounds__(256) void ens_dots_kernel(EnsArgs e, int ncols) {
+    const int b = blockIdx.y;
+    if (e.stop[b]) return;
+    slice_dots(ens_slice(e, b, ncols), e.pa + (int64_t)b * e.pstride, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(256) void ens_update_dots_kernel(EnsArgs e, int ncols, int j) {
+    __shared__ double hs[kMaxBasis];
+    const int b = blockIdx.y;
+    if (e.stop[b]) return;
+    const ArnoldiArgs a = ens_slice(e, b, ncols);
+    gather_coeffs(e.pa + (int64_t)b * e.pstride, ncols, gridDim.x, hs, e.H + (int64_t)b * e.hstride + (int64_t)j * e.ldh,
+                  true);
+    slice_update(a, hs, blockIdx.x, gridDim.x);
+    __syncthreads();
+    slice_dots(a, e.pb + (int64_t)b * e.pstride, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(256) void ens_update_sumsq_kernel(EnsArgs e, int ncols, int j) {
+    __shared__ double hs[kMaxBasis];
+    __shared__ double sh[4];
+    const int b = blockIdx.y;
+    if (e.stop[b]) return;
+    const ArnoldiArgs a = ens_slice(e, b, ncols);
+    gather_coeffs(e.pb + (int64_t)b * e.pstride, ncols, gridDim.x, hs, e.H + (int64_t)b * e.hstride + (int64_t)j * e.ldh,
+                  false);
+    slice_update(a, hs, blockIdx.x, gridDim.x);
+    __syncthreads();
+    double s = 0.0;
+    for (int64_t r = (int64_t)blockIdx.x * kSlice + threadIdx.x; r < a.n; r += (int64_t)gridDim.x * kSlice) {
+#pragma unroll
+        for (int q = 0; q < kSlice / 256; ++q) {
+            const int64_t rr = r + q * 256;
+            if (rr < a.n) s += a.w[rr] * a.w[rr];
+        }
+    }
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) e.pa[(int64_t)b * e.pstride + blockIdx.x] = s;
+}
+
+// Closes step j of every running body (||w|| from its partial sums, Givens,
+// estimate, stop flag; the body's workgroup 0) and, with write_next, writes
+// v_{j+1} = w / ||w|| into column j + 1 of its basis.  A body that stops here
+// is counted in *nstop and stamped (wall clock) in stamps[b].
+__global__ __launch_bounds__(256) void ens_close_kernel(EnsArgs e, int j, int write_next) {
+    __shared__ double nrm;
+    const int b = blockIdx.y;
+    if (e.stop[b]) return;
+    if (threadIdx.x < kWave) {
+        const double t = wave_sum_strided(e.pa + (int64_t)b * e.pstride, gridDim.x);
+        if (threadIdx.x == 0) nrm = __builtin_sqrt(t);
+    }
+    __syncthreads();
+    const double hn = nrm;
+    if (blockIdx.x == 0) {
+        const int64_t mo = (int64_t)b * e.mstride;
+        arnoldi_close(hn, e.H + (int64_t)b * e.hstride, e.ldh, j, e.cs + mo, e.sn + mo, e.g + mo, e.scal + 2 * b,
+                      e.est + mo, e.stop + b, e.abs_tol[b]);
+        if (threadIdx.x == 0 && e.stop[b] != 0) {
+            atomicAdd(e.nstop, 1);
+            if (e.stamps) e.stamps[b] = (int64_t)wall_clock64();
+        }
+    }
+    if (write_next) {
+        const double sc = hn > 0.0 ? 1.0 / hn : 0.0;
+        const int64_t o = e.koff[b], n = e.koff[b + 1] - o;
+        double* v = e.V + (int64_t)(j + 1) * e.ld + o;
+        const double* w = e.w + o;
+        for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += (int64_t)gridDim.x * 256) v[r] = w[r] * sc;
+    }
+}
+
+// Column 0 of every running body's basis: v_0 = r / ||r||, and g = (||r||, 0, ...).
+__global__ __launch_bounds__(256) void ens_open_kernel(EnsArgs e, const double* __restrict__ r,
+                                                       const double* __restrict__ rnorm) {
+    const int b = blockIdx.y;
+    if (e.stop[b]) return;
+    const double rn = rnorm[b];
+    const int64_t o = e.koff[b], n = e.koff[b + 1] - o;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        e.V[o + i] = r[o + i] / rn;
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < e.mstride; i += 256) e.g[(int64_t)b * e.mstride + i] = i == 0 ? rn : 0.0;
+}
+
+// dst[map[i]] = src[i] (to_operator) or dst[i] = src[map[i]].
+__global__ __launch_bounds__(256) void ens_permute_kernel(const double* __restrict__ src, double* __restrict__ dst,
+                                                          const int32_t* __restrict__ map, int64_t n, int to_operator) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        if (to_operator) dst[map[i]] = src[i];
+        else dst[i] = src[map[i]];
+    }
+}
+
+// out[b] = ||v_b|| over body b's rows (one workgroup per body, fixed order).
+__global__ __launch_bounds__(256) void ens_norms_kernel(const double* __restrict__ v, const int64_t* __restrict__ koff,
+                                                        double* __restrict__ out) {
+    __shared__ double sh[4];
+    const int b = blockIdx.x;
+    double s = 0.0;
+    for (int64_t i = koff[b] + threadIdx.x; i < koff[b + 1]; i += 256) s += v[i] * v[i];
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) out[b] = __builtin_sqrt(s);
+}
+
+// Iterations of body b in the cycle just run: the stopping step, all m if it
+// ran without stopping, none if it did not take part (or failed).
+__device__ __forceinline__ int ens_steps(int32_t stop, int m) {
+    return stop == 0 ? m : (stop > 0 && stop <= m ? stop : 0);
+}
+
+// Per body: H(0:k, 0:k) y = g by columns, the triangle staged in LDS (one
+// workgroup per body); kdone[b] = k.
+__global__ __launch_bounds__(256) void ens_trisolve_kernel(EnsArgs e, int m, double* __restrict__ y_all,
+                                                           int32_t* __restrict__ kdone) {
+    extern __shared__ double tri[];
+    __shared__ double yi;
+    const int b = blockIdx.x;
+    const int k = ens_steps(e.stop[b], m);
+    if (threadIdx.x == 0) kdone[b] = k;
+    const double* H = e.H + (int64_t)b * e.hstride;
+    const double* g = e.g + (int64_t)b * e.mstride;
+    double* y = y_all + (int64_t)b * e.mstride;
+    double* rhs = tri + (int64_t)m * (m + 1) / 2;
+    for (int j = threadIdx.x; j < k; j += 256) rhs[j] = g[j];
+    for (int q = threadIdx.x; q < k * k; q += 256) {
+        const int i = q / k, j = q - i * k;
+        if (j <= i) tri[i * (i + 1) / 2 + j] = H[(int64_t)i * e.ldh + j];
+    }
+    __syncthreads();
+    for (int i = k - 1; i >= 0; --i) {
+        const double* col = tri + (int64_t)i * (i + 1) / 2;
+        if (threadIdx.x == 0) {
+            yi = rhs[i] / col[i];
+            y[i] = yi;
+        }
+        __syncthreads();
+        const double v = yi;
+        for (int j = threadIdx.x; j < i; j += 256) rhs[j] -= col[j] * v;
+        __syncthreads();
+    }
+}
+
+// out_b = V_b(:, 0:k_b) y_b for every body (zero rows where k_b = 0).
+__global__ __launch_bounds__(256) void ens_gemv_kernel(EnsArgs e, const double* __restrict__ y_all,
+                                                       const int32_t* __restrict__ kdone, double* __restrict__ out) {
+    const int b = blockIdx.y;
+    const int k = kdone[b];
+    const int64_t o = e.koff[b], n = e.koff[b + 1] - o;
+    const double* y = y_all + (int64_t)b * e.mstride;
+    for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += (int64_t)gridDim.x * 256) {
+        double s = 0.0;
+        for (int c = 0; c < k; ++c) s += e.V[(int64_t)c * e.ld + o + r] * y[c];
+        out[o + r] = s;
+    }
+}
+
+// Per body with flag[b] set: x = 0 and r = rhs over its rows (map: Krylov
+// index -> operator index): the zero-rhs and rubbish-guess rules per body.
+__global__ __launch_bounds__(256) void ens_reset_kernel(const int32_t* __restrict__ flag,
+                                                        const int64_t* __restrict__ koff, int nbody,
+                                                        const int32_t* __restrict__ map, double* __restrict__ x,
+                                                        double* __restrict__ r, const double* __restrict__ rhs) {
+    const int b = blockIdx.y;
+    if (b >= nbody || !flag[b]) return;
+    for (int64_t i = koff[b] + (int64_t)blockIdx.x * 256 + threadIdx.x; i < koff[b + 1]; i += (int64_t)gridDim.x * 256) {
+        const int32_t o = map[i];
+        x[o] = 0.0;
+        if (r) r[o] = rhs[o];
+    }
+}
+
+// stamps[b] = now for every body (the start of a timestep).
+__global__ void ens_stamp_kernel(int64_t* __restrict__ stamps, int nbody) {
+    const int64_t t = (int64_t)wall_clock64();
+    for (int b = threadIdx.x; b < nbody; b += blockDim.x) stamps[b] = t;
+}
+
 }  // namespace
 
 static inline int blocks_for(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
@@ -1447,6 +1637,54 @@ hipError_t k_body_pose(const double* pose, const double* base_ref, const double*
 
 hipError_t k_body_advance(const double* sol, int64_t off_body, double dt, double* pose, hipStream_t s) {
     hipLaunchKernelGGL(body_advance_kernel, dim3(1), dim3(64), 0, s, sol, off_body, dt, pose);
+    return hipGetLastError();
+}
+
+// ---- independent bodies in lock-step ----------------------------------------
+hipError_t k_ens_arnoldi_step(const EnsArgs& e, int j, int G, hipStream_t s) {
+    if (j + 1 > kMaxBasis) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)G, (unsigned)e.nbody);
+    hipLaunchKernelGGL(ens_dots_kernel, grid, dim3(256), 0, s, e, j + 1);
+    hipLaunchKernelGGL(ens_update_dots_kernel, grid, dim3(256), 0, s, e, j + 1, j);
+    hipLaunchKernelGGL(ens_update_sumsq_kernel, grid, dim3(256), 0, s, e, j + 1, j);
+    return hipGetLastError();
+}
+hipError_t k_ens_close(const EnsArgs& e, int j, bool write_next, int G, hipStream_t s) {
+    hipLaunchKernelGGL(ens_close_kernel, dim3((unsigned)G, (unsigned)e.nbody), dim3(256), 0, s, e, j,
+                       write_next ? 1 : 0);
+    return hipGetLastError();
+}
+hipError_t k_ens_open(const EnsArgs& e, const double* r, const double* rnorm, int G, hipStream_t s) {
+    hipLaunchKernelGGL(ens_open_kernel, dim3((unsigned)G, (unsigned)e.nbody), dim3(256), 0, s, e, r, rnorm);
+    return hipGetLastError();
+}
+hipError_t k_ens_permute(const double* src, double* dst, const int32_t* map, int64_t n, bool to_operator,
+                         hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ens_permute_kernel, dim3((unsigned)std::min<int64_t>(blocks_for(n), 4096)), dim3(256), 0, s,
+                       src, dst, map, n, to_operator ? 1 : 0);
+    return hipGetLastError();
+}
+hipError_t k_ens_norms(const double* v, const int64_t* koff, int nbody, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(ens_norms_kernel, dim3((unsigned)nbody), dim3(256), 0, s, v, koff, out);
+    return hipGetLastError();
+}
+hipError_t k_ens_update(const EnsArgs& e, int m, double* y, int32_t* kdone, double* out, int G, hipStream_t s) {
+    const size_t shmem = sizeof(double) * ((size_t)m * (m + 1) / 2 + m);
+    if (shmem > 60 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(ens_trisolve_kernel, dim3((unsigned)e.nbody), dim3(256), shmem, s, e, m, y, kdone);
+    hipLaunchKernelGGL(ens_gemv_kernel, dim3((unsigned)G, (unsigned)e.nbody), dim3(256), 0, s, e, (const double*)y,
+                       (const int32_t*)kdone, out);
+    return hipGetLastError();
+}
+hipError_t k_ens_reset(const int32_t* flag, const int64_t* koff, int nbody, const int32_t* map, double* x, double* r,
+                       const double* rhs, int G, hipStream_t s) {
+    hipLaunchKernelGGL(ens_reset_kernel, dim3((unsigned)G, (unsigned)nbody), dim3(256), 0, s, flag, koff, nbody, map, x,
+                       r, rhs);
+    return hipGetLastError();
+}
+hipError_t k_ens_stamp(int64_t* stamps, int nbody, hipStream_t s) {
+    hipLaunchKernelGGL(ens_stamp_kernel, dim3(1), dim3(256), 0, s, stamps, nbody);
     return hipGetLastError();
 }
 

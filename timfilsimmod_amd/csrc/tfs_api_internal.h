@@ -116,6 +116,9 @@ TileShape default_shape(int precision, int ns, int nb, int num_cus);
 Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err);
 // Positions changed: the near-pair lists must be rebuilt before the next matvec.
 void invalidate_near(tfs_rpy* h);
+// Builds the near-pair list of `precision` (and, for FP32, the fixed-point
+// positions) if stale; allocates and synchronises, so never under capture.
+int ensure_near(tfs_rpy* h, int precision);
 
 // Matvec with explicit views: forces fs (seg_dof comps) / fb (3 comps) and
 // outputs vs (seg_out_slots comps) / vb (3 comps), with the write rules of

@@ -54,6 +54,44 @@ hipError_t k_fill(double* p, double v, int64_t n, hipStream_t s) {
 
 }  // namespace
 
+// Independent bodies solved in lock-step (tfs_solver_set_independent_bodies):
+// one GMRES per body, every batched launch covering all bodies.  The operator
+// (matvec, couplings, preconditioner) works on the usual layout over all
+// bodies -- [lambda_s | lambda_b | phi_dot | theta_dot], each part body after
+// body -- and the Krylov vectors live in the bodies' own layout, body b's
+// unknowns contiguous at koff[b]; map[i] is the operator index of Krylov
+// index i.
+struct EnsState {
+    int nbody = 0;
+    std::vector<int32_t> fil, blob;   // per body
+    std::vector<int64_t> koff;        // nbody + 1
+    int64_t ntot = 0;
+    int G = 1;                        // workgroups per body of the 2-D launches
+    void* arena = nullptr;
+    int64_t* d_koff = nullptr;
+    int32_t *d_map = nullptr, *d_fil_body = nullptr;
+    double *V = nullptr, *w = nullptr, *tk = nullptr, *rk = nullptr;
+    int64_t ldv = 0;
+    double *H = nullptr, *cs = nullptr, *sn = nullptr, *g = nullptr, *est = nullptr, *y = nullptr, *scal = nullptr;
+    double *abs_tol = nullptr, *norms = nullptr, *pa = nullptr, *pb = nullptr;
+    int64_t hstride = 0, pstride = 0;
+    int mstride = 0;
+    int32_t *stop = nullptr, *nstop = nullptr, *kdone = nullptr, *flag = nullptr;
+    int64_t* stamps = nullptr;        // nbody + 1: per-body last stop, then the step's start
+    int32_t* h_i32 = nullptr;         // pinned: 2 nbody + 8
+    double* h_f64 = nullptr;          // pinned: nbody
+    int64_t* h_stamps = nullptr;      // pinned: nbody + 1
+    // per-body results of the last solve
+    std::vector<int32_t> iters, conv;
+    std::vector<double> relres;
+    // captured cycles (one per precision) and the pointers / values they hold
+    bool use_graphs = true;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    std::vector<uintptr_t> key[2];
+    int64_t graph_launches = 0, eager_cycles = 0;
+};
+
 struct tfs_solver {
     tfs_solver_desc desc{};
     SaddleDims dims{};
@@ -93,6 +131,7 @@ struct tfs_solver {
     double *x_bodies = nullptr, *bodyZ = nullptr, *body_sinv = nullptr, *body_t = nullptr;
     bool bodies_set = false, body_pc_valid = false;
     tfs::F32Stage f32;  // FP32 entry points (tfs_solver_*_f32)
+    EnsState* ens = nullptr;  // independent bodies in lock-step (null: one system)
 };
 
 namespace {
@@ -133,6 +172,7 @@ int apply_A(tfs_solver* s, const double* in, double* out, int precision, tfs_sol
         sad.dyn_angle = dm.dyn_angle;
         sad.off_phase = dm.off_phase;
         sad.off_angle = dm.off_angle;
+        if (s->ens) sad.fil_body = s->ens->d_fil_body;
         if (dm.free_body) {
             sad.x_bodies = s->x_bodies;
             sad.free_body = 1;
@@ -249,34 +289,48 @@ int ensure_block_precond(tfs_solver* s) {
         hipError_t e = hipMemcpyAsync(xb.data(), s->rpy->x_blobs, xb.size() * 8, hipMemcpyDeviceToHost, s->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
         if (e != hipSuccess) return hip_fail(e, "block preconditioner: blob positions");
-        double lo[3], hi[3], span[3];
-        for (int k = 0; k < 3; ++k) lo[k] = hi[k] = xb[k];
-        for (int i = 0; i < d.nblob; ++i)
-            for (int k = 0; k < 3; ++k) {
-                lo[k] = std::min(lo[k], xb[3 * i + k]);
-                hi[k] = std::max(hi[k], xb[3 * i + k]);
-            }
-        for (int k = 0; k < 3; ++k) span[k] = hi[k] - lo[k];
-        std::vector<std::pair<uint32_t, int32_t>> key(d.nblob);
-        for (int i = 0; i < d.nblob; ++i) key[i] = {morton3(&xb[3 * i], lo, span), i};
+        // Morton order within each body (independent bodies: their own boxes,
+        // and no patch holds blobs of two bodies)
+        std::vector<int32_t> first(1, 0);
+        if (s->ens)
+            for (int32_t c : s->ens->blob) first.push_back(first.back() + c);
+        else
+            first.push_back(d.nblob);
+        std::vector<std::pair<uint64_t, int32_t>> key(d.nblob);
+        for (size_t bi = 0; bi + 1 < first.size(); ++bi) {
+            const int i0 = first[bi], i1 = first[bi + 1];
+            if (i1 <= i0) continue;
+            double lo[3], hi[3], span[3];
+            for (int k = 0; k < 3; ++k) lo[k] = hi[k] = xb[3 * (size_t)i0 + k];
+            for (int i = i0; i < i1; ++i)
+                for (int k = 0; k < 3; ++k) {
+                    lo[k] = std::min(lo[k], xb[3 * i + k]);
+                    hi[k] = std::max(hi[k], xb[3 * i + k]);
+                }
+            for (int k = 0; k < 3; ++k) span[k] = hi[k] - lo[k];
+            for (int i = i0; i < i1; ++i) key[i] = {((uint64_t)bi << 32) | morton3(&xb[3 * i], lo, span), i};
+        }
         std::stable_sort(key.begin(), key.end(),
-                         [](const std::pair<uint32_t, int32_t>& a, const std::pair<uint32_t, int32_t>& b) {
+                         [](const std::pair<uint64_t, int32_t>& a, const std::pair<uint64_t, int32_t>& b) {
                              return a.first < b.first;
                          });
-        const int np = (d.nblob + kPatchBlobs - 1) / kPatchBlobs;
-        std::vector<int32_t> blob(d.nblob), start(np + 1), size(np), pivoff(np);
-        std::vector<int64_t> off(np);
+        // patches of kPatchBlobs consecutive blobs, a new one at every body
+        std::vector<int32_t> blob(d.nblob), start, size, pivoff;
+        std::vector<int64_t> off;
         int64_t total = 0;
         for (int i = 0; i < d.nblob; ++i) blob[i] = key[i].second;
-        for (int p = 0; p < np; ++p) {
-            start[p] = p * kPatchBlobs;
-            const int cnt = std::min(kPatchBlobs, d.nblob - start[p]);
-            size[p] = 3 * cnt;
-            off[p] = total;
-            pivoff[p] = 3 * start[p];
-            total += (int64_t)size[p] * size[p];
+        for (int i = 0; i < d.nblob;) {
+            int j = i + 1;
+            while (j < d.nblob && j - i < kPatchBlobs && (key[j].first >> 32) == (key[i].first >> 32)) ++j;
+            start.push_back(i);
+            size.push_back(3 * (j - i));
+            off.push_back(total);
+            pivoff.push_back(3 * i);
+            total += (int64_t)size.back() * size.back();
+            i = j;
         }
-        start[np] = d.nblob;
+        const int np = (int)start.size();
+        start.push_back(d.nblob);
         if (s->npatch != np) {
             for (void* q : {(void*)s->patch_inv, (void*)s->patch_off, (void*)s->patch_blob, (void*)s->patch_start,
                             (void*)s->patch_size, (void*)s->patch_piv, (void*)s->patch_pivoff, (void*)s->patch_status})
@@ -549,6 +603,338 @@ int gmres(tfs_solver* s, int precision, const double* b, double* x, double* r, d
     return TFS_OK;
 }
 
+// ---- independent bodies in lock-step --------------------------------------
+
+EnsArgs ens_args(tfs_solver* s) {
+    EnsState& E = *s->ens;
+    EnsArgs a;
+    a.koff = E.d_koff;
+    a.nbody = E.nbody;
+    a.V = E.V;
+    a.ld = E.ldv;
+    a.w = E.w;
+    a.H = E.H;
+    a.ldh = s->m + 1;
+    a.hstride = E.hstride;
+    a.cs = E.cs;
+    a.sn = E.sn;
+    a.g = E.g;
+    a.est = E.est;
+    a.mstride = E.mstride;
+    a.scal = E.scal;
+    a.stop = E.stop;
+    a.nstop = E.nstop;
+    a.abs_tol = E.abs_tol;
+    a.pa = E.pa;
+    a.pb = E.pb;
+    a.pstride = E.pstride;
+    a.stamps = E.stamps;
+    return a;
+}
+
+// Per-body norms of an operator-layout vector, read back into norms[].
+int ens_norms(tfs_solver* s, const double* v, std::vector<double>& norms) {
+    EnsState& E = *s->ens;
+    int rc = check(k_ens_permute(v, E.rk, E.d_map, E.ntot, false, s->stream), "per-body norms");
+    if (!rc) rc = check(k_ens_norms(E.rk, E.d_koff, E.nbody, E.norms, s->stream), "per-body norms");
+    if (!rc) rc = check(hipMemcpyAsync(E.h_f64, E.norms, 8LL * E.nbody, hipMemcpyDeviceToHost, s->stream), "per-body norms");
+    if (!rc) rc = check(hipStreamSynchronize(s->stream), "per-body norms");
+    if (rc) return rc;
+    norms.assign(E.h_f64, E.h_f64 + E.nbody);
+    return TFS_OK;
+}
+
+// Lock-step steps j0 <= j < j1 of the current cycle (no host round trip):
+// w = A z with the stopped bodies' work skipped, w to the Krylov layout, the
+// CGS2 step and close of every running body, v_{j+1} and z = P^{-1} v_{j+1}.
+int ens_enqueue_steps(tfs_solver* s, int precision, int j0, int j1) {
+    EnsState& E = *s->ens;
+    const EnsArgs a = ens_args(s);
+    const int m = s->m;
+    for (int j = j0; j < j1; ++j) {
+        s->rpy->body_skip = E.stop;
+        int rc = apply_A(s, s->z, s->w, precision, nullptr);
+        s->rpy->body_skip = nullptr;
+        if (!rc) rc = check(k_ens_permute(s->w, E.w, E.d_map, E.ntot, false, s->stream), "w to the Krylov layout");
+        if (!rc) rc = check(k_ens_arnoldi_step(a, j, E.G, s->stream), "lock-step Arnoldi");
+        if (!rc) rc = check(k_ens_close(a, j, j + 1 < m, E.G, s->stream), "lock-step close");
+        if (!rc && j + 1 < m) {
+            rc = check(k_ens_permute(E.V + (int64_t)(j + 1) * E.ldv, s->t, E.d_map, E.ntot, true, s->stream),
+                       "v to the operator layout");
+            if (!rc) rc = check(k_precond(s->t, s->z, precond_args(s), s->dims, s->stream), "preconditioner");
+        }
+        if (rc) return rc;
+    }
+    return TFS_OK;
+}
+
+// The m steps of a cycle as one hipGraph, captured again only when something
+// the launches hold by value or pointer changed (the work list, partial
+// buffer, near-pair lists, fixed-point positions and their scale).
+int ens_cycle_graph(tfs_solver* s, int precision) {
+    EnsState& E = *s->ens;
+    tfs_rpy* h = s->rpy;
+    int rc = ensure_near(h, precision);
+    if (rc) return rc;
+    int err = TFS_OK;
+    const Plan* p = get_plan(h, TFS_BLOCK_ALL, h->shape[precision], &err);
+    if (!p) return err;
+    uint64_t lam_bits = 0;
+    std::memcpy(&lam_bits, &h->fx_lambda, 8);
+    const std::vector<uintptr_t> key = {(uintptr_t)p->d_tiles,
+                                        (uintptr_t)p->d_items,
+                                        (uintptr_t)h->partial,
+                                        (uintptr_t)h->near[precision].cnt,
+                                        (uintptr_t)h->near[precision].idx,
+                                        (uintptr_t)h->near[precision].cap,
+                                        (uintptr_t)h->x_fx,
+                                        (uintptr_t)(precision == 1 ? lam_bits : 0),
+                                        (uintptr_t)h->shape[precision].block,
+                                        (uintptr_t)h->shape[precision].per_thread,
+                                        (uintptr_t)h->shape[precision].chunk,
+                                        (uintptr_t)h->shape[precision].persistent,
+                                        (uintptr_t)s->precond_kind,
+                                        (uintptr_t)s->npatch};
+    if (E.exec[precision] && E.key[precision] == key) return TFS_OK;
+    if (E.exec[precision]) (void)hipGraphExecDestroy(E.exec[precision]);
+    if (E.graph[precision]) (void)hipGraphDestroy(E.graph[precision]);
+    E.exec[precision] = nullptr;
+    E.graph[precision] = nullptr;
+    const bool prof = h->profiling, timing = h->body_timing;
+    h->profiling = false;
+    h->body_timing = false;
+    hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        h->profiling = prof;
+        h->body_timing = timing;
+        return hip_fail(e, "lock-step cycle: begin capture");
+    }
+    rc = ens_enqueue_steps(s, precision, 0, s->m);
+    hipError_t e2 = hipStreamEndCapture(s->stream, &E.graph[precision]);
+    h->profiling = prof;
+    h->body_timing = timing;
+    if (rc || e2 != hipSuccess) {
+        if (E.graph[precision]) (void)hipGraphDestroy(E.graph[precision]);
+        E.graph[precision] = nullptr;
+        return rc ? rc : hip_fail(e2, "lock-step cycle: end capture");
+    }
+    e = hipGraphInstantiate(&E.exec[precision], E.graph[precision], nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(E.graph[precision]);
+        E.graph[precision] = nullptr;
+        E.exec[precision] = nullptr;
+        return hip_fail(e, "lock-step cycle: instantiate");
+    }
+    E.key[precision] = key;
+    return TFS_OK;
+}
+
+// GMRES(m) cycles of every body with active[b] on A x = rhs (matvecs in
+// `precision`), each to its own abs_tol[b]; r holds rhs - A x on entry and is
+// work space after.  x (operator layout) is updated in place; est[b] gets the
+// last residual estimate of a body that ran, iters[b] its steps.
+int gmres_batched(tfs_solver* s, int precision, const double* rhs, double* x, double* r, std::vector<char>& active,
+                  const std::vector<double>& abs_tol, std::vector<double>& est, int max_cycles, tfs_solve_stats* st) {
+    EnsState& E = *s->ens;
+    const int nb = E.nbody, m = s->m;
+    for (int b = 0; b < nb; ++b) E.h_f64[b] = abs_tol[b];
+    int rc = check(hipMemcpyAsync(E.abs_tol, E.h_f64, 8LL * nb, hipMemcpyHostToDevice, s->stream), "tolerances");
+    if (!rc) rc = check(hipStreamSynchronize(s->stream), "tolerances");
+    if (rc) return rc;
+    const EnsArgs a = ens_args(s);
+    std::vector<double> rn;
+    std::vector<double> est_all((size_t)nb * E.mstride);
+    std::vector<int32_t> kd(nb), stp(nb);
+    for (int cyc = 0; cyc < max_cycles; ++cyc) {
+        if (cyc > 0) {
+            // the true residual of the bodies still running
+            for (int b = 0; b < nb; ++b) E.h_i32[b] = active[b] ? 0 : 1;
+            rc = check(hipMemcpyAsync(E.flag, E.h_i32, 4LL * nb, hipMemcpyHostToDevice, s->stream), "active bodies");
+            s->rpy->body_skip = E.flag;
+            if (!rc) rc = residual(s, rhs, x, false, r, precision, st);
+            s->rpy->body_skip = nullptr;
+            if (rc) return rc;
+        }
+        rc = ens_norms(s, r, rn);
+        if (rc) return rc;
+        int n_active = 0;
+        for (int b = 0; b < nb; ++b) {
+            if (!active[b]) continue;
+            if (!std::isfinite(rn[b])) return fail(TFS_ERR_NON_FINITE, "GMRES: non-finite residual of body " + std::to_string(b));
+            est[b] = rn[b];
+            if (rn[b] <= abs_tol[b]) active[b] = 0;
+            else ++n_active;
+        }
+        if (!n_active) break;
+        st->cycles++;
+        // open the cycle: stop flags, H = 0, v_0 = r / ||r||, z = P^{-1} v_0
+        for (int b = 0; b < nb; ++b) E.h_i32[b] = active[b] ? 0 : kEnsInactive;
+        rc = check(hipMemcpyAsync(E.stop, E.h_i32, 4LL * nb, hipMemcpyHostToDevice, s->stream), "stop flags");
+        if (!rc) rc = check(k_fill_int(E.nstop, 0, 1, s->stream), "stop count");
+        if (!rc) rc = check(k_fill(E.H, 0.0, (int64_t)nb * E.hstride, s->stream), "H");
+        if (!rc) rc = check(k_ens_open(a, E.rk, E.norms, E.G, s->stream), "v0");
+        if (!rc) rc = check(k_ens_permute(E.V, s->t, E.d_map, E.ntot, true, s->stream), "v0 to the operator layout");
+        if (!rc) rc = check(k_precond(s->t, s->z, precond_args(s), s->dims, s->stream), "preconditioner");
+        if (rc) return rc;
+        int launched = m;
+        if (E.use_graphs) {
+            rc = ens_cycle_graph(s, precision);
+            if (!rc) rc = check(hipGraphLaunch(E.exec[precision], s->stream), "lock-step cycle graph");
+            if (rc) return rc;
+            E.graph_launches++;
+        } else {
+            // steps enqueued four at a time; stop once every running body stopped
+            E.eager_cycles++;
+            launched = 0;
+            for (int j0 = 0; j0 < m; j0 += 4) {
+                const int j1 = std::min(m, j0 + 4);
+                rc = ens_enqueue_steps(s, precision, j0, j1);
+                if (!rc) rc = check(hipMemcpyAsync(E.h_i32 + nb, E.nstop, 4, hipMemcpyDeviceToHost, s->stream), "stop count");
+                if (!rc) rc = check(hipStreamSynchronize(s->stream), "stop count");
+                if (rc) return rc;
+                launched = j1;
+                if (E.h_i32[nb] >= n_active) break;
+            }
+        }
+        // x += P^{-1} V y per body
+        rc = check(k_ens_update(a, m, E.y, E.kdone, E.tk, E.G, s->stream), "lock-step update");
+        if (!rc) rc = check(k_ens_permute(E.tk, s->t, E.d_map, E.ntot, true, s->stream), "update to the operator layout");
+        if (!rc) rc = check(k_precond(s->t, s->z, precond_args(s), s->dims, s->stream), "preconditioner");
+        if (!rc) rc = check(k_axpby(1.0, s->z, 1.0, x, (int)s->dims.n, s->stream), "x += d");
+        if (!rc) rc = check(hipMemcpyAsync(kd.data(), E.kdone, 4LL * nb, hipMemcpyDeviceToHost, s->stream), "cycle steps");
+        if (!rc) rc = check(hipMemcpyAsync(stp.data(), E.stop, 4LL * nb, hipMemcpyDeviceToHost, s->stream), "cycle stops");
+        if (!rc) rc = check(hipMemcpyAsync(est_all.data(), E.est, 8 * est_all.size(), hipMemcpyDeviceToHost, s->stream), "estimates");
+        if (!rc) rc = check(hipStreamSynchronize(s->stream), "cycle results");
+        if (rc) return rc;
+        int steps = 0;
+        for (int b = 0; b < nb; ++b) {
+            if (!active[b]) continue;
+            if (stp[b] < 0) return fail(TFS_ERR_NON_FINITE, "GMRES: non-finite residual estimate of body " + std::to_string(b));
+            const int k = std::min(kd[b], launched);
+            E.iters[b] += k;
+            steps = std::max(steps, k);
+            if (k > 0) est[b] = est_all[(size_t)b * E.mstride + k - 1];
+            if (est[b] <= abs_tol[b]) active[b] = 0;
+        }
+        st->iterations += steps;
+        if (precision == TFS_PRECISION_FP64) st->fp64_matvecs += launched;
+        else st->fp32_matvecs += launched;
+    }
+    return TFS_OK;
+}
+
+// solve_device for independent bodies: every body's solve follows the
+// single-system rules (zero rhs, good warm start, rubbish guess, mixed
+// precision refinement) with its own norms and tolerances.
+int solve_batched(tfs_solver* s, const double* b, double* x, tfs_solve_stats* st) {
+    EnsState& E = *s->ens;
+    const tfs_solver_desc& D = s->desc;
+    const int nb = E.nbody;
+    const int n = (int)s->dims.n;
+    *st = tfs_solve_stats{};
+    E.iters.assign(nb, 0);
+    // stamps[0..nb) = each body's last Arnoldi stop, stamps[nb] = the start
+    int rc0 = check(k_ens_stamp(E.stamps, nb + 1, s->stream), "solve start");
+    if (rc0) return rc0;
+    E.conv.assign(nb, 0);
+    E.relres.assign(nb, 0.0);
+    int rc = ensure_block_precond(s);
+    if (rc) return rc;
+    std::vector<double> bn, rn;
+    rc = ens_norms(s, b, bn);
+    if (rc) return rc;
+    std::vector<char> active(nb, 1);
+    std::vector<double> abs_tol(nb), est(nb, 0.0);
+    for (int i = 0; i < nb; ++i) {
+        if (!std::isfinite(bn[i])) return fail(TFS_ERR_NON_FINITE, "solve: non-finite right-hand side of body " + std::to_string(i));
+        abs_tol[i] = D.tol * bn[i];
+        E.h_i32[i] = bn[i] == 0.0 ? 1 : 0;  // zero rhs: zero solution (mobility_solver.cpp:2127-2194)
+    }
+    rc = check(hipMemcpyAsync(E.flag, E.h_i32, 4LL * nb, hipMemcpyHostToDevice, s->stream), "zero rhs");
+    if (!rc) rc = check(k_ens_reset(E.flag, E.d_koff, nb, E.d_map, x, nullptr, b, E.G, s->stream), "zero rhs");
+    if (!rc) rc = residual(s, b, x, false, s->r, TFS_PRECISION_FP64, st);
+    if (!rc) rc = ens_norms(s, s->r, rn);
+    if (rc) return rc;
+    int any_reset = 0;
+    for (int i = 0; i < nb; ++i) {
+        E.h_i32[i] = 0;
+        if (bn[i] == 0.0) {
+            active[i] = 0;
+            rn[i] = 0.0;
+            continue;
+        }
+        if (!std::isfinite(rn[i])) return fail(TFS_ERR_NON_FINITE, "solve: non-finite initial residual of body " + std::to_string(i));
+        if (rn[i] <= abs_tol[i]) active[i] = 0;        // good warm start (:2304-2308)
+        else if (rn[i] > bn[i]) {                      // rubbish initial guess (:2310-2317)
+            E.h_i32[i] = 1;
+            rn[i] = bn[i];
+            any_reset = 1;
+        }
+    }
+    if (any_reset) {
+        rc = check(hipMemcpyAsync(E.flag, E.h_i32, 4LL * nb, hipMemcpyHostToDevice, s->stream), "reset");
+        if (!rc) rc = check(k_ens_reset(E.flag, E.d_koff, nb, E.d_map, x, s->r, b, E.G, s->stream), "reset");
+        if (rc) return rc;
+    }
+    double init = 0.0;
+    for (int i = 0; i < nb; ++i) init = std::max(init, bn[i] > 0 ? rn[i] / bn[i] : 0.0);
+    st->initial_relres = init;
+    if (D.mode == TFS_SOLVE_FP64) {
+        rc = gmres_batched(s, TFS_PRECISION_FP64, b, x, s->r, active, abs_tol, est, D.max_cycles, st);
+        if (rc) return rc;
+        for (int i = 0; i < nb; ++i)
+            if (bn[i] > 0.0 && E.iters[i] > 0) rn[i] = est[i];
+    } else {
+        std::vector<double> inner(nb);
+        for (int it = 0; it < D.max_refinements; ++it) {
+            std::vector<char> act(nb, 0);
+            int n_act = 0;
+            for (int i = 0; i < nb; ++i) {
+                act[i] = bn[i] > 0.0 && rn[i] > abs_tol[i];
+                n_act += act[i];
+                inner[i] = std::max(D.inner_tol * rn[i], 0.5 * abs_tol[i]);
+            }
+            if (!n_act) break;
+            st->refinements++;
+            rc = check(hipMemcpyAsync(s->rin, s->r, (size_t)n * 8, hipMemcpyDeviceToDevice, s->stream), "inner rhs");
+            if (!rc) rc = check(k_fill(s->d, 0.0, n, s->stream), "d = 0");
+            if (!rc) rc = gmres_batched(s, TFS_PRECISION_FP32_ACC64, s->rin, s->d, s->r, act, inner, est, D.max_cycles, st);
+            if (!rc) rc = check(k_axpby(1.0, s->d, 1.0, x, n, s->stream), "x += d");
+            if (!rc) rc = residual(s, b, x, false, s->r, TFS_PRECISION_FP64, st);
+            if (!rc) rc = ens_norms(s, s->r, rn);
+            if (rc) return rc;
+            for (int i = 0; i < nb; ++i)
+                if (!std::isfinite(rn[i])) return fail(TFS_ERR_NON_FINITE, "solve: non-finite residual of body " + std::to_string(i));
+        }
+    }
+    bool all = true;
+    double worst = 0.0;
+    for (int i = 0; i < nb; ++i) {
+        E.relres[i] = bn[i] > 0.0 ? rn[i] / bn[i] : 0.0;
+        E.conv[i] = bn[i] == 0.0 || rn[i] <= abs_tol[i];
+        all = all && E.conv[i];
+        worst = std::max(worst, E.relres[i]);
+    }
+    st->converged = all ? 1 : 0;
+    st->relres = worst;
+    return all ? TFS_OK : fail(TFS_ERR_NOT_CONVERGED, "solve: tolerance not met by every body");
+}
+
+void free_ens(tfs_solver* s) {
+    EnsState* E = s->ens;
+    if (!E) return;
+    for (int p = 0; p < 2; ++p) {
+        if (E->exec[p]) (void)hipGraphExecDestroy(E->exec[p]);
+        if (E->graph[p]) (void)hipGraphDestroy(E->graph[p]);
+    }
+    if (E->arena) (void)hipFree(E->arena);
+    if (E->h_i32) (void)hipHostFree(E->h_i32);
+    if (E->h_f64) (void)hipHostFree(E->h_f64);
+    if (E->h_stamps) (void)hipHostFree(E->h_stamps);
+    delete E;
+    s->ens = nullptr;
+}
+
 int alloc_solver(tfs_solver* s) {
     const int64_t n = s->dims.n;
     s->ldv = ((n + 31) / 32) * 32;
@@ -599,6 +985,7 @@ hipMemcpyKind kind_out(int32_t memory) {
 
 // Core solve on device vectors b (rhs) and x (warm start in, solution out).
 int solve_device(tfs_solver* s, const double* b, double* x, tfs_solve_stats* st) {
+    if (s->ens) return solve_batched(s, b, x, st);
     const int n = (int)s->dims.n;
     const tfs_solver_desc& D = s->desc;
     *st = tfs_solve_stats{};
@@ -755,6 +1142,7 @@ int tfs_solver_destroy(tfs_solver* s) {
     if (s->host_scal) hipHostFree(s->host_scal);
     if (s->host_flag) hipHostFree(s->host_flag);
     free_stage(s->f32);
+    free_ens(s);
     for (hipEvent_t ev : s->flag_ev)
         if (ev) hipEventDestroy(ev);
     if (s->rpy) tfs_rpy_destroy(s->rpy);
@@ -1003,6 +1391,144 @@ int tfs_solver_residual_f32(tfs_solver* s, const float* rhs, const float* x, int
     if (!rc) rc = norm(s, s->b, &bn);
     if (rc) return rc;
     *relres = bn > 0 ? rn / bn : rn;
+    return TFS_OK;
+}
+
+int tfs_solver_set_independent_bodies(tfs_solver* s, int32_t num_bodies, const int32_t* fil_per_body,
+                                      const int32_t* blobs_per_body) {
+    g_last_error.clear();
+    if (!s || num_bodies < 1 || !fil_per_body || !blobs_per_body)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_independent_bodies: bad arguments");
+    const SaddleDims& d = s->dims;
+    if (d.free_body) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_independent_bodies: not with free bodies");
+    int64_t nf = 0, nbl = 0;
+    for (int b = 0; b < num_bodies; ++b) {
+        if (fil_per_body[b] < 0 || blobs_per_body[b] < 0 || fil_per_body[b] + blobs_per_body[b] == 0)
+            return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_independent_bodies: every body needs particles");
+        nf += fil_per_body[b];
+        nbl += blobs_per_body[b];
+    }
+    if (nf != d.nfil || nbl != d.nblob)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_independent_bodies: counts do not add up to the solver's sizes");
+    const int m = s->m;
+    // the per-body back substitution stages its triangle in LDS
+    if (num_bodies > 1 && m > 120)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_independent_bodies: restart above 120");
+    int rc = check(hipStreamSynchronize(s->stream), "tfs_solver_set_independent_bodies");
+    if (rc) return rc;
+    free_ens(s);
+    std::vector<int32_t> segs(num_bodies);
+    for (int b = 0; b < num_bodies; ++b) segs[b] = fil_per_body[b] * d.nseg;
+    rc = tfs_rpy_set_bodies(s->rpy, num_bodies, segs.data(), blobs_per_body);
+    s->blob_pc_valid = false;
+    if (rc || num_bodies == 1) return rc;  // one body: the single-system solve
+    EnsState* E = new EnsState();
+    s->ens = E;
+    E->nbody = num_bodies;
+    E->fil.assign(fil_per_body, fil_per_body + num_bodies);
+    E->blob.assign(blobs_per_body, blobs_per_body + num_bodies);
+    // Krylov layout of body b: its lambda_s rows, lambda_b rows, phase rows, angle rows
+    std::vector<int32_t> map, fil_body(std::max(d.nfil, 1), 0);
+    map.reserve(d.n);
+    E->koff.assign(1, 0);
+    int64_t max_n = 0;
+    for (int b = 0, f0 = 0, b0 = 0; b < num_bodies; ++b) {
+        for (int64_t i = 3LL * f0 * d.nseg; i < 3LL * (f0 + fil_per_body[b]) * d.nseg; ++i) map.push_back((int32_t)i);
+        for (int64_t i = 3LL * b0; i < 3LL * (b0 + blobs_per_body[b]); ++i) map.push_back((int32_t)(d.off_blob + i));
+        for (int f = f0; f < f0 + fil_per_body[b]; ++f) {
+            fil_body[f] = b;
+            if (d.dyn_phase) map.push_back((int32_t)(d.off_phase + f));
+        }
+        if (d.dyn_angle)
+            for (int f = f0; f < f0 + fil_per_body[b]; ++f) map.push_back((int32_t)(d.off_angle + f));
+        E->koff.push_back((int64_t)map.size());
+        max_n = std::max(max_n, E->koff[b + 1] - E->koff[b]);
+        f0 += fil_per_body[b];
+        b0 += blobs_per_body[b];
+    }
+    E->ntot = (int64_t)map.size();
+    if (E->ntot != d.n) {
+        free_ens(s);
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_independent_bodies: layout size");
+    }
+    const int nb = num_bodies;
+    E->G = arnoldi_blocks(max_n);
+    E->ldv = ((E->ntot + 31) / 32) * 32;
+    E->mstride = m + 1;
+    E->hstride = (int64_t)(m + 1) * m;
+    E->pstride = (int64_t)E->G * (m + 1);
+    // one device arena, every part 256-byte aligned
+    std::vector<std::pair<void**, int64_t>> parts = {
+        {(void**)&E->d_koff, 8LL * (nb + 1)}, {(void**)&E->d_map, 4 * E->ntot},
+        {(void**)&E->d_fil_body, 4LL * fil_body.size()}, {(void**)&E->V, 8 * E->ldv * (m + 1)},
+        {(void**)&E->w, 8 * E->ldv}, {(void**)&E->tk, 8 * E->ldv}, {(void**)&E->rk, 8 * E->ldv},
+        {(void**)&E->H, 8 * nb * E->hstride}, {(void**)&E->cs, 8LL * nb * E->mstride},
+        {(void**)&E->sn, 8LL * nb * E->mstride}, {(void**)&E->g, 8LL * nb * E->mstride},
+        {(void**)&E->est, 8LL * nb * E->mstride}, {(void**)&E->y, 8LL * nb * E->mstride},
+        {(void**)&E->scal, 16LL * nb}, {(void**)&E->abs_tol, 8LL * nb}, {(void**)&E->norms, 8LL * nb},
+        {(void**)&E->pa, 8 * nb * E->pstride}, {(void**)&E->pb, 8 * nb * E->pstride},
+        {(void**)&E->stop, 4LL * nb}, {(void**)&E->nstop, 4}, {(void**)&E->kdone, 4LL * nb},
+        {(void**)&E->flag, 4LL * nb}, {(void**)&E->stamps, 8LL * (nb + 1)}};
+    int64_t total = 0;
+    for (auto& q : parts) total += (q.second + 255) / 256 * 256;
+    hipError_t e = hipMalloc(&E->arena, total);
+    if (e != hipSuccess) {
+        free_ens(s);
+        return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_solver_set_independent_bodies: device memory");
+    }
+    char* p = (char*)E->arena;
+    for (auto& q : parts) {
+        *q.first = p;
+        p += (q.second + 255) / 256 * 256;
+    }
+    e = hipHostMalloc((void**)&E->h_i32, 4 * (2LL * nb + 8), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&E->h_f64, 8LL * nb, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&E->h_stamps, 8LL * (nb + 1), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMemsetAsync(E->arena, 0, total, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(E->d_koff, E->koff.data(), 8LL * (nb + 1), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(E->d_map, map.data(), 4 * E->ntot, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(E->d_fil_body, fil_body.data(), 4 * fil_body.size(), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+        free_ens(s);
+        return hip_fail(e, "tfs_solver_set_independent_bodies: set-up");
+    }
+    return TFS_OK;
+}
+
+int tfs_solver_set_graphs(tfs_solver* s, int32_t enable) {
+    if (!s) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_graphs: null handle");
+    if (!s->ens) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_set_graphs: no independent bodies set");
+    s->ens->use_graphs = enable != 0;
+    return TFS_OK;
+}
+
+int tfs_solver_body_stats(tfs_solver* s, int32_t* iterations, int32_t* converged, double* relres, double* solve_ms) {
+    if (!s) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_body_stats: null handle");
+    if (!s->ens) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_body_stats: no independent bodies set");
+    EnsState& E = *s->ens;
+    const int nb = E.nbody;
+    if (E.iters.size() != (size_t)nb) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_body_stats: no solve yet");
+    if (iterations) std::copy(E.iters.begin(), E.iters.end(), iterations);
+    if (converged) std::copy(E.conv.begin(), E.conv.end(), converged);
+    if (relres) std::copy(E.relres.begin(), E.relres.end(), relres);
+    if (solve_ms) {
+        int khz = 0;
+        hipError_t e = hipMemcpyAsync(E.h_stamps, E.stamps, 8LL * (nb + 1), hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->rpy->device);
+        if (e != hipSuccess) return hip_fail(e, "tfs_solver_body_stats");
+        if (khz <= 0) return fail(TFS_ERR_HIP, "tfs_solver_body_stats: no wall clock rate");
+        for (int b = 0; b < nb; ++b) solve_ms[b] = double(E.h_stamps[b] - E.h_stamps[nb]) / khz;
+    }
+    return TFS_OK;
+}
+
+int tfs_solver_cycle_counts(tfs_solver* s, int64_t* graph_cycles, int64_t* eager_cycles) {
+    if (!s) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_solver_cycle_counts: null handle");
+    if (graph_cycles) *graph_cycles = s->ens ? s->ens->graph_launches : 0;
+    if (eager_cycles) *eager_cycles = s->ens ? s->ens->eager_cycles : 0;
     return TFS_OK;
 }
 

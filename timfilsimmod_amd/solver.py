@@ -91,6 +91,10 @@ L.extra_signatures({
     "tfs_solver_precondition_f32": (ctypes.c_int, [_dp, _dp, _dp, _i32]),
     "tfs_solver_solve_f32": (ctypes.c_int, [_dp, _dp, _dp, _i32, ctypes.POINTER(SolveStats)]),
     "tfs_solver_residual_f32": (ctypes.c_int, [_dp, _dp, _dp, _i32, _i32, ctypes.POINTER(ctypes.c_double)]),
+    "tfs_solver_set_independent_bodies": (ctypes.c_int, [_dp, _i32, _dp, _dp]),
+    "tfs_solver_set_graphs": (ctypes.c_int, [_dp, _i32]),
+    "tfs_solver_body_stats": (ctypes.c_int, [_dp, _dp, _dp, _dp, _dp]),
+    "tfs_solver_cycle_counts": (ctypes.c_int, [_dp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
 })
 
 
@@ -141,6 +145,7 @@ class Solver:
         n = ctypes.c_int64()
         L.check(L.lib().tfs_solver_size(self.h, ctypes.byref(n)))
         self.n = n.value
+        self._nbody = 1
 
     def close(self):
         if self.h:
@@ -195,6 +200,38 @@ class Solver:
         L.check(L.lib().tfs_solver_residual(self.h, L.ptr(rhs), L.ptr(x), precision, L.MEMORY_HOST,
                                             ctypes.byref(out)))
         return out.value
+
+    def set_independent_bodies(self, fil_per_body, blobs_per_body):
+        """Solve the bodies held one after another as independent systems in
+        lock-step (tfs_solver_set_independent_bodies)."""
+        self._nbody = set_independent_bodies(self.h, fil_per_body, blobs_per_body)
+
+    def set_graphs(self, enable: bool):
+        L.check(L.lib().tfs_solver_set_graphs(self.h, int(bool(enable))))
+
+    def body_stats(self):
+        return body_stats(self.h, self._nbody)
+
+
+def set_independent_bodies(handle, fil_per_body, blobs_per_body):
+    """tfs_solver_set_independent_bodies on a solver handle."""
+    fil = np.ascontiguousarray(fil_per_body, dtype=np.int32)
+    blobs = np.ascontiguousarray(blobs_per_body, dtype=np.int32)
+    if fil.size != blobs.size:
+        raise ValueError("set_independent_bodies: one filament and one blob count per body")
+    L.check(L.lib().tfs_solver_set_independent_bodies(handle, fil.size, L.ptr(fil), L.ptr(blobs)))
+    return fil.size
+
+
+def body_stats(handle, nbody: int) -> dict:
+    """Per-body results of the last solve (tfs_solver_body_stats)."""
+    it, cv = np.zeros(nbody, np.int32), np.zeros(nbody, np.int32)
+    rr, ms = np.zeros(nbody), np.zeros(nbody)
+    L.check(L.lib().tfs_solver_body_stats(handle, L.ptr(it), L.ptr(cv), L.ptr(rr), L.ptr(ms)))
+    gc, ec = ctypes.c_int64(), ctypes.c_int64()
+    L.check(L.lib().tfs_solver_cycle_counts(handle, ctypes.byref(gc), ctypes.byref(ec)))
+    return {"iterations": it, "converged": cv.astype(bool), "relres": rr, "solve_ms": ms,
+            "graph_cycles": gc.value, "eager_cycles": ec.value}
 
 
 class ElasticVelocities:
@@ -444,3 +481,78 @@ class BeatSimulation:
         vs, vb = np.zeros(6 * ns), np.zeros(3 * nb)
         L.check(L.lib().tfs_beat_velocities(self.h, L.ptr(vs), L.ptr(vb) if nb else None))
         return vs, vb
+
+
+@dataclass
+class _Concatenated:
+    """The Body fields BeatSimulation reads, for bodies held one after another."""
+
+    num_fil: int
+    num_seg: int
+    num_blobs: int
+    a: float
+    centre: np.ndarray
+
+
+class EnsembleBeat(BeatSimulation):
+    """Prescribed-beat time loop of independent bodies of uneven size on one
+    handle: the bodies' arrays one after another, every timestep's solve one
+    lock-step GMRES per body (tfs_solver_set_independent_bodies).  Each body
+    follows exactly the trajectory its own BeatSimulation would (same setup,
+    same solver settings).  Bodies are held still (no prescribed motion)."""
+
+    def __init__(self, bodies, *, setups=None, graphs: bool = True, **kw):
+        if not bodies:
+            raise ValueError("EnsembleBeat: no bodies")
+        nseg, a, dl = bodies[0].num_seg, bodies[0].a, bodies[0].dl
+        if any(b.num_seg != nseg or b.a != a or b.dl != dl for b in bodies):
+            raise ValueError("EnsembleBeat: every body needs the same segment count, radius and spacing")
+        setups = setups or [beat_setup(b, phase_seed=3 + i) for i, b in enumerate(bodies)]
+        cat = lambda k: np.ascontiguousarray(np.concatenate([np.asarray(getattr(s, k)).reshape(-1)
+                                                             for s in setups]))
+        s0 = setups[0]
+        su = BeatSetup(frames=cat("frames"), bases=cat("bases"), x_blobs=cat("x_blobs"), s_values=s0.s_values,
+                       omega0=cat("omega0"), phase0=cat("phase0"), angle0=cat("angle0"),
+                       gen_phase=s0.gen_phase, gen_angle=s0.gen_angle, fil_length=s0.fil_length)
+        body = _Concatenated(num_fil=sum(b.num_fil for b in bodies), num_seg=nseg,
+                             num_blobs=sum(b.num_blobs for b in bodies), a=a, centre=np.zeros(3))
+        super().__init__(body, setup=su, **kw)
+        self.fil_per_body = np.array([b.num_fil for b in bodies], dtype=np.int32)
+        self.blobs_per_body = np.array([b.num_blobs for b in bodies], dtype=np.int32)
+        self.num_bodies = len(bodies)
+        sp = self._solver()
+        set_independent_bodies(sp, self.fil_per_body, self.blobs_per_body)
+        if not graphs:
+            L.check(L.lib().tfs_solver_set_graphs(sp, 0))
+
+    def _solver(self):
+        sp = ctypes.c_void_p()
+        L.check(L.lib().tfs_beat_solver(self.h, ctypes.byref(sp)))
+        return sp
+
+    def body_stats(self) -> dict:
+        return body_stats(self._solver(), self.num_bodies)
+
+    def split(self, state: dict) -> list:
+        """Per-body pieces of state(): phase, angle, x_segs and solution (the
+        solution in the one-body layout [lambda_s | lambda_b | phase | angle])."""
+        nf, nb, nseg = self.fil_per_body, self.blobs_per_body, self.num_seg
+        fo = np.concatenate([[0], np.cumsum(nf)])
+        bo = np.concatenate([[0], np.cumsum(nb)])
+        sol = state["solution"]
+        ns3 = 3 * int(fo[-1]) * nseg
+        nfil = int(fo[-1])
+        off_phase = ns3 + 3 * int(bo[-1])
+        dyn_p, dyn_a = self.desc.solver.dynamic_phase, self.desc.solver.dynamic_angle
+        off_angle = off_phase + (nfil if dyn_p else 0)
+        out = []
+        for i in range(self.num_bodies):
+            f0, f1, b0, b1 = int(fo[i]), int(fo[i + 1]), int(bo[i]), int(bo[i + 1])
+            parts = [sol[3 * f0 * nseg:3 * f1 * nseg], sol[ns3 + 3 * b0:ns3 + 3 * b1]]
+            if dyn_p:
+                parts.append(sol[off_phase + f0:off_phase + f1])
+            if dyn_a:
+                parts.append(sol[off_angle + f0:off_angle + f1])
+            out.append({"phase": state["phase"][f0:f1], "angle": state["angle"][f0:f1],
+                        "x_segs": state["x_segs"][3 * f0 * nseg:3 * f1 * nseg], "solution": np.concatenate(parts)})
+        return out
