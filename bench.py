@@ -164,8 +164,8 @@ def flow_leg(torch, stream):
 def ensemble_leg(torch, stream, steps: int = 5):
     """64 independent bodies of uneven size (filaments U[100,1000] x 20 segments,
     blobs U[2000,9000]) in one block-diagonal 6-DOF matvec launch: autotuned
-    shape, hipGraph replays; per-body completion p50/p99 from one timed
-    persistent launch."""
+    shape, hipGraph replays; per-body completion p50/p99 (ms from the launch's
+    first item) read from the stamps of the last timed replay."""
     from timfilsimmod_amd import _lib as L
     from timfilsimmod_amd.ensemble import make_ensemble
     from timfilsimmod_amd.mobility import Graph, RpyHandle
@@ -186,6 +186,7 @@ def ensemble_leg(torch, stream, steps: int = 5):
         tuned_ms = h.autotune(prec, reps=1)
         shape = h.tile_shape(prec)
         h.set_precision(prec)
+        h.body_timing(True)     # every replay stores per-item start/finish stamps
         g = Graph(h, L.BLOCK_ALL, 1)
         g.launch()
         torch.cuda.synchronize()
@@ -195,24 +196,77 @@ def ensemble_leg(torch, stream, steps: int = 5):
             g.launch()
         t1.record()
         torch.cuda.synchronize()
+        bt = h.body_times(64)   # the last timed replay
         g.close()
-        ms = t0.elapsed_time(t1) / steps
-        # per-body completion inside one persistent launch
-        keep = shape["persistent"]
-        if keep == 0:
-            h.set_persistent(prec, 2)
-        h.body_timing(True)
-        h.evaluate()
-        bt = h.body_times(64)
         h.body_timing(False)
-        h.set_persistent(prec, keep)
+        ms = t0.elapsed_time(t1) / steps
         out[name] = {"ms_per_launch": ms, "body_matvecs_per_s": 64 * 1000.0 / ms,
                      "TFLOP_s": flops / (ms * 1e-3) / 1e12, "autotuned_shape": shape,
                      "autotune_best_ms": tuned_ms,
                      "per_body_completion_ms_p50": float(np.percentile(bt, 50)),
                      "per_body_completion_ms_p99": float(np.percentile(bt, 99)),
-                     "per_body_completion_ms_max": float(bt.max())}
+                     "per_body_completion_ms_max": float(bt.max()),
+                     "per_body_completion_p99_over_p50": float(np.percentile(bt, 99) / np.percentile(bt, 50))}
     h.close()
+    return out
+
+
+def ensemble_timestep_leg(torch, stream, steps: int = 2, warmup: int = 2):
+    """The 64 uneven bodies (filaments U[100,1000] x 20 segments, blobs
+    U[2000,9000]) taking prescribed-beat timesteps together on one handle:
+    one lock-step GMRES per body in every launch, each GMRES cycle a captured
+    hipGraph, converged bodies dropping out of the pair launches.  Same solver
+    settings as the single-body timestep.  Reports aggregate body-timesteps/s
+    and each body's solve time (device clock, solve start to the body's last
+    Arnoldi stop) over the timed steps, mixed and FP64, and the bodies'
+    mixed-vs-FP64 trajectory deviation after the same steps."""
+    from timfilsimmod_amd.ensemble import make_ensemble
+    from timfilsimmod_amd.solver import SOLVE_FP64, SOLVE_MIXED, EnsembleBeat
+    e = make_ensemble(64)
+    out = {"bodies": 64, "segments": e.num_segs, "blobs": e.num_blobs, "warmup_steps": warmup,
+           "timed_steps": steps}
+    states = {}
+    for mode, name in ((SOLVE_MIXED, "mixed"), (SOLVE_FP64, "fp64")):
+        print(f"bench: ensemble timestep leg, {name}", file=sys.stderr, flush=True)
+        sim = EnsembleBeat(e.bodies, mode=mode, tol=1e-8, restart=50, inner_tol=INNER_TOL, max_cycles=40,
+                           max_refinements=20, stream=stream, warm_start=WARM_START, precond=PRECOND)
+        out["system_size"] = sim.n
+        for _ in range(warmup):
+            sim.step()
+        torch.cuda.synchronize()
+        stats, solve_ms, iters, relres = [], [], [], []
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(steps):
+            stats.append(sim.step())
+            bs = sim.body_stats()
+            solve_ms.append(bs["solve_ms"])
+            iters.append(bs["iterations"])
+            relres.append(bs["relres"].max())
+        t1.record()
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / steps
+        bs = sim.body_stats()
+        solve_ms = np.concatenate(solve_ms)
+        out[name] = {"ms_per_step": ms, "ensemble_timesteps_per_s": 1000.0 / ms,
+                     "body_timesteps_per_s": 64 * 1000.0 / ms,
+                     "per_body_solve_ms_p50": float(np.percentile(solve_ms, 50)),
+                     "per_body_solve_ms_p99": float(np.percentile(solve_ms, 99)),
+                     "per_body_solve_ms_max": float(solve_ms.max()),
+                     "gmres_iterations_per_body_mean": float(np.mean(iters)),
+                     "gmres_iterations_per_body_max": int(np.max(iters)),
+                     "lockstep_iterations_per_step": float(np.mean([s["iterations"] for s in stats])),
+                     "fp32_matvecs_per_step": float(np.mean([s["fp32_matvecs"] for s in stats])),
+                     "fp64_matvecs_per_step": float(np.mean([s["fp64_matvecs"] for s in stats])),
+                     "final_relres_max": float(max(relres)),
+                     "converged_all": bool(all(s["converged"] for s in stats)),
+                     "graph_cycles": bs["graph_cycles"], "eager_cycles": bs["eager_cycles"]}
+        states[name] = sim.split(sim.state())
+        sim.close()
+        torch.cuda.empty_cache()
+    out["speedup_mixed_over_fp64"] = out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    out["trajectory_dev_over_DL_max"] = float(max(np.abs(a["x_segs"] - b["x_segs"]).max()
+                                                  for a, b in zip(states["mixed"], states["fp64"])) / DL)
     return out
 
 
@@ -438,12 +492,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--matvec-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="timestep,matvec,flow,ensemble,elastic",
-                    help="comma list of timestep, matvec, flow, ensemble, copy (copy: a 256 MiB device "
-                         "copy for calibrating HBM byte counters; not part of the default run)")
+    ap.add_argument("--legs", default="timestep,matvec,flow,ensemble,ensemble_step,elastic",
+                    help="comma list of timestep, matvec, flow, ensemble, ensemble_step, elastic, copy "
+                         "(copy: a 256 MiB device copy for calibrating HBM byte counters; not part of the "
+                         "default run)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
-    unknown = legs - {"timestep", "matvec", "flow", "ensemble", "copy", "elastic"}
+    unknown = legs - {"timestep", "matvec", "flow", "ensemble", "ensemble_step", "copy", "elastic"}
     if unknown:
         print(f"bench.py: unknown legs {sorted(unknown)}", file=sys.stderr)
         sys.exit(2)
@@ -558,6 +613,10 @@ def main():
     if "ensemble" in legs:
         print("bench: ensemble leg", file=sys.stderr, flush=True)
         ens = ensemble_leg(torch, stream)
+    ens_step = None
+    if "ensemble_step" in legs:
+        ens_step = ensemble_timestep_leg(torch, stream)
+        torch.cuda.empty_cache()
     elastic = None
     if "elastic" in legs:
         print("bench: elastic-step leg", file=sys.stderr, flush=True)
@@ -606,6 +665,7 @@ def main():
         "matvec": matvec,
         "flow_field": flow,
         "ensemble": ens,
+        "ensemble_timestep": ens_step,
         "elastic_step": elastic,
         "roofline": {"bound": "valu", "achieved": tf32, "peak": PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
                      "frac": tf32 / PEAK_FP32_VALU_TFLOPS, "traffic": measured_traffic(),

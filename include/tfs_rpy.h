@@ -149,7 +149,10 @@ int tfs_rpy_positions_changed(tfs_rpy *h);
  * (segs_per_body[i] segments and blobs_per_body[i] blobs each, summing to the
  * handle's sizes); evaluations couple particles of the same body only, i.e.
  * a block-diagonal grand mobility.  The reference couples all NSWIM bodies
- * (cuda_functions.cu:399, 504, 580, 675); one body is the default. */
+ * (cuda_functions.cu:399, 504, 580, 675); one body is the default.  Unless a
+ * tile shape was set or autotuned, work items are then sized so that the
+ * largest body alone fills the device, and the bodies' target tiles are
+ * interleaved through the launch so that they finish together. */
 int tfs_rpy_set_bodies(tfs_rpy *h, int32_t num_bodies, const int32_t *segs_per_body,
                        const int32_t *blobs_per_body);
 
@@ -158,7 +161,7 @@ int tfs_rpy_set_bodies(tfs_rpy *h, int32_t num_bodies, const int32_t *segs_per_b
 int tfs_rpy_set_persistent(tfs_rpy *h, int32_t precision, int32_t workgroups_per_cu);
 /* Current launch shape of a precision: block, per_thread, chunk, persistent. */
 int tfs_rpy_get_tile_shape(tfs_rpy *h, int32_t precision, int32_t shape[4]);
-/* Per-body completion times of the last persistent launch, in ms from its
+/* Per-body completion times of the last launch (either form), in ms from its
  * first work item's start (enable before the launch; reading synchronises). */
 int tfs_rpy_body_timing(tfs_rpy *h, int32_t enable);
 int tfs_rpy_body_times(tfs_rpy *h, double *ms_per_body);

@@ -223,23 +223,33 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
         for (int k = 0; k < VT; ++k) slot[k * TILE + tid + r * BLOCK] = T(k < 3 ? acc[r][k] : lam * acc[r][k]);
 }
 
+// One workgroup per item.  With `stamps`, thread 0 stores the item's start
+// and finish (wall_clock64 ticks; equal for an item of a skipped body).
 template <typename T, int SEGDOF, int BLOCK, int R>
 __global__ __launch_bounds__(BLOCK) void rpy_pairs_kernel(const MatvecItem* __restrict__ items,
                                                           const MatvecTile* __restrict__ tiles,
-                                                          MatvecArgs a, T* __restrict__ partial) {
+                                                          MatvecArgs a, T* __restrict__ partial,
+                                                          int64_t* __restrict__ stamps) {
     constexpr int REC = rec_len_max<T>();
     __shared__ __align__(16) T lds[BLOCK * REC];
     if (a.skip && *a.skip) return;
+    const int64_t t_start = stamps ? (int64_t)wall_clock64() : 0;
     const MatvecItem it = items[blockIdx.x];
     const MatvecTile tl = tiles[it.tile];
-    if (a.body_skip && a.body_skip[tl.body]) return;
-    T* slot = partial + (size_t)blockIdx.x * (kMaxDof * BLOCK * R);
-    if (tl.cls == 0) {
-        if (it.src_cls == 0) run_item<SEGDOF, SEGDOF, T, BLOCK, R>(it, tl, a, slot, lds);
-        else run_item<SEGDOF, 3, T, BLOCK, R>(it, tl, a, slot, lds);
-    } else {
-        if (it.src_cls == 0) run_item<3, SEGDOF, T, BLOCK, R>(it, tl, a, slot, lds);
-        else run_item<3, 3, T, BLOCK, R>(it, tl, a, slot, lds);
+    if (!(a.body_skip && a.body_skip[tl.body])) {
+        T* slot = partial + (size_t)blockIdx.x * (kMaxDof * BLOCK * R);
+        if (tl.cls == 0) {
+            if (it.src_cls == 0) run_item<SEGDOF, SEGDOF, T, BLOCK, R>(it, tl, a, slot, lds);
+            else run_item<SEGDOF, 3, T, BLOCK, R>(it, tl, a, slot, lds);
+        } else {
+            if (it.src_cls == 0) run_item<3, SEGDOF, T, BLOCK, R>(it, tl, a, slot, lds);
+            else run_item<3, 3, T, BLOCK, R>(it, tl, a, slot, lds);
+        }
+    }
+    if (stamps && threadIdx.x == 0) {
+        const int64_t t_end = (a.body_skip && a.body_skip[tl.body]) ? t_start : (int64_t)wall_clock64();
+        stamps[2 * (int64_t)blockIdx.x] = t_start;
+        stamps[2 * (int64_t)blockIdx.x + 1] = t_end;
     }
 }
 
@@ -294,10 +304,11 @@ __global__ __launch_bounds__(kWallBlock) void rpy_pairs_wall_kernel(const Matvec
 }
 
 // Persistent form: a fixed grid of `gridDim.x` workgroups walks the work list
-// with stride gridDim.x (the list is ordered by body, items of similar size),
-// so every workgroup's loop count and exit are uniform -- no atomics and no
-// lane-divergent code in the loop.  With timing on, every lane of a workgroup
-// stores the same per-item start/finish stamps (wall_clock64 ticks).
+// with stride gridDim.x (the bodies' tiles interleaved, items of similar
+// size), so every workgroup's loop count and exit are uniform -- no atomics
+// and no lane-divergent code in the loop.  With timing on, every lane of a
+// workgroup stores the same per-item start/finish stamps (wall_clock64
+// ticks; equal for an item of a skipped body).
 template <typename T, int SEGDOF, int BLOCK, int R>
 __global__ __launch_bounds__(BLOCK) void rpy_pairs_persistent_kernel(const MatvecItem* __restrict__ items,
                                                                      const MatvecTile* __restrict__ tiles,
@@ -310,7 +321,10 @@ __global__ __launch_bounds__(BLOCK) void rpy_pairs_persistent_kernel(const Matve
         if (ps.item_stamps) ps.item_stamps[2 * item] = (int64_t)wall_clock64();
         const MatvecItem it = items[item];
         const MatvecTile tl = tiles[it.tile];
-        if (a.body_skip && a.body_skip[tl.body]) continue;  // uniform over the workgroup
+        if (a.body_skip && a.body_skip[tl.body]) {  // uniform over the workgroup
+            if (ps.item_stamps) ps.item_stamps[2 * item + 1] = ps.item_stamps[2 * item];
+            continue;
+        }
         T* slot = partial + (size_t)item * (kMaxDof * BLOCK * R);
         if (tl.cls == 0) {
             if (it.src_cls == 0) run_item<SEGDOF, SEGDOF, T, BLOCK, R>(it, tl, a, slot, lds);
@@ -789,7 +803,7 @@ hipError_t launch_pairs(const TileShape& sh, const MatvecArgs& a, const MatvecIt
                            d_items, d_tiles, a, reinterpret_cast<T*>(partial), n_items, ps);
     } else {
         hipLaunchKernelGGL((rpy_pairs_kernel<T, SEGDOF, BLOCK, R>), dim3(n_items), dim3(BLOCK), 0, stream,
-                           d_items, d_tiles, a, reinterpret_cast<T*>(partial));
+                           d_items, d_tiles, a, reinterpret_cast<T*>(partial), ps.item_stamps);
     }
     return hipGetLastError();
 }
@@ -814,29 +828,49 @@ void build_worklist(const std::vector<BodyRange>& bodies, int tile_targets, int 
                     std::vector<MatvecItem>& items) {
     tiles.clear();
     items.clear();
+    // Target tiles of every body; with several bodies the tiles are then
+    // interleaved in proportion to each body's tile count (tile k of T_b at
+    // position (k + 1/2) / T_b), so that every body's work is spread over the
+    // whole launch and the bodies finish together.
+    struct Pending {
+        MatvecTile tl;
+        double pos;
+    };
+    std::vector<Pending> pend;
     for (const BodyRange& body : bodies) {
-        auto add_tiles = [&](int cls, int t_lo, int t_hi, bool from_segs, bool from_blobs, int write) {
+        const size_t first = pend.size();
+        auto add_tiles = [&](int cls, int t_lo, int t_hi, int write) {
             if (write < 0) return;
             for (int t0 = t_lo; t0 < t_hi; t0 += tile_targets) {
-                MatvecTile tl;
+                MatvecTile tl{};
                 tl.cls = cls;
                 tl.t_begin = t0;
                 tl.t_end = std::min(t0 + tile_targets, t_hi);
-                tl.first_item = (int)items.size();
                 tl.write_mode = write;
                 tl.body = (int32_t)(&body - bodies.data());
-                const int tile_id = (int)tiles.size();
-                auto add_src = [&](int src_cls, int lo, int hi) {
-                    for (int s0 = lo; s0 < hi; s0 += chunk) items.push_back({tile_id, src_cls, s0, std::min(s0 + chunk, hi)});
-                };
-                if (from_segs) add_src(0, body.seg_begin, body.seg_end);
-                if (from_blobs) add_src(1, body.blob_begin, body.blob_end);
-                tl.n_items = (int)items.size() - tl.first_item;
-                tiles.push_back(tl);
+                pend.push_back({tl, 0.0});
             }
         };
-        add_tiles(0, body.seg_begin, body.seg_end, ss, sb, seg_write);
-        add_tiles(1, body.blob_begin, body.blob_end, bs, bb, blob_write);
+        add_tiles(0, body.seg_begin, body.seg_end, seg_write);
+        add_tiles(1, body.blob_begin, body.blob_end, blob_write);
+        const size_t nt = pend.size() - first;
+        for (size_t k = 0; k < nt; ++k) pend[first + k].pos = (double(k) + 0.5) / double(nt);
+    }
+    if (bodies.size() > 1)
+        std::stable_sort(pend.begin(), pend.end(), [](const Pending& x, const Pending& y) { return x.pos < y.pos; });
+    for (Pending& pt : pend) {
+        MatvecTile tl = pt.tl;
+        const BodyRange& body = bodies[tl.body];
+        const bool from_segs = tl.cls == 0 ? ss : bs, from_blobs = tl.cls == 0 ? sb : bb;
+        tl.first_item = (int)items.size();
+        const int tile_id = (int)tiles.size();
+        auto add_src = [&](int src_cls, int lo, int hi) {
+            for (int s0 = lo; s0 < hi; s0 += chunk) items.push_back({tile_id, src_cls, s0, std::min(s0 + chunk, hi)});
+        };
+        if (from_segs) add_src(0, body.seg_begin, body.seg_end);
+        if (from_blobs) add_src(1, body.blob_begin, body.blob_end);
+        tl.n_items = (int)items.size() - tl.first_item;
+        tiles.push_back(tl);
     }
 }
 

@@ -78,6 +78,9 @@ struct tfs_rpy {
     size_t item_stamps_len = 0;
     const tfs::Plan* stamped_plan = nullptr;
     bool body_timing = false;
+    // Tile shape set by tfs_rpy_set_tile_shape or tfs_rpy_autotune (else the
+    // default follows the body list).
+    bool shape_chosen[2] = {false, false};
     // Stop flag passed to the kernels of later evaluations (set by a solver
     // around its speculative Krylov iterations; null otherwise).
     const int32_t* skip_flag = nullptr;

@@ -54,6 +54,24 @@ TileShape default_shape(int precision, int ns, int nb, int num_cus) {
     return s;
 }
 
+// The largest body's segment and blob counts: with independent bodies, work
+// items are sized so that this body alone fills the device (a body whose
+// neighbours have converged still runs at full width).
+static void largest_body(const tfs_rpy* h, int* ns, int* nb) {
+    *ns = h->desc.num_segs;
+    *nb = h->desc.num_blobs;
+    if (h->bodies.size() < 2) return;
+    long best = -1;
+    for (const BodyRange& b : h->bodies) {
+        const long n = (long)(b.seg_end - b.seg_begin) + (b.blob_end - b.blob_begin);
+        if (n > best) {
+            best = n;
+            *ns = b.seg_end - b.seg_begin;
+            *nb = b.blob_end - b.blob_begin;
+        }
+    }
+}
+
 Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err) {
     bool ss = blocks & TFS_BLOCK_SS, sb = blocks & TFS_BLOCK_SB;
     bool bb = blocks & TFS_BLOCK_BB, bs = blocks & TFS_BLOCK_BS;
@@ -410,7 +428,7 @@ int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, V
     }
     PersistState ps;
     ps.num_cus = h->num_cus;
-    if (sh.persistent > 0 && h->body_timing) {
+    if (h->body_timing) {
         int rc = ensure_stamps(h, p->items.size());
         if (rc) return rc;
         ps.item_stamps = h->d_item_stamps;
@@ -712,6 +730,7 @@ int tfs_rpy_set_tile_shape(tfs_rpy* h, int32_t precision, int32_t block, int32_t
     if (chunk < block || chunk % block)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_tile_shape: chunk must be a positive multiple of block");
     h->shape[precision] = TileShape{block, per_thread, chunk, h->shape[precision].persistent};
+    h->shape_chosen[precision] = true;
     return TFS_OK;
 }
 
@@ -742,6 +761,14 @@ int tfs_rpy_set_bodies(tfs_rpy* h, int32_t num_bodies, const int32_t* segs_per_b
     free_persist(h);
     h->bodies = bodies;
     invalidate_near(h);
+    int lns = 0, lnb = 0;
+    largest_body(h, &lns, &lnb);
+    for (int pr = 0; pr < 2; ++pr)
+        if (!h->shape_chosen[pr]) {
+            const int pers = h->shape[pr].persistent;
+            h->shape[pr] = default_shape(pr, lns, lnb, h->num_cus);
+            h->shape[pr].persistent = pers;
+        }
     return TFS_OK;
 }
 
@@ -817,8 +844,11 @@ int tfs_rpy_autotune(tfs_rpy* h, int32_t precision, uint32_t blocks, int32_t rep
     TileShape best = h->shape[precision];
     double best_t = 1e300;
     int rc = TFS_OK;
-    const int ns = h->desc.num_segs, nb = h->desc.num_blobs;
-    const double pairs = double(ns + nb) * double(ns + nb) / double(h->bodies.size());
+    // Chunk candidates are sized on the largest body (all of it for one body),
+    // whose items alone should fill the device.
+    int ns = 0, nb = 0;
+    largest_body(h, &ns, &nb);
+    const double pairs = double(ns + nb) * double(ns + nb);
     for (int block : {64, 128, 256}) {
         if (block == 64 && pairs > 1e8) continue;  // large problems: 64-thread groups never win
         for (int r : {1, 2}) {
@@ -856,6 +886,7 @@ int tfs_rpy_autotune(tfs_rpy* h, int32_t precision, uint32_t blocks, int32_t rep
     h->profiling = prof;
     h->desc.precision = saved_precision;
     h->shape[precision] = best;
+    h->shape_chosen[precision] = true;
     if (best_ms) *best_ms = best_t;
     return rc;
 }
@@ -887,7 +918,7 @@ int tfs_rpy_body_timing(tfs_rpy* h, int32_t enable) {
 int tfs_rpy_body_times(tfs_rpy* h, double* ms_per_body) {
     if (!h || !ms_per_body) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_body_times: null argument");
     const Plan* p = h->stamped_plan;
-    if (!p) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_body_times: no timed persistent launch yet");
+    if (!p) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_body_times: no timed launch yet");
     const size_t ni = p->items.size(), nb = h->bodies.size();
     std::vector<int64_t> st(2 * ni);
     hipError_t e = hipMemcpyAsync(st.data(), h->d_item_stamps, sizeof(int64_t) * 2 * ni, hipMemcpyDeviceToHost,
