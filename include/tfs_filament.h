@@ -15,7 +15,9 @@
  * for the reference's clamped-base build with a follower end force
  * (INSTABILITY_CILIA): the base segment is clamped by a multiplier, a force
  * -END_FORCE_MAGNITUDE t acts on the last segment, and an optional kick acts
- * on two segments during the first step (filament.cpp:1533-1552).
+ * on two segments during the first step (filament.cpp:1533-1552); and, with
+ * tfs_filament_set_base, for its GEOMETRIC_CILIA (driven free base) and
+ * CONSTANT_BASE_ROTATION (turning clamped base) builds.
  *
  * Layouts: segments body -> filament -> segment (as x_segs, f_segs of
  * tfs_rpy.h); quaternions (w, x, y, z); per filament the 3 (NSEG + 1)
@@ -55,6 +57,28 @@ typedef struct tfs_filament tfs_filament;
 
 int tfs_filament_create(const tfs_filament_desc *desc, tfs_filament **out);
 int tfs_filament_destroy(tfs_filament *h);
+/* The base segment (the reference's CILIA_TYPE, config.hpp:346-348):
+ *   TFS_BASE_CLAMPED (default): held by the clamp multiplier, the Broyden
+ *     unknowns' clamp slot (INSTABILITY_CILIA, CILIA_TYPE 0).  values, if
+ *     given, are per filament the Lie-algebra element [3] the base turns by
+ *     every step (CONSTANT_BASE_ROTATION, CILIA_TYPE 2: BASE_ROTATION_RATE DT
+ *     about the base's tangent, filament.cpp:487-492); NULL keeps it still.
+ *   TFS_BASE_DRIVEN: free, with its Lie-algebra element in the clamp's slot of
+ *     the unknowns (filament.cpp:1918-1927), its rotation moving every later
+ *     segment in J0 (:1789-1800), and values per filament the lab-frame
+ *     torque [3] on segment 0 (GEOMETRIC_CILIA, CILIA_TYPE 1: the reference's
+ *     base_torque_magnitude_factor DEFAULT_BASE_TORQUE_MAGNITUDE R(body_q)
+ *     (my_vertical x fast_beating_direction), filament.cpp:1523-1531, whose
+ *     factor keeps its initial value because find_my_angle() returns 0,
+ *     :1433-1447); NULL: no torque.  Bodies held still only.
+ * The end force and first-step kick stay as set (zero for CILIA_TYPE 1, 2).
+ * The reference's CILIA_TYPE 2 branch does not compile as shipped
+ * (clamp_lambda exists only for CILIA_TYPE 0, filament.hpp:66-70) and would
+ * reset the base's rotation to the body's every step (filament.cpp:660-665);
+ * this follows its evident intent. */
+#define TFS_BASE_CLAMPED 0
+#define TFS_BASE_DRIVEN 1
+int tfs_filament_set_base(tfs_filament *h, int32_t mode, const double *values);
 /* Initial state (filament::initial_setup): segment-0 positions [3 per
  * filament] and quaternions [4 per segment]; multipliers and Lie-algebra
  * elements zero, the positions from the robot arm, their history equal. */

@@ -127,9 +127,11 @@ int tfs_rpy_set_precision(tfs_rpy *h, int32_t precision);
  * TFS_DOMAIN_HALF_SPACE, the Swan-Brady wall tensor of
  * cuda_functions.cu:12-203 for particles above z = 0, which like the
  * reference assumes radius 1 and viscosity 1 (other values are rejected) and
- * has no overlap branch.  Half-space evaluations take FP64 tiles only.  The
- * self term is applied to the same particle (the reference tests the
- * class-local indices, which would also match segment i with blob i). */
+ * has no overlap branch.  FP32 tiles evaluate the wall tensor in float (as
+ * the reference's Real = float build does) from exact fixed-point
+ * separations, with FP64 sums across source batches.  The self term is
+ * applied to the same particle (the reference tests the class-local indices,
+ * which would also match segment i with blob i). */
 int tfs_rpy_set_domain(tfs_rpy *h, int32_t domain);
 
 /* Override the launch shape (threads per workgroup 64/128/256, targets per

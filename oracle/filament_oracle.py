@@ -3,9 +3,15 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the device
 filament kernels; the product never imports it.
 
-One filament of NSEG segments, quaternions (w, x, y, z) per segment, the
-reference's clamped-base build with a follower end force (INSTABILITY_CILIA,
-the configuration of the elastic "RPY matvec + GMRES + Broyden" case):
+One filament of NSEG segments, quaternions (w, x, y, z) per segment.  The
+base segment is either clamped (Params.base == "clamped") -- the reference's
+INSTABILITY_CILIA build (CILIA_TYPE 0, with a follower end force and a
+first-step kick; the configuration of the elastic "RPY matvec + GMRES +
+Broyden" case) and CONSTANT_BASE_ROTATION (CILIA_TYPE 2, no end force, the
+base turned by a fixed Lie-algebra element every step, Filament.u0) -- or free
+and driven (Params.base == "driven", GEOMETRIC_CILIA, CILIA_TYPE 1): no clamp,
+the base segment's Lie-algebra element is an unknown, and a driving torque
+Filament.drive acts on it (filament.cpp:1523-1531).
 
   quaternion algebra            src/general/quaternion.cpp:80-110 (product),
                                 :183-195, 245-270 (conj, sqrt), :285-370
@@ -23,7 +29,22 @@ the configuration of the elastic "RPY matvec + GMRES + Broyden" case):
 
 Unknowns of one filament (the Broyden vector's filament block, 6 NSEG):
 [tether lambda, lambda_0 .. lambda_{N-2}] (3 NSEG), then [clamp lambda,
-u_1 .. u_{N-1}] (3 NSEG).
+u_1 .. u_{N-1}] (3 NSEG); with a driven base u_0 takes the clamp's place
+(filament.cpp:1918-1927).
+
+The reference's CONSTANT_BASE_ROTATION branch does not compile as shipped
+(clamp_lambda is declared only for INSTABILITY_CILIA, filament.hpp:66-70,
+but used for CILIA_TYPE 2 at filament.cpp:606, 1565-1571), and its
+accept_state_from_rigid_body would reset the base's Lie-algebra element to
+the body's every step (filament.cpp:660-665), which undoes the rotation set
+up at filament.cpp:487-492.  This restatement follows the evident intent:
+the base segment turns by u0 every step, held by the clamp multiplier.  With
+u0 = 0 it is the clamped build without end force.  GEOMETRIC_CILIA's driving
+torque factor switches only when find_my_angle() passes CRITICAL_ANGLE, and
+find_my_angle() returns 0 (filament.cpp:1433-1447), so the torque stays
+base_torque_magnitude_factor * DEFAULT_BASE_TORQUE_MAGNITUDE * R(body_q)
+(my_vertical x fast_beating_direction) with its initial factor: the caller
+passes that vector as Filament.drive.
 """
 from __future__ import annotations
 
@@ -137,6 +158,7 @@ class Params:
     dt: float = 1.0
     end_force: float = 0.0               # END_FORCE_MAGNITUDE (follower force on the last segment)
     strain_twist: np.ndarray = field(default_factory=lambda: np.zeros(3))
+    base: str = "clamped"                # "clamped" (CILIA_TYPE 0, 2) or "driven" (CILIA_TYPE 1)
 
 
 @dataclass
@@ -157,6 +179,8 @@ class Filament:
     clamp: np.ndarray
     clampm1: np.ndarray
     clampm2: np.ndarray
+    u0: np.ndarray = field(default_factory=lambda: np.zeros(3))      # clamped base: its turn per step
+    drive: np.ndarray = field(default_factory=lambda: np.zeros(3))   # driven base: torque on segment 0
 
     @staticmethod
     def at_rest(base, q, nseg, dl):
@@ -165,7 +189,7 @@ class Filament:
         f = Filament(q=q.copy(), qm1=q.copy(), u=np.zeros((nseg, 3)), um1=np.zeros((nseg, 3)),
                      x=np.zeros((nseg, 3)), xm1=None, xm2=None, lam=np.zeros((nseg - 1, 3)),
                      lamm1=np.zeros((nseg - 1, 3)), lamm2=np.zeros((nseg - 1, 3)), tether=z3(), tetherm1=z3(),
-                     tetherm2=z3(), clamp=z3(), clampm1=z3(), clampm2=z3())
+                     tetherm2=z3(), clamp=z3(), clampm1=z3(), clampm2=z3(), u0=z3(), drive=z3())
         f.x[0] = base
         robot_arm(f, dl)
         f.xm1, f.xm2 = f.x.copy(), f.x.copy()
@@ -220,7 +244,10 @@ def internal_forces(f: Filament, p: Params, perturbation=None):
     if perturbation:
         for k, v in perturbation.items():
             out[k, :3] += v
-    out[0, 3:] += dexpinv_transpose(f.u[0], f.clamp)
+    if p.base == "driven":
+        out[0, 3:] += f.drive                                      # filament.cpp:1523-1531
+    else:
+        out[0, 3:] += dexpinv_transpose(f.u[0], f.clamp)          # :1555-1571
     return out
 
 
@@ -287,6 +314,12 @@ def approx_jacobian(f: Filament, forces, bdf2, p: Params):
         J[3 * j:3 * j + 3, jp:jp + 3] = 0.5 * temp
         for i in range(j + 1, N):
             J[3 * i:3 * i + 3, jp:jp + 3] = temp
+    if p.base == "driven":
+        # the robot arm moves every later segment with the base's tangent
+        # (filament.cpp:1789-1800)
+        t0 = 0.5 * p.dl * rcross(frame(f.q[0])[0])
+        for i in range(1, N):
+            J[3 * i:3 * i + 3, 3 * N:3 * N + 3] = t0
     tN = frame(f.q[N - 1])[0]
     J[3 * (N - 1):3 * N, 3 * (2 * N - 1):6 * N] -= fac * p.end_force * rcross(tN)
     fac2 = (p.dl * p.dt / (24.0 * PI * p.mu * p.a ** 3) if bdf2 else p.dl * p.dt / (16.0 * PI * p.mu * p.a ** 3))
@@ -296,6 +329,8 @@ def approx_jacobian(f: Filament, forces, bdf2, p: Params):
             tmat = -rcross(frame(f.q[k])[0])
             J[ip:ip + 3, 3 * j:3 * j + 3] = fac2 * (tmat + 0.5 * umat @ tmat + umat @ umat @ tmat / 12.0)
     J[3 * N:, 3 * N:] = lie_algebra_block(f, forces, bdf2, p)
+    if p.base == "driven":
+        return J
     fac3 = -p.dt / (12.0 * PI * p.mu * p.a ** 3) if bdf2 else -p.dt / (8.0 * PI * p.mu * p.a ** 3)
     umat = rcross(f.u[0])
     blk = np.eye(3) + 0.5 * umat + umat @ umat / 12.0
@@ -311,7 +346,11 @@ def update(f: Filament, du, p: Params):
     du = np.asarray(du).reshape(2, N, 3)
     f.tether = f.tether + du[0, 0]
     f.lam = f.lam + du[0, 1:]
-    f.clamp = f.clamp + du[1, 0]
+    if p.base == "driven":
+        f.u[0] = f.u[0] + du[1, 0]
+        f.q[0] = qmul(lie_exp(f.u[0]), f.qm1[0])
+    else:
+        f.clamp = f.clamp + du[1, 0]
     for i in range(1, N):
         f.u[i] = f.u[i] + du[1, i]
         f.q[i] = qmul(lie_exp(f.u[i]), f.qm1[i])
@@ -321,9 +360,11 @@ def update(f: Filament, du, p: Params):
 def initial_guess(f: Filament, nt, p: Params, num_euler_steps=1):
     """Start of step nt for a filament on a body held still
     (filament::initial_guess, filament.cpp:1083-1385): segment history shift
-    and u extrapolation (segment::initial_guess), the base segment set back to
-    the body's rotation, zero here (accept_state_from_rigid_body,
-    filament.cpp:649-672), the multipliers' extrapolation, the robot arm."""
+    and u extrapolation (segment::initial_guess); a clamped base segment then
+    turns by its fixed u0 (zero for the INSTABILITY_CILIA build, whose base
+    takes the still body's rotation, accept_state_from_rigid_body,
+    filament.cpp:649-672), a driven one keeps its extrapolated u; then the
+    multipliers' extrapolation and the robot arm."""
     N = p.nseg
     f.xm2, f.xm1 = f.xm1.copy(), f.x.copy()
     if nt <= num_euler_steps:
@@ -333,8 +374,9 @@ def initial_guess(f: Filament, nt, p: Params, num_euler_steps=1):
     f.qm1 = f.q.copy()
     for i in range(N):
         f.q[i] = qmul(lie_exp(f.u[i]), f.qm1[i])
-    f.u[0] = 0.0
-    f.q[0] = qmul(lie_exp(f.u[0]), f.qm1[0])
+    if p.base != "driven":
+        f.u[0] = f.u0
+        f.q[0] = qmul(lie_exp(f.u[0]), f.qm1[0])
     if nt == 1:
         f.lamm1, f.tetherm1, f.clampm1 = f.lam.copy(), f.tether.copy(), f.clamp.copy()
     elif nt == 2:

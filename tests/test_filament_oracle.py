@@ -218,3 +218,103 @@ def test_multiplier_extrapolation_is_exact_on_polynomials():
         g.lam = lin(nt - 1)
         F.initial_guess(g, nt, p)
     assert np.allclose(g.lam, lin(2), atol=1e-12)
+
+
+# ---- the base segment's other builds (CILIA_TYPE 1 and 2) --------------------------
+def test_driven_base_forces_balance():
+    p = F.Params(nseg=10, base="driven")
+    f, rng = bent_filament(10, 0.3, 14)
+    f.lam = rng.standard_normal((9, 3)) * 50
+    f.tether = rng.standard_normal(3) * 20
+    f.clamp = rng.standard_normal(3) * 20          # ignored by a driven base
+    f.drive = rng.standard_normal(3) * 40
+    out = F.internal_forces(f, p)
+    assert np.allclose(out[:, :3].sum(0), f.tether, atol=1e-10)
+    moment = (np.cross(f.x, out[:, :3]) + out[:, 3:]).sum(0)
+    assert np.allclose(moment, np.cross(f.x[0], f.tether) + f.drive, atol=1e-8)
+
+
+@pytest.mark.parametrize("bdf2", [False, True])
+def test_driven_base_jacobian_is_the_self_mobility_jacobian(bdf2):
+    # a free, driven base: its Lie-algebra element is the unknown in the
+    # clamp's slot and moves every later segment through the robot arm
+    N = 6
+    p = F.Params(nseg=N, dt=0.3, kt=1500.0, base="driven")
+    q0 = quat(Rotation.random(random_state=17))
+    f = F.Filament.at_rest(np.array([0.5, 1.0, -2.0]), np.tile(q0, (N, 1)), N, p.dl)
+    rng = np.random.default_rng(18)
+    f.lam = rng.standard_normal((N - 1, 3)) * 30
+    f.tether = rng.standard_normal(3) * 10
+    f.drive = rng.standard_normal(3) * 25
+    J, fd = jacobian_fd(f, p, bdf2)
+    scale = np.abs(fd).max()
+    assert np.abs(J - fd).max() <= 2e-7 * scale
+    # the base column is not the clamp's: rows of later segments' positions see it
+    assert np.abs(J[3:3 * N, 3 * N:3 * N + 3]).max() > 0.1
+    # a planted sign flip of the robot-arm column is seen
+    Jbad = J.copy()
+    Jbad[3:3 * N, 3 * N:3 * N + 3] *= -1
+    assert np.abs(Jbad - fd).max() > 1e-2 * scale
+
+
+def test_driven_base_update_turns_the_whole_filament():
+    N, dl = 5, 2.2
+    p = F.Params(nseg=N, dl=dl, base="driven")
+    q0 = quat(Rotation.from_rotvec([0.0, 0.3, 0.0]))
+    f = F.Filament.at_rest(np.array([1.0, 2.0, 3.0]), np.tile(q0, (N, 1)), N, dl)
+    t = F.frame(q0)[0]
+    dz = np.zeros(6 * N)
+    dz[3 * N:3 * N + 3] = [0.0, 0.0, 0.4]           # turn the base segment only
+    g = f.copy()
+    F.update(g, dz, p)
+    t0 = F.frame(F.qmul(F.lie_exp(np.array([0.0, 0.0, 0.4])), q0))[0]
+    assert np.allclose(g.u[0], [0.0, 0.0, 0.4]) and np.allclose(g.clamp, 0.0)
+    assert np.allclose(g.x[0], f.x[0], atol=1e-14)
+    for i in range(1, N):
+        assert np.allclose(g.x[i], f.x[0] + 0.5 * dl * (t0 + t) + (i - 1) * dl * t, atol=1e-13)
+    # the clamped build puts the same slot into the clamp multiplier
+    h = f.copy()
+    F.update(h, dz, F.Params(nseg=N, dl=dl))
+    assert np.allclose(h.clamp, [0.0, 0.0, 0.4]) and np.allclose(h.x, f.x, atol=1e-14)
+
+
+def test_constant_base_rotation_turns_the_base_every_step():
+    N = 6
+    p = F.Params(nseg=N)
+    q0 = quat(Rotation.random(random_state=23))
+    f = F.Filament.at_rest(np.zeros(3), np.tile(q0, (N, 1)), N, p.dl)
+    t, n, _ = F.frame(q0)
+    theta = 0.05
+    f.u0 = theta * t                                 # BASE_ROTATION_RATE DT about the tangent
+    x0 = f.x.copy()
+    for nt in range(4):
+        F.initial_guess(f, nt, p)
+        rot_k = Rotation.from_rotvec((nt + 1) * theta * t)
+        tk, nk, _ = F.frame(f.q[0])
+        assert np.allclose(tk, t, atol=1e-13) and np.allclose(nk, rot_k.apply(n), atol=1e-12)
+        assert np.allclose(f.u[0], theta * t) and np.allclose(f.x, x0, atol=1e-12)
+        for i in range(1, N):                        # the others stay at rest (no forces acted)
+            assert np.allclose(f.q[i], q0, atol=1e-14)
+
+
+def test_constant_base_rotation_clamp_columns_of_the_jacobian():
+    # with the base turned (u_0 != 0) the clamp block fac dexpinv dexpinv^T is
+    # the series of the exact derivative to fourth order in |u_0|; the rest of
+    # J0 is first order in the twist the turn puts between segments 0 and 1
+    N = 5
+    errs = []
+    for theta in (0.02, 0.04):
+        p = F.Params(nseg=N, dt=0.25)
+        q0 = quat(Rotation.random(random_state=29))
+        f = F.Filament.at_rest(np.zeros(3), np.tile(q0, (N, 1)), N, p.dl)
+        f.u0 = theta * F.frame(q0)[0]
+        F.initial_guess(f, 0, p)
+        rng = np.random.default_rng(30)
+        f.lam = rng.standard_normal((N - 1, 3)) * 30
+        f.clamp = rng.standard_normal(3) * 10
+        J, fd = jacobian_fd(f, p, False)
+        scale = np.abs(fd).max()
+        cl = slice(3 * N, 3 * N + 3)
+        assert np.abs(J[:, cl] - fd[:, cl]).max() <= 1e-7 * np.abs(fd[:, cl]).max()
+        errs.append(np.abs(J - fd).max() / scale)
+    assert errs[0] < 2e-3 and 1.7 < errs[1] / errs[0] < 2.5

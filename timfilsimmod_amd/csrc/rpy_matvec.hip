@@ -254,52 +254,99 @@ __global__ __launch_bounds__(BLOCK) void rpy_pairs_kernel(const MatvecItem* __re
 }
 
 // Half-space domain (no-slip wall at z = 0): every pair of an item through
-// pair_wall in FP64, one target per lane; the self pair (same global index)
-// takes the wall self-mobility.  Partial slots as run_item.
-template <int VT, int FS>
+// pair_wall, one target per lane; the self pair (same global index) takes the
+// wall self-mobility.  FP64 tiles stage positions as they are.  FP32 tiles
+// take separations from the fixed-point positions (exact integer
+// differences, scaled by 1/lambda) and heights as floats of the FP64
+// positions, sum one LDS batch in FP32 and flush it into FP64 accumulators.
+// Partial slots as run_item, in the evaluation type.
+template <int VT, int FS, typename T>
 __device__ void run_item_wall(const MatvecItem it, const MatvecTile tl, const MatvecArgs& a,
-                              double* __restrict__ slot, double* __restrict__ lds) {
-    constexpr int REC = rec_len<double, FS>();
+                              T* __restrict__ slot, T* __restrict__ lds) {
+    constexpr bool F32 = sizeof(T) == 4;
+    constexpr int F0 = F32 ? 4 : 3;          // FP32 records: X bits (3), height, strengths
+    constexpr int REC = F0 + FS;
     constexpr int TILE = kWallBlock;
     const int tid = threadIdx.x;
     int t = tl.t_begin + tid;
     t = t < tl.t_end ? t : tl.t_end - 1;
+    const CView xt = tl.cls == 0 ? a.xs : a.xb;
+    const double zi_d = xt.p[(int64_t)t * xt.ld_p + 2 * xt.ld_c];
     double xi[3];
-    load_target(a, tl.cls, t, xi);
+    int32_t Xi[3];
+    if constexpr (F32) load_target(a, tl.cls, t, Xi);
+    else load_target(a, tl.cls, t, xi);
+    const T zi = T(zi_d);
+    const T inv_lam = T(F32 ? 1.0 / a.lam : 1.0);
     const int tg = (tl.cls == 0 ? 0 : a.ns) + t;
     const int src_base = it.src_cls == 0 ? 0 : a.ns;
+    const CView xsrc = it.src_cls == 0 ? a.xs : a.xb;
+    const CView fsrc = it.src_cls == 0 ? a.fs : a.fb;
     double acc[VT];
+    T part[VT];
 #pragma unroll
     for (int k = 0; k < VT; ++k) acc[k] = 0.0;
     for (int s0 = it.s_begin; s0 < it.s_end; s0 += kWallBlock) {
         const int n = (it.s_end - s0) < kWallBlock ? (it.s_end - s0) : kWallBlock;
         __syncthreads();
-        if (tid < n) stage_source<double, FS>(lds + tid * REC, a, it.src_cls, s0 + tid, true);
-        __syncthreads();
-        for (int j = 0; j < n; ++j) {
-            const double* rec = lds + j * REC;
-            pair_wall<VT, FS>(acc, src_base + s0 + j == tg, xi, rec, rec + 3);
+        if (tid < n) {
+            T* rec = lds + tid * REC;
+            const int s = s0 + tid;
+            if constexpr (F32) {
+                const int4 q = a.xfx[src_base + s];
+                rec[0] = __int_as_float(q.x);
+                rec[1] = __int_as_float(q.y);
+                rec[2] = __int_as_float(q.z);
+                rec[3] = float(xsrc.p[(int64_t)s * xsrc.ld_p + 2 * xsrc.ld_c]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) rec[k] = xsrc.p[(int64_t)s * xsrc.ld_p + k * xsrc.ld_c];
+            }
+#pragma unroll
+            for (int k = 0; k < FS; ++k) rec[F0 + k] = T(fsrc.p[(int64_t)s * fsrc.ld_p + k * fsrc.ld_c]);
         }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < VT; ++k) part[k] = T(0);
+        for (int j = 0; j < n; ++j) {
+            const T* rec = lds + j * REC;
+            T dx, dy, dz, zj;
+            if constexpr (F32) {
+                dx = float(Xi[0] - __float_as_int(rec[0])) * inv_lam;
+                dy = float(Xi[1] - __float_as_int(rec[1])) * inv_lam;
+                dz = float(Xi[2] - __float_as_int(rec[2])) * inv_lam;
+                zj = rec[3];
+            } else {
+                dx = xi[0] - rec[0];
+                dy = xi[1] - rec[1];
+                dz = xi[2] - rec[2];
+                zj = rec[2];
+            }
+            pair_wall<T, VT, FS>(part, src_base + s0 + j == tg, dx, dy, dz, zi, zj, rec + F0);
+        }
+#pragma unroll
+        for (int k = 0; k < VT; ++k) acc[k] += double(part[k]);
     }
 #pragma unroll
-    for (int k = 0; k < VT; ++k) slot[k * TILE + tid] = acc[k];
+    for (int k = 0; k < VT; ++k) slot[k * TILE + tid] = T(acc[k]);
 }
 
-template <int SEGDOF>
+template <int SEGDOF, typename T>
 __global__ __launch_bounds__(kWallBlock) void rpy_pairs_wall_kernel(const MatvecItem* __restrict__ items,
                                                                     const MatvecTile* __restrict__ tiles,
-                                                                    MatvecArgs a, double* __restrict__ partial) {
-    __shared__ __align__(16) double lds[kWallBlock * rec_len_max<double>()];
+                                                                    MatvecArgs a, T* __restrict__ partial) {
+    __shared__ __align__(16) T lds[kWallBlock * (4 + 6)];
     if (a.skip && *a.skip) return;
     const MatvecItem it = items[blockIdx.x];
     const MatvecTile tl = tiles[it.tile];
-    double* slot = partial + (size_t)blockIdx.x * (kMaxDof * kWallBlock);
+    if (a.body_skip && a.body_skip[tl.body]) return;
+    T* slot = partial + (size_t)blockIdx.x * (kMaxDof * kWallBlock);
     if (tl.cls == 0) {
-        if (it.src_cls == 0) run_item_wall<SEGDOF, SEGDOF>(it, tl, a, slot, lds);
-        else run_item_wall<SEGDOF, 3>(it, tl, a, slot, lds);
+        if (it.src_cls == 0) run_item_wall<SEGDOF, SEGDOF, T>(it, tl, a, slot, lds);
+        else run_item_wall<SEGDOF, 3, T>(it, tl, a, slot, lds);
     } else {
-        if (it.src_cls == 0) run_item_wall<3, SEGDOF>(it, tl, a, slot, lds);
-        else run_item_wall<3, 3>(it, tl, a, slot, lds);
+        if (it.src_cls == 0) run_item_wall<3, SEGDOF, T>(it, tl, a, slot, lds);
+        else run_item_wall<3, 3, T>(it, tl, a, slot, lds);
     }
 }
 
@@ -955,30 +1002,49 @@ hipError_t launch_barrier(const MatvecArgs& a, NearGrid& grid, const BarrierPara
     return hipGetLastError();
 }
 
-hipError_t launch_matvec_wall(int seg_dof, const MatvecArgs& a, const OutArgs& o, const MatvecTile* d_tiles,
-                              int n_tiles, const MatvecItem* d_items, int n_items, double* partial,
-                              hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end) {
+hipError_t launch_matvec_wall(int precision, int seg_dof, const MatvecArgs& a, const OutArgs& o,
+                              const MatvecTile* d_tiles, int n_tiles, const MatvecItem* d_items, int n_items,
+                              double* partial, hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end) {
     if (n_tiles == 0) return hipSuccess;
+    float* pf = reinterpret_cast<float*>(partial);
     if (n_items > 0) {
         if (ev_begin) TFS_HIP_CHECK(hipEventRecord(ev_begin, stream));
-        if (seg_dof == 6)
-            hipLaunchKernelGGL(rpy_pairs_wall_kernel<6>, dim3(n_items), dim3(kWallBlock), 0, stream, d_items, d_tiles,
-                               a, partial);
-        else
-            hipLaunchKernelGGL(rpy_pairs_wall_kernel<3>, dim3(n_items), dim3(kWallBlock), 0, stream, d_items, d_tiles,
-                               a, partial);
+        const dim3 grid((unsigned)n_items), block(kWallBlock);
+        if (precision == 0) {
+            if (seg_dof == 6)
+                hipLaunchKernelGGL((rpy_pairs_wall_kernel<6, double>), grid, block, 0, stream, d_items, d_tiles, a,
+                                   partial);
+            else
+                hipLaunchKernelGGL((rpy_pairs_wall_kernel<3, double>), grid, block, 0, stream, d_items, d_tiles, a,
+                                   partial);
+        } else {
+            if (seg_dof == 6)
+                hipLaunchKernelGGL((rpy_pairs_wall_kernel<6, float>), grid, block, 0, stream, d_items, d_tiles, a, pf);
+            else
+                hipLaunchKernelGGL((rpy_pairs_wall_kernel<3, float>), grid, block, 0, stream, d_items, d_tiles, a, pf);
+        }
         TFS_HIP_CHECK(hipGetLastError());
         if (ev_end) TFS_HIP_CHECK(hipEventRecord(ev_end, stream));
     }
     const int tile_blocks = n_tiles * ((kWallBlock + 63) / 64);
     const int fil_blocks = o.sad.in ? (o.sad.nfil + 3) / 4 : 0;
     const NearList none{};
-    if (seg_dof == 6)
-        hipLaunchKernelGGL((rpy_reduce_kernel<6, double>), dim3(tile_blocks + fil_blocks), dim3(256), 0, stream,
-                           d_tiles, (const double*)partial, kWallBlock, tile_blocks, o, a, none);
-    else
-        hipLaunchKernelGGL((rpy_reduce_kernel<3, double>), dim3(tile_blocks + fil_blocks), dim3(256), 0, stream,
-                           d_tiles, (const double*)partial, kWallBlock, tile_blocks, o, a, none);
+    const dim3 grid((unsigned)(tile_blocks + fil_blocks));
+    if (precision == 0) {
+        if (seg_dof == 6)
+            hipLaunchKernelGGL((rpy_reduce_kernel<6, double>), grid, dim3(256), 0, stream, d_tiles,
+                               (const double*)partial, kWallBlock, tile_blocks, o, a, none);
+        else
+            hipLaunchKernelGGL((rpy_reduce_kernel<3, double>), grid, dim3(256), 0, stream, d_tiles,
+                               (const double*)partial, kWallBlock, tile_blocks, o, a, none);
+    } else {
+        if (seg_dof == 6)
+            hipLaunchKernelGGL((rpy_reduce_kernel<6, float>), grid, dim3(256), 0, stream, d_tiles, (const float*)pf,
+                               kWallBlock, tile_blocks, o, a, none);
+        else
+            hipLaunchKernelGGL((rpy_reduce_kernel<3, float>), grid, dim3(256), 0, stream, d_tiles, (const float*)pf,
+                               kWallBlock, tile_blocks, o, a, none);
+    }
     return hipGetLastError();
 }
 

@@ -248,24 +248,26 @@ __device__ __forceinline__ void pair_near(double* __restrict__ v, double dx, dou
 }
 
 // Half-space pair tensor (no-slip wall at z = 0; cuda_functions.cu:12-203,
-// Swan & Brady 2007, radius 1 and viscosity 1 as the reference assumes), in
-// FP64.  Accumulates into v the velocity (VT = 3) or velocity and angular
-// velocity (VT = 6) of particle i at xi due to the force (FS = 3) or force
-// and torque (FS = 6) f on particle j at xj; `same` marks i and j as one
+// Swan & Brady 2007, radius 1 and viscosity 1 as the reference assumes),
+// evaluated in T (double, or float as the reference's Real = float build
+// does).  Accumulates into v the velocity (VT = 3) or velocity and angular
+// velocity (VT = 6) of particle i due to the force (FS = 3) or force and
+// torque (FS = 6) f on particle j, given their separation (dx, dy, dz) =
+// xi - xj and heights zi, zj above the wall; `same` marks i and j as one
 // particle (the wall self-mobility).  Distinct particles take the unbounded
 // far-field form plus the image corrections; as in the reference, the
 // rotation-translation wall terms enter with the opposite sign to the
 // paper's printed alternating-tensor convention.
-template <int VT, int FS>
-__device__ inline void pair_wall(double* __restrict__ v, bool same, const double* xi, const double* xj,
-                                 const double* __restrict__ f) {
+template <typename T, int VT, int FS>
+__device__ inline void pair_wall(T* __restrict__ v, bool same, T dx, T dy, T dz, T zi, T zj,
+                                 const T* __restrict__ f) {
     const double PI = 3.14159265358979323846;
-    const double k6 = 1.0 / (6.0 * PI), k8 = 1.0 / (8.0 * PI);
+    const T k6 = T(1.0 / (6.0 * PI)), k8 = T(1.0 / (8.0 * PI));
     if (same) {
-        const double g1 = 1.0 / xi[2], g3 = g1 * g1 * g1, g4 = g3 * g1, g5 = g4 * g1;
-        const double tpar = k6 - (9.0 * g1 - 2.0 * g3 + g5) * (k6 / 16.0);
-        const double tperp = k6 - (9.0 * g1 - 4.0 * g3 + g5) * (k6 / 8.0);
-        const double cpl = g4 * (k6 * 3.0 / 32.0);
+        const T g1 = T(1) / zi, g3 = g1 * g1 * g1, g4 = g3 * g1, g5 = g4 * g1;
+        const T tpar = k6 - (T(9.0) * g1 - T(2.0) * g3 + g5) * (k6 / T(16.0));
+        const T tperp = k6 - (T(9.0) * g1 - T(4.0) * g3 + g5) * (k6 / T(8.0));
+        const T cpl = g4 * (k6 * T(3.0) / T(32.0));
         v[0] += tpar * f[0];
         v[1] += tpar * f[1];
         v[2] += tperp * f[2];
@@ -277,8 +279,8 @@ __device__ inline void pair_wall(double* __restrict__ v, bool same, const double
             v[3] -= cpl * f[1];
             v[4] += cpl * f[0];
             if constexpr (FS == 6) {
-                const double rpar = k8 - g3 * (k6 * 15.0 / 64.0);
-                const double rperp = k8 - g3 * (k6 * 3.0 / 32.0);
+                const T rpar = k8 - g3 * (k6 * T(15.0) / T(64.0));
+                const T rperp = k8 - g3 * (k6 * T(3.0) / T(32.0));
                 v[3] += rpar * f[3];
                 v[4] += rpar * f[4];
                 v[5] += rperp * f[5];
@@ -288,15 +290,14 @@ __device__ inline void pair_wall(double* __restrict__ v, bool same, const double
     }
     // unbounded far field between the particles
     {
-        const double dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
-        const double q = rsqrt_refined(dx * dx + dy * dy + dz * dz);
-        const double ux = dx * q, uy = dy * q, uz = dz * q;
-        const double uf = ux * f[0] + uy * f[1] + uz * f[2];
-        const double ca = k8 * q, cb = 1.0 + (2.0 / 3.0) * q * q, cc = (1.0 - 2.0 * q * q) * uf;
+        const T q = rsqrt_refined(dx * dx + dy * dy + dz * dz);
+        const T ux = dx * q, uy = dy * q, uz = dz * q;
+        const T uf = ux * f[0] + uy * f[1] + uz * f[2];
+        const T ca = k8 * q, cb = T(1.0) + (T(2.0) / T(3.0)) * q * q, cc = (T(1.0) - T(2.0) * q * q) * uf;
         v[0] += ca * (cb * f[0] + cc * ux);
         v[1] += ca * (cb * f[1] + cc * uy);
         v[2] += ca * (cb * f[2] + cc * uz);
-        const double c2 = ca * q;
+        const T c2 = ca * q;
         if constexpr (FS == 6) {
             v[0] += c2 * (f[4] * uz - f[5] * uy);
             v[1] += c2 * (f[5] * ux - f[3] * uz);
@@ -307,45 +308,45 @@ __device__ inline void pair_wall(double* __restrict__ v, bool same, const double
             v[4] += c2 * (f[2] * ux - f[0] * uz);
             v[5] += c2 * (f[0] * uy - f[1] * ux);
             if constexpr (FS == 6) {
-                const double c3 = 0.5 * c2 * q, ut = ux * f[3] + uy * f[4] + uz * f[5];
-                v[3] += c3 * (3.0 * ut * ux - f[3]);
-                v[4] += c3 * (3.0 * ut * uy - f[4]);
-                v[5] += c3 * (3.0 * ut * uz - f[5]);
+                const T c3 = T(0.5) * c2 * q, ut = ux * f[3] + uy * f[4] + uz * f[5];
+                v[3] += c3 * (T(3.0) * ut * ux - f[3]);
+                v[4] += c3 * (T(3.0) * ut * uy - f[4]);
+                v[5] += c3 * (T(3.0) * ut * uz - f[5]);
             }
         }
     }
     // image corrections: separation from the image of j at (xj, yj, -zj)
-    const double Rx = xi[0] - xj[0], Ry = xi[1] - xj[1], Rz = xi[2] + xj[2];
-    const double p1 = rsqrt_refined(Rx * Rx + Ry * Ry + Rz * Rz);
-    const double p2 = p1 * p1, p3 = p2 * p1, p4 = p3 * p1, p5 = p4 * p1;
-    const double ex = Rx * p1, ey = Ry * p1, ez = Rz * p1, ez2 = ez * ez;
-    const double h = xj[2] / (xi[2] + xj[2]);
-    const double ef = ex * f[0] + ey * f[1] + ez * f[2];
+    const T Rx = dx, Ry = dy, Rz = zi + zj;
+    const T p1 = rsqrt_refined(Rx * Rx + Ry * Ry + Rz * Rz);
+    const T p2 = p1 * p1, p3 = p2 * p1, p4 = p3 * p1, p5 = p4 * p1;
+    const T ex = Rx * p1, ey = Ry * p1, ez = Rz * p1, ez2 = ez * ez;
+    const T h = zj / Rz;
+    const T ef = ex * f[0] + ey * f[1] + ez * f[2];
     {
-        const double t0 = -(0.75 * p1 * (1.0 + 2.0 * h * (1.0 - h) * ez2) +
-                            0.5 * (p3 * (1.0 - 3.0 * ez2) - p5 * (1.0 - 5.0 * ez2)));
-        const double t12 = -(0.75 * p1 * (1.0 - 6.0 * h * (1.0 - h) * ez2) - 1.5 * p3 * (1.0 - 5.0 * ez2) +
-                             2.5 * p5 * (1.0 - 7.0 * ez2)) * ef +
+        const T t0 = -(T(0.75) * p1 * (T(1.0) + T(2.0) * h * (T(1.0) - h) * ez2) +
+                            T(0.5) * (p3 * (T(1.0) - T(3.0) * ez2) - p5 * (T(1.0) - T(5.0) * ez2)));
+        const T t12 = -(T(0.75) * p1 * (T(1.0) - T(6.0) * h * (T(1.0) - h) * ez2) - T(1.5) * p3 * (T(1.0) - T(5.0) * ez2) +
+                             T(2.5) * p5 * (T(1.0) - T(7.0) * ez2)) * ef +
                            ez * f[2] *
-                               (1.5 * h * p1 * (1.0 - 6.0 * (1.0 - h) * ez2) - 3.0 * p3 * (1.0 - 5.0 * ez2) +
-                                5.0 * p5 * (2.0 - 7.0 * ez2));
-        const double t3 = (1.5 * h * p1 - 5.0 * p5) * ef;
-        const double t4 = -((3.0 * h * h * p1 + 3.0 * p3) * ez2 + (2.0 - 15.0 * ez2) * p5) * f[2];
+                               (T(1.5) * h * p1 * (T(1.0) - T(6.0) * (T(1.0) - h) * ez2) - T(3.0) * p3 * (T(1.0) - T(5.0) * ez2) +
+                                T(5.0) * p5 * (T(2.0) - T(7.0) * ez2));
+        const T t3 = (T(1.5) * h * p1 - T(5.0) * p5) * ef;
+        const T t4 = -((T(3.0) * h * h * p1 + T(3.0) * p3) * ez2 + (T(2.0) - T(15.0) * ez2) * p5) * f[2];
         v[0] += k6 * (t0 * f[0] + t12 * ex);
         v[1] += k6 * (t0 * f[1] + t12 * ey);
         v[2] += k6 * (t0 * f[2] + (t12 + t3) * ez + t4);
     }
     if constexpr (VT == 6) {
-        const double u0 = 0.75 * p2;
-        const double u12 = f[2] * (9.0 * h * ez2 * p2 + (1.5 - 15.0 * ez2) * p4) - ez * ef * (4.5 * h * p2 - 7.5 * p4);
-        const double u3 = -1.5 * ez * (h * p2 - p4);
+        const T u0 = T(0.75) * p2;
+        const T u12 = f[2] * (T(9.0) * h * ez2 * p2 + (T(1.5) - T(15.0) * ez2) * p4) - ez * ef * (T(4.5) * h * p2 - T(7.5) * p4);
+        const T u3 = -T(1.5) * ez * (h * p2 - p4);
         v[3] -= k6 * (u0 * (f[1] * ez - f[2] * ey) - u12 * ey + u3 * f[1]);
         v[4] -= k6 * (u0 * (f[2] * ex - f[0] * ez) + u12 * ex - u3 * f[0]);
         v[5] -= k6 * u0 * (f[0] * ey - f[1] * ex);
         if constexpr (FS == 6) {
-            const double et = ex * f[3] + ey * f[4] + ez * f[5];
-            const double w0 = p3 * (0.375 - 2.25 * ez2), w1 = -1.125 * p3 * et, w2 = 2.25 * ez * p3 * et;
-            const double w3 = 2.25 * p3 * (ex * f[4] - ey * f[3]);
+            const T et = ex * f[3] + ey * f[4] + ez * f[5];
+            const T w0 = p3 * (T(0.375) - T(2.25) * ez2), w1 = -T(1.125) * p3 * et, w2 = T(2.25) * ez * p3 * et;
+            const T w3 = T(2.25) * p3 * (ex * f[4] - ey * f[3]);
             v[3] += k6 * (w0 * f[3] + w1 * ex - w3 * ey);
             v[4] += k6 * (w0 * f[4] + w1 * ey + w3 * ex);
             v[5] += k6 * (w0 * f[5] + w1 * ez + w2);
@@ -354,12 +355,12 @@ __device__ inline void pair_wall(double* __restrict__ v, bool same, const double
     if constexpr (FS == 6) {
         // translation from torque: transpose of the rotation-from-force block
         // seen from j (image weight 1 - h, e_x and e_y reversed)
-        const double hh = 1.0 - h, fx = -ex, fy = -ey;
-        const double ext = fx * f[4] - fy * f[3];
-        const double q0 = 0.75 * p2;
-        const double q1 = (9.0 * hh * ez2 * p2 + (1.5 - 15.0 * ez2) * p4) * ext;
-        const double q2 = -ez * (4.5 * hh * p2 - 7.5 * p4) * ext;
-        const double q3 = -1.5 * ez * (hh * p2 - p4);
+        const T hh = T(1.0) - h, fx = -ex, fy = -ey;
+        const T ext = fx * f[4] - fy * f[3];
+        const T q0 = T(0.75) * p2;
+        const T q1 = (T(9.0) * hh * ez2 * p2 + (T(1.5) - T(15.0) * ez2) * p4) * ext;
+        const T q2 = -ez * (T(4.5) * hh * p2 - T(7.5) * p4) * ext;
+        const T q3 = -T(1.5) * ez * (hh * p2 - p4);
         v[0] -= k6 * (q0 * (f[5] * fy - f[4] * ez) + q2 * fx - q3 * f[4]);
         v[1] -= k6 * (q0 * (f[3] * ez - f[5] * fx) + q2 * fy + q3 * f[3]);
         v[2] -= k6 * (q0 * (f[4] * fx - f[3] * fy) + q2 * ez + q1);

@@ -173,6 +173,7 @@ struct FilParams {
     double dl, kb, kt, mu, a, dt, end_force;
     double st[3];
     int pert_a, pert_b;
+    int base_mode;  // TFS_BASE_CLAMPED or TFS_BASE_DRIVEN
 };
 
 // Body-frame elastic moment of the link q1 -> q2 (filament.cpp:1451-1465).
@@ -238,6 +239,8 @@ struct FilState {
     double *bases;                     // 3 per filament
     double *pert;                      // 6 per filament (may be null)
     double *clampblk;                  // 18 per filament: elastic clamping blocks 1, 2 (row-major)
+    double *base_val;                  // 3 per filament (may be null): clamped base: its turn per
+                                       // step; driven base: the torque on segment 0
     // A free body per swimmer (all null while the bodies are held still):
     // position at n+1, n, n-1; Lie-algebra element at n+1, n; quaternion at
     // n+1, n; Q_init = R(q) of the initial guess (3, 3, 3, 3, 3, 4, 4, 9 each).
@@ -312,7 +315,15 @@ __global__ void initial_guess_kernel(FilState S, FilParams P, int nt) {
         }
         const Q q = load_q(S.q + 4 * g);
         store_q(S.qm1 + 4 * g, q);
-        if (i == 0 && !S.bx) {
+        if (i == 0 && !S.bx && P.base_mode == TFS_BASE_DRIVEN) {
+            // a driven base segment is free: its u was extrapolated above
+            store_q(S.q + 4 * g, qmul(lie_exp(S.u + 3 * g), q));
+        } else if (i == 0 && !S.bx && S.base_val) {
+            // clamped, turned by a fixed element every step
+#pragma unroll
+            for (int k = 0; k < 3; ++k) S.u[3 * g + k] = S.base_val[3 * f + k];
+            store_q(S.q + 4 * g, qmul(lie_exp(S.u + 3 * g), q));
+        } else if (i == 0 && !S.bx) {
             // clamped to a body that does not turn: u_0 = 0, q_0 unchanged
 #pragma unroll
             for (int k = 0; k < 3; ++k) S.u[3 * g + k] = 0.0;
@@ -392,8 +403,15 @@ __global__ void forces_kernel(FilState S, FilParams P, int nt, double* __restric
         }
     }
     if (i == 0) {
-        double ct[3];
-        dexpinv_transpose(S.u + 3 * g, mult + 3 * N, ct);
+        // tether force; the clamp torque (filament.cpp:1555-1571) or the
+        // driving torque of a free base (:1523-1531)
+        double ct[3] = {0.0, 0.0, 0.0};
+        if (P.base_mode == TFS_BASE_DRIVEN) {
+            if (S.base_val)
+                for (int k = 0; k < 3; ++k) ct[k] = S.base_val[3 * f + k];
+        } else {
+            dexpinv_transpose(S.u + 3 * g, mult + 3 * N, ct);
+        }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             F[k] += mult[k];
@@ -446,6 +464,9 @@ __global__ __launch_bounds__(64) void jacobian_kernel(FilState S, FilParams P, i
     const double Tfac = 1.0 / (8.0 * pi * mu * a * a * a);
     M3 I3 = zero3();
     I3.a[0] = I3.a[4] = I3.a[8] = 1.0;
+    const bool driven = P.base_mode == TFS_BASE_DRIVEN;
+    double t0[3];
+    tangent(load_q(S.q + 4 * s0), t0);
     for (int k = threadIdx.x; k < N; k += blockDim.x) {
         const int64_t g = s0 + k;
         const Q qk = load_q(S.q + 4 * g);
@@ -464,6 +485,9 @@ __global__ __launch_bounds__(64) void jacobian_kernel(FilState S, FilParams P, i
             if (k == N - 1 && j == N - 1) B = axpy(-fac * P.end_force, rcross(tj), B);
             put3(J, n6, 3 * k, 3 * (N + j), B);
         }
+        // a free base turns every later segment with its tangent
+        // (filament.cpp:1789-1800)
+        if (driven && k >= 1) put3(J, n6, 3 * k, 3 * N, scale(0.5 * P.dl, rcross(t0)));
         // Lie rows, multiplier columns 3k (k >= 1) and 3(k+1) (k+1 <= N-1)
         const M3 um = rcross(S.u + 3 * g);
         const M3 tm = scale(-1.0, rcross(t));
@@ -505,7 +529,7 @@ __global__ __launch_bounds__(64) void jacobian_kernel(FilState S, FilParams P, i
             B = axpy(1.0 / 12.0, mm(L, mm(L, left)), B);
             // segment 1's block on the clamp column is zero (filament.cpp:1887);
             // its value before that is elastic_clamping_block2
-            if (k != 1) {
+            if (k != 1 || driven) {
                 put3(J, n6, 3 * N + 3 * k, 3 * N + 3 * k - 3, scale(-tsf, B));
             } else if (S.clampblk) {
                 const M3 c2 = scale(-tsf, B);
@@ -514,7 +538,7 @@ __global__ __launch_bounds__(64) void jacobian_kernel(FilState S, FilParams P, i
             }
             for (int c = 0; c < 3; ++c) lamsum[c] += mult[3 * k + c];
         }
-        if (k == 0) {
+        if (k == 0 && !driven) {
             // the clamp multiplier replaces u_0 (filament.cpp:1866-1888)
             const M3 blk = axpy(1.0 / 12.0, mm(um, um), axpy(0.5, um, I3));
             const M3 blkT = axpy(1.0 / 12.0, mm(um, um), axpy(-0.5, um, I3));
@@ -531,7 +555,7 @@ __global__ __launch_bounds__(64) void jacobian_kernel(FilState S, FilParams P, i
             const M3 hi = axpy(1.0, rcross(ou), axpy(1.0, mm(L, Om), mm(L, mm(L, dOm))));
             B = axpy(1.0 / 12.0, hi, B);
             const M3 diag = axpy(-tsf, B, I3);
-            if (k > 0) {
+            if (k > 0 || driven) {
                 put3(J, n6, 3 * N + 3 * k, 3 * N + 3 * k, diag);
             } else {
 #pragma unroll
@@ -556,6 +580,13 @@ __global__ void update_kernel(FilState S, FilParams P, const double* __restrict_
     double* m = S.mult + (int64_t)f * nm;
 #pragma unroll
     for (int k = 0; k < 3; ++k) m[3 * i + k] += d[3 * i + k];   // tether (i = 0) or lambda_{i-1}
+    if (i == 0 && P.base_mode == TFS_BASE_DRIVEN) {
+        // a free base: u_0 takes the clamp's place (filament.cpp:1918-1927)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) S.u[3 * g + k] += d[3 * N + k];
+        store_q(S.q + 4 * g, qmul(lie_exp(S.u + 3 * g), load_q(S.qm1 + 4 * g)));
+        return;
+    }
     if (i == 0) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) m[3 * N + k] += d[3 * N + k];   // clamp
@@ -911,6 +942,7 @@ struct tfs_filament {
     // free bodies (tfs_filament_set_body)
     double* body_arena = nullptr;
     double *blob_refs = nullptr, *n_ref = nullptr, *n_rot = nullptr, *part = nullptr;
+    double* base_slab = nullptr;  // 3 per filament, in the arena (tfs_filament_set_base)
     int64_t nblob = 0;
     int32_t blobs_per_swim = 0;
     bool jacobian_done = false;
@@ -976,12 +1008,13 @@ int tfs_filament_create(const tfs_filament_desc* desc, tfs_filament** out) {
     P.end_force = D.end_force;
     for (int k = 0; k < 3; ++k) P.st[k] = D.strain_twist[k];
     P.pert_a = P.pert_b = -1;
+    P.base_mode = TFS_BASE_CLAMPED;
     const int64_t ns = num_segs(h), nf = P.nfil, nm = 3LL * (P.nseg + 1) * nf;
     std::vector<std::pair<double**, int64_t>> parts = {
         {&h->S.q, 4 * ns},   {&h->S.qm1, 4 * ns},  {&h->S.u, 3 * ns},      {&h->S.um1, 3 * ns},
         {&h->S.x, 3 * ns},   {&h->S.xm1, 3 * ns},  {&h->S.xm2, 3 * ns},    {&h->S.mult, nm},
         {&h->S.mult_m1, nm}, {&h->S.mult_m2, nm},  {&h->S.bases, 3 * nf},  {&h->S.pert, 6 * nf},
-        {&h->S.clampblk, 18 * nf}};
+        {&h->S.clampblk, 18 * nf}, {&h->base_slab, 3 * nf}};
     int64_t total = 0;
     for (auto& p : parts) total += (p.second + 31) / 32 * 32;
     if (hipMalloc(&h->arena, total * 8) != hipSuccess) {
@@ -1050,6 +1083,25 @@ int tfs_filament_set_perturbation(tfs_filament* h, int32_t seg_a, int32_t seg_b,
     h->S.pert = pert;
     h->P.pert_a = seg_a;
     h->P.pert_b = seg_b;
+    return TFS_OK;
+}
+
+int tfs_filament_set_base(tfs_filament* h, int32_t mode, const double* values) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_set_base: null handle");
+    if (mode != TFS_BASE_CLAMPED && mode != TFS_BASE_DRIVEN)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_set_base: unknown base mode");
+    if (mode == TFS_BASE_DRIVEN && h->S.bx)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_set_base: a driven base needs a body held still");
+    hipError_t e = hipSuccess;
+    if (values) {
+        const hipMemcpyKind k = h->desc.memory == TFS_MEMORY_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+        e = hipMemcpyAsync(h->base_slab, values, 24LL * h->P.nfil, k, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return hip_fail(e, "tfs_filament_set_base");
+    }
+    h->S.base_val = values ? h->base_slab : nullptr;
+    h->P.base_mode = mode;
+    h->jacobian_done = false;
     return TFS_OK;
 }
 
@@ -1185,6 +1237,8 @@ int tfs_filament_set_body(tfs_filament* h, const double* x_bodies, const double*
     if (!h || !x_bodies || !q_bodies || !fil_refs || (blobs_per_swim > 0 && !blob_refs) || blobs_per_swim < 0)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_set_body: null argument or negative blob count");
     const int64_t nsw = h->desc.num_swim, nf = h->P.nfil, nb = nsw * blobs_per_swim;
+    if (h->P.base_mode == TFS_BASE_DRIVEN)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_set_body: a driven base needs a body held still");
     if (h->body_arena && nb != h->nblob)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_set_body: the blob count cannot change");
     if (!h->body_arena) {

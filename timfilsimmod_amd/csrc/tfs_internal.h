@@ -181,13 +181,14 @@ hipError_t launch_matvec(int precision, int seg_dof, const TileShape& shape, con
                          const PersistState& ps, hipStream_t stream, hipEvent_t ev_begin = nullptr,
                          hipEvent_t ev_end = nullptr);
 
-// Half-space domain (no-slip wall at z = 0): FP64 pair kernel over a work
-// list of kWallBlock-target tiles (one target per lane), then the reduce
-// kernel without near-pair terms.
+// Half-space domain (no-slip wall at z = 0): the wall pair kernel (FP64, or
+// FP32 tiles from fixed-point separations) over a work list of
+// kWallBlock-target tiles (one target per lane), then the reduce kernel
+// without near-pair terms.
 constexpr int kWallBlock = 256;
-hipError_t launch_matvec_wall(int seg_dof, const MatvecArgs& a, const OutArgs& o, const MatvecTile* d_tiles,
-                              int n_tiles, const MatvecItem* d_items, int n_items, double* partial,
-                              hipStream_t stream, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+hipError_t launch_matvec_wall(int precision, int seg_dof, const MatvecArgs& a, const OutArgs& o,
+                              const MatvecTile* d_tiles, int n_tiles, const MatvecItem* d_items, int n_items,
+                              double* partial, hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end);
 
 // Flow-field evaluation (Stokeslet + rotlet mode and RPY mode).
 struct FlowArgs {

@@ -385,8 +385,6 @@ void free_plans(tfs_rpy* h) {
 int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, View vs, View vb,
                int seg_out_slots, const SaddleEpilogue* sad) {
     const bool wall = h->domain == TFS_DOMAIN_HALF_SPACE;
-    if (wall && precision != TFS_PRECISION_FP64)
-        return fail(TFS_ERR_INVALID_ARGUMENT, "half-space domain: FP64 tiles only");
     TileShape sh = h->shape[precision];
     if (wall) {
         // one target per lane, ~16 workgroups per CU over the work list
@@ -402,6 +400,7 @@ int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, V
     if (!p) return err;
     if (p->tiles.empty()) return TFS_OK;
     if (!wall) err = ensure_near(h, precision);
+    else if (precision == TFS_PRECISION_FP32_ACC64) err = ensure_fixed(h);  // separations in fixed point
     if (err) return err;
     MatvecArgs a = matvec_args(h, fs, fb);
     OutArgs o;
@@ -434,7 +433,7 @@ int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, V
         ps.item_stamps = h->d_item_stamps;
         h->stamped_plan = p;
     }
-    hipError_t e = wall ? launch_matvec_wall(h->desc.seg_dof, a, o, p->d_tiles, (int)p->tiles.size(), p->d_items,
+    hipError_t e = wall ? launch_matvec_wall(precision, h->desc.seg_dof, a, o, p->d_tiles, (int)p->tiles.size(), p->d_items,
                                              (int)p->items.size(), h->partial, h->stream, eb, ee)
                         : launch_matvec(precision, h->desc.seg_dof, sh, a, o, p->d_tiles, (int)p->tiles.size(),
                                         p->d_items, (int)p->items.size(), h->partial, h->near[precision], ps,

@@ -50,7 +50,11 @@ L.extra_signatures({
     "tfs_elastic_backup_size": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "tfs_elastic_backup": (ctypes.c_int, [_vp, _vp, _vp]),
     "tfs_elastic_restore": (ctypes.c_int, [_vp, _vp]),
+    "tfs_filament_set_base": (ctypes.c_int, [_vp, _i32, _vp]),
 })
+
+BASE_CLAMPED = 0   # CILIA_TYPE 0 (INSTABILITY_CILIA), and 2 (CONSTANT_BASE_ROTATION) with values
+BASE_DRIVEN = 1    # CILIA_TYPE 1 (GEOMETRIC_CILIA): free base segment with a driving torque
 
 
 class FilamentModel:
@@ -103,6 +107,14 @@ class FilamentModel:
     def set_perturbation(self, seg_a, seg_b, forces):
         self._pert = self._arr(forces)
         L.check(L.lib().tfs_filament_set_perturbation(self.h, int(seg_a), int(seg_b), L.ptr(self._pert)))
+
+    def set_base(self, mode, values=None):
+        """tfs_filament_set_base: BASE_CLAMPED with values = per filament the
+        base's turn per step [3] (None: held still), or BASE_DRIVEN with values
+        = per filament the lab-frame torque on the base segment [3]."""
+        self._base = None if values is None else self._arr(values)
+        L.check(L.lib().tfs_filament_set_base(self.h, int(mode), L.ptr(self._base) if self._base is not None
+                                              else None))
 
     def set_body(self, x_bodies, q_bodies, fil_refs, blobs_per_swim=0, blob_refs=None):
         """Free bodies (swimmer.cpp:433-467, 1072-1098): each swimmer's body
