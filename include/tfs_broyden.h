@@ -66,6 +66,19 @@ int tfs_broyden_set_jacobian(tfs_broyden *b, const double *blocks);
  * tfs_broyden_set_jacobian or _inverse removes the coupling. */
 int tfs_broyden_set_body_coupling(tfs_broyden *b, int32_t num_swim, int32_t fil_per_swim, const double *A,
                                   const double *B, const double *S0);
+/* Free bodies in the Broyden-for-everything form (swimmer::jacobian_inv_mult,
+ * swimmer.cpp:931-1010): the blocks are, per swimmer, fil_per_swim filament
+ * blocks, one of 6 and blobs_per_swim blob blocks of 3 (their J0 blocks
+ * -I / (6 pi mu a), inverse blob_mob_fac I); J0^{-1} eliminates the blob rows
+ * with that diagonal mobility and couples the body through the Schur matrix
+ * S0 - sum B J^{-1} A, where S0 carries the blob terms (as
+ * tfs_filament_body_coupling_blobs writes it).  blob_disp [3 per blob]: the
+ * blobs' displacements Q_init ref; blob_mob_fac = -6 pi mu a; dt_fac = 1/DT
+ * (backward Euler) or 1.5/DT (BDF2). */
+int tfs_broyden_set_body_coupling_blobs(tfs_broyden *b, int32_t num_swim, int32_t fil_per_swim,
+                                        int32_t blobs_per_swim, const double *A, const double *B,
+                                        const double *S0, const double *blob_disp, double blob_mob_fac,
+                                        double dt_fac);
 /* Start a timestep: iter = 0 and error = the residual of the initial guess
  * (compute_errors before the loop, cilia_sim_main.cu:394-411).  error == NULL
  * keeps the residual that tfs_broyden_compute_errors left in slot 0. */
@@ -101,6 +114,16 @@ typedef struct tfs_broyden_state {
      * form above. */
     int32_t blobs_per_swim;
     const double *x_blobs, *v_blobs;           /* 3 per blob                                  */
+    /* free_bodies (with blobs_per_swim > 0): free bodies in the Broyden-for-everything
+     * form (PRESCRIBED_BODY_VELOCITIES false, mobility_solver.cpp:2676-2681, 2786-2847):
+     * the body rows are its force and torque balance -- the segment forces f_segs and
+     * their torques about body_x (swimmer::forces_and_torques, swimmer.cpp:469-499) plus
+     * the blob forces f_blobs and theirs, tolerance TOL -- and the blob rows take the
+     * velocities the body's positions imply (backward Euler, or BDF2 with dexp);
+     * v_bodies is then unused. */
+    int32_t free_bodies;
+    const double *f_segs;                      /* 6 per segment                               */
+    const double *f_blobs;                     /* 3 per blob                                  */
 } tfs_broyden_state;
 
 /* Residuals of the BDF2 (or backward-Euler) equations for positions and

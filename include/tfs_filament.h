@@ -128,6 +128,18 @@ int tfs_filament_blob_positions(tfs_filament *h, double *x_blobs);
  * so for spherical bodies, swimmer.cpp:818-843); 0 leaves them out. */
 int tfs_filament_body_coupling(tfs_filament *h, int32_t nt, const double *body_mobility, double sphere_radius,
                                double *A, double *B, double *S0);
+/* The same for the Broyden-for-everything form with free bodies
+ * (swimmer::prepare_jacobian_inv, swimmer.cpp:565-676), after
+ * tfs_filament_jacobian and with the blobs given to tfs_filament_set_body: A
+ * as above (the filaments' rows, the body's columns); B per filament the body's
+ * force and torque balance against its unknowns (6 x 6 NSEG; the reference's
+ * D); S0 per swimmer its 6 x 6 matrix before the Schur corrections -- the
+ * blobs' terms with the diagonal blob mobility -6 pi mu blob_radius / DT (or
+ * -9 pi mu a / DT in BDF2 steps) and the blob forces f_blobs [3 per blob], and
+ * the filaments' terms; blob_disp [3 per blob] the blobs' displacements
+ * Q_init ref.  For tfs_broyden_set_body_coupling_blobs. */
+int tfs_filament_body_coupling_blobs(tfs_filament *h, int32_t nt, const double *f_blobs, double blob_radius,
+                                     double *A, double *B, double *S0, double *blob_disp);
 
 /* Waits for the handle's work (device-memory outputs are then complete). */
 int tfs_filament_sync(tfs_filament *h);

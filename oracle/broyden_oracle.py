@@ -134,16 +134,21 @@ def dexpinv(u, v):
     return (v - 0.5 * c) - fac * w
 
 
-def compute_errors(st, num_swim, num_fil, num_seg, bdf2, dt, dl, tol, blobs_per_swim=0):
+def compute_errors(st, num_swim, num_fil, num_seg, bdf2, dt, dl, tol, blobs_per_swim=0, free_bodies=False):
     """(error, error_is_too_large) of mobility_solver.cpp:2661-2946.
 
     ``st`` holds seg_x, seg_xm1, seg_xm2, seg_u, seg_um1 (3 per segment),
     v_segs (6 per segment) and body_x, body_xm1, body_xm2, body_u, body_um1
     (3 per body), v_bodies (6 per body).  blobs_per_swim > 0 is the
-    Broyden-for-everything form with bodies of prescribed velocity
-    (USE_BROYDEN_FOR_EVERYTHING, :2674-2684, 2774-2850): per swimmer the 3 NBLOB
-    blob rows V + Omega x (x_blob - x_body) - v_blob follow the body's rows;
-    st then also holds x_blobs and v_blobs (3 per blob)."""
+    Broyden-for-everything form (USE_BROYDEN_FOR_EVERYTHING, :2674-2684,
+    2774-2850): per swimmer the 3 NBLOB blob rows V + Omega x (x_blob - x_body)
+    - v_blob follow the body's rows; st then also holds x_blobs and v_blobs (3
+    per blob).  With prescribed body velocities (free_bodies False) V, Omega
+    are v_bodies and the body rows its position / Lie-algebra equations; with
+    free bodies the body rows are its force and torque balance
+    -swimmer.f + sum (f_blob, (x_blob - x_body) x f_blob) (:2676-2681,
+    2837-2847; st then holds f_segs, 6 per segment, and f_blobs, 3 per blob),
+    and V, Omega are the velocities the body's positions imply (:2786-2806)."""
     c1, c2, c3 = -4.0 / 3.0, 1.0 / 3.0, -2.0 / 3.0
     per = 6 * (num_fil * num_seg + 1) + 3 * blobs_per_swim
     n = num_swim * per
@@ -178,13 +183,40 @@ def compute_errors(st, num_swim, num_fil, num_seg, bdf2, dt, dl, tol, blobs_per_
                                  g3("seg_um1", g), vs[6 * g:6 * g + 3], vs[6 * g + 3:6 * g + 6], pid,
                                  pid + 3 * num_seg)
         pid = base + 6 * num_fil * num_seg
-        too_large |= one(g3("body_x", s), g3("body_xm1", s), g3("body_xm2", s), g3("body_u", s),
-                         g3("body_um1", s), vb[6 * s:6 * s + 3], vb[6 * s + 3:6 * s + 6], pid, pid + 3)
+        X = g3("body_x", s)
+        if free_bodies and blobs_per_swim:
+            fs = np.asarray(st["f_segs"], dtype=np.float64)
+            fb = np.asarray(st["f_blobs"], dtype=np.float64)
+            eb = np.zeros(6)          # -swimmer.f: the segments' force and torque about X
+            for i in range(num_fil):
+                for j in range(num_seg):
+                    g = j + num_seg * (i + s * num_fil)
+                    Fg, Tg = fs[6 * g:6 * g + 3], fs[6 * g + 3:6 * g + 6]
+                    eb[:3] += Fg
+                    eb[3:] += Tg + np.cross(g3("seg_x", g) - X, Fg)
+            for m in range(blobs_per_swim):
+                g = s * blobs_per_swim + m
+                fm = fb[3 * g:3 * g + 3]
+                eb[:3] += fm
+                eb[3:] += np.cross(g3("x_blobs", g) - X, fm)
+            err[pid:pid + 6] = eb
+            too_large |= bool(np.any(np.abs(eb) > tol))
+            if bdf2:
+                V = -(X + c1 * g3("body_xm1", s) + c2 * g3("body_xm2", s)) / (c3 * dt)
+                w = (c2 * g3("body_um1", s) - g3("body_u", s)) / (c3 * dt)
+                Om = dexp(g3("body_u", s), w)
+            else:
+                V = (X - g3("body_xm1", s)) / dt
+                Om = g3("body_u", s) / dt
+        else:
+            too_large |= one(X, g3("body_xm1", s), g3("body_xm2", s), g3("body_u", s),
+                             g3("body_um1", s), vb[6 * s:6 * s + 3], vb[6 * s + 3:6 * s + 6], pid, pid + 3)
+            V, Om = vb[6 * s:6 * s + 3], vb[6 * s + 3:6 * s + 6]
         for m in range(blobs_per_swim):
             g = s * blobs_per_swim + m
-            diff = g3("x_blobs", g) - g3("body_x", s)
-            w = np.cross(vb[6 * s + 3:6 * s + 6], diff)
-            e = vb[6 * s:6 * s + 3] + w - g3("v_blobs", g)
+            diff = g3("x_blobs", g) - X
+            w = np.cross(Om, diff)
+            e = V + w - g3("v_blobs", g)
             rid = base + 6 * (num_fil * num_seg + 1) + 3 * m
             err[rid:rid + 3] = e
             too_large |= bool(np.any(np.abs(e) > tol))

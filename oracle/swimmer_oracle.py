@@ -19,9 +19,13 @@ reference solves it by GMRES; a dense solve here):
   swimmer::prepare_jacobian_inv               swimmer.cpp:535-553, 688-880 (the
                                               GMRES branch, free body)
   swimmer::jacobian_inv_mult                  swimmer.cpp:915-921, 1012-1066
+  the Broyden-for-everything form, free body  swimmer.cpp:565-676 (bfe_coupling),
+                                              931-1010 (BfeInverse); the loop
+                                              free_bfe_loop
 
 Broyden vector of the swimmer: NFIL filament blocks of 6 NSEG, then the body's
-6 unknowns [position (3), Lie-algebra element (3)].
+6 unknowns [position (3), Lie-algebra element (3)] (then, Broyden for
+everything, 3 NBLOB blob forces).
 
 What pins it (tests/test_oracle_swimmer.py): the coupled inverse against a
 dense solve of the assembled block Jacobian; anchoring and quadratic
@@ -35,7 +39,12 @@ swimmer residual they linearise (self-mobility segments, the body's 6 x 6
 mobility, the sphere term from the pinned pair tensor), Euler and BDF2, with
 and without the sphere, the reference's own simplifications checked as their
 exact closed-form differences; and blob_force_guess as exact on linear and
-quadratic force histories.
+quadratic force histories.  tests/test_oracle_bfe_free.py pins the
+Broyden-for-everything form with a free body the same way: BfeInverse as the
+exact inverse of its block matrix, the blocks against central differences of
+the residual with self-mobility segments and blobs (the reference's left-out
+end-force lever and BDF2 angular-velocity terms checked as their closed
+forms), and the loop reaching the GMRES form's trajectory.
 """
 from __future__ import annotations
 
@@ -402,6 +411,164 @@ def free_swimmer_loop(b: Body, fils, refs, blob_refs, p: F.Params, steps, tol, p
             ref.end_of_iter(e, big)
         out.append({"iters": ref.iter, "converged": not big, "x": cat("x"), "body": b.copy(),
                     "forces": [fc.copy() for fc in forces], "f_blobs": lam, "v_body": V, "error": e})
+    return out
+
+
+def bfe_coupling(b: Body, fils, refs, blob_refs, f_blobs, nt, p: F.Params, blob_radius=1.0,
+                 num_euler_steps=1):
+    """swimmer::prepare_jacobian_inv, Broyden-for-everything branch with a free
+    body (swimmer.cpp:565-676), before the Schur matrix is inverted: S (6 x 6,
+    with the blob terms of the diagonal blob approximation), per filament D_n
+    (the body rows' dependence on its unknowns, 6 x 6 NSEG: force and torque
+    balance) and B_n (its rows' dependence on the body's unknowns, 6 NSEG x 6),
+    and the blob displacements d_m = Q_init ref_m; jacobian_B_blocks are
+    D_n J_n^{-1} (bfe_inverse)."""
+    N, dl, EF = p.nseg, p.dl, p.end_force
+    Q, u = b.Q_init, b.u
+    bmf = (-6.0 if nt < num_euler_steps else -9.0) * PI * p.mu * blob_radius / p.dt
+    refs_b = np.asarray(blob_refs, dtype=float).reshape(-1, 3)
+    fb = np.asarray(f_blobs, dtype=float).reshape(-1, 3)
+    nb = len(refs_b)
+    Smat = np.zeros((6, 6))
+    Smat[:3, :3] = -nb * bmf * np.eye(3)
+    disp = refs_b @ Q.T
+    for m in range(nb):
+        rd = F.rcross(disp[m])
+        Smat[3:, 3:] += F.rcross(fb[m]) @ rd
+        Smat[:3, 3:] -= bmf * rd
+        Smat[3:, :3] += bmf * rd
+        Smat[3:, 3:] += bmf * rd @ rd
+    U = F.rcross(u)
+    Ds, Bs = [], []
+    for f, ref in zip(fils, refs):
+        d = Q @ ref
+        Smat[3:, 3:] += F.rcross(f.tether) @ F.rcross(d)
+        Smat[3:, 3:] += 0.5 * F.rcross(f.clamp) + (F.rcross(np.cross(u, f.clamp)) - U @ F.rcross(f.clamp)) / 12.0
+        t_base, t_end = F.frame(f.q[0])[0], F.frame(f.q[N - 1])[0]
+        Smat[3:, 3:] += F.rcross(-EF * t_end) @ F.rcross(0.5 * dl * t_base)
+        D = np.zeros((6, 6 * N))
+        D[:3, :3] = np.eye(3)
+        D[3:, :3] = F.rcross(-d)
+        D[3:, 3 * N:3 * N + 3] = U @ U / 12.0 - 0.5 * U + np.eye(3)
+        D[:3, 6 * N - 3:] = F.rcross(-EF * t_end)
+        D[3:, 6 * N - 3:] = F.rcross(0.5 * dl * t_end - (f.x[N - 1] - b.x)) @ D[:3, 6 * N - 3:]
+        for m in range(1, N - 1):
+            D[3:, 3 * (N + m):3 * (N + m) + 3] = D[:3, 6 * N - 3:] @ F.rcross(dl * F.frame(f.q[m])[0])
+        B = np.zeros((6 * N, 6))
+        Du = F.rcross((0.5 * dl + np.linalg.norm(d)) * d / np.linalg.norm(d))
+        for m in range(N):
+            B[3 * m:3 * m + 3, :3] = np.eye(3)
+            B[3 * m:3 * m + 3, 3:] = Du
+        Ds.append(D)
+        Bs.append(B)
+    return Smat, Ds, Bs, disp
+
+
+def bfe_clamping(fils, forces, nt, p: F.Params, Bs, num_euler_steps=1):
+    """B_n's Lie-algebra rows of segments 0 and 1: the elastic clamping blocks
+    (swimmer.cpp:645-646)."""
+    for f, Fi, B in zip(fils, forces, Bs):
+        c1, c2 = clamping_blocks(f, Fi, nt >= num_euler_steps, p)
+        N = p.nseg
+        B[3 * N:3 * N + 3, 3:6] = c1
+        B[3 * N + 3:3 * N + 6, 3:6] = c2
+
+
+class BfeInverse:
+    """J0^{-1} of one swimmer in the Broyden-for-everything form with a free
+    body (swimmer::jacobian_inv_mult, swimmer.cpp:931-1010): the blob rows
+    eliminated with the diagonal blob mobility, the body's Schur matrix,
+    then the filaments."""
+
+    def __init__(self, J_blocks, Smat, Ds, Bs, disp, nt, p: F.Params, blob_radius=1.0, num_euler_steps=1):
+        self.Jinv = [np.linalg.inv(J) for J in J_blocks]
+        self.JB = [D @ Ji for D, Ji in zip(Ds, self.Jinv)]
+        Sf = Smat.copy()
+        for JB, B in zip(self.JB, Bs):
+            Sf -= JB @ B
+        self.Sinv = np.linalg.inv(Sf)
+        self.Bs, self.disp = Bs, np.asarray(disp)
+        self.fac = -6.0 * PI * p.mu * blob_radius
+        self.dt_fac = 1.0 / p.dt if nt < num_euler_steps else 1.5 / p.dt
+        self.n6 = J_blocks[0].shape[0]
+
+    def apply(self, x):
+        n6, nf, fac = self.n6, len(self.Jinv), self.fac
+        b1 = x[:n6 * nf]
+        b3 = x[n6 * nf:n6 * nf + 6].copy()
+        b2 = x[n6 * nf + 6:].reshape(-1, 3).copy()
+        for n in range(nf):
+            b3 -= self.JB[n] @ b1[n6 * n:n6 * (n + 1)]
+        for m, d in enumerate(self.disp):
+            b3[:3] -= fac * b2[m]
+            b3[3:] -= fac * np.cross(d, b2[m])
+        out = np.zeros_like(x)
+        oB = self.Sinv @ b3
+        out[n6 * nf:n6 * nf + 6] = oB
+        for m, d in enumerate(self.disp):
+            b2[m] -= self.dt_fac * (oB[:3] + np.cross(oB[3:], d))
+        out[n6 * nf + 6:] = fac * b2.reshape(-1)
+        for n in range(nf):
+            out[n6 * n:n6 * (n + 1)] = self.Jinv[n] @ (b1[n6 * n:n6 * (n + 1)] - self.Bs[n] @ oB)
+        return out
+
+
+def free_bfe_loop(b: Body, fils, refs, blob_refs, p: F.Params, steps, tol, perturbation, blob_radius=1.0,
+                  max_iter=400, confidence=0.1):
+    """The elastic time loop for one free swimmer in the Broyden-for-everything
+    form (USE_BROYDEN_FOR_EVERYTHING, PRESCRIBED_BODY_VELOCITIES false): the
+    blob forces are Broyden unknowns after the body's six, extrapolated at each
+    step's start (rigid_body.cpp:447-478) and updated with the body
+    (rigid_body.cpp:506-520); one plain matvec per evaluation; the residual's
+    body rows are the force and torque balance and its blob rows the rigid
+    motion the body's positions imply; J0^{-1} through bfe_coupling /
+    BfeInverse.  Returns per step the Broyden iterations and the state."""
+    from oracle import broyden_oracle as BO
+    from oracle import oracle as O
+
+    N, nfil = p.nseg, len(fils)
+    nb3 = np.asarray(blob_refs).size
+    n6 = 6 * N
+    n = n6 * nfil + 6 + nb3
+    cat = lambda name: np.concatenate([getattr(fl, name).reshape(-1) for fl in fils])
+    fb, fbm1, fbm2 = np.zeros(nb3), np.zeros(nb3), np.zeros(nb3)
+    kw = dict(seg_dof=6, a_blob=blob_radius)
+    out = []
+    for nt in range(steps):
+        swimmer_initial_guess(b, fils, refs, nt, p)
+        fb, fbm1, fbm2 = blob_force_guess(fb, fbm1, fbm2, nt)
+
+        def evaluate(fb):
+            kicks = [{4: perturbation[i, :3], 12: perturbation[i, 3:]} if nt == 0 else None for i in range(nfil)]
+            forces = [F.internal_forces(fl, p, kicks[i]) for i, fl in enumerate(fils)]
+            x, f = cat("x"), np.concatenate([fc.reshape(-1) for fc in forces])
+            xb = blob_positions(b, blob_refs)
+            vb = O.block("bb", x, xb, f, fb, **kw)
+            vb = O.block("bs", x, xb, f, fb, vb, bs_accumulate=True, **kw)
+            vs = O.block("sb", x, xb, f, fb, O.block("ss", x, xb, f, fb, **kw), **kw)
+            st = {"seg_x": x, "seg_xm1": cat("xm1"), "seg_xm2": cat("xm2"), "seg_u": cat("u"),
+                  "seg_um1": cat("um1"), "v_segs": vs, "body_x": b.x, "body_xm1": b.xm1, "body_xm2": b.xm2,
+                  "body_u": b.u, "body_um1": b.um1, "v_bodies": np.zeros(6), "x_blobs": xb, "v_blobs": vb,
+                  "f_segs": f, "f_blobs": fb}
+            e, big = BO.compute_errors(st, 1, nfil, N, nt >= 1, p.dt, p.dl, tol, blobs_per_swim=nb3 // 3,
+                                       free_bodies=True)
+            return forces, e, big
+
+        forces, e, big = evaluate(fb)
+        Js = [F.approx_jacobian(fl, forces[i], nt >= 1, p) for i, fl in enumerate(fils)]
+        Smat, Ds, Bs, disp = bfe_coupling(b, fils, refs, blob_refs, fb, nt, p, blob_radius)
+        bfe_clamping(fils, forces, nt, p, Bs)
+        jinv = BfeInverse(Js, Smat, Ds, Bs, disp, nt, p, blob_radius).apply
+        ref = BroydenOp(n, max_iter, confidence, jinv)
+        ref.start(e)
+        while big and ref.iter < max_iter:
+            du = ref.find_update()
+            swimmer_update(b, fils, refs, du[:n6 * nfil + 6], p)
+            fb = fb + du[n6 * nfil + 6:]
+            forces, e, big = evaluate(fb)
+            ref.end_of_iter(e, big)
+        out.append({"iters": ref.iter, "converged": not big, "x": cat("x"), "body": b.copy(), "f_blobs": fb.copy(),
+                    "error": e})
     return out
 
 

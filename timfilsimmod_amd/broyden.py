@@ -34,6 +34,7 @@ class BroydenState(ctypes.Structure):
         ("body_x", ctypes.c_void_p), ("body_xm1", ctypes.c_void_p), ("body_xm2", ctypes.c_void_p),
         ("body_u", ctypes.c_void_p), ("body_um1", ctypes.c_void_p), ("v_bodies", ctypes.c_void_p),
         ("blobs_per_swim", ctypes.c_int32), ("x_blobs", ctypes.c_void_p), ("v_blobs", ctypes.c_void_p),
+        ("free_bodies", ctypes.c_int32), ("f_segs", ctypes.c_void_p), ("f_blobs", ctypes.c_void_p),
     ]
 
 
@@ -45,6 +46,8 @@ L.extra_signatures({
     "tfs_broyden_set_jacobian_inverse": (ctypes.c_int, [_vp, _vp]),
     "tfs_broyden_set_jacobian": (ctypes.c_int, [_vp, _vp]),
     "tfs_broyden_set_body_coupling": (ctypes.c_int, [_vp, _i32, _i32, _vp, _vp, _vp]),
+    "tfs_broyden_set_body_coupling_blobs": (ctypes.c_int, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp,
+                                                           ctypes.c_double, ctypes.c_double]),
     "tfs_broyden_start": (ctypes.c_int, [_vp, _vp]),
     "tfs_broyden_find_update": (ctypes.c_int, [_vp, _vp]),
     "tfs_broyden_end_of_iter": (ctypes.c_int, [_vp, _vp, _i32]),
@@ -120,6 +123,16 @@ class BroydenSolver:
         L.check(L.lib().tfs_broyden_set_body_coupling(self.h, int(num_swim), int(fil_per_swim), L.ptr(a), L.ptr(b),
                                                       L.ptr(s)))
 
+    def set_body_coupling_blobs(self, num_swim, fil_per_swim, blobs_per_swim, A, B, S0, blob_disp, blob_mob_fac,
+                                dt_fac):
+        """Free bodies in the Broyden-for-everything form (swimmer.cpp:931-1010),
+        after set_jacobian; A, B, S0, blob_disp as
+        FilamentModel.body_coupling_blobs writes them."""
+        a, b, s, d = _f64(A), _f64(B), _f64(S0), _f64(blob_disp)
+        L.check(L.lib().tfs_broyden_set_body_coupling_blobs(self.h, int(num_swim), int(fil_per_swim),
+                                                            int(blobs_per_swim), L.ptr(a), L.ptr(b), L.ptr(s),
+                                                            L.ptr(d), float(blob_mob_fac), float(dt_fac)))
+
     def start(self, error=None):
         e = _f64(error)
         L.check(L.lib().tfs_broyden_start(self.h, L.ptr(e)))
@@ -145,17 +158,19 @@ class BroydenSolver:
         return v.value
 
     def compute_errors(self, state: dict, slot: int, *, num_swim, num_fil, num_seg, bdf2, dt, dl, tol=TOL,
-                       blobs_per_swim=0):
+                       blobs_per_swim=0, free_bodies=False):
         """Residuals into slot 0 (error) or 1 (new_error); returns error_is_too_large.
         ``state`` maps the tfs_broyden_state array names to arrays; blobs_per_swim > 0
-        is the Broyden-for-everything form (x_blobs, v_blobs in ``state``)."""
+        is the Broyden-for-everything form (x_blobs, v_blobs in ``state``), with
+        free_bodies its free-body rows (f_segs, f_blobs in ``state``)."""
         st = BroydenState()
         st.num_swim, st.num_fil, st.num_seg, st.bdf2 = num_swim, num_fil, num_seg, int(bool(bdf2))
         st.dt, st.dl, st.tol = dt, dl, tol
         st.blobs_per_swim = int(blobs_per_swim)
+        st.free_bodies = int(bool(free_bodies))
         keep = []
         for name, _ in BroydenState._fields_[7:]:
-            if name == "blobs_per_swim":
+            if name in ("blobs_per_swim", "free_bodies"):
                 continue
             a = _f64(state.get(name))
             keep.append(a)

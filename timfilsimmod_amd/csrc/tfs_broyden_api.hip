@@ -69,12 +69,13 @@ __global__ __launch_bounds__(256) void block_transpose_kernel(const double* __re
 // workgroup per filament.
 __global__ __launch_bounds__(256) void coupling_prep_kernel(const double* __restrict__ J,
                                                             const int64_t* __restrict__ mat_off, int fil_per_swim,
-                                                            int n6, const double* __restrict__ A,
+                                                            int blocks_per_swim, int n6,
+                                                            const double* __restrict__ A,
                                                             const double* __restrict__ B, double* __restrict__ W,
                                                             double* __restrict__ BJ) {
     const int f = blockIdx.x;
     const int s = f / fil_per_swim, i = f % fil_per_swim;
-    const double* Ji = J + mat_off[s * (fil_per_swim + 1) + i];
+    const double* Ji = J + mat_off[(int64_t)s * blocks_per_swim + i];
     const int64_t o = (int64_t)f * 6 * n6;
     for (int r = threadIdx.x; r < n6; r += blockDim.x) {
         double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -114,17 +115,24 @@ __global__ __launch_bounds__(64) void coupling_schur_kernel(const double* __rest
     S[36 * s + t] = acc;
 }
 
-// After the block-diagonal product y = alpha diag(J_i^{-1}, S^{-1}) x: the
-// body rows become alpha S^{-1} (x_b - sum_i BJ_i x_i) and the filament rows
-// y_i - W_i y_b (swimmer.cpp:1012-1066).  One workgroup per swimmer.
+// After the block-diagonal product y = alpha diag(J_i^{-1}, S^{-1}, fac I ...)
+// x: the body rows become alpha S^{-1} (x_b - sum_i BJ_i x_i - fac sum_m (x_m,
+// d_m x x_m)) and the filament rows y_i - W_i y_b (swimmer.cpp:1012-1066); with
+// blob blocks (the Broyden-for-everything form, swimmer.cpp:931-1010) the blob
+// rows y_m - fac dt_fac (y_b + y_b,rot x d_m).  One workgroup per swimmer.
+struct BlobCoupling {
+    int nblob = 0;               // blobs per swimmer (0: none)
+    const double* disp = nullptr;  // 3 per blob: Q_init ref
+    double fac = 0.0, dt_fac = 0.0;
+};
 __global__ __launch_bounds__(256) void coupling_apply_kernel(const double* __restrict__ W,
                                                              const double* __restrict__ BJ,
                                                              const double* __restrict__ Sinv, int fil_per_swim,
-                                                             int n6, const double* __restrict__ x,
+                                                             int n6, BlobCoupling bc, const double* __restrict__ x,
                                                              double* __restrict__ y, double alpha) {
     const int s = blockIdx.x;
     const int64_t rows = (int64_t)fil_per_swim * n6;
-    const int64_t r0 = (int64_t)s * (rows + 6);
+    const int64_t r0 = (int64_t)s * (rows + 6 + 3LL * bc.nblob);
     const int64_t o = (int64_t)s * fil_per_swim * 6 * n6;
     double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int64_t e = threadIdx.x; e < rows; e += blockDim.x) {
@@ -132,6 +140,16 @@ __global__ __launch_bounds__(256) void coupling_apply_kernel(const double* __res
         const double xv = x[r0 + e];
 #pragma unroll
         for (int k = 0; k < 6; ++k) g[k] += BJ[o + i * 6 * n6 + (int64_t)k * n6 + c] * xv;
+    }
+    for (int64_t m = threadIdx.x; m < bc.nblob; m += blockDim.x) {
+        const double* xb = x + r0 + rows + 6 + 3 * m;
+        const double* d = bc.disp + 3 * ((int64_t)s * bc.nblob + m);
+        g[0] += bc.fac * xb[0];
+        g[1] += bc.fac * xb[1];
+        g[2] += bc.fac * xb[2];
+        g[3] += bc.fac * (d[1] * xb[2] - d[2] * xb[1]);
+        g[4] += bc.fac * (d[2] * xb[0] - d[0] * xb[2]);
+        g[5] += bc.fac * (d[0] * xb[1] - d[1] * xb[0]);
     }
     __shared__ double sm[6][256];
     __shared__ double ob[6];
@@ -161,6 +179,14 @@ __global__ __launch_bounds__(256) void coupling_apply_kernel(const double* __res
         for (int k = 0; k < 6; ++k) a += W[o + i * 6 * n6 + r * 6 + k] * ob[k];
         y[r0 + e] -= a;
     }
+    const double fd = bc.fac * bc.dt_fac;
+    for (int64_t m = threadIdx.x; m < bc.nblob; m += blockDim.x) {
+        double* yb = y + r0 + rows + 6 + 3 * m;
+        const double* d = bc.disp + 3 * ((int64_t)s * bc.nblob + m);
+        yb[0] -= fd * (ob[0] + (ob[4] * d[2] - ob[5] * d[1]));
+        yb[1] -= fd * (ob[1] + (ob[5] * d[0] - ob[3] * d[2]));
+        yb[2] -= fd * (ob[2] + (ob[3] * d[1] - ob[4] * d[0]));
+    }
 }
 
 // s[1] = 1 / s[0]
@@ -189,7 +215,76 @@ struct ResidualArgs {
     const double *bx, *bxm1, *bxm2, *bu, *bum1, *bv;
     int nblob;                        // blobs per swimmer (Broyden-for-everything form) or 0
     const double *xb, *vb;            // 3 per blob
+    // free bodies in the Broyden-for-everything form: the body rows are the
+    // force and torque balance (written by body_balance_kernel) and the blob
+    // rows take the velocities the body's positions imply
+    int free_bfe;
+    const double *fs, *fb;            // 6 per segment, 3 per blob
 };
+
+// out = dexp(u, v) = v + alpha u x v + beta u x (u x v) (quaternion.cpp:574-608).
+__device__ inline void dexp(double* out, const double* u, const double* v) {
+#pragma clang fp contract(off)
+    const double t2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const double t = sqrt(t2);
+    double alpha, beta;
+    if (t < 1e-5) {
+        const double t4 = t2 * t2;
+        beta = (1.0 / 3.0 + t2 / 72.0) / (2.0 + t2 / 6.0 + t4 / 72.0);
+        alpha = 1.0 / 6.0 + 2.0 * beta * (1.0 - t2 / 12.0);
+    } else {
+        alpha = sin(0.5 * t);
+        alpha *= 2.0 * alpha / t2;
+        beta = (t - sin(t)) / (t * t2);
+    }
+    const double c[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+    const double cc[3] = {u[1] * c[2] - u[2] * c[1], u[2] * c[0] - u[0] * c[2], u[0] * c[1] - u[1] * c[0]};
+    for (int k = 0; k < 3; ++k) out[k] = v[k] + alpha * c[k] + beta * cc[k];
+}
+
+// The body rows of a free body in the Broyden-for-everything form
+// (mobility_solver.cpp:2676-2681, 2837-2847): -swimmer.f (its segments' forces
+// and their torques about the body's centre, swimmer.cpp:469-499) plus the blob
+// forces and their torques.  One workgroup per swimmer.
+__global__ __launch_bounds__(256) void body_balance_kernel(ResidualArgs a, double* __restrict__ err) {
+    const int s = blockIdx.x;
+    const int64_t nsw = (int64_t)a.nfil * a.nseg;
+    const int64_t per = 6 * (nsw + 1) + 3LL * a.nblob;
+    const double* X = a.bx + 3 * s;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = threadIdx.x; i < nsw + a.nblob; i += 256) {
+        const double *f, *x;
+        double T[3] = {0.0, 0.0, 0.0};
+        if (i < nsw) {
+            const int64_t g = s * nsw + i;
+            f = a.fs + 6 * g;
+            x = a.x + 3 * g;
+            T[0] = f[3];
+            T[1] = f[4];
+            T[2] = f[5];
+        } else {
+            const int64_t g = s * (int64_t)a.nblob + (i - nsw);
+            f = a.fb + 3 * g;
+            x = a.xb + 3 * g;
+        }
+        const double r[3] = {x[0] - X[0], x[1] - X[1], x[2] - X[2]};
+        acc[0] += f[0];
+        acc[1] += f[1];
+        acc[2] += f[2];
+        acc[3] += T[0] + (r[1] * f[2] - r[2] * f[1]);
+        acc[4] += T[1] + (r[2] * f[0] - r[0] * f[2]);
+        acc[5] += T[2] + (r[0] * f[1] - r[1] * f[0]);
+    }
+    __shared__ double sm[6][256];
+    for (int k = 0; k < 6; ++k) sm[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 6; ++k) sm[k][threadIdx.x] += sm[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) err[s * per + 6 * nsw + threadIdx.x] = sm[threadIdx.x][0];
+}
 
 // One thread per segment (then one per body): 3 position and 3 Lie-algebra
 // residuals in the reference's storage order (mobility_solver.cpp:2661-2946),
@@ -212,6 +307,26 @@ __global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a, double* _
         const int64_t m = g - nsegs - a.nswim;
         const int64_t n = m / a.nblob, k = m % a.nblob;
         const double* V = a.bv + 6 * n;
+        double Vf[6];
+        if (a.free_bfe) {
+            // the body's velocities from its positions (mobility_solver.cpp:2786-2806)
+            const double *x = a.bx + 3 * n, *xm1 = a.bxm1 + 3 * n, *u = a.bu + 3 * n;
+            if (a.bdf2) {
+                const double *xm2 = a.bxm2 + 3 * n, *um1 = a.bum1 + 3 * n;
+                double w[3];
+                for (int c = 0; c < 3; ++c) {
+                    Vf[c] = -(x[c] + c1 * xm1[c] + c2 * xm2[c]) / (c3 * a.dt);
+                    w[c] = (c2 * um1[c] - u[c]) / (c3 * a.dt);
+                }
+                dexp(Vf + 3, u, w);
+            } else {
+                for (int c = 0; c < 3; ++c) {
+                    Vf[c] = (x[c] - xm1[c]) / a.dt;
+                    Vf[3 + c] = u[c] / a.dt;
+                }
+            }
+            V = Vf;
+        }
         double d[3];
         for (int c = 0; c < 3; ++c) d[c] = a.xb[3 * m + c] - a.bx[3 * n + c];
         const double w[3] = {V[4] * d[2] - V[5] * d[1], V[5] * d[0] - V[3] * d[2], V[3] * d[1] - V[4] * d[0]};
@@ -239,6 +354,16 @@ __global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a, double* _
             um1 = a.um1 ? a.um1 + 3 * g : nullptr;
             vel = a.v + 6 * g;
             om = a.v + 6 * g + 3;
+        } else if (a.free_bfe) {
+            // force and torque balance, formed by body_balance_kernel (tolerance TOL)
+            const int64_t n = g - nsegs;
+            pid = n * per + 6 * (int64_t)a.nfil * a.nseg;
+            for (int d = 0; d < 6; ++d) {
+                const double e = err[pid + d];
+                if (!isfinite(e)) nonfinite = 1.0;
+                ratio = fmax(ratio, fabs(e) / a.lie_tol);
+            }
+            x = nullptr;
         } else {
             const int64_t n = g - nsegs;
             pid = n * per + 6 * (int64_t)a.nfil * a.nseg;
@@ -252,9 +377,9 @@ __global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a, double* _
             om = a.bv + 6 * n + 3;
         }
         double W[3];
-        dexpinv(W, u, om);
+        if (x) dexpinv(W, u, om);
         const double c3dt = c3 * a.dt;
-        for (int d = 0; d < 3; ++d) {
+        for (int d = 0; d < 3 && x; ++d) {
             double ep, el;
             if (a.bdf2) {
                 ep = x[d] + c1 * xm1[d] + c2 * xm2[d] + c3dt * vel[d];
@@ -337,11 +462,14 @@ struct tfs_broyden {
     std::vector<int32_t> host_status;
     std::vector<int32_t> host_sizes;
     std::vector<int64_t> host_moff;
-    // free-body coupling (tfs_broyden_set_body_coupling)
+    // free-body coupling (tfs_broyden_set_body_coupling[_blobs])
     bool coupled = false;
     int cpl_swim = 0, cpl_fil = 0, cpl_n6 = 0;
     double* cpl_arena = nullptr;
     double *W = nullptr, *BJ = nullptr, *Sinv = nullptr, *Sbuf = nullptr, *cpl_in = nullptr;
+    BlobCoupling cpl_blobs;
+    double* cpl_disp = nullptr;  // device, 3 per blob
+    int64_t cpl_disp_len = 0;
 };
 
 namespace {
@@ -358,7 +486,8 @@ int apply_jinv(tfs_broyden* b, const double* x, double* y, double alpha) {
     int rc = check(hipGetLastError(), "J0^{-1}");
     if (!rc && b->coupled) {
         hipLaunchKernelGGL(coupling_apply_kernel, dim3(b->cpl_swim), dim3(256), 0, b->stream, (const double*)b->W,
-                           (const double*)b->BJ, (const double*)b->Sinv, b->cpl_fil, b->cpl_n6, x, y, alpha);
+                           (const double*)b->BJ, (const double*)b->Sinv, b->cpl_fil, b->cpl_n6, b->cpl_blobs, x, y,
+                           alpha);
         rc = check(hipGetLastError(), "J0^{-1} body coupling");
     }
     return rc;
@@ -478,6 +607,7 @@ int tfs_broyden_destroy(tfs_broyden* b) {
     (void)hipFree(b->piv);
     (void)hipFree(b->status);
     if (b->cpl_arena) (void)hipFree(b->cpl_arena);
+    if (b->cpl_disp) (void)hipFree(b->cpl_disp);
     if (b->host_scal) (void)hipHostFree(b->host_scal);
     if (b->own_stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -517,17 +647,25 @@ int tfs_broyden_set_jacobian(tfs_broyden* b, const double* blocks) {
     return check(hipGetLastError(), "tfs_broyden_set_jacobian");
 }
 
-int tfs_broyden_set_body_coupling(tfs_broyden* b, int32_t num_swim, int32_t fil_per_swim, const double* A,
-                                  const double* B, const double* S0) {
-    if (!b || !A || !B || !S0) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_set_body_coupling: null argument");
-    if (num_swim < 1 || fil_per_swim < 1 || (int64_t)num_swim * (fil_per_swim + 1) != b->nblocks)
-        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_set_body_coupling: block count is not num_swim * (fil_per_swim + 1)");
+}  // extern "C"
+
+namespace {
+
+int set_coupling(tfs_broyden* b, int32_t num_swim, int32_t fil_per_swim, int32_t nblob, const double* A,
+                 const double* B, const double* S0, const double* disp, double fac, double dt_fac) {
+    if (!b || !A || !B || !S0 || (nblob > 0 && !disp))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_set_body_coupling: null argument");
+    const int per_swim = fil_per_swim + 1 + nblob;
+    if (num_swim < 1 || fil_per_swim < 1 || nblob < 0 || (int64_t)num_swim * per_swim != b->nblocks)
+        return fail(TFS_ERR_INVALID_ARGUMENT,
+                    "tfs_broyden_set_body_coupling: block count is not num_swim * (fil_per_swim + 1 + blobs)");
     const int n6 = b->host_sizes[0];
     for (int s = 0; s < num_swim; ++s)
-        for (int i = 0; i <= fil_per_swim; ++i)
-            if (b->host_sizes[(size_t)s * (fil_per_swim + 1) + i] != (i < fil_per_swim ? n6 : 6))
+        for (int i = 0; i < per_swim; ++i)
+            if (b->host_sizes[(size_t)s * per_swim + i] != (i < fil_per_swim ? n6 : i == fil_per_swim ? 6 : 3))
                 return fail(TFS_ERR_INVALID_ARGUMENT,
-                            "tfs_broyden_set_body_coupling: blocks are not fil_per_swim equal blocks and a 6 x 6 per swimmer");
+                            "tfs_broyden_set_body_coupling: blocks are not fil_per_swim equal blocks, a 6 x 6 and "
+                            "3 x 3 blob blocks per swimmer");
     const int64_t lab = (int64_t)num_swim * fil_per_swim * 6 * n6;
     if (b->cpl_arena && (b->cpl_swim != num_swim || b->cpl_fil != fil_per_swim)) {
         (void)hipFree(b->cpl_arena);
@@ -561,8 +699,27 @@ int tfs_broyden_set_body_coupling(tfs_broyden* b, int32_t num_swim, int32_t fil_
         dS0 = b->cpl_in + 2 * lab;
     }
     if (rc) return rc;
+    b->cpl_blobs = BlobCoupling{};
+    if (!rc && nblob > 0) {
+        const int64_t len = 3LL * num_swim * nblob;
+        if (b->cpl_disp_len < len) {
+            if (b->cpl_disp) (void)hipFree(b->cpl_disp);
+            b->cpl_disp = nullptr;
+            b->cpl_disp_len = 0;
+            if (hipMalloc(&b->cpl_disp, len * 8) != hipSuccess)
+                return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_broyden_set_body_coupling: device memory");
+            b->cpl_disp_len = len;
+        }
+        rc = copy_in(b, b->cpl_disp, disp, len, "tfs_broyden_set_body_coupling");
+        b->cpl_blobs.nblob = nblob;
+        b->cpl_blobs.disp = b->cpl_disp;
+        b->cpl_blobs.fac = fac;
+        b->cpl_blobs.dt_fac = dt_fac;
+    }
+    if (rc) return rc;
     hipLaunchKernelGGL(coupling_prep_kernel, dim3((unsigned)(num_swim * fil_per_swim)), dim3(256), 0, b->stream,
-                       (const double*)b->J, (const int64_t*)b->mat_off, fil_per_swim, n6, dA, dB, b->W, b->BJ);
+                       (const double*)b->J, (const int64_t*)b->mat_off, fil_per_swim, per_swim, n6, dA, dB, b->W,
+                       b->BJ);
     rc = check(hipGetLastError(), "body coupling");
     if (!rc) {
         hipLaunchKernelGGL(coupling_schur_kernel, dim3((unsigned)num_swim), dim3(64), 0, b->stream, dS0,
@@ -604,7 +761,7 @@ int tfs_broyden_set_body_coupling(tfs_broyden* b, int32_t num_swim, int32_t fil_
         for (int r = 0; r < 6; ++r)
             for (int c = 0; c < 6; ++c) colmajor[36 * (size_t)s + 6 * c + r] = Si[36 * (size_t)s + 6 * r + c];
     for (int s = 0; s < num_swim && !rc; ++s) {
-        const size_t blk = (size_t)s * (fil_per_swim + 1) + fil_per_swim;
+        const size_t blk = (size_t)s * per_swim + fil_per_swim;
         rc = check(hipMemcpyAsync(b->J + b->host_moff[blk], colmajor.data() + 36 * (size_t)s, 288,
                                   hipMemcpyHostToDevice, b->stream),
                    "S^{-1} block");
@@ -612,6 +769,23 @@ int tfs_broyden_set_body_coupling(tfs_broyden* b, int32_t num_swim, int32_t fil_
     if (!rc) rc = check(hipStreamSynchronize(b->stream), "tfs_broyden_set_body_coupling");
     if (!rc) b->coupled = true;
     return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tfs_broyden_set_body_coupling(tfs_broyden* b, int32_t num_swim, int32_t fil_per_swim, const double* A,
+                                  const double* B, const double* S0) {
+    return set_coupling(b, num_swim, fil_per_swim, 0, A, B, S0, nullptr, 0.0, 0.0);
+}
+
+int tfs_broyden_set_body_coupling_blobs(tfs_broyden* b, int32_t num_swim, int32_t fil_per_swim,
+                                        int32_t blobs_per_swim, const double* A, const double* B, const double* S0,
+                                        const double* blob_disp, double blob_mob_fac, double dt_fac) {
+    if (blobs_per_swim < 1)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_set_body_coupling_blobs: blobs_per_swim < 1");
+    return set_coupling(b, num_swim, fil_per_swim, blobs_per_swim, A, B, S0, blob_disp, blob_mob_fac, dt_fac);
 }
 
 int tfs_broyden_start(tfs_broyden* b, const double* error) {
@@ -694,6 +868,10 @@ int tfs_broyden_compute_errors(tfs_broyden* b, const tfs_broyden_state* st, int3
                     "tfs_broyden_compute_errors: 6*(NSWIM*NFIL*NSEG + NSWIM) + 3*NSWIM*blobs_per_swim != n");
     if (st->blobs_per_swim > 0 && (!st->x_blobs || !st->v_blobs))
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_compute_errors: blob rows need x_blobs and v_blobs");
+    if (st->free_bodies && (st->blobs_per_swim <= 0 || !st->f_blobs ||
+                            (st->num_fil > 0 && st->num_seg > 0 && !st->f_segs)))
+        return fail(TFS_ERR_INVALID_ARGUMENT,
+                    "tfs_broyden_compute_errors: free bodies take the Broyden-for-everything form with f_segs and f_blobs");
     const bool nsegs = (int64_t)st->num_fil * st->num_seg > 0;
     if (!(st->dt > 0) || !(st->dl > 0) || !(st->tol > 0) || !st->body_x || !st->body_xm1 || !st->body_u ||
         !st->v_bodies || (st->bdf2 && (!st->body_xm2 || !st->body_um1)) ||
@@ -736,14 +914,25 @@ int tfs_broyden_compute_errors(tfs_broyden* b, const tfs_broyden_state* st, int3
     a.nblob = st->blobs_per_swim;
     a.xb = stage(st->x_blobs, 3 * nblob_tot);
     a.vb = stage(st->v_blobs, 3 * nblob_tot);
+    a.free_bfe = st->free_bodies ? 1 : 0;
+    if (a.free_bfe) {
+        a.fs = stage(st->f_segs, 6 * nseg_tot);
+        a.fb = stage(st->f_blobs, 3 * nblob_tot);
+    }
     int rc = TFS_OK;
     if (nblob_tot && (!a.xb || !a.vb)) rc = fail(TFS_ERR_OUT_OF_MEMORY, "tfs_broyden_compute_errors: staging");
+    if (!rc && a.free_bfe && (!a.fb || (nseg_tot && !a.fs)))
+        rc = fail(TFS_ERR_OUT_OF_MEMORY, "tfs_broyden_compute_errors: staging");
     if (!rc && ((nsegs && (!a.x || !a.xm1 || !a.u || !a.v || (a.bdf2 && (!a.xm2 || !a.um1)))) || !a.bx ||
                 !a.bxm1 || !a.bu || !a.bv || (a.bdf2 && (!a.bxm2 || !a.bum1))))
         rc = fail(TFS_ERR_OUT_OF_MEMORY, "tfs_broyden_compute_errors: staging");
     const int blocks = (int)((nseg_tot + nb + nblob_tot + 255) / 256);
     if (!rc && 2LL * blocks > std::max<int64_t>((int64_t)b->G * (b->max_iter + 1), 2LL * b->RB))
         rc = fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_compute_errors: state larger than the engine");
+    if (!rc && a.free_bfe) {
+        hipLaunchKernelGGL(body_balance_kernel, dim3((unsigned)a.nswim), dim3(256), 0, b->stream, a, b->slot[slot]);
+        rc = check(hipGetLastError(), "force and torque balance");
+    }
     if (!rc) {
         hipLaunchKernelGGL(residual_kernel, dim3(blocks), dim3(256), 0, b->stream, a, b->slot[slot], b->partial);
         hipLaunchKernelGGL(residual_finish_kernel, dim3(1), dim3(256), 0, b->stream, b->partial, blocks,

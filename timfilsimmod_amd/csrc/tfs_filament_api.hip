@@ -777,9 +777,10 @@ struct CouplingOut {
     const double* n_rot;   // 36 per body
     double* A;             // per filament 6N x 6, swimmer layout
     double* B;             // per filament 6 x 6N
-    double* part;          // 9 per filament: its term of omega_lie_deriv
+    double* part;          // 9 per filament: its term of omega_lie_deriv (or of S, bfe)
     double sphere;         // radius of the sphere correction (0: none)
     int bdf2;
+    int bfe;               // Broyden-for-everything form: B is the force and torque balance's D
 };
 
 // swimmer::prepare_jacobian_inv, GMRES branch with a free body
@@ -811,8 +812,9 @@ __global__ __launch_bounds__(64) void coupling_kernel(FilState S, FilParams P, C
 #pragma unroll
     for (int k = 0; k < 3; ++k) efN[k] = P.end_force * tN[k];
     const M3 T = rcross(efN);
-    const double* nr = O.n_rot + 36 * sw;
-    const M3 N00 = block_of(nr, 0, 0), N01 = block_of(nr, 0, 1), N10 = block_of(nr, 1, 0), N11 = block_of(nr, 1, 1);
+    const double* nr = O.bfe ? nullptr : O.n_rot + 36 * sw;
+    const M3 N00 = nr ? block_of(nr, 0, 0) : zero3(), N01 = nr ? block_of(nr, 0, 1) : zero3(),
+             N10 = nr ? block_of(nr, 1, 0) : zero3(), N11 = nr ? block_of(nr, 1, 1) : zero3();
     const int swim_f = f % P.fil_per_swim;
     double* Bf = O.B + (int64_t)sw * P.fil_per_swim * 6 * n6 + (int64_t)swim_f * 6 * n6;
     double* Af = O.A + (int64_t)sw * P.fil_per_swim * 6 * n6 + (int64_t)swim_f * 6 * n6;
@@ -837,6 +839,17 @@ __global__ __launch_bounds__(64) void coupling_kernel(FilState S, FilParams P, C
             for (int k = 0; k < 3; ++k) tm[k] *= P.dl;
             bot = mm(T, rcross(tm));
         }
+        if (O.bfe) {
+            // the force and torque balance's dependence (swimmer.cpp:605-640):
+            // D = -[top; bot]
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    Bf[(int64_t)r * n6 + 3 * c + j] = -top.a[3 * r + j];
+                    Bf[(int64_t)(3 + r) * n6 + 3 * c + j] = -bot.a[3 * r + j];
+                }
+        } else {
         // body_mobility * B
         M3 nt_ = axpy(1.0, mm(N01, bot), mm(N00, top));
         M3 nb_ = axpy(1.0, mm(N11, bot), mm(N10, top));
@@ -872,6 +885,7 @@ __global__ __launch_bounds__(64) void coupling_kernel(FilState S, FilParams P, C
                 Bf[(int64_t)r * n6 + 3 * c + j] = dt_fac * nt_.a[3 * r + j];
                 Bf[(int64_t)(3 + r) * n6 + 3 * c + j] = dt_fac * nb_.a[3 * r + j];
             }
+        }
         // A: [I | Dm] on the position rows, the clamping blocks on the Lie
         // rows of segments 0 and 1
         M3 al = zero3(), ar = zero3();
@@ -890,7 +904,25 @@ __global__ __launch_bounds__(64) void coupling_kernel(FilState S, FilParams P, C
                 Af[(int64_t)(3 * c + r) * 6 + 3 + j] = ar.a[3 * r + j];
             }
     }
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && O.bfe) {
+        // this filament's terms of the Schur matrix (swimmer.cpp:597-617)
+        const double* tether = mult;
+        const double* clamp = mult + 3 * N;
+        const M3 U = rcross(S.bu + 3 * sw);
+        const M3 rc = rcross(clamp);
+        double uxl[3];
+        cross(S.bu + 3 * sw, clamp, uxl);
+        M3 part = mm(rcross(tether), rcross(d));
+        part = axpy(0.5, rc, part);
+        part = axpy(1.0 / 12.0, axpy(-1.0, mm(U, rc), rcross(uxl)), part);
+        double tb[3];
+        tangent(load_q(S.q + 4 * s0), tb);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) tb[k] *= 0.5 * P.dl;
+        part = axpy(-1.0, mm(T, rcross(tb)), part);   // rcross(-EF t_end) rcross(DL t_base / 2)
+#pragma unroll
+        for (int e = 0; e < 9; ++e) O.part[9 * (int64_t)f + e] = part.a[e];
+    } else if (threadIdx.x == 0) {
         const double* tether = mult;
         const double* clamp = mult + 3 * N;
         M3 part = scale(-1.0, mm(rcross(tether), rcross(d)));
@@ -902,6 +934,65 @@ __global__ __launch_bounds__(64) void coupling_kernel(FilState S, FilParams P, C
         part = axpy(1.0, mm(T, Dm), part);
 #pragma unroll
         for (int e = 0; e < 9; ++e) O.part[9 * (int64_t)f + e] = part.a[e];
+    }
+}
+
+// The Schur matrix of the Broyden-for-everything form before its filament
+// corrections (swimmer.cpp:565-596): the blob terms with the diagonal blob
+// mobility and the blob forces' torque derivative, plus the filaments' terms;
+// and the blob displacements d_m = Q_init ref_m.  One workgroup per swimmer.
+__global__ __launch_bounds__(256) void bfe_body_block_kernel(FilState S, FilParams P, int nblob,
+                                                             const double* __restrict__ blob_refs,
+                                                             const double* __restrict__ f_blobs,
+                                                             const double* __restrict__ part, double bmf,
+                                                             double* __restrict__ S0, double* __restrict__ disp) {
+    const int sw = blockIdx.x;
+    M3 Qi;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) Qi.a[e] = S.bQ[9 * sw + e];
+    double acc[27];
+#pragma unroll
+    for (int e = 0; e < 27; ++e) acc[e] = 0.0;
+    for (int m = threadIdx.x; m < nblob; m += blockDim.x) {
+        const int64_t g = (int64_t)sw * nblob + m;
+        double d[3];
+        mv(Qi, blob_refs + 3 * g, d);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) disp[3 * g + k] = d[k];
+        const M3 rd = rcross(d);
+        const M3 fr = mm(rcross(f_blobs + 3 * g), rd);
+        const M3 rr = mm(rd, rd);
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            acc[e] += fr.a[e];
+            acc[9 + e] += rd.a[e];
+            acc[18 + e] += rr.a[e];
+        }
+    }
+    __shared__ double sm[27][256];
+#pragma unroll
+    for (int e = 0; e < 27; ++e) sm[e][threadIdx.x] = acc[e];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int e = 0; e < 27; ++e) sm[e][threadIdx.x] += sm[e][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double* s = S0 + 36 * sw;
+        for (int e = 0; e < 36; ++e) s[e] = 0.0;
+        for (int k = 0; k < 3; ++k) s[7 * k] = -nblob * bmf;
+        M3 rot;
+        for (int e = 0; e < 9; ++e) rot.a[e] = sm[e][0] + bmf * sm[18 + e][0];
+        for (int i = 0; i < P.fil_per_swim; ++i)
+            for (int e = 0; e < 9; ++e) rot.a[e] += part[9 * ((int64_t)sw * P.fil_per_swim + i) + e];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                s[6 * r + 3 + c] = -bmf * sm[9 + 3 * r + c][0];
+                s[6 * (3 + r) + c] = bmf * sm[9 + 3 * r + c][0];
+                s[6 * (3 + r) + 3 + c] = rot.a[3 * r + c];
+            }
     }
 }
 
@@ -1336,6 +1427,56 @@ int tfs_filament_body_coupling(tfs_filament* h, int32_t nt, const double* body_m
     return check(e, "tfs_filament_body_coupling");
 }
 
+int tfs_filament_body_coupling_blobs(tfs_filament* h, int32_t nt, const double* f_blobs, double blob_radius,
+                                     double* A, double* B, double* S0, double* blob_disp) {
+    if (!h || !f_blobs || !A || !B || !S0 || !blob_disp || nt < 0)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_body_coupling_blobs: null argument or nt < 0");
+    if (!h->S.bx || h->blobs_per_swim < 1)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_body_coupling_blobs: no free body with blobs");
+    if (!h->jacobian_done)
+        return fail(TFS_ERR_INVALID_ARGUMENT,
+                    "tfs_filament_body_coupling_blobs: call tfs_filament_jacobian at this state first");
+    if (!(blob_radius > 0.0) || !std::isfinite(blob_radius))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_filament_body_coupling_blobs: blob radius must be > 0");
+    const int64_t nsw = h->desc.num_swim, n6 = 6LL * h->P.nseg, lab = (int64_t)h->P.nfil * 6 * n6;
+    const int64_t nb3 = 3 * nsw * h->blobs_per_swim;
+    const bool host = h->desc.memory == TFS_MEMORY_HOST;
+    double *dA = A, *dB = B, *dS = S0, *dD = blob_disp;
+    const double* dF = f_blobs;
+    hipError_t e = hipSuccess;
+    if (host) {
+        double* st = staging(h, 2 * lab + 36 * nsw + 2 * nb3);
+        if (!st) return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_filament_body_coupling_blobs: staging memory");
+        dA = st;
+        dB = st + lab;
+        dS = st + 2 * lab;
+        dD = dS + 36 * nsw;
+        double* fst = dD + nb3;
+        e = hipMemcpyAsync(fst, f_blobs, 8 * nb3, hipMemcpyHostToDevice, h->stream);
+        dF = fst;
+    }
+    // -6 pi mu a / DT (backward Euler) or -9 pi mu a / DT (BDF2), swimmer.cpp:572
+    const int bdf2 = nt >= 1 ? 1 : 0;
+    const double bmf = (bdf2 ? -9.0 : -6.0) * kPi * h->P.mu * blob_radius / h->P.dt;
+    if (e == hipSuccess) {
+        CouplingOut o{nullptr, dA, dB, h->part, 0.0, bdf2, 1};
+        hipLaunchKernelGGL(coupling_kernel, dim3((unsigned)h->P.nfil), dim3(64), 0, h->stream, h->S, h->P, o);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(bfe_body_block_kernel, dim3((unsigned)nsw), dim3(256), 0, h->stream, h->S, h->P,
+                           (int)h->blobs_per_swim, (const double*)h->blob_refs, dF, (const double*)h->part, bmf, dS,
+                           dD);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && host) e = hipMemcpyAsync(A, dA, 8 * lab, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && host) e = hipMemcpyAsync(B, dB, 8 * lab, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && host) e = hipMemcpyAsync(S0, dS, 288 * nsw, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && host) e = hipMemcpyAsync(blob_disp, dD, 8 * nb3, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && host) e = hipStreamSynchronize(h->stream);
+    return check(e, "tfs_filament_body_coupling_blobs");
+}
+
 // ---- elastic implicit timestep --------------------------------------------
 
 struct tfs_elastic {
@@ -1351,6 +1492,7 @@ struct tfs_elastic {
     int64_t nb_swim = 0, per = 0;
     tfs_rpy* rpy = nullptr;
     double *v_blobs = nullptr, *fbm1 = nullptr, *fbm2 = nullptr, *bfe_blocks = nullptr, *bfe_du = nullptr;
+    double* bdisp = nullptr;  // free bodies in that form: the blobs' displacements Q_init ref
 };
 
 int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
@@ -1362,12 +1504,14 @@ int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
     const tfs_filament* fh = desc->filaments;
     const bool free_bodies = fh->S.bx != nullptr;
     const bool bfe = desc->broyden_blobs > 0;
-    if (bfe && (free_bodies || !(desc->blob_radius > 0)))
+    if (bfe && !(desc->blob_radius > 0))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_create: the Broyden-for-everything form needs blob_radius > 0");
+    if (bfe && free_bodies && fh->blobs_per_swim != desc->broyden_blobs)
         return fail(TFS_ERR_INVALID_ARGUMENT,
-                    "tfs_elastic_create: the Broyden-for-everything form takes bodies held still and blob_radius > 0");
+                    "tfs_elastic_create: broyden_blobs must equal the blobs given to tfs_filament_set_body");
     if (!free_bodies && (!desc->x_bodies || !desc->v_bodies || !desc->x_blobs))
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_create: null body or blob array");
-    if (free_bodies && (!desc->body_mobility || !(desc->sphere_radius >= 0.0)))
+    if (free_bodies && !bfe && (!desc->body_mobility || !(desc->sphere_radius >= 0.0)))
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_create: free bodies need the body mobility (and radius >= 0)");
     if (fh->desc.memory != TFS_MEMORY_DEVICE)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_create: the filament handle must use device memory");
@@ -1389,7 +1533,8 @@ int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
         {&e->f_segs, 6 * ns}, {&e->v_segs, 6 * ns},        {&e->du, e->n},  {&e->blocks, nb},
         {&e->zeros, 6 * nswim}, {&e->x_blobs, 3 * fh->nblob}, {&e->v_bodies, 6 * nswim}, {&e->A, lab},
         {&e->B, lab},          {&e->S0, 36 * nswim},        {&e->v_blobs, 3 * nbt},   {&e->fbm1, 3 * nbt},
-        {&e->fbm2, 3 * nbt},   {&e->bfe_blocks, bfe_len},   {&e->bfe_du, bfe ? nswim * e->per : 0}};
+        {&e->fbm2, 3 * nbt},   {&e->bfe_blocks, bfe_len},   {&e->bfe_du, bfe ? nswim * e->per : 0},
+        {&e->bdisp, free_bodies && bfe ? 3 * nbt : 0}};
     int64_t total = 0;
     for (auto& p : parts) total += (p.second + 31) / 32 * 32;
     if (hipMalloc(&e->arena, total * 8) != hipSuccess) {
@@ -1410,7 +1555,7 @@ int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
     hipError_t err = hipMemcpy(e->blocks, host.data(), 8 * nb, hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemset(e->zeros, 0, 48 * nswim);
     if (err == hipSuccess) err = hipMemset(e->v_bodies, 0, 48 * nswim);
-    if (err == hipSuccess && free_bodies && fh->nblob &&
+    if (err == hipSuccess && free_bodies && fh->nblob && desc->velocities &&
         tfs_velocities_set_body_frame(desc->velocities, fh->blob_refs) != TFS_OK) {
         const std::string msg = g_last_error;
         tfs_elastic_destroy(e);
@@ -1504,8 +1649,13 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
     }
     if (e->rpy) {
         st.blobs_per_swim = (int32_t)e->nb_swim;
-        st.x_blobs = D.x_blobs;
+        st.x_blobs = e->free_bodies ? e->x_blobs : D.x_blobs;
         st.v_blobs = e->v_blobs;
+        if (e->free_bodies) {
+            st.free_bodies = 1;
+            st.f_segs = e->f_segs;
+            st.f_blobs = D.f_blobs;
+        }
     }
     const int64_t nswim = fh->desc.num_swim, fil = fh->desc.fil_per_swim, n6 = 6LL * fh->P.nseg;
     const int64_t nb3 = 3 * e->nb_swim;
@@ -1536,7 +1686,7 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
         if (!rc && e->rpy) {
             // USE_BROYDEN_FOR_EVERYTHING: one plain matvec with the blob forces given
             rc = tfs_rpy_upload(e->rpy, TFS_SEG_POSITIONS, fh->S.x);
-            if (!rc) rc = tfs_rpy_upload(e->rpy, TFS_BLOB_POSITIONS, D.x_blobs);
+            if (!rc) rc = tfs_rpy_upload(e->rpy, TFS_BLOB_POSITIONS, e->free_bodies ? e->x_blobs : D.x_blobs);
             if (!rc) rc = tfs_rpy_apply(e->rpy, TFS_BLOCK_ALL, e->f_segs, D.f_blobs, e->v_segs, e->v_blobs);
             if (!rc) rc = tfs_filament_sync(fh);
         } else if (!rc && e->free_bodies)
@@ -1572,7 +1722,18 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
             if (!rc) rc = tfs_filament_sync(fh);
         }
         if (!rc) rc = tfs_broyden_set_jacobian(D.broyden, e->rpy ? e->bfe_blocks : e->blocks);
-        if (!rc && e->free_bodies) {
+        if (!rc && e->free_bodies && e->rpy) {
+            // swimmer::prepare_jacobian_inv, Broyden for everything: the blobs
+            // eliminated with their diagonal mobility, the body's Schur matrix
+            const double fac = -6.0 * kPi * fh->P.mu * D.blob_radius;
+            const double dt_fac = (nt >= 1 ? 1.5 : 1.0) / fh->P.dt;
+            rc = tfs_filament_body_coupling_blobs(fh, nt, D.f_blobs, D.blob_radius, e->A, e->B, e->S0, e->bdisp);
+            if (!rc) rc = tfs_filament_sync(fh);
+            if (!rc)
+                rc = tfs_broyden_set_body_coupling_blobs(D.broyden, fh->desc.num_swim, fh->desc.fil_per_swim,
+                                                         (int32_t)e->nb_swim, e->A, e->B, e->S0, e->bdisp, fac,
+                                                         dt_fac);
+        } else if (!rc && e->free_bodies) {
             // swimmer::prepare_jacobian_inv: the body's Schur complement
             rc = tfs_filament_body_coupling(fh, nt, D.body_mobility, D.sphere_radius, e->A, e->B, e->S0);
             if (!rc) rc = tfs_filament_sync(fh);

@@ -51,6 +51,7 @@ L.extra_signatures({
     "tfs_elastic_backup": (ctypes.c_int, [_vp, _vp, _vp]),
     "tfs_elastic_restore": (ctypes.c_int, [_vp, _vp]),
     "tfs_filament_set_base": (ctypes.c_int, [_vp, _i32, _vp]),
+    "tfs_filament_body_coupling_blobs": (ctypes.c_int, [_vp, _i32, _vp, ctypes.c_double, _vp, _vp, _vp, _vp]),
 })
 
 BASE_CLAMPED = 0   # CILIA_TYPE 0 (INSTABILITY_CILIA), and 2 (CONSTANT_BASE_ROTATION) with values
@@ -142,6 +143,22 @@ class FilamentModel:
         L.check(L.lib().tfs_filament_body_coupling(self.h, int(nt), L.ptr(nm), float(sphere_radius), L.ptr(A),
                                                    L.ptr(B), L.ptr(S0)))
         return A, B, S0
+
+    def body_coupling_blobs(self, nt, f_blobs, blob_radius=1.0, A=None, B=None, S0=None, blob_disp=None):
+        """J0's body coupling in the Broyden-for-everything form with free bodies
+        (swimmer.cpp:565-676) after jacobian() at the same state: per filament A
+        (6N x 6) and B (6 x 6N: the force and torque balance's dependence), per
+        swimmer S0 (6 x 6, with the blob terms), the blobs' displacements."""
+        n6, nf, ns = 6 * self.desc.num_seg, self.nfil, self.desc.num_swim
+        nb3 = np.asarray(f_blobs).size if not hasattr(f_blobs, "data_ptr") else f_blobs.numel()
+        A = np.zeros(nf * 6 * n6) if A is None else A
+        B = np.zeros(nf * 6 * n6) if B is None else B
+        S0 = np.zeros(36 * ns) if S0 is None else S0
+        blob_disp = np.zeros(nb3) if blob_disp is None else blob_disp
+        fb = self._arr(f_blobs)
+        L.check(L.lib().tfs_filament_body_coupling_blobs(self.h, int(nt), L.ptr(fb), float(blob_radius), L.ptr(A),
+                                                         L.ptr(B), L.ptr(S0), L.ptr(blob_disp)))
+        return A, B, S0, blob_disp
 
     def initial_guess(self, nt):
         L.check(L.lib().tfs_filament_initial_guess(self.h, int(nt)))
