@@ -135,7 +135,8 @@ int tfs_rpy_set_precision(tfs_rpy *h, int32_t precision);
 int tfs_rpy_set_domain(tfs_rpy *h, int32_t domain);
 
 /* Override the launch shape (threads per workgroup 64/128/256, targets per
- * thread 1/2, sources per work item: a multiple of block) for a precision. */
+ * thread 1/2, sources per work item: a multiple of block) for a precision;
+ * every lane holds its own targets (no source split). */
 int tfs_rpy_set_tile_shape(tfs_rpy *h, int32_t precision, int32_t block, int32_t per_thread,
                            int32_t chunk);
 
@@ -159,10 +160,26 @@ int tfs_rpy_set_bodies(tfs_rpy *h, int32_t num_bodies, const int32_t *segs_per_b
                        const int32_t *blobs_per_body);
 
 /* Persistent launches (0 = off): a fixed grid of workgroups_per_cu workgroups
- * per CU walks the work list instead of one workgroup per item. */
+ * per CU instead of one workgroup per item; each takes a contiguous range of
+ * the work list and keeps a tile's sums in registers across its items,
+ * writing one partial slot per tile it touches.  The source split has no
+ * persistent form: a shape set with tfs_rpy_set_split_shape is refused, the
+ * default shape gives way to the plain default (block 256, 2 targets per
+ * thread). */
 int tfs_rpy_set_persistent(tfs_rpy *h, int32_t precision, int32_t workgroups_per_cu);
 /* Current launch shape of a precision: block, per_thread, chunk, persistent. */
 int tfs_rpy_get_tile_shape(tfs_rpy *h, int32_t precision, int32_t shape[4]);
+/* Launch shape with the source split (no counterpart in the reference, one
+ * thread per target, cuda_functions.cu:362-757): the block/64 waves of a
+ * workgroup (block 256 or 512) share one tile of 64 x per_thread targets and
+ * split every LDS batch of the item's `chunk` sources between them, adding
+ * their sums in LDS in wave order before the item's one store, so an item
+ * writes block/64 times fewer partial bytes.  One workgroup per item (no
+ * persistent form).  The default shape of both precisions; tfs_rpy_set_tile_shape
+ * selects the form without it. */
+int tfs_rpy_set_split_shape(tfs_rpy *h, int32_t precision, int32_t block, int32_t per_thread, int32_t chunk);
+/* 1 if the precision's current shape uses the source split. */
+int tfs_rpy_get_source_split(tfs_rpy *h, int32_t precision, int32_t *split);
 /* Per-body completion times of the last launch (either form), in ms from its
  * first work item's start (enable before the launch; reading synchronises). */
 int tfs_rpy_body_timing(tfs_rpy *h, int32_t enable);

@@ -1,7 +1,8 @@
 """Independent bodies in one launch (block-diagonal grand mobility): every
 body's velocities match the FP64 oracle run on that body alone; the
-persistent (queued) launch, every autotuned shape and hipGraph replays give
-the same numbers; per-body completion times come back for every body."""
+persistent launch, the source split, every autotuned shape and hipGraph
+replays give the same numbers; per-body completion times come back for every
+body."""
 import numpy as np
 import pytest
 
@@ -71,7 +72,8 @@ def test_autotune_keeps_results_and_picks_a_valid_shape():
             best = h.autotune(prec, reps=2)
             shape = h.tile_shape(prec)
             print(prec, shape, best)
-            assert shape["block"] in (64, 128, 256) and shape["per_thread"] in (1, 2) and best > 0
+            assert shape["block"] in (64, 128, 256, 512) and shape["per_thread"] in (1, 2) and best > 0
+            assert shape["split"] == 1 or shape["block"] != 512
             h.set_precision(prec)
             h.evaluate()
             got = read(h, e)
@@ -80,12 +82,16 @@ def test_autotune_keeps_results_and_picks_a_valid_shape():
         h.close()
 
 
-@pytest.mark.parametrize("persistent", [0, 2])
+@pytest.mark.parametrize("persistent", [0, 2, "split"])
 def test_graph_replays_match_eager_every_time(persistent):
     e = make_ensemble(sizes=SIZES, num_seg=6)
     h = handle_for(e, 6)
     try:
-        h.set_persistent(0, persistent)
+        if persistent == "split":
+            h.set_split_shape(0, 256, 2, 256)
+        else:
+            h.set_tile_shape(0, 256, 2, 256)
+            h.set_persistent(0, persistent)
         h.evaluate()
         eager = read(h, e)
         g = h.graph(L.BLOCK_ALL, 2)

@@ -16,8 +16,8 @@ something other than the GPU itself.
   oracle's right-hand side (mobility_solver.cpp:240-440).
 * The 64-body uneven ensemble (filaments U[100, 1000] x 20 segments, blobs
   U[2000, 9000]) as one block-diagonal 6-DOF matvec launch, in FP64 and with
-  FP32 tiles, default, autotuned and persistent launch shapes and a hipGraph
-  replay: the smallest, median and largest bodies and five seeded random ones
+  FP32 tiles, default (source split), autotuned, plain and persistent launch
+  shapes and a hipGraph replay: the smallest, median and largest bodies and five seeded random ones
   against the oracle's evaluate_full_mobility of that body alone
   (rpy_mobility_solver.cu:330-337).
 """
@@ -199,11 +199,19 @@ def test_ensemble_64_bodies_match_per_body_oracle():
             print(f"  {name}: autotuned shape {h.tile_shape(prec)}", flush=True)
             h.evaluate()
             results["autotuned"] = read(h, e)
-            keep = h.tile_shape(prec)["persistent"]
+            keep = h.tile_shape(prec)
+            h.set_tile_shape(prec, 256, 2, 1024)
+            h.evaluate()
+            results["plain"] = read(h, e)
             h.set_persistent(prec, 2)
             h.evaluate()
             results["persistent"] = read(h, e)
-            h.set_persistent(prec, keep)
+            h.set_persistent(prec, 0)
+            if keep["split"]:
+                h.set_split_shape(prec, keep["block"], keep["per_thread"], keep["chunk"])
+            else:
+                h.set_tile_shape(prec, keep["block"], keep["per_thread"], keep["chunk"])
+                h.set_persistent(prec, keep["persistent"])
             g = h.graph(L.BLOCK_ALL, 1)
             rng = np.random.default_rng(1)
             for k in range(3):

@@ -106,51 +106,47 @@ __global__ __launch_bounds__(256) void block_gauss_jordan_kernel(double* __restr
 
 // In-place inverse of symmetric positive definite row-major blocks of at most
 // 64 rows by Gauss-Jordan elimination without pivoting (SPD: every pivot is
-// positive), one wave per block, four blocks per workgroup.  Lane j keeps
-// column j in registers; the pivot row and column come from the owning lanes
-// by readlane, so no LDS and no barrier is needed.  status[b] = 1 if a pivot
-// is not positive and finite (the block is then left partly eliminated).
-__global__ __launch_bounds__(256) void spd_inverse_wave_kernel(double* __restrict__ A,
-                                                               const int64_t* __restrict__ off,
-                                                               const int32_t* __restrict__ sizes, int nblocks,
-                                                               int32_t* __restrict__ status) {
+// positive), one workgroup per block: the block sits in LDS, and each of the
+// n steps stages the pivot row and column, then every thread updates its
+// elements (16 of a 64 x 64 block).  status[b] = 1 if a pivot is not positive
+// and finite (the block is then left as loaded).
+__global__ __launch_bounds__(256) void spd_inverse_lds_kernel(double* __restrict__ A,
+                                                              const int64_t* __restrict__ off,
+                                                              const int32_t* __restrict__ sizes,
+                                                              int32_t* __restrict__ status) {
     constexpr int W = 64;
-    const int b = blockIdx.x * 4 + (int)(threadIdx.x / W);
-    const int lane = threadIdx.x & (W - 1);
-    if (b >= nblocks) return;
+    __shared__ double S[W][W + 1];
+    __shared__ double rowk[W], colk[W];
+    __shared__ int bad;
+    const int b = blockIdx.x;
     const int n = sizes[b];
     double* M = A + off[b];
-    const bool mine = lane < n;
-    double col[W];
-#pragma unroll
-    for (int i = 0; i < W; ++i) col[i] = (i < n && mine) ? M[(int64_t)i * n + lane] : 0.0;
-    int bad = 0;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < n * n; e += 256) S[e / n][e % n] = M[e];
+    if (tid == 0) bad = 0;
+    __syncthreads();
     for (int k = 0; k < n; ++k) {
-        double akj = 0.0;
-#pragma unroll
-        for (int i = 0; i < W; ++i)
-            if (i == k) akj = col[i];
-        const double d = lane_bcast(akj, k);
+        const double d = S[k][k];
         if (!(d > 0.0) || !__builtin_isfinite(d)) {
-            bad = 1;
-            break;  // uniform: every lane read the same d
+            if (tid == 0) bad = 1;
+            break;  // uniform: every thread read the same d
         }
         const double inv = 1.0 / d;
-        const double rk = lane == k ? inv : akj * inv;
-#pragma unroll
-        for (int i = 0; i < W; ++i) {
-            if (i < n) {
-                const double f = lane_bcast(col[i], k);
-                col[i] = i == k ? rk : (lane == k ? -f * inv : col[i] - f * rk);
-            }
+        // the pivot row scaled (its new values) and the pivot column (old values)
+        if (tid < n) rowk[tid] = tid == k ? inv : S[k][tid] * inv;
+        else if (tid >= 128 && tid - 128 < n) colk[tid - 128] = S[tid - 128][k];
+        __syncthreads();
+        for (int e = tid; e < n * n; e += 256) {
+            const int i = e / n, j = e % n;
+            const double f = colk[i], rk = rowk[j];
+            S[i][j] = i == k ? rk : (j == k ? -f * inv : S[i][j] - f * rk);
         }
+        __syncthreads();
     }
-    if (!bad && mine) {
-#pragma unroll
-        for (int i = 0; i < W; ++i)
-            if (i < n) M[(int64_t)i * n + lane] = col[i];
-    }
-    if (lane == 0) status[b] = bad;
+    __syncthreads();
+    if (!bad)
+        for (int e = tid; e < n * n; e += 256) M[e] = S[e / n][e % n];
+    if (tid == 0) status[b] = bad;
 }
 
 // ---- blocked Gauss-Jordan on 64 x 64 tiles ------------------------------------
@@ -267,8 +263,7 @@ hipError_t k_batched_inverse_spd(double* A, const int64_t* off, const int32_t* s
                                  int32_t* status, hipStream_t stream) {
     if (nblocks <= 0) return hipSuccess;
     if (max_size > 64) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(spd_inverse_wave_kernel, dim3((unsigned)((nblocks + 3) / 4)), dim3(256), 0, stream, A, off,
-                       sizes, nblocks, status);
+    hipLaunchKernelGGL(spd_inverse_lds_kernel, dim3((unsigned)nblocks), dim3(256), 0, stream, A, off, sizes, status);
     return hipGetLastError();
 }
 

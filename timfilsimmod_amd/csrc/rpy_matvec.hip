@@ -21,8 +21,13 @@
 // Every pair is counted exactly once.
 //
 // Splitting the source range into chunks is what fills the chip: 21,800
-// targets are only 341 wave64s, far fewer than the 1,024 SIMDs of an MI355X;
-// with chunks of ~1k sources the production body gives ~2k workgroups.
+// targets are only 341 wave64s, far fewer than the 1,024 SIMDs of an MI355X.
+// The default shape also splits each item's sources between the waves of its
+// workgroup (the waves share one tile of 64 x per_thread targets and add
+// their sums in LDS before the item's one store): 12,800 segments + 9,000
+// blobs in FP32 tiles are ~10k items of 64 targets x 768 sources, each
+// writing 64 x 3 FP32 sums, a quarter of what four waves with their own
+// targets would write.
 //
 // Two evaluation types:
 //   double : the FP64 path (parity with the FP64 oracle at <= 1e-12).
@@ -42,6 +47,7 @@ namespace tfs {
 namespace {
 
 constexpr int kMaxDof = 6;
+constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 
 // LDS record length (in T) of one staged source with FS strength components
 // for targets with VT velocity components.  FP64: x y z f0..f(FS-1) [pad].
@@ -132,16 +138,37 @@ __device__ __forceinline__ void stage_source(T* rec, const MatvecArgs& a, int sr
     }
 }
 
-// The pairs of one work item.  FP32 tiles work in the fixed-point frame
-// d' = lambda d (lambda = 2^k, exact): with forces staged as lambda c8 F and
-// torques as lambda^2 c8 tau, the translational sums come out in physical
-// units and the rotational ones divided by lambda, which the FP64 flush
-// multiplies back.
-template <int VT, int FS, typename T, int BLOCK, int R>
-__device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile tl, const MatvecArgs& a,
-                                         T* __restrict__ slot, T* __restrict__ lds) {
+// Target coordinates of a tile for the lanes of this thread (clamped into
+// the tile; surplus lanes compute a duplicate and their slot entries are
+// never read).  A tile has LANES x R targets: LANES = the workgroup size, or
+// 64 when the waves split the sources (every wave then holds the same
+// targets).
+template <typename T>
+using TargetT = typename std::conditional<sizeof(T) == 4, int32_t, double>::type;
+
+template <typename T, int LANES, int R>
+__device__ __forceinline__ void load_targets(const MatvecTile& tl, const MatvecArgs& a, TargetT<T> (&xi)[R][3]) {
+    const int lane = (int)threadIdx.x & (LANES - 1);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int t = tl.t_begin + lane + r * LANES;
+        t = t < tl.t_end ? t : tl.t_end - 1;
+        load_target(a, tl.cls, t, xi[r]);
+    }
+}
+
+// The pairs of one work item added into acc (the first VT entries of each
+// target's AV).  FP32 tiles work in the fixed-point frame d' = lambda d
+// (lambda = 2^k, exact): with forces staged as lambda c8 F and torques as
+// lambda^2 c8 tau, the translational sums come out in physical units and the
+// rotational ones divided by lambda, which store_slot multiplies back.  FP32
+// sums one LDS batch in `part` and flushes it into the FP64 accumulators.
+// With SPLIT, wave w evaluates records [64 w, 64 w + 64) of every batch.
+template <int VT, int FS, typename T, int BLOCK, int R, int AV, bool SPLIT = false>
+__device__ __forceinline__ void accumulate_item(const MatvecItem it, const MatvecTile tl, const MatvecArgs& a,
+                                                const TargetT<T> (&xi)[R][3], double (&acc)[R][AV],
+                                                T* __restrict__ lds) {
     constexpr int REC = rec_len<T, FS, VT>();
-    constexpr int TILE = BLOCK * R;
     constexpr int F0 = 3;
     constexpr bool F32 = sizeof(T) == 4;
     const int tid = threadIdx.x;
@@ -156,28 +183,12 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
     const T m3s3 = T(-3) * s3;
     const double c8 = 1.0 / (8.0 * PI * a.mu);  // folded into the staged strengths
 
-    // Targets of this lane (clamped into the tile; surplus lanes compute a
-    // duplicate and their slot entries are never read).
-    using XT = typename std::conditional<F32, int32_t, double>::type;
-    XT xi[R][3];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        int t = tl.t_begin + tid + r * BLOCK;
-        t = t < tl.t_end ? t : tl.t_end - 1;
-        load_target(a, tl.cls, t, xi[r]);
-    }
-
-    // FP64 accumulates straight into acc; FP32 sums one LDS batch in `part`
-    // and flushes it into the FP64 accumulators.
-    double acc[R][VT];
     T part[R][F32 ? VT : 1];
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-#pragma unroll
-        for (int k = 0; k < VT; ++k) acc[r][k] = 0.0;
-
+    // this wave's records of a batch start at j0 (uniform over the wave)
+    const int j0 = SPLIT ? __builtin_amdgcn_readfirstlane(tid >> 6) * 64 : 0;
     for (int s0 = it.s_begin; s0 < it.s_end; s0 += BLOCK) {
         const int n = (it.s_end - s0) < BLOCK ? (it.s_end - s0) : BLOCK;
+        const int j1 = SPLIT ? (n < j0 + 64 ? n : j0 + 64) : n;
         __syncthreads();  // the previous batch has been consumed
         if (tid < n) stage_source<T, FS, VT>(lds + tid * REC, a, it.src_cls, s0 + tid, true, lam * c8, lam * lam * c8);
         __syncthreads();
@@ -189,7 +200,7 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
         }
 
 #pragma unroll 2
-        for (int j = 0; j < n; ++j) {
+        for (int j = j0; j < j1; ++j) {
             const T* rec = lds + j * REC;
             T f[FS];
 #pragma unroll
@@ -213,37 +224,95 @@ __device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile t
                 for (int k = 0; k < VT; ++k) acc[r][k] += double(part[r][k]);
         }
     }
+}
 
-    // FP32 tiles store their item sums in FP32: with one LDS batch per item the
-    // sum is an FP32 batch sum already, so nothing is lost, and the reduce
-    // kernel reads half the bytes.
+// A tile's sums (VT per target) into its partial slot (LANES x R targets).
+// FP32 tiles store them in FP32: the reduce kernel reads half the bytes.
+template <int VT, typename T, int LANES, int R, int AV>
+__device__ __forceinline__ void store_slot(T* __restrict__ slot, const double (&acc)[R][AV], double lam) {
+    constexpr int TILE = LANES * R;
+    const int lane = (int)threadIdx.x & (LANES - 1);
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
-        for (int k = 0; k < VT; ++k) slot[k * TILE + tid + r * BLOCK] = T(k < 3 ? acc[r][k] : lam * acc[r][k]);
+        for (int k = 0; k < VT; ++k)
+            slot[k * TILE + lane + r * LANES] = T(k < 3 ? acc[r][k] : lam * acc[r][k]);
+}
+
+// Source split: waves 1 .. S-1 hand their sums to wave 0 through LDS, which
+// adds them in wave order (a fixed order: results are reproducible).
+template <int VT, int BLOCK, int R, int AV>
+__device__ __forceinline__ void combine_waves(double (&acc)[R][AV], double* __restrict__ buf) {
+    const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    __syncthreads();  // every wave is done with the last batch's records
+    if (w > 0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int k = 0; k < VT; ++k) buf[(((w - 1) * R + r) * VT + k) * 64 + lane] = acc[r][k];
+    }
+    __syncthreads();
+    if (w == 0) {
+        for (int v = 1; v < BLOCK / 64; ++v)
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int k = 0; k < VT; ++k) acc[r][k] += buf[(((v - 1) * R + r) * VT + k) * 64 + lane];
+    }
+}
+
+// LDS of the one-workgroup-per-item kernel, in T: a batch of the largest
+// records, or (source split) the sums of BLOCK/64 - 1 waves, if larger.
+template <typename T, int BLOCK, int R, bool SPLIT>
+constexpr int item_lds_len() {
+    constexpr int batch = BLOCK * rec_len_max<T>();
+    constexpr int sums = SPLIT ? (BLOCK / 64 - 1) * 64 * R * kMaxDof * (int)(sizeof(double) / sizeof(T)) : 0;
+    return batch > sums ? batch : sums;
+}
+
+// The pairs of one work item into its own slot.
+template <int VT, int FS, typename T, int BLOCK, int R, bool SPLIT>
+__device__ __forceinline__ void run_item(const MatvecItem it, const MatvecTile tl, const MatvecArgs& a,
+                                         T* __restrict__ slot, T* __restrict__ lds) {
+    constexpr int LANES = SPLIT ? 64 : BLOCK;
+    TargetT<T> xi[R][3];
+    load_targets<T, LANES, R>(tl, a, xi);
+    double acc[R][VT];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int k = 0; k < VT; ++k) acc[r][k] = 0.0;
+    accumulate_item<VT, FS, T, BLOCK, R, VT, SPLIT>(it, tl, a, xi, acc, lds);
+    if constexpr (SPLIT) {
+        combine_waves<VT, BLOCK, R, VT>(acc, reinterpret_cast<double*>(lds));
+        if (threadIdx.x >= 64) return;
+    }
+    store_slot<VT, T, LANES, R, VT>(slot, acc, sizeof(T) == 4 ? a.lam : 1.0);
 }
 
 // One workgroup per item.  With `stamps`, thread 0 stores the item's start
 // and finish (wall_clock64 ticks; equal for an item of a skipped body).
-template <typename T, int SEGDOF, int BLOCK, int R>
+// SPLIT: the BLOCK/64 waves share the tile's 64 x R targets and split every
+// batch of sources (BLOCK/64 times fewer partial bytes per item).
+template <typename T, int SEGDOF, int BLOCK, int R, bool SPLIT>
 __global__ __launch_bounds__(BLOCK) void rpy_pairs_kernel(const MatvecItem* __restrict__ items,
                                                           const MatvecTile* __restrict__ tiles,
                                                           MatvecArgs a, T* __restrict__ partial,
                                                           int64_t* __restrict__ stamps) {
-    constexpr int REC = rec_len_max<T>();
-    __shared__ __align__(16) T lds[BLOCK * REC];
+    constexpr int TILE = (SPLIT ? 64 : BLOCK) * R;
+    __shared__ __align__(16) T lds[item_lds_len<T, BLOCK, R, SPLIT>()];
     if (a.skip && *a.skip) return;
     const int64_t t_start = stamps ? (int64_t)wall_clock64() : 0;
     const MatvecItem it = items[blockIdx.x];
     const MatvecTile tl = tiles[it.tile];
     if (!(a.body_skip && a.body_skip[tl.body])) {
-        T* slot = partial + (size_t)blockIdx.x * (kMaxDof * BLOCK * R);
+        T* slot = partial + (size_t)blockIdx.x * (kMaxDof * TILE);
         if (tl.cls == 0) {
-            if (it.src_cls == 0) run_item<SEGDOF, SEGDOF, T, BLOCK, R>(it, tl, a, slot, lds);
-            else run_item<SEGDOF, 3, T, BLOCK, R>(it, tl, a, slot, lds);
+            if (it.src_cls == 0) run_item<SEGDOF, SEGDOF, T, BLOCK, R, SPLIT>(it, tl, a, slot, lds);
+            else run_item<SEGDOF, 3, T, BLOCK, R, SPLIT>(it, tl, a, slot, lds);
         } else {
-            if (it.src_cls == 0) run_item<3, SEGDOF, T, BLOCK, R>(it, tl, a, slot, lds);
-            else run_item<3, 3, T, BLOCK, R>(it, tl, a, slot, lds);
+            if (it.src_cls == 0) run_item<3, SEGDOF, T, BLOCK, R, SPLIT>(it, tl, a, slot, lds);
+            else run_item<3, 3, T, BLOCK, R, SPLIT>(it, tl, a, slot, lds);
         }
     }
     if (stamps && threadIdx.x == 0) {
@@ -350,21 +419,35 @@ __global__ __launch_bounds__(kWallBlock) void rpy_pairs_wall_kernel(const Matvec
     }
 }
 
-// Persistent form: a fixed grid of `gridDim.x` workgroups walks the work list
-// with stride gridDim.x (the bodies' tiles interleaved, items of similar
-// size), so every workgroup's loop count and exit are uniform -- no atomics
-// and no lane-divergent code in the loop.  With timing on, every lane of a
-// workgroup stores the same per-item start/finish stamps (wall_clock64
-// ticks; equal for an item of a skipped body).
+// Persistent form: workgroup w takes the contiguous items [w k, (w+1) k) of
+// the work list (items of one tile are contiguous, of similar size) and keeps
+// one tile's sums in registers across its items, storing them once per tile
+// segment, into the slot of the segment's first item: the reduce kernel reads
+// only those slots (segment starts are the tiles' first items and the
+// multiples of k), about k times fewer partial bytes than one slot per item.
+// Loop counts and exits are uniform over the workgroup.  With timing on,
+// every lane of a workgroup stores the same per-item start/finish stamps
+// (wall_clock64 ticks; equal for an item of a skipped body).
 template <typename T, int SEGDOF, int BLOCK, int R>
 __global__ __launch_bounds__(BLOCK) void rpy_pairs_persistent_kernel(const MatvecItem* __restrict__ items,
                                                                      const MatvecTile* __restrict__ tiles,
                                                                      MatvecArgs a, T* __restrict__ partial,
-                                                                     int n_items, PersistState ps) {
+                                                                     int n_items, int k, PersistState ps) {
     constexpr int REC = rec_len_max<T>();
     __shared__ __align__(16) T lds[BLOCK * REC];
     if (a.skip && *a.skip) return;
-    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int i0 = blockIdx.x * k;
+    const int i1 = i0 + k < n_items ? i0 + k : n_items;
+    const double lam = sizeof(T) == 4 ? a.lam : 1.0;
+    TargetT<T> xi[R][3];
+    double acc[R][SEGDOF];
+    int cur = -1, cur_cls = 0, seg = 0;
+    auto flush = [&]() {
+        T* slot = partial + (size_t)seg * (kMaxDof * BLOCK * R);
+        if (cur_cls == 0) store_slot<SEGDOF, T, BLOCK, R, SEGDOF>(slot, acc, lam);
+        else store_slot<3, T, BLOCK, R, SEGDOF>(slot, acc, lam);
+    };
+    for (int item = i0; item < i1; ++item) {
         if (ps.item_stamps) ps.item_stamps[2 * item] = (int64_t)wall_clock64();
         const MatvecItem it = items[item];
         const MatvecTile tl = tiles[it.tile];
@@ -372,16 +455,27 @@ __global__ __launch_bounds__(BLOCK) void rpy_pairs_persistent_kernel(const Matve
             if (ps.item_stamps) ps.item_stamps[2 * item + 1] = ps.item_stamps[2 * item];
             continue;
         }
-        T* slot = partial + (size_t)item * (kMaxDof * BLOCK * R);
+        if (it.tile != cur) {
+            if (cur >= 0) flush();
+            cur = it.tile;
+            cur_cls = tl.cls;
+            seg = item;
+            load_targets<T, BLOCK, R>(tl, a, xi);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int c = 0; c < SEGDOF; ++c) acc[r][c] = 0.0;
+        }
         if (tl.cls == 0) {
-            if (it.src_cls == 0) run_item<SEGDOF, SEGDOF, T, BLOCK, R>(it, tl, a, slot, lds);
-            else run_item<SEGDOF, 3, T, BLOCK, R>(it, tl, a, slot, lds);
+            if (it.src_cls == 0) accumulate_item<SEGDOF, SEGDOF, T, BLOCK, R, SEGDOF>(it, tl, a, xi, acc, lds);
+            else accumulate_item<SEGDOF, 3, T, BLOCK, R, SEGDOF>(it, tl, a, xi, acc, lds);
         } else {
-            if (it.src_cls == 0) run_item<3, SEGDOF, T, BLOCK, R>(it, tl, a, slot, lds);
-            else run_item<3, 3, T, BLOCK, R>(it, tl, a, slot, lds);
+            if (it.src_cls == 0) accumulate_item<3, SEGDOF, T, BLOCK, R, SEGDOF>(it, tl, a, xi, acc, lds);
+            else accumulate_item<3, 3, T, BLOCK, R, SEGDOF>(it, tl, a, xi, acc, lds);
         }
         if (ps.item_stamps) ps.item_stamps[2 * item + 1] = (int64_t)wall_clock64();
     }
+    if (cur >= 0) flush();
 }
 
 // ---------------------------------------------------------------------------
@@ -690,7 +784,8 @@ __device__ void saddle_filament_rows(const SaddleEpilogue& e, int m, int lane) {
 template <int SEGDOF, typename P>
 __global__ __launch_bounds__(256) void rpy_reduce_kernel(const MatvecTile* __restrict__ tiles,
                                                          const P* __restrict__ partial, int tile_targets,
-                                                         int tile_blocks, OutArgs o, MatvecArgs a, NearList near) {
+                                                         int tile_blocks, int merge_k, OutArgs o, MatvecArgs a,
+                                                         NearList near) {
     __shared__ double grp_acc[3][6][64];
     if (a.skip && *a.skip) return;
     const int grp = threadIdx.x >> 6;
@@ -711,20 +806,26 @@ __global__ __launch_bounds__(256) void rpy_reduce_kernel(const MatvecTile* __res
     const size_t slot_len = (size_t)kMaxDof * tile_targets;
     const int lc = l < tile_targets ? l : tile_targets - 1;
     const P* p = partial + (size_t)tl.first_item * slot_len + lc;
+    // Filled slots: every item's, or (persistent form, merge_k items per
+    // workgroup) the tile's first item and the multiples of merge_k after it.
+    const int q0 = merge_k > 0 ? tl.first_item / merge_k : 0;
+    const int nslots = merge_k > 0 && tl.n_items > 0 ? (tl.first_item + tl.n_items - 1) / merge_k - q0 + 1
+                                                     : tl.n_items;
+    auto slot_of = [&](int q) { return merge_k > 0 ? (q == 0 ? 0 : (q0 + q) * merge_k - tl.first_item) : q; };
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int m0 = grp; m0 < tl.n_items; m0 += 16) {
+    for (int m0 = grp; m0 < nslots; m0 += 16) {
         double v[4][6];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int m = m0 + 4 * u;
-            const int mm = m < tl.n_items ? m : grp;
+            const int mm = slot_of(m < nslots ? m : grp);
 #pragma unroll
             for (int k = 0; k < 6; ++k)
                 v[u][k] = k < vt ? double(p[(size_t)mm * slot_len + (size_t)k * tile_targets]) : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const bool in = m0 + 4 * u < tl.n_items;
+            const bool in = m0 + 4 * u < nslots;
 #pragma unroll
             for (int k = 0; k < 6; ++k) acc[k] += in ? v[u][k] : 0.0;
         }
@@ -840,16 +941,24 @@ __global__ __launch_bounds__(256) void near_sort_kernel(const int32_t* __restric
     }
 }
 
-template <typename T, int SEGDOF, int BLOCK, int R>
+// Items per workgroup of the persistent form: the work list over
+// persistent x CUs workgroups, as contiguous ranges.
+int persistent_items(const TileShape& sh, int n_items, int num_cus) {
+    const long g = (long)sh.persistent * (num_cus > 0 ? num_cus : 256);
+    return (int)std::max<long>(1, (n_items + g - 1) / g);
+}
+
+template <typename T, int SEGDOF, int BLOCK, int R, bool SPLIT>
 hipError_t launch_pairs(const TileShape& sh, const MatvecArgs& a, const MatvecItem* d_items, int n_items,
                         const MatvecTile* d_tiles, double* partial, const PersistState& ps, hipStream_t stream) {
-    if (sh.persistent > 0) {
-        int grid = sh.persistent * ps.num_cus;
-        if (grid > n_items) grid = n_items;
-        hipLaunchKernelGGL((rpy_pairs_persistent_kernel<T, SEGDOF, BLOCK, R>), dim3(grid), dim3(BLOCK), 0, stream,
-                           d_items, d_tiles, a, reinterpret_cast<T*>(partial), n_items, ps);
+    if (SPLIT && sh.persistent > 0) return hipErrorInvalidValue;  // no persistent form of the split
+    if (!SPLIT && sh.persistent > 0) {
+        const int k = persistent_items(sh, n_items, ps.num_cus);
+        hipLaunchKernelGGL((rpy_pairs_persistent_kernel<T, SEGDOF, BLOCK, R>), dim3((unsigned)((n_items + k - 1) / k)),
+                           dim3(BLOCK), 0, stream, d_items, d_tiles, a, reinterpret_cast<T*>(partial), n_items, k,
+                           ps);
     } else {
-        hipLaunchKernelGGL((rpy_pairs_kernel<T, SEGDOF, BLOCK, R>), dim3(n_items), dim3(BLOCK), 0, stream,
+        hipLaunchKernelGGL((rpy_pairs_kernel<T, SEGDOF, BLOCK, R, SPLIT>), dim3(n_items), dim3(BLOCK), 0, stream,
                            d_items, d_tiles, a, reinterpret_cast<T*>(partial), ps.item_stamps);
     }
     return hipGetLastError();
@@ -859,11 +968,11 @@ template <typename T, int SEGDOF>
 hipError_t dispatch_shape(const TileShape& sh, const MatvecArgs& a, const MatvecItem* d_items, int n_items,
                           const MatvecTile* d_tiles, double* partial, const PersistState& ps,
                           hipStream_t stream) {
-#define TFS_SHAPE(B, R)                                                                       \
-    if (sh.block == B && sh.per_thread == R)                                                 \
-        return launch_pairs<T, SEGDOF, B, R>(sh, a, d_items, n_items, d_tiles, partial, ps, stream);
-    TFS_SHAPE(64, 1) TFS_SHAPE(64, 2) TFS_SHAPE(128, 1) TFS_SHAPE(128, 2) TFS_SHAPE(256, 1)
-    TFS_SHAPE(256, 2)
+#define TFS_SHAPE(B, R, S)                                                                    \
+    if (sh.block == B && sh.per_thread == R && sh.split == S)                                \
+        return launch_pairs<T, SEGDOF, B, R, S>(sh, a, d_items, n_items, d_tiles, partial, ps, stream);
+    TFS_SHAPE(64, 1, 0) TFS_SHAPE(64, 2, 0) TFS_SHAPE(128, 1, 0) TFS_SHAPE(128, 2, 0) TFS_SHAPE(256, 1, 0)
+    TFS_SHAPE(256, 2, 0) TFS_SHAPE(256, 1, 1) TFS_SHAPE(256, 2, 1) TFS_SHAPE(512, 1, 1) TFS_SHAPE(512, 2, 1)
 #undef TFS_SHAPE
     return hipErrorInvalidValue;
 }
@@ -911,12 +1020,35 @@ void build_worklist(const std::vector<BodyRange>& bodies, int tile_targets, int 
         const bool from_segs = tl.cls == 0 ? ss : bs, from_blobs = tl.cls == 0 ? sb : bb;
         tl.first_item = (int)items.size();
         const int tile_id = (int)tiles.size();
+        std::vector<MatvecItem> chunks;  // the tile's items in source order
         auto add_src = [&](int src_cls, int lo, int hi) {
-            for (int s0 = lo; s0 < hi; s0 += chunk) items.push_back({tile_id, src_cls, s0, std::min(s0 + chunk, hi)});
+            for (int s0 = lo; s0 < hi; s0 += chunk) chunks.push_back({tile_id, src_cls, s0, std::min(s0 + chunk, hi)});
         };
         if (from_segs) add_src(0, body.seg_begin, body.seg_end);
         if (from_blobs) add_src(1, body.blob_begin, body.blob_end);
-        tl.n_items = (int)items.size() - tl.first_item;
+        // Workgroup i of a launch runs on XCD i mod 8.  Chunk c of every tile
+        // goes to a position p with p = c (mod 8) where one is free, so each
+        // XCD's L2 holds about an eighth of the sources instead of all of them.
+        const int n = (int)chunks.size();
+        std::vector<int> next(kXcds, 0), order(n, -1);
+        std::vector<char> used(n, 0);
+        for (int i = 0; i < n; ++i) {
+            const int want = (tl.first_item + i) % kXcds;
+            int& c = next[want];
+            while (c * kXcds + want < n && used[c * kXcds + want]) ++c;
+            if (c * kXcds + want < n) {
+                order[i] = c * kXcds + want;
+                used[order[i]] = 1;
+            }
+        }
+        for (int i = 0, c = 0; i < n; ++i) {
+            if (order[i] >= 0) continue;
+            while (used[c]) ++c;
+            order[i] = c;
+            used[c] = 1;
+        }
+        for (int i = 0; i < n; ++i) items.push_back(chunks[order[i]]);
+        tl.n_items = n;
         tiles.push_back(tl);
     }
 }
@@ -1033,17 +1165,17 @@ hipError_t launch_matvec_wall(int precision, int seg_dof, const MatvecArgs& a, c
     if (precision == 0) {
         if (seg_dof == 6)
             hipLaunchKernelGGL((rpy_reduce_kernel<6, double>), grid, dim3(256), 0, stream, d_tiles,
-                               (const double*)partial, kWallBlock, tile_blocks, o, a, none);
+                               (const double*)partial, kWallBlock, tile_blocks, 0, o, a, none);
         else
             hipLaunchKernelGGL((rpy_reduce_kernel<3, double>), grid, dim3(256), 0, stream, d_tiles,
-                               (const double*)partial, kWallBlock, tile_blocks, o, a, none);
+                               (const double*)partial, kWallBlock, tile_blocks, 0, o, a, none);
     } else {
         if (seg_dof == 6)
             hipLaunchKernelGGL((rpy_reduce_kernel<6, float>), grid, dim3(256), 0, stream, d_tiles, (const float*)pf,
-                               kWallBlock, tile_blocks, o, a, none);
+                               kWallBlock, tile_blocks, 0, o, a, none);
         else
             hipLaunchKernelGGL((rpy_reduce_kernel<3, float>), grid, dim3(256), 0, stream, d_tiles, (const float*)pf,
-                               kWallBlock, tile_blocks, o, a, none);
+                               kWallBlock, tile_blocks, 0, o, a, none);
     }
     return hipGetLastError();
 }
@@ -1066,26 +1198,28 @@ hipError_t launch_matvec(int precision, int seg_dof, const TileShape& shape, con
         if (e != hipSuccess) return e;
         if (ev_end) TFS_HIP_CHECK(hipEventRecord(ev_end, stream));
     }
-    const int tt = shape.block * shape.per_thread;
+    const int tt = tile_targets(shape);
     const int tile_blocks = n_tiles * ((tt + 63) / 64);
     const int fil_blocks = o.sad.in ? (o.sad.nfil + 3) / 4 : 0;
     const dim3 grid((unsigned)(tile_blocks + fil_blocks));
-    // The partial slots hold the evaluation type (FP32 sums for FP32 tiles).
+    // The partial slots hold the evaluation type (FP32 sums for FP32 tiles);
+    // the persistent form fills only its tile segments' first slots.
+    const int merge_k = shape.persistent > 0 && n_items > 0 ? persistent_items(shape, n_items, ps.num_cus) : 0;
     const float* pf = reinterpret_cast<const float*>(partial);
     if (precision == 0) {
         if (seg_dof == 6)
             hipLaunchKernelGGL((rpy_reduce_kernel<6, double>), grid, dim3(256), 0, stream, d_tiles,
-                               (const double*)partial, tt, tile_blocks, o, a, near);
+                               (const double*)partial, tt, tile_blocks, merge_k, o, a, near);
         else
             hipLaunchKernelGGL((rpy_reduce_kernel<3, double>), grid, dim3(256), 0, stream, d_tiles,
-                               (const double*)partial, tt, tile_blocks, o, a, near);
+                               (const double*)partial, tt, tile_blocks, merge_k, o, a, near);
     } else {
         if (seg_dof == 6)
             hipLaunchKernelGGL((rpy_reduce_kernel<6, float>), grid, dim3(256), 0, stream, d_tiles, pf, tt,
-                               tile_blocks, o, a, near);
+                               tile_blocks, merge_k, o, a, near);
         else
             hipLaunchKernelGGL((rpy_reduce_kernel<3, float>), grid, dim3(256), 0, stream, d_tiles, pf, tt,
-                               tile_blocks, o, a, near);
+                               tile_blocks, merge_k, o, a, near);
     }
     return hipGetLastError();
 }

@@ -94,7 +94,7 @@ struct tfs_rpy {
         int precision;
     };
     std::vector<EvPair> ev_used, ev_free;
-    std::map<std::tuple<uint32_t, int, int, int>, tfs::Plan> plans;
+    std::map<std::tuple<uint32_t, int, int, int, int>, tfs::Plan> plans;
     tfs::F32Stage f32;  // FP32 entry points (tfs_rpy_*_f32)
 };
 
@@ -115,7 +115,7 @@ int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* where);
 // TFS_OK for hipSuccess, else hip_fail(e, where).
 int hip_fail_if(hipError_t e, const char* where);
-TileShape default_shape(int precision, int ns, int nb, int num_cus);
+TileShape default_shape(int precision, int ns, int nb, int num_cus, int seg_dof, int num_bodies);
 Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err);
 // Positions changed: the near-pair lists must be rebuilt before the next matvec.
 void invalidate_near(tfs_rpy* h);

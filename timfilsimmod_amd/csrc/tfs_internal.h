@@ -87,12 +87,17 @@ struct BarrierParams {
 
 // Launch shape of the pair kernel; chosen by the autotuner.
 struct TileShape {
-    int block;       // threads per workgroup (64, 128, 256)
+    int block;       // threads per workgroup (64, 128, 256; 512 with split)
     int per_thread;  // targets per thread (1 or 2)
     int chunk;       // sources per work item (a multiple of block)
-    int persistent;  // 0: one workgroup per item; >0: that many workgroups per CU pulling
-                     // items from an atomic queue
+    int persistent;  // 0: one workgroup per item; >0: that many workgroups per CU, each
+                     // taking a contiguous range of the work list
+    int split = 0;   // 1: the block/64 waves share one tile of 64 x per_thread targets and
+                     // split every LDS batch of sources between them
 };
+
+// Targets per tile (and per partial slot) of a launch shape.
+inline int tile_targets(const TileShape& s) { return (s.split ? 64 : s.block) * s.per_thread; }
 
 // State of a persistent launch: optional per-item start/finish stamps
 // (wall_clock64 ticks), two per item.

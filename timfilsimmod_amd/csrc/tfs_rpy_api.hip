@@ -35,22 +35,46 @@ int hip_fail(hipError_t e, const char* where) {
 
 int hip_fail_if(hipError_t e, const char* where) { return e == hipSuccess ? TFS_OK : hip_fail(e, where); }
 
-TileShape default_shape(int precision, int ns, int nb, int num_cus) {
-    TileShape s;
-    s.block = 256;
-    s.per_thread = 2;
-    s.persistent = 0;
-    (void)precision;
-    const int tt = s.block * s.per_thread;
+// s.chunk for ~per_cu items per CU over a body of ns segments and nb blobs
+// (a multiple of the block).
+static void size_chunk(TileShape& s, int ns, int nb, int num_cus, long per_cu) {
+    const int tt = tile_targets(s);
     const long n_tiles = (ns + tt - 1) / tt + (nb + tt - 1) / tt;
     const long n_src = (long)ns + nb;
-    // Aim for ~16 workgroups per CU over the whole work list (measured best at
-    // the production size for both precisions: short items even out the tail).
-    const long want_items = 16L * (num_cus > 0 ? num_cus : 256);
+    const long want_items = per_cu * (num_cus > 0 ? num_cus : 256);
     long chunk = n_tiles > 0 ? (n_src * n_tiles + want_items - 1) / want_items : n_src;
     chunk = ((chunk + s.block - 1) / s.block) * s.block;
     if (chunk < s.block) chunk = s.block;
     s.chunk = (int)chunk;
+}
+
+TileShape default_shape(int precision, int ns, int nb, int num_cus, int seg_dof, int num_bodies) {
+    // Measured on MI355X (pair kernel, ms per launch): one body of 12,800
+    // segments + 9,000 blobs, 3-DOF segments, is fastest with the source
+    // split and 64-target tiles in FP32 (0.266 against 0.279 for the best
+    // plain shape) and 128-target tiles in FP64 (0.430 against 0.444), with
+    // items sized for ~48 and ~32 per CU; the 6-DOF matvec of that body
+    // likewise (FP32 0.391 with 64-target tiles, 0.400 with 128; FP64 0.716,
+    // against 0.418 and 0.792 without the split).  64 uneven bodies in one
+    // launch have work to spare: there the split with 128-target tiles and
+    // ~512-source items is within 2% of the best plain shape in both
+    // precisions, and 64-target tiles lose 5-10%.
+    (void)seg_dof;
+    TileShape s;
+    s.block = 256;
+    s.persistent = 0;
+    s.split = 1;
+    s.per_thread = num_bodies == 1 && precision == 1 ? 1 : 2;
+    const long per_cu = num_bodies > 1 ? 64 : s.per_thread == 1 ? 48 : 32;
+    size_chunk(s, ns, nb, num_cus, per_cu);
+    return s;
+}
+
+// Without the split: 512-target tiles, ~16 items per CU (short items even
+// out the tail).
+static TileShape plain_shape(int ns, int nb, int num_cus) {
+    TileShape s{256, 2, 256, 0, 0};
+    size_chunk(s, ns, nb, num_cus, 16);
     return s;
 }
 
@@ -82,13 +106,12 @@ Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err) {
     if (bb) blob_write = 0;
     else if (bs) blob_write = h->desc.bs_mode == TFS_BS_OVERWRITE ? 0 : 1;
     const uint32_t eff = (ss ? 1u : 0u) | (sb ? 2u : 0u) | (bb ? 4u : 0u) | (bs ? 8u : 0u);
-    const auto key = std::make_tuple(eff, sh.block, sh.per_thread, sh.chunk);
+    const auto key = std::make_tuple(eff, sh.block, sh.per_thread, sh.chunk, sh.split);
     auto it = h->plans.find(key);
     if (it != h->plans.end()) return &it->second;
     Plan p;
     p.blocks = eff;
-    build_worklist(h->bodies, sh.block * sh.per_thread, sh.chunk, ss, sb, bb, bs, seg_write, blob_write, p.tiles,
-                   p.items);
+    build_worklist(h->bodies, tile_targets(sh), sh.chunk, ss, sb, bb, bs, seg_write, blob_write, p.tiles, p.items);
     if (!p.tiles.empty()) {
         hipError_t e = hipMalloc(&p.d_tiles, sizeof(MatvecTile) * p.tiles.size());
         if (e == hipSuccess && !p.items.empty())
@@ -105,7 +128,7 @@ Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err) {
             *err = hip_fail(e, "work list upload");
             return nullptr;
         }
-        const size_t need = p.items.size() * 6 * (size_t)(sh.block * sh.per_thread);
+        const size_t need = p.items.size() * 6 * (size_t)tile_targets(sh);
         if (need > h->partial_len) {
             if (h->partial) {
                 hipStreamSynchronize(h->stream);
@@ -511,7 +534,8 @@ int tfs_rpy_create(const tfs_rpy_desc* desc, tfs_rpy** out) {
         tfs_rpy_destroy(h);
         return hip_fail(e, "tfs_rpy_create: clear");
     }
-    for (int pr = 0; pr < 2; ++pr) h->shape[pr] = default_shape(pr, desc->num_segs, desc->num_blobs, cus);
+    for (int pr = 0; pr < 2; ++pr)
+        h->shape[pr] = default_shape(pr, desc->num_segs, desc->num_blobs, cus, desc->seg_dof, 1);
     h->bodies.push_back(BodyRange{0, desc->num_segs, 0, desc->num_blobs});
     e = hipHostMalloc((void**)&h->host_int, 64, hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -717,20 +741,33 @@ int tfs_rpy_set_precision(tfs_rpy* h, int32_t precision) {
     return TFS_OK;
 }
 
-int tfs_rpy_set_tile_shape(tfs_rpy* h, int32_t precision, int32_t block, int32_t per_thread,
-                           int32_t chunk) {
-    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_tile_shape: null handle");
+static int set_shape(tfs_rpy* h, const char* fn, int32_t precision, int32_t block, int32_t per_thread,
+                     int32_t chunk, int split) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, std::string(fn) + ": null handle");
     if (precision != 0 && precision != 1)
-        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_tile_shape: unknown precision");
-    if (block != 64 && block != 128 && block != 256)
-        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_tile_shape: block must be 64, 128 or 256");
+        return fail(TFS_ERR_INVALID_ARGUMENT, std::string(fn) + ": unknown precision");
+    if (split ? (block != 256 && block != 512) : (block != 64 && block != 128 && block != 256))
+        return fail(TFS_ERR_INVALID_ARGUMENT,
+                    std::string(fn) + (split ? ": block must be 256 or 512" : ": block must be 64, 128 or 256"));
     if (per_thread != 1 && per_thread != 2)
-        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_tile_shape: per_thread must be 1 or 2");
+        return fail(TFS_ERR_INVALID_ARGUMENT, std::string(fn) + ": per_thread must be 1 or 2");
     if (chunk < block || chunk % block)
-        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_tile_shape: chunk must be a positive multiple of block");
-    h->shape[precision] = TileShape{block, per_thread, chunk, h->shape[precision].persistent};
+        return fail(TFS_ERR_INVALID_ARGUMENT, std::string(fn) + ": chunk must be a positive multiple of block");
+    const int pers = h->shape[precision].persistent;
+    if (split && pers > 0)
+        return fail(TFS_ERR_INVALID_ARGUMENT, std::string(fn) + ": the source split has no persistent form");
+    h->shape[precision] = TileShape{block, per_thread, chunk, pers, split};
     h->shape_chosen[precision] = true;
     return TFS_OK;
+}
+
+int tfs_rpy_set_tile_shape(tfs_rpy* h, int32_t precision, int32_t block, int32_t per_thread,
+                           int32_t chunk) {
+    return set_shape(h, "tfs_rpy_set_tile_shape", precision, block, per_thread, chunk, 0);
+}
+
+int tfs_rpy_set_split_shape(tfs_rpy* h, int32_t precision, int32_t block, int32_t per_thread, int32_t chunk) {
+    return set_shape(h, "tfs_rpy_set_split_shape", precision, block, per_thread, chunk, 1);
 }
 
 int tfs_rpy_positions_changed(tfs_rpy* h) {
@@ -763,11 +800,8 @@ int tfs_rpy_set_bodies(tfs_rpy* h, int32_t num_bodies, const int32_t* segs_per_b
     int lns = 0, lnb = 0;
     largest_body(h, &lns, &lnb);
     for (int pr = 0; pr < 2; ++pr)
-        if (!h->shape_chosen[pr]) {
-            const int pers = h->shape[pr].persistent;
-            h->shape[pr] = default_shape(pr, lns, lnb, h->num_cus);
-            h->shape[pr].persistent = pers;
-        }
+        if (!h->shape_chosen[pr])
+            h->shape[pr] = default_shape(pr, lns, lnb, h->num_cus, h->desc.seg_dof, (int)h->bodies.size());
     return TFS_OK;
 }
 
@@ -848,18 +882,25 @@ int tfs_rpy_autotune(tfs_rpy* h, int32_t precision, uint32_t blocks, int32_t rep
     int ns = 0, nb = 0;
     largest_body(h, &ns, &nb);
     const double pairs = double(ns + nb) * double(ns + nb);
-    for (int block : {64, 128, 256}) {
+    for (int cand = 0; cand < 8 && !rc; ++cand) {
+        const int split = cand / 4, block = 64 << (cand % 4);
+        if (split ? block < 256 : block > 256) continue;  // the split needs >= 4 waves
         if (block == 64 && pairs > 1e8) continue;  // large problems: 64-thread groups never win
         for (int r : {1, 2}) {
-            const int tt = block * r;
-            const TileShape base = default_shape(precision, ns, nb, h->num_cus);
-            // chunk candidates around the one that gives ~8 workgroups per CU for this tile size
+            TileShape probe{block, r, block, 0, split};
+            const int tt = tile_targets(probe);
+            // chunk candidates: the largest body's work list at ~16, 32 and 64 items per CU
             const long n_tiles = (ns + tt - 1) / tt + (nb + tt - 1) / tt;
-            long c0 = n_tiles > 0 ? ((long)(ns + nb) * n_tiles + 8L * h->num_cus - 1) / (8L * h->num_cus) : base.chunk;
-            for (long mult : {1L, 2L, 4L}) {
-                long chunk = std::max<long>(block, ((c0 * mult / 2 + block - 1) / block) * block);
+            long prev = -1;
+            for (long per_cu : {16L, 32L, 64L}) {
+                const long want = per_cu * (h->num_cus > 0 ? h->num_cus : 256);
+                const long c0 = n_tiles > 0 ? ((long)(ns + nb) * n_tiles + want - 1) / want : (long)(ns + nb);
+                const long chunk = std::max<long>(block, ((c0 + block - 1) / block) * block);
+                if (chunk == prev) continue;
+                prev = chunk;
                 for (int pers : {0, 2}) {
-                    h->shape[precision] = TileShape{block, r, (int)chunk, pers};
+                    if (split && pers) continue;  // no persistent form of the split
+                    h->shape[precision] = TileShape{block, r, (int)chunk, pers, split};
                     rc = tfs_rpy_evaluate(h, blocks);  // builds the plan; warm-up
                     if (rc) break;
                     hipEventRecord(e0, h->stream);
@@ -893,6 +934,15 @@ int tfs_rpy_autotune(tfs_rpy* h, int32_t precision, uint32_t blocks, int32_t rep
 int tfs_rpy_set_persistent(tfs_rpy* h, int32_t precision, int32_t workgroups_per_cu) {
     if (!h || (precision != 0 && precision != 1) || workgroups_per_cu < 0 || workgroups_per_cu > 64)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_persistent: bad arguments");
+    if (workgroups_per_cu > 0 && h->shape[precision].split) {
+        // the split has no persistent form: a default shape gives way to the plain default
+        if (h->shape_chosen[precision])
+            return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_persistent: the source split has no persistent form");
+        int ns = 0, nb = 0;
+        largest_body(h, &ns, &nb);
+        h->shape[precision] = plain_shape(ns, nb, h->num_cus);
+        h->shape_chosen[precision] = true;
+    }
     h->shape[precision].persistent = workgroups_per_cu;
     return TFS_OK;
 }
@@ -905,6 +955,13 @@ int tfs_rpy_get_tile_shape(tfs_rpy* h, int32_t precision, int32_t shape[4]) {
     shape[1] = s.per_thread;
     shape[2] = s.chunk;
     shape[3] = s.persistent;
+    return TFS_OK;
+}
+
+int tfs_rpy_get_source_split(tfs_rpy* h, int32_t precision, int32_t* split) {
+    if (!h || !split || (precision != 0 && precision != 1))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_get_source_split: bad arguments");
+    *split = h->shape[precision].split;
     return TFS_OK;
 }
 

@@ -693,6 +693,7 @@ int ens_cycle_graph(tfs_solver* s, int precision) {
                                         (uintptr_t)h->shape[precision].per_thread,
                                         (uintptr_t)h->shape[precision].chunk,
                                         (uintptr_t)h->shape[precision].persistent,
+                                        (uintptr_t)h->shape[precision].split,
                                         (uintptr_t)s->precond_kind,
                                         (uintptr_t)s->npatch};
     if (E.exec[precision] && E.key[precision] == key) return TFS_OK;

@@ -126,10 +126,17 @@ class RpyHandle:
     def set_persistent(self, precision: int, workgroups_per_cu: int):
         L.check(L.lib().tfs_rpy_set_persistent(self.h, precision, workgroups_per_cu))
 
+    def set_split_shape(self, precision: int, block: int, per_thread: int, chunk: int):
+        """Launch shape whose waves share one tile of targets and split its sources."""
+        L.check(L.lib().tfs_rpy_set_split_shape(self.h, precision, block, per_thread, chunk))
+
     def tile_shape(self, precision: int):
         out = (ctypes.c_int32 * 4)()
         L.check(L.lib().tfs_rpy_get_tile_shape(self.h, precision, out))
-        return {"block": out[0], "per_thread": out[1], "chunk": out[2], "persistent": out[3]}
+        split = ctypes.c_int32()
+        L.check(L.lib().tfs_rpy_get_source_split(self.h, precision, ctypes.byref(split)))
+        return {"block": out[0], "per_thread": out[1], "chunk": out[2], "persistent": out[3],
+                "split": split.value}
 
     def body_timing(self, enable: bool = True):
         L.check(L.lib().tfs_rpy_body_timing(self.h, int(enable)))
