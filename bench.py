@@ -17,7 +17,10 @@ the same state beside it.
 
 Also timed: the full 6-DOF grand-mobility matvec (dimension 6*12,800 +
 3*9,000) in FP64 and FP32-tile/FP64-accumulate forms -- the "RPY matvec GB/s"
-part of the metric -- with its relative L2 error against FP64.
+part of the metric -- with its relative L2 error against FP64; 200 timesteps
+of the production body mixed and FP64 against an FP64 truth solved to 1e-12;
+the flow field, the 64-body ensemble (matvec and timestep) and the elastic
+Cosserat steps.
 
 Prints one JSON line.  `--gpus` above 1 is not supported (one GPU).
 """
@@ -213,6 +216,44 @@ def ensemble_leg(torch, stream, steps: int = 5):
                      "per_body_completion_ms_max": float(bt.max()),
                      "per_body_completion_p99_over_p50": float(np.percentile(bt, 99) / np.percentile(bt, 50))}
     h.close()
+    return out
+
+
+def trajectory_leg(torch, body, stream, steps: int = 200, every: int = 20):
+    """The production body over `steps` prescribed-beat timesteps three ways:
+    mixed precision and FP64 at the headline tolerance 1e-8, and an FP64
+    truth at 1e-12.  Reports each one's largest deviation from the truth over
+    states taken every `every` steps (segment positions / DL, solution
+    relative L2), and their step rates."""
+    from timfilsimmod_amd.solver import SOLVE_FP64, SOLVE_MIXED, BeatSimulation
+    runs = {}
+    for name, mode, tol in (("truth", SOLVE_FP64, 1e-12), ("fp64", SOLVE_FP64, 1e-8), ("mixed", SOLVE_MIXED, 1e-8)):
+        print(f"bench: trajectory leg, {name} ({steps} steps, tol {tol})", file=sys.stderr, flush=True)
+        sim = BeatSimulation(body, mode=mode, tol=tol, restart=50, inner_tol=INNER_TOL, max_cycles=60,
+                             max_refinements=20, stream=stream, warm_start=WARM_START, precond=PRECOND)
+        stats, states = [], []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            stats.append(sim.step())
+            if k % every == every - 1:
+                states.append(sim.state())
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        sim.close()
+        runs[name] = (stats, states, wall)
+    out = {"steps": steps, "states_compared": steps // every}
+    truth = runs["truth"][1]
+    for name, (stats, states, wall) in runs.items():
+        entry = {"tol": 1e-12 if name == "truth" else 1e-8, "timesteps_per_s_wall": steps / wall,
+                 "gmres_iterations_mean": float(np.mean([s["iterations"] for s in stats])),
+                 "converged_all": bool(all(s["converged"] for s in stats))}
+        if name != "truth":
+            entry["x_segs_dev_over_DL_max"] = float(max(np.abs(a["x_segs"] - b["x_segs"]).max()
+                                                        for a, b in zip(states, truth)) / DL)
+            entry["solution_rel_err_max"] = float(max(np.linalg.norm(a["solution"] - b["solution"]) /
+                                                      np.linalg.norm(b["solution"]) for a, b in zip(states, truth)))
+        out[name] = entry
     return out
 
 
@@ -497,13 +538,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--matvec-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="timestep,matvec,flow,ensemble,ensemble_step,elastic",
-                    help="comma list of timestep, matvec, flow, ensemble, ensemble_step, elastic, copy "
+    ap.add_argument("--legs", default="timestep,trajectory,matvec,flow,ensemble,ensemble_step,elastic",
+                    help="comma list of timestep, trajectory, matvec, flow, ensemble, ensemble_step, elastic, copy "
                          "(copy: a 256 MiB device copy for calibrating HBM byte counters; not part of the "
                          "default run)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
-    unknown = legs - {"timestep", "matvec", "flow", "ensemble", "ensemble_step", "copy", "elastic"}
+    unknown = legs - {"timestep", "trajectory", "matvec", "flow", "ensemble", "ensemble_step", "copy", "elastic"}
     if unknown:
         print(f"bench.py: unknown legs {sorted(unknown)}", file=sys.stderr)
         sys.exit(2)
@@ -612,6 +653,10 @@ def main():
                   "algorithmic_gflop": flops6 / 1e9, "compulsory_bytes": bytes6}
 
     flow = ens = None
+    traj = None
+    if "trajectory" in legs:
+        traj = trajectory_leg(torch, body, stream)
+        torch.cuda.empty_cache()
     if "flow" in legs:
         print("bench: flow-field leg", file=sys.stderr, flush=True)
         flow = flow_leg(torch, stream)
@@ -668,6 +713,7 @@ def main():
                      "trajectory_dev_over_DL_after_steps": dev,
                      "solution_rel_err_mixed_vs_fp64": sol_err,
                      "warmup_plus_timed_steps": args.warmup + args.steps},
+        "trajectory_200": traj,
         "matvec": matvec,
         "flow_field": flow,
         "ensemble": ens,
