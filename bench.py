@@ -316,12 +316,13 @@ def ensemble_timestep_leg(torch, stream, steps: int = 2, warmup: int = 2):
     return out
 
 
-def elastic_leg(torch, steps: int = 3):
+def elastic_leg(torch, stream, steps: int = 3):
     """The small elastic configuration on the device: one rigid sphere of
     radius 20a with 2,000 blobs and 16 filaments x 20 segments of the
     reference's Cosserat model (clamped, follower end force of dimensionless
     strength 220, a first-step kick), each timestep through tfs_elastic_step:
-    blob forces by GMRES(30) to 1e-10, Broyden to 1e-8."""
+    blob forces from the dense inverse of the blob mobility, Broyden to 1e-8.
+    Every handle of a step runs on the bench's stream."""
     from scipy.spatial.transform import Rotation
 
     from timfilsimmod_amd import _lib as L
@@ -338,11 +339,12 @@ def elastic_leg(torch, steps: int = 3):
     qs = np.repeat(q0[:, None, :], nseg, axis=1)
     pert = np.random.default_rng(0).standard_normal((nfil, 6)) * 0.1 * efm
     dev = lambda a: torch.tensor(np.asarray(a, dtype=np.float64), device="cuda")
-    model = FM.FilamentModel(1, nfil, nseg, dl=dl, kb=kb, kt=kb, dt=dt, end_force=efm, memory=L.MEMORY_DEVICE)
+    model = FM.FilamentModel(1, nfil, nseg, dl=dl, kb=kb, kt=kb, dt=dt, end_force=efm, memory=L.MEMORY_DEVICE,
+                             stream=stream)
     ev = ElasticVelocities(1, nfil * nseg, nb, mode=SOLVE_FP64, restart=30, max_cycles=200, tol=1e-10,
-                           memory=L.MEMORY_DEVICE)
+                           memory=L.MEMORY_DEVICE, stream=stream)
     ev.set_preconditioner(BLOB_DIRECT)
-    eng = BroydenSolver(model.block_sizes(), memory=L.MEMORY_DEVICE)
+    eng = BroydenSolver(model.block_sizes(), memory=L.MEMORY_DEVICE, stream=stream)
     keep = [dev(body.x_blobs), dev(np.zeros(3)), dev(np.zeros(6)), dev(np.zeros(3 * nb))]
     model.set_state(dev(body.fil_base.reshape(-1)), dev(qs.reshape(-1)))
     model.set_perturbation(4, 12, dev(pert.reshape(-1)))
@@ -379,7 +381,8 @@ def elastic_leg(torch, steps: int = 3):
     # solve with body rows from the stored body-frame inverse, J0 with the
     # body's Schur complement
     fil_refs = dev((body.fil_base - body.centre).reshape(-1))
-    model = FM.FilamentModel(1, nfil, nseg, dl=dl, kb=kb, kt=kb, dt=dt, end_force=efm, memory=L.MEMORY_DEVICE)
+    model = FM.FilamentModel(1, nfil, nseg, dl=dl, kb=kb, kt=kb, dt=dt, end_force=efm, memory=L.MEMORY_DEVICE,
+                             stream=stream)
     model.set_state(dev(body.fil_base.reshape(-1)), dev(qs.reshape(-1)))
     model.set_body(dev(body.centre), dev([1.0, 0.0, 0.0, 0.0]), fil_refs, nb, dev(body.blob_refs.reshape(-1)))
     model.set_perturbation(4, 12, dev(pert.reshape(-1)))
@@ -388,9 +391,9 @@ def elastic_leg(torch, steps: int = 3):
     nmob = dev(evh.body_mobility(body.x_blobs, body.centre)[0].reshape(-1))
     evh.close()
     ev = ElasticVelocities(1, nfil * nseg, nb, mode=SOLVE_FP64, restart=30, max_cycles=200, tol=1e-10,
-                           memory=L.MEMORY_DEVICE, free_bodies=True)
+                           memory=L.MEMORY_DEVICE, free_bodies=True, stream=stream)
     ev.set_preconditioner(BLOB_DIRECT)
-    eng = BroydenSolver(model.block_sizes(), memory=L.MEMORY_DEVICE)
+    eng = BroydenSolver(model.block_sizes(), memory=L.MEMORY_DEVICE, stream=stream)
     fb = dev(np.zeros(3 * nb))
     step = FM.ElasticStep(model, ev, eng, None, None, None, fb, tol=1e-8, max_iter=400, body_mobility=nmob,
                           sphere_radius=float(body.radius))
@@ -403,10 +406,11 @@ def elastic_leg(torch, steps: int = 3):
                                     "body-frame dense inverse, J0 with the body's Schur complement)", "steps": rows,
                         "ms_per_step_mean": float(np.mean([r["ms"] for r in rows]))}
     # the body held still with the blob forces as Broyden unknowns
-    model = FM.FilamentModel(1, nfil, nseg, dl=dl, kb=kb, kt=kb, dt=dt, end_force=efm, memory=L.MEMORY_DEVICE)
+    model = FM.FilamentModel(1, nfil, nseg, dl=dl, kb=kb, kt=kb, dt=dt, end_force=efm, memory=L.MEMORY_DEVICE,
+                             stream=stream)
     model.set_state(dev(body.fil_base.reshape(-1)), dev(qs.reshape(-1)))
     model.set_perturbation(4, 12, dev(pert.reshape(-1)))
-    eng = BroydenSolver(model.block_sizes(blobs_per_swim=nb), memory=L.MEMORY_DEVICE)
+    eng = BroydenSolver(model.block_sizes(blobs_per_swim=nb), memory=L.MEMORY_DEVICE, stream=stream)
     keep = [dev(body.x_blobs), dev(np.zeros(3)), dev(np.zeros(6)), dev(np.zeros(3 * nb))]
     step = FM.ElasticStep(model, None, eng, *keep, tol=1e-8, max_iter=400, broyden_blobs=nb, blob_radius=1.0)
     rows = timed(step)
@@ -671,7 +675,7 @@ def main():
     elastic = None
     if "elastic" in legs:
         print("bench: elastic-step leg", file=sys.stderr, flush=True)
-        elastic = elastic_leg(torch)
+        elastic = elastic_leg(torch, stream)
         if not args.no_cpu_baseline:
             print("bench: elastic-step CPU baseline", file=sys.stderr, flush=True)
             elastic["cpu_baseline"] = elastic_cpu_baseline()

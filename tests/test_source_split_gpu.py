@@ -11,7 +11,9 @@ full_mobility) on the same seeded inputs:
 * independent bodies (block-diagonal mobility) in one launch;
 * repeated evaluations are bitwise identical (a fixed summation order);
 * the argument rules: no persistent form, block 256 or 512 with the split,
-  64/128/256 without."""
+  64/128/256 without;
+* the default shapes: the split for a body that fills the device (12,800
+  segments + 9,000 blobs), one wave per 64 x 64 item for one that does not."""
 import numpy as np
 import pytest
 
@@ -61,7 +63,6 @@ def test_split_shapes_match_oracle(seg_dof, precision):
     ref = np.concatenate([rs, rb])
     h = handle(ns, nb, xs, xb, fs, fb, seg_dof, precision)
     try:
-        assert h.tile_shape(precision)["split"] == 1
         h.evaluate()
         assert rel(read(h, ns, nb), ref) <= TOL[precision]
         h.set_tile_shape(precision, 256, 2, 256)
@@ -105,9 +106,7 @@ def test_split_independent_bodies_match_per_body_oracle(precision):
         h.upload(L.BLOB_POSITIONS, e.x_blobs)
         h.upload(L.SEG_FORCES, e.f_segs)
         h.upload(L.BLOB_FORCES, e.f_blobs)
-        shape = h.tile_shape(precision)
-        print("  default split shape", shape, flush=True)
-        assert shape["split"] == 1 and shape["block"] in (256, 512)
+        h.set_split_shape(precision, 256, 2, 256)
         h.evaluate()
         got = read(h, e.num_segs, e.num_blobs)
         assert rel(got, ref) <= TOL[precision]
@@ -121,24 +120,49 @@ def test_split_argument_rules():
     h = handle(ns, nb, xs, xb, fs, fb, 6, L.PRECISION_FP64)
     p = L.PRECISION_FP64
     try:
-        assert h.tile_shape(p)["split"] == 1
         for bad in ((128, 1, 128), (64, 1, 64), (512, 3, 512), (512, 1, 256), (1024, 1, 1024)):
             with pytest.raises(RuntimeError):
                 h.set_split_shape(p, *bad)
         with pytest.raises(RuntimeError):
             h.set_tile_shape(p, 512, 1, 512)           # 512 threads only with the split
-        h.set_persistent(p, 2)                          # the default gives way to the plain form
-        assert h.tile_shape(p)["split"] == 0 and h.tile_shape(p)["persistent"] == 2
+        h.set_split_shape(p, 256, 1, 256)
         with pytest.raises(RuntimeError):
-            h.set_split_shape(p, 256, 1, 256)           # no persistent form of the split
+            h.set_persistent(p, 2)                      # a chosen split shape has no persistent form
+        h.set_tile_shape(p, 128, 1, 128)
+        h.set_persistent(p, 2)
+        with pytest.raises(RuntimeError):
+            h.set_split_shape(p, 256, 1, 256)
         h.set_persistent(p, 0)
         h.set_split_shape(p, 512, 2, 512)
         assert h.tile_shape(p) == {"block": 512, "per_thread": 2, "chunk": 512, "persistent": 0, "split": 1}
-        with pytest.raises(RuntimeError):
-            h.set_persistent(p, 2)                      # a chosen split shape stays
         h.evaluate()
         got = read(h, ns, nb)
         rs, rb = O.full_mobility(xs, xb, fs, fb, seg_dof=6, bs_accumulate=True)
         assert rel(got, np.concatenate([rs, rb])) <= TOL[p]
     finally:
         h.close()
+    # a default split shape gives way to the plain default for a persistent launch
+    big = RpyHandle(12800, 9000, seg_dof=3, precision=p, bs_accumulate=True)
+    try:
+        assert big.tile_shape(p)["split"] == 1
+        big.set_persistent(p, 2)
+        assert big.tile_shape(p) == {"block": 256, "per_thread": 2, "chunk": 256, "persistent": 2, "split": 0}
+    finally:
+        big.close()
+
+
+@pytest.mark.parametrize("seg_dof", [3, 6])
+def test_default_shapes(seg_dof):
+    big = RpyHandle(12800, 9000, seg_dof=seg_dof, precision=L.PRECISION_FP32_ACC64, bs_accumulate=True)
+    small = RpyHandle(320, 2000, seg_dof=seg_dof, precision=L.PRECISION_FP64, bs_accumulate=True)
+    try:
+        f32, f64 = big.tile_shape(L.PRECISION_FP32_ACC64), big.tile_shape(L.PRECISION_FP64)
+        print("12,800 + 9,000:", f32, f64, flush=True)
+        assert f32 == {"block": 256, "per_thread": 1, "chunk": 768, "persistent": 0, "split": 1}
+        assert f64 == {"block": 256, "per_thread": 2, "chunk": 512, "persistent": 0, "split": 1}
+        s64 = small.tile_shape(L.PRECISION_FP64)
+        print("320 + 2,000:", s64, flush=True)
+        assert s64 == {"block": 64, "per_thread": 1, "chunk": 64, "persistent": 0, "split": 0}
+    finally:
+        big.close()
+        small.close()

@@ -1090,7 +1090,7 @@ static hipError_t sort_into_grid(const MatvecArgs& a, NearGrid& grid, int n, dou
 hipError_t build_near_list(int precision, const MatvecArgs& a, NearGrid& grid, const int32_t* body_ranges, int nbody,
                            NearList& near, int n_targets, int* max_count, int* host_scratch, hipStream_t stream) {
     const int n = n_targets;
-    *max_count = 0;
+    if (max_count) *max_count = 0;
     if (n <= 0) return hipSuccess;
     const double h = 2.0 * std::max(a.a_seg, a.a_blob) * (1.0 + 1e-5);
     const double inv_h = 1.0 / h;
@@ -1117,6 +1117,7 @@ hipError_t build_near_list(int precision, const MatvecArgs& a, NearGrid& grid, c
                            (int64_t)n, near.max_cnt);
     TFS_HIP_CHECK(hipGetLastError());
     TFS_HIP_CHECK(hipMemcpyAsync(host_scratch, near.max_cnt, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    if (!max_count) return hipSuccess;  // the caller reads host_scratch after its own wait
     TFS_HIP_CHECK(hipStreamSynchronize(stream));
     *max_count = *host_scratch;
     return hipSuccess;

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/tfs_rpy.h"
+#include "../../include/tfs_solver.h"
 #include "tfs_internal.h"
 
 namespace tfs {
@@ -59,6 +60,11 @@ struct tfs_rpy {
     std::vector<tfs::BodyRange> bodies;
     tfs::NearList near[2];
     bool near_valid[2] = {false, false};
+    // Rebuilt near lists whose capacity check is left to the caller: the
+    // largest count is copied to host_int[8 + precision] without a wait, and
+    // tfs::rpy_near_check reads it after the caller's own synchronisation.
+    bool defer_near_check = false;
+    bool near_pending[2] = {false, false};
     tfs::NearGrid grid;
     int domain = 0;  // TFS_DOMAIN_UNBOUNDED or TFS_DOMAIN_HALF_SPACE
     // Fixed-point positions for FP32 tiles (valid with the near lists):
@@ -105,10 +111,24 @@ namespace tfs {
 
 // The blob-motion mode of a velocities handle (tfs_velocities_set_blob_motion).
 int32_t velocities_blob_motion(const tfs_velocities* v);
+// tfs_velocities_compute(_free) whose device-memory results are ordered on
+// the handle's stream instead of complete on return.
+int velocities_compute_ordered(tfs_velocities* v, const double* x_segs, const double* f_segs, const double* x_blobs,
+                               const double* x_bodies, const double* bodies, const double* q_bodies, double* v_segs,
+                               double* f_blobs, double* v_bodies_io, tfs_solve_stats* stats, bool free_bodies);
+hipStream_t velocities_stream(const tfs_velocities* v);
+tfs_rpy* velocities_rpy(const tfs_velocities* v);
+// Deferred near-list checks (tfs_rpy::defer_near_check): after `waited` has
+// been synchronised, reads the largest neighbour count of every list rebuilt
+// since the last check; a list that overflowed its capacity is dropped with
+// the capacity grown, and *overflow is set: the evaluations since the rebuild
+// must be repeated.
+int rpy_near_check(tfs_rpy* h, hipStream_t waited, bool* overflow);
 // Norm of the latest residual given to a Broyden engine: the new_error of its
 // last end_of_iter (slot 0 after an update, slot 1 otherwise), else the
 // error it started from.  Synchronises.
 int broyden_latest_norm(tfs_broyden* b, double* out);
+hipStream_t broyden_stream(const tfs_broyden* b);
 
 extern thread_local std::string g_last_error;
 int fail(int code, const std::string& msg);

@@ -189,9 +189,125 @@ __global__ __launch_bounds__(256) void coupling_apply_kernel(const double* __res
     }
 }
 
-// s[1] = 1 / s[0]
-__global__ void reciprocal_kernel(double* s) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) s[1] = s[0] > 0.0 ? 1.0 / s[0] : 0.0;
+// As block_gemv_kernel for blocks of 64 rows or more: 512 threads per block,
+// four column quarters of 128 rows each, the quarters added in order through
+// LDS.
+__global__ __launch_bounds__(512) void block_gemv_wide_kernel(const double* __restrict__ A,
+                                                              const int64_t* __restrict__ mat_off,
+                                                              const int32_t* __restrict__ row_off,
+                                                              const int32_t* __restrict__ sizes,
+                                                              const double* __restrict__ x, double* __restrict__ y,
+                                                              double alpha) {
+    __shared__ double part[4][128];
+    const int b = blockIdx.x;
+    const int bs = sizes[b];
+    const double* Ab = A + mat_off[b];
+    const int r0 = row_off[b];
+    const int rl = threadIdx.x & 127, q = threadIdx.x >> 7;
+    const int cq = (bs + 3) / 4;
+    const int c0 = q * cq, c1 = c0 + cq < bs ? c0 + cq : bs;
+    for (int rb = 0; rb < bs; rb += 128) {
+        const int r = rb + rl;
+        double s0 = 0.0, s1 = 0.0;
+        if (r < bs) {
+            int c = c0;
+            for (; c + 2 <= c1; c += 2) {
+                s0 += Ab[(int64_t)c * bs + r] * x[r0 + c];
+                s1 += Ab[(int64_t)(c + 1) * bs + r] * x[r0 + c + 1];
+            }
+            if (c < c1) s0 += Ab[(int64_t)c * bs + r] * x[r0 + c];
+        }
+        part[q][rl] = s0 + s1;
+        __syncthreads();
+        if (q == 0 && r < bs) y[r0 + r] = alpha * ((part[0][rl] + part[1][rl]) + (part[2][rl] + part[3][rl]));
+        __syncthreads();
+    }
+}
+
+// h[c] = D(:, c) . e for c < k: one wave per column over all rows (four
+// accumulation chains, then a fixed shuffle tree).
+__global__ __launch_bounds__(256) void lowrank_dots_kernel(const double* __restrict__ D, int64_t ld, int k,
+                                                           const double* __restrict__ e, int n,
+                                                           double* __restrict__ h) {
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= k) return;
+    const double* d = D + (int64_t)c * ld;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int r = lane;
+    for (; r + 192 < n; r += 256) {
+        s0 += d[r] * e[r];
+        s1 += d[r + 64] * e[r + 64];
+        s2 += d[r + 128] * e[r + 128];
+        s3 += d[r + 192] * e[r + 192];
+    }
+    for (; r < n; r += 64) s0 += d[r] * e[r];
+    double s = (s0 + s1) + (s2 + s3);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) h[c] = s;
+}
+
+// out[r] -= sum_{c < k} C(r, c) h[c] (four accumulation chains).
+__global__ __launch_bounds__(256) void lowrank_gemv_kernel(const double* __restrict__ C, int64_t ld, int k,
+                                                           const double* __restrict__ h, double* __restrict__ out,
+                                                           int n) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= n) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int c = 0;
+    for (; c + 4 <= k; c += 4) {
+        s0 += C[(int64_t)c * ld + r] * h[c];
+        s1 += C[(int64_t)(c + 1) * ld + r] * h[c + 1];
+        s2 += C[(int64_t)(c + 2) * ld + r] * h[c + 2];
+        s3 += C[(int64_t)(c + 3) * ld + r] * h[c + 3];
+    }
+    for (; c < k; ++c) s0 += C[(int64_t)c * ld + r] * h[c];
+    out[r] -= (s0 + s1) + (s2 + s3);
+}
+
+// D_k = s1 - s0 and partial[b] = sum over workgroup b's rows of D_k^2.
+__global__ __launch_bounds__(256) void diff_sumsq_kernel(const double* __restrict__ s1, const double* __restrict__ s0,
+                                                         double* __restrict__ Dk, int n,
+                                                         double* __restrict__ partial) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int r = blockIdx.x * 256 + threadIdx.x; r < n; r += 256 * gridDim.x) {
+        const double d = s1[r] - s0[r];
+        Dk[r] = d;
+        s += d * d;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// C_k and D_k divided by ||D_k|| = sqrt(sum_b partial[b]) (every workgroup
+// sums the G partials itself, in the same order); scal[0] = ||D_k||.
+__global__ __launch_bounds__(256) void scale_pair_kernel(double* __restrict__ Ck, double* __restrict__ Dk, int n,
+                                                         const double* __restrict__ partial, int G,
+                                                         double* __restrict__ scal) {
+    __shared__ double sh[4];
+    __shared__ double inv;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < G; b += 256) s += partial[b];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double nrm = __builtin_sqrt((sh[0] + sh[1]) + (sh[2] + sh[3]));
+        inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+        if (blockIdx.x == 0) scal[0] = nrm;
+    }
+    __syncthreads();
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r < n) {
+        Ck[r] *= inv;
+        Dk[r] *= inv;
+    }
 }
 
 // out = dexpinv(u, v) = v - u x v / 2 - fac u x (u x v),
@@ -476,13 +592,20 @@ namespace {
 
 bool host_mem(const tfs_broyden* b) { return b->desc.memory == TFS_MEMORY_HOST; }
 
+// Up to this many unknowns the low-rank sums take one wave per stored column.
+constexpr int kWaveDotsMax = 65536;
+
 hipMemcpyKind to_device(const tfs_broyden* b) {
     return host_mem(b) ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
 }
 
 int apply_jinv(tfs_broyden* b, const double* x, double* y, double alpha) {
-    hipLaunchKernelGGL(block_gemv_kernel, dim3(b->nblocks), dim3(128), 0, b->stream, b->J, b->mat_off, b->row_off,
-                       b->sizes, x, y, alpha);
+    if (b->max_block >= 64)
+        hipLaunchKernelGGL(block_gemv_wide_kernel, dim3(b->nblocks), dim3(512), 0, b->stream, b->J, b->mat_off,
+                           b->row_off, b->sizes, x, y, alpha);
+    else
+        hipLaunchKernelGGL(block_gemv_kernel, dim3(b->nblocks), dim3(128), 0, b->stream, b->J, b->mat_off,
+                           b->row_off, b->sizes, x, y, alpha);
     int rc = check(hipGetLastError(), "J0^{-1}");
     if (!rc && b->coupled) {
         hipLaunchKernelGGL(coupling_apply_kernel, dim3(b->cpl_swim), dim3(256), 0, b->stream, (const double*)b->W,
@@ -498,6 +621,13 @@ int inverse_update(tfs_broyden* b, const double* e, double* out) {
     int rc = apply_jinv(b, e, out, -b->desc.confidence);
     const int k = std::min(b->iter, b->max_iter - 1);
     if (rc || k == 0) return rc;
+    if (b->n <= kWaveDotsMax) {
+        hipLaunchKernelGGL(lowrank_dots_kernel, dim3((unsigned)((k + 3) / 4)), dim3(256), 0, b->stream,
+                           (const double*)b->D, (int64_t)b->ld, k, e, b->n, b->h);
+        hipLaunchKernelGGL(lowrank_gemv_kernel, dim3((unsigned)((b->n + 255) / 256)), dim3(256), 0, b->stream,
+                           (const double*)b->C, (int64_t)b->ld, k, (const double*)b->h, out, b->n);
+        return check(hipGetLastError(), "low-rank update");
+    }
     rc = check(k_dot_cols(b->D, b->ld, k, e, b->n, b->partial, b->G, b->stream), "D^T e");
     if (!rc) rc = check(k_reduce_cols(b->partial, b->G, k, b->h, nullptr, b->stream), "reduce");
     if (!rc) rc = check(k_gemv(b->C, b->ld, k, b->h, out, b->n, true, b->stream), "C h");
@@ -819,14 +949,14 @@ int tfs_broyden_end_of_iter(tfs_broyden* b, const double* new_error, int32_t err
         double* Ck = b->C + (int64_t)b->iter * b->ld;
         double* Dk = b->D + (int64_t)b->iter * b->ld;
         if (!rc) rc = inverse_update(b, b->slot[1], Ck);
-        if (!rc) rc = check(k_sub(b->slot[1], b->slot[0], Dk, n, b->stream), "D_k");
-        if (!rc) rc = check(k_norm(Dk, n, b->partial, b->G, b->scal, b->stream), "||D_k||");
         if (!rc) {
-            hipLaunchKernelGGL(reciprocal_kernel, dim3(1), dim3(64), 0, b->stream, b->scal);
-            rc = check(hipGetLastError(), "1/||D_k||");
+            // D_k = new - old, then C_k and D_k over ||D_k||
+            hipLaunchKernelGGL(diff_sumsq_kernel, dim3((unsigned)b->G), dim3(256), 0, b->stream,
+                               (const double*)b->slot[1], (const double*)b->slot[0], Dk, n, b->partial);
+            hipLaunchKernelGGL(scale_pair_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream, Ck,
+                               Dk, n, (const double*)b->partial, b->G, b->scal);
+            rc = check(hipGetLastError(), "D_k, C_k / ||D_k||");
         }
-        if (!rc) rc = check(k_scale(Ck, Ck, b->scal, 1, n, b->stream), "C_k /= ||D_k||");
-        if (!rc) rc = check(k_scale(Dk, Dk, b->scal, 1, n, b->stream), "D_k /= ||D_k||");
         if (rc) return rc;
         std::swap(b->slot[0], b->slot[1]);   // error <- new_error
         b->latest = 0;
@@ -954,6 +1084,8 @@ int tfs_broyden_compute_errors(tfs_broyden* b, const tfs_broyden_state* st, int3
 }
 
 }  // extern "C"
+
+hipStream_t tfs::broyden_stream(const tfs_broyden* b) { return b->stream; }
 
 int tfs::broyden_latest_norm(tfs_broyden* b, double* out) {
     int rc = check(k_norm(b->slot[b->latest], b->n, b->partial, b->G, b->scal, b->stream), "||residual||");

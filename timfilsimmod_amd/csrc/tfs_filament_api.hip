@@ -1677,28 +1677,65 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
         if (rc0) return rc0;
         motion.v = D.velocities;
     }
+    // Work queued on stream `from` is ordered before later work on `to`: the
+    // handles may run on streams of their own (then the host waits), and one
+    // shared stream needs no waits.
+    const hipStream_t s_fil = fh->stream;
+    const hipStream_t s_vel = e->rpy ? e->rpy->stream : tfs::velocities_stream(D.velocities);
+    const hipStream_t s_bro = tfs::broyden_stream(D.broyden);
+    auto order = [](hipStream_t from, hipStream_t to) -> int {
+        return from == to ? TFS_OK : check(hipStreamSynchronize(from), "tfs_elastic_step: stream order");
+    };
     // forces -> velocities -> residual in `slot`
     auto evaluate = [&](int slot, int32_t* too_large) -> int {
         int rc = e->free_bodies ? tfs_filament_blob_positions(fh, e->x_blobs) : TFS_OK;
         if (!rc) rc = tfs_filament_forces(fh, nt, e->f_segs);
-        if (!rc) rc = tfs_filament_sync(fh);
+        if (!rc) rc = order(s_fil, s_vel);
         tfs_solve_stats vs{};
         if (!rc && e->rpy) {
             // USE_BROYDEN_FOR_EVERYTHING: one plain matvec with the blob forces given
             rc = tfs_rpy_upload(e->rpy, TFS_SEG_POSITIONS, fh->S.x);
             if (!rc) rc = tfs_rpy_upload(e->rpy, TFS_BLOB_POSITIONS, e->free_bodies ? e->x_blobs : D.x_blobs);
             if (!rc) rc = tfs_rpy_apply(e->rpy, TFS_BLOCK_ALL, e->f_segs, D.f_blobs, e->v_segs, e->v_blobs);
-            if (!rc) rc = tfs_filament_sync(fh);
         } else if (!rc && e->free_bodies)
-            rc = tfs_velocities_compute_free(D.velocities, fh->S.x, e->f_segs, e->x_blobs, fh->S.bx, nullptr,
-                                             fh->S.bq, e->v_segs, D.f_blobs, e->v_bodies, &vs);
+            rc = tfs::velocities_compute_ordered(D.velocities, fh->S.x, e->f_segs, e->x_blobs, fh->S.bx, nullptr,
+                                                 fh->S.bq, e->v_segs, D.f_blobs, e->v_bodies, &vs, true);
         else if (!rc)
-            rc = tfs_velocities_compute(D.velocities, fh->S.x, e->f_segs, D.x_blobs, D.x_bodies, D.v_bodies,
-                                        e->v_segs, D.f_blobs, &vs);
+            rc = tfs::velocities_compute_ordered(D.velocities, fh->S.x, e->f_segs, D.x_blobs, D.x_bodies,
+                                                 D.v_bodies, nullptr, e->v_segs, D.f_blobs, nullptr, &vs, false);
         S.gmres_iterations += vs.iterations;
         S.velocity_solves++;
+        if (!rc) rc = order(s_vel, s_bro);
         if (!rc) rc = tfs_broyden_compute_errors(D.broyden, &st, slot, too_large);
         return rc;
+    };
+    // The near-pair lists rebuilt in an evaluation have their capacity checked
+    // at the residual's own wait (one host round trip per Broyden iteration);
+    // an overflow grows the lists and repeats the evaluation, a pure function
+    // of the state.
+    tfs_rpy* rpyh = e->rpy ? e->rpy : tfs::velocities_rpy(D.velocities);
+    struct DeferScope {
+        tfs_rpy* h = nullptr;
+        bool saved = false;
+        ~DeferScope() {
+            if (!h) return;
+            bool overflow = false;
+            (void)hipStreamSynchronize(h->stream);
+            (void)tfs::rpy_near_check(h, h->stream, &overflow);
+            h->defer_near_check = saved;
+        }
+    } defer;
+    defer.h = rpyh;
+    defer.saved = rpyh->defer_near_check;
+    rpyh->defer_near_check = true;
+    auto evaluate_checked = [&](int slot, int32_t* too_large) -> int {
+        for (int attempt = 0; attempt < 8; ++attempt) {
+            int rc = evaluate(slot, too_large);
+            bool overflow = false;
+            if (!rc) rc = tfs::rpy_near_check(rpyh, s_bro, &overflow);
+            if (rc || !overflow) return rc;
+        }
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_step: too many overlapping neighbours");
     };
     int rc = tfs_filament_initial_guess(fh, nt);
     if (!rc && e->rpy && nb3) {
@@ -1707,7 +1744,7 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
         rc = check(hipGetLastError(), "blob force guess");
     }
     int32_t big = 0;
-    if (!rc) rc = evaluate(0, &big);
+    if (!rc) rc = evaluate_checked(0, &big);
     if (!rc && D.velocities && !e->free_bodies) rc = tfs_velocities_set_blob_motion(D.velocities, TFS_BLOBS_FIXED);
     if (!rc && big) {
         rc = tfs_filament_jacobian(fh, nt, e->f_segs, e->blocks);
@@ -1749,6 +1786,7 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
             // the filament and body entries to the filament layout, the blob
             // entries onto the blob forces
             rc = tfs_broyden_find_update(D.broyden, e->bfe_du);
+            if (!rc) rc = order(s_bro, s_fil);
             for (int64_t s = 0; s < nswim && !rc; ++s)
                 rc = check(hipMemcpyAsync(e->du + s * (fil * n6 + 6), e->bfe_du + s * e->per, 8 * (fil * n6 + 6),
                                           hipMemcpyDeviceToDevice, fh->stream),
@@ -1762,8 +1800,9 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
         } else {
             rc = tfs_broyden_find_update(D.broyden, e->du);
         }
+        if (!rc && !e->rpy) rc = order(s_bro, s_fil);
         if (!rc) rc = tfs_filament_update(fh, e->du);
-        if (!rc) rc = evaluate(1, &big);
+        if (!rc) rc = evaluate_checked(1, &big);
         if (!rc) rc = tfs_broyden_end_of_iter(D.broyden, nullptr, big);
         if (!rc) rc = tfs_broyden_status(D.broyden, &iter, nullptr);
     }

@@ -59,6 +59,10 @@ TileShape default_shape(int precision, int ns, int nb, int num_cus, int seg_dof,
     // launch have work to spare: there the split with 128-target tiles and
     // ~512-source items is within 2% of the best plain shape in both
     // precisions, and 64-target tiles lose 5-10%.
+    // A body too small to fill the device at that shape (under ~2 items per
+    // CU, e.g. 320 segments + 2,000 blobs: 190 items) takes the finest shape,
+    // one wave per item of 64 targets x 64 sources: its launches are bound by
+    // the longest item's latency, not by throughput.
     (void)seg_dof;
     TileShape s;
     s.block = 256;
@@ -67,6 +71,10 @@ TileShape default_shape(int precision, int ns, int nb, int num_cus, int seg_dof,
     s.per_thread = num_bodies == 1 && precision == 1 ? 1 : 2;
     const long per_cu = num_bodies > 1 ? 64 : s.per_thread == 1 ? 48 : 32;
     size_chunk(s, ns, nb, num_cus, per_cu);
+    const int tt = tile_targets(s);
+    const long items = ((long)(ns + tt - 1) / tt + (nb + tt - 1) / tt) *
+                       (((long)ns + s.chunk - 1) / s.chunk + ((long)nb + s.chunk - 1) / s.chunk);
+    if (items < 2L * (num_cus > 0 ? num_cus : 256)) s = TileShape{64, 1, 64, 0, 0};
     return s;
 }
 
@@ -359,6 +367,14 @@ int ensure_near(tfs_rpy* h, int precision) {
                 return fail(TFS_ERR_OUT_OF_MEMORY, "near-pair lists");
             }
         }
+        if (h->defer_near_check && nl.cnt) {
+            hipError_t e = build_near_list(precision, a, h->grid, h->d_body_range, (int)h->bodies.size(), nl, nt,
+                                           nullptr, h->host_int + 8 + precision, h->stream);
+            if (e != hipSuccess) return hip_fail(e, "near-pair lists");
+            h->near_valid[precision] = true;
+            h->near_pending[precision] = true;
+            return TFS_OK;
+        }
         int max_count = 0;
         hipError_t e = build_near_list(precision, a, h->grid, h->d_body_range, (int)h->bodies.size(), nl, nt,
                                        &max_count, h->host_int, h->stream);
@@ -374,6 +390,27 @@ int ensure_near(tfs_rpy* h, int precision) {
         nl.cap = cap;
     }
     return fail(TFS_ERR_INVALID_ARGUMENT, "near-pair lists: too many overlapping neighbours");
+}
+
+int rpy_near_check(tfs_rpy* h, hipStream_t waited, bool* overflow) {
+    *overflow = false;
+    for (int p = 0; p < 2; ++p) {
+        if (!h->near_pending[p]) continue;
+        if (h->stream != waited && hipStreamSynchronize(h->stream) != hipSuccess)
+            return fail(TFS_ERR_HIP, "near-pair lists: synchronise");
+        h->near_pending[p] = false;
+        NearList& nl = h->near[p];
+        const int m = h->host_int[8 + p];
+        if (m <= nl.cap) continue;
+        int cap = nl.cap;
+        while (cap < m) cap *= 2;
+        hipStreamSynchronize(h->stream);
+        free_near(nl);
+        nl.cap = cap;
+        h->near_valid[p] = false;
+        *overflow = true;
+    }
+    return TFS_OK;
 }
 
 int ensure_stamps(tfs_rpy* h, size_t n_items) {

@@ -221,6 +221,36 @@ __global__ __launch_bounds__(256) void free_direct_forces_kernel(const double* _
     for (int c = 0; c < 3; ++c) out[3 * m + c] = R[3 * c] * b[0] + R[3 * c + 1] * b[1] + R[3 * c + 2] * b[2];
 }
 
+// Up to eight device-to-device copies of doubles in one launch.
+struct CopyList {
+    const double* src[8];
+    double* dst[8];
+    int64_t n[8];
+    int count = 0;
+    int64_t longest = 0;
+    void add(const double* s, double* d, int64_t len) {
+        if (s == d || len <= 0) return;
+        src[count] = s;
+        dst[count] = d;
+        n[count] = len;
+        ++count;
+        longest = len > longest ? len : longest;
+    }
+};
+
+__global__ __launch_bounds__(256) void copy_list_kernel(CopyList L) {
+    for (int k = 0; k < L.count; ++k)
+        for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < L.n[k]; i += (int64_t)gridDim.x * 256)
+            L.dst[k][i] = L.src[k][i];
+}
+
+hipError_t copy_list(const CopyList& L, hipStream_t st) {
+    if (L.count == 0) return hipSuccess;
+    const int64_t g = (L.longest + 255) / 256;
+    hipLaunchKernelGGL(copy_list_kernel, dim3((unsigned)(g < 1024 ? g : 1024)), dim3(256), 0, st, L);
+    return hipGetLastError();
+}
+
 // flag = 1 if a[i] != b[i] for some i (every writer stores the same value).
 __global__ __launch_bounds__(256) void differs_kernel(const double* __restrict__ a, const double* __restrict__ b,
                                                      int64_t n, int32_t* __restrict__ flag) {
@@ -492,7 +522,7 @@ namespace {
 // result).
 int compute(tfs_velocities* v, const double* x_segs, const double* f_segs, const double* x_blobs,
             const double* x_bodies, const double* bodies, const double* q_bodies, double* v_segs, double* f_blobs,
-            double* v_bodies_io, tfs_solve_stats* stats, bool free_bodies) {
+            double* v_bodies_io, tfs_solve_stats* stats, bool free_bodies, bool sync_end = true) {
     const hipMemcpyKind kin = v->desc.memory == TFS_MEMORY_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
     const hipMemcpyKind kout = v->desc.memory == TFS_MEMORY_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
     tfs_rpy* h = v->rpy;
@@ -523,40 +553,59 @@ int compute(tfs_velocities* v, const double* x_segs, const double* f_segs, const
             blobs_moved = *v->h_flag != 0;
         }
     }
-    if (!rc) rc = check(hipMemcpyAsync(h->x_blobs, x_blobs, 24 * nb, kin, st), "blob positions");
-    if (!rc && ns) rc = check(hipMemcpyAsync(h->x_segs, x_segs, 24 * ns, kin, st), "segment positions");
-    if (!rc && ns) rc = check(hipMemcpyAsync(h->f_segs, f_segs, 48 * ns, kin, st), "segment forces");
+    // With device memory the segment forces are read in place, the segment
+    // velocities are summed straight into the caller's array, and the small
+    // inputs arrive in one copy launch; a direct solve needs no warm start.
+    const bool dev = v->desc.memory == TFS_MEMORY_DEVICE;
+    const bool direct_fixed = v->direct && !free_bodies;
+    const double* fs_in = dev && ns ? f_segs : h->f_segs;
+    double* vs_out = dev && ns ? v_segs : h->v_segs;
     double* sol = free_bodies ? v->d_sol : h->f_blobs;
-    if (!rc) rc = check(hipMemcpyAsync(sol, f_blobs, 24 * nb, kin, st), "blob warm start");
-    if (!rc && free_bodies)
-        rc = check(hipMemcpyAsync(sol + 3 * nb, v_bodies_io, 48 * v->nswim, kin, st), "body warm start");
-    if (!rc) rc = check(hipMemcpyAsync(v->d_bodies, x_bodies, 24 * v->nswim, kin, st), "body positions");
-    if (!rc && bodies)
-        rc = check(hipMemcpyAsync(v->d_bodies + 3 * v->nswim, bodies, 48 * v->nswim, kin, st), "body velocities");
-    if (!rc && q_bodies) rc = check(hipMemcpyAsync(v->d_q, q_bodies, 32 * v->nswim, kin, st), "body orientations");
+    if (!rc && dev) {
+        CopyList L;
+        L.add(x_blobs, h->x_blobs, 3 * nb);
+        if (ns) L.add(x_segs, h->x_segs, 3 * ns);
+        if (!v->direct) L.add(f_blobs, sol, 3 * nb);
+        if (free_bodies && !v->direct) L.add(v_bodies_io, sol + 3 * nb, 6 * v->nswim);
+        L.add(x_bodies, v->d_bodies, 3 * v->nswim);
+        if (bodies) L.add(bodies, v->d_bodies + 3 * v->nswim, 6 * v->nswim);
+        if (q_bodies) L.add(q_bodies, v->d_q, 4 * v->nswim);
+        rc = check(copy_list(L, st), "inputs");
+    } else if (!rc) {
+        rc = check(hipMemcpyAsync(h->x_blobs, x_blobs, 24 * nb, kin, st), "blob positions");
+        if (!rc && ns) rc = check(hipMemcpyAsync(h->x_segs, x_segs, 24 * ns, kin, st), "segment positions");
+        if (!rc && ns) rc = check(hipMemcpyAsync(h->f_segs, f_segs, 48 * ns, kin, st), "segment forces");
+        if (!rc) rc = check(hipMemcpyAsync(sol, f_blobs, 24 * nb, kin, st), "blob warm start");
+        if (!rc && free_bodies)
+            rc = check(hipMemcpyAsync(sol + 3 * nb, v_bodies_io, 48 * v->nswim, kin, st), "body warm start");
+        if (!rc) rc = check(hipMemcpyAsync(v->d_bodies, x_bodies, 24 * v->nswim, kin, st), "body positions");
+        if (!rc && bodies)
+            rc = check(hipMemcpyAsync(v->d_bodies + 3 * v->nswim, bodies, 48 * v->nswim, kin, st), "body velocities");
+        if (!rc && q_bodies) rc = check(hipMemcpyAsync(v->d_q, q_bodies, 32 * v->nswim, kin, st), "body orientations");
+    }
     if (!rc) rc = tfs_rpy_positions_changed(h);
-    // Mss_mult '=', Mbs_mult '='
-    if (!rc && ns) rc = tfs_rpy_apply(h, TFS_BLOCK_SS, h->f_segs, nullptr, h->v_segs, nullptr);
-    if (!rc && ns) rc = tfs_rpy_apply(h, TFS_BLOCK_BS, h->f_segs, nullptr, nullptr, h->v_blobs);
+    // Mss_mult '=', Mbs_mult '=' (one launch)
+    if (!rc && ns) rc = tfs_rpy_apply(h, TFS_BLOCK_SS | TFS_BLOCK_BS, fs_in, nullptr, vs_out, h->v_blobs);
+    // the blob solve's geometry (the direct free-body solve works in the body
+    // frame and needs none)
+    if (!rc && blobs_moved && !(free_bodies && v->direct))
+        rc = tfs_solver_set_geometry(v->blobs, nullptr, h->x_blobs, nullptr, nullptr, TFS_MEMORY_DEVICE);
+    if (!rc) v->have_blobs = true;
+    if (!rc && blobs_moved) v->direct_valid = false;
+    if (!rc && direct_fixed && !v->direct_valid) rc = build_direct(v, v->rpy->x_blobs);
     if (!rc && free_bodies) {
-        hipLaunchKernelGGL(free_rhs_kernel, dim3((unsigned)v->nswim), dim3(256), 0, st, h->x_segs, h->f_segs,
+        hipLaunchKernelGGL(free_rhs_kernel, dim3((unsigned)v->nswim), dim3(256), 0, st, h->x_segs, fs_in,
                            h->v_blobs, nb, (int)v->desc.segs_per_swim, v->d_bodies,
                            bodies ? v->d_bodies + 3 * v->nswim : nullptr, v->d_rhs);
         rc = check(hipGetLastError(), "free-body rhs");
         if (!rc) rc = tfs_solver_set_bodies(v->blobs, v->d_bodies, TFS_MEMORY_DEVICE);
     } else if (!rc) {
+        // the direct solve reads its right-hand side from the padded vector
         hipLaunchKernelGGL(blob_rhs_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, h->v_blobs,
                            h->x_blobs, v->d_bodies, v->d_bodies + 3 * v->nswim, (int)nb, v->desc.blobs_per_swim,
-                           ns > 0 ? 1 : 0, v->d_rhs);
+                           ns > 0 ? 1 : 0, direct_fixed ? v->xpad : v->d_rhs);
         rc = check(hipGetLastError(), "blob rhs");
     }
-    // GMRES on M_bb for the blob forces, warm-started from the given ones
-    // (or, direct, one product with the stored inverse)
-    // (the direct free-body solve works in the body frame and needs no geometry)
-    if (!rc && blobs_moved && !(free_bodies && v->direct))
-        rc = tfs_solver_set_geometry(v->blobs, nullptr, h->x_blobs, nullptr, nullptr, TFS_MEMORY_DEVICE);
-    if (!rc) v->have_blobs = true;
-    if (!rc && blobs_moved) v->direct_valid = false;
     int solve_rc = TFS_OK;
     if (!rc && v->direct && free_bodies) {
         // one free body: in its frame with the stored reference inverse
@@ -583,10 +632,8 @@ int compute(tfs_velocities* v, const double* x_segs, const double* f_segs, const
             stats->converged = 1;
         }
     } else if (!rc && v->direct) {
-        if (!v->direct_valid) rc = build_direct(v, v->rpy->x_blobs);
-        if (!rc) rc = check(hipMemcpyAsync(v->xpad, v->d_rhs, 24 * nb, hipMemcpyDeviceToDevice, st), "direct rhs");
-        if (!rc) rc = check(k_symv_dense(v->Ainv, v->n_pad, v->xpad, v->ypad, v->sym_partial, v->chunks, st), "direct solve");
-        if (!rc) rc = check(hipMemcpyAsync(h->f_blobs, v->ypad, 24 * nb, hipMemcpyDeviceToDevice, st), "direct result");
+        // blob forces = M_bb^{-1} rhs, left in ypad (its first 3 nb entries)
+        rc = check(k_symv_dense(v->Ainv, v->n_pad, v->xpad, v->ypad, v->sym_partial, v->chunks, st), "direct solve");
         if (stats) {
             *stats = tfs_solve_stats{};
             stats->converged = 1;
@@ -598,13 +645,21 @@ int compute(tfs_velocities* v, const double* x_segs, const double* f_segs, const
             rc = check(hipMemcpyAsync(h->f_blobs, sol, 24 * nb, hipMemcpyDeviceToDevice, st), "blob forces");
     }
     const std::string solve_msg = solve_rc ? g_last_error : std::string();
+    const double* fb_solved = direct_fixed ? v->ypad : h->f_blobs;
     // Msb_mult '+='
-    if (!rc && ns) rc = tfs_rpy_apply(h, TFS_BLOCK_SB, nullptr, h->f_blobs, h->v_segs, nullptr);
-    if (!rc && ns) rc = check(hipMemcpyAsync(v_segs, h->v_segs, 48 * ns, kout, st), "segment velocities");
-    if (!rc) rc = check(hipMemcpyAsync(f_blobs, h->f_blobs, 24 * nb, kout, st), "blob forces");
-    if (!rc && free_bodies)
-        rc = check(hipMemcpyAsync(v_bodies_io, sol + 3 * nb, 48 * v->nswim, kout, st), "body velocities");
-    if (!rc) rc = check(hipStreamSynchronize(st), "tfs_velocities_compute");
+    if (!rc && ns) rc = tfs_rpy_apply(h, TFS_BLOCK_SB, nullptr, fb_solved, vs_out, nullptr);
+    if (!rc && dev) {
+        CopyList L;
+        L.add(fb_solved, f_blobs, 3 * nb);
+        if (free_bodies) L.add(sol + 3 * nb, v_bodies_io, 6 * v->nswim);
+        rc = check(copy_list(L, st), "outputs");
+    } else if (!rc) {
+        if (ns) rc = check(hipMemcpyAsync(v_segs, h->v_segs, 48 * ns, kout, st), "segment velocities");
+        if (!rc) rc = check(hipMemcpyAsync(f_blobs, fb_solved, 24 * nb, kout, st), "blob forces");
+        if (!rc && free_bodies)
+            rc = check(hipMemcpyAsync(v_bodies_io, sol + 3 * nb, 48 * v->nswim, kout, st), "body velocities");
+    }
+    if (!rc && sync_end) rc = check(hipStreamSynchronize(st), "tfs_velocities_compute");
     if (rc) return rc;
     if (solve_rc) {
         g_last_error = solve_msg;
@@ -690,6 +745,18 @@ int compute_f32(tfs_velocities* v, const float* x_segs, const float* f_segs, con
 }  // namespace
 
 int32_t tfs::velocities_blob_motion(const tfs_velocities* v) { return v->blob_motion; }
+
+int tfs::velocities_compute_ordered(tfs_velocities* v, const double* x_segs, const double* f_segs,
+                                    const double* x_blobs, const double* x_bodies, const double* bodies,
+                                    const double* q_bodies, double* v_segs, double* f_blobs, double* v_bodies_io,
+                                    tfs_solve_stats* stats, bool free_bodies) {
+    return compute(v, x_segs, f_segs, x_blobs, x_bodies, bodies, q_bodies, v_segs, f_blobs, v_bodies_io, stats,
+                   free_bodies, v->desc.memory != TFS_MEMORY_DEVICE);
+}
+
+hipStream_t tfs::velocities_stream(const tfs_velocities* v) { return v->stream; }
+
+tfs_rpy* tfs::velocities_rpy(const tfs_velocities* v) { return v->rpy; }
 
 extern "C" {
 
