@@ -40,9 +40,10 @@ struct DenseWork {
 // trailing update and the column panel are 64^3-FMA tile products, one
 // workgroup per tile.  work.status[k] = 1 if step k met a singular pivot.
 hipError_t k_spd_inverse_blocked(double* A, int n_pad, const DenseWork& work, hipStream_t stream);
-// y[0..n) = A x for a symmetric n_pad x n_pad A (x and y of n_pad entries, x
-// zero past n): column chunks into partial (chunks x n_pad), then summed in
-// chunk order.
+// y = A x for a symmetric n_pad x n_pad A (n_pad a multiple of 64; x and y
+// of n_pad entries) from its lower triangle: per 64 x 64 tile the row sums
+// and, off the diagonal, the column sums into partial (chunks x n_pad,
+// chunks = n_pad / 64), then summed in a fixed order.
 hipError_t k_symv_dense(const double* A, int n_pad, const double* x, double* y, double* partial, int chunks,
                         hipStream_t stream);
 

@@ -229,32 +229,93 @@ __global__ __launch_bounds__(256) void gj_status_kernel(const int32_t* __restric
     if (threadIdx.x == 0) out[k] = st[0];
 }
 
-__global__ __launch_bounds__(256) void symv_partial_kernel(const double* __restrict__ A, int n_pad,
-                                                          const double* __restrict__ x, double* __restrict__ partial,
-                                                          int chunk) {
-    const int r = blockIdx.x * 256 + threadIdx.x;
-    const int c0 = blockIdx.y * chunk, c1 = min(c0 + chunk, n_pad);
-    if (r >= n_pad) return;
-    // symmetric: row r = column r; four independent chains keep more loads in flight
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int c = c0;
-    for (; c + 4 <= c1; c += 4) {
-        a0 += A[(int64_t)c * n_pad + r] * x[c];
-        a1 += A[(int64_t)(c + 1) * n_pad + r] * x[c + 1];
-        a2 += A[(int64_t)(c + 2) * n_pad + r] * x[c + 2];
-        a3 += A[(int64_t)(c + 3) * n_pad + r] * x[c + 3];
+// y = A x for the symmetric n_pad x n_pad A (column major) from its lower
+// triangle, which is read once: workgroup b takes tile (I, J), I >= J, of
+// 64 x 64 (b = I (I + 1) / 2 + J) and writes its row sums A_IJ x_J to
+// partial[(I nt + J) 64 + .] and, off the diagonal, its column sums
+// A_IJ^T x_I to partial[(J nt + I) 64 + .], so that every (R, K) slot of the
+// nt x nt x 64 partials is written once.  Wave g holds rows I*64 + lane and
+// columns J*64 + 16 g .. + 16; the column sums go down the wave by recursive
+// halving (each lane keeps half the columns per exchange), 17 shuffles for
+// 16 columns.
+__global__ __launch_bounds__(256) void symv_tiles_kernel(const double* __restrict__ A, int n_pad,
+                                                        const double* __restrict__ x,
+                                                        double* __restrict__ partial) {
+    __shared__ double rows[4][64];
+    const int nt = n_pad / 64;
+    const int b = (int)blockIdx.x;
+    int I = (int)((__builtin_sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+    while ((I + 1) * (I + 2) / 2 <= b) ++I;
+    while (I * (I + 1) / 2 > b) --I;
+    const int J = b - I * (I + 1) / 2;
+    const int lane = (int)threadIdx.x & 63, g = (int)threadIdx.x >> 6;
+    const int r = I * 64 + lane;
+    const int c0 = J * 64 + g * 16;
+    const double* col = A + (int64_t)c0 * n_pad + r;
+    double v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = col[(int64_t)k * n_pad];
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; k += 2) {
+        s0 += v[k] * x[c0 + k];
+        s1 += v[k + 1] * x[c0 + k + 1];
     }
-    for (; c < c1; ++c) a0 += A[(int64_t)c * n_pad + r] * x[c];
-    partial[(int64_t)blockIdx.y * n_pad + r] = (a0 + a1) + (a2 + a3);
+    rows[g][lane] = s0 + s1;
+    if (I != J) {  // uniform over the workgroup
+        const double xr = x[r];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] *= xr;
+        // halving: bit 5 picks columns 8-15, bit 4 the upper four, bit 3 two, bit 2 one
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const bool hi = lane & 32;
+            const double send = hi ? v[k] : v[k + 8];
+            v[k] = (hi ? v[k + 8] : v[k]) + __shfl_xor(send, 32, 64);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool hi = lane & 16;
+            const double send = hi ? v[k] : v[k + 4];
+            v[k] = (hi ? v[k + 4] : v[k]) + __shfl_xor(send, 16, 64);
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const bool hi = lane & 8;
+            const double send = hi ? v[k] : v[k + 2];
+            v[k] = (hi ? v[k + 2] : v[k]) + __shfl_xor(send, 8, 64);
+        }
+        {
+            const bool hi = lane & 4;
+            const double send = hi ? v[0] : v[1];
+            v[0] = (hi ? v[1] : v[0]) + __shfl_xor(send, 4, 64);
+        }
+        v[0] += __shfl_xor(v[0], 2, 64);
+        v[0] += __shfl_xor(v[0], 1, 64);
+        if ((lane & 3) == 0) {
+            const int c = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
+            partial[((int64_t)J * nt + I) * 64 + g * 16 + c] = v[0];
+        }
+    }
+    __syncthreads();
+    if (g == 0) partial[((int64_t)I * nt + J) * 64 + lane] = (rows[0][lane] + rows[1][lane]) + (rows[2][lane] + rows[3][lane]);
 }
 
-__global__ __launch_bounds__(256) void symv_sum_kernel(const double* __restrict__ partial, int n_pad, int chunks,
-                                                      double* __restrict__ y) {
-    const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r >= n_pad) return;
-    double acc = 0.0;
-    for (int q = 0; q < chunks; ++q) acc += partial[(int64_t)q * n_pad + r];
-    y[r] = acc;
+// y[R 64 + l] = sum over K in order of partial[(R nt + K) 64 + l].
+__global__ __launch_bounds__(256) void symv_tiles_sum_kernel(const double* __restrict__ partial, int n_pad,
+                                                            double* __restrict__ y) {
+    const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    if (i >= n_pad) return;
+    const int nt = n_pad / 64, R = i / 64, l = i % 64;
+    const double* p = partial + (int64_t)R * nt * 64 + l;
+    double a0 = 0.0, a1 = 0.0;
+    int K = 0;
+    for (; K + 2 <= nt; K += 2) {
+        a0 += p[(int64_t)K * 64];
+        a1 += p[(int64_t)(K + 1) * 64];
+    }
+    if (K < nt) a0 += p[(int64_t)K * 64];
+    y[i] = a0 + a1;
 }
 
 }  // namespace
@@ -299,11 +360,12 @@ hipError_t k_spd_inverse_blocked(double* A, int n_pad, const DenseWork& w, hipSt
 
 hipError_t k_symv_dense(const double* A, int n_pad, const double* x, double* y, double* partial, int chunks,
                         hipStream_t stream) {
-    const int chunk = (n_pad + chunks - 1) / chunks;
-    hipLaunchKernelGGL(symv_partial_kernel, dim3((unsigned)((n_pad + 255) / 256), (unsigned)chunks), dim3(256), 0,
-                       stream, A, n_pad, x, partial, chunk);
-    hipLaunchKernelGGL(symv_sum_kernel, dim3((unsigned)((n_pad + 255) / 256)), dim3(256), 0, stream, partial, n_pad,
-                       chunks, y);
+    const int nt = n_pad / 64;
+    if (n_pad % 64 || chunks != nt) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(symv_tiles_kernel, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, stream, A, n_pad, x,
+                       partial);
+    hipLaunchKernelGGL(symv_tiles_sum_kernel, dim3((unsigned)((n_pad + 255) / 256)), dim3(256), 0, stream,
+                       (const double*)partial, n_pad, y);
     return hipGetLastError();
 }
 

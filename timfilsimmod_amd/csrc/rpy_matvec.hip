@@ -48,6 +48,9 @@ namespace {
 
 constexpr int kMaxDof = 6;
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
+// Up to this many particles the near-pair lists test every pair (bit masks of
+// n^2 / 8 bytes) instead of sorting the particles into a hashed grid.
+constexpr int kNearPairsMax = 6144;
 
 // LDS record length (in T) of one staged source with FS strength components
 // for targets with VT velocity components.  FP64: x y z f0..f(FS-1) [pad].
@@ -650,6 +653,75 @@ __global__ __launch_bounds__(256) void near_grid_kernel(MatvecArgs a, int n, dou
     }
 }
 
+// All pairs, for problems of a few thousand particles: one wave per 64
+// targets x 64 sources sets bit k of mask[c n + t] when source 64 c + k
+// overlaps target t, classified exactly as near_grid_kernel does (the pair
+// kernel's staging and separation, the same r^2 bound); body_of null: one
+// body.  Workgroup (0, 0) clears max_cnt for near_emit_kernel.
+template <typename T>
+__global__ __launch_bounds__(64) void near_pairs_kernel(MatvecArgs a, int n, const int32_t* __restrict__ body_of,
+                                                       uint64_t* __restrict__ mask, int32_t* __restrict__ max_cnt) {
+    constexpr bool F32 = sizeof(T) == 4;
+    constexpr int REC = rec_len<T, 3>();
+    __shared__ __align__(16) T recs[64 * REC];
+    __shared__ int32_t sbody[64];
+    __shared__ int32_t scls[64];
+    const int lane = (int)threadIdx.x;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && lane == 0) *max_cnt = 0;
+    const int j0 = (int)blockIdx.y * 64;
+    const int jl = j0 + lane;
+    if (jl < n) {
+        const int jcls = jl < a.ns ? 0 : 1;
+        stage_source<T, 3>(recs + lane * REC, a, jcls, jcls == 0 ? jl : jl - a.ns, false);
+        sbody[lane] = body_of ? body_of[jl] : 0;
+        scls[lane] = jcls;
+    }
+    __syncthreads();
+    const int tg = (int)blockIdx.x * 64 + lane;
+    if (tg >= n) return;
+    const int cls = tg < a.ns ? 0 : 1;
+    using XT = typename std::conditional<F32, int32_t, double>::type;
+    XT xi[3];
+    load_target(a, cls, cls == 0 ? tg : tg - a.ns, xi);
+    const int body = body_of ? body_of[tg] : 0;
+    const double ai = cls == 0 ? a.a_seg : a.a_blob;
+    const double lam = F32 ? a.lam : 1.0;
+    const T sum_seg = T(lam * lam * ((ai + a.a_seg) * (ai + a.a_seg)));
+    const T sum_blob = T(lam * lam * ((ai + a.a_blob) * (ai + a.a_blob)));
+    const int m = n - j0 < 64 ? n - j0 : 64;
+    uint64_t bits = 0;
+    for (int k = 0; k < m; ++k) {
+        if (sbody[k] != body) continue;
+        T ddx, ddy, ddz;
+        const T r2 = separation(xi, recs + k * REC, ddx, ddy, ddz);
+        if (!(r2 > (scls[k] == 0 ? sum_seg : sum_blob))) bits |= uint64_t(1) << k;
+    }
+    mask[(int64_t)blockIdx.y * n + tg] = bits;
+}
+
+// One thread per target: the set bits of its masks, in increasing source
+// order, into the near list (entries past cap dropped, cnt the full count,
+// max_cnt the largest count past cap).
+__global__ __launch_bounds__(256) void near_emit_kernel(const uint64_t* __restrict__ mask, int n, int nchunk,
+                                                       int32_t* __restrict__ cnt, int32_t* __restrict__ idx, int cap,
+                                                       int32_t* __restrict__ max_cnt) {
+    const int tg = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    if (tg >= n) return;
+    int32_t* e = idx + (int64_t)tg * cap;
+    int c = 0;
+    for (int q = 0; q < nchunk; ++q) {
+        uint64_t b = mask[(int64_t)q * n + tg];
+        while (b) {
+            const int k = __builtin_ctzll(b);
+            b &= b - 1;
+            if (c < cap) e[c] = q * 64 + k;
+            ++c;
+        }
+    }
+    cnt[tg] = c;
+    if (c > cap) atomicMax(max_cnt, c);
+}
+
 __global__ void fill_int_kernel(int32_t* p, int32_t v, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1054,9 +1126,12 @@ void build_worklist(const std::vector<BodyRange>& bodies, int tile_targets, int 
 }
 
 void free_grid(NearGrid& g) {
-    for (int32_t* p : {g.start, g.fill, g.bucket_of, g.order, g.body_of})
+    for (int32_t* p : {g.start, g.fill, g.bucket_of, g.order, g.body_of, g.pair_body})
         if (p) (void)hipFree(p);
+    if (g.mask) (void)hipFree(g.mask);
+    const bool may_alloc = g.may_alloc;
     g = NearGrid{};
+    g.may_alloc = may_alloc;
 }
 
 // Sorts the n particles (segments, then blobs) into the grid's buckets for
@@ -1092,6 +1167,46 @@ hipError_t build_near_list(int precision, const MatvecArgs& a, NearGrid& grid, c
     const int n = n_targets;
     if (max_count) *max_count = 0;
     if (n <= 0) return hipSuccess;
+    const int nchunk = (n + 63) / 64;
+    if (n <= kNearPairsMax && (grid.may_alloc || grid.mask_len >= (int64_t)nchunk * n)) {
+        if (grid.mask_len < (int64_t)nchunk * n) {
+            TFS_HIP_CHECK(hipStreamSynchronize(stream));
+            if (grid.mask) (void)hipFree(grid.mask);
+            grid.mask = nullptr;
+            grid.mask_len = 0;
+            TFS_HIP_CHECK(hipMalloc(&grid.mask, sizeof(uint64_t) * (size_t)nchunk * n));
+            grid.mask_len = (int64_t)nchunk * n;
+        }
+        const int32_t* body_of = nullptr;
+        if (nbody > 1) {
+            if (grid.pair_body_len < n) {
+                TFS_HIP_CHECK(hipStreamSynchronize(stream));
+                if (grid.pair_body) (void)hipFree(grid.pair_body);
+                grid.pair_body = nullptr;
+                grid.pair_body_len = 0;
+                TFS_HIP_CHECK(hipMalloc(&grid.pair_body, sizeof(int32_t) * n));
+                grid.pair_body_len = n;
+            }
+            hipLaunchKernelGGL(grid_body_kernel, dim3((unsigned)nbody), dim3(256), 0, stream,
+                               reinterpret_cast<const int4*>(body_ranges), a.ns, grid.pair_body);
+            body_of = grid.pair_body;
+        }
+        const dim3 g2((unsigned)nchunk, (unsigned)nchunk);
+        if (precision == 0)
+            hipLaunchKernelGGL(near_pairs_kernel<double>, g2, dim3(64), 0, stream, a, n, body_of, grid.mask,
+                               near.max_cnt);
+        else
+            hipLaunchKernelGGL(near_pairs_kernel<float>, g2, dim3(64), 0, stream, a, n, body_of, grid.mask,
+                               near.max_cnt);
+        hipLaunchKernelGGL(near_emit_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                           (const uint64_t*)grid.mask, n, nchunk, near.cnt, near.idx, near.cap, near.max_cnt);
+        TFS_HIP_CHECK(hipGetLastError());
+        TFS_HIP_CHECK(hipMemcpyAsync(host_scratch, near.max_cnt, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+        if (!max_count) return hipSuccess;
+        TFS_HIP_CHECK(hipStreamSynchronize(stream));
+        *max_count = *host_scratch;
+        return hipSuccess;
+    }
     const double h = 2.0 * std::max(a.a_seg, a.a_blob) * (1.0 + 1e-5);
     const double inv_h = 1.0 / h;
     TFS_HIP_CHECK(sort_into_grid(a, grid, n, inv_h, stream));

@@ -73,6 +73,13 @@ struct NearGrid {
     int32_t* body_of = nullptr;    // n: body index of each particle
     int table = 0;
     int n = 0;
+    // Small problems (build_near_list's all-pairs form): overlap bit masks,
+    // 64 sources per word, and a body index per particle of its own.
+    uint64_t* mask = nullptr;
+    int64_t mask_len = 0;
+    int32_t* pair_body = nullptr;
+    int pair_body_len = 0;
+    bool may_alloc = false;        // the all-pairs form may allocate its buffers
 };
 
 // Barrier forces (cuda_functions.cu:1240-1436): strength

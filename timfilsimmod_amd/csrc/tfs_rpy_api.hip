@@ -532,6 +532,7 @@ int tfs_rpy_create(const tfs_rpy_desc* desc, tfs_rpy** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(TFS_ERR_NO_DEVICE, "tfs_rpy_create: no HIP device visible");
     tfs_rpy* h = new tfs_rpy();
+    h->grid.may_alloc = true;
     h->desc = *desc;
     hipGetDevice(&h->device);
     hipDeviceProp_t prop;

@@ -306,6 +306,7 @@ int build_direct(tfs_velocities* v, const double* x) {
     hipStream_t st = v->stream;
     if (!v->Ainv) {
         v->n_pad = n_pad;
+        v->chunks = nt;  // k_symv_dense's partials: one row of tiles per 64 rows
         if (hipMalloc(&v->Ainv, 8 * (size_t)n_pad * n_pad) != hipSuccess)
             return fail(TFS_ERR_OUT_OF_MEMORY, "direct blob solve: " + std::to_string(8.0 * n_pad * n_pad / 1e9) +
                                                    " GB for the dense blob mobility");
