@@ -135,8 +135,9 @@ int tfs_rpy_set_precision(tfs_rpy *h, int32_t precision);
 int tfs_rpy_set_domain(tfs_rpy *h, int32_t domain);
 
 /* Override the launch shape (threads per workgroup 64/128/256, targets per
- * thread 1/2, sources per work item: a multiple of block) for a precision;
- * every lane holds its own targets (no source split). */
+ * thread 1/2, sources per work item: a multiple of block, or a divisor of it
+ * of at least 32) for a precision; every lane holds its own targets (no
+ * source split). */
 int tfs_rpy_set_tile_shape(tfs_rpy *h, int32_t precision, int32_t block, int32_t per_thread,
                            int32_t chunk);
 

@@ -13,7 +13,7 @@ full_mobility) on the same seeded inputs:
 * the argument rules: no persistent form, block 256 or 512 with the split,
   64/128/256 without;
 * the default shapes: the split for a body that fills the device (12,800
-  segments + 9,000 blobs), one wave per 64 x 64 item for one that does not."""
+  segments + 9,000 blobs), one wave per 64 x 32 item for one that does not."""
 import numpy as np
 import pytest
 
@@ -162,7 +162,7 @@ def test_default_shapes(seg_dof):
         assert f64 == {"block": 256, "per_thread": 2, "chunk": 512, "persistent": 0, "split": 1}
         s64 = small.tile_shape(L.PRECISION_FP64)
         print("320 + 2,000:", s64, flush=True)
-        assert s64 == {"block": 64, "per_thread": 1, "chunk": 64, "persistent": 0, "split": 0}
+        assert s64 == {"block": 64, "per_thread": 1, "chunk": 32, "persistent": 0, "split": 0}
     finally:
         big.close()
         small.close()

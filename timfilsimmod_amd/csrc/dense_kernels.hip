@@ -301,21 +301,27 @@ __global__ __launch_bounds__(256) void symv_tiles_kernel(const double* __restric
     if (g == 0) partial[((int64_t)I * nt + J) * 64 + lane] = (rows[0][lane] + rows[1][lane]) + (rows[2][lane] + rows[3][lane]);
 }
 
-// y[R 64 + l] = sum over K in order of partial[(R nt + K) 64 + l].
+// y[R 64 + l] = sum over K of partial[(R nt + K) 64 + l]: workgroup R, wave
+// w summing K in [w q, (w + 1) q) (q = ceil(nt / 4)), the four quarters added
+// in order through LDS.
 __global__ __launch_bounds__(256) void symv_tiles_sum_kernel(const double* __restrict__ partial, int n_pad,
                                                             double* __restrict__ y) {
-    const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
-    if (i >= n_pad) return;
-    const int nt = n_pad / 64, R = i / 64, l = i % 64;
+    __shared__ double quarter[4][64];
+    const int nt = n_pad / 64, R = (int)blockIdx.x;
+    const int l = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+    const int q = (nt + 3) / 4;
+    const int k0 = w * q, k1 = k0 + q < nt ? k0 + q : nt;
     const double* p = partial + (int64_t)R * nt * 64 + l;
     double a0 = 0.0, a1 = 0.0;
-    int K = 0;
-    for (; K + 2 <= nt; K += 2) {
+    int K = k0;
+    for (; K + 2 <= k1; K += 2) {
         a0 += p[(int64_t)K * 64];
         a1 += p[(int64_t)(K + 1) * 64];
     }
-    if (K < nt) a0 += p[(int64_t)K * 64];
-    y[i] = a0 + a1;
+    if (K < k1) a0 += p[(int64_t)K * 64];
+    quarter[w][l] = a0 + a1;
+    __syncthreads();
+    if (w == 0) y[R * 64 + l] = (quarter[0][l] + quarter[1][l]) + (quarter[2][l] + quarter[3][l]);
 }
 
 }  // namespace
@@ -364,8 +370,8 @@ hipError_t k_symv_dense(const double* A, int n_pad, const double* x, double* y, 
     if (n_pad % 64 || chunks != nt) return hipErrorInvalidValue;
     hipLaunchKernelGGL(symv_tiles_kernel, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, stream, A, n_pad, x,
                        partial);
-    hipLaunchKernelGGL(symv_tiles_sum_kernel, dim3((unsigned)((n_pad + 255) / 256)), dim3(256), 0, stream,
-                       (const double*)partial, n_pad, y);
+    hipLaunchKernelGGL(symv_tiles_sum_kernel, dim3((unsigned)nt), dim3(256), 0, stream, (const double*)partial,
+                       n_pad, y);
     return hipGetLastError();
 }
 

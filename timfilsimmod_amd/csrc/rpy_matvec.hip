@@ -690,36 +690,51 @@ __global__ __launch_bounds__(64) void near_pairs_kernel(MatvecArgs a, int n, con
     const T sum_blob = T(lam * lam * ((ai + a.a_blob) * (ai + a.a_blob)));
     const int m = n - j0 < 64 ? n - j0 : 64;
     uint64_t bits = 0;
+#pragma unroll 8
     for (int k = 0; k < m; ++k) {
-        if (sbody[k] != body) continue;
         T ddx, ddy, ddz;
         const T r2 = separation(xi, recs + k * REC, ddx, ddy, ddz);
-        if (!(r2 > (scls[k] == 0 ? sum_seg : sum_blob))) bits |= uint64_t(1) << k;
+        const bool near = !(r2 > (scls[k] == 0 ? sum_seg : sum_blob)) && sbody[k] == body;
+        bits |= near ? uint64_t(1) << k : uint64_t(0);
     }
     mask[(int64_t)blockIdx.y * n + tg] = bits;
 }
 
-// One thread per target: the set bits of its masks, in increasing source
-// order, into the near list (entries past cap dropped, cnt the full count,
-// max_cnt the largest count past cap).
+// One wave per target: lane q takes the masks of chunks q, q + 64, ...; an
+// exclusive scan of the lanes' bit counts places each lane's sources, so the
+// list comes out in increasing source order (entries past cap dropped, cnt
+// the full count, max_cnt the largest count past cap).
 __global__ __launch_bounds__(256) void near_emit_kernel(const uint64_t* __restrict__ mask, int n, int nchunk,
                                                        int32_t* __restrict__ cnt, int32_t* __restrict__ idx, int cap,
                                                        int32_t* __restrict__ max_cnt) {
-    const int tg = (int)blockIdx.x * 256 + (int)threadIdx.x;
-    if (tg >= n) return;
+    const int tg = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 6);
+    const int lane = (int)threadIdx.x & 63;
+    if (tg >= n) return;  // uniform over the wave
     int32_t* e = idx + (int64_t)tg * cap;
-    int c = 0;
-    for (int q = 0; q < nchunk; ++q) {
-        uint64_t b = mask[(int64_t)q * n + tg];
+    int base = 0;
+    for (int q0 = 0; q0 < nchunk; q0 += 64) {
+        const int q = q0 + lane;
+        uint64_t b = q < nchunk ? mask[(int64_t)q * n + tg] : uint64_t(0);
+        const int pc = __popcll(b);
+        int incl = pc;  // inclusive scan over the wave
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int t = __shfl_up(incl, off, 64);
+            incl += lane >= off ? t : 0;
+        }
+        int c = base + incl - pc;
         while (b) {
             const int k = __builtin_ctzll(b);
             b &= b - 1;
             if (c < cap) e[c] = q * 64 + k;
             ++c;
         }
+        base += __shfl(incl, 63, 64);
     }
-    cnt[tg] = c;
-    if (c > cap) atomicMax(max_cnt, c);
+    if (lane == 0) {
+        cnt[tg] = base;
+        if (base > cap) atomicMax(max_cnt, base);
+    }
 }
 
 __global__ void fill_int_kernel(int32_t* p, int32_t v, int64_t n) {
@@ -1198,7 +1213,7 @@ hipError_t build_near_list(int precision, const MatvecArgs& a, NearGrid& grid, c
         else
             hipLaunchKernelGGL(near_pairs_kernel<float>, g2, dim3(64), 0, stream, a, n, body_of, grid.mask,
                                near.max_cnt);
-        hipLaunchKernelGGL(near_emit_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+        hipLaunchKernelGGL(near_emit_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, stream,
                            (const uint64_t*)grid.mask, n, nchunk, near.cnt, near.idx, near.cap, near.max_cnt);
         TFS_HIP_CHECK(hipGetLastError());
         TFS_HIP_CHECK(hipMemcpyAsync(host_scratch, near.max_cnt, sizeof(int32_t), hipMemcpyDeviceToHost, stream));

@@ -211,11 +211,17 @@ __global__ __launch_bounds__(512) void block_gemv_wide_kernel(const double* __re
         double s0 = 0.0, s1 = 0.0;
         if (r < bs) {
             int c = c0;
-            for (; c + 2 <= c1; c += 2) {
-                s0 += Ab[(int64_t)c * bs + r] * x[r0 + c];
-                s1 += Ab[(int64_t)(c + 1) * bs + r] * x[r0 + c + 1];
+            for (; c + 8 <= c1; c += 8) {  // eight loads in flight per batch
+                double a[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) a[u] = Ab[(int64_t)(c + u) * bs + r];
+#pragma unroll
+                for (int u = 0; u < 8; u += 2) {
+                    s0 += a[u] * x[r0 + c + u];
+                    s1 += a[u + 1] * x[r0 + c + u + 1];
+                }
             }
-            if (c < c1) s0 += Ab[(int64_t)c * bs + r] * x[r0 + c];
+            for (; c < c1; ++c) s0 += Ab[(int64_t)c * bs + r] * x[r0 + c];
         }
         part[q][rl] = s0 + s1;
         __syncthreads();
@@ -224,28 +230,30 @@ __global__ __launch_bounds__(512) void block_gemv_wide_kernel(const double* __re
     }
 }
 
-// h[c] = D(:, c) . e for c < k: one wave per column over all rows (four
-// accumulation chains, then a fixed shuffle tree).
+// h[c] = D(:, c) . e for c < k: workgroup c, its four waves over rows
+// 256 i + 64 w + lane (four accumulation chains), a fixed shuffle tree per
+// wave and the waves added in order.
 __global__ __launch_bounds__(256) void lowrank_dots_kernel(const double* __restrict__ D, int64_t ld, int k,
                                                            const double* __restrict__ e, int n,
                                                            double* __restrict__ h) {
-    const int lane = threadIdx.x & 63;
-    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (c >= k) return;
+    __shared__ double wsum[4];
+    const int c = (int)blockIdx.x;
     const double* d = D + (int64_t)c * ld;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int r = lane;
-    for (; r + 192 < n; r += 256) {
+    int r = (int)threadIdx.x;
+    for (; r + 768 < n; r += 1024) {
         s0 += d[r] * e[r];
-        s1 += d[r + 64] * e[r + 64];
-        s2 += d[r + 128] * e[r + 128];
-        s3 += d[r + 192] * e[r + 192];
+        s1 += d[r + 256] * e[r + 256];
+        s2 += d[r + 512] * e[r + 512];
+        s3 += d[r + 768] * e[r + 768];
     }
-    for (; r < n; r += 64) s0 += d[r] * e[r];
+    for (; r < n; r += 256) s0 += d[r] * e[r];
     double s = (s0 + s1) + (s2 + s3);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if (lane == 0) h[c] = s;
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) h[c] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
 }
 
 // out[r] -= sum_{c < k} C(r, c) h[c] (four accumulation chains).
@@ -256,11 +264,14 @@ __global__ __launch_bounds__(256) void lowrank_gemv_kernel(const double* __restr
     if (r >= n) return;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int c = 0;
-    for (; c + 4 <= k; c += 4) {
-        s0 += C[(int64_t)c * ld + r] * h[c];
-        s1 += C[(int64_t)(c + 1) * ld + r] * h[c + 1];
-        s2 += C[(int64_t)(c + 2) * ld + r] * h[c + 2];
-        s3 += C[(int64_t)(c + 3) * ld + r] * h[c + 3];
+    for (; c + 8 <= k; c += 8) {  // eight loads in flight per batch
+        double a[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = C[(int64_t)(c + u) * ld + r];
+        s0 += a[0] * h[c] + a[4] * h[c + 4];
+        s1 += a[1] * h[c + 1] + a[5] * h[c + 5];
+        s2 += a[2] * h[c + 2] + a[6] * h[c + 6];
+        s3 += a[3] * h[c + 3] + a[7] * h[c + 7];
     }
     for (; c < k; ++c) s0 += C[(int64_t)c * ld + r] * h[c];
     out[r] -= (s0 + s1) + (s2 + s3);
@@ -622,7 +633,7 @@ int inverse_update(tfs_broyden* b, const double* e, double* out) {
     const int k = std::min(b->iter, b->max_iter - 1);
     if (rc || k == 0) return rc;
     if (b->n <= kWaveDotsMax) {
-        hipLaunchKernelGGL(lowrank_dots_kernel, dim3((unsigned)((k + 3) / 4)), dim3(256), 0, b->stream,
+        hipLaunchKernelGGL(lowrank_dots_kernel, dim3((unsigned)k), dim3(256), 0, b->stream,
                            (const double*)b->D, (int64_t)b->ld, k, e, b->n, b->h);
         hipLaunchKernelGGL(lowrank_gemv_kernel, dim3((unsigned)((b->n + 255) / 256)), dim3(256), 0, b->stream,
                            (const double*)b->C, (int64_t)b->ld, k, (const double*)b->h, out, b->n);

@@ -61,8 +61,9 @@ TileShape default_shape(int precision, int ns, int nb, int num_cus, int seg_dof,
     // precisions, and 64-target tiles lose 5-10%.
     // A body too small to fill the device at that shape (under ~2 items per
     // CU, e.g. 320 segments + 2,000 blobs: 190 items) takes the finest shape,
-    // one wave per item of 64 targets x 64 sources: its launches are bound by
-    // the longest item's latency, not by throughput.
+    // one wave per item of 64 targets x 32 sources (the LDS batch then holds
+    // half a wave's worth): its launches are bound by the longest item's
+    // latency, not by throughput.
     (void)seg_dof;
     TileShape s;
     s.block = 256;
@@ -74,7 +75,7 @@ TileShape default_shape(int precision, int ns, int nb, int num_cus, int seg_dof,
     const int tt = tile_targets(s);
     const long items = ((long)(ns + tt - 1) / tt + (nb + tt - 1) / tt) *
                        (((long)ns + s.chunk - 1) / s.chunk + ((long)nb + s.chunk - 1) / s.chunk);
-    if (items < 2L * (num_cus > 0 ? num_cus : 256)) s = TileShape{64, 1, 64, 0, 0};
+    if (items < 2L * (num_cus > 0 ? num_cus : 256)) s = TileShape{64, 1, 32, 0, 0};
     return s;
 }
 
@@ -789,8 +790,11 @@ static int set_shape(tfs_rpy* h, const char* fn, int32_t precision, int32_t bloc
                     std::string(fn) + (split ? ": block must be 256 or 512" : ": block must be 64, 128 or 256"));
     if (per_thread != 1 && per_thread != 2)
         return fail(TFS_ERR_INVALID_ARGUMENT, std::string(fn) + ": per_thread must be 1 or 2");
-    if (chunk < block || chunk % block)
-        return fail(TFS_ERR_INVALID_ARGUMENT, std::string(fn) + ": chunk must be a positive multiple of block");
+    const bool part_batch = !split && chunk >= 32 && chunk < block && block % chunk == 0;
+    if (!part_batch && (chunk < block || chunk % block))
+        return fail(TFS_ERR_INVALID_ARGUMENT,
+                    std::string(fn) + ": chunk must be a positive multiple of block (or, without the split, a "
+                                      "divisor of it of at least 32)");
     const int pers = h->shape[precision].persistent;
     if (split && pers > 0)
         return fail(TFS_ERR_INVALID_ARGUMENT, std::string(fn) + ": the source split has no persistent form");
