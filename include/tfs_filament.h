@@ -227,6 +227,17 @@ int tfs_elastic_destroy(tfs_elastic *e);
 /* Step nt.  TFS_ERR_NOT_CONVERGED if max_iter is reached (state kept),
  * TFS_ERR_NON_FINITE if a residual is not finite (cilia_sim_main.cu:469-487). */
 int tfs_elastic_step(tfs_elastic *e, int32_t nt, tfs_elastic_stats *stats);
+/* Captured Broyden iterations (no counterpart in the reference, whose loop
+ * is host code, cilia_sim_main.cu:394-502): with graphs on (the default),
+ * when the three handles share one stream, the pair blocks are FP64 and the
+ * blob forces come from the dense direct solve or the Broyden-for-everything
+ * matvec, each step captures its iteration (find_update, update, forces,
+ * velocities, residual) and the engine's end of iteration as two hipGraphs
+ * and replays them with one host wait per iteration; results equal the
+ * launch-by-launch loop's bit for bit.  0: always launch by launch. */
+int tfs_elastic_set_graphs(tfs_elastic *e, int32_t enable);
+/* Broyden iterations run as graph replays and launch by launch since creation. */
+int tfs_elastic_iteration_counts(tfs_elastic *e, int64_t *graph_iterations, int64_t *eager_iterations);
 
 /* The state of the step in the reference's backup record, after its step
  * number (swimmer::write_backup, filament::write_backup and

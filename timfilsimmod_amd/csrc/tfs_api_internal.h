@@ -7,6 +7,7 @@
 
 #include "../../include/tfs_rpy.h"
 #include "../../include/tfs_solver.h"
+#include "../../include/tfs_broyden.h"
 #include "tfs_internal.h"
 
 namespace tfs {
@@ -64,6 +65,7 @@ struct tfs_rpy {
     // largest count is copied to host_int[8 + precision] without a wait, and
     // tfs::rpy_near_check reads it after the caller's own synchronisation.
     bool defer_near_check = false;
+    bool body_table_valid = false;  // d_body_range holds `bodies`
     bool near_pending[2] = {false, false};
     tfs::NearGrid grid;
     int domain = 0;  // TFS_DOMAIN_UNBOUNDED or TFS_DOMAIN_HALF_SPACE
@@ -118,6 +120,9 @@ int velocities_compute_ordered(tfs_velocities* v, const double* x_segs, const do
                                double* f_blobs, double* v_bodies_io, tfs_solve_stats* stats, bool free_bodies);
 hipStream_t velocities_stream(const tfs_velocities* v);
 tfs_rpy* velocities_rpy(const tfs_velocities* v);
+// A device-memory handle whose next computes enqueue fixed launches only (the
+// dense direct blob solve formed, the blobs' motion declared): capturable.
+bool velocities_graph_ready(const tfs_velocities* v);
 // Deferred near-list checks (tfs_rpy::defer_near_check): after `waited` has
 // been synchronised, reads the largest neighbour count of every list rebuilt
 // since the last check; a list that overflowed its capacity is dropped with
@@ -129,6 +134,20 @@ int rpy_near_check(tfs_rpy* h, hipStream_t waited, bool* overflow);
 // error it started from.  Synchronises.
 int broyden_latest_norm(tfs_broyden* b, double* out);
 hipStream_t broyden_stream(const tfs_broyden* b);
+// Device-counted Broyden iterations, for hipGraph capture (device memory, at
+// most 65,536 unknowns): begin copies the iteration count to the device;
+// find_update and end_of_iter (residual in slot 1, its too-large flag read on
+// the device) then take it from there, with the update committed by copies
+// instead of slot swaps; finish hands the count and the latest slot back.
+bool broyden_dev_supported(const tfs_broyden* b);
+int broyden_dev_begin(tfs_broyden* b);
+int broyden_dev_find_update(tfs_broyden* b, double* update);
+int broyden_dev_end_of_iter(tfs_broyden* b);
+void broyden_dev_finish(tfs_broyden* b, int32_t iter, int32_t latest);
+// tfs_broyden_compute_errors in two halves: the launches and the copy of the
+// flags (no wait), and the flags once the stream has been waited for.
+int broyden_errors_enqueue(tfs_broyden* b, const tfs_broyden_state* st, int32_t slot);
+int broyden_errors_result(tfs_broyden* b, int32_t* too_large);
 
 extern thread_local std::string g_last_error;
 int fail(int code, const std::string& msg);

@@ -235,9 +235,11 @@ __global__ __launch_bounds__(512) void block_gemv_wide_kernel(const double* __re
 // wave and the waves added in order.
 __global__ __launch_bounds__(256) void lowrank_dots_kernel(const double* __restrict__ D, int64_t ld, int k,
                                                            const double* __restrict__ e, int n,
-                                                           double* __restrict__ h) {
+                                                           double* __restrict__ h,
+                                                           const int32_t* __restrict__ d_k) {
     __shared__ double wsum[4];
     const int c = (int)blockIdx.x;
+    if (d_k && c >= (*d_k < k ? *d_k : k)) return;  // device count: k is the largest
     const double* d = D + (int64_t)c * ld;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int r = (int)threadIdx.x;
@@ -259,9 +261,10 @@ __global__ __launch_bounds__(256) void lowrank_dots_kernel(const double* __restr
 // out[r] -= sum_{c < k} C(r, c) h[c] (four accumulation chains).
 __global__ __launch_bounds__(256) void lowrank_gemv_kernel(const double* __restrict__ C, int64_t ld, int k,
                                                            const double* __restrict__ h, double* __restrict__ out,
-                                                           int n) {
+                                                           int n, const int32_t* __restrict__ d_k) {
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r >= n) return;
+    if (d_k) k = *d_k < k ? *d_k : k;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int c = 0;
     for (; c + 8 <= k; c += 8) {  // eight loads in flight per batch
@@ -275,6 +278,50 @@ __global__ __launch_bounds__(256) void lowrank_gemv_kernel(const double* __restr
     }
     for (; c < k; ++c) s0 += C[(int64_t)c * ld + r] * h[c];
     out[r] -= (s0 + s1) + (s2 + s3);
+}
+
+// Device-counted end of iteration: when the residual was too large
+// (gate[0] > 1) and iteration it = *d_iter < max_iter - 1, C(:, it) and
+// D(:, it) = tC, tD over ||tD|| (every workgroup sums the G partials of
+// ||tD||^2 itself, in order) and the new residual becomes the engine's error
+// (slot0 = slot1).
+__global__ __launch_bounds__(256) void commit_update_kernel(const double* __restrict__ tC,
+                                                            const double* __restrict__ tD, double* __restrict__ C,
+                                                            double* __restrict__ D, int64_t ld, int n,
+                                                            double* __restrict__ slot0,
+                                                            const double* __restrict__ slot1,
+                                                            const double* __restrict__ partial, int G,
+                                                            const double* __restrict__ gate,
+                                                            const int32_t* __restrict__ d_iter, int max_iter) {
+    __shared__ double sh[4];
+    __shared__ double inv;
+    const int it = *d_iter;
+    if (!(gate[0] > 1.0) || it >= max_iter - 1) return;  // uniform
+    double s = 0.0;
+    for (int b = threadIdx.x; b < G; b += 256) s += partial[b];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double nrm = __builtin_sqrt((sh[0] + sh[1]) + (sh[2] + sh[3]));
+        inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+    }
+    __syncthreads();
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r < n) {
+        C[(int64_t)it * ld + r] = tC[r] * inv;
+        D[(int64_t)it * ld + r] = tD[r] * inv;
+        slot0[r] = slot1[r];
+    }
+}
+
+__global__ void set_count_kernel(int32_t* __restrict__ c, int32_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = v;
+}
+
+__global__ void advance_count_kernel(int32_t* __restrict__ c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c += 1;
 }
 
 // D_k = s1 - s0 and partial[b] = sum over workgroup b's rows of D_k^2.
@@ -585,6 +632,10 @@ struct tfs_broyden {
     int32_t* piv = nullptr;
     int32_t* status = nullptr;
     int max_block = 0;
+    // device-counted iterations (tfs::broyden_dev_*): the iteration count on
+    // the device, the next C and D columns before their commit
+    int32_t* d_iter = nullptr;
+    double *tC = nullptr, *tD = nullptr;
     double* host_scal = nullptr;
     std::vector<int32_t> host_status;
     std::vector<int32_t> host_sizes;
@@ -634,9 +685,10 @@ int inverse_update(tfs_broyden* b, const double* e, double* out) {
     if (rc || k == 0) return rc;
     if (b->n <= kWaveDotsMax) {
         hipLaunchKernelGGL(lowrank_dots_kernel, dim3((unsigned)k), dim3(256), 0, b->stream,
-                           (const double*)b->D, (int64_t)b->ld, k, e, b->n, b->h);
+                           (const double*)b->D, (int64_t)b->ld, k, e, b->n, b->h, (const int32_t*)nullptr);
         hipLaunchKernelGGL(lowrank_gemv_kernel, dim3((unsigned)((b->n + 255) / 256)), dim3(256), 0, b->stream,
-                           (const double*)b->C, (int64_t)b->ld, k, (const double*)b->h, out, b->n);
+                           (const double*)b->C, (int64_t)b->ld, k, (const double*)b->h, out, b->n,
+                           (const int32_t*)nullptr);
         return check(hipGetLastError(), "low-rank update");
     }
     rc = check(k_dot_cols(b->D, b->ld, k, e, b->n, b->partial, b->G, b->stream), "D^T e");
@@ -703,9 +755,10 @@ int tfs_broyden_create(const tfs_broyden_desc* desc, tfs_broyden** out) {
         b->own_stream = e == hipSuccess;
     }
     const int64_t parts[] = {b->ld * D.max_iter, b->ld * D.max_iter, b->ld, b->ld, b->ld, D.max_iter + 1,
-                             std::max<int64_t>((int64_t)b->G * (D.max_iter + 1), 2LL * b->RB), 8, mats, mats};
+                             std::max<int64_t>((int64_t)b->G * (D.max_iter + 1), 2LL * b->RB), 8, mats, mats,
+                             b->ld, b->ld};
     double** dst[] = {&b->C, &b->D, &b->slot[0], &b->slot[1], &b->t, &b->h, &b->partial, &b->scal, &b->J,
-                      &b->Jstage};
+                      &b->Jstage, &b->tC, &b->tD};
     int64_t total = 0;
     for (int64_t l : parts) total += round32(l);
     if (e == hipSuccess) e = hipMalloc(&b->arena, total * 8);
@@ -714,6 +767,7 @@ int tfs_broyden_create(const tfs_broyden_desc* desc, tfs_broyden** out) {
     if (e == hipSuccess) e = hipMalloc(&b->sizes, sizeof(int32_t) * D.num_blocks);
     if (e == hipSuccess) e = hipMalloc(&b->piv, sizeof(int32_t) * D.n);
     if (e == hipSuccess) e = hipMalloc(&b->status, sizeof(int32_t) * D.num_blocks);
+    if (e == hipSuccess) e = hipMalloc(&b->d_iter, sizeof(int32_t));
     if (e == hipSuccess) e = hipHostMalloc((void**)&b->host_scal, 64, hipHostMallocDefault);
     if (e == hipSuccess) {
         double* p = b->arena;
@@ -747,6 +801,7 @@ int tfs_broyden_destroy(tfs_broyden* b) {
     (void)hipFree(b->sizes);
     (void)hipFree(b->piv);
     (void)hipFree(b->status);
+    if (b->d_iter) (void)hipFree(b->d_iter);
     if (b->cpl_arena) (void)hipFree(b->cpl_arena);
     if (b->cpl_disp) (void)hipFree(b->cpl_disp);
     if (b->host_scal) (void)hipHostFree(b->host_scal);
@@ -998,8 +1053,12 @@ int tfs_broyden_status(tfs_broyden* b, int32_t* iter, double* error_norm) {
     return TFS_OK;
 }
 
-int tfs_broyden_compute_errors(tfs_broyden* b, const tfs_broyden_state* st, int32_t slot,
-                               int32_t* error_is_too_large) {
+}  // extern "C"
+
+// Checks the state, stages host arrays (host memory), enqueues the residual
+// kernels and the copy of its two flags (too large, not finite) into
+// host_scal[0..1], without waiting.
+static int errors_enqueue(tfs_broyden* b, const tfs_broyden_state* st, int32_t slot, std::vector<void*>& staged) {
     if (!b || !st || (slot != 0 && slot != 1))
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_compute_errors: null argument or bad slot");
     if (st->num_swim < 1 || st->num_fil < 0 || st->num_seg < 0 || st->blobs_per_swim < 0 ||
@@ -1029,7 +1088,6 @@ int tfs_broyden_compute_errors(tfs_broyden* b, const tfs_broyden_state* st, int3
     a.dt = st->dt;
     a.pos_tol = st->dl * st->tol;
     a.lie_tol = st->tol;
-    std::vector<void*> staged;
     auto stage = [&](const double* src, int64_t count) -> const double* {
         if (!src || count == 0) return src;
         if (!host_mem(b)) return src;
@@ -1080,23 +1138,88 @@ int tfs_broyden_compute_errors(tfs_broyden* b, const tfs_broyden_state* st, int3
                            b->scal + 2);
         rc = check(hipGetLastError(), "residual");
     }
-    if (!rc) {
-        hipError_t e = hipMemcpyAsync(b->host_scal, b->scal + 2, 16, hipMemcpyDeviceToHost, b->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
-        if (e != hipSuccess) rc = hip_fail(e, "tfs_broyden_compute_errors");
-    } else {
-        (void)hipStreamSynchronize(b->stream);
-    }
-    for (void* p : staged) (void)hipFree(p);
-    if (rc) return rc;
+    if (!rc) rc = check(hipMemcpyAsync(b->host_scal, b->scal + 2, 16, hipMemcpyDeviceToHost, b->stream),
+                        "tfs_broyden_compute_errors");
+    return rc;
+}
+
+// The flags errors_enqueue copied, once the stream has been waited for.
+static int errors_result(tfs_broyden* b, int32_t* error_is_too_large) {
     if (error_is_too_large) *error_is_too_large = b->host_scal[0] > 1.0 ? 1 : 0;
     if (b->host_scal[1] != 0.0) return fail(TFS_ERR_NON_FINITE, "tfs_broyden_compute_errors: residual not finite");
     return TFS_OK;
 }
 
+extern "C" {
+
+int tfs_broyden_compute_errors(tfs_broyden* b, const tfs_broyden_state* st, int32_t slot,
+                               int32_t* error_is_too_large) {
+    std::vector<void*> staged;
+    int rc = errors_enqueue(b, st, slot, staged);
+    hipError_t e = b ? hipStreamSynchronize(b->stream) : hipSuccess;
+    for (void* p : staged) (void)hipFree(p);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "tfs_broyden_compute_errors");
+    return errors_result(b, error_is_too_large);
+}
+
 }  // extern "C"
 
 hipStream_t tfs::broyden_stream(const tfs_broyden* b) { return b->stream; }
+
+// ---- Device-counted iterations: every launch has fixed arguments, so a
+// caller can capture find_update and end_of_iter in hipGraphs ----
+
+bool tfs::broyden_dev_supported(const tfs_broyden* b) { return !host_mem(b) && b->n <= kWaveDotsMax; }
+
+int tfs::broyden_dev_begin(tfs_broyden* b) {
+    hipLaunchKernelGGL(set_count_kernel, dim3(1), dim3(64), 0, b->stream, b->d_iter, (int32_t)b->iter);
+    return check(hipGetLastError(), "device iteration count");
+}
+
+// inverse_update with the iteration count read on the device:
+// out = -c J0^{-1} e - C(:, 0:k) (D(:, 0:k)^T e), k = min(*d_iter, max_iter - 1).
+static int inverse_update_dev(tfs_broyden* b, const double* e, double* out) {
+    int rc = apply_jinv(b, e, out, -b->desc.confidence);
+    const int kmax = b->max_iter - 1;
+    if (rc || kmax < 1) return rc;
+    hipLaunchKernelGGL(lowrank_dots_kernel, dim3((unsigned)kmax), dim3(256), 0, b->stream, (const double*)b->D,
+                       (int64_t)b->ld, kmax, e, b->n, b->h, (const int32_t*)b->d_iter);
+    hipLaunchKernelGGL(lowrank_gemv_kernel, dim3((unsigned)((b->n + 255) / 256)), dim3(256), 0, b->stream,
+                       (const double*)b->C, (int64_t)b->ld, kmax, (const double*)b->h, out, b->n,
+                       (const int32_t*)b->d_iter);
+    return check(hipGetLastError(), "low-rank update");
+}
+
+int tfs::broyden_dev_find_update(tfs_broyden* b, double* update) { return inverse_update_dev(b, b->slot[0], update); }
+
+int tfs::broyden_dev_end_of_iter(tfs_broyden* b) {
+    const int n = b->n;
+    int rc = inverse_update_dev(b, b->slot[1], b->tC);
+    if (rc) return rc;
+    hipLaunchKernelGGL(diff_sumsq_kernel, dim3((unsigned)b->G), dim3(256), 0, b->stream, (const double*)b->slot[1],
+                       (const double*)b->slot[0], b->tD, n, b->partial);
+    hipLaunchKernelGGL(commit_update_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream,
+                       (const double*)b->tC, (const double*)b->tD, b->C, b->D, (int64_t)b->ld, n, b->slot[0],
+                       (const double*)b->slot[1], (const double*)b->partial, b->G, (const double*)(b->scal + 2),
+                       (const int32_t*)b->d_iter, b->max_iter);
+    hipLaunchKernelGGL(advance_count_kernel, dim3(1), dim3(64), 0, b->stream, b->d_iter);
+    return check(hipGetLastError(), "end of iteration");
+}
+
+void tfs::broyden_dev_finish(tfs_broyden* b, int32_t iter, int32_t latest) {
+    b->iter = iter;
+    b->latest = latest;
+}
+
+int tfs::broyden_errors_enqueue(tfs_broyden* b, const tfs_broyden_state* st, int32_t slot) {
+    std::vector<void*> staged;  // device memory: nothing is staged
+    int rc = errors_enqueue(b, st, slot, staged);
+    for (void* p : staged) (void)hipFree(p);
+    return rc;
+}
+
+int tfs::broyden_errors_result(tfs_broyden* b, int32_t* too_large) { return errors_result(b, too_large); }
 
 int tfs::broyden_latest_norm(tfs_broyden* b, double* out) {
     int rc = check(k_norm(b->slot[b->latest], b->n, b->partial, b->G, b->scal, b->stream), "||residual||");

@@ -1493,6 +1493,8 @@ struct tfs_elastic {
     tfs_rpy* rpy = nullptr;
     double *v_blobs = nullptr, *fbm1 = nullptr, *fbm2 = nullptr, *bfe_blocks = nullptr, *bfe_du = nullptr;
     double* bdisp = nullptr;  // free bodies in that form: the blobs' displacements Q_init ref
+    bool graphs = true;       // Broyden iterations as captured hipGraphs where the handles allow
+    int64_t graph_iters = 0, eager_iters = 0;
 };
 
 int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
@@ -1686,8 +1688,9 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
     auto order = [](hipStream_t from, hipStream_t to) -> int {
         return from == to ? TFS_OK : check(hipStreamSynchronize(from), "tfs_elastic_step: stream order");
     };
-    // forces -> velocities -> residual in `slot`
-    auto evaluate = [&](int slot, int32_t* too_large) -> int {
+    // forces -> velocities -> residual in `slot` (enqueue_only: the residual's
+    // flags are left for tfs::broyden_errors_result, nothing waits)
+    auto evaluate = [&](int slot, int32_t* too_large, bool enqueue_only = false) -> int {
         int rc = e->free_bodies ? tfs_filament_blob_positions(fh, e->x_blobs) : TFS_OK;
         if (!rc) rc = tfs_filament_forces(fh, nt, e->f_segs);
         if (!rc) rc = order(s_fil, s_vel);
@@ -1703,10 +1706,14 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
         else if (!rc)
             rc = tfs::velocities_compute_ordered(D.velocities, fh->S.x, e->f_segs, D.x_blobs, D.x_bodies,
                                                  D.v_bodies, nullptr, e->v_segs, D.f_blobs, nullptr, &vs, false);
-        S.gmres_iterations += vs.iterations;
-        S.velocity_solves++;
+        if (!enqueue_only) {
+            S.gmres_iterations += vs.iterations;
+            S.velocity_solves++;
+        }
         if (!rc) rc = order(s_vel, s_bro);
-        if (!rc) rc = tfs_broyden_compute_errors(D.broyden, &st, slot, too_large);
+        if (!rc)
+            rc = enqueue_only ? tfs::broyden_errors_enqueue(D.broyden, &st, slot)
+                              : tfs_broyden_compute_errors(D.broyden, &st, slot, too_large);
         return rc;
     };
     // The near-pair lists rebuilt in an evaluation have their capacity checked
@@ -1780,31 +1787,104 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
         }
     }
     if (!rc) rc = tfs_broyden_start(D.broyden, nullptr);
-    int32_t iter = 0;
-    while (!rc && big && iter < D.max_iter) {
+    // find_update -> the state's update (device_count: the engine's iteration
+    // count read on the device)
+    auto advance = [&](bool device_count) -> int {
+        int rc2;
         if (e->rpy) {
             // the filament and body entries to the filament layout, the blob
             // entries onto the blob forces
-            rc = tfs_broyden_find_update(D.broyden, e->bfe_du);
-            if (!rc) rc = order(s_bro, s_fil);
-            for (int64_t s = 0; s < nswim && !rc; ++s)
-                rc = check(hipMemcpyAsync(e->du + s * (fil * n6 + 6), e->bfe_du + s * e->per, 8 * (fil * n6 + 6),
-                                          hipMemcpyDeviceToDevice, fh->stream),
-                           "update");
-            if (!rc && nb3) {
+            rc2 = device_count ? tfs::broyden_dev_find_update(D.broyden, e->bfe_du)
+                               : tfs_broyden_find_update(D.broyden, e->bfe_du);
+            if (!rc2) rc2 = order(s_bro, s_fil);
+            for (int64_t s = 0; s < nswim && !rc2; ++s)
+                rc2 = check(hipMemcpyAsync(e->du + s * (fil * n6 + 6), e->bfe_du + s * e->per, 8 * (fil * n6 + 6),
+                                           hipMemcpyDeviceToDevice, fh->stream),
+                            "update");
+            if (!rc2 && nb3) {
                 hipLaunchKernelGGL(blob_force_update_kernel, dim3(grid_for(nswim * nb3, 256)), dim3(256), 0,
                                    fh->stream, D.f_blobs, (const double*)e->bfe_du, nb3, (int)nswim, e->per,
                                    fil * n6 + 6);
-                rc = check(hipGetLastError(), "blob force update");
+                rc2 = check(hipGetLastError(), "blob force update");
             }
         } else {
-            rc = tfs_broyden_find_update(D.broyden, e->du);
+            rc2 = device_count ? tfs::broyden_dev_find_update(D.broyden, e->du)
+                               : tfs_broyden_find_update(D.broyden, e->du);
         }
-        if (!rc && !e->rpy) rc = order(s_bro, s_fil);
-        if (!rc) rc = tfs_filament_update(fh, e->du);
+        if (!rc2 && !e->rpy) rc2 = order(s_bro, s_fil);
+        if (!rc2) rc2 = tfs_filament_update(fh, e->du);
+        return rc2;
+    };
+    int32_t iter = 0;
+    // Captured iterations: one stream for every handle, device memory, fixed
+    // launches (FP64 pair blocks, the dense direct blob solve or the plain
+    // matvec), an engine small enough for its device-counted form.  Graph 1 is
+    // find_update -> update -> forces -> velocities -> residual, graph 2 the
+    // engine's end of iteration; the host waits once per iteration, for the
+    // residual's flags and the near lists' capacity check.
+    const bool graph_mode = !rc && big && e->graphs && s_fil == s_vel && s_vel == s_bro &&
+                            tfs::broyden_dev_supported(D.broyden) && rpyh->desc.precision == TFS_PRECISION_FP64 &&
+                            rpyh->desc.memory == TFS_MEMORY_DEVICE &&
+                            (e->rpy || tfs::velocities_graph_ready(D.velocities));
+    if (graph_mode) {
+        struct Graphs {
+            hipGraph_t g[2] = {nullptr, nullptr};
+            hipGraphExec_t x[2] = {nullptr, nullptr};
+            void clear() {
+                for (int i = 0; i < 2; ++i) {
+                    if (x[i]) (void)hipGraphExecDestroy(x[i]);
+                    if (g[i]) (void)hipGraphDestroy(g[i]);
+                    x[i] = nullptr;
+                    g[i] = nullptr;
+                }
+            }
+            ~Graphs() { clear(); }
+        } G;
+        auto capture = [&](int which) -> int {
+            hipError_t ce = hipStreamBeginCapture(s_fil, hipStreamCaptureModeThreadLocal);
+            if (ce != hipSuccess) return hip_fail(ce, "tfs_elastic_step: begin capture");
+            int rc2 = which == 0 ? advance(true) : tfs::broyden_dev_end_of_iter(D.broyden);
+            if (!rc2 && which == 0) rc2 = evaluate(1, nullptr, true);
+            hipError_t ee = hipStreamEndCapture(s_fil, &G.g[which]);
+            if (rc2) return rc2;
+            if (ee != hipSuccess) return hip_fail(ee, "tfs_elastic_step: end capture");
+            ce = hipGraphInstantiate(&G.x[which], G.g[which], nullptr, nullptr, 0);
+            return ce == hipSuccess ? TFS_OK : hip_fail(ce, "tfs_elastic_step: instantiate");
+        };
+        rc = tfs::broyden_dev_begin(D.broyden);
+        int32_t latest = 0;
+        while (!rc && big && iter < D.max_iter) {
+            if (!G.x[0]) rc = capture(0);
+            if (!rc && !G.x[1]) rc = capture(1);
+            if (!rc) rc = check(hipGraphLaunch(G.x[0], s_fil), "tfs_elastic_step: iteration graph");
+            if (!rc) rc = check(hipStreamSynchronize(s_fil), "tfs_elastic_step: iteration graph");
+            S.velocity_solves++;
+            if (!rc) rc = tfs::broyden_errors_result(D.broyden, &big);
+            bool overflow = false;
+            if (!rc) {
+                rpyh->near_pending[0] = true;
+                rc = tfs::rpy_near_check(rpyh, s_fil, &overflow);
+            }
+            if (!rc && overflow) {
+                // grown lists: the evaluation again, eagerly, and new graphs
+                G.clear();
+                rc = evaluate_checked(1, &big);
+            }
+            if (!rc)
+                rc = G.x[1] ? check(hipGraphLaunch(G.x[1], s_fil), "tfs_elastic_step: end-of-iteration graph")
+                            : tfs::broyden_dev_end_of_iter(D.broyden);
+            latest = big && iter < D.max_iter - 1 ? 0 : 1;
+            ++iter;
+            e->graph_iters++;
+        }
+        if (!rc) tfs::broyden_dev_finish(D.broyden, iter, latest);
+    }
+    while (!graph_mode && !rc && big && iter < D.max_iter) {
+        rc = advance(false);
         if (!rc) rc = evaluate_checked(1, &big);
         if (!rc) rc = tfs_broyden_end_of_iter(D.broyden, nullptr, big);
         if (!rc) rc = tfs_broyden_status(D.broyden, &iter, nullptr);
+        e->eager_iters++;
     }
     double norm = 0.0;
     if (!rc) rc = tfs::broyden_latest_norm(D.broyden, &norm);
@@ -1813,6 +1893,19 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
     S.converged = (!rc && !big) ? 1 : 0;
     if (rc) return rc;
     return big ? fail(TFS_ERR_NOT_CONVERGED, "tfs_elastic_step: Broyden iterations exhausted") : TFS_OK;
+}
+
+int tfs_elastic_set_graphs(tfs_elastic* e, int32_t enable) {
+    if (!e) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_set_graphs: null handle");
+    e->graphs = enable != 0;
+    return TFS_OK;
+}
+
+int tfs_elastic_iteration_counts(tfs_elastic* e, int64_t* graph_iterations, int64_t* eager_iterations) {
+    if (!e) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_elastic_iteration_counts: null handle");
+    if (graph_iterations) *graph_iterations = e->graph_iters;
+    if (eager_iterations) *eager_iterations = e->eager_iters;
+    return TFS_OK;
 }
 
 }  // extern "C"

@@ -237,6 +237,7 @@ __global__ __launch_bounds__(256) void to_fixed_kernel(const double* __restrict_
 // The body table on the device (BodyRange records, plus room for 6 doubles
 // per body), uploaded on the handle's stream from h->bodies.
 int ensure_body_table(tfs_rpy* h) {
+    if (h->body_table_valid) return TFS_OK;
     const size_t nbody = h->bodies.size();
     hipError_t e = hipSuccess;
     if (h->body_cap < nbody) {
@@ -250,7 +251,10 @@ int ensure_body_table(tfs_rpy* h) {
     }
     if (e != hipSuccess) return fail(TFS_ERR_OUT_OF_MEMORY, "body table");
     e = hipMemcpyAsync(h->d_body_range, h->bodies.data(), sizeof(int4) * nbody, hipMemcpyHostToDevice, h->stream);
-    return e == hipSuccess ? TFS_OK : hip_fail(e, "body table");
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);  // the host table may change after return
+    if (e != hipSuccess) return hip_fail(e, "body table");
+    h->body_table_valid = true;
+    return TFS_OK;
 }
 
 // The fixed-point frame of the bodies' bounding boxes (6 per body: min xyz,
@@ -838,6 +842,7 @@ int tfs_rpy_set_bodies(tfs_rpy* h, int32_t num_bodies, const int32_t* segs_per_b
     free_plans(h);
     free_persist(h);
     h->bodies = bodies;
+    h->body_table_valid = false;
     invalidate_near(h);
     int lns = 0, lnb = 0;
     largest_body(h, &lns, &lnb);

@@ -599,7 +599,7 @@ int compute(tfs_velocities* v, const double* x_segs, const double* f_segs, const
                            h->v_blobs, nb, (int)v->desc.segs_per_swim, v->d_bodies,
                            bodies ? v->d_bodies + 3 * v->nswim : nullptr, v->d_rhs);
         rc = check(hipGetLastError(), "free-body rhs");
-        if (!rc) rc = tfs_solver_set_bodies(v->blobs, v->d_bodies, TFS_MEMORY_DEVICE);
+        if (!rc && !v->direct) rc = tfs_solver_set_bodies(v->blobs, v->d_bodies, TFS_MEMORY_DEVICE);
     } else if (!rc) {
         // the direct solve reads its right-hand side from the padded vector
         hipLaunchKernelGGL(blob_rhs_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, h->v_blobs,
@@ -758,6 +758,11 @@ int tfs::velocities_compute_ordered(tfs_velocities* v, const double* x_segs, con
 hipStream_t tfs::velocities_stream(const tfs_velocities* v) { return v->stream; }
 
 tfs_rpy* tfs::velocities_rpy(const tfs_velocities* v) { return v->rpy; }
+
+bool tfs::velocities_graph_ready(const tfs_velocities* v) {
+    return v->direct && v->desc.memory == TFS_MEMORY_DEVICE && v->have_blobs && v->blob_motion != TFS_BLOBS_CHECK &&
+           (v->desc.free_bodies ? v->free_direct_valid : v->direct_valid);
+}
 
 extern "C" {
 

@@ -42,6 +42,8 @@ L.extra_signatures({
     "tfs_elastic_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_vp)]),
     "tfs_elastic_destroy": (ctypes.c_int, [_vp]),
     "tfs_elastic_step": (ctypes.c_int, [_vp, _i32, ctypes.c_void_p]),
+    "tfs_elastic_set_graphs": (ctypes.c_int, [_vp, _i32]),
+    "tfs_elastic_iteration_counts": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "tfs_filament_device_array": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_vp)]),
     "tfs_filament_read": (ctypes.c_int, [_vp, _i32, _vp]),
     "tfs_filament_set_body": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _vp]),
@@ -259,6 +261,16 @@ class ElasticStep:
         if v.size != n.value:
             raise ValueError(f"backup holds {v.size} values, this step needs {n.value}")
         L.check(L.lib().tfs_elastic_restore(self.h, L.ptr(v)))
+
+    def set_graphs(self, enable: bool):
+        """Captured Broyden iterations on (default) or off (tfs_elastic_set_graphs)."""
+        L.check(L.lib().tfs_elastic_set_graphs(self.h, int(bool(enable))))
+
+    def iteration_counts(self):
+        """(iterations replayed as graphs, iterations launched one by one)."""
+        g, e = ctypes.c_int64(), ctypes.c_int64()
+        L.check(L.lib().tfs_elastic_iteration_counts(self.h, ctypes.byref(g), ctypes.byref(e)))
+        return g.value, e.value
 
     def step(self, nt, *, allow_unconverged=False):
         st = ElasticStats()
