@@ -142,12 +142,29 @@ hipStream_t broyden_stream(const tfs_broyden* b);
 bool broyden_dev_supported(const tfs_broyden* b);
 int broyden_dev_begin(tfs_broyden* b);
 int broyden_dev_find_update(tfs_broyden* b, double* update);
-int broyden_dev_end_of_iter(tfs_broyden* b);
+// pending (device, may be null): the end of iteration runs only while it is
+// nonzero, and clears it.
+int broyden_dev_end_of_iter(tfs_broyden* b, int32_t* pending);
 void broyden_dev_finish(tfs_broyden* b, int32_t iter, int32_t latest);
-// tfs_broyden_compute_errors in two halves: the launches and the copy of the
-// flags (no wait), and the flags once the stream has been waited for.
-int broyden_errors_enqueue(tfs_broyden* b, const tfs_broyden_state* st, int32_t slot);
+// tfs_broyden_compute_errors in two halves: the launches and (copy_flags)
+// the copy of the flags to the host (no wait), and the flags once the stream
+// has been waited for; broyden_copy_error_flags enqueues that copy alone.
+int broyden_errors_enqueue(tfs_broyden* b, const tfs_broyden_state* st, int32_t slot, bool copy_flags = true);
+int broyden_copy_error_flags(tfs_broyden* b);
 int broyden_errors_result(tfs_broyden* b, int32_t* too_large);
+// Device address of the residual's two flags: its too-large measure (> 1:
+// too large) and nonzero if it was not finite.
+const double* broyden_error_flags(const tfs_broyden* b);
+// While flags is set, every residual's last kernel also counts one more
+// iteration in flags[0] unless flags[2] (stopped) is set, marks its end of
+// iteration pending in flags[1], and sets flags[2] to 1 once the residual is
+// small enough or max_iter iterations have run, 2 if it is not finite.
+void broyden_set_bookkeeping(tfs_broyden* b, int32_t* flags, int max_iter);
+// The too-large measure of the residual whose flags were last read (> 1: too large).
+double broyden_last_measure(const tfs_broyden* b);
+// Grows the near-pair lists of `precision` to at least `cap` entries per
+// particle (rebuilt at the next evaluation).
+int rpy_near_reserve(tfs_rpy* h, int precision, int cap);
 
 extern thread_local std::string g_last_error;
 int fail(int code, const std::string& msg);

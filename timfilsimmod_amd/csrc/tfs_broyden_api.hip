@@ -189,22 +189,22 @@ __global__ __launch_bounds__(256) void coupling_apply_kernel(const double* __res
     }
 }
 
-// As block_gemv_kernel for blocks of 64 rows or more: 512 threads per block,
-// four column quarters of 128 rows each, the quarters added in order through
-// LDS.
-__global__ __launch_bounds__(512) void block_gemv_wide_kernel(const double* __restrict__ A,
-                                                              const int64_t* __restrict__ mat_off,
-                                                              const int32_t* __restrict__ row_off,
-                                                              const int32_t* __restrict__ sizes,
-                                                              const double* __restrict__ x, double* __restrict__ y,
-                                                              double alpha) {
-    __shared__ double part[4][128];
+// As block_gemv_kernel for blocks of 64 rows or more: 1024 threads per
+// block, eight column groups of 128 rows each, the groups added in a fixed
+// order through LDS.
+__global__ __launch_bounds__(1024) void block_gemv_wide_kernel(const double* __restrict__ A,
+                                                               const int64_t* __restrict__ mat_off,
+                                                               const int32_t* __restrict__ row_off,
+                                                               const int32_t* __restrict__ sizes,
+                                                               const double* __restrict__ x, double* __restrict__ y,
+                                                               double alpha) {
+    __shared__ double part[8][128];
     const int b = blockIdx.x;
     const int bs = sizes[b];
     const double* Ab = A + mat_off[b];
     const int r0 = row_off[b];
     const int rl = threadIdx.x & 127, q = threadIdx.x >> 7;
-    const int cq = (bs + 3) / 4;
+    const int cq = (bs + 7) / 8;
     const int c0 = q * cq, c1 = c0 + cq < bs ? c0 + cq : bs;
     for (int rb = 0; rb < bs; rb += 128) {
         const int r = rb + rl;
@@ -225,7 +225,9 @@ __global__ __launch_bounds__(512) void block_gemv_wide_kernel(const double* __re
         }
         part[q][rl] = s0 + s1;
         __syncthreads();
-        if (q == 0 && r < bs) y[r0 + r] = alpha * ((part[0][rl] + part[1][rl]) + (part[2][rl] + part[3][rl]));
+        if (q == 0 && r < bs)
+            y[r0 + r] = alpha * (((part[0][rl] + part[1][rl]) + (part[2][rl] + part[3][rl])) +
+                                 ((part[4][rl] + part[5][rl]) + (part[6][rl] + part[7][rl])));
         __syncthreads();
     }
 }
@@ -292,11 +294,12 @@ __global__ __launch_bounds__(256) void commit_update_kernel(const double* __rest
                                                             const double* __restrict__ slot1,
                                                             const double* __restrict__ partial, int G,
                                                             const double* __restrict__ gate,
-                                                            const int32_t* __restrict__ d_iter, int max_iter) {
+                                                            const int32_t* __restrict__ d_iter, int max_iter,
+                                                            const int32_t* __restrict__ pending) {
     __shared__ double sh[4];
     __shared__ double inv;
     const int it = *d_iter;
-    if (!(gate[0] > 1.0) || it >= max_iter - 1) return;  // uniform
+    if ((pending && !*pending) || !(gate[0] > 1.0) || it >= max_iter - 1) return;  // uniform
     double s = 0.0;
     for (int b = threadIdx.x; b < G; b += 256) s += partial[b];
 #pragma unroll
@@ -316,12 +319,65 @@ __global__ __launch_bounds__(256) void commit_update_kernel(const double* __rest
     }
 }
 
+// The end of iteration for up to kEndSmall unknowns in one workgroup:
+// tD = slot1 - slot0 and its norm, the commit of C(:, it) = tC / ||tD||,
+// D(:, it) = tD / ||tD|| and slot0 = slot1.  Device-counted (d_iter set): the
+// commit is gated by the residual's flag and the count, the count advances,
+// and nothing runs unless an iteration is pending; otherwise iteration it_fixed
+// commits unconditionally.  The norm sums 1024 per-thread partials in a fixed
+// tree.
+__global__ __launch_bounds__(1024) void end_small_kernel(const double* __restrict__ tC, double* __restrict__ tD,
+                                                         double* __restrict__ C, double* __restrict__ D,
+                                                         int64_t ld, int n, double* __restrict__ slot0,
+                                                         const double* __restrict__ slot1,
+                                                         const double* __restrict__ gate, int32_t* __restrict__ d_iter,
+                                                         int it_fixed, int max_iter, int32_t* __restrict__ pending) {
+    __shared__ double sh[16];
+    __shared__ double inv;
+    const int tid = (int)threadIdx.x;
+    if (pending && !*pending) return;  // uniform
+    const int it = d_iter ? *d_iter : it_fixed;
+    const bool commit = d_iter ? gate[0] > 1.0 && it < max_iter - 1 : true;
+    double s = 0.0;
+    for (int r = tid; r < n; r += 1024) {
+        const double d = slot1[r] - slot0[r];
+        tD[r] = d;
+        s += d * d;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((tid & 63) == 0) sh[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < 16; ++w) t += sh[w];
+        const double nrm = __builtin_sqrt(t);
+        inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+    }
+    __syncthreads();
+    if (commit) {
+        for (int r = tid; r < n; r += 1024) {
+            C[(int64_t)it * ld + r] = tC[r] * inv;
+            D[(int64_t)it * ld + r] = tD[r] * inv;
+            slot0[r] = slot1[r];
+        }
+    }
+    __syncthreads();  // every thread has read *d_iter
+    if (tid == 0 && d_iter) {
+        *d_iter = it + 1;
+        if (pending) *pending = 0;
+    }
+}
+
 __global__ void set_count_kernel(int32_t* __restrict__ c, int32_t v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *c = v;
 }
 
-__global__ void advance_count_kernel(int32_t* __restrict__ c) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *c += 1;
+// One more iteration, if one is pending (which it then consumes).
+__global__ void advance_count_kernel(int32_t* __restrict__ c, int32_t* __restrict__ pending) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || (pending && !*pending)) return;
+    *c += 1;
+    if (pending) *pending = 0;
 }
 
 // D_k = s1 - s0 and partial[b] = sum over workgroup b's rows of D_k^2.
@@ -585,8 +641,10 @@ __global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a, double* _
 }
 
 // out[0] = max ratio, out[1] = nonfinite flag over nb partial pairs (one block).
+// (book: see tfs::broyden_set_bookkeeping, may be null)
 __global__ __launch_bounds__(256) void residual_finish_kernel(const double* __restrict__ partial, int nb,
-                                                              double* __restrict__ out) {
+                                                              double* __restrict__ out, int32_t* __restrict__ book,
+                                                              int book_max) {
     __shared__ double sm[256];
     __shared__ double sn[256];
     double m = 0.0, f = 0.0;
@@ -607,6 +665,15 @@ __global__ __launch_bounds__(256) void residual_finish_kernel(const double* __re
     if (threadIdx.x == 0) {
         out[0] = sm[0];
         out[1] = sn[0];
+        // one more iteration, unless the iterations had stopped; stop once the
+        // residual is small enough, not finite, or book_max iterations have run
+        if (book && !book[2]) {
+            const bool nonfinite = sn[0] != 0.0;
+            book[0] += 1;
+            book[1] = nonfinite ? 0 : 1;
+            if (nonfinite) book[2] = 2;
+            else if (!(sm[0] > 1.0) || book[0] >= book_max) book[2] = 1;
+        }
     }
 }
 
@@ -636,6 +703,8 @@ struct tfs_broyden {
     // the device, the next C and D columns before their commit
     int32_t* d_iter = nullptr;
     double *tC = nullptr, *tD = nullptr;
+    int32_t* book = nullptr;  // iteration bookkeeping after each residual (tfs::broyden_set_bookkeeping)
+    int book_max = 0;
     double* host_scal = nullptr;
     std::vector<int32_t> host_status;
     std::vector<int32_t> host_sizes;
@@ -656,6 +725,8 @@ bool host_mem(const tfs_broyden* b) { return b->desc.memory == TFS_MEMORY_HOST; 
 
 // Up to this many unknowns the low-rank sums take one wave per stored column.
 constexpr int kWaveDotsMax = 65536;
+// Up to this many unknowns the device-counted end of iteration is one workgroup.
+constexpr int kEndSmall = 8192;
 
 hipMemcpyKind to_device(const tfs_broyden* b) {
     return host_mem(b) ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
@@ -663,7 +734,7 @@ hipMemcpyKind to_device(const tfs_broyden* b) {
 
 int apply_jinv(tfs_broyden* b, const double* x, double* y, double alpha) {
     if (b->max_block >= 64)
-        hipLaunchKernelGGL(block_gemv_wide_kernel, dim3(b->nblocks), dim3(512), 0, b->stream, b->J, b->mat_off,
+        hipLaunchKernelGGL(block_gemv_wide_kernel, dim3(b->nblocks), dim3(1024), 0, b->stream, b->J, b->mat_off,
                            b->row_off, b->sizes, x, y, alpha);
     else
         hipLaunchKernelGGL(block_gemv_kernel, dim3(b->nblocks), dim3(128), 0, b->stream, b->J, b->mat_off,
@@ -1014,6 +1085,22 @@ int tfs_broyden_end_of_iter(tfs_broyden* b, const double* new_error, int32_t err
         int rc = new_error ? copy_in(b, b->slot[1], new_error, n, "tfs_broyden_end_of_iter") : TFS_OK;
         double* Ck = b->C + (int64_t)b->iter * b->ld;
         double* Dk = b->D + (int64_t)b->iter * b->ld;
+        if (!rc && n <= kEndSmall) {
+            // one workgroup, as the device-counted form: slot 0 takes the new
+            // error by a copy
+            rc = inverse_update(b, b->slot[1], b->tC);
+            if (!rc) {
+                hipLaunchKernelGGL(end_small_kernel, dim3(1), dim3(1024), 0, b->stream, (const double*)b->tC, b->tD,
+                                   b->C, b->D, (int64_t)b->ld, n, b->slot[0], (const double*)b->slot[1],
+                                   (const double*)nullptr, (int32_t*)nullptr, b->iter, b->max_iter,
+                                   (int32_t*)nullptr);
+                rc = check(hipGetLastError(), "D_k, C_k / ||D_k||");
+            }
+            if (rc) return rc;
+            b->latest = 0;
+            b->iter++;
+            return TFS_OK;
+        }
         if (!rc) rc = inverse_update(b, b->slot[1], Ck);
         if (!rc) {
             // D_k = new - old, then C_k and D_k over ||D_k||
@@ -1056,9 +1143,10 @@ int tfs_broyden_status(tfs_broyden* b, int32_t* iter, double* error_norm) {
 }  // extern "C"
 
 // Checks the state, stages host arrays (host memory), enqueues the residual
-// kernels and the copy of its two flags (too large, not finite) into
-// host_scal[0..1], without waiting.
-static int errors_enqueue(tfs_broyden* b, const tfs_broyden_state* st, int32_t slot, std::vector<void*>& staged) {
+// kernels and (copy_flags) the copy of its two flags (too large, not finite)
+// into host_scal[0..1], without waiting.
+static int errors_enqueue(tfs_broyden* b, const tfs_broyden_state* st, int32_t slot, std::vector<void*>& staged,
+                          bool copy_flags = true) {
     if (!b || !st || (slot != 0 && slot != 1))
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_broyden_compute_errors: null argument or bad slot");
     if (st->num_swim < 1 || st->num_fil < 0 || st->num_seg < 0 || st->blobs_per_swim < 0 ||
@@ -1135,11 +1223,12 @@ static int errors_enqueue(tfs_broyden* b, const tfs_broyden_state* st, int32_t s
     if (!rc) {
         hipLaunchKernelGGL(residual_kernel, dim3(blocks), dim3(256), 0, b->stream, a, b->slot[slot], b->partial);
         hipLaunchKernelGGL(residual_finish_kernel, dim3(1), dim3(256), 0, b->stream, b->partial, blocks,
-                           b->scal + 2);
+                           b->scal + 2, b->book, b->book_max);
         rc = check(hipGetLastError(), "residual");
     }
-    if (!rc) rc = check(hipMemcpyAsync(b->host_scal, b->scal + 2, 16, hipMemcpyDeviceToHost, b->stream),
-                        "tfs_broyden_compute_errors");
+    if (!rc && copy_flags)
+        rc = check(hipMemcpyAsync(b->host_scal, b->scal + 2, 16, hipMemcpyDeviceToHost, b->stream),
+                   "tfs_broyden_compute_errors");
     return rc;
 }
 
@@ -1193,17 +1282,23 @@ static int inverse_update_dev(tfs_broyden* b, const double* e, double* out) {
 
 int tfs::broyden_dev_find_update(tfs_broyden* b, double* update) { return inverse_update_dev(b, b->slot[0], update); }
 
-int tfs::broyden_dev_end_of_iter(tfs_broyden* b) {
+int tfs::broyden_dev_end_of_iter(tfs_broyden* b, int32_t* pending) {
     const int n = b->n;
     int rc = inverse_update_dev(b, b->slot[1], b->tC);
     if (rc) return rc;
+    if (n <= kEndSmall) {
+        hipLaunchKernelGGL(end_small_kernel, dim3(1), dim3(1024), 0, b->stream, (const double*)b->tC, b->tD, b->C,
+                           b->D, (int64_t)b->ld, n, b->slot[0], (const double*)b->slot[1],
+                           (const double*)(b->scal + 2), b->d_iter, 0, b->max_iter, pending);
+        return check(hipGetLastError(), "end of iteration");
+    }
     hipLaunchKernelGGL(diff_sumsq_kernel, dim3((unsigned)b->G), dim3(256), 0, b->stream, (const double*)b->slot[1],
                        (const double*)b->slot[0], b->tD, n, b->partial);
     hipLaunchKernelGGL(commit_update_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream,
                        (const double*)b->tC, (const double*)b->tD, b->C, b->D, (int64_t)b->ld, n, b->slot[0],
                        (const double*)b->slot[1], (const double*)b->partial, b->G, (const double*)(b->scal + 2),
-                       (const int32_t*)b->d_iter, b->max_iter);
-    hipLaunchKernelGGL(advance_count_kernel, dim3(1), dim3(64), 0, b->stream, b->d_iter);
+                       (const int32_t*)b->d_iter, b->max_iter, (const int32_t*)pending);
+    hipLaunchKernelGGL(advance_count_kernel, dim3(1), dim3(64), 0, b->stream, b->d_iter, pending);
     return check(hipGetLastError(), "end of iteration");
 }
 
@@ -1212,14 +1307,28 @@ void tfs::broyden_dev_finish(tfs_broyden* b, int32_t iter, int32_t latest) {
     b->latest = latest;
 }
 
-int tfs::broyden_errors_enqueue(tfs_broyden* b, const tfs_broyden_state* st, int32_t slot) {
+int tfs::broyden_errors_enqueue(tfs_broyden* b, const tfs_broyden_state* st, int32_t slot, bool copy_flags) {
     std::vector<void*> staged;  // device memory: nothing is staged
-    int rc = errors_enqueue(b, st, slot, staged);
+    int rc = errors_enqueue(b, st, slot, staged, copy_flags);
     for (void* p : staged) (void)hipFree(p);
     return rc;
 }
 
 int tfs::broyden_errors_result(tfs_broyden* b, int32_t* too_large) { return errors_result(b, too_large); }
+
+const double* tfs::broyden_error_flags(const tfs_broyden* b) { return b->scal + 2; }
+
+void tfs::broyden_set_bookkeeping(tfs_broyden* b, int32_t* flags, int max_iter) {
+    b->book = flags;
+    b->book_max = max_iter;
+}
+
+double tfs::broyden_last_measure(const tfs_broyden* b) { return b->host_scal[0]; }
+
+int tfs::broyden_copy_error_flags(tfs_broyden* b) {
+    return check(hipMemcpyAsync(b->host_scal, b->scal + 2, 16, hipMemcpyDeviceToHost, b->stream),
+                 "residual flags");
+}
 
 int tfs::broyden_latest_norm(tfs_broyden* b, double* out) {
     int rc = check(k_norm(b->slot[b->latest], b->n, b->partial, b->G, b->scal, b->stream), "||residual||");

@@ -569,9 +569,10 @@ __global__ __launch_bounds__(64) void jacobian_kernel(FilState S, FilParams P, i
 // segment's Lie-algebra element and orientation, and its three multiplier
 // entries (segment 0 also the clamp's); the robot arm follows in
 // robot_arm_kernel.
-__global__ void update_kernel(FilState S, FilParams P, const double* __restrict__ du) {
+__global__ void update_kernel(FilState S, FilParams P, const double* __restrict__ du,
+                              const int32_t* __restrict__ stop) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= (int64_t)P.nfil * P.nseg) return;
+    if (g >= (int64_t)P.nfil * P.nseg || (stop && *stop)) return;
     const int N = P.nseg, n6 = 6 * N;
     const int f = (int)(g / N), i = (int)(g % N);
     const int swim = f / P.fil_per_swim, fs = f % P.fil_per_swim;
@@ -610,6 +611,33 @@ __global__ void robot_arm_kernel(FilState S, FilParams P) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= P.nfil) return;
     robot_arm(S, P, f);
+}
+
+// The robot arm for filaments of at most 64 segments: one wave per filament,
+// lane i the segment's step 0.5 DL (t_{i-1} + t_i), the positions x_0 plus an
+// inclusive scan of the steps over the lanes.
+__global__ __launch_bounds__(64) void robot_arm_wave_kernel(FilState S, FilParams P) {
+    const int f = blockIdx.x, i = threadIdx.x, N = P.nseg;
+    const int64_t s0 = (int64_t)f * N;
+    double t[3] = {0.0, 0.0, 0.0};
+    if (i < N) tangent(load_q(S.q + 4 * (s0 + i)), t);
+    double d[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double tp = __shfl_up(t[k], 1, 64);
+        d[k] = i >= 1 && i < N ? 0.5 * P.dl * (tp + t[k]) : 0.0;
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double v = __shfl_up(d[k], off, 64);
+            d[k] += i >= off ? v : 0.0;
+        }
+    if (i >= 1 && i < N) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) S.x[3 * (s0 + i) + k] = S.x[3 * s0 + k] + d[k];
+    }
 }
 
 // rigid_body::initial_guess (rigid_body.cpp:395-447), one thread per body.
@@ -662,9 +690,9 @@ __global__ void blob_force_guess_kernel(double* __restrict__ fb, double* __restr
 // rigid_body::update's blob part (rigid_body.cpp:506-520): swimmer s's blob
 // forces take the entries after its body's 6 in the Broyden update.
 __global__ void blob_force_update_kernel(double* __restrict__ fb, const double* __restrict__ du, int64_t nb3,
-                                         int nswim, int64_t per, int64_t off) {
+                                         int nswim, int64_t per, int64_t off, const int32_t* __restrict__ stop) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nb3 * nswim) return;
+    if (i >= nb3 * nswim || (stop && *stop)) return;
     const int64_t s = i / nb3, k = i % nb3;
     fb[i] += du[s * per + off + k];
 }
@@ -680,9 +708,10 @@ __global__ void body_rot_kernel(FilState S, int nswim) {
 
 // rigid_body::update (rigid_body.cpp:506-535) with the body's 6 entries of the
 // Broyden update (after its filaments' blocks).
-__global__ void body_update_kernel(FilState S, FilParams P, int nswim, const double* __restrict__ du) {
+__global__ void body_update_kernel(FilState S, FilParams P, int nswim, const double* __restrict__ du,
+                                   const int32_t* __restrict__ stop) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nswim) return;
+    if (b >= nswim || (stop && *stop)) return;
     const int64_t n6 = 6LL * P.nseg;
     const double* d = du + (int64_t)b * (P.fil_per_swim * n6 + 6) + P.fil_per_swim * n6;
 #pragma unroll
@@ -1037,6 +1066,9 @@ struct tfs_filament {
     int64_t nblob = 0;
     int32_t blobs_per_swim = 0;
     bool jacobian_done = false;
+    // while tfs_elastic_step captures its iterations: a device flag that
+    // turns the state's update into no-ops once the iterations have stopped
+    const int32_t* update_stop = nullptr;
 };
 
 namespace {
@@ -1267,12 +1299,16 @@ int tfs_filament_update(tfs_filament* h, const double* update) {
     }
     if (e == hipSuccess && h->S.bx) {
         hipLaunchKernelGGL(body_update_kernel, dim3(grid_for(h->desc.num_swim, 64)), dim3(64), 0, h->stream, h->S,
-                           h->P, (int)h->desc.num_swim, du);
+                           h->P, (int)h->desc.num_swim, du, h->update_stop);
         e = hipGetLastError();
     }
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(update_kernel, dim3(grid_for(num_segs(h), 256)), dim3(256), 0, h->stream, h->S, h->P, du);
-        hipLaunchKernelGGL(robot_arm_kernel, dim3(grid_for(h->P.nfil, 64)), dim3(64), 0, h->stream, h->S, h->P);
+        hipLaunchKernelGGL(update_kernel, dim3(grid_for(num_segs(h), 256)), dim3(256), 0, h->stream, h->S, h->P, du,
+                           h->update_stop);
+        if (h->P.nseg <= 64)
+            hipLaunchKernelGGL(robot_arm_wave_kernel, dim3((unsigned)h->P.nfil), dim3(64), 0, h->stream, h->S, h->P);
+        else
+            hipLaunchKernelGGL(robot_arm_kernel, dim3(grid_for(h->P.nfil, 64)), dim3(64), 0, h->stream, h->S, h->P);
         e = hipGetLastError();
     }
     if (e == hipSuccess && host) e = hipStreamSynchronize(h->stream);
@@ -1479,6 +1515,17 @@ int tfs_filament_body_coupling_blobs(tfs_filament* h, int32_t nt, const double* 
 
 // ---- elastic implicit timestep --------------------------------------------
 
+namespace {
+
+__global__ void set_flags_kernel(int32_t* __restrict__ c, int32_t runs, int32_t pending, int32_t stop) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    c[0] = runs;
+    c[1] = pending;
+    c[2] = stop;
+}
+
+}  // namespace
+
 struct tfs_elastic {
     tfs_elastic_desc desc{};
     double* arena = nullptr;
@@ -1495,6 +1542,11 @@ struct tfs_elastic {
     double* bdisp = nullptr;  // free bodies in that form: the blobs' displacements Q_init ref
     bool graphs = true;       // Broyden iterations as captured hipGraphs where the handles allow
     int64_t graph_iters = 0, eager_iters = 0;
+    // captured iterations' bookkeeping: [0] iterations run, [1] one awaits its
+    // end of iteration, [2] stopped (2: residual not finite); device and a
+    // pinned host image
+    int32_t* d_flags = nullptr;
+    int32_t* h_flags = nullptr;
 };
 
 int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
@@ -1539,7 +1591,10 @@ int tfs_elastic_create(const tfs_elastic_desc* desc, tfs_elastic** out) {
         {&e->bdisp, free_bodies && bfe ? 3 * nbt : 0}};
     int64_t total = 0;
     for (auto& p : parts) total += (p.second + 31) / 32 * 32;
-    if (hipMalloc(&e->arena, total * 8) != hipSuccess) {
+    if (hipMalloc(&e->arena, total * 8) != hipSuccess || hipMalloc(&e->d_flags, 4 * sizeof(int32_t)) != hipSuccess ||
+        hipHostMalloc((void**)&e->h_flags, 4 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+        if (e->arena) (void)hipFree(e->arena);
+        if (e->d_flags) (void)hipFree(e->d_flags);
         delete e;
         return fail(TFS_ERR_OUT_OF_MEMORY, "tfs_elastic_create: device memory");
     }
@@ -1611,6 +1666,8 @@ int tfs_elastic_destroy(tfs_elastic* e) {
     if (!e) return TFS_OK;
     if (e->rpy) tfs_rpy_destroy(e->rpy);
     if (e->arena) (void)hipFree(e->arena);
+    if (e->d_flags) (void)hipFree(e->d_flags);
+    if (e->h_flags) (void)hipHostFree(e->h_flags);
     delete e;
     return TFS_OK;
 }
@@ -1690,7 +1747,7 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
     };
     // forces -> velocities -> residual in `slot` (enqueue_only: the residual's
     // flags are left for tfs::broyden_errors_result, nothing waits)
-    auto evaluate = [&](int slot, int32_t* too_large, bool enqueue_only = false) -> int {
+    auto evaluate = [&](int slot, int32_t* too_large, bool enqueue_only = false, bool copy_flags = true) -> int {
         int rc = e->free_bodies ? tfs_filament_blob_positions(fh, e->x_blobs) : TFS_OK;
         if (!rc) rc = tfs_filament_forces(fh, nt, e->f_segs);
         if (!rc) rc = order(s_fil, s_vel);
@@ -1712,7 +1769,7 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
         }
         if (!rc) rc = order(s_vel, s_bro);
         if (!rc)
-            rc = enqueue_only ? tfs::broyden_errors_enqueue(D.broyden, &st, slot)
+            rc = enqueue_only ? tfs::broyden_errors_enqueue(D.broyden, &st, slot, copy_flags)
                               : tfs_broyden_compute_errors(D.broyden, &st, slot, too_large);
         return rc;
     };
@@ -1735,6 +1792,13 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
     defer.h = rpyh;
     defer.saved = rpyh->defer_near_check;
     rpyh->defer_near_check = true;
+    // captured iterations are replayed several per wait when no near list can
+    // overflow: a capacity of every particle (all-pairs lists, small bodies)
+    if (e->graphs && s_fil == s_vel && s_vel == s_bro && rpyh->desc.precision == TFS_PRECISION_FP64 &&
+        rpyh->desc.num_segs + rpyh->desc.num_blobs <= kNearPairsMax) {
+        const int rc0 = tfs::rpy_near_reserve(rpyh, 0, rpyh->desc.num_segs + rpyh->desc.num_blobs);
+        if (rc0) return rc0;
+    }
     auto evaluate_checked = [&](int slot, int32_t* too_large) -> int {
         for (int attempt = 0; attempt < 8; ++attempt) {
             int rc = evaluate(slot, too_large);
@@ -1804,7 +1868,7 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
             if (!rc2 && nb3) {
                 hipLaunchKernelGGL(blob_force_update_kernel, dim3(grid_for(nswim * nb3, 256)), dim3(256), 0,
                                    fh->stream, D.f_blobs, (const double*)e->bfe_du, nb3, (int)nswim, e->per,
-                                   fil * n6 + 6);
+                                   fil * n6 + 6, fh->update_stop);
                 rc2 = check(hipGetLastError(), "blob force update");
             }
         } else {
@@ -1818,10 +1882,17 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
     int32_t iter = 0;
     // Captured iterations: one stream for every handle, device memory, fixed
     // launches (FP64 pair blocks, the dense direct blob solve or the plain
-    // matvec), an engine small enough for its device-counted form.  Graph 1 is
-    // find_update -> update -> forces -> velocities -> residual, graph 2 the
-    // engine's end of iteration; the host waits once per iteration, for the
-    // residual's flags and the near lists' capacity check.
+    // matvec), an engine small enough for its device-counted form.  The
+    // captured iteration is the engine's end of the previous iteration (while
+    // one is pending), find_update, the state's update (a no-op once stopped),
+    // forces, velocities, residual and the bookkeeping that counts the
+    // iteration and stops the iterations; every other launch recomputes the
+    // same values from an unchanged state, so the iteration is replayed
+    // `batch` times per host wait.  A last graph ends the last iteration.
+    // With near lists of a capacity no count can exceed, the batch is the
+    // number of iterations the residual's observed rate of decrease still
+    // needs to reach the tolerance (1 to 8; replays past the stop cost one
+    // evaluation each); otherwise 1, with the capacity checked at each wait.
     const bool graph_mode = !rc && big && e->graphs && s_fil == s_vel && s_vel == s_bro &&
                             tfs::broyden_dev_supported(D.broyden) && rpyh->desc.precision == TFS_PRECISION_FP64 &&
                             rpyh->desc.memory == TFS_MEMORY_DEVICE &&
@@ -1843,41 +1914,76 @@ int tfs_elastic_step(tfs_elastic* e, int32_t nt, tfs_elastic_stats* stats) {
         auto capture = [&](int which) -> int {
             hipError_t ce = hipStreamBeginCapture(s_fil, hipStreamCaptureModeThreadLocal);
             if (ce != hipSuccess) return hip_fail(ce, "tfs_elastic_step: begin capture");
-            int rc2 = which == 0 ? advance(true) : tfs::broyden_dev_end_of_iter(D.broyden);
-            if (!rc2 && which == 0) rc2 = evaluate(1, nullptr, true);
+            fh->update_stop = e->d_flags + 2;
+            tfs::broyden_set_bookkeeping(D.broyden, e->d_flags, D.max_iter);
+            int rc2 = tfs::broyden_dev_end_of_iter(D.broyden, e->d_flags + 1);
+            if (!rc2 && which == 0) rc2 = advance(true);
+            if (!rc2 && which == 0) rc2 = evaluate(1, nullptr, true, false);
+            tfs::broyden_set_bookkeeping(D.broyden, nullptr, 0);
+            fh->update_stop = nullptr;
             hipError_t ee = hipStreamEndCapture(s_fil, &G.g[which]);
             if (rc2) return rc2;
             if (ee != hipSuccess) return hip_fail(ee, "tfs_elastic_step: end capture");
             ce = hipGraphInstantiate(&G.x[which], G.g[which], nullptr, nullptr, 0);
             return ce == hipSuccess ? TFS_OK : hip_fail(ce, "tfs_elastic_step: instantiate");
         };
+        const bool roomy = rpyh->near[0].cap >= rpyh->desc.num_segs + rpyh->desc.num_blobs;
         rc = tfs::broyden_dev_begin(D.broyden);
-        int32_t latest = 0;
-        while (!rc && big && iter < D.max_iter) {
+        if (!rc) {
+            hipLaunchKernelGGL(set_flags_kernel, dim3(1), dim3(64), 0, s_fil, e->d_flags, 0, 0, 0);
+            rc = check(hipGetLastError(), "tfs_elastic_step: bookkeeping");
+        }
+        int32_t runs = 0, stop = 0, runs_prev = -1;
+        double measure_prev = 0.0;
+        while (!rc && !stop) {
             if (!G.x[0]) rc = capture(0);
-            if (!rc && !G.x[1]) rc = capture(1);
-            if (!rc) rc = check(hipGraphLaunch(G.x[0], s_fil), "tfs_elastic_step: iteration graph");
-            if (!rc) rc = check(hipStreamSynchronize(s_fil), "tfs_elastic_step: iteration graph");
-            S.velocity_solves++;
-            if (!rc) rc = tfs::broyden_errors_result(D.broyden, &big);
-            bool overflow = false;
-            if (!rc) {
-                rpyh->near_pending[0] = true;
-                rc = tfs::rpy_near_check(rpyh, s_fil, &overflow);
+            const double measure = tfs::broyden_last_measure(D.broyden);
+            int batch = 1;
+            if (roomy && runs_prev >= 0 && runs > runs_prev && measure > 1.0 && measure < measure_prev) {
+                const double per_iter = std::log(measure_prev / measure) / (runs - runs_prev);
+                const double needed = std::log(measure) / per_iter;
+                batch = (int)std::max(1.0, std::min(8.0, std::floor(needed)));
             }
+            runs_prev = runs;
+            measure_prev = measure;
+            for (int r = 0; r < batch && !rc; ++r)
+                rc = check(hipGraphLaunch(G.x[0], s_fil), "tfs_elastic_step: iteration graph");
+            if (!rc)
+                rc = check(hipMemcpyAsync(e->h_flags, e->d_flags, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s_fil),
+                           "tfs_elastic_step: bookkeeping");
+            if (!rc) rc = tfs::broyden_copy_error_flags(D.broyden);
+            if (!rc) rc = check(hipStreamSynchronize(s_fil), "tfs_elastic_step: iteration graph");
+            if (rc) break;
+            const int32_t before = runs;
+            runs = e->h_flags[0];
+            stop = e->h_flags[2];
+            S.velocity_solves += runs - before;
+            e->graph_iters += runs - before;
+            rc = tfs::broyden_errors_result(D.broyden, &big);
+            if (rc) break;
+            bool overflow = false;
+            rpyh->near_pending[0] = !roomy;
+            rc = tfs::rpy_near_check(rpyh, s_fil, &overflow);
             if (!rc && overflow) {
-                // grown lists: the evaluation again, eagerly, and new graphs
+                // (one iteration per wait here) grown lists: the evaluation
+                // again, eagerly, its bookkeeping redone, and new graphs
                 G.clear();
                 rc = evaluate_checked(1, &big);
+                S.velocity_solves++;
+                stop = !big || runs >= D.max_iter ? 1 : 0;
+                if (!rc) {
+                    hipLaunchKernelGGL(set_flags_kernel, dim3(1), dim3(64), 0, s_fil, e->d_flags, runs, 1, stop);
+                    rc = check(hipGetLastError(), "tfs_elastic_step: bookkeeping");
+                }
             }
-            if (!rc)
-                rc = G.x[1] ? check(hipGraphLaunch(G.x[1], s_fil), "tfs_elastic_step: end-of-iteration graph")
-                            : tfs::broyden_dev_end_of_iter(D.broyden);
-            latest = big && iter < D.max_iter - 1 ? 0 : 1;
-            ++iter;
-            e->graph_iters++;
         }
-        if (!rc) tfs::broyden_dev_finish(D.broyden, iter, latest);
+        if (!rc && runs > 0) {
+            if (!G.x[1]) rc = capture(1);
+            if (!rc) rc = check(hipGraphLaunch(G.x[1], s_fil), "tfs_elastic_step: end-of-iteration graph");
+        }
+        iter = runs;
+        if (!rc) tfs::broyden_dev_finish(D.broyden, iter, 1);
+        if (!rc && stop == 2) rc = fail(TFS_ERR_NON_FINITE, "tfs_broyden_compute_errors: residual not finite");
     }
     while (!graph_mode && !rc && big && iter < D.max_iter) {
         rc = advance(false);

@@ -103,6 +103,10 @@ struct TileShape {
                      // split every LDS batch of sources between them
 };
 
+// Up to this many particles the near-pair lists test every pair (bit masks of
+// n^2 / 8 bytes) instead of sorting the particles into a hashed grid.
+constexpr int kNearPairsMax = 6144;
+
 // Targets per tile (and per partial slot) of a launch shape.
 inline int tile_targets(const TileShape& s) { return (s.split ? 64 : s.block) * s.per_thread; }
 

@@ -373,11 +373,13 @@ int ensure_near(tfs_rpy* h, int precision) {
             }
         }
         if (h->defer_near_check && nl.cnt) {
+            // a capacity of every particle cannot overflow: nothing to check
+            const bool roomy = nl.cap >= nt;
             hipError_t e = build_near_list(precision, a, h->grid, h->d_body_range, (int)h->bodies.size(), nl, nt,
-                                           nullptr, h->host_int + 8 + precision, h->stream);
+                                           nullptr, roomy ? nullptr : h->host_int + 8 + precision, h->stream);
             if (e != hipSuccess) return hip_fail(e, "near-pair lists");
             h->near_valid[precision] = true;
-            h->near_pending[precision] = true;
+            h->near_pending[precision] = !roomy;
             return TFS_OK;
         }
         int max_count = 0;
@@ -395,6 +397,16 @@ int ensure_near(tfs_rpy* h, int precision) {
         nl.cap = cap;
     }
     return fail(TFS_ERR_INVALID_ARGUMENT, "near-pair lists: too many overlapping neighbours");
+}
+
+int rpy_near_reserve(tfs_rpy* h, int precision, int cap) {
+    NearList& nl = h->near[precision];
+    if (nl.cap >= cap) return TFS_OK;
+    if (nl.cnt && hipStreamSynchronize(h->stream) != hipSuccess) return fail(TFS_ERR_HIP, "near-pair lists: synchronise");
+    free_near(nl);
+    nl.cap = cap;
+    h->near_valid[precision] = false;
+    return TFS_OK;
 }
 
 int rpy_near_check(tfs_rpy* h, hipStream_t waited, bool* overflow) {
