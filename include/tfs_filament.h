@@ -231,9 +231,11 @@ int tfs_elastic_step(tfs_elastic *e, int32_t nt, tfs_elastic_stats *stats);
  * is host code, cilia_sim_main.cu:394-502): with graphs on (the default),
  * when the three handles share one stream, the pair blocks are FP64 and the
  * blob forces come from the dense direct solve or the Broyden-for-everything
- * matvec, each step captures its iteration (find_update, update, forces,
- * velocities, residual) and the engine's end of iteration as two hipGraphs
- * and replays them with one host wait per iteration; results equal the
+ * matvec, each step captures one iteration (the engine's end of the previous
+ * iteration, find_update, update, forces, velocities, residual) as a hipGraph
+ * with its bookkeeping on the device, and replays it in batches of up to 8
+ * per host wait (the state's update is a no-op once the iterations have
+ * stopped; every other launch recomputes the same values); results equal the
  * launch-by-launch loop's bit for bit.  0: always launch by launch. */
 int tfs_elastic_set_graphs(tfs_elastic *e, int32_t enable);
 /* Broyden iterations run as graph replays and launch by launch since creation. */
