@@ -501,11 +501,11 @@ def copy_leg(torch, nbytes: int = 256 << 20, reps: int = 3):
     return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GB_s": 2 * nbytes / (ms * 1e-3) / 1e9}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_rpy_pairs_float3_production_body.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_rpy_pairs_sym_production_body.md")
 
 
 def measured_traffic():
-    """HBM bytes per launch of the FP32 <3,3> pair kernel, as recorded from the
+    """HBM bytes per launch of the FP32 <3,3> pair kernel (the symmetric form), as recorded from the
     two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; scaled by the copy
     calibration) in the committed profile summary; None if it is absent."""
     try:
@@ -725,7 +725,8 @@ def main():
         "elastic_step": elastic,
         "roofline": {"bound": "valu", "achieved": tf32, "peak": PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
                      "frac": tf32 / PEAK_FP32_VALU_TFLOPS, "traffic": measured_traffic(),
-                     "kernel": "rpy_pairs_kernel<float,3> (timestep matvec, 17.11 GFLOP per launch)",
+                     "kernel": "rpy_pairs_sym_kernel, FP32 <3,3> symmetric form (timestep matvec, 17.11 GFLOP "
+                               "of ordered-pair work per launch, each unordered pair evaluated once)",
                      "launches": k32_n, "matvecs_used": used32, "avg_ms": k32_ms / max(used32, 1),
                      "fp64_kernel": {"achieved": tf64, "peak": PEAK_FP64_VALU_TFLOPS,
                                      "frac": tf64 / PEAK_FP64_VALU_TFLOPS,

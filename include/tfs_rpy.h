@@ -181,6 +181,22 @@ int tfs_rpy_get_tile_shape(tfs_rpy *h, int32_t precision, int32_t shape[4]);
 int tfs_rpy_set_split_shape(tfs_rpy *h, int32_t precision, int32_t block, int32_t per_thread, int32_t chunk);
 /* 1 if the precision's current shape uses the source split. */
 int tfs_rpy_get_source_split(tfs_rpy *h, int32_t precision, int32_t *split);
+/* Symmetric form (no counterpart in the reference, which evaluates every
+ * ordered pair, cuda_functions.cu:362-757): the far-field tensor of a pair is
+ * the same seen from either particle, so each unordered pair is evaluated once
+ * and its term added to both particles' sums -- one 1/r, A and B for two
+ * terms.  64-particle tiles in groups of four per workgroup of 256 threads;
+ * results agree with the other shapes to FP32 rounding and are the same on
+ * every run.  FP32 tiles (precision TFS_PRECISION_FP32_ACC64), 3-DOF segments
+ * and one body only (refused otherwise); evaluations whose blocks lack SS, SB
+ * or BS, and the half-space domain, take the default source-split shape
+ * instead.  No persistent form.  The default shape of a one-body FP32 handle
+ * with 3-DOF segments whose groups of four tiles pair into at least four items
+ * per CU (e.g. 12,800 segments + 9,000 blobs: 3,741 items); enable = 0
+ * selects the source-split default instead. */
+int tfs_rpy_set_symmetric(tfs_rpy *h, int32_t precision, int32_t enable);
+/* 1 if the precision's current shape is the symmetric form. */
+int tfs_rpy_get_symmetric(tfs_rpy *h, int32_t precision, int32_t *enabled);
 /* Per-body completion times of the last launch (either form), in ms from its
  * first work item's start (enable before the launch; reading synchronises). */
 int tfs_rpy_body_timing(tfs_rpy *h, int32_t enable);
@@ -199,7 +215,8 @@ int tfs_graph_destroy(tfs_graph *g);
 /* Autotuner (no counterpart in the reference, whose launch shape is fixed at
  * THREADS_PER_BLOCK 64, config.hpp:294): times every launch shape (block 64/128/256,
  * 1 or 2 targets per thread, three chunk sizes, queued or one workgroup per
- * item) on the handle's current data and keeps the fastest for `precision`;
+ * item, with and without the source split, and the symmetric form where it
+ * applies) on the handle's current data and keeps the fastest for `precision`;
  * *best_ms is its mean time per evaluation. */
 int tfs_rpy_autotune(tfs_rpy *h, int32_t precision, uint32_t blocks, int32_t reps, double *best_ms);
 

@@ -13,7 +13,8 @@ full_mobility) on the same seeded inputs:
 * the argument rules: no persistent form, block 256 or 512 with the split,
   64/128/256 without;
 * the default shapes: the split for a body that fills the device (12,800
-  segments + 9,000 blobs), one wave per 64 x 32 item for one that does not."""
+  segments + 9,000 blobs; with 3-DOF segments in FP32 the symmetric form comes
+  first), one wave per 64 x 32 item for one that does not."""
 import numpy as np
 import pytest
 
@@ -158,6 +159,15 @@ def test_default_shapes(seg_dof):
     try:
         f32, f64 = big.tile_shape(L.PRECISION_FP32_ACC64), big.tile_shape(L.PRECISION_FP64)
         print("12,800 + 9,000:", f32, f64, flush=True)
+        if seg_dof == 3:
+            # the symmetric form (tests/test_symmetric_gpu.py) is the default of
+            # a large one-body FP32 handle with 3-DOF segments; the split its
+            # alternative
+            assert big.symmetric(L.PRECISION_FP32_ACC64)
+            assert f32 == {"block": 256, "per_thread": 1, "chunk": 0, "persistent": 0, "split": 0}
+            big.set_symmetric(L.PRECISION_FP32_ACC64, False)
+            f32 = big.tile_shape(L.PRECISION_FP32_ACC64)
+        assert not big.symmetric(L.PRECISION_FP32_ACC64)
         assert f32 == {"block": 256, "per_thread": 1, "chunk": 768, "persistent": 0, "split": 1}
         assert f64 == {"block": 256, "per_thread": 2, "chunk": 512, "persistent": 0, "split": 1}
         s64 = small.tile_shape(L.PRECISION_FP64)

@@ -85,6 +85,8 @@ SIGNATURES = {
     "tfs_rpy_get_tile_shape": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_i32)]),
     "tfs_rpy_set_split_shape": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32]),
     "tfs_rpy_get_source_split": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_i32)]),
+    "tfs_rpy_set_symmetric": (ctypes.c_int, [_vp, _i32, _i32]),
+    "tfs_rpy_get_symmetric": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_i32)]),
     "tfs_rpy_body_timing": (ctypes.c_int, [_vp, _i32]),
     "tfs_rpy_body_times": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
     "tfs_rpy_graph_create": (ctypes.c_int, [_vp, _u32, _i32, ctypes.POINTER(_vp)]),

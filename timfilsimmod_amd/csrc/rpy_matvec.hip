@@ -322,6 +322,200 @@ __global__ __launch_bounds__(BLOCK) void rpy_pairs_kernel(const MatvecItem* __re
     }
 }
 
+// ---- symmetric form (FP32 tiles, 3-DOF particles, one body) ----------------
+//
+// The far-field tensor of a pair is the same seen from either particle: the
+// separation enters twice (its sign cancels) and the radii symmetrically, so
+// M_ji = M_ij and one evaluation of 1/r, A and B serves both.  Tiles of 64
+// particles are grouped kSymTiles to a workgroup; item (ga, gb), ga <= gb,
+// evaluates the tile pairs of groups ga and gb, wave w owning tile w of group
+// ga.  In stage t the wave takes source tile (w + t) mod kSymTiles of group gb
+// from LDS, lane l pairing its particle with source (l + kSymRot k) mod 64 at
+// step k: the lane's own sum gets the source's term and the source's sum gets
+// the lane's term.  The source sums live in one register per component that
+// moves one lane per step (a DPP wave rotation folded into the add), so after
+// 64 steps lane l holds the sum of source (l + 63 kSymRot) mod 64.  Each stage
+// adds those sums into the source tile's FP64 LDS sums; the stages are fenced
+// by barriers and the writer of a tile's sums in stage t is wave
+// (tile - t) mod kSymTiles, a fixed order.  A pair of distinct groups is
+// evaluated once for both tiles; in a diagonal item (ga == gb) the pairs with
+// tile a < tile b are evaluated once, a == b one-sided (the self pair and the
+// overlapping pairs take the clamped far field, which the reduce kernel swaps
+// for the exact value), and a > b not at all.
+//
+// Partial slots: tile t of group g has n_groups slots, slot q holding the
+// sums it received from group q (its own sums when q > g, the source sums
+// when q < g, both added for q == g), each written once per launch, so the
+// reduce kernel sums them in slot order.
+constexpr int kSymTiles = 4;
+constexpr int kSymRot = 1;  // DPP wave_rol:1 -- lane l reads lane l + 1 (mod 64)
+__device__ __forceinline__ float rotate_lanes(float x) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), kSymRot > 0 ? 0x134 : 0x13C, 0xF, 0xF, false));
+}
+
+// LDS records of one source tile, twice over (index j and j + 64 hold
+// particle j, so that lane l reads record l + kSymRot k + (kSymRot < 0 ? 64 :
+// 0) without a wrap): X Y Z (integer bits) and the first strength in a 16-byte
+// array, the other two strengths in an 8-byte array -- consecutive lanes read
+// consecutive records, which no two lanes of a read phase share banks for.
+struct SymTileLds {
+    float4 xf[128];
+    float2 yz[128];
+};
+
+// The 64 steps of one stage.  BOTH: add the lane's terms into the rotating
+// source sums pj as well (an off-diagonal tile pair); otherwise only the
+// lane's own sum pi (the tile against itself).
+template <bool BOTH>
+__device__ __forceinline__ void sym_stage(const SymTileLds& recs, int lane, const int32_t (&Xi)[3],
+                                          const float (&fi)[3], float sum_a2, float s3, float m3s3, float (&pi)[3],
+                                          float (&pj)[3]) {
+    const int j0 = lane + (kSymRot > 0 ? 0 : 64);
+    // Not unrolled: 48 VGPRs, eight waves per SIMD to hide the LDS and
+    // dependency latency (measured on MI355X, production body: 0.212 ms per
+    // launch, against 0.219 unrolled twice (59 VGPRs) and 0.239 eight times
+    // (119 VGPRs, four waves per SIMD)).
+#pragma unroll 1
+    for (int k = 0; k < 64; ++k) {
+        const float4 q = recs.xf[j0 + kSymRot * k];
+        const float2 g = recs.yz[j0 + kSymRot * k];
+        const float rec[3] = {q.x, q.y, q.z};
+        float dx, dy, dz;
+        const float r2 = separation(Xi, rec, dx, dy, dz);
+        const float fj0 = q.w, fj1 = g.x, fj2 = g.y;
+        const float rm1 = rsqrt_refined(fmax_t(r2, sum_a2));
+        const float rm2 = rm1 * rm1;
+        const float rm3 = rm1 * rm2;
+        const float A = __builtin_fmaf(s3, rm3, rm1);
+        const float u = m3s3 * rm2;
+        const float B = __builtin_fmaf(u, rm3, rm3);
+        const float Bd = B * __builtin_fmaf(dz, fj2, __builtin_fmaf(dy, fj1, dx * fj0));
+        pi[0] = __builtin_fmaf(Bd, dx, __builtin_fmaf(A, fj0, pi[0]));
+        pi[1] = __builtin_fmaf(Bd, dy, __builtin_fmaf(A, fj1, pi[1]));
+        pi[2] = __builtin_fmaf(Bd, dz, __builtin_fmaf(A, fj2, pi[2]));
+        if constexpr (BOTH) {
+            // the source's term: the same A, B and separation (d_ji = -d
+            // enters twice), the lane's strengths
+            const float Be = B * __builtin_fmaf(dz, fi[2], __builtin_fmaf(dy, fi[1], dx * fi[0]));
+            pj[0] = __builtin_fmaf(Be, dx, __builtin_fmaf(A, fi[0], rotate_lanes(pj[0])));
+            pj[1] = __builtin_fmaf(Be, dy, __builtin_fmaf(A, fi[1], rotate_lanes(pj[1])));
+            pj[2] = __builtin_fmaf(Be, dz, __builtin_fmaf(A, fi[2], rotate_lanes(pj[2])));
+        }
+    }
+}
+
+// One particle of a tile for the symmetric form: fixed-point position and
+// strengths scaled by fsc (zero, with the tile's first position, for lanes
+// past the tile's end).
+__device__ __forceinline__ void sym_particle(const MatvecTile& tl, const MatvecArgs& a, int lane, double fsc,
+                                             int32_t (&X)[3], float (&f)[3]) {
+    const int s = tl.t_begin + lane;
+    const bool ok = s < tl.t_end;
+    const int sc = ok ? s : tl.t_begin;
+    const int4 q = a.xfx[(tl.cls == 0 ? 0 : a.ns) + sc];
+    X[0] = q.x;
+    X[1] = q.y;
+    X[2] = q.z;
+    const CView fv = tl.cls == 0 ? a.fs : a.fb;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) f[k] = ok ? float(fsc * fv.p[(int64_t)sc * fv.ld_p + k * fv.ld_c]) : 0.0f;
+}
+
+// One workgroup of kSymTiles waves per item (tile = ga, src_cls = gb).
+// `bb`: blob-blob pairs are evaluated (off when an overwriting BS block
+// replaces BB).
+__global__ __launch_bounds__(64 * kSymTiles) void rpy_pairs_sym_kernel(const MatvecItem* __restrict__ items,
+                                                                        const MatvecTile* __restrict__ tiles,
+                                                                        int n_tiles, int n_groups, int bb,
+                                                                        MatvecArgs a, float* __restrict__ partial) {
+    __shared__ SymTileLds recs[kSymTiles];
+    __shared__ double jsum[kSymTiles][3][64];
+    if (a.skip && *a.skip) return;
+    const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const MatvecItem it = items[blockIdx.x];
+    const int ga = it.tile, gb = it.src_cls;
+    const double PI = 3.14159265358979323846;
+    const double lam = a.lam;
+    const double fsc = lam / (8.0 * PI * a.mu);
+    // wave w stages source tile w of group gb (twice: records lane and lane + 64)
+    {
+        const int tb = kSymTiles * gb + w;
+        if (tb < n_tiles) {
+            int32_t X[3];
+            float f[3];
+            sym_particle(tiles[tb], a, lane, fsc, X, f);
+            const float4 q = make_float4(__int_as_float(X[0]), __int_as_float(X[1]), __int_as_float(X[2]), f[0]);
+            const float2 g = make_float2(f[1], f[2]);
+            recs[w].xf[lane] = q;
+            recs[w].xf[lane + 64] = q;
+            recs[w].yz[lane] = g;
+            recs[w].yz[lane + 64] = g;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) jsum[w][k][lane] = 0.0;
+    }
+    const int ta = kSymTiles * ga + w;
+    const bool has_a = ta < n_tiles;
+    int32_t Xi[3] = {0, 0, 0};
+    float fi[3] = {0.0f, 0.0f, 0.0f};
+    int cls_a = 0;
+    if (has_a) {
+        const MatvecTile tl = tiles[ta];
+        cls_a = tl.cls;
+        sym_particle(tl, a, lane, fsc, Xi, fi);
+    }
+    __syncthreads();
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int t = 0; t < kSymTiles; ++t) {
+        const int bq = (w + t) & (kSymTiles - 1);
+        const int tb = kSymTiles * gb + bq;
+        int mode = 0;  // 0 nothing, 1 one-sided, 2 both particles
+        int cls_b = 0;
+        if (has_a && tb < n_tiles) {
+            cls_b = tiles[tb].cls;
+            mode = (ga < gb || ta < tb) ? 2 : (ta == tb ? 1 : 0);
+            if (cls_a == 1 && cls_b == 1 && !bb) mode = 0;
+        }
+        if (mode) {
+            const double ai = cls_a == 0 ? a.a_seg : a.a_blob;
+            const double aj = cls_b == 0 ? a.a_seg : a.a_blob;
+            const float sum_a2 = float(lam * lam * ((ai + aj) * (ai + aj)));
+            const float s3 = float(lam * lam * ((ai * ai + aj * aj) / 3.0));
+            const float m3s3 = -3.0f * s3;
+            float pi[3] = {0.0f, 0.0f, 0.0f}, pj[3] = {0.0f, 0.0f, 0.0f};
+            if (mode == 2) sym_stage<true>(recs[bq], lane, Xi, fi, sum_a2, s3, m3s3, pi, pj);
+            else sym_stage<false>(recs[bq], lane, Xi, fi, sum_a2, s3, m3s3, pi, pj);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc[k] += double(pi[k]);
+            if (mode == 2) {
+                const int j = (lane + 63 * kSymRot) & 63;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) jsum[bq][k][j] += double(pj[k]);
+            }
+        }
+        __syncthreads();  // this stage's source sums are in before the next writer's
+    }
+    // slot (tile, partner group): kMaxDof x 64 floats, components k < 3 used
+    auto slot = [&](int tile, int q) { return partial + ((size_t)tile * n_groups + q) * (kMaxDof * 64); };
+    if (ga < gb) {
+        if (has_a) {
+            float* s = slot(ta, gb);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) s[k * 64 + lane] = float(acc[k]);
+        }
+        const int tb = kSymTiles * gb + w;
+        if (tb < n_tiles) {
+            float* s = slot(tb, ga);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) s[k * 64 + lane] = float(jsum[w][k][lane]);
+        }
+    } else if (has_a) {
+        float* s = slot(ta, ga);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s[k * 64 + lane] = float(acc[k] + jsum[w][k][lane]);
+    }
+}
+
 // Half-space domain (no-slip wall at z = 0): every pair of an item through
 // pair_wall, one target per lane; the self pair (same global index) takes the
 // wall self-mobility.  FP64 tiles stage positions as they are.  FP32 tiles
@@ -1137,6 +1331,36 @@ void build_worklist(const std::vector<BodyRange>& bodies, int tile_targets, int 
     }
 }
 
+int build_sym_worklist(const BodyRange& body, int seg_write, int blob_write, std::vector<MatvecTile>& tiles,
+                       std::vector<MatvecItem>& items) {
+    tiles.clear();
+    items.clear();
+    auto add_tiles = [&](int cls, int lo, int hi, int write) {
+        for (int t0 = lo; t0 < hi; t0 += 64) {
+            MatvecTile tl{};
+            tl.cls = cls;
+            tl.t_begin = t0;
+            tl.t_end = std::min(t0 + 64, hi);
+            tl.write_mode = write;
+            tl.body = 0;
+            tiles.push_back(tl);
+        }
+    };
+    add_tiles(0, body.seg_begin, body.seg_end, seg_write);
+    add_tiles(1, body.blob_begin, body.blob_end, blob_write);
+    const int n_tiles = (int)tiles.size();
+    const int n_groups = (n_tiles + kSymTiles - 1) / kSymTiles;
+    for (int t = 0; t < n_tiles; ++t) {
+        tiles[t].first_item = t * n_groups;
+        tiles[t].n_items = n_groups;
+    }
+    // off-diagonal items (both tiles' sums, the costlier ones) first
+    for (int ga = 0; ga < n_groups; ++ga)
+        for (int gb = ga + 1; gb < n_groups; ++gb) items.push_back({ga, gb, 0, 0});
+    for (int g = 0; g < n_groups; ++g) items.push_back({g, g, 0, 0});
+    return n_groups;
+}
+
 void free_grid(NearGrid& g) {
     for (int32_t* p : {g.start, g.fill, g.bucket_of, g.order, g.body_of, g.pair_body})
         if (p) (void)hipFree(p);
@@ -1315,10 +1539,17 @@ hipError_t launch_matvec(int precision, int seg_dof, const TileShape& shape, con
                          const MatvecItem* d_items, int n_items, double* partial, const NearList& near,
                          const PersistState& ps, hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end) {
     if (n_tiles == 0) return hipSuccess;
+    if (shape.symmetric && (precision != 1 || seg_dof != 3 || !a.xfx)) return hipErrorInvalidValue;
     if (n_items > 0) {
         hipError_t e;
         if (ev_begin) TFS_HIP_CHECK(hipEventRecord(ev_begin, stream));
-        if (precision == 0) {
+        if (shape.symmetric) {
+            const int n_groups = (n_tiles + kSymTiles - 1) / kSymTiles;
+            hipLaunchKernelGGL(rpy_pairs_sym_kernel, dim3((unsigned)n_items), dim3(64 * kSymTiles), 0, stream, d_items,
+                               d_tiles, n_tiles, n_groups, (o.blocks & 4u) ? 1 : 0, a,
+                               reinterpret_cast<float*>(partial));
+            e = hipGetLastError();
+        } else if (precision == 0) {
             e = seg_dof == 6 ? dispatch_shape<double, 6>(shape, a, d_items, n_items, d_tiles, partial, ps, stream)
                              : dispatch_shape<double, 3>(shape, a, d_items, n_items, d_tiles, partial, ps, stream);
         } else {

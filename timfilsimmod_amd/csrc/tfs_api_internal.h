@@ -102,7 +102,7 @@ struct tfs_rpy {
         int precision;
     };
     std::vector<EvPair> ev_used, ev_free;
-    std::map<std::tuple<uint32_t, int, int, int, int>, tfs::Plan> plans;
+    std::map<std::tuple<uint32_t, int, int, int, int, int>, tfs::Plan> plans;
     tfs::F32Stage f32;  // FP32 entry points (tfs_rpy_*_f32)
 };
 
@@ -173,6 +173,13 @@ int hip_fail(hipError_t e, const char* where);
 int hip_fail_if(hipError_t e, const char* where);
 TileShape default_shape(int precision, int ns, int nb, int num_cus, int seg_dof, int num_bodies);
 Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err);
+// Whether the symmetric form can evaluate `blocks` of this handle in
+// `precision` (FP32 tiles, 3-DOF segments, one body, unbounded domain, the SS,
+// SB and BS blocks all active).
+bool symmetric_applies(const tfs_rpy* h, int precision, uint32_t blocks);
+// The launch shape of one evaluation: the handle's shape, or its default
+// source-split shape where a symmetric shape does not apply.
+TileShape effective_shape(const tfs_rpy* h, int precision, uint32_t blocks);
 // Positions changed: the near-pair lists must be rebuilt before the next matvec.
 void invalidate_near(tfs_rpy* h);
 // Builds the near-pair list of `precision` (and, for FP32, the fixed-point

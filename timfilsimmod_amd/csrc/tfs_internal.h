@@ -101,6 +101,8 @@ struct TileShape {
                      // taking a contiguous range of the work list
     int split = 0;   // 1: the block/64 waves share one tile of 64 x per_thread targets and
                      // split every LDS batch of sources between them
+    int symmetric = 0;  // 1: each pair once for both particles (FP32 tiles, 3-DOF, one
+                        // body; 64-particle tiles in groups of four, 256 threads)
 };
 
 // Up to this many particles the near-pair lists test every pair (bit masks of
@@ -108,7 +110,7 @@ struct TileShape {
 constexpr int kNearPairsMax = 6144;
 
 // Targets per tile (and per partial slot) of a launch shape.
-inline int tile_targets(const TileShape& s) { return (s.split ? 64 : s.block) * s.per_thread; }
+inline int tile_targets(const TileShape& s) { return s.symmetric ? 64 : (s.split ? 64 : s.block) * s.per_thread; }
 
 // State of a persistent launch: optional per-item start/finish stamps
 // (wall_clock64 ticks), two per item.
@@ -172,6 +174,12 @@ struct OutArgs {
 void build_worklist(const std::vector<BodyRange>& bodies, int tile_targets, int chunk, bool ss, bool sb,
                     bool bb, bool bs, int seg_write, int blob_write, std::vector<MatvecTile>& tiles,
                     std::vector<MatvecItem>& items);
+// The symmetric form's work list over one body: 64-particle tiles (segments,
+// then blobs; every tile n_groups partial slots from tile * n_groups) and one
+// item (tile = ga, src_cls = gb) per pair of tile groups ga <= gb, the
+// diagonal ones last.  Returns the number of tile groups.
+int build_sym_worklist(const BodyRange& body, int seg_write, int blob_write, std::vector<MatvecTile>& tiles,
+                       std::vector<MatvecItem>& items);
 
 // Rebuilds `near` for the given precision: every pair of particles of one
 // body whose squared separation, in the pair kernel's own arithmetic for that

@@ -79,6 +79,23 @@ TileShape default_shape(int precision, int ns, int nb, int num_cus, int seg_dof,
     return s;
 }
 
+// The default launch shape of a handle: the symmetric form for one body with
+// FP32 tiles and 3-DOF segments large enough to give every CU ~4 of its items
+// (measured on MI355X, 12,800 segments + 9,000 blobs: 0.212 ms per launch
+// against 0.266 for the best source-split shape), else default_shape.
+static TileShape preferred_shape(int precision, int ns, int nb, int num_cus, int seg_dof, int num_bodies) {
+    if (precision == TFS_PRECISION_FP32_ACC64 && seg_dof == 3 && num_bodies == 1) {
+        const long tiles = (ns + 63L) / 64 + (nb + 63L) / 64;
+        const long groups = (tiles + 3) / 4;
+        if (groups * (groups + 1) / 2 >= 4L * (num_cus > 0 ? num_cus : 256)) {
+            TileShape s{256, 1, 0, 0, 0};
+            s.symmetric = 1;
+            return s;
+        }
+    }
+    return default_shape(precision, ns, nb, num_cus, seg_dof, num_bodies);
+}
+
 // Without the split: 512-target tiles, ~16 items per CU (short items even
 // out the tail).
 static TileShape plain_shape(int ns, int nb, int num_cus) {
@@ -115,12 +132,20 @@ Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err) {
     if (bb) blob_write = 0;
     else if (bs) blob_write = h->desc.bs_mode == TFS_BS_OVERWRITE ? 0 : 1;
     const uint32_t eff = (ss ? 1u : 0u) | (sb ? 2u : 0u) | (bb ? 4u : 0u) | (bs ? 8u : 0u);
-    const auto key = std::make_tuple(eff, sh.block, sh.per_thread, sh.chunk, sh.split);
+    const auto key = std::make_tuple(eff, sh.block, sh.per_thread, sh.chunk, sh.split, sh.symmetric);
     auto it = h->plans.find(key);
     if (it != h->plans.end()) return &it->second;
     Plan p;
     p.blocks = eff;
-    build_worklist(h->bodies, tile_targets(sh), sh.chunk, ss, sb, bb, bs, seg_write, blob_write, p.tiles, p.items);
+    size_t slots = 0;
+    if (sh.symmetric) {
+        // every tile a slot per tile group
+        const int n_groups = build_sym_worklist(h->bodies[0], seg_write, blob_write, p.tiles, p.items);
+        slots = p.tiles.size() * (size_t)n_groups;
+    } else {
+        build_worklist(h->bodies, tile_targets(sh), sh.chunk, ss, sb, bb, bs, seg_write, blob_write, p.tiles, p.items);
+        slots = p.items.size();
+    }
     if (!p.tiles.empty()) {
         hipError_t e = hipMalloc(&p.d_tiles, sizeof(MatvecTile) * p.tiles.size());
         if (e == hipSuccess && !p.items.empty())
@@ -137,7 +162,7 @@ Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err) {
             *err = hip_fail(e, "work list upload");
             return nullptr;
         }
-        const size_t need = p.items.size() * 6 * (size_t)tile_targets(sh);
+        const size_t need = slots * 6 * (size_t)tile_targets(sh);
         if (need > h->partial_len) {
             if (h->partial) {
                 hipStreamSynchronize(h->stream);
@@ -155,6 +180,20 @@ Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err) {
     }
     auto res = h->plans.emplace(key, std::move(p));
     return &res.first->second;
+}
+
+bool symmetric_applies(const tfs_rpy* h, int precision, uint32_t blocks) {
+    return precision == TFS_PRECISION_FP32_ACC64 && h->desc.seg_dof == 3 && h->bodies.size() == 1 &&
+           h->domain == TFS_DOMAIN_UNBOUNDED && (blocks & TFS_BLOCK_SS) && (blocks & TFS_BLOCK_SB) &&
+           (blocks & TFS_BLOCK_BS);
+}
+
+TileShape effective_shape(const tfs_rpy* h, int precision, uint32_t blocks) {
+    const TileShape& sh = h->shape[precision];
+    if (!sh.symmetric || symmetric_applies(h, precision, blocks)) return sh;
+    int ns = 0, nb = 0;
+    largest_body(h, &ns, &nb);
+    return default_shape(precision, ns, nb, h->num_cus, h->desc.seg_dof, (int)h->bodies.size());
 }
 
 void invalidate_near(tfs_rpy* h) {
@@ -462,7 +501,7 @@ void free_plans(tfs_rpy* h) {
 int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, View vs, View vb,
                int seg_out_slots, const SaddleEpilogue* sad) {
     const bool wall = h->domain == TFS_DOMAIN_HALF_SPACE;
-    TileShape sh = h->shape[precision];
+    TileShape sh = effective_shape(h, precision, blocks);
     if (wall) {
         // one target per lane, ~16 workgroups per CU over the work list
         const int ns = h->desc.num_segs, nb = h->desc.num_blobs;
@@ -504,7 +543,9 @@ int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, V
     }
     PersistState ps;
     ps.num_cus = h->num_cus;
-    if (h->body_timing) {
+    if (h->body_timing && sh.symmetric) {
+        h->stamped_plan = nullptr;  // the symmetric form keeps no per-item stamps
+    } else if (h->body_timing) {
         int rc = ensure_stamps(h, p->items.size());
         if (rc) return rc;
         ps.item_stamps = h->d_item_stamps;
@@ -590,7 +631,7 @@ int tfs_rpy_create(const tfs_rpy_desc* desc, tfs_rpy** out) {
         return hip_fail(e, "tfs_rpy_create: clear");
     }
     for (int pr = 0; pr < 2; ++pr)
-        h->shape[pr] = default_shape(pr, desc->num_segs, desc->num_blobs, cus, desc->seg_dof, 1);
+        h->shape[pr] = preferred_shape(pr, desc->num_segs, desc->num_blobs, cus, desc->seg_dof, 1);
     h->bodies.push_back(BodyRange{0, desc->num_segs, 0, desc->num_blobs});
     e = hipHostMalloc((void**)&h->host_int, 64, hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -828,6 +869,41 @@ int tfs_rpy_set_split_shape(tfs_rpy* h, int32_t precision, int32_t block, int32_
     return set_shape(h, "tfs_rpy_set_split_shape", precision, block, per_thread, chunk, 1);
 }
 
+int tfs_rpy_set_symmetric(tfs_rpy* h, int32_t precision, int32_t enable) {
+    if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_symmetric: null handle");
+    if (precision != 0 && precision != 1)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_symmetric: unknown precision");
+    if (!enable) {
+        if (h->shape[precision].symmetric) {
+            int ns = 0, nb = 0;
+            largest_body(h, &ns, &nb);
+            h->shape[precision] = default_shape(precision, ns, nb, h->num_cus, h->desc.seg_dof, (int)h->bodies.size());
+            h->shape_chosen[precision] = true;
+        }
+        return TFS_OK;
+    }
+    if (precision != TFS_PRECISION_FP32_ACC64)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_symmetric: the symmetric form has FP32 tiles only");
+    if (h->desc.seg_dof != 3)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_symmetric: the symmetric form needs 3-DOF segments");
+    if (h->bodies.size() != 1)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_symmetric: the symmetric form covers one body");
+    if (h->shape[precision].persistent > 0)
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_symmetric: the symmetric form has no persistent form");
+    TileShape sh{64 * 4, 1, 0, 0, 0};
+    sh.symmetric = 1;
+    h->shape[precision] = sh;
+    h->shape_chosen[precision] = true;
+    return TFS_OK;
+}
+
+int tfs_rpy_get_symmetric(tfs_rpy* h, int32_t precision, int32_t* enabled) {
+    if (!h || !enabled || (precision != 0 && precision != 1))
+        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_get_symmetric: bad arguments");
+    *enabled = h->shape[precision].symmetric;
+    return TFS_OK;
+}
+
 int tfs_rpy_positions_changed(tfs_rpy* h) {
     if (!h) return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_positions_changed: null handle");
     invalidate_near(h);
@@ -860,7 +936,7 @@ int tfs_rpy_set_bodies(tfs_rpy* h, int32_t num_bodies, const int32_t* segs_per_b
     largest_body(h, &lns, &lnb);
     for (int pr = 0; pr < 2; ++pr)
         if (!h->shape_chosen[pr])
-            h->shape[pr] = default_shape(pr, lns, lnb, h->num_cus, h->desc.seg_dof, (int)h->bodies.size());
+            h->shape[pr] = preferred_shape(pr, lns, lnb, h->num_cus, h->desc.seg_dof, (int)h->bodies.size());
     return TFS_OK;
 }
 
@@ -941,6 +1017,27 @@ int tfs_rpy_autotune(tfs_rpy* h, int32_t precision, uint32_t blocks, int32_t rep
     int ns = 0, nb = 0;
     largest_body(h, &ns, &nb);
     const double pairs = double(ns + nb) * double(ns + nb);
+    auto time_shape = [&](const TileShape& sh) {
+        h->shape[precision] = sh;
+        rc = tfs_rpy_evaluate(h, blocks);  // builds the plan; warm-up
+        if (rc) return;
+        hipEventRecord(e0, h->stream);
+        for (int i = 0; i < reps && !rc; ++i) rc = tfs_rpy_evaluate(h, blocks);
+        hipEventRecord(e1, h->stream);
+        if (rc) return;
+        hipEventSynchronize(e1);
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (ms / reps < best_t) {
+            best_t = ms / reps;
+            best = h->shape[precision];
+        }
+    };
+    if (symmetric_applies(h, precision, blocks)) {
+        TileShape sym{256, 1, 0, 0, 0};
+        sym.symmetric = 1;
+        time_shape(sym);
+    }
     for (int cand = 0; cand < 8 && !rc; ++cand) {
         const int split = cand / 4, block = 64 << (cand % 4);
         if (split ? block < 256 : block > 256) continue;  // the split needs >= 4 waves
@@ -959,20 +1056,8 @@ int tfs_rpy_autotune(tfs_rpy* h, int32_t precision, uint32_t blocks, int32_t rep
                 prev = chunk;
                 for (int pers : {0, 2}) {
                     if (split && pers) continue;  // no persistent form of the split
-                    h->shape[precision] = TileShape{block, r, (int)chunk, pers, split};
-                    rc = tfs_rpy_evaluate(h, blocks);  // builds the plan; warm-up
+                    time_shape(TileShape{block, r, (int)chunk, pers, split});
                     if (rc) break;
-                    hipEventRecord(e0, h->stream);
-                    for (int i = 0; i < reps && !rc; ++i) rc = tfs_rpy_evaluate(h, blocks);
-                    hipEventRecord(e1, h->stream);
-                    if (rc) break;
-                    hipEventSynchronize(e1);
-                    float ms = 0.0f;
-                    hipEventElapsedTime(&ms, e0, e1);
-                    if (ms / reps < best_t) {
-                        best_t = ms / reps;
-                        best = h->shape[precision];
-                    }
                 }
                 if (rc) break;
             }
@@ -993,10 +1078,13 @@ int tfs_rpy_autotune(tfs_rpy* h, int32_t precision, uint32_t blocks, int32_t rep
 int tfs_rpy_set_persistent(tfs_rpy* h, int32_t precision, int32_t workgroups_per_cu) {
     if (!h || (precision != 0 && precision != 1) || workgroups_per_cu < 0 || workgroups_per_cu > 64)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_persistent: bad arguments");
-    if (workgroups_per_cu > 0 && h->shape[precision].split) {
-        // the split has no persistent form: a default shape gives way to the plain default
+    if (workgroups_per_cu > 0 && (h->shape[precision].split || h->shape[precision].symmetric)) {
+        // the split and the symmetric form have no persistent form: a default
+        // shape gives way to the plain default
         if (h->shape_chosen[precision])
-            return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_persistent: the source split has no persistent form");
+            return fail(TFS_ERR_INVALID_ARGUMENT, h->shape[precision].split
+                                                      ? "tfs_rpy_set_persistent: the source split has no persistent form"
+                                                      : "tfs_rpy_set_persistent: the symmetric form has no persistent form");
         int ns = 0, nb = 0;
         largest_body(h, &ns, &nb);
         h->shape[precision] = plain_shape(ns, nb, h->num_cus);

@@ -677,7 +677,8 @@ int ens_cycle_graph(tfs_solver* s, int precision) {
     int rc = ensure_near(h, precision);
     if (rc) return rc;
     int err = TFS_OK;
-    const Plan* p = get_plan(h, TFS_BLOCK_ALL, h->shape[precision], &err);
+    const TileShape sh = effective_shape(h, precision, TFS_BLOCK_ALL);
+    const Plan* p = get_plan(h, TFS_BLOCK_ALL, sh, &err);
     if (!p) return err;
     uint64_t lam_bits = 0;
     std::memcpy(&lam_bits, &h->fx_lambda, 8);
@@ -689,11 +690,12 @@ int ens_cycle_graph(tfs_solver* s, int precision) {
                                         (uintptr_t)h->near[precision].cap,
                                         (uintptr_t)h->x_fx,
                                         (uintptr_t)(precision == 1 ? lam_bits : 0),
-                                        (uintptr_t)h->shape[precision].block,
-                                        (uintptr_t)h->shape[precision].per_thread,
-                                        (uintptr_t)h->shape[precision].chunk,
-                                        (uintptr_t)h->shape[precision].persistent,
-                                        (uintptr_t)h->shape[precision].split,
+                                        (uintptr_t)sh.block,
+                                        (uintptr_t)sh.per_thread,
+                                        (uintptr_t)sh.chunk,
+                                        (uintptr_t)sh.persistent,
+                                        (uintptr_t)sh.split,
+                                        (uintptr_t)sh.symmetric,
                                         (uintptr_t)s->precond_kind,
                                         (uintptr_t)s->npatch};
     if (E.exec[precision] && E.key[precision] == key) return TFS_OK;

@@ -130,6 +130,15 @@ class RpyHandle:
         """Launch shape whose waves share one tile of targets and split its sources."""
         L.check(L.lib().tfs_rpy_set_split_shape(self.h, precision, block, per_thread, chunk))
 
+    def set_symmetric(self, precision: int, enable: bool = True):
+        """Each unordered pair evaluated once for both particles (FP32 tiles, 3-DOF, one body)."""
+        L.check(L.lib().tfs_rpy_set_symmetric(self.h, precision, int(enable)))
+
+    def symmetric(self, precision: int) -> bool:
+        out = ctypes.c_int32()
+        L.check(L.lib().tfs_rpy_get_symmetric(self.h, precision, ctypes.byref(out)))
+        return bool(out.value)
+
     def tile_shape(self, precision: int):
         out = (ctypes.c_int32 * 4)()
         L.check(L.lib().tfs_rpy_get_tile_shape(self.h, precision, out))
