@@ -11,11 +11,17 @@ particle's sums rotating one lane per step) against the FP64 oracle
 * agreement with the default source-split shape;
 * evaluations of a block subset take the default shape (bitwise equal to a
   handle without the symmetric form);
-* the argument rules: FP32 tiles, 3-DOF segments and one body only, no
-  persistent form; enable = 0 restores the default shape;
+* independent bodies (block-diagonal mobility, pairs within a body only) of
+  uneven sizes, whose groups of four tiles straddle bodies, against the
+  per-body oracle, with per-body completion times from the same launch;
+* the argument rules: FP32 tiles and 3-DOF segments only, no persistent form;
+  enable = 0 restores the default shape;
 * a mixed-precision GMRES solve of the saddle-point system with the symmetric
   form on the solver's handle is as close to an FP64 solve at tolerance 1e-12
-  as the same solve with the default shape, in as many iterations (+-2)."""
+  as the same solve with the default shape, in as many iterations (+-2);
+* the lock-step solve of independent bodies (converged bodies' items exit at
+  once) with the symmetric form meets every body's FP64 tolerance and agrees
+  with the same ensemble solved with the default shape."""
 import numpy as np
 import pytest
 
@@ -124,15 +130,11 @@ def test_symmetric_argument_rules():
     xs, xb, fs, fb = cluster(ns, nb, 5.0, 3)
     h = handle(ns, nb, xs, xb, fs, fb)
     h6 = RpyHandle(ns, nb, seg_dof=6, precision=P32, bs_accumulate=True)
-    hb = handle(ns, nb, xs, xb, fs, fb)
     try:
         with pytest.raises(RuntimeError):
             h.set_symmetric(L.PRECISION_FP64)           # FP32 tiles only
         with pytest.raises(RuntimeError):
             h6.set_symmetric(P32)                       # 3-DOF segments only
-        hb.set_bodies([32, 32], [40, 24])
-        with pytest.raises(RuntimeError):
-            hb.set_symmetric(P32)                       # one body only
         h.set_symmetric(P32)
         with pytest.raises(RuntimeError):
             h.set_persistent(P32, 2)                    # no persistent form
@@ -144,7 +146,77 @@ def test_symmetric_argument_rules():
     finally:
         h.close()
         h6.close()
-        hb.close()
+
+
+# (num_fil, num_blobs, radius): bodies of one or two tiles next to bodies of
+# several groups, one without blobs, one without filaments
+BODIES = [(3, 40, 5.0), (1, 0, 3.0), (0, 70, 6.0), (30, 400, 14.0), (2, 33, 4.0), (45, 900, 18.0), (5, 120, 7.0)]
+
+
+@pytest.mark.parametrize("bs_acc", [True, False])
+def test_symmetric_independent_bodies_match_per_body_oracle(bs_acc):
+    from timfilsimmod_amd.ensemble import make_ensemble
+
+    e = make_ensemble(sizes=BODIES, num_seg=10)
+    print(f"case symmetric, {len(e.bodies)} independent bodies, bs_accumulate={bs_acc}", flush=True)
+    vs, vb = [], []
+    for b in e.bodies:
+        s_, b_ = O.full_mobility(b.x_segs, b.x_blobs, b.f_segs, b.f_blobs, seg_dof=3, bs_accumulate=bs_acc)
+        vs.append(s_)
+        vb.append(b_)
+    ref = np.concatenate(vs + vb)
+    h = RpyHandle(e.num_segs, e.num_blobs, seg_dof=3, precision=P32, bs_accumulate=bs_acc)
+    try:
+        h.set_bodies(e.segs_per_body, e.blobs_per_body)
+        h.upload(L.SEG_POSITIONS, e.x_segs)
+        h.upload(L.BLOB_POSITIONS, e.x_blobs)
+        h.upload(L.SEG_FORCES, e.f_segs)
+        h.upload(L.BLOB_FORCES, e.f_blobs)
+        h.set_symmetric(P32)
+        h.body_timing(True)
+        h.evaluate()
+        got = read(h, e.num_segs, e.num_blobs)
+        err = rel(got, ref)
+        print(f"  rel err {err:.2e}", flush=True)
+        assert err <= TOL
+        ms = h.body_times(len(e.bodies))
+        print("  per-body completion ms", ms, flush=True)
+        assert np.all(np.isfinite(ms)) and np.all(ms >= 0) and ms.max() > 0
+        h.body_timing(False)
+        h.evaluate()
+        assert np.array_equal(read(h, e.num_segs, e.num_blobs), got)
+    finally:
+        h.close()
+
+
+def test_lockstep_ensemble_with_symmetric_form():
+    from timfilsimmod_amd import geometry as G
+    from timfilsimmod_amd.solver import PRECOND_BLOCKS, SOLVE_MIXED, EnsembleBeat, beat_setup
+
+    sizes = [(4, 120, 8.0), (9, 300, 10.0), (2, 0, 6.0), (6, 60, 7.0)]
+    bodies = [G.make_body(nf, 10, nb, r, force_seed=10 + i, perturb_seed=20 + i, centre=(60.0 * i, 0.0, 0.0))
+              for i, (nf, nb, r) in enumerate(sizes)]
+    setups = [beat_setup(b, phase_seed=3 + i) for i, b in enumerate(bodies)]
+    kw = dict(mode=SOLVE_MIXED, restart=30, tol=1e-10, inner_tol=1e-4, max_refinements=20, max_cycles=100,
+              precond=PRECOND_BLOCKS)
+    print("case lock-step mixed ensemble, 4 bodies, symmetric form on the solver's handle", flush=True)
+    out = {}
+    for sym in (False, True):
+        ens = EnsembleBeat(bodies, setups=setups, **kw)
+        try:
+            if sym:
+                L.check(L.lib().tfs_rpy_set_symmetric(ens.rpy_handle(), P32, 1))
+            for _ in range(3):
+                st = ens.step()
+                bs = ens.body_stats()
+                assert st["converged"] == 1 and bs["converged"].all()
+                assert (bs["relres"] <= 1e-10).all()
+            out[sym] = ens.split(ens.state())
+        finally:
+            ens.close()
+    for i in range(len(bodies)):
+        assert rel(out[True][i]["phase"], out[False][i]["phase"]) <= 1e-9, i
+        assert rel(out[True][i]["solution"], out[False][i]["solution"]) <= 1e-6, i
 
 
 def test_gmres_with_symmetric_form_meets_tolerance():

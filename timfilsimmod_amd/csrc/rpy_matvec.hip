@@ -347,7 +347,7 @@ __global__ __launch_bounds__(BLOCK) void rpy_pairs_kernel(const MatvecItem* __re
 // sums it received from group q (its own sums when q > g, the source sums
 // when q < g, both added for q == g), each written once per launch, so the
 // reduce kernel sums them in slot order.
-constexpr int kSymTiles = 4;
+constexpr int kSymTiles = kSymGroupTiles;
 constexpr int kSymRot = 1;  // DPP wave_rol:1 -- lane l reads lane l + 1 (mod 64)
 __device__ __forceinline__ float rotate_lanes(float x) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), kSymRot > 0 ? 0x134 : 0x13C, 0xF, 0xF, false));
@@ -423,17 +423,36 @@ __device__ __forceinline__ void sym_particle(const MatvecTile& tl, const MatvecA
 
 // One workgroup of kSymTiles waves per item (tile = ga, src_cls = gb).
 // `bb`: blob-blob pairs are evaluated (off when an overwriting BS block
-// replaces BB).
+// replaces BB).  Tile pairs of different bodies are skipped, and so is an item
+// whose bodies all have their stop flag set; with `stamps`, thread 0 stores
+// the item's start and finish (wall_clock64 ticks).
 __global__ __launch_bounds__(64 * kSymTiles) void rpy_pairs_sym_kernel(const MatvecItem* __restrict__ items,
                                                                         const MatvecTile* __restrict__ tiles,
-                                                                        int n_tiles, int n_groups, int bb,
-                                                                        MatvecArgs a, float* __restrict__ partial) {
+                                                                        int n_tiles, int bb, MatvecArgs a,
+                                                                        float* __restrict__ partial,
+                                                                        int64_t* __restrict__ stamps) {
     __shared__ SymTileLds recs[kSymTiles];
     __shared__ double jsum[kSymTiles][3][64];
     if (a.skip && *a.skip) return;
+    const int64_t t_start = stamps ? (int64_t)wall_clock64() : 0;
     const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
     const MatvecItem it = items[blockIdx.x];
     const int ga = it.tile, gb = it.src_cls;
+    if (a.body_skip) {
+        // the bodies of the item's tiles are a contiguous range
+        const int b0 = tiles[kSymTiles * ga].body;
+        const int tl_last = kSymTiles * gb + kSymTiles < n_tiles ? kSymTiles * gb + kSymTiles : n_tiles;
+        const int b1 = tiles[tl_last - 1].body;
+        bool any = false;
+        for (int b = b0; b <= b1 && !any; ++b) any = a.body_skip[b] == 0;
+        if (!any) {
+            if (stamps && threadIdx.x == 0) {
+                stamps[2 * (int64_t)blockIdx.x] = t_start;
+                stamps[2 * (int64_t)blockIdx.x + 1] = t_start;
+            }
+            return;
+        }
+    }
     const double PI = 3.14159265358979323846;
     const double lam = a.lam;
     const double fsc = lam / (8.0 * PI * a.mu);
@@ -458,11 +477,12 @@ __global__ __launch_bounds__(64 * kSymTiles) void rpy_pairs_sym_kernel(const Mat
     const bool has_a = ta < n_tiles;
     int32_t Xi[3] = {0, 0, 0};
     float fi[3] = {0.0f, 0.0f, 0.0f};
-    int cls_a = 0;
+    MatvecTile tla{};
+    bool run_a = false;
     if (has_a) {
-        const MatvecTile tl = tiles[ta];
-        cls_a = tl.cls;
-        sym_particle(tl, a, lane, fsc, Xi, fi);
+        tla = tiles[ta];
+        run_a = !(a.body_skip && a.body_skip[tla.body]);
+        sym_particle(tla, a, lane, fsc, Xi, fi);
     }
     __syncthreads();
     double acc[3] = {0.0, 0.0, 0.0};
@@ -471,13 +491,14 @@ __global__ __launch_bounds__(64 * kSymTiles) void rpy_pairs_sym_kernel(const Mat
         const int tb = kSymTiles * gb + bq;
         int mode = 0;  // 0 nothing, 1 one-sided, 2 both particles
         int cls_b = 0;
-        if (has_a && tb < n_tiles) {
-            cls_b = tiles[tb].cls;
+        if (run_a && tb < n_tiles) {
+            const MatvecTile tlb = tiles[tb];
+            cls_b = tlb.cls;
             mode = (ga < gb || ta < tb) ? 2 : (ta == tb ? 1 : 0);
-            if (cls_a == 1 && cls_b == 1 && !bb) mode = 0;
+            if ((tla.cls == 1 && cls_b == 1 && !bb) || tlb.body != tla.body) mode = 0;
         }
         if (mode) {
-            const double ai = cls_a == 0 ? a.a_seg : a.a_blob;
+            const double ai = tla.cls == 0 ? a.a_seg : a.a_blob;
             const double aj = cls_b == 0 ? a.a_seg : a.a_blob;
             const float sum_a2 = float(lam * lam * ((ai + aj) * (ai + aj)));
             const float s3 = float(lam * lam * ((ai * ai + aj * aj) / 3.0));
@@ -495,24 +516,35 @@ __global__ __launch_bounds__(64 * kSymTiles) void rpy_pairs_sym_kernel(const Mat
         }
         __syncthreads();  // this stage's source sums are in before the next writer's
     }
-    // slot (tile, partner group): kMaxDof x 64 floats, components k < 3 used
-    auto slot = [&](int tile, int q) { return partial + ((size_t)tile * n_groups + q) * (kMaxDof * 64); };
+    // slot of (tile, partner group q) if q is one of the groups of the tile's
+    // body: kMaxDof x 64 floats, components k < 3 used
+    auto slot = [&](const MatvecTile& tl, int q) -> float* {
+        return (q >= tl.group0 && q < tl.group0 + tl.n_items)
+                   ? partial + ((size_t)tl.first_item + (q - tl.group0)) * (kMaxDof * 64)
+                   : nullptr;
+    };
     if (ga < gb) {
         if (has_a) {
-            float* s = slot(ta, gb);
+            if (float* s = slot(tla, gb)) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) s[k * 64 + lane] = float(acc[k]);
+                for (int k = 0; k < 3; ++k) s[k * 64 + lane] = float(acc[k]);
+            }
         }
         const int tb = kSymTiles * gb + w;
         if (tb < n_tiles) {
-            float* s = slot(tb, ga);
+            if (float* s = slot(tiles[tb], ga)) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) s[k * 64 + lane] = float(jsum[w][k][lane]);
+                for (int k = 0; k < 3; ++k) s[k * 64 + lane] = float(jsum[w][k][lane]);
+            }
         }
     } else if (has_a) {
-        float* s = slot(ta, ga);
+        float* s = slot(tla, ga);
 #pragma unroll
         for (int k = 0; k < 3; ++k) s[k * 64 + lane] = float(acc[k] + jsum[w][k][lane]);
+    }
+    if (stamps && threadIdx.x == 0) {
+        stamps[2 * (int64_t)blockIdx.x] = t_start;
+        stamps[2 * (int64_t)blockIdx.x + 1] = (int64_t)wall_clock64();
     }
 }
 
@@ -1331,34 +1363,50 @@ void build_worklist(const std::vector<BodyRange>& bodies, int tile_targets, int 
     }
 }
 
-int build_sym_worklist(const BodyRange& body, int seg_write, int blob_write, std::vector<MatvecTile>& tiles,
-                       std::vector<MatvecItem>& items) {
+size_t build_sym_worklist(const std::vector<BodyRange>& bodies, int seg_write, int blob_write,
+                          std::vector<MatvecTile>& tiles, std::vector<MatvecItem>& items) {
     tiles.clear();
     items.clear();
-    auto add_tiles = [&](int cls, int lo, int hi, int write) {
-        for (int t0 = lo; t0 < hi; t0 += 64) {
-            MatvecTile tl{};
-            tl.cls = cls;
-            tl.t_begin = t0;
-            tl.t_end = std::min(t0 + 64, hi);
-            tl.write_mode = write;
-            tl.body = 0;
-            tiles.push_back(tl);
-        }
-    };
-    add_tiles(0, body.seg_begin, body.seg_end, seg_write);
-    add_tiles(1, body.blob_begin, body.blob_end, blob_write);
+    for (size_t b = 0; b < bodies.size(); ++b) {
+        auto add_tiles = [&](int cls, int lo, int hi, int write) {
+            for (int t0 = lo; t0 < hi; t0 += 64) {
+                MatvecTile tl{};
+                tl.cls = cls;
+                tl.t_begin = t0;
+                tl.t_end = std::min(t0 + 64, hi);
+                tl.write_mode = write;
+                tl.body = (int32_t)b;
+                tiles.push_back(tl);
+            }
+        };
+        add_tiles(0, bodies[b].seg_begin, bodies[b].seg_end, seg_write);
+        add_tiles(1, bodies[b].blob_begin, bodies[b].blob_end, blob_write);
+    }
     const int n_tiles = (int)tiles.size();
     const int n_groups = (n_tiles + kSymTiles - 1) / kSymTiles;
+    // each body's groups [gfirst, glast]; each group's bodies [blo, bhi]
+    std::vector<int> gfirst(bodies.size(), -1), glast(bodies.size(), -1), blo(n_groups), bhi(n_groups);
     for (int t = 0; t < n_tiles; ++t) {
-        tiles[t].first_item = t * n_groups;
-        tiles[t].n_items = n_groups;
+        const int b = tiles[t].body, g = t / kSymTiles;
+        if (gfirst[b] < 0) gfirst[b] = g;
+        glast[b] = g;
+        if (t % kSymTiles == 0) blo[g] = b;
+        bhi[g] = b;
     }
-    // off-diagonal items (both tiles' sums, the costlier ones) first
+    size_t slots = 0;
+    for (MatvecTile& tl : tiles) {
+        tl.group0 = gfirst[tl.body];
+        tl.first_item = (int32_t)slots;
+        tl.n_items = glast[tl.body] - gfirst[tl.body] + 1;
+        slots += (size_t)tl.n_items;
+    }
+    // off-diagonal items (both tiles' sums, the costlier ones) first; groups
+    // are in body order, so ga < gb share a body when gb starts no later than
+    // the last body of ga
     for (int ga = 0; ga < n_groups; ++ga)
-        for (int gb = ga + 1; gb < n_groups; ++gb) items.push_back({ga, gb, 0, 0});
+        for (int gb = ga + 1; gb < n_groups && blo[gb] <= bhi[ga]; ++gb) items.push_back({ga, gb, 0, 0});
     for (int g = 0; g < n_groups; ++g) items.push_back({g, g, 0, 0});
-    return n_groups;
+    return slots;
 }
 
 void free_grid(NearGrid& g) {
@@ -1544,10 +1592,9 @@ hipError_t launch_matvec(int precision, int seg_dof, const TileShape& shape, con
         hipError_t e;
         if (ev_begin) TFS_HIP_CHECK(hipEventRecord(ev_begin, stream));
         if (shape.symmetric) {
-            const int n_groups = (n_tiles + kSymTiles - 1) / kSymTiles;
             hipLaunchKernelGGL(rpy_pairs_sym_kernel, dim3((unsigned)n_items), dim3(64 * kSymTiles), 0, stream, d_items,
-                               d_tiles, n_tiles, n_groups, (o.blocks & 4u) ? 1 : 0, a,
-                               reinterpret_cast<float*>(partial));
+                               d_tiles, n_tiles, (o.blocks & 4u) ? 1 : 0, a, reinterpret_cast<float*>(partial),
+                               ps.item_stamps);
             e = hipGetLastError();
         } else if (precision == 0) {
             e = seg_dof == 6 ? dispatch_shape<double, 6>(shape, a, d_items, n_items, d_tiles, partial, ps, stream)

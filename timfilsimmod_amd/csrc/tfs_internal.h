@@ -48,6 +48,8 @@ struct MatvecTile {
     int32_t n_items;
     int32_t write_mode;  // 0 overwrite, 1 accumulate into the output
     int32_t body;        // index into the body list
+    int32_t group0;      // symmetric form: first tile group of the body (slot of group q:
+                         // first_item + q - group0)
 };
 
 // Overlapping pairs (r^2 <= (ai+aj)^2 as evaluated by the pair kernel of one
@@ -108,6 +110,9 @@ struct TileShape {
 // Up to this many particles the near-pair lists test every pair (bit masks of
 // n^2 / 8 bytes) instead of sorting the particles into a hashed grid.
 constexpr int kNearPairsMax = 6144;
+
+// Tiles of 64 particles per group (and waves per workgroup) of the symmetric form.
+constexpr int kSymGroupTiles = 4;
 
 // Targets per tile (and per partial slot) of a launch shape.
 inline int tile_targets(const TileShape& s) { return s.symmetric ? 64 : (s.split ? 64 : s.block) * s.per_thread; }
@@ -174,12 +179,13 @@ struct OutArgs {
 void build_worklist(const std::vector<BodyRange>& bodies, int tile_targets, int chunk, bool ss, bool sb,
                     bool bb, bool bs, int seg_write, int blob_write, std::vector<MatvecTile>& tiles,
                     std::vector<MatvecItem>& items);
-// The symmetric form's work list over one body: 64-particle tiles (segments,
-// then blobs; every tile n_groups partial slots from tile * n_groups) and one
-// item (tile = ga, src_cls = gb) per pair of tile groups ga <= gb, the
-// diagonal ones last.  Returns the number of tile groups.
-int build_sym_worklist(const BodyRange& body, int seg_write, int blob_write, std::vector<MatvecTile>& tiles,
-                       std::vector<MatvecItem>& items);
+// The symmetric form's work list: 64-particle tiles (each body's segments,
+// then its blobs, body after body) in groups of four, one item (tile = ga,
+// src_cls = gb) per pair of groups ga <= gb that share a body, the diagonal
+// ones last; every tile one partial slot per group its body overlaps.
+// Returns the number of partial slots.
+size_t build_sym_worklist(const std::vector<BodyRange>& bodies, int seg_write, int blob_write,
+                          std::vector<MatvecTile>& tiles, std::vector<MatvecItem>& items);
 
 // Rebuilds `near` for the given precision: every pair of particles of one
 // body whose squared separation, in the pair kernel's own arithmetic for that

@@ -79,12 +79,14 @@ TileShape default_shape(int precision, int ns, int nb, int num_cus, int seg_dof,
     return s;
 }
 
-// The default launch shape of a handle: the symmetric form for one body with
-// FP32 tiles and 3-DOF segments large enough to give every CU ~4 of its items
-// (measured on MI355X, 12,800 segments + 9,000 blobs: 0.212 ms per launch
-// against 0.266 for the best source-split shape), else default_shape.
+// The default launch shape of a handle: the symmetric form for FP32 tiles and
+// 3-DOF segments when the (largest) body alone gives every CU ~4 items
+// (measured on MI355X: 12,800 segments + 9,000 blobs, 0.212 ms per launch
+// against 0.266 for the best source-split shape; 64 uneven bodies in
+// lock-step, a mixed timestep in 831 ms against 1,027), else default_shape.
 static TileShape preferred_shape(int precision, int ns, int nb, int num_cus, int seg_dof, int num_bodies) {
-    if (precision == TFS_PRECISION_FP32_ACC64 && seg_dof == 3 && num_bodies == 1) {
+    (void)num_bodies;
+    if (precision == TFS_PRECISION_FP32_ACC64 && seg_dof == 3) {
         const long tiles = (ns + 63L) / 64 + (nb + 63L) / 64;
         const long groups = (tiles + 3) / 4;
         if (groups * (groups + 1) / 2 >= 4L * (num_cus > 0 ? num_cus : 256)) {
@@ -139,9 +141,8 @@ Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err) {
     p.blocks = eff;
     size_t slots = 0;
     if (sh.symmetric) {
-        // every tile a slot per tile group
-        const int n_groups = build_sym_worklist(h->bodies[0], seg_write, blob_write, p.tiles, p.items);
-        slots = p.tiles.size() * (size_t)n_groups;
+        slots = build_sym_worklist(h->bodies, seg_write, blob_write, p.tiles, p.items);
+        p.symmetric = true;
     } else {
         build_worklist(h->bodies, tile_targets(sh), sh.chunk, ss, sb, bb, bs, seg_write, blob_write, p.tiles, p.items);
         slots = p.items.size();
@@ -183,9 +184,8 @@ Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err) {
 }
 
 bool symmetric_applies(const tfs_rpy* h, int precision, uint32_t blocks) {
-    return precision == TFS_PRECISION_FP32_ACC64 && h->desc.seg_dof == 3 && h->bodies.size() == 1 &&
-           h->domain == TFS_DOMAIN_UNBOUNDED && (blocks & TFS_BLOCK_SS) && (blocks & TFS_BLOCK_SB) &&
-           (blocks & TFS_BLOCK_BS);
+    return precision == TFS_PRECISION_FP32_ACC64 && h->desc.seg_dof == 3 && h->domain == TFS_DOMAIN_UNBOUNDED &&
+           (blocks & TFS_BLOCK_SS) && (blocks & TFS_BLOCK_SB) && (blocks & TFS_BLOCK_BS);
 }
 
 TileShape effective_shape(const tfs_rpy* h, int precision, uint32_t blocks) {
@@ -543,9 +543,7 @@ int run_matvec(tfs_rpy* h, uint32_t blocks, int precision, CView fs, CView fb, V
     }
     PersistState ps;
     ps.num_cus = h->num_cus;
-    if (h->body_timing && sh.symmetric) {
-        h->stamped_plan = nullptr;  // the symmetric form keeps no per-item stamps
-    } else if (h->body_timing) {
+    if (h->body_timing) {
         int rc = ensure_stamps(h, p->items.size());
         if (rc) return rc;
         ps.item_stamps = h->d_item_stamps;
@@ -886,8 +884,6 @@ int tfs_rpy_set_symmetric(tfs_rpy* h, int32_t precision, int32_t enable) {
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_symmetric: the symmetric form has FP32 tiles only");
     if (h->desc.seg_dof != 3)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_symmetric: the symmetric form needs 3-DOF segments");
-    if (h->bodies.size() != 1)
-        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_symmetric: the symmetric form covers one body");
     if (h->shape[precision].persistent > 0)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_symmetric: the symmetric form has no persistent form");
     TileShape sh{64 * 4, 1, 0, 0, 0};
@@ -1134,9 +1130,19 @@ int tfs_rpy_body_times(tfs_rpy* h, double* ms_per_body) {
     int64_t t0 = INT64_MAX;
     for (size_t i = 0; i < ni; ++i) t0 = std::min(t0, st[2 * i]);
     std::vector<int64_t> done(nb, t0);
+    const int nt = (int)p->tiles.size();
     for (size_t i = 0; i < ni; ++i) {
-        const int b = p->tiles[p->items[i].tile].body;
-        done[b] = std::max(done[b], st[2 * i + 1]);
+        if (p->symmetric) {
+            // an item pairs tile groups ga <= gb: the bodies both hold
+            const int ga = p->items[i].tile, gb = p->items[i].src_cls;
+            const int g = kSymGroupTiles;
+            const int lo = std::max(p->tiles[g * ga].body, p->tiles[g * gb].body);
+            const int hi = std::min(p->tiles[std::min(g * ga + g, nt) - 1].body, p->tiles[std::min(g * gb + g, nt) - 1].body);
+            for (int b = lo; b <= hi; ++b) done[b] = std::max(done[b], st[2 * i + 1]);
+        } else {
+            const int b = p->tiles[p->items[i].tile].body;
+            done[b] = std::max(done[b], st[2 * i + 1]);
+        }
     }
     for (size_t b = 0; b < nb; ++b) ms_per_body[b] = double(done[b] - t0) / double(khz);
     return TFS_OK;
