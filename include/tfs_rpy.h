@@ -188,14 +188,14 @@ int tfs_rpy_get_source_split(tfs_rpy *h, int32_t precision, int32_t *split);
  * terms.  64-particle tiles in groups of four per workgroup of 256 threads;
  * results agree with the other shapes to FP32 rounding and are the same on
  * every run.  Independent bodies (tfs_rpy_set_bodies) pair within a body only,
- * and the items of bodies whose stop flag is set return at once.  FP32 tiles
- * (precision TFS_PRECISION_FP32_ACC64) and 3-DOF segments only (refused
- * otherwise); evaluations whose blocks lack SS, SB
+ * and the items of bodies whose stop flag is set return at once.  Both
+ * precisions; 3-DOF segments only (refused otherwise); evaluations whose
+ * blocks lack SS, SB
  * or BS, and the half-space domain, take the default source-split shape
- * instead.  No persistent form.  The default shape of an FP32 handle with
- * 3-DOF segments whose (largest) body's groups of four tiles pair into at
- * least four items per CU (e.g. 12,800 segments + 9,000 blobs: 3,741 items);
- * enable = 0 selects the source-split default instead. */
+ * instead.  No persistent form.  The default shape (both precisions) of a
+ * handle with 3-DOF segments whose (largest) body's groups of four tiles pair
+ * into at least four items per CU (e.g. 12,800 segments + 9,000 blobs: 3,741
+ * items); enable = 0 selects the source-split default instead. */
 int tfs_rpy_set_symmetric(tfs_rpy *h, int32_t precision, int32_t enable);
 /* 1 if the precision's current shape is the symmetric form. */
 int tfs_rpy_get_symmetric(tfs_rpy *h, int32_t precision, int32_t *enabled);

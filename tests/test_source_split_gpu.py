@@ -13,7 +13,7 @@ full_mobility) on the same seeded inputs:
 * the argument rules: no persistent form, block 256 or 512 with the split,
   64/128/256 without;
 * the default shapes: the split for a body that fills the device (12,800
-  segments + 9,000 blobs; with 3-DOF segments in FP32 the symmetric form comes
+  segments + 9,000 blobs; with 3-DOF segments the symmetric form comes
   first), one wave per 64 x 32 item for one that does not."""
 import numpy as np
 import pytest
@@ -142,10 +142,10 @@ def test_split_argument_rules():
         assert rel(got, np.concatenate([rs, rb])) <= TOL[p]
     finally:
         h.close()
-    # a default split shape gives way to the plain default for a persistent launch
+    # a default split (or symmetric) shape gives way to the plain default for a persistent launch
     big = RpyHandle(12800, 9000, seg_dof=3, precision=p, bs_accumulate=True)
     try:
-        assert big.tile_shape(p)["split"] == 1
+        assert big.symmetric(p)
         big.set_persistent(p, 2)
         assert big.tile_shape(p) == {"block": 256, "per_thread": 2, "chunk": 256, "persistent": 2, "split": 0}
     finally:
@@ -161,13 +161,13 @@ def test_default_shapes(seg_dof):
         print("12,800 + 9,000:", f32, f64, flush=True)
         if seg_dof == 3:
             # the symmetric form (tests/test_symmetric_gpu.py) is the default of
-            # a large one-body FP32 handle with 3-DOF segments; the split its
-            # alternative
-            assert big.symmetric(L.PRECISION_FP32_ACC64)
-            assert f32 == {"block": 256, "per_thread": 1, "chunk": 0, "persistent": 0, "split": 0}
-            big.set_symmetric(L.PRECISION_FP32_ACC64, False)
-            f32 = big.tile_shape(L.PRECISION_FP32_ACC64)
-        assert not big.symmetric(L.PRECISION_FP32_ACC64)
+            # a large handle with 3-DOF segments; the split its alternative
+            for p in (L.PRECISION_FP32_ACC64, L.PRECISION_FP64):
+                assert big.symmetric(p)
+                assert big.tile_shape(p) == {"block": 256, "per_thread": 1, "chunk": 0, "persistent": 0, "split": 0}
+                big.set_symmetric(p, False)
+            f32, f64 = big.tile_shape(L.PRECISION_FP32_ACC64), big.tile_shape(L.PRECISION_FP64)
+        assert not big.symmetric(L.PRECISION_FP32_ACC64) and not big.symmetric(L.PRECISION_FP64)
         assert f32 == {"block": 256, "per_thread": 1, "chunk": 768, "persistent": 0, "split": 1}
         assert f64 == {"block": 256, "per_thread": 2, "chunk": 512, "persistent": 0, "split": 1}
         s64 = small.tile_shape(L.PRECISION_FP64)

@@ -3,7 +3,8 @@ pair evaluated once, its term added to both particles' sums, the second
 particle's sums rotating one lane per step) against the FP64 oracle
 (oracle/rpy_oracle.c full_mobility) on the same seeded inputs:
 
-* FP32 tiles, 3-DOF segments, <= 1e-6 relative L2, over sizes that leave the
+* FP32 tiles <= 1e-6 and FP64 tiles <= 1e-12 relative L2, 3-DOF segments,
+  over sizes that leave the
   last tile and the last group of four tiles partly empty, one particle class
   only, overlapping pairs, unequal radii, and both BS write rules (the
   overwriting BS drops the blob-blob pairs);
@@ -14,8 +15,8 @@ particle's sums rotating one lane per step) against the FP64 oracle
 * independent bodies (block-diagonal mobility, pairs within a body only) of
   uneven sizes, whose groups of four tiles straddle bodies, against the
   per-body oracle, with per-body completion times from the same launch;
-* the argument rules: FP32 tiles and 3-DOF segments only, no persistent form;
-  enable = 0 restores the default shape;
+* the argument rules: 3-DOF segments only, no persistent form; enable = 0
+  restores the default shape;
 * a mixed-precision GMRES solve of the saddle-point system with the symmetric
   form on the solver's handle is as close to an FP64 solve at tolerance 1e-12
   as the same solve with the default shape, in as many iterations (+-2);
@@ -33,6 +34,8 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-6
 P32 = L.PRECISION_FP32_ACC64
+P64 = L.PRECISION_FP64
+TOLS = {P32: 1e-6, P64: 1e-12}
 
 
 def rel(a, b):
@@ -45,8 +48,9 @@ def cluster(ns, nb, box, seed):
             rng.normal(size=6 * ns), rng.normal(size=3 * nb))
 
 
-def handle(ns, nb, xs, xb, fs, fb, bs_acc=True, a_blob=1.0):
-    h = RpyHandle(ns, nb, seg_dof=3, precision=P32, bs_accumulate=bs_acc, seg_radius=1.0, blob_radius=a_blob)
+def handle(ns, nb, xs, xb, fs, fb, bs_acc=True, a_blob=1.0, precision=P32):
+    h = RpyHandle(ns, nb, seg_dof=3, precision=precision, bs_accumulate=bs_acc, seg_radius=1.0,
+                  blob_radius=a_blob)
     if ns:
         h.upload(L.SEG_POSITIONS, xs)
         h.upload(L.SEG_FORCES, fs)
@@ -66,18 +70,19 @@ def read(h, ns, nb):
     return np.concatenate([vs, vb])
 
 
+@pytest.mark.parametrize("precision", [P32, P64])
 @pytest.mark.parametrize("bs_acc", [True, False])
 @pytest.mark.parametrize("ns,nb", [(700, 530), (100, 37), (129, 0), (0, 200), (257, 511), (3000, 2100)])
-def test_symmetric_matches_oracle(ns, nb, bs_acc):
-    print(f"case symmetric ns={ns} nb={nb} bs_accumulate={bs_acc}", flush=True)
+def test_symmetric_matches_oracle(ns, nb, bs_acc, precision):
+    print(f"case symmetric ns={ns} nb={nb} bs_accumulate={bs_acc} precision={precision}", flush=True)
     box = 1.1 * (ns + nb) ** (1 / 3) + 2.0  # dense enough for overlapping pairs
     xs, xb, fs, fb = cluster(ns, nb, box, ns * 3 + nb)
     rs, rb = O.full_mobility(xs, xb, fs, fb, seg_dof=3, bs_accumulate=bs_acc, a_seg=1.0, a_blob=1.3)
     ref = np.concatenate([rs, rb])
-    h = handle(ns, nb, xs, xb, fs, fb, bs_acc, a_blob=1.3)
+    h = handle(ns, nb, xs, xb, fs, fb, bs_acc, a_blob=1.3, precision=precision)
     try:
-        h.set_symmetric(P32)
-        assert h.symmetric(P32)
+        h.set_symmetric(precision)
+        assert h.symmetric(precision)
         h.evaluate()
         got = read(h, ns, nb)
         if not np.any(ref):
@@ -86,7 +91,7 @@ def test_symmetric_matches_oracle(ns, nb, bs_acc):
         else:
             err = rel(got, ref)
             print(f"  rel err {err:.2e}", flush=True)
-            assert err <= TOL
+            assert err <= TOLS[precision]
         if ns:
             assert np.all(got[:6 * ns].reshape(ns, 6)[:, 3:] == 0.0)
         h.evaluate()
@@ -132,9 +137,9 @@ def test_symmetric_argument_rules():
     h6 = RpyHandle(ns, nb, seg_dof=6, precision=P32, bs_accumulate=True)
     try:
         with pytest.raises(RuntimeError):
-            h.set_symmetric(L.PRECISION_FP64)           # FP32 tiles only
-        with pytest.raises(RuntimeError):
             h6.set_symmetric(P32)                       # 3-DOF segments only
+        with pytest.raises(RuntimeError):
+            h6.set_symmetric(P64)
         h.set_symmetric(P32)
         with pytest.raises(RuntimeError):
             h.set_persistent(P32, 2)                    # no persistent form
@@ -153,8 +158,9 @@ def test_symmetric_argument_rules():
 BODIES = [(3, 40, 5.0), (1, 0, 3.0), (0, 70, 6.0), (30, 400, 14.0), (2, 33, 4.0), (45, 900, 18.0), (5, 120, 7.0)]
 
 
+@pytest.mark.parametrize("precision", [P32, P64])
 @pytest.mark.parametrize("bs_acc", [True, False])
-def test_symmetric_independent_bodies_match_per_body_oracle(bs_acc):
+def test_symmetric_independent_bodies_match_per_body_oracle(bs_acc, precision):
     from timfilsimmod_amd.ensemble import make_ensemble
 
     e = make_ensemble(sizes=BODIES, num_seg=10)
@@ -165,20 +171,20 @@ def test_symmetric_independent_bodies_match_per_body_oracle(bs_acc):
         vs.append(s_)
         vb.append(b_)
     ref = np.concatenate(vs + vb)
-    h = RpyHandle(e.num_segs, e.num_blobs, seg_dof=3, precision=P32, bs_accumulate=bs_acc)
+    h = RpyHandle(e.num_segs, e.num_blobs, seg_dof=3, precision=precision, bs_accumulate=bs_acc)
     try:
         h.set_bodies(e.segs_per_body, e.blobs_per_body)
         h.upload(L.SEG_POSITIONS, e.x_segs)
         h.upload(L.BLOB_POSITIONS, e.x_blobs)
         h.upload(L.SEG_FORCES, e.f_segs)
         h.upload(L.BLOB_FORCES, e.f_blobs)
-        h.set_symmetric(P32)
+        h.set_symmetric(precision)
         h.body_timing(True)
         h.evaluate()
         got = read(h, e.num_segs, e.num_blobs)
         err = rel(got, ref)
         print(f"  rel err {err:.2e}", flush=True)
-        assert err <= TOL
+        assert err <= TOLS[precision]
         ms = h.body_times(len(e.bodies))
         print("  per-body completion ms", ms, flush=True)
         assert np.all(np.isfinite(ms)) and np.all(ms >= 0) and ms.max() > 0

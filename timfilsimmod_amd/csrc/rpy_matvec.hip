@@ -322,7 +322,7 @@ __global__ __launch_bounds__(BLOCK) void rpy_pairs_kernel(const MatvecItem* __re
     }
 }
 
-// ---- symmetric form (FP32 tiles, 3-DOF particles, one body) ----------------
+// ---- symmetric form (3-DOF particles) -----------------------------------------
 //
 // The far-field tensor of a pair is the same seen from either particle: the
 // separation enters twice (its sign cancels) and the radii symmetrically, so
@@ -352,61 +352,105 @@ constexpr int kSymRot = 1;  // DPP wave_rol:1 -- lane l reads lane l + 1 (mod 64
 __device__ __forceinline__ float rotate_lanes(float x) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), kSymRot > 0 ? 0x134 : 0x13C, 0xF, 0xF, false));
 }
+__device__ __forceinline__ double rotate_lanes(double x) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), kSymRot > 0 ? 0x134 : 0x13C, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), kSymRot > 0 ? 0x134 : 0x13C, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
 
 // LDS records of one source tile, twice over (index j and j + 64 hold
 // particle j, so that lane l reads record l + kSymRot k + (kSymRot < 0 ? 64 :
-// 0) without a wrap): X Y Z (integer bits) and the first strength in a 16-byte
-// array, the other two strengths in an 8-byte array -- consecutive lanes read
-// consecutive records, which no two lanes of a read phase share banks for.
-struct SymTileLds {
+// 0) without a wrap), split into arrays of 16 or 8 bytes per record so that
+// consecutive lanes, reading consecutive records, share no banks within a
+// read phase.  FP32: X Y Z (fixed-point integers as their bits) and the first
+// strength, then the other two strengths.  FP64: x y, z and the first
+// strength, the other two.
+template <typename T>
+struct SymTileLds;
+template <>
+struct SymTileLds<float> {
     float4 xf[128];
     float2 yz[128];
 };
+template <>
+struct SymTileLds<double> {
+    double2 xy[128];
+    double2 zf[128];
+    double2 gh[128];
+};
+
+__device__ __forceinline__ void sym_record(const SymTileLds<float>& L, int j, float (&rec)[3], float (&f)[3]) {
+    const float4 q = L.xf[j];
+    const float2 g = L.yz[j];
+    rec[0] = q.x;
+    rec[1] = q.y;
+    rec[2] = q.z;
+    f[0] = q.w;
+    f[1] = g.x;
+    f[2] = g.y;
+}
+__device__ __forceinline__ void sym_record(const SymTileLds<double>& L, int j, double (&rec)[3], double (&f)[3]) {
+    const double2 a = L.xy[j], b = L.zf[j], c = L.gh[j];
+    rec[0] = a.x;
+    rec[1] = a.y;
+    rec[2] = b.x;
+    f[0] = b.y;
+    f[1] = c.x;
+    f[2] = c.y;
+}
+__device__ __forceinline__ void sym_put(SymTileLds<float>& L, int j, const int32_t (&X)[3], const float (&f)[3]) {
+    L.xf[j] = make_float4(__int_as_float(X[0]), __int_as_float(X[1]), __int_as_float(X[2]), f[0]);
+    L.yz[j] = make_float2(f[1], f[2]);
+}
+__device__ __forceinline__ void sym_put(SymTileLds<double>& L, int j, const double (&X)[3], const double (&f)[3]) {
+    L.xy[j] = make_double2(X[0], X[1]);
+    L.zf[j] = make_double2(X[2], f[0]);
+    L.gh[j] = make_double2(f[1], f[2]);
+}
 
 // The 64 steps of one stage.  BOTH: add the lane's terms into the rotating
 // source sums pj as well (an off-diagonal tile pair); otherwise only the
-// lane's own sum pi (the tile against itself).
-template <bool BOTH>
-__device__ __forceinline__ void sym_stage(const SymTileLds& recs, int lane, const int32_t (&Xi)[3],
-                                          const float (&fi)[3], float sum_a2, float s3, float m3s3, float (&pi)[3],
-                                          float (&pj)[3]) {
+// lane's own sum pi (the tile against itself).  FP32 tiles work in the
+// fixed-point frame (separation from integer coordinates, s3 and (ai+aj)^2
+// scaled by lambda^2).
+template <bool BOTH, typename T>
+__device__ __forceinline__ void sym_stage(const SymTileLds<T>& recs, int lane, const TargetT<T> (&Xi)[3],
+                                          const T (&fi)[3], T sum_a2, T s3, T m3s3, T (&pi)[3], T (&pj)[3]) {
     const int j0 = lane + (kSymRot > 0 ? 0 : 64);
-    // Not unrolled: 48 VGPRs, eight waves per SIMD to hide the LDS and
+    // Not unrolled: FP32 48 VGPRs, eight waves per SIMD to hide the LDS and
     // dependency latency (measured on MI355X, production body: 0.212 ms per
     // launch, against 0.219 unrolled twice (59 VGPRs) and 0.239 eight times
     // (119 VGPRs, four waves per SIMD)).
 #pragma unroll 1
     for (int k = 0; k < 64; ++k) {
-        const float4 q = recs.xf[j0 + kSymRot * k];
-        const float2 g = recs.yz[j0 + kSymRot * k];
-        const float rec[3] = {q.x, q.y, q.z};
-        float dx, dy, dz;
-        const float r2 = separation(Xi, rec, dx, dy, dz);
-        const float fj0 = q.w, fj1 = g.x, fj2 = g.y;
-        const float rm1 = rsqrt_refined(fmax_t(r2, sum_a2));
-        const float rm2 = rm1 * rm1;
-        const float rm3 = rm1 * rm2;
-        const float A = __builtin_fmaf(s3, rm3, rm1);
-        const float u = m3s3 * rm2;
-        const float B = __builtin_fmaf(u, rm3, rm3);
-        const float Bd = B * __builtin_fmaf(dz, fj2, __builtin_fmaf(dy, fj1, dx * fj0));
-        pi[0] = __builtin_fmaf(Bd, dx, __builtin_fmaf(A, fj0, pi[0]));
-        pi[1] = __builtin_fmaf(Bd, dy, __builtin_fmaf(A, fj1, pi[1]));
-        pi[2] = __builtin_fmaf(Bd, dz, __builtin_fmaf(A, fj2, pi[2]));
+        T rec[3], fj[3];
+        sym_record(recs, j0 + kSymRot * k, rec, fj);
+        T dx, dy, dz;
+        const T r2 = separation(Xi, rec, dx, dy, dz);
+        const T rm1 = rsqrt_refined(fmax_t(r2, sum_a2));
+        const T rm2 = rm1 * rm1;
+        const T rm3 = rm1 * rm2;
+        const T A = fma_t(s3, rm3, rm1);
+        const T u = m3s3 * rm2;
+        const T B = fma_t(u, rm3, rm3);
+        const T Bd = B * fma_t(dz, fj[2], fma_t(dy, fj[1], dx * fj[0]));
+        pi[0] = fma_t(Bd, dx, fma_t(A, fj[0], pi[0]));
+        pi[1] = fma_t(Bd, dy, fma_t(A, fj[1], pi[1]));
+        pi[2] = fma_t(Bd, dz, fma_t(A, fj[2], pi[2]));
         if constexpr (BOTH) {
             // the source's term: the same A, B and separation (d_ji = -d
             // enters twice), the lane's strengths
-            const float Be = B * __builtin_fmaf(dz, fi[2], __builtin_fmaf(dy, fi[1], dx * fi[0]));
-            pj[0] = __builtin_fmaf(Be, dx, __builtin_fmaf(A, fi[0], rotate_lanes(pj[0])));
-            pj[1] = __builtin_fmaf(Be, dy, __builtin_fmaf(A, fi[1], rotate_lanes(pj[1])));
-            pj[2] = __builtin_fmaf(Be, dz, __builtin_fmaf(A, fi[2], rotate_lanes(pj[2])));
+            const T Be = B * fma_t(dz, fi[2], fma_t(dy, fi[1], dx * fi[0]));
+            pj[0] = fma_t(Be, dx, fma_t(A, fi[0], rotate_lanes(pj[0])));
+            pj[1] = fma_t(Be, dy, fma_t(A, fi[1], rotate_lanes(pj[1])));
+            pj[2] = fma_t(Be, dz, fma_t(A, fi[2], rotate_lanes(pj[2])));
         }
     }
 }
 
-// One particle of a tile for the symmetric form: fixed-point position and
-// strengths scaled by fsc (zero, with the tile's first position, for lanes
-// past the tile's end).
+// One particle of a tile for the symmetric form: its position (FP32 tiles:
+// fixed-point) and strengths scaled by fsc (zero, with the tile's first
+// position, for lanes past the tile's end).
 __device__ __forceinline__ void sym_particle(const MatvecTile& tl, const MatvecArgs& a, int lane, double fsc,
                                              int32_t (&X)[3], float (&f)[3]) {
     const int s = tl.t_begin + lane;
@@ -420,18 +464,33 @@ __device__ __forceinline__ void sym_particle(const MatvecTile& tl, const MatvecA
 #pragma unroll
     for (int k = 0; k < 3; ++k) f[k] = ok ? float(fsc * fv.p[(int64_t)sc * fv.ld_p + k * fv.ld_c]) : 0.0f;
 }
+__device__ __forceinline__ void sym_particle(const MatvecTile& tl, const MatvecArgs& a, int lane, double fsc,
+                                             double (&X)[3], double (&f)[3]) {
+    const int s = tl.t_begin + lane;
+    const bool ok = s < tl.t_end;
+    const int sc = ok ? s : tl.t_begin;
+    const CView xv = tl.cls == 0 ? a.xs : a.xb;
+    const CView fv = tl.cls == 0 ? a.fs : a.fb;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        X[k] = xv.p[(int64_t)sc * xv.ld_p + k * xv.ld_c];
+        f[k] = ok ? fsc * fv.p[(int64_t)sc * fv.ld_p + k * fv.ld_c] : 0.0;
+    }
+}
 
 // One workgroup of kSymTiles waves per item (tile = ga, src_cls = gb).
 // `bb`: blob-blob pairs are evaluated (off when an overwriting BS block
 // replaces BB).  Tile pairs of different bodies are skipped, and so is an item
 // whose bodies all have their stop flag set; with `stamps`, thread 0 stores
 // the item's start and finish (wall_clock64 ticks).
+template <typename T>
 __global__ __launch_bounds__(64 * kSymTiles) void rpy_pairs_sym_kernel(const MatvecItem* __restrict__ items,
                                                                         const MatvecTile* __restrict__ tiles,
                                                                         int n_tiles, int bb, MatvecArgs a,
-                                                                        float* __restrict__ partial,
+                                                                        T* __restrict__ partial,
                                                                         int64_t* __restrict__ stamps) {
-    __shared__ SymTileLds recs[kSymTiles];
+    constexpr bool F32 = sizeof(T) == 4;
+    __shared__ SymTileLds<T> recs[kSymTiles];
     __shared__ double jsum[kSymTiles][3][64];
     if (a.skip && *a.skip) return;
     const int64_t t_start = stamps ? (int64_t)wall_clock64() : 0;
@@ -454,29 +513,25 @@ __global__ __launch_bounds__(64 * kSymTiles) void rpy_pairs_sym_kernel(const Mat
         }
     }
     const double PI = 3.14159265358979323846;
-    const double lam = a.lam;
+    const double lam = F32 ? a.lam : 1.0;
     const double fsc = lam / (8.0 * PI * a.mu);
     // wave w stages source tile w of group gb (twice: records lane and lane + 64)
     {
         const int tb = kSymTiles * gb + w;
         if (tb < n_tiles) {
-            int32_t X[3];
-            float f[3];
+            TargetT<T> X[3];
+            T f[3];
             sym_particle(tiles[tb], a, lane, fsc, X, f);
-            const float4 q = make_float4(__int_as_float(X[0]), __int_as_float(X[1]), __int_as_float(X[2]), f[0]);
-            const float2 g = make_float2(f[1], f[2]);
-            recs[w].xf[lane] = q;
-            recs[w].xf[lane + 64] = q;
-            recs[w].yz[lane] = g;
-            recs[w].yz[lane + 64] = g;
+            sym_put(recs[w], lane, X, f);
+            sym_put(recs[w], lane + 64, X, f);
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) jsum[w][k][lane] = 0.0;
     }
     const int ta = kSymTiles * ga + w;
     const bool has_a = ta < n_tiles;
-    int32_t Xi[3] = {0, 0, 0};
-    float fi[3] = {0.0f, 0.0f, 0.0f};
+    TargetT<T> Xi[3] = {0, 0, 0};
+    T fi[3] = {T(0), T(0), T(0)};
     MatvecTile tla{};
     bool run_a = false;
     if (has_a) {
@@ -500,12 +555,12 @@ __global__ __launch_bounds__(64 * kSymTiles) void rpy_pairs_sym_kernel(const Mat
         if (mode) {
             const double ai = tla.cls == 0 ? a.a_seg : a.a_blob;
             const double aj = cls_b == 0 ? a.a_seg : a.a_blob;
-            const float sum_a2 = float(lam * lam * ((ai + aj) * (ai + aj)));
-            const float s3 = float(lam * lam * ((ai * ai + aj * aj) / 3.0));
-            const float m3s3 = -3.0f * s3;
-            float pi[3] = {0.0f, 0.0f, 0.0f}, pj[3] = {0.0f, 0.0f, 0.0f};
-            if (mode == 2) sym_stage<true>(recs[bq], lane, Xi, fi, sum_a2, s3, m3s3, pi, pj);
-            else sym_stage<false>(recs[bq], lane, Xi, fi, sum_a2, s3, m3s3, pi, pj);
+            const T sum_a2 = T(lam * lam * ((ai + aj) * (ai + aj)));
+            const T s3 = T(lam * lam * ((ai * ai + aj * aj) / 3.0));
+            const T m3s3 = T(-3) * s3;
+            T pi[3] = {T(0), T(0), T(0)}, pj[3] = {T(0), T(0), T(0)};
+            if (mode == 2) sym_stage<true, T>(recs[bq], lane, Xi, fi, sum_a2, s3, m3s3, pi, pj);
+            else sym_stage<false, T>(recs[bq], lane, Xi, fi, sum_a2, s3, m3s3, pi, pj);
 #pragma unroll
             for (int k = 0; k < 3; ++k) acc[k] += double(pi[k]);
             if (mode == 2) {
@@ -518,29 +573,29 @@ __global__ __launch_bounds__(64 * kSymTiles) void rpy_pairs_sym_kernel(const Mat
     }
     // slot of (tile, partner group q) if q is one of the groups of the tile's
     // body: kMaxDof x 64 floats, components k < 3 used
-    auto slot = [&](const MatvecTile& tl, int q) -> float* {
+    auto slot = [&](const MatvecTile& tl, int q) -> T* {
         return (q >= tl.group0 && q < tl.group0 + tl.n_items)
                    ? partial + ((size_t)tl.first_item + (q - tl.group0)) * (kMaxDof * 64)
                    : nullptr;
     };
     if (ga < gb) {
         if (has_a) {
-            if (float* s = slot(tla, gb)) {
+            if (T* s = slot(tla, gb)) {
 #pragma unroll
-                for (int k = 0; k < 3; ++k) s[k * 64 + lane] = float(acc[k]);
+                for (int k = 0; k < 3; ++k) s[k * 64 + lane] = T(acc[k]);
             }
         }
         const int tb = kSymTiles * gb + w;
         if (tb < n_tiles) {
-            if (float* s = slot(tiles[tb], ga)) {
+            if (T* s = slot(tiles[tb], ga)) {
 #pragma unroll
-                for (int k = 0; k < 3; ++k) s[k * 64 + lane] = float(jsum[w][k][lane]);
+                for (int k = 0; k < 3; ++k) s[k * 64 + lane] = T(jsum[w][k][lane]);
             }
         }
     } else if (has_a) {
-        float* s = slot(tla, ga);
+        T* s = slot(tla, ga);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) s[k * 64 + lane] = float(acc[k] + jsum[w][k][lane]);
+        for (int k = 0; k < 3; ++k) s[k * 64 + lane] = T(acc[k] + jsum[w][k][lane]);
     }
     if (stamps && threadIdx.x == 0) {
         stamps[2 * (int64_t)blockIdx.x] = t_start;
@@ -1587,14 +1642,18 @@ hipError_t launch_matvec(int precision, int seg_dof, const TileShape& shape, con
                          const MatvecItem* d_items, int n_items, double* partial, const NearList& near,
                          const PersistState& ps, hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end) {
     if (n_tiles == 0) return hipSuccess;
-    if (shape.symmetric && (precision != 1 || seg_dof != 3 || !a.xfx)) return hipErrorInvalidValue;
+    if (shape.symmetric && (seg_dof != 3 || (precision == 1 && !a.xfx))) return hipErrorInvalidValue;
     if (n_items > 0) {
         hipError_t e;
         if (ev_begin) TFS_HIP_CHECK(hipEventRecord(ev_begin, stream));
         if (shape.symmetric) {
-            hipLaunchKernelGGL(rpy_pairs_sym_kernel, dim3((unsigned)n_items), dim3(64 * kSymTiles), 0, stream, d_items,
-                               d_tiles, n_tiles, (o.blocks & 4u) ? 1 : 0, a, reinterpret_cast<float*>(partial),
-                               ps.item_stamps);
+            const int bb = (o.blocks & 4u) ? 1 : 0;
+            if (precision == 1)
+                hipLaunchKernelGGL(rpy_pairs_sym_kernel<float>, dim3((unsigned)n_items), dim3(64 * kSymTiles), 0, stream,
+                                   d_items, d_tiles, n_tiles, bb, a, reinterpret_cast<float*>(partial), ps.item_stamps);
+            else
+                hipLaunchKernelGGL(rpy_pairs_sym_kernel<double>, dim3((unsigned)n_items), dim3(64 * kSymTiles), 0,
+                                   stream, d_items, d_tiles, n_tiles, bb, a, partial, ps.item_stamps);
             e = hipGetLastError();
         } else if (precision == 0) {
             e = seg_dof == 6 ? dispatch_shape<double, 6>(shape, a, d_items, n_items, d_tiles, partial, ps, stream)

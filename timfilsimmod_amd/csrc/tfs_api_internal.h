@@ -175,8 +175,8 @@ int hip_fail_if(hipError_t e, const char* where);
 TileShape default_shape(int precision, int ns, int nb, int num_cus, int seg_dof, int num_bodies);
 Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err);
 // Whether the symmetric form can evaluate `blocks` of this handle in
-// `precision` (FP32 tiles, 3-DOF segments, unbounded domain, the SS, SB and BS
-// blocks all active).
+// `precision` (3-DOF segments, unbounded domain, the SS, SB and BS blocks all
+// active).
 bool symmetric_applies(const tfs_rpy* h, int precision, uint32_t blocks);
 // The launch shape of one evaluation: the handle's shape, or its default
 // source-split shape where a symmetric shape does not apply.

@@ -79,14 +79,16 @@ TileShape default_shape(int precision, int ns, int nb, int num_cus, int seg_dof,
     return s;
 }
 
-// The default launch shape of a handle: the symmetric form for FP32 tiles and
-// 3-DOF segments when the (largest) body alone gives every CU ~4 items
-// (measured on MI355X: 12,800 segments + 9,000 blobs, 0.212 ms per launch
-// against 0.266 for the best source-split shape; 64 uneven bodies in
-// lock-step, a mixed timestep in 831 ms against 1,027), else default_shape.
+// The default launch shape of a handle: the symmetric form for 3-DOF
+// segments when the (largest) body alone gives every CU ~4 items (measured on
+// MI355X, 12,800 segments + 9,000 blobs: 0.212 ms per FP32 launch against
+// 0.266 for the best source-split shape, 0.358 ms per FP64 launch against
+// 0.432; 64 uneven bodies in lock-step, a mixed timestep in 831 ms against
+// 1,027), else default_shape.
 static TileShape preferred_shape(int precision, int ns, int nb, int num_cus, int seg_dof, int num_bodies) {
     (void)num_bodies;
-    if (precision == TFS_PRECISION_FP32_ACC64 && seg_dof == 3) {
+    (void)precision;
+    if (seg_dof == 3) {
         const long tiles = (ns + 63L) / 64 + (nb + 63L) / 64;
         const long groups = (tiles + 3) / 4;
         if (groups * (groups + 1) / 2 >= 4L * (num_cus > 0 ? num_cus : 256)) {
@@ -184,8 +186,9 @@ Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err) {
 }
 
 bool symmetric_applies(const tfs_rpy* h, int precision, uint32_t blocks) {
-    return precision == TFS_PRECISION_FP32_ACC64 && h->desc.seg_dof == 3 && h->domain == TFS_DOMAIN_UNBOUNDED &&
-           (blocks & TFS_BLOCK_SS) && (blocks & TFS_BLOCK_SB) && (blocks & TFS_BLOCK_BS);
+    (void)precision;
+    return h->desc.seg_dof == 3 && h->domain == TFS_DOMAIN_UNBOUNDED && (blocks & TFS_BLOCK_SS) &&
+           (blocks & TFS_BLOCK_SB) && (blocks & TFS_BLOCK_BS);
 }
 
 TileShape effective_shape(const tfs_rpy* h, int precision, uint32_t blocks) {
@@ -880,8 +883,6 @@ int tfs_rpy_set_symmetric(tfs_rpy* h, int32_t precision, int32_t enable) {
         }
         return TFS_OK;
     }
-    if (precision != TFS_PRECISION_FP32_ACC64)
-        return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_symmetric: the symmetric form has FP32 tiles only");
     if (h->desc.seg_dof != 3)
         return fail(TFS_ERR_INVALID_ARGUMENT, "tfs_rpy_set_symmetric: the symmetric form needs 3-DOF segments");
     if (h->shape[precision].persistent > 0)

@@ -131,7 +131,7 @@ class RpyHandle:
         L.check(L.lib().tfs_rpy_set_split_shape(self.h, precision, block, per_thread, chunk))
 
     def set_symmetric(self, precision: int, enable: bool = True):
-        """Each unordered pair evaluated once for both particles (FP32 tiles, 3-DOF, one body)."""
+        """Each unordered pair evaluated once for both particles (3-DOF segments)."""
         L.check(L.lib().tfs_rpy_set_symmetric(self.h, precision, int(enable)))
 
     def symmetric(self, precision: int) -> bool:
