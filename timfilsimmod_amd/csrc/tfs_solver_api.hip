@@ -30,6 +30,12 @@ constexpr int kMaxRestart = 350;
 // host looks one batch behind; up to 2*kEstBatch - 1 iterations enqueued past
 // convergence run as empty launches.
 constexpr int kEstBatch = 4;
+// Mixed-precision refinement: the inner FP32-tile solve of a correction d
+// stops at max(inner_tol ||r||, kInnerFloor abs_tol).  The FP64 residual of
+// x + d then exceeds the inner residual only by the FP32 operator's error on
+// d, ~1e-7 ||A d|| ~ 1e-7 ||r||, far inside the remaining (1 - kInnerFloor)
+// abs_tol, so one refinement meets the outer tolerance.
+constexpr double kInnerFloor = 0.8;
 
 __global__ void fill_kernel(double* p, double v, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -895,7 +901,7 @@ int solve_batched(tfs_solver* s, const double* b, double* x, tfs_solve_stats* st
             for (int i = 0; i < nb; ++i) {
                 act[i] = bn[i] > 0.0 && rn[i] > abs_tol[i];
                 n_act += act[i];
-                inner[i] = std::max(D.inner_tol * rn[i], 0.5 * abs_tol[i]);
+                inner[i] = std::max(D.inner_tol * rn[i], kInnerFloor * abs_tol[i]);
             }
             if (!n_act) break;
             st->refinements++;
@@ -1042,7 +1048,7 @@ int solve_device(tfs_solver* s, const double* b, double* x, tfs_solve_stats* st)
             double inner_res = 0.0;
             if (!rc)
                 rc = gmres(s, TFS_PRECISION_FP32_ACC64, s->rin, s->d, s->r, rnorm,
-                           std::max(D.inner_tol * rnorm, 0.5 * abs_tol), D.max_cycles, st, &inner_conv,
+                           std::max(D.inner_tol * rnorm, kInnerFloor * abs_tol), D.max_cycles, st, &inner_conv,
                            &inner_res);
             if (!rc) rc = check(k_axpby(1.0, s->d, 1.0, x, n, s->stream), "x += d");
             if (!rc) rc = residual(s, b, x, false, s->r, TFS_PRECISION_FP64, st);
