@@ -8,7 +8,8 @@ particle's sums rotating one lane per step) against the FP64 oracle
   last tile and the last group of four tiles partly empty, one particle class
   only, overlapping pairs, unequal radii, and both BS write rules (the
   overwriting BS drops the blob-blob pairs);
-* repeated evaluations bitwise identical (a fixed summation order);
+* repeated evaluations bitwise identical (a fixed summation order), and hipGraph
+  replays bitwise equal to the eager launch with eager work between replays;
 * agreement with the default source-split shape;
 * evaluations of a block subset take the default shape (bitwise equal to a
   handle without the symmetric form);
@@ -128,6 +129,29 @@ def test_symmetric_agrees_with_split_and_falls_back_for_block_subsets():
     finally:
         h.close()
         plain.close()
+
+
+@pytest.mark.parametrize("precision", [P32, P64])
+def test_symmetric_graph_replays_match_eager(precision):
+    ns, nb = 2000, 1500
+    xs, xb, fs, fb = cluster(ns, nb, 16.0, 9)
+    h = handle(ns, nb, xs, xb, fs, fb, precision=precision)
+    try:
+        h.set_symmetric(precision)
+        h.evaluate()
+        eager = read(h, ns, nb)
+        g = h.graph(L.BLOCK_ALL, 2)
+        rng = np.random.default_rng(0)
+        for k in range(4):
+            # eager work between replays: overwrite the outputs with noise
+            h.upload(L.SEG_VELOCITIES, rng.normal(size=6 * ns))
+            h.upload(L.BLOB_VELOCITIES, rng.normal(size=3 * nb))
+            h.sync()
+            g.launch()
+            assert np.array_equal(read(h, ns, nb), eager), k
+        g.close()
+    finally:
+        h.close()
 
 
 def test_symmetric_argument_rules():

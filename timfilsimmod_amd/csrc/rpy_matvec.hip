@@ -572,10 +572,10 @@ __global__ __launch_bounds__(64 * kSymTiles) void rpy_pairs_sym_kernel(const Mat
         __syncthreads();  // this stage's source sums are in before the next writer's
     }
     // slot of (tile, partner group q) if q is one of the groups of the tile's
-    // body: kMaxDof x 64 floats, components k < 3 used
+    // body: 3 x 64 values
     auto slot = [&](const MatvecTile& tl, int q) -> T* {
         return (q >= tl.group0 && q < tl.group0 + tl.n_items)
-                   ? partial + ((size_t)tl.first_item + (q - tl.group0)) * (kMaxDof * 64)
+                   ? partial + ((size_t)tl.first_item + (q - tl.group0)) * (kSymSlotDof * 64)
                    : nullptr;
     };
     if (ga < gb) {
@@ -1140,7 +1140,8 @@ __device__ void saddle_filament_rows(const SaddleEpilogue& e, int m, int lane) {
 }
 
 // Adds the partial slots of each tile and the overlapping pairs, and applies
-// the write rule.  A workgroup covers 64 targets of one tile with four groups
+// the write rule.  A slot holds slot_dof x tile_targets values (component k
+// of target l at k * tile_targets + l).  A workgroup covers 64 targets of one tile with four groups
 // of 64 threads: group g sums the items g, g+4, g+8, ... (loads of four items
 // issued together) and then the near-list entries g, g+4, ... of its target
 // (the lists are sorted when built); the four group sums are added in group
@@ -1149,8 +1150,8 @@ __device__ void saddle_filament_rows(const SaddleEpilogue& e, int m, int lane) {
 template <int SEGDOF, typename P>
 __global__ __launch_bounds__(256) void rpy_reduce_kernel(const MatvecTile* __restrict__ tiles,
                                                          const P* __restrict__ partial, int tile_targets,
-                                                         int tile_blocks, int merge_k, OutArgs o, MatvecArgs a,
-                                                         NearList near) {
+                                                         int tile_blocks, int merge_k, int slot_dof, OutArgs o,
+                                                         MatvecArgs a, NearList near) {
     __shared__ double grp_acc[3][6][64];
     if (a.skip && *a.skip) return;
     const int grp = threadIdx.x >> 6;
@@ -1168,7 +1169,7 @@ __global__ __launch_bounds__(256) void rpy_reduce_kernel(const MatvecTile* __res
     const bool active = l < tile_targets && t < tl.t_end;
     const bool seg_t = tl.cls == 0;
     const int vt = seg_t ? SEGDOF : 3;
-    const size_t slot_len = (size_t)kMaxDof * tile_targets;
+    const size_t slot_len = (size_t)slot_dof * tile_targets;
     const int lc = l < tile_targets ? l : tile_targets - 1;
     const P* p = partial + (size_t)tl.first_item * slot_len + lc;
     // Filled slots: every item's, or (persistent form, merge_k items per
@@ -1622,17 +1623,17 @@ hipError_t launch_matvec_wall(int precision, int seg_dof, const MatvecArgs& a, c
     if (precision == 0) {
         if (seg_dof == 6)
             hipLaunchKernelGGL((rpy_reduce_kernel<6, double>), grid, dim3(256), 0, stream, d_tiles,
-                               (const double*)partial, kWallBlock, tile_blocks, 0, o, a, none);
+                               (const double*)partial, kWallBlock, tile_blocks, 0, kMaxDof, o, a, none);
         else
             hipLaunchKernelGGL((rpy_reduce_kernel<3, double>), grid, dim3(256), 0, stream, d_tiles,
-                               (const double*)partial, kWallBlock, tile_blocks, 0, o, a, none);
+                               (const double*)partial, kWallBlock, tile_blocks, 0, kMaxDof, o, a, none);
     } else {
         if (seg_dof == 6)
             hipLaunchKernelGGL((rpy_reduce_kernel<6, float>), grid, dim3(256), 0, stream, d_tiles, (const float*)pf,
-                               kWallBlock, tile_blocks, 0, o, a, none);
+                               kWallBlock, tile_blocks, 0, kMaxDof, o, a, none);
         else
             hipLaunchKernelGGL((rpy_reduce_kernel<3, float>), grid, dim3(256), 0, stream, d_tiles, (const float*)pf,
-                               kWallBlock, tile_blocks, 0, o, a, none);
+                               kWallBlock, tile_blocks, 0, kMaxDof, o, a, none);
     }
     return hipGetLastError();
 }
@@ -1672,21 +1673,22 @@ hipError_t launch_matvec(int precision, int seg_dof, const TileShape& shape, con
     // The partial slots hold the evaluation type (FP32 sums for FP32 tiles);
     // the persistent form fills only its tile segments' first slots.
     const int merge_k = shape.persistent > 0 && n_items > 0 ? persistent_items(shape, n_items, ps.num_cus) : 0;
+    const int slot_dof = shape.symmetric ? kSymSlotDof : kMaxDof;
     const float* pf = reinterpret_cast<const float*>(partial);
     if (precision == 0) {
         if (seg_dof == 6)
             hipLaunchKernelGGL((rpy_reduce_kernel<6, double>), grid, dim3(256), 0, stream, d_tiles,
-                               (const double*)partial, tt, tile_blocks, merge_k, o, a, near);
+                               (const double*)partial, tt, tile_blocks, merge_k, slot_dof, o, a, near);
         else
             hipLaunchKernelGGL((rpy_reduce_kernel<3, double>), grid, dim3(256), 0, stream, d_tiles,
-                               (const double*)partial, tt, tile_blocks, merge_k, o, a, near);
+                               (const double*)partial, tt, tile_blocks, merge_k, slot_dof, o, a, near);
     } else {
         if (seg_dof == 6)
             hipLaunchKernelGGL((rpy_reduce_kernel<6, float>), grid, dim3(256), 0, stream, d_tiles, pf, tt,
-                               tile_blocks, merge_k, o, a, near);
+                               tile_blocks, merge_k, slot_dof, o, a, near);
         else
             hipLaunchKernelGGL((rpy_reduce_kernel<3, float>), grid, dim3(256), 0, stream, d_tiles, pf, tt,
-                               tile_blocks, merge_k, o, a, near);
+                               tile_blocks, merge_k, slot_dof, o, a, near);
     }
     return hipGetLastError();
 }

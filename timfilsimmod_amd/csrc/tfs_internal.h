@@ -111,8 +111,13 @@ struct TileShape {
 // n^2 / 8 bytes) instead of sorting the particles into a hashed grid.
 constexpr int kNearPairsMax = 6144;
 
-// Tiles of 64 particles per group (and waves per workgroup) of the symmetric form.
+// Tiles of 64 particles per group (and waves per workgroup) of the symmetric
+// form; values per particle in its partial slots (3 x 64 per slot); the
+// largest partial buffer its default shape may take (the slots grow as
+// tiles x groups).
 constexpr int kSymGroupTiles = 4;
+constexpr int kSymSlotDof = 3;
+constexpr double kSymMaxPartialBytes = 4.0 * (1 << 30);
 
 // Targets per tile (and per partial slot) of a launch shape.
 inline int tile_targets(const TileShape& s) { return s.symmetric ? 64 : (s.split ? 64 : s.block) * s.per_thread; }

@@ -85,10 +85,13 @@ TileShape default_shape(int precision, int ns, int nb, int num_cus, int seg_dof,
 // 0.266 for the best source-split shape, 0.358 ms per FP64 launch against
 // 0.432; 64 uneven bodies in lock-step, a mixed timestep in 831 ms against
 // 1,027), else default_shape.
-static TileShape preferred_shape(int precision, int ns, int nb, int num_cus, int seg_dof, int num_bodies) {
+// `sym_slots`: the symmetric form's partial slots over all bodies (its buffer
+// takes 3 x 64 doubles each), which must stay below kSymMaxPartialBytes.
+static TileShape preferred_shape(int precision, int ns, int nb, int num_cus, int seg_dof, int num_bodies,
+                                 double sym_slots) {
     (void)num_bodies;
     (void)precision;
-    if (seg_dof == 3) {
+    if (seg_dof == 3 && sym_slots * kSymSlotDof * 64 * sizeof(double) <= kSymMaxPartialBytes) {
         const long tiles = (ns + 63L) / 64 + (nb + 63L) / 64;
         const long groups = (tiles + 3) / 4;
         if (groups * (groups + 1) / 2 >= 4L * (num_cus > 0 ? num_cus : 256)) {
@@ -124,6 +127,17 @@ static void largest_body(const tfs_rpy* h, int* ns, int* nb) {
             *nb = b.blob_end - b.blob_begin;
         }
     }
+}
+
+// The symmetric form's partial slots for these bodies (each tile one per
+// group of four tiles its body overlaps).
+static double sym_slot_count(const std::vector<BodyRange>& bodies) {
+    double slots = 0.0;
+    for (const BodyRange& b : bodies) {
+        const double t = double((b.seg_end - b.seg_begin + 63) / 64 + (b.blob_end - b.blob_begin + 63) / 64);
+        slots += t * (std::ceil(t / kSymGroupTiles) + 1.0);
+    }
+    return slots;
 }
 
 Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err) {
@@ -165,7 +179,7 @@ Plan* get_plan(tfs_rpy* h, uint32_t blocks, const TileShape& sh, int* err) {
             *err = hip_fail(e, "work list upload");
             return nullptr;
         }
-        const size_t need = slots * 6 * (size_t)tile_targets(sh);
+        const size_t need = slots * (size_t)(sh.symmetric ? kSymSlotDof : 6) * (size_t)tile_targets(sh);
         if (need > h->partial_len) {
             if (h->partial) {
                 hipStreamSynchronize(h->stream);
@@ -632,7 +646,8 @@ int tfs_rpy_create(const tfs_rpy_desc* desc, tfs_rpy** out) {
         return hip_fail(e, "tfs_rpy_create: clear");
     }
     for (int pr = 0; pr < 2; ++pr)
-        h->shape[pr] = preferred_shape(pr, desc->num_segs, desc->num_blobs, cus, desc->seg_dof, 1);
+        h->shape[pr] = preferred_shape(pr, desc->num_segs, desc->num_blobs, cus, desc->seg_dof, 1,
+                                       sym_slot_count({BodyRange{0, desc->num_segs, 0, desc->num_blobs}}));
     h->bodies.push_back(BodyRange{0, desc->num_segs, 0, desc->num_blobs});
     e = hipHostMalloc((void**)&h->host_int, 64, hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -933,7 +948,8 @@ int tfs_rpy_set_bodies(tfs_rpy* h, int32_t num_bodies, const int32_t* segs_per_b
     largest_body(h, &lns, &lnb);
     for (int pr = 0; pr < 2; ++pr)
         if (!h->shape_chosen[pr])
-            h->shape[pr] = preferred_shape(pr, lns, lnb, h->num_cus, h->desc.seg_dof, (int)h->bodies.size());
+            h->shape[pr] = preferred_shape(pr, lns, lnb, h->num_cus, h->desc.seg_dof, (int)h->bodies.size(),
+                                           sym_slot_count(h->bodies));
     return TFS_OK;
 }
 
