@@ -68,6 +68,9 @@ __device__ __forceinline__ void flow_sources(const FlowArgs& a, CView xsrc, CVie
     const T c8 = T(1.0 / (8.0 * PI * a.mu));
     const T aj2 = T(aj * aj);
     T part[R][3];
+    // FP32 Stokeslet mode: the rotlet's negative cross-product halves, summed
+    // apart and subtracted at the flush (every operation an accumulate)
+    T negp[R][3];
     for (int s0 = 0; s0 < nsrc; s0 += BLOCK) {
         const int n = (nsrc - s0) < BLOCK ? (nsrc - s0) : BLOCK;
         __syncthreads();
@@ -91,7 +94,10 @@ __device__ __forceinline__ void flow_sources(const FlowArgs& a, CView xsrc, CVie
         }
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < R; ++r) part[r][0] = part[r][1] = part[r][2] = T(0);
+        for (int r = 0; r < R; ++r) {
+            part[r][0] = part[r][1] = part[r][2] = T(0);
+            negp[r][0] = negp[r][1] = negp[r][2] = T(0);
+        }
         for (int j = 0; j < n; ++j) {
             const T* rec = lds + j * REC;
             constexpr int F0 = F32 ? 6 : 3;
@@ -121,9 +127,14 @@ __device__ __forceinline__ void flow_sources(const FlowArgs& a, CView xsrc, CVie
                     part[r][1] = fma_t(sd, dy, fma_t(rm1, f[1], part[r][1]));
                     part[r][2] = fma_t(sd, dz, fma_t(rm1, f[2], part[r][2]));
                     if constexpr (FS == 6) {
-                        part[r][0] = fma_t(rm3, f[4] * dz - f[5] * dy, part[r][0]);
-                        part[r][1] = fma_t(rm3, f[5] * dx - f[3] * dz, part[r][1]);
-                        part[r][2] = fma_t(rm3, f[3] * dy - f[4] * dx, part[r][2]);
+                        // (tau x d) / r^3 as + and - halves
+                        const T ex = rm3 * dx, ey = rm3 * dy, ez = rm3 * dz;
+                        part[r][0] = fma_t(f[4], ez, part[r][0]);
+                        negp[r][0] = fma_t(f[5], ey, negp[r][0]);
+                        part[r][1] = fma_t(f[5], ex, part[r][1]);
+                        negp[r][1] = fma_t(f[3], ez, negp[r][1]);
+                        part[r][2] = fma_t(f[3], ey, part[r][2]);
+                        negp[r][2] = fma_t(f[4], ex, negp[r][2]);
                     }
                     if constexpr (BLAKE) {
                         // Blakelet image system, flow_field_evaluator.cu:83-116, evaluated in
@@ -179,7 +190,7 @@ __device__ __forceinline__ void flow_sources(const FlowArgs& a, CView xsrc, CVie
 #pragma unroll
         for (int r = 0; r < R; ++r)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) acc[r][k] += double(part[r][k]);
+            for (int k = 0; k < 3; ++k) acc[r][k] += double(part[r][k]) - double(negp[r][k]);
     }
 }
 
