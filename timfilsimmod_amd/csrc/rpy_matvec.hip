@@ -333,8 +333,8 @@ __global__ __launch_bounds__(BLOCK) void rpy_pairs_kernel(const MatvecItem* __re
 // from LDS, lane l pairing its particle with source (l + kSymRot k) mod 64 at
 // step k: the lane's own sum gets the source's term and the source's sum gets
 // the lane's term.  The source sums live in one register per component that
-// moves one lane per step (a DPP wave rotation folded into the add), so after
-// 64 steps lane l holds the sum of source (l + 63 kSymRot) mod 64.  Each stage
+// moves one lane per step (a DPP wave rotation ahead of the accumulates), so
+// after 64 steps lane l holds the sum of source (l + 63 kSymRot) mod 64.  Each stage
 // adds those sums into the source tile's FP64 LDS sums; the stages are fenced
 // by barriers and the writer of a tile's sums in stage t is wave
 // (tile - t) mod kSymTiles, a fixed order.  A pair of distinct groups is
@@ -343,10 +343,11 @@ __global__ __launch_bounds__(BLOCK) void rpy_pairs_kernel(const MatvecItem* __re
 // overlapping pairs take the clamped far field, which the reduce kernel swaps
 // for the exact value), and a > b not at all.
 //
-// Partial slots: tile t of group g has n_groups slots, slot q holding the
-// sums it received from group q (its own sums when q > g, the source sums
-// when q < g, both added for q == g), each written once per launch, so the
-// reduce kernel sums them in slot order.
+// Partial slots: tile t of group g has one slot per group q its body overlaps
+// (groups run body after body; tile pairs of different bodies are skipped),
+// holding the sums it received from group q (its own sums when q > g, the
+// source sums when q < g, both added for q == g), each written once per
+// launch, so the reduce kernel sums them in slot order.
 constexpr int kSymTiles = kSymGroupTiles;
 constexpr int kSymRot = 1;  // DPP wave_rol:1 -- lane l reads lane l + 1 (mod 64)
 __device__ __forceinline__ float rotate_lanes(float x) {
