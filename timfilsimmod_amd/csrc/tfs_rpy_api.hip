@@ -89,8 +89,6 @@ TileShape default_shape(int precision, int ns, int nb, int num_cus, int seg_dof,
 // takes 3 x 64 doubles each), which must stay below kSymMaxPartialBytes.
 static TileShape preferred_shape(int precision, int ns, int nb, int num_cus, int seg_dof, int num_bodies,
                                  double sym_slots) {
-    (void)num_bodies;
-    (void)precision;
     if (seg_dof == 3 && sym_slots * kSymSlotDof * 64 * sizeof(double) <= kSymMaxPartialBytes) {
         const long tiles = (ns + 63L) / 64 + (nb + 63L) / 64;
         const long groups = (tiles + 3) / 4;
