@@ -71,6 +71,10 @@ def test_null_handles_are_errors_not_crashes():
     assert lib.tfs_rpy_destroy(None) == L.TFS_OK
     assert lib.tfs_beat_set_warm_start(None, 1) == L.TFS_ERR_INVALID_ARGUMENT
     assert lib.tfs_solver_set_preconditioner(None, 1) == L.TFS_ERR_INVALID_ARGUMENT
+    assert lib.tfs_rpy_set_symmetric(None, 1, 1) == L.TFS_ERR_INVALID_ARGUMENT
+    out = ctypes.c_int32(7)
+    assert lib.tfs_rpy_get_symmetric(None, 1, ctypes.byref(out)) == L.TFS_ERR_INVALID_ARGUMENT
+    assert out.value == 7
 
 
 def test_create_reports_missing_device_on_cpu_host():
