@@ -118,7 +118,10 @@ struct tfs_solver {
     double *pa = nullptr, *pb = nullptr;  // per-workgroup partial sums of the Arnoldi step
     int GA = 1;                           // workgroups of the Arnoldi step
     int32_t* stop = nullptr;              // device stop flag of the current GMRES cycle
-    int32_t* host_flag = nullptr;         // pinned, 3 slots
+    // The GMRES stop flag as the close kernel mirrors it: pinned host memory
+    // mapped for the device (host_stop) and its device address.
+    int32_t* host_stop = nullptr;
+    int32_t* dev_stop_mirror = nullptr;
     hipEvent_t flag_ev[2] = {nullptr, nullptr};
     double* host_scal = nullptr;  // pinned, kEstBatch + 8 doubles
     double gs = 0, gb = 0;        // 6 pi mu a for segments / blobs
@@ -534,12 +537,15 @@ int gmres(tfs_solver* s, int precision, const double* b, double* x, double* r, d
         if (!rc) rc = check(k_axpby(1.0 / rnorm, r, 0.0, s->V, n, s->stream), "v0");
         if (rc) return rc;
         // The iterations are enqueued ahead of the host: every kEstBatch of
-        // them the stop flag is copied back asynchronously, and the host looks
-        // at the copy of the batch before (so one batch is always in flight).
-        // Launches enqueued after the step that met the tolerance see the flag
-        // and do nothing.
+        // them an event is recorded, and the host waits for the event of the
+        // batch before (so one batch is always in flight) and then reads the
+        // stop flag that the close kernel mirrors into host memory.  Launches
+        // enqueued after the step that met the tolerance see the flag and do
+        // nothing.  (The previous cycle's work has finished: the mirror is the
+        // host's to clear.)
         if (!rc) rc = check(k_fill_int(s->stop, 0, 1, s->stream), "stop flag");
         if (rc) return rc;
+        __atomic_store_n(s->host_stop, 0, __ATOMIC_SEQ_CST);
         int enq = 0, batches = 0;
         bool seen = false;
         while (enq < m && !seen) {
@@ -553,8 +559,8 @@ int gmres(tfs_solver* s, int precision, const double* b, double* x, double* r, d
                 if (!rc) rc = body_correct(s, r, 1.0 / rnorm, nullptr, 0, vj, s->z, nullptr);
             } else {
                 rc = check(k_arnoldi_close_next(s->pa, s->GA, s->H, ldh, j - 1, s->cs, s->sn, s->g, s->scal, s->est,
-                                                s->stop, abs_tol, s->w, vj, s->z, precond_args(s), s->dims,
-                                                s->stream),
+                                                s->stop, s->dev_stop_mirror, abs_tol, s->w, vj, s->z, precond_args(s),
+                                                s->dims, s->stream),
                            "close and normalise");
                 if (!rc) rc = body_correct(s, s->w, 1.0, s->pa, s->GA, vj, s->z, s->stop);
             }
@@ -567,28 +573,26 @@ int gmres(tfs_solver* s, int precision, const double* b, double* x, double* r, d
             ++enq;
             if (enq % kEstBatch != 0 && enq < m) continue;
             const int slot = batches & 1;
-            hipError_t e = hipMemcpyAsync(s->host_flag + slot, s->stop, sizeof(int32_t), hipMemcpyDeviceToHost,
-                                          s->stream);
-            if (e == hipSuccess) e = hipEventRecord(s->flag_ev[slot], s->stream);
+            hipError_t e = hipEventRecord(s->flag_ev[slot], s->stream);
             if (e == hipSuccess && batches > 0) {
                 e = hipEventSynchronize(s->flag_ev[slot ^ 1]);
-                if (e == hipSuccess && s->host_flag[slot ^ 1] != 0) seen = true;
+                if (e == hipSuccess && __atomic_load_n(s->host_stop, __ATOMIC_SEQ_CST) != 0) seen = true;
             }
             if (e != hipSuccess) return hip_fail(e, "GMRES stop flag");
             ++batches;
         }
         // close the last enqueued step
         rc = check(k_arnoldi_close_next(s->pa, s->GA, s->H, ldh, enq - 1, s->cs, s->sn, s->g, s->scal, s->est, s->stop,
-                                        abs_tol, s->w, nullptr, nullptr, precond_args(s), s->dims, s->stream),
+                                        s->dev_stop_mirror, abs_tol, s->w, nullptr, nullptr, precond_args(s), s->dims,
+                                        s->stream),
                    "close");
         if (rc) return rc;
         int32_t flag = 0;
         double est = rnorm;
         {
-            hipError_t e = hipMemcpyAsync(s->host_flag + 2, s->stop, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+            const hipError_t e = hipStreamSynchronize(s->stream);
             if (e != hipSuccess) return hip_fail(e, "GMRES stop flag");
-            flag = s->host_flag[2];
+            flag = __atomic_load_n(s->host_stop, __ATOMIC_SEQ_CST);
         }
         if (flag < 0) return fail(TFS_ERR_NON_FINITE, "GMRES: non-finite residual estimate");
         const int k = flag > 0 ? flag : enq;
@@ -978,7 +982,8 @@ int alloc_solver(tfs_solver* s) {
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return hip_fail(e, "solver: clear");
     e = hipHostMalloc((void**)&s->host_scal, sizeof(double) * (kEstBatch + 8), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&s->host_flag, sizeof(int32_t) * 4, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->host_stop, sizeof(int32_t) * 4, hipHostMallocMapped);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&s->dev_stop_mirror, s->host_stop, 0);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->flag_ev[0], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->flag_ev[1], hipEventDisableTiming);
     if (e != hipSuccess) return hip_fail(e, "solver: pinned scalar");
@@ -1149,7 +1154,7 @@ int tfs_solver_destroy(tfs_solver* s) {
     if (s->stream) hipStreamSynchronize(s->stream);
     if (s->arena) hipFree(s->arena);
     if (s->host_scal) hipHostFree(s->host_scal);
-    if (s->host_flag) hipHostFree(s->host_flag);
+    if (s->host_stop) hipHostFree(s->host_stop);
     free_stage(s->f32);
     free_ens(s);
     for (hipEvent_t ev : s->flag_ev)
