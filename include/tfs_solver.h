@@ -16,7 +16,8 @@
  *     phase speeds (if dynamic_phase) | shape-rotation speeds (if dynamic_angle) ]
  *
  * The Krylov solver is GMRES(m) with right preconditioning and FP64 Arnoldi
- * (classical Gram-Schmidt applied twice).  restart = min(n, 350) with
+ * (classical Gram-Schmidt applied twice; once in the inner solve of
+ * TFS_SOLVE_MIXED, whose result the FP64 residual checks).  restart = min(n, 350) with
  * max_cycles = 1 and tol = 1e-4 reproduces the reference's single-cycle solve
  * (config.hpp:312-321); restarted cycles and mixed-precision iterative
  * refinement are additions.  All statuses are TFS_* codes of tfs_rpy.h.

@@ -88,14 +88,15 @@ hipError_t k_axpby(double a, const double* x, double b, double* y, int n, hipStr
 // x <- extrapolated warm start of the given order from x, h1, h2; h2 <- h1, h1 <- old x.
 hipError_t k_extrapolate(double* x, double* h1, double* h2, int n, int order, hipStream_t s);
 hipError_t k_sub(const double* x, const double* y, double* out, int n, hipStream_t s);
-// One CGS2 Arnoldi step on w against V(:, 0:j] (three launches of G =
-// arnoldi_blocks(n) workgroups; pa and pb hold G x (j+1) partial sums each):
-// H(:, j) = h1 + h2, w orthogonalised, and G partial sums of ||w||^2 left in
-// pa for k_arnoldi_close_next.  Every launch does nothing while *stop is
-// nonzero.
+// One Arnoldi step on w against V(:, 0:j] by classical Gram-Schmidt, with
+// passes = 2 (CGS2: three launches of G = arnoldi_blocks(n) workgroups,
+// H(:, j) = h1 + h2) or passes = 1 (CGS: two launches, H(:, j) = h1); pa and
+// pb hold G x (j+1) partial sums each.  w is left orthogonalised and G partial
+// sums of ||w||^2 in pa for k_arnoldi_close_next.  Every launch does nothing
+// while *stop is nonzero.
 hipError_t k_arnoldi_step(const double* V, int64_t ld, int j, double* w, int n, double* pa, double* pb, int G,
                           double* H, int ldh, double* cs, double* sn, double* g, double* scal, double* est,
-                          int32_t* stop, double abs_tol, hipStream_t s);
+                          int32_t* stop, double abs_tol, int passes, hipStream_t s);
 // Closes step j -- ||w|| from pa, H(j+1, j) rotated, Givens rotations
 // updated, scal[0] = est[j] = |g(j+1)|, scal[1] = 1/||w||, *stop = j + 1 when
 // est[j] <= abs_tol (-(j + 1) if it is not finite), the same value to

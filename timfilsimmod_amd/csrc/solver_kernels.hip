@@ -400,11 +400,11 @@ __global__ __launch_bounds__(256) void arnoldi_update_dots_kernel(ArnoldiArgs a,
 
 __global__ __launch_bounds__(256) void arnoldi_update_sumsq_kernel(ArnoldiArgs a, const double* __restrict__ p_in,
                                                                    double* __restrict__ p_out,
-                                                                   double* __restrict__ Hcol) {
+                                                                   double* __restrict__ Hcol, bool first) {
     __shared__ double hs[kMaxBasis];
     __shared__ double sh[4];
     if (*a.skip) return;
-    gather_coeffs(p_in, a.ncols, gridDim.x, hs, Hcol, false);
+    gather_coeffs(p_in, a.ncols, gridDim.x, hs, Hcol, first);
     slice_update(a, hs, blockIdx.x, gridDim.x);
     __syncthreads();
     double s = 0.0;
@@ -1511,13 +1511,22 @@ hipError_t k_axpby(double a, const double* x, double b, double* y, int n, hipStr
 }
 hipError_t k_arnoldi_step(const double* V, int64_t ld, int j, double* w, int n, double* pa, double* pb, int G,
                           double* H, int ldh, double* cs, double* sn, double* g, double* scal, double* est,
-                          int32_t* stop, double abs_tol, hipStream_t s) {
-    if (j + 1 > kMaxBasis) return hipErrorInvalidValue;
+                          int32_t* stop, double abs_tol, int passes, hipStream_t s) {
+    if (j + 1 > kMaxBasis || (passes != 1 && passes != 2)) return hipErrorInvalidValue;
     const ArnoldiArgs a{V, ld, j + 1, w, n, stop};
     double* Hcol = H + (int64_t)j * ldh;
-    hipLaunchKernelGGL(arnoldi_dots_kernel, dim3(G), dim3(256), 0, s, a, pa);
-    hipLaunchKernelGGL(arnoldi_update_dots_kernel, dim3(G), dim3(256), 0, s, a, (const double*)pa, pb, Hcol);
-    hipLaunchKernelGGL(arnoldi_update_sumsq_kernel, dim3(G), dim3(256), 0, s, a, (const double*)pb, pa, Hcol);
+    if (passes == 2) {
+        hipLaunchKernelGGL(arnoldi_dots_kernel, dim3(G), dim3(256), 0, s, a, pa);
+        hipLaunchKernelGGL(arnoldi_update_dots_kernel, dim3(G), dim3(256), 0, s, a, (const double*)pa, pb, Hcol);
+        hipLaunchKernelGGL(arnoldi_update_sumsq_kernel, dim3(G), dim3(256), 0, s, a, (const double*)pb, pa, Hcol,
+                           false);
+    } else {
+        // one pass: the projections go to pb, so that the norm partials
+        // written to pa overlap no coefficient partial still being read
+        hipLaunchKernelGGL(arnoldi_dots_kernel, dim3(G), dim3(256), 0, s, a, pb);
+        hipLaunchKernelGGL(arnoldi_update_sumsq_kernel, dim3(G), dim3(256), 0, s, a, (const double*)pb, pa, Hcol,
+                           true);
+    }
     return hipGetLastError();
 }
 // Waves for a preconditioner launch: one per filament (or per patch in the

@@ -515,6 +515,10 @@ int gmres(tfs_solver* s, int precision, const double* b, double* x, double* r, d
     const int n = (int)s->dims.n;
     const int m = s->m;
     const int ldh = m + 1;
+    // FP64 solves orthogonalise twice (CGS2, orthogonal to working precision);
+    // the inner solve of mixed refinement, whose result the FP64 residual
+    // checks, once.
+    const int gs_passes = precision == TFS_PRECISION_FP64 ? 2 : 1;
     *converged = rnorm <= abs_tol;
     *final_res = rnorm;
     for (int cyc = 0; cyc < max_cycles && !*converged; ++cyc) {
@@ -567,7 +571,7 @@ int gmres(tfs_solver* s, int precision, const double* b, double* x, double* r, d
             if (!rc) rc = apply_A(s, s->z, s->w, precision, nullptr, s->stop);
             if (!rc)
                 rc = check(k_arnoldi_step(s->V, s->ldv, j, s->w, n, s->pa, s->pb, s->GA, s->H, ldh, s->cs, s->sn,
-                                          s->g, s->scal, s->est, s->stop, abs_tol, s->stream),
+                                          s->g, s->scal, s->est, s->stop, abs_tol, gs_passes, s->stream),
                            "arnoldi");
             if (rc) return rc;
             ++enq;
