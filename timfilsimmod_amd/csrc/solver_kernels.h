@@ -155,8 +155,9 @@ struct EnsArgs {
     int64_t pstride = 0;
     int64_t* stamps = nullptr;        // per body: wall clock of its last stop (may be null)
 };
-// CGS2 step j of every running body (three launches, G x nbody workgroups).
-hipError_t k_ens_arnoldi_step(const EnsArgs& e, int j, int G, hipStream_t s);
+// Gram-Schmidt step j of every running body, G x nbody workgroups per launch:
+// passes = 2 (CGS2, three launches) or 1 (CGS, two launches).
+hipError_t k_ens_arnoldi_step(const EnsArgs& e, int j, int G, int passes, hipStream_t s);
 // Close step j of every running body; with write_next, v_{j+1} into the basis.
 hipError_t k_ens_close(const EnsArgs& e, int j, bool write_next, int G, hipStream_t s);
 // v_0 = r / ||r|| and g = (||r||, 0, ...) for every running body (r in the Krylov layout).

@@ -1282,10 +1282,10 @@ __device__ __forceinline__ ArnoldiArgs ens_slice(const EnsArgs& e, int b, int nc
     return ArnoldiArgs{e.V + e.koff[b], e.ld, ncols, e.w + e.koff[b], (int)(e.koff[b + 1] - e.koff[b]), nullptr};
 }
 
-__global__ __launch_bounds__(256) void ens_dots_kernel(EnsArgs e, int ncols) {
+__global__ __launch_bounds__(256) void ens_dots_kernel(EnsArgs e, int ncols, int to_pb) {
     const int b = blockIdx.y;
     if (e.stop[b]) return;
-    slice_dots(ens_slice(e, b, ncols), e.pa + (int64_t)b * e.pstride, blockIdx.x, gridDim.x);
+    slice_dots(ens_slice(e, b, ncols), (to_pb ? e.pb : e.pa) + (int64_t)b * e.pstride, blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(256) void ens_update_dots_kernel(EnsArgs e, int ncols, int j) {
@@ -1300,14 +1300,14 @@ __global__ __launch_bounds__(256) void ens_update_dots_kernel(EnsArgs e, int nco
     slice_dots(a, e.pb + (int64_t)b * e.pstride, blockIdx.x, gridDim.x);
 }
 
-__global__ __launch_bounds__(256) void ens_update_sumsq_kernel(EnsArgs e, int ncols, int j) {
+__global__ __launch_bounds__(256) void ens_update_sumsq_kernel(EnsArgs e, int ncols, int j, int first) {
     __shared__ double hs[kMaxBasis];
     __shared__ double sh[4];
     const int b = blockIdx.y;
     if (e.stop[b]) return;
     const ArnoldiArgs a = ens_slice(e, b, ncols);
     gather_coeffs(e.pb + (int64_t)b * e.pstride, ncols, gridDim.x, hs, e.H + (int64_t)b * e.hstride + (int64_t)j * e.ldh,
-                  false);
+                  first != 0);
     slice_update(a, hs, blockIdx.x, gridDim.x);
     __syncthreads();
     double s = 0.0;
@@ -1655,12 +1655,18 @@ hipError_t k_body_advance(const double* sol, int64_t off_body, double dt, double
 }
 
 // ---- independent bodies in lock-step ----------------------------------------
-hipError_t k_ens_arnoldi_step(const EnsArgs& e, int j, int G, hipStream_t s) {
-    if (j + 1 > kMaxBasis) return hipErrorInvalidValue;
+hipError_t k_ens_arnoldi_step(const EnsArgs& e, int j, int G, int passes, hipStream_t s) {
+    if (j + 1 > kMaxBasis || (passes != 1 && passes != 2)) return hipErrorInvalidValue;
     const dim3 grid((unsigned)G, (unsigned)e.nbody);
-    hipLaunchKernelGGL(ens_dots_kernel, grid, dim3(256), 0, s, e, j + 1);
-    hipLaunchKernelGGL(ens_update_dots_kernel, grid, dim3(256), 0, s, e, j + 1, j);
-    hipLaunchKernelGGL(ens_update_sumsq_kernel, grid, dim3(256), 0, s, e, j + 1, j);
+    if (passes == 2) {
+        hipLaunchKernelGGL(ens_dots_kernel, grid, dim3(256), 0, s, e, j + 1, 0);
+        hipLaunchKernelGGL(ens_update_dots_kernel, grid, dim3(256), 0, s, e, j + 1, j);
+        hipLaunchKernelGGL(ens_update_sumsq_kernel, grid, dim3(256), 0, s, e, j + 1, j, 0);
+    } else {
+        // one pass: the projections go to pb (the norm partials go to pa)
+        hipLaunchKernelGGL(ens_dots_kernel, grid, dim3(256), 0, s, e, j + 1, 1);
+        hipLaunchKernelGGL(ens_update_sumsq_kernel, grid, dim3(256), 0, s, e, j + 1, j, 1);
+    }
     return hipGetLastError();
 }
 hipError_t k_ens_close(const EnsArgs& e, int j, bool write_next, int G, hipStream_t s) {

@@ -660,7 +660,8 @@ int ens_norms(tfs_solver* s, const double* v, std::vector<double>& norms) {
 
 // Lock-step steps j0 <= j < j1 of the current cycle (no host round trip):
 // w = A z with the stopped bodies' work skipped, w to the Krylov layout, the
-// CGS2 step and close of every running body, v_{j+1} and z = P^{-1} v_{j+1}.
+// Gram-Schmidt step (twice in FP64 solves, once in the inner solve of mixed
+// refinement) and close of every running body, v_{j+1} and z = P^{-1} v_{j+1}.
 int ens_enqueue_steps(tfs_solver* s, int precision, int j0, int j1) {
     EnsState& E = *s->ens;
     const EnsArgs a = ens_args(s);
@@ -670,7 +671,9 @@ int ens_enqueue_steps(tfs_solver* s, int precision, int j0, int j1) {
         int rc = apply_A(s, s->z, s->w, precision, nullptr);
         s->rpy->body_skip = nullptr;
         if (!rc) rc = check(k_ens_permute(s->w, E.w, E.d_map, E.ntot, false, s->stream), "w to the Krylov layout");
-        if (!rc) rc = check(k_ens_arnoldi_step(a, j, E.G, s->stream), "lock-step Arnoldi");
+        if (!rc)
+            rc = check(k_ens_arnoldi_step(a, j, E.G, precision == TFS_PRECISION_FP64 ? 2 : 1, s->stream),
+                       "lock-step Arnoldi");
         if (!rc) rc = check(k_ens_close(a, j, j + 1 < m, E.G, s->stream), "lock-step close");
         if (!rc && j + 1 < m) {
             rc = check(k_ens_permute(E.V + (int64_t)(j + 1) * E.ldv, s->t, E.d_map, E.ntot, true, s->stream),
