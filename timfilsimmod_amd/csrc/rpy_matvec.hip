@@ -807,48 +807,72 @@ __global__ __launch_bounds__(256) void grid_count_kernel(MatvecArgs a, int n, do
 
 // start = exclusive prefix sum of fill (table entries, a power of two >= 1024),
 // start[table] = total; fill is cleared for the scatter.  One workgroup, in
-// chunks of 16384 entries staged through LDS with coalesced loads and stores:
-// each thread scans 16 consecutive entries, the 1024 thread sums are scanned
-// in LDS, and a running carry joins the chunks.
+// chunks of up to 16384 entries (1024 x per).  The chunk is loaded and stored
+// coalesced, every load (and store) of a thread issued together, and staged in
+// LDS with one pad word per 16 entries, so that each thread's `per`
+// consecutive entries (a stride of 17 words across the lanes) share no bank.
+// Every thread sums its entries, the sums are scanned across each wave by
+// shuffles and across the sixteen waves from their totals, and a running
+// carry joins the chunks.
 __global__ __launch_bounds__(1024) void grid_scan_kernel(int32_t* __restrict__ fill, int32_t* __restrict__ start,
                                                         int table) {
-    constexpr int kChunk = 16384, kPer = kChunk / 1024;
-    __shared__ int32_t buf[kChunk];
-    __shared__ int32_t part[1024];
-    const int tid = threadIdx.x;
+    constexpr int kChunk = 16384, kMaxPer = kChunk / 1024, kWaves = 1024 / 64;
+    __shared__ int32_t buf[kChunk + kChunk / 16];
+    __shared__ int32_t wsum[kWaves];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const auto pad = [](int i) { return i + (i >> 4); };
     int32_t carry = 0;
     for (int c0 = 0; c0 < table; c0 += kChunk) {
-        const int len = table - c0 < kChunk ? table - c0 : kChunk;   // table >= 1024: len is a multiple of 1024
-        for (int i = tid; i < len; i += 1024) buf[i] = fill[c0 + i];
-        __syncthreads();
+        const int len = table - c0 < kChunk ? table - c0 : kChunk;
         const int per = len / 1024;
+        int32_t v[kMaxPer];
+#pragma unroll
+        for (int k = 0; k < kMaxPer; ++k) v[k] = k < per ? fill[c0 + k * 1024 + tid] : 0;
+#pragma unroll
+        for (int k = 0; k < kMaxPer; ++k)
+            if (k < per) buf[pad(k * 1024 + tid)] = v[k];
+        __syncthreads();
+        const int b = tid * per;  // this thread's first entry of the chunk
         int32_t sum = 0;
-        for (int i = 0; i < per; ++i) sum += buf[tid * per + i];
-        part[tid] = sum;
+#pragma unroll
+        for (int i = 0; i < kMaxPer; ++i)
+            if (i < per) sum += buf[pad(b + i)];
+        int32_t inc = sum;  // inclusive scan of the threads' sums across the wave
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int32_t t = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += t;
+        }
+        if (lane == 63) wsum[wid] = inc;
         __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            const int32_t v = tid >= off ? part[tid - off] : 0;
-            __syncthreads();
-            part[tid] += v;
-            __syncthreads();
+        int32_t before = 0, total = 0;
+#pragma unroll
+        for (int k = 0; k < kWaves; ++k) {
+            const int32_t t = wsum[k];
+            before += k < wid ? t : 0;
+            total += t;
         }
-        int32_t run = carry + part[tid] - sum;  // exclusive prefix of this thread's entries
-        for (int i = 0; i < per; ++i) {
-            const int32_t cnt = buf[tid * per + i];
-            buf[tid * per + i] = run;
-            run += cnt;
-        }
-        const int32_t total = part[1023];
+        int32_t run = carry + before + inc - sum;  // exclusive prefix of this thread's entries
+#pragma unroll
+        for (int i = 0; i < kMaxPer; ++i)
+            if (i < per) {
+                const int32_t cnt = buf[pad(b + i)];
+                buf[pad(b + i)] = run;
+                run += cnt;
+            }
         __syncthreads();
-        for (int i = tid; i < len; i += 1024) {
-            start[c0 + i] = buf[i];
-            fill[c0 + i] = 0;
-        }
+#pragma unroll
+        for (int k = 0; k < kMaxPer; ++k) v[k] = k < per ? buf[pad(k * 1024 + tid)] : 0;
+#pragma unroll
+        for (int k = 0; k < kMaxPer; ++k)
+            if (k < per) {
+                start[c0 + k * 1024 + tid] = v[k];
+                fill[c0 + k * 1024 + tid] = 0;
+            }
         carry += total;
-        __syncthreads();
+        __syncthreads();  // buf and wsum are written again by the next chunk
     }
     if (tid == 0) start[table] = carry;
-    (void)kPer;
 }
 
 __global__ __launch_bounds__(256) void grid_scatter_kernel(int n, const int32_t* __restrict__ bucket_of,
