@@ -100,12 +100,12 @@ hipError_t k_arnoldi_step(const double* V, int64_t ld, int j, double* w, int n, 
 // Closes step j -- ||w|| from pa, H(j+1, j) rotated, Givens rotations
 // updated, scal[0] = est[j] = |g(j+1)|, scal[1] = 1/||w||, *stop = j + 1 when
 // est[j] <= abs_tol (-(j + 1) if it is not finite), the same value to
-// *stop_mirror (host memory mapped for the device, may be null) -- and, when
-// v is not null, writes v = w / ||w|| and out = P^{-1} v in the same launch.
-// Nothing happens while *stop is nonzero.
+// *stop_mirror and est[j] to *est_mirror (host memory mapped for the device,
+// either may be null) -- and, when v is not null, writes v = w / ||w|| and
+// out = P^{-1} v in the same launch.  Nothing happens while *stop is nonzero.
 hipError_t k_arnoldi_close_next(const double* pa, int G, double* H, int ldh, int j, double* cs, double* sn,
                                 double* g, double* scal, double* est, int32_t* stop, int32_t* stop_mirror,
-                                double abs_tol,
+                                double* est_mirror, double abs_tol,
                                 const double* w, double* v, double* out, const PrecondArgs& P,
                                 const SaddleDims& d, hipStream_t s);
 int arnoldi_blocks(int64_t n);

@@ -424,12 +424,13 @@ __global__ __launch_bounds__(256) void arnoldi_update_sumsq_kernel(ArnoldiArgs a
 // thread 0's sequential rotations read no memory; est[j] = |g(j+1)|,
 // scal[1] = 1/||w||.  The first step whose estimate meets abs_tol sets
 // *stop = j + 1 (a non-finite estimate sets -(j + 1)); later launches of the
-// solve then skip.  The value also goes to *stop_mirror if given (host memory
-// the host polls).  Called by every thread of one workgroup.
+// solve then skip.  The value also goes to *stop_mirror and est[j] to
+// *est_mirror if given (host memory the host reads).  Called by every thread
+// of one workgroup.
 __device__ void arnoldi_close(double hn, double* __restrict__ H, int ldh, int j, double* __restrict__ cs,
                               double* __restrict__ sn, double* __restrict__ g, double* __restrict__ scal,
                               double* __restrict__ est, int32_t* __restrict__ stop, double abs_tol,
-                              int32_t* __restrict__ stop_mirror = nullptr) {
+                              int32_t* __restrict__ stop_mirror = nullptr, double* __restrict__ est_mirror = nullptr) {
     __shared__ double col[kMaxBasis + 1], lc[kMaxBasis], ls[kMaxBasis];
     double* Hc = H + (int64_t)j * ldh;
     for (int i = threadIdx.x; i <= j; i += blockDim.x) {
@@ -469,6 +470,10 @@ __device__ void arnoldi_close(double hn, double* __restrict__ H, int ldh, int j,
         g[j] = c * gj;
         scal[0] = __builtin_fabs(g[j + 1]);
         est[j] = scal[0];
+        if (est_mirror)
+            __hip_atomic_store(reinterpret_cast<unsigned long long*>(est_mirror),
+                               (unsigned long long)__double_as_longlong(scal[0]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
         const int32_t sv_stop = !__builtin_isfinite(scal[0]) ? -(j + 1) : (scal[0] <= abs_tol ? j + 1 : 0);
         if (sv_stop) {
             *stop = sv_stop;
@@ -737,8 +742,8 @@ __global__ __launch_bounds__(256) void scale_precond_kernel(const double* __rest
 __global__ __launch_bounds__(256) void arnoldi_close_next_kernel(
     const double* __restrict__ p_in, int G, double* __restrict__ H, int ldh, int j, double* __restrict__ cs,
     double* __restrict__ sn, double* __restrict__ g, double* __restrict__ scal, double* __restrict__ est,
-    int32_t* __restrict__ stop, int32_t* __restrict__ stop_mirror, double abs_tol, const double* __restrict__ w,
-    double* __restrict__ v, double* __restrict__ out, PrecondArgs P, SaddleDims d) {
+    int32_t* __restrict__ stop, int32_t* __restrict__ stop_mirror, double* __restrict__ est_mirror, double abs_tol,
+    const double* __restrict__ w, double* __restrict__ v, double* __restrict__ out, PrecondArgs P, SaddleDims d) {
     __shared__ double nrm;
     if (*stop) return;
     if (threadIdx.x < kWave) {
@@ -747,7 +752,7 @@ __global__ __launch_bounds__(256) void arnoldi_close_next_kernel(
     }
     __syncthreads();
     const double hn = nrm;
-    if (blockIdx.x == 0) arnoldi_close(hn, H, ldh, j, cs, sn, g, scal, est, stop, abs_tol, stop_mirror);
+    if (blockIdx.x == 0) arnoldi_close(hn, H, ldh, j, cs, sn, g, scal, est, stop, abs_tol, stop_mirror, est_mirror);
     if (v) {
         const double sc = hn > 0.0 ? 1.0 / hn : 0.0;
         if (P.kind == 1) block_precond_body(P, d, w, sc, v, out);
@@ -1538,12 +1543,12 @@ static int64_t precond_work(const PrecondArgs& P, const SaddleDims& d) {
 }
 hipError_t k_arnoldi_close_next(const double* pa, int G, double* H, int ldh, int j, double* cs, double* sn,
                                 double* g, double* scal, double* est, int32_t* stop, int32_t* stop_mirror,
-                                double abs_tol, const double* w, double* v, double* out, const PrecondArgs& P,
+                                double* est_mirror, double abs_tol, const double* w, double* v, double* out, const PrecondArgs& P,
                                 const SaddleDims& d, hipStream_t s) {
     if (j + 1 > kMaxBasis) return hipErrorInvalidValue;
     const int grid = v ? blocks_for(precond_work(P, d)) : 1;
     hipLaunchKernelGGL(arnoldi_close_next_kernel, dim3(grid), dim3(256), 0, s, pa, G, H, ldh, j, cs, sn, g, scal,
-                       est, stop, stop_mirror, abs_tol, w, v, out, P, d);
+                       est, stop, stop_mirror, est_mirror, abs_tol, w, v, out, P, d);
     return hipGetLastError();
 }
 int arnoldi_blocks(int64_t n) {

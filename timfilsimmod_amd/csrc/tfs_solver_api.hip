@@ -122,6 +122,10 @@ struct tfs_solver {
     // mapped for the device (host_stop) and its device address.
     int32_t* host_stop = nullptr;
     int32_t* dev_stop_mirror = nullptr;
+    // The residual estimate of the last step the close kernel closed, mirrored
+    // the same way (the cycle's final estimate once its launches are done).
+    double* host_est = nullptr;
+    double* dev_est_mirror = nullptr;
     hipEvent_t flag_ev[2] = {nullptr, nullptr};
     double* host_scal = nullptr;  // pinned, kEstBatch + 8 doubles
     double gs = 0, gb = 0;        // 6 pi mu a for segments / blobs
@@ -563,7 +567,8 @@ int gmres(tfs_solver* s, int precision, const double* b, double* x, double* r, d
                 if (!rc) rc = body_correct(s, r, 1.0 / rnorm, nullptr, 0, vj, s->z, nullptr);
             } else {
                 rc = check(k_arnoldi_close_next(s->pa, s->GA, s->H, ldh, j - 1, s->cs, s->sn, s->g, s->scal, s->est,
-                                                s->stop, s->dev_stop_mirror, abs_tol, s->w, vj, s->z, precond_args(s),
+                                                s->stop, s->dev_stop_mirror, s->dev_est_mirror, abs_tol, s->w, vj,
+                                                s->z, precond_args(s),
                                                 s->dims, s->stream),
                            "close and normalise");
                 if (!rc) rc = body_correct(s, s->w, 1.0, s->pa, s->GA, vj, s->z, s->stop);
@@ -587,21 +592,24 @@ int gmres(tfs_solver* s, int precision, const double* b, double* x, double* r, d
         }
         // close the last enqueued step
         rc = check(k_arnoldi_close_next(s->pa, s->GA, s->H, ldh, enq - 1, s->cs, s->sn, s->g, s->scal, s->est, s->stop,
-                                        s->dev_stop_mirror, abs_tol, s->w, nullptr, nullptr, precond_args(s), s->dims,
+                                        s->dev_stop_mirror, s->dev_est_mirror, abs_tol, s->w, nullptr, nullptr, precond_args(s), s->dims,
                                         s->stream),
                    "close");
         if (rc) return rc;
+        // Once the launches are done, the mirrors hold the stop value and the
+        // estimate of the last step closed: step k - 1, where the cycle
+        // stopped at k or, unstopped, the last one enqueued (closes after a
+        // stop do nothing).
         int32_t flag = 0;
         double est = rnorm;
         {
             const hipError_t e = hipStreamSynchronize(s->stream);
             if (e != hipSuccess) return hip_fail(e, "GMRES stop flag");
             flag = __atomic_load_n(s->host_stop, __ATOMIC_SEQ_CST);
+            __atomic_load(s->host_est, &est, __ATOMIC_SEQ_CST);
         }
         if (flag < 0) return fail(TFS_ERR_NON_FINITE, "GMRES: non-finite residual estimate");
         const int k = flag > 0 ? flag : enq;
-        rc = read_scalar(s, s->est + (k - 1), &est);
-        if (rc) return rc;
         st->iterations += k;
         if (precision == TFS_PRECISION_FP64) st->fp64_matvecs += k;
         else st->fp32_matvecs += k;
@@ -991,6 +999,8 @@ int alloc_solver(tfs_solver* s) {
     e = hipHostMalloc((void**)&s->host_scal, sizeof(double) * (kEstBatch + 8), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->host_stop, sizeof(int32_t) * 4, hipHostMallocMapped);
     if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&s->dev_stop_mirror, s->host_stop, 0);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->host_est, sizeof(double), hipHostMallocMapped);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&s->dev_est_mirror, s->host_est, 0);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->flag_ev[0], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->flag_ev[1], hipEventDisableTiming);
     if (e != hipSuccess) return hip_fail(e, "solver: pinned scalar");
@@ -1162,6 +1172,7 @@ int tfs_solver_destroy(tfs_solver* s) {
     if (s->arena) hipFree(s->arena);
     if (s->host_scal) hipHostFree(s->host_scal);
     if (s->host_stop) hipHostFree(s->host_stop);
+    if (s->host_est) hipHostFree(s->host_est);
     free_stage(s->f32);
     free_ens(s);
     for (hipEvent_t ev : s->flag_ev)
