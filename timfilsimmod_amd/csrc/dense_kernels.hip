@@ -108,8 +108,9 @@ __global__ __launch_bounds__(256) void block_gauss_jordan_kernel(double* __restr
 // 64 rows by Gauss-Jordan elimination without pivoting (SPD: every pivot is
 // positive), one workgroup per block: the block sits in LDS, and each of the
 // n steps stages the pivot row and column, then every thread updates its
-// elements (16 of a 64 x 64 block).  status[b] = 1 if a pivot is not positive
-// and finite (the block is then left as loaded).
+// elements (column = lane, rows wave, wave + 4, ...: 16 of a 64 x 64 block,
+// no index division).  status[b] = 1 if a pivot is not positive and finite
+// (the block is then left as loaded).
 __global__ __launch_bounds__(256) void spd_inverse_lds_kernel(double* __restrict__ A,
                                                               const int64_t* __restrict__ off,
                                                               const int32_t* __restrict__ sizes,
@@ -136,10 +137,13 @@ __global__ __launch_bounds__(256) void spd_inverse_lds_kernel(double* __restrict
         if (tid < n) rowk[tid] = tid == k ? inv : S[k][tid] * inv;
         else if (tid >= 128 && tid - 128 < n) colk[tid - 128] = S[tid - 128][k];
         __syncthreads();
-        for (int e = tid; e < n * n; e += 256) {
-            const int i = e / n, j = e % n;
-            const double f = colk[i], rk = rowk[j];
-            S[i][j] = i == k ? rk : (j == k ? -f * inv : S[i][j] - f * rk);
+        const int j = tid & (W - 1);
+        if (j < n) {
+            const double rk = rowk[j];
+            for (int i = tid / W; i < n; i += 256 / W) {
+                const double f = colk[i];
+                S[i][j] = i == k ? rk : (j == k ? -f * inv : S[i][j] - f * rk);
+            }
         }
         __syncthreads();
     }
